@@ -1,0 +1,672 @@
+// gen.cpp -- deterministic synthetic H.264 syntax for the benchmark workloads.
+//
+// The generator produces, per picture in decoding order, exactly what a serial
+// host parser would hand to the engine: the picture descriptor (slice header /
+// PPS / reference-list state) and one h264g_mb record per macroblock plus the
+// coefficient-level stream.  Syntax is valid for the reference decoder:
+//   * intra prediction modes are drawn only among the modes whose neighbouring
+//     samples are available (the reference writes nothing for the others,
+//     H264PictureBase.cpp:1176-1186),
+//   * QPY follows mb_qp_delta semantics (7-37): MBs without mb_qp_delta inherit
+//     QPY,PREV (H264MacroBlock.cpp:1098-1124, 1194-1198),
+//   * P_Skip MVs are the 8.4.1.1 prediction (H264InterPrediction.cpp:685-738),
+//   * reference lists follow the default initialisation of 8.2.4.2 for frames.
+#include "h264g.h"
+#include "h264g_internal.h"
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+#include <algorithm>
+
+namespace {
+
+// ---------------------------------------------------------------- random
+struct Rng {
+    uint64_t s[4];
+    static uint64_t splitmix(uint64_t &x) {
+        uint64_t z = (x += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    }
+    explicit Rng(uint64_t seed) {
+        uint64_t x = seed;
+        for (int i = 0; i < 4; i++) s[i] = splitmix(x);
+    }
+    static uint64_t rotl(uint64_t v, int k) { return (v << k) | (v >> (64 - k)); }
+    uint64_t next() {  // xoshiro256**
+        uint64_t r = rotl(s[1] * 5, 7) * 9;
+        uint64_t t = s[1] << 17;
+        s[2] ^= s[0]; s[3] ^= s[1]; s[1] ^= s[2]; s[0] ^= s[3];
+        s[2] ^= t; s[3] = rotl(s[3], 45);
+        return r;
+    }
+    // uniform integer in [lo, hi]
+    int32_t uni(int32_t lo, int32_t hi) {
+        uint64_t span = (uint64_t)((int64_t)hi - lo + 1);
+        return (int32_t)(lo + (int64_t)(next() % span));
+    }
+    bool permille(int32_t p) { return (int32_t)(next() % 1000) < p; }
+    double unit() { return (double)(next() >> 11) * (1.0 / 9007199254740992.0); }
+};
+
+inline int clip3(int lo, int hi, int v) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// ------------------------------------------------------------- GOP layout
+struct PicPlan {
+    int slice_type;
+    int display;     // display index
+    int is_ref;
+    int idr;
+    int frame_num;   // decoding-order index inside the GOP (PicNum for frames)
+};
+
+std::vector<PicPlan> plan_gop(int n_frames, int b_frames, int gop) {
+    // Anchors (I/P) at display indices 0, b+1, 2(b+1), ... and the last frame;
+    // each anchor is decoded before the B pictures that precede it in display order.
+    std::vector<PicPlan> out;
+    if (gop <= 0) gop = n_frames;
+    for (int g0 = 0; g0 < n_frames; g0 += gop) {
+        int g1 = std::min(n_frames, g0 + gop);
+        int fn = 0;
+        int prev_anchor = g0;
+        out.push_back({H264G_SLICE_I, g0, 1, 1, fn++});
+        while (prev_anchor < g1 - 1) {
+            int a = std::min(prev_anchor + b_frames + 1, g1 - 1);
+            out.push_back({H264G_SLICE_P, a, 1, 0, fn++});
+            for (int d = prev_anchor + 1; d < a; d++) out.push_back({H264G_SLICE_B, d, 0, 0, fn});
+            prev_anchor = a;
+        }
+    }
+    return out;
+}
+
+struct Picture {
+    h264g_pic_desc desc;
+    std::vector<h264g_mb> mbs;
+    std::vector<int16_t> levels;
+};
+
+// Per-4x4 motion state of the picture being generated (for P_Skip prediction).
+struct Motion {
+    int16_t mv[2][2];
+    int8_t ref[2];
+    uint8_t intra;
+};
+
+}  // namespace
+
+struct h264g_gen {
+    h264g_gen_params p;
+    int dpb_slots;
+    std::vector<Picture> pics;
+    uint64_t digest;
+};
+
+namespace {
+
+// Appends one MB's levels (masks first, then values) from dense scan-order blocks.
+struct LevelWriter {
+    std::vector<uint16_t> masks;
+    std::vector<int16_t> values;
+    void block(const int16_t *coef, int n, int first) {
+        // n <= 16 entries per mask word; 64-entry blocks call this four times.
+        uint16_t m = 0;
+        for (int k = first; k < n; k++)
+            if (coef[k]) { m |= (uint16_t)(1u << k); values.push_back(coef[k]); }
+        masks.push_back(m);
+    }
+    void flush(std::vector<int16_t> &out) {
+        for (uint16_t m : masks) out.push_back((int16_t)m);
+        out.insert(out.end(), values.begin(), values.end());
+        masks.clear(); values.clear();
+    }
+};
+
+class PictureGen {
+public:
+    PictureGen(const h264g_gen_params &p, Rng &rng, h264g_pic_desc &desc, Picture &pic)
+        : P(p), R(rng), D(desc), pic(pic), W(p.width_mbs), H(p.height_mbs) {
+        motion.assign((size_t)W * 4 * H * 4, Motion());
+        mbclass.assign((size_t)W * H, 0);
+    }
+
+    void run() {
+        int slice_qp = R.uni(P.qp_lo, P.qp_hi);
+        qp_prev = slice_qp;
+        pic.mbs.resize((size_t)W * H);
+        for (int my = 0; my < H; my++)
+            for (int mx = 0; mx < W; mx++) gen_mb(mx, my);
+    }
+
+private:
+    const h264g_gen_params &P;
+    Rng &R;
+    h264g_pic_desc &D;
+    Picture &pic;
+    int W, H;
+    int qp_prev = 26;
+    std::vector<Motion> motion;
+    std::vector<uint8_t> mbclass;  // 1 = intra
+
+    Motion &mo(int bx, int by) { return motion[(size_t)by * W * 4 + bx]; }
+    bool mb_exists(int mx, int my) const { return mx >= 0 && my >= 0 && mx < W && my < H; }
+
+    int16_t draw_level() {
+        if (R.permille(P.zero_permille)) return 0;
+        double b = P.laplace_b_x100 / 100.0;
+        double u = R.unit();
+        if (u < 1e-300) u = 1e-300;
+        int mag = 1 + (int)std::floor(-b * std::log(u));
+        if (mag > 64) mag = 64;
+        return (int16_t)((R.next() & 1) ? -mag : mag);
+    }
+
+    int draw_mv_comp() {
+        if (P.mv_mode == 0) return R.uni(-P.mv_range, P.mv_range);
+        double u1 = R.unit(), u2 = R.unit();
+        if (u1 < 1e-300) u1 = 1e-300;
+        double g = std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+        int v = (int)std::lround(g * P.mv_range);
+        return clip3(P.mv_clamp_lo, P.mv_clamp_hi, v);
+    }
+
+    // --- intra mode choice among modes whose neighbours are available ---------
+    int pick_nxn_mode(bool left, bool top, bool topleft) {
+        int cand[9], n = 0;
+        cand[n++] = 2;                                   // DC
+        if (top) { cand[n++] = 0; cand[n++] = 3; cand[n++] = 7; }
+        if (left) { cand[n++] = 1; cand[n++] = 8; }
+        if (top && left && topleft) { cand[n++] = 4; cand[n++] = 5; cand[n++] = 6; }
+        return cand[R.uni(0, n - 1)];
+    }
+    int pick_16_mode(bool left, bool top, bool topleft) {
+        int cand[4], n = 0;
+        cand[n++] = 2;
+        if (top) cand[n++] = 0;
+        if (left) cand[n++] = 1;
+        if (top && left && topleft) cand[n++] = 3;
+        return cand[R.uni(0, n - 1)];
+    }
+    int pick_chroma_mode(bool left, bool top, bool topleft) {
+        int cand[4], n = 0;
+        cand[n++] = 0;
+        if (left) cand[n++] = 1;
+        if (top) cand[n++] = 2;
+        if (top && left && topleft) cand[n++] = 3;
+        return cand[R.uni(0, n - 1)];
+    }
+
+    // --- 8.4.1.1 P_Skip motion vector (non-MBAFF frames) -----------------------
+    // Neighbour motion is read per 4x4 block, which equals MvL0[mbPartIdxN][subMbPartIdxN]
+    // and RefIdxL0[mbPartIdxN] of the covering partition (H264InterPrediction.cpp:1757-2001).
+    void neighbour(int bx, int by, bool &avail, int &ref, int mv[2]) {
+        mv[0] = mv[1] = 0;
+        ref = -1;
+        avail = bx >= 0 && by >= 0 && bx < W * 4 && by < H * 4;
+        if (!avail) return;
+        const Motion &m = mo(bx, by);
+        if (m.intra || m.ref[0] < 0) return;
+        ref = m.ref[0];
+        mv[0] = m.mv[0][0];
+        mv[1] = m.mv[0][1];
+    }
+
+    void pskip_mv(int mx, int my, int out[2]) {
+        int x0 = mx * 4, y0 = my * 4;
+        bool aA, aB, aC;
+        int rA, rB, rC, mA[2], mB[2], mC[2];
+        neighbour(x0 - 1, y0, aA, rA, mA);
+        neighbour(x0, y0 - 1, aB, rB, mB);
+        // C at (16, -1): the macroblock above-right, else D at (-1, -1).
+        bool c_exists = mb_exists(mx + 1, my - 1);
+        if (c_exists) neighbour(x0 + 4, y0 - 1, aC, rC, mC);
+        else neighbour(x0 - 1, y0 - 1, aC, rC, mC);
+        if (!aA || !aB || (rA == 0 && mA[0] == 0 && mA[1] == 0) ||
+            (rB == 0 && mB[0] == 0 && mB[1] == 0)) {
+            out[0] = out[1] = 0;
+            return;
+        }
+        // median prediction with refIdx = 0 (8.4.1.3.1)
+        if (!aB && !aC && aA) {
+            mB[0] = mC[0] = mA[0]; mB[1] = mC[1] = mA[1];
+            rB = rC = rA;
+        }
+        int cnt = (rA == 0) + (rB == 0) + (rC == 0);
+        if (cnt == 1) {
+            const int *m = rA == 0 ? mA : (rB == 0 ? mB : mC);
+            out[0] = m[0]; out[1] = m[1];
+            return;
+        }
+        for (int c = 0; c < 2; c++) {
+            int a = mA[c], b = mB[c], cc = mC[c];
+            out[c] = a + b + cc - std::min(a, std::min(b, cc)) - std::max(a, std::max(b, cc));
+        }
+    }
+
+    // --- one macroblock -------------------------------------------------------
+    void gen_mb(int mx, int my) {
+        h264g_mb &mb = pic.mbs[(size_t)my * W + mx];
+        std::memset(&mb, 0, sizeof(mb));
+        for (int l = 0; l < 2; l++)
+            for (int q = 0; q < 4; q++) mb.ref_idx[l][q] = -1;
+        const int st = D.slice_type;
+        bool A = mb_exists(mx - 1, my), B = mb_exists(mx, my - 1);
+        bool Dn = mb_exists(mx - 1, my - 1);
+
+        // ---- choose class
+        int cls;
+        if (st == H264G_SLICE_I) {
+            cls = R.permille(500) ? H264G_MB_I16x16 : (P.t8x8 && R.permille(500) ? H264G_MB_I8x8 : H264G_MB_I4x4);
+        } else if (P.p_pattern && st == H264G_SLICE_P) {
+            int u = R.uni(0, 999);
+            cls = u < 100 ? H264G_MB_P_SKIP : H264G_MB_P;
+            if (cls == H264G_MB_P) mb.part_shape = (u < 400) ? H264G_PART_8x8 : H264G_PART_16x16;
+        } else {
+            if (R.permille(P.intra_in_inter_permille)) {
+                int u = R.uni(0, 2);
+                cls = u == 0 ? H264G_MB_I16x16 : (u == 1 ? H264G_MB_I4x4 : (P.t8x8 ? H264G_MB_I8x8 : H264G_MB_I4x4));
+            } else if (st == H264G_SLICE_P && R.permille(P.skip_permille)) {
+                cls = H264G_MB_P_SKIP;
+            } else {
+                cls = st == H264G_SLICE_P ? H264G_MB_P : H264G_MB_B;
+                int u = R.uni(0, 99);
+                mb.part_shape = u < 40 ? H264G_PART_16x16 : (u < 55 ? H264G_PART_16x8 : (u < 70 ? H264G_PART_8x16 : H264G_PART_8x8));
+            }
+        }
+        if (P.ipcm_permille && cls != H264G_MB_P_SKIP && R.permille(P.ipcm_permille)) cls = H264G_MB_IPCM;
+        mb.mb_type = (uint8_t)cls;
+        bool intra = cls <= H264G_MB_IPCM;
+        mbclass[(size_t)my * W + mx] = intra ? 1 : 0;
+
+        LevelWriter lw;
+        mb.level_offset = (uint32_t)pic.levels.size();
+
+        if (cls == H264G_MB_IPCM) {
+            for (int i = 0; i < 384; i++) pic.levels.push_back((int16_t)R.uni(0, 255));
+            mb.qp_y = (uint8_t)qp_prev;  // mb_qp_delta not present: QPY = QPY,PREV
+            mark_intra(mx, my);
+            return;
+        }
+
+        // ---- prediction syntax
+        if (cls == H264G_MB_I4x4 || cls == H264G_MB_I8x8) {
+            int n = cls == H264G_MB_I4x4 ? 4 : 2;   // blocks per row
+            int step = 4 / n;                       // in 4x4 units
+            for (int by = 0; by < n; by++)
+                for (int bx = 0; bx < n; bx++) {
+                    bool left = bx > 0 || A, top = by > 0 || B;
+                    bool tl = (bx > 0 && by > 0) || (bx == 0 && by > 0 && A) || (bx > 0 && by == 0 && B) || (bx == 0 && by == 0 && Dn);
+                    int mode = pick_nxn_mode(left, top, tl);
+                    int blk = cls == H264G_MB_I4x4 ? blk4x4_index(bx * step, by * step) : (by * 2 + bx);
+                    mb.intra_pred_modes[blk] = (uint8_t)mode;
+                }
+            if (cls == H264G_MB_I8x8) mb.flags |= H264G_MBF_T8x8;
+        } else if (cls == H264G_MB_I16x16) {
+            mb.intra16x16_pred_mode = (uint8_t)pick_16_mode(A, B, Dn);
+        }
+        if (intra) mb.intra_chroma_pred_mode = (uint8_t)pick_chroma_mode(A, B, Dn);
+
+        if (cls == H264G_MB_P_SKIP) {
+            int mv[2];
+            pskip_mv(mx, my, mv);
+            for (int b = 0; b < 16; b++) { mb.mv[0][b][0] = (int16_t)mv[0]; mb.mv[0][b][1] = (int16_t)mv[1]; }
+            for (int q = 0; q < 4; q++) mb.ref_idx[0][q] = 0;
+            mb.qp_y = (uint8_t)qp_prev;
+            store_motion(mx, my, mb);
+            return;
+        }
+        if (cls == H264G_MB_P || cls == H264G_MB_B) gen_inter(mb);
+
+        // ---- transform size for inter MBs (eligible: no sub-8x8 partitions)
+        bool inter = cls == H264G_MB_P || cls == H264G_MB_B;
+        if (inter && P.t8x8) {
+            bool eligible = true;
+            if (mb.part_shape == H264G_PART_8x8)
+                for (int q = 0; q < 4; q++) if (mb.sub_part_shape[q] != H264G_SUB_8x8) eligible = false;
+            if (eligible && R.permille(500)) mb.flags |= H264G_MBF_T8x8;
+        }
+        bool t8 = (mb.flags & H264G_MBF_T8x8) != 0;
+
+        // ---- residual levels (dense, scan order) and CBP derived from them
+        int16_t dc16[16] = {0};
+        int16_t luma[16][16];      // 4x4 blocks by luma4x4BlkIdx (or I16 AC at 1..15)
+        int16_t luma8[4][64];
+        int16_t cdc[2][4], cac[2][4][16];
+        std::memset(luma, 0, sizeof(luma));
+        std::memset(luma8, 0, sizeof(luma8));
+        std::memset(cac, 0, sizeof(cac));
+        if (cls == H264G_MB_I16x16) for (int k = 0; k < 16; k++) dc16[k] = draw_level();
+        int cbp_luma = 0;
+        for (int q = 0; q < 4; q++) {
+            bool nz = false;
+            if (t8) {
+                for (int k = 0; k < 64; k++) { luma8[q][k] = draw_level(); nz |= luma8[q][k] != 0; }
+            } else {
+                for (int b = 0; b < 4; b++)
+                    for (int k = (cls == H264G_MB_I16x16 ? 1 : 0); k < 16; k++) {
+                        luma[q * 4 + b][k] = draw_level();
+                        nz |= luma[q * 4 + b][k] != 0;
+                    }
+            }
+            if (nz) cbp_luma |= 1 << q;
+        }
+        if (cls == H264G_MB_I16x16 && cbp_luma) cbp_luma = 15;  // I16x16 signals 0 or 15
+        bool cdc_nz = false, cac_nz = false;
+        for (int c = 0; c < 2; c++) {
+            for (int k = 0; k < 4; k++) { cdc[c][k] = draw_level(); cdc_nz |= cdc[c][k] != 0; }
+            for (int b = 0; b < 4; b++)
+                for (int k = 1; k < 16; k++) { cac[c][b][k] = draw_level(); cac_nz |= cac[c][b][k] != 0; }
+        }
+        int cbp_chroma = cac_nz ? 2 : (cdc_nz ? 1 : 0);
+        if (cbp_chroma < 2) std::memset(cac, 0, sizeof(cac));
+        if (inter && cbp_luma == 0) mb.flags &= (uint8_t)~H264G_MBF_T8x8;  // not signalled without luma CBP
+        t8 = (mb.flags & H264G_MBF_T8x8) != 0;
+        mb.cbp = (uint8_t)(cbp_luma | (cbp_chroma << 4));
+
+        // ---- QP: mb_qp_delta present only with residual or for I16x16
+        bool has_delta = cbp_luma || cbp_chroma || cls == H264G_MB_I16x16;
+        if (has_delta) qp_prev = R.uni(P.qp_lo, P.qp_hi);
+        mb.qp_y = (uint8_t)qp_prev;
+
+        // ---- emit levels
+        if (cls == H264G_MB_I16x16) lw.block(dc16, 16, 0);
+        for (int q = 0; q < 4; q++) {
+            if (!(cbp_luma & (1 << q))) continue;
+            if (t8) {
+                for (int w = 0; w < 4; w++) {
+                    int16_t part[16];
+                    std::memcpy(part, &luma8[q][w * 16], sizeof(part));
+                    lw.block(part, 16, 0);
+                }
+            } else {
+                for (int b = 0; b < 4; b++) lw.block(luma[q * 4 + b], 16, cls == H264G_MB_I16x16 ? 1 : 0);
+            }
+        }
+        if (cbp_chroma >= 1) for (int c = 0; c < 2; c++) lw.block(cdc[c], 4, 0);
+        if (cbp_chroma == 2)
+            for (int c = 0; c < 2; c++)
+                for (int b = 0; b < 4; b++) lw.block(cac[c][b], 16, 1);
+        lw.flush(pic.levels);
+
+        if (intra) mark_intra(mx, my);
+        else store_motion(mx, my, mb);
+    }
+
+    static int blk4x4_index(int x4, int y4) {  // luma4x4BlkIdx of the 4x4 at (x4, y4)
+        return 8 * (y4 / 2) + 4 * (x4 / 2) + 2 * (y4 % 2) + (x4 % 2);
+    }
+
+    void mark_intra(int mx, int my) {
+        for (int y = 0; y < 4; y++)
+            for (int x = 0; x < 4; x++) {
+                Motion &m = mo(mx * 4 + x, my * 4 + y);
+                m = Motion();
+                m.intra = 1;
+                m.ref[0] = m.ref[1] = -1;
+            }
+    }
+
+    void store_motion(int mx, int my, const h264g_mb &mb) {
+        for (int y = 0; y < 4; y++)
+            for (int x = 0; x < 4; x++) {
+                Motion &m = mo(mx * 4 + x, my * 4 + y);
+                int q = (y / 2) * 2 + (x / 2);
+                m.intra = 0;
+                for (int l = 0; l < 2; l++) {
+                    m.ref[l] = mb.ref_idx[l][q];
+                    m.mv[l][0] = mb.mv[l][y * 4 + x][0];
+                    m.mv[l][1] = mb.mv[l][y * 4 + x][1];
+                }
+            }
+    }
+
+    // Draws partitions, prediction lists, reference indices and final MVs.
+    void gen_inter(h264g_mb &mb) {
+        const bool isB = mb.mb_type == H264G_MB_B;
+        auto draw_pred = [&]() -> int {  // 1 = L0, 2 = L1, 3 = Bi
+            if (!isB) return 1;
+            int u = R.uni(0, 99);
+            return u < 40 ? 3 : (u < 70 ? 1 : 2);
+        };
+        auto draw_ref = [&](int l) -> int { return R.uni(0, D.num_ref_idx_active[l] - 1); };
+        // With implicit weights a bi-predicted partition must not use the same picture
+        // from both lists: the reference divides by td = POC1 - POC0 = 0 there
+        // (H264InterPrediction.cpp:2958) before its 32/32 fallback test.
+        auto draw_pred_refs = [&](int &pred, int ref[2]) {
+            pred = draw_pred();
+            for (int l = 0; l < 2; l++) ref[l] = (pred & (1 << l)) ? draw_ref(l) : -1;
+            if (pred == 3 && D.weighted_bipred_idc == 2) {
+                for (int t = 0; t < 8 && D.ref_slot[0][ref[0]] == D.ref_slot[1][ref[1]]; t++) ref[1] = draw_ref(1);
+                if (D.ref_slot[0][ref[0]] == D.ref_slot[1][ref[1]]) { pred = 1; ref[1] = -1; }
+            }
+        };
+        // rectangles in 4x4 units: (x, y, w, h) with their quadrant list
+        auto fill = [&](int x0, int y0, int w, int h, int l, int ref, int mvx, int mvy) {
+            for (int y = y0; y < y0 + h; y++)
+                for (int x = x0; x < x0 + w; x++) {
+                    mb.mv[l][y * 4 + x][0] = (int16_t)mvx;
+                    mb.mv[l][y * 4 + x][1] = (int16_t)mvy;
+                }
+            for (int y = y0; y < y0 + h; y += 2)
+                for (int x = x0; x < x0 + w; x += 2) mb.ref_idx[l][(y / 2) * 2 + x / 2] = (int8_t)ref;
+        };
+        int shape = mb.part_shape;
+        if (shape != H264G_PART_8x8) {
+            int np = shape == H264G_PART_16x16 ? 1 : 2;
+            for (int p = 0; p < np; p++) {
+                int x0 = 0, y0 = 0, w = 4, h = 4;
+                if (shape == H264G_PART_16x8) { h = 2; y0 = 2 * p; }
+                if (shape == H264G_PART_8x16) { w = 2; x0 = 2 * p; }
+                int pred, refs[2];
+                draw_pred_refs(pred, refs);
+                for (int l = 0; l < 2; l++)
+                    if (pred & (1 << l)) {
+                        int mvx = draw_mv_comp(), mvy = draw_mv_comp();
+                        fill(x0, y0, w, h, l, refs[l], mvx, mvy);
+                    }
+            }
+        } else {
+            for (int q = 0; q < 4; q++) {
+                int sub = R.uni(0, 3);
+                mb.sub_part_shape[q] = (uint8_t)sub;
+                int pred, refs[2];
+                draw_pred_refs(pred, refs);
+                int qx = (q % 2) * 2, qy = (q / 2) * 2;
+                int sw = (sub == H264G_SUB_8x8 || sub == H264G_SUB_8x4) ? 2 : 1;
+                int sh = (sub == H264G_SUB_8x8 || sub == H264G_SUB_4x8) ? 2 : 1;
+                for (int l = 0; l < 2; l++) {
+                    if (!(pred & (1 << l))) continue;
+                    int ref = refs[l];
+                    for (int sy = 0; sy < 2; sy += sh)
+                        for (int sx = 0; sx < 2; sx += sw) {
+                            int mvx = draw_mv_comp(), mvy = draw_mv_comp();
+                            fill(qx + sx, qy + sy, sw, sh, l, ref, mvx, mvy);
+                        }
+                }
+            }
+        }
+    }
+};
+
+uint64_t fnv1a(uint64_t h, const void *data, size_t n) {
+    const uint8_t *p = (const uint8_t *)data;
+    for (size_t i = 0; i < n; i++) { h ^= p[i]; h *= 0x100000001B3ull; }
+    return h;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t h264g_gen_default_params(int32_t config, h264g_gen_params *o) {
+    if (!o) return H264G_E_ARG;
+    std::memset(o, 0, sizeof(*o));
+    o->config = config;
+    o->seed = 0x5EED0000ull + (uint64_t)config;
+    o->max_refs = 1;
+    o->mv_mode = 0; o->mv_range = 64; o->mv_clamp_lo = -512; o->mv_clamp_hi = 511;
+    o->qp_lo = 20; o->qp_hi = 36;
+    o->zero_permille = 700; o->laplace_b_x100 = 200;
+    o->skip_permille = 100;
+    if (config == 1) {
+        o->width_mbs = 22; o->height_mbs = 18; o->n_frames = 2;
+        o->b_frames = 0; o->p_pattern = 1;
+        return H264G_OK;
+    }
+    if (config == 2 || config == 0) {
+        o->width_mbs = 120; o->height_mbs = 68; o->n_frames = 8;
+        o->b_frames = 2; o->max_refs = 4; o->t8x8 = 1;
+        o->mv_mode = 1; o->mv_range = 24;
+        o->qp_lo = 18; o->qp_hi = 40;
+        o->zero_permille = 600; o->laplace_b_x100 = 300;
+        o->intra_in_inter_permille = 50;
+        o->weighted_pred_flag = 0; o->weighted_bipred_idc = 1;
+        o->filter_offsets = 1;
+        return H264G_OK;
+    }
+    return H264G_E_ARG;
+}
+
+int32_t h264g_gen_create(const h264g_gen_params *params, h264g_gen **out) {
+    if (!params || !out) return H264G_E_ARG;
+    const h264g_gen_params &p = *params;
+    if (p.width_mbs <= 0 || p.height_mbs <= 0 || p.n_frames <= 0 || p.max_refs < 1 || p.max_refs > 16 ||
+        p.qp_lo < 0 || p.qp_hi > 51 || p.qp_lo > p.qp_hi || p.qp_hi - p.qp_lo > 25)
+        return h264g_set_error(H264G_E_ARG, "gen: bad params");
+    h264g_gen *g = new h264g_gen();
+    g->p = p;
+    std::vector<PicPlan> plan = plan_gop(p.n_frames, p.b_frames, p.gop);
+
+    // DPB bookkeeping: reference frames in decoding order with their slot, POC, frame_num.
+    struct RefEntry { int slot, poc, frame_num; };
+    std::vector<RefEntry> refs;
+    int n_slots = p.max_refs + 1;
+    std::vector<int> slot_busy(n_slots, 0);
+    g->dpb_slots = n_slots;
+    g->pics.resize(plan.size());
+    for (size_t i = 0; i < plan.size(); i++) {
+        const PicPlan &pp = plan[i];
+        if (pp.idr) { refs.clear(); std::fill(slot_busy.begin(), slot_busy.end(), 0); }
+        Picture &pic = g->pics[i];
+        h264g_pic_desc &d = pic.desc;
+        std::memset(&d, 0, sizeof(d));
+        Rng rng(p.seed * 0x100000001B3ull + (uint64_t)i * 0x9E3779B97F4A7C15ull + 1);
+        d.slice_type = pp.slice_type;
+        d.structure = H264G_FRAME;
+        d.poc = 2 * pp.display;
+        d.disable_deblocking_filter_idc = p.disable_deblocking_filter_idc;
+        if (p.filter_offsets) {
+            d.filter_offset_a = 2 * rng.uni(-2, 2);
+            d.filter_offset_b = 2 * rng.uni(-2, 2);
+        }
+        d.chroma_qp_index_offset = p.chroma_qp_index_offset;
+        d.second_chroma_qp_index_offset = p.second_chroma_qp_index_offset;
+        d.weighted_pred_flag = p.weighted_pred_flag;
+        d.weighted_bipred_idc = p.weighted_bipred_idc;
+        for (int l = 0; l < 2; l++)
+            for (int r = 0; r < 32; r++) d.ref_slot[l][r] = -1;
+
+        // ---- reference lists (8.2.4.2.1 / 8.2.4.2.3 for frames)
+        std::vector<RefEntry> l0, l1;
+        if (pp.slice_type == H264G_SLICE_P) {
+            l0 = refs;
+            std::sort(l0.begin(), l0.end(), [](const RefEntry &a, const RefEntry &b) { return a.frame_num > b.frame_num; });
+        } else if (pp.slice_type == H264G_SLICE_B) {
+            std::vector<RefEntry> before, after;
+            for (auto &r : refs) (r.poc < d.poc ? before : after).push_back(r);
+            std::sort(before.begin(), before.end(), [](const RefEntry &a, const RefEntry &b) { return a.poc > b.poc; });
+            std::sort(after.begin(), after.end(), [](const RefEntry &a, const RefEntry &b) { return a.poc < b.poc; });
+            l0 = before; l0.insert(l0.end(), after.begin(), after.end());
+            l1 = after; l1.insert(l1.end(), before.begin(), before.end());
+            if (l1.size() > 1 && l0.size() == l1.size()) {
+                bool same = true;
+                for (size_t k = 0; k < l0.size(); k++) same &= l0[k].slot == l1[k].slot;
+                if (same) std::swap(l1[0], l1[1]);
+            }
+        }
+        std::vector<RefEntry> *lists[2] = {&l0, &l1};
+        for (int l = 0; l < 2; l++) {
+            int n = (int)std::min<size_t>(lists[l]->size(), (size_t)p.max_refs);
+            d.num_ref_idx_active[l] = n;
+            for (int r = 0; r < n; r++) {
+                d.ref_slot[l][r] = (*lists[l])[r].slot;
+                d.ref_poc[l][r] = (*lists[l])[r].poc;
+            }
+        }
+        // ---- explicit weighted prediction tables (pred_weight_table syntax)
+        bool explicit_wp = (pp.slice_type == H264G_SLICE_P && p.weighted_pred_flag == 1) ||
+                           (pp.slice_type == H264G_SLICE_B && p.weighted_bipred_idc == 1);
+        if (explicit_wp) {
+            d.luma_log2_weight_denom = 5;
+            d.chroma_log2_weight_denom = 5;
+            int nl = pp.slice_type == H264G_SLICE_B ? 2 : 1;
+            for (int l = 0; l < nl; l++) {
+                for (int r = 0; r < d.num_ref_idx_active[l]; r++) {
+                    d.luma_weight[l][r] = (int16_t)rng.uni(16, 48);
+                    d.luma_offset[l][r] = (int16_t)rng.uni(-8, 8);
+                    for (int c = 0; c < 2; c++) {
+                        d.chroma_weight[l][r][c] = (int16_t)rng.uni(16, 48);
+                        d.chroma_offset[l][r][c] = (int16_t)rng.uni(-8, 8);
+                    }
+                }
+                d.luma_weight_flag_last[l] = d.num_ref_idx_active[l] > 0 ? 1 : 0;
+            }
+        }
+        // ---- destination slot
+        int dst = -1;
+        for (int s = 0; s < n_slots; s++) if (!slot_busy[s]) { dst = s; break; }
+        if (dst < 0) { delete g; return h264g_set_error(H264G_E_STATE, "gen: no free DPB slot"); }
+        d.dst_slot = dst;
+
+        // ---- macroblocks
+        PictureGen pg(p, rng, d, pic);
+        pg.run();
+
+        // ---- reference marking: sliding window over max_refs frames
+        if (pp.is_ref) {
+            if ((int)refs.size() == p.max_refs) {
+                auto oldest = std::min_element(refs.begin(), refs.end(), [](const RefEntry &a, const RefEntry &b) { return a.frame_num < b.frame_num; });
+                slot_busy[oldest->slot] = 0;
+                refs.erase(oldest);
+            }
+            refs.push_back({dst, d.poc, pp.frame_num});
+            slot_busy[dst] = 1;
+        }
+    }
+    uint64_t h = 0xCBF29CE484222325ull;
+    for (auto &pic : g->pics) {
+        h = fnv1a(h, &pic.desc, sizeof(pic.desc));
+        h = fnv1a(h, pic.mbs.data(), pic.mbs.size() * sizeof(h264g_mb));
+        h = fnv1a(h, pic.levels.data(), pic.levels.size() * sizeof(int16_t));
+    }
+    g->digest = h;
+    *out = g;
+    return H264G_OK;
+}
+
+int32_t h264g_gen_destroy(h264g_gen *gen) {
+    delete gen;
+    return H264G_OK;
+}
+
+int32_t h264g_gen_num_pictures(const h264g_gen *gen) { return gen ? (int32_t)gen->pics.size() : H264G_E_ARG; }
+
+int32_t h264g_gen_dpb_slots(const h264g_gen *gen) { return gen ? gen->dpb_slots : H264G_E_ARG; }
+
+uint64_t h264g_gen_digest(const h264g_gen *gen) { return gen ? gen->digest : 0; }
+
+int32_t h264g_gen_picture(h264g_gen *gen, int32_t idx, h264g_pic_desc *desc, const h264g_mb **mbs,
+                          uint32_t *n_mbs, const int16_t **levels, uint64_t *n_levels) {
+    if (!gen || idx < 0 || idx >= (int32_t)gen->pics.size()) return H264G_E_ARG;
+    const Picture &p = gen->pics[idx];
+    if (desc) *desc = p.desc;
+    if (mbs) *mbs = p.mbs.data();
+    if (n_mbs) *n_mbs = (uint32_t)p.mbs.size();
+    if (levels) *levels = p.levels.data();
+    if (n_levels) *n_levels = p.levels.size();
+    return H264G_OK;
+}
+
+}  // extern "C"

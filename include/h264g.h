@@ -1,0 +1,286 @@
+/*
+ * h264g.h -- C ABI of the MI355X (gfx950) H.264 picture-reconstruction engine.
+ *
+ * The engine owns the data-parallel half of an H.264 decoder: dequantisation and
+ * inverse transforms (clause 8.5), inter prediction with fractional-sample
+ * interpolation and weighted prediction (8.4.2/8.4.3), intra prediction (8.3)
+ * and the deblocking filter (8.7).  Parsing and entropy decoding stay on the
+ * host, which hands per-picture syntax arrays to the engine through this ABI.
+ *
+ * Where the reference decoder (jfu222/h264_video_decoder_demo) performs the same
+ * work, each entry point names the reference interface it replaces:
+ *
+ *   per-MB reconstruction dispatch   h264_video_decoder_demo/H264SliceData.cpp:386-487
+ *   P_Skip / B_Skip reconstruction   h264_video_decoder_demo/H264SliceData.cpp:222-227,333-338
+ *   per-picture deblocking           h264_video_decoder_demo/H264PictureBase.cpp:707
+ *   DPB picture storage              h264_video_decoder_demo/H264PictureBase.cpp:160-182
+ *   output callback (samples read)   h264_video_decoder_demo/H264VideoDecoder.h:22, main.cpp:25-31
+ *
+ * Conventions (same as the reference, h264_video_decoder_demo/CommonFunction.h:40-48):
+ * every function returns int32: 0 = success, negative = error (H264G_E_*).
+ * Handles are opaque.  Input arrays are copied before the call returns, so the
+ * caller keeps ownership.  Calls on one stream must be serialised by the caller.
+ * All structures are plain C with fixed-width fields; no torch types cross the ABI.
+ */
+#ifndef H264G_H
+#define H264G_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ errors */
+#define H264G_OK             0
+#define H264G_E_ARG         -1   /* invalid argument / malformed syntax record      */
+#define H264G_E_NODEV       -2   /* no usable gfx950 device                          */
+#define H264G_E_NOMEM       -3   /* host or device allocation failed                 */
+#define H264G_E_BUDGET      -4   /* stream admission refused: DPB HBM budget        */
+#define H264G_E_STATE       -5   /* call not valid in the current state              */
+#define H264G_E_DEVICE      -6   /* a HIP runtime call failed                        */
+#define H264G_E_SYNTAX      -7   /* syntax that the reference rejects (e.g. WP sum)  */
+#define H264G_E_TIMEOUT     -8   /* a device wavefront wait exceeded its spin limit  */
+
+/* ----------------------------------------------------------- enumerations */
+/* slice_type % 5, as in h264_video_decoder_demo/H264CommonFunc.h (P=0, B=1, I=2). */
+#define H264G_SLICE_P   0
+#define H264G_SLICE_B   1
+#define H264G_SLICE_I   2
+
+/* picture structure (h264_video_decoder_demo/H264CommonFunc.h H264_PICTURE_CODED_TYPE) */
+#define H264G_FRAME         0
+#define H264G_TOP_FIELD     1
+#define H264G_BOTTOM_FIELD  2
+
+/* Macroblock class.  The exact mb_type name follows from the class, the partition
+ * shape and which lists each partition uses (ref_idx >= 0), the way
+ * CH264MacroBlock::MbPartPredMode resolves it (H264MacroBlock.cpp:509). */
+#define H264G_MB_I4x4     0   /* I_NxN, transform_size_8x8_flag = 0 */
+#define H264G_MB_I8x8     1   /* I_NxN, transform_size_8x8_flag = 1 */
+#define H264G_MB_I16x16   2   /* I_16x16_<mode>_<cbp>               */
+#define H264G_MB_IPCM     3   /* I_PCM                              */
+#define H264G_MB_P        4   /* P_L0_16x16 / P_L0_L0_16x8 / P_L0_L0_8x16 / P_8x8 */
+#define H264G_MB_P_SKIP   5   /* P_Skip (final MV supplied by the host)        */
+#define H264G_MB_B        6   /* B_{L0,L1,Bi}_*, B_8x8 with non-direct subtypes */
+
+/* Partition shapes (MbPartWidth x MbPartHeight) and sub-partition shapes. */
+#define H264G_PART_16x16  0
+#define H264G_PART_16x8   1
+#define H264G_PART_8x16   2
+#define H264G_PART_8x8    3
+#define H264G_SUB_8x8     0
+#define H264G_SUB_8x4     1
+#define H264G_SUB_4x8     2
+#define H264G_SUB_4x4     3
+
+/* h264g_mb.flags */
+#define H264G_MBF_T8x8    0x01u   /* transform_size_8x8_flag                 */
+#define H264G_MBF_FIELD   0x02u   /* mb_field_decoding_flag (MBAFF pairs)    */
+
+/* ----------------------------------------------------------------- records */
+/*
+ * One fixed-size record per macroblock, in macroblock-address order.  The fields
+ * are the ones CH264MacroBlock carries into reconstruction after parsing
+ * (H264MacroBlock.h:113-223): types, QPY, CBP, resolved intra prediction modes,
+ * reference indices and the final motion vectors (mvp + mvd, or the P_Skip
+ * prediction), plus where the MB's coefficient levels start.
+ *
+ * mv[list][blk][c]: quarter-sample MV of the 4x4 luma block blk = 4*y4 + x4
+ * (raster order inside the MB), component c (0 = x, 1 = y).  Every 4x4 block of a
+ * partition carries the partition's MV.  Unused lists are ignored.
+ *
+ * ref_idx[list][q]: reference index of the 8x8 quadrant q (raster order) in
+ * RefPicList<list>, or -1 when the quadrant does not predict from that list.
+ */
+typedef struct h264g_mb {
+    uint8_t  mb_type;                 /* H264G_MB_*                                  */
+    uint8_t  flags;                   /* H264G_MBF_*                                 */
+    uint8_t  qp_y;                    /* QPY, 0..51                                  */
+    uint8_t  cbp;                     /* CodedBlockPatternLuma | Chroma << 4         */
+    uint8_t  intra16x16_pred_mode;    /* Intra16x16PredMode, 0..3                    */
+    uint8_t  intra_chroma_pred_mode;  /* intra_chroma_pred_mode, 0..3                */
+    uint8_t  part_shape;              /* H264G_PART_*   (inter)                      */
+    uint8_t  reserved0;
+    uint8_t  sub_part_shape[4];       /* H264G_SUB_*    (part_shape == 8x8)          */
+    uint8_t  intra_pred_modes[16];    /* Intra4x4PredMode[luma4x4BlkIdx] or
+                                         Intra8x8PredMode[luma8x8BlkIdx] (first 4)  */
+    int8_t   ref_idx[2][4];
+    int16_t  mv[2][16][2];
+    uint32_t level_offset;            /* first int16 of this MB's level block        */
+    uint32_t reserved1;
+} h264g_mb;
+
+/*
+ * Coefficient levels of one macroblock, starting at levels[level_offset]:
+ *
+ *   uint16 masks[n_masks]   then   int16 values[sum of popcount(masks)]
+ *
+ * Each mask covers one transform block; bit k set means the level at scan position
+ * k (zig-zag, or field scan for field MBs / field pictures) is non-zero, and the
+ * values follow in mask order.  Blocks appear in the order residual() parses them
+ * (H264MacroBlock.cpp:1592-1856):
+ *   1. Intra16x16DCLevel                       (one mask)       if I16x16
+ *   2. for i8x8 in 0..3 with CodedBlockPatternLuma bit i8x8:
+ *        transform 8x8: one 64-entry block     (four masks, scan positions 0..63)
+ *        otherwise: luma4x4BlkIdx 4*i8x8+0..3 (one mask each; for I16x16 the AC
+ *        levels sit at scan positions 1..15, bit 0 is zero)
+ *   3. ChromaDCLevel Cb, Cr                    (one mask each, bits 0..3) if cbp>>4 >= 1
+ *   4. ChromaACLevel Cb 0..3, Cr 0..3          (one mask each, positions 1..15) if cbp>>4 == 2
+ * All masks of the MB come first, then all values.  I_PCM MBs instead carry
+ * 384 samples (256 Y, 64 Cb, 64 Cr) as int16 with no masks.
+ */
+
+/* ---------------------------------------------------------- descriptors */
+typedef struct h264g_config {
+    int32_t  device;              /* HIP device ordinal                               */
+    uint64_t dpb_budget_bytes;    /* HBM budget for all streams' DPB slots (0 = none) */
+    int32_t  max_batch_pictures;  /* pictures per batched launch (0 = default 256)    */
+} h264g_config;
+
+typedef struct h264g_stream_desc {
+    int32_t  pic_width_in_mbs;    /* PicWidthInMbs                                    */
+    int32_t  frame_height_in_mbs; /* FrameHeightInMbs                                 */
+    int32_t  chroma_array_type;   /* 1 (4:2:0) -- the only format this engine accepts */
+    int32_t  bit_depth;           /* 8                                                */
+    int32_t  frame_mbs_only;      /* frame_mbs_only_flag                              */
+    int32_t  mbaff;               /* mb_adaptive_frame_field_flag                     */
+    int32_t  dpb_slots;           /* DPB pictures to reserve, 1..17                   */
+} h264g_stream_desc;
+
+/*
+ * Per-picture state the reconstruction reads from the slice header, PPS and the
+ * reference lists (one slice per picture).  Weighted-prediction syntax is passed
+ * raw, as pred_weight_table() leaves it in CH264SliceHeader
+ * (H264SliceHeader.cpp:586-665), so the engine derives the per-partition weights
+ * itself with the reference's rules (H264InterPrediction.cpp:2833-3050).
+ */
+typedef struct h264g_pic_desc {
+    int32_t  slice_type;                      /* H264G_SLICE_*                         */
+    int32_t  structure;                       /* H264G_FRAME / _TOP_FIELD / _BOTTOM_FIELD */
+    int32_t  mbaff;                           /* MbaffFrameFlag                        */
+    int32_t  constrained_intra_pred_flag;
+    int32_t  disable_deblocking_filter_idc;
+    int32_t  filter_offset_a;                 /* slice_alpha_c0_offset_div2 << 1       */
+    int32_t  filter_offset_b;                 /* slice_beta_offset_div2 << 1           */
+    int32_t  chroma_qp_index_offset;
+    int32_t  second_chroma_qp_index_offset;
+    int32_t  weighted_pred_flag;
+    int32_t  weighted_bipred_idc;
+    int32_t  luma_log2_weight_denom;
+    int32_t  chroma_log2_weight_denom;
+    int32_t  num_ref_idx_active[2];           /* RefPicListXLength                     */
+    int32_t  poc;                             /* PicOrderCnt of the current picture    */
+    int32_t  dst_slot;                        /* DPB slot that receives the picture    */
+    int32_t  ref_slot[2][32];                 /* RefPicListX[i] -> DPB slot            */
+    int32_t  ref_poc[2][32];                  /* PicOrderCnt(RefPicListX[i])           */
+    int32_t  ref_long_term[2][32];            /* marked "used for long-term reference" */
+    int32_t  luma_weight_flag_last[2];        /* last parsed luma_weight_lX_flag       */
+    int16_t  luma_weight[2][32];
+    int16_t  luma_offset[2][32];
+    int16_t  chroma_weight[2][32][2];
+    int16_t  chroma_offset[2][32][2];
+} h264g_pic_desc;
+
+typedef struct h264g_ctx h264g_ctx;
+typedef struct h264g_pic h264g_pic;
+
+/* ------------------------------------------------------------- lifecycle */
+/* Replaces: decoder construction + CH264PicturesGOP allocation
+ * (H264VideoDecoder.cpp:56, H264PicturesGOP.cpp:37-47). */
+int32_t h264g_create(const h264g_config *cfg, h264g_ctx **out);
+int32_t h264g_destroy(h264g_ctx *ctx);
+
+/* Reserve a stream's device DPB.  Fails with H264G_E_BUDGET when the HBM budget
+ * would be exceeded (serving admission).  Replaces the per-frame my_malloc of
+ * CH264PictureBase::init (H264PictureBase.cpp:160-182). */
+int32_t h264g_stream_open(h264g_ctx *ctx, const h264g_stream_desc *desc, uint32_t *stream_id);
+int32_t h264g_stream_close(h264g_ctx *ctx, uint32_t stream_id);
+
+/* -------------------------------------------------------- plain path */
+/* Stage one picture's syntax: copies the records and levels to device memory
+ * (input arrays stay owned by the caller).  Replaces the per-MB dispatch at
+ * H264SliceData.cpp:386-487 (the host calls this where the reference called the
+ * reconstruction functions, once per picture). */
+int32_t h264g_picture_begin(h264g_ctx *ctx, uint32_t stream_id, const h264g_pic_desc *desc,
+                            const h264g_mb *mbs, uint32_t n_mbs,
+                            const int16_t *levels, uint64_t n_levels, h264g_pic **out);
+
+/* Enqueue reconstruction + deblocking of one picture on the context's HIP stream.
+ * Replaces Deblocking_filter_process at H264PictureBase.cpp:707. */
+int32_t h264g_picture_submit(h264g_ctx *ctx, h264g_pic *pic);
+
+/* Enqueue many staged pictures as one batched job: one persistent launch per
+ * stage drains a device queue of macroblock tiles from all of them.  The
+ * pictures must not reference each other (e.g. different streams). */
+int32_t h264g_batch_submit(h264g_ctx *ctx, h264g_pic *const *pics, uint32_t n_pics);
+
+/* Wait for all enqueued work; returns the first device error word if any. */
+int32_t h264g_sync(h264g_ctx *ctx);
+
+/* Copy a decoded DPB slot to host planes (stride = width).  Call before the
+ * output callback fires, as do_callback does (H264VideoDecoder.cpp:383-435). */
+int32_t h264g_slot_readback(h264g_ctx *ctx, uint32_t stream_id, int32_t slot,
+                            uint8_t *y, uint8_t *cb, uint8_t *cr);
+/* Upload host planes into a DPB slot (tests, resuming a stream at an IDR). */
+int32_t h264g_slot_upload(h264g_ctx *ctx, uint32_t stream_id, int32_t slot,
+                          const uint8_t *y, const uint8_t *cb, const uint8_t *cr);
+
+/* Release a staged picture's device syntax buffers. */
+int32_t h264g_picture_release(h264g_ctx *ctx, h264g_pic *pic);
+
+/* Algorithmic bytes of a staged picture for the roofline report:
+ * out[0] = MC + residual (inter), out[1] = intra, out[2] = deblocking. */
+int32_t h264g_picture_alg_bytes(const h264g_pic *pic, uint64_t out[3]);
+
+/* Last error message of the calling thread (static storage). */
+const char *h264g_last_error(void);
+
+/* ----------------------------------------------------- syntax generator */
+/* Deterministic synthetic syntax (configs of the benchmark).  Host-only. */
+typedef struct h264g_gen h264g_gen;
+
+typedef struct h264g_gen_params {
+    int32_t  config;              /* 1 = CIF I+P, 2 = 1080p IPBBPBBP High, 0 = custom  */
+    uint64_t seed;
+    int32_t  width_mbs, height_mbs;
+    int32_t  n_frames;
+    int32_t  gop;                 /* frames between IDRs (0 = one GOP)                */
+    int32_t  b_frames;            /* B pictures between anchors                        */
+    int32_t  max_refs;
+    int32_t  t8x8;                /* allow transform_size_8x8_flag / Intra8x8          */
+    int32_t  mv_mode;             /* 0 uniform [-mv_range, mv_range], 1 Gaussian       */
+    int32_t  mv_range;            /* uniform half-range, or Gaussian sigma (qpel)      */
+    int32_t  mv_clamp_lo, mv_clamp_hi;
+    int32_t  qp_lo, qp_hi;
+    int32_t  zero_permille;       /* probability a level is zero, in 1/1000            */
+    int32_t  laplace_b_x100;      /* Laplacian scale of level magnitudes, x100         */
+    int32_t  intra_in_inter_permille;
+    int32_t  skip_permille;
+    int32_t  weighted_pred_flag, weighted_bipred_idc;
+    int32_t  filter_offsets;      /* draw slice alpha/beta offsets in [-2,2]           */
+    int32_t  disable_deblocking_filter_idc;
+    int32_t  chroma_qp_index_offset, second_chroma_qp_index_offset;
+    int32_t  ipcm_permille;
+    int32_t  p_pattern;           /* 1 = config-1 P mix (60/30/10), 0 = general mix    */
+} h264g_gen_params;
+
+int32_t h264g_gen_default_params(int32_t config, h264g_gen_params *out);
+int32_t h264g_gen_create(const h264g_gen_params *params, h264g_gen **out);
+int32_t h264g_gen_destroy(h264g_gen *gen);
+/* Pictures are produced in decoding order. */
+int32_t h264g_gen_num_pictures(const h264g_gen *gen);
+int32_t h264g_gen_picture(h264g_gen *gen, int32_t idx, h264g_pic_desc *desc,
+                          const h264g_mb **mbs, uint32_t *n_mbs,
+                          const int16_t **levels, uint64_t *n_levels);
+/* Number of DPB slots the stream needs. */
+int32_t h264g_gen_dpb_slots(const h264g_gen *gen);
+/* FNV-1a-64 of all generated syntax (determinism check across machines). */
+uint64_t h264g_gen_digest(const h264g_gen *gen);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* H264G_H */
