@@ -1,0 +1,730 @@
+// api.cpp -- host side of libh264g: the C ABI of include/h264g.h.
+//
+// Staging a picture (h264g_picture_begin) validates the host parser's syntax
+// records, derives the per-MB values the kernels need (DPB slots of the
+// reference indices, resolved weighted-prediction entries, non-zero masks for
+// the boundary strength, chroma / deblocking QPs) and builds the three device
+// work queues: inter MBs, intra MBs and all MBs, the latter two ordered by the
+// wavefront time t = x + 2y so that every dependency of a queue entry precedes it.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <new>
+#include <vector>
+
+#include "h264g.h"
+#include "h264g_dev.h"
+#include "h264g_internal.h"
+
+namespace {
+
+thread_local char g_err[512] = "";
+
+int chroma_qp(int qpy, int offset) {  // 8.5.8, Table 8-15 (H264PictureBase.cpp:4748-4813)
+    static const uint8_t tab[22] = {29, 30, 31, 32, 32, 33, 34, 34, 35, 35, 36, 36, 37, 37, 37, 38, 38, 38, 39, 39, 39, 39};
+    int qpi = std::min(51, std::max(0, qpy + offset));
+    return qpi < 30 ? qpi : tab[qpi - 30];
+}
+
+bool cls_inter(int c) { return c == H264G_MB_P || c == H264G_MB_P_SKIP || c == H264G_MB_B; }
+bool cls_intra(int c) { return c == H264G_MB_I4x4 || c == H264G_MB_I8x8 || c == H264G_MB_I16x16; }
+
+#define HIPCHK(expr)                                                                     \
+    do {                                                                                 \
+        hipError_t e_ = (expr);                                                          \
+        if (e_ != hipSuccess) return h264g_set_error(H264G_E_DEVICE, "%s: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+struct Stream {
+    h264g_stream_desc desc;
+    int W = 0, H = 0;
+    uint8_t *dpb = nullptr;
+    uint64_t slot_bytes = 0;
+    uint32_t *flags = nullptr;
+    uint64_t bytes = 0;
+    bool open = false;
+};
+
+struct Batch;
+
+}  // namespace
+
+struct h264g_pic {
+    h264g_ctx *ctx = nullptr;
+    uint32_t stream = 0;
+    h264g_pic_desc desc;
+    int W = 0, H = 0;
+    void *dmem = nullptr;          // one device block for everything below
+    h264g_mb *d_mbs = nullptr;
+    DevAux *d_aux = nullptr;
+    int16_t *d_levels = nullptr;
+    DevWeight *d_wt = nullptr;
+    uint32_t *d_items = nullptr;   // inter | intra | deblock queues (picture index 0)
+    DevPic *d_pic = nullptr;
+    uint32_t n_inter = 0, n_intra = 0, n_mbs = 0;
+    DevPic hpic;                   // host copy (batches re-pack it)
+    std::vector<uint32_t> intra_t, all_t;  // host copies of the t-ordered queues (MB addresses)
+    std::vector<uint32_t> inter_list;
+    uint64_t alg[3] = {0, 0, 0};
+};
+
+namespace {
+struct Batch {
+    void *dmem = nullptr;
+    DevPic *d_pics = nullptr;
+    uint32_t *d_inter = nullptr, *d_intra = nullptr, *d_dbk = nullptr;
+    uint32_t n_inter = 0, n_intra = 0, n_dbk = 0;
+};
+}  // namespace
+
+struct h264g_batch : Batch {};
+
+struct h264g_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint32_t *d_err = nullptr;
+    DevCtl *d_ctl = nullptr;       // [0] intra, [1] deblock
+    uint32_t epoch = 0;
+    int waves = 2048;
+    uint64_t budget = 0, used = 0;
+    std::vector<Stream> streams;
+    std::vector<h264g_batch *> deferred;  // batches freed at the next sync
+    bool profile = false;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    std::mutex mu;
+};
+
+extern "C" int32_t h264g_set_error(int32_t code, const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+namespace {
+
+// --------------------------------------------------------------- weights
+// 8.4.2.3 mode selection and 8.4.3 values (H264InterPrediction.cpp:2526-2613, 2833-3050).
+// Explicit entries read the tables at refIdx -1 for an unused list, which in the
+// reference aliases the preceding slice-header member: luma_weight_lX[-1] is
+// luma_weight_lX_flag (the last one parsed) and luma_offset_lX[-1] is
+// luma_weight_lX[31] (H264SliceHeader.h:78-80).  L1-only luma uses that (w0, o0).
+int derive_weight(const h264g_pic_desc &d, int r0, int r1, DevWeight &w) {
+    std::memset(&w, 0, sizeof(w));
+    int pf0 = r0 >= 0, pf1 = r1 >= 0;
+    if (d.slice_type == H264G_SLICE_P) w.mode = d.weighted_pred_flag == 1;
+    else if (d.slice_type == H264G_SLICE_B) w.mode = d.weighted_bipred_idc == 1 || (d.weighted_bipred_idc == 2 && pf0 && pf1);
+    if (!w.mode) return 0;
+    if (d.slice_type == H264G_SLICE_B && d.weighted_bipred_idc == 2) {
+        int poc0 = d.ref_poc[0][r0], poc1 = d.ref_poc[1][r1];
+        int ww0 = 32, ww1 = 32;
+        // POC1 == POC0: the reference divides by zero (H264InterPrediction.cpp:2958);
+        // the spec's 32/32 fallback applies.
+        if (poc1 != poc0 && !d.ref_long_term[0][r0] && !d.ref_long_term[1][r1]) {
+            int tb = std::min(127, std::max(-128, d.poc - poc0)), td = std::min(127, std::max(-128, poc1 - poc0));
+            int tx = (16384 + std::abs(td / 2)) / td;
+            int dsf = std::min(1023, std::max(-1024, (tb * tx + 32) >> 6));
+            if (!((dsf >> 2) < -64 || (dsf >> 2) > 128)) { ww0 = 64 - (dsf >> 2); ww1 = dsf >> 2; }
+        }
+        for (int c = 0; c < 3; c++) { w.logWD[c] = 5; w.w0[c] = (int16_t)ww0; w.w1[c] = (int16_t)ww1; }
+        return 0;
+    }
+    w.logWD[0] = (int16_t)d.luma_log2_weight_denom;
+    w.logWD[1] = w.logWD[2] = (int16_t)d.chroma_log2_weight_denom;
+    w.w0[0] = (int16_t)(pf0 ? d.luma_weight[0][r0] : d.luma_weight_flag_last[0]);
+    w.o0[0] = (int16_t)(pf0 ? d.luma_offset[0][r0] : d.luma_weight[0][31]);
+    w.w1[0] = (int16_t)(pf1 ? d.luma_weight[1][r1] : d.luma_weight_flag_last[1]);
+    w.o1[0] = (int16_t)(pf1 ? d.luma_offset[1][r1] : d.luma_weight[1][31]);
+    for (int c = 0; c < 2; c++) {
+        w.w0[1 + c] = pf0 ? d.chroma_weight[0][r0][c] : 0;
+        w.o0[1 + c] = pf0 ? d.chroma_offset[0][r0][c] : 0;
+        w.w1[1 + c] = pf1 ? d.chroma_weight[1][r1][c] : 0;
+        w.o1[1 + c] = pf1 ? d.chroma_offset[1][r1][c] : 0;
+    }
+    if (pf0 && pf1) {  // H264InterPrediction.cpp:3038-3047 (its Cr test mixes in w0Cb)
+        auto bad = [](int s, int lw) { return !(-128 <= s && s <= (lw == 7 ? 127 : 128)); };
+        if (bad(w.w0[0] + w.w1[0], w.logWD[0]) || bad(w.w0[1] + w.w1[1], w.logWD[1]) ||
+            !(-128 <= w.w0[2] + w.w1[2] && w.w0[1] + w.w1[2] <= (w.logWD[2] == 7 ? 127 : 128)))
+            return h264g_set_error(H264G_E_SYNTAX, "explicit bi-prediction weight sum out of range");
+    }
+    return 0;
+}
+
+// Mask index -> level slot, the order residual() parses the blocks (H264MacroBlock.cpp:1592-1856).
+int level_slot(int i, int i16, int cbp) {
+    int idx = 0;
+    if (i16) { if (i == idx) return 0; idx++; }
+    for (int q = 0; q < 4; q++)
+        if (cbp & (1 << q))
+            for (int b = 0; b < 4; b++) { if (i == idx) return 1 + 4 * q + b; idx++; }
+    int cc = cbp >> 4;
+    if (cc >= 1) for (int c = 0; c < 2; c++) { if (i == idx) return 17 + c; idx++; }
+    if (cc == 2) for (int k = 0; k < 8; k++) { if (i == idx) return 19 + k; idx++; }
+    return -1;
+}
+int count_masks(int i16, int cbp) {
+    int n = i16 ? 1 : 0;
+    for (int q = 0; q < 4; q++) if (cbp & (1 << q)) n += 4;
+    if ((cbp >> 4) >= 1) n += 2;
+    if ((cbp >> 4) == 2) n += 8;
+    return n;
+}
+
+// Intra prediction modes must only use available neighbours; the reference leaves
+// the block unwritten otherwise (H264PictureBase.cpp:1176-1186), which has no defined
+// output, so such syntax is rejected.
+bool nxn_mode_ok(int mode, bool left, bool top, bool tl) {
+    switch (mode) {
+    case 0: case 3: case 7: return top;
+    case 1: case 8: return left;
+    case 2: return true;
+    case 4: case 5: case 6: return top && left && tl;
+    default: return false;
+    }
+}
+
+int stage_picture(h264g_ctx *ctx, Stream &st, const h264g_pic_desc &d, const h264g_mb *mbs, uint32_t n_mbs,
+                  const int16_t *levels, uint64_t n_levels, h264g_pic *pic) {
+    const int W = st.W, H = st.H;
+    if (d.structure != H264G_FRAME || d.mbaff) return h264g_set_error(H264G_E_ARG, "only progressive frame pictures are supported");
+    if (d.slice_type < 0 || d.slice_type > 2) return h264g_set_error(H264G_E_ARG, "slice_type %d", d.slice_type);
+    if ((int)n_mbs != W * H) return h264g_set_error(H264G_E_ARG, "n_mbs %u != %d", n_mbs, W * H);
+    const int nslots = st.desc.dpb_slots;
+    if (d.dst_slot < 0 || d.dst_slot >= nslots) return h264g_set_error(H264G_E_ARG, "dst_slot %d", d.dst_slot);
+    for (int l = 0; l < 2; l++) {
+        if (d.num_ref_idx_active[l] < 0 || d.num_ref_idx_active[l] > 32) return h264g_set_error(H264G_E_ARG, "num_ref_idx_active");
+        for (int r = 0; r < d.num_ref_idx_active[l]; r++)
+            if (d.ref_slot[l][r] < 0 || d.ref_slot[l][r] >= nslots || d.ref_slot[l][r] == d.dst_slot)
+                return h264g_set_error(H264G_E_ARG, "ref_slot[%d][%d] = %d", l, r, d.ref_slot[l][r]);
+    }
+    if (d.disable_deblocking_filter_idc < 0 || d.disable_deblocking_filter_idc > 2 || d.filter_offset_a < -12 || d.filter_offset_a > 12 ||
+        d.filter_offset_b < -12 || d.filter_offset_b > 12)
+        return h264g_set_error(H264G_E_ARG, "deblocking parameters");
+    std::vector<DevAux> aux(n_mbs);
+    std::vector<DevWeight> wt;
+    std::map<int, int> wmap;
+    {
+        DevWeight w0;
+        std::memset(&w0, 0, sizeof(w0));
+        wt.push_back(w0);  // entry 0: intra / unused
+    }
+    uint64_t alg_inter = 0, alg_intra = 0;
+    for (uint32_t a = 0; a < n_mbs; a++) {
+        const h264g_mb &m = mbs[a];
+        DevAux &x = aux[a];
+        std::memset(&x, 0, sizeof(x));
+        for (int l = 0; l < 2; l++) for (int q = 0; q < 4; q++) x.refid[l][q] = -1;
+        const int mbx = a % W, mby = a / W, cls = m.mb_type;
+        if (cls > H264G_MB_B) return h264g_set_error(H264G_E_ARG, "mb %u: mb_type %d", a, cls);
+        if (m.qp_y > 51) return h264g_set_error(H264G_E_ARG, "mb %u: qp %d", a, m.qp_y);
+        if (m.level_offset > n_levels) return h264g_set_error(H264G_E_ARG, "mb %u: level_offset", a);
+        const int cbp = m.cbp, i16 = cls == H264G_MB_I16x16;
+        const int t8 = (m.flags & H264G_MBF_T8x8) ? 1 : 0;
+        if ((cbp & 15) && (cbp >> 4) > 2) return h264g_set_error(H264G_E_ARG, "mb %u: cbp", a);
+        if ((cbp >> 4) > 2) return h264g_set_error(H264G_E_ARG, "mb %u: cbp", a);
+        if (i16 && (cbp & 15) != 0 && (cbp & 15) != 15) return h264g_set_error(H264G_E_ARG, "mb %u: Intra16x16 cbp", a);
+        if (cls == H264G_MB_P_SKIP && cbp) return h264g_set_error(H264G_E_ARG, "mb %u: P_Skip with cbp", a);
+        if (cls == H264G_MB_I8x8 && !t8) return h264g_set_error(H264G_E_ARG, "mb %u: I8x8 without transform_size_8x8_flag", a);
+        if ((cls == H264G_MB_I4x4 || cls == H264G_MB_I16x16) && t8)
+            return h264g_set_error(H264G_E_ARG, "mb %u: transform_size_8x8_flag on an MB type that has no 8x8 transform", a);
+        // transform_size_8x8_flag is only parsed for inter MBs with luma residual and
+        // no sub-8x8 partitions (7.3.5, H264MacroBlock.cpp:1069-1071).
+        if (cls_inter(cls) && t8 && ((cbp & 15) == 0 || (m.part_shape == H264G_PART_8x8 &&
+                                                        (m.sub_part_shape[0] | m.sub_part_shape[1] | m.sub_part_shape[2] | m.sub_part_shape[3]))))
+            return h264g_set_error(H264G_E_ARG, "mb %u: transform_size_8x8_flag without luma residual or with sub-8x8 partitions", a);
+        if (m.flags & H264G_MBF_FIELD) return h264g_set_error(H264G_E_ARG, "mb %u: field MB in a frame picture", a);
+        if (d.slice_type == H264G_SLICE_I && cls_inter(cls)) return h264g_set_error(H264G_E_ARG, "mb %u: inter MB in I slice", a);
+        if (d.slice_type != H264G_SLICE_B && cls == H264G_MB_B) return h264g_set_error(H264G_E_ARG, "mb %u: B MB outside B slice", a);
+        if (d.slice_type != H264G_SLICE_P && (cls == H264G_MB_P || cls == H264G_MB_P_SKIP)) return h264g_set_error(H264G_E_ARG, "mb %u: P MB outside P slice", a);
+        x.qdb[0] = (uint8_t)(cls == H264G_MB_IPCM ? 0 : m.qp_y);
+        // Deblocking swaps the two chroma offsets (Deblock.cpp:813-815 passes iCbCr where
+        // isChromaCb is expected; H264PictureBase.cpp:4816-4829).
+        x.qdb[1] = (uint8_t)chroma_qp(x.qdb[0], d.second_chroma_qp_index_offset);
+        x.qdb[2] = (uint8_t)chroma_qp(x.qdb[0], d.chroma_qp_index_offset);
+        x.qpc[0] = (uint8_t)chroma_qp(m.qp_y, d.chroma_qp_index_offset);
+        x.qpc[1] = (uint8_t)chroma_qp(m.qp_y, d.second_chroma_qp_index_offset);
+        if (cls == H264G_MB_IPCM) {
+            if (m.level_offset + 384 > n_levels) return h264g_set_error(H264G_E_ARG, "mb %u: PCM samples", a);
+            x.kind = AUX_PCM;
+            // RefIdxLX[0] = 0 and PredFlagLX = 0 from the zeroed MB state (Deblock.cpp:1175-1183)
+            for (int l = 0; l < 2; l++)
+                for (int q = 0; q < 4; q++) x.refid[l][q] = (int8_t)(d.num_ref_idx_active[l] > 0 ? d.ref_slot[l][0] : -1);
+            alg_inter += 384 + 384 * 2 + 16;
+            continue;
+        }
+        if (cls_intra(cls)) x.kind |= AUX_INTRA;
+        else x.kind |= AUX_INTER;
+        if (t8) x.kind |= AUX_T8;
+        // ---- levels: masks -> non-zero bits
+        int nmask = count_masks(i16, cbp);
+        if (cls == H264G_MB_P_SKIP) nmask = 0;
+        x.nmask = (uint8_t)nmask;
+        uint64_t nvals = 0;
+        if (m.level_offset + (uint64_t)nmask > n_levels) return h264g_set_error(H264G_E_ARG, "mb %u: masks beyond levels", a);
+        for (int i = 0; i < nmask; i++) {
+            uint16_t mk = (uint16_t)levels[m.level_offset + i];
+            int slot = level_slot(i, i16, cbp);
+            int bits = (slot == 17 || slot == 18) ? 0x000F : ((slot == 0 || (slot >= 1 && slot <= 16 && !i16)) ? 0xFFFF : 0xFFFE);
+            if (slot >= 1 && slot <= 16 && x.kind & AUX_T8) bits = 0xFFFF;
+            if (mk & ~bits) return h264g_set_error(H264G_E_ARG, "mb %u: level mask %d has positions the block lacks", a, i);
+            nvals += __builtin_popcount(mk);
+            if (slot >= 1 && slot <= 16 && mk) {
+                if (x.kind & AUX_T8) x.nz8 |= (uint8_t)(1 << ((slot - 1) >> 2));
+                else x.nz4 |= (uint16_t)(1 << (slot - 1));
+            }
+        }
+        if (m.level_offset + nmask + nvals > n_levels) return h264g_set_error(H264G_E_ARG, "mb %u: values beyond levels", a);
+        if (nmask) x.kind |= AUX_RESID;
+        // ---- intra modes vs neighbour availability
+        if (cls_intra(cls)) {
+            auto avail = [&](int dx, int dy) {
+                int nx = mbx + dx, ny = mby + dy;
+                if (nx < 0 || ny < 0 || nx >= W) return false;
+                if (d.constrained_intra_pred_flag && cls_inter(mbs[ny * W + nx].mb_type)) return false;
+                return true;
+            };
+            bool A = avail(-1, 0), B = avail(0, -1), D = avail(-1, -1);
+            if (cls == H264G_MB_I4x4 || cls == H264G_MB_I8x8) {
+                int n = cls == H264G_MB_I4x4 ? 4 : 2;
+                for (int by = 0; by < n; by++)
+                    for (int bx = 0; bx < n; bx++) {
+                        bool left = bx > 0 || A, top = by > 0 || B;
+                        bool tl = (bx > 0 && by > 0) || (bx == 0 && by > 0 && A) || (bx > 0 && by == 0 && B) || (bx == 0 && by == 0 && D);
+                        int blk = cls == H264G_MB_I4x4 ? (8 * (by / 2) + 4 * (bx / 2) + 2 * (by % 2) + (bx % 2)) : by * 2 + bx;
+                        if (!nxn_mode_ok(m.intra_pred_modes[blk], left, top, tl))
+                            return h264g_set_error(H264G_E_SYNTAX, "mb %u: intra mode %d of block %d uses unavailable samples", a, m.intra_pred_modes[blk], blk);
+                    }
+            } else {
+                int md = m.intra16x16_pred_mode;
+                if (md > 3 || (md == 0 && !B) || (md == 1 && !A) || (md == 3 && !(A && B && D)))
+                    return h264g_set_error(H264G_E_SYNTAX, "mb %u: Intra16x16 mode %d", a, md);
+            }
+            int cm = m.intra_chroma_pred_mode;
+            if (cm > 3 || (cm == 1 && !A) || (cm == 2 && !B) || (cm == 3 && !(A && B && D)))
+                return h264g_set_error(H264G_E_SYNTAX, "mb %u: intra chroma mode %d", a, cm);
+            alg_intra += 384 + 2 * (uint64_t)(nmask + nvals) + 16 * 2 + 48;
+            continue;
+        }
+        // ---- inter: reference slots, prediction flags, weight entries
+        if (m.part_shape > 3) return h264g_set_error(H264G_E_ARG, "mb %u: part_shape", a);
+        for (int q = 0; q < 4; q++) {
+            int r[2] = {m.ref_idx[0][q], m.ref_idx[1][q]};
+            if (r[0] < 0 && r[1] < 0) return h264g_set_error(H264G_E_ARG, "mb %u: quadrant %d predicts from no list", a, q);
+            if (cls != H264G_MB_B && r[1] >= 0) return h264g_set_error(H264G_E_ARG, "mb %u: list 1 in a P MB", a);
+            for (int l = 0; l < 2; l++) {
+                if (r[l] < -1 || r[l] >= d.num_ref_idx_active[l]) return h264g_set_error(H264G_E_ARG, "mb %u: ref_idx[%d][%d] = %d", a, l, q, r[l]);
+                if (r[l] >= 0) {
+                    x.refid[l][q] = (int8_t)d.ref_slot[l][r[l]];
+                    x.pf |= (uint8_t)(1 << (4 * l + q));
+                }
+            }
+            int key = (r[0] + 1) * 64 + (r[1] + 1);
+            auto it = wmap.find(key);
+            if (it == wmap.end()) {
+                DevWeight w;
+                int rc = derive_weight(d, r[0], r[1], w);
+                if (rc) return rc;
+                it = wmap.emplace(key, (int)wt.size()).first;
+                wt.push_back(w);
+                if (wt.size() > 255) return h264g_set_error(H264G_E_ARG, "too many weight entries");
+            }
+            x.wp[q] = (uint8_t)it->second;
+        }
+        // ---- algorithmic bytes of this MB (reference footprints per 4x4 block group)
+        for (int b = 0; b < 16; b++) {
+            int q = (b / 8) * 2 + (b % 4) / 2;
+            for (int l = 0; l < 2; l++) {
+                if (m.ref_idx[l][q] < 0) continue;
+                int fx = m.mv[l][b][0] & 3, fy = m.mv[l][b][1] & 3, cfx = m.mv[l][b][0] & 7, cfy = m.mv[l][b][1] & 7;
+                alg_inter += (uint64_t)(4 + 5 * (fx != 0)) * (4 + 5 * (fy != 0)) + 2ull * (2 + (cfx != 0)) * (2 + (cfy != 0));
+            }
+        }
+        alg_inter += 384 + 2 * (uint64_t)(nmask + nvals) + sizeof(h264g_mb) + sizeof(DevAux);
+    }
+    // ---- queues
+    pic->inter_list.clear();
+    pic->intra_t.clear();
+    pic->all_t.clear();
+    for (uint32_t a = 0; a < n_mbs; a++)
+        if (aux[a].kind & (AUX_INTER | AUX_PCM)) pic->inter_list.push_back(a);
+    for (int t = 0; t <= (W - 1) + 2 * (H - 1); t++)
+        for (int y = 0; y < H; y++) {
+            int xx = t - 2 * y;
+            if (xx < 0) break;
+            if (xx >= W) continue;
+            uint32_t a = (uint32_t)(y * W + xx);
+            pic->all_t.push_back(a);
+            if (aux[a].kind & AUX_INTRA) pic->intra_t.push_back(a);
+        }
+    pic->n_inter = (uint32_t)pic->inter_list.size();
+    pic->n_intra = (uint32_t)pic->intra_t.size();
+    pic->n_mbs = n_mbs;
+    pic->alg[0] = alg_inter;
+    pic->alg[1] = alg_intra;
+    pic->alg[2] = 2ull * 384 * n_mbs + (uint64_t)n_mbs * (sizeof(DevAux) + 4 * 32);
+    // ---- device block: mbs | aux | levels | weights | items | pic
+    auto al = [](size_t v) { return (v + 255) & ~(size_t)255; };
+    size_t o_mbs = 0, o_aux = al(o_mbs + n_mbs * sizeof(h264g_mb)), o_lv = al(o_aux + n_mbs * sizeof(DevAux));
+    size_t o_wt = al(o_lv + n_levels * sizeof(int16_t) + 16), o_it = al(o_wt + wt.size() * sizeof(DevWeight));
+    size_t n_items = pic->n_inter + pic->n_intra + n_mbs;
+    size_t o_pic = al(o_it + n_items * sizeof(uint32_t)), total = al(o_pic + sizeof(DevPic));
+    void *dm = nullptr;
+    HIPCHK(hipMalloc(&dm, total));
+    pic->dmem = dm;
+    char *base = (char *)dm;
+    pic->d_mbs = (h264g_mb *)(base + o_mbs);
+    pic->d_aux = (DevAux *)(base + o_aux);
+    pic->d_levels = (int16_t *)(base + o_lv);
+    pic->d_wt = (DevWeight *)(base + o_wt);
+    pic->d_items = (uint32_t *)(base + o_it);
+    pic->d_pic = (DevPic *)(base + o_pic);
+    std::vector<uint32_t> items;
+    items.reserve(n_items);
+    items.insert(items.end(), pic->inter_list.begin(), pic->inter_list.end());
+    items.insert(items.end(), pic->intra_t.begin(), pic->intra_t.end());
+    items.insert(items.end(), pic->all_t.begin(), pic->all_t.end());
+    DevPic &P = pic->hpic;
+    std::memset(&P, 0, sizeof(P));
+    P.mbs = pic->d_mbs; P.aux = pic->d_aux; P.levels = pic->d_levels; P.wt = pic->d_wt;
+    P.dpb = st.dpb; P.slot_bytes = st.slot_bytes; P.flags = st.flags; P.dst = d.dst_slot;
+    P.W = W; P.H = H; P.offA = d.filter_offset_a; P.offB = d.filter_offset_b;
+    P.dbk_idc = d.disable_deblocking_filter_idc; P.cip = d.constrained_intra_pred_flag;
+    HIPCHK(hipMemcpy(pic->d_mbs, mbs, n_mbs * sizeof(h264g_mb), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(pic->d_aux, aux.data(), n_mbs * sizeof(DevAux), hipMemcpyHostToDevice));
+    if (n_levels) HIPCHK(hipMemcpy(pic->d_levels, levels, n_levels * sizeof(int16_t), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(pic->d_wt, wt.data(), wt.size() * sizeof(DevWeight), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(pic->d_items, items.data(), items.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(pic->d_pic, &P, sizeof(DevPic), hipMemcpyHostToDevice));
+    (void)ctx;
+    return 0;
+}
+
+int launch_stages(h264g_ctx *ctx, const DevPic *d_pics, const uint32_t *inter, uint32_t n_inter, const uint32_t *intra,
+                  uint32_t n_intra, const uint32_t *dbk, uint32_t n_dbk, bool dbk_off) {
+    hipStream_t s = ctx->stream;
+    if (ctx->profile) HIPCHK(hipEventRecord(ctx->ev[0], s));
+    int e = h264g_launch_inter(d_pics, inter, n_inter, ctx->d_err, s);
+    if (e) return h264g_set_error(H264G_E_DEVICE, "inter launch: %s", hipGetErrorString((hipError_t)e));
+    if (ctx->profile) HIPCHK(hipEventRecord(ctx->ev[1], s));
+    e = h264g_launch_intra(d_pics, intra, n_intra, &ctx->d_ctl[0], ++ctx->epoch, ctx->d_err, ctx->waves, s);
+    if (e) return h264g_set_error(H264G_E_DEVICE, "intra launch: %s", hipGetErrorString((hipError_t)e));
+    if (ctx->profile) HIPCHK(hipEventRecord(ctx->ev[2], s));
+    if (!dbk_off) {
+        e = h264g_launch_deblock(d_pics, dbk, n_dbk, &ctx->d_ctl[1], ++ctx->epoch, ctx->d_err, ctx->waves, s);
+        if (e) return h264g_set_error(H264G_E_DEVICE, "deblock launch: %s", hipGetErrorString((hipError_t)e));
+    }
+    if (ctx->profile) HIPCHK(hipEventRecord(ctx->ev[3], s));
+    return 0;
+}
+
+int free_batch(h264g_batch *b) {
+    if (!b) return 0;
+    if (b->dmem) (void)hipFree(b->dmem);
+    delete b;
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *h264g_last_error(void) { return g_err; }
+
+int32_t h264g_create(const h264g_config *cfg, h264g_ctx **out) {
+    if (!out) return h264g_set_error(H264G_E_ARG, "out is NULL");
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return h264g_set_error(H264G_E_NODEV, "no HIP device");
+    int dev = cfg ? cfg->device : 0;
+    if (dev < 0 || dev >= ndev) return h264g_set_error(H264G_E_NODEV, "device %d of %d", dev, ndev);
+    HIPCHK(hipSetDevice(dev));
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, dev));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return h264g_set_error(H264G_E_NODEV, "device %d is %s, the kernels are built for gfx950", dev, prop.gcnArchName);
+    h264g_ctx *c = new (std::nothrow) h264g_ctx();
+    if (!c) return h264g_set_error(H264G_E_NOMEM, "ctx");
+    c->device = dev;
+    c->budget = cfg ? cfg->dpb_budget_bytes : 0;
+    c->profile = cfg && cfg->profile;
+    c->waves = (cfg && cfg->waves > 0) ? cfg->waves : prop.multiProcessorCount * 8;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipMalloc(&c->d_err, sizeof(uint32_t)) != hipSuccess || hipMalloc(&c->d_ctl, 4 * sizeof(DevCtl)) != hipSuccess ||
+        hipMemset(c->d_err, 0, sizeof(uint32_t)) != hipSuccess || hipMemset(c->d_ctl, 0, 4 * sizeof(DevCtl)) != hipSuccess) {
+        h264g_destroy(c);
+        return h264g_set_error(H264G_E_DEVICE, "context allocation failed");
+    }
+    if (c->profile)
+        for (auto &e : c->ev)
+            if (hipEventCreate(&e) != hipSuccess) { h264g_destroy(c); return h264g_set_error(H264G_E_DEVICE, "event"); }
+    *out = c;
+    return H264G_OK;
+}
+
+int32_t h264g_destroy(h264g_ctx *c) {
+    if (!c) return H264G_OK;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (auto *b : c->deferred) free_batch(b);
+    for (auto &s : c->streams) {
+        if (s.dpb) (void)hipFree(s.dpb);
+        if (s.flags) (void)hipFree(s.flags);
+    }
+    for (auto &e : c->ev) if (e) (void)hipEventDestroy(e);
+    if (c->d_err) (void)hipFree(c->d_err);
+    if (c->d_ctl) (void)hipFree(c->d_ctl);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return H264G_OK;
+}
+
+int32_t h264g_stream_open(h264g_ctx *c, const h264g_stream_desc *d, uint32_t *id) {
+    if (!c || !d || !id) return h264g_set_error(H264G_E_ARG, "NULL argument");
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (d->pic_width_in_mbs <= 0 || d->frame_height_in_mbs <= 0 || d->pic_width_in_mbs * d->frame_height_in_mbs >= (1 << 20))
+        return h264g_set_error(H264G_E_ARG, "picture size");
+    if (d->chroma_array_type != 1 || d->bit_depth != 8) return h264g_set_error(H264G_E_ARG, "only 8-bit 4:2:0 is supported");
+    if (d->dpb_slots < 1 || d->dpb_slots > 17) return h264g_set_error(H264G_E_ARG, "dpb_slots");
+    if (!d->frame_mbs_only || d->mbaff) return h264g_set_error(H264G_E_ARG, "field coding is not supported");
+    Stream s;
+    s.desc = *d;
+    s.W = d->pic_width_in_mbs;
+    s.H = d->frame_height_in_mbs;
+    s.slot_bytes = (uint64_t)s.W * s.H * 384;
+    uint64_t need = s.slot_bytes * d->dpb_slots + (uint64_t)s.W * s.H * 4;
+    if (c->budget && c->used + need > c->budget)
+        return h264g_set_error(H264G_E_BUDGET, "DPB budget: %llu + %llu > %llu", (unsigned long long)c->used,
+                               (unsigned long long)need, (unsigned long long)c->budget);
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipMalloc(&s.dpb, s.slot_bytes * d->dpb_slots));
+    if (hipMalloc(&s.flags, (size_t)s.W * s.H * 4) != hipSuccess) {
+        (void)hipFree(s.dpb);
+        return h264g_set_error(H264G_E_NOMEM, "flags");
+    }
+    HIPCHK(hipMemset(s.dpb, 0, s.slot_bytes * d->dpb_slots));
+    HIPCHK(hipMemset(s.flags, 0, (size_t)s.W * s.H * 4));
+    s.bytes = need;
+    s.open = true;
+    c->used += need;
+    uint32_t i = 0;
+    for (; i < c->streams.size(); i++) if (!c->streams[i].open) break;
+    if (i == c->streams.size()) c->streams.push_back(s);
+    else c->streams[i] = s;
+    *id = i;
+    return H264G_OK;
+}
+
+int32_t h264g_stream_close(h264g_ctx *c, uint32_t id) {
+    if (!c) return h264g_set_error(H264G_E_ARG, "ctx");
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (id >= c->streams.size() || !c->streams[id].open) return h264g_set_error(H264G_E_ARG, "stream %u", id);
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    Stream &s = c->streams[id];
+    (void)hipFree(s.dpb);
+    (void)hipFree(s.flags);
+    c->used -= s.bytes;
+    s = Stream();
+    return H264G_OK;
+}
+
+int32_t h264g_picture_begin(h264g_ctx *c, uint32_t sid, const h264g_pic_desc *d, const h264g_mb *mbs, uint32_t n_mbs,
+                            const int16_t *levels, uint64_t n_levels, h264g_pic **out) {
+    if (!c || !d || !mbs || !out || (!levels && n_levels)) return h264g_set_error(H264G_E_ARG, "NULL argument");
+    *out = nullptr;
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (sid >= c->streams.size() || !c->streams[sid].open) return h264g_set_error(H264G_E_ARG, "stream %u", sid);
+    HIPCHK(hipSetDevice(c->device));
+    h264g_pic *p = new (std::nothrow) h264g_pic();
+    if (!p) return h264g_set_error(H264G_E_NOMEM, "pic");
+    p->ctx = c;
+    p->stream = sid;
+    p->desc = *d;
+    p->W = c->streams[sid].W;
+    p->H = c->streams[sid].H;
+    int rc = stage_picture(c, c->streams[sid], *d, mbs, n_mbs, levels, n_levels, p);
+    if (rc) {
+        if (p->dmem) (void)hipFree(p->dmem);
+        delete p;
+        return rc;
+    }
+    *out = p;
+    return H264G_OK;
+}
+
+int32_t h264g_picture_submit(h264g_ctx *c, h264g_pic *p) {
+    if (!c || !p || p->ctx != c) return h264g_set_error(H264G_E_ARG, "bad picture");
+    std::lock_guard<std::mutex> lk(c->mu);
+    HIPCHK(hipSetDevice(c->device));
+    const uint32_t *it = p->d_items;
+    return launch_stages(c, p->d_pic, it, p->n_inter, it + p->n_inter, p->n_intra, it + p->n_inter + p->n_intra, p->n_mbs,
+                         p->desc.disable_deblocking_filter_idc == 1);
+}
+
+int32_t h264g_picture_release(h264g_ctx *c, h264g_pic *p) {
+    if (!p) return H264G_OK;
+    if (c) {
+        (void)hipSetDevice(c->device);
+        (void)hipStreamSynchronize(c->stream);
+    }
+    if (p->dmem) (void)hipFree(p->dmem);
+    delete p;
+    return H264G_OK;
+}
+
+int32_t h264g_picture_alg_bytes(const h264g_pic *p, uint64_t out[3]) {
+    if (!p || !out) return h264g_set_error(H264G_E_ARG, "NULL argument");
+    for (int i = 0; i < 3; i++) out[i] = p->alg[i];
+    return H264G_OK;
+}
+
+int32_t h264g_batch_create(h264g_ctx *c, h264g_pic *const *pics, uint32_t n, h264g_batch **out) {
+    if (!c || !pics || !out || n == 0) return h264g_set_error(H264G_E_ARG, "NULL argument");
+    *out = nullptr;
+    if (n >= (1u << (32 - WI_PIC_SHIFT))) return h264g_set_error(H264G_E_ARG, "too many pictures in one batch");
+    std::lock_guard<std::mutex> lk(c->mu);
+    HIPCHK(hipSetDevice(c->device));
+    std::vector<uint32_t> seen;
+    std::vector<DevPic> hp(n);
+    std::vector<uint32_t> inter, intra, dbk;
+    int maxt = 0;
+    for (uint32_t i = 0; i < n; i++) {
+        const h264g_pic *p = pics[i];
+        if (!p || p->ctx != c) return h264g_set_error(H264G_E_ARG, "picture %u", i);
+        if (std::find(seen.begin(), seen.end(), p->stream) != seen.end())
+            return h264g_set_error(H264G_E_ARG, "two pictures of stream %u in one batch", p->stream);
+        seen.push_back(p->stream);
+        hp[i] = p->hpic;
+        for (uint32_t a : p->inter_list) inter.push_back((i << WI_PIC_SHIFT) | a);
+        maxt = std::max(maxt, (p->W - 1) + 2 * (p->H - 1));
+    }
+    // Wavefront queues of all pictures merged by t = x + 2y (counting sort): every
+    // dependency of an entry has a smaller t and therefore an earlier ticket.
+    auto merge = [&](bool intra_only, std::vector<uint32_t> &outq) {
+        std::vector<std::vector<uint32_t>> byt(maxt + 1);
+        for (uint32_t i = 0; i < n; i++) {
+            const h264g_pic *p = pics[i];
+            const std::vector<uint32_t> &q = intra_only ? p->intra_t : p->all_t;
+            for (uint32_t a : q) {
+                int t = (int)(a % p->W) + 2 * (int)(a / p->W);
+                byt[t].push_back((i << WI_PIC_SHIFT) | a);
+            }
+        }
+        for (auto &v : byt) outq.insert(outq.end(), v.begin(), v.end());
+    };
+    merge(true, intra);
+    bool any_dbk = false;
+    for (uint32_t i = 0; i < n; i++) any_dbk |= pics[i]->desc.disable_deblocking_filter_idc != 1;
+    if (any_dbk) merge(false, dbk);
+    auto al = [](size_t v) { return (v + 255) & ~(size_t)255; };
+    size_t o_p = 0, o_i = al(n * sizeof(DevPic)), o_a = al(o_i + inter.size() * 4), o_d = al(o_a + intra.size() * 4);
+    size_t total = al(o_d + dbk.size() * 4 + 4);
+    h264g_batch *b = new (std::nothrow) h264g_batch();
+    if (!b) return h264g_set_error(H264G_E_NOMEM, "batch");
+    if (hipMalloc(&b->dmem, total) != hipSuccess) { delete b; return h264g_set_error(H264G_E_NOMEM, "batch device memory"); }
+    char *base = (char *)b->dmem;
+    b->d_pics = (DevPic *)(base + o_p);
+    b->d_inter = (uint32_t *)(base + o_i);
+    b->d_intra = (uint32_t *)(base + o_a);
+    b->d_dbk = (uint32_t *)(base + o_d);
+    b->n_inter = (uint32_t)inter.size();
+    b->n_intra = (uint32_t)intra.size();
+    b->n_dbk = (uint32_t)dbk.size();
+    if (hipMemcpy(b->d_pics, hp.data(), n * sizeof(DevPic), hipMemcpyHostToDevice) != hipSuccess ||
+        (!inter.empty() && hipMemcpy(b->d_inter, inter.data(), inter.size() * 4, hipMemcpyHostToDevice) != hipSuccess) ||
+        (!intra.empty() && hipMemcpy(b->d_intra, intra.data(), intra.size() * 4, hipMemcpyHostToDevice) != hipSuccess) ||
+        (!dbk.empty() && hipMemcpy(b->d_dbk, dbk.data(), dbk.size() * 4, hipMemcpyHostToDevice) != hipSuccess)) {
+        free_batch(b);
+        return h264g_set_error(H264G_E_DEVICE, "batch upload");
+    }
+    *out = b;
+    return H264G_OK;
+}
+
+int32_t h264g_batch_run(h264g_ctx *c, h264g_batch *b) {
+    if (!c || !b) return h264g_set_error(H264G_E_ARG, "NULL argument");
+    std::lock_guard<std::mutex> lk(c->mu);
+    HIPCHK(hipSetDevice(c->device));
+    return launch_stages(c, b->d_pics, b->d_inter, b->n_inter, b->d_intra, b->n_intra, b->d_dbk, b->n_dbk, b->n_dbk == 0);
+}
+
+int32_t h264g_batch_destroy(h264g_ctx *c, h264g_batch *b) {
+    if (c) {
+        (void)hipSetDevice(c->device);
+        (void)hipStreamSynchronize(c->stream);
+    }
+    return free_batch(b);
+}
+
+int32_t h264g_batch_submit(h264g_ctx *c, h264g_pic *const *pics, uint32_t n) {
+    h264g_batch *b = nullptr;
+    int rc = h264g_batch_create(c, pics, n, &b);
+    if (rc) return rc;
+    rc = h264g_batch_run(c, b);
+    std::lock_guard<std::mutex> lk(c->mu);
+    c->deferred.push_back(b);
+    return rc;
+}
+
+int32_t h264g_sync(h264g_ctx *c) {
+    if (!c) return h264g_set_error(H264G_E_ARG, "ctx");
+    std::lock_guard<std::mutex> lk(c->mu);
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (auto *b : c->deferred) free_batch(b);
+    c->deferred.clear();
+    uint32_t err = 0;
+    HIPCHK(hipMemcpy(&err, c->d_err, 4, hipMemcpyDeviceToHost));
+    if (err) {
+        HIPCHK(hipMemset(c->d_err, 0, 4));
+        return h264g_set_error(H264G_E_TIMEOUT, "device error word 0x%x (wavefront wait exceeded its spin limit)", err);
+    }
+    return H264G_OK;
+}
+
+int32_t h264g_stage_times(h264g_ctx *c, float out[3]) {
+    if (!c || !out || !c->profile) return h264g_set_error(H264G_E_STATE, "profiling is off");
+    HIPCHK(hipSetDevice(c->device));
+    for (int i = 0; i < 3; i++) HIPCHK(hipEventElapsedTime(&out[i], c->ev[i], c->ev[i + 1]));
+    return H264G_OK;
+}
+
+int32_t h264g_slot_readback(h264g_ctx *c, uint32_t sid, int32_t slot, uint8_t *y, uint8_t *cb, uint8_t *cr) {
+    if (!c || !y || !cb || !cr) return h264g_set_error(H264G_E_ARG, "NULL argument");
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (sid >= c->streams.size() || !c->streams[sid].open) return h264g_set_error(H264G_E_ARG, "stream %u", sid);
+    Stream &s = c->streams[sid];
+    if (slot < 0 || slot >= s.desc.dpb_slots) return h264g_set_error(H264G_E_ARG, "slot %d", slot);
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const uint8_t *base = s.dpb + (uint64_t)slot * s.slot_bytes;
+    size_t ny = (size_t)s.W * s.H * 256, nc = (size_t)s.W * s.H * 64;
+    HIPCHK(hipMemcpy(y, base, ny, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(cb, base + ny, nc, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(cr, base + ny + nc, nc, hipMemcpyDeviceToHost));
+    return H264G_OK;
+}
+
+int32_t h264g_slot_upload(h264g_ctx *c, uint32_t sid, int32_t slot, const uint8_t *y, const uint8_t *cb, const uint8_t *cr) {
+    if (!c || !y || !cb || !cr) return h264g_set_error(H264G_E_ARG, "NULL argument");
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (sid >= c->streams.size() || !c->streams[sid].open) return h264g_set_error(H264G_E_ARG, "stream %u", sid);
+    Stream &s = c->streams[sid];
+    if (slot < 0 || slot >= s.desc.dpb_slots) return h264g_set_error(H264G_E_ARG, "slot %d", slot);
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    uint8_t *base = s.dpb + (uint64_t)slot * s.slot_bytes;
+    size_t ny = (size_t)s.W * s.H * 256, nc = (size_t)s.W * s.H * 64;
+    HIPCHK(hipMemcpy(base, y, ny, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(base + ny, cb, nc, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(base + ny + nc, cr, nc, hipMemcpyHostToDevice));
+    return H264G_OK;
+}
+
+}  // extern "C"

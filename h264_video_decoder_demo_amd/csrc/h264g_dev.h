@@ -1,0 +1,78 @@
+// h264g_dev.h -- layouts shared by the host side of libh264g and its gfx950 kernels.
+#pragma once
+#include <stdint.h>
+#include "h264g.h"
+
+// Per-MB values the host derives once while staging a picture (picture_begin):
+// everything the kernels would otherwise recompute per sample.
+struct DevAux {
+    int8_t   refid[2][4];     // reference picture per list and 8x8 quadrant as a DPB slot,
+                              // -1 = none: the MC source when the list is used, and the
+                              // picture identity the bS derivation compares
+    uint8_t  pf;              // PredFlagLX of quadrant q at bit (4 * X + q)
+    uint8_t  kind;            // AUX_* bits below
+    uint16_t nz4;             // luma4x4BlkIdx b has non-zero levels (4x4-transform MBs)
+    uint8_t  wp[4];           // weight-table entry per quadrant
+    uint8_t  nz8;             // 8x8 block has non-zero levels (8x8-transform MBs)
+    uint8_t  nmask;           // number of level masks of the MB
+    uint8_t  qpc[2];          // QP'C of Cb / Cr for dequantisation
+    uint8_t  qdb[3];          // deblocking qP: luma, Cb edges, Cr edges
+    uint8_t  pad0;
+    uint32_t reserved;
+};
+static_assert(sizeof(DevAux) == 28, "DevAux layout");
+
+#define AUX_INTRA   0x01u  // Intra_4x4 / Intra_8x8 / Intra_16x16 prediction (bS 3/4)
+#define AUX_PCM     0x02u
+#define AUX_T8      0x04u  // transform_size_8x8_flag
+#define AUX_INTER   0x08u
+#define AUX_RESID   0x10u  // has coefficient levels
+
+// Resolved weighted-prediction parameters of one (refIdxL0, refIdxL1) combination.
+struct DevWeight {
+    int16_t mode;             // 0 default (8.4.2.3.1), 1 weighted (8.4.2.3.2)
+    int16_t logWD[3];
+    int16_t w0[3], w1[3], o0[3], o1[3];
+    // L1-only luma takes (w0, o0) -- handled by the host filling w0/o0 for such entries
+};
+
+// One picture as the kernels see it.
+struct DevPic {
+    const h264g_mb *mbs;
+    const DevAux *aux;
+    const int16_t *levels;
+    const DevWeight *wt;
+    uint8_t *dpb;             // slot 0 of the stream's DPB
+    uint64_t slot_bytes;      // bytes per DPB slot (Y + Cb + Cr)
+    uint32_t *flags;          // per-MB wavefront progress words of the stream
+    int32_t  dst;             // destination slot
+    int32_t  W, H;            // in macroblocks
+    int32_t  offA, offB;      // FilterOffsetA / FilterOffsetB
+    int32_t  dbk_idc;         // disable_deblocking_filter_idc
+    int32_t  cip;             // constrained_intra_pred_flag
+    int32_t  pad[3];
+};
+
+// Work item: picture index in the batch and macroblock address.
+#define WI_PIC_SHIFT 20u
+#define WI_MB_MASK   ((1u << WI_PIC_SHIFT) - 1u)
+
+// Control block of one persistent launch: ticket dispenser + finished-wave count.
+struct DevCtl {
+    uint32_t ticket;
+    uint32_t done;
+    uint32_t pad[14];         // own 64-byte line
+};
+
+// Device error bits (ORed into the context's error word).
+#define DEVERR_TIMEOUT  0x1u
+
+extern "C" {
+// Kernel launchers (kernels.hip); all enqueue on `stream` and return a hipError_t.
+int h264g_launch_inter(const DevPic *pics, const uint32_t *items, uint32_t n, uint32_t *err, void *stream);
+int h264g_launch_intra(const DevPic *pics, const uint32_t *items, uint32_t n, DevCtl *ctl, uint32_t epoch,
+                       uint32_t *err, int waves, void *stream);
+int h264g_launch_deblock(const DevPic *pics, const uint32_t *items, uint32_t n, DevCtl *ctl, uint32_t epoch,
+                         uint32_t *err, int waves, void *stream);
+int h264g_launch_copy(const uint8_t *src, uint8_t *dst, uint64_t bytes, void *stream);
+}

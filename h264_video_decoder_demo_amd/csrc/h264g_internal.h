@@ -3,4 +3,4 @@
 #include <stdint.h>
 
 // Records an error message for h264g_last_error() and returns `code`.
-int32_t h264g_set_error(int32_t code, const char *fmt, ...);
+extern "C" int32_t h264g_set_error(int32_t code, const char *fmt, ...);

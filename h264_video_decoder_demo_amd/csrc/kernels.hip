@@ -1,0 +1,1018 @@
+// kernels.hip -- gfx950 kernels of the H.264 picture-reconstruction engine.
+//
+// One wavefront (64 lanes) reconstructs one macroblock.  Three stages per picture,
+// in the reference's order (inter MBs never read the current picture, intra MBs read
+// unfiltered neighbours, deblocking runs after the whole picture is built,
+// H264PictureBase.cpp:707):
+//   inter   -- fully parallel over inter (and I_PCM) MBs: fractional-sample MC from
+//              the device-resident DPB + weighted prediction + residual.
+//   intra   -- persistent wavefront over intra MBs: each wave takes the next ticket of a
+//              queue ordered by t = x + 2y, waits on its A/B/C/D neighbours' progress
+//              words, reconstructs, publishes its own word.
+//   deblock -- the same wavefront scheme over all MBs with the raster-order edge filter.
+// The queues hold MBs of any number of pictures (batching across streams).
+#include <hip/hip_runtime.h>
+#include "h264g_dev.h"
+
+namespace {
+
+constexpr int WAVE = 64;
+
+// ------------------------------------------------------------------ tables
+// Table 8-13 / 8-14 zig-zag (scan position -> raster index), H264PictureBase.cpp:4542-4744
+__constant__ uint8_t c_zz4[16] = {0, 1, 4, 8, 5, 2, 3, 6, 9, 12, 13, 10, 7, 11, 14, 15};
+__constant__ uint8_t c_zz8[64] = {
+    0, 1, 8, 16, 9, 2, 3, 10, 17, 24, 32, 25, 18, 11, 4, 5, 12, 19, 26, 33, 40, 48, 41, 34, 27, 20, 13, 6, 7, 14, 21, 28,
+    35, 42, 49, 56, 57, 50, 43, 36, 29, 22, 15, 23, 30, 37, 44, 51, 58, 59, 52, 45, 38, 31, 39, 46, 53, 60, 61, 54, 47, 55, 62, 63};
+// normAdjust values v (8-315, 8-318), H264PictureBase.cpp:4900-4950
+__constant__ uint8_t c_v4[6][3] = {{10, 16, 13}, {11, 18, 14}, {13, 20, 16}, {14, 23, 18}, {16, 25, 20}, {18, 29, 23}};
+__constant__ uint8_t c_v8[6][6] = {{20, 18, 32, 19, 25, 24}, {22, 19, 35, 21, 28, 26}, {26, 23, 42, 24, 33, 31},
+                                   {28, 25, 45, 26, 35, 33}, {32, 28, 51, 30, 40, 38}, {36, 32, 58, 34, 46, 43}};
+// Tables 8-16 / 8-17, Deblock.cpp:1330-1409
+__constant__ uint8_t c_alpha[52] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 4, 4, 5, 6, 7, 8, 9, 10, 12, 13,
+                                    15, 17, 20, 22, 25, 28, 32, 36, 40, 45, 50, 56, 63, 71, 80, 90, 101, 113, 127, 144, 162, 182, 203, 226, 255, 255};
+__constant__ uint8_t c_beta[52] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 2, 2, 2, 3, 3, 3, 3, 4, 4, 4,
+                                   6, 6, 7, 7, 8, 8, 9, 9, 10, 10, 11, 11, 12, 12, 13, 13, 14, 14, 15, 15, 16, 16, 17, 17, 18, 18};
+__constant__ uint8_t c_tc0[3][52] = {
+    {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 2, 2, 2, 2, 3, 3, 3, 4, 4, 4, 5, 6, 6, 7, 8, 9, 10, 11, 13},
+    {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 2, 2, 2, 2, 3, 3, 3, 4, 4, 5, 5, 6, 7, 8, 8, 10, 11, 12, 13, 15, 17},
+    {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 2, 2, 2, 2, 3, 3, 3, 4, 4, 4, 5, 6, 6, 7, 8, 9, 10, 11, 13, 14, 16, 18, 20, 23, 25}};
+
+__device__ __forceinline__ int clip3(int lo, int hi, int v) { return v < lo ? lo : (v > hi ? hi : v); }
+__device__ __forceinline__ int clip1(int v) { return clip3(0, 255, v); }
+__device__ __forceinline__ int iabs(int v) { return v < 0 ? -v : v; }
+__device__ __forceinline__ int tap6(int a, int b, int c, int d, int e, int f) { return a - 5 * b + 20 * c + 20 * d - 5 * e + f; }
+
+__device__ __forceinline__ int ls4(int m, int i, int j) {
+    int cls = ((i | j) & 1) == 0 ? 0 : (((i & j) & 1) ? 1 : 2);
+    return 16 * c_v4[m][cls];
+}
+__device__ __forceinline__ int ls8(int m, int i, int j) {
+    int cls;
+    if ((i & 3) == 0 && (j & 3) == 0) cls = 0;
+    else if ((i & 1) && (j & 1)) cls = 1;
+    else if ((i & 3) == 2 && (j & 3) == 2) cls = 2;
+    else if (((i & 3) == 0 && (j & 1)) || ((i & 1) && (j & 3) == 0)) cls = 3;
+    else if (((i & 3) == 0 && (j & 3) == 2) || ((i & 3) == 2 && (j & 3) == 0)) cls = 4;
+    else cls = 5;
+    return 16 * c_v8[m][cls];
+}
+
+// luma4x4BlkIdx -> (x, y) in the MB (6.4.3)
+__device__ __forceinline__ int blk_x(int b) { return ((b >> 2) & 1) * 8 + (b & 1) * 4; }
+__device__ __forceinline__ int blk_y(int b) { return ((b >> 3) & 1) * 8 + ((b >> 1) & 1) * 4; }
+
+__device__ __forceinline__ int wave_lane() { return threadIdx.x & (WAVE - 1); }
+
+// ------------------------------------------------------------ residuals
+// LDS scratch of one wave: dense scan-order levels of the MB (slots: 0 = Intra16x16 DC,
+// 1 + luma4x4BlkIdx (8x8 block q = slots 1+4q..4+4q), 17/18 = chroma DC Cb/Cr,
+// 19 + 4c + b = chroma AC), then the residual planes.
+struct WaveRes {
+    int32_t coef[27][16];
+    int16_t rl[256];      // luma residual, raster 16x16
+    int16_t rc[2][64];    // chroma residual, raster 8x8
+};
+
+__device__ int level_slot(int i, int i16, int cbp, int t8) {
+    int idx = 0;
+    if (i16) { if (i == idx) return 0; idx++; }
+    for (int q = 0; q < 4; q++)
+        if (cbp & (1 << q))
+            for (int b = 0; b < 4; b++) { if (i == idx) return 1 + 4 * q + b; idx++; }
+    int cc = cbp >> 4;
+    if (cc >= 1) for (int c = 0; c < 2; c++) { if (i == idx) return 17 + c; idx++; }
+    if (cc == 2) for (int k = 0; k < 8; k++) { if (i == idx) return 19 + k; idx++; }
+    (void)t8;
+    return -1;
+}
+
+// 8.5.12 on one lane: scan-order c -> residual r (raster), dc_pass keeps c[0] as is.
+__device__ void residual4x4(const int32_t *scan, int qp, int dc_pass, int32_t r[16]) {
+    int32_t d[16];
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+        int pos = c_zz4[k], i = pos >> 2, j = pos & 3;
+        int32_t v = scan[k];
+        int32_t o;
+        if (k == 0 && dc_pass) o = v;
+        else if (qp >= 24) o = (v * ls4(qp % 6, i, j)) << (qp / 6 - 4);
+        else o = (v * ls4(qp % 6, i, j) + (1 << (3 - qp / 6))) >> (4 - qp / 6);
+        d[pos] = o;
+    }
+    int32_t f[16];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        int32_t e0 = d[i * 4] + d[i * 4 + 2], e1 = d[i * 4] - d[i * 4 + 2];
+        int32_t e2 = (d[i * 4 + 1] >> 1) - d[i * 4 + 3], e3 = d[i * 4 + 1] + (d[i * 4 + 3] >> 1);
+        f[i * 4] = e0 + e3; f[i * 4 + 1] = e1 + e2; f[i * 4 + 2] = e1 - e2; f[i * 4 + 3] = e0 - e3;
+    }
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        int32_t g0 = f[j] + f[8 + j], g1 = f[j] - f[8 + j], g2 = (f[4 + j] >> 1) - f[12 + j], g3 = f[4 + j] + (f[12 + j] >> 1);
+        r[j] = (g0 + g3 + 32) >> 6; r[4 + j] = (g1 + g2 + 32) >> 6;
+        r[8 + j] = (g1 - g2 + 32) >> 6; r[12 + j] = (g0 - g3 + 32) >> 6;
+    }
+}
+
+__device__ __forceinline__ void idct8(int32_t *v, int s) {  // 8.5.13.2, in place, stride s
+    int32_t d0 = v[0], d1 = v[s], d2 = v[2 * s], d3 = v[3 * s], d4 = v[4 * s], d5 = v[5 * s], d6 = v[6 * s], d7 = v[7 * s];
+    int32_t e0 = d0 + d4, e1 = -d3 + d5 - d7 - (d7 >> 1), e2 = d0 - d4, e3 = d1 + d7 - d3 - (d3 >> 1);
+    int32_t e4 = (d2 >> 1) - d6, e5 = -d1 + d7 + d5 + (d5 >> 1), e6 = d2 + (d6 >> 1), e7 = d3 + d5 + d1 + (d1 >> 1);
+    int32_t f0 = e0 + e6, f1 = e1 + (e7 >> 2), f2 = e2 + e4, f3 = e3 + (e5 >> 2);
+    int32_t f4 = e2 - e4, f5 = (e3 >> 2) - e5, f6 = e0 - e6, f7 = e7 - (e1 >> 2);
+    v[0] = f0 + f7; v[s] = f2 + f5; v[2 * s] = f4 + f3; v[3 * s] = f6 + f1;
+    v[4 * s] = f6 - f1; v[5 * s] = f4 - f3; v[6 * s] = f2 - f5; v[7 * s] = f0 - f7;
+}
+
+// Expands the MB's levels into ws->coef and computes all residual blocks into ws->rl /
+// ws->rc.  Must be called by the whole wave.  Returns false when the MB has no levels.
+__device__ bool build_residual(const h264g_mb *mb, const DevAux &ax, const int16_t *levels, WaveRes *ws) {
+    const int lane = wave_lane();
+    if (!(ax.kind & AUX_RESID)) return false;
+    const int cbp = mb->cbp;
+    const int i16 = mb->mb_type == H264G_MB_I16x16;
+    const int t8 = (ax.kind & AUX_T8) ? 1 : 0;
+    const int qp = mb->qp_y;
+    int32_t *cf = &ws->coef[0][0];
+    for (int k = lane; k < 27 * 16; k += WAVE) cf[k] = 0;
+    __syncthreads();
+    const int16_t *lv = levels + mb->level_offset;
+    const int nmask = ax.nmask;
+    int mask = lane < nmask ? (uint16_t)lv[lane] : 0;
+    int cnt = __popc(mask);
+    int pre = cnt;  // inclusive prefix sum over lanes
+#pragma unroll
+    for (int o = 1; o < 32; o <<= 1) {
+        int t = __shfl_up(pre, o, WAVE);
+        if (lane >= o) pre += t;
+    }
+    pre -= cnt;
+    if (lane < nmask) {
+        int slot = level_slot(lane, i16, cbp, t8);
+        const int16_t *vals = lv + nmask + pre;
+        int n = 0;
+        while (mask) {
+            int k = __ffs(mask) - 1;
+            mask &= mask - 1;
+            ws->coef[slot][k] = vals[n++];
+        }
+    }
+    __syncthreads();
+    if (lane < 16) {
+        if (t8) {
+            if (lane < 4) {  // one lane per 8x8 block
+                int q = lane;
+                int32_t m[64];
+                const int32_t *src = &ws->coef[1 + 4 * q][0];
+#pragma unroll
+                for (int k = 0; k < 64; k++) {
+                    int pos = c_zz8[k], i = pos >> 3, j = pos & 7;
+                    int32_t v = src[k];
+                    m[pos] = qp >= 36 ? (v * ls8(qp % 6, i, j)) << (qp / 6 - 6)
+                                      : (v * ls8(qp % 6, i, j) + (1 << (5 - qp / 6))) >> (6 - qp / 6);
+                }
+#pragma unroll
+                for (int i = 0; i < 8; i++) idct8(&m[i * 8], 1);
+#pragma unroll
+                for (int j = 0; j < 8; j++) idct8(&m[j], 8);
+                int x0 = (q & 1) * 8, y0 = (q >> 1) * 8;
+#pragma unroll
+                for (int k = 0; k < 64; k++) ws->rl[(y0 + (k >> 3)) * 16 + x0 + (k & 7)] = (int16_t)((m[k] + 32) >> 6);
+            }
+        } else {
+            int b = lane;
+            int32_t scan[16];
+#pragma unroll
+            for (int k = 0; k < 16; k++) scan[k] = ws->coef[1 + b][k];
+            int dcpass = 0;
+            if (i16) {  // 8.5.10: 4x4 Hadamard of the DC levels, then scaling
+                int32_t c[16], g[16];
+#pragma unroll
+                for (int k = 0; k < 16; k++) c[c_zz4[k]] = ws->coef[0][k];
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    int32_t a = c[j], bb = c[4 + j], cc = c[8 + j], dd = c[12 + j];
+                    g[j] = a + bb + cc + dd; g[4 + j] = a + bb - cc - dd; g[8 + j] = a - bb - cc + dd; g[12 + j] = a - bb + cc - dd;
+                }
+                int row = blk_y(b) >> 2, col = blk_x(b) >> 2;
+                int32_t a = g[row * 4], bb = g[row * 4 + 1], cc = g[row * 4 + 2], dd = g[row * 4 + 3];
+                int32_t f = col == 0 ? a + bb + cc + dd : col == 1 ? a + bb - cc - dd : col == 2 ? a - bb - cc + dd : a - bb + cc - dd;
+                int s = ls4(qp % 6, 0, 0);
+                scan[0] = qp >= 36 ? (f * s) << (qp / 6 - 6) : (f * s + (1 << (5 - qp / 6))) >> (6 - qp / 6);
+                dcpass = 1;
+            }
+            int32_t r[16];
+            residual4x4(scan, qp, dcpass, r);
+            int x0 = blk_x(b), y0 = blk_y(b);
+#pragma unroll
+            for (int k = 0; k < 16; k++) ws->rl[(y0 + (k >> 2)) * 16 + x0 + (k & 3)] = (int16_t)r[k];
+        }
+    } else if (lane < 24) {  // chroma: 8.5.11 DC then 8.5.12 per 4x4 block
+        int c = (lane - 16) >> 2, b = (lane - 16) & 3;
+        int qpc = ax.qpc[c];
+        const int32_t *dc = ws->coef[17 + c];
+        int32_t e00 = dc[0] + dc[2], e01 = dc[1] + dc[3], e10 = dc[0] - dc[2], e11 = dc[1] - dc[3];
+        int32_t f = b == 0 ? e00 + e01 : b == 1 ? e00 - e01 : b == 2 ? e10 + e11 : e10 - e11;
+        int32_t scan[16];
+#pragma unroll
+        for (int k = 0; k < 16; k++) scan[k] = ws->coef[19 + 4 * c + b][k];
+        scan[0] = ((f * ls4(qpc % 6, 0, 0)) << (qpc / 6)) >> 5;
+        int32_t r[16];
+        residual4x4(scan, qpc, 1, r);
+        int x0 = (b & 1) * 4, y0 = (b >> 1) * 4;
+#pragma unroll
+        for (int k = 0; k < 16; k++) ws->rc[c][(y0 + (k >> 2)) * 8 + x0 + (k & 3)] = (int16_t)r[k];
+    }
+    __syncthreads();
+    return true;
+}
+
+// ------------------------------------------------------------ inter MC
+__device__ __forceinline__ int ldc(const uint8_t *p, int w, int h, int x, int y) {
+    return p[clip3(0, h - 1, y) * w + clip3(0, w - 1, x)];
+}
+
+// Four horizontally adjacent luma prediction samples at integer position (x0, y0) with
+// fractional offset (xF, yF), 8.4.2.2.1 (H264InterPrediction.cpp:2344-2480).
+__device__ void luma4(const uint8_t *ref, int w, int h, int x0, int y0, int xF, int yF, int out[4]) {
+    if (xF == 0 && yF == 0) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) out[i] = ldc(ref, w, h, x0 + i, y0);
+        return;
+    }
+    int win[6][9];
+#pragma unroll
+    for (int r = 0; r < 6; r++)
+#pragma unroll
+        for (int c = 0; c < 9; c++) win[r][c] = ldc(ref, w, h, x0 - 2 + c, y0 - 2 + r);
+    if (yF == 0) {  // a, b, c
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            int b = clip1((tap6(win[2][i], win[2][i + 1], win[2][i + 2], win[2][i + 3], win[2][i + 4], win[2][i + 5]) + 16) >> 5);
+            out[i] = xF == 2 ? b : (xF == 1 ? (win[2][i + 2] + b + 1) >> 1 : (win[2][i + 3] + b + 1) >> 1);
+        }
+        return;
+    }
+    if (xF == 0) {  // d, h, n
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            int hh = clip1((tap6(win[0][i + 2], win[1][i + 2], win[2][i + 2], win[3][i + 2], win[4][i + 2], win[5][i + 2]) + 16) >> 5);
+            out[i] = yF == 2 ? hh : (yF == 1 ? (win[2][i + 2] + hh + 1) >> 1 : (win[3][i + 2] + hh + 1) >> 1);
+        }
+        return;
+    }
+    // both fractional: b (row 0), s (row 1), h (col 0), m (col 1), j
+    int v[5];  // clipped vertical half samples h at columns i = 0..4
+#pragma unroll
+    for (int i = 0; i < 5; i++)
+        v[i] = clip1((tap6(win[0][i + 2], win[1][i + 2], win[2][i + 2], win[3][i + 2], win[4][i + 2], win[5][i + 2]) + 16) >> 5);
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        int b = clip1((tap6(win[2][i], win[2][i + 1], win[2][i + 2], win[2][i + 3], win[2][i + 4], win[2][i + 5]) + 16) >> 5);
+        int s = clip1((tap6(win[3][i], win[3][i + 1], win[3][i + 2], win[3][i + 3], win[3][i + 4], win[3][i + 5]) + 16) >> 5);
+        int hh = v[i], m = v[i + 1];
+        int o;
+        if ((xF & 1) && (yF & 1)) {  // e, g, p, r
+            int hv = xF == 1 ? hh : m;
+            int bs = yF == 1 ? b : s;
+            o = (bs + hv + 1) >> 1;
+        } else {
+            int j1 = 0;
+#pragma unroll
+            for (int r = 0; r < 6; r++) {
+                int b1 = tap6(win[r][i], win[r][i + 1], win[r][i + 2], win[r][i + 3], win[r][i + 4], win[r][i + 5]);
+                const int wgt = r == 0 || r == 5 ? 1 : (r == 1 || r == 4 ? -5 : 20);
+                j1 += wgt * b1;
+            }
+            int j = clip1((j1 + 512) >> 10);
+            if (xF == 2 && yF == 2) o = j;
+            else if (xF == 2) o = (j + (yF == 1 ? b : s) + 1) >> 1;  // f, q
+            else o = (j + (xF == 1 ? hh : m) + 1) >> 1;                // i, k
+        }
+        out[i] = o;
+    }
+}
+
+// 8.4.2.3.1 / 8.4.2.3.2 (H264InterPrediction.cpp:2526-2829); the host resolved the
+// weights, including the L1-only luma entries of quirk Q2, into w0/o0.
+__device__ __forceinline__ int weigh(const DevWeight &w, int c, int pf0, int pf1, int p0, int p1) {
+    if (!w.mode) {
+        if (pf0 && pf1) return (p0 + p1 + 1) >> 1;
+        return pf0 ? p0 : p1;
+    }
+    int lw = w.logWD[c];
+    if (pf0 && pf1) return clip1(((p0 * w.w0[c] + p1 * w.w1[c] + (1 << lw)) >> (lw + 1)) + ((w.o0[c] + w.o1[c] + 1) >> 1));
+    int p = pf0 ? p0 : p1;
+    int ww = (pf0 || c == 0) ? w.w0[c] : w.w1[c];
+    int oo = (pf0 || c == 0) ? w.o0[c] : w.o1[c];
+    if (lw >= 1) return clip1(((p * ww + (1 << (lw - 1))) >> lw) + oo);
+    return clip1(p * ww + oo);
+}
+
+__device__ __forceinline__ uint8_t *slot_base(const DevPic &P, int s) { return P.dpb + (uint64_t)s * P.slot_bytes; }
+
+// Inter MB (or I_PCM copy): prediction + residual, written to the destination slot.
+__device__ void inter_mb(const DevPic &P, int addr, WaveRes *ws) {
+    const int lane = wave_lane();
+    const h264g_mb *mb = &P.mbs[addr];
+    const DevAux ax = P.aux[addr];
+    const int W = P.W, H = P.H, wl = W * 16, hl = H * 16, wc = W * 8, hc = H * 8;
+    const int mbx = addr % W, mby = addr / W;
+    uint8_t *dst = slot_base(P, P.dst);
+    uint8_t *dy = dst, *dcb = dst + (uint64_t)wl * hl, *dcr = dcb + (uint64_t)wc * hc;
+    if (ax.kind & AUX_PCM) {  // 8.3.5 (H264PictureBase.cpp:2449-2499)
+        const int16_t *pcm = P.levels + mb->level_offset;
+        for (int i = lane; i < 256; i += WAVE) dy[(mby * 16 + (i >> 4)) * wl + mbx * 16 + (i & 15)] = (uint8_t)pcm[i];
+        dcb[(mby * 8 + (lane >> 3)) * wc + mbx * 8 + (lane & 7)] = (uint8_t)pcm[256 + lane];
+        dcr[(mby * 8 + (lane >> 3)) * wc + mbx * 8 + (lane & 7)] = (uint8_t)pcm[320 + lane];
+        return;
+    }
+    const bool res = build_residual(mb, ax, P.levels, ws);
+    // ---- luma: lane -> 4x4 block b (raster), row r
+    {
+        int b = lane >> 2, r = lane & 3, bx = b & 3, by = b >> 2, q = (by >> 1) * 2 + (bx >> 1);
+        int pf0 = (ax.pf >> q) & 1, pf1 = (ax.pf >> (4 + q)) & 1;
+        int pl[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+        int xAL = mbx * 16 + bx * 4, yAL = mby * 16 + by * 4 + r;
+        for (int l = 0; l < 2; l++) {
+            if (!((l == 0) ? pf0 : pf1)) continue;
+            const uint8_t *ref = slot_base(P, ax.refid[l][q]);
+            int mvx = mb->mv[l][b][0], mvy = mb->mv[l][b][1];
+            luma4(ref, wl, hl, xAL + (mvx >> 2), yAL + (mvy >> 2), mvx & 3, mvy & 3, pl[l]);
+        }
+        const DevWeight w = P.wt[ax.wp[q]];
+        uint32_t packed = 0;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            int v = weigh(w, 0, pf0, pf1, pl[0][i], pl[1][i]);
+            if (res) v = clip1(v + ws->rl[(by * 4 + r) * 16 + bx * 4 + i]);
+            packed |= (uint32_t)v << (8 * i);
+        }
+        *(uint32_t *)&dy[(uint64_t)yAL * wl + xAL] = packed;
+    }
+    // ---- chroma: lane -> component c, 4x4 luma block b, chroma row r (2 samples)
+    {
+        int c = lane >> 5, i = lane & 31, b = i >> 1, r = i & 1, bx = b & 3, by = b >> 2, q = (by >> 1) * 2 + (bx >> 1);
+        int pf0 = (ax.pf >> q) & 1, pf1 = (ax.pf >> (4 + q)) & 1;
+        int pc[2][2] = {{0, 0}, {0, 0}};
+        int xC = mbx * 8 + bx * 2, yC = mby * 8 + by * 2 + r;
+        for (int l = 0; l < 2; l++) {
+            if (!((l == 0) ? pf0 : pf1)) continue;
+            const uint8_t *ref = slot_base(P, ax.refid[l][q]) + (uint64_t)wl * hl + (uint64_t)c * wc * hc;
+            int mvx = mb->mv[l][b][0], mvy = mb->mv[l][b][1];
+            int xi = xC + (mvx >> 3), yi = yC + (mvy >> 3), xF = mvx & 7, yF = mvy & 7;
+            int A0 = ldc(ref, wc, hc, xi, yi), A1 = ldc(ref, wc, hc, xi + 1, yi), A2 = ldc(ref, wc, hc, xi + 2, yi);
+            int B0 = ldc(ref, wc, hc, xi, yi + 1), B1 = ldc(ref, wc, hc, xi + 1, yi + 1), B2 = ldc(ref, wc, hc, xi + 2, yi + 1);
+            pc[l][0] = ((8 - xF) * (8 - yF) * A0 + xF * (8 - yF) * A1 + (8 - xF) * yF * B0 + xF * yF * B1 + 32) >> 6;
+            pc[l][1] = ((8 - xF) * (8 - yF) * A1 + xF * (8 - yF) * A2 + (8 - xF) * yF * B1 + xF * yF * B2 + 32) >> 6;
+        }
+        const DevWeight w = P.wt[ax.wp[q]];
+        uint8_t *plane = c == 0 ? dcb : dcr;
+        int v0 = weigh(w, 1 + c, pf0, pf1, pc[0][0], pc[1][0]);
+        int v1 = weigh(w, 1 + c, pf0, pf1, pc[0][1], pc[1][1]);
+        if (res) {
+            v0 = clip1(v0 + ws->rc[c][(by * 2 + r) * 8 + bx * 2]);
+            v1 = clip1(v1 + ws->rc[c][(by * 2 + r) * 8 + bx * 2 + 1]);
+        }
+        *(uint16_t *)&plane[(uint64_t)yC * wc + xC] = (uint16_t)(v0 | (v1 << 8));
+    }
+}
+
+// ------------------------------------------------------------ wavefront helpers
+__device__ __forceinline__ uint32_t ld_acquire(const uint32_t *p) {
+    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_release(uint32_t *p, uint32_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+constexpr uint32_t SPIN_LIMIT = 1u << 20;
+
+// Lane 0 waits until flags[i] == epoch for each listed neighbour.  Bounded: after
+// SPIN_LIMIT polls, or as soon as another wave has raised the error word, the wave
+// stops waiting (its output is then wrong, and h264g_sync reports the error).
+__device__ void wait_deps(const uint32_t *flags, const int *deps, int nd, uint32_t epoch, uint32_t *err) {
+    if (wave_lane() == 0) {
+        for (int k = 0; k < nd; k++) {
+            if (deps[k] < 0) continue;
+            uint32_t it = 0;
+            while (ld_acquire(&flags[deps[k]]) != epoch) {
+                __builtin_amdgcn_s_sleep(1);
+                ++it;
+                if (it > SPIN_LIMIT) { atomicOr(err, DEVERR_TIMEOUT); break; }
+                if ((it & 255u) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) break;
+            }
+        }
+    }
+    __syncthreads();
+    __atomic_thread_fence(__ATOMIC_ACQUIRE);
+}
+
+__device__ void publish(uint32_t *flag, uint32_t epoch) {
+    __syncthreads();
+    __threadfence();
+    if (wave_lane() == 0) st_release(flag, epoch);
+}
+
+// Next ticket of a persistent launch (whole wave), or -1.
+__device__ int next_ticket(DevCtl *ctl, uint32_t n) {
+    uint32_t t = 0;
+    if (wave_lane() == 0) t = atomicAdd(&ctl->ticket, 1u);
+    t = __shfl(t, 0, WAVE);
+    return t < n ? (int)t : -1;
+}
+
+// The last wave of a launch resets the control block for the next launch on the stream.
+__device__ void retire(DevCtl *ctl, uint32_t total_waves) {
+    if (wave_lane() == 0) {
+        uint32_t d = atomicAdd(&ctl->done, 1u);
+        if (d == total_waves - 1) {
+            __hip_atomic_store(&ctl->ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&ctl->done, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+}
+
+__device__ __forceinline__ bool cls_inter(int c) { return c == H264G_MB_P || c == H264G_MB_P_SKIP || c == H264G_MB_B; }
+__device__ __forceinline__ bool cls_intra(int c) { return c == H264G_MB_I4x4 || c == H264G_MB_I8x8 || c == H264G_MB_I16x16; }
+
+// ------------------------------------------------------------ intra (8.3)
+// LDS tile of one wave: luma T[y+1][x+1] for x in [-1, 23], y in [-1, 15] (row -1 holds
+// the top and top-right neighbours, column -1 the left ones), chroma C[c][y+1][x+1] for
+// x, y in [-1, 7]; availability of the neighbouring MBs A, B, C, D.
+struct WaveIntra {
+    WaveRes res;
+    uint8_t T[17][25];
+    uint8_t C[2][9][9];
+    int16_t pf[25];        // filtered 8x8 reference samples p'[-1,-1..7] then p'[0..15,-1]
+};
+
+struct Avail { int A, B, C, D; };
+
+// Sample p[x, y] relative to the 4x4 / 8x8 block at (xO, yO) of the current MB, from the
+// tile, or -1 when not available for intra prediction (6.4.12.1, H264PictureBase.cpp:1099-1155).
+__device__ __forceinline__ int tile_p(const WaveIntra *wi, const Avail &av, int xO, int yO, int x, int y, int blk4) {
+    int xx = xO + x, yy = yO + y;
+    if (yy < 0) {
+        if (xx < 0) return av.D ? wi->T[0][0] : -1;
+        if (xx < 16) return av.B ? wi->T[0][xx + 1] : -1;
+        return av.C ? wi->T[0][xx + 1] : -1;
+    }
+    if (xx < 0) return av.A ? wi->T[yy + 1][0] : -1;
+    if (xx >= 16) return -1;
+    if (blk4 >= 0 && x > 3 && (blk4 == 3 || blk4 == 11)) return -1;  // H264PictureBase.cpp:1131
+    return wi->T[yy + 1][xx + 1];
+}
+
+// 8.3.1.2 for pixel (x, y) of the 4x4 block blk (H264PictureBase.cpp:1062-1419).
+__device__ int pred4x4(const WaveIntra *wi, const Avail &av, int blk, int mode, int x, int y) {
+    int xO = blk_x(blk), yO = blk_y(blk);
+    int top[8], left[4], tl = tile_p(wi, av, xO, yO, -1, -1, blk);
+#pragma unroll
+    for (int i = 0; i < 8; i++) top[i] = tile_p(wi, av, xO, yO, i, -1, blk);
+#pragma unroll
+    for (int i = 0; i < 4; i++) left[i] = tile_p(wi, av, xO, yO, -1, i, blk);
+    if (top[4] < 0 && top[5] < 0 && top[6] < 0 && top[7] < 0 && top[3] >= 0) top[4] = top[5] = top[6] = top[7] = top[3];
+    int ta = top[0] >= 0 && top[1] >= 0 && top[2] >= 0 && top[3] >= 0;
+    int la = left[0] >= 0 && left[1] >= 0 && left[2] >= 0 && left[3] >= 0;
+#define PT(i) ((i) < 0 ? tl : top[i])
+#define PL(i) ((i) < 0 ? tl : left[i])
+    switch (mode) {
+    case 0: return top[x];
+    case 1: return left[y];
+    case 2:
+        if (ta && la) return (top[0] + top[1] + top[2] + top[3] + left[0] + left[1] + left[2] + left[3] + 4) >> 3;
+        if (la) return (left[0] + left[1] + left[2] + left[3] + 2) >> 2;
+        if (ta) return (top[0] + top[1] + top[2] + top[3] + 2) >> 2;
+        return 128;
+    case 3:
+        if (x == 3 && y == 3) return (top[6] + 3 * top[7] + 2) >> 2;
+        return (top[x + y] + 2 * top[x + y + 1] + top[x + y + 2] + 2) >> 2;
+    case 4:
+        if (x > y) return (PT(x - y - 2) + 2 * PT(x - y - 1) + top[x - y] + 2) >> 2;
+        if (x < y) return (PL(y - x - 2) + 2 * PL(y - x - 1) + left[y - x] + 2) >> 2;
+        return (top[0] + 2 * tl + left[0] + 2) >> 2;
+    case 5: {
+        int z = 2 * x - y;
+        if (z >= 0 && !(z & 1)) return (PT(x - (y >> 1) - 1) + top[x - (y >> 1)] + 1) >> 1;
+        if (z >= 0) return (PT(x - (y >> 1) - 2) + 2 * PT(x - (y >> 1) - 1) + top[x - (y >> 1)] + 2) >> 2;
+        if (z == -1) return (left[0] + 2 * tl + top[0] + 2) >> 2;
+        return (PL(y - 1) + 2 * PL(y - 2) + PL(y - 3) + 2) >> 2;
+    }
+    case 6: {
+        int z = 2 * y - x;
+        if (z >= 0 && !(z & 1)) return (PL(y - (x >> 1) - 1) + left[y - (x >> 1)] + 1) >> 1;
+        if (z >= 0) return (PL(y - (x >> 1) - 2) + 2 * PL(y - (x >> 1) - 1) + left[y - (x >> 1)] + 2) >> 2;
+        if (z == -1) return (left[0] + 2 * tl + top[0] + 2) >> 2;
+        return (PT(x - 1) + 2 * PT(x - 2) + PT(x - 3) + 2) >> 2;
+    }
+    case 7:
+        if (!(y & 1)) return (top[x + (y >> 1)] + top[x + (y >> 1) + 1] + 1) >> 1;
+        return (top[x + (y >> 1)] + 2 * top[x + (y >> 1) + 1] + top[x + (y >> 1) + 2] + 2) >> 2;
+    default: {
+        int z = x + 2 * y;
+        if (z < 5 && !(z & 1)) return (left[y + (x >> 1)] + left[y + (x >> 1) + 1] + 1) >> 1;
+        if (z < 5) return (left[y + (x >> 1)] + 2 * left[y + (x >> 1) + 1] + left[y + (x >> 1) + 2] + 2) >> 2;
+        if (z == 5) return (left[2] + 3 * left[3] + 2) >> 2;
+        return left[3];
+    }
+    }
+#undef PT
+#undef PL
+}
+
+// 8.3.2.2 for pixel (x, y) of the 8x8 block blk using the filtered samples in wi->pf:
+// pf[0] = p'[-1,-1], pf[1 + y] = p'[-1, y], pf[9 + x] = p'[x, -1] (H264PictureBase.cpp:1601-1843).
+__device__ int pred8x8(const WaveIntra *wi, int mode, int x, int y) {
+    const int16_t *pf = wi->pf;
+#define FT(i) ((i) < 0 ? pf[0] : pf[9 + (i)])
+#define FL(i) ((i) < 0 ? pf[0] : pf[1 + (i)])
+    int ta = pf[9] >= 0, la = pf[1] >= 0;  // all-or-nothing after filtering
+    switch (mode) {
+    case 0: return FT(x);
+    case 1: return FL(y);
+    case 2: {
+        int st = 0, sl = 0;
+        for (int k = 0; k < 8; k++) { st += FT(k); sl += FL(k); }
+        if (ta && la) return (st + sl + 8) >> 4;
+        if (la) return (sl + 4) >> 3;
+        if (ta) return (st + 4) >> 3;
+        return 128;
+    }
+    case 3:
+        if (x == 7 && y == 7) return (FT(14) + 3 * FT(15) + 2) >> 2;
+        return (FT(x + y) + 2 * FT(x + y + 1) + FT(x + y + 2) + 2) >> 2;
+    case 4:
+        if (x > y) return (FT(x - y - 2) + 2 * FT(x - y - 1) + FT(x - y) + 2) >> 2;
+        if (x < y) return (FL(y - x - 2) + 2 * FL(y - x - 1) + FL(y - x) + 2) >> 2;
+        return (FT(0) + 2 * pf[0] + FL(0) + 2) >> 2;
+    case 5: {
+        int z = 2 * x - y;
+        if (z >= 0 && !(z & 1)) return (FT(x - (y >> 1) - 1) + FT(x - (y >> 1)) + 1) >> 1;
+        if (z >= 0) return (FT(x - (y >> 1) - 2) + 2 * FT(x - (y >> 1) - 1) + FT(x - (y >> 1)) + 2) >> 2;
+        if (z == -1) return (FL(0) + 2 * pf[0] + FT(0) + 2) >> 2;
+        return (FL(y - 2 * x - 1) + 2 * FL(y - 2 * x - 2) + FL(y - 2 * x - 3) + 2) >> 2;
+    }
+    case 6: {
+        int z = 2 * y - x;
+        if (z >= 0 && !(z & 1)) return (FL(y - (x >> 1) - 1) + FL(y - (x >> 1)) + 1) >> 1;
+        if (z >= 0) return (FL(y - (x >> 1) - 2) + 2 * FL(y - (x >> 1) - 1) + FL(y - (x >> 1)) + 2) >> 2;
+        if (z == -1) return (FL(0) + 2 * pf[0] + FT(0) + 2) >> 2;
+        return (FT(x - 2 * y - 1) + 2 * FT(x - 2 * y - 2) + FT(x - 2 * y - 3) + 2) >> 2;
+    }
+    case 7:
+        if (!(y & 1)) return (FT(x + (y >> 1)) + FT(x + (y >> 1) + 1) + 1) >> 1;
+        return (FT(x + (y >> 1)) + 2 * FT(x + (y >> 1) + 1) + FT(x + (y >> 1) + 2) + 2) >> 2;
+    default: {
+        int z = x + 2 * y;
+        if (z < 13 && !(z & 1)) return (FL(y + (x >> 1)) + FL(y + (x >> 1) + 1) + 1) >> 1;
+        if (z < 13) return (FL(y + (x >> 1)) + 2 * FL(y + (x >> 1) + 1) + FL(y + (x >> 1) + 2) + 2) >> 2;
+        if (z == 13) return (FL(6) + 3 * FL(7) + 2) >> 2;
+        return FL(7);
+    }
+    }
+#undef FT
+#undef FL
+}
+
+__device__ void intra_mb(const DevPic &P, int addr, WaveIntra *wi, const Avail &av) {
+    const int lane = wave_lane();
+    const h264g_mb *mb = &P.mbs[addr];
+    const DevAux ax = P.aux[addr];
+    const int W = P.W, H = P.H, wl = W * 16, hl = H * 16, wc = W * 8, hc = H * 8;
+    const int mbx = addr % W, mby = addr / W;
+    uint8_t *dst = slot_base(P, P.dst);
+    uint8_t *dy = dst, *dcb = dst + (uint64_t)wl * hl, *dcr = dcb + (uint64_t)wc * hc;
+    const bool res = build_residual(mb, ax, P.levels, &wi->res);
+    // ---- neighbours into the tile
+    if (lane < 25) {  // row -1, x = lane - 1
+        int x = lane - 1;
+        bool ok = x < 0 ? av.D : (x < 16 ? av.B : av.C);
+        if (ok) wi->T[0][lane] = dy[(uint64_t)(mby * 16 - 1) * wl + mbx * 16 + x];
+    } else if (lane < 41) {  // column -1
+        int y = lane - 25;
+        if (av.A) wi->T[y + 1][0] = dy[(uint64_t)(mby * 16 + y) * wl + mbx * 16 - 1];
+    } else if (lane < 59) {  // chroma row -1 (x = -1..7) for both planes
+        int k = lane - 41, c = k / 9, x = k % 9 - 1;
+        bool ok = x < 0 ? av.D : av.B;
+        const uint8_t *pl = c == 0 ? dcb : dcr;
+        if (ok) wi->C[c][0][x + 1] = pl[(uint64_t)(mby * 8 - 1) * wc + mbx * 8 + x];
+    }
+    if (lane < 16) {  // chroma column -1
+        int c = lane >> 3, y = lane & 7;
+        const uint8_t *pl = c == 0 ? dcb : dcr;
+        if (av.A) wi->C[c][y + 1][0] = pl[(uint64_t)(mby * 8 + y) * wc + mbx * 8 - 1];
+    }
+    __syncthreads();
+    // ---- luma
+    if (mb->mb_type == H264G_MB_I4x4) {
+        for (int blk = 0; blk < 16; blk++) {
+            if (lane < 16) {
+                int x = lane & 3, y = lane >> 2;
+                int v = pred4x4(wi, av, blk, mb->intra_pred_modes[blk], x, y);
+                int xx = blk_x(blk) + x, yy = blk_y(blk) + y;
+                if (res) v = clip1(v + wi->res.rl[yy * 16 + xx]);
+                wi->T[yy + 1][xx + 1] = (uint8_t)v;
+            }
+            __syncthreads();
+        }
+    } else if (mb->mb_type == H264G_MB_I8x8) {
+        for (int blk = 0; blk < 4; blk++) {
+            int xO = (blk & 1) * 8, yO = (blk >> 1) * 8;
+            // 8.3.2.2.1 reference sample filtering (H264PictureBase.cpp:1535-1599)
+            if (lane < 25) {
+                int p[18];  // p[-1..16]: neighbours along the filtered line of this lane
+                int v = -1;
+                if (lane == 0) {  // p'[-1,-1]
+                    int pm = tile_p(wi, av, xO, yO, -1, -1, -1), pt = tile_p(wi, av, xO, yO, 0, -1, -1), pl = tile_p(wi, av, xO, yO, -1, 0, -1);
+                    if (pm >= 0) {
+                        if (pt < 0 || pl < 0) v = pt >= 0 ? (3 * pm + pt + 2) >> 2 : (pl >= 0 ? (3 * pm + pl + 2) >> 2 : pm);
+                        else v = (pt + 2 * pm + pl + 2) >> 2;
+                    }
+                } else if (lane < 9) {  // p'[-1, y]
+                    int y = lane - 1, all = 1;
+                    for (int k = 0; k < 8; k++) { p[k + 1] = tile_p(wi, av, xO, yO, -1, k, -1); all &= p[k + 1] >= 0; }
+                    int pm = tile_p(wi, av, xO, yO, -1, -1, -1);
+                    if (all) {
+                        if (y == 0) v = pm >= 0 ? (pm + 2 * p[1] + p[2] + 2) >> 2 : (3 * p[1] + p[2] + 2) >> 2;
+                        else if (y == 7) v = (p[7] + 3 * p[8] + 2) >> 2;
+                        else v = (p[y] + 2 * p[y + 1] + p[y + 2] + 2) >> 2;
+                    }
+                } else {  // p'[x, -1]
+                    int x = lane - 9, all = 1;
+                    for (int k = 0; k < 16; k++) p[k + 1] = tile_p(wi, av, xO, yO, k, -1, -1);
+                    int up8 = 1;
+                    for (int k = 8; k < 16; k++) up8 &= p[k + 1] < 0;
+                    if (up8 && p[8] >= 0) for (int k = 8; k < 16; k++) p[k + 1] = p[8];
+                    for (int k = 0; k < 16; k++) all &= p[k + 1] >= 0;
+                    int pm = tile_p(wi, av, xO, yO, -1, -1, -1);
+                    if (all) {
+                        if (x == 0) v = pm >= 0 ? (pm + 2 * p[1] + p[2] + 2) >> 2 : (3 * p[1] + p[2] + 2) >> 2;
+                        else if (x == 15) v = (p[15] + 3 * p[16] + 2) >> 2;
+                        else v = (p[x] + 2 * p[x + 1] + p[x + 2] + 2) >> 2;
+                    }
+                }
+                wi->pf[lane] = (int16_t)v;
+            }
+            __syncthreads();
+            int x = lane & 7, y = lane >> 3;
+            int v = pred8x8(wi, mb->intra_pred_modes[blk], x, y);
+            if (res) v = clip1(v + wi->res.rl[(yO + y) * 16 + xO + x]);
+            __syncthreads();
+            wi->T[yO + y + 1][xO + x + 1] = (uint8_t)v;
+            __syncthreads();
+        }
+    } else {  // Intra_16x16, 8.3.3 (H264PictureBase.cpp:1847-2056)
+        int mode = mb->intra16x16_pred_mode;
+        int dc = 128, a = 0, bq = 0, cq = 0;
+        if (mode == 2) {
+            int st = 0, sl = 0;
+            for (int k = 0; k < 16; k++) { st += wi->T[0][k + 1]; sl += wi->T[k + 1][0]; }
+            if (av.A && av.B) dc = (st + sl + 16) >> 5;
+            else if (av.A) dc = (sl + 8) >> 4;
+            else if (av.B) dc = (st + 8) >> 4;
+        } else if (mode == 3) {
+            int Hs = 0, Vs = 0;
+            for (int k = 0; k < 8; k++) {
+                Hs += (k + 1) * (wi->T[0][8 + k + 1] - wi->T[0][6 - k + 1]);
+                Vs += (k + 1) * (wi->T[8 + k + 1][0] - wi->T[6 - k + 1][0]);
+            }
+            a = 16 * (wi->T[16][0] + wi->T[0][16]);
+            bq = (5 * Hs + 32) >> 6;
+            cq = (5 * Vs + 32) >> 6;
+        }
+        for (int k = 0; k < 4; k++) {
+            int i = lane + k * WAVE, x = i & 15, y = i >> 4;
+            int v = mode == 0 ? wi->T[0][x + 1] : mode == 1 ? wi->T[y + 1][0] : mode == 2 ? dc : clip1((a + bq * (x - 7) + cq * (y - 7) + 16) >> 5);
+            if (res) v = clip1(v + wi->res.rl[y * 16 + x]);
+            wi->T[y + 1][x + 1] = (uint8_t)v;  // only this lane reads/writes pixel i
+        }
+        __syncthreads();
+    }
+    // ---- chroma, 8.3.4 (H264PictureBase.cpp:2076-2381); DC tests "> 0" (quirk Q1)
+    {
+        int mode = mb->intra_chroma_pred_mode;
+        for (int k = 0; k < 2; k++) {
+            int i = lane + k * WAVE, c = i >> 6, x = i & 7, y = (i >> 3) & 7;
+            const uint8_t (*Cp)[9] = wi->C[c];
+            int v;
+            if (mode == 0) {
+                int xO = x & 4, yO = y & 4;
+                int t = av.B && Cp[0][xO + 1] > 0 && Cp[0][xO + 2] > 0 && Cp[0][xO + 3] > 0 && Cp[0][xO + 4] > 0;
+                int l = av.A && Cp[yO + 1][0] > 0 && Cp[yO + 2][0] > 0 && Cp[yO + 3][0] > 0 && Cp[yO + 4][0] > 0;
+                int st = Cp[0][xO + 1] + Cp[0][xO + 2] + Cp[0][xO + 3] + Cp[0][xO + 4];
+                int sl = Cp[yO + 1][0] + Cp[yO + 2][0] + Cp[yO + 3][0] + Cp[yO + 4][0];
+                if ((xO == 0 && yO == 0) || (xO > 0 && yO > 0)) v = (t && l) ? (st + sl + 4) >> 3 : (l ? (sl + 2) >> 2 : (t ? (st + 2) >> 2 : 128));
+                else if (xO > 0) v = t ? (st + 2) >> 2 : (l ? (sl + 2) >> 2 : 128);
+                else v = l ? (sl + 2) >> 2 : (t ? (st + 2) >> 2 : 128);
+            } else if (mode == 1) {
+                v = Cp[y + 1][0];
+            } else if (mode == 2) {
+                v = Cp[0][x + 1];
+            } else {
+                int Hs = 0, Vs = 0;
+                for (int j = 0; j < 4; j++) {
+                    Hs += (j + 1) * (Cp[0][4 + j + 1] - Cp[0][2 - j + 1]);
+                    Vs += (j + 1) * (Cp[4 + j + 1][0] - Cp[2 - j + 1][0]);
+                }
+                int aa = 16 * (Cp[8][0] + Cp[0][8]), bb = (34 * Hs + 32) >> 6, cc = (34 * Vs + 32) >> 6;
+                v = clip1((aa + bb * (x - 3) + cc * (y - 3) + 16) >> 5);
+            }
+            if (res) v = clip1(v + wi->res.rc[c][y * 8 + x]);
+            uint8_t *pl = c == 0 ? dcb : dcr;
+            pl[(uint64_t)(mby * 8 + y) * wc + mbx * 8 + x] = (uint8_t)v;
+        }
+    }
+    // ---- luma tile -> picture
+    for (int k = 0; k < 4; k++) {
+        int i = lane + k * WAVE, x = i & 15, y = i >> 4;
+        dy[(uint64_t)(mby * 16 + y) * wl + mbx * 16 + x] = wi->T[y + 1][x + 1];
+    }
+    (void)H;
+}
+
+// ------------------------------------------------------------ deblocking (8.7)
+struct MbView {
+    int cls, kind, nz4, nz8;
+    int refid[2][4];
+    int pf;
+    int qdb[3];
+};
+
+__device__ __forceinline__ void load_view(const DevPic &P, int addr, MbView &v) {
+    const DevAux ax = P.aux[addr];
+    v.cls = P.mbs[addr].mb_type;
+    v.kind = ax.kind;
+    v.nz4 = ax.nz4;
+    v.nz8 = ax.nz8;
+    v.pf = ax.pf;
+#pragma unroll
+    for (int l = 0; l < 2; l++)
+#pragma unroll
+        for (int q = 0; q < 4; q++) v.refid[l][q] = ax.refid[l][q];
+    v.qdb[0] = ax.qdb[0]; v.qdb[1] = ax.qdb[1]; v.qdb[2] = ax.qdb[2];
+}
+
+__device__ __forceinline__ bool mv_far(int ax, int ay, int bx, int by) { return iabs(ax - bx) >= 4 || iabs(ay - by) >= 4; }
+
+// 8.7.2.1 (Deblock.cpp:994-1310) with quirk Q3; I_PCM counts as neither intra nor inter.
+__device__ int bs_luma(const DevPic &P, int ap, const MbView &Pv, int aq, const MbView &Qv, int mb_edge, int xp, int yp, int xq, int yq) {
+    bool pin = Pv.kind & AUX_INTRA, qin = Qv.kind & AUX_INTRA;
+    if (mb_edge && (pin || qin)) return 4;
+    if (pin || qin) return 3;
+    int bp4 = 8 * (yp >> 3) + 4 * (xp >> 3) + 2 * ((yp & 7) >> 2) + ((xp & 7) >> 2);
+    int bq4 = 8 * (yq >> 3) + 4 * (xq >> 3) + 2 * ((yq & 7) >> 2) + ((xq & 7) >> 2);
+    int bp8 = 2 * (yp >> 3) + (xp >> 3), bq8 = 2 * (yq >> 3) + (xq >> 3);
+    bool pt8 = Pv.kind & AUX_T8, qt8 = Qv.kind & AUX_T8;
+    if ((pt8 && ((Pv.nz8 >> bp8) & 1)) || (!pt8 && ((Pv.nz4 >> bp4) & 1)) ||
+        (qt8 && ((Qv.nz8 >> bq8) & 1)) || (!qt8 && ((Qv.nz4 >> bq4) & 1)))
+        return 2;
+    int qp = 2 * (yp >> 3) + (xp >> 3), qq = 2 * (yq >> 3) + (xq >> 3);
+    int rp = (yp >> 2) * 4 + (xp >> 2), rq = (yq >> 2) * 4 + (xq >> 2);
+    int p0 = Pv.refid[0][qp], p1 = Pv.refid[1][qp], q0 = Qv.refid[0][qq], q1 = Qv.refid[1][qq];
+    int fp0 = (Pv.pf >> qp) & 1, fp1 = (Pv.pf >> (4 + qp)) & 1, fq0 = (Qv.pf >> qq) & 1, fq1 = (Qv.pf >> (4 + qq)) & 1;
+    if (!(((p0 == q0 && p1 == q1) || (p0 == q1 && p1 == q0)) && fp0 + fp1 == fq0 + fq1)) return 1;
+    const h264g_mb *mp = &P.mbs[ap], *mq = &P.mbs[aq];
+    int mp0x = mp->mv[0][rp][0], mp0y = mp->mv[0][rp][1], mp1x = mp->mv[1][rp][0], mp1y = mp->mv[1][rp][1];
+    int mq0x = mq->mv[0][rq][0], mq0y = mq->mv[0][rq][1], mq1x = mq->mv[1][rq][0], mq1y = mq->mv[1][rq][1];
+    if (fp0 && !fp1 && fq0 && !fq1 && mv_far(mp0x, mp0y, mq0x, mq0y)) return 1;
+    if (fp0 && !fp1 && !fq0 && fq1 && mv_far(mp0x, mp0y, mq1x, mq1y)) return 1;
+    if (!fp0 && fp1 && fq0 && !fq1 && mv_far(mp1x, mp1y, mq0x, mq0y)) return 1;
+    if (!fp0 && fp1 && !fq0 && fq1 && mv_far(mp1x, mp1y, mq1x, mq1y)) return 1;
+    if (fp0 && fp1 && p0 != p1 && fq0 && fq1 && ((q0 == p0 && q1 == p1) || (q0 == p1 && q1 == p0))) {
+        if (q0 == p0 && (mv_far(mp0x, mp0y, mq0x, mq0y) || mv_far(mp1x, mp1y, mq1x, mq1y))) return 1;
+        if (q0 == p1 && (mv_far(mp1x, mp1y, mq0x, mq0y) || mv_far(mp0x, mp0y, mq1x, mq1y))) return 1;
+    }
+    if (fp0 && fp1 && p0 == p1 && fq0 && fq1 && q0 == q1 && q0 == p0) {
+        if (mv_far(mp0x, mp0y, mq0x, mq0y) || (mv_far(mp1x, mp1y, mq1x, mq1y) && mv_far(mp0x, mp0y, mq1x, mq1y)) ||
+            mv_far(mp1x, mp1y, mq0x, mq0y))
+            return 1;
+    }
+    return 0;
+}
+
+// 8.7.2.2 - 8.7.2.4 on samples p[0..3] (p0 first) and q[0..3] (Deblock.cpp:1314-1522).
+__device__ void filter_line(int p[4], int q[4], int bS, bool chroma, int qavg, int offA, int offB) {
+    int indexA = clip3(0, 51, qavg + offA), indexB = clip3(0, 51, qavg + offB);
+    int alpha = c_alpha[indexA], beta = c_beta[indexB];
+    int p0 = p[0], p1 = p[1], p2 = p[2], p3 = p[3], q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3];
+    if (!(bS != 0 && iabs(p0 - q0) < alpha && iabs(p1 - p0) < beta && iabs(q1 - q0) < beta)) return;
+    int ap = iabs(p2 - p0), aq = iabs(q2 - q0);
+    if (bS < 4) {
+        int tc0 = c_tc0[bS - 1][indexA];
+        int tc = chroma ? tc0 + 1 : tc0 + (ap < beta) + (aq < beta);
+        int delta = clip3(-tc, tc, (((q0 - p0) << 2) + (p1 - q1) + 4) >> 3);
+        p[0] = clip1(p0 + delta);
+        q[0] = clip1(q0 - delta);
+        if (!chroma && ap < beta) p[1] = (p1 + clip3(-tc0, tc0, (p2 + ((p0 + q0 + 1) >> 1) - (p1 << 1)) >> 1)) & 255;
+        if (!chroma && aq < beta) q[1] = (q1 + clip3(-tc0, tc0, (q2 + ((p0 + q0 + 1) >> 1) - (q1 << 1)) >> 1)) & 255;
+        return;
+    }
+    bool strong = iabs(p0 - q0) < ((alpha >> 2) + 2);
+    if (!chroma && ap < beta && strong) {
+        p[0] = (p2 + 2 * p1 + 2 * p0 + 2 * q0 + q1 + 4) >> 3;
+        p[1] = (p2 + p1 + p0 + q0 + 2) >> 2;
+        p[2] = (2 * p3 + 3 * p2 + p1 + p0 + q0 + 4) >> 3;
+    } else {
+        p[0] = (2 * p1 + p0 + q1 + 2) >> 2;
+    }
+    if (!chroma && aq < beta && strong) {
+        q[0] = (p1 + 2 * p0 + 2 * q0 + 2 * q1 + q2 + 4) >> 3;
+        q[1] = (p0 + q0 + q1 + q2 + 2) >> 2;
+        q[2] = (2 * q3 + 3 * q2 + q1 + q0 + p0 + 4) >> 3;
+    } else {
+        q[0] = (2 * q1 + q0 + p1 + 2) >> 2;
+    }
+}
+
+// LDS working set of one deblocking wave: luma rows/cols [-4, 15], chroma [-4, 7].
+struct WaveDbk {
+    uint8_t L[20][20];
+    uint8_t C[2][12][12];
+};
+
+__device__ void deblock_mb(const DevPic &P, int addr, WaveDbk *wd) {
+    const int lane = wave_lane();
+    const int W = P.W, H = P.H, wl = W * 16, hl = H * 16, wc = W * 8, hc = H * 8;
+    const int mbx = addr % W, mby = addr / W;
+    uint8_t *dst = slot_base(P, P.dst);
+    uint8_t *dy = dst, *dcb = dst + (uint64_t)wl * hl, *dcr = dcb + (uint64_t)wc * hc;
+    const bool left = mbx > 0, top = mby > 0;
+    // ---- load: luma 20x20 (rows/cols -4..15) and chroma 12x12, where they exist
+    for (int i = lane; i < 400; i += WAVE) {
+        int y = i / 20 - 4, x = i % 20 - 4;
+        if ((x < 0 && !left) || (y < 0 && !top) || (x < 0 && y < 0)) continue;
+        wd->L[y + 4][x + 4] = dy[(uint64_t)(mby * 16 + y) * wl + mbx * 16 + x];
+    }
+    for (int i = lane; i < 288; i += WAVE) {
+        int c = i / 144, k = i % 144, y = k / 12 - 4, x = k % 12 - 4;
+        if ((x < 0 && !left) || (y < 0 && !top) || (x < 0 && y < 0)) continue;
+        const uint8_t *pl = c == 0 ? dcb : dcr;
+        wd->C[c][y + 4][x + 4] = pl[(uint64_t)(mby * 8 + y) * wc + mbx * 8 + x];
+    }
+    MbView Qv, Lv, Tv;
+    load_view(P, addr, Qv);
+    if (left) load_view(P, addr - 1, Lv);
+    if (top) load_view(P, addr - W, Tv);
+    __syncthreads();
+    const bool t8 = Qv.kind & AUX_T8;
+    // ---- edges: dir 0 = vertical edges (left to right), dir 1 = horizontal (top to bottom)
+    for (int dir = 0; dir < 2; dir++) {
+        for (int s = 0; s < 4; s++) {
+            int e = 4 * s;
+            int plane = lane < 16 ? 0 : (lane < 24 ? 1 : (lane >= 32 && lane < 40 ? 2 : -1));
+            bool active = plane >= 0;
+            if (plane == 0 && ((s & 1) && t8)) active = false;       // internal 4 / 12 with 8x8 transform
+            if (plane > 0 && s >= 2) active = false;                 // chroma edges 0 and 4 only
+            if (e == 0 && (dir == 0 ? !left : !top)) active = false;
+            if (active) {
+                int k = plane == 0 ? lane : (plane == 1 ? lane - 16 : lane - 32);
+                int n = plane == 0 ? 16 : 8, sc = plane == 0 ? 1 : 2;
+                int xq = dir == 0 ? e : k, yq = dir == 0 ? k : e;
+                int xp = dir == 0 ? (e ? e - 1 : n - 1) : k, yp = dir == 0 ? k : (e ? e - 1 : n - 1);
+                const MbView &Pv = e ? Qv : (dir == 0 ? Lv : Tv);
+                int ap = e ? addr : (dir == 0 ? addr - 1 : addr - W);
+                int bS = bs_luma(P, ap, Pv, addr, Qv, e == 0, xp * sc, yp * sc, xq * sc, yq * sc);
+                int qavg = (Pv.qdb[plane] + Qv.qdb[plane] + 1) >> 1;
+                int pv[4], qv[4];
+                if (plane == 0) {
+                    for (int i = 0; i < 4; i++) {
+                        if (dir == 0) { qv[i] = wd->L[k + 4][e + i + 4]; pv[i] = wd->L[k + 4][e - i - 1 + 4]; }
+                        else { qv[i] = wd->L[e + i + 4][k + 4]; pv[i] = wd->L[e - i - 1 + 4][k + 4]; }
+                    }
+                } else {
+                    uint8_t (*C)[12] = wd->C[plane - 1];
+                    for (int i = 0; i < 4; i++) {
+                        if (dir == 0) { qv[i] = C[k + 4][e + i + 4]; pv[i] = C[k + 4][e - i - 1 + 4]; }
+                        else { qv[i] = C[e + i + 4][k + 4]; pv[i] = C[e - i - 1 + 4][k + 4]; }
+                    }
+                }
+                if (P.dbk_idc != 1) filter_line(pv, qv, bS, plane > 0, qavg, P.offA, P.offB);
+                if (plane == 0) {
+                    for (int i = 0; i < 3; i++) {
+                        if (dir == 0) { wd->L[k + 4][e + i + 4] = (uint8_t)qv[i]; wd->L[k + 4][e - i - 1 + 4] = (uint8_t)pv[i]; }
+                        else { wd->L[e + i + 4][k + 4] = (uint8_t)qv[i]; wd->L[e - i - 1 + 4][k + 4] = (uint8_t)pv[i]; }
+                    }
+                } else {
+                    uint8_t (*C)[12] = wd->C[plane - 1];
+                    for (int i = 0; i < 3; i++) {
+                        if (dir == 0) { C[k + 4][e + i + 4] = (uint8_t)qv[i]; C[k + 4][e - i - 1 + 4] = (uint8_t)pv[i]; }
+                        else { C[e + i + 4][k + 4] = (uint8_t)qv[i]; C[e - i - 1 + 4][k + 4] = (uint8_t)pv[i]; }
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    // ---- write back the MB and the strips its MB edges modified (3 columns / rows)
+    for (int i = lane; i < 400; i += WAVE) {
+        int y = i / 20 - 4, x = i % 20 - 4;
+        if (x < -3 || y < -3 || (x < 0 && y < 0) || (x < 0 && !left) || (y < 0 && !top)) continue;
+        dy[(uint64_t)(mby * 16 + y) * wl + mbx * 16 + x] = wd->L[y + 4][x + 4];
+    }
+    for (int i = lane; i < 288; i += WAVE) {
+        int c = i / 144, k = i % 144, y = k / 12 - 4, x = k % 12 - 4;
+        if (x < -3 || y < -3 || (x < 0 && y < 0) || (x < 0 && !left) || (y < 0 && !top)) continue;
+        uint8_t *pl = c == 0 ? dcb : dcr;
+        pl[(uint64_t)(mby * 8 + y) * wc + mbx * 8 + x] = wd->C[c][y + 4][x + 4];
+    }
+    (void)H; (void)hl; (void)hc;
+}
+
+// ------------------------------------------------------------ kernels
+// Every kernel runs one wave per workgroup: the LDS working sets are per wave and
+// __syncthreads is then a barrier of that wave alone, whatever path its MB takes.
+__global__ __launch_bounds__(WAVE) void k_inter(const DevPic *__restrict__ pics, const uint32_t *__restrict__ items, uint32_t n,
+                                                uint32_t *err) {
+    __shared__ WaveRes lds;
+    const uint32_t i = blockIdx.x;
+    if (i >= n) return;
+    const uint32_t it = items[i];
+    const DevPic &P = pics[it >> WI_PIC_SHIFT];
+    inter_mb(P, (int)(it & WI_MB_MASK), &lds);
+    (void)err;
+}
+
+__global__ __launch_bounds__(WAVE) void k_intra(const DevPic *__restrict__ pics, const uint32_t *__restrict__ items, uint32_t n,
+                                                DevCtl *ctl, uint32_t epoch, uint32_t *err) {
+    __shared__ WaveIntra lds;
+    for (;;) {
+        int t = next_ticket(ctl, n);
+        if (t < 0) break;
+        const uint32_t it = items[t];
+        const DevPic &P = pics[it >> WI_PIC_SHIFT];
+        const int addr = (int)(it & WI_MB_MASK), W = P.W, mbx = addr % W, mby = addr / W;
+        Avail av;
+        av.A = mbx > 0; av.B = mby > 0; av.C = mby > 0 && mbx < W - 1; av.D = mbx > 0 && mby > 0;
+        int nb[4] = {av.A ? addr - 1 : -1, av.B ? addr - W : -1, av.C ? addr - W + 1 : -1, av.D ? addr - W - 1 : -1};
+        int deps[4];
+        for (int k = 0; k < 4; k++) {
+            deps[k] = -1;
+            if (nb[k] < 0) continue;
+            int c = P.mbs[nb[k]].mb_type;
+            if (P.cip && cls_inter(c)) { if (k == 0) av.A = 0; else if (k == 1) av.B = 0; else if (k == 2) av.C = 0; else av.D = 0; }
+            if (cls_intra(c)) deps[k] = nb[k];
+        }
+        wait_deps(P.flags, deps, 4, epoch, err);
+        intra_mb(P, addr, &lds, av);
+        publish(&P.flags[addr], epoch);
+    }
+    retire(ctl, gridDim.x);
+}
+
+__global__ __launch_bounds__(WAVE) void k_deblock(const DevPic *__restrict__ pics, const uint32_t *__restrict__ items, uint32_t n,
+                                                  DevCtl *ctl, uint32_t epoch, uint32_t *err) {
+    __shared__ WaveDbk lds;
+    for (;;) {
+        int t = next_ticket(ctl, n);
+        if (t < 0) break;
+        const uint32_t it = items[t];
+        const DevPic &P = pics[it >> WI_PIC_SHIFT];
+        const int addr = (int)(it & WI_MB_MASK), W = P.W, mbx = addr % W, mby = addr / W;
+        int deps[2] = {mbx > 0 ? addr - 1 : -1, mby > 0 ? (mbx < W - 1 ? addr - W + 1 : addr - W) : -1};
+        wait_deps(P.flags, deps, 2, epoch, err);
+        deblock_mb(P, addr, &lds);
+        publish(&P.flags[addr], epoch);
+    }
+    retire(ctl, gridDim.x);
+}
+
+__global__ void k_copy(const uint4 *src, uint4 *dst, uint64_t n16) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n16; i += (uint64_t)gridDim.x * blockDim.x) dst[i] = src[i];
+}
+
+}  // namespace
+
+extern "C" {
+
+int h264g_launch_inter(const DevPic *pics, const uint32_t *items, uint32_t n, uint32_t *err, void *stream) {
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(k_inter, dim3(n), dim3(WAVE), 0, (hipStream_t)stream, pics, items, n, err);
+    return (int)hipGetLastError();
+}
+
+int h264g_launch_intra(const DevPic *pics, const uint32_t *items, uint32_t n, DevCtl *ctl, uint32_t epoch, uint32_t *err,
+                       int waves, void *stream) {
+    if (n == 0) return 0;
+    uint32_t g = (uint32_t)waves < n ? (uint32_t)waves : n;
+    hipLaunchKernelGGL(k_intra, dim3(g), dim3(WAVE), 0, (hipStream_t)stream, pics, items, n, ctl, epoch, err);
+    return (int)hipGetLastError();
+}
+
+int h264g_launch_deblock(const DevPic *pics, const uint32_t *items, uint32_t n, DevCtl *ctl, uint32_t epoch, uint32_t *err,
+                         int waves, void *stream) {
+    if (n == 0) return 0;
+    uint32_t g = (uint32_t)waves < n ? (uint32_t)waves : n;
+    hipLaunchKernelGGL(k_deblock, dim3(g), dim3(WAVE), 0, (hipStream_t)stream, pics, items, n, ctl, epoch, err);
+    return (int)hipGetLastError();
+}
+
+int h264g_launch_copy(const uint8_t *src, uint8_t *dst, uint64_t bytes, void *stream) {
+    uint64_t n16 = bytes / 16;
+    if (n16 == 0) return 0;
+    hipLaunchKernelGGL(k_copy, dim3(1024), dim3(256), 0, (hipStream_t)stream, (const uint4 *)src, (uint4 *)dst, n16);
+    return (int)hipGetLastError();
+}
+
+}  // extern "C"

@@ -1,0 +1,224 @@
+"""Host-side handles over the libh264g C ABI: engine context, streams, staged
+pictures, batches, and the deterministic syntax generator.
+
+Everything here is plumbing around ``include/h264g.h``; the reconstruction
+itself runs in the gfx950 kernels of the library.
+"""
+from __future__ import annotations
+
+import ctypes
+from ctypes import byref, c_float, c_uint32, c_uint64, c_void_p
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from ._lib import MB_DTYPE, Config, GenParams, PicDesc, StreamDesc, check, lib
+
+
+def _ptr(a: np.ndarray):
+    assert a.flags["C_CONTIGUOUS"]
+    return ctypes.c_void_p(a.ctypes.data) if a.size else None
+
+
+# ------------------------------------------------------------------ generator
+class Generator:
+    """Deterministic synthetic syntax for one stream (pictures in decoding order)."""
+
+    def __init__(self, config: int = 1, seed=None, **overrides):
+        self.params = GenParams()
+        check(lib().h264g_gen_default_params(config, byref(self.params)))
+        if seed is not None:
+            self.params.seed = seed
+        for k, v in overrides.items():
+            if not hasattr(self.params, k):
+                raise AttributeError(f"unknown generator parameter {k}")
+            setattr(self.params, k, v)
+        h = c_void_p()
+        check(lib().h264g_gen_create(byref(self.params), byref(h)))
+        self._h = h
+
+    def close(self):
+        if self._h:
+            lib().h264g_gen_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def width_mbs(self) -> int:
+        return self.params.width_mbs
+
+    @property
+    def height_mbs(self) -> int:
+        return self.params.height_mbs
+
+    @property
+    def num_pictures(self) -> int:
+        return check(lib().h264g_gen_num_pictures(self._h))
+
+    @property
+    def dpb_slots(self) -> int:
+        return check(lib().h264g_gen_dpb_slots(self._h))
+
+    @property
+    def digest(self) -> int:
+        return lib().h264g_gen_digest(self._h)
+
+    def picture(self, idx: int):
+        """(PicDesc, mbs structured array, levels int16 array), copies owned by the caller."""
+        desc = PicDesc()
+        mbs_p, nmb = c_void_p(), c_uint32()
+        lv_p, nlv = c_void_p(), c_uint64()
+        check(lib().h264g_gen_picture(self._h, idx, byref(desc), byref(mbs_p), byref(nmb), byref(lv_p), byref(nlv)))
+        raw = ctypes.string_at(mbs_p.value, nmb.value * MB_DTYPE.itemsize)
+        mbs = np.frombuffer(raw, MB_DTYPE).copy()
+        if nlv.value:
+            levels = np.frombuffer(ctypes.string_at(lv_p.value, nlv.value * 2), np.int16).copy()
+        else:
+            levels = np.zeros(0, np.int16)
+        return desc, mbs, levels
+
+    def pictures(self):
+        return [self.picture(i) for i in range(self.num_pictures)]
+
+
+# ------------------------------------------------------------------ engine
+@dataclass
+class StagedPicture:
+    handle: c_void_p
+    stream: int
+    desc: PicDesc
+    width_mbs: int
+    height_mbs: int
+
+
+class Engine:
+    """One libh264g context on one HIP device (one HIP stream inside)."""
+
+    def __init__(self, device: int = 0, dpb_budget_bytes: int = 0, profile: bool = False, waves: int = 0,
+                 max_batch_pictures: int = 0):
+        cfg = Config(device=device, dpb_budget_bytes=dpb_budget_bytes, max_batch_pictures=max_batch_pictures,
+                     profile=1 if profile else 0, waves=waves)
+        h = c_void_p()
+        check(lib().h264g_create(byref(cfg), byref(h)))
+        self._h = h
+        self.streams = {}
+
+    def close(self):
+        if self._h:
+            lib().h264g_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- streams
+    def open_stream(self, width_mbs: int, height_mbs: int, dpb_slots: int) -> int:
+        d = StreamDesc(pic_width_in_mbs=width_mbs, frame_height_in_mbs=height_mbs, chroma_array_type=1, bit_depth=8,
+                       frame_mbs_only=1, mbaff=0, dpb_slots=dpb_slots)
+        sid = c_uint32()
+        check(lib().h264g_stream_open(self._h, byref(d), byref(sid)))
+        self.streams[sid.value] = (width_mbs, height_mbs, dpb_slots)
+        return sid.value
+
+    def close_stream(self, sid: int):
+        check(lib().h264g_stream_close(self._h, sid))
+        self.streams.pop(sid, None)
+
+    # ---- pictures
+    def begin(self, sid: int, desc: PicDesc, mbs: np.ndarray, levels: np.ndarray) -> StagedPicture:
+        mbs = np.ascontiguousarray(mbs, MB_DTYPE)
+        levels = np.ascontiguousarray(levels, np.int16)
+        h = c_void_p()
+        check(lib().h264g_picture_begin(self._h, sid, byref(desc), _ptr(mbs), mbs.size, _ptr(levels), levels.size,
+                                        byref(h)))
+        W, H, _ = self.streams[sid]
+        return StagedPicture(h, sid, desc, W, H)
+
+    def submit(self, pic: StagedPicture):
+        check(lib().h264g_picture_submit(self._h, pic.handle))
+
+    def release(self, pic: StagedPicture):
+        if pic.handle:
+            check(lib().h264g_picture_release(self._h, pic.handle))
+            pic.handle = None
+
+    def alg_bytes(self, pic: StagedPicture):
+        out = (c_uint64 * 3)()
+        check(lib().h264g_picture_alg_bytes(pic.handle, out))
+        return list(out)
+
+    # ---- batches
+    def batch_submit(self, pics):
+        arr = (c_void_p * len(pics))(*[p.handle.value for p in pics])
+        check(lib().h264g_batch_submit(self._h, arr, len(pics)))
+
+    def batch_create(self, pics) -> c_void_p:
+        arr = (c_void_p * len(pics))(*[p.handle.value for p in pics])
+        b = c_void_p()
+        check(lib().h264g_batch_create(self._h, arr, len(pics), byref(b)))
+        return b
+
+    def batch_run(self, b: c_void_p):
+        check(lib().h264g_batch_run(self._h, b))
+
+    def batch_destroy(self, b: c_void_p):
+        check(lib().h264g_batch_destroy(self._h, b))
+
+    # ---- completion and data
+    def sync(self):
+        check(lib().h264g_sync(self._h))
+
+    def stage_times(self):
+        out = (c_float * 3)()
+        check(lib().h264g_stage_times(self._h, out))
+        return list(out)
+
+    def readback(self, sid: int, slot: int):
+        W, H, _ = self.streams[sid]
+        y = np.empty((H * 16, W * 16), np.uint8)
+        cb = np.empty((H * 8, W * 8), np.uint8)
+        cr = np.empty((H * 8, W * 8), np.uint8)
+        check(lib().h264g_slot_readback(self._h, sid, slot, _ptr(y), _ptr(cb), _ptr(cr)))
+        return y, cb, cr
+
+    def upload(self, sid: int, slot: int, y: np.ndarray, cb: np.ndarray, cr: np.ndarray):
+        y, cb, cr = (np.ascontiguousarray(a, np.uint8) for a in (y, cb, cr))
+        check(lib().h264g_slot_upload(self._h, sid, slot, _ptr(y), _ptr(cb), _ptr(cr)))
+
+
+def decode_stream_plain(eng: Engine, gen: Generator, readback: bool = True):
+    """Decodes a generated stream picture by picture (plain path).
+
+    Returns the reconstructed (Y, Cb, Cr) of every picture in decoding order."""
+    sid = eng.open_stream(gen.width_mbs, gen.height_mbs, gen.dpb_slots)
+    out = []
+    try:
+        for i in range(gen.num_pictures):
+            desc, mbs, levels = gen.picture(i)
+            pic = eng.begin(sid, desc, mbs, levels)
+            eng.submit(pic)
+            eng.sync()
+            if readback:
+                out.append(eng.readback(sid, desc.dst_slot))
+            eng.release(pic)
+    finally:
+        eng.close_stream(sid)
+    return out
+
+
+__all__ = ["Generator", "Engine", "StagedPicture", "decode_stream_plain", "_lib"]

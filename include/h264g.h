@@ -136,7 +136,9 @@ typedef struct h264g_mb {
 typedef struct h264g_config {
     int32_t  device;              /* HIP device ordinal                               */
     uint64_t dpb_budget_bytes;    /* HBM budget for all streams' DPB slots (0 = none) */
-    int32_t  max_batch_pictures;  /* pictures per batched launch (0 = default 256)    */
+    int32_t  max_batch_pictures;  /* pictures per batched launch (0 = default 4096)   */
+    int32_t  profile;             /* 1 = record per-stage HIP events                  */
+    int32_t  waves;               /* persistent waves per wavefront launch (0 = auto) */
 } h264g_config;
 
 typedef struct h264g_stream_desc {
@@ -213,8 +215,22 @@ int32_t h264g_picture_submit(h264g_ctx *ctx, h264g_pic *pic);
 
 /* Enqueue many staged pictures as one batched job: one persistent launch per
  * stage drains a device queue of macroblock tiles from all of them.  The
- * pictures must not reference each other (e.g. different streams). */
+ * pictures must not reference each other (e.g. different streams).  No
+ * counterpart in the reference, which decodes one picture at a time. */
 int32_t h264g_batch_submit(h264g_ctx *ctx, h264g_pic *const *pics, uint32_t n_pics);
+
+/* The same in two steps: build the batch's device work queues once
+ * (h264g_batch_create), then enqueue its stage launches (h264g_batch_run) as
+ * often as wanted, e.g. inside a timed region with all inputs already in HBM. */
+typedef struct h264g_batch h264g_batch;
+int32_t h264g_batch_create(h264g_ctx *ctx, h264g_pic *const *pics, uint32_t n_pics, h264g_batch **out);
+int32_t h264g_batch_run(h264g_ctx *ctx, h264g_batch *batch);
+int32_t h264g_batch_destroy(h264g_ctx *ctx, h264g_batch *batch);
+
+/* Device-side timing of the last h264g_batch_run / h264g_picture_submit per
+ * stage, in milliseconds (HIP events on the launch stream): out[0] inter,
+ * out[1] intra, out[2] deblocking.  Requires h264g_config.profile = 1. */
+int32_t h264g_stage_times(h264g_ctx *ctx, float out[3]);
 
 /* Wait for all enqueued work; returns the first device error word if any. */
 int32_t h264g_sync(h264g_ctx *ctx);

@@ -4,7 +4,7 @@
 #include <cstdio>
 #include <cstdint>
 
-int32_t h264g_set_error(int32_t code, const char *fmt, ...) {
+extern "C" int32_t h264g_set_error(int32_t code, const char *fmt, ...) {
     va_list ap;
     va_start(ap, fmt);
     vfprintf(stderr, fmt, ap);

@@ -1,0 +1,113 @@
+"""Writes tests/golden/golden.json: per-picture hashes of the reference decoder's
+own reconstruction on generated syntax.
+
+The reference's reconstruction sources (jfu222/h264_video_decoder_demo, compiled
+unmodified by oracle/Makefile into oracle/_ref/) are driven by oracle/ref_harness.cpp
+on the syntax of csrc/gen.cpp.  For each case the fixture records the generator's
+digest (so a changed generator is caught) and SHA-256 of every output picture's Y,
+Cb and Cr planes, before and after deblocking.  The reference has no tests or
+golden vectors of its own for this path; these are the pins.
+
+Run in a container that has /root/reference:  python tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import hashlib
+import json
+import os
+import subprocess
+import sys
+import tempfile
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+HARNESS = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+
+# name -> (generator config, overrides as generator parameter names)
+CASES = {
+    # 352x288, I + P, the plain-path workload
+    "cif_ip": (1, {}),
+    "cif_ip_seed2": (1, {"seed": 2}),
+    "cif_ip_seed3": (1, {"seed": 3}),
+    # config-1 syntax over more frames with I_PCM, chroma QP offsets, idc 2
+    "cif_pcm_cqp": (1, {"width_mbs": 30, "height_mbs": 20, "n_frames": 6, "ipcm_permille": 30,
+                        "chroma_qp_index_offset": -4, "second_chroma_qp_index_offset": 3,
+                        "disable_deblocking_filter_idc": 2}),
+    # High-profile IPBBPBBP syntax at a reduced size
+    "high_small": (2, {"width_mbs": 40, "height_mbs": 24}),
+    "high_small_seed7": (2, {"width_mbs": 23, "height_mbs": 17, "seed": 7}),
+    # implicit bi-prediction weights, explicit P weights
+    "high_implicit": (2, {"width_mbs": 24, "height_mbs": 16, "weighted_pred_flag": 1, "weighted_bipred_idc": 2,
+                          "chroma_qp_index_offset": 2, "second_chroma_qp_index_offset": -3}),
+    # default weights everywhere, deblocking off, I_PCM
+    "high_default_nodbk": (2, {"width_mbs": 24, "height_mbs": 16, "weighted_pred_flag": 0, "weighted_bipred_idc": 0,
+                               "disable_deblocking_filter_idc": 1, "ipcm_permille": 20}),
+    # one reference, no B pictures, single-MB-wide and single-MB-tall pictures
+    "narrow": (2, {"width_mbs": 1, "height_mbs": 12, "max_refs": 1, "b_frames": 0}),
+    "flat": (2, {"width_mbs": 14, "height_mbs": 1, "max_refs": 2}),
+    # the full 1920x1088 High-profile workload (8 pictures)
+    "high_1080p": (2, {}),
+}
+
+_CLI = {"n_frames": "--frames", "width_mbs": "--width", "height_mbs": "--height", "seed": "--seed",
+        "ipcm_permille": "--ipcm", "disable_deblocking_filter_idc": "--deblock", "max_refs": "--refs",
+        "b_frames": "--bframes"}
+
+
+def harness_args(config: int, ov: dict) -> list:
+    args = ["--config", str(config)]
+    for k, v in ov.items():
+        if k in _CLI:
+            args += [_CLI[k], str(v)]
+    if "weighted_pred_flag" in ov or "weighted_bipred_idc" in ov:
+        args += ["--wp", f"{ov.get('weighted_pred_flag', 0)},{ov.get('weighted_bipred_idc', 1)}"]
+    if "chroma_qp_index_offset" in ov or "second_chroma_qp_index_offset" in ov:
+        args += ["--cqp", f"{ov.get('chroma_qp_index_offset', 0)},{ov.get('second_chroma_qp_index_offset', 0)}"]
+    return args
+
+
+def plane_hashes(buf: bytes, W: int, H: int):
+    ny, nc = W * H * 256, W * H * 64
+    return [hashlib.sha256(buf[:ny]).hexdigest(), hashlib.sha256(buf[ny:ny + nc]).hexdigest(),
+            hashlib.sha256(buf[ny + nc:]).hexdigest()]
+
+
+def run_case(config: int, ov: dict):
+    with tempfile.TemporaryDirectory() as d:
+        r = subprocess.run([HARNESS, *harness_args(config, ov), "--predeblock", "--out", d], capture_output=True,
+                           text=True, check=True)
+        info = json.loads(r.stdout.strip().splitlines()[-1])
+        n = info["pictures"]
+        files = sorted(os.listdir(d))
+        W, H = None, None
+        out = {"digest": info["digest"], "pictures": n, "pic": [], "pre": []}
+        for i in range(n):
+            for kind in ("pre", "pic"):
+                buf = open(os.path.join(d, f"{kind}_{i:03d}.yuv"), "rb").read()
+                if W is None:
+                    # sizes come from the generator defaults + overrides
+                    from h264_video_decoder_demo_amd import Generator
+                    g = Generator(config, **ov)
+                    W, H = g.width_mbs, g.height_mbs
+                assert len(buf) == W * H * 384, (kind, i, len(buf))
+                out[kind].append(plane_hashes(buf, W, H))
+        out["width_mbs"], out["height_mbs"] = W, H
+        assert info["mv_mismatches"] == 0, info
+        del files
+        return out
+
+
+def main():
+    if not os.path.exists(HARNESS):
+        subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "ref", "-j8"], check=True)
+    sys.path.insert(0, ROOT)
+    golden = {}
+    for name, (config, ov) in CASES.items():
+        golden[name] = {"config": config, "overrides": ov, **run_case(config, ov)}
+        print(name, golden[name]["digest"], golden[name]["pictures"], flush=True)
+    with open(os.path.join(HERE, "golden.json"), "w") as f:
+        json.dump(golden, f, indent=1, sort_keys=True)
+
+
+if __name__ == "__main__":
+    main()
