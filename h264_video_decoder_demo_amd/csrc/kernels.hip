@@ -389,48 +389,54 @@ __device__ __forceinline__ void st_release(uint32_t *p, uint32_t v) {
 
 constexpr uint32_t SPIN_LIMIT = 1u << 20;
 
-// Lane 0 waits until flags[i] == epoch for each listed neighbour.  Bounded: after
-// SPIN_LIMIT polls, or as soon as another wave has raised the error word, the wave
-// stops waiting (its output is then wrong, and h264g_sync reports the error).
+// The persistent loops below keep all control flow around the ticket, the waits and
+// the publication wave-uniform: every lane executes the same atomics, loads and stores
+// (lane-identical addresses and values), and each value a branch depends on is made
+// scalar with readfirstlane.  Lane-conditional regions here (e.g. "lane 0 publishes,
+// then lane 0 takes the next ticket") let the compiler thread the two regions together
+// across the loop latch; its CFG structurizer then leaves lanes 1..63 iterating on
+// their own, which never terminates.
+__device__ __forceinline__ uint32_t uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// Waits until flags[i] == epoch for each listed neighbour.  Bounded: after SPIN_LIMIT
+// polls, or once another wave has raised the error word, the wave stops waiting (its
+// output is then wrong, and h264g_sync reports the error).
 __device__ void wait_deps(const uint32_t *flags, const int *deps, int nd, uint32_t epoch, uint32_t *err) {
-    if (wave_lane() == 0) {
-        for (int k = 0; k < nd; k++) {
-            if (deps[k] < 0) continue;
-            uint32_t it = 0;
-            while (ld_acquire(&flags[deps[k]]) != epoch) {
-                __builtin_amdgcn_s_sleep(1);
-                ++it;
-                if (it > SPIN_LIMIT) { atomicOr(err, DEVERR_TIMEOUT); break; }
-                if ((it & 255u) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) break;
+    for (int k = 0; k < nd; k++) {
+        const int d = (int)uniform((uint32_t)deps[k]);
+        if (d < 0) continue;
+        for (uint32_t it = 0;; ++it) {
+            if (uniform(ld_acquire(&flags[d])) == epoch) break;
+            if (it >= SPIN_LIMIT) {
+                atomicOr(err, DEVERR_TIMEOUT);
+                break;
             }
+            if ((it & 255u) == 255u && uniform(__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) != 0u) break;
+            __builtin_amdgcn_s_sleep(1);
         }
     }
-    __syncthreads();
     __atomic_thread_fence(__ATOMIC_ACQUIRE);
 }
 
 __device__ void publish(uint32_t *flag, uint32_t epoch) {
     __syncthreads();
     __threadfence();
-    if (wave_lane() == 0) st_release(flag, epoch);
+    st_release(flag, epoch);
 }
 
-// Next ticket of a persistent launch (whole wave), or -1.
+// Next ticket of a persistent launch (same value in every lane), or -1.  Only lane 0
+// adds 1; the compiler's wave-level atomic combining issues one atomic for the wave.
 __device__ int next_ticket(DevCtl *ctl, uint32_t n) {
-    uint32_t t = 0;
-    if (wave_lane() == 0) t = atomicAdd(&ctl->ticket, 1u);
-    t = __shfl(t, 0, WAVE);
+    const uint32_t t = uniform(atomicAdd(&ctl->ticket, wave_lane() == 0 ? 1u : 0u));
     return t < n ? (int)t : -1;
 }
 
 // The last wave of a launch resets the control block for the next launch on the stream.
 __device__ void retire(DevCtl *ctl, uint32_t total_waves) {
-    if (wave_lane() == 0) {
-        uint32_t d = atomicAdd(&ctl->done, 1u);
-        if (d == total_waves - 1) {
-            __hip_atomic_store(&ctl->ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(&ctl->done, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
+    const uint32_t d = uniform(atomicAdd(&ctl->done, wave_lane() == 0 ? 1u : 0u));
+    if (d == total_waves - 1) {
+        __hip_atomic_store(&ctl->ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&ctl->done, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
