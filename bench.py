@@ -1,0 +1,269 @@
+#!/usr/bin/env python3
+"""Throughput of bit-exact H.264 picture reconstruction on one MI355X.
+
+Workload (default): 64 independent streams of uneven size -- 16 x 352x288,
+24 x 1280x720, 16 x 1920x1088, 8 x 3840x2160 -- each 16 frames of High-profile
+IPBB syntax (8x8 transform on half the eligible MBs, explicit weighted
+bi-prediction, partitions down to 4x4, 4 references), drawn per stream from
+seed 0x5EED0004 + stream index by the library's generator.  All syntax is staged
+in HBM before timing.
+
+One step = the whole job: the 16 frame indices in decoding order, each as one
+batched launch per stage (inter, intra, deblocking) over the 64 streams' pictures.
+The same pictures are also run through the plain per-picture path (one launch per
+stage per picture), and the batched output is checked byte-for-byte against it.
+
+Prints one JSON line (metric, value = batched Mpix/s, plain Mpix/s, p50/p99 frame
+latency, the roofline of the dominant kernel, and the reference decoder's CPU
+reconstruction timed on this host's cores).
+"""
+from __future__ import annotations
+
+import argparse
+import concurrent.futures as cf
+import hashlib
+import json
+import os
+import statistics
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "bit-exact recon Mpix/s, batched vs plain on 1 GPU; p50/p99 frame latency"
+HBM_PEAK_GBS = 8000.0
+SEED4 = 0x5EED0004
+MIX = [(16, 22, 18), (24, 80, 45), (16, 120, 68), (8, 240, 135)]  # (streams, width MBs, height MBs)
+STAGES = ("inter", "intra", "deblock")
+KERNELS = ("k_inter", "k_intra", "k_deblock")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_config4.json")
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def stream_specs(frames):
+    specs, sid = [], 0
+    for n, W, H in MIX:
+        for _ in range(n):
+            specs.append({"width_mbs": W, "height_mbs": H, "n_frames": frames, "seed": SEED4 + sid})
+            sid += 1
+    return specs
+
+
+def pct(vals, q):
+    v = sorted(vals)
+    return v[min(len(v) - 1, max(0, int(round(q / 100.0 * (len(v) - 1)))))]
+
+
+def dpb_hash(eng, sids, slots):
+    h = hashlib.sha256()
+    for sid in sids:
+        for s in range(slots[sid]):
+            for p in eng.readback(sid, s):
+                h.update(p.tobytes())
+    return h.hexdigest()
+
+
+def cpu_baseline(n_proc, frames):
+    """The reference decoder's reconstruction (oracle/_ref/ref_harness: its own
+    sources, timed around reconstruction + deblocking only) on a bounded sample of
+    the workload, one process per core; falls back to the C oracle (port)."""
+    harness = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+    W, H = 80, 45
+    seeds = [SEED4 + 16 + i for i in range(n_proc)]  # the first 720p streams of the mix
+    sample = f"{n_proc} x 1280x720 streams of the mix (seeds 0x5EED0004+16..), first {frames} frames each"
+    if os.path.exists(harness):
+        with tempfile.TemporaryDirectory() as tmp:
+            procs = []
+            for i, sd in enumerate(seeds):
+                d = os.path.join(tmp, str(i))
+                os.makedirs(d)
+                procs.append(subprocess.Popen([harness, "--config", "2", "--width", str(W), "--height", str(H),
+                                               "--frames", str(frames), "--seed", str(sd), "--time", "--out", d],
+                                              stdout=subprocess.PIPE, text=True))
+            rates, mpix, secs = [], 0.0, 0.0
+            for p in procs:
+                out, _ = p.communicate(timeout=600)
+                if p.returncode != 0:
+                    raise RuntimeError("reference harness failed")
+                r = json.loads(out.strip().splitlines()[-1])
+                rates.append(r["mpix"] / r["recon_seconds"])
+                mpix += r["mpix"]
+                secs += r["recon_seconds"]
+        return {"value": round(sum(rates), 3), "unit": "Mpix/s", "cores": n_proc, "kind": "reference",
+                "sample": sample + "; reference sources compiled by oracle/Makefile, one process per core",
+                "per_core_mpix_s": round(mpix / secs, 3), "cpu_seconds": round(secs, 2)}
+    from h264_video_decoder_demo_amd import Generator
+    from oracle.refcpu import Dpb
+
+    def one(sd):
+        g = Generator(2, width_mbs=W, height_mbs=H, n_frames=frames, seed=sd)
+        dpb = Dpb(W, H, g.dpb_slots)
+        pics = [g.picture(i) for i in range(g.num_pictures)]
+        t = time.perf_counter()
+        for desc, mbs, lv in pics:
+            dpb.decode(desc, mbs, lv, 2)
+        return W * H * 256 * len(pics) / 1e6, time.perf_counter() - t
+
+    with cf.ThreadPoolExecutor(n_proc) as ex:
+        res = list(ex.map(one, seeds))
+    return {"value": round(sum(m / s for m, s in res), 3), "unit": "Mpix/s", "cores": n_proc, "kind": "port",
+            "sample": sample + "; C oracle oracle/refcpu.c, one thread per core",
+            "per_core_mpix_s": round(sum(m for m, _ in res) / sum(s for _, s in res), 3)}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3, help="timed passes over the whole 64-stream job")
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--frames", type=int, default=16)
+    ap.add_argument("--plain-steps", type=int, default=1)
+    ap.add_argument("--cpu-procs", type=int, default=8)
+    ap.add_argument("--cpu-frames", type=int, default=8)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--waves", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this engine runs on one GPU; streams would be spread over GPUs by an outer "
+              "scheduler (no collectives involved). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    from h264_video_decoder_demo_amd import Engine, Generator
+
+    specs = stream_specs(args.frames)
+    t = time.perf_counter()
+    with cf.ThreadPoolExecutor(16) as ex:
+        gens = list(ex.map(lambda s: Generator(2, **s), specs))
+    log(f"generated {len(gens)} streams in {time.perf_counter() - t:.1f} s")
+
+    eng = Engine(device=0, profile=True, waves=args.waves)
+    sids, slots, pics = [], {}, []
+    alg = [0, 0, 0]
+    for g in gens:
+        sid = eng.open_stream(g.width_mbs, g.height_mbs, g.dpb_slots)
+        sids.append(sid)
+        slots[sid] = g.dpb_slots
+        row = []
+        for i in range(g.num_pictures):
+            p = eng.begin_raw(sid, *g.picture_raw(i))
+            for k, v in enumerate(eng.alg_bytes(p)):
+                alg[k] += v
+            row.append(p)
+        pics.append(row)
+    nframes = min(len(r) for r in pics)
+    mpix_step = sum(g.width_mbs * g.height_mbs * 256 * nframes for g in gens) / 1e6
+    batches = [eng.batch_create([row[f] for row in pics]) for f in range(nframes)]
+    log(f"staged {sum(len(r) for r in pics)} pictures ({mpix_step:.1f} Mpix per step) in "
+        f"{time.perf_counter() - t:.1f} s")
+
+    # ---- batched: warmup, then K timed passes over the whole job
+    for _ in range(args.warmup):
+        for b in batches:
+            eng.batch_run(b)
+    eng.sync()
+    eng.stage_times()  # reset the per-stage accumulators
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        for b in batches:
+            eng.batch_run(b)
+    eng.sync()
+    t_batched = time.perf_counter() - t0
+    runs, stage_ms = eng.stage_times()
+    batched = mpix_step * args.steps / t_batched
+    h_batched = dpb_hash(eng, sids, slots)
+    log(f"batched: {batched:.1f} Mpix/s ({1e3 * t_batched / args.steps:.2f} ms per step); "
+        f"stages ms/step {[round(x / args.steps, 3) for x in stage_ms]}")
+
+    # ---- batched frame latency: each frame index's batch submitted and waited for alone
+    lat_b = []
+    for _ in range(max(1, args.steps)):
+        for b in batches:
+            t1 = time.perf_counter()
+            eng.batch_run(b)
+            eng.sync()
+            lat_b.append(1e3 * (time.perf_counter() - t1))
+    eng.stage_times()
+
+    plain = None
+    lat_p = []
+    if not args.no_plain:
+        # ---- plain: one picture at a time (one launch per stage per picture)
+        t0 = time.perf_counter()
+        for _ in range(args.plain_steps):
+            for f in range(nframes):
+                for row in pics:
+                    eng.submit(row[f])
+        eng.sync()
+        t_plain = time.perf_counter() - t0
+        plain = mpix_step * args.plain_steps / t_plain
+        h_plain = dpb_hash(eng, sids, slots)
+        if h_plain != h_batched:
+            raise SystemExit("bench.py: batched and plain outputs differ")
+        for f in range(nframes):
+            for row in pics:
+                t1 = time.perf_counter()
+                eng.submit(row[f])
+                eng.sync()
+                lat_p.append(1e3 * (time.perf_counter() - t1))
+        eng.stage_times()
+        log(f"plain: {plain:.1f} Mpix/s; batched/plain = {batched / plain:.1f}x")
+
+    # ---- roofline of the dominant kernel (algorithmic bytes / event-timed duration)
+    dom = max(range(3), key=lambda k: stage_ms[k])
+    launches = args.steps * nframes
+    per_launch_bytes = alg[dom] / nframes
+    avg_ms = stage_ms[dom] / launches
+    achieved = per_launch_bytes / (avg_ms * 1e-3) / 1e9
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE):
+        with open(TRAFFIC_FILE) as f:
+            traffic = json.load(f).get(KERNELS[dom])
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": KERNELS[dom],
+                "alg_bytes_per_launch": int(per_launch_bytes), "avg_launch_ms": round(avg_ms, 4)}
+    stages = {}
+    for k in range(3):
+        ms = stage_ms[k] / launches
+        stages[STAGES[k]] = {"avg_launch_ms": round(ms, 4), "alg_bytes_per_launch": int(alg[k] / nframes),
+                             "alg_GBs": round(alg[k] / nframes / (ms * 1e-3) / 1e9, 1) if ms > 0 else None}
+    mc_dbk = (alg[0] + alg[2]) / nframes / ((stage_ms[0] + stage_ms[2]) / launches * 1e-3) / 1e9
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        t = time.perf_counter()
+        cpu = cpu_baseline(args.cpu_procs, args.cpu_frames)
+        log(f"cpu baseline: {cpu['value']} Mpix/s on {cpu['cores']} cores ({time.perf_counter() - t:.1f} s)")
+
+    out = {
+        "metric": METRIC, "value": round(batched, 1), "unit": "Mpix/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(1e3 * t_batched / args.steps, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "u8", "data": "synthetic",
+        "config": {"workload": "64 streams x 16 frames, mixed 352x288/1280x720/1920x1088/3840x2160 "
+                               "High-profile IPBB syntax, batched launch per stage per frame index",
+                   "streams": len(gens), "frames_per_stream": nframes, "pictures_per_step": len(gens) * nframes,
+                   "mpix_per_step": round(mpix_step, 3)},
+        "plain_value": round(plain, 1) if plain else None,
+        "batched_over_plain": round(batched / plain, 2) if plain else None,
+        "latency_ms": {"batched_frame_p50": round(pct(lat_b, 50), 3), "batched_frame_p99": round(pct(lat_b, 99), 3),
+                       "plain_picture_p50": round(pct(lat_p, 50), 3) if lat_p else None,
+                       "plain_picture_p99": round(pct(lat_p, 99), 3) if lat_p else None},
+        "stages": stages,
+        "mc_deblock_alg_GBs": round(mc_dbk, 1),
+        "bitexact": {"batched_equals_plain": (plain is not None), "dpb_sha256": h_batched[:16]},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    for b in batches:
+        eng.batch_destroy(b)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

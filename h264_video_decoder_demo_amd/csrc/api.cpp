@@ -95,7 +95,8 @@ struct h264g_ctx {
     std::vector<Stream> streams;
     std::vector<h264g_batch *> deferred;  // batches freed at the next sync
     bool profile = false;
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    std::vector<hipEvent_t> ev;    // 4 events per profiled run: inter | intra | deblock | end
+    size_t ev_used = 0;            // events recorded since the last h264g_stage_times
     std::mutex mu;
 };
 
@@ -189,6 +190,30 @@ bool nxn_mode_ok(int mode, bool left, bool top, bool tl) {
     }
 }
 
+// Partitions of an inter MB as (x, y, w, h) in 4x4-block units (mb_pred /
+// sub_mb_pred shapes, Tables 7-13 / 7-17).  Returns their number.
+int partitions(const h264g_mb &m, int out[16][4]) {
+    auto put = [&](int &n, int x, int y, int w, int h) { out[n][0] = x; out[n][1] = y; out[n][2] = w; out[n][3] = h; n++; };
+    int n = 0;
+    const int shape = m.mb_type == H264G_MB_P_SKIP ? H264G_PART_16x16 : m.part_shape;
+    switch (shape) {
+    case H264G_PART_16x16: put(n, 0, 0, 4, 4); break;
+    case H264G_PART_16x8: put(n, 0, 0, 4, 2); put(n, 0, 2, 4, 2); break;
+    case H264G_PART_8x16: put(n, 0, 0, 2, 4); put(n, 2, 0, 2, 4); break;
+    default:
+        for (int q = 0; q < 4; q++) {
+            const int x0 = (q & 1) * 2, y0 = (q >> 1) * 2;
+            switch (m.sub_part_shape[q]) {
+            case H264G_SUB_8x8: put(n, x0, y0, 2, 2); break;
+            case H264G_SUB_8x4: put(n, x0, y0, 2, 1); put(n, x0, y0 + 1, 2, 1); break;
+            case H264G_SUB_4x8: put(n, x0, y0, 1, 2); put(n, x0 + 1, y0, 1, 2); break;
+            default: for (int k = 0; k < 4; k++) put(n, x0 + (k & 1), y0 + (k >> 1), 1, 1); break;
+            }
+        }
+    }
+    return n;
+}
+
 int stage_picture(h264g_ctx *ctx, Stream &st, const h264g_pic_desc &d, const h264g_mb *mbs, uint32_t n_mbs,
                   const int16_t *levels, uint64_t n_levels, h264g_pic *pic) {
     const int W = st.W, H = st.H;
@@ -214,7 +239,7 @@ int stage_picture(h264g_ctx *ctx, Stream &st, const h264g_pic_desc &d, const h26
         std::memset(&w0, 0, sizeof(w0));
         wt.push_back(w0);  // entry 0: intra / unused
     }
-    uint64_t alg_inter = 0, alg_intra = 0;
+    uint64_t alg_inter = 0, alg_intra = 0, n_part_lists = 0;
     for (uint32_t a = 0; a < n_mbs; a++) {
         const h264g_mb &m = mbs[a];
         DevAux &x = aux[a];
@@ -313,6 +338,9 @@ int stage_picture(h264g_ctx *ctx, Stream &st, const h264g_pic_desc &d, const h26
         }
         // ---- inter: reference slots, prediction flags, weight entries
         if (m.part_shape > 3) return h264g_set_error(H264G_E_ARG, "mb %u: part_shape", a);
+        if (m.part_shape == H264G_PART_8x8)
+            for (int q = 0; q < 4; q++)
+                if (m.sub_part_shape[q] > 3) return h264g_set_error(H264G_E_ARG, "mb %u: sub_part_shape", a);
         for (int q = 0; q < 4; q++) {
             int r[2] = {m.ref_idx[0][q], m.ref_idx[1][q]};
             if (r[0] < 0 && r[1] < 0) return h264g_set_error(H264G_E_ARG, "mb %u: quadrant %d predicts from no list", a, q);
@@ -336,16 +364,25 @@ int stage_picture(h264g_ctx *ctx, Stream &st, const h264g_pic_desc &d, const h26
             }
             x.wp[q] = (uint8_t)it->second;
         }
-        // ---- algorithmic bytes of this MB (reference footprints per 4x4 block group)
-        for (int b = 0; b < 16; b++) {
-            int q = (b / 8) * 2 + (b % 4) / 2;
+        // ---- algorithmic bytes of this MB: per partition and list used, the luma
+        // footprint (w + 5) x (h + 5) with the 6-tap halo where the MV is fractional,
+        // the two bilinear chroma footprints, the MB's output and its levels, and the
+        // compact per-partition motion record (8 + 5 bytes per partition-list).
+        int parts[16][4];
+        const int np = partitions(m, parts);
+        for (int k = 0; k < np; k++) {
+            const int px = parts[k][0], py = parts[k][1], pw = parts[k][2], ph = parts[k][3];
+            const int b = py * 4 + px, q = (py >> 1) * 2 + (px >> 1);
             for (int l = 0; l < 2; l++) {
                 if (m.ref_idx[l][q] < 0) continue;
-                int fx = m.mv[l][b][0] & 3, fy = m.mv[l][b][1] & 3, cfx = m.mv[l][b][0] & 7, cfy = m.mv[l][b][1] & 7;
-                alg_inter += (uint64_t)(4 + 5 * (fx != 0)) * (4 + 5 * (fy != 0)) + 2ull * (2 + (cfx != 0)) * (2 + (cfy != 0));
+                const int w = 4 * pw, h = 4 * ph;
+                const int fx = m.mv[l][b][0] & 3, fy = m.mv[l][b][1] & 3, cfx = m.mv[l][b][0] & 7, cfy = m.mv[l][b][1] & 7;
+                alg_inter += (uint64_t)(w + 5 * (fx != 0)) * (h + 5 * (fy != 0)) +
+                             2ull * (w / 2 + (cfx != 0)) * (h / 2 + (cfy != 0)) + 5;
+                n_part_lists++;
             }
         }
-        alg_inter += 384 + 2 * (uint64_t)(nmask + nvals) + sizeof(h264g_mb) + sizeof(DevAux);
+        alg_inter += 384 + 2 * (uint64_t)(nmask + nvals) + 8;
     }
     // ---- queues
     pic->inter_list.clear();
@@ -367,7 +404,9 @@ int stage_picture(h264g_ctx *ctx, Stream &st, const h264g_pic_desc &d, const h26
     pic->n_mbs = n_mbs;
     pic->alg[0] = alg_inter;
     pic->alg[1] = alg_intra;
-    pic->alg[2] = 2ull * 384 * n_mbs + (uint64_t)n_mbs * (sizeof(DevAux) + 4 * 32);
+    // deblocking: one read and one write of the picture, the bS inputs (motion records,
+    // 8 bytes per MB and 5 per partition-list) and 4 bytes of non-zero / QP bits per MB
+    pic->alg[2] = 2ull * 384 * n_mbs + 8ull * n_mbs + 5 * n_part_lists + 4ull * n_mbs;
     // ---- device block: mbs | aux | levels | weights | items | pic
     auto al = [](size_t v) { return (v + 255) & ~(size_t)255; };
     size_t o_mbs = 0, o_aux = al(o_mbs + n_mbs * sizeof(h264g_mb)), o_lv = al(o_aux + n_mbs * sizeof(DevAux));
@@ -408,18 +447,28 @@ int stage_picture(h264g_ctx *ctx, Stream &st, const h264g_pic_desc &d, const h26
 int launch_stages(h264g_ctx *ctx, const DevPic *d_pics, const uint32_t *inter, uint32_t n_inter, const uint32_t *intra,
                   uint32_t n_intra, const uint32_t *dbk, uint32_t n_dbk, bool dbk_off) {
     hipStream_t s = ctx->stream;
-    if (ctx->profile) HIPCHK(hipEventRecord(ctx->ev[0], s));
+    hipEvent_t *ev = nullptr;
+    if (ctx->profile) {
+        while (ctx->ev.size() < ctx->ev_used + 4) {
+            hipEvent_t e;
+            HIPCHK(hipEventCreate(&e));
+            ctx->ev.push_back(e);
+        }
+        ev = &ctx->ev[ctx->ev_used];
+        ctx->ev_used += 4;
+    }
+    if (ev) HIPCHK(hipEventRecord(ev[0], s));
     int e = h264g_launch_inter(d_pics, inter, n_inter, ctx->d_err, s);
     if (e) return h264g_set_error(H264G_E_DEVICE, "inter launch: %s", hipGetErrorString((hipError_t)e));
-    if (ctx->profile) HIPCHK(hipEventRecord(ctx->ev[1], s));
+    if (ev) HIPCHK(hipEventRecord(ev[1], s));
     e = h264g_launch_intra(d_pics, intra, n_intra, &ctx->d_ctl[0], ++ctx->epoch, ctx->d_err, ctx->waves, s);
     if (e) return h264g_set_error(H264G_E_DEVICE, "intra launch: %s", hipGetErrorString((hipError_t)e));
-    if (ctx->profile) HIPCHK(hipEventRecord(ctx->ev[2], s));
+    if (ev) HIPCHK(hipEventRecord(ev[2], s));
     if (!dbk_off) {
         e = h264g_launch_deblock(d_pics, dbk, n_dbk, &ctx->d_ctl[1], ++ctx->epoch, ctx->d_err, ctx->waves, s);
         if (e) return h264g_set_error(H264G_E_DEVICE, "deblock launch: %s", hipGetErrorString((hipError_t)e));
     }
-    if (ctx->profile) HIPCHK(hipEventRecord(ctx->ev[3], s));
+    if (ev) HIPCHK(hipEventRecord(ev[3], s));
     return 0;
 }
 
@@ -460,9 +509,6 @@ int32_t h264g_create(const h264g_config *cfg, h264g_ctx **out) {
         h264g_destroy(c);
         return h264g_set_error(H264G_E_DEVICE, "context allocation failed");
     }
-    if (c->profile)
-        for (auto &e : c->ev)
-            if (hipEventCreate(&e) != hipSuccess) { h264g_destroy(c); return h264g_set_error(H264G_E_DEVICE, "event"); }
     *out = c;
     return H264G_OK;
 }
@@ -690,9 +736,19 @@ int32_t h264g_sync(h264g_ctx *c) {
 
 int32_t h264g_stage_times(h264g_ctx *c, float out[3]) {
     if (!c || !out || !c->profile) return h264g_set_error(H264G_E_STATE, "profiling is off");
+    std::lock_guard<std::mutex> lk(c->mu);
     HIPCHK(hipSetDevice(c->device));
-    for (int i = 0; i < 3; i++) HIPCHK(hipEventElapsedTime(&out[i], c->ev[i], c->ev[i + 1]));
-    return H264G_OK;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    out[0] = out[1] = out[2] = 0.f;
+    const size_t runs = c->ev_used / 4;
+    for (size_t r = 0; r < runs; r++)
+        for (int i = 0; i < 3; i++) {
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, c->ev[4 * r + i], c->ev[4 * r + i + 1]));
+            out[i] += ms;
+        }
+    c->ev_used = 0;
+    return (int32_t)runs;
 }
 
 int32_t h264g_slot_readback(h264g_ctx *c, uint32_t sid, int32_t slot, uint8_t *y, uint8_t *cb, uint8_t *cr) {

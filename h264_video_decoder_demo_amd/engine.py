@@ -86,6 +86,15 @@ class Generator:
     def pictures(self):
         return [self.picture(i) for i in range(self.num_pictures)]
 
+    def picture_raw(self, idx: int):
+        """(PicDesc, mbs pointer, n_mbs, levels pointer, n_levels) into the generator's own
+        storage (valid while the generator lives); no copies."""
+        desc = PicDesc()
+        mbs_p, nmb = c_void_p(), c_uint32()
+        lv_p, nlv = c_void_p(), c_uint64()
+        check(lib().h264g_gen_picture(self._h, idx, byref(desc), byref(mbs_p), byref(nmb), byref(lv_p), byref(nlv)))
+        return desc, mbs_p, nmb.value, lv_p, nlv.value
+
 
 # ------------------------------------------------------------------ engine
 @dataclass
@@ -149,6 +158,13 @@ class Engine:
         W, H, _ = self.streams[sid]
         return StagedPicture(h, sid, desc, W, H)
 
+    def begin_raw(self, sid: int, desc: PicDesc, mbs_p, n_mbs: int, lv_p, n_levels: int) -> StagedPicture:
+        """picture_begin on raw host pointers (e.g. from Generator.picture_raw)."""
+        h = c_void_p()
+        check(lib().h264g_picture_begin(self._h, sid, byref(desc), mbs_p, n_mbs, lv_p, n_levels, byref(h)))
+        W, H, _ = self.streams[sid]
+        return StagedPicture(h, sid, desc, W, H)
+
     def submit(self, pic: StagedPicture):
         check(lib().h264g_picture_submit(self._h, pic.handle))
 
@@ -184,9 +200,10 @@ class Engine:
         check(lib().h264g_sync(self._h))
 
     def stage_times(self):
+        """(runs, [inter_ms, intra_ms, deblock_ms]) summed since the previous call."""
         out = (c_float * 3)()
-        check(lib().h264g_stage_times(self._h, out))
-        return list(out)
+        runs = check(lib().h264g_stage_times(self._h, out))
+        return runs, list(out)
 
     def readback(self, sid: int, slot: int):
         W, H, _ = self.streams[sid]

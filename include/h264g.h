@@ -227,9 +227,11 @@ int32_t h264g_batch_create(h264g_ctx *ctx, h264g_pic *const *pics, uint32_t n_pi
 int32_t h264g_batch_run(h264g_ctx *ctx, h264g_batch *batch);
 int32_t h264g_batch_destroy(h264g_ctx *ctx, h264g_batch *batch);
 
-/* Device-side timing of the last h264g_batch_run / h264g_picture_submit per
- * stage, in milliseconds (HIP events on the launch stream): out[0] inter,
- * out[1] intra, out[2] deblocking.  Requires h264g_config.profile = 1. */
+/* Device-side time per stage, in milliseconds, summed over every
+ * h264g_batch_run / h264g_picture_submit since the previous call (HIP events
+ * recorded around each stage on the launch stream): out[0] inter, out[1] intra,
+ * out[2] deblocking.  Waits for the stream; returns the number of runs summed.
+ * Requires h264g_config.profile = 1. */
 int32_t h264g_stage_times(h264g_ctx *ctx, float out[3]);
 
 /* Wait for all enqueued work; returns the first device error word if any. */
