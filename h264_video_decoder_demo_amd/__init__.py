@@ -7,3 +7,4 @@ parsing stays on the host.  ``decoder.H264VideoDecoder`` mirrors the reference's
 decode/output-callback interface (H264VideoDecoder.h).
 """
 from .engine import Engine, Generator, StagedPicture, decode_stream_plain  # noqa: F401
+from .decoder import H264VideoDecoder, H264Picture, H264_DECODE_ERROR_CODE_FILE_END  # noqa: F401

@@ -1,0 +1,162 @@
+"""Host-side mirror of the reference decoder's public interface.
+
+``H264VideoDecoder`` keeps the shape of CH264VideoDecoder
+(h264_video_decoder_demo/H264VideoDecoder.h:22-45): the user registers an output
+callback with ``set_output_frame_callback_functuin(callback, user_data)`` and calls
+``open(source)``; every decoded picture is handed to
+``callback(picture, user_data, error_code)`` in output (POC) order, a non-zero
+return stops decoding, and a final ``callback(None, user_data,
+H264_DECODE_ERROR_CODE_FILE_END)`` ends the stream (H264VideoDecoder.cpp:340-436).
+The output reorder follows CH264PicturesGOP::getOneOutPicture
+(H264PicturesGOP.cpp:89-165).
+
+Reconstruction runs on the GPU through libh264g; ``source`` supplies the parsed
+per-picture syntax (the generator, or a syntax file written by ``syntax_io``), in
+place of the reference's bitstream parser.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+from .engine import Engine
+
+# H264_DECODE_ERROR_CODE (H264CommonFunc.h:477-484)
+H264_DECODE_ERROR_CODE_UNKNOWN = 0
+H264_DECODE_ERROR_CODE_FILE_END = 1
+H264_DECODE_ERROR_CODE_OPEN_FILE_FAILED = 2
+H264_DECODE_ERROR_CODE_NEED_MORE_DATA = 3
+H264_DECODE_ERROR_CODE_NO_MORE_FRAMES = 4
+
+_SLICE_NAMES = {0: "P", 1: "B", 2: "I"}
+
+
+@dataclass
+class H264Picture:
+    """The fields of CH264Picture::m_picture_frame that output callbacks read
+    (main.cpp:25-31): planar samples with stride = width, sizes and POC."""
+    m_pic_buff_luma: np.ndarray
+    m_pic_buff_cb: np.ndarray
+    m_pic_buff_cr: np.ndarray
+    PicWidthInSamplesL: int
+    PicHeightInSamplesL: int
+    PicOrderCnt: int
+    slice_type: int
+    m_PicNumCnt: int  # decoding order
+
+    @property
+    def slice_type_str(self) -> str:
+        return _SLICE_NAMES.get(self.slice_type % 5, "?")
+
+    def yuv420p(self) -> bytes:
+        """Y, Cb, Cr planes back to back (the layout of CH264PictureBase's buffer)."""
+        return self.m_pic_buff_luma.tobytes() + self.m_pic_buff_cb.tobytes() + self.m_pic_buff_cr.tobytes()
+
+
+class _OutputReorder:
+    """CH264PicturesGOP::getOneOutPicture: holds up to max_num_reorder_frames decoded
+    pictures and releases the smallest POC."""
+
+    def __init__(self, max_num_reorder_frames: int):
+        self.depth = max_num_reorder_frames
+        self.buf = []
+
+    def push(self, pic):
+        if self.depth == 0:
+            return pic
+        if len(self.buf) < self.depth:
+            self.buf.append(pic)
+            return None
+        i = min(range(len(self.buf)), key=lambda k: self.buf[k].PicOrderCnt)
+        if pic.PicOrderCnt < self.buf[i].PicOrderCnt:
+            return pic
+        out, self.buf[i] = self.buf[i], pic
+        return out
+
+    def flush_one(self):
+        if not self.buf:
+            return None
+        i = min(range(len(self.buf)), key=lambda k: self.buf[k].PicOrderCnt)
+        return self.buf.pop(i)
+
+
+class H264VideoDecoder:
+    def __init__(self, engine: Engine | None = None, device: int = 0):
+        self._engine = engine
+        self._device = device
+        self.m_output_frame_callback = None
+        self.m_userData = None
+
+    def init(self) -> int:
+        return 0
+
+    def unInit(self) -> int:
+        return 0
+
+    def set_output_frame_callback_functuin(self, output_frame_callback, userData) -> int:
+        self.m_output_frame_callback = output_frame_callback
+        self.m_userData = userData
+        return 0
+
+    def _emit(self, pic) -> int:
+        if self.m_output_frame_callback is None:
+            return 0
+        return -1 if self.m_output_frame_callback(pic, self.m_userData, 0) != 0 else 0
+
+    def open(self, source) -> int:
+        """Decodes every picture of ``source`` (an object with width_mbs, height_mbs,
+        dpb_slots, num_pictures, num_reorder_frames and picture(i) -> (desc, mbs,
+        levels), e.g. Generator or syntax_io.SyntaxFile, or a path to a syntax file)."""
+        if isinstance(source, str):
+            from .syntax_io import SyntaxFile
+            try:
+                source = SyntaxFile(source)
+            except OSError:
+                if self.m_output_frame_callback is not None:
+                    return self.m_output_frame_callback(None, self.m_userData, H264_DECODE_ERROR_CODE_OPEN_FILE_FAILED)
+                return -1
+        eng = self._engine or Engine(device=self._device)
+        own = self._engine is None
+        W, H = source.width_mbs, source.height_mbs
+        sid = eng.open_stream(W, H, source.dpb_slots)
+        reorder = _OutputReorder(getattr(source, "num_reorder_frames", 0))
+        ret = 0
+        stopped = False
+        try:
+            for i in range(source.num_pictures):
+                desc, mbs, levels = source.picture(i)
+                pic = eng.begin(sid, desc, mbs, levels)
+                eng.submit(pic)
+                eng.sync()
+                y, cb, cr = eng.readback(sid, desc.dst_slot)  # host-visible before any callback
+                eng.release(pic)
+                if i > 0 and source.is_idr(i):
+                    while True:  # an IDR flushes the reorder buffer (H264VideoDecoder.cpp:383-410)
+                        out = reorder.flush_one()
+                        if out is None:
+                            break
+                        if self._emit(out):
+                            stopped = True
+                            break
+                    if stopped:
+                        break
+                out = reorder.push(H264Picture(y, cb, cr, W * 16, H * 16, desc.poc, desc.slice_type, i))
+                if out is not None and self._emit(out):
+                    stopped = True
+                    break
+            if not stopped:
+                while True:
+                    out = reorder.flush_one()
+                    if out is None:
+                        break
+                    if self._emit(out):
+                        stopped = True
+                        break
+        finally:
+            eng.close_stream(sid)
+            if own:
+                eng.close()
+        if self.m_output_frame_callback is not None:
+            ret = self.m_output_frame_callback(None, self.m_userData, H264_DECODE_ERROR_CODE_FILE_END)
+        return ret

@@ -69,6 +69,19 @@ class Generator:
     def digest(self) -> int:
         return lib().h264g_gen_digest(self._h)
 
+    @property
+    def num_reorder_frames(self) -> int:
+        """Output reorder depth of the stream (B pictures between anchors)."""
+        return self.params.b_frames
+
+    def is_idr(self, idx: int) -> bool:
+        """The generator codes every I picture as an IDR picture."""
+        desc = PicDesc()
+        mbs_p, nmb = c_void_p(), c_uint32()
+        lv_p, nlv = c_void_p(), c_uint64()
+        check(lib().h264g_gen_picture(self._h, idx, byref(desc), byref(mbs_p), byref(nmb), byref(lv_p), byref(nlv)))
+        return desc.slice_type == _lib.SLICE_I
+
     def picture(self, idx: int):
         """(PicDesc, mbs structured array, levels int16 array), copies owned by the caller."""
         desc = PicDesc()

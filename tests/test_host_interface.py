@@ -1,0 +1,112 @@
+"""The host-side mirror of the reference's decode / output-callback interface and
+the syntax stream files (CPU parts), plus the GPU decode through it."""
+import numpy as np
+import pytest
+
+from h264_video_decoder_demo_amd import Generator
+from h264_video_decoder_demo_amd.decoder import (H264_DECODE_ERROR_CODE_FILE_END, H264VideoDecoder,
+                                                 _OutputReorder)
+from h264_video_decoder_demo_amd.syntax_io import SyntaxFile, save_stream
+
+from helpers import make_generator, plane_hashes
+
+
+class _P:
+    def __init__(self, poc):
+        self.PicOrderCnt = poc
+
+
+def _run_reorder(depth, pocs, idr=()):
+    r = _OutputReorder(depth)
+    out = []
+    for i, poc in enumerate(pocs):
+        if i in idr:
+            while (o := r.flush_one()) is not None:
+                out.append(o.PicOrderCnt)
+        o = r.push(_P(poc))
+        if o is not None:
+            out.append(o.PicOrderCnt)
+    while (o := r.flush_one()) is not None:
+        out.append(o.PicOrderCnt)
+    return out
+
+
+def test_reorder_outputs_ipbb_in_poc_order():
+    # decoding order I0 P6 B2 B4 P12 B8 B10 P18 B14 B16 (POC = 2 x display index)
+    dec = [0, 6, 2, 4, 12, 8, 10, 18, 14, 16]
+    assert _run_reorder(2, dec) == sorted(dec)
+
+
+def test_reorder_depth_zero_is_decoding_order():
+    assert _run_reorder(0, [0, 2, 4, 6]) == [0, 2, 4, 6]
+
+
+def test_reorder_idr_flushes_the_previous_gop():
+    dec = [0, 6, 2, 4, 0, 6, 2, 4]  # two GOPs, POC restarts at the IDR (index 4)
+    assert _run_reorder(2, dec, idr={4}) == [0, 2, 4, 6, 0, 2, 4, 6]
+
+
+def test_generator_output_order_is_poc_order():
+    g = Generator(2, width_mbs=4, height_mbs=3)
+    pocs = [g.picture(i)[0].poc for i in range(g.num_pictures)]
+    assert _run_reorder(g.num_reorder_frames, pocs) == sorted(pocs)
+
+
+def test_syntax_file_round_trip(tmp_path):
+    g = Generator(2, width_mbs=6, height_mbs=5, ipcm_permille=20)
+    path = str(tmp_path / "s.h264g.npz")
+    save_stream(g, path)
+    f = SyntaxFile(path)
+    assert (f.width_mbs, f.height_mbs, f.dpb_slots, f.num_reorder_frames, f.num_pictures) == \
+        (g.width_mbs, g.height_mbs, g.dpb_slots, g.num_reorder_frames, g.num_pictures)
+    for i in range(g.num_pictures):
+        d0, m0, l0 = g.picture(i)
+        d1, m1, l1 = f.picture(i)
+        assert bytes(d0) == bytes(d1)
+        assert m0.tobytes() == m1.tobytes()
+        np.testing.assert_array_equal(l0, l1)
+        assert f.is_idr(i) == g.is_idr(i)
+
+
+@pytest.mark.gpu
+def test_decoder_outputs_reference_pictures_in_poc_order(engine, golden, tmp_path):
+    case = golden["high_small"]
+    gen = make_generator(case)
+    poc_of = {i: gen.picture(i)[0].poc for i in range(gen.num_pictures)}
+    seen = []
+
+    def cb(pic, user, err):
+        if pic is None:
+            user.append(("end", err))
+            return 0
+        user.append((pic.m_PicNumCnt, pic.PicOrderCnt, plane_hashes(pic.m_pic_buff_luma, pic.m_pic_buff_cb,
+                                                                    pic.m_pic_buff_cr)))
+        assert (pic.PicWidthInSamplesL, pic.PicHeightInSamplesL) == (gen.width_mbs * 16, gen.height_mbs * 16)
+        return 0
+
+    dec = H264VideoDecoder(engine=engine)
+    assert dec.set_output_frame_callback_functuin(cb, seen) == 0
+    path = str(tmp_path / "s.h264g.npz")
+    save_stream(gen, path)
+    assert dec.open(path) == 0
+    assert seen[-1] == ("end", H264_DECODE_ERROR_CODE_FILE_END)
+    frames = seen[:-1]
+    assert [f[1] for f in frames] == sorted(poc_of.values())
+    for idx, poc, hashes in frames:
+        assert poc == poc_of[idx]
+        assert hashes == case["pic"][idx], f"picture {idx}"
+
+
+@pytest.mark.gpu
+def test_decoder_stops_when_the_callback_returns_nonzero(engine):
+    gen = Generator(1, width_mbs=4, height_mbs=4, n_frames=5, b_frames=0)
+    calls = []
+
+    def cb(pic, user, err):
+        calls.append(None if pic is None else pic.PicOrderCnt)
+        return 1 if pic is not None and len(calls) == 2 else 0
+
+    dec = H264VideoDecoder(engine=engine)
+    dec.set_output_frame_callback_functuin(cb, None)
+    dec.open(gen)
+    assert calls == [0, 2, None]
