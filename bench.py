@@ -24,7 +24,6 @@ import concurrent.futures as cf
 import hashlib
 import json
 import os
-import statistics
 import subprocess
 import sys
 import tempfile
@@ -129,7 +128,13 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--waves", type=int, default=0)
+    ap.add_argument("--profile-run", action="store_true",
+                    help="short run for rocprofv3: 1 timed step, no warmup, no latency / plain / CPU legs, "
+                         "then the library's calibration copy of --copy-mib MiB")
+    ap.add_argument("--copy-mib", type=int, default=1024)
     args = ap.parse_args()
+    if args.profile_run:
+        args.steps, args.warmup, args.no_plain, args.no_cpu_baseline = 1, 0, True, True
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this engine runs on one GPU; streams would be spread over GPUs by an outer "
               "scheduler (no collectives involved). Run with --gpus 1.", file=sys.stderr)
@@ -180,6 +185,14 @@ def main():
     h_batched = dpb_hash(eng, sids, slots)
     log(f"batched: {batched:.1f} Mpix/s ({1e3 * t_batched / args.steps:.2f} ms per step); "
         f"stages ms/step {[round(x / args.steps, 3) for x in stage_ms]}")
+
+    if args.profile_run:
+        from h264_video_decoder_demo_amd._lib import check, lib
+        check(lib().h264g_calibration_copy(eng._h, args.copy_mib << 20, 1))
+        log(f"calibration copy of {args.copy_mib} MiB done")
+        print(json.dumps({"profile_run": True, "mpix_per_step": mpix_step, "seconds": t_batched,
+                          "alg_bytes_per_step": alg, "copy_bytes": args.copy_mib << 20}), flush=True)
+        return
 
     # ---- batched frame latency: each frame index's batch submitted and waited for alone
     lat_b = []

@@ -751,6 +751,25 @@ int32_t h264g_stage_times(h264g_ctx *c, float out[3]) {
     return (int32_t)runs;
 }
 
+int32_t h264g_calibration_copy(h264g_ctx *c, uint64_t bytes, int32_t reps) {
+    if (!c || bytes == 0 || bytes % 16 || reps < 1) return h264g_set_error(H264G_E_ARG, "calibration copy arguments");
+    std::lock_guard<std::mutex> lk(c->mu);
+    HIPCHK(hipSetDevice(c->device));
+    void *a = nullptr, *b = nullptr;
+    if (hipMalloc(&a, bytes) != hipSuccess) return h264g_set_error(H264G_E_NOMEM, "calibration buffer");
+    if (hipMalloc(&b, bytes) != hipSuccess) { (void)hipFree(a); return h264g_set_error(H264G_E_NOMEM, "calibration buffer"); }
+    int rc = 0;
+    if (hipMemsetAsync(a, 1, bytes, c->stream) != hipSuccess) rc = h264g_set_error(H264G_E_DEVICE, "calibration memset");
+    for (int r = 0; r < reps && !rc; r++) {
+        int e = h264g_launch_copy((const uint8_t *)a, (uint8_t *)b, bytes, c->stream);
+        if (e) rc = h264g_set_error(H264G_E_DEVICE, "copy launch: %s", hipGetErrorString((hipError_t)e));
+    }
+    if (hipStreamSynchronize(c->stream) != hipSuccess && !rc) rc = h264g_set_error(H264G_E_DEVICE, "calibration sync");
+    (void)hipFree(a);
+    (void)hipFree(b);
+    return rc;
+}
+
 int32_t h264g_slot_readback(h264g_ctx *c, uint32_t sid, int32_t slot, uint8_t *y, uint8_t *cb, uint8_t *cr) {
     if (!c || !y || !cb || !cr) return h264g_set_error(H264G_E_ARG, "NULL argument");
     std::lock_guard<std::mutex> lk(c->mu);

@@ -252,6 +252,12 @@ int32_t h264g_picture_release(h264g_ctx *ctx, h264g_pic *pic);
  * out[0] = MC + residual (inter), out[1] = intra, out[2] = deblocking. */
 int32_t h264g_picture_alg_bytes(const h264g_pic *pic, uint64_t out[3]);
 
+/* Measurement aid with no counterpart in the reference: runs the library's plain
+ * copy kernel (16-byte loads and stores) `reps` times over a device buffer of
+ * `bytes` (a multiple of 16) into another, and waits.  Its HBM counter readings
+ * calibrate the FETCH_SIZE / WRITE_SIZE of the reconstruction kernels. */
+int32_t h264g_calibration_copy(h264g_ctx *ctx, uint64_t bytes, int32_t reps);
+
 /* Last error message of the calling thread (static storage). */
 const char *h264g_last_error(void);
 
