@@ -88,8 +88,7 @@ struct h264g_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     uint32_t *d_err = nullptr;
-    DevCtl *d_ctl = nullptr;       // [0] intra, [1] deblock
-    uint32_t epoch = 0;
+    DevCtl *d_ctl = nullptr;       // [0] intra, [1] deblock, [2].ticket = shared launch epoch
     int waves = 2048;
     uint64_t budget = 0, used = 0;
     std::vector<Stream> streams;
@@ -461,11 +460,11 @@ int launch_stages(h264g_ctx *ctx, const DevPic *d_pics, const uint32_t *inter, u
     int e = h264g_launch_inter(d_pics, inter, n_inter, ctx->d_err, s);
     if (e) return h264g_set_error(H264G_E_DEVICE, "inter launch: %s", hipGetErrorString((hipError_t)e));
     if (ev) HIPCHK(hipEventRecord(ev[1], s));
-    e = h264g_launch_intra(d_pics, intra, n_intra, &ctx->d_ctl[0], ++ctx->epoch, ctx->d_err, ctx->waves, s);
+    e = h264g_launch_intra(d_pics, intra, n_intra, &ctx->d_ctl[0], &ctx->d_ctl[2].ticket, ctx->d_err, ctx->waves, s);
     if (e) return h264g_set_error(H264G_E_DEVICE, "intra launch: %s", hipGetErrorString((hipError_t)e));
     if (ev) HIPCHK(hipEventRecord(ev[2], s));
     if (!dbk_off) {
-        e = h264g_launch_deblock(d_pics, dbk, n_dbk, &ctx->d_ctl[1], ++ctx->epoch, ctx->d_err, ctx->waves, s);
+        e = h264g_launch_deblock(d_pics, dbk, n_dbk, &ctx->d_ctl[1], &ctx->d_ctl[2].ticket, ctx->d_err, ctx->waves, s);
         if (e) return h264g_set_error(H264G_E_DEVICE, "deblock launch: %s", hipGetErrorString((hipError_t)e));
     }
     if (ev) HIPCHK(hipEventRecord(ev[3], s));
@@ -484,6 +483,9 @@ int free_batch(h264g_batch *b) {
 extern "C" {
 
 const char *h264g_last_error(void) { return g_err; }
+
+// Progress words start at 0, so the first wavefront launch uses epoch 1.
+static const uint32_t kFirstEpoch = 1;
 
 int32_t h264g_create(const h264g_config *cfg, h264g_ctx **out) {
     if (!out) return h264g_set_error(H264G_E_ARG, "out is NULL");
@@ -505,7 +507,8 @@ int32_t h264g_create(const h264g_config *cfg, h264g_ctx **out) {
     c->waves = (cfg && cfg->waves > 0) ? cfg->waves : prop.multiProcessorCount * 8;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipMalloc(&c->d_err, sizeof(uint32_t)) != hipSuccess || hipMalloc(&c->d_ctl, 4 * sizeof(DevCtl)) != hipSuccess ||
-        hipMemset(c->d_err, 0, sizeof(uint32_t)) != hipSuccess || hipMemset(c->d_ctl, 0, 4 * sizeof(DevCtl)) != hipSuccess) {
+        hipMemset(c->d_err, 0, sizeof(uint32_t)) != hipSuccess || hipMemset(c->d_ctl, 0, 4 * sizeof(DevCtl)) != hipSuccess ||
+        hipMemcpy(&c->d_ctl[2].ticket, &kFirstEpoch, sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess) {
         h264g_destroy(c);
         return h264g_set_error(H264G_E_DEVICE, "context allocation failed");
     }

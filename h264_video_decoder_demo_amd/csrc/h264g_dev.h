@@ -70,9 +70,11 @@ struct DevCtl {
 extern "C" {
 // Kernel launchers (kernels.hip); all enqueue on `stream` and return a hipError_t.
 int h264g_launch_inter(const DevPic *pics, const uint32_t *items, uint32_t n, uint32_t *err, void *stream);
-int h264g_launch_intra(const DevPic *pics, const uint32_t *items, uint32_t n, DevCtl *ctl, uint32_t epoch,
+// `epoch` points at the device word shared by all wavefront launches of a context
+// (initially 1; each launch's last wave advances it).
+int h264g_launch_intra(const DevPic *pics, const uint32_t *items, uint32_t n, DevCtl *ctl, uint32_t *epoch,
                        uint32_t *err, int waves, void *stream);
-int h264g_launch_deblock(const DevPic *pics, const uint32_t *items, uint32_t n, DevCtl *ctl, uint32_t epoch,
+int h264g_launch_deblock(const DevPic *pics, const uint32_t *items, uint32_t n, DevCtl *ctl, uint32_t *epoch,
                          uint32_t *err, int waves, void *stream);
 int h264g_launch_copy(const uint8_t *src, uint8_t *dst, uint64_t bytes, void *stream);
 }

@@ -431,12 +431,23 @@ __device__ int next_ticket(DevCtl *ctl, uint32_t n) {
     return t < n ? (int)t : -1;
 }
 
-// The last wave of a launch resets the control block for the next launch on the stream.
-__device__ void retire(DevCtl *ctl, uint32_t total_waves) {
+// The launch's epoch: progress words equal to it are this launch's.  Read once at
+// kernel start; it only changes when the launch's last wave retires, after every
+// wave of the launch has read it.
+__device__ __forceinline__ uint32_t launch_epoch(const uint32_t *epoch) {
+    return uniform(__hip_atomic_load(epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+}
+
+// The last wave of a launch resets the control block for the next launch on the stream
+// and advances the shared epoch, so the next wavefront launch (of either kind) uses a
+// value no progress word holds yet.  Launch arguments therefore stay identical from one
+// launch to the next, which lets a captured graph replay them.
+__device__ void retire(DevCtl *ctl, uint32_t total_waves, uint32_t *epoch, uint32_t cur) {
     const uint32_t d = uniform(atomicAdd(&ctl->done, wave_lane() == 0 ? 1u : 0u));
     if (d == total_waves - 1) {
         __hip_atomic_store(&ctl->ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __hip_atomic_store(&ctl->done, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(epoch, cur + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
@@ -941,8 +952,9 @@ __global__ __launch_bounds__(WAVE) void k_inter(const DevPic *__restrict__ pics,
 }
 
 __global__ __launch_bounds__(WAVE) void k_intra(const DevPic *__restrict__ pics, const uint32_t *__restrict__ items, uint32_t n,
-                                                DevCtl *ctl, uint32_t epoch, uint32_t *err) {
+                                                DevCtl *ctl, uint32_t *epoch_p, uint32_t *err) {
     __shared__ WaveIntra lds;
+    const uint32_t epoch = launch_epoch(epoch_p);
     for (;;) {
         int t = next_ticket(ctl, n);
         if (t < 0) break;
@@ -964,12 +976,13 @@ __global__ __launch_bounds__(WAVE) void k_intra(const DevPic *__restrict__ pics,
         intra_mb(P, addr, &lds, av);
         publish(&P.flags[addr], epoch);
     }
-    retire(ctl, gridDim.x);
+    retire(ctl, gridDim.x, epoch_p, epoch);
 }
 
 __global__ __launch_bounds__(WAVE) void k_deblock(const DevPic *__restrict__ pics, const uint32_t *__restrict__ items, uint32_t n,
-                                                  DevCtl *ctl, uint32_t epoch, uint32_t *err) {
+                                                  DevCtl *ctl, uint32_t *epoch_p, uint32_t *err) {
     __shared__ WaveDbk lds;
+    const uint32_t epoch = launch_epoch(epoch_p);
     for (;;) {
         int t = next_ticket(ctl, n);
         if (t < 0) break;
@@ -981,7 +994,7 @@ __global__ __launch_bounds__(WAVE) void k_deblock(const DevPic *__restrict__ pic
         deblock_mb(P, addr, &lds);
         publish(&P.flags[addr], epoch);
     }
-    retire(ctl, gridDim.x);
+    retire(ctl, gridDim.x, epoch_p, epoch);
 }
 
 __global__ void k_copy(const uint4 *src, uint4 *dst, uint64_t n16) {
@@ -998,7 +1011,7 @@ int h264g_launch_inter(const DevPic *pics, const uint32_t *items, uint32_t n, ui
     return (int)hipGetLastError();
 }
 
-int h264g_launch_intra(const DevPic *pics, const uint32_t *items, uint32_t n, DevCtl *ctl, uint32_t epoch, uint32_t *err,
+int h264g_launch_intra(const DevPic *pics, const uint32_t *items, uint32_t n, DevCtl *ctl, uint32_t *epoch, uint32_t *err,
                        int waves, void *stream) {
     if (n == 0) return 0;
     uint32_t g = (uint32_t)waves < n ? (uint32_t)waves : n;
@@ -1006,7 +1019,7 @@ int h264g_launch_intra(const DevPic *pics, const uint32_t *items, uint32_t n, De
     return (int)hipGetLastError();
 }
 
-int h264g_launch_deblock(const DevPic *pics, const uint32_t *items, uint32_t n, DevCtl *ctl, uint32_t epoch, uint32_t *err,
+int h264g_launch_deblock(const DevPic *pics, const uint32_t *items, uint32_t n, DevCtl *ctl, uint32_t *epoch, uint32_t *err,
                          int waves, void *stream) {
     if (n == 0) return 0;
     uint32_t g = (uint32_t)waves < n ? (uint32_t)waves : n;
