@@ -128,6 +128,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--waves", type=int, default=0)
+    ap.add_argument("--no-graphs", action="store_true", help="launch batch stages directly instead of hipGraph replay")
     ap.add_argument("--profile-run", action="store_true",
                     help="short run for rocprofv3: 1 timed step, no warmup, no latency / plain / CPU legs, "
                          "then the library's calibration copy of --copy-mib MiB")
@@ -148,7 +149,7 @@ def main():
         gens = list(ex.map(lambda s: Generator(2, **s), specs))
     log(f"generated {len(gens)} streams in {time.perf_counter() - t:.1f} s")
 
-    eng = Engine(device=0, profile=True, waves=args.waves)
+    eng = Engine(device=0, profile=False, waves=args.waves, graphs=not args.no_graphs)
     sids, slots, pics = [], {}, []
     alg = [0, 0, 0]
     for g in gens:
@@ -168,23 +169,35 @@ def main():
     log(f"staged {sum(len(r) for r in pics)} pictures ({mpix_step:.1f} Mpix per step) in "
         f"{time.perf_counter() - t:.1f} s")
 
-    # ---- batched: warmup, then K timed passes over the whole job
-    for _ in range(args.warmup):
+    # ---- batched: warmup (captures each batch's stage graph), then K timed passes
+    for _ in range(max(1, args.warmup)):
         for b in batches:
             eng.batch_run(b)
     eng.sync()
-    eng.stage_times()  # reset the per-stage accumulators
     t0 = time.perf_counter()
     for _ in range(args.steps):
         for b in batches:
             eng.batch_run(b)
     eng.sync()
     t_batched = time.perf_counter() - t0
-    runs, stage_ms = eng.stage_times()
     batched = mpix_step * args.steps / t_batched
     h_batched = dpb_hash(eng, sids, slots)
-    log(f"batched: {batched:.1f} Mpix/s ({1e3 * t_batched / args.steps:.2f} ms per step); "
-        f"stages ms/step {[round(x / args.steps, 3) for x in stage_ms]}")
+    log(f"batched: {batched:.1f} Mpix/s ({1e3 * t_batched / args.steps:.2f} ms per step)")
+
+    # ---- the same K passes with HIP events around every stage launch (direct launches):
+    # per-stage kernel durations for the roofline
+    eng.set_profile(True)
+    eng.stage_times()  # reset the accumulators
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        for b in batches:
+            eng.batch_run(b)
+    eng.sync()
+    t_events = time.perf_counter() - t0
+    runs, stage_ms = eng.stage_times()
+    eng.set_profile(False)
+    log(f"batched with events: {mpix_step * args.steps / t_events:.1f} Mpix/s; "
+        f"stage ms per step {[round(x / args.steps, 3) for x in stage_ms]}")
 
     if args.profile_run:
         from h264_video_decoder_demo_amd._lib import check, lib
@@ -202,7 +215,6 @@ def main():
             eng.batch_run(b)
             eng.sync()
             lat_b.append(1e3 * (time.perf_counter() - t1))
-    eng.stage_times()
 
     plain = None
     lat_p = []
@@ -225,7 +237,6 @@ def main():
                 eng.submit(row[f])
                 eng.sync()
                 lat_p.append(1e3 * (time.perf_counter() - t1))
-        eng.stage_times()
         log(f"plain: {plain:.1f} Mpix/s; batched/plain = {batched / plain:.1f}x")
 
     # ---- roofline of the dominant kernel (algorithmic bytes / event-timed duration)
@@ -262,6 +273,8 @@ def main():
                                "High-profile IPBB syntax, batched launch per stage per frame index",
                    "streams": len(gens), "frames_per_stream": nframes, "pictures_per_step": len(gens) * nframes,
                    "mpix_per_step": round(mpix_step, 3)},
+        "graphs": not args.no_graphs,
+        "batched_event_timed_value": round(mpix_step * args.steps / t_events, 1),
         "plain_value": round(plain, 1) if plain else None,
         "batched_over_plain": round(batched / plain, 2) if plain else None,
         "latency_ms": {"batched_frame_p50": round(pct(lat_b, 50), 3), "batched_frame_p99": round(pct(lat_b, 99), 3),

@@ -80,7 +80,7 @@ class StreamDesc(ctypes.Structure):
 class Config(ctypes.Structure):
     _fields_ = [
         ("device", c_int32), ("dpb_budget_bytes", c_uint64), ("max_batch_pictures", c_int32),
-        ("profile", c_int32), ("waves", c_int32),
+        ("profile", c_int32), ("waves", c_int32), ("no_graphs", c_int32),
     ]
 
 
@@ -111,6 +111,7 @@ _SIGNATURES = {
     "h264g_batch_run": (c_int32, [c_void_p, c_void_p]),
     "h264g_batch_destroy": (c_int32, [c_void_p, c_void_p]),
     "h264g_stage_times": (c_int32, [c_void_p, POINTER(c_float)]),
+    "h264g_set_profile": (c_int32, [c_void_p, c_int32]),
     "h264g_sync": (c_int32, [c_void_p]),
     "h264g_slot_readback": (c_int32, [c_void_p, c_uint32, c_int32, c_void_p, c_void_p, c_void_p]),
     "h264g_slot_upload": (c_int32, [c_void_p, c_uint32, c_int32, c_void_p, c_void_p, c_void_p]),

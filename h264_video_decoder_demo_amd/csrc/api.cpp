@@ -79,6 +79,7 @@ struct Batch {
     DevPic *d_pics = nullptr;
     uint32_t *d_inter = nullptr, *d_intra = nullptr, *d_dbk = nullptr;
     uint32_t n_inter = 0, n_intra = 0, n_dbk = 0;
+    hipGraphExec_t graph = nullptr;  // the captured stage sequence (kernel nodes only)
 };
 }  // namespace
 
@@ -94,6 +95,7 @@ struct h264g_ctx {
     std::vector<Stream> streams;
     std::vector<h264g_batch *> deferred;  // batches freed at the next sync
     bool profile = false;
+    bool no_graphs = false;
     std::vector<hipEvent_t> ev;    // 4 events per profiled run: inter | intra | deblock | end
     size_t ev_used = 0;            // events recorded since the last h264g_stage_times
     std::mutex mu;
@@ -473,6 +475,7 @@ int launch_stages(h264g_ctx *ctx, const DevPic *d_pics, const uint32_t *inter, u
 
 int free_batch(h264g_batch *b) {
     if (!b) return 0;
+    if (b->graph) (void)hipGraphExecDestroy(b->graph);
     if (b->dmem) (void)hipFree(b->dmem);
     delete b;
     return 0;
@@ -504,6 +507,7 @@ int32_t h264g_create(const h264g_config *cfg, h264g_ctx **out) {
     c->device = dev;
     c->budget = cfg ? cfg->dpb_budget_bytes : 0;
     c->profile = cfg && cfg->profile;
+    c->no_graphs = cfg && cfg->no_graphs;
     c->waves = (cfg && cfg->waves > 0) ? cfg->waves : prop.multiProcessorCount * 8;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipMalloc(&c->d_err, sizeof(uint32_t)) != hipSuccess || hipMalloc(&c->d_ctl, 4 * sizeof(DevCtl)) != hipSuccess ||
@@ -700,7 +704,23 @@ int32_t h264g_batch_run(h264g_ctx *c, h264g_batch *b) {
     if (!c || !b) return h264g_set_error(H264G_E_ARG, "NULL argument");
     std::lock_guard<std::mutex> lk(c->mu);
     HIPCHK(hipSetDevice(c->device));
-    return launch_stages(c, b->d_pics, b->d_inter, b->n_inter, b->d_intra, b->n_intra, b->d_dbk, b->n_dbk, b->n_dbk == 0);
+    if (c->profile || c->no_graphs)
+        return launch_stages(c, b->d_pics, b->d_inter, b->n_inter, b->d_intra, b->n_intra, b->d_dbk, b->n_dbk, b->n_dbk == 0);
+    if (!b->graph) {
+        // Capture the three stage launches once; every launch argument (including the
+        // wavefront epoch, which lives in device memory) is the same on each replay.
+        HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        int rc = launch_stages(c, b->d_pics, b->d_inter, b->n_inter, b->d_intra, b->n_intra, b->d_dbk, b->n_dbk, b->n_dbk == 0);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(c->stream, &g);
+        if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+        if (e != hipSuccess) return h264g_set_error(H264G_E_DEVICE, "stream capture: %s", hipGetErrorString(e));
+        e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) { b->graph = nullptr; return h264g_set_error(H264G_E_DEVICE, "graph instantiate: %s", hipGetErrorString(e)); }
+    }
+    HIPCHK(hipGraphLaunch(b->graph, c->stream));
+    return H264G_OK;
 }
 
 int32_t h264g_batch_destroy(h264g_ctx *c, h264g_batch *b) {
@@ -734,6 +754,13 @@ int32_t h264g_sync(h264g_ctx *c) {
         HIPCHK(hipMemset(c->d_err, 0, 4));
         return h264g_set_error(H264G_E_TIMEOUT, "device error word 0x%x (wavefront wait exceeded its spin limit)", err);
     }
+    return H264G_OK;
+}
+
+int32_t h264g_set_profile(h264g_ctx *c, int32_t on) {
+    if (!c) return h264g_set_error(H264G_E_ARG, "ctx");
+    std::lock_guard<std::mutex> lk(c->mu);
+    c->profile = on != 0;
     return H264G_OK;
 }
 

@@ -123,9 +123,9 @@ class Engine:
     """One libh264g context on one HIP device (one HIP stream inside)."""
 
     def __init__(self, device: int = 0, dpb_budget_bytes: int = 0, profile: bool = False, waves: int = 0,
-                 max_batch_pictures: int = 0):
+                 max_batch_pictures: int = 0, graphs: bool = True):
         cfg = Config(device=device, dpb_budget_bytes=dpb_budget_bytes, max_batch_pictures=max_batch_pictures,
-                     profile=1 if profile else 0, waves=waves)
+                     profile=1 if profile else 0, waves=waves, no_graphs=0 if graphs else 1)
         h = c_void_p()
         check(lib().h264g_create(byref(cfg), byref(h)))
         self._h = h
@@ -211,6 +211,9 @@ class Engine:
     # ---- completion and data
     def sync(self):
         check(lib().h264g_sync(self._h))
+
+    def set_profile(self, on: bool):
+        check(lib().h264g_set_profile(self._h, 1 if on else 0))
 
     def stage_times(self):
         """(runs, [inter_ms, intra_ms, deblock_ms]) summed since the previous call."""

@@ -139,6 +139,9 @@ typedef struct h264g_config {
     int32_t  max_batch_pictures;  /* pictures per batched launch (0 = default 4096)   */
     int32_t  profile;             /* 1 = record per-stage HIP events                  */
     int32_t  waves;               /* persistent waves per wavefront launch (0 = auto) */
+    int32_t  no_graphs;           /* 1 = launch batch stages directly; 0 = replay each
+                                     batch's stage sequence as a captured hipGraph
+                                     (profiled contexts always launch directly)       */
 } h264g_config;
 
 typedef struct h264g_stream_desc {
@@ -233,6 +236,11 @@ int32_t h264g_batch_destroy(h264g_ctx *ctx, h264g_batch *batch);
  * out[2] deblocking.  Waits for the stream; returns the number of runs summed.
  * Requires h264g_config.profile = 1. */
 int32_t h264g_stage_times(h264g_ctx *ctx, float out[3]);
+
+/* Turn per-stage event recording on or off for later launches (as
+ * h264g_config.profile).  While it is on, batches launch their stages directly
+ * instead of replaying their captured graphs. */
+int32_t h264g_set_profile(h264g_ctx *ctx, int32_t on);
 
 /* Wait for all enqueued work; returns the first device error word if any. */
 int32_t h264g_sync(h264g_ctx *ctx);
