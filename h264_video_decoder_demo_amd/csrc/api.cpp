@@ -46,6 +46,7 @@ struct Stream {
     uint8_t *dpb = nullptr;
     uint64_t slot_bytes = 0;
     uint32_t *flags = nullptr;
+    uint64_t *rowprog = nullptr;
     uint64_t bytes = 0;
     bool open = false;
 };
@@ -64,9 +65,9 @@ struct h264g_pic {
     DevAux *d_aux = nullptr;
     int16_t *d_levels = nullptr;
     DevWeight *d_wt = nullptr;
-    uint32_t *d_items = nullptr;   // inter | intra | deblock queues (picture index 0)
+    uint32_t *d_items = nullptr;   // inter | intra | deblock (per MB) | deblock (rows) queues
     DevPic *d_pic = nullptr;
-    uint32_t n_inter = 0, n_intra = 0, n_mbs = 0;
+    uint32_t n_inter = 0, n_intra = 0, n_mbs = 0, n_rows = 0;
     DevPic hpic;                   // host copy (batches re-pack it)
     std::vector<uint32_t> intra_t, all_t;  // host copies of the t-ordered queues (MB addresses)
     std::vector<uint32_t> inter_list;
@@ -79,6 +80,7 @@ struct Batch {
     DevPic *d_pics = nullptr;
     uint32_t *d_inter = nullptr, *d_intra = nullptr, *d_dbk = nullptr;
     uint32_t n_inter = 0, n_intra = 0, n_dbk = 0;
+    bool dbk_rows = false;           // d_dbk holds MB rows (row-pipelined deblocking)
     hipGraphExec_t graph = nullptr;  // the captured stage sequence (kernel nodes only)
 };
 }  // namespace
@@ -96,6 +98,7 @@ struct h264g_ctx {
     std::vector<h264g_batch *> deferred;  // batches freed at the next sync
     bool profile = false;
     bool no_graphs = false;
+    bool dbk_per_mb = false;       // deblocking as a per-MB wavefront instead of MB rows
     std::vector<hipEvent_t> ev;    // 4 events per profiled run: inter | intra | deblock | end
     size_t ev_used = 0;            // events recorded since the last h264g_stage_times
     std::mutex mu;
@@ -412,7 +415,8 @@ int stage_picture(h264g_ctx *ctx, Stream &st, const h264g_pic_desc &d, const h26
     auto al = [](size_t v) { return (v + 255) & ~(size_t)255; };
     size_t o_mbs = 0, o_aux = al(o_mbs + n_mbs * sizeof(h264g_mb)), o_lv = al(o_aux + n_mbs * sizeof(DevAux));
     size_t o_wt = al(o_lv + n_levels * sizeof(int16_t) + 16), o_it = al(o_wt + wt.size() * sizeof(DevWeight));
-    size_t n_items = pic->n_inter + pic->n_intra + n_mbs;
+    pic->n_rows = (uint32_t)H;
+    size_t n_items = pic->n_inter + pic->n_intra + n_mbs + (size_t)H;
     size_t o_pic = al(o_it + n_items * sizeof(uint32_t)), total = al(o_pic + sizeof(DevPic));
     void *dm = nullptr;
     HIPCHK(hipMalloc(&dm, total));
@@ -429,10 +433,11 @@ int stage_picture(h264g_ctx *ctx, Stream &st, const h264g_pic_desc &d, const h26
     items.insert(items.end(), pic->inter_list.begin(), pic->inter_list.end());
     items.insert(items.end(), pic->intra_t.begin(), pic->intra_t.end());
     items.insert(items.end(), pic->all_t.begin(), pic->all_t.end());
+    for (int y = 0; y < H; y++) items.push_back((uint32_t)y);
     DevPic &P = pic->hpic;
     std::memset(&P, 0, sizeof(P));
     P.mbs = pic->d_mbs; P.aux = pic->d_aux; P.levels = pic->d_levels; P.wt = pic->d_wt;
-    P.dpb = st.dpb; P.slot_bytes = st.slot_bytes; P.flags = st.flags; P.dst = d.dst_slot;
+    P.dpb = st.dpb; P.slot_bytes = st.slot_bytes; P.flags = st.flags; P.rowprog = st.rowprog; P.dst = d.dst_slot;
     P.W = W; P.H = H; P.offA = d.filter_offset_a; P.offB = d.filter_offset_b;
     P.dbk_idc = d.disable_deblocking_filter_idc; P.cip = d.constrained_intra_pred_flag;
     HIPCHK(hipMemcpy(pic->d_mbs, mbs, n_mbs * sizeof(h264g_mb), hipMemcpyHostToDevice));
@@ -445,8 +450,9 @@ int stage_picture(h264g_ctx *ctx, Stream &st, const h264g_pic_desc &d, const h26
     return 0;
 }
 
+// dbk holds MB addresses in t order (per-MB wavefront) or MB rows (dbk_rows).
 int launch_stages(h264g_ctx *ctx, const DevPic *d_pics, const uint32_t *inter, uint32_t n_inter, const uint32_t *intra,
-                  uint32_t n_intra, const uint32_t *dbk, uint32_t n_dbk, bool dbk_off) {
+                  uint32_t n_intra, const uint32_t *dbk, uint32_t n_dbk, bool dbk_off, bool dbk_rows) {
     hipStream_t s = ctx->stream;
     hipEvent_t *ev = nullptr;
     if (ctx->profile) {
@@ -466,7 +472,8 @@ int launch_stages(h264g_ctx *ctx, const DevPic *d_pics, const uint32_t *inter, u
     if (e) return h264g_set_error(H264G_E_DEVICE, "intra launch: %s", hipGetErrorString((hipError_t)e));
     if (ev) HIPCHK(hipEventRecord(ev[2], s));
     if (!dbk_off) {
-        e = h264g_launch_deblock(d_pics, dbk, n_dbk, &ctx->d_ctl[1], &ctx->d_ctl[2].ticket, ctx->d_err, ctx->waves, s);
+        e = dbk_rows ? h264g_launch_deblock_rows(d_pics, dbk, n_dbk, &ctx->d_ctl[1], &ctx->d_ctl[2].ticket, ctx->d_err, ctx->waves, s)
+                     : h264g_launch_deblock(d_pics, dbk, n_dbk, &ctx->d_ctl[1], &ctx->d_ctl[2].ticket, ctx->d_err, ctx->waves, s);
         if (e) return h264g_set_error(H264G_E_DEVICE, "deblock launch: %s", hipGetErrorString((hipError_t)e));
     }
     if (ev) HIPCHK(hipEventRecord(ev[3], s));
@@ -508,6 +515,7 @@ int32_t h264g_create(const h264g_config *cfg, h264g_ctx **out) {
     c->budget = cfg ? cfg->dpb_budget_bytes : 0;
     c->profile = cfg && cfg->profile;
     c->no_graphs = cfg && cfg->no_graphs;
+    c->dbk_per_mb = cfg && cfg->deblock_per_mb;
     c->waves = (cfg && cfg->waves > 0) ? cfg->waves : prop.multiProcessorCount * 8;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipMalloc(&c->d_err, sizeof(uint32_t)) != hipSuccess || hipMalloc(&c->d_ctl, 4 * sizeof(DevCtl)) != hipSuccess ||
@@ -528,6 +536,7 @@ int32_t h264g_destroy(h264g_ctx *c) {
     for (auto &s : c->streams) {
         if (s.dpb) (void)hipFree(s.dpb);
         if (s.flags) (void)hipFree(s.flags);
+        if (s.rowprog) (void)hipFree(s.rowprog);
     }
     for (auto &e : c->ev) if (e) (void)hipEventDestroy(e);
     if (c->d_err) (void)hipFree(c->d_err);
@@ -550,7 +559,7 @@ int32_t h264g_stream_open(h264g_ctx *c, const h264g_stream_desc *d, uint32_t *id
     s.W = d->pic_width_in_mbs;
     s.H = d->frame_height_in_mbs;
     s.slot_bytes = (uint64_t)s.W * s.H * 384;
-    uint64_t need = s.slot_bytes * d->dpb_slots + (uint64_t)s.W * s.H * 4;
+    uint64_t need = s.slot_bytes * d->dpb_slots + (uint64_t)s.W * s.H * 4 + (uint64_t)s.H * 8;
     if (c->budget && c->used + need > c->budget)
         return h264g_set_error(H264G_E_BUDGET, "DPB budget: %llu + %llu > %llu", (unsigned long long)c->used,
                                (unsigned long long)need, (unsigned long long)c->budget);
@@ -560,6 +569,12 @@ int32_t h264g_stream_open(h264g_ctx *c, const h264g_stream_desc *d, uint32_t *id
         (void)hipFree(s.dpb);
         return h264g_set_error(H264G_E_NOMEM, "flags");
     }
+    if (hipMalloc(&s.rowprog, (size_t)s.H * 8) != hipSuccess) {
+        (void)hipFree(s.dpb);
+        (void)hipFree(s.flags);
+        return h264g_set_error(H264G_E_NOMEM, "row progress");
+    }
+    HIPCHK(hipMemset(s.rowprog, 0, (size_t)s.H * 8));
     HIPCHK(hipMemset(s.dpb, 0, s.slot_bytes * d->dpb_slots));
     HIPCHK(hipMemset(s.flags, 0, (size_t)s.W * s.H * 4));
     s.bytes = need;
@@ -582,6 +597,7 @@ int32_t h264g_stream_close(h264g_ctx *c, uint32_t id) {
     Stream &s = c->streams[id];
     (void)hipFree(s.dpb);
     (void)hipFree(s.flags);
+    (void)hipFree(s.rowprog);
     c->used -= s.bytes;
     s = Stream();
     return H264G_OK;
@@ -616,8 +632,10 @@ int32_t h264g_picture_submit(h264g_ctx *c, h264g_pic *p) {
     std::lock_guard<std::mutex> lk(c->mu);
     HIPCHK(hipSetDevice(c->device));
     const uint32_t *it = p->d_items;
-    return launch_stages(c, p->d_pic, it, p->n_inter, it + p->n_inter, p->n_intra, it + p->n_inter + p->n_intra, p->n_mbs,
-                         p->desc.disable_deblocking_filter_idc == 1);
+    const uint32_t *dbk = it + p->n_inter + p->n_intra;
+    const bool rows = !c->dbk_per_mb;
+    return launch_stages(c, p->d_pic, it, p->n_inter, it + p->n_inter, p->n_intra, rows ? dbk + p->n_mbs : dbk,
+                         rows ? p->n_rows : p->n_mbs, p->desc.disable_deblocking_filter_idc == 1, rows);
 }
 
 int32_t h264g_picture_release(h264g_ctx *c, h264g_pic *p) {
@@ -674,7 +692,16 @@ int32_t h264g_batch_create(h264g_ctx *c, h264g_pic *const *pics, uint32_t n, h26
     merge(true, intra);
     bool any_dbk = false;
     for (uint32_t i = 0; i < n; i++) any_dbk |= pics[i]->desc.disable_deblocking_filter_idc != 1;
-    if (any_dbk) merge(false, dbk);
+    const bool rows = !c->dbk_per_mb;
+    if (any_dbk && !rows) merge(false, dbk);
+    if (any_dbk && rows) {
+        // MB rows of all pictures, row index major: the row above always comes first.
+        int maxh = 0;
+        for (uint32_t i = 0; i < n; i++) maxh = std::max(maxh, pics[i]->H);
+        for (int y = 0; y < maxh; y++)
+            for (uint32_t i = 0; i < n; i++)
+                if (y < pics[i]->H) dbk.push_back((i << WI_PIC_SHIFT) | (uint32_t)y);
+    }
     auto al = [](size_t v) { return (v + 255) & ~(size_t)255; };
     size_t o_p = 0, o_i = al(n * sizeof(DevPic)), o_a = al(o_i + inter.size() * 4), o_d = al(o_a + intra.size() * 4);
     size_t total = al(o_d + dbk.size() * 4 + 4);
@@ -689,6 +716,7 @@ int32_t h264g_batch_create(h264g_ctx *c, h264g_pic *const *pics, uint32_t n, h26
     b->n_inter = (uint32_t)inter.size();
     b->n_intra = (uint32_t)intra.size();
     b->n_dbk = (uint32_t)dbk.size();
+    b->dbk_rows = rows;
     if (hipMemcpy(b->d_pics, hp.data(), n * sizeof(DevPic), hipMemcpyHostToDevice) != hipSuccess ||
         (!inter.empty() && hipMemcpy(b->d_inter, inter.data(), inter.size() * 4, hipMemcpyHostToDevice) != hipSuccess) ||
         (!intra.empty() && hipMemcpy(b->d_intra, intra.data(), intra.size() * 4, hipMemcpyHostToDevice) != hipSuccess) ||
@@ -705,12 +733,12 @@ int32_t h264g_batch_run(h264g_ctx *c, h264g_batch *b) {
     std::lock_guard<std::mutex> lk(c->mu);
     HIPCHK(hipSetDevice(c->device));
     if (c->profile || c->no_graphs)
-        return launch_stages(c, b->d_pics, b->d_inter, b->n_inter, b->d_intra, b->n_intra, b->d_dbk, b->n_dbk, b->n_dbk == 0);
+        return launch_stages(c, b->d_pics, b->d_inter, b->n_inter, b->d_intra, b->n_intra, b->d_dbk, b->n_dbk, b->n_dbk == 0, b->dbk_rows);
     if (!b->graph) {
         // Capture the three stage launches once; every launch argument (including the
         // wavefront epoch, which lives in device memory) is the same on each replay.
         HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        int rc = launch_stages(c, b->d_pics, b->d_inter, b->n_inter, b->d_intra, b->n_intra, b->d_dbk, b->n_dbk, b->n_dbk == 0);
+        int rc = launch_stages(c, b->d_pics, b->d_inter, b->n_inter, b->d_intra, b->n_intra, b->d_dbk, b->n_dbk, b->n_dbk == 0, b->dbk_rows);
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamEndCapture(c->stream, &g);
         if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }

@@ -45,12 +45,14 @@ struct DevPic {
     uint8_t *dpb;             // slot 0 of the stream's DPB
     uint64_t slot_bytes;      // bytes per DPB slot (Y + Cb + Cr)
     uint32_t *flags;          // per-MB wavefront progress words of the stream
+    uint64_t *rowprog;        // per-MB-row progress of row-pipelined deblocking:
+                              // epoch << 32 | MBs of the row finished
     int32_t  dst;             // destination slot
     int32_t  W, H;            // in macroblocks
     int32_t  offA, offB;      // FilterOffsetA / FilterOffsetB
     int32_t  dbk_idc;         // disable_deblocking_filter_idc
     int32_t  cip;             // constrained_intra_pred_flag
-    int32_t  pad[3];
+    int32_t  pad[1];
 };
 
 // Work item: picture index in the batch and macroblock address.
@@ -76,5 +78,7 @@ int h264g_launch_intra(const DevPic *pics, const uint32_t *items, uint32_t n, De
                        uint32_t *err, int waves, void *stream);
 int h264g_launch_deblock(const DevPic *pics, const uint32_t *items, uint32_t n, DevCtl *ctl, uint32_t *epoch,
                          uint32_t *err, int waves, void *stream);
+int h264g_launch_deblock_rows(const DevPic *pics, const uint32_t *items, uint32_t n, DevCtl *ctl, uint32_t *epoch,
+                              uint32_t *err, int waves, void *stream);
 int h264g_launch_copy(const uint8_t *src, uint8_t *dst, uint64_t bytes, void *stream);
 }

@@ -418,9 +418,9 @@ __device__ void wait_deps(const uint32_t *flags, const int *deps, int nd, uint32
     __atomic_thread_fence(__ATOMIC_ACQUIRE);
 }
 
+// The barrier orders every lane's stores before the (wave-wide) release store.
 __device__ void publish(uint32_t *flag, uint32_t epoch) {
     __syncthreads();
-    __threadfence();
     st_release(flag, epoch);
 }
 
@@ -849,32 +849,48 @@ struct WaveDbk {
     uint8_t C[2][12][12];
 };
 
-__device__ void deblock_mb(const DevPic &P, int addr, WaveDbk *wd) {
+// Plane pointers of the picture being deblocked.
+struct DbkPlanes {
+    uint8_t *y, *cb, *cr;
+    int wl, wc;
+};
+__device__ __forceinline__ DbkPlanes dbk_planes(const DevPic &P) {
+    DbkPlanes d;
+    d.wl = P.W * 16;
+    d.wc = P.W * 8;
+    d.y = slot_base(P, P.dst);
+    d.cb = d.y + (uint64_t)d.wl * (P.H * 16);
+    d.cr = d.cb + (uint64_t)d.wc * (P.H * 8);
+    return d;
+}
+
+// Tile loads.  Region bits: 1 = the MB itself (rows / cols >= 0), 2 = the 4 rows above
+// it (cols >= 0), 4 = the 4 columns left of it (rows >= 0).  The corner is never used.
+__device__ void dbk_load(const DbkPlanes &d, int mbx, int mby, int region, WaveDbk *wd) {
     const int lane = wave_lane();
-    const int W = P.W, H = P.H, wl = W * 16, hl = H * 16, wc = W * 8, hc = H * 8;
-    const int mbx = addr % W, mby = addr / W;
-    uint8_t *dst = slot_base(P, P.dst);
-    uint8_t *dy = dst, *dcb = dst + (uint64_t)wl * hl, *dcr = dcb + (uint64_t)wc * hc;
-    const bool left = mbx > 0, top = mby > 0;
-    // ---- load: luma 20x20 (rows/cols -4..15) and chroma 12x12, where they exist
     for (int i = lane; i < 400; i += WAVE) {
-        int y = i / 20 - 4, x = i % 20 - 4;
-        if ((x < 0 && !left) || (y < 0 && !top) || (x < 0 && y < 0)) continue;
-        wd->L[y + 4][x + 4] = dy[(uint64_t)(mby * 16 + y) * wl + mbx * 16 + x];
+        const int y = i / 20 - 4, x = i % 20 - 4;
+        const int r = (y >= 0 && x >= 0) ? 1 : (y < 0 && x >= 0 ? 2 : (x < 0 && y >= 0 ? 4 : 0));
+        if (!(r & region)) continue;
+        wd->L[y + 4][x + 4] = d.y[(uint64_t)(mby * 16 + y) * d.wl + mbx * 16 + x];
     }
     for (int i = lane; i < 288; i += WAVE) {
-        int c = i / 144, k = i % 144, y = k / 12 - 4, x = k % 12 - 4;
-        if ((x < 0 && !left) || (y < 0 && !top) || (x < 0 && y < 0)) continue;
-        const uint8_t *pl = c == 0 ? dcb : dcr;
-        wd->C[c][y + 4][x + 4] = pl[(uint64_t)(mby * 8 + y) * wc + mbx * 8 + x];
+        const int c = i / 144, k = i % 144, y = k / 12 - 4, x = k % 12 - 4;
+        const int r = (y >= 0 && x >= 0) ? 1 : (y < 0 && x >= 0 ? 2 : (x < 0 && y >= 0 ? 4 : 0));
+        if (!(r & region)) continue;
+        const uint8_t *pl = c == 0 ? d.cb : d.cr;
+        wd->C[c][y + 4][x + 4] = pl[(uint64_t)(mby * 8 + y) * d.wc + mbx * 8 + x];
     }
-    MbView Qv, Lv, Tv;
-    load_view(P, addr, Qv);
-    if (left) load_view(P, addr - 1, Lv);
-    if (top) load_view(P, addr - W, Tv);
-    __syncthreads();
+}
+
+// The 8 edge steps of one MB on the tile, in the reference's order (8.7, Deblock.cpp:76-635):
+// vertical edges left to right, then horizontal edges top to bottom; luma in lanes 0-15,
+// Cb in 16-23, Cr in 32-39.  Lv / Tv are the left / top MB views (read only when present).
+__device__ void dbk_filter(const DevPic &P, int addr, const MbView &Qv, const MbView &Lv, const MbView &Tv, bool left,
+                           bool top, WaveDbk *wd) {
+    const int lane = wave_lane();
+    const int W = P.W;
     const bool t8 = Qv.kind & AUX_T8;
-    // ---- edges: dir 0 = vertical edges (left to right), dir 1 = horizontal (top to bottom)
     for (int dir = 0; dir < 2; dir++) {
         for (int s = 0; s < 4; s++) {
             int e = 4 * s;
@@ -922,19 +938,104 @@ __device__ void deblock_mb(const DevPic &P, int addr, WaveDbk *wd) {
             __syncthreads();
         }
     }
-    // ---- write back the MB and the strips its MB edges modified (3 columns / rows)
+}
+
+// Writes back the MB and the strips its MB edges modified (3 columns / rows).
+__device__ void dbk_store(const DbkPlanes &d, int mbx, int mby, bool left, bool top, const WaveDbk *wd) {
+    const int lane = wave_lane();
     for (int i = lane; i < 400; i += WAVE) {
         int y = i / 20 - 4, x = i % 20 - 4;
         if (x < -3 || y < -3 || (x < 0 && y < 0) || (x < 0 && !left) || (y < 0 && !top)) continue;
-        dy[(uint64_t)(mby * 16 + y) * wl + mbx * 16 + x] = wd->L[y + 4][x + 4];
+        d.y[(uint64_t)(mby * 16 + y) * d.wl + mbx * 16 + x] = wd->L[y + 4][x + 4];
     }
     for (int i = lane; i < 288; i += WAVE) {
         int c = i / 144, k = i % 144, y = k / 12 - 4, x = k % 12 - 4;
         if (x < -3 || y < -3 || (x < 0 && y < 0) || (x < 0 && !left) || (y < 0 && !top)) continue;
-        uint8_t *pl = c == 0 ? dcb : dcr;
-        pl[(uint64_t)(mby * 8 + y) * wc + mbx * 8 + x] = wd->C[c][y + 4][x + 4];
+        uint8_t *pl = c == 0 ? d.cb : d.cr;
+        pl[(uint64_t)(mby * 8 + y) * d.wc + mbx * 8 + x] = wd->C[c][y + 4][x + 4];
     }
-    (void)H; (void)hl; (void)hc;
+}
+
+// One MB with every neighbour sample loaded from the picture (per-MB wavefront).
+__device__ void deblock_mb(const DevPic &P, int addr, WaveDbk *wd) {
+    const int W = P.W;
+    const int mbx = addr % W, mby = addr / W;
+    const DbkPlanes d = dbk_planes(P);
+    const bool left = mbx > 0, top = mby > 0;
+    dbk_load(d, mbx, mby, 1 | (top ? 2 : 0) | (left ? 4 : 0), wd);
+    MbView Qv, Lv, Tv;
+    load_view(P, addr, Qv);
+    if (left) load_view(P, addr - 1, Lv);
+    if (top) load_view(P, addr - W, Tv);
+    __syncthreads();
+    dbk_filter(P, addr, Qv, Lv, Tv, left, top, wd);
+    dbk_store(d, mbx, mby, left, top, wd);
+}
+
+// After MB x of a row: its columns 12..15 become the left columns of MB x + 1.
+__device__ void dbk_carry_left(WaveDbk *wd) {
+    const int lane = wave_lane();
+    __syncthreads();
+    for (int i = lane; i < 16 * 4 + 2 * 8 * 4; i += WAVE) {
+        if (i < 64) {
+            const int y = i >> 2, x = i & 3;
+            wd->L[y + 4][x] = wd->L[y + 4][16 + x];
+        } else {
+            const int k = i - 64, c = k >> 5, y = (k >> 2) & 7, x = k & 3;
+            wd->C[c][y + 4][x] = wd->C[c][y + 4][8 + x];
+        }
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ uint64_t ld_acquire64(const uint64_t *p) {
+    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Waits until row progress word *w >= want (epoch-tagged: epoch << 32 | MBs done).
+__device__ void wait_row(const uint64_t *w, uint64_t want, uint32_t *err) {
+    for (uint32_t it = 0;; ++it) {
+        const uint64_t v = ld_acquire64(w);
+        const uint32_t lo = uniform((uint32_t)v), hi = uniform((uint32_t)(v >> 32));
+        if ((((uint64_t)hi << 32) | lo) >= want) break;
+        if (it >= SPIN_LIMIT) {
+            atomicOr(err, DEVERR_TIMEOUT);
+            break;
+        }
+        if ((it & 255u) == 255u && uniform(__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) != 0u) break;
+        __builtin_amdgcn_s_sleep(1);
+    }
+    __atomic_thread_fence(__ATOMIC_ACQUIRE);
+}
+
+// One whole MB row, left to right, in raster order.  MB (x, y) needs MB (x + 1, y - 1)
+// done (its left-edge filter changed columns 13..15 of (x, y - 1), which (x, y)'s top
+// edge reads), i.e. the row above at progress min(x + 2, W).  The left neighbour's
+// columns stay in LDS from the previous step.
+__device__ void deblock_row(const DevPic &P, int mby, uint64_t tag, uint32_t *err, WaveDbk *wd) {
+    const int W = P.W;
+    const DbkPlanes d = dbk_planes(P);
+    const bool top = mby > 0;
+    MbView Lv, Qv, Tv;
+    for (int mbx = 0; mbx < W; mbx++) {
+        const int addr = mby * W + mbx;
+        const bool left = mbx > 0;
+        dbk_load(d, mbx, mby, 1, wd);  // the MB's own samples: final before this launch
+        if (left) Lv = Qv;
+        load_view(P, addr, Qv);
+        if (top) {
+            load_view(P, addr - W, Tv);
+            wait_row(&P.rowprog[mby - 1], tag | (uint32_t)min(mbx + 2, W), err);
+            dbk_load(d, mbx, mby, 2, wd);
+        }
+        __syncthreads();
+        dbk_filter(P, addr, Qv, Lv, Tv, left, top, wd);
+        dbk_store(d, mbx, mby, left, top, wd);
+        // publish: the barrier orders every lane's stores before the release store
+        __syncthreads();
+        __hip_atomic_store(&P.rowprog[mby], tag | (uint32_t)(mbx + 1), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        dbk_carry_left(wd);
+    }
 }
 
 // ------------------------------------------------------------ kernels
@@ -997,6 +1098,23 @@ __global__ __launch_bounds__(WAVE) void k_deblock(const DevPic *__restrict__ pic
     retire(ctl, gridDim.x, epoch_p, epoch);
 }
 
+// Row-pipelined deblocking: a ticket is one MB row of one picture (queue ordered by
+// row index across pictures, so the row above always holds an earlier ticket).
+__global__ __launch_bounds__(WAVE) void k_deblock_rows(const DevPic *__restrict__ pics, const uint32_t *__restrict__ items,
+                                                       uint32_t n, DevCtl *ctl, uint32_t *epoch_p, uint32_t *err) {
+    __shared__ WaveDbk lds;
+    const uint32_t epoch = launch_epoch(epoch_p);
+    const uint64_t tag = (uint64_t)epoch << 32;
+    for (;;) {
+        int t = next_ticket(ctl, n);
+        if (t < 0) break;
+        const uint32_t it = items[t];
+        const DevPic &P = pics[it >> WI_PIC_SHIFT];
+        deblock_row(P, (int)(it & WI_MB_MASK), tag, err, &lds);
+    }
+    retire(ctl, gridDim.x, epoch_p, epoch);
+}
+
 __global__ void k_copy(const uint4 *src, uint4 *dst, uint64_t n16) {
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n16; i += (uint64_t)gridDim.x * blockDim.x) dst[i] = src[i];
 }
@@ -1024,6 +1142,14 @@ int h264g_launch_deblock(const DevPic *pics, const uint32_t *items, uint32_t n, 
     if (n == 0) return 0;
     uint32_t g = (uint32_t)waves < n ? (uint32_t)waves : n;
     hipLaunchKernelGGL(k_deblock, dim3(g), dim3(WAVE), 0, (hipStream_t)stream, pics, items, n, ctl, epoch, err);
+    return (int)hipGetLastError();
+}
+
+int h264g_launch_deblock_rows(const DevPic *pics, const uint32_t *items, uint32_t n, DevCtl *ctl, uint32_t *epoch,
+                              uint32_t *err, int waves, void *stream) {
+    if (n == 0) return 0;
+    uint32_t g = (uint32_t)waves < n ? (uint32_t)waves : n;
+    hipLaunchKernelGGL(k_deblock_rows, dim3(g), dim3(WAVE), 0, (hipStream_t)stream, pics, items, n, ctl, epoch, err);
     return (int)hipGetLastError();
 }
 

@@ -123,9 +123,10 @@ class Engine:
     """One libh264g context on one HIP device (one HIP stream inside)."""
 
     def __init__(self, device: int = 0, dpb_budget_bytes: int = 0, profile: bool = False, waves: int = 0,
-                 max_batch_pictures: int = 0, graphs: bool = True):
+                 max_batch_pictures: int = 0, graphs: bool = True, deblock_rows: bool = True):
         cfg = Config(device=device, dpb_budget_bytes=dpb_budget_bytes, max_batch_pictures=max_batch_pictures,
-                     profile=1 if profile else 0, waves=waves, no_graphs=0 if graphs else 1)
+                     profile=1 if profile else 0, waves=waves, no_graphs=0 if graphs else 1,
+                     deblock_per_mb=0 if deblock_rows else 1)
         h = c_void_p()
         check(lib().h264g_create(byref(cfg), byref(h)))
         self._h = h

@@ -142,6 +142,8 @@ typedef struct h264g_config {
     int32_t  no_graphs;           /* 1 = launch batch stages directly; 0 = replay each
                                      batch's stage sequence as a captured hipGraph
                                      (profiled contexts always launch directly)       */
+    int32_t  deblock_per_mb;      /* 1 = deblock as a wavefront of single MBs; 0 = one
+                                     wave streams each MB row (row-pipelined)         */
 } h264g_config;
 
 typedef struct h264g_stream_desc {

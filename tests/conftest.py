@@ -28,3 +28,18 @@ def engine():
     eng = Engine(device=0)
     yield eng
     eng.close()
+
+
+@pytest.fixture(scope="session")
+def engine_per_mb():
+    """A second context that deblocks as a wavefront of single MBs."""
+    from h264_video_decoder_demo_amd import Engine
+    eng = Engine(device=0, deblock_rows=False)
+    yield eng
+    eng.close()
+
+
+@pytest.fixture(params=["rows", "per_mb"])
+def any_engine(request):
+    """Both deblocking schedules (row-pipelined and per-MB wavefront)."""
+    return request.getfixturevalue("engine" if request.param == "rows" else "engine_per_mb")
