@@ -20,10 +20,15 @@ constexpr int WAVE = 64;
 
 // ------------------------------------------------------------------ tables
 // Table 8-13 / 8-14 zig-zag (scan position -> raster index), H264PictureBase.cpp:4542-4744
-__constant__ uint8_t c_zz4[16] = {0, 1, 4, 8, 5, 2, 3, 6, 9, 12, 13, 10, 7, 11, 14, 15};
+// (constexpr so that fully unrolled loops index register arrays with constants)
+__device__ constexpr uint8_t c_zz4[16] = {0, 1, 4, 8, 5, 2, 3, 6, 9, 12, 13, 10, 7, 11, 14, 15};
 __constant__ uint8_t c_zz8[64] = {
     0, 1, 8, 16, 9, 2, 3, 10, 17, 24, 32, 25, 18, 11, 4, 5, 12, 19, 26, 33, 40, 48, 41, 34, 27, 20, 13, 6, 7, 14, 21, 28,
     35, 42, 49, 56, 57, 50, 43, 36, 29, 22, 15, 23, 30, 37, 44, 51, 58, 59, 52, 45, 38, 31, 39, 46, 53, 60, 61, 54, 47, 55, 62, 63};
+// raster position -> 8x8 zig-zag scan index (inverse of c_zz8)
+__constant__ uint8_t c_izz8[64] = {
+    0, 1, 5, 6, 14, 15, 27, 28, 2, 4, 7, 13, 16, 26, 29, 42, 3, 8, 12, 17, 25, 30, 41, 43, 9, 11, 18, 24, 31, 40, 44, 53,
+    10, 19, 23, 32, 39, 45, 52, 54, 20, 22, 33, 38, 46, 51, 55, 60, 21, 34, 37, 47, 50, 56, 59, 61, 35, 36, 48, 49, 57, 58, 62, 63};
 // normAdjust values v (8-315, 8-318), H264PictureBase.cpp:4900-4950
 __constant__ uint8_t c_v4[6][3] = {{10, 16, 13}, {11, 18, 14}, {13, 20, 16}, {14, 23, 18}, {16, 25, 20}, {18, 29, 23}};
 __constant__ uint8_t c_v8[6][6] = {{20, 18, 32, 19, 25, 24}, {22, 19, 35, 21, 28, 26}, {26, 23, 42, 24, 33, 31},
@@ -70,6 +75,7 @@ __device__ __forceinline__ int wave_lane() { return threadIdx.x & (WAVE - 1); }
 // 19 + 4c + b = chroma AC), then the residual planes.
 struct WaveRes {
     int32_t coef[27][16];
+    int32_t t8[4][64];    // 8x8 transform: rows after the horizontal pass
     int16_t rl[256];      // luma residual, raster 16x16
     int16_t rc[2][64];    // chroma residual, raster 8x8
 };
@@ -127,7 +133,7 @@ __device__ __forceinline__ void idct8(int32_t *v, int s) {  // 8.5.13.2, in plac
 
 // Expands the MB's levels into ws->coef and computes all residual blocks into ws->rl /
 // ws->rc.  Must be called by the whole wave.  Returns false when the MB has no levels.
-__device__ bool build_residual(const h264g_mb *mb, const DevAux &ax, const int16_t *levels, WaveRes *ws) {
+__device__ __forceinline__ bool build_residual(const h264g_mb *mb, const DevAux &ax, const int16_t *levels, WaveRes *ws) {
     const int lane = wave_lane();
     if (!(ax.kind & AUX_RESID)) return false;
     const int cbp = mb->cbp;
@@ -159,57 +165,58 @@ __device__ bool build_residual(const h264g_mb *mb, const DevAux &ax, const int16
         }
     }
     __syncthreads();
-    if (lane < 16) {
-        if (t8) {
-            if (lane < 4) {  // one lane per 8x8 block
-                int q = lane;
-                int32_t m[64];
-                const int32_t *src = &ws->coef[1 + 4 * q][0];
+    if (t8) {
+        // 8.5.13: lanes 0-31 = (8x8 block q, row i): dequantise + horizontal pass ...
+        if (lane < 32) {
+            const int q = lane >> 3, i = lane & 7;
+            const int32_t *src = &ws->coef[1 + 4 * q][0];
+            int32_t d[8];
 #pragma unroll
-                for (int k = 0; k < 64; k++) {
-                    int pos = c_zz8[k], i = pos >> 3, j = pos & 7;
-                    int32_t v = src[k];
-                    m[pos] = qp >= 36 ? (v * ls8(qp % 6, i, j)) << (qp / 6 - 6)
-                                      : (v * ls8(qp % 6, i, j) + (1 << (5 - qp / 6))) >> (6 - qp / 6);
-                }
-#pragma unroll
-                for (int i = 0; i < 8; i++) idct8(&m[i * 8], 1);
-#pragma unroll
-                for (int j = 0; j < 8; j++) idct8(&m[j], 8);
-                int x0 = (q & 1) * 8, y0 = (q >> 1) * 8;
-#pragma unroll
-                for (int k = 0; k < 64; k++) ws->rl[(y0 + (k >> 3)) * 16 + x0 + (k & 7)] = (int16_t)((m[k] + 32) >> 6);
+            for (int j = 0; j < 8; j++) {
+                const int32_t v = src[c_izz8[i * 8 + j]];
+                d[j] = qp >= 36 ? (v * ls8(qp % 6, i, j)) << (qp / 6 - 6)
+                                : (v * ls8(qp % 6, i, j) + (1 << (5 - qp / 6))) >> (6 - qp / 6);
             }
-        } else {
-            int b = lane;
-            int32_t scan[16];
+            idct8(d, 1);
 #pragma unroll
-            for (int k = 0; k < 16; k++) scan[k] = ws->coef[1 + b][k];
-            int dcpass = 0;
-            if (i16) {  // 8.5.10: 4x4 Hadamard of the DC levels, then scaling
-                int32_t c[16], g[16];
+            for (int j = 0; j < 8; j++) ws->t8[q][i * 8 + j] = d[j];
+        }
+    } else if (lane < 16) {
+        int b = lane;
+        int32_t scan[16];
 #pragma unroll
-                for (int k = 0; k < 16; k++) c[c_zz4[k]] = ws->coef[0][k];
+        for (int k = 0; k < 16; k++) scan[k] = ws->coef[1 + b][k];
+        int dcpass = 0;
+        if (i16) {  // 8.5.10: f = (H c H)[row][col] with H the 4x4 Hadamard, then scaling
+            const int row = blk_y(b) >> 2, col = blk_x(b) >> 2;
+            // H[r][i] is -1 where: r = 1, i >= 2; r = 2, i in {1, 2}; r = 3, i odd
+            auto hneg = [](int r, int i) { return r == 1 ? (i >> 1) : r == 2 ? ((i == 1) | (i == 2)) : r == 3 ? (i & 1) : 0; };
+            int32_t f = 0;
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                int32_t acc = 0;
 #pragma unroll
                 for (int j = 0; j < 4; j++) {
-                    int32_t a = c[j], bb = c[4 + j], cc = c[8 + j], dd = c[12 + j];
-                    g[j] = a + bb + cc + dd; g[4 + j] = a + bb - cc - dd; g[8 + j] = a - bb - cc + dd; g[12 + j] = a - bb + cc - dd;
-                }
-                int row = blk_y(b) >> 2, col = blk_x(b) >> 2;
-                int32_t a = g[row * 4], bb = g[row * 4 + 1], cc = g[row * 4 + 2], dd = g[row * 4 + 3];
-                int32_t f = col == 0 ? a + bb + cc + dd : col == 1 ? a + bb - cc - dd : col == 2 ? a - bb - cc + dd : a - bb + cc - dd;
-                int s = ls4(qp % 6, 0, 0);
-                scan[0] = qp >= 36 ? (f * s) << (qp / 6 - 6) : (f * s + (1 << (5 - qp / 6))) >> (6 - qp / 6);
-                dcpass = 1;
-            }
-            int32_t r[16];
-            residual4x4(scan, qp, dcpass, r);
-            int x0 = blk_x(b), y0 = blk_y(b);
+                    int32_t v = 0;
 #pragma unroll
-            for (int k = 0; k < 16; k++) ws->rl[(y0 + (k >> 2)) * 16 + x0 + (k & 3)] = (int16_t)r[k];
+                    for (int k = 0; k < 16; k++)
+                        if (c_zz4[k] == i * 4 + j) v = ws->coef[0][k];  // resolved at compile time
+                    acc += hneg(col, j) ? -v : v;
+                }
+                f += hneg(row, i) ? -acc : acc;
+            }
+            int s = ls4(qp % 6, 0, 0);
+            scan[0] = qp >= 36 ? (f * s) << (qp / 6 - 6) : (f * s + (1 << (5 - qp / 6))) >> (6 - qp / 6);
+            dcpass = 1;
         }
-    } else if (lane < 24) {  // chroma: 8.5.11 DC then 8.5.12 per 4x4 block
-        int c = (lane - 16) >> 2, b = (lane - 16) & 3;
+        int32_t r[16];
+        residual4x4(scan, qp, dcpass, r);
+        int x0 = blk_x(b), y0 = blk_y(b);
+#pragma unroll
+        for (int k = 0; k < 16; k++) ws->rl[(y0 + (k >> 2)) * 16 + x0 + (k & 3)] = (int16_t)r[k];
+    }
+    if (lane >= 32 && lane < 40) {  // chroma: 8.5.11 DC then 8.5.12 per 4x4 block
+        int c = (lane - 32) >> 2, b = (lane - 32) & 3;
         int qpc = ax.qpc[c];
         const int32_t *dc = ws->coef[17 + c];
         int32_t e00 = dc[0] + dc[2], e01 = dc[1] + dc[3], e10 = dc[0] - dc[2], e11 = dc[1] - dc[3];
@@ -223,6 +230,20 @@ __device__ bool build_residual(const h264g_mb *mb, const DevAux &ax, const int16
         int x0 = (b & 1) * 4, y0 = (b >> 1) * 4;
 #pragma unroll
         for (int k = 0; k < 16; k++) ws->rc[c][(y0 + (k >> 2)) * 8 + x0 + (k & 3)] = (int16_t)r[k];
+    }
+    if (t8) {
+        __syncthreads();
+        // ... then lanes 0-31 = (block q, column j): vertical pass and (x + 32) >> 6
+        if (lane < 32) {
+            const int q = lane >> 3, j = lane & 7;
+            int32_t d[8];
+#pragma unroll
+            for (int i = 0; i < 8; i++) d[i] = ws->t8[q][i * 8 + j];
+            idct8(d, 1);
+            const int x0 = (q & 1) * 8, y0 = (q >> 1) * 8;
+#pragma unroll
+            for (int i = 0; i < 8; i++) ws->rl[(y0 + i) * 16 + x0 + j] = (int16_t)((d[i] + 32) >> 6);
+        }
     }
     __syncthreads();
     return true;
@@ -294,6 +315,16 @@ __device__ void luma4(const uint8_t *ref, int w, int h, int x0, int y0, int xF, 
     }
 }
 
+// Two horizontally adjacent chroma prediction samples at (xC, yC) with a 1/8-sample
+// MV, 8.4.2.2.2 (H264InterPrediction.cpp:2485-2522), coordinates clamped.
+__device__ __forceinline__ void chroma2(const uint8_t *ref, int wc, int hc, int xC, int yC, int mvx, int mvy, int &o0, int &o1) {
+    const int xi = xC + (mvx >> 3), yi = yC + (mvy >> 3), xF = mvx & 7, yF = mvy & 7;
+    const int A0 = ldc(ref, wc, hc, xi, yi), A1 = ldc(ref, wc, hc, xi + 1, yi), A2 = ldc(ref, wc, hc, xi + 2, yi);
+    const int B0 = ldc(ref, wc, hc, xi, yi + 1), B1 = ldc(ref, wc, hc, xi + 1, yi + 1), B2 = ldc(ref, wc, hc, xi + 2, yi + 1);
+    o0 = ((8 - xF) * (8 - yF) * A0 + xF * (8 - yF) * A1 + (8 - xF) * yF * B0 + xF * yF * B1 + 32) >> 6;
+    o1 = ((8 - xF) * (8 - yF) * A1 + xF * (8 - yF) * A2 + (8 - xF) * yF * B1 + xF * yF * B2 + 32) >> 6;
+}
+
 // 8.4.2.3.1 / 8.4.2.3.2 (H264InterPrediction.cpp:2526-2829); the host resolved the
 // weights, including the L1-only luma entries of quirk Q2, into w0/o0.
 __device__ __forceinline__ int weigh(const DevWeight &w, int c, int pf0, int pf1, int p0, int p1) {
@@ -316,7 +347,7 @@ __device__ __forceinline__ uint8_t *slot_base(const DevPic &P, int s) { return P
 __device__ void inter_mb(const DevPic &P, int addr, WaveRes *ws) {
     const int lane = wave_lane();
     const h264g_mb *mb = &P.mbs[addr];
-    const DevAux ax = P.aux[addr];
+    const DevAux &ax = P.aux[addr];  // per-lane fields (refid[q], wp[q]) are read from memory
     const int W = P.W, H = P.H, wl = W * 16, hl = H * 16, wc = W * 8, hc = H * 8;
     const int mbx = addr % W, mby = addr / W;
     uint8_t *dst = slot_base(P, P.dst);
@@ -333,19 +364,21 @@ __device__ void inter_mb(const DevPic &P, int addr, WaveRes *ws) {
     {
         int b = lane >> 2, r = lane & 3, bx = b & 3, by = b >> 2, q = (by >> 1) * 2 + (bx >> 1);
         int pf0 = (ax.pf >> q) & 1, pf1 = (ax.pf >> (4 + q)) & 1;
-        int pl[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+        int p0[4] = {0, 0, 0, 0}, p1[4] = {0, 0, 0, 0};
         int xAL = mbx * 16 + bx * 4, yAL = mby * 16 + by * 4 + r;
-        for (int l = 0; l < 2; l++) {
-            if (!((l == 0) ? pf0 : pf1)) continue;
-            const uint8_t *ref = slot_base(P, ax.refid[l][q]);
-            int mvx = mb->mv[l][b][0], mvy = mb->mv[l][b][1];
-            luma4(ref, wl, hl, xAL + (mvx >> 2), yAL + (mvy >> 2), mvx & 3, mvy & 3, pl[l]);
+        if (pf0) {
+            const int mvx = mb->mv[0][b][0], mvy = mb->mv[0][b][1];
+            luma4(slot_base(P, ax.refid[0][q]), wl, hl, xAL + (mvx >> 2), yAL + (mvy >> 2), mvx & 3, mvy & 3, p0);
         }
-        const DevWeight w = P.wt[ax.wp[q]];
+        if (pf1) {
+            const int mvx = mb->mv[1][b][0], mvy = mb->mv[1][b][1];
+            luma4(slot_base(P, ax.refid[1][q]), wl, hl, xAL + (mvx >> 2), yAL + (mvy >> 2), mvx & 3, mvy & 3, p1);
+        }
+        const DevWeight &w = P.wt[ax.wp[q]];
         uint32_t packed = 0;
 #pragma unroll
         for (int i = 0; i < 4; i++) {
-            int v = weigh(w, 0, pf0, pf1, pl[0][i], pl[1][i]);
+            int v = weigh(w, 0, pf0, pf1, p0[i], p1[i]);
             if (res) v = clip1(v + ws->rl[(by * 4 + r) * 16 + bx * 4 + i]);
             packed |= (uint32_t)v << (8 * i);
         }
@@ -355,22 +388,15 @@ __device__ void inter_mb(const DevPic &P, int addr, WaveRes *ws) {
     {
         int c = lane >> 5, i = lane & 31, b = i >> 1, r = i & 1, bx = b & 3, by = b >> 2, q = (by >> 1) * 2 + (bx >> 1);
         int pf0 = (ax.pf >> q) & 1, pf1 = (ax.pf >> (4 + q)) & 1;
-        int pc[2][2] = {{0, 0}, {0, 0}};
         int xC = mbx * 8 + bx * 2, yC = mby * 8 + by * 2 + r;
-        for (int l = 0; l < 2; l++) {
-            if (!((l == 0) ? pf0 : pf1)) continue;
-            const uint8_t *ref = slot_base(P, ax.refid[l][q]) + (uint64_t)wl * hl + (uint64_t)c * wc * hc;
-            int mvx = mb->mv[l][b][0], mvy = mb->mv[l][b][1];
-            int xi = xC + (mvx >> 3), yi = yC + (mvy >> 3), xF = mvx & 7, yF = mvy & 7;
-            int A0 = ldc(ref, wc, hc, xi, yi), A1 = ldc(ref, wc, hc, xi + 1, yi), A2 = ldc(ref, wc, hc, xi + 2, yi);
-            int B0 = ldc(ref, wc, hc, xi, yi + 1), B1 = ldc(ref, wc, hc, xi + 1, yi + 1), B2 = ldc(ref, wc, hc, xi + 2, yi + 1);
-            pc[l][0] = ((8 - xF) * (8 - yF) * A0 + xF * (8 - yF) * A1 + (8 - xF) * yF * B0 + xF * yF * B1 + 32) >> 6;
-            pc[l][1] = ((8 - xF) * (8 - yF) * A1 + xF * (8 - yF) * A2 + (8 - xF) * yF * B1 + xF * yF * B2 + 32) >> 6;
-        }
-        const DevWeight w = P.wt[ax.wp[q]];
+        int c00 = 0, c01 = 0, c10 = 0, c11 = 0;
+        const uint64_t coff = (uint64_t)wl * hl + (uint64_t)c * wc * hc;
+        if (pf0) chroma2(slot_base(P, ax.refid[0][q]) + coff, wc, hc, xC, yC, mb->mv[0][b][0], mb->mv[0][b][1], c00, c01);
+        if (pf1) chroma2(slot_base(P, ax.refid[1][q]) + coff, wc, hc, xC, yC, mb->mv[1][b][0], mb->mv[1][b][1], c10, c11);
+        const DevWeight &w = P.wt[ax.wp[q]];
         uint8_t *plane = c == 0 ? dcb : dcr;
-        int v0 = weigh(w, 1 + c, pf0, pf1, pc[0][0], pc[1][0]);
-        int v1 = weigh(w, 1 + c, pf0, pf1, pc[0][1], pc[1][1]);
+        int v0 = weigh(w, 1 + c, pf0, pf1, c00, c10);
+        int v1 = weigh(w, 1 + c, pf0, pf1, c01, c11);
         if (res) {
             v0 = clip1(v0 + ws->rc[c][(by * 2 + r) * 8 + bx * 2]);
             v1 = clip1(v1 + ws->rc[c][(by * 2 + r) * 8 + bx * 2 + 1]);
@@ -752,10 +778,11 @@ __device__ void intra_mb(const DevPic &P, int addr, WaveIntra *wi, const Avail &
 // ------------------------------------------------------------ deblocking (8.7)
 struct MbView {
     int cls, kind, nz4, nz8;
-    int refid[2][4];
+    uint32_t ref[2];  // reference DPB slot per quadrant q (int8 in byte q), per list
     int pf;
     int qdb[3];
 };
+__device__ __forceinline__ int view_ref(const MbView &v, int l, int q) { return (int8_t)(v.ref[l] >> (8 * q)); }
 
 __device__ __forceinline__ void load_view(const DevPic &P, int addr, MbView &v) {
     const DevAux ax = P.aux[addr];
@@ -766,8 +793,8 @@ __device__ __forceinline__ void load_view(const DevPic &P, int addr, MbView &v) 
     v.pf = ax.pf;
 #pragma unroll
     for (int l = 0; l < 2; l++)
-#pragma unroll
-        for (int q = 0; q < 4; q++) v.refid[l][q] = ax.refid[l][q];
+        v.ref[l] = (uint32_t)(uint8_t)ax.refid[l][0] | ((uint32_t)(uint8_t)ax.refid[l][1] << 8) |
+                   ((uint32_t)(uint8_t)ax.refid[l][2] << 16) | ((uint32_t)(uint8_t)ax.refid[l][3] << 24);
     v.qdb[0] = ax.qdb[0]; v.qdb[1] = ax.qdb[1]; v.qdb[2] = ax.qdb[2];
 }
 
@@ -787,7 +814,7 @@ __device__ int bs_luma(const DevPic &P, int ap, const MbView &Pv, int aq, const 
         return 2;
     int qp = 2 * (yp >> 3) + (xp >> 3), qq = 2 * (yq >> 3) + (xq >> 3);
     int rp = (yp >> 2) * 4 + (xp >> 2), rq = (yq >> 2) * 4 + (xq >> 2);
-    int p0 = Pv.refid[0][qp], p1 = Pv.refid[1][qp], q0 = Qv.refid[0][qq], q1 = Qv.refid[1][qq];
+    int p0 = view_ref(Pv, 0, qp), p1 = view_ref(Pv, 1, qp), q0 = view_ref(Qv, 0, qq), q1 = view_ref(Qv, 1, qq);
     int fp0 = (Pv.pf >> qp) & 1, fp1 = (Pv.pf >> (4 + qp)) & 1, fq0 = (Qv.pf >> qq) & 1, fq1 = (Qv.pf >> (4 + qq)) & 1;
     if (!(((p0 == q0 && p1 == q1) || (p0 == q1 && p1 == q0)) && fp0 + fp1 == fq0 + fq1)) return 1;
     const h264g_mb *mp = &P.mbs[ap], *mq = &P.mbs[aq];
