@@ -776,13 +776,16 @@ __device__ void intra_mb(const DevPic &P, int addr, WaveIntra *wi, const Avail &
 }
 
 // ------------------------------------------------------------ deblocking (8.7)
+// Scalar fields only, so that selecting between views and per-lane field choices stay
+// in registers.
 struct MbView {
     int cls, kind, nz4, nz8;
-    uint32_t ref[2];  // reference DPB slot per quadrant q (int8 in byte q), per list
+    uint32_t ref0, ref1;  // reference DPB slot per quadrant q (int8 in byte q), lists 0 / 1
     int pf;
-    int qdb[3];
+    uint32_t qp;          // deblocking qP of luma, Cb edges, Cr edges in bytes 0, 1, 2
 };
-__device__ __forceinline__ int view_ref(const MbView &v, int l, int q) { return (int8_t)(v.ref[l] >> (8 * q)); }
+__device__ __forceinline__ int view_ref(const MbView &v, int l, int q) { return (int8_t)((l ? v.ref1 : v.ref0) >> (8 * q)); }
+__device__ __forceinline__ int view_qp(const MbView &v, int plane) { return (v.qp >> (8 * plane)) & 255; }
 
 __device__ __forceinline__ void load_view(const DevPic &P, int addr, MbView &v) {
     const DevAux ax = P.aux[addr];
@@ -791,17 +794,17 @@ __device__ __forceinline__ void load_view(const DevPic &P, int addr, MbView &v) 
     v.nz4 = ax.nz4;
     v.nz8 = ax.nz8;
     v.pf = ax.pf;
-#pragma unroll
-    for (int l = 0; l < 2; l++)
-        v.ref[l] = (uint32_t)(uint8_t)ax.refid[l][0] | ((uint32_t)(uint8_t)ax.refid[l][1] << 8) |
-                   ((uint32_t)(uint8_t)ax.refid[l][2] << 16) | ((uint32_t)(uint8_t)ax.refid[l][3] << 24);
-    v.qdb[0] = ax.qdb[0]; v.qdb[1] = ax.qdb[1]; v.qdb[2] = ax.qdb[2];
+    v.ref0 = (uint32_t)(uint8_t)ax.refid[0][0] | ((uint32_t)(uint8_t)ax.refid[0][1] << 8) |
+             ((uint32_t)(uint8_t)ax.refid[0][2] << 16) | ((uint32_t)(uint8_t)ax.refid[0][3] << 24);
+    v.ref1 = (uint32_t)(uint8_t)ax.refid[1][0] | ((uint32_t)(uint8_t)ax.refid[1][1] << 8) |
+             ((uint32_t)(uint8_t)ax.refid[1][2] << 16) | ((uint32_t)(uint8_t)ax.refid[1][3] << 24);
+    v.qp = (uint32_t)ax.qdb[0] | ((uint32_t)ax.qdb[1] << 8) | ((uint32_t)ax.qdb[2] << 16);
 }
 
 __device__ __forceinline__ bool mv_far(int ax, int ay, int bx, int by) { return iabs(ax - bx) >= 4 || iabs(ay - by) >= 4; }
 
 // 8.7.2.1 (Deblock.cpp:994-1310) with quirk Q3; I_PCM counts as neither intra nor inter.
-__device__ int bs_luma(const DevPic &P, int ap, const MbView &Pv, int aq, const MbView &Qv, int mb_edge, int xp, int yp, int xq, int yq) {
+__device__ __forceinline__ int bs_luma(const DevPic &P, int ap, const MbView &Pv, int aq, const MbView &Qv, int mb_edge, int xp, int yp, int xq, int yq) {
     bool pin = Pv.kind & AUX_INTRA, qin = Qv.kind & AUX_INTRA;
     if (mb_edge && (pin || qin)) return 4;
     if (pin || qin) return 3;
@@ -837,7 +840,7 @@ __device__ int bs_luma(const DevPic &P, int ap, const MbView &Pv, int aq, const 
 }
 
 // 8.7.2.2 - 8.7.2.4 on samples p[0..3] (p0 first) and q[0..3] (Deblock.cpp:1314-1522).
-__device__ void filter_line(int p[4], int q[4], int bS, bool chroma, int qavg, int offA, int offB) {
+__device__ __forceinline__ void filter_line(int p[4], int q[4], int bS, bool chroma, int qavg, int offA, int offB) {
     int indexA = clip3(0, 51, qavg + offA), indexB = clip3(0, 51, qavg + offB);
     int alpha = c_alpha[indexA], beta = c_beta[indexB];
     int p0 = p[0], p1 = p[1], p2 = p[2], p3 = p[3], q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3];
@@ -910,61 +913,57 @@ __device__ void dbk_load(const DbkPlanes &d, int mbx, int mby, int region, WaveD
     }
 }
 
+// One edge step: the edge at position e (0, 4, 8, 12 luma samples) in direction DIR
+// (0 = vertical edge, 1 = horizontal), luma in lanes 0-15, Cb in 16-23, Cr in 32-39.
+// Pv is the view of the MB on the p side (the left / top MB for e = 0, else the MB itself).
+template <int DIR>
+__device__ __forceinline__ void dbk_edge(const DevPic &P, int addr, int e, int ap, const MbView &Pv, const MbView &Qv,
+                                         WaveDbk *wd) {
+    const int lane = wave_lane();
+    const int plane = lane < 16 ? 0 : (lane < 24 ? 1 : (lane >= 32 && lane < 40 ? 2 : -1));
+    bool active = plane >= 0;
+    if (plane == 0 && ((e & 4) && (Qv.kind & AUX_T8))) active = false;  // internal 4 / 12 with 8x8 transform
+    if (plane > 0 && e >= 8) active = false;                             // chroma edges 0 and 4 only
+    if (!active) return;
+    const int k = plane == 0 ? lane : (plane == 1 ? lane - 16 : lane - 32);
+    const int n = plane == 0 ? 16 : 8, sc = plane == 0 ? 1 : 2;
+    const int xq = DIR == 0 ? e : k, yq = DIR == 0 ? k : e;
+    const int xp = DIR == 0 ? (e ? e - 1 : n - 1) : k, yp = DIR == 0 ? k : (e ? e - 1 : n - 1);
+    const int bS = bs_luma(P, ap, Pv, addr, Qv, e == 0, xp * sc, yp * sc, xq * sc, yq * sc);
+    const int qavg = (view_qp(Pv, plane) + view_qp(Qv, plane) + 1) >> 1;
+    uint8_t *base;
+    int stride;
+    if (plane == 0) { base = &wd->L[4][4]; stride = 20; }
+    else { base = &wd->C[plane - 1][4][4]; stride = 12; }
+    // sample i on the q side / p side of the line k
+    const int step = DIR == 0 ? 1 : stride, line = DIR == 0 ? k * stride : k;
+    uint8_t *q0 = base + line + e * step;
+    int pv[4], qv[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) { qv[i] = q0[i * step]; pv[i] = q0[-(i + 1) * step]; }
+    if (P.dbk_idc != 1) filter_line(pv, qv, bS, plane > 0, qavg, P.offA, P.offB);
+#pragma unroll
+    for (int i = 0; i < 3; i++) { q0[i * step] = (uint8_t)qv[i]; q0[-(i + 1) * step] = (uint8_t)pv[i]; }
+}
+
+template <int DIR>
+__device__ __forceinline__ void dbk_pass(const DevPic &P, int addr, int ap_edge, const MbView &Ev, bool edge, const MbView &Qv,
+                                         WaveDbk *wd) {
+    if (edge) dbk_edge<DIR>(P, addr, 0, ap_edge, Ev, Qv, wd);
+    __syncthreads();
+    for (int e = 4; e < 16; e += 4) {
+        dbk_edge<DIR>(P, addr, e, addr, Qv, Qv, wd);
+        __syncthreads();
+    }
+}
+
 // The 8 edge steps of one MB on the tile, in the reference's order (8.7, Deblock.cpp:76-635):
-// vertical edges left to right, then horizontal edges top to bottom; luma in lanes 0-15,
-// Cb in 16-23, Cr in 32-39.  Lv / Tv are the left / top MB views (read only when present).
+// vertical edges left to right, then horizontal edges top to bottom.  Lv / Tv are the
+// left / top MB views (read only when present).
 __device__ void dbk_filter(const DevPic &P, int addr, const MbView &Qv, const MbView &Lv, const MbView &Tv, bool left,
                            bool top, WaveDbk *wd) {
-    const int lane = wave_lane();
-    const int W = P.W;
-    const bool t8 = Qv.kind & AUX_T8;
-    for (int dir = 0; dir < 2; dir++) {
-        for (int s = 0; s < 4; s++) {
-            int e = 4 * s;
-            int plane = lane < 16 ? 0 : (lane < 24 ? 1 : (lane >= 32 && lane < 40 ? 2 : -1));
-            bool active = plane >= 0;
-            if (plane == 0 && ((s & 1) && t8)) active = false;       // internal 4 / 12 with 8x8 transform
-            if (plane > 0 && s >= 2) active = false;                 // chroma edges 0 and 4 only
-            if (e == 0 && (dir == 0 ? !left : !top)) active = false;
-            if (active) {
-                int k = plane == 0 ? lane : (plane == 1 ? lane - 16 : lane - 32);
-                int n = plane == 0 ? 16 : 8, sc = plane == 0 ? 1 : 2;
-                int xq = dir == 0 ? e : k, yq = dir == 0 ? k : e;
-                int xp = dir == 0 ? (e ? e - 1 : n - 1) : k, yp = dir == 0 ? k : (e ? e - 1 : n - 1);
-                const MbView &Pv = e ? Qv : (dir == 0 ? Lv : Tv);
-                int ap = e ? addr : (dir == 0 ? addr - 1 : addr - W);
-                int bS = bs_luma(P, ap, Pv, addr, Qv, e == 0, xp * sc, yp * sc, xq * sc, yq * sc);
-                int qavg = (Pv.qdb[plane] + Qv.qdb[plane] + 1) >> 1;
-                int pv[4], qv[4];
-                if (plane == 0) {
-                    for (int i = 0; i < 4; i++) {
-                        if (dir == 0) { qv[i] = wd->L[k + 4][e + i + 4]; pv[i] = wd->L[k + 4][e - i - 1 + 4]; }
-                        else { qv[i] = wd->L[e + i + 4][k + 4]; pv[i] = wd->L[e - i - 1 + 4][k + 4]; }
-                    }
-                } else {
-                    uint8_t (*C)[12] = wd->C[plane - 1];
-                    for (int i = 0; i < 4; i++) {
-                        if (dir == 0) { qv[i] = C[k + 4][e + i + 4]; pv[i] = C[k + 4][e - i - 1 + 4]; }
-                        else { qv[i] = C[e + i + 4][k + 4]; pv[i] = C[e - i - 1 + 4][k + 4]; }
-                    }
-                }
-                if (P.dbk_idc != 1) filter_line(pv, qv, bS, plane > 0, qavg, P.offA, P.offB);
-                if (plane == 0) {
-                    for (int i = 0; i < 3; i++) {
-                        if (dir == 0) { wd->L[k + 4][e + i + 4] = (uint8_t)qv[i]; wd->L[k + 4][e - i - 1 + 4] = (uint8_t)pv[i]; }
-                        else { wd->L[e + i + 4][k + 4] = (uint8_t)qv[i]; wd->L[e - i - 1 + 4][k + 4] = (uint8_t)pv[i]; }
-                    }
-                } else {
-                    uint8_t (*C)[12] = wd->C[plane - 1];
-                    for (int i = 0; i < 3; i++) {
-                        if (dir == 0) { C[k + 4][e + i + 4] = (uint8_t)qv[i]; C[k + 4][e - i - 1 + 4] = (uint8_t)pv[i]; }
-                        else { C[e + i + 4][k + 4] = (uint8_t)qv[i]; C[e - i - 1 + 4][k + 4] = (uint8_t)pv[i]; }
-                    }
-                }
-            }
-            __syncthreads();
-        }
-    }
+    dbk_pass<0>(P, addr, addr - 1, Lv, left, Qv, wd);
+    dbk_pass<1>(P, addr, addr - P.W, Tv, top, Qv, wd);
 }
 
 // Writes back the MB and the strips its MB edges modified (3 columns / rows).
