@@ -129,7 +129,9 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--waves", type=int, default=0)
     ap.add_argument("--no-graphs", action="store_true", help="launch batch stages directly instead of hipGraph replay")
-    ap.add_argument("--deblock-per-mb", action="store_true", help="deblock as a per-MB wavefront instead of MB rows")
+    ap.add_argument("--deblock", choices=["picture_wg", "mb_wavefront", "mb_rows"], default="picture_wg",
+                    help="deblocking schedule")
+    ap.add_argument("--deblock-waves", type=int, default=0, help="waves per workgroup of the picture_wg schedule")
     ap.add_argument("--profile-run", action="store_true",
                     help="short run for rocprofv3: 1 timed step, no warmup, no latency / plain / CPU legs, "
                          "then the library's calibration copy of --copy-mib MiB")
@@ -150,8 +152,9 @@ def main():
         gens = list(ex.map(lambda s: Generator(2, **s), specs))
     log(f"generated {len(gens)} streams in {time.perf_counter() - t:.1f} s")
 
-    eng = Engine(device=0, profile=False, waves=args.waves, graphs=not args.no_graphs,
-                 deblock_rows=not args.deblock_per_mb)
+    schedule = {"picture_wg": 0, "mb_wavefront": 1, "mb_rows": 2}[args.deblock]
+    eng = Engine(device=0, profile=False, waves=args.waves, graphs=not args.no_graphs, deblock_schedule=schedule,
+                 deblock_waves=args.deblock_waves)
     sids, slots, pics = [], {}, []
     alg = [0, 0, 0]
     for g in gens:
@@ -276,7 +279,7 @@ def main():
                    "streams": len(gens), "frames_per_stream": nframes, "pictures_per_step": len(gens) * nframes,
                    "mpix_per_step": round(mpix_step, 3)},
         "graphs": not args.no_graphs,
-        "deblock_schedule": "per-MB wavefront" if args.deblock_per_mb else "row-pipelined",
+        "deblock_schedule": args.deblock,
         "batched_event_timed_value": round(mpix_step * args.steps / t_events, 1),
         "plain_value": round(plain, 1) if plain else None,
         "batched_over_plain": round(batched / plain, 2) if plain else None,

@@ -78,9 +78,10 @@ namespace {
 struct Batch {
     void *dmem = nullptr;
     DevPic *d_pics = nullptr;
-    uint32_t *d_inter = nullptr, *d_intra = nullptr, *d_dbk = nullptr;
-    uint32_t n_inter = 0, n_intra = 0, n_dbk = 0;
-    bool dbk_rows = false;           // d_dbk holds MB rows (row-pipelined deblocking)
+    uint32_t n_pics = 0;
+    uint32_t *d_inter = nullptr, *d_intra = nullptr, *d_all = nullptr, *d_rows = nullptr;
+    uint32_t n_inter = 0, n_intra = 0, n_all = 0, n_rows = 0;
+    bool dbk_off = false;            // no picture of the batch is deblocked
     hipGraphExec_t graph = nullptr;  // the captured stage sequence (kernel nodes only)
 };
 }  // namespace
@@ -98,7 +99,8 @@ struct h264g_ctx {
     std::vector<h264g_batch *> deferred;  // batches freed at the next sync
     bool profile = false;
     bool no_graphs = false;
-    bool dbk_per_mb = false;       // deblocking as a per-MB wavefront instead of MB rows
+    int dbk_schedule = H264G_DEBLOCK_PICTURE_WG;
+    int dbk_waves = 16;            // waves per workgroup of the per-picture schedule
     std::vector<hipEvent_t> ev;    // 4 events per profiled run: inter | intra | deblock | end
     size_t ev_used = 0;            // events recorded since the last h264g_stage_times
     std::mutex mu;
@@ -417,7 +419,7 @@ int stage_picture(h264g_ctx *ctx, Stream &st, const h264g_pic_desc &d, const h26
     size_t o_wt = al(o_lv + n_levels * sizeof(int16_t) + 16), o_it = al(o_wt + wt.size() * sizeof(DevWeight));
     pic->n_rows = (uint32_t)H;
     size_t n_items = pic->n_inter + pic->n_intra + n_mbs + (size_t)H;
-    size_t o_pic = al(o_it + n_items * sizeof(uint32_t)), total = al(o_pic + sizeof(DevPic));
+    size_t o_bs = al(o_it + n_items * sizeof(uint32_t)), o_pic = al(o_bs + (size_t)n_mbs * 32), total = al(o_pic + sizeof(DevPic));
     void *dm = nullptr;
     HIPCHK(hipMalloc(&dm, total));
     pic->dmem = dm;
@@ -428,6 +430,7 @@ int stage_picture(h264g_ctx *ctx, Stream &st, const h264g_pic_desc &d, const h26
     pic->d_wt = (DevWeight *)(base + o_wt);
     pic->d_items = (uint32_t *)(base + o_it);
     pic->d_pic = (DevPic *)(base + o_pic);
+    uint8_t *d_bs = (uint8_t *)(base + o_bs);
     std::vector<uint32_t> items;
     items.reserve(n_items);
     items.insert(items.end(), pic->inter_list.begin(), pic->inter_list.end());
@@ -437,7 +440,7 @@ int stage_picture(h264g_ctx *ctx, Stream &st, const h264g_pic_desc &d, const h26
     DevPic &P = pic->hpic;
     std::memset(&P, 0, sizeof(P));
     P.mbs = pic->d_mbs; P.aux = pic->d_aux; P.levels = pic->d_levels; P.wt = pic->d_wt;
-    P.dpb = st.dpb; P.slot_bytes = st.slot_bytes; P.flags = st.flags; P.rowprog = st.rowprog; P.dst = d.dst_slot;
+    P.dpb = st.dpb; P.slot_bytes = st.slot_bytes; P.flags = st.flags; P.rowprog = st.rowprog; P.bs = d_bs; P.dst = d.dst_slot;
     P.W = W; P.H = H; P.offA = d.filter_offset_a; P.offB = d.filter_offset_b;
     P.dbk_idc = d.disable_deblocking_filter_idc; P.cip = d.constrained_intra_pred_flag;
     HIPCHK(hipMemcpy(pic->d_mbs, mbs, n_mbs * sizeof(h264g_mb), hipMemcpyHostToDevice));
@@ -450,9 +453,18 @@ int stage_picture(h264g_ctx *ctx, Stream &st, const h264g_pic_desc &d, const h26
     return 0;
 }
 
-// dbk holds MB addresses in t order (per-MB wavefront) or MB rows (dbk_rows).
-int launch_stages(h264g_ctx *ctx, const DevPic *d_pics, const uint32_t *inter, uint32_t n_inter, const uint32_t *intra,
-                  uint32_t n_intra, const uint32_t *dbk, uint32_t n_dbk, bool dbk_off, bool dbk_rows) {
+// The launches of one picture or batch.  all = every MB (t order: it serves the bS
+// pre-pass and the per-MB deblocking wavefront); rows = MB rows (row index major) for
+// the cross-CU row schedule.
+struct StageSet {
+    const DevPic *pics;
+    uint32_t n_pics;
+    const uint32_t *inter, *intra, *all, *rows;
+    uint32_t n_inter, n_intra, n_all, n_rows;
+    bool dbk_off;
+};
+
+int launch_stages(h264g_ctx *ctx, const StageSet &q) {
     hipStream_t s = ctx->stream;
     hipEvent_t *ev = nullptr;
     if (ctx->profile) {
@@ -465,15 +477,23 @@ int launch_stages(h264g_ctx *ctx, const DevPic *d_pics, const uint32_t *inter, u
         ctx->ev_used += 4;
     }
     if (ev) HIPCHK(hipEventRecord(ev[0], s));
-    int e = h264g_launch_inter(d_pics, inter, n_inter, ctx->d_err, s);
+    int e = h264g_launch_inter(q.pics, q.inter, q.n_inter, ctx->d_err, s);
     if (e) return h264g_set_error(H264G_E_DEVICE, "inter launch: %s", hipGetErrorString((hipError_t)e));
     if (ev) HIPCHK(hipEventRecord(ev[1], s));
-    e = h264g_launch_intra(d_pics, intra, n_intra, &ctx->d_ctl[0], &ctx->d_ctl[2].ticket, ctx->d_err, ctx->waves, s);
+    e = h264g_launch_intra(q.pics, q.intra, q.n_intra, &ctx->d_ctl[0], &ctx->d_ctl[2].ticket, ctx->d_err, ctx->waves, s);
     if (e) return h264g_set_error(H264G_E_DEVICE, "intra launch: %s", hipGetErrorString((hipError_t)e));
     if (ev) HIPCHK(hipEventRecord(ev[2], s));
-    if (!dbk_off) {
-        e = dbk_rows ? h264g_launch_deblock_rows(d_pics, dbk, n_dbk, &ctx->d_ctl[1], &ctx->d_ctl[2].ticket, ctx->d_err, ctx->waves, s)
-                     : h264g_launch_deblock(d_pics, dbk, n_dbk, &ctx->d_ctl[1], &ctx->d_ctl[2].ticket, ctx->d_err, ctx->waves, s);
+    if (!q.dbk_off) {
+        e = h264g_launch_bs(q.pics, q.all, q.n_all, s);
+        if (e) return h264g_set_error(H264G_E_DEVICE, "bS launch: %s", hipGetErrorString((hipError_t)e));
+        switch (ctx->dbk_schedule) {
+        case H264G_DEBLOCK_PICTURE_WG: e = h264g_launch_deblock_pic(q.pics, q.n_pics, ctx->dbk_waves, ctx->d_err, s); break;
+        case H264G_DEBLOCK_MB_WAVEFRONT:
+            e = h264g_launch_deblock(q.pics, q.all, q.n_all, &ctx->d_ctl[1], &ctx->d_ctl[2].ticket, ctx->d_err, ctx->waves, s);
+            break;
+        default:
+            e = h264g_launch_deblock_rows(q.pics, q.rows, q.n_rows, &ctx->d_ctl[1], &ctx->d_ctl[2].ticket, ctx->d_err, ctx->waves, s);
+        }
         if (e) return h264g_set_error(H264G_E_DEVICE, "deblock launch: %s", hipGetErrorString((hipError_t)e));
     }
     if (ev) HIPCHK(hipEventRecord(ev[3], s));
@@ -515,7 +535,12 @@ int32_t h264g_create(const h264g_config *cfg, h264g_ctx **out) {
     c->budget = cfg ? cfg->dpb_budget_bytes : 0;
     c->profile = cfg && cfg->profile;
     c->no_graphs = cfg && cfg->no_graphs;
-    c->dbk_per_mb = cfg && cfg->deblock_per_mb;
+    c->dbk_schedule = cfg ? cfg->deblock_schedule : H264G_DEBLOCK_PICTURE_WG;
+    c->dbk_waves = (cfg && cfg->deblock_waves > 0) ? cfg->deblock_waves : 16;
+    if (c->dbk_schedule < 0 || c->dbk_schedule > 2 || c->dbk_waves > 16) {
+        h264g_destroy(c);
+        return h264g_set_error(H264G_E_ARG, "deblock_schedule %d / deblock_waves %d", cfg->deblock_schedule, cfg->deblock_waves);
+    }
     c->waves = (cfg && cfg->waves > 0) ? cfg->waves : prop.multiProcessorCount * 8;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipMalloc(&c->d_err, sizeof(uint32_t)) != hipSuccess || hipMalloc(&c->d_ctl, 4 * sizeof(DevCtl)) != hipSuccess ||
@@ -553,6 +578,7 @@ int32_t h264g_stream_open(h264g_ctx *c, const h264g_stream_desc *d, uint32_t *id
         return h264g_set_error(H264G_E_ARG, "picture size");
     if (d->chroma_array_type != 1 || d->bit_depth != 8) return h264g_set_error(H264G_E_ARG, "only 8-bit 4:2:0 is supported");
     if (d->dpb_slots < 1 || d->dpb_slots > 17) return h264g_set_error(H264G_E_ARG, "dpb_slots");
+    if (d->frame_height_in_mbs > 1024) return h264g_set_error(H264G_E_ARG, "frame_height_in_mbs > 1024");
     if (!d->frame_mbs_only || d->mbaff) return h264g_set_error(H264G_E_ARG, "field coding is not supported");
     Stream s;
     s.desc = *d;
@@ -632,10 +658,19 @@ int32_t h264g_picture_submit(h264g_ctx *c, h264g_pic *p) {
     std::lock_guard<std::mutex> lk(c->mu);
     HIPCHK(hipSetDevice(c->device));
     const uint32_t *it = p->d_items;
-    const uint32_t *dbk = it + p->n_inter + p->n_intra;
-    const bool rows = !c->dbk_per_mb;
-    return launch_stages(c, p->d_pic, it, p->n_inter, it + p->n_inter, p->n_intra, rows ? dbk + p->n_mbs : dbk,
-                         rows ? p->n_rows : p->n_mbs, p->desc.disable_deblocking_filter_idc == 1, rows);
+    StageSet q;
+    q.pics = p->d_pic;
+    q.n_pics = 1;
+    q.inter = it;
+    q.n_inter = p->n_inter;
+    q.intra = it + p->n_inter;
+    q.n_intra = p->n_intra;
+    q.all = q.intra + p->n_intra;
+    q.n_all = p->n_mbs;
+    q.rows = q.all + p->n_mbs;
+    q.n_rows = p->n_rows;
+    q.dbk_off = p->desc.disable_deblocking_filter_idc == 1;
+    return launch_stages(c, q);
 }
 
 int32_t h264g_picture_release(h264g_ctx *c, h264g_pic *p) {
@@ -692,35 +727,39 @@ int32_t h264g_batch_create(h264g_ctx *c, h264g_pic *const *pics, uint32_t n, h26
     merge(true, intra);
     bool any_dbk = false;
     for (uint32_t i = 0; i < n; i++) any_dbk |= pics[i]->desc.disable_deblocking_filter_idc != 1;
-    const bool rows = !c->dbk_per_mb;
-    if (any_dbk && !rows) merge(false, dbk);
-    if (any_dbk && rows) {
+    std::vector<uint32_t> all, rows;
+    if (any_dbk) merge(false, all);
+    if (any_dbk && c->dbk_schedule == H264G_DEBLOCK_MB_ROWS) {
         // MB rows of all pictures, row index major: the row above always comes first.
         int maxh = 0;
         for (uint32_t i = 0; i < n; i++) maxh = std::max(maxh, pics[i]->H);
         for (int y = 0; y < maxh; y++)
             for (uint32_t i = 0; i < n; i++)
-                if (y < pics[i]->H) dbk.push_back((i << WI_PIC_SHIFT) | (uint32_t)y);
+                if (y < pics[i]->H) rows.push_back((i << WI_PIC_SHIFT) | (uint32_t)y);
     }
     auto al = [](size_t v) { return (v + 255) & ~(size_t)255; };
-    size_t o_p = 0, o_i = al(n * sizeof(DevPic)), o_a = al(o_i + inter.size() * 4), o_d = al(o_a + intra.size() * 4);
-    size_t total = al(o_d + dbk.size() * 4 + 4);
+    size_t o_p = 0, o_i = al(n * sizeof(DevPic)), o_a = al(o_i + inter.size() * 4), o_l = al(o_a + intra.size() * 4);
+    size_t o_r = al(o_l + all.size() * 4), total = al(o_r + rows.size() * 4 + 4);
     h264g_batch *b = new (std::nothrow) h264g_batch();
     if (!b) return h264g_set_error(H264G_E_NOMEM, "batch");
     if (hipMalloc(&b->dmem, total) != hipSuccess) { delete b; return h264g_set_error(H264G_E_NOMEM, "batch device memory"); }
     char *base = (char *)b->dmem;
     b->d_pics = (DevPic *)(base + o_p);
+    b->n_pics = n;
     b->d_inter = (uint32_t *)(base + o_i);
     b->d_intra = (uint32_t *)(base + o_a);
-    b->d_dbk = (uint32_t *)(base + o_d);
+    b->d_all = (uint32_t *)(base + o_l);
+    b->d_rows = (uint32_t *)(base + o_r);
     b->n_inter = (uint32_t)inter.size();
     b->n_intra = (uint32_t)intra.size();
-    b->n_dbk = (uint32_t)dbk.size();
-    b->dbk_rows = rows;
+    b->n_all = (uint32_t)all.size();
+    b->n_rows = (uint32_t)rows.size();
+    b->dbk_off = !any_dbk;
     if (hipMemcpy(b->d_pics, hp.data(), n * sizeof(DevPic), hipMemcpyHostToDevice) != hipSuccess ||
         (!inter.empty() && hipMemcpy(b->d_inter, inter.data(), inter.size() * 4, hipMemcpyHostToDevice) != hipSuccess) ||
         (!intra.empty() && hipMemcpy(b->d_intra, intra.data(), intra.size() * 4, hipMemcpyHostToDevice) != hipSuccess) ||
-        (!dbk.empty() && hipMemcpy(b->d_dbk, dbk.data(), dbk.size() * 4, hipMemcpyHostToDevice) != hipSuccess)) {
+        (!all.empty() && hipMemcpy(b->d_all, all.data(), all.size() * 4, hipMemcpyHostToDevice) != hipSuccess) ||
+        (!rows.empty() && hipMemcpy(b->d_rows, rows.data(), rows.size() * 4, hipMemcpyHostToDevice) != hipSuccess)) {
         free_batch(b);
         return h264g_set_error(H264G_E_DEVICE, "batch upload");
     }
@@ -728,17 +767,35 @@ int32_t h264g_batch_create(h264g_ctx *c, h264g_pic *const *pics, uint32_t n, h26
     return H264G_OK;
 }
 
+namespace {
+StageSet batch_stages(const h264g_batch *b) {
+    StageSet q;
+    q.pics = b->d_pics;
+    q.n_pics = b->n_pics;
+    q.inter = b->d_inter;
+    q.n_inter = b->n_inter;
+    q.intra = b->d_intra;
+    q.n_intra = b->n_intra;
+    q.all = b->d_all;
+    q.n_all = b->n_all;
+    q.rows = b->d_rows;
+    q.n_rows = b->n_rows;
+    q.dbk_off = b->dbk_off;
+    return q;
+}
+}  // namespace
+
 int32_t h264g_batch_run(h264g_ctx *c, h264g_batch *b) {
     if (!c || !b) return h264g_set_error(H264G_E_ARG, "NULL argument");
     std::lock_guard<std::mutex> lk(c->mu);
     HIPCHK(hipSetDevice(c->device));
     if (c->profile || c->no_graphs)
-        return launch_stages(c, b->d_pics, b->d_inter, b->n_inter, b->d_intra, b->n_intra, b->d_dbk, b->n_dbk, b->n_dbk == 0, b->dbk_rows);
+        return launch_stages(c, batch_stages(b));
     if (!b->graph) {
         // Capture the three stage launches once; every launch argument (including the
         // wavefront epoch, which lives in device memory) is the same on each replay.
         HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        int rc = launch_stages(c, b->d_pics, b->d_inter, b->n_inter, b->d_intra, b->n_intra, b->d_dbk, b->n_dbk, b->n_dbk == 0, b->dbk_rows);
+        int rc = launch_stages(c, batch_stages(b));
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamEndCapture(c->stream, &g);
         if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }

@@ -47,12 +47,13 @@ struct DevPic {
     uint32_t *flags;          // per-MB wavefront progress words of the stream
     uint64_t *rowprog;        // per-MB-row progress of row-pipelined deblocking:
                               // epoch << 32 | MBs of the row finished
+    uint8_t  *bs;             // boundary strengths, 32 per MB: [dir][edge][4-line segment]
     int32_t  dst;             // destination slot
     int32_t  W, H;            // in macroblocks
     int32_t  offA, offB;      // FilterOffsetA / FilterOffsetB
     int32_t  dbk_idc;         // disable_deblocking_filter_idc
     int32_t  cip;             // constrained_intra_pred_flag
-    int32_t  pad[1];
+    int32_t  pad[3];
 };
 
 // Work item: picture index in the batch and macroblock address.
@@ -80,5 +81,8 @@ int h264g_launch_deblock(const DevPic *pics, const uint32_t *items, uint32_t n, 
                          uint32_t *err, int waves, void *stream);
 int h264g_launch_deblock_rows(const DevPic *pics, const uint32_t *items, uint32_t n, DevCtl *ctl, uint32_t *epoch,
                               uint32_t *err, int waves, void *stream);
+// One workgroup of `waves` waves per picture (pics[0..n_pics)); rows of up to 1024 MBs.
+int h264g_launch_deblock_pic(const DevPic *pics, uint32_t n_pics, int waves, uint32_t *err, void *stream);
+int h264g_launch_bs(const DevPic *pics, const uint32_t *items, uint32_t n, void *stream);
 int h264g_launch_copy(const uint8_t *src, uint8_t *dst, uint64_t bytes, void *stream);
 }

@@ -69,6 +69,16 @@ __device__ __forceinline__ int blk_y(int b) { return ((b >> 3) & 1) * 8 + ((b >>
 
 __device__ __forceinline__ int wave_lane() { return threadIdx.x & (WAVE - 1); }
 
+// Orders the LDS accesses of the lanes of one wave (LDS instructions of a wave execute in
+// order; the fences keep the compiler from moving memory operations across).  Every
+// LDS working set below belongs to one wave, so this is the only barrier the per-MB code
+// needs, also when several waves share a workgroup.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // ------------------------------------------------------------ residuals
 // LDS scratch of one wave: dense scan-order levels of the MB (slots: 0 = Intra16x16 DC,
 // 1 + luma4x4BlkIdx (8x8 block q = slots 1+4q..4+4q), 17/18 = chroma DC Cb/Cr,
@@ -142,7 +152,7 @@ __device__ __forceinline__ bool build_residual(const h264g_mb *mb, const DevAux 
     const int qp = mb->qp_y;
     int32_t *cf = &ws->coef[0][0];
     for (int k = lane; k < 27 * 16; k += WAVE) cf[k] = 0;
-    __syncthreads();
+    wave_sync();
     const int16_t *lv = levels + mb->level_offset;
     const int nmask = ax.nmask;
     int mask = lane < nmask ? (uint16_t)lv[lane] : 0;
@@ -164,7 +174,7 @@ __device__ __forceinline__ bool build_residual(const h264g_mb *mb, const DevAux 
             ws->coef[slot][k] = vals[n++];
         }
     }
-    __syncthreads();
+    wave_sync();
     if (t8) {
         // 8.5.13: lanes 0-31 = (8x8 block q, row i): dequantise + horizontal pass ...
         if (lane < 32) {
@@ -232,7 +242,7 @@ __device__ __forceinline__ bool build_residual(const h264g_mb *mb, const DevAux 
         for (int k = 0; k < 16; k++) ws->rc[c][(y0 + (k >> 2)) * 8 + x0 + (k & 3)] = (int16_t)r[k];
     }
     if (t8) {
-        __syncthreads();
+        wave_sync();
         // ... then lanes 0-31 = (block q, column j): vertical pass and (x + 32) >> 6
         if (lane < 32) {
             const int q = lane >> 3, j = lane & 7;
@@ -245,7 +255,7 @@ __device__ __forceinline__ bool build_residual(const h264g_mb *mb, const DevAux 
             for (int i = 0; i < 8; i++) ws->rl[(y0 + i) * 16 + x0 + j] = (int16_t)((d[i] + 32) >> 6);
         }
     }
-    __syncthreads();
+    wave_sync();
     return true;
 }
 
@@ -446,7 +456,7 @@ __device__ void wait_deps(const uint32_t *flags, const int *deps, int nd, uint32
 
 // The barrier orders every lane's stores before the (wave-wide) release store.
 __device__ void publish(uint32_t *flag, uint32_t epoch) {
-    __syncthreads();
+    wave_sync();
     st_release(flag, epoch);
 }
 
@@ -647,7 +657,7 @@ __device__ void intra_mb(const DevPic &P, int addr, WaveIntra *wi, const Avail &
         const uint8_t *pl = c == 0 ? dcb : dcr;
         if (av.A) wi->C[c][y + 1][0] = pl[(uint64_t)(mby * 8 + y) * wc + mbx * 8 - 1];
     }
-    __syncthreads();
+    wave_sync();
     // ---- luma
     if (mb->mb_type == H264G_MB_I4x4) {
         for (int blk = 0; blk < 16; blk++) {
@@ -658,7 +668,7 @@ __device__ void intra_mb(const DevPic &P, int addr, WaveIntra *wi, const Avail &
                 if (res) v = clip1(v + wi->res.rl[yy * 16 + xx]);
                 wi->T[yy + 1][xx + 1] = (uint8_t)v;
             }
-            __syncthreads();
+            wave_sync();
         }
     } else if (mb->mb_type == H264G_MB_I8x8) {
         for (int blk = 0; blk < 4; blk++) {
@@ -698,13 +708,13 @@ __device__ void intra_mb(const DevPic &P, int addr, WaveIntra *wi, const Avail &
                 }
                 wi->pf[lane] = (int16_t)v;
             }
-            __syncthreads();
+            wave_sync();
             int x = lane & 7, y = lane >> 3;
             int v = pred8x8(wi, mb->intra_pred_modes[blk], x, y);
             if (res) v = clip1(v + wi->res.rl[(yO + y) * 16 + xO + x]);
-            __syncthreads();
+            wave_sync();
             wi->T[yO + y + 1][xO + x + 1] = (uint8_t)v;
-            __syncthreads();
+            wave_sync();
         }
     } else {  // Intra_16x16, 8.3.3 (H264PictureBase.cpp:1847-2056)
         int mode = mb->intra16x16_pred_mode;
@@ -731,7 +741,7 @@ __device__ void intra_mb(const DevPic &P, int addr, WaveIntra *wi, const Avail &
             if (res) v = clip1(v + wi->res.rl[y * 16 + x]);
             wi->T[y + 1][x + 1] = (uint8_t)v;  // only this lane reads/writes pixel i
         }
-        __syncthreads();
+        wave_sync();
     }
     // ---- chroma, 8.3.4 (H264PictureBase.cpp:2076-2381); DC tests "> 0" (quirk Q1)
     {
@@ -929,7 +939,11 @@ __device__ __forceinline__ void dbk_edge(const DevPic &P, int addr, int e, int a
     const int n = plane == 0 ? 16 : 8, sc = plane == 0 ? 1 : 2;
     const int xq = DIR == 0 ? e : k, yq = DIR == 0 ? k : e;
     const int xp = DIR == 0 ? (e ? e - 1 : n - 1) : k, yp = DIR == 0 ? k : (e ? e - 1 : n - 1);
-    const int bS = bs_luma(P, ap, Pv, addr, Qv, e == 0, xp * sc, yp * sc, xq * sc, yq * sc);
+    // boundary strength of the luma segment this line maps to (chroma line k -> luma 2k,
+    // chroma edge e -> luma edge 2e), precomputed by k_bs
+    const int el = plane == 0 ? e : 2 * e, seg = plane == 0 ? (k >> 2) : (k >> 1);
+    const int bS = P.bs[(size_t)addr * 32 + DIR * 16 + (el >> 2) * 4 + seg];
+    (void)ap; (void)xp; (void)yp; (void)xq; (void)yq; (void)sc;
     const int qavg = (view_qp(Pv, plane) + view_qp(Qv, plane) + 1) >> 1;
     uint8_t *base;
     int stride;
@@ -950,10 +964,10 @@ template <int DIR>
 __device__ __forceinline__ void dbk_pass(const DevPic &P, int addr, int ap_edge, const MbView &Ev, bool edge, const MbView &Qv,
                                          WaveDbk *wd) {
     if (edge) dbk_edge<DIR>(P, addr, 0, ap_edge, Ev, Qv, wd);
-    __syncthreads();
+    wave_sync();
     for (int e = 4; e < 16; e += 4) {
         dbk_edge<DIR>(P, addr, e, addr, Qv, Qv, wd);
-        __syncthreads();
+        wave_sync();
     }
 }
 
@@ -993,7 +1007,7 @@ __device__ void deblock_mb(const DevPic &P, int addr, WaveDbk *wd) {
     load_view(P, addr, Qv);
     if (left) load_view(P, addr - 1, Lv);
     if (top) load_view(P, addr - W, Tv);
-    __syncthreads();
+    wave_sync();
     dbk_filter(P, addr, Qv, Lv, Tv, left, top, wd);
     dbk_store(d, mbx, mby, left, top, wd);
 }
@@ -1001,7 +1015,7 @@ __device__ void deblock_mb(const DevPic &P, int addr, WaveDbk *wd) {
 // After MB x of a row: its columns 12..15 become the left columns of MB x + 1.
 __device__ void dbk_carry_left(WaveDbk *wd) {
     const int lane = wave_lane();
-    __syncthreads();
+    wave_sync();
     for (int i = lane; i < 16 * 4 + 2 * 8 * 4; i += WAVE) {
         if (i < 64) {
             const int y = i >> 2, x = i & 3;
@@ -1011,7 +1025,7 @@ __device__ void dbk_carry_left(WaveDbk *wd) {
             wd->C[c][y + 4][x] = wd->C[c][y + 4][8 + x];
         }
     }
-    __syncthreads();
+    wave_sync();
 }
 
 __device__ __forceinline__ uint64_t ld_acquire64(const uint64_t *p) {
@@ -1034,11 +1048,50 @@ __device__ void wait_row(const uint64_t *w, uint64_t want, uint32_t *err) {
     __atomic_thread_fence(__ATOMIC_ACQUIRE);
 }
 
+// Row progress across CUs: epoch-tagged 64-bit words in global memory (agent scope).
+struct RowSyncGlobal {
+    uint64_t *prog;
+    uint64_t tag;
+    uint32_t *err;
+    __device__ void wait(int row, int need) { wait_row(&prog[row], tag | (uint32_t)need, err); }
+    __device__ void post(int row, int done) {
+        wave_sync();  // every lane's stores before the (wave-wide) release store
+        __hip_atomic_store(&prog[row], tag | (uint32_t)done, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+};
+
+// Row progress of one picture inside one workgroup: counters in LDS, workgroup scope.
+// The waves share the CU's caches, so no L2 write-back or invalidation is involved.
+constexpr uint32_t LDS_SPIN_LIMIT = 1u << 22;
+struct RowSyncLds {
+    uint32_t *prog;
+    uint32_t *err;
+    __device__ void wait(int row, int need) {
+        for (uint32_t it = 0;; ++it) {
+            const uint32_t v = uniform(__hip_atomic_load(&prog[row], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP));
+            if ((int)v >= need) break;
+            if (it >= LDS_SPIN_LIMIT) {
+                atomicOr(err, DEVERR_TIMEOUT);
+                break;
+            }
+            if ((it & 1023u) == 1023u && uniform(__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) != 0u) break;
+            __builtin_amdgcn_s_sleep(1);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    }
+    __device__ void post(int row, int done) {
+        wave_sync();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __hip_atomic_store(&prog[row], (uint32_t)done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+};
+
 // One whole MB row, left to right, in raster order.  MB (x, y) needs MB (x + 1, y - 1)
 // done (its left-edge filter changed columns 13..15 of (x, y - 1), which (x, y)'s top
 // edge reads), i.e. the row above at progress min(x + 2, W).  The left neighbour's
 // columns stay in LDS from the previous step.
-__device__ void deblock_row(const DevPic &P, int mby, uint64_t tag, uint32_t *err, WaveDbk *wd) {
+template <class Sync>
+__device__ void deblock_row(const DevPic &P, int mby, Sync &sync, WaveDbk *wd) {
     const int W = P.W;
     const DbkPlanes d = dbk_planes(P);
     const bool top = mby > 0;
@@ -1051,22 +1104,20 @@ __device__ void deblock_row(const DevPic &P, int mby, uint64_t tag, uint32_t *er
         load_view(P, addr, Qv);
         if (top) {
             load_view(P, addr - W, Tv);
-            wait_row(&P.rowprog[mby - 1], tag | (uint32_t)min(mbx + 2, W), err);
+            sync.wait(mby - 1, min(mbx + 2, W));
             dbk_load(d, mbx, mby, 2, wd);
         }
-        __syncthreads();
+        wave_sync();
         dbk_filter(P, addr, Qv, Lv, Tv, left, top, wd);
         dbk_store(d, mbx, mby, left, top, wd);
-        // publish: the barrier orders every lane's stores before the release store
-        __syncthreads();
-        __hip_atomic_store(&P.rowprog[mby], tag | (uint32_t)(mbx + 1), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        sync.post(mby, mbx + 1);
         dbk_carry_left(wd);
     }
 }
 
 // ------------------------------------------------------------ kernels
-// Every kernel runs one wave per workgroup: the LDS working sets are per wave and
-// __syncthreads is then a barrier of that wave alone, whatever path its MB takes.
+// The LDS working sets are per wave and the per-MB code synchronises with wave barriers,
+// so a workgroup may hold one wave (most kernels) or several (k_deblock_pic).
 __global__ __launch_bounds__(WAVE) void k_inter(const DevPic *__restrict__ pics, const uint32_t *__restrict__ items, uint32_t n,
                                                 uint32_t *err) {
     __shared__ WaveRes lds;
@@ -1136,9 +1187,50 @@ __global__ __launch_bounds__(WAVE) void k_deblock_rows(const DevPic *__restrict_
         if (t < 0) break;
         const uint32_t it = items[t];
         const DevPic &P = pics[it >> WI_PIC_SHIFT];
-        deblock_row(P, (int)(it & WI_MB_MASK), tag, err, &lds);
+        RowSyncGlobal sync{P.rowprog, tag, err};
+        deblock_row(P, (int)(it & WI_MB_MASK), sync, &lds);
     }
     retire(ctl, gridDim.x, epoch_p, epoch);
+}
+
+// One workgroup per picture: its waves take the picture's MB rows in turn (row y on
+// wave y mod waves) and pass row progress through LDS.
+constexpr int DBK_WG_MAX_WAVES = 16;
+constexpr int DBK_WG_MAX_ROWS = 1024;
+__global__ __launch_bounds__(WAVE * DBK_WG_MAX_WAVES) void k_deblock_pic(const DevPic *__restrict__ pics, uint32_t *err) {
+    __shared__ WaveDbk tiles[DBK_WG_MAX_WAVES];
+    __shared__ uint32_t prog[DBK_WG_MAX_ROWS];
+    const DevPic &P = pics[blockIdx.x];
+    if (P.dbk_idc == 1) return;
+    const int nw = blockDim.x / WAVE, w = threadIdx.x / WAVE;
+    for (int i = threadIdx.x; i < P.H; i += blockDim.x) prog[i] = 0;
+    __syncthreads();
+    RowSyncLds sync{prog, err};
+    for (int y = w; y < P.H; y += nw) deblock_row(P, y, sync, &tiles[w]);
+}
+
+// Boundary strengths of every edge segment of every MB (8.7.2.1, Deblock.cpp:994-1310),
+// fully parallel: they depend on syntax only, not on samples.  Half a wave per MB; lane
+// j = 16 * dir + 4 * edge + segment (4 luma lines each).
+__global__ __launch_bounds__(WAVE) void k_bs(const DevPic *__restrict__ pics, const uint32_t *__restrict__ items, uint32_t n) {
+    const int lane = wave_lane();
+    const uint32_t i = blockIdx.x * 2 + (lane >> 5);
+    if (i >= n) return;
+    const uint32_t it = items[i];
+    const DevPic &P = pics[it >> WI_PIC_SHIFT];
+    const int addr = (int)(it & WI_MB_MASK), W = P.W, mbx = addr % W, mby = addr / W;
+    const int j = lane & 31, dir = j >> 4, e = ((j >> 2) & 3) * 4, k = (j & 3) * 4;
+    int bS = 0;
+    if (e > 0 || (dir == 0 ? mbx > 0 : mby > 0)) {
+        const int ap = e ? addr : (dir == 0 ? addr - 1 : addr - W);
+        MbView Qv, Pv;
+        load_view(P, addr, Qv);
+        load_view(P, ap, Pv);
+        const int xq = dir == 0 ? e : k, yq = dir == 0 ? k : e;
+        const int xp = dir == 0 ? (e ? e - 1 : 15) : k, yp = dir == 0 ? k : (e ? e - 1 : 15);
+        bS = bs_luma(P, ap, Pv, addr, Qv, e == 0, xp, yp, xq, yq);
+    }
+    P.bs[(size_t)addr * 32 + j] = (uint8_t)bS;
 }
 
 __global__ void k_copy(const uint4 *src, uint4 *dst, uint64_t n16) {
@@ -1176,6 +1268,19 @@ int h264g_launch_deblock_rows(const DevPic *pics, const uint32_t *items, uint32_
     if (n == 0) return 0;
     uint32_t g = (uint32_t)waves < n ? (uint32_t)waves : n;
     hipLaunchKernelGGL(k_deblock_rows, dim3(g), dim3(WAVE), 0, (hipStream_t)stream, pics, items, n, ctl, epoch, err);
+    return (int)hipGetLastError();
+}
+
+int h264g_launch_deblock_pic(const DevPic *pics, uint32_t n_pics, int waves, uint32_t *err, void *stream) {
+    if (n_pics == 0) return 0;
+    if (waves < 1 || waves > DBK_WG_MAX_WAVES) return (int)hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_deblock_pic, dim3(n_pics), dim3(WAVE * waves), 0, (hipStream_t)stream, pics, err);
+    return (int)hipGetLastError();
+}
+
+int h264g_launch_bs(const DevPic *pics, const uint32_t *items, uint32_t n, void *stream) {
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(k_bs, dim3((n + 1) / 2), dim3(WAVE), 0, (hipStream_t)stream, pics, items, n);
     return (int)hipGetLastError();
 }
 

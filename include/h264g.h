@@ -142,9 +142,20 @@ typedef struct h264g_config {
     int32_t  no_graphs;           /* 1 = launch batch stages directly; 0 = replay each
                                      batch's stage sequence as a captured hipGraph
                                      (profiled contexts always launch directly)       */
-    int32_t  deblock_per_mb;      /* 1 = deblock as a wavefront of single MBs; 0 = one
-                                     wave streams each MB row (row-pipelined)         */
+    int32_t  deblock_schedule;    /* H264G_DEBLOCK_*                                  */
+    int32_t  deblock_waves;       /* waves per workgroup of H264G_DEBLOCK_PICTURE_WG,
+                                     1..16 (0 = 16)                                  */
 } h264g_config;
+
+/* Deblocking schedules (all give the raster-order result of 8.7):
+ * PICTURE_WG   one workgroup per picture; its waves take MB rows in turn and pass row
+ *              progress through LDS (default)
+ * MB_WAVEFRONT persistent waves over single MBs ordered by t = x + 2y, progress words
+ *              in device memory
+ * MB_ROWS      persistent waves over MB rows of all pictures, progress in device memory */
+#define H264G_DEBLOCK_PICTURE_WG   0
+#define H264G_DEBLOCK_MB_WAVEFRONT 1
+#define H264G_DEBLOCK_MB_ROWS      2
 
 typedef struct h264g_stream_desc {
     int32_t  pic_width_in_mbs;    /* PicWidthInMbs                                    */

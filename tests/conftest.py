@@ -30,16 +30,26 @@ def engine():
     eng.close()
 
 
+_ENGINES = {}
+
+
 @pytest.fixture(scope="session")
-def engine_per_mb():
-    """A second context that deblocks as a wavefront of single MBs."""
+def engines():
+    """Contexts per deblocking schedule (created on first use, closed at the end)."""
+    yield _ENGINES
+    for e in _ENGINES.values():
+        e.close()
+
+
+@pytest.fixture(params=["picture_wg", "mb_wavefront", "mb_rows", "picture_wg_4waves"])
+def any_engine(request, engines):
+    """Every deblocking schedule (and a small per-picture workgroup)."""
     from h264_video_decoder_demo_amd import Engine
-    eng = Engine(device=0, deblock_rows=False)
-    yield eng
-    eng.close()
-
-
-@pytest.fixture(params=["rows", "per_mb"])
-def any_engine(request):
-    """Both deblocking schedules (row-pipelined and per-MB wavefront)."""
-    return request.getfixturevalue("engine" if request.param == "rows" else "engine_per_mb")
+    from h264_video_decoder_demo_amd import _lib
+    kw = {"picture_wg": dict(deblock_schedule=_lib.DEBLOCK_PICTURE_WG),
+          "mb_wavefront": dict(deblock_schedule=_lib.DEBLOCK_MB_WAVEFRONT),
+          "mb_rows": dict(deblock_schedule=_lib.DEBLOCK_MB_ROWS),
+          "picture_wg_4waves": dict(deblock_schedule=_lib.DEBLOCK_PICTURE_WG, deblock_waves=4)}[request.param]
+    if request.param not in engines:
+        engines[request.param] = Engine(device=0, **kw)
+    return engines[request.param]
