@@ -132,6 +132,8 @@ def main():
     ap.add_argument("--deblock", choices=["picture_wg", "mb_wavefront", "mb_rows"], default="picture_wg",
                     help="deblocking schedule")
     ap.add_argument("--deblock-waves", type=int, default=0, help="waves per workgroup of the picture_wg schedule")
+    ap.add_argument("--intra", choices=["picture_wg", "mb_wavefront"], default="picture_wg", help="intra schedule")
+    ap.add_argument("--intra-waves", type=int, default=0, help="waves per workgroup of the picture_wg intra schedule")
     ap.add_argument("--profile-run", action="store_true",
                     help="short run for rocprofv3: 1 timed step, no warmup, no latency / plain / CPU legs, "
                          "then the library's calibration copy of --copy-mib MiB")
@@ -154,7 +156,8 @@ def main():
 
     schedule = {"picture_wg": 0, "mb_wavefront": 1, "mb_rows": 2}[args.deblock]
     eng = Engine(device=0, profile=False, waves=args.waves, graphs=not args.no_graphs, deblock_schedule=schedule,
-                 deblock_waves=args.deblock_waves)
+                 deblock_waves=args.deblock_waves, intra_schedule={"picture_wg": 0, "mb_wavefront": 1}[args.intra],
+                 intra_waves=args.intra_waves)
     sids, slots, pics = [], {}, []
     alg = [0, 0, 0]
     for g in gens:
@@ -280,6 +283,7 @@ def main():
                    "mpix_per_step": round(mpix_step, 3)},
         "graphs": not args.no_graphs,
         "deblock_schedule": args.deblock,
+        "intra_schedule": args.intra,
         "batched_event_timed_value": round(mpix_step * args.steps / t_events, 1),
         "plain_value": round(plain, 1) if plain else None,
         "batched_over_plain": round(batched / plain, 2) if plain else None,

@@ -81,11 +81,12 @@ class Config(ctypes.Structure):
     _fields_ = [
         ("device", c_int32), ("dpb_budget_bytes", c_uint64), ("max_batch_pictures", c_int32),
         ("profile", c_int32), ("waves", c_int32), ("no_graphs", c_int32), ("deblock_schedule", c_int32),
-        ("deblock_waves", c_int32),
+        ("deblock_waves", c_int32), ("intra_schedule", c_int32), ("intra_waves", c_int32),
     ]
 
 
 DEBLOCK_PICTURE_WG, DEBLOCK_MB_WAVEFRONT, DEBLOCK_MB_ROWS = 0, 1, 2
+INTRA_PICTURE_WG, INTRA_MB_WAVEFRONT = 0, 1
 
 
 class GenParams(ctypes.Structure):

@@ -100,6 +100,8 @@ struct h264g_ctx {
     bool profile = false;
     bool no_graphs = false;
     int dbk_schedule = H264G_DEBLOCK_PICTURE_WG;
+    int intra_schedule = H264G_INTRA_PICTURE_WG;
+    int intra_waves = 8;           // waves per workgroup of the per-picture intra schedule
     int dbk_waves = 16;            // waves per workgroup of the per-picture schedule
     std::vector<hipEvent_t> ev;    // 4 events per profiled run: inter | intra | deblock | end
     size_t ev_used = 0;            // events recorded since the last h264g_stage_times
@@ -418,7 +420,15 @@ int stage_picture(h264g_ctx *ctx, Stream &st, const h264g_pic_desc &d, const h26
     size_t o_mbs = 0, o_aux = al(o_mbs + n_mbs * sizeof(h264g_mb)), o_lv = al(o_aux + n_mbs * sizeof(DevAux));
     size_t o_wt = al(o_lv + n_levels * sizeof(int16_t) + 16), o_it = al(o_wt + wt.size() * sizeof(DevWeight));
     pic->n_rows = (uint32_t)H;
-    size_t n_items = pic->n_inter + pic->n_intra + n_mbs + (size_t)H;
+    // intra MBs per row: offsets, then columns (the per-picture intra schedule)
+    std::vector<uint32_t> irow(H + 1, 0), icol;
+    for (int y = 0; y < H; y++) {
+        irow[y] = (uint32_t)icol.size();
+        for (int xx = 0; xx < W; xx++)
+            if (aux[y * W + xx].kind & AUX_INTRA) icol.push_back((uint32_t)xx);
+    }
+    irow[H] = (uint32_t)icol.size();
+    size_t n_items = pic->n_inter + pic->n_intra + n_mbs + (size_t)H + irow.size() + icol.size();
     size_t o_bs = al(o_it + n_items * sizeof(uint32_t)), o_pic = al(o_bs + (size_t)n_mbs * 32), total = al(o_pic + sizeof(DevPic));
     void *dm = nullptr;
     HIPCHK(hipMalloc(&dm, total));
@@ -437,10 +447,14 @@ int stage_picture(h264g_ctx *ctx, Stream &st, const h264g_pic_desc &d, const h26
     items.insert(items.end(), pic->intra_t.begin(), pic->intra_t.end());
     items.insert(items.end(), pic->all_t.begin(), pic->all_t.end());
     for (int y = 0; y < H; y++) items.push_back((uint32_t)y);
+    const size_t irow_at = items.size();
+    items.insert(items.end(), irow.begin(), irow.end());
+    items.insert(items.end(), icol.begin(), icol.end());
     DevPic &P = pic->hpic;
     std::memset(&P, 0, sizeof(P));
     P.mbs = pic->d_mbs; P.aux = pic->d_aux; P.levels = pic->d_levels; P.wt = pic->d_wt;
     P.dpb = st.dpb; P.slot_bytes = st.slot_bytes; P.flags = st.flags; P.rowprog = st.rowprog; P.bs = d_bs; P.dst = d.dst_slot;
+    P.irow = pic->d_items + irow_at; P.n_intra = pic->n_intra;
     P.W = W; P.H = H; P.offA = d.filter_offset_a; P.offB = d.filter_offset_b;
     P.dbk_idc = d.disable_deblocking_filter_idc; P.cip = d.constrained_intra_pred_flag;
     HIPCHK(hipMemcpy(pic->d_mbs, mbs, n_mbs * sizeof(h264g_mb), hipMemcpyHostToDevice));
@@ -480,7 +494,10 @@ int launch_stages(h264g_ctx *ctx, const StageSet &q) {
     int e = h264g_launch_inter(q.pics, q.inter, q.n_inter, ctx->d_err, s);
     if (e) return h264g_set_error(H264G_E_DEVICE, "inter launch: %s", hipGetErrorString((hipError_t)e));
     if (ev) HIPCHK(hipEventRecord(ev[1], s));
-    e = h264g_launch_intra(q.pics, q.intra, q.n_intra, &ctx->d_ctl[0], &ctx->d_ctl[2].ticket, ctx->d_err, ctx->waves, s);
+    if (q.n_intra)
+        e = ctx->intra_schedule == H264G_INTRA_PICTURE_WG
+                ? h264g_launch_intra_pic(q.pics, q.n_pics, ctx->intra_waves, ctx->d_err, s)
+                : h264g_launch_intra(q.pics, q.intra, q.n_intra, &ctx->d_ctl[0], &ctx->d_ctl[2].ticket, ctx->d_err, ctx->waves, s);
     if (e) return h264g_set_error(H264G_E_DEVICE, "intra launch: %s", hipGetErrorString((hipError_t)e));
     if (ev) HIPCHK(hipEventRecord(ev[2], s));
     if (!q.dbk_off) {
@@ -537,6 +554,12 @@ int32_t h264g_create(const h264g_config *cfg, h264g_ctx **out) {
     c->no_graphs = cfg && cfg->no_graphs;
     c->dbk_schedule = cfg ? cfg->deblock_schedule : H264G_DEBLOCK_PICTURE_WG;
     c->dbk_waves = (cfg && cfg->deblock_waves > 0) ? cfg->deblock_waves : 16;
+    c->intra_schedule = cfg ? cfg->intra_schedule : H264G_INTRA_PICTURE_WG;
+    c->intra_waves = (cfg && cfg->intra_waves > 0) ? cfg->intra_waves : 8;
+    if (c->intra_schedule < 0 || c->intra_schedule > 1 || c->intra_waves > 8) {
+        h264g_destroy(c);
+        return h264g_set_error(H264G_E_ARG, "intra_schedule %d / intra_waves %d", cfg->intra_schedule, cfg->intra_waves);
+    }
     if (c->dbk_schedule < 0 || c->dbk_schedule > 2 || c->dbk_waves > 16) {
         h264g_destroy(c);
         return h264g_set_error(H264G_E_ARG, "deblock_schedule %d / deblock_waves %d", cfg->deblock_schedule, cfg->deblock_waves);

@@ -48,12 +48,14 @@ struct DevPic {
     uint64_t *rowprog;        // per-MB-row progress of row-pipelined deblocking:
                               // epoch << 32 | MBs of the row finished
     uint8_t  *bs;             // boundary strengths, 32 per MB: [dir][edge][4-line segment]
+    const uint32_t *irow;     // intra MBs per row: H + 1 offsets, then their columns
+    uint32_t n_intra;         // intra-predicted MBs of the picture
     int32_t  dst;             // destination slot
     int32_t  W, H;            // in macroblocks
     int32_t  offA, offB;      // FilterOffsetA / FilterOffsetB
     int32_t  dbk_idc;         // disable_deblocking_filter_idc
     int32_t  cip;             // constrained_intra_pred_flag
-    int32_t  pad[3];
+    int32_t  pad[4];
 };
 
 // Work item: picture index in the batch and macroblock address.
@@ -81,7 +83,8 @@ int h264g_launch_deblock(const DevPic *pics, const uint32_t *items, uint32_t n, 
                          uint32_t *err, int waves, void *stream);
 int h264g_launch_deblock_rows(const DevPic *pics, const uint32_t *items, uint32_t n, DevCtl *ctl, uint32_t *epoch,
                               uint32_t *err, int waves, void *stream);
-// One workgroup of `waves` waves per picture (pics[0..n_pics)); rows of up to 1024 MBs.
+// One workgroup of `waves` waves per picture (pics[0..n_pics)); pictures of up to 1024 MB rows.
+int h264g_launch_intra_pic(const DevPic *pics, uint32_t n_pics, int waves, uint32_t *err, void *stream);
 int h264g_launch_deblock_pic(const DevPic *pics, uint32_t n_pics, int waves, uint32_t *err, void *stream);
 int h264g_launch_bs(const DevPic *pics, const uint32_t *items, uint32_t n, void *stream);
 int h264g_launch_copy(const uint8_t *src, uint8_t *dst, uint64_t bytes, void *stream);

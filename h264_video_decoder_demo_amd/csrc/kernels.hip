@@ -1116,6 +1116,7 @@ __device__ void deblock_row(const DevPic &P, int mby, Sync &sync, WaveDbk *wd) {
 }
 
 // ------------------------------------------------------------ kernels
+constexpr int DBK_WG_MAX_ROWS = 1024;  // MB rows of a picture with LDS row progress
 // The LDS working sets are per wave and the per-MB code synchronises with wave barriers,
 // so a workgroup may hold one wave (most kernels) or several (k_deblock_pic).
 __global__ __launch_bounds__(WAVE) void k_inter(const DevPic *__restrict__ pics, const uint32_t *__restrict__ items, uint32_t n,
@@ -1129,6 +1130,24 @@ __global__ __launch_bounds__(WAVE) void k_inter(const DevPic *__restrict__ pics,
     (void)err;
 }
 
+// Neighbour availability of an intra MB for intra prediction (6.4.11, with
+// constrained_intra_pred_flag); deps[k] = address of neighbour k (A, B, C, D) when it is
+// itself intra-predicted in this picture (its samples come from the intra stage), else -1.
+__device__ __forceinline__ Avail intra_avail(const DevPic &P, int addr, int deps[4]) {
+    const int W = P.W, mbx = addr % W, mby = addr / W;
+    Avail av;
+    av.A = mbx > 0; av.B = mby > 0; av.C = mby > 0 && mbx < W - 1; av.D = mbx > 0 && mby > 0;
+    const int nb[4] = {av.A ? addr - 1 : -1, av.B ? addr - W : -1, av.C ? addr - W + 1 : -1, av.D ? addr - W - 1 : -1};
+    for (int k = 0; k < 4; k++) {
+        deps[k] = -1;
+        if (nb[k] < 0) continue;
+        const int c = P.mbs[nb[k]].mb_type;
+        if (P.cip && cls_inter(c)) { if (k == 0) av.A = 0; else if (k == 1) av.B = 0; else if (k == 2) av.C = 0; else av.D = 0; }
+        if (cls_intra(c)) deps[k] = nb[k];
+    }
+    return av;
+}
+
 __global__ __launch_bounds__(WAVE) void k_intra(const DevPic *__restrict__ pics, const uint32_t *__restrict__ items, uint32_t n,
                                                 DevCtl *ctl, uint32_t *epoch_p, uint32_t *err) {
     __shared__ WaveIntra lds;
@@ -1138,23 +1157,44 @@ __global__ __launch_bounds__(WAVE) void k_intra(const DevPic *__restrict__ pics,
         if (t < 0) break;
         const uint32_t it = items[t];
         const DevPic &P = pics[it >> WI_PIC_SHIFT];
-        const int addr = (int)(it & WI_MB_MASK), W = P.W, mbx = addr % W, mby = addr / W;
-        Avail av;
-        av.A = mbx > 0; av.B = mby > 0; av.C = mby > 0 && mbx < W - 1; av.D = mbx > 0 && mby > 0;
-        int nb[4] = {av.A ? addr - 1 : -1, av.B ? addr - W : -1, av.C ? addr - W + 1 : -1, av.D ? addr - W - 1 : -1};
+        const int addr = (int)(it & WI_MB_MASK);
         int deps[4];
-        for (int k = 0; k < 4; k++) {
-            deps[k] = -1;
-            if (nb[k] < 0) continue;
-            int c = P.mbs[nb[k]].mb_type;
-            if (P.cip && cls_inter(c)) { if (k == 0) av.A = 0; else if (k == 1) av.B = 0; else if (k == 2) av.C = 0; else av.D = 0; }
-            if (cls_intra(c)) deps[k] = nb[k];
-        }
+        const Avail av = intra_avail(P, addr, deps);
         wait_deps(P.flags, deps, 4, epoch, err);
         intra_mb(P, addr, &lds, av);
         publish(&P.flags[addr], epoch);
     }
     retire(ctl, gridDim.x, epoch_p, epoch);
+}
+
+// Intra stage with one workgroup per picture: its waves take MB rows in turn and walk
+// the row's intra MBs (P.irow: row offsets, then columns).  Row progress p in LDS means
+// "every intra MB of the row left of column p is done"; MB (x, y) needs the row above at
+// min(x + 2, W) (neighbours B, C, D).  The samples of the neighbours pass through device
+// memory inside the CU, ordered by workgroup-scope release / acquire.
+constexpr int INTRA_WG_MAX_WAVES = 8;
+__global__ __launch_bounds__(WAVE * INTRA_WG_MAX_WAVES) void k_intra_pic(const DevPic *__restrict__ pics, uint32_t *err) {
+    __shared__ WaveIntra tiles[INTRA_WG_MAX_WAVES];
+    __shared__ uint32_t prog[DBK_WG_MAX_ROWS];
+    const DevPic &P = pics[blockIdx.x];
+    if (P.n_intra == 0) return;
+    const int nw = blockDim.x / WAVE, w = threadIdx.x / WAVE, W = P.W, H = P.H;
+    for (int i = threadIdx.x; i < H; i += blockDim.x) prog[i] = 0;
+    __syncthreads();
+    RowSyncLds sync{prog, err};
+    const uint32_t *off = P.irow, *col = P.irow + H + 1;
+    for (int y = w; y < H; y += nw) {
+        const uint32_t b = off[y], e = off[y + 1];
+        sync.post(y, b < e ? (int)col[b] : W);
+        for (uint32_t i = b; i < e; i++) {
+            const int x = (int)col[i], addr = y * W + x;
+            if (y > 0) sync.wait(y - 1, min(x + 2, W));
+            int deps[4];
+            const Avail av = intra_avail(P, addr, deps);
+            intra_mb(P, addr, &tiles[w], av);
+            sync.post(y, i + 1 < e ? (int)col[i + 1] : W);
+        }
+    }
 }
 
 __global__ __launch_bounds__(WAVE) void k_deblock(const DevPic *__restrict__ pics, const uint32_t *__restrict__ items, uint32_t n,
@@ -1196,7 +1236,6 @@ __global__ __launch_bounds__(WAVE) void k_deblock_rows(const DevPic *__restrict_
 // One workgroup per picture: its waves take the picture's MB rows in turn (row y on
 // wave y mod waves) and pass row progress through LDS.
 constexpr int DBK_WG_MAX_WAVES = 16;
-constexpr int DBK_WG_MAX_ROWS = 1024;
 __global__ __launch_bounds__(WAVE * DBK_WG_MAX_WAVES) void k_deblock_pic(const DevPic *__restrict__ pics, uint32_t *err) {
     __shared__ WaveDbk tiles[DBK_WG_MAX_WAVES];
     __shared__ uint32_t prog[DBK_WG_MAX_ROWS];
@@ -1268,6 +1307,13 @@ int h264g_launch_deblock_rows(const DevPic *pics, const uint32_t *items, uint32_
     if (n == 0) return 0;
     uint32_t g = (uint32_t)waves < n ? (uint32_t)waves : n;
     hipLaunchKernelGGL(k_deblock_rows, dim3(g), dim3(WAVE), 0, (hipStream_t)stream, pics, items, n, ctl, epoch, err);
+    return (int)hipGetLastError();
+}
+
+int h264g_launch_intra_pic(const DevPic *pics, uint32_t n_pics, int waves, uint32_t *err, void *stream) {
+    if (n_pics == 0) return 0;
+    if (waves < 1 || waves > INTRA_WG_MAX_WAVES) return (int)hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_intra_pic, dim3(n_pics), dim3(WAVE * waves), 0, (hipStream_t)stream, pics, err);
     return (int)hipGetLastError();
 }
 

@@ -145,7 +145,16 @@ typedef struct h264g_config {
     int32_t  deblock_schedule;    /* H264G_DEBLOCK_*                                  */
     int32_t  deblock_waves;       /* waves per workgroup of H264G_DEBLOCK_PICTURE_WG,
                                      1..16 (0 = 16)                                  */
+    int32_t  intra_schedule;      /* H264G_INTRA_*                                    */
+    int32_t  intra_waves;         /* waves per workgroup of H264G_INTRA_PICTURE_WG,
+                                     1..8 (0 = 8)                                    */
 } h264g_config;
+
+/* Intra schedules: PICTURE_WG = one workgroup per picture, waves walk the intra MBs of
+ * MB rows in turn with LDS row progress (default); MB_WAVEFRONT = persistent waves over
+ * the intra MBs of all pictures ordered by t = x + 2y, progress words in device memory. */
+#define H264G_INTRA_PICTURE_WG   0
+#define H264G_INTRA_MB_WAVEFRONT 1
 
 /* Deblocking schedules (all give the raster-order result of 8.7):
  * PICTURE_WG   one workgroup per picture; its waves take MB rows in turn and pass row

@@ -41,15 +41,16 @@ def engines():
         e.close()
 
 
-@pytest.fixture(params=["picture_wg", "mb_wavefront", "mb_rows", "picture_wg_4waves"])
+@pytest.fixture(params=["picture_wg", "mb_wavefront", "mb_rows", "picture_wg_small"])
 def any_engine(request, engines):
-    """Every deblocking schedule (and a small per-picture workgroup)."""
+    """Every intra / deblocking schedule, and small per-picture workgroups."""
     from h264_video_decoder_demo_amd import Engine
     from h264_video_decoder_demo_amd import _lib
-    kw = {"picture_wg": dict(deblock_schedule=_lib.DEBLOCK_PICTURE_WG),
-          "mb_wavefront": dict(deblock_schedule=_lib.DEBLOCK_MB_WAVEFRONT),
-          "mb_rows": dict(deblock_schedule=_lib.DEBLOCK_MB_ROWS),
-          "picture_wg_4waves": dict(deblock_schedule=_lib.DEBLOCK_PICTURE_WG, deblock_waves=4)}[request.param]
+    kw = {"picture_wg": dict(deblock_schedule=_lib.DEBLOCK_PICTURE_WG, intra_schedule=_lib.INTRA_PICTURE_WG),
+          "mb_wavefront": dict(deblock_schedule=_lib.DEBLOCK_MB_WAVEFRONT, intra_schedule=_lib.INTRA_MB_WAVEFRONT),
+          "mb_rows": dict(deblock_schedule=_lib.DEBLOCK_MB_ROWS, intra_schedule=_lib.INTRA_MB_WAVEFRONT),
+          "picture_wg_small": dict(deblock_schedule=_lib.DEBLOCK_PICTURE_WG, deblock_waves=3,
+                                   intra_schedule=_lib.INTRA_PICTURE_WG, intra_waves=2)}[request.param]
     if request.param not in engines:
         engines[request.param] = Engine(device=0, **kw)
     return engines[request.param]
