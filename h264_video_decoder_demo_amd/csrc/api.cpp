@@ -560,7 +560,7 @@ int32_t h264g_create(const h264g_config *cfg, h264g_ctx **out) {
         h264g_destroy(c);
         return h264g_set_error(H264G_E_ARG, "intra_schedule %d / intra_waves %d", cfg->intra_schedule, cfg->intra_waves);
     }
-    if (c->dbk_schedule < 0 || c->dbk_schedule > 2 || c->dbk_waves > 16) {
+    if (c->dbk_schedule < 0 || c->dbk_schedule > 2 || c->dbk_waves < 2 || c->dbk_waves > 16) {
         h264g_destroy(c);
         return h264g_set_error(H264G_E_ARG, "deblock_schedule %d / deblock_waves %d", cfg->deblock_schedule, cfg->deblock_waves);
     }

@@ -884,7 +884,7 @@ __device__ __forceinline__ void filter_line(int p[4], int q[4], int bS, bool chr
 }
 
 // LDS working set of one deblocking wave: luma rows/cols [-4, 15], chroma [-4, 7].
-struct WaveDbk {
+struct alignas(16) WaveDbk {
     uint8_t L[20][20];
     uint8_t C[2][12][12];
 };
@@ -928,7 +928,7 @@ __device__ void dbk_load(const DbkPlanes &d, int mbx, int mby, int region, WaveD
 // Pv is the view of the MB on the p side (the left / top MB for e = 0, else the MB itself).
 template <int DIR>
 __device__ __forceinline__ void dbk_edge(const DevPic &P, int addr, int e, int ap, const MbView &Pv, const MbView &Qv,
-                                         WaveDbk *wd) {
+                                         const uint8_t *bsrec, WaveDbk *wd) {
     const int lane = wave_lane();
     const int plane = lane < 16 ? 0 : (lane < 24 ? 1 : (lane >= 32 && lane < 40 ? 2 : -1));
     bool active = plane >= 0;
@@ -942,7 +942,7 @@ __device__ __forceinline__ void dbk_edge(const DevPic &P, int addr, int e, int a
     // boundary strength of the luma segment this line maps to (chroma line k -> luma 2k,
     // chroma edge e -> luma edge 2e), precomputed by k_bs
     const int el = plane == 0 ? e : 2 * e, seg = plane == 0 ? (k >> 2) : (k >> 1);
-    const int bS = P.bs[(size_t)addr * 32 + DIR * 16 + (el >> 2) * 4 + seg];
+    const int bS = bsrec[DIR * 16 + (el >> 2) * 4 + seg];
     (void)ap; (void)xp; (void)yp; (void)xq; (void)yq; (void)sc;
     const int qavg = (view_qp(Pv, plane) + view_qp(Qv, plane) + 1) >> 1;
     uint8_t *base;
@@ -962,11 +962,11 @@ __device__ __forceinline__ void dbk_edge(const DevPic &P, int addr, int e, int a
 
 template <int DIR>
 __device__ __forceinline__ void dbk_pass(const DevPic &P, int addr, int ap_edge, const MbView &Ev, bool edge, const MbView &Qv,
-                                         WaveDbk *wd) {
-    if (edge) dbk_edge<DIR>(P, addr, 0, ap_edge, Ev, Qv, wd);
+                                         const uint8_t *bsrec, WaveDbk *wd) {
+    if (edge) dbk_edge<DIR>(P, addr, 0, ap_edge, Ev, Qv, bsrec, wd);
     wave_sync();
     for (int e = 4; e < 16; e += 4) {
-        dbk_edge<DIR>(P, addr, e, addr, Qv, Qv, wd);
+        dbk_edge<DIR>(P, addr, e, addr, Qv, Qv, bsrec, wd);
         wave_sync();
     }
 }
@@ -974,10 +974,11 @@ __device__ __forceinline__ void dbk_pass(const DevPic &P, int addr, int ap_edge,
 // The 8 edge steps of one MB on the tile, in the reference's order (8.7, Deblock.cpp:76-635):
 // vertical edges left to right, then horizontal edges top to bottom.  Lv / Tv are the
 // left / top MB views (read only when present).
+// bsrec: the MB's 32 boundary strengths (device memory or an LDS copy).
 __device__ void dbk_filter(const DevPic &P, int addr, const MbView &Qv, const MbView &Lv, const MbView &Tv, bool left,
-                           bool top, WaveDbk *wd) {
-    dbk_pass<0>(P, addr, addr - 1, Lv, left, Qv, wd);
-    dbk_pass<1>(P, addr, addr - P.W, Tv, top, Qv, wd);
+                           bool top, const uint8_t *bsrec, WaveDbk *wd) {
+    dbk_pass<0>(P, addr, addr - 1, Lv, left, Qv, bsrec, wd);
+    dbk_pass<1>(P, addr, addr - P.W, Tv, top, Qv, bsrec, wd);
 }
 
 // Writes back the MB and the strips its MB edges modified (3 columns / rows).
@@ -1008,7 +1009,7 @@ __device__ void deblock_mb(const DevPic &P, int addr, WaveDbk *wd) {
     if (left) load_view(P, addr - 1, Lv);
     if (top) load_view(P, addr - W, Tv);
     wave_sync();
-    dbk_filter(P, addr, Qv, Lv, Tv, left, top, wd);
+    dbk_filter(P, addr, Qv, Lv, Tv, left, top, P.bs + (size_t)addr * 32, wd);
     dbk_store(d, mbx, mby, left, top, wd);
 }
 
@@ -1108,7 +1109,7 @@ __device__ void deblock_row(const DevPic &P, int mby, Sync &sync, WaveDbk *wd) {
             dbk_load(d, mbx, mby, 2, wd);
         }
         wave_sync();
-        dbk_filter(P, addr, Qv, Lv, Tv, left, top, wd);
+        dbk_filter(P, addr, Qv, Lv, Tv, left, top, P.bs + (size_t)addr * 32, wd);
         dbk_store(d, mbx, mby, left, top, wd);
         sync.post(mby, mbx + 1);
         dbk_carry_left(wd);
@@ -1117,6 +1118,7 @@ __device__ void deblock_row(const DevPic &P, int mby, Sync &sync, WaveDbk *wd) {
 
 // ------------------------------------------------------------ kernels
 constexpr int DBK_WG_MAX_ROWS = 1024;  // MB rows of a picture with LDS row progress
+constexpr int DBK_WG_MAX_WAVES = 16;
 // The LDS working sets are per wave and the per-MB code synchronises with wave barriers,
 // so a workgroup may hold one wave (most kernels) or several (k_deblock_pic).
 __global__ __launch_bounds__(WAVE) void k_inter(const DevPic *__restrict__ pics, const uint32_t *__restrict__ items, uint32_t n,
@@ -1233,19 +1235,166 @@ __global__ __launch_bounds__(WAVE) void k_deblock_rows(const DevPic *__restrict_
     retire(ctl, gridDim.x, epoch_p, epoch);
 }
 
+// ---- per-picture workgroup deblocking with LDS strips
+// Row y's wave hands the bottom 4 luma / chroma rows of each MB (the p samples of the
+// next row's top edge) to row y + 1's wave through a ring of LDS slots, so no sample
+// travels through device memory between waves.  Each sample is written to the picture
+// by exactly one wave: row y stores rows 0..12 (chroma 0..4) of its MBs, row y + 1 the
+// rows 13..15 (chroma 5..7) after its top-edge filtering (the last row stores all).
+constexpr int STRIP_SLOTS = 8;
+struct alignas(16) StripSlot {
+    uint8_t y[4][16];         // luma rows 12..15
+    uint8_t c[2][4][8];       // Cb / Cr rows 4..7
+};
+struct DbkWgShared {
+    WaveDbk tiles[DBK_WG_MAX_WAVES];
+    StripSlot ring[DBK_WG_MAX_WAVES][STRIP_SLOTS];
+    alignas(16) uint8_t bsrec[DBK_WG_MAX_WAVES][32];
+    uint32_t prog[DBK_WG_MAX_ROWS];   // row y: strips of MBs 0 .. prog - 1 complete
+    uint32_t cons[DBK_WG_MAX_ROWS];   // row y: strips of MBs 0 .. cons - 1 read by row y + 1
+};
+
+__device__ __forceinline__ void lds_release() {
+    wave_sync();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+}
+__device__ void lds_wait(const uint32_t *w, uint32_t need, uint32_t *err) {
+    for (uint32_t it = 0;; ++it) {
+        if (uniform(__hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) >= need) break;
+        if (it >= LDS_SPIN_LIMIT) {
+            atomicOr(err, DEVERR_TIMEOUT);
+            break;
+        }
+        if ((it & 1023u) == 1023u && uniform(__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) != 0u) break;
+        __builtin_amdgcn_s_sleep(1);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+__device__ __forceinline__ void lds_post(uint32_t *w, uint32_t v) {
+    lds_release();
+    __hip_atomic_store(w, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// Body of MB (mbx, mby) and its boundary strengths, one dword per lane (prefetch registers).
+struct DbkPrefetch {
+    uint32_t luma, chroma, bs;
+};
+__device__ __forceinline__ DbkPrefetch dbk_fetch(const DevPic &P, const DbkPlanes &d, int mbx, int mby) {
+    const int lane = wave_lane();
+    DbkPrefetch f;
+    f.luma = *(const uint32_t *)&d.y[(uint64_t)(mby * 16 + (lane >> 2)) * d.wl + mbx * 16 + (lane & 3) * 4];
+    f.chroma = 0;
+    f.bs = 0;
+    if (lane < 32) {
+        const uint8_t *pl = (lane >> 4) ? d.cr : d.cb;
+        f.chroma = *(const uint32_t *)&pl[(uint64_t)(mby * 8 + ((lane >> 1) & 7)) * d.wc + mbx * 8 + (lane & 1) * 4];
+    }
+    if (lane < 8) f.bs = *(const uint32_t *)&P.bs[(size_t)(mby * P.W + mbx) * 32 + lane * 4];
+    return f;
+}
+__device__ __forceinline__ void dbk_place(const DbkPrefetch &f, WaveDbk *wd, uint8_t *bsrec) {
+    const int lane = wave_lane();
+    *(uint32_t *)&wd->L[(lane >> 2) + 4][4 + (lane & 3) * 4] = f.luma;
+    if (lane < 32) *(uint32_t *)&wd->C[lane >> 4][((lane >> 1) & 7) + 4][4 + (lane & 1) * 4] = f.chroma;
+    if (lane < 8) *(uint32_t *)&bsrec[lane * 4] = f.bs;
+}
+
+// Stores of step x of a row (see above): rows 0..12 (last row 0..15) of MB x cols 0..12
+// and of MB x - 1 cols 13..15, the top neighbour's rows 13..15, and at the row end the
+// last MB's cols 13..15; chroma alike with rows / cols 5..7.
+__device__ void dbk_store_wg(const DbkPlanes &d, int mbx, int mby, bool left, bool top, bool last, bool rowend,
+                             const WaveDbk *wd) {
+    const int lane = wave_lane();
+    const int ly = last ? 15 : 12, cy = last ? 7 : 4;
+    for (int i = lane; i < 400; i += WAVE) {
+        const int y = i / 20 - 4, x = i % 20 - 4;
+        const bool body = y >= 0 && y <= ly && (x <= 12 ? x >= (left ? -3 : 0) : rowend);
+        const bool up = top && y >= -3 && y < 0 && x >= 0;
+        if (body || up) d.y[(uint64_t)(mby * 16 + y) * d.wl + mbx * 16 + x] = wd->L[y + 4][x + 4];
+    }
+    for (int i = lane; i < 288; i += WAVE) {
+        const int c = i / 144, k = i % 144, y = k / 12 - 4, x = k % 12 - 4;
+        const bool body = y >= 0 && y <= cy && (x <= 4 ? x >= (left ? -3 : 0) : rowend);
+        const bool up = top && y >= -3 && y < 0 && x >= 0;
+        if (body || up) (c ? d.cr : d.cb)[(uint64_t)(mby * 8 + y) * d.wc + mbx * 8 + x] = wd->C[c][y + 4][x + 4];
+    }
+}
+
+// Strip parts: A = MB x's columns 0..12 (chroma 0..4) from the tile's current MB;
+// B = columns 13..15 (chroma 5..7) of the MB whose right columns sit at tile column tc
+// (1 for MB x - 1 after its right neighbour's left edge, 17 for the last MB of the row).
+__device__ void strip_put(StripSlot *sl, const WaveDbk *wd, bool part_a, int tc) {
+    const int lane = wave_lane();
+    for (int i = lane; i < 128; i += WAVE) {
+        if (i < 64) {
+            const int r = i >> 4, x = i & 15;
+            if (part_a ? x <= 12 : x >= 13) sl->y[r][x] = wd->L[16 + r][part_a ? 4 + x : tc + x - 13];
+        } else {
+            const int k = i - 64, c = k >> 5, r = (k >> 3) & 3, x = k & 7;
+            if (part_a ? x <= 4 : x >= 5) sl->c[c][r][x] = wd->C[c][8 + r][part_a ? 4 + x : (tc == 17 ? 9 : tc) + x - 5];
+        }
+    }
+}
+// The top neighbour's bottom rows into the tile (rows -4..-1 above the MB).
+__device__ void strip_get(const StripSlot *sl, WaveDbk *wd) {
+    const int lane = wave_lane();
+    if (lane < 16) *(uint32_t *)&wd->L[lane >> 2][4 + (lane & 3) * 4] = *(const uint32_t *)&sl->y[lane >> 2][(lane & 3) * 4];
+    else if (lane < 32) {
+        const int k = lane - 16, c = k >> 3, r = (k >> 1) & 3, h = k & 1;
+        *(uint32_t *)&wd->C[c][r][4 + h * 4] = *(const uint32_t *)&sl->c[c][r][h * 4];
+    }
+}
+
+// One MB row of a picture (wave w of nw, row y); see the strip scheme above.
+__device__ void deblock_row_wg(const DevPic &P, int y, int nw, int w, DbkWgShared &sh, uint32_t *err) {
+    const int W = P.W, H = P.H;
+    const DbkPlanes d = dbk_planes(P);
+    const bool top = y > 0, last = y == H - 1;
+    WaveDbk *wd = &sh.tiles[w];
+    uint8_t *bsrec = sh.bsrec[w];
+    StripSlot *ring = sh.ring[w];
+    const StripSlot *above = top ? sh.ring[(y - 1) % nw] : nullptr;
+    // this wave's ring held the strips of row y - nw: row y - nw + 1 must have read them all
+    if (!last && y >= nw) lds_wait(&sh.cons[y - nw], (uint32_t)W, err);
+    MbView Lv, Qv, Tv;
+    DbkPrefetch f = dbk_fetch(P, d, 0, y);
+    for (int x = 0; x < W; x++) {
+        const int addr = y * W + x;
+        const bool left = x > 0, rowend = x == W - 1;
+        dbk_place(f, wd, bsrec);
+        if (!rowend) f = dbk_fetch(P, d, x + 1, y);  // next MB, in flight during this one
+        if (left) Lv = Qv;
+        load_view(P, addr, Qv);
+        if (top) {
+            load_view(P, addr - W, Tv);
+            lds_wait(&sh.prog[y - 1], (uint32_t)x + 1, err);
+            strip_get(&above[x % STRIP_SLOTS], wd);
+            lds_post(&sh.cons[y - 1], (uint32_t)x + 1);
+        }
+        wave_sync();
+        dbk_filter(P, addr, Qv, Lv, Tv, left, top, bsrec, wd);
+        dbk_store_wg(d, x, y, left, top, last, rowend, wd);
+        if (!last) {
+            if (x >= STRIP_SLOTS) lds_wait(&sh.cons[y], (uint32_t)(x - STRIP_SLOTS + 1), err);
+            strip_put(&ring[x % STRIP_SLOTS], wd, true, 0);
+            if (left) strip_put(&ring[(x - 1) % STRIP_SLOTS], wd, false, 1);
+            if (rowend) strip_put(&ring[x % STRIP_SLOTS], wd, false, 17);
+            lds_post(&sh.prog[y], rowend ? (uint32_t)W : (uint32_t)x);
+        }
+        dbk_carry_left(wd);
+    }
+}
+
 // One workgroup per picture: its waves take the picture's MB rows in turn (row y on
-// wave y mod waves) and pass row progress through LDS.
-constexpr int DBK_WG_MAX_WAVES = 16;
+// wave y mod waves, at least 2 waves).
 __global__ __launch_bounds__(WAVE * DBK_WG_MAX_WAVES) void k_deblock_pic(const DevPic *__restrict__ pics, uint32_t *err) {
-    __shared__ WaveDbk tiles[DBK_WG_MAX_WAVES];
-    __shared__ uint32_t prog[DBK_WG_MAX_ROWS];
+    __shared__ DbkWgShared sh;
     const DevPic &P = pics[blockIdx.x];
     if (P.dbk_idc == 1) return;
     const int nw = blockDim.x / WAVE, w = threadIdx.x / WAVE;
-    for (int i = threadIdx.x; i < P.H; i += blockDim.x) prog[i] = 0;
+    for (int i = threadIdx.x; i < P.H; i += blockDim.x) sh.prog[i] = sh.cons[i] = 0;
     __syncthreads();
-    RowSyncLds sync{prog, err};
-    for (int y = w; y < P.H; y += nw) deblock_row(P, y, sync, &tiles[w]);
+    for (int y = w; y < P.H; y += nw) deblock_row_wg(P, y, nw, w, sh, err);
 }
 
 // Boundary strengths of every edge segment of every MB (8.7.2.1, Deblock.cpp:994-1310),
@@ -1319,7 +1468,7 @@ int h264g_launch_intra_pic(const DevPic *pics, uint32_t n_pics, int waves, uint3
 
 int h264g_launch_deblock_pic(const DevPic *pics, uint32_t n_pics, int waves, uint32_t *err, void *stream) {
     if (n_pics == 0) return 0;
-    if (waves < 1 || waves > DBK_WG_MAX_WAVES) return (int)hipErrorInvalidValue;
+    if (waves < 2 || waves > DBK_WG_MAX_WAVES) return (int)hipErrorInvalidValue;
     hipLaunchKernelGGL(k_deblock_pic, dim3(n_pics), dim3(WAVE * waves), 0, (hipStream_t)stream, pics, err);
     return (int)hipGetLastError();
 }

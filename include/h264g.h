@@ -144,7 +144,7 @@ typedef struct h264g_config {
                                      (profiled contexts always launch directly)       */
     int32_t  deblock_schedule;    /* H264G_DEBLOCK_*                                  */
     int32_t  deblock_waves;       /* waves per workgroup of H264G_DEBLOCK_PICTURE_WG,
-                                     1..16 (0 = 16)                                  */
+                                     2..16 (0 = 16)                                  */
     int32_t  intra_schedule;      /* H264G_INTRA_*                                    */
     int32_t  intra_waves;         /* waves per workgroup of H264G_INTRA_PICTURE_WG,
                                      1..8 (0 = 8)                                    */
@@ -157,8 +157,8 @@ typedef struct h264g_config {
 #define H264G_INTRA_MB_WAVEFRONT 1
 
 /* Deblocking schedules (all give the raster-order result of 8.7):
- * PICTURE_WG   one workgroup per picture; its waves take MB rows in turn and pass row
- *              progress through LDS (default)
+ * PICTURE_WG   one workgroup per picture; its waves take MB rows in turn and hand each
+ *              MB's bottom rows to the next row's wave through LDS (default)
  * MB_WAVEFRONT persistent waves over single MBs ordered by t = x + 2y, progress words
  *              in device memory
  * MB_ROWS      persistent waves over MB rows of all pictures, progress in device memory */
