@@ -88,6 +88,7 @@ struct WaveRes {
     int32_t t8[4][64];    // 8x8 transform: rows after the horizontal pass
     int16_t rl[256];      // luma residual, raster 16x16
     int16_t rc[2][64];    // chroma residual, raster 8x8
+    uint8_t win[16][9][12];  // inter: 9x9 reference window of each 4x4 luma block (one list)
 };
 
 __device__ int level_slot(int i, int i16, int cbp, int t8) {
@@ -264,19 +265,17 @@ __device__ __forceinline__ int ldc(const uint8_t *p, int w, int h, int x, int y)
     return p[clip3(0, h - 1, y) * w + clip3(0, w - 1, x)];
 }
 
+__device__ __forceinline__ uint8_t *slot_base(const DevPic &P, int s) { return P.dpb + (uint64_t)s * P.slot_bytes; }
+
 // Four horizontally adjacent luma prediction samples at integer position (x0, y0) with
-// fractional offset (xF, yF), 8.4.2.2.1 (H264InterPrediction.cpp:2344-2480).
-__device__ void luma4(const uint8_t *ref, int w, int h, int x0, int y0, int xF, int yF, int out[4]) {
+// fractional offset (xF, yF), 8.4.2.2.1 (H264InterPrediction.cpp:2344-2480), from the 6x9
+// window w[r][c] = sample (x0 - 2 + c, y0 - 2 + r) (coordinates clamped to the picture).
+__device__ __forceinline__ void luma4_win(const int win[6][9], int xF, int yF, int out[4]) {
     if (xF == 0 && yF == 0) {
 #pragma unroll
-        for (int i = 0; i < 4; i++) out[i] = ldc(ref, w, h, x0 + i, y0);
+        for (int i = 0; i < 4; i++) out[i] = win[2][i + 2];
         return;
     }
-    int win[6][9];
-#pragma unroll
-    for (int r = 0; r < 6; r++)
-#pragma unroll
-        for (int c = 0; c < 9; c++) win[r][c] = ldc(ref, w, h, x0 - 2 + c, y0 - 2 + r);
     if (yF == 0) {  // a, b, c
 #pragma unroll
         for (int i = 0; i < 4; i++) {
@@ -293,7 +292,6 @@ __device__ void luma4(const uint8_t *ref, int w, int h, int x0, int y0, int xF, 
         }
         return;
     }
-    // both fractional: b (row 0), s (row 1), h (col 0), m (col 1), j
     int v[5];  // clipped vertical half samples h at columns i = 0..4
 #pragma unroll
     for (int i = 0; i < 5; i++)
@@ -305,24 +303,69 @@ __device__ void luma4(const uint8_t *ref, int w, int h, int x0, int y0, int xF, 
         int hh = v[i], m = v[i + 1];
         int o;
         if ((xF & 1) && (yF & 1)) {  // e, g, p, r
-            int hv = xF == 1 ? hh : m;
-            int bs = yF == 1 ? b : s;
-            o = (bs + hv + 1) >> 1;
+            o = ((yF == 1 ? b : s) + (xF == 1 ? hh : m) + 1) >> 1;
         } else {
             int j1 = 0;
 #pragma unroll
             for (int r = 0; r < 6; r++) {
-                int b1 = tap6(win[r][i], win[r][i + 1], win[r][i + 2], win[r][i + 3], win[r][i + 4], win[r][i + 5]);
-                const int wgt = r == 0 || r == 5 ? 1 : (r == 1 || r == 4 ? -5 : 20);
-                j1 += wgt * b1;
+                const int b1 = tap6(win[r][i], win[r][i + 1], win[r][i + 2], win[r][i + 3], win[r][i + 4], win[r][i + 5]);
+                j1 += (r == 0 || r == 5 ? 1 : (r == 1 || r == 4 ? -5 : 20)) * b1;
             }
-            int j = clip1((j1 + 512) >> 10);
+            const int j = clip1((j1 + 512) >> 10);
             if (xF == 2 && yF == 2) o = j;
             else if (xF == 2) o = (j + (yF == 1 ? b : s) + 1) >> 1;  // f, q
             else o = (j + (xF == 1 ? hh : m) + 1) >> 1;                // i, k
         }
         out[i] = o;
     }
+}
+
+// Luma prediction samples of list l for the whole MB (lane = 4x4 block b, row r; four
+// samples each).  The 9x9 windows of all 16 blocks are staged into LDS with dword loads
+// of whole window rows (clamped byte loads only where a window leaves the picture), then
+// every lane reads its 6x9 window from LDS.  Whole wave; out is left 0 where the lane's
+// quadrant does not use list l.
+__device__ void luma_list(const DevPic &P, const h264g_mb *mb, const DevAux &ax, int l, WaveRes *ws, int out[4]) {
+    const int lane = wave_lane();
+    const int W = P.W, wl = W * 16, hl = P.H * 16;
+    const int addr = (int)(mb - P.mbs), bx0 = (addr % W) * 16, by0 = (addr / W) * 16;
+    // phase 1: 16 blocks x 9 rows x 3 dwords
+    for (int i = lane; i < 16 * 27; i += WAVE) {
+        const int b = i / 27, k = i % 27, rr = k / 3, dw = k % 3;
+        const int q = ((b >> 3) << 1) | ((b & 3) >> 1);
+        if (!((ax.pf >> (4 * l + q)) & 1)) continue;
+        const uint8_t *ref = slot_base(P, ax.refid[l][q]);
+        const int x0 = bx0 + (b & 3) * 4 + (mb->mv[l][b][0] >> 2), y0 = by0 + (b >> 2) * 4 + (mb->mv[l][b][1] >> 2);
+        const int xs = x0 - 2, a = xs & ~3, off = xs - a, y = y0 - 2 + rr;
+        uint32_t v;
+        if (a >= 0 && a + 12 <= wl && y >= 0 && y < hl) {
+            v = *(const uint32_t *)&ref[(uint64_t)y * wl + a + 4 * dw];
+        } else {
+            v = 0;
+#pragma unroll
+            for (int t = 0; t < 4; t++) v |= (uint32_t)ldc(ref, wl, hl, a + 4 * dw + t, y) << (8 * t);
+        }
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            const int p = 4 * dw + t - off;  // position in the window row
+            if (p >= 0 && p < 12) ws->win[b][rr][p] = (uint8_t)(v >> (8 * t));
+        }
+    }
+    wave_sync();
+    // phase 2
+    const int b = lane >> 2, r = lane & 3, q = ((b >> 3) << 1) | ((b & 3) >> 1);
+    if ((ax.pf >> (4 * l + q)) & 1) {
+        int win[6][9];
+#pragma unroll
+        for (int rr = 0; rr < 6; rr++) {
+            const uint32_t *row = (const uint32_t *)&ws->win[b][r + rr][0];
+            const uint32_t d0 = row[0], d1 = row[1], d2 = row[2];
+#pragma unroll
+            for (int c = 0; c < 9; c++) win[rr][c] = ((c < 4 ? d0 : (c < 8 ? d1 : d2)) >> (8 * (c & 3))) & 255;
+        }
+        luma4_win(win, mb->mv[l][b][0] & 3, mb->mv[l][b][1] & 3, out);
+    }
+    wave_sync();  // the window buffer is reused by the next list
 }
 
 // Two horizontally adjacent chroma prediction samples at (xC, yC) with a 1/8-sample
@@ -351,7 +394,6 @@ __device__ __forceinline__ int weigh(const DevWeight &w, int c, int pf0, int pf1
     return clip1(p * ww + oo);
 }
 
-__device__ __forceinline__ uint8_t *slot_base(const DevPic &P, int s) { return P.dpb + (uint64_t)s * P.slot_bytes; }
 
 // Inter MB (or I_PCM copy): prediction + residual, written to the destination slot.
 __device__ void inter_mb(const DevPic &P, int addr, WaveRes *ws) {
@@ -376,14 +418,8 @@ __device__ void inter_mb(const DevPic &P, int addr, WaveRes *ws) {
         int pf0 = (ax.pf >> q) & 1, pf1 = (ax.pf >> (4 + q)) & 1;
         int p0[4] = {0, 0, 0, 0}, p1[4] = {0, 0, 0, 0};
         int xAL = mbx * 16 + bx * 4, yAL = mby * 16 + by * 4 + r;
-        if (pf0) {
-            const int mvx = mb->mv[0][b][0], mvy = mb->mv[0][b][1];
-            luma4(slot_base(P, ax.refid[0][q]), wl, hl, xAL + (mvx >> 2), yAL + (mvy >> 2), mvx & 3, mvy & 3, p0);
-        }
-        if (pf1) {
-            const int mvx = mb->mv[1][b][0], mvy = mb->mv[1][b][1];
-            luma4(slot_base(P, ax.refid[1][q]), wl, hl, xAL + (mvx >> 2), yAL + (mvy >> 2), mvx & 3, mvy & 3, p1);
-        }
+        if (ax.pf & 0x0F) luma_list(P, mb, ax, 0, ws, p0);  // MB-uniform conditions
+        if (ax.pf & 0xF0) luma_list(P, mb, ax, 1, ws, p1);
         const DevWeight &w = P.wt[ax.wp[q]];
         uint32_t packed = 0;
 #pragma unroll
