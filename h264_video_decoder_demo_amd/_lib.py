@@ -117,6 +117,10 @@ _SIGNATURES = {
     "h264g_batch_destroy": (c_int32, [c_void_p, c_void_p]),
     "h264g_stage_times": (c_int32, [c_void_p, POINTER(c_float)]),
     "h264g_set_profile": (c_int32, [c_void_p, c_int32]),
+    "h264g_fence_record": (c_int32, [c_void_p, POINTER(c_void_p)]),
+    "h264g_fence_query": (c_int32, [c_void_p, c_void_p]),
+    "h264g_fence_wait": (c_int32, [c_void_p, c_void_p]),
+    "h264g_fence_destroy": (c_int32, [c_void_p, c_void_p]),
     "h264g_sync": (c_int32, [c_void_p]),
     "h264g_slot_readback": (c_int32, [c_void_p, c_uint32, c_int32, c_void_p, c_void_p, c_void_p]),
     "h264g_slot_upload": (c_int32, [c_void_p, c_uint32, c_int32, c_void_p, c_void_p, c_void_p]),

@@ -60,7 +60,8 @@ struct h264g_pic {
     uint32_t stream = 0;
     h264g_pic_desc desc;
     int W = 0, H = 0;
-    void *dmem = nullptr;          // one device block for everything below
+    void *dmem = nullptr;          // one device block for everything below (pooled)
+    size_t dbytes = 0;
     h264g_mb *d_mbs = nullptr;
     DevAux *d_aux = nullptr;
     int16_t *d_levels = nullptr;
@@ -76,7 +77,8 @@ struct h264g_pic {
 
 namespace {
 struct Batch {
-    void *dmem = nullptr;
+    void *dmem = nullptr;          // pooled device block
+    size_t dbytes = 0;
     DevPic *d_pics = nullptr;
     uint32_t n_pics = 0;
     uint32_t *d_inter = nullptr, *d_intra = nullptr, *d_all = nullptr, *d_rows = nullptr;
@@ -96,7 +98,11 @@ struct h264g_ctx {
     int waves = 2048;
     uint64_t budget = 0, used = 0;
     std::vector<Stream> streams;
-    std::vector<h264g_batch *> deferred;  // batches freed at the next sync
+    // Device blocks of released pictures / batches, reusable once `ready` has completed on
+    // the stream (so releasing never waits for the GPU and staging rarely allocates).
+    struct PoolBlock { void *ptr; size_t size; hipEvent_t ready; };
+    std::vector<PoolBlock> pool;
+    size_t pool_bytes = 0;
     bool profile = false;
     bool no_graphs = false;
     int dbk_schedule = H264G_DEBLOCK_PICTURE_WG;
@@ -117,6 +123,55 @@ extern "C" int32_t h264g_set_error(int32_t code, const char *fmt, ...) {
 }
 
 namespace {
+
+// ------------------------------------------------------------ block pool
+constexpr size_t kPoolMaxBytes = 8ull << 30;
+
+size_t pool_class(size_t n) {  // sizes are rounded up so that blocks can be reused
+    size_t c = 64 << 10;
+    while (c < n && c < (64ull << 20)) c <<= 1;
+    return c >= n ? c : (n + (16ull << 20) - 1) & ~((16ull << 20) - 1);
+}
+
+// A device block of at least n bytes (caller holds ctx->mu).
+void *pool_get(h264g_ctx *c, size_t n) {
+    const size_t want = pool_class(n);
+    for (size_t i = 0; i < c->pool.size(); i++) {
+        auto &b = c->pool[i];
+        if (b.size == want && hipEventQuery(b.ready) == hipSuccess) {
+            void *p = b.ptr;
+            (void)hipEventDestroy(b.ready);
+            c->pool_bytes -= b.size;
+            c->pool.erase(c->pool.begin() + (long)i);
+            return p;
+        }
+    }
+    void *p = nullptr;
+    if (hipMalloc(&p, want) != hipSuccess) return nullptr;
+    return p;
+}
+
+// Returns a block; it becomes reusable when the work enqueued so far has completed.
+void pool_put(h264g_ctx *c, void *p, size_t n) {
+    if (!p) return;
+    hipEvent_t e = nullptr;
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess || hipEventRecord(e, c->stream) != hipSuccess) {
+        (void)hipStreamSynchronize(c->stream);  // cannot track it: free it once idle
+        if (e) (void)hipEventDestroy(e);
+        (void)hipFree(p);
+        return;
+    }
+    c->pool.push_back({p, pool_class(n), e});
+    c->pool_bytes += pool_class(n);
+    while (c->pool_bytes > kPoolMaxBytes && !c->pool.empty()) {  // trim the oldest
+        auto &b = c->pool.front();
+        (void)hipEventSynchronize(b.ready);
+        (void)hipEventDestroy(b.ready);
+        (void)hipFree(b.ptr);
+        c->pool_bytes -= b.size;
+        c->pool.erase(c->pool.begin());
+    }
+}
 
 // --------------------------------------------------------------- weights
 // 8.4.2.3 mode selection and 8.4.3 values (H264InterPrediction.cpp:2526-2613, 2833-3050).
@@ -430,9 +485,10 @@ int stage_picture(h264g_ctx *ctx, Stream &st, const h264g_pic_desc &d, const h26
     irow[H] = (uint32_t)icol.size();
     size_t n_items = pic->n_inter + pic->n_intra + n_mbs + (size_t)H + irow.size() + icol.size();
     size_t o_bs = al(o_it + n_items * sizeof(uint32_t)), o_pic = al(o_bs + (size_t)n_mbs * 32), total = al(o_pic + sizeof(DevPic));
-    void *dm = nullptr;
-    HIPCHK(hipMalloc(&dm, total));
+    void *dm = pool_get(ctx, total);
+    if (!dm) return h264g_set_error(H264G_E_NOMEM, "picture device memory (%zu bytes)", total);
     pic->dmem = dm;
+    pic->dbytes = total;
     char *base = (char *)dm;
     pic->d_mbs = (h264g_mb *)(base + o_mbs);
     pic->d_aux = (DevAux *)(base + o_aux);
@@ -463,7 +519,6 @@ int stage_picture(h264g_ctx *ctx, Stream &st, const h264g_pic_desc &d, const h26
     HIPCHK(hipMemcpy(pic->d_wt, wt.data(), wt.size() * sizeof(DevWeight), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(pic->d_items, items.data(), items.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(pic->d_pic, &P, sizeof(DevPic), hipMemcpyHostToDevice));
-    (void)ctx;
     return 0;
 }
 
@@ -517,10 +572,14 @@ int launch_stages(h264g_ctx *ctx, const StageSet &q) {
     return 0;
 }
 
-int free_batch(h264g_batch *b) {
+// Caller holds ctx->mu.  A batch with a graph is destroyed only when idle.
+int free_batch(h264g_ctx *c, h264g_batch *b) {
     if (!b) return 0;
-    if (b->graph) (void)hipGraphExecDestroy(b->graph);
-    if (b->dmem) (void)hipFree(b->dmem);
+    if (b->graph) {
+        (void)hipStreamSynchronize(c->stream);
+        (void)hipGraphExecDestroy(b->graph);
+    }
+    pool_put(c, b->dmem, b->dbytes);
     delete b;
     return 0;
 }
@@ -580,7 +639,11 @@ int32_t h264g_destroy(h264g_ctx *c) {
     if (!c) return H264G_OK;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    for (auto *b : c->deferred) free_batch(b);
+    for (auto &b : c->pool) {
+        (void)hipEventDestroy(b.ready);
+        (void)hipFree(b.ptr);
+    }
+    c->pool.clear();
     for (auto &s : c->streams) {
         if (s.dpb) (void)hipFree(s.dpb);
         if (s.flags) (void)hipFree(s.flags);
@@ -668,7 +731,7 @@ int32_t h264g_picture_begin(h264g_ctx *c, uint32_t sid, const h264g_pic_desc *d,
     p->H = c->streams[sid].H;
     int rc = stage_picture(c, c->streams[sid], *d, mbs, n_mbs, levels, n_levels, p);
     if (rc) {
-        if (p->dmem) (void)hipFree(p->dmem);
+        pool_put(c, p->dmem, p->dbytes);
         delete p;
         return rc;
     }
@@ -698,11 +761,10 @@ int32_t h264g_picture_submit(h264g_ctx *c, h264g_pic *p) {
 
 int32_t h264g_picture_release(h264g_ctx *c, h264g_pic *p) {
     if (!p) return H264G_OK;
-    if (c) {
-        (void)hipSetDevice(c->device);
-        (void)hipStreamSynchronize(c->stream);
-    }
-    if (p->dmem) (void)hipFree(p->dmem);
+    if (!c || p->ctx != c) return h264g_set_error(H264G_E_ARG, "bad picture");
+    std::lock_guard<std::mutex> lk(c->mu);
+    (void)hipSetDevice(c->device);
+    pool_put(c, p->dmem, p->dbytes);  // reused only after the launches enqueued so far
     delete p;
     return H264G_OK;
 }
@@ -765,7 +827,9 @@ int32_t h264g_batch_create(h264g_ctx *c, h264g_pic *const *pics, uint32_t n, h26
     size_t o_r = al(o_l + all.size() * 4), total = al(o_r + rows.size() * 4 + 4);
     h264g_batch *b = new (std::nothrow) h264g_batch();
     if (!b) return h264g_set_error(H264G_E_NOMEM, "batch");
-    if (hipMalloc(&b->dmem, total) != hipSuccess) { delete b; return h264g_set_error(H264G_E_NOMEM, "batch device memory"); }
+    b->dmem = pool_get(c, total);
+    b->dbytes = total;
+    if (!b->dmem) { delete b; return h264g_set_error(H264G_E_NOMEM, "batch device memory"); }
     char *base = (char *)b->dmem;
     b->d_pics = (DevPic *)(base + o_p);
     b->n_pics = n;
@@ -783,7 +847,7 @@ int32_t h264g_batch_create(h264g_ctx *c, h264g_pic *const *pics, uint32_t n, h26
         (!intra.empty() && hipMemcpy(b->d_intra, intra.data(), intra.size() * 4, hipMemcpyHostToDevice) != hipSuccess) ||
         (!all.empty() && hipMemcpy(b->d_all, all.data(), all.size() * 4, hipMemcpyHostToDevice) != hipSuccess) ||
         (!rows.empty() && hipMemcpy(b->d_rows, rows.data(), rows.size() * 4, hipMemcpyHostToDevice) != hipSuccess)) {
-        free_batch(b);
+        free_batch(c, b);
         return h264g_set_error(H264G_E_DEVICE, "batch upload");
     }
     *out = b;
@@ -832,20 +896,21 @@ int32_t h264g_batch_run(h264g_ctx *c, h264g_batch *b) {
 }
 
 int32_t h264g_batch_destroy(h264g_ctx *c, h264g_batch *b) {
-    if (c) {
-        (void)hipSetDevice(c->device);
-        (void)hipStreamSynchronize(c->stream);
-    }
-    return free_batch(b);
+    if (!c) return h264g_set_error(H264G_E_ARG, "ctx");
+    if (!b) return H264G_OK;
+    std::lock_guard<std::mutex> lk(c->mu);
+    (void)hipSetDevice(c->device);
+    return free_batch(c, b);
 }
 
 int32_t h264g_batch_submit(h264g_ctx *c, h264g_pic *const *pics, uint32_t n) {
     h264g_batch *b = nullptr;
     int rc = h264g_batch_create(c, pics, n, &b);
     if (rc) return rc;
-    rc = h264g_batch_run(c, b);
     std::lock_guard<std::mutex> lk(c->mu);
-    c->deferred.push_back(b);
+    HIPCHK(hipSetDevice(c->device));
+    rc = launch_stages(c, batch_stages(b));  // used once: launched directly, not captured
+    free_batch(c, b);                        // its block is reused after these launches
     return rc;
 }
 
@@ -854,14 +919,47 @@ int32_t h264g_sync(h264g_ctx *c) {
     std::lock_guard<std::mutex> lk(c->mu);
     HIPCHK(hipSetDevice(c->device));
     HIPCHK(hipStreamSynchronize(c->stream));
-    for (auto *b : c->deferred) free_batch(b);
-    c->deferred.clear();
     uint32_t err = 0;
     HIPCHK(hipMemcpy(&err, c->d_err, 4, hipMemcpyDeviceToHost));
     if (err) {
         HIPCHK(hipMemset(c->d_err, 0, 4));
         return h264g_set_error(H264G_E_TIMEOUT, "device error word 0x%x (wavefront wait exceeded its spin limit)", err);
     }
+    return H264G_OK;
+}
+
+int32_t h264g_fence_record(h264g_ctx *c, h264g_fence **out) {
+    if (!c || !out) return h264g_set_error(H264G_E_ARG, "NULL argument");
+    *out = nullptr;
+    std::lock_guard<std::mutex> lk(c->mu);
+    HIPCHK(hipSetDevice(c->device));
+    hipEvent_t e = nullptr;
+    HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    if (hipEventRecord(e, c->stream) != hipSuccess) {
+        (void)hipEventDestroy(e);
+        return h264g_set_error(H264G_E_DEVICE, "fence record");
+    }
+    *out = reinterpret_cast<h264g_fence *>(e);
+    return H264G_OK;
+}
+
+int32_t h264g_fence_query(h264g_ctx *c, h264g_fence *f) {
+    if (!c || !f) return h264g_set_error(H264G_E_ARG, "NULL argument");
+    hipError_t e = hipEventQuery(reinterpret_cast<hipEvent_t>(f));
+    if (e == hipSuccess) return 1;
+    if (e == hipErrorNotReady) return 0;
+    return h264g_set_error(H264G_E_DEVICE, "fence query: %s", hipGetErrorString(e));
+}
+
+int32_t h264g_fence_wait(h264g_ctx *c, h264g_fence *f) {
+    if (!c || !f) return h264g_set_error(H264G_E_ARG, "NULL argument");
+    HIPCHK(hipEventSynchronize(reinterpret_cast<hipEvent_t>(f)));
+    return H264G_OK;
+}
+
+int32_t h264g_fence_destroy(h264g_ctx *c, h264g_fence *f) {
+    (void)c;
+    if (f) (void)hipEventDestroy(reinterpret_cast<hipEvent_t>(f));
     return H264G_OK;
 }
 

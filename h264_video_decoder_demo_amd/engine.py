@@ -217,6 +217,21 @@ class Engine:
     def sync(self):
         check(lib().h264g_sync(self._h))
 
+    def fence(self) -> c_void_p:
+        """A fence after everything enqueued so far (poll with done(), free with drop_fence())."""
+        f = c_void_p()
+        check(lib().h264g_fence_record(self._h, byref(f)))
+        return f
+
+    def done(self, f: c_void_p) -> bool:
+        return check(lib().h264g_fence_query(self._h, f)) == 1
+
+    def wait(self, f: c_void_p):
+        check(lib().h264g_fence_wait(self._h, f))
+
+    def drop_fence(self, f: c_void_p):
+        lib().h264g_fence_destroy(self._h, f)
+
     def set_profile(self, on: bool):
         check(lib().h264g_set_profile(self._h, 1 if on else 0))
 

@@ -259,6 +259,16 @@ int32_t h264g_batch_destroy(h264g_ctx *ctx, h264g_batch *batch);
  * Requires h264g_config.profile = 1. */
 int32_t h264g_stage_times(h264g_ctx *ctx, float out[3]);
 
+/* Completion fences on the context's stream (serving loops): a fence records the
+ * point after everything enqueued so far; query returns 1 when that work is done,
+ * 0 when not yet, negative on error; wait blocks.  No counterpart in the reference,
+ * which decodes synchronously. */
+typedef struct h264g_fence h264g_fence;
+int32_t h264g_fence_record(h264g_ctx *ctx, h264g_fence **out);
+int32_t h264g_fence_query(h264g_ctx *ctx, h264g_fence *fence);
+int32_t h264g_fence_wait(h264g_ctx *ctx, h264g_fence *fence);
+int32_t h264g_fence_destroy(h264g_ctx *ctx, h264g_fence *fence);
+
 /* Turn per-stage event recording on or off for later launches (as
  * h264g_config.profile).  While it is on, batches launch their stages directly
  * instead of replaying their captured graphs. */
@@ -275,7 +285,8 @@ int32_t h264g_slot_readback(h264g_ctx *ctx, uint32_t stream_id, int32_t slot,
 int32_t h264g_slot_upload(h264g_ctx *ctx, uint32_t stream_id, int32_t slot,
                           const uint8_t *y, const uint8_t *cb, const uint8_t *cr);
 
-/* Release a staged picture's device syntax buffers. */
+/* Release a staged picture's device syntax buffers.  Does not wait: the memory is
+ * reused only after the launches enqueued before the release have completed. */
 int32_t h264g_picture_release(h264g_ctx *ctx, h264g_pic *pic);
 
 /* Algorithmic bytes of a staged picture for the roofline report:
