@@ -134,6 +134,8 @@ def main():
     ap.add_argument("--deblock-waves", type=int, default=0, help="waves per workgroup of the picture_wg schedule")
     ap.add_argument("--intra", choices=["picture_wg", "mb_wavefront"], default="picture_wg", help="intra schedule")
     ap.add_argument("--intra-waves", type=int, default=0, help="waves per workgroup of the picture_wg intra schedule")
+    ap.add_argument("--serve-seconds", type=float, default=3.0,
+                    help="length of the serving run (Poisson arrivals, 128 streams, 400 frames/s); 0 = skip")
     ap.add_argument("--profile-run", action="store_true",
                     help="short run for rocprofv3: 1 timed step, no warmup, no latency / plain / CPU legs, "
                          "then the library's calibration copy of --copy-mib MiB")
@@ -267,6 +269,36 @@ def main():
                              "alg_GBs": round(alg[k] / nframes / (ms * 1e-3) / 1e9, 1) if ms > 0 else None}
     mc_dbk = (alg[0] + alg[2]) / nframes / ((stage_ms[0] + stage_ms[2]) / launches * 1e-3) / 1e9
 
+    serve = None
+    if args.serve_seconds > 0:
+        # ---- serving: 128 streams of the serving mix, GOP 32, Poisson arrivals of
+        # 400 frames/s in total, 2 ms batch windows, DPB budget 200 GB
+        from h264_video_decoder_demo_amd.serve import ServeLoop, make_specs
+        for b in batches:
+            eng.batch_destroy(b)
+        batches = []
+        for row in pics:
+            for p in row:
+                eng.release(p)
+        for sid in sids:
+            eng.close_stream(sid)
+        specs = make_specs(128, total_fps=400.0, duration_s=args.serve_seconds)
+        t = time.perf_counter()
+        with cf.ThreadPoolExecutor(16) as ex:
+            sgens = list(ex.map(lambda sp: Generator(2, width_mbs=sp.width_mbs, height_mbs=sp.height_mbs,
+                                                     n_frames=sp.n_frames, seed=sp.seed, gop=32), specs))
+        log(f"serving: generated {sum(g.num_pictures for g in sgens)} frames of {len(specs)} streams in "
+            f"{time.perf_counter() - t:.1f} s")
+        seng = Engine(device=0, dpb_budget_bytes=200 * 10**9, waves=args.waves, deblock_schedule=schedule,
+                      deblock_waves=args.deblock_waves, intra_schedule={"picture_wg": 0, "mb_wavefront": 1}[args.intra],
+                      intra_waves=args.intra_waves)
+        res = ServeLoop(seng, specs, window_ms=2.0, gens=sgens).run()
+        seng.close()
+        serve = {"streams": len(specs), "offered_fps": 400.0, "frames": res.frames, "seconds": round(res.wall_s, 3),
+                 "mpix_s": round(res.mpix_per_s, 1), "p50_ms": round(res.pct(50), 3), "p99_ms": round(res.pct(99), 3),
+                 "max_batch": res.max_batch, "window_ms": 2.0, "dpb_budget_gb": 200}
+        log(f"serving: {res.frames} frames, p50 {serve['p50_ms']} ms, p99 {serve['p99_ms']} ms")
+
     cpu = None
     if not args.no_cpu_baseline:
         t = time.perf_counter()
@@ -287,9 +319,12 @@ def main():
         "batched_event_timed_value": round(mpix_step * args.steps / t_events, 1),
         "plain_value": round(plain, 1) if plain else None,
         "batched_over_plain": round(batched / plain, 2) if plain else None,
-        "latency_ms": {"batched_frame_p50": round(pct(lat_b, 50), 3), "batched_frame_p99": round(pct(lat_b, 99), 3),
+        "latency_ms": {"serving_frame_p50": serve["p50_ms"] if serve else None,
+                       "serving_frame_p99": serve["p99_ms"] if serve else None,
+                       "batched_frame_p50": round(pct(lat_b, 50), 3), "batched_frame_p99": round(pct(lat_b, 99), 3),
                        "plain_picture_p50": round(pct(lat_p, 50), 3) if lat_p else None,
                        "plain_picture_p99": round(pct(lat_p, 99), 3) if lat_p else None},
+        "serving": serve,
         "stages": stages,
         "mc_deblock_alg_GBs": round(mc_dbk, 1),
         "bitexact": {"batched_equals_plain": (plain is not None), "dpb_sha256": h_batched[:16]},
@@ -299,6 +334,7 @@ def main():
     for b in batches:
         eng.batch_destroy(b)
     print(json.dumps(out), flush=True)
+    eng.close()
 
 
 if __name__ == "__main__":
