@@ -124,6 +124,7 @@ _SIGNATURES = {
     "h264g_sync": (c_int32, [c_void_p]),
     "h264g_slot_readback": (c_int32, [c_void_p, c_uint32, c_int32, c_void_p, c_void_p, c_void_p]),
     "h264g_slot_upload": (c_int32, [c_void_p, c_uint32, c_int32, c_void_p, c_void_p, c_void_p]),
+    "h264g_slot_to_bgr24": (c_int32, [c_void_p, c_uint32, c_int32, c_void_p, c_uint32, c_int32]),
     "h264g_picture_release": (c_int32, [c_void_p, c_void_p]),
     "h264g_picture_alg_bytes": (c_int32, [c_void_p, POINTER(c_uint64)]),
     "h264g_calibration_copy": (c_int32, [c_void_p, c_uint64, c_int32]),

@@ -1022,6 +1022,26 @@ int32_t h264g_slot_readback(h264g_ctx *c, uint32_t sid, int32_t slot, uint8_t *y
     return H264G_OK;
 }
 
+int32_t h264g_slot_to_bgr24(h264g_ctx *c, uint32_t sid, int32_t slot, uint8_t *bgr, uint32_t stride, int32_t flip) {
+    if (!c || !bgr) return h264g_set_error(H264G_E_ARG, "NULL argument");
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (sid >= c->streams.size() || !c->streams[sid].open) return h264g_set_error(H264G_E_ARG, "stream %u", sid);
+    Stream &s = c->streams[sid];
+    if (slot < 0 || slot >= s.desc.dpb_slots) return h264g_set_error(H264G_E_ARG, "slot %d", slot);
+    const int wl = s.W * 16, hl = s.H * 16;
+    if (stride < (uint32_t)wl * 3) return h264g_set_error(H264G_E_ARG, "stride %u < %d", stride, wl * 3);
+    HIPCHK(hipSetDevice(c->device));
+    const size_t bytes = (size_t)stride * hl;
+    uint8_t *d = (uint8_t *)pool_get(c, bytes);
+    if (!d) return h264g_set_error(H264G_E_NOMEM, "conversion buffer");
+    int e = h264g_launch_bgr24(s.dpb + (uint64_t)slot * s.slot_bytes, wl, hl, d, stride, flip != 0, c->stream);
+    hipError_t he = e ? (hipError_t)e : hipStreamSynchronize(c->stream);
+    if (he == hipSuccess) he = hipMemcpy(bgr, d, bytes, hipMemcpyDeviceToHost);
+    pool_put(c, d, bytes);
+    if (he != hipSuccess) return h264g_set_error(H264G_E_DEVICE, "bgr24 conversion: %s", hipGetErrorString(he));
+    return H264G_OK;
+}
+
 int32_t h264g_slot_upload(h264g_ctx *c, uint32_t sid, int32_t slot, const uint8_t *y, const uint8_t *cb, const uint8_t *cr) {
     if (!c || !y || !cb || !cr) return h264g_set_error(H264G_E_ARG, "NULL argument");
     std::lock_guard<std::mutex> lk(c->mu);

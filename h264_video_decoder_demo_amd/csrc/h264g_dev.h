@@ -87,5 +87,6 @@ int h264g_launch_deblock_rows(const DevPic *pics, const uint32_t *items, uint32_
 int h264g_launch_intra_pic(const DevPic *pics, uint32_t n_pics, int waves, uint32_t *err, void *stream);
 int h264g_launch_deblock_pic(const DevPic *pics, uint32_t n_pics, int waves, uint32_t *err, void *stream);
 int h264g_launch_bs(const DevPic *pics, const uint32_t *items, uint32_t n, void *stream);
+int h264g_launch_bgr24(const uint8_t *yuv, int wl, int hl, uint8_t *bgr, uint32_t stride, int flip, void *stream);
 int h264g_launch_copy(const uint8_t *src, uint8_t *dst, uint64_t bytes, void *stream);
 }

@@ -1457,6 +1457,31 @@ __global__ __launch_bounds__(WAVE) void k_bs(const DevPic *__restrict__ pics, co
     P.bs[(size_t)addr * 32 + j] = (uint8_t)bS;
 }
 
+// YUV420P -> BGR24 of one DPB slot (CH264PictureBase::convertYuv420pToBgr24,
+// H264PictureBase.cpp:440-495): one thread per 2x2 luma block (one chroma sample);
+// BT.601 integer coefficients over 1000 with C division (toward zero), clipped.
+__global__ __launch_bounds__(256) void k_bgr24(const uint8_t *__restrict__ yuv, int wl, int hl, uint8_t *__restrict__ bgr,
+                                                uint32_t stride, int flip) {
+    const int wc = wl >> 1, hc = hl >> 1;
+    const int cx = blockIdx.x * blockDim.x + threadIdx.x, cy = blockIdx.y;
+    if (cx >= wc || cy >= hc) return;
+    const uint8_t *u = yuv + (size_t)wl * hl, *v = u + (size_t)wc * hc;
+    const int U = u[(size_t)cy * wc + cx] - 128, V = v[(size_t)cy * wc + cx] - 128;
+    const int gb = 2018 * U, gg = -813 * V - 391 * U, gr = 1596 * V;
+#pragma unroll
+    for (int dy = 0; dy < 2; dy++) {
+        const int y = 2 * cy + dy;
+        uint8_t *o = bgr + (size_t)(flip ? hl - 1 - y : y) * stride + (size_t)(2 * cx) * 3;
+#pragma unroll
+        for (int dx = 0; dx < 2; dx++) {
+            const int Y = 1164 * (yuv[(size_t)y * wl + 2 * cx + dx] - 16);
+            o[3 * dx + 0] = (uint8_t)clip1((Y + gb) / 1000);
+            o[3 * dx + 1] = (uint8_t)clip1((Y + gg) / 1000);
+            o[3 * dx + 2] = (uint8_t)clip1((Y + gr) / 1000);
+        }
+    }
+}
+
 __global__ void k_copy(const uint4 *src, uint4 *dst, uint64_t n16) {
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n16; i += (uint64_t)gridDim.x * blockDim.x) dst[i] = src[i];
 }
@@ -1512,6 +1537,12 @@ int h264g_launch_deblock_pic(const DevPic *pics, uint32_t n_pics, int waves, uin
 int h264g_launch_bs(const DevPic *pics, const uint32_t *items, uint32_t n, void *stream) {
     if (n == 0) return 0;
     hipLaunchKernelGGL(k_bs, dim3((n + 1) / 2), dim3(WAVE), 0, (hipStream_t)stream, pics, items, n);
+    return (int)hipGetLastError();
+}
+
+int h264g_launch_bgr24(const uint8_t *yuv, int wl, int hl, uint8_t *bgr, uint32_t stride, int flip, void *stream) {
+    const int wc = wl / 2, hc = hl / 2;
+    hipLaunchKernelGGL(k_bgr24, dim3((wc + 255) / 256, hc), dim3(256), 0, (hipStream_t)stream, yuv, wl, hl, bgr, stride, flip);
     return (int)hipGetLastError();
 }
 

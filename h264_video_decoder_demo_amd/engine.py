@@ -249,6 +249,14 @@ class Engine:
         check(lib().h264g_slot_readback(self._h, sid, slot, _ptr(y), _ptr(cb), _ptr(cr)))
         return y, cb, cr
 
+    def to_bgr24(self, sid: int, slot: int, flip: bool = False) -> np.ndarray:
+        """BGR24 image of a DPB slot, rows padded to 4 bytes (the reference's BMP layout)."""
+        W, H, _ = self.streams[sid]
+        stride = (W * 16 * 3 + 3) // 4 * 4
+        out = np.zeros((H * 16, stride), np.uint8)
+        check(lib().h264g_slot_to_bgr24(self._h, sid, slot, _ptr(out), stride, 1 if flip else 0))
+        return out
+
     def upload(self, sid: int, slot: int, y: np.ndarray, cb: np.ndarray, cr: np.ndarray):
         y, cb, cr = (np.ascontiguousarray(a, np.uint8) for a in (y, cb, cr))
         check(lib().h264g_slot_upload(self._h, sid, slot, _ptr(y), _ptr(cb), _ptr(cr)))

@@ -281,6 +281,13 @@ int32_t h264g_sync(h264g_ctx *ctx);
  * output callback fires, as do_callback does (H264VideoDecoder.cpp:383-435). */
 int32_t h264g_slot_readback(h264g_ctx *ctx, uint32_t stream_id, int32_t slot,
                             uint8_t *y, uint8_t *cb, uint8_t *cr);
+/* Convert a decoded DPB slot to BGR24 on the device and copy it to host memory:
+ * rows of `stride` bytes (>= 3 * width), bottom-up when flip = 1.  Replaces
+ * CH264PictureBase::convertYuv420pToBgr24 / ...FlipLines (H264PictureBase.cpp:440-495)
+ * used by saveToBmpFile and the player. */
+int32_t h264g_slot_to_bgr24(h264g_ctx *ctx, uint32_t stream_id, int32_t slot, uint8_t *bgr, uint32_t stride,
+                            int32_t flip);
+
 /* Upload host planes into a DPB slot (tests, resuming a stream at an IDR). */
 int32_t h264g_slot_upload(h264g_ctx *ctx, uint32_t stream_id, int32_t slot,
                           const uint8_t *y, const uint8_t *cb, const uint8_t *cr);

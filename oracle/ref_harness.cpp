@@ -25,7 +25,9 @@
 //
 // Usage: ref_harness --config N [--frames F --width W --height H --seed S]
 //                    [--ipcm P] [--wp P,B] [--cqp a,b] [--deblock idc]
-//                    --out DIR [--predeblock] [--time]
+//                    --out DIR [--predeblock] [--bgr] [--time]
+// --bgr also writes every picture converted by the reference's
+// CH264PictureBase::convertYuv420pToBgr24 (rows padded to 4 bytes) as bgr_NNN.raw.
 #include "H264PicturesGOP.h"
 #include "H264Picture.h"
 #include "H264PictureBase.h"
@@ -136,6 +138,7 @@ struct Harness {
     CH264Picture *dpb[16];
     CH264SliceHeader sh;
     bool predeblock = false;
+    bool bgr = false;
     std::string out;
     double recon_seconds = 0;
     long mismatches = 0;
@@ -513,6 +516,16 @@ struct Harness {
         size_t n = (size_t)W * H * 256 * 3 / 2;
         fwrite(pic.m_pic_buff_luma, 1, n, f);  // Y, Cb, Cr are one contiguous block
         fclose(f);
+        if (bgr && std::strcmp(tag, "pic") == 0) {
+            const uint32_t w = (uint32_t)W * 16, h = (uint32_t)H * 16, stride = (w * 3 + 3) / 4 * 4;
+            std::vector<uint8_t> buf((size_t)stride * h, 0);
+            if (pic.convertYuv420pToBgr24(w, h, pic.m_pic_buff_luma, buf.data(), stride) != 0) fail("convertYuv420pToBgr24");
+            snprintf(path, sizeof(path), "%s/bgr_%03d.raw", out.c_str(), idx);
+            f = fopen(path, "wb");
+            if (!f) fail("cannot write output");
+            fwrite(buf.data(), 1, buf.size(), f);
+            fclose(f);
+        }
     }
 
     void run() {
@@ -640,6 +653,7 @@ int main(int argc, char **argv) {
         else if (a == "--bframes") p.b_frames = atoi(next());
         else if (a == "--out") hs.out = next();
         else if (a == "--predeblock") hs.predeblock = true;
+        else if (a == "--bgr") hs.bgr = true;
         else if (a == "--time") timing = true;
         else fail("unknown argument");
     }

@@ -983,3 +983,21 @@ int refcpu_decode_picture(const h264g_pic_desc *d, const h264g_mb *mbs, int W, i
     (void)imin; (void)imax;
     return rc;
 }
+
+/* ------------------------------------------------ output conversion */
+/* H264PictureBase.cpp:440-466 (and the bottom-up variant at :469-495): BT.601 integer
+ * coefficients over 1000, C division (truncation toward zero), clipped to 0..255;
+ * chroma sampled at (x / 2, y / 2). */
+void refcpu_yuv420p_to_bgr24(int width, int height, const uint8_t *yuv, uint8_t *bgr, int stride, int flip) {
+    const uint8_t *u = yuv + (size_t)width * height, *v = u + (size_t)width * height / 4;
+    for (int y = 0; y < height; y++)
+        for (int x = 0; x < width; x++) {
+            const int Y = yuv[(size_t)y * width + x] - 16;
+            const int U = u[(size_t)(y / 2) * (width / 2) + x / 2] - 128, V = v[(size_t)(y / 2) * (width / 2) + x / 2] - 128;
+            const int b = (1164 * Y + 2018 * U) / 1000, g = (1164 * Y - 813 * V - 391 * U) / 1000, r = (1164 * Y + 1596 * V) / 1000;
+            uint8_t *o = bgr + (size_t)(flip ? height - 1 - y : y) * stride + (size_t)x * 3;
+            o[0] = (uint8_t)CLIP3(0, 255, b);
+            o[1] = (uint8_t)CLIP3(0, 255, g);
+            o[2] = (uint8_t)CLIP3(0, 255, r);
+        }
+}

@@ -43,6 +43,11 @@ int refcpu_chroma_sample(const uint8_t *ref, int w, int h, int xInt, int yInt, i
 /* 8.7.2.3 / 8.7.2.4 on one line of 8 samples p3..p0 q0..q3 (in/out). */
 void refcpu_filter_line(uint8_t p[4], uint8_t q[4], int bS, int chroma, int qp_avg, int offset_a, int offset_b);
 
+/* YUV420P -> BGR24 of a decoded picture (CH264PictureBase::convertYuv420pToBgr24,
+ * H264PictureBase.cpp:440-466): bgr rows of `stride` bytes; flip = 1 writes rows
+ * bottom-up (convertYuv420pToBgr24FlipLines). */
+void refcpu_yuv420p_to_bgr24(int width, int height, const uint8_t *yuv, uint8_t *bgr, int stride, int flip);
+
 #ifdef __cplusplus
 }
 #endif

@@ -43,6 +43,7 @@ def lib() -> ctypes.CDLL:
         L.refcpu_chroma_sample.restype = c_int
         L.refcpu_chroma_sample.argtypes = [c_void_p, c_int, c_int, c_int, c_int, c_int, c_int]
         L.refcpu_filter_line.argtypes = [c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_int]
+        L.refcpu_yuv420p_to_bgr24.argtypes = [c_int, c_int, c_void_p, c_void_p, c_int, c_int]
         _lib = L
     return _lib
 
@@ -122,3 +123,13 @@ def filter_line(p, q, bS: int, chroma: int, qp_avg: int, off_a: int = 0, off_b: 
     qq = np.ascontiguousarray(q, np.uint8).copy()
     lib().refcpu_filter_line(_ptr(pp), _ptr(qq), bS, chroma, qp_avg, off_a, off_b)
     return pp, qq
+
+
+def yuv420p_to_bgr24(y: np.ndarray, cb: np.ndarray, cr: np.ndarray, flip: int = 0) -> np.ndarray:
+    """BGR24 image (rows padded to 4 bytes, as the reference's BMP output) of one picture."""
+    h, w = y.shape
+    yuv = np.concatenate([np.ascontiguousarray(p, np.uint8).reshape(-1) for p in (y, cb, cr)])
+    stride = (w * 3 + 3) // 4 * 4
+    out = np.zeros(stride * h, np.uint8)
+    lib().refcpu_yuv420p_to_bgr24(w, h, _ptr(yuv), _ptr(out), stride, flip)
+    return out.reshape(h, stride)

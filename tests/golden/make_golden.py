@@ -5,7 +5,8 @@ The reference's reconstruction sources (jfu222/h264_video_decoder_demo, compiled
 unmodified by oracle/Makefile into oracle/_ref/) are driven by oracle/ref_harness.cpp
 on the syntax of csrc/gen.cpp.  For each case the fixture records the generator's
 digest (so a changed generator is caught) and SHA-256 of every output picture's Y,
-Cb and Cr planes, before and after deblocking.  The reference has no tests or
+Cb and Cr planes, before and after deblocking, and of the BGR24 image the
+reference's convertYuv420pToBgr24 makes of it.  The reference has no tests or
 golden vectors of its own for this path; these are the pins.
 
 Run in a container that has /root/reference:  python tests/golden/make_golden.py
@@ -74,13 +75,13 @@ def plane_hashes(buf: bytes, W: int, H: int):
 
 def run_case(config: int, ov: dict):
     with tempfile.TemporaryDirectory() as d:
-        r = subprocess.run([HARNESS, *harness_args(config, ov), "--predeblock", "--out", d], capture_output=True,
-                           text=True, check=True)
+        r = subprocess.run([HARNESS, *harness_args(config, ov), "--predeblock", "--bgr", "--out", d],
+                           capture_output=True, text=True, check=True)
         info = json.loads(r.stdout.strip().splitlines()[-1])
         n = info["pictures"]
         files = sorted(os.listdir(d))
         W, H = None, None
-        out = {"digest": info["digest"], "pictures": n, "pic": [], "pre": []}
+        out = {"digest": info["digest"], "pictures": n, "pic": [], "pre": [], "bgr": []}
         for i in range(n):
             for kind in ("pre", "pic"):
                 buf = open(os.path.join(d, f"{kind}_{i:03d}.yuv"), "rb").read()
@@ -91,6 +92,9 @@ def run_case(config: int, ov: dict):
                     W, H = g.width_mbs, g.height_mbs
                 assert len(buf) == W * H * 384, (kind, i, len(buf))
                 out[kind].append(plane_hashes(buf, W, H))
+            bgr = open(os.path.join(d, f"bgr_{i:03d}.raw"), "rb").read()
+            assert len(bgr) == (W * 16 * 3 + 3) // 4 * 4 * H * 16
+            out["bgr"].append(hashlib.sha256(bgr).hexdigest())
         out["width_mbs"], out["height_mbs"] = W, H
         assert info["mv_mismatches"] == 0, info
         del files

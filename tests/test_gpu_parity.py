@@ -145,3 +145,25 @@ def test_graph_replays_match_eager_runs(engine, golden):
     finally:
         for s in sids.values():
             engine.close_stream(s)
+
+
+@pytest.mark.parametrize("name", ["cif_ip", "high_small", "flat"])
+def test_bgr24_matches_reference(engine, golden, name):
+    """Output conversion on the device against the reference's convertYuv420pToBgr24."""
+    import hashlib
+    case = golden[name]
+    gen = make_generator(case)
+    sid = engine.open_stream(gen.width_mbs, gen.height_mbs, gen.dpb_slots)
+    try:
+        for i in range(gen.num_pictures):
+            desc, mbs, levels = gen.picture(i)
+            pic = engine.begin(sid, desc, mbs, levels)
+            engine.submit(pic)
+            engine.sync()
+            bgr = engine.to_bgr24(sid, desc.dst_slot)
+            assert hashlib.sha256(bgr.tobytes()).hexdigest() == case["bgr"][i], f"{name} picture {i}"
+            flipped = engine.to_bgr24(sid, desc.dst_slot, flip=True)
+            np.testing.assert_array_equal(flipped[::-1], bgr)
+            engine.release(pic)
+    finally:
+        engine.close_stream(sid)

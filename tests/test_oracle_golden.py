@@ -35,3 +35,14 @@ def test_oracle_matches_reference(golden, name):
         if 1 in out:
             assert plane_hashes(*out[1]) == case["pre"][i], f"{name} picture {i} before deblocking"
         assert plane_hashes(*out[2]) == case["pic"][i], f"{name} picture {i}"
+
+
+@pytest.mark.parametrize("name", ["cif_ip", "high_small", "flat"])
+def test_oracle_bgr24_matches_reference(golden, name):
+    """YUV420P -> BGR24 output conversion against the reference's convertYuv420pToBgr24."""
+    import hashlib
+    from oracle.refcpu import yuv420p_to_bgr24
+    case = golden[name]
+    for i, desc, out in oracle_decode(make_generator(case), (2,)):
+        bgr = yuv420p_to_bgr24(*out[2])
+        assert hashlib.sha256(bgr.tobytes()).hexdigest() == case["bgr"][i], f"{name} picture {i}"

@@ -306,3 +306,24 @@ def test_filter_known_answers():
     # FilterOffsetA shifts indexA: qPav 24 + 6 behaves as qPav 30 for alpha / tC0
     pp, qq = refcpu.filter_line(p, q, 2, 1, 24, 6, 6)
     assert list(pp) == [102, 100, 100, 100] and list(qq) == [108, 110, 110, 110]
+
+
+def test_bgr24_closed_form_and_truncation():
+    """Gray stays gray; the division truncates toward zero (C), not down."""
+    y = np.full((2, 4), 16, np.uint8)
+    cb = np.full((1, 2), 128, np.uint8)
+    cr = np.full((1, 2), 128, np.uint8)
+    out = refcpu.yuv420p_to_bgr24(y, cb, cr)
+    assert (out[:, :12] == 0).all() and out.shape == (2, 12)
+    y[:] = 235
+    assert (refcpu.yuv420p_to_bgr24(y, cb, cr)[:, :12] == 254).all()  # 1164 * 219 / 1000 = 254.9 -> 254
+    # b = (1164 * (17 - 16) + 2018 * (127 - 128)) / 1000 = -854 / 1000 -> 0 (trunc), clipped 0
+    y[:] = 17
+    cb[:] = 127
+    px = refcpu.yuv420p_to_bgr24(y, cb, cr)[0, :3]
+    assert list(px) == [0, (1164 + 391) // 1000, 1164 // 1000]
+    # flip writes rows bottom-up
+    y = np.arange(8, dtype=np.uint8).reshape(2, 4) * 30 + 16
+    a = refcpu.yuv420p_to_bgr24(y, cb, cr, flip=0)
+    b = refcpu.yuv420p_to_bgr24(y, cb, cr, flip=1)
+    assert (a[0] == b[1]).all() and (a[1] == b[0]).all()
