@@ -36,8 +36,8 @@ METRIC = "bit-exact recon Mpix/s, batched vs plain on 1 GPU; p50/p99 frame laten
 HBM_PEAK_GBS = 8000.0
 SEED4 = 0x5EED0004
 MIX = [(16, 22, 18), (24, 80, 45), (16, 120, 68), (8, 240, 135)]  # (streams, width MBs, height MBs)
-STAGES = ("inter", "intra", "deblock")
-KERNELS = ("k_inter", "k_intra", "k_deblock")
+STAGES = ("inter", "intra", "bs", "deblock")
+KERNELS = ("k_inter", "k_intra", "k_bs", "k_deblock")
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_config4.json")
 
 
@@ -116,6 +116,38 @@ def cpu_baseline(n_proc, frames):
             "per_core_mpix_s": round(sum(m for m, _ in res) / sum(s for _, s in res), 3)}
 
 
+def summarize_pmc(fetch_csv: str, write_csv: str, copy_bytes: int, out_json: str) -> dict:
+    """HBM traffic per launch from two rocprofv3 --pmc runs of `bench.py --profile-run`
+    (FETCH_SIZE, then WRITE_SIZE; KiB per dispatch), each scaled by what the counter
+    reads for the calibration copy of copy_bytes (read once, written once)."""
+    import csv
+    import collections
+
+    def per_kernel(path, counter):
+        acc = collections.defaultdict(list)
+        with open(path) as f:
+            for row in csv.DictReader(f):
+                if row.get("Counter_Name") != counter:
+                    continue
+                name = row["Kernel_Name"]
+                for k in KERNELS + ("k_copy", "k_bs", "k_deblock_pic", "k_intra_pic", "k_deblock_rows"):
+                    if f"::{k}(" in name:
+                        acc[k].append(float(row["Counter_Value"]))
+        return {k: sum(v) / len(v) for k, v in acc.items()}
+
+    fetch, write = per_kernel(fetch_csv, "FETCH_SIZE"), per_kernel(write_csv, "WRITE_SIZE")
+    f_scale = fetch["k_copy"] * 1024.0 / copy_bytes   # counter bytes per true byte
+    w_scale = write["k_copy"] * 1024.0 / copy_bytes
+    out = {"fetch_scale": f_scale, "write_scale": w_scale, "copy_bytes": copy_bytes}
+    for k in fetch:
+        if k == "k_copy":
+            continue
+        out[k] = round(fetch[k] * 1024.0 / f_scale + write.get(k, 0.0) * 1024.0 / w_scale)
+    with open(out_json, "w") as f:
+        json.dump(out, f, indent=1, sort_keys=True)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -140,7 +172,12 @@ def main():
                     help="short run for rocprofv3: 1 timed step, no warmup, no latency / plain / CPU legs, "
                          "then the library's calibration copy of --copy-mib MiB")
     ap.add_argument("--copy-mib", type=int, default=1024)
+    ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="write profiles/traffic_config4.json from two rocprofv3 --pmc runs of --profile-run")
     args = ap.parse_args()
+    if args.summarize_pmc:
+        print(json.dumps(summarize_pmc(*args.summarize_pmc, args.copy_mib << 20, TRAFFIC_FILE)))
+        return
     if args.profile_run:
         args.steps, args.warmup, args.no_plain, args.no_cpu_baseline = 1, 0, True, True
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -161,7 +198,7 @@ def main():
                  deblock_waves=args.deblock_waves, intra_schedule={"picture_wg": 0, "mb_wavefront": 1}[args.intra],
                  intra_waves=args.intra_waves)
     sids, slots, pics = [], {}, []
-    alg = [0, 0, 0]
+    alg = [0, 0, 0, 0]
     for g in gens:
         sid = eng.open_stream(g.width_mbs, g.height_mbs, g.dpb_slots)
         sids.append(sid)
@@ -169,8 +206,11 @@ def main():
         row = []
         for i in range(g.num_pictures):
             p = eng.begin_raw(sid, *g.picture_raw(i))
-            for k, v in enumerate(eng.alg_bytes(p)):
-                alg[k] += v
+            ab = eng.alg_bytes(p)
+            alg[0] += ab[0]
+            alg[1] += ab[1]
+            alg[3] += ab[2]
+            alg[2] += g.width_mbs * g.height_mbs * (28 + 172 + 32)  # bS pass: aux + records in, 32 B out per MB
             row.append(p)
         pics.append(row)
     nframes = min(len(r) for r in pics)
@@ -250,24 +290,26 @@ def main():
         log(f"plain: {plain:.1f} Mpix/s; batched/plain = {batched / plain:.1f}x")
 
     # ---- roofline of the dominant kernel (algorithmic bytes / event-timed duration)
-    dom = max(range(3), key=lambda k: stage_ms[k])
+    dom = max(range(4), key=lambda k: stage_ms[k])
     launches = args.steps * nframes
     per_launch_bytes = alg[dom] / nframes
     avg_ms = stage_ms[dom] / launches
     achieved = per_launch_bytes / (avg_ms * 1e-3) / 1e9
+    kname = {0: "k_inter", 1: "k_intra_pic" if args.intra == "picture_wg" else "k_intra", 2: "k_bs",
+             3: {"picture_wg": "k_deblock_pic", "mb_wavefront": "k_deblock", "mb_rows": "k_deblock_rows"}[args.deblock]}[dom]
     traffic = None
     if os.path.exists(TRAFFIC_FILE):
         with open(TRAFFIC_FILE) as f:
-            traffic = json.load(f).get(KERNELS[dom])
+            traffic = json.load(f).get(kname)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": KERNELS[dom],
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": kname,
                 "alg_bytes_per_launch": int(per_launch_bytes), "avg_launch_ms": round(avg_ms, 4)}
     stages = {}
-    for k in range(3):
+    for k in range(4):
         ms = stage_ms[k] / launches
         stages[STAGES[k]] = {"avg_launch_ms": round(ms, 4), "alg_bytes_per_launch": int(alg[k] / nframes),
                              "alg_GBs": round(alg[k] / nframes / (ms * 1e-3) / 1e9, 1) if ms > 0 else None}
-    mc_dbk = (alg[0] + alg[2]) / nframes / ((stage_ms[0] + stage_ms[2]) / launches * 1e-3) / 1e9
+    mc_dbk = (alg[0] + alg[3]) / nframes / ((stage_ms[0] + stage_ms[3]) / launches * 1e-3) / 1e9
 
     serve = None
     if args.serve_seconds > 0:

@@ -109,7 +109,7 @@ struct h264g_ctx {
     int intra_schedule = H264G_INTRA_PICTURE_WG;
     int intra_waves = 8;           // waves per workgroup of the per-picture intra schedule
     int dbk_waves = 16;            // waves per workgroup of the per-picture schedule
-    std::vector<hipEvent_t> ev;    // 4 events per profiled run: inter | intra | deblock | end
+    std::vector<hipEvent_t> ev;    // 5 events per profiled run: inter | intra | bS | deblock | end
     size_t ev_used = 0;            // events recorded since the last h264g_stage_times
     std::mutex mu;
 };
@@ -537,13 +537,13 @@ int launch_stages(h264g_ctx *ctx, const StageSet &q) {
     hipStream_t s = ctx->stream;
     hipEvent_t *ev = nullptr;
     if (ctx->profile) {
-        while (ctx->ev.size() < ctx->ev_used + 4) {
+        while (ctx->ev.size() < ctx->ev_used + 5) {
             hipEvent_t e;
             HIPCHK(hipEventCreate(&e));
             ctx->ev.push_back(e);
         }
         ev = &ctx->ev[ctx->ev_used];
-        ctx->ev_used += 4;
+        ctx->ev_used += 5;
     }
     if (ev) HIPCHK(hipEventRecord(ev[0], s));
     int e = h264g_launch_inter(q.pics, q.inter, q.n_inter, ctx->d_err, s);
@@ -558,6 +558,7 @@ int launch_stages(h264g_ctx *ctx, const StageSet &q) {
     if (!q.dbk_off) {
         e = h264g_launch_bs(q.pics, q.all, q.n_all, s);
         if (e) return h264g_set_error(H264G_E_DEVICE, "bS launch: %s", hipGetErrorString((hipError_t)e));
+        if (ev) HIPCHK(hipEventRecord(ev[3], s));
         switch (ctx->dbk_schedule) {
         case H264G_DEBLOCK_PICTURE_WG: e = h264g_launch_deblock_pic(q.pics, q.n_pics, ctx->dbk_waves, ctx->d_err, s); break;
         case H264G_DEBLOCK_MB_WAVEFRONT:
@@ -568,7 +569,8 @@ int launch_stages(h264g_ctx *ctx, const StageSet &q) {
         }
         if (e) return h264g_set_error(H264G_E_DEVICE, "deblock launch: %s", hipGetErrorString((hipError_t)e));
     }
-    if (ev) HIPCHK(hipEventRecord(ev[3], s));
+    else if (ev) HIPCHK(hipEventRecord(ev[3], s));
+    if (ev) HIPCHK(hipEventRecord(ev[4], s));
     return 0;
 }
 
@@ -970,17 +972,17 @@ int32_t h264g_set_profile(h264g_ctx *c, int32_t on) {
     return H264G_OK;
 }
 
-int32_t h264g_stage_times(h264g_ctx *c, float out[3]) {
+int32_t h264g_stage_times(h264g_ctx *c, float out[4]) {
     if (!c || !out || !c->profile) return h264g_set_error(H264G_E_STATE, "profiling is off");
     std::lock_guard<std::mutex> lk(c->mu);
     HIPCHK(hipSetDevice(c->device));
     HIPCHK(hipStreamSynchronize(c->stream));
-    out[0] = out[1] = out[2] = 0.f;
-    const size_t runs = c->ev_used / 4;
+    out[0] = out[1] = out[2] = out[3] = 0.f;
+    const size_t runs = c->ev_used / 5;
     for (size_t r = 0; r < runs; r++)
-        for (int i = 0; i < 3; i++) {
+        for (int i = 0; i < 4; i++) {
             float ms = 0.f;
-            HIPCHK(hipEventElapsedTime(&ms, c->ev[4 * r + i], c->ev[4 * r + i + 1]));
+            HIPCHK(hipEventElapsedTime(&ms, c->ev[5 * r + i], c->ev[5 * r + i + 1]));
             out[i] += ms;
         }
     c->ev_used = 0;

@@ -236,8 +236,8 @@ class Engine:
         check(lib().h264g_set_profile(self._h, 1 if on else 0))
 
     def stage_times(self):
-        """(runs, [inter_ms, intra_ms, deblock_ms]) summed since the previous call."""
-        out = (c_float * 3)()
+        """(runs, [inter_ms, intra_ms, bs_ms, deblock_ms]) summed since the previous call."""
+        out = (c_float * 4)()
         runs = check(lib().h264g_stage_times(self._h, out))
         return runs, list(out)
 

@@ -255,9 +255,10 @@ int32_t h264g_batch_destroy(h264g_ctx *ctx, h264g_batch *batch);
 /* Device-side time per stage, in milliseconds, summed over every
  * h264g_batch_run / h264g_picture_submit since the previous call (HIP events
  * recorded around each stage on the launch stream): out[0] inter, out[1] intra,
- * out[2] deblocking.  Waits for the stream; returns the number of runs summed.
- * Requires h264g_config.profile = 1. */
-int32_t h264g_stage_times(h264g_ctx *ctx, float out[3]);
+ * out[2] boundary strengths, out[3] deblocking.  Waits for the stream; returns the
+ * number of runs summed.  Requires profiling (h264g_config.profile or
+ * h264g_set_profile). */
+int32_t h264g_stage_times(h264g_ctx *ctx, float out[4]);
 
 /* Completion fences on the context's stream (serving loops): a fence records the
  * point after everything enqueued so far; query returns 1 when that work is done,
