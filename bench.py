@@ -130,7 +130,7 @@ def summarize_pmc(fetch_csv: str, write_csv: str, copy_bytes: int, out_json: str
                 if row.get("Counter_Name") != counter:
                     continue
                 name = row["Kernel_Name"]
-                for k in KERNELS + ("k_copy", "k_bs", "k_deblock_pic", "k_intra_pic", "k_deblock_rows"):
+                for k in KERNELS + ("k_copy", "k_bs", "k_deblock_band", "k_intra_band", "k_deblock_rows"):
                     if f"::{k}(" in name:
                         acc[k].append(float(row["Counter_Value"]))
         return {k: sum(v) / len(v) for k, v in acc.items()}
@@ -161,11 +161,11 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--waves", type=int, default=0)
     ap.add_argument("--no-graphs", action="store_true", help="launch batch stages directly instead of hipGraph replay")
-    ap.add_argument("--deblock", choices=["picture_wg", "mb_wavefront", "mb_rows"], default="picture_wg",
+    ap.add_argument("--deblock", choices=["bands", "mb_wavefront", "mb_rows"], default="bands",
                     help="deblocking schedule")
-    ap.add_argument("--deblock-waves", type=int, default=0, help="waves per workgroup of the picture_wg schedule")
-    ap.add_argument("--intra", choices=["picture_wg", "mb_wavefront"], default="picture_wg", help="intra schedule")
-    ap.add_argument("--intra-waves", type=int, default=0, help="waves per workgroup of the picture_wg intra schedule")
+    ap.add_argument("--deblock-waves", type=int, default=0, help="MB rows per band of the bands schedule (0 = 16)")
+    ap.add_argument("--intra", choices=["bands", "mb_wavefront"], default="bands", help="intra schedule")
+    ap.add_argument("--intra-waves", type=int, default=0, help="MB rows per band of the bands intra schedule (0 = 8)")
     ap.add_argument("--serve-seconds", type=float, default=3.0,
                     help="length of the serving run (Poisson arrivals, 128 streams, 400 frames/s); 0 = skip")
     ap.add_argument("--profile-run", action="store_true",
@@ -193,9 +193,9 @@ def main():
         gens = list(ex.map(lambda s: Generator(2, **s), specs))
     log(f"generated {len(gens)} streams in {time.perf_counter() - t:.1f} s")
 
-    schedule = {"picture_wg": 0, "mb_wavefront": 1, "mb_rows": 2}[args.deblock]
+    schedule = {"bands": 0, "mb_wavefront": 1, "mb_rows": 2}[args.deblock]
     eng = Engine(device=0, profile=False, waves=args.waves, graphs=not args.no_graphs, deblock_schedule=schedule,
-                 deblock_waves=args.deblock_waves, intra_schedule={"picture_wg": 0, "mb_wavefront": 1}[args.intra],
+                 deblock_waves=args.deblock_waves, intra_schedule={"bands": 0, "mb_wavefront": 1}[args.intra],
                  intra_waves=args.intra_waves)
     sids, slots, pics = [], {}, []
     alg = [0, 0, 0, 0]
@@ -295,8 +295,8 @@ def main():
     per_launch_bytes = alg[dom] / nframes
     avg_ms = stage_ms[dom] / launches
     achieved = per_launch_bytes / (avg_ms * 1e-3) / 1e9
-    kname = {0: "k_inter", 1: "k_intra_pic" if args.intra == "picture_wg" else "k_intra", 2: "k_bs",
-             3: {"picture_wg": "k_deblock_pic", "mb_wavefront": "k_deblock", "mb_rows": "k_deblock_rows"}[args.deblock]}[dom]
+    kname = {0: "k_inter", 1: "k_intra_band" if args.intra == "bands" else "k_intra", 2: "k_bs",
+             3: {"bands": "k_deblock_band", "mb_wavefront": "k_deblock", "mb_rows": "k_deblock_rows"}[args.deblock]}[dom]
     traffic = None
     if os.path.exists(TRAFFIC_FILE):
         with open(TRAFFIC_FILE) as f:
@@ -332,7 +332,7 @@ def main():
         log(f"serving: generated {sum(g.num_pictures for g in sgens)} frames of {len(specs)} streams in "
             f"{time.perf_counter() - t:.1f} s")
         seng = Engine(device=0, dpb_budget_bytes=200 * 10**9, waves=args.waves, deblock_schedule=schedule,
-                      deblock_waves=args.deblock_waves, intra_schedule={"picture_wg": 0, "mb_wavefront": 1}[args.intra],
+                      deblock_waves=args.deblock_waves, intra_schedule={"bands": 0, "mb_wavefront": 1}[args.intra],
                       intra_waves=args.intra_waves)
         res = ServeLoop(seng, specs, window_ms=2.0, gens=sgens).run()
         seng.close()

@@ -85,8 +85,8 @@ class Config(ctypes.Structure):
     ]
 
 
-DEBLOCK_PICTURE_WG, DEBLOCK_MB_WAVEFRONT, DEBLOCK_MB_ROWS = 0, 1, 2
-INTRA_PICTURE_WG, INTRA_MB_WAVEFRONT = 0, 1
+DEBLOCK_BANDS, DEBLOCK_MB_WAVEFRONT, DEBLOCK_MB_ROWS = 0, 1, 2
+INTRA_BANDS, INTRA_MB_WAVEFRONT = 0, 1
 
 
 class GenParams(ctypes.Structure):

@@ -47,6 +47,7 @@ struct Stream {
     uint64_t slot_bytes = 0;
     uint32_t *flags = nullptr;
     uint64_t *rowprog = nullptr;
+    uint8_t *gstrip = nullptr;     // deblocking strips between row bands
     uint64_t bytes = 0;
     bool open = false;
 };
@@ -82,7 +83,8 @@ struct Batch {
     DevPic *d_pics = nullptr;
     uint32_t n_pics = 0;
     uint32_t *d_inter = nullptr, *d_intra = nullptr, *d_all = nullptr, *d_rows = nullptr;
-    uint32_t n_inter = 0, n_intra = 0, n_all = 0, n_rows = 0;
+    uint32_t *d_ibands = nullptr, *d_dbands = nullptr;   // (picture, band) items, band-major
+    uint32_t n_inter = 0, n_intra = 0, n_all = 0, n_rows = 0, n_ibands = 0, n_dbands = 0;
     bool dbk_off = false;            // no picture of the batch is deblocked
     hipGraphExec_t graph = nullptr;  // the captured stage sequence (kernel nodes only)
 };
@@ -105,10 +107,10 @@ struct h264g_ctx {
     size_t pool_bytes = 0;
     bool profile = false;
     bool no_graphs = false;
-    int dbk_schedule = H264G_DEBLOCK_PICTURE_WG;
-    int intra_schedule = H264G_INTRA_PICTURE_WG;
-    int intra_waves = 8;           // waves per workgroup of the per-picture intra schedule
-    int dbk_waves = 16;            // waves per workgroup of the per-picture schedule
+    int dbk_schedule = H264G_DEBLOCK_BANDS;
+    int intra_schedule = H264G_INTRA_BANDS;
+    int intra_waves = 8;           // rows per band (waves per workgroup) of the intra band schedule
+    int dbk_waves = 16;            // rows per band (waves per workgroup) of the deblocking band schedule
     std::vector<hipEvent_t> ev;    // 5 events per profiled run: inter | intra | bS | deblock | end
     size_t ev_used = 0;            // events recorded since the last h264g_stage_times
     std::mutex mu;
@@ -509,7 +511,7 @@ int stage_picture(h264g_ctx *ctx, Stream &st, const h264g_pic_desc &d, const h26
     DevPic &P = pic->hpic;
     std::memset(&P, 0, sizeof(P));
     P.mbs = pic->d_mbs; P.aux = pic->d_aux; P.levels = pic->d_levels; P.wt = pic->d_wt;
-    P.dpb = st.dpb; P.slot_bytes = st.slot_bytes; P.flags = st.flags; P.rowprog = st.rowprog; P.bs = d_bs; P.dst = d.dst_slot;
+    P.dpb = st.dpb; P.slot_bytes = st.slot_bytes; P.flags = st.flags; P.rowprog = st.rowprog; P.gstrip = st.gstrip; P.bs = d_bs; P.dst = d.dst_slot;
     P.irow = pic->d_items + irow_at; P.n_intra = pic->n_intra;
     P.W = W; P.H = H; P.offA = d.filter_offset_a; P.offB = d.filter_offset_b;
     P.dbk_idc = d.disable_deblocking_filter_idc; P.cip = d.constrained_intra_pred_flag;
@@ -528,8 +530,8 @@ int stage_picture(h264g_ctx *ctx, Stream &st, const h264g_pic_desc &d, const h26
 struct StageSet {
     const DevPic *pics;
     uint32_t n_pics;
-    const uint32_t *inter, *intra, *all, *rows;
-    uint32_t n_inter, n_intra, n_all, n_rows;
+    const uint32_t *inter, *intra, *all, *rows, *ibands, *dbands;
+    uint32_t n_inter, n_intra, n_all, n_rows, n_ibands, n_dbands;
     bool dbk_off;
 };
 
@@ -550,8 +552,9 @@ int launch_stages(h264g_ctx *ctx, const StageSet &q) {
     if (e) return h264g_set_error(H264G_E_DEVICE, "inter launch: %s", hipGetErrorString((hipError_t)e));
     if (ev) HIPCHK(hipEventRecord(ev[1], s));
     if (q.n_intra)
-        e = ctx->intra_schedule == H264G_INTRA_PICTURE_WG
-                ? h264g_launch_intra_pic(q.pics, q.n_pics, ctx->intra_waves, ctx->d_err, s)
+        e = ctx->intra_schedule == H264G_INTRA_BANDS
+                ? h264g_launch_intra_band(q.pics, q.ibands, q.n_ibands, &ctx->d_ctl[0], &ctx->d_ctl[2].ticket, ctx->intra_waves,
+                                          ctx->d_err, s)
                 : h264g_launch_intra(q.pics, q.intra, q.n_intra, &ctx->d_ctl[0], &ctx->d_ctl[2].ticket, ctx->d_err, ctx->waves, s);
     if (e) return h264g_set_error(H264G_E_DEVICE, "intra launch: %s", hipGetErrorString((hipError_t)e));
     if (ev) HIPCHK(hipEventRecord(ev[2], s));
@@ -560,7 +563,10 @@ int launch_stages(h264g_ctx *ctx, const StageSet &q) {
         if (e) return h264g_set_error(H264G_E_DEVICE, "bS launch: %s", hipGetErrorString((hipError_t)e));
         if (ev) HIPCHK(hipEventRecord(ev[3], s));
         switch (ctx->dbk_schedule) {
-        case H264G_DEBLOCK_PICTURE_WG: e = h264g_launch_deblock_pic(q.pics, q.n_pics, ctx->dbk_waves, ctx->d_err, s); break;
+        case H264G_DEBLOCK_BANDS:
+            e = h264g_launch_deblock_band(q.pics, q.dbands, q.n_dbands, &ctx->d_ctl[1], &ctx->d_ctl[2].ticket, ctx->dbk_waves,
+                                          ctx->d_err, s);
+            break;
         case H264G_DEBLOCK_MB_WAVEFRONT:
             e = h264g_launch_deblock(q.pics, q.all, q.n_all, &ctx->d_ctl[1], &ctx->d_ctl[2].ticket, ctx->d_err, ctx->waves, s);
             break;
@@ -613,15 +619,15 @@ int32_t h264g_create(const h264g_config *cfg, h264g_ctx **out) {
     c->budget = cfg ? cfg->dpb_budget_bytes : 0;
     c->profile = cfg && cfg->profile;
     c->no_graphs = cfg && cfg->no_graphs;
-    c->dbk_schedule = cfg ? cfg->deblock_schedule : H264G_DEBLOCK_PICTURE_WG;
+    c->dbk_schedule = cfg ? cfg->deblock_schedule : H264G_DEBLOCK_BANDS;
     c->dbk_waves = (cfg && cfg->deblock_waves > 0) ? cfg->deblock_waves : 16;
-    c->intra_schedule = cfg ? cfg->intra_schedule : H264G_INTRA_PICTURE_WG;
+    c->intra_schedule = cfg ? cfg->intra_schedule : H264G_INTRA_BANDS;
     c->intra_waves = (cfg && cfg->intra_waves > 0) ? cfg->intra_waves : 8;
     if (c->intra_schedule < 0 || c->intra_schedule > 1 || c->intra_waves > 8) {
         h264g_destroy(c);
         return h264g_set_error(H264G_E_ARG, "intra_schedule %d / intra_waves %d", cfg->intra_schedule, cfg->intra_waves);
     }
-    if (c->dbk_schedule < 0 || c->dbk_schedule > 2 || c->dbk_waves < 2 || c->dbk_waves > 16) {
+    if (c->dbk_schedule < 0 || c->dbk_schedule > 2 || c->dbk_waves < 1 || c->dbk_waves > 16) {
         h264g_destroy(c);
         return h264g_set_error(H264G_E_ARG, "deblock_schedule %d / deblock_waves %d", cfg->deblock_schedule, cfg->deblock_waves);
     }
@@ -650,6 +656,7 @@ int32_t h264g_destroy(h264g_ctx *c) {
         if (s.dpb) (void)hipFree(s.dpb);
         if (s.flags) (void)hipFree(s.flags);
         if (s.rowprog) (void)hipFree(s.rowprog);
+        if (s.gstrip) (void)hipFree(s.gstrip);
     }
     for (auto &e : c->ev) if (e) (void)hipEventDestroy(e);
     if (c->d_err) (void)hipFree(c->d_err);
@@ -673,7 +680,10 @@ int32_t h264g_stream_open(h264g_ctx *c, const h264g_stream_desc *d, uint32_t *id
     s.W = d->pic_width_in_mbs;
     s.H = d->frame_height_in_mbs;
     s.slot_bytes = (uint64_t)s.W * s.H * 384;
-    uint64_t need = s.slot_bytes * d->dpb_slots + (uint64_t)s.W * s.H * 4 + (uint64_t)s.H * 8;
+    // DPB slots, per-MB progress words, per-row progress words, and one row of
+    // deblocking strips (128 B per MB) per band of c->dbk_waves rows
+    const uint64_t strip_bytes = (uint64_t)((s.H + c->dbk_waves - 1) / c->dbk_waves) * s.W * 128;
+    uint64_t need = s.slot_bytes * d->dpb_slots + (uint64_t)s.W * s.H * 4 + (uint64_t)s.H * 8 + strip_bytes;
     if (c->budget && c->used + need > c->budget)
         return h264g_set_error(H264G_E_BUDGET, "DPB budget: %llu + %llu > %llu", (unsigned long long)c->used,
                                (unsigned long long)need, (unsigned long long)c->budget);
@@ -687,6 +697,12 @@ int32_t h264g_stream_open(h264g_ctx *c, const h264g_stream_desc *d, uint32_t *id
         (void)hipFree(s.dpb);
         (void)hipFree(s.flags);
         return h264g_set_error(H264G_E_NOMEM, "row progress");
+    }
+    if (hipMalloc(&s.gstrip, strip_bytes) != hipSuccess) {
+        (void)hipFree(s.dpb);
+        (void)hipFree(s.flags);
+        (void)hipFree(s.rowprog);
+        return h264g_set_error(H264G_E_NOMEM, "deblocking strips");
     }
     HIPCHK(hipMemset(s.rowprog, 0, (size_t)s.H * 8));
     HIPCHK(hipMemset(s.dpb, 0, s.slot_bytes * d->dpb_slots));
@@ -712,6 +728,7 @@ int32_t h264g_stream_close(h264g_ctx *c, uint32_t id) {
     (void)hipFree(s.dpb);
     (void)hipFree(s.flags);
     (void)hipFree(s.rowprog);
+    (void)hipFree(s.gstrip);
     c->used -= s.bytes;
     s = Stream();
     return H264G_OK;
@@ -757,6 +774,9 @@ int32_t h264g_picture_submit(h264g_ctx *c, h264g_pic *p) {
     q.n_all = p->n_mbs;
     q.rows = q.all + p->n_mbs;
     q.n_rows = p->n_rows;
+    q.ibands = q.dbands = q.rows;  // band b of picture 0 is item b
+    q.n_ibands = (uint32_t)((p->H + c->intra_waves - 1) / c->intra_waves);
+    q.n_dbands = (uint32_t)((p->H + c->dbk_waves - 1) / c->dbk_waves);
     q.dbk_off = p->desc.disable_deblocking_filter_idc == 1;
     return launch_stages(c, q);
 }
@@ -816,17 +836,30 @@ int32_t h264g_batch_create(h264g_ctx *c, h264g_pic *const *pics, uint32_t n, h26
     for (uint32_t i = 0; i < n; i++) any_dbk |= pics[i]->desc.disable_deblocking_filter_idc != 1;
     std::vector<uint32_t> all, rows;
     if (any_dbk) merge(false, all);
+    int maxh = 0;
+    for (uint32_t i = 0; i < n; i++) maxh = std::max(maxh, pics[i]->H);
+    // bands of R rows of all pictures, band index major: the band above always comes first
+    auto bands = [&](int R, bool intra, std::vector<uint32_t> &out) {
+        for (int b = 0; b * R < maxh; b++)
+            for (uint32_t i = 0; i < n; i++) {
+                const h264g_pic *p = pics[i];
+                if (b * R < p->H && (intra ? p->n_intra != 0 : p->desc.disable_deblocking_filter_idc != 1))
+                    out.push_back((i << WI_PIC_SHIFT) | (uint32_t)b);
+            }
+    };
+    std::vector<uint32_t> ibands, dbands;
+    if (!intra.empty()) bands(c->intra_waves, true, ibands);
+    if (any_dbk) bands(c->dbk_waves, false, dbands);
     if (any_dbk && c->dbk_schedule == H264G_DEBLOCK_MB_ROWS) {
         // MB rows of all pictures, row index major: the row above always comes first.
-        int maxh = 0;
-        for (uint32_t i = 0; i < n; i++) maxh = std::max(maxh, pics[i]->H);
         for (int y = 0; y < maxh; y++)
             for (uint32_t i = 0; i < n; i++)
                 if (y < pics[i]->H) rows.push_back((i << WI_PIC_SHIFT) | (uint32_t)y);
     }
     auto al = [](size_t v) { return (v + 255) & ~(size_t)255; };
     size_t o_p = 0, o_i = al(n * sizeof(DevPic)), o_a = al(o_i + inter.size() * 4), o_l = al(o_a + intra.size() * 4);
-    size_t o_r = al(o_l + all.size() * 4), total = al(o_r + rows.size() * 4 + 4);
+    size_t o_r = al(o_l + all.size() * 4), o_ib = al(o_r + rows.size() * 4), o_db = al(o_ib + ibands.size() * 4);
+    size_t total = al(o_db + dbands.size() * 4 + 4);
     h264g_batch *b = new (std::nothrow) h264g_batch();
     if (!b) return h264g_set_error(H264G_E_NOMEM, "batch");
     b->dmem = pool_get(c, total);
@@ -839,16 +872,22 @@ int32_t h264g_batch_create(h264g_ctx *c, h264g_pic *const *pics, uint32_t n, h26
     b->d_intra = (uint32_t *)(base + o_a);
     b->d_all = (uint32_t *)(base + o_l);
     b->d_rows = (uint32_t *)(base + o_r);
+    b->d_ibands = (uint32_t *)(base + o_ib);
+    b->d_dbands = (uint32_t *)(base + o_db);
     b->n_inter = (uint32_t)inter.size();
     b->n_intra = (uint32_t)intra.size();
     b->n_all = (uint32_t)all.size();
     b->n_rows = (uint32_t)rows.size();
+    b->n_ibands = (uint32_t)ibands.size();
+    b->n_dbands = (uint32_t)dbands.size();
     b->dbk_off = !any_dbk;
     if (hipMemcpy(b->d_pics, hp.data(), n * sizeof(DevPic), hipMemcpyHostToDevice) != hipSuccess ||
         (!inter.empty() && hipMemcpy(b->d_inter, inter.data(), inter.size() * 4, hipMemcpyHostToDevice) != hipSuccess) ||
         (!intra.empty() && hipMemcpy(b->d_intra, intra.data(), intra.size() * 4, hipMemcpyHostToDevice) != hipSuccess) ||
         (!all.empty() && hipMemcpy(b->d_all, all.data(), all.size() * 4, hipMemcpyHostToDevice) != hipSuccess) ||
-        (!rows.empty() && hipMemcpy(b->d_rows, rows.data(), rows.size() * 4, hipMemcpyHostToDevice) != hipSuccess)) {
+        (!rows.empty() && hipMemcpy(b->d_rows, rows.data(), rows.size() * 4, hipMemcpyHostToDevice) != hipSuccess) ||
+        (!ibands.empty() && hipMemcpy(b->d_ibands, ibands.data(), ibands.size() * 4, hipMemcpyHostToDevice) != hipSuccess) ||
+        (!dbands.empty() && hipMemcpy(b->d_dbands, dbands.data(), dbands.size() * 4, hipMemcpyHostToDevice) != hipSuccess)) {
         free_batch(c, b);
         return h264g_set_error(H264G_E_DEVICE, "batch upload");
     }
@@ -869,6 +908,10 @@ StageSet batch_stages(const h264g_batch *b) {
     q.n_all = b->n_all;
     q.rows = b->d_rows;
     q.n_rows = b->n_rows;
+    q.ibands = b->d_ibands;
+    q.n_ibands = b->n_ibands;
+    q.dbands = b->d_dbands;
+    q.n_dbands = b->n_dbands;
     q.dbk_off = b->dbk_off;
     return q;
 }

@@ -55,7 +55,8 @@ struct DevPic {
     int32_t  offA, offB;      // FilterOffsetA / FilterOffsetB
     int32_t  dbk_idc;         // disable_deblocking_filter_idc
     int32_t  cip;             // constrained_intra_pred_flag
-    int32_t  pad[4];
+    uint8_t  *gstrip;         // deblocking strips between bands of the stream: W x 128 B per band
+    int32_t  pad[2];
 };
 
 // Work item: picture index in the batch and macroblock address.
@@ -84,8 +85,10 @@ int h264g_launch_deblock(const DevPic *pics, const uint32_t *items, uint32_t n, 
 int h264g_launch_deblock_rows(const DevPic *pics, const uint32_t *items, uint32_t n, DevCtl *ctl, uint32_t *epoch,
                               uint32_t *err, int waves, void *stream);
 // One workgroup of `waves` waves per picture (pics[0..n_pics)); pictures of up to 1024 MB rows.
-int h264g_launch_intra_pic(const DevPic *pics, uint32_t n_pics, int waves, uint32_t *err, void *stream);
-int h264g_launch_deblock_pic(const DevPic *pics, uint32_t n_pics, int waves, uint32_t *err, void *stream);
+int h264g_launch_intra_band(const DevPic *pics, const uint32_t *bands, uint32_t n, DevCtl *ctl, uint32_t *epoch,
+                            int waves, uint32_t *err, void *stream);
+int h264g_launch_deblock_band(const DevPic *pics, const uint32_t *bands, uint32_t n, DevCtl *ctl, uint32_t *epoch,
+                              int waves, uint32_t *err, void *stream);
 int h264g_launch_bs(const DevPic *pics, const uint32_t *items, uint32_t n, void *stream);
 int h264g_launch_bgr24(const uint8_t *yuv, int wl, int hl, uint8_t *bgr, uint32_t stride, int flip, void *stream);
 int h264g_launch_copy(const uint8_t *src, uint8_t *dst, uint64_t bytes, void *stream);

@@ -1153,10 +1153,9 @@ __device__ void deblock_row(const DevPic &P, int mby, Sync &sync, WaveDbk *wd) {
 }
 
 // ------------------------------------------------------------ kernels
-constexpr int DBK_WG_MAX_ROWS = 1024;  // MB rows of a picture with LDS row progress
 constexpr int DBK_WG_MAX_WAVES = 16;
 // The LDS working sets are per wave and the per-MB code synchronises with wave barriers,
-// so a workgroup may hold one wave (most kernels) or several (k_deblock_pic).
+// so a workgroup may hold one wave (most kernels) or several (the band kernels).
 __global__ __launch_bounds__(WAVE) void k_inter(const DevPic *__restrict__ pics, const uint32_t *__restrict__ items, uint32_t n,
                                                 uint32_t *err) {
     __shared__ WaveRes lds;
@@ -1205,34 +1204,111 @@ __global__ __launch_bounds__(WAVE) void k_intra(const DevPic *__restrict__ pics,
     retire(ctl, gridDim.x, epoch_p, epoch);
 }
 
-// Intra stage with one workgroup per picture: its waves take MB rows in turn and walk
-// the row's intra MBs (P.irow: row offsets, then columns).  Row progress p in LDS means
-// "every intra MB of the row left of column p is done"; MB (x, y) needs the row above at
-// min(x + 2, W) (neighbours B, C, D).  The samples of the neighbours pass through device
-// memory inside the CU, ordered by workgroup-scope release / acquire.
-constexpr int INTRA_WG_MAX_WAVES = 8;
-__global__ __launch_bounds__(WAVE * INTRA_WG_MAX_WAVES) void k_intra_pic(const DevPic *__restrict__ pics, uint32_t *err) {
-    __shared__ WaveIntra tiles[INTRA_WG_MAX_WAVES];
-    __shared__ uint32_t prog[DBK_WG_MAX_ROWS];
-    const DevPic &P = pics[blockIdx.x];
-    if (P.n_intra == 0) return;
-    const int nw = blockDim.x / WAVE, w = threadIdx.x / WAVE, W = P.W, H = P.H;
-    for (int i = threadIdx.x; i < H; i += blockDim.x) prog[i] = 0;
+// ---- bands of MB rows, one workgroup per band
+// A band is `waves` consecutive MB rows of one picture, one row per wave.  Inside the
+// band the rows synchronise through LDS counters (workgroup scope: the waves share the
+// CU's caches).  A band's last row hands its results to the next band's first row, in
+// another workgroup and possibly another XCD, through device memory: it publishes its
+// progress word (P.rowprog[y], epoch << 32 | progress) with an agent-scope release every
+// BAND_PUBLISH_EVERY MBs and at the row end; the reader polls it relaxed and then
+// acquires at agent scope.  Work items are (picture, band) in band-major order and a
+// workgroup takes its item by ticket when it starts, so the band above always belongs to
+// a workgroup that started earlier and never waits on a later one.
+constexpr int BAND_PUBLISH_EVERY = 8;
+
+// Takes the workgroup's ticket (into *slot) and clears `n` LDS counters.
+__device__ __forceinline__ uint32_t band_ticket(DevCtl *ctl, uint32_t *slot, uint32_t *ctr, int n) {
+    if (threadIdx.x == 0) *slot = atomicAdd(&ctl->ticket, 1u);
+    if ((int)threadIdx.x < n) ctr[threadIdx.x] = 0;
     __syncthreads();
-    RowSyncLds sync{prog, err};
-    const uint32_t *off = P.irow, *col = P.irow + H + 1;
-    for (int y = w; y < H; y += nw) {
-        const uint32_t b = off[y], e = off[y + 1];
-        sync.post(y, b < e ? (int)col[b] : W);
-        for (uint32_t i = b; i < e; i++) {
-            const int x = (int)col[i], addr = y * W + x;
-            if (y > 0) sync.wait(y - 1, min(x + 2, W));
-            int deps[4];
-            const Avail av = intra_avail(P, addr, deps);
-            intra_mb(P, addr, &tiles[w], av);
-            sync.post(y, i + 1 < e ? (int)col[i + 1] : W);
+    return uniform(*slot);
+}
+// Every wave of the workgroup is done: the launch's last workgroup resets the control
+// block and advances the epoch (see retire).
+__device__ __forceinline__ void band_retire(DevCtl *ctl, uint32_t *epoch, uint32_t cur) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t d = atomicAdd(&ctl->done, 1u);
+        if (d == gridDim.x - 1) {
+            __hip_atomic_store(&ctl->ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&ctl->done, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(epoch, cur + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
     }
+}
+// Progress of a band's last row for the next band (every lane stores the same word, so
+// each lane's earlier stores are released).
+__device__ __forceinline__ void band_publish(uint64_t *w, uint64_t tag, uint32_t v) {
+    wave_sync();
+    __hip_atomic_store(w, tag | v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+// Waits until the previous band's last row published at least `need`; `seen` caches the
+// last value read, so most calls return without touching memory.
+__device__ void band_wait(const uint64_t *w, uint64_t tag, uint32_t need, uint32_t &seen, uint32_t *err) {
+    if (seen >= need) return;
+    for (uint32_t it = 0;; ++it) {
+        const uint64_t v = __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t lo = uniform((uint32_t)v), hi = uniform((uint32_t)(v >> 32));
+        if (((uint64_t)hi << 32 | lo) >= (tag | need)) {
+            seen = lo;
+            break;
+        }
+        if (it >= SPIN_LIMIT) {
+            atomicOr(err, DEVERR_TIMEOUT);
+            break;
+        }
+        if ((it & 255u) == 255u && uniform(__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) != 0u) break;
+        __builtin_amdgcn_s_sleep(1);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+}
+
+// Intra stage over bands: the wave of row y walks the row's intra MBs (P.irow: row
+// offsets, then columns).  Row progress p means "every intra MB of the row left of
+// column p is done"; MB (x, y) needs the row above at min(x + 2, W) (neighbours B, C, D).
+// Neighbour samples pass through the picture in device memory.
+constexpr int INTRA_WG_MAX_WAVES = 8;
+__global__ __launch_bounds__(WAVE * INTRA_WG_MAX_WAVES) void k_intra_band(const DevPic *__restrict__ pics,
+                                                                          const uint32_t *__restrict__ items, uint32_t n,
+                                                                          DevCtl *ctl, uint32_t *epoch_p, uint32_t *err) {
+    __shared__ WaveIntra tiles[INTRA_WG_MAX_WAVES];
+    __shared__ uint32_t prog[INTRA_WG_MAX_WAVES];
+    __shared__ uint32_t s_ticket;
+    const uint32_t epoch = launch_epoch(epoch_p);
+    const uint64_t tag = (uint64_t)epoch << 32;
+    const int nw = blockDim.x / WAVE, w = (int)uniform(threadIdx.x / WAVE);
+    const uint32_t t = band_ticket(ctl, &s_ticket, prog, nw);
+    if (t < n) {
+        const uint32_t it = items[t];
+        const DevPic &P = pics[it >> WI_PIC_SHIFT];
+        const int W = P.W, H = P.H, y0 = (int)(it & WI_MB_MASK) * nw, y = y0 + w, y1 = min(H, y0 + nw);
+        if (P.n_intra != 0 && y < H) {
+            const bool from_g = y > 0 && y == y0, to_g = y < H - 1 && y == y1 - 1;
+            RowSyncLds sync{prog, err};
+            uint32_t seen = 0;
+            const uint32_t *off = P.irow, *col = P.irow + H + 1;
+            const uint32_t b = uniform(off[y]), e = uniform(off[y + 1]);
+            auto post = [&](uint32_t done, bool force) {
+                if (to_g) {
+                    if (force) band_publish(&P.rowprog[y], tag, done);
+                } else if (y < H - 1) {
+                    sync.post(y - y0, (int)done);
+                }
+            };
+            post(b < e ? uniform(col[b]) : (uint32_t)W, true);
+            for (uint32_t i = b; i < e; i++) {
+                const int x = (int)uniform(col[i]), addr = y * W + x;
+                if (from_g) band_wait(&P.rowprog[y - 1], tag, (uint32_t)min(x + 2, W), seen, err);
+                else if (y > 0) sync.wait(y - 1 - y0, min(x + 2, W));
+                int deps[4];
+                const Avail av = intra_avail(P, addr, deps);
+                intra_mb(P, addr, &tiles[w], av);
+                const bool end = i + 1 == e;
+                post(end ? (uint32_t)W : uniform(col[i + 1]), end || (i - b) % BAND_PUBLISH_EVERY == BAND_PUBLISH_EVERY - 1);
+            }
+        }
+    }
+    band_retire(ctl, epoch_p, epoch);
 }
 
 __global__ __launch_bounds__(WAVE) void k_deblock(const DevPic *__restrict__ pics, const uint32_t *__restrict__ items, uint32_t n,
@@ -1271,12 +1347,16 @@ __global__ __launch_bounds__(WAVE) void k_deblock_rows(const DevPic *__restrict_
     retire(ctl, gridDim.x, epoch_p, epoch);
 }
 
-// ---- per-picture workgroup deblocking with LDS strips
+// ---- deblocking over bands with LDS strips
 // Row y's wave hands the bottom 4 luma / chroma rows of each MB (the p samples of the
-// next row's top edge) to row y + 1's wave through a ring of LDS slots, so no sample
-// travels through device memory between waves.  Each sample is written to the picture
-// by exactly one wave: row y stores rows 0..12 (chroma 0..4) of its MBs, row y + 1 the
-// rows 13..15 (chroma 5..7) after its top-edge filtering (the last row stores all).
+// next row's top edge) to row y + 1's wave as a strip: through a ring of STRIP_SLOTS LDS
+// slots inside a band, through a full-width strip array in device memory (P.gstrip, one
+// row of W strips per band boundary) from a band's last row to the next band.  No sample
+// otherwise travels between waves: each sample is written to the picture by exactly one
+// wave, row y storing rows 0..12 (chroma 0..4) of its MBs and row y + 1 the rows 13..15
+// (chroma 5..7) after its top-edge filtering (the picture's last row stores all).  Row y
+// writes ring slot x mod STRIP_SLOTS only after row y + 1 has read strip x - STRIP_SLOTS;
+// every row of a band has its own wave, so the reader is always running.
 constexpr int STRIP_SLOTS = 8;
 struct alignas(16) StripSlot {
     uint8_t y[4][16];         // luma rows 12..15
@@ -1284,10 +1364,10 @@ struct alignas(16) StripSlot {
 };
 struct DbkWgShared {
     WaveDbk tiles[DBK_WG_MAX_WAVES];
-    StripSlot ring[DBK_WG_MAX_WAVES][STRIP_SLOTS];
+    StripSlot ring[DBK_WG_MAX_WAVES][STRIP_SLOTS];   // ring of local row r, read by row r + 1
     alignas(16) uint8_t bsrec[DBK_WG_MAX_WAVES][32];
-    uint32_t prog[DBK_WG_MAX_ROWS];   // row y: strips of MBs 0 .. prog - 1 complete
-    uint32_t cons[DBK_WG_MAX_ROWS];   // row y: strips of MBs 0 .. cons - 1 read by row y + 1
+    uint32_t prog[DBK_WG_MAX_WAVES];   // local row r: strips of MBs 0 .. prog - 1 complete
+    uint32_t cons[DBK_WG_MAX_WAVES];   // local row r: strips of MBs 0 .. cons - 1 read by row r + 1
 };
 
 __device__ __forceinline__ void lds_release() {
@@ -1359,7 +1439,7 @@ __device__ void dbk_store_wg(const DbkPlanes &d, int mbx, int mby, bool left, bo
 // Strip parts: A = MB x's columns 0..12 (chroma 0..4) from the tile's current MB;
 // B = columns 13..15 (chroma 5..7) of the MB whose right columns sit at tile column tc
 // (1 for MB x - 1 after its right neighbour's left edge, 17 for the last MB of the row).
-__device__ void strip_put(StripSlot *sl, const WaveDbk *wd, bool part_a, int tc) {
+__device__ __forceinline__ void strip_put(StripSlot *sl, const WaveDbk *wd, bool part_a, int tc) {
     const int lane = wave_lane();
     for (int i = lane; i < 128; i += WAVE) {
         if (i < 64) {
@@ -1372,7 +1452,7 @@ __device__ void strip_put(StripSlot *sl, const WaveDbk *wd, bool part_a, int tc)
     }
 }
 // The top neighbour's bottom rows into the tile (rows -4..-1 above the MB).
-__device__ void strip_get(const StripSlot *sl, WaveDbk *wd) {
+__device__ __forceinline__ void strip_get(const StripSlot *sl, WaveDbk *wd) {
     const int lane = wave_lane();
     if (lane < 16) *(uint32_t *)&wd->L[lane >> 2][4 + (lane & 3) * 4] = *(const uint32_t *)&sl->y[lane >> 2][(lane & 3) * 4];
     else if (lane < 32) {
@@ -1381,17 +1461,20 @@ __device__ void strip_get(const StripSlot *sl, WaveDbk *wd) {
     }
 }
 
-// One MB row of a picture (wave w of nw, row y); see the strip scheme above.
-__device__ void deblock_row_wg(const DevPic &P, int y, int nw, int w, DbkWgShared &sh, uint32_t *err) {
-    const int W = P.W, H = P.H;
+// MB row y of the band starting at row y0 (band index b, rows y0 .. y1 - 1) on wave w.
+__device__ void deblock_row_band(const DevPic &P, int y, int y0, int y1, int b, int w, DbkWgShared &sh, uint64_t tag,
+                                 uint32_t *err) {
+    const int W = P.W, H = P.H, ly = y - y0;
     const DbkPlanes d = dbk_planes(P);
     const bool top = y > 0, last = y == H - 1;
+    const bool from_g = top && y == y0, to_g = !last && y == y1 - 1;
     WaveDbk *wd = &sh.tiles[w];
     uint8_t *bsrec = sh.bsrec[w];
-    StripSlot *ring = sh.ring[w];
-    const StripSlot *above = top ? sh.ring[(y - 1) % nw] : nullptr;
-    // this wave's ring held the strips of row y - nw: row y - nw + 1 must have read them all
-    if (!last && y >= nw) lds_wait(&sh.cons[y - nw], (uint32_t)W, err);
+    StripSlot *ring = sh.ring[ly];
+    const StripSlot *above = ly > 0 ? sh.ring[ly - 1] : nullptr;
+    StripSlot *gout = (StripSlot *)P.gstrip + (size_t)b * W;              // this band -> next
+    const StripSlot *gin = (const StripSlot *)P.gstrip + (size_t)(b - 1) * W;  // previous band -> this
+    uint32_t seen = 0;
     MbView Lv, Qv, Tv;
     DbkPrefetch f = dbk_fetch(P, d, 0, y);
     for (int x = 0; x < W; x++) {
@@ -1403,34 +1486,53 @@ __device__ void deblock_row_wg(const DevPic &P, int y, int nw, int w, DbkWgShare
         load_view(P, addr, Qv);
         if (top) {
             load_view(P, addr - W, Tv);
-            lds_wait(&sh.prog[y - 1], (uint32_t)x + 1, err);
-            strip_get(&above[x % STRIP_SLOTS], wd);
-            lds_post(&sh.cons[y - 1], (uint32_t)x + 1);
+            if (from_g) {
+                band_wait(&P.rowprog[y - 1], tag, (uint32_t)x + 1, seen, err);
+                strip_get(&gin[x], wd);
+            } else {
+                lds_wait(&sh.prog[ly - 1], (uint32_t)x + 1, err);
+                strip_get(&above[x % STRIP_SLOTS], wd);
+                lds_post(&sh.cons[ly - 1], (uint32_t)x + 1);
+            }
         }
         wave_sync();
         dbk_filter(P, addr, Qv, Lv, Tv, left, top, bsrec, wd);
         dbk_store_wg(d, x, y, left, top, last, rowend, wd);
-        if (!last) {
-            if (x >= STRIP_SLOTS) lds_wait(&sh.cons[y], (uint32_t)(x - STRIP_SLOTS + 1), err);
+        if (to_g) {
+            strip_put(&gout[x], wd, true, 0);
+            if (left) strip_put(&gout[x - 1], wd, false, 1);
+            if (rowend) strip_put(&gout[x], wd, false, 17);
+            if (rowend || x % BAND_PUBLISH_EVERY == BAND_PUBLISH_EVERY - 1)
+                band_publish(&P.rowprog[y], tag, rowend ? (uint32_t)W : (uint32_t)x);
+        } else if (!last) {
+            if (x >= STRIP_SLOTS) lds_wait(&sh.cons[ly], (uint32_t)(x - STRIP_SLOTS + 1), err);
             strip_put(&ring[x % STRIP_SLOTS], wd, true, 0);
             if (left) strip_put(&ring[(x - 1) % STRIP_SLOTS], wd, false, 1);
             if (rowend) strip_put(&ring[x % STRIP_SLOTS], wd, false, 17);
-            lds_post(&sh.prog[y], rowend ? (uint32_t)W : (uint32_t)x);
+            lds_post(&sh.prog[ly], rowend ? (uint32_t)W : (uint32_t)x);
         }
         dbk_carry_left(wd);
     }
 }
 
-// One workgroup per picture: its waves take the picture's MB rows in turn (row y on
-// wave y mod waves, at least 2 waves).
-__global__ __launch_bounds__(WAVE * DBK_WG_MAX_WAVES) void k_deblock_pic(const DevPic *__restrict__ pics, uint32_t *err) {
+// One workgroup per band of `waves` MB rows (row y0 + w on wave w).
+__global__ __launch_bounds__(WAVE * DBK_WG_MAX_WAVES) void k_deblock_band(const DevPic *__restrict__ pics,
+                                                                          const uint32_t *__restrict__ items, uint32_t n,
+                                                                          DevCtl *ctl, uint32_t *epoch_p, uint32_t *err) {
     __shared__ DbkWgShared sh;
-    const DevPic &P = pics[blockIdx.x];
-    if (P.dbk_idc == 1) return;
-    const int nw = blockDim.x / WAVE, w = threadIdx.x / WAVE;
-    for (int i = threadIdx.x; i < P.H; i += blockDim.x) sh.prog[i] = sh.cons[i] = 0;
-    __syncthreads();
-    for (int y = w; y < P.H; y += nw) deblock_row_wg(P, y, nw, w, sh, err);
+    __shared__ uint32_t s_ticket;
+    const uint32_t epoch = launch_epoch(epoch_p);
+    const uint64_t tag = (uint64_t)epoch << 32;
+    const int nw = blockDim.x / WAVE, w = (int)uniform(threadIdx.x / WAVE);
+    if ((int)threadIdx.x < nw) sh.cons[threadIdx.x] = 0;
+    const uint32_t t = band_ticket(ctl, &s_ticket, sh.prog, nw);
+    if (t < n) {
+        const uint32_t it = items[t];
+        const DevPic &P = pics[it >> WI_PIC_SHIFT];
+        const int b = (int)(it & WI_MB_MASK), y0 = b * nw, y = y0 + w;
+        if (P.dbk_idc != 1 && y < P.H) deblock_row_band(P, y, y0, min(P.H, y0 + nw), b, w, sh, tag, err);
+    }
+    band_retire(ctl, epoch_p, epoch);
 }
 
 // Boundary strengths of every edge segment of every MB (8.7.2.1, Deblock.cpp:994-1310),
@@ -1520,17 +1622,19 @@ int h264g_launch_deblock_rows(const DevPic *pics, const uint32_t *items, uint32_
     return (int)hipGetLastError();
 }
 
-int h264g_launch_intra_pic(const DevPic *pics, uint32_t n_pics, int waves, uint32_t *err, void *stream) {
-    if (n_pics == 0) return 0;
+int h264g_launch_intra_band(const DevPic *pics, const uint32_t *bands, uint32_t n, DevCtl *ctl, uint32_t *epoch,
+                            int waves, uint32_t *err, void *stream) {
+    if (n == 0) return 0;
     if (waves < 1 || waves > INTRA_WG_MAX_WAVES) return (int)hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_intra_pic, dim3(n_pics), dim3(WAVE * waves), 0, (hipStream_t)stream, pics, err);
+    hipLaunchKernelGGL(k_intra_band, dim3(n), dim3(WAVE * waves), 0, (hipStream_t)stream, pics, bands, n, ctl, epoch, err);
     return (int)hipGetLastError();
 }
 
-int h264g_launch_deblock_pic(const DevPic *pics, uint32_t n_pics, int waves, uint32_t *err, void *stream) {
-    if (n_pics == 0) return 0;
-    if (waves < 2 || waves > DBK_WG_MAX_WAVES) return (int)hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_deblock_pic, dim3(n_pics), dim3(WAVE * waves), 0, (hipStream_t)stream, pics, err);
+int h264g_launch_deblock_band(const DevPic *pics, const uint32_t *bands, uint32_t n, DevCtl *ctl, uint32_t *epoch,
+                              int waves, uint32_t *err, void *stream) {
+    if (n == 0) return 0;
+    if (waves < 1 || waves > DBK_WG_MAX_WAVES) return (int)hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_deblock_band, dim3(n), dim3(WAVE * waves), 0, (hipStream_t)stream, pics, bands, n, ctl, epoch, err);
     return (int)hipGetLastError();
 }
 

@@ -143,26 +143,30 @@ typedef struct h264g_config {
                                      batch's stage sequence as a captured hipGraph
                                      (profiled contexts always launch directly)       */
     int32_t  deblock_schedule;    /* H264G_DEBLOCK_*                                  */
-    int32_t  deblock_waves;       /* waves per workgroup of H264G_DEBLOCK_PICTURE_WG,
-                                     2..16 (0 = 16)                                  */
+    int32_t  deblock_waves;       /* rows per band = waves per workgroup of
+                                     H264G_DEBLOCK_BANDS, 1..16 (0 = 16)             */
     int32_t  intra_schedule;      /* H264G_INTRA_*                                    */
-    int32_t  intra_waves;         /* waves per workgroup of H264G_INTRA_PICTURE_WG,
-                                     1..8 (0 = 8)                                    */
+    int32_t  intra_waves;         /* rows per band = waves per workgroup of
+                                     H264G_INTRA_BANDS, 1..8 (0 = 8)                 */
 } h264g_config;
 
-/* Intra schedules: PICTURE_WG = one workgroup per picture, waves walk the intra MBs of
- * MB rows in turn with LDS row progress (default); MB_WAVEFRONT = persistent waves over
- * the intra MBs of all pictures ordered by t = x + 2y, progress words in device memory. */
-#define H264G_INTRA_PICTURE_WG   0
+/* Intra schedules: BANDS = pictures cut into bands of `intra_waves` MB rows, one
+ * workgroup per band and one wave per row walking the row's intra MBs, LDS row progress
+ * inside a band and device-memory progress between bands (default); MB_WAVEFRONT =
+ * persistent waves over the intra MBs of all pictures ordered by t = x + 2y, progress
+ * words in device memory. */
+#define H264G_INTRA_BANDS        0
 #define H264G_INTRA_MB_WAVEFRONT 1
 
 /* Deblocking schedules (all give the raster-order result of 8.7):
- * PICTURE_WG   one workgroup per picture; its waves take MB rows in turn and hand each
- *              MB's bottom rows to the next row's wave through LDS (default)
+ * BANDS        pictures cut into bands of `deblock_waves` MB rows, one workgroup per
+ *              band and one wave per row; each MB's bottom rows go to the next row's
+ *              wave through LDS inside a band and through device memory between bands
+ *              (default)
  * MB_WAVEFRONT persistent waves over single MBs ordered by t = x + 2y, progress words
  *              in device memory
  * MB_ROWS      persistent waves over MB rows of all pictures, progress in device memory */
-#define H264G_DEBLOCK_PICTURE_WG   0
+#define H264G_DEBLOCK_BANDS        0
 #define H264G_DEBLOCK_MB_WAVEFRONT 1
 #define H264G_DEBLOCK_MB_ROWS      2
 

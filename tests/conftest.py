@@ -41,16 +41,19 @@ def engines():
         e.close()
 
 
-@pytest.fixture(params=["picture_wg", "mb_wavefront", "mb_rows", "picture_wg_small"])
+@pytest.fixture(params=["bands", "mb_wavefront", "mb_rows", "bands_small", "bands_single_row"])
 def any_engine(request, engines):
-    """Every intra / deblocking schedule, and small per-picture workgroups."""
+    """Every intra / deblocking schedule; bands of 3 / 2 rows and of single rows (every row
+    a band boundary)."""
     from h264_video_decoder_demo_amd import Engine
     from h264_video_decoder_demo_amd import _lib
-    kw = {"picture_wg": dict(deblock_schedule=_lib.DEBLOCK_PICTURE_WG, intra_schedule=_lib.INTRA_PICTURE_WG),
+    kw = {"bands": dict(deblock_schedule=_lib.DEBLOCK_BANDS, intra_schedule=_lib.INTRA_BANDS),
           "mb_wavefront": dict(deblock_schedule=_lib.DEBLOCK_MB_WAVEFRONT, intra_schedule=_lib.INTRA_MB_WAVEFRONT),
           "mb_rows": dict(deblock_schedule=_lib.DEBLOCK_MB_ROWS, intra_schedule=_lib.INTRA_MB_WAVEFRONT),
-          "picture_wg_small": dict(deblock_schedule=_lib.DEBLOCK_PICTURE_WG, deblock_waves=3,
-                                   intra_schedule=_lib.INTRA_PICTURE_WG, intra_waves=2)}[request.param]
+          "bands_small": dict(deblock_schedule=_lib.DEBLOCK_BANDS, deblock_waves=3,
+                              intra_schedule=_lib.INTRA_BANDS, intra_waves=2),
+          "bands_single_row": dict(deblock_schedule=_lib.DEBLOCK_BANDS, deblock_waves=1,
+                                   intra_schedule=_lib.INTRA_BANDS, intra_waves=1)}[request.param]
     if request.param not in engines:
         engines[request.param] = Engine(device=0, **kw)
     return engines[request.param]

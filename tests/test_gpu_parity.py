@@ -11,6 +11,8 @@ import ctypes
 import numpy as np
 import pytest
 
+from h264_video_decoder_demo_amd import Generator
+
 from helpers import first_diff, make_generator, oracle_decode, plane_hashes
 
 pytestmark = pytest.mark.gpu
@@ -165,5 +167,27 @@ def test_bgr24_matches_reference(engine, golden, name):
             flipped = engine.to_bgr24(sid, desc.dst_slot, flip=True)
             np.testing.assert_array_equal(flipped[::-1], bgr)
             engine.release(pic)
+    finally:
+        engine.close_stream(sid)
+
+
+@pytest.mark.parametrize("width_mbs", [240, 330, 340])
+def test_wide_pictures_match_oracle(any_engine, width_mbs):
+    """Pictures as wide as 2160p and wider: the per-picture deblocking sizes its LDS strip
+    rings by the width (330 MBs is the widest that fits 16 waves) and hands wider
+    pictures to the row-pipelined kernel; every schedule matches the oracle."""
+    engine = any_engine
+    gen = Generator(2, width_mbs=width_mbs, height_mbs=5, n_frames=3, seed=11 + width_mbs)
+    want = {i: plane_hashes(*out[2]) for i, _, out in oracle_decode(gen)}
+    sid = engine.open_stream(gen.width_mbs, gen.height_mbs, gen.dpb_slots)
+    try:
+        for i in range(gen.num_pictures):
+            desc, mbs, levels = gen.picture(i)
+            pic = engine.begin(sid, desc, mbs, levels)
+            engine.submit(pic)
+            engine.sync()
+            got = engine.readback(sid, desc.dst_slot)
+            engine.release(pic)
+            assert plane_hashes(*got) == want[i], f"width {width_mbs} picture {i}"
     finally:
         engine.close_stream(sid)
