@@ -885,15 +885,32 @@ __device__ __forceinline__ int bs_luma(const DevPic &P, int ap, const MbView &Pv
     return 0;
 }
 
-// 8.7.2.2 - 8.7.2.4 on samples p[0..3] (p0 first) and q[0..3] (Deblock.cpp:1314-1522).
-__device__ __forceinline__ void filter_line(int p[4], int q[4], int bS, bool chroma, int qavg, int offA, int offB) {
-    int indexA = clip3(0, 51, qavg + offA), indexB = clip3(0, 51, qavg + offB);
-    int alpha = c_alpha[indexA], beta = c_beta[indexB];
+// Tables 8-16 (alpha', beta') and 8-17 (tC0), copied into LDS by every deblocking
+// workgroup: lookups with lane-varying indices then cost an LDS read, not a vector
+// memory access.
+struct DbkTables {
+    uint8_t alpha[52], beta[52], tc0[3][52];
+};
+__device__ void dbk_tables_init(DbkTables *t) {
+    for (int i = threadIdx.x; i < 52; i += blockDim.x) {
+        t->alpha[i] = c_alpha[i];
+        t->beta[i] = c_beta[i];
+        t->tc0[0][i] = c_tc0[0][i];
+        t->tc0[1][i] = c_tc0[1][i];
+        t->tc0[2][i] = c_tc0[2][i];
+    }
+}
+
+// 8.7.2.2 - 8.7.2.4 on samples p[0..3] (p0 first) and q[0..3] (Deblock.cpp:1314-1522);
+// indexA / indexB = Clip3(0, 51, qPav + filterOffsetA / B).
+__device__ __forceinline__ void filter_line(int p[4], int q[4], int bS, bool chroma, int indexA, int indexB,
+                                            const DbkTables *tb) {
+    int alpha = tb->alpha[indexA], beta = tb->beta[indexB];
     int p0 = p[0], p1 = p[1], p2 = p[2], p3 = p[3], q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3];
     if (!(bS != 0 && iabs(p0 - q0) < alpha && iabs(p1 - p0) < beta && iabs(q1 - q0) < beta)) return;
     int ap = iabs(p2 - p0), aq = iabs(q2 - q0);
     if (bS < 4) {
-        int tc0 = c_tc0[bS - 1][indexA];
+        int tc0 = tb->tc0[bS - 1][indexA];
         int tc = chroma ? tc0 + 1 : tc0 + (ap < beta) + (aq < beta);
         int delta = clip3(-tc, tc, (((q0 - p0) << 2) + (p1 - q1) + 4) >> 3);
         p[0] = clip1(p0 + delta);
@@ -959,62 +976,100 @@ __device__ void dbk_load(const DbkPlanes &d, int mbx, int mby, int region, WaveD
     }
 }
 
-// One edge step: the edge at position e (0, 4, 8, 12 luma samples) in direction DIR
-// (0 = vertical edge, 1 = horizontal), luma in lanes 0-15, Cb in 16-23, Cr in 32-39.
-// Pv is the view of the MB on the p side (the left / top MB for e = 0, else the MB itself).
-template <int DIR>
-__device__ __forceinline__ void dbk_edge(const DevPic &P, int addr, int e, int ap, const MbView &Pv, const MbView &Qv,
-                                         const uint8_t *bsrec, WaveDbk *wd) {
-    const int lane = wave_lane();
-    const int plane = lane < 16 ? 0 : (lane < 24 ? 1 : (lane >= 32 && lane < 40 ? 2 : -1));
-    bool active = plane >= 0;
-    if (plane == 0 && ((e & 4) && (Qv.kind & AUX_T8))) active = false;  // internal 4 / 12 with 8x8 transform
-    if (plane > 0 && e >= 8) active = false;                             // chroma edges 0 and 4 only
-    if (!active) return;
-    const int k = plane == 0 ? lane : (plane == 1 ? lane - 16 : lane - 32);
-    const int n = plane == 0 ? 16 : 8, sc = plane == 0 ? 1 : 2;
-    const int xq = DIR == 0 ? e : k, yq = DIR == 0 ? k : e;
-    const int xp = DIR == 0 ? (e ? e - 1 : n - 1) : k, yp = DIR == 0 ? k : (e ? e - 1 : n - 1);
-    // boundary strength of the luma segment this line maps to (chroma line k -> luma 2k,
-    // chroma edge e -> luma edge 2e), precomputed by k_bs
-    const int el = plane == 0 ? e : 2 * e, seg = plane == 0 ? (k >> 2) : (k >> 1);
-    const int bS = bsrec[DIR * 16 + (el >> 2) * 4 + seg];
-    (void)ap; (void)xp; (void)yp; (void)xq; (void)yq; (void)sc;
-    const int qavg = (view_qp(Pv, plane) + view_qp(Qv, plane) + 1) >> 1;
-    uint8_t *base;
-    int stride;
-    if (plane == 0) { base = &wd->L[4][4]; stride = 20; }
-    else { base = &wd->C[plane - 1][4][4]; stride = 12; }
-    // sample i on the q side / p side of the line k
-    const int step = DIR == 0 ? 1 : stride, line = DIR == 0 ? k * stride : k;
-    uint8_t *q0 = base + line + e * step;
-    int pv[4], qv[4];
-#pragma unroll
-    for (int i = 0; i < 4; i++) { qv[i] = q0[i * step]; pv[i] = q0[-(i + 1) * step]; }
-    if (P.dbk_idc != 1) filter_line(pv, qv, bS, plane > 0, qavg, P.offA, P.offB);
-#pragma unroll
-    for (int i = 0; i < 3; i++) { q0[i * step] = (uint8_t)qv[i]; q0[-(i + 1) * step] = (uint8_t)pv[i]; }
+// Edge J of a line held as 20 bytes in w[0..4] (little-endian; the edge lies between
+// words J and J + 1); chroma lines hold 12 bytes and use J = 0, 1 (luma edges 0 and 8
+// for the bS).
+template <int J>
+__device__ __forceinline__ void dbk_line_edge(uint32_t (&w)[5], bool act, bool luma, int seg, const uint8_t *bsdir,
+                                              int iA, int iB, const DbkTables *tb) {
+    if (!act) return;
+    const int bS = bsdir[(luma ? J : 2 * J) * 4 + seg];
+    const uint32_t a = w[J], b = w[J + 1];
+    int pp[4] = {(int)(a >> 24), (int)((a >> 16) & 255u), (int)((a >> 8) & 255u), (int)(a & 255u)};
+    int qq[4] = {(int)(b & 255u), (int)((b >> 8) & 255u), (int)((b >> 16) & 255u), (int)(b >> 24)};
+    filter_line(pp, qq, bS, !luma, iA, iB, tb);
+    w[J] = (a & 255u) | ((uint32_t)pp[2] << 8) | ((uint32_t)pp[1] << 16) | ((uint32_t)pp[0] << 24);
+    w[J + 1] = (uint32_t)qq[0] | ((uint32_t)qq[1] << 8) | ((uint32_t)qq[2] << 16) | (b & 0xff000000u);
 }
 
+// The edges of one direction (DIR 0: vertical edges, 1: horizontal) of the MB in the
+// tile.  Lanes 0-15 take luma, 16-23 Cb, 24-31 Cr; lane k of a plane owns line k of the
+// MB (a row for vertical edges, a column for horizontal ones) with the 4 samples before
+// it, keeps the line in registers and runs its edges in order: left / top MB edge (when
+// `edge`), then the internal ones (luma 4 / 12 skipped with the 8x8 transform, chroma
+// only 4).  Lines of one direction touch disjoint samples, so this gives the reference's
+// edge-by-edge order (8.7, Deblock.cpp:76-635).  Ev is the view of the left / top MB.
 template <int DIR>
-__device__ __forceinline__ void dbk_pass(const DevPic &P, int addr, int ap_edge, const MbView &Ev, bool edge, const MbView &Qv,
-                                         const uint8_t *bsrec, WaveDbk *wd) {
-    if (edge) dbk_edge<DIR>(P, addr, 0, ap_edge, Ev, Qv, bsrec, wd);
-    wave_sync();
-    for (int e = 4; e < 16; e += 4) {
-        dbk_edge<DIR>(P, addr, e, addr, Qv, Qv, bsrec, wd);
-        wave_sync();
+__device__ __forceinline__ void dbk_lines(const DevPic &P, const MbView &Ev, bool edge, const MbView &Qv,
+                                          const uint8_t *bsrec, WaveDbk *wd, const DbkTables *tb) {
+    const int lane = wave_lane();
+    if (lane >= 32) return;
+    const int plane = lane < 16 ? 0 : (lane < 24 ? 1 : 2);
+    const bool luma = plane == 0;
+    const int k = luma ? lane : (lane & 7);
+    uint8_t *base = luma ? &wd->L[0][0] : &wd->C[plane - 1][0][0];
+    const int stride = luma ? 20 : 12;
+    uint32_t w[5];
+    if (DIR == 0) {
+        const uint32_t *row = (const uint32_t *)(base + (k + 4) * stride);
+        w[0] = row[0];
+        w[1] = row[1];
+        w[2] = row[2];
+        w[3] = luma ? row[3] : 0u;
+        w[4] = luma ? row[4] : 0u;
+    } else {
+        const uint8_t *c = base + k + 4;
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+            w[i] = (uint32_t)c[(4 * i) * stride] | ((uint32_t)c[(4 * i + 1) * stride] << 8) |
+                   ((uint32_t)c[(4 * i + 2) * stride] << 16) | ((uint32_t)c[(4 * i + 3) * stride] << 24);
+#pragma unroll
+        for (int i = 3; i < 5; i++)
+            w[i] = luma ? ((uint32_t)c[(4 * i) * stride] | ((uint32_t)c[(4 * i + 1) * stride] << 8) |
+                           ((uint32_t)c[(4 * i + 2) * stride] << 16) | ((uint32_t)c[(4 * i + 3) * stride] << 24))
+                        : 0u;
+    }
+    const int qpq = view_qp(Qv, plane), qpe = edge ? view_qp(Ev, plane) : qpq;
+    const int qav = (qpe + qpq + 1) >> 1;  // internal edges: (qpq + qpq + 1) >> 1 = qpq
+    const int iA0 = clip3(0, 51, qav + P.offA), iB0 = clip3(0, 51, qav + P.offB);
+    const int iAi = clip3(0, 51, qpq + P.offA), iBi = clip3(0, 51, qpq + P.offB);
+    const bool t8 = Qv.kind & AUX_T8, on = P.dbk_idc != 1;
+    const int seg = luma ? (k >> 2) : (k >> 1);  // 4-line luma segment of the line
+    const uint8_t *bsdir = bsrec + DIR * 16;       // bS per [edge][segment], from k_bs
+    dbk_line_edge<0>(w, on && edge, luma, seg, bsdir, iA0, iB0, tb);
+    dbk_line_edge<1>(w, on && (luma ? !t8 : true), luma, seg, bsdir, iAi, iBi, tb);
+    dbk_line_edge<2>(w, on && luma, luma, seg, bsdir, iAi, iBi, tb);
+    dbk_line_edge<3>(w, on && luma && !t8, luma, seg, bsdir, iAi, iBi, tb);
+    if (DIR == 0) {
+        uint32_t *row = (uint32_t *)(base + (k + 4) * stride);
+        row[0] = w[0];
+        row[1] = w[1];
+        row[2] = w[2];
+        if (luma) {
+            row[3] = w[3];
+            row[4] = w[4];
+        }
+    } else {
+        uint8_t *c = base + k + 4;
+#pragma unroll
+        for (int i = 1; i < 11; i++) c[i * stride] = (uint8_t)(w[i >> 2] >> (8 * (i & 3)));
+        if (luma) {
+#pragma unroll
+            for (int i = 11; i < 19; i++) c[i * stride] = (uint8_t)(w[i >> 2] >> (8 * (i & 3)));
+        }
     }
 }
 
-// The 8 edge steps of one MB on the tile, in the reference's order (8.7, Deblock.cpp:76-635):
-// vertical edges left to right, then horizontal edges top to bottom.  Lv / Tv are the
-// left / top MB views (read only when present).
-// bsrec: the MB's 32 boundary strengths (device memory or an LDS copy).
-__device__ void dbk_filter(const DevPic &P, int addr, const MbView &Qv, const MbView &Lv, const MbView &Tv, bool left,
-                           bool top, const uint8_t *bsrec, WaveDbk *wd) {
-    dbk_pass<0>(P, addr, addr - 1, Lv, left, Qv, bsrec, wd);
-    dbk_pass<1>(P, addr, addr - P.W, Tv, top, Qv, bsrec, wd);
+// The MB's vertical edges, then its horizontal edges, on the tile (8.7, Deblock.cpp:76-635).
+// Lv / Tv are the left / top MB views (read only when present).  bsrec: the MB's 32
+// boundary strengths (device memory or an LDS copy).  The caller has synchronised the
+// tile; it is complete when this returns.
+__device__ __forceinline__ void dbk_filter(const DevPic &P, const MbView &Qv, const MbView &Lv, const MbView &Tv, bool left, bool top,
+                           const uint8_t *bsrec, WaveDbk *wd, const DbkTables *tb) {
+    dbk_lines<0>(P, Lv, left, Qv, bsrec, wd, tb);
+    wave_sync();
+    dbk_lines<1>(P, Tv, top, Qv, bsrec, wd, tb);
+    wave_sync();
 }
 
 // Writes back the MB and the strips its MB edges modified (3 columns / rows).
@@ -1034,7 +1089,7 @@ __device__ void dbk_store(const DbkPlanes &d, int mbx, int mby, bool left, bool 
 }
 
 // One MB with every neighbour sample loaded from the picture (per-MB wavefront).
-__device__ void deblock_mb(const DevPic &P, int addr, WaveDbk *wd) {
+__device__ void deblock_mb(const DevPic &P, int addr, WaveDbk *wd, const DbkTables *tb) {
     const int W = P.W;
     const int mbx = addr % W, mby = addr / W;
     const DbkPlanes d = dbk_planes(P);
@@ -1045,7 +1100,7 @@ __device__ void deblock_mb(const DevPic &P, int addr, WaveDbk *wd) {
     if (left) load_view(P, addr - 1, Lv);
     if (top) load_view(P, addr - W, Tv);
     wave_sync();
-    dbk_filter(P, addr, Qv, Lv, Tv, left, top, P.bs + (size_t)addr * 32, wd);
+    dbk_filter(P, Qv, Lv, Tv, left, top, P.bs + (size_t)addr * 32, wd, tb);
     dbk_store(d, mbx, mby, left, top, wd);
 }
 
@@ -1128,7 +1183,7 @@ struct RowSyncLds {
 // edge reads), i.e. the row above at progress min(x + 2, W).  The left neighbour's
 // columns stay in LDS from the previous step.
 template <class Sync>
-__device__ void deblock_row(const DevPic &P, int mby, Sync &sync, WaveDbk *wd) {
+__device__ void deblock_row(const DevPic &P, int mby, Sync &sync, WaveDbk *wd, const DbkTables *tb) {
     const int W = P.W;
     const DbkPlanes d = dbk_planes(P);
     const bool top = mby > 0;
@@ -1145,7 +1200,7 @@ __device__ void deblock_row(const DevPic &P, int mby, Sync &sync, WaveDbk *wd) {
             dbk_load(d, mbx, mby, 2, wd);
         }
         wave_sync();
-        dbk_filter(P, addr, Qv, Lv, Tv, left, top, P.bs + (size_t)addr * 32, wd);
+        dbk_filter(P, Qv, Lv, Tv, left, top, P.bs + (size_t)addr * 32, wd, tb);
         dbk_store(d, mbx, mby, left, top, wd);
         sync.post(mby, mbx + 1);
         dbk_carry_left(wd);
@@ -1314,6 +1369,9 @@ __global__ __launch_bounds__(WAVE * INTRA_WG_MAX_WAVES) void k_intra_band(const 
 __global__ __launch_bounds__(WAVE) void k_deblock(const DevPic *__restrict__ pics, const uint32_t *__restrict__ items, uint32_t n,
                                                   DevCtl *ctl, uint32_t *epoch_p, uint32_t *err) {
     __shared__ WaveDbk lds;
+    __shared__ DbkTables tb;
+    dbk_tables_init(&tb);
+    __syncthreads();
     const uint32_t epoch = launch_epoch(epoch_p);
     for (;;) {
         int t = next_ticket(ctl, n);
@@ -1323,7 +1381,7 @@ __global__ __launch_bounds__(WAVE) void k_deblock(const DevPic *__restrict__ pic
         const int addr = (int)(it & WI_MB_MASK), W = P.W, mbx = addr % W, mby = addr / W;
         int deps[2] = {mbx > 0 ? addr - 1 : -1, mby > 0 ? (mbx < W - 1 ? addr - W + 1 : addr - W) : -1};
         wait_deps(P.flags, deps, 2, epoch, err);
-        deblock_mb(P, addr, &lds);
+        deblock_mb(P, addr, &lds, &tb);
         publish(&P.flags[addr], epoch);
     }
     retire(ctl, gridDim.x, epoch_p, epoch);
@@ -1334,6 +1392,9 @@ __global__ __launch_bounds__(WAVE) void k_deblock(const DevPic *__restrict__ pic
 __global__ __launch_bounds__(WAVE) void k_deblock_rows(const DevPic *__restrict__ pics, const uint32_t *__restrict__ items,
                                                        uint32_t n, DevCtl *ctl, uint32_t *epoch_p, uint32_t *err) {
     __shared__ WaveDbk lds;
+    __shared__ DbkTables tb;
+    dbk_tables_init(&tb);
+    __syncthreads();
     const uint32_t epoch = launch_epoch(epoch_p);
     const uint64_t tag = (uint64_t)epoch << 32;
     for (;;) {
@@ -1342,7 +1403,7 @@ __global__ __launch_bounds__(WAVE) void k_deblock_rows(const DevPic *__restrict_
         const uint32_t it = items[t];
         const DevPic &P = pics[it >> WI_PIC_SHIFT];
         RowSyncGlobal sync{P.rowprog, tag, err};
-        deblock_row(P, (int)(it & WI_MB_MASK), sync, &lds);
+        deblock_row(P, (int)(it & WI_MB_MASK), sync, &lds, &tb);
     }
     retire(ctl, gridDim.x, epoch_p, epoch);
 }
@@ -1368,6 +1429,7 @@ struct DbkWgShared {
     alignas(16) uint8_t bsrec[DBK_WG_MAX_WAVES][32];
     uint32_t prog[DBK_WG_MAX_WAVES];   // local row r: strips of MBs 0 .. prog - 1 complete
     uint32_t cons[DBK_WG_MAX_WAVES];   // local row r: strips of MBs 0 .. cons - 1 read by row r + 1
+    DbkTables tb;
 };
 
 __device__ __forceinline__ void lds_release() {
@@ -1496,7 +1558,7 @@ __device__ void deblock_row_band(const DevPic &P, int y, int y0, int y1, int b, 
             }
         }
         wave_sync();
-        dbk_filter(P, addr, Qv, Lv, Tv, left, top, bsrec, wd);
+        dbk_filter(P, Qv, Lv, Tv, left, top, bsrec, wd, &sh.tb);
         dbk_store_wg(d, x, y, left, top, last, rowend, wd);
         if (to_g) {
             strip_put(&gout[x], wd, true, 0);
@@ -1525,6 +1587,7 @@ __global__ __launch_bounds__(WAVE * DBK_WG_MAX_WAVES) void k_deblock_band(const 
     const uint64_t tag = (uint64_t)epoch << 32;
     const int nw = blockDim.x / WAVE, w = (int)uniform(threadIdx.x / WAVE);
     if ((int)threadIdx.x < nw) sh.cons[threadIdx.x] = 0;
+    dbk_tables_init(&sh.tb);
     const uint32_t t = band_ticket(ctl, &s_ticket, sh.prog, nw);
     if (t < n) {
         const uint32_t it = items[t];
