@@ -535,80 +535,76 @@ struct WaveIntra {
     uint8_t T[17][25];
     uint8_t C[2][9][9];
     int16_t pf[25];        // filtered 8x8 reference samples p'[-1,-1..7] then p'[0..15,-1]
+    int16_t U[25];         // the same samples before filtering (-1: not available)
+    int16_t X[2][16];      // neighbour lines of two 4x4 blocks (see pred4x4)
 };
 
-struct Avail { int A, B, C, D; };
+// Availability of the neighbouring MBs A, B, C, D for intra prediction as bits 0..3 of
+// one word (a struct of four fields selected per lane would be indexed in scratch).
+struct Avail {
+    uint32_t m;
+    __device__ bool A() const { return m & 1u; }
+    __device__ bool B() const { return m & 2u; }
+    __device__ bool C() const { return m & 4u; }
+    __device__ bool D() const { return m & 8u; }
+};
 
 // Sample p[x, y] relative to the 4x4 / 8x8 block at (xO, yO) of the current MB, from the
 // tile, or -1 when not available for intra prediction (6.4.12.1, H264PictureBase.cpp:1099-1155).
 __device__ __forceinline__ int tile_p(const WaveIntra *wi, const Avail &av, int xO, int yO, int x, int y, int blk4) {
     int xx = xO + x, yy = yO + y;
     if (yy < 0) {
-        if (xx < 0) return av.D ? wi->T[0][0] : -1;
-        if (xx < 16) return av.B ? wi->T[0][xx + 1] : -1;
-        return av.C ? wi->T[0][xx + 1] : -1;
+        if (xx < 0) return av.D() ? wi->T[0][0] : -1;
+        if (xx < 16) return av.B() ? wi->T[0][xx + 1] : -1;
+        return av.C() ? wi->T[0][xx + 1] : -1;
     }
-    if (xx < 0) return av.A ? wi->T[yy + 1][0] : -1;
+    if (xx < 0) return av.A() ? wi->T[yy + 1][0] : -1;
     if (xx >= 16) return -1;
     if (blk4 >= 0 && x > 3 && (blk4 == 3 || blk4 == 11)) return -1;  // H264PictureBase.cpp:1131
     return wi->T[yy + 1][xx + 1];
 }
 
-// 8.3.1.2 for pixel (x, y) of the 4x4 block blk (H264PictureBase.cpp:1062-1419).
-__device__ int pred4x4(const WaveIntra *wi, const Avail &av, int blk, int mode, int x, int y) {
-    int xO = blk_x(blk), yO = blk_y(blk);
-    int top[8], left[4], tl = tile_p(wi, av, xO, yO, -1, -1, blk);
-#pragma unroll
-    for (int i = 0; i < 8; i++) top[i] = tile_p(wi, av, xO, yO, i, -1, blk);
-#pragma unroll
-    for (int i = 0; i < 4; i++) left[i] = tile_p(wi, av, xO, yO, -1, i, blk);
-    if (top[4] < 0 && top[5] < 0 && top[6] < 0 && top[7] < 0 && top[3] >= 0) top[4] = top[5] = top[6] = top[7] = top[3];
-    int ta = top[0] >= 0 && top[1] >= 0 && top[2] >= 0 && top[3] >= 0;
-    int la = left[0] >= 0 && left[1] >= 0 && left[2] >= 0 && left[3] >= 0;
-#define PT(i) ((i) < 0 ? tl : top[i])
-#define PL(i) ((i) < 0 ? tl : left[i])
+// 8.3.1.2 for pixel (x, y) of a 4x4 block (H264PictureBase.cpp:1062-1419), from the
+// block's neighbour line X[0..14] = L3 L3 L2 L1 L0 TL T0 .. T7 T7 (L = p[-1, y],
+// T = p[x, -1], TL = p[-1, -1]; T4..T7 already substituted by T3 when unavailable;
+// -1 = not available).  Every directional mode reads a 2-tap (X[i] + X[i+1] + 1) >> 1
+// or a 3-tap (X[c-1] + 2 X[c] + X[c+1] + 2) >> 2 of this line: the equations of
+// 8.3.1.2.4 - 8.3.1.2.9 with p[-1, y] and p[x, -1] mapped to their X positions (the
+// duplicated end samples give the corner cases, e.g. Diagonal_Down_Left at (3, 3) and
+// Horizontal_Up at zHU = 5).
+__device__ __forceinline__ int xtap2(const int16_t *X, int i) { return (X[i] + X[i + 1] + 1) >> 1; }
+__device__ __forceinline__ int xtap3(const int16_t *X, int c) { return (X[c - 1] + 2 * X[c] + X[c + 1] + 2) >> 2; }
+__device__ int pred4x4(const int16_t *X, int mode, int x, int y) {
     switch (mode) {
-    case 0: return top[x];
-    case 1: return left[y];
-    case 2:
-        if (ta && la) return (top[0] + top[1] + top[2] + top[3] + left[0] + left[1] + left[2] + left[3] + 4) >> 3;
-        if (la) return (left[0] + left[1] + left[2] + left[3] + 2) >> 2;
-        if (ta) return (top[0] + top[1] + top[2] + top[3] + 2) >> 2;
+    case 0: return X[6 + x];  // vertical
+    case 1: return X[4 - y];  // horizontal
+    case 2: {                 // DC: the top / left samples are available all or none
+        const bool ta = X[6] >= 0, la = X[4] >= 0;
+        const int st = X[6] + X[7] + X[8] + X[9], sl = X[1] + X[2] + X[3] + X[4];
+        if (ta && la) return (st + sl + 4) >> 3;
+        if (la) return (sl + 2) >> 2;
+        if (ta) return (st + 2) >> 2;
         return 128;
-    case 3:
-        if (x == 3 && y == 3) return (top[6] + 3 * top[7] + 2) >> 2;
-        return (top[x + y] + 2 * top[x + y + 1] + top[x + y + 2] + 2) >> 2;
-    case 4:
-        if (x > y) return (PT(x - y - 2) + 2 * PT(x - y - 1) + top[x - y] + 2) >> 2;
-        if (x < y) return (PL(y - x - 2) + 2 * PL(y - x - 1) + left[y - x] + 2) >> 2;
-        return (top[0] + 2 * tl + left[0] + 2) >> 2;
-    case 5: {
-        int z = 2 * x - y;
-        if (z >= 0 && !(z & 1)) return (PT(x - (y >> 1) - 1) + top[x - (y >> 1)] + 1) >> 1;
-        if (z >= 0) return (PT(x - (y >> 1) - 2) + 2 * PT(x - (y >> 1) - 1) + top[x - (y >> 1)] + 2) >> 2;
-        if (z == -1) return (left[0] + 2 * tl + top[0] + 2) >> 2;
-        return (PL(y - 1) + 2 * PL(y - 2) + PL(y - 3) + 2) >> 2;
     }
-    case 6: {
-        int z = 2 * y - x;
-        if (z >= 0 && !(z & 1)) return (PL(y - (x >> 1) - 1) + left[y - (x >> 1)] + 1) >> 1;
-        if (z >= 0) return (PL(y - (x >> 1) - 2) + 2 * PL(y - (x >> 1) - 1) + left[y - (x >> 1)] + 2) >> 2;
-        if (z == -1) return (left[0] + 2 * tl + top[0] + 2) >> 2;
-        return (PT(x - 1) + 2 * PT(x - 2) + PT(x - 3) + 2) >> 2;
+    case 3: return xtap3(X, 7 + x + y);  // diagonal down left
+    case 4: return xtap3(X, 5 + x - y);  // diagonal down right
+    case 5: {                            // vertical right
+        const int z = 2 * x - y;
+        if (z >= 0) return (z & 1) ? xtap3(X, 5 + x - (y >> 1)) : xtap2(X, 5 + x - (y >> 1));
+        return z == -1 ? xtap3(X, 5) : xtap3(X, 6 - y);
     }
-    case 7:
-        if (!(y & 1)) return (top[x + (y >> 1)] + top[x + (y >> 1) + 1] + 1) >> 1;
-        return (top[x + (y >> 1)] + 2 * top[x + (y >> 1) + 1] + top[x + (y >> 1) + 2] + 2) >> 2;
-    default: {
-        int z = x + 2 * y;
-        if (z < 5 && !(z & 1)) return (left[y + (x >> 1)] + left[y + (x >> 1) + 1] + 1) >> 1;
-        if (z < 5) return (left[y + (x >> 1)] + 2 * left[y + (x >> 1) + 1] + left[y + (x >> 1) + 2] + 2) >> 2;
-        if (z == 5) return (left[2] + 3 * left[3] + 2) >> 2;
-        return left[3];
+    case 6: {                            // horizontal down
+        const int z = 2 * y - x;
+        if (z >= 0) return (z & 1) ? xtap3(X, 5 - y + (x >> 1)) : xtap2(X, 4 - y + (x >> 1));
+        return z == -1 ? xtap3(X, 5) : xtap3(X, 4 + x);
+    }
+    case 7: return (y & 1) ? xtap3(X, 7 + x + (y >> 1)) : xtap2(X, 6 + x + (y >> 1));  // vertical left
+    default: {                                                                         // horizontal up
+        const int z = x + 2 * y;
+        if (z < 5) return (z & 1) ? xtap3(X, 3 - y - (x >> 1)) : xtap2(X, 3 - y - (x >> 1));
+        return z == 5 ? xtap3(X, 1) : X[1];
     }
     }
-#undef PT
-#undef PL
 }
 
 // 8.3.2.2 for pixel (x, y) of the 8x8 block blk using the filtered samples in wi->pf:
@@ -665,10 +661,10 @@ __device__ int pred8x8(const WaveIntra *wi, int mode, int x, int y) {
 #undef FL
 }
 
-__device__ void intra_mb(const DevPic &P, int addr, WaveIntra *wi, const Avail &av) {
+__device__ __forceinline__ void intra_mb(const DevPic &P, int addr, WaveIntra *wi, const Avail &av) {
     const int lane = wave_lane();
     const h264g_mb *mb = &P.mbs[addr];
-    const DevAux ax = P.aux[addr];
+    const DevAux &ax = P.aux[addr];
     const int W = P.W, H = P.H, wl = W * 16, hl = H * 16, wc = W * 8, hc = H * 8;
     const int mbx = addr % W, mby = addr / W;
     uint8_t *dst = slot_base(P, P.dst);
@@ -677,69 +673,102 @@ __device__ void intra_mb(const DevPic &P, int addr, WaveIntra *wi, const Avail &
     // ---- neighbours into the tile
     if (lane < 25) {  // row -1, x = lane - 1
         int x = lane - 1;
-        bool ok = x < 0 ? av.D : (x < 16 ? av.B : av.C);
+        bool ok = x < 0 ? av.D() : (x < 16 ? av.B() : av.C());
         if (ok) wi->T[0][lane] = dy[(uint64_t)(mby * 16 - 1) * wl + mbx * 16 + x];
     } else if (lane < 41) {  // column -1
         int y = lane - 25;
-        if (av.A) wi->T[y + 1][0] = dy[(uint64_t)(mby * 16 + y) * wl + mbx * 16 - 1];
+        if (av.A()) wi->T[y + 1][0] = dy[(uint64_t)(mby * 16 + y) * wl + mbx * 16 - 1];
     } else if (lane < 59) {  // chroma row -1 (x = -1..7) for both planes
         int k = lane - 41, c = k / 9, x = k % 9 - 1;
-        bool ok = x < 0 ? av.D : av.B;
+        bool ok = x < 0 ? av.D() : av.B();
         const uint8_t *pl = c == 0 ? dcb : dcr;
         if (ok) wi->C[c][0][x + 1] = pl[(uint64_t)(mby * 8 - 1) * wc + mbx * 8 + x];
     }
     if (lane < 16) {  // chroma column -1
         int c = lane >> 3, y = lane & 7;
         const uint8_t *pl = c == 0 ? dcb : dcr;
-        if (av.A) wi->C[c][y + 1][0] = pl[(uint64_t)(mby * 8 + y) * wc + mbx * 8 - 1];
+        if (av.A()) wi->C[c][y + 1][0] = pl[(uint64_t)(mby * 8 + y) * wc + mbx * 8 - 1];
     }
     wave_sync();
     // ---- luma
     if (mb->mb_type == H264G_MB_I4x4) {
-        for (int blk = 0; blk < 16; blk++) {
-            if (lane < 16) {
-                int x = lane & 3, y = lane >> 2;
-                int v = pred4x4(wi, av, blk, mb->intra_pred_modes[blk], x, y);
-                int xx = blk_x(blk) + x, yy = blk_y(blk) + y;
-                if (res) v = clip1(v + wi->res.rl[yy * 16 + xx]);
-                wi->T[yy + 1][xx + 1] = (uint8_t)v;
+        // 4x4 blocks in wavefront order t = bx + 2 by, two blocks per step (lanes 0-15 and
+        // 16-31): every neighbour a block reads has a smaller t, and a top-right block that
+        // the reference's block order has not decoded yet (blocks 3, 11) counts as not
+        // available by rule (tile_p), so the result equals the block-index order.
+        const int g = lane >> 4, j = lane & 15;
+        for (int t = 0; t < 10; t++) {
+            const int by = (t < 2 ? 0 : (t - 2) >> 1) + g, bx = t - 2 * by;
+            const bool has = lane < 32 && bx >= 0 && by < 4;
+            const int blk = 8 * (by >> 1) + 4 * (bx >> 1) + 2 * (by & 1) + (bx & 1), xO = bx * 4, yO = by * 4;
+            if (has && j < 15) {  // neighbour line X (see pred4x4)
+                int v;
+                if (j < 5) v = tile_p(wi, av, xO, yO, -1, j == 0 ? 3 : 4 - j, blk);
+                else if (j == 5) v = tile_p(wi, av, xO, yO, -1, -1, blk);
+                else {
+                    const int tx = j == 14 ? 7 : j - 6;
+                    v = tile_p(wi, av, xO, yO, tx, -1, blk);
+                    if (tx >= 4 && v < 0) {  // p[4..7, -1] <- p[3, -1] (H264PictureBase.cpp:1161-1167)
+                        const int t3 = tile_p(wi, av, xO, yO, 3, -1, blk);
+                        if (t3 >= 0) v = t3;
+                    }
+                }
+                wi->X[g][j] = (int16_t)v;
+            }
+            wave_sync();
+            if (has) {
+                const int x = j & 3, y = j >> 2;
+                int v = pred4x4(wi->X[g], mb->intra_pred_modes[blk], x, y);
+                if (res) v = clip1(v + wi->res.rl[(yO + y) * 16 + xO + x]);
+                wi->T[yO + y + 1][xO + x + 1] = (uint8_t)v;
             }
             wave_sync();
         }
     } else if (mb->mb_type == H264G_MB_I8x8) {
         for (int blk = 0; blk < 4; blk++) {
-            int xO = (blk & 1) * 8, yO = (blk >> 1) * 8;
-            // 8.3.2.2.1 reference sample filtering (H264PictureBase.cpp:1535-1599)
+            const int xO = (blk & 1) * 8, yO = (blk >> 1) * 8;
+            // the 25 reference samples before filtering: U[0] = p[-1, -1], U[1 + y] = p[-1, y],
+            // U[9 + x] = p[x, -1] with p[8..15, -1] <- p[7, -1] when not available
             if (lane < 25) {
-                int p[18];  // p[-1..16]: neighbours along the filtered line of this lane
+                int v;
+                if (lane == 0) v = tile_p(wi, av, xO, yO, -1, -1, -1);
+                else if (lane < 9) v = tile_p(wi, av, xO, yO, -1, lane - 1, -1);
+                else {
+                    const int x = lane - 9;
+                    v = tile_p(wi, av, xO, yO, x, -1, -1);
+                    if (x >= 8 && v < 0) {
+                        const int t7 = tile_p(wi, av, xO, yO, 7, -1, -1);
+                        if (t7 >= 0) v = t7;
+                    }
+                }
+                wi->U[lane] = (int16_t)v;
+            }
+            wave_sync();
+            // 8.3.2.2.1 reference sample filtering (H264PictureBase.cpp:1535-1599); the left
+            // column and the 16 top samples are available all or none
+            if (lane < 25) {
+                const int16_t *U = wi->U;
+                const int pm = U[0];
                 int v = -1;
-                if (lane == 0) {  // p'[-1,-1]
-                    int pm = tile_p(wi, av, xO, yO, -1, -1, -1), pt = tile_p(wi, av, xO, yO, 0, -1, -1), pl = tile_p(wi, av, xO, yO, -1, 0, -1);
+                if (lane == 0) {  // p'[-1, -1]
+                    const int pt = U[9], pl = U[1];
                     if (pm >= 0) {
                         if (pt < 0 || pl < 0) v = pt >= 0 ? (3 * pm + pt + 2) >> 2 : (pl >= 0 ? (3 * pm + pl + 2) >> 2 : pm);
                         else v = (pt + 2 * pm + pl + 2) >> 2;
                     }
                 } else if (lane < 9) {  // p'[-1, y]
-                    int y = lane - 1, all = 1;
-                    for (int k = 0; k < 8; k++) { p[k + 1] = tile_p(wi, av, xO, yO, -1, k, -1); all &= p[k + 1] >= 0; }
-                    int pm = tile_p(wi, av, xO, yO, -1, -1, -1);
-                    if (all) {
-                        if (y == 0) v = pm >= 0 ? (pm + 2 * p[1] + p[2] + 2) >> 2 : (3 * p[1] + p[2] + 2) >> 2;
-                        else if (y == 7) v = (p[7] + 3 * p[8] + 2) >> 2;
-                        else v = (p[y] + 2 * p[y + 1] + p[y + 2] + 2) >> 2;
+                    const int y = lane - 1;
+                    if (U[1] >= 0) {
+                        if (y == 0) v = pm >= 0 ? (pm + 2 * U[1] + U[2] + 2) >> 2 : (3 * U[1] + U[2] + 2) >> 2;
+                        else if (y == 7) v = (U[7] + 3 * U[8] + 2) >> 2;
+                        else v = (U[y] + 2 * U[y + 1] + U[y + 2] + 2) >> 2;
                     }
                 } else {  // p'[x, -1]
-                    int x = lane - 9, all = 1;
-                    for (int k = 0; k < 16; k++) p[k + 1] = tile_p(wi, av, xO, yO, k, -1, -1);
-                    int up8 = 1;
-                    for (int k = 8; k < 16; k++) up8 &= p[k + 1] < 0;
-                    if (up8 && p[8] >= 0) for (int k = 8; k < 16; k++) p[k + 1] = p[8];
-                    for (int k = 0; k < 16; k++) all &= p[k + 1] >= 0;
-                    int pm = tile_p(wi, av, xO, yO, -1, -1, -1);
-                    if (all) {
-                        if (x == 0) v = pm >= 0 ? (pm + 2 * p[1] + p[2] + 2) >> 2 : (3 * p[1] + p[2] + 2) >> 2;
-                        else if (x == 15) v = (p[15] + 3 * p[16] + 2) >> 2;
-                        else v = (p[x] + 2 * p[x + 1] + p[x + 2] + 2) >> 2;
+                    const int x = lane - 9;
+                    if (U[9] >= 0 && U[17] >= 0) {
+                        if (x == 0) v = pm >= 0 ? (pm + 2 * U[9] + U[10] + 2) >> 2 : (3 * U[9] + U[10] + 2) >> 2;
+                        else if (x == 15) v = (U[23] + 3 * U[24] + 2) >> 2;
+                        else v = (U[8 + x] + 2 * U[9 + x] + U[10 + x] + 2) >> 2;
                     }
                 }
                 wi->pf[lane] = (int16_t)v;
@@ -758,9 +787,9 @@ __device__ void intra_mb(const DevPic &P, int addr, WaveIntra *wi, const Avail &
         if (mode == 2) {
             int st = 0, sl = 0;
             for (int k = 0; k < 16; k++) { st += wi->T[0][k + 1]; sl += wi->T[k + 1][0]; }
-            if (av.A && av.B) dc = (st + sl + 16) >> 5;
-            else if (av.A) dc = (sl + 8) >> 4;
-            else if (av.B) dc = (st + 8) >> 4;
+            if (av.A() && av.B()) dc = (st + sl + 16) >> 5;
+            else if (av.A()) dc = (sl + 8) >> 4;
+            else if (av.B()) dc = (st + 8) >> 4;
         } else if (mode == 3) {
             int Hs = 0, Vs = 0;
             for (int k = 0; k < 8; k++) {
@@ -788,8 +817,8 @@ __device__ void intra_mb(const DevPic &P, int addr, WaveIntra *wi, const Avail &
             int v;
             if (mode == 0) {
                 int xO = x & 4, yO = y & 4;
-                int t = av.B && Cp[0][xO + 1] > 0 && Cp[0][xO + 2] > 0 && Cp[0][xO + 3] > 0 && Cp[0][xO + 4] > 0;
-                int l = av.A && Cp[yO + 1][0] > 0 && Cp[yO + 2][0] > 0 && Cp[yO + 3][0] > 0 && Cp[yO + 4][0] > 0;
+                int t = av.B() && Cp[0][xO + 1] > 0 && Cp[0][xO + 2] > 0 && Cp[0][xO + 3] > 0 && Cp[0][xO + 4] > 0;
+                int l = av.A() && Cp[yO + 1][0] > 0 && Cp[yO + 2][0] > 0 && Cp[yO + 3][0] > 0 && Cp[yO + 4][0] > 0;
                 int st = Cp[0][xO + 1] + Cp[0][xO + 2] + Cp[0][xO + 3] + Cp[0][xO + 4];
                 int sl = Cp[yO + 1][0] + Cp[yO + 2][0] + Cp[yO + 3][0] + Cp[yO + 4][0];
                 if ((xO == 0 && yO == 0) || (xO > 0 && yO > 0)) v = (t && l) ? (st + sl + 4) >> 3 : (l ? (sl + 2) >> 2 : (t ? (st + 2) >> 2 : 128));
@@ -1228,13 +1257,14 @@ __global__ __launch_bounds__(WAVE) void k_inter(const DevPic *__restrict__ pics,
 __device__ __forceinline__ Avail intra_avail(const DevPic &P, int addr, int deps[4]) {
     const int W = P.W, mbx = addr % W, mby = addr / W;
     Avail av;
-    av.A = mbx > 0; av.B = mby > 0; av.C = mby > 0 && mbx < W - 1; av.D = mbx > 0 && mby > 0;
-    const int nb[4] = {av.A ? addr - 1 : -1, av.B ? addr - W : -1, av.C ? addr - W + 1 : -1, av.D ? addr - W - 1 : -1};
+    av.m = (mbx > 0 ? 1u : 0u) | (mby > 0 ? 2u : 0u) | (mby > 0 && mbx < W - 1 ? 4u : 0u) | (mbx > 0 && mby > 0 ? 8u : 0u);
+    const int nb[4] = {addr - 1, addr - W, addr - W + 1, addr - W - 1};
+#pragma unroll
     for (int k = 0; k < 4; k++) {
         deps[k] = -1;
-        if (nb[k] < 0) continue;
+        if (!(av.m & (1u << k))) continue;
         const int c = P.mbs[nb[k]].mb_type;
-        if (P.cip && cls_inter(c)) { if (k == 0) av.A = 0; else if (k == 1) av.B = 0; else if (k == 2) av.C = 0; else av.D = 0; }
+        if (P.cip && cls_inter(c)) av.m &= ~(1u << k);
         if (cls_intra(c)) deps[k] = nb[k];
     }
     return av;
