@@ -965,10 +965,13 @@ __device__ __forceinline__ void filter_line(int p[4], int q[4], int bS, bool chr
     }
 }
 
-// LDS working set of one deblocking wave: luma rows/cols [-4, 15], chroma [-4, 7].
+// LDS working set of one deblocking wave, in MB-relative coordinates: luma rows [-4, 15]
+// x columns [-16, 15] (the MB and the whole MB to its left), chroma rows [-4, 7] x
+// columns [-8, 7].  Rows -4..-1 hold the bottom rows of the MB above.
+constexpr int DT_LX = 16, DT_CX = 8;  // tile column of MB column 0 (luma, chroma)
 struct alignas(16) WaveDbk {
-    uint8_t L[20][20];
-    uint8_t C[2][12][12];
+    uint8_t L[20][32];
+    uint8_t C[2][12][16];
 };
 
 // Plane pointers of the picture being deblocked.
@@ -994,14 +997,14 @@ __device__ void dbk_load(const DbkPlanes &d, int mbx, int mby, int region, WaveD
         const int y = i / 20 - 4, x = i % 20 - 4;
         const int r = (y >= 0 && x >= 0) ? 1 : (y < 0 && x >= 0 ? 2 : (x < 0 && y >= 0 ? 4 : 0));
         if (!(r & region)) continue;
-        wd->L[y + 4][x + 4] = d.y[(uint64_t)(mby * 16 + y) * d.wl + mbx * 16 + x];
+        wd->L[y + 4][x + DT_LX] = d.y[(uint64_t)(mby * 16 + y) * d.wl + mbx * 16 + x];
     }
     for (int i = lane; i < 288; i += WAVE) {
         const int c = i / 144, k = i % 144, y = k / 12 - 4, x = k % 12 - 4;
         const int r = (y >= 0 && x >= 0) ? 1 : (y < 0 && x >= 0 ? 2 : (x < 0 && y >= 0 ? 4 : 0));
         if (!(r & region)) continue;
         const uint8_t *pl = c == 0 ? d.cb : d.cr;
-        wd->C[c][y + 4][x + 4] = pl[(uint64_t)(mby * 8 + y) * d.wc + mbx * 8 + x];
+        wd->C[c][y + 4][x + DT_CX] = pl[(uint64_t)(mby * 8 + y) * d.wc + mbx * 8 + x];
     }
 }
 
@@ -1036,8 +1039,9 @@ __device__ __forceinline__ void dbk_lines(const DevPic &P, const MbView &Ev, boo
     const int plane = lane < 16 ? 0 : (lane < 24 ? 1 : 2);
     const bool luma = plane == 0;
     const int k = luma ? lane : (lane & 7);
-    uint8_t *base = luma ? &wd->L[0][0] : &wd->C[plane - 1][0][0];
-    const int stride = luma ? 20 : 12;
+    // base: tile row 0 at MB column -4; a row holds columns -4 .. 15 (chroma -4 .. 7)
+    uint8_t *base = luma ? &wd->L[0][DT_LX - 4] : &wd->C[plane - 1][0][DT_CX - 4];
+    const int stride = luma ? 32 : 16;
     uint32_t w[5];
     if (DIR == 0) {
         const uint32_t *row = (const uint32_t *)(base + (k + 4) * stride);
@@ -1107,13 +1111,13 @@ __device__ void dbk_store(const DbkPlanes &d, int mbx, int mby, bool left, bool 
     for (int i = lane; i < 400; i += WAVE) {
         int y = i / 20 - 4, x = i % 20 - 4;
         if (x < -3 || y < -3 || (x < 0 && y < 0) || (x < 0 && !left) || (y < 0 && !top)) continue;
-        d.y[(uint64_t)(mby * 16 + y) * d.wl + mbx * 16 + x] = wd->L[y + 4][x + 4];
+        d.y[(uint64_t)(mby * 16 + y) * d.wl + mbx * 16 + x] = wd->L[y + 4][x + DT_LX];
     }
     for (int i = lane; i < 288; i += WAVE) {
         int c = i / 144, k = i % 144, y = k / 12 - 4, x = k % 12 - 4;
         if (x < -3 || y < -3 || (x < 0 && y < 0) || (x < 0 && !left) || (y < 0 && !top)) continue;
         uint8_t *pl = c == 0 ? d.cb : d.cr;
-        pl[(uint64_t)(mby * 8 + y) * d.wc + mbx * 8 + x] = wd->C[c][y + 4][x + 4];
+        pl[(uint64_t)(mby * 8 + y) * d.wc + mbx * 8 + x] = wd->C[c][y + 4][x + DT_CX];
     }
 }
 
@@ -1140,10 +1144,10 @@ __device__ void dbk_carry_left(WaveDbk *wd) {
     for (int i = lane; i < 16 * 4 + 2 * 8 * 4; i += WAVE) {
         if (i < 64) {
             const int y = i >> 2, x = i & 3;
-            wd->L[y + 4][x] = wd->L[y + 4][16 + x];
+            wd->L[y + 4][DT_LX - 4 + x] = wd->L[y + 4][DT_LX + 12 + x];
         } else {
             const int k = i - 64, c = k >> 5, y = (k >> 2) & 7, x = k & 3;
-            wd->C[c][y + 4][x] = wd->C[c][y + 4][8 + x];
+            wd->C[c][y + 4][DT_CX - 4 + x] = wd->C[c][y + 4][DT_CX + 4 + x];
         }
     }
     wave_sync();
@@ -1240,11 +1244,21 @@ __device__ void deblock_row(const DevPic &P, int mby, Sync &sync, WaveDbk *wd, c
 constexpr int DBK_WG_MAX_WAVES = 16;
 // The LDS working sets are per wave and the per-MB code synchronises with wave barriers,
 // so a workgroup may hold one wave (most kernels) or several (the band kernels).
+// Workgroups are dispatched to the 8 XCDs round-robin (workgroup b on XCD b mod 8), each
+// XCD with its own L2.  Item of workgroup b (of n, one item each) such that every XCD
+// takes one contiguous run of items: neighbouring MBs then share that XCD's L2 for
+// their overlapping reference lines and for the cache lines their rows share.
+constexpr uint32_t N_XCD = 8;
+__device__ __forceinline__ uint32_t xcd_item(uint32_t b, uint32_t n) {
+    const uint32_t x = b % N_XCD, q = n / N_XCD, r = n % N_XCD;
+    return x * q + min(x, r) + b / N_XCD;
+}
+
 __global__ __launch_bounds__(WAVE) void k_inter(const DevPic *__restrict__ pics, const uint32_t *__restrict__ items, uint32_t n,
                                                 uint32_t *err) {
     __shared__ WaveRes lds;
-    const uint32_t i = blockIdx.x;
-    if (i >= n) return;
+    if (blockIdx.x >= n) return;
+    const uint32_t i = xcd_item(blockIdx.x, n);
     const uint32_t it = items[i];
     const DevPic &P = pics[it >> WI_PIC_SHIFT];
     inter_mb(P, (int)(it & WI_MB_MASK), &lds);
@@ -1445,9 +1459,12 @@ __global__ __launch_bounds__(WAVE) void k_deblock_rows(const DevPic *__restrict_
 // row of W strips per band boundary) from a band's last row to the next band.  No sample
 // otherwise travels between waves: each sample is written to the picture by exactly one
 // wave, row y storing rows 0..12 (chroma 0..4) of its MBs and row y + 1 the rows 13..15
-// (chroma 5..7) after its top-edge filtering (the picture's last row stores all).  Row y
-// writes ring slot x mod STRIP_SLOTS only after row y + 1 has read strip x - STRIP_SLOTS;
-// every row of a band has its own wave, so the reader is always running.
+// (chroma 5..7) after its top-edge filtering (the picture's last row stores all).  An MB
+// is stored, and its strip written, one step after its own filtering, when its right
+// neighbour's left edge has finished its columns 13..15; the row's last MB at its own
+// step.  Row y writes strip x into ring slot x mod STRIP_SLOTS only after row y + 1 has
+// read strip x - STRIP_SLOTS; every row of a band has its own wave, so the reader is
+// always running.
 constexpr int STRIP_SLOTS = 8;
 struct alignas(16) StripSlot {
     uint8_t y[4][16];         // luma rows 12..15
@@ -1502,55 +1519,68 @@ __device__ __forceinline__ DbkPrefetch dbk_fetch(const DevPic &P, const DbkPlane
 }
 __device__ __forceinline__ void dbk_place(const DbkPrefetch &f, WaveDbk *wd, uint8_t *bsrec) {
     const int lane = wave_lane();
-    *(uint32_t *)&wd->L[(lane >> 2) + 4][4 + (lane & 3) * 4] = f.luma;
-    if (lane < 32) *(uint32_t *)&wd->C[lane >> 4][((lane >> 1) & 7) + 4][4 + (lane & 1) * 4] = f.chroma;
+    *(uint32_t *)&wd->L[(lane >> 2) + 4][DT_LX + (lane & 3) * 4] = f.luma;
+    if (lane < 32) *(uint32_t *)&wd->C[lane >> 4][((lane >> 1) & 7) + 4][DT_CX + (lane & 1) * 4] = f.chroma;
     if (lane < 8) *(uint32_t *)&bsrec[lane * 4] = f.bs;
 }
 
-// Stores of step x of a row (see above): rows 0..12 (last row 0..15) of MB x cols 0..12
-// and of MB x - 1 cols 13..15, the top neighbour's rows 13..15, and at the row end the
-// last MB's cols 13..15; chroma alike with rows / cols 5..7.
-__device__ void dbk_store_wg(const DbkPlanes &d, int mbx, int mby, bool left, bool top, bool last, bool rowend,
-                             const WaveDbk *wd) {
+// Rows 0 .. ry (chroma 0 .. cy) of the MB whose column 0 sits at tile column `col` (0: the
+// left MB, DT_LX: the current one) to the picture: one aligned 16-byte luma row per lane
+// 0..15, one 8-byte chroma row per lane 32..47.
+__device__ __forceinline__ void dbk_store_mb(const DbkPlanes &d, int mbx, int mby, int col, int ry, int cy,
+                                             const WaveDbk *wd) {
     const int lane = wave_lane();
-    const int ly = last ? 15 : 12, cy = last ? 7 : 4;
-    for (int i = lane; i < 400; i += WAVE) {
-        const int y = i / 20 - 4, x = i % 20 - 4;
-        const bool body = y >= 0 && y <= ly && (x <= 12 ? x >= (left ? -3 : 0) : rowend);
-        const bool up = top && y >= -3 && y < 0 && x >= 0;
-        if (body || up) d.y[(uint64_t)(mby * 16 + y) * d.wl + mbx * 16 + x] = wd->L[y + 4][x + 4];
+    if (lane < 16) {
+        if (lane <= ry)
+            *(uint4 *)&d.y[(uint64_t)(mby * 16 + lane) * d.wl + mbx * 16] = *(const uint4 *)&wd->L[lane + 4][col];
+    } else if (lane >= 32 && lane < 48) {
+        const int k = lane - 32, c = k >> 3, r = k & 7;
+        if (r <= cy)
+            *(uint2 *)&(c ? d.cr : d.cb)[(uint64_t)(mby * 8 + r) * d.wc + mbx * 8] = *(const uint2 *)&wd->C[c][r + 4][col >> 1];
     }
-    for (int i = lane; i < 288; i += WAVE) {
-        const int c = i / 144, k = i % 144, y = k / 12 - 4, x = k % 12 - 4;
-        const bool body = y >= 0 && y <= cy && (x <= 4 ? x >= (left ? -3 : 0) : rowend);
-        const bool up = top && y >= -3 && y < 0 && x >= 0;
-        if (body || up) (c ? d.cr : d.cb)[(uint64_t)(mby * 8 + y) * d.wc + mbx * 8 + x] = wd->C[c][y + 4][x + 4];
+}
+// Rows 13..15 (chroma 5..7) of the MB above the current one, final after the current MB's
+// top edge: lanes 0..2 luma, 32..37 chroma.
+__device__ __forceinline__ void dbk_store_top(const DbkPlanes &d, int mbx, int mby, const WaveDbk *wd) {
+    const int lane = wave_lane();
+    if (lane < 3) {
+        *(uint4 *)&d.y[(uint64_t)(mby * 16 - 3 + lane) * d.wl + mbx * 16] = *(const uint4 *)&wd->L[lane + 1][DT_LX];
+    } else if (lane >= 32 && lane < 38) {
+        const int k = lane - 32, c = k >= 3, r = k - 3 * c;
+        *(uint2 *)&(c ? d.cr : d.cb)[(uint64_t)(mby * 8 - 3 + r) * d.wc + mbx * 8] = *(const uint2 *)&wd->C[c][r + 1][DT_CX];
     }
 }
 
-// Strip parts: A = MB x's columns 0..12 (chroma 0..4) from the tile's current MB;
-// B = columns 13..15 (chroma 5..7) of the MB whose right columns sit at tile column tc
-// (1 for MB x - 1 after its right neighbour's left edge, 17 for the last MB of the row).
-__device__ __forceinline__ void strip_put(StripSlot *sl, const WaveDbk *wd, bool part_a, int tc) {
+// The strip (rows 12..15, chroma 4..7) of the MB at tile column `col`: one dword per lane.
+__device__ __forceinline__ void strip_put(StripSlot *sl, const WaveDbk *wd, int col) {
     const int lane = wave_lane();
-    for (int i = lane; i < 128; i += WAVE) {
-        if (i < 64) {
-            const int r = i >> 4, x = i & 15;
-            if (part_a ? x <= 12 : x >= 13) sl->y[r][x] = wd->L[16 + r][part_a ? 4 + x : tc + x - 13];
-        } else {
-            const int k = i - 64, c = k >> 5, r = (k >> 3) & 3, x = k & 7;
-            if (part_a ? x <= 4 : x >= 5) sl->c[c][r][x] = wd->C[c][8 + r][part_a ? 4 + x : (tc == 17 ? 9 : tc) + x - 5];
-        }
+    if (lane < 16) {
+        *(uint32_t *)&sl->y[lane >> 2][(lane & 3) * 4] = *(const uint32_t *)&wd->L[16 + (lane >> 2)][col + (lane & 3) * 4];
+    } else if (lane < 32) {
+        const int k = lane - 16, c = k >> 3, r = (k >> 1) & 3, h = k & 1;
+        *(uint32_t *)&sl->c[c][r][h * 4] = *(const uint32_t *)&wd->C[c][8 + r][(col >> 1) + h * 4];
     }
 }
-// The top neighbour's bottom rows into the tile (rows -4..-1 above the MB).
+// The MB above's strip into the tile rows -4..-1 above the current MB.
 __device__ __forceinline__ void strip_get(const StripSlot *sl, WaveDbk *wd) {
     const int lane = wave_lane();
-    if (lane < 16) *(uint32_t *)&wd->L[lane >> 2][4 + (lane & 3) * 4] = *(const uint32_t *)&sl->y[lane >> 2][(lane & 3) * 4];
-    else if (lane < 32) {
+    if (lane < 16) {
+        *(uint32_t *)&wd->L[lane >> 2][DT_LX + (lane & 3) * 4] = *(const uint32_t *)&sl->y[lane >> 2][(lane & 3) * 4];
+    } else if (lane < 32) {
         const int k = lane - 16, c = k >> 3, r = (k >> 1) & 3, h = k & 1;
-        *(uint32_t *)&wd->C[c][r][4 + h * 4] = *(const uint32_t *)&sl->c[c][r][h * 4];
+        *(uint32_t *)&wd->C[c][r][DT_CX + h * 4] = *(const uint32_t *)&sl->c[c][r][h * 4];
     }
+}
+// After MB x: it becomes the left MB of MB x + 1 (rows 0..15).
+__device__ __forceinline__ void dbk_carry_mb(WaveDbk *wd) {
+    const int lane = wave_lane();
+    wave_sync();
+    const uint32_t l = *(const uint32_t *)&wd->L[4 + (lane >> 2)][DT_LX + (lane & 3) * 4];
+    uint32_t c = 0;
+    if (lane < 32) c = *(const uint32_t *)&wd->C[lane >> 4][4 + ((lane >> 1) & 7)][DT_CX + (lane & 1) * 4];
+    *(uint32_t *)&wd->L[4 + (lane >> 2)][(lane & 3) * 4] = l;
+    if (lane < 32) *(uint32_t *)&wd->C[lane >> 4][4 + ((lane >> 1) & 7)][(lane & 1) * 4] = c;
+    wave_sync();
 }
 
 // MB row y of the band starting at row y0 (band index b, rows y0 .. y1 - 1) on wave w.
@@ -1589,21 +1619,26 @@ __device__ void deblock_row_band(const DevPic &P, int y, int y0, int y1, int b, 
         }
         wave_sync();
         dbk_filter(P, Qv, Lv, Tv, left, top, bsrec, wd, &sh.tb);
-        dbk_store_wg(d, x, y, left, top, last, rowend, wd);
+        // MB x - 1 is final now (its columns 13..15 took MB x's left edge), except rows
+        // 13..15 that the next row's top edge filters; MB x too at the row end
+        const int ry = last ? 15 : 12, cy = last ? 7 : 4;
+        if (left) dbk_store_mb(d, x - 1, y, 0, ry, cy, wd);
+        if (rowend) dbk_store_mb(d, x, y, DT_LX, ry, cy, wd);
+        if (top) dbk_store_top(d, x, y, wd);
         if (to_g) {
-            strip_put(&gout[x], wd, true, 0);
-            if (left) strip_put(&gout[x - 1], wd, false, 1);
-            if (rowend) strip_put(&gout[x], wd, false, 17);
+            if (left) strip_put(&gout[x - 1], wd, 0);
+            if (rowend) strip_put(&gout[x], wd, DT_LX);
             if (rowend || x % BAND_PUBLISH_EVERY == BAND_PUBLISH_EVERY - 1)
                 band_publish(&P.rowprog[y], tag, rowend ? (uint32_t)W : (uint32_t)x);
         } else if (!last) {
-            if (x >= STRIP_SLOTS) lds_wait(&sh.cons[ly], (uint32_t)(x - STRIP_SLOTS + 1), err);
-            strip_put(&ring[x % STRIP_SLOTS], wd, true, 0);
-            if (left) strip_put(&ring[(x - 1) % STRIP_SLOTS], wd, false, 1);
-            if (rowend) strip_put(&ring[x % STRIP_SLOTS], wd, false, 17);
+            // the slots of strips x - 1 (and x at the row end) held strips x - 1 - S (x - S)
+            const int need = (rowend ? x + 1 : x) - STRIP_SLOTS;
+            if (need > 0) lds_wait(&sh.cons[ly], (uint32_t)need, err);
+            if (left) strip_put(&ring[(x - 1) % STRIP_SLOTS], wd, 0);
+            if (rowend) strip_put(&ring[x % STRIP_SLOTS], wd, DT_LX);
             lds_post(&sh.prog[ly], rowend ? (uint32_t)W : (uint32_t)x);
         }
-        dbk_carry_left(wd);
+        dbk_carry_mb(wd);
     }
 }
 
