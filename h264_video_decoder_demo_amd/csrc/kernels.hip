@@ -661,15 +661,21 @@ __device__ int pred8x8(const WaveIntra *wi, int mode, int x, int y) {
 #undef FL
 }
 
-__device__ __forceinline__ void intra_mb(const DevPic &P, int addr, WaveIntra *wi, const Avail &av) {
+// The residual of an intra MB into wi->res (it needs no neighbour samples, so the
+// schedules run it before waiting for the MBs above); false when the MB has none.
+__device__ __forceinline__ bool intra_residual(const DevPic &P, int addr, WaveIntra *wi) {
+    return build_residual(&P.mbs[addr], P.aux[addr], P.levels, &wi->res);
+}
+
+// Prediction and reconstruction of an intra MB whose residual intra_residual left in
+// wi->res (res = it returned true); the neighbours it reads are final.
+__device__ __forceinline__ void intra_mb(const DevPic &P, int addr, WaveIntra *wi, const Avail &av, bool res) {
     const int lane = wave_lane();
     const h264g_mb *mb = &P.mbs[addr];
-    const DevAux &ax = P.aux[addr];
     const int W = P.W, H = P.H, wl = W * 16, hl = H * 16, wc = W * 8, hc = H * 8;
     const int mbx = addr % W, mby = addr / W;
     uint8_t *dst = slot_base(P, P.dst);
     uint8_t *dy = dst, *dcb = dst + (uint64_t)wl * hl, *dcr = dcb + (uint64_t)wc * hc;
-    const bool res = build_residual(mb, ax, P.levels, &wi->res);
     // ---- neighbours into the tile
     if (lane < 25) {  // row -1, x = lane - 1
         int x = lane - 1;
@@ -1296,8 +1302,9 @@ __global__ __launch_bounds__(WAVE) void k_intra(const DevPic *__restrict__ pics,
         const int addr = (int)(it & WI_MB_MASK);
         int deps[4];
         const Avail av = intra_avail(P, addr, deps);
+        const bool res = intra_residual(P, addr, &lds);
         wait_deps(P.flags, deps, 4, epoch, err);
-        intra_mb(P, addr, &lds, av);
+        intra_mb(P, addr, &lds, av, res);
         publish(&P.flags[addr], epoch);
     }
     retire(ctl, gridDim.x, epoch_p, epoch);
@@ -1397,11 +1404,12 @@ __global__ __launch_bounds__(WAVE * INTRA_WG_MAX_WAVES) void k_intra_band(const 
             post(b < e ? uniform(col[b]) : (uint32_t)W, true);
             for (uint32_t i = b; i < e; i++) {
                 const int x = (int)uniform(col[i]), addr = y * W + x;
-                if (from_g) band_wait(&P.rowprog[y - 1], tag, (uint32_t)min(x + 2, W), seen, err);
-                else if (y > 0) sync.wait(y - 1 - y0, min(x + 2, W));
                 int deps[4];
                 const Avail av = intra_avail(P, addr, deps);
-                intra_mb(P, addr, &tiles[w], av);
+                const bool res = intra_residual(P, addr, &tiles[w]);
+                if (from_g) band_wait(&P.rowprog[y - 1], tag, (uint32_t)min(x + 2, W), seen, err);
+                else if (y > 0) sync.wait(y - 1 - y0, min(x + 2, W));
+                intra_mb(P, addr, &tiles[w], av, res);
                 const bool end = i + 1 == e;
                 post(end ? (uint32_t)W : uniform(col[i + 1]), end || (i - b) % BAND_PUBLISH_EVERY == BAND_PUBLISH_EVERY - 1);
             }
