@@ -1316,11 +1316,13 @@ __global__ __launch_bounds__(WAVE) void k_intra(const DevPic *__restrict__ pics,
 // CU's caches).  A band's last row hands its results to the next band's first row, in
 // another workgroup and possibly another XCD, through device memory: it publishes its
 // progress word (P.rowprog[y], epoch << 32 | progress) with an agent-scope release every
-// BAND_PUBLISH_EVERY MBs and at the row end; the reader polls it relaxed and then
+// few MBs (INTRA_PUBLISH_EVERY intra MBs, DBK_PUBLISH_EVERY MBs: each publish writes back
+// the L2, each read of a new value invalidates it, so the period trades that cost against
+// the next band's lag) and at the row end; the reader polls it relaxed and then
 // acquires at agent scope.  Work items are (picture, band) in band-major order and a
 // workgroup takes its item by ticket when it starts, so the band above always belongs to
 // a workgroup that started earlier and never waits on a later one.
-constexpr int BAND_PUBLISH_EVERY = 8;
+constexpr int INTRA_PUBLISH_EVERY = 1, DBK_PUBLISH_EVERY = 4;
 
 // Takes the workgroup's ticket (into *slot) and clears `n` LDS counters.
 __device__ __forceinline__ uint32_t band_ticket(DevCtl *ctl, uint32_t *slot, uint32_t *ctr, int n) {
@@ -1411,7 +1413,7 @@ __global__ __launch_bounds__(WAVE * INTRA_WG_MAX_WAVES) void k_intra_band(const 
                 else if (y > 0) sync.wait(y - 1 - y0, min(x + 2, W));
                 intra_mb(P, addr, &tiles[w], av, res);
                 const bool end = i + 1 == e;
-                post(end ? (uint32_t)W : uniform(col[i + 1]), end || (i - b) % BAND_PUBLISH_EVERY == BAND_PUBLISH_EVERY - 1);
+                post(end ? (uint32_t)W : uniform(col[i + 1]), end || (i - b) % INTRA_PUBLISH_EVERY == INTRA_PUBLISH_EVERY - 1);
             }
         }
     }
@@ -1636,7 +1638,7 @@ __device__ void deblock_row_band(const DevPic &P, int y, int y0, int y1, int b, 
         if (to_g) {
             if (left) strip_put(&gout[x - 1], wd, 0);
             if (rowend) strip_put(&gout[x], wd, DT_LX);
-            if (rowend || x % BAND_PUBLISH_EVERY == BAND_PUBLISH_EVERY - 1)
+            if (rowend || x % DBK_PUBLISH_EVERY == DBK_PUBLISH_EVERY - 1)
                 band_publish(&P.rowprog[y], tag, rowend ? (uint32_t)W : (uint32_t)x);
         } else if (!last) {
             // the slots of strips x - 1 (and x at the row end) held strips x - 1 - S (x - S)
