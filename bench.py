@@ -155,7 +155,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--frames", type=int, default=16)
     ap.add_argument("--plain-steps", type=int, default=1)
-    ap.add_argument("--cpu-procs", type=int, default=8)
+    ap.add_argument("--cpu-procs", type=int, default=16)
     ap.add_argument("--cpu-frames", type=int, default=16)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
