@@ -486,7 +486,8 @@ int stage_picture(h264g_ctx *ctx, Stream &st, const h264g_pic_desc &d, const h26
     }
     irow[H] = (uint32_t)icol.size();
     size_t n_items = pic->n_inter + pic->n_intra + n_mbs + (size_t)H + irow.size() + icol.size();
-    size_t o_bs = al(o_it + n_items * sizeof(uint32_t)), o_pic = al(o_bs + (size_t)n_mbs * 32), total = al(o_pic + sizeof(DevPic));
+    size_t o_bs = al(o_it + n_items * sizeof(uint32_t)), o_dbq = al(o_bs + (size_t)n_mbs * 32);
+    size_t o_pic = al(o_dbq + (size_t)n_mbs * 4), total = al(o_pic + sizeof(DevPic));
     void *dm = pool_get(ctx, total);
     if (!dm) return h264g_set_error(H264G_E_NOMEM, "picture device memory (%zu bytes)", total);
     pic->dmem = dm;
@@ -499,6 +500,7 @@ int stage_picture(h264g_ctx *ctx, Stream &st, const h264g_pic_desc &d, const h26
     pic->d_items = (uint32_t *)(base + o_it);
     pic->d_pic = (DevPic *)(base + o_pic);
     uint8_t *d_bs = (uint8_t *)(base + o_bs);
+    uint32_t *d_dbq = (uint32_t *)(base + o_dbq);
     std::vector<uint32_t> items;
     items.reserve(n_items);
     items.insert(items.end(), pic->inter_list.begin(), pic->inter_list.end());
@@ -511,7 +513,7 @@ int stage_picture(h264g_ctx *ctx, Stream &st, const h264g_pic_desc &d, const h26
     DevPic &P = pic->hpic;
     std::memset(&P, 0, sizeof(P));
     P.mbs = pic->d_mbs; P.aux = pic->d_aux; P.levels = pic->d_levels; P.wt = pic->d_wt;
-    P.dpb = st.dpb; P.slot_bytes = st.slot_bytes; P.flags = st.flags; P.rowprog = st.rowprog; P.gstrip = st.gstrip; P.bs = d_bs; P.dst = d.dst_slot;
+    P.dpb = st.dpb; P.slot_bytes = st.slot_bytes; P.flags = st.flags; P.rowprog = st.rowprog; P.gstrip = st.gstrip; P.bs = d_bs; P.dbq = d_dbq; P.dst = d.dst_slot;
     P.irow = pic->d_items + irow_at; P.n_intra = pic->n_intra;
     P.W = W; P.H = H; P.offA = d.filter_offset_a; P.offB = d.filter_offset_b;
     P.dbk_idc = d.disable_deblocking_filter_idc; P.cip = d.constrained_intra_pred_flag;

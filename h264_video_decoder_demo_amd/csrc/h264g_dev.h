@@ -47,7 +47,7 @@ struct DevPic {
     uint32_t *flags;          // per-MB wavefront progress words of the stream
     uint64_t *rowprog;        // per-MB-row progress of row-pipelined deblocking:
                               // epoch << 32 | MBs of the row finished
-    uint8_t  *bs;             // boundary strengths, 32 per MB: [dir][edge][4-line segment]
+    uint8_t  *bs;             // boundary strengths, 32 per MB: [dir][4-line segment][edge]
     const uint32_t *irow;     // intra MBs per row: H + 1 offsets, then their columns
     uint32_t n_intra;         // intra-predicted MBs of the picture
     int32_t  dst;             // destination slot
@@ -56,7 +56,7 @@ struct DevPic {
     int32_t  dbk_idc;         // disable_deblocking_filter_idc
     int32_t  cip;             // constrained_intra_pred_flag
     uint8_t  *gstrip;         // deblocking strips between bands of the stream: W x 128 B per band
-    int32_t  pad[2];
+    uint32_t *dbq;            // per-MB deblocking word, written by k_bs
 };
 
 // Work item: picture index in the batch and macroblock address.

@@ -857,16 +857,16 @@ __device__ __forceinline__ void intra_mb(const DevPic &P, int addr, WaveIntra *w
 }
 
 // ------------------------------------------------------------ deblocking (8.7)
-// Scalar fields only, so that selecting between views and per-lane field choices stay
-// in registers.
+// What the bS derivation reads of an MB.  Scalar fields only, so that selecting between
+// views and per-lane field choices stay in registers.
 struct MbView {
     int cls, kind, nz4, nz8;
     uint32_t ref0, ref1;  // reference DPB slot per quadrant q (int8 in byte q), lists 0 / 1
     int pf;
-    uint32_t qp;          // deblocking qP of luma, Cb edges, Cr edges in bytes 0, 1, 2
 };
 __device__ __forceinline__ int view_ref(const MbView &v, int l, int q) { return (int8_t)((l ? v.ref1 : v.ref0) >> (8 * q)); }
-__device__ __forceinline__ int view_qp(const MbView &v, int plane) { return (v.qp >> (8 * plane)) & 255; }
+__device__ __forceinline__ int dbq_qp(uint32_t q, int plane) { return (q >> (8 * plane)) & 255; }
+__device__ __forceinline__ bool dbq_t8(uint32_t q) { return (q >> 24) & 1u; }
 
 __device__ __forceinline__ void load_view(const DevPic &P, int addr, MbView &v) {
     const DevAux ax = P.aux[addr];
@@ -879,8 +879,9 @@ __device__ __forceinline__ void load_view(const DevPic &P, int addr, MbView &v) 
              ((uint32_t)(uint8_t)ax.refid[0][2] << 16) | ((uint32_t)(uint8_t)ax.refid[0][3] << 24);
     v.ref1 = (uint32_t)(uint8_t)ax.refid[1][0] | ((uint32_t)(uint8_t)ax.refid[1][1] << 8) |
              ((uint32_t)(uint8_t)ax.refid[1][2] << 16) | ((uint32_t)(uint8_t)ax.refid[1][3] << 24);
-    v.qp = (uint32_t)ax.qdb[0] | ((uint32_t)ax.qdb[1] << 8) | ((uint32_t)ax.qdb[2] << 16);
 }
+// Deblocking word of an MB (P.dbq, written by k_bs): qP of luma, Cb edges, Cr edges in
+// bytes 0..2, transform_size_8x8_flag in bit 24.
 
 __device__ __forceinline__ bool mv_far(int ax, int ay, int bx, int by) { return iabs(ax - bx) >= 4 || iabs(ay - by) >= 4; }
 
@@ -937,15 +938,13 @@ __device__ void dbk_tables_init(DbkTables *t) {
 }
 
 // 8.7.2.2 - 8.7.2.4 on samples p[0..3] (p0 first) and q[0..3] (Deblock.cpp:1314-1522);
-// indexA / indexB = Clip3(0, 51, qPav + filterOffsetA / B).
-__device__ __forceinline__ void filter_line(int p[4], int q[4], int bS, bool chroma, int indexA, int indexB,
-                                            const DbkTables *tb) {
-    int alpha = tb->alpha[indexA], beta = tb->beta[indexB];
+// alpha = alpha'(indexA), beta = beta'(indexB), tc0 = tC0'(bS, indexA) (used for bS 1..3),
+// with indexA / indexB = Clip3(0, 51, qPav + filterOffsetA / B).
+__device__ __forceinline__ void filter_line(int p[4], int q[4], int bS, bool chroma, int alpha, int beta, int tc0) {
     int p0 = p[0], p1 = p[1], p2 = p[2], p3 = p[3], q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3];
     if (!(bS != 0 && iabs(p0 - q0) < alpha && iabs(p1 - p0) < beta && iabs(q1 - q0) < beta)) return;
     int ap = iabs(p2 - p0), aq = iabs(q2 - q0);
     if (bS < 4) {
-        int tc0 = tb->tc0[bS - 1][indexA];
         int tc = chroma ? tc0 + 1 : tc0 + (ap < beta) + (aq < beta);
         int delta = clip3(-tc, tc, (((q0 - p0) << 2) + (p1 - q1) + 4) >> 3);
         p[0] = clip1(p0 + delta);
@@ -1018,14 +1017,12 @@ __device__ void dbk_load(const DbkPlanes &d, int mbx, int mby, int region, WaveD
 // words J and J + 1); chroma lines hold 12 bytes and use J = 0, 1 (luma edges 0 and 8
 // for the bS).
 template <int J>
-__device__ __forceinline__ void dbk_line_edge(uint32_t (&w)[5], bool act, bool luma, int seg, const uint8_t *bsdir,
-                                              int iA, int iB, const DbkTables *tb) {
+__device__ __forceinline__ void dbk_line_edge(uint32_t (&w)[5], bool act, bool luma, int bS, int alpha, int beta, int tc0) {
     if (!act) return;
-    const int bS = bsdir[(luma ? J : 2 * J) * 4 + seg];
     const uint32_t a = w[J], b = w[J + 1];
     int pp[4] = {(int)(a >> 24), (int)((a >> 16) & 255u), (int)((a >> 8) & 255u), (int)(a & 255u)};
     int qq[4] = {(int)(b & 255u), (int)((b >> 8) & 255u), (int)((b >> 16) & 255u), (int)(b >> 24)};
-    filter_line(pp, qq, bS, !luma, iA, iB, tb);
+    filter_line(pp, qq, bS, !luma, alpha, beta, tc0);
     w[J] = (a & 255u) | ((uint32_t)pp[2] << 8) | ((uint32_t)pp[1] << 16) | ((uint32_t)pp[0] << 24);
     w[J + 1] = (uint32_t)qq[0] | ((uint32_t)qq[1] << 8) | ((uint32_t)qq[2] << 16) | (b & 0xff000000u);
 }
@@ -1036,9 +1033,10 @@ __device__ __forceinline__ void dbk_line_edge(uint32_t (&w)[5], bool act, bool l
 // it, keeps the line in registers and runs its edges in order: left / top MB edge (when
 // `edge`), then the internal ones (luma 4 / 12 skipped with the 8x8 transform, chroma
 // only 4).  Lines of one direction touch disjoint samples, so this gives the reference's
-// edge-by-edge order (8.7, Deblock.cpp:76-635).  Ev is the view of the left / top MB.
+// edge-by-edge order (8.7, Deblock.cpp:76-635).  qq / eq: deblocking words (see k_bs) of
+// the MB and of the left / top MB.
 template <int DIR>
-__device__ __forceinline__ void dbk_lines(const DevPic &P, const MbView &Ev, bool edge, const MbView &Qv,
+__device__ __forceinline__ void dbk_lines(const DevPic &P, uint32_t eq, bool edge, uint32_t qq,
                                           const uint8_t *bsrec, WaveDbk *wd, const DbkTables *tb) {
     const int lane = wave_lane();
     if (lane >= 32) return;
@@ -1068,17 +1066,24 @@ __device__ __forceinline__ void dbk_lines(const DevPic &P, const MbView &Ev, boo
                            ((uint32_t)c[(4 * i + 2) * stride] << 16) | ((uint32_t)c[(4 * i + 3) * stride] << 24))
                         : 0u;
     }
-    const int qpq = view_qp(Qv, plane), qpe = edge ? view_qp(Ev, plane) : qpq;
+    const int qpq = dbq_qp(qq, plane), qpe = edge ? dbq_qp(eq, plane) : qpq;
     const int qav = (qpe + qpq + 1) >> 1;  // internal edges: (qpq + qpq + 1) >> 1 = qpq
     const int iA0 = clip3(0, 51, qav + P.offA), iB0 = clip3(0, 51, qav + P.offB);
     const int iAi = clip3(0, 51, qpq + P.offA), iBi = clip3(0, 51, qpq + P.offB);
-    const bool t8 = Qv.kind & AUX_T8, on = P.dbk_idc != 1;
+    const bool t8 = dbq_t8(qq), on = P.dbk_idc != 1;
     const int seg = luma ? (k >> 2) : (k >> 1);  // 4-line luma segment of the line
-    const uint8_t *bsdir = bsrec + DIR * 16;       // bS per [edge][segment], from k_bs
-    dbk_line_edge<0>(w, on && edge, luma, seg, bsdir, iA0, iB0, tb);
-    dbk_line_edge<1>(w, on && (luma ? !t8 : true), luma, seg, bsdir, iAi, iBi, tb);
-    dbk_line_edge<2>(w, on && luma, luma, seg, bsdir, iAi, iBi, tb);
-    dbk_line_edge<3>(w, on && luma && !t8, luma, seg, bsdir, iAi, iBi, tb);
+    // bS of this segment's edges 0..3 (k_bs stores them [direction][segment][edge]); chroma
+    // edge j lies on luma edge 2j.  All table lookups are issued before the filtering.
+    const uint32_t bsv = *(const uint32_t *)&bsrec[DIR * 16 + seg * 4];
+    const int bs0 = bsv & 255u, bs1 = luma ? (bsv >> 8) & 255u : (bsv >> 16) & 255u, bs2 = (bsv >> 16) & 255u,
+              bs3 = bsv >> 24;
+    const int a0 = tb->alpha[iA0], b0 = tb->beta[iB0], ai = tb->alpha[iAi], bi = tb->beta[iBi];
+    const int t0 = tb->tc0[bs0 >= 1 && bs0 <= 3 ? bs0 - 1 : 0][iA0], t1 = tb->tc0[bs1 >= 1 && bs1 <= 3 ? bs1 - 1 : 0][iAi];
+    const int t2 = tb->tc0[bs2 >= 1 && bs2 <= 3 ? bs2 - 1 : 0][iAi], t3 = tb->tc0[bs3 >= 1 && bs3 <= 3 ? bs3 - 1 : 0][iAi];
+    dbk_line_edge<0>(w, on && edge, luma, bs0, a0, b0, t0);
+    dbk_line_edge<1>(w, on && (luma ? !t8 : true), luma, bs1, ai, bi, t1);
+    dbk_line_edge<2>(w, on && luma, luma, bs2, ai, bi, t2);
+    dbk_line_edge<3>(w, on && luma && !t8, luma, bs3, ai, bi, t3);
     if (DIR == 0) {
         uint32_t *row = (uint32_t *)(base + (k + 4) * stride);
         row[0] = w[0];
@@ -1100,14 +1105,14 @@ __device__ __forceinline__ void dbk_lines(const DevPic &P, const MbView &Ev, boo
 }
 
 // The MB's vertical edges, then its horizontal edges, on the tile (8.7, Deblock.cpp:76-635).
-// Lv / Tv are the left / top MB views (read only when present).  bsrec: the MB's 32
+// lq / tq are the deblocking words of the left / top MB (read only when present).  bsrec: the MB's 32
 // boundary strengths (device memory or an LDS copy).  The caller has synchronised the
 // tile; it is complete when this returns.
-__device__ __forceinline__ void dbk_filter(const DevPic &P, const MbView &Qv, const MbView &Lv, const MbView &Tv, bool left, bool top,
+__device__ __forceinline__ void dbk_filter(const DevPic &P, uint32_t qq, uint32_t lq, uint32_t tq, bool left, bool top,
                            const uint8_t *bsrec, WaveDbk *wd, const DbkTables *tb) {
-    dbk_lines<0>(P, Lv, left, Qv, bsrec, wd, tb);
+    dbk_lines<0>(P, lq, left, qq, bsrec, wd, tb);
     wave_sync();
-    dbk_lines<1>(P, Tv, top, Qv, bsrec, wd, tb);
+    dbk_lines<1>(P, tq, top, qq, bsrec, wd, tb);
     wave_sync();
 }
 
@@ -1134,12 +1139,9 @@ __device__ void deblock_mb(const DevPic &P, int addr, WaveDbk *wd, const DbkTabl
     const DbkPlanes d = dbk_planes(P);
     const bool left = mbx > 0, top = mby > 0;
     dbk_load(d, mbx, mby, 1 | (top ? 2 : 0) | (left ? 4 : 0), wd);
-    MbView Qv, Lv, Tv;
-    load_view(P, addr, Qv);
-    if (left) load_view(P, addr - 1, Lv);
-    if (top) load_view(P, addr - W, Tv);
+    const uint32_t qq = P.dbq[addr], lq = left ? P.dbq[addr - 1] : 0u, tq = top ? P.dbq[addr - W] : 0u;
     wave_sync();
-    dbk_filter(P, Qv, Lv, Tv, left, top, P.bs + (size_t)addr * 32, wd, tb);
+    dbk_filter(P, qq, lq, tq, left, top, P.bs + (size_t)addr * 32, wd, tb);
     dbk_store(d, mbx, mby, left, top, wd);
 }
 
@@ -1226,20 +1228,20 @@ __device__ void deblock_row(const DevPic &P, int mby, Sync &sync, WaveDbk *wd, c
     const int W = P.W;
     const DbkPlanes d = dbk_planes(P);
     const bool top = mby > 0;
-    MbView Lv, Qv, Tv;
+    uint32_t lq = 0u, qq = 0u, tq = 0u;
     for (int mbx = 0; mbx < W; mbx++) {
         const int addr = mby * W + mbx;
         const bool left = mbx > 0;
         dbk_load(d, mbx, mby, 1, wd);  // the MB's own samples: final before this launch
-        if (left) Lv = Qv;
-        load_view(P, addr, Qv);
+        lq = qq;
+        qq = P.dbq[addr];
         if (top) {
-            load_view(P, addr - W, Tv);
+            tq = P.dbq[addr - W];
             sync.wait(mby - 1, min(mbx + 2, W));
             dbk_load(d, mbx, mby, 2, wd);
         }
         wave_sync();
-        dbk_filter(P, Qv, Lv, Tv, left, top, P.bs + (size_t)addr * 32, wd, tb);
+        dbk_filter(P, qq, lq, tq, left, top, P.bs + (size_t)addr * 32, wd, tb);
         dbk_store(d, mbx, mby, left, top, wd);
         sync.post(mby, mbx + 1);
         dbk_carry_left(wd);
@@ -1606,18 +1608,44 @@ __device__ void deblock_row_band(const DevPic &P, int y, int y0, int y1, int b, 
     const StripSlot *above = ly > 0 ? sh.ring[ly - 1] : nullptr;
     StripSlot *gout = (StripSlot *)P.gstrip + (size_t)b * W;              // this band -> next
     const StripSlot *gin = (const StripSlot *)P.gstrip + (size_t)(b - 1) * W;  // previous band -> this
+    const int ry = last ? 15 : 12, cy = last ? 7 : 4;  // rows this row stores of its own MBs
     uint32_t seen = 0;
-    MbView Lv, Qv, Tv;
+    // strip s of this row to the row below (slot s mod STRIP_SLOTS), then progress `done`
+    auto hand_down = [&](int s, int col, uint32_t done, bool end) {
+        if (to_g) {
+            strip_put(&gout[s], wd, col);
+            if (end || done % DBK_PUBLISH_EVERY == 0) band_publish(&P.rowprog[y], tag, done);
+        } else {
+            if (s - STRIP_SLOTS >= 0) lds_wait(&sh.cons[ly], (uint32_t)(s - STRIP_SLOTS + 1), err);  // slot read?
+            strip_put(&ring[s % STRIP_SLOTS], wd, col);
+            lds_post(&sh.prog[ly], done);
+        }
+    };
     DbkPrefetch f = dbk_fetch(P, d, 0, y);
+    // deblocking words of the MB and the MB above, one MB ahead like the samples
+    uint32_t qq = 0u, qn = P.dbq[y * W], tn = top ? P.dbq[(y - 1) * W] : 0u;
     for (int x = 0; x < W; x++) {
         const int addr = y * W + x;
         const bool left = x > 0, rowend = x == W - 1;
+        const uint32_t lq = qq, tq = tn;
+        qq = qn;
         dbk_place(f, wd, bsrec);
-        if (!rowend) f = dbk_fetch(P, d, x + 1, y);  // next MB, in flight during this one
-        if (left) Lv = Qv;
-        load_view(P, addr, Qv);
+        if (!rowend) {
+            f = dbk_fetch(P, d, x + 1, y);  // next MB, in flight during this one
+            qn = P.dbq[addr + 1];
+            if (top) tn = P.dbq[addr - W + 1];
+        }
+        wave_sync();
+        dbk_lines<0>(P, lq, left, qq, bsrec, wd, &sh.tb);  // vertical edges: need nothing from above
+        wave_sync();
+        // MB x - 1 is final now (its columns 13..15 took MB x's left edge) except rows 13..15,
+        // which the next row's top edge filters: store it and hand its strip down before
+        // waiting for the row above, so the row below runs about one step behind
+        if (left) {
+            dbk_store_mb(d, x - 1, y, 0, ry, cy, wd);
+            if (!last) hand_down(x - 1, 0, (uint32_t)x, false);
+        }
         if (top) {
-            load_view(P, addr - W, Tv);
             if (from_g) {
                 band_wait(&P.rowprog[y - 1], tag, (uint32_t)x + 1, seen, err);
                 strip_get(&gin[x], wd);
@@ -1626,27 +1654,14 @@ __device__ void deblock_row_band(const DevPic &P, int y, int y0, int y1, int b, 
                 strip_get(&above[x % STRIP_SLOTS], wd);
                 lds_post(&sh.cons[ly - 1], (uint32_t)x + 1);
             }
+            wave_sync();
         }
+        dbk_lines<1>(P, tq, top, qq, bsrec, wd, &sh.tb);  // horizontal edges
         wave_sync();
-        dbk_filter(P, Qv, Lv, Tv, left, top, bsrec, wd, &sh.tb);
-        // MB x - 1 is final now (its columns 13..15 took MB x's left edge), except rows
-        // 13..15 that the next row's top edge filters; MB x too at the row end
-        const int ry = last ? 15 : 12, cy = last ? 7 : 4;
-        if (left) dbk_store_mb(d, x - 1, y, 0, ry, cy, wd);
-        if (rowend) dbk_store_mb(d, x, y, DT_LX, ry, cy, wd);
         if (top) dbk_store_top(d, x, y, wd);
-        if (to_g) {
-            if (left) strip_put(&gout[x - 1], wd, 0);
-            if (rowend) strip_put(&gout[x], wd, DT_LX);
-            if (rowend || x % DBK_PUBLISH_EVERY == DBK_PUBLISH_EVERY - 1)
-                band_publish(&P.rowprog[y], tag, rowend ? (uint32_t)W : (uint32_t)x);
-        } else if (!last) {
-            // the slots of strips x - 1 (and x at the row end) held strips x - 1 - S (x - S)
-            const int need = (rowend ? x + 1 : x) - STRIP_SLOTS;
-            if (need > 0) lds_wait(&sh.cons[ly], (uint32_t)need, err);
-            if (left) strip_put(&ring[(x - 1) % STRIP_SLOTS], wd, 0);
-            if (rowend) strip_put(&ring[x % STRIP_SLOTS], wd, DT_LX);
-            lds_post(&sh.prog[ly], rowend ? (uint32_t)W : (uint32_t)x);
+        if (rowend) {
+            dbk_store_mb(d, x, y, DT_LX, ry, cy, wd);
+            if (!last) hand_down(x, DT_LX, (uint32_t)W, true);
         }
         dbk_carry_mb(wd);
     }
@@ -1674,8 +1689,10 @@ __global__ __launch_bounds__(WAVE * DBK_WG_MAX_WAVES) void k_deblock_band(const 
 }
 
 // Boundary strengths of every edge segment of every MB (8.7.2.1, Deblock.cpp:994-1310),
-// fully parallel: they depend on syntax only, not on samples.  Half a wave per MB; lane
-// j = 16 * dir + 4 * edge + segment (4 luma lines each).
+// stored [direction][4-line segment][edge] so that a deblocking line reads its four edges
+// as one dword, and each MB's deblocking word (P.dbq: qP of luma / Cb edges / Cr edges in
+// bytes 0..2, transform_size_8x8_flag in bit 24); fully parallel: they depend on syntax
+// only, not on samples.  Half a wave per MB; lane j = 16 * dir + 4 * edge + segment.
 __global__ __launch_bounds__(WAVE) void k_bs(const DevPic *__restrict__ pics, const uint32_t *__restrict__ items, uint32_t n) {
     const int lane = wave_lane();
     const uint32_t i = blockIdx.x * 2 + (lane >> 5);
@@ -1694,7 +1711,12 @@ __global__ __launch_bounds__(WAVE) void k_bs(const DevPic *__restrict__ pics, co
         const int xp = dir == 0 ? (e ? e - 1 : 15) : k, yp = dir == 0 ? k : (e ? e - 1 : 15);
         bS = bs_luma(P, ap, Pv, addr, Qv, e == 0, xp, yp, xq, yq);
     }
-    P.bs[(size_t)addr * 32 + j] = (uint8_t)bS;
+    P.bs[(size_t)addr * 32 + (j & 16) + ((j & 3) << 2) + ((j >> 2) & 3)] = (uint8_t)bS;  // [dir][segment][edge]
+    if (j == 0) {
+        const DevAux &ax = P.aux[addr];
+        P.dbq[addr] = (uint32_t)ax.qdb[0] | ((uint32_t)ax.qdb[1] << 8) | ((uint32_t)ax.qdb[2] << 16) |
+                      ((ax.kind & AUX_T8) ? 1u << 24 : 0u);
+    }
 }
 
 // YUV420P -> BGR24 of one DPB slot (CH264PictureBase::convertYuv420pToBgr24,
