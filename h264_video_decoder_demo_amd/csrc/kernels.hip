@@ -1275,7 +1275,8 @@ __global__ __launch_bounds__(WAVE) void k_inter(const DevPic *__restrict__ pics,
 
 // Neighbour availability of an intra MB for intra prediction (6.4.11, with
 // constrained_intra_pred_flag); deps[k] = address of neighbour k (A, B, C, D) when it is
-// itself intra-predicted in this picture (its samples come from the intra stage), else -1.
+// itself intra-predicted in this picture (its samples come from the intra stage) and its
+// samples are read, else -1.
 __device__ __forceinline__ Avail intra_avail(const DevPic &P, int addr, int deps[4]) {
     const int W = P.W, mbx = addr % W, mby = addr / W;
     Avail av;
@@ -1289,6 +1290,14 @@ __device__ __forceinline__ Avail intra_avail(const DevPic &P, int addr, int deps
         if (P.cip && cls_inter(c)) av.m &= ~(1u << k);
         if (cls_intra(c)) deps[k] = nb[k];
     }
+    // Samples of C (above right) are read only by Intra_8x8 (the top reference samples
+    // p[8..15, -1] of block 1 and their filtering, 8.3.2.2) and by Intra_4x4 block 5 in
+    // the modes that use p[4..7, -1] (Diagonal_Down_Left 3, Vertical_Left 7); without
+    // them the MB does not wait for C.
+    const h264g_mb &m = P.mbs[addr];
+    const bool uses_c = m.mb_type == H264G_MB_I8x8 ||
+                        (m.mb_type == H264G_MB_I4x4 && (m.intra_pred_modes[5] == 3 || m.intra_pred_modes[5] == 7));
+    if (!uses_c) deps[2] = -1;
     return av;
 }
 
@@ -1375,8 +1384,9 @@ __device__ void band_wait(const uint64_t *w, uint64_t tag, uint32_t need, uint32
 
 // Intra stage over bands: the wave of row y walks the row's intra MBs (P.irow: row
 // offsets, then columns).  Row progress p means "every intra MB of the row left of
-// column p is done"; MB (x, y) needs the row above at min(x + 2, W) (neighbours B, C, D).
-// Neighbour samples pass through the picture in device memory.
+// column p is done"; MB (x, y) needs the row above at min(x + 2, W) when it reads its
+// intra neighbour C, at x + 1 when it reads only B / D (intra_avail).  Neighbour samples
+// pass through the picture in device memory.
 constexpr int INTRA_WG_MAX_WAVES = 8;
 __global__ __launch_bounds__(WAVE * INTRA_WG_MAX_WAVES) void k_intra_band(const DevPic *__restrict__ pics,
                                                                           const uint32_t *__restrict__ items, uint32_t n,
@@ -1411,8 +1421,12 @@ __global__ __launch_bounds__(WAVE * INTRA_WG_MAX_WAVES) void k_intra_band(const 
                 int deps[4];
                 const Avail av = intra_avail(P, addr, deps);
                 const bool res = intra_residual(P, addr, &tiles[w]);
-                if (from_g) band_wait(&P.rowprog[y - 1], tag, (uint32_t)min(x + 2, W), seen, err);
-                else if (y > 0) sync.wait(y - 1 - y0, min(x + 2, W));
+                // the row above up to C (x + 1) or up to B (x) when only B / D are read and intra
+                const int need = deps[2] >= 0 ? min(x + 2, W) : (deps[1] >= 0 || deps[3] >= 0 ? x + 1 : 0);
+                if (need > 0) {
+                    if (from_g) band_wait(&P.rowprog[y - 1], tag, (uint32_t)need, seen, err);
+                    else sync.wait(y - 1 - y0, need);
+                }
                 intra_mb(P, addr, &tiles[w], av, res);
                 const bool end = i + 1 == e;
                 post(end ? (uint32_t)W : uniform(col[i + 1]), end || (i - b) % INTRA_PUBLISH_EVERY == INTRA_PUBLISH_EVERY - 1);
