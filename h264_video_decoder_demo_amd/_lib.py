@@ -82,6 +82,7 @@ class Config(ctypes.Structure):
         ("device", c_int32), ("dpb_budget_bytes", c_uint64), ("max_batch_pictures", c_int32),
         ("profile", c_int32), ("waves", c_int32), ("no_graphs", c_int32), ("deblock_schedule", c_int32),
         ("deblock_waves", c_int32), ("intra_schedule", c_int32), ("intra_waves", c_int32),
+        ("lane_split_mbs", c_int32),
     ]
 
 

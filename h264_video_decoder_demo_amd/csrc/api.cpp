@@ -49,6 +49,7 @@ struct Stream {
     uint64_t *rowprog = nullptr;
     uint8_t *gstrip = nullptr;     // deblocking strips between row bands
     uint64_t bytes = 0;
+    int lane = 0;                  // HIP stream (lane) of the context that runs its pictures
     bool open = false;
 };
 
@@ -77,6 +78,7 @@ struct h264g_pic {
 };
 
 namespace {
+// The pictures of one batch on one lane.
 struct Batch {
     void *dmem = nullptr;          // pooled device block
     size_t dbytes = 0;
@@ -90,19 +92,26 @@ struct Batch {
 };
 }  // namespace
 
-struct h264g_batch : Batch {};
+struct h264g_batch {
+    Batch sub[2];                  // pictures of lane 0 / lane 1
+};
 
 struct h264g_ctx {
     int device = 0;
-    hipStream_t stream = nullptr;
+    // Two lanes (HIP streams): streams of pictures with at least split_mbs MBs run on lane
+    // 1, the others on lane 0, so a batch's long intra / deblocking chains of large
+    // pictures overlap the other pictures' stages instead of ending every stage.
+    hipStream_t stream = nullptr;  // lane 0
+    hipStream_t aux = nullptr;     // lane 1
+    int split_mbs = 16384;         // 0: one lane
     uint32_t *d_err = nullptr;
-    DevCtl *d_ctl = nullptr;       // [0] intra, [1] deblock, [2].ticket = shared launch epoch
+    DevCtl *d_ctl = nullptr;       // per lane L: [4L] intra, [4L + 1] deblock, [4L + 2].ticket = launch epoch
     int waves = 2048;
     uint64_t budget = 0, used = 0;
     std::vector<Stream> streams;
     // Device blocks of released pictures / batches, reusable once `ready` has completed on
     // the stream (so releasing never waits for the GPU and staging rarely allocates).
-    struct PoolBlock { void *ptr; size_t size; hipEvent_t ready; };
+    struct PoolBlock { void *ptr; size_t size; hipEvent_t ready[2]; };  // one event per lane
     std::vector<PoolBlock> pool;
     size_t pool_bytes = 0;
     bool profile = false;
@@ -126,6 +135,14 @@ extern "C" int32_t h264g_set_error(int32_t code, const char *fmt, ...) {
 
 namespace {
 
+hipStream_t lane_stream(const h264g_ctx *c, int lane) { return lane ? c->aux : c->stream; }
+
+hipError_t sync_lanes(h264g_ctx *c) {
+    hipError_t e = hipStreamSynchronize(c->stream);
+    hipError_t e2 = hipStreamSynchronize(c->aux);
+    return e != hipSuccess ? e : e2;
+}
+
 // ------------------------------------------------------------ block pool
 constexpr size_t kPoolMaxBytes = 8ull << 30;
 
@@ -140,9 +157,10 @@ void *pool_get(h264g_ctx *c, size_t n) {
     const size_t want = pool_class(n);
     for (size_t i = 0; i < c->pool.size(); i++) {
         auto &b = c->pool[i];
-        if (b.size == want && hipEventQuery(b.ready) == hipSuccess) {
+        if (b.size == want && hipEventQuery(b.ready[0]) == hipSuccess && hipEventQuery(b.ready[1]) == hipSuccess) {
             void *p = b.ptr;
-            (void)hipEventDestroy(b.ready);
+            (void)hipEventDestroy(b.ready[0]);
+            (void)hipEventDestroy(b.ready[1]);
             c->pool_bytes -= b.size;
             c->pool.erase(c->pool.begin() + (long)i);
             return p;
@@ -153,22 +171,29 @@ void *pool_get(h264g_ctx *c, size_t n) {
     return p;
 }
 
-// Returns a block; it becomes reusable when the work enqueued so far has completed.
+// Returns a block; it becomes reusable when the work enqueued so far on both lanes has
+// completed.
 void pool_put(h264g_ctx *c, void *p, size_t n) {
     if (!p) return;
-    hipEvent_t e = nullptr;
-    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess || hipEventRecord(e, c->stream) != hipSuccess) {
-        (void)hipStreamSynchronize(c->stream);  // cannot track it: free it once idle
-        if (e) (void)hipEventDestroy(e);
+    hipEvent_t e[2] = {nullptr, nullptr};
+    bool ok = true;
+    for (int l = 0; l < 2; l++)
+        ok = ok && hipEventCreateWithFlags(&e[l], hipEventDisableTiming) == hipSuccess &&
+             hipEventRecord(e[l], lane_stream(c, l)) == hipSuccess;
+    if (!ok) {
+        (void)sync_lanes(c);  // cannot track it: free it once idle
+        for (auto ev : e) if (ev) (void)hipEventDestroy(ev);
         (void)hipFree(p);
         return;
     }
-    c->pool.push_back({p, pool_class(n), e});
+    c->pool.push_back({p, pool_class(n), {e[0], e[1]}});
     c->pool_bytes += pool_class(n);
     while (c->pool_bytes > kPoolMaxBytes && !c->pool.empty()) {  // trim the oldest
         auto &b = c->pool.front();
-        (void)hipEventSynchronize(b.ready);
-        (void)hipEventDestroy(b.ready);
+        for (auto ev : b.ready) {
+            (void)hipEventSynchronize(ev);
+            (void)hipEventDestroy(ev);
+        }
         (void)hipFree(b.ptr);
         c->pool_bytes -= b.size;
         c->pool.erase(c->pool.begin());
@@ -537,8 +562,10 @@ struct StageSet {
     bool dbk_off;
 };
 
-int launch_stages(h264g_ctx *ctx, const StageSet &q) {
-    hipStream_t s = ctx->stream;
+int launch_stages(h264g_ctx *ctx, const StageSet &q, int lane) {
+    hipStream_t s = lane_stream(ctx, lane);
+    DevCtl *ctl = &ctx->d_ctl[4 * lane];
+    uint32_t *epoch = &ctl[2].ticket;
     hipEvent_t *ev = nullptr;
     if (ctx->profile) {
         while (ctx->ev.size() < ctx->ev_used + 5) {
@@ -555,9 +582,9 @@ int launch_stages(h264g_ctx *ctx, const StageSet &q) {
     if (ev) HIPCHK(hipEventRecord(ev[1], s));
     if (q.n_intra)
         e = ctx->intra_schedule == H264G_INTRA_BANDS
-                ? h264g_launch_intra_band(q.pics, q.ibands, q.n_ibands, &ctx->d_ctl[0], &ctx->d_ctl[2].ticket, ctx->intra_waves,
+                ? h264g_launch_intra_band(q.pics, q.ibands, q.n_ibands, &ctl[0], epoch, ctx->intra_waves,
                                           ctx->d_err, s)
-                : h264g_launch_intra(q.pics, q.intra, q.n_intra, &ctx->d_ctl[0], &ctx->d_ctl[2].ticket, ctx->d_err, ctx->waves, s);
+                : h264g_launch_intra(q.pics, q.intra, q.n_intra, &ctl[0], epoch, ctx->d_err, ctx->waves, s);
     if (e) return h264g_set_error(H264G_E_DEVICE, "intra launch: %s", hipGetErrorString((hipError_t)e));
     if (ev) HIPCHK(hipEventRecord(ev[2], s));
     if (!q.dbk_off) {
@@ -566,14 +593,14 @@ int launch_stages(h264g_ctx *ctx, const StageSet &q) {
         if (ev) HIPCHK(hipEventRecord(ev[3], s));
         switch (ctx->dbk_schedule) {
         case H264G_DEBLOCK_BANDS:
-            e = h264g_launch_deblock_band(q.pics, q.dbands, q.n_dbands, &ctx->d_ctl[1], &ctx->d_ctl[2].ticket, ctx->dbk_waves,
+            e = h264g_launch_deblock_band(q.pics, q.dbands, q.n_dbands, &ctl[1], epoch, ctx->dbk_waves,
                                           ctx->d_err, s);
             break;
         case H264G_DEBLOCK_MB_WAVEFRONT:
-            e = h264g_launch_deblock(q.pics, q.all, q.n_all, &ctx->d_ctl[1], &ctx->d_ctl[2].ticket, ctx->d_err, ctx->waves, s);
+            e = h264g_launch_deblock(q.pics, q.all, q.n_all, &ctl[1], epoch, ctx->d_err, ctx->waves, s);
             break;
         default:
-            e = h264g_launch_deblock_rows(q.pics, q.rows, q.n_rows, &ctx->d_ctl[1], &ctx->d_ctl[2].ticket, ctx->d_err, ctx->waves, s);
+            e = h264g_launch_deblock_rows(q.pics, q.rows, q.n_rows, &ctl[1], epoch, ctx->d_err, ctx->waves, s);
         }
         if (e) return h264g_set_error(H264G_E_DEVICE, "deblock launch: %s", hipGetErrorString((hipError_t)e));
     }
@@ -585,11 +612,11 @@ int launch_stages(h264g_ctx *ctx, const StageSet &q) {
 // Caller holds ctx->mu.  A batch with a graph is destroyed only when idle.
 int free_batch(h264g_ctx *c, h264g_batch *b) {
     if (!b) return 0;
-    if (b->graph) {
-        (void)hipStreamSynchronize(c->stream);
-        (void)hipGraphExecDestroy(b->graph);
+    if (b->sub[0].graph || b->sub[1].graph) (void)sync_lanes(c);
+    for (Batch &sb : b->sub) {
+        if (sb.graph) (void)hipGraphExecDestroy(sb.graph);
+        pool_put(c, sb.dmem, sb.dbytes);
     }
-    pool_put(c, b->dmem, b->dbytes);
     delete b;
     return 0;
 }
@@ -634,10 +661,13 @@ int32_t h264g_create(const h264g_config *cfg, h264g_ctx **out) {
         return h264g_set_error(H264G_E_ARG, "deblock_schedule %d / deblock_waves %d", cfg->deblock_schedule, cfg->deblock_waves);
     }
     c->waves = (cfg && cfg->waves > 0) ? cfg->waves : prop.multiProcessorCount * 8;
+    c->split_mbs = !cfg || cfg->lane_split_mbs == 0 ? 16384 : (cfg->lane_split_mbs < 0 ? 0 : cfg->lane_split_mbs);
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipMalloc(&c->d_err, sizeof(uint32_t)) != hipSuccess || hipMalloc(&c->d_ctl, 4 * sizeof(DevCtl)) != hipSuccess ||
-        hipMemset(c->d_err, 0, sizeof(uint32_t)) != hipSuccess || hipMemset(c->d_ctl, 0, 4 * sizeof(DevCtl)) != hipSuccess ||
-        hipMemcpy(&c->d_ctl[2].ticket, &kFirstEpoch, sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess) {
+        hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking) != hipSuccess ||
+        hipMalloc(&c->d_err, sizeof(uint32_t)) != hipSuccess || hipMalloc(&c->d_ctl, 8 * sizeof(DevCtl)) != hipSuccess ||
+        hipMemset(c->d_err, 0, sizeof(uint32_t)) != hipSuccess || hipMemset(c->d_ctl, 0, 8 * sizeof(DevCtl)) != hipSuccess ||
+        hipMemcpy(&c->d_ctl[2].ticket, &kFirstEpoch, sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(&c->d_ctl[6].ticket, &kFirstEpoch, sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess) {
         h264g_destroy(c);
         return h264g_set_error(H264G_E_DEVICE, "context allocation failed");
     }
@@ -649,8 +679,9 @@ int32_t h264g_destroy(h264g_ctx *c) {
     if (!c) return H264G_OK;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->aux) (void)hipStreamSynchronize(c->aux);
     for (auto &b : c->pool) {
-        (void)hipEventDestroy(b.ready);
+        for (auto ev : b.ready) (void)hipEventDestroy(ev);
         (void)hipFree(b.ptr);
     }
     c->pool.clear();
@@ -664,6 +695,7 @@ int32_t h264g_destroy(h264g_ctx *c) {
     if (c->d_err) (void)hipFree(c->d_err);
     if (c->d_ctl) (void)hipFree(c->d_ctl);
     if (c->stream) (void)hipStreamDestroy(c->stream);
+    if (c->aux) (void)hipStreamDestroy(c->aux);
     delete c;
     return H264G_OK;
 }
@@ -682,6 +714,7 @@ int32_t h264g_stream_open(h264g_ctx *c, const h264g_stream_desc *d, uint32_t *id
     s.W = d->pic_width_in_mbs;
     s.H = d->frame_height_in_mbs;
     s.slot_bytes = (uint64_t)s.W * s.H * 384;
+    s.lane = c->split_mbs > 0 && s.W * s.H >= c->split_mbs ? 1 : 0;
     // DPB slots, per-MB progress words, per-row progress words, and one row of
     // deblocking strips (128 B per MB) per band of c->dbk_waves rows
     const uint64_t strip_bytes = (uint64_t)((s.H + c->dbk_waves - 1) / c->dbk_waves) * s.W * 128;
@@ -725,7 +758,7 @@ int32_t h264g_stream_close(h264g_ctx *c, uint32_t id) {
     std::lock_guard<std::mutex> lk(c->mu);
     if (id >= c->streams.size() || !c->streams[id].open) return h264g_set_error(H264G_E_ARG, "stream %u", id);
     (void)hipSetDevice(c->device);
-    (void)hipStreamSynchronize(c->stream);
+    (void)sync_lanes(c);
     Stream &s = c->streams[id];
     (void)hipFree(s.dpb);
     (void)hipFree(s.flags);
@@ -780,7 +813,7 @@ int32_t h264g_picture_submit(h264g_ctx *c, h264g_pic *p) {
     q.n_ibands = (uint32_t)((p->H + c->intra_waves - 1) / c->intra_waves);
     q.n_dbands = (uint32_t)((p->H + c->dbk_waves - 1) / c->dbk_waves);
     q.dbk_off = p->desc.disable_deblocking_filter_idc == 1;
-    return launch_stages(c, q);
+    return launch_stages(c, q, c->streams[p->stream].lane);
 }
 
 int32_t h264g_picture_release(h264g_ctx *c, h264g_pic *p) {
@@ -799,22 +832,15 @@ int32_t h264g_picture_alg_bytes(const h264g_pic *p, uint64_t out[3]) {
     return H264G_OK;
 }
 
-int32_t h264g_batch_create(h264g_ctx *c, h264g_pic *const *pics, uint32_t n, h264g_batch **out) {
-    if (!c || !pics || !out || n == 0) return h264g_set_error(H264G_E_ARG, "NULL argument");
-    *out = nullptr;
-    if (n >= (1u << (32 - WI_PIC_SHIFT))) return h264g_set_error(H264G_E_ARG, "too many pictures in one batch");
-    std::lock_guard<std::mutex> lk(c->mu);
-    HIPCHK(hipSetDevice(c->device));
-    std::vector<uint32_t> seen;
+namespace {
+// Device lists of the pictures of one lane of a batch (caller holds ctx->mu).
+int build_sub(h264g_ctx *c, const std::vector<const h264g_pic *> &pics, Batch *b) {
+    const uint32_t n = (uint32_t)pics.size();
     std::vector<DevPic> hp(n);
     std::vector<uint32_t> inter, intra, dbk;
     int maxt = 0;
     for (uint32_t i = 0; i < n; i++) {
         const h264g_pic *p = pics[i];
-        if (!p || p->ctx != c) return h264g_set_error(H264G_E_ARG, "picture %u", i);
-        if (std::find(seen.begin(), seen.end(), p->stream) != seen.end())
-            return h264g_set_error(H264G_E_ARG, "two pictures of stream %u in one batch", p->stream);
-        seen.push_back(p->stream);
         hp[i] = p->hpic;
         for (uint32_t a : p->inter_list) inter.push_back((i << WI_PIC_SHIFT) | a);
         maxt = std::max(maxt, (p->W - 1) + 2 * (p->H - 1));
@@ -862,11 +888,9 @@ int32_t h264g_batch_create(h264g_ctx *c, h264g_pic *const *pics, uint32_t n, h26
     size_t o_p = 0, o_i = al(n * sizeof(DevPic)), o_a = al(o_i + inter.size() * 4), o_l = al(o_a + intra.size() * 4);
     size_t o_r = al(o_l + all.size() * 4), o_ib = al(o_r + rows.size() * 4), o_db = al(o_ib + ibands.size() * 4);
     size_t total = al(o_db + dbands.size() * 4 + 4);
-    h264g_batch *b = new (std::nothrow) h264g_batch();
-    if (!b) return h264g_set_error(H264G_E_NOMEM, "batch");
     b->dmem = pool_get(c, total);
     b->dbytes = total;
-    if (!b->dmem) { delete b; return h264g_set_error(H264G_E_NOMEM, "batch device memory"); }
+    if (!b->dmem) return h264g_set_error(H264G_E_NOMEM, "batch device memory");
     char *base = (char *)b->dmem;
     b->d_pics = (DevPic *)(base + o_p);
     b->n_pics = n;
@@ -889,16 +913,44 @@ int32_t h264g_batch_create(h264g_ctx *c, h264g_pic *const *pics, uint32_t n, h26
         (!all.empty() && hipMemcpy(b->d_all, all.data(), all.size() * 4, hipMemcpyHostToDevice) != hipSuccess) ||
         (!rows.empty() && hipMemcpy(b->d_rows, rows.data(), rows.size() * 4, hipMemcpyHostToDevice) != hipSuccess) ||
         (!ibands.empty() && hipMemcpy(b->d_ibands, ibands.data(), ibands.size() * 4, hipMemcpyHostToDevice) != hipSuccess) ||
-        (!dbands.empty() && hipMemcpy(b->d_dbands, dbands.data(), dbands.size() * 4, hipMemcpyHostToDevice) != hipSuccess)) {
-        free_batch(c, b);
+        (!dbands.empty() && hipMemcpy(b->d_dbands, dbands.data(), dbands.size() * 4, hipMemcpyHostToDevice) != hipSuccess))
         return h264g_set_error(H264G_E_DEVICE, "batch upload");
+    return H264G_OK;
+}
+}  // namespace
+
+int32_t h264g_batch_create(h264g_ctx *c, h264g_pic *const *pics, uint32_t n, h264g_batch **out) {
+    if (!c || !pics || !out || n == 0) return h264g_set_error(H264G_E_ARG, "NULL argument");
+    *out = nullptr;
+    if (n >= (1u << (32 - WI_PIC_SHIFT))) return h264g_set_error(H264G_E_ARG, "too many pictures in one batch");
+    std::lock_guard<std::mutex> lk(c->mu);
+    HIPCHK(hipSetDevice(c->device));
+    std::vector<uint32_t> seen;
+    std::vector<const h264g_pic *> lane[2];
+    for (uint32_t i = 0; i < n; i++) {
+        const h264g_pic *p = pics[i];
+        if (!p || p->ctx != c) return h264g_set_error(H264G_E_ARG, "picture %u", i);
+        if (std::find(seen.begin(), seen.end(), p->stream) != seen.end())
+            return h264g_set_error(H264G_E_ARG, "two pictures of stream %u in one batch", p->stream);
+        seen.push_back(p->stream);
+        lane[c->streams[p->stream].lane].push_back(p);
+    }
+    h264g_batch *b = new (std::nothrow) h264g_batch();
+    if (!b) return h264g_set_error(H264G_E_NOMEM, "batch");
+    for (int l = 0; l < 2; l++) {
+        if (lane[l].empty()) continue;
+        int rc = build_sub(c, lane[l], &b->sub[l]);
+        if (rc) {
+            free_batch(c, b);
+            return rc;
+        }
     }
     *out = b;
     return H264G_OK;
 }
 
 namespace {
-StageSet batch_stages(const h264g_batch *b) {
+StageSet batch_stages(const Batch *b) {
     StageSet q;
     q.pics = b->d_pics;
     q.n_pics = b->n_pics;
@@ -923,22 +975,30 @@ int32_t h264g_batch_run(h264g_ctx *c, h264g_batch *b) {
     if (!c || !b) return h264g_set_error(H264G_E_ARG, "NULL argument");
     std::lock_guard<std::mutex> lk(c->mu);
     HIPCHK(hipSetDevice(c->device));
-    if (c->profile || c->no_graphs)
-        return launch_stages(c, batch_stages(b));
-    if (!b->graph) {
-        // Capture the three stage launches once; every launch argument (including the
-        // wavefront epoch, which lives in device memory) is the same on each replay.
-        HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        int rc = launch_stages(c, batch_stages(b));
-        hipGraph_t g = nullptr;
-        hipError_t e = hipStreamEndCapture(c->stream, &g);
-        if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
-        if (e != hipSuccess) return h264g_set_error(H264G_E_DEVICE, "stream capture: %s", hipGetErrorString(e));
-        e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(g);
-        if (e != hipSuccess) { b->graph = nullptr; return h264g_set_error(H264G_E_DEVICE, "graph instantiate: %s", hipGetErrorString(e)); }
+    for (int l = 0; l < 2; l++) {
+        Batch &sb = b->sub[l];
+        if (!sb.n_pics) continue;
+        if (c->profile || c->no_graphs) {  // direct launches (profiled runs record events on the lane)
+            int rc = launch_stages(c, batch_stages(&sb), l);
+            if (rc) return rc;
+            continue;
+        }
+        hipStream_t s = lane_stream(c, l);
+        if (!sb.graph) {
+            // Capture the stage launches once; every launch argument (including the
+            // wavefront epoch, which lives in device memory) is the same on each replay.
+            HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            int rc = launch_stages(c, batch_stages(&sb), l);
+            hipGraph_t g = nullptr;
+            hipError_t e = hipStreamEndCapture(s, &g);
+            if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+            if (e != hipSuccess) return h264g_set_error(H264G_E_DEVICE, "stream capture: %s", hipGetErrorString(e));
+            e = hipGraphInstantiate(&sb.graph, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e != hipSuccess) { sb.graph = nullptr; return h264g_set_error(H264G_E_DEVICE, "graph instantiate: %s", hipGetErrorString(e)); }
+        }
+        HIPCHK(hipGraphLaunch(sb.graph, s));
     }
-    HIPCHK(hipGraphLaunch(b->graph, c->stream));
     return H264G_OK;
 }
 
@@ -956,8 +1016,9 @@ int32_t h264g_batch_submit(h264g_ctx *c, h264g_pic *const *pics, uint32_t n) {
     if (rc) return rc;
     std::lock_guard<std::mutex> lk(c->mu);
     HIPCHK(hipSetDevice(c->device));
-    rc = launch_stages(c, batch_stages(b));  // used once: launched directly, not captured
-    free_batch(c, b);                        // its block is reused after these launches
+    for (int l = 0; l < 2 && !rc; l++)  // used once: launched directly, not captured
+        if (b->sub[l].n_pics) rc = launch_stages(c, batch_stages(&b->sub[l]), l);
+    free_batch(c, b);                   // its blocks are reused after these launches
     return rc;
 }
 
@@ -965,7 +1026,7 @@ int32_t h264g_sync(h264g_ctx *c) {
     if (!c) return h264g_set_error(H264G_E_ARG, "ctx");
     std::lock_guard<std::mutex> lk(c->mu);
     HIPCHK(hipSetDevice(c->device));
-    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(sync_lanes(c));
     uint32_t err = 0;
     HIPCHK(hipMemcpy(&err, c->d_err, 4, hipMemcpyDeviceToHost));
     if (err) {
@@ -975,44 +1036,60 @@ int32_t h264g_sync(h264g_ctx *c) {
     return H264G_OK;
 }
 
+// A fence is one event per lane, recorded after the work enqueued so far on it.
+struct h264g_fence {
+    hipEvent_t ev[2];
+};
+
 int32_t h264g_fence_record(h264g_ctx *c, h264g_fence **out) {
     if (!c || !out) return h264g_set_error(H264G_E_ARG, "NULL argument");
     *out = nullptr;
     std::lock_guard<std::mutex> lk(c->mu);
     HIPCHK(hipSetDevice(c->device));
-    hipEvent_t e = nullptr;
-    HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-    if (hipEventRecord(e, c->stream) != hipSuccess) {
-        (void)hipEventDestroy(e);
-        return h264g_set_error(H264G_E_DEVICE, "fence record");
+    h264g_fence *f = new (std::nothrow) h264g_fence{{nullptr, nullptr}};
+    if (!f) return h264g_set_error(H264G_E_NOMEM, "fence");
+    for (int l = 0; l < 2; l++) {
+        if (hipEventCreateWithFlags(&f->ev[l], hipEventDisableTiming) != hipSuccess ||
+            hipEventRecord(f->ev[l], lane_stream(c, l)) != hipSuccess) {
+            for (auto e : f->ev) if (e) (void)hipEventDestroy(e);
+            delete f;
+            return h264g_set_error(H264G_E_DEVICE, "fence record");
+        }
     }
-    *out = reinterpret_cast<h264g_fence *>(e);
+    *out = f;
     return H264G_OK;
 }
 
 int32_t h264g_fence_query(h264g_ctx *c, h264g_fence *f) {
     if (!c || !f) return h264g_set_error(H264G_E_ARG, "NULL argument");
-    hipError_t e = hipEventQuery(reinterpret_cast<hipEvent_t>(f));
-    if (e == hipSuccess) return 1;
-    if (e == hipErrorNotReady) return 0;
-    return h264g_set_error(H264G_E_DEVICE, "fence query: %s", hipGetErrorString(e));
+    for (auto ev : f->ev) {
+        hipError_t e = hipEventQuery(ev);
+        if (e == hipErrorNotReady) return 0;
+        if (e != hipSuccess) return h264g_set_error(H264G_E_DEVICE, "fence query: %s", hipGetErrorString(e));
+    }
+    return 1;
 }
 
 int32_t h264g_fence_wait(h264g_ctx *c, h264g_fence *f) {
     if (!c || !f) return h264g_set_error(H264G_E_ARG, "NULL argument");
-    HIPCHK(hipEventSynchronize(reinterpret_cast<hipEvent_t>(f)));
+    for (auto ev : f->ev) HIPCHK(hipEventSynchronize(ev));
     return H264G_OK;
 }
 
 int32_t h264g_fence_destroy(h264g_ctx *c, h264g_fence *f) {
     (void)c;
-    if (f) (void)hipEventDestroy(reinterpret_cast<hipEvent_t>(f));
+    if (f) {
+        for (auto ev : f->ev) (void)hipEventDestroy(ev);
+        delete f;
+    }
     return H264G_OK;
 }
 
 int32_t h264g_set_profile(h264g_ctx *c, int32_t on) {
     if (!c) return h264g_set_error(H264G_E_ARG, "ctx");
     std::lock_guard<std::mutex> lk(c->mu);
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(sync_lanes(c));
     c->profile = on != 0;
     return H264G_OK;
 }
@@ -1021,7 +1098,7 @@ int32_t h264g_stage_times(h264g_ctx *c, float out[4]) {
     if (!c || !out || !c->profile) return h264g_set_error(H264G_E_STATE, "profiling is off");
     std::lock_guard<std::mutex> lk(c->mu);
     HIPCHK(hipSetDevice(c->device));
-    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(sync_lanes(c));
     out[0] = out[1] = out[2] = out[3] = 0.f;
     const size_t runs = c->ev_used / 5;
     for (size_t r = 0; r < runs; r++)
@@ -1060,7 +1137,7 @@ int32_t h264g_slot_readback(h264g_ctx *c, uint32_t sid, int32_t slot, uint8_t *y
     Stream &s = c->streams[sid];
     if (slot < 0 || slot >= s.desc.dpb_slots) return h264g_set_error(H264G_E_ARG, "slot %d", slot);
     HIPCHK(hipSetDevice(c->device));
-    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipStreamSynchronize(lane_stream(c, s.lane)));  // the lane that writes the stream
     const uint8_t *base = s.dpb + (uint64_t)slot * s.slot_bytes;
     size_t ny = (size_t)s.W * s.H * 256, nc = (size_t)s.W * s.H * 64;
     HIPCHK(hipMemcpy(y, base, ny, hipMemcpyDeviceToHost));
@@ -1081,8 +1158,9 @@ int32_t h264g_slot_to_bgr24(h264g_ctx *c, uint32_t sid, int32_t slot, uint8_t *b
     const size_t bytes = (size_t)stride * hl;
     uint8_t *d = (uint8_t *)pool_get(c, bytes);
     if (!d) return h264g_set_error(H264G_E_NOMEM, "conversion buffer");
-    int e = h264g_launch_bgr24(s.dpb + (uint64_t)slot * s.slot_bytes, wl, hl, d, stride, flip != 0, c->stream);
-    hipError_t he = e ? (hipError_t)e : hipStreamSynchronize(c->stream);
+    hipStream_t ls = lane_stream(c, s.lane);  // after the stream's pictures
+    int e = h264g_launch_bgr24(s.dpb + (uint64_t)slot * s.slot_bytes, wl, hl, d, stride, flip != 0, ls);
+    hipError_t he = e ? (hipError_t)e : hipStreamSynchronize(ls);
     if (he == hipSuccess) he = hipMemcpy(bgr, d, bytes, hipMemcpyDeviceToHost);
     pool_put(c, d, bytes);
     if (he != hipSuccess) return h264g_set_error(H264G_E_DEVICE, "bgr24 conversion: %s", hipGetErrorString(he));
@@ -1096,7 +1174,7 @@ int32_t h264g_slot_upload(h264g_ctx *c, uint32_t sid, int32_t slot, const uint8_
     Stream &s = c->streams[sid];
     if (slot < 0 || slot >= s.desc.dpb_slots) return h264g_set_error(H264G_E_ARG, "slot %d", slot);
     HIPCHK(hipSetDevice(c->device));
-    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipStreamSynchronize(lane_stream(c, s.lane)));
     uint8_t *base = s.dpb + (uint64_t)slot * s.slot_bytes;
     size_t ny = (size_t)s.W * s.H * 256, nc = (size_t)s.W * s.H * 64;
     HIPCHK(hipMemcpy(base, y, ny, hipMemcpyHostToDevice));

@@ -124,14 +124,16 @@ class Engine:
 
     def __init__(self, device: int = 0, dpb_budget_bytes: int = 0, profile: bool = False, waves: int = 0,
                  max_batch_pictures: int = 0, graphs: bool = True, deblock_schedule: int = 0, deblock_waves: int = 0,
-                 intra_schedule: int = 0, intra_waves: int = 0):
+                 intra_schedule: int = 0, intra_waves: int = 0, lane_split_mbs: int = 0):
         """deblock_schedule: _lib.DEBLOCK_BANDS (default), DEBLOCK_MB_WAVEFRONT or DEBLOCK_MB_ROWS;
         intra_schedule: _lib.INTRA_BANDS (default) or INTRA_MB_WAVEFRONT.  deblock_waves /
-        intra_waves: MB rows per band of the band schedules (0 = 16 / 8)."""
+        intra_waves: MB rows per band of the band schedules (0 = 16 / 8).  lane_split_mbs:
+        streams with pictures of at least this many MBs run on the context's second HIP
+        stream (0 = 16384, negative = one stream)."""
         cfg = Config(device=device, dpb_budget_bytes=dpb_budget_bytes, max_batch_pictures=max_batch_pictures,
                      profile=1 if profile else 0, waves=waves, no_graphs=0 if graphs else 1,
                      deblock_schedule=deblock_schedule, deblock_waves=deblock_waves,
-                     intra_schedule=intra_schedule, intra_waves=intra_waves)
+                     intra_schedule=intra_schedule, intra_waves=intra_waves, lane_split_mbs=lane_split_mbs)
         h = c_void_p()
         check(lib().h264g_create(byref(cfg), byref(h)))
         self._h = h

@@ -148,6 +148,10 @@ typedef struct h264g_config {
     int32_t  intra_schedule;      /* H264G_INTRA_*                                    */
     int32_t  intra_waves;         /* rows per band = waves per workgroup of
                                      H264G_INTRA_BANDS, 1..8 (0 = 8)                 */
+    int32_t  lane_split_mbs;      /* streams of at least this many MBs per picture run
+                                     on a second HIP stream, so their long intra /
+                                     deblocking chains overlap the other pictures'
+                                     stages (0 = 16384; negative = one stream)       */
 } h264g_config;
 
 /* Intra schedules: BANDS = pictures cut into bands of `intra_waves` MB rows, one
@@ -256,11 +260,12 @@ int32_t h264g_batch_create(h264g_ctx *ctx, h264g_pic *const *pics, uint32_t n_pi
 int32_t h264g_batch_run(h264g_ctx *ctx, h264g_batch *batch);
 int32_t h264g_batch_destroy(h264g_ctx *ctx, h264g_batch *batch);
 
-/* Device-side time per stage, in milliseconds, summed over every
- * h264g_batch_run / h264g_picture_submit since the previous call (HIP events
- * recorded around each stage on the launch stream): out[0] inter, out[1] intra,
- * out[2] boundary strengths, out[3] deblocking.  Waits for the stream; returns the
- * number of runs summed.  Requires profiling (h264g_config.profile or
+/* Device-side time per stage, in milliseconds, summed over every stage sequence
+ * launched since the previous call (HIP events recorded around each stage on the
+ * launch stream; a batch with pictures on both lanes launches one sequence per
+ * lane, and the two lanes' sequences may overlap): out[0] inter, out[1] intra,
+ * out[2] boundary strengths, out[3] deblocking.  Waits for both lanes; returns the
+ * number of sequences summed.  Requires profiling (h264g_config.profile or
  * h264g_set_profile). */
 int32_t h264g_stage_times(h264g_ctx *ctx, float out[4]);
 

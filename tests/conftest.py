@@ -23,9 +23,11 @@ def golden():
 
 @pytest.fixture(scope="session")
 def engine():
-    """One engine context for all GPU tests of the session (fails loudly without the HIP library)."""
+    """One engine context for all GPU tests of the session (fails loudly without the HIP
+    library).  Streams of at least 500 MBs per picture run on its second HIP stream, so
+    batches of the golden cases use both lanes."""
     from h264_video_decoder_demo_amd import Engine
-    eng = Engine(device=0)
+    eng = Engine(device=0, lane_split_mbs=500)
     yield eng
     eng.close()
 
@@ -41,10 +43,10 @@ def engines():
         e.close()
 
 
-@pytest.fixture(params=["bands", "mb_wavefront", "mb_rows", "bands_small", "bands_single_row"])
+@pytest.fixture(params=["bands", "mb_wavefront", "mb_rows", "bands_small", "bands_single_row", "bands_two_lanes"])
 def any_engine(request, engines):
     """Every intra / deblocking schedule; bands of 3 / 2 rows and of single rows (every row
-    a band boundary)."""
+    a band boundary); streams of 500 MBs and more on the second HIP stream."""
     from h264_video_decoder_demo_amd import Engine
     from h264_video_decoder_demo_amd import _lib
     kw = {"bands": dict(deblock_schedule=_lib.DEBLOCK_BANDS, intra_schedule=_lib.INTRA_BANDS),
@@ -53,7 +55,8 @@ def any_engine(request, engines):
           "bands_small": dict(deblock_schedule=_lib.DEBLOCK_BANDS, deblock_waves=3,
                               intra_schedule=_lib.INTRA_BANDS, intra_waves=2),
           "bands_single_row": dict(deblock_schedule=_lib.DEBLOCK_BANDS, deblock_waves=1,
-                                   intra_schedule=_lib.INTRA_BANDS, intra_waves=1)}[request.param]
+                                   intra_schedule=_lib.INTRA_BANDS, intra_waves=1),
+          "bands_two_lanes": dict(lane_split_mbs=500)}[request.param]
     if request.param not in engines:
         engines[request.param] = Engine(device=0, **kw)
     return engines[request.param]
