@@ -46,7 +46,6 @@ __constant__ uint8_t c_tc0[3][52] = {
 __device__ __forceinline__ int clip3(int lo, int hi, int v) { return v < lo ? lo : (v > hi ? hi : v); }
 __device__ __forceinline__ int clip1(int v) { return clip3(0, 255, v); }
 __device__ __forceinline__ int iabs(int v) { return v < 0 ? -v : v; }
-__device__ __forceinline__ int tap6(int a, int b, int c, int d, int e, int f) { return a - 5 * b + 20 * c + 20 * d - 5 * e + f; }
 
 __device__ __forceinline__ int ls4(int m, int i, int j) {
     int cls = ((i | j) & 1) == 0 ? 0 : (((i & j) & 1) ? 1 : 2);
@@ -83,13 +82,28 @@ __device__ __forceinline__ void wave_sync() {
 // LDS scratch of one wave: dense scan-order levels of the MB (slots: 0 = Intra16x16 DC,
 // 1 + luma4x4BlkIdx (8x8 block q = slots 1+4q..4+4q), 17/18 = chroma DC Cb/Cr,
 // 19 + 4c + b = chroma AC), then the residual planes.
+// Luma interpolation of one list (inter MBs, after the residual is built).  Window row rr
+// of 4x4 block b holds reference row y0 - 2 + rr at columns x0 - 2 .. x0 + 5, where
+// (x0, y0) is the block's integer sample position; b1 is the unrounded horizontal 6-tap
+// value between columns x0 + i and x0 + i + 1 (8-241).
+struct WaveMc {
+    uint32_t raw[16][9][2];   // window columns 0..7, packed bytes
+    uint32_t hb[16][5];       // b = Clip1((b1 + 16) >> 5), i = 0..3, of window rows 2..6
+    int16_t b1t[16][4][10];   // b1 of column i, window rows 0..8 (+ one pad row): [b][i][rr]
+};
+
 struct WaveRes {
-    int32_t coef[27][16];
-    int32_t t8[4][64];    // 8x8 transform: rows after the horizontal pass
+    union {
+        struct {
+            int32_t coef[27][16];
+            int32_t t8[4][64];    // 8x8 transform: rows after the horizontal pass
+        };
+        WaveMc mc;                // live only after the residual planes are built
+    };
     int16_t rl[256];      // luma residual, raster 16x16
     int16_t rc[2][64];    // chroma residual, raster 8x8
-    uint8_t win[16][9][12];  // inter: 9x9 reference window of each 4x4 luma block (one list)
 };
+static_assert(sizeof(WaveMc) <= 27 * 16 * 4 + 4 * 64 * 4, "interpolation buffer overlays the levels");
 
 __device__ int level_slot(int i, int i16, int cbp, int t8) {
     int idx = 0;
@@ -267,105 +281,147 @@ __device__ __forceinline__ int ldc(const uint8_t *p, int w, int h, int x, int y)
 
 __device__ __forceinline__ uint8_t *slot_base(const DevPic &P, int s) { return P.dpb + (uint64_t)s * P.slot_bytes; }
 
-// Four horizontally adjacent luma prediction samples at integer position (x0, y0) with
-// fractional offset (xF, yF), 8.4.2.2.1 (H264InterPrediction.cpp:2344-2480), from the 6x9
-// window w[r][c] = sample (x0 - 2 + c, y0 - 2 + r) (coordinates clamped to the picture).
-__device__ __forceinline__ void luma4_win(const int win[6][9], int xF, int yF, int out[4]) {
-    if (xF == 0 && yF == 0) {
-#pragma unroll
-        for (int i = 0; i < 4; i++) out[i] = win[2][i + 2];
-        return;
-    }
-    if (yF == 0) {  // a, b, c
-#pragma unroll
-        for (int i = 0; i < 4; i++) {
-            int b = clip1((tap6(win[2][i], win[2][i + 1], win[2][i + 2], win[2][i + 3], win[2][i + 4], win[2][i + 5]) + 16) >> 5);
-            out[i] = xF == 2 ? b : (xF == 1 ? (win[2][i + 2] + b + 1) >> 1 : (win[2][i + 3] + b + 1) >> 1);
-        }
-        return;
-    }
-    if (xF == 0) {  // d, h, n
-#pragma unroll
-        for (int i = 0; i < 4; i++) {
-            int hh = clip1((tap6(win[0][i + 2], win[1][i + 2], win[2][i + 2], win[3][i + 2], win[4][i + 2], win[5][i + 2]) + 16) >> 5);
-            out[i] = yF == 2 ? hh : (yF == 1 ? (win[2][i + 2] + hh + 1) >> 1 : (win[3][i + 2] + hh + 1) >> 1);
-        }
-        return;
-    }
-    int v[5];  // clipped vertical half samples h at columns i = 0..4
-#pragma unroll
-    for (int i = 0; i < 5; i++)
-        v[i] = clip1((tap6(win[0][i + 2], win[1][i + 2], win[2][i + 2], win[3][i + 2], win[4][i + 2], win[5][i + 2]) + 16) >> 5);
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-        int b = clip1((tap6(win[2][i], win[2][i + 1], win[2][i + 2], win[2][i + 3], win[2][i + 4], win[2][i + 5]) + 16) >> 5);
-        int s = clip1((tap6(win[3][i], win[3][i + 1], win[3][i + 2], win[3][i + 3], win[3][i + 4], win[3][i + 5]) + 16) >> 5);
-        int hh = v[i], m = v[i + 1];
-        int o;
-        if ((xF & 1) && (yF & 1)) {  // e, g, p, r
-            o = ((yF == 1 ? b : s) + (xF == 1 ? hh : m) + 1) >> 1;
-        } else {
-            int j1 = 0;
-#pragma unroll
-            for (int r = 0; r < 6; r++) {
-                const int b1 = tap6(win[r][i], win[r][i + 1], win[r][i + 2], win[r][i + 3], win[r][i + 4], win[r][i + 5]);
-                j1 += (r == 0 || r == 5 ? 1 : (r == 1 || r == 4 ? -5 : 20)) * b1;
-            }
-            const int j = clip1((j1 + 512) >> 10);
-            if (xF == 2 && yF == 2) o = j;
-            else if (xF == 2) o = (j + (yF == 1 ? b : s) + 1) >> 1;  // f, q
-            else o = (j + (xF == 1 ? hh : m) + 1) >> 1;                // i, k
-        }
-        out[i] = o;
-    }
+// (P1 + P2 + 1) >> 1 of every luma position (xF, yF), 8.4.2.2.1 (H264InterPrediction.cpp:2344-2480),
+// with P1, P2 among 0 = G (integer sample), 1 = b (horizontal half sample), 2 = h
+// (vertical half sample), 3 = j: 4 bits (P1 | P2 << 2) at bit 4 * (4 * yF + xF).  The
+// quarter positions right of or below the half samples read them one column right
+// (m instead of h, G at x + 1) or one row down (s instead of b, G at y + 1).
+constexpr uint64_t mc_sel_table() {
+    const int t[16][2] = {{0, 0}, {0, 1}, {1, 1}, {0, 1}, {0, 2}, {1, 2}, {1, 3}, {1, 2},
+                          {2, 2}, {2, 3}, {3, 3}, {2, 3}, {0, 2}, {1, 2}, {1, 3}, {1, 2}};
+    uint64_t v = 0;
+    for (int i = 0; i < 16; i++) v |= (uint64_t)(t[i][0] | t[i][1] << 2) << (4 * i);
+    return v;
 }
+constexpr uint64_t MC_SEL = mc_sel_table();
+// 6-tap (1, -5, 20, 20, -5, 1) as signed byte weights for v_dot4: samples s..s+3 and
+// s+4, s+5 (bytes 0-1; bytes 2-3 for the second column of a pair).  The samples enter
+// as x ^ 0x80 = x - 128, so every tap sum is short by 128 * 32 = 4096.
+constexpr int TAP_LO = 0x1414fb01, TAP_HI = 0x000001fb, TAP_HI2 = 0x01fb0000, TAP_BIAS = 4096;
 
-// Luma prediction samples of list l for the whole MB (lane = 4x4 block b, row r; four
-// samples each).  The 9x9 windows of all 16 blocks are staged into LDS with dword loads
-// of whole window rows (clamped byte loads only where a window leaves the picture), then
-// every lane reads its 6x9 window from LDS.  Whole wave; out is left 0 where the lane's
-// quadrant does not use list l.
-__device__ void luma_list(const DevPic &P, const h264g_mb *mb, const DevAux &ax, int l, WaveRes *ws, int out[4]) {
+__device__ __forceinline__ uint32_t align8(uint32_t hi, uint32_t lo, uint32_t bytes) {
+    return __builtin_amdgcn_alignbyte(hi, lo, bytes);
+}
+__device__ __forceinline__ int sdot4(uint32_t a, int w, int acc) { return __builtin_amdgcn_sdot4((int)a, w, acc, false); }
+__device__ __forceinline__ int sdot2(uint32_t a, short w0, short w1, int acc) {
+    typedef short short2_t __attribute__((ext_vector_type(2)));
+    return __builtin_amdgcn_sdot2(__builtin_bit_cast(short2_t, a), (short2_t){w0, w1}, acc, false);
+}
+// Low bytes of v0..v3 as one dword, through v_perm.  Written with shifts and ORs, hipcc
+// fuses Clip1(x >> n) of two values into v_ashr_pk_u8_i32 and ORs the other two bytes
+// onto its destination, whose upper half then still holds the register's old contents
+// on gfx950; the permutes keep that instruction out.
+__device__ __forceinline__ uint32_t pack4(int v0, int v1, int v2, int v3) {
+    const uint32_t lo = __builtin_amdgcn_perm((uint32_t)v1, (uint32_t)v0, 0x0c0c0400u);
+    const uint32_t hi = __builtin_amdgcn_perm((uint32_t)v3, (uint32_t)v2, 0x0c0c0400u);
+    return __builtin_amdgcn_perm(hi, lo, 0x05040100u);
+}
+// Per-byte (a + b + 1) >> 1.
+__device__ __forceinline__ uint32_t avg4(uint32_t a, uint32_t b) { return (a | b) - (((a ^ b) >> 1) & 0x7f7f7f7fu); }
+
+// Luma prediction samples of list l: lane = (4x4 block b = lane & 15, row r = lane >> 4),
+// four samples packed in a dword (0 where the lane's quadrant does not use list l).
+// Stage 1 (lane = block b, window rows r, r + 4, r + 8): dword loads of the window row,
+// the horizontal 6-tap values b1 of its four columns (v_dot4), all into LDS.  Stage 2:
+// the lane's vertical 6-taps on window bytes (h, or m) and on b1 (j), and the pair of
+// the position's inputs.  Both stages are the same for every (xF, yF), so lanes with
+// different quarter-sample positions do not diverge; the wave skips what no lane uses.
+__device__ __forceinline__ uint32_t luma_list(const DevPic &P, const h264g_mb *mb, const DevAux &ax, int l, WaveRes *ws) {
     const int lane = wave_lane();
     const int W = P.W, wl = W * 16, hl = P.H * 16;
     const int addr = (int)(mb - P.mbs), bx0 = (addr % W) * 16, by0 = (addr / W) * 16;
-    // phase 1: 16 blocks x 9 rows x 3 dwords
-    for (int i = lane; i < 16 * 27; i += WAVE) {
-        const int b = i / 27, k = i % 27, rr = k / 3, dw = k % 3;
-        const int q = ((b >> 3) << 1) | ((b & 3) >> 1);
-        if (!((ax.pf >> (4 * l + q)) & 1)) continue;
+    const int b = lane & 15, r = lane >> 4, bx = b & 3, by = b >> 2, q = ((by >> 1) << 1) | (bx >> 1);
+    const bool use = (ax.pf >> (4 * l + q)) & 1;
+    const int mvx = mb->mv[l][b][0], mvy = mb->mv[l][b][1];
+    const int xF = mvx & 3, yF = mvy & 3;
+    const int sel = (int)((MC_SEL >> (4 * (4 * yF + xF))) & 15), s1 = sel & 3, s2 = sel >> 2;
+    const uint32_t uses = use ? (1u << s1) | (1u << s2) : 0u;
+    const bool need_b = __ballot(uses & 2) != 0, need_h = __ballot(uses & 4) != 0, need_j = __ballot(uses & 8) != 0;
+    const int x0 = bx0 + bx * 4 + (mvx >> 2), y0 = by0 + by * 4 + (mvy >> 2);
+    WaveMc &mc = ws->mc;
+    if (use) {
         const uint8_t *ref = slot_base(P, ax.refid[l][q]);
-        const int x0 = bx0 + (b & 3) * 4 + (mb->mv[l][b][0] >> 2), y0 = by0 + (b >> 2) * 4 + (mb->mv[l][b][1] >> 2);
-        const int xs = x0 - 2, a = xs & ~3, off = xs - a, y = y0 - 2 + rr;
-        uint32_t v;
-        if (a >= 0 && a + 12 <= wl && y >= 0 && y < hl) {
-            v = *(const uint32_t *)&ref[(uint64_t)y * wl + a + 4 * dw];
-        } else {
-            v = 0;
+        const int xs = x0 - 2, xa = xs & ~3, off = xs & 3;
+        const bool inside = xa >= 0 && xa + 12 <= wl;
 #pragma unroll
-            for (int t = 0; t < 4; t++) v |= (uint32_t)ldc(ref, wl, hl, a + 4 * dw + t, y) << (8 * t);
-        }
+        for (int k = 0; k < 3; k++) {
+            const int rr = r + 4 * k;
+            if (rr < 9) {
+                const uint8_t *row = ref + (uint64_t)clip3(0, hl - 1, y0 - 2 + rr) * wl;
+                uint32_t w0, w1, w2;
+                if (inside) {
+                    const uint32_t *p = (const uint32_t *)(row + xa);
+                    const uint32_t d0 = p[0], d1 = p[1], d2 = p[2];
+                    w0 = align8(d1, d0, off); w1 = align8(d2, d1, off); w2 = d2 >> (8 * off);
+                } else {  // the window leaves the picture: clamped columns (8.4.2.2.1)
+                    int v[9];
 #pragma unroll
-        for (int t = 0; t < 4; t++) {
-            const int p = 4 * dw + t - off;  // position in the window row
-            if (p >= 0 && p < 12) ws->win[b][rr][p] = (uint8_t)(v >> (8 * t));
+                    for (int t = 0; t < 9; t++) v[t] = row[clip3(0, wl - 1, xs + t)];
+                    w0 = pack4(v[0], v[1], v[2], v[3]); w1 = pack4(v[4], v[5], v[6], v[7]); w2 = (uint32_t)v[8];
+                }
+                mc.raw[b][rr][0] = w0;
+                mc.raw[b][rr][1] = w1;
+                if (need_b || need_j) {
+                    const uint32_t s0 = w0 ^ 0x80808080u, sx = w1 ^ 0x80808080u, sy = w2 ^ 0x80u;
+                    int hv[4];
+#pragma unroll
+                    for (int i = 0; i < 4; i++) {
+                        const uint32_t lo = i ? align8(sx, s0, i) : s0, hi = i ? align8(sy, sx, i) : sx;
+                        hv[i] = sdot4(hi, TAP_HI, sdot4(lo, TAP_LO, TAP_BIAS));
+                        mc.b1t[b][i][rr] = (int16_t)hv[i];
+                    }
+                    if (rr >= 2 && rr <= 6)
+                        mc.hb[b][rr - 2] = pack4(clip1((hv[0] + 16) >> 5), clip1((hv[1] + 16) >> 5), clip1((hv[2] + 16) >> 5),
+                                                 clip1((hv[3] + 16) >> 5));
+                }
+            }
         }
     }
     wave_sync();
-    // phase 2
-    const int b = lane >> 2, r = lane & 3, q = ((b >> 3) << 1) | ((b & 3) >> 1);
-    if ((ax.pf >> (4 * l + q)) & 1) {
-        int win[6][9];
+    uint32_t out = 0;
+    if (use) {
+        const int dx = xF == 3, dy = yF == 3;
+        uint32_t R[6];
 #pragma unroll
-        for (int rr = 0; rr < 6; rr++) {
-            const uint32_t *row = (const uint32_t *)&ws->win[b][r + rr][0];
-            const uint32_t d0 = row[0], d1 = row[1], d2 = row[2];
-#pragma unroll
-            for (int c = 0; c < 9; c++) win[rr][c] = ((c < 4 ? d0 : (c < 8 ? d1 : d2)) >> (8 * (c & 3))) & 255;
+        for (int k = 0; k < 6; k++) R[k] = align8(mc.raw[b][r + k][1], mc.raw[b][r + k][0], 2 + dx);
+        uint32_t cand[4];
+        cand[0] = dy ? R[3] : R[2];
+        cand[1] = need_b ? mc.hb[b][r + dy] : 0u;
+        cand[2] = 0;
+        if (need_h) {  // vertical 6-tap of the four columns: byte transpose, then v_dot4
+            const uint32_t t0 = __builtin_amdgcn_perm(R[1], R[0], 0x05010400u), t1 = __builtin_amdgcn_perm(R[1], R[0], 0x07030602u);
+            const uint32_t t2 = __builtin_amdgcn_perm(R[3], R[2], 0x05010400u), t3 = __builtin_amdgcn_perm(R[3], R[2], 0x07030602u);
+            const uint32_t c0 = __builtin_amdgcn_perm(t2, t0, 0x05040100u) ^ 0x80808080u;
+            const uint32_t c1 = __builtin_amdgcn_perm(t2, t0, 0x07060302u) ^ 0x80808080u;
+            const uint32_t c2 = __builtin_amdgcn_perm(t3, t1, 0x05040100u) ^ 0x80808080u;
+            const uint32_t c3 = __builtin_amdgcn_perm(t3, t1, 0x07060302u) ^ 0x80808080u;
+            const uint32_t e01 = __builtin_amdgcn_perm(R[5], R[4], 0x05010400u) ^ 0x80808080u;
+            const uint32_t e23 = __builtin_amdgcn_perm(R[5], R[4], 0x07030602u) ^ 0x80808080u;
+            const int h0 = sdot4(e01, TAP_HI, sdot4(c0, TAP_LO, TAP_BIAS + 16));
+            const int h1 = sdot4(e01, TAP_HI2, sdot4(c1, TAP_LO, TAP_BIAS + 16));
+            const int h2 = sdot4(e23, TAP_HI, sdot4(c2, TAP_LO, TAP_BIAS + 16));
+            const int h3 = sdot4(e23, TAP_HI2, sdot4(c3, TAP_LO, TAP_BIAS + 16));
+            cand[2] = pack4(clip1(h0 >> 5), clip1(h1 >> 5), clip1(h2 >> 5), clip1(h3 >> 5));
         }
-        luma4_win(win, mb->mv[l][b][0] & 3, mb->mv[l][b][1] & 3, out);
+        cand[3] = 0;
+        if (need_j) {  // vertical 6-tap over b1 (8-243) with v_dot2 on row pairs
+            const uint32_t sh = 2 * (r & 1);
+            int jv[4];
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const uint32_t *c = (const uint32_t *)&mc.b1t[b][i][r & ~1];
+                const uint32_t d0 = c[0], d1 = c[1], d2 = c[2], d3 = c[3];
+                const uint32_t p0 = align8(d1, d0, sh), p1 = align8(d2, d1, sh), p2 = align8(d3, d2, sh);
+                jv[i] = clip1(sdot2(p2, -5, 1, sdot2(p1, 20, 20, sdot2(p0, 1, -5, 512))) >> 10);
+            }
+            cand[3] = pack4(jv[0], jv[1], jv[2], jv[3]);
+        }
+        const uint32_t a = s1 == 0 ? cand[0] : s1 == 1 ? cand[1] : s1 == 2 ? cand[2] : cand[3];
+        const uint32_t c = s2 == 0 ? cand[0] : s2 == 1 ? cand[1] : s2 == 2 ? cand[2] : cand[3];
+        out = avg4(a, c);
     }
-    wave_sync();  // the window buffer is reused by the next list
+    wave_sync();  // the buffer is reused by the next list
+    return out;
 }
 
 // Two horizontally adjacent chroma prediction samples at (xC, yC) with a 1/8-sample
@@ -412,23 +468,33 @@ __device__ void inter_mb(const DevPic &P, int addr, WaveRes *ws) {
         return;
     }
     const bool res = build_residual(mb, ax, P.levels, ws);
-    // ---- luma: lane -> 4x4 block b (raster), row r
+    // ---- luma: lane -> 4x4 block b = lane & 15 (raster), row r = lane >> 4
     {
-        int b = lane >> 2, r = lane & 3, bx = b & 3, by = b >> 2, q = (by >> 1) * 2 + (bx >> 1);
-        int pf0 = (ax.pf >> q) & 1, pf1 = (ax.pf >> (4 + q)) & 1;
-        int p0[4] = {0, 0, 0, 0}, p1[4] = {0, 0, 0, 0};
-        int xAL = mbx * 16 + bx * 4, yAL = mby * 16 + by * 4 + r;
-        if (ax.pf & 0x0F) luma_list(P, mb, ax, 0, ws, p0);  // MB-uniform conditions
-        if (ax.pf & 0xF0) luma_list(P, mb, ax, 1, ws, p1);
-        const DevWeight &w = P.wt[ax.wp[q]];
-        uint32_t packed = 0;
-#pragma unroll
-        for (int i = 0; i < 4; i++) {
-            int v = weigh(w, 0, pf0, pf1, p0[i], p1[i]);
-            if (res) v = clip1(v + ws->rl[(by * 4 + r) * 16 + bx * 4 + i]);
-            packed |= (uint32_t)v << (8 * i);
+        const int b = lane & 15, r = lane >> 4, bx = b & 3, by = b >> 2, q = (by >> 1) * 2 + (bx >> 1);
+        const int pf0 = (ax.pf >> q) & 1, pf1 = (ax.pf >> (4 + q)) & 1;
+        uint32_t p0 = 0, p1 = 0;
+#pragma nounroll
+        for (int l = 0; l < 2; l++) {  // one inlined copy of the interpolation for both lists
+            if (!(ax.pf & (0x0F << (4 * l)))) continue;  // MB-uniform
+            const uint32_t v = luma_list(P, mb, ax, l, ws);
+            if (l) p1 = v; else p0 = v;
         }
-        *(uint32_t *)&dy[(uint64_t)yAL * wl + xAL] = packed;
+        const DevWeight &w = P.wt[ax.wp[q]];
+        uint32_t packed;
+        if (!w.mode) {
+            packed = pf0 && pf1 ? avg4(p0, p1) : (pf0 ? p0 : p1);
+        } else {
+            packed = 0;
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+                packed |= (uint32_t)weigh(w, 0, pf0, pf1, (p0 >> (8 * i)) & 255, (p1 >> (8 * i)) & 255) << (8 * i);
+        }
+        if (res) {
+            const int16_t *rl = &ws->rl[(by * 4 + r) * 16 + bx * 4];
+            packed = pack4(clip1((int)(packed & 255) + rl[0]), clip1((int)((packed >> 8) & 255) + rl[1]),
+                           clip1((int)((packed >> 16) & 255) + rl[2]), clip1((int)(packed >> 24) + rl[3]));
+        }
+        *(uint32_t *)&dy[(uint64_t)(mby * 16 + by * 4 + r) * wl + mbx * 16 + bx * 4] = packed;
     }
     // ---- chroma: lane -> component c, 4x4 luma block b, chroma row r (2 samples)
     {
