@@ -507,12 +507,16 @@ int stage_picture(h264g_ctx *ctx, Stream &st, const h264g_pic_desc &d, const h26
     for (int y = 0; y < H; y++) {
         irow[y] = (uint32_t)icol.size();
         for (int xx = 0; xx < W; xx++)
-            if (aux[y * W + xx].kind & AUX_INTRA) icol.push_back((uint32_t)xx);
+            if (aux[y * W + xx].kind & AUX_INTRA) {
+                aux[y * W + xx].ires = (uint32_t)icol.size();
+                icol.push_back((uint32_t)xx);
+            }
     }
     irow[H] = (uint32_t)icol.size();
     size_t n_items = pic->n_inter + pic->n_intra + n_mbs + (size_t)H + irow.size() + icol.size();
     size_t o_bs = al(o_it + n_items * sizeof(uint32_t)), o_dbq = al(o_bs + (size_t)n_mbs * 32);
-    size_t o_pic = al(o_dbq + (size_t)n_mbs * 4), total = al(o_pic + sizeof(DevPic));
+    size_t o_ires = al(o_dbq + (size_t)n_mbs * 4), o_pic = al(o_ires + (size_t)pic->n_intra * 768);
+    size_t total = al(o_pic + sizeof(DevPic));
     void *dm = pool_get(ctx, total);
     if (!dm) return h264g_set_error(H264G_E_NOMEM, "picture device memory (%zu bytes)", total);
     pic->dmem = dm;
@@ -539,6 +543,7 @@ int stage_picture(h264g_ctx *ctx, Stream &st, const h264g_pic_desc &d, const h26
     std::memset(&P, 0, sizeof(P));
     P.mbs = pic->d_mbs; P.aux = pic->d_aux; P.levels = pic->d_levels; P.wt = pic->d_wt;
     P.dpb = st.dpb; P.slot_bytes = st.slot_bytes; P.flags = st.flags; P.rowprog = st.rowprog; P.gstrip = st.gstrip; P.bs = d_bs; P.dbq = d_dbq; P.dst = d.dst_slot;
+    P.ires = (int16_t *)(base + o_ires);
     P.irow = pic->d_items + irow_at; P.n_intra = pic->n_intra;
     P.W = W; P.H = H; P.offA = d.filter_offset_a; P.offB = d.filter_offset_b;
     P.dbk_idc = d.disable_deblocking_filter_idc; P.cip = d.constrained_intra_pred_flag;
@@ -577,7 +582,7 @@ int launch_stages(h264g_ctx *ctx, const StageSet &q, int lane) {
         ctx->ev_used += 5;
     }
     if (ev) HIPCHK(hipEventRecord(ev[0], s));
-    int e = h264g_launch_inter(q.pics, q.inter, q.n_inter, ctx->d_err, s);
+    int e = h264g_launch_inter(q.pics, q.inter, q.n_inter, q.intra, q.n_intra, ctx->d_err, s);
     if (e) return h264g_set_error(H264G_E_DEVICE, "inter launch: %s", hipGetErrorString((hipError_t)e));
     if (ev) HIPCHK(hipEventRecord(ev[1], s));
     if (q.n_intra)

@@ -18,7 +18,7 @@ struct DevAux {
     uint8_t  qpc[2];          // QP'C of Cb / Cr for dequantisation
     uint8_t  qdb[3];          // deblocking qP: luma, Cb edges, Cr edges
     uint8_t  pad0;
-    uint32_t reserved;
+    uint32_t ires;            // intra MBs: index of the MB's residual in DevPic::ires
 };
 static_assert(sizeof(DevAux) == 28, "DevAux layout");
 
@@ -57,6 +57,8 @@ struct DevPic {
     int32_t  cip;             // constrained_intra_pred_flag
     uint8_t  *gstrip;         // deblocking strips between bands of the stream: W x 128 B per band
     uint32_t *dbq;            // per-MB deblocking word, written by k_bs
+    int16_t  *ires;           // residual of each intra MB with levels (DevAux::ires): luma
+                              // 16x16 then Cb, Cr 8x8, raster; written by the inter stage
 };
 
 // Work item: picture index in the batch and macroblock address.
@@ -75,7 +77,8 @@ struct DevCtl {
 
 extern "C" {
 // Kernel launchers (kernels.hip); all enqueue on `stream` and return a hipError_t.
-int h264g_launch_inter(const DevPic *pics, const uint32_t *items, uint32_t n, uint32_t *err, void *stream);
+int h264g_launch_inter(const DevPic *pics, const uint32_t *items, uint32_t n, const uint32_t *intra_items, uint32_t n_intra,
+                       uint32_t *err, void *stream);
 // `epoch` points at the device word shared by all wavefront launches of a context
 // (initially 1; each launch's last wave advances it).
 int h264g_launch_intra(const DevPic *pics, const uint32_t *items, uint32_t n, DevCtl *ctl, uint32_t *epoch,

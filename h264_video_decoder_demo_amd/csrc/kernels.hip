@@ -92,7 +92,7 @@ struct WaveMc {
     int16_t b1t[16][4][10];   // b1 of column i, window rows 0..8 (+ one pad row): [b][i][rr]
 };
 
-struct WaveRes {
+struct alignas(16) WaveRes {
     union {
         struct {
             int32_t coef[27][16];
@@ -727,14 +727,38 @@ __device__ int pred8x8(const WaveIntra *wi, int mode, int x, int y) {
 #undef FL
 }
 
-// The residual of an intra MB into wi->res (it needs no neighbour samples, so the
-// schedules run it before waiting for the MBs above); false when the MB has none.
-__device__ __forceinline__ bool intra_residual(const DevPic &P, int addr, WaveIntra *wi) {
-    return build_residual(&P.mbs[addr], P.aux[addr], P.levels, &wi->res);
+// Residuals of intra MBs.  They need no neighbour samples, so the inter stage, which is
+// parallel over MBs, builds them (intra_res_build) and leaves them in P.ires; the intra
+// schedules load an MB's residual (intra_res_fetch, lanes 0-47 one 16-byte piece each)
+// before they wait for its neighbours and put it in the tile after the wait
+// (intra_res_place).
+static_assert(offsetof(WaveRes, rc) == offsetof(WaveRes, rl) + 512 && offsetof(WaveRes, rl) % 16 == 0,
+              "luma and chroma residual planes are one 768-byte run");
+__device__ __forceinline__ void intra_res_build(const DevPic &P, int addr, WaveRes *ws) {
+    const DevAux &ax = P.aux[addr];
+    if (!build_residual(&P.mbs[addr], ax, P.levels, ws)) return;
+    const int lane = wave_lane();
+    if (lane < 48) ((uint4 *)(P.ires + (size_t)ax.ires * 384))[lane] = ((const uint4 *)ws->rl)[lane];
+}
+struct IntraRes {
+    uint4 v;
+    bool res;
+};
+__device__ __forceinline__ IntraRes intra_res_fetch(const DevPic &P, int addr) {
+    const DevAux &ax = P.aux[addr];
+    IntraRes r;
+    r.res = (ax.kind & AUX_RESID) != 0;
+    r.v = make_uint4(0, 0, 0, 0);
+    if (r.res && wave_lane() < 48) r.v = ((const uint4 *)(P.ires + (size_t)ax.ires * 384))[wave_lane()];
+    return r;
+}
+__device__ __forceinline__ void intra_res_place(const IntraRes &r, WaveIntra *wi) {
+    if (r.res && wave_lane() < 48) ((uint4 *)wi->res.rl)[wave_lane()] = r.v;
 }
 
-// Prediction and reconstruction of an intra MB whose residual intra_residual left in
-// wi->res (res = it returned true); the neighbours it reads are final.
+// Prediction and reconstruction of an intra MB whose residual intra_res_place left in
+// wi->res (res: the MB has levels); the neighbours it reads are final.  The LDS writes of
+// the residual are ordered before its first wave barrier.
 __device__ __forceinline__ void intra_mb(const DevPic &P, int addr, WaveIntra *wi, const Avail &av, bool res) {
     const int lane = wave_lane();
     const h264g_mb *mb = &P.mbs[addr];
@@ -1328,14 +1352,18 @@ __device__ __forceinline__ uint32_t xcd_item(uint32_t b, uint32_t n) {
     return x * q + min(x, r) + b / N_XCD;
 }
 
+// Workgroups 0 .. n - 1 reconstruct the inter (and I_PCM) MBs of `items`, the following
+// ni build the residuals of the intra MBs of `iitems`.
 __global__ __launch_bounds__(WAVE) void k_inter(const DevPic *__restrict__ pics, const uint32_t *__restrict__ items, uint32_t n,
-                                                uint32_t *err) {
+                                                const uint32_t *__restrict__ iitems, uint32_t ni, uint32_t *err) {
     __shared__ WaveRes lds;
-    if (blockIdx.x >= n) return;
-    const uint32_t i = xcd_item(blockIdx.x, n);
-    const uint32_t it = items[i];
+    const uint32_t total = n + ni;
+    if (blockIdx.x >= total) return;
+    const uint32_t i = xcd_item(blockIdx.x, total);
+    const uint32_t it = i < n ? items[i] : iitems[i - n];
     const DevPic &P = pics[it >> WI_PIC_SHIFT];
-    inter_mb(P, (int)(it & WI_MB_MASK), &lds);
+    if (i < n) inter_mb(P, (int)(it & WI_MB_MASK), &lds);
+    else intra_res_build(P, (int)(it & WI_MB_MASK), &lds);
     (void)err;
 }
 
@@ -1379,9 +1407,10 @@ __global__ __launch_bounds__(WAVE) void k_intra(const DevPic *__restrict__ pics,
         const int addr = (int)(it & WI_MB_MASK);
         int deps[4];
         const Avail av = intra_avail(P, addr, deps);
-        const bool res = intra_residual(P, addr, &lds);
+        const IntraRes r = intra_res_fetch(P, addr);
         wait_deps(P.flags, deps, 4, epoch, err);
-        intra_mb(P, addr, &lds, av, res);
+        intra_res_place(r, &lds);
+        intra_mb(P, addr, &lds, av, r.res);
         publish(&P.flags[addr], epoch);
     }
     retire(ctl, gridDim.x, epoch_p, epoch);
@@ -1486,14 +1515,15 @@ __global__ __launch_bounds__(WAVE * INTRA_WG_MAX_WAVES) void k_intra_band(const 
                 const int x = (int)uniform(col[i]), addr = y * W + x;
                 int deps[4];
                 const Avail av = intra_avail(P, addr, deps);
-                const bool res = intra_residual(P, addr, &tiles[w]);
+                const IntraRes r = intra_res_fetch(P, addr);  // in flight during the wait
                 // the row above up to C (x + 1) or up to B (x) when only B / D are read and intra
                 const int need = deps[2] >= 0 ? min(x + 2, W) : (deps[1] >= 0 || deps[3] >= 0 ? x + 1 : 0);
                 if (need > 0) {
                     if (from_g) band_wait(&P.rowprog[y - 1], tag, (uint32_t)need, seen, err);
                     else sync.wait(y - 1 - y0, need);
                 }
-                intra_mb(P, addr, &tiles[w], av, res);
+                intra_res_place(r, &tiles[w]);
+                intra_mb(P, addr, &tiles[w], av, r.res);
                 const bool end = i + 1 == e;
                 post(end ? (uint32_t)W : uniform(col[i + 1]), end || (i - b) % INTRA_PUBLISH_EVERY == INTRA_PUBLISH_EVERY - 1);
             }
@@ -1832,9 +1862,11 @@ __global__ void k_copy(const uint4 *src, uint4 *dst, uint64_t n16) {
 
 extern "C" {
 
-int h264g_launch_inter(const DevPic *pics, const uint32_t *items, uint32_t n, uint32_t *err, void *stream) {
-    if (n == 0) return 0;
-    hipLaunchKernelGGL(k_inter, dim3(n), dim3(WAVE), 0, (hipStream_t)stream, pics, items, n, err);
+int h264g_launch_inter(const DevPic *pics, const uint32_t *items, uint32_t n, const uint32_t *intra_items, uint32_t n_intra,
+                       uint32_t *err, void *stream) {
+    if (n + n_intra == 0) return 0;
+    hipLaunchKernelGGL(k_inter, dim3(n + n_intra), dim3(WAVE), 0, (hipStream_t)stream, pics, items, n, intra_items, n_intra,
+                       err);
     return (int)hipGetLastError();
 }
 
