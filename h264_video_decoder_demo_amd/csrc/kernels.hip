@@ -1027,39 +1027,6 @@ __device__ void dbk_tables_init(DbkTables *t) {
     }
 }
 
-// 8.7.2.2 - 8.7.2.4 on samples p[0..3] (p0 first) and q[0..3] (Deblock.cpp:1314-1522);
-// alpha = alpha'(indexA), beta = beta'(indexB), tc0 = tC0'(bS, indexA) (used for bS 1..3),
-// with indexA / indexB = Clip3(0, 51, qPav + filterOffsetA / B).
-__device__ __forceinline__ void filter_line(int p[4], int q[4], int bS, bool chroma, int alpha, int beta, int tc0) {
-    int p0 = p[0], p1 = p[1], p2 = p[2], p3 = p[3], q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3];
-    if (!(bS != 0 && iabs(p0 - q0) < alpha && iabs(p1 - p0) < beta && iabs(q1 - q0) < beta)) return;
-    int ap = iabs(p2 - p0), aq = iabs(q2 - q0);
-    if (bS < 4) {
-        int tc = chroma ? tc0 + 1 : tc0 + (ap < beta) + (aq < beta);
-        int delta = clip3(-tc, tc, (((q0 - p0) << 2) + (p1 - q1) + 4) >> 3);
-        p[0] = clip1(p0 + delta);
-        q[0] = clip1(q0 - delta);
-        if (!chroma && ap < beta) p[1] = (p1 + clip3(-tc0, tc0, (p2 + ((p0 + q0 + 1) >> 1) - (p1 << 1)) >> 1)) & 255;
-        if (!chroma && aq < beta) q[1] = (q1 + clip3(-tc0, tc0, (q2 + ((p0 + q0 + 1) >> 1) - (q1 << 1)) >> 1)) & 255;
-        return;
-    }
-    bool strong = iabs(p0 - q0) < ((alpha >> 2) + 2);
-    if (!chroma && ap < beta && strong) {
-        p[0] = (p2 + 2 * p1 + 2 * p0 + 2 * q0 + q1 + 4) >> 3;
-        p[1] = (p2 + p1 + p0 + q0 + 2) >> 2;
-        p[2] = (2 * p3 + 3 * p2 + p1 + p0 + q0 + 4) >> 3;
-    } else {
-        p[0] = (2 * p1 + p0 + q1 + 2) >> 2;
-    }
-    if (!chroma && aq < beta && strong) {
-        q[0] = (p1 + 2 * p0 + 2 * q0 + 2 * q1 + q2 + 4) >> 3;
-        q[1] = (p0 + q0 + q1 + q2 + 2) >> 2;
-        q[2] = (2 * q3 + 3 * q2 + q1 + q0 + p0 + 4) >> 3;
-    } else {
-        q[0] = (2 * q1 + q0 + p1 + 2) >> 2;
-    }
-}
-
 // LDS working set of one deblocking wave, in MB-relative coordinates: luma rows [-4, 15]
 // x columns [-16, 15] (the MB and the whole MB to its left), chroma rows [-4, 7] x
 // columns [-8, 7].  Rows -4..-1 hold the bottom rows of the MB above.
@@ -1103,18 +1070,50 @@ __device__ void dbk_load(const DbkPlanes &d, int mbx, int mby, int region, WaveD
     }
 }
 
+__device__ __forceinline__ int absdiff(int a, int b) { return (int)__builtin_amdgcn_sad_u16((uint32_t)a, (uint32_t)b, 0u); }
+__device__ __forceinline__ bool any_lane(bool v) { return __ballot(v) != 0; }
+
 // Edge J of a line held as 20 bytes in w[0..4] (little-endian; the edge lies between
 // words J and J + 1); chroma lines hold 12 bytes and use J = 0, 1 (luma edges 0 and 8
-// for the bS).
+// for the bS).  8.7.2.2 - 8.7.2.4 (Deblock.cpp:1314-1522): alpha = alpha'(indexA), beta =
+// beta'(indexB), tc0 = tC0'(bS, indexA) (used for bS 1..3), with indexA / indexB =
+// Clip3(0, 51, qPav + filterOffsetA / B).  Every lane takes the same path: the wave
+// skips a filter that no lane applies (wave-uniform branches) and selects per lane
+// otherwise, so no branch waits on a per-line condition.
 template <int J>
 __device__ __forceinline__ void dbk_line_edge(uint32_t (&w)[5], bool act, bool luma, int bS, int alpha, int beta, int tc0) {
-    if (!act) return;
+    const bool on = act && bS != 0;
+    if (!any_lane(on)) return;
     const uint32_t a = w[J], b = w[J + 1];
-    int pp[4] = {(int)(a >> 24), (int)((a >> 16) & 255u), (int)((a >> 8) & 255u), (int)(a & 255u)};
-    int qq[4] = {(int)(b & 255u), (int)((b >> 8) & 255u), (int)((b >> 16) & 255u), (int)(b >> 24)};
-    filter_line(pp, qq, bS, !luma, alpha, beta, tc0);
-    w[J] = (a & 255u) | ((uint32_t)pp[2] << 8) | ((uint32_t)pp[1] << 16) | ((uint32_t)pp[0] << 24);
-    w[J + 1] = (uint32_t)qq[0] | ((uint32_t)qq[1] << 8) | ((uint32_t)qq[2] << 16) | (b & 0xff000000u);
+    const int p0 = (int)(a >> 24), p1 = (int)((a >> 16) & 255u), p2 = (int)((a >> 8) & 255u), p3 = (int)(a & 255u);
+    const int q0 = (int)(b & 255u), q1 = (int)((b >> 8) & 255u), q2 = (int)((b >> 16) & 255u), q3 = (int)(b >> 24);
+    const int d0 = absdiff(p0, q0);
+    const bool filt = on & (d0 < alpha) & (absdiff(p1, p0) < beta) & (absdiff(q1, q0) < beta);
+    if (!any_lane(filt)) return;
+    const bool apl = absdiff(p2, p0) < beta, aql = absdiff(q2, q0) < beta;
+    int np0 = p0, np1 = p1, np2 = p2, nq0 = q0, nq1 = q1, nq2 = q2;
+    const bool fn = filt & (bS < 4), fs = filt & (bS >= 4);
+    if (any_lane(fn)) {  // bS < 4 (8.7.2.3)
+        const int tc = luma ? tc0 + (int)apl + (int)aql : tc0 + 1;
+        const int delta = clip3(-tc, tc, (((q0 - p0) << 2) + (p1 - q1) + 4) >> 3);
+        const int avg = (p0 + q0 + 1) >> 1;
+        np0 = fn ? clip1(p0 + delta) : np0;
+        nq0 = fn ? clip1(q0 - delta) : nq0;
+        np1 = fn & luma & apl ? (p1 + clip3(-tc0, tc0, (p2 + avg - (p1 << 1)) >> 1)) & 255 : np1;
+        nq1 = fn & luma & aql ? (q1 + clip3(-tc0, tc0, (q2 + avg - (q1 << 1)) >> 1)) & 255 : nq1;
+    }
+    if (any_lane(fs)) {  // bS == 4 (8.7.2.4)
+        const bool strong = d0 < ((alpha >> 2) + 2);
+        const bool sp = luma & apl & strong, sq = luma & aql & strong;
+        np0 = fs ? (sp ? (p2 + 2 * p1 + 2 * p0 + 2 * q0 + q1 + 4) >> 3 : (2 * p1 + p0 + q1 + 2) >> 2) : np0;
+        np1 = fs & sp ? (p2 + p1 + p0 + q0 + 2) >> 2 : np1;
+        np2 = fs & sp ? (2 * p3 + 3 * p2 + p1 + p0 + q0 + 4) >> 3 : np2;
+        nq0 = fs ? (sq ? (p1 + 2 * p0 + 2 * q0 + 2 * q1 + q2 + 4) >> 3 : (2 * q1 + q0 + p1 + 2) >> 2) : nq0;
+        nq1 = fs & sq ? (p0 + q0 + q1 + q2 + 2) >> 2 : nq1;
+        nq2 = fs & sq ? (2 * q3 + 3 * q2 + q1 + q0 + p0 + 4) >> 3 : nq2;
+    }
+    w[J] = __builtin_amdgcn_perm(pack4(0, np2, np1, np0), a, 0x07060500u);
+    w[J + 1] = __builtin_amdgcn_perm(b, pack4(nq0, nq1, nq2, 0), 0x07020100u);
 }
 
 // The edges of one direction (DIR 0: vertical edges, 1: horizontal) of the MB in the
@@ -1167,9 +1166,12 @@ __device__ __forceinline__ void dbk_lines(const DevPic &P, uint32_t eq, bool edg
     const uint32_t bsv = *(const uint32_t *)&bsrec[DIR * 16 + seg * 4];
     const int bs0 = bsv & 255u, bs1 = luma ? (bsv >> 8) & 255u : (bsv >> 16) & 255u, bs2 = (bsv >> 16) & 255u,
               bs3 = bsv >> 24;
-    const int a0 = tb->alpha[iA0], b0 = tb->beta[iB0], ai = tb->alpha[iAi], bi = tb->beta[iBi];
-    const int t0 = tb->tc0[bs0 >= 1 && bs0 <= 3 ? bs0 - 1 : 0][iA0], t1 = tb->tc0[bs1 >= 1 && bs1 <= 3 ? bs1 - 1 : 0][iAi];
-    const int t2 = tb->tc0[bs2 >= 1 && bs2 <= 3 ? bs2 - 1 : 0][iAi], t3 = tb->tc0[bs3 >= 1 && bs3 <= 3 ? bs3 - 1 : 0][iAi];
+    int a0 = tb->alpha[iA0], b0 = tb->beta[iB0], ai = tb->alpha[iAi], bi = tb->beta[iBi];
+    int t0 = tb->tc0[bs0 >= 1 && bs0 <= 3 ? bs0 - 1 : 0][iA0], t1 = tb->tc0[bs1 >= 1 && bs1 <= 3 ? bs1 - 1 : 0][iAi];
+    int t2 = tb->tc0[bs2 >= 1 && bs2 <= 3 ? bs2 - 1 : 0][iAi], t3 = tb->tc0[bs3 >= 1 && bs3 <= 3 ? bs3 - 1 : 0][iAi];
+    // one LDS round trip for the eight lookups: without this the compiler moves each into
+    // the edge that first uses it and waits on it there
+    asm volatile("" : "+v"(a0), "+v"(b0), "+v"(ai), "+v"(bi), "+v"(t0), "+v"(t1), "+v"(t2), "+v"(t3));
     dbk_line_edge<0>(w, on && edge, luma, bs0, a0, b0, t0);
     dbk_line_edge<1>(w, on && (luma ? !t8 : true), luma, bs1, ai, bi, t1);
     dbk_line_edge<2>(w, on && luma, luma, bs2, ai, bi, t2);
