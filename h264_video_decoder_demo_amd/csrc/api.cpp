@@ -722,7 +722,7 @@ int32_t h264g_stream_open(h264g_ctx *c, const h264g_stream_desc *d, uint32_t *id
     s.lane = c->split_mbs > 0 && s.W * s.H >= c->split_mbs ? 1 : 0;
     // DPB slots, per-MB progress words, per-row progress words, and one row of
     // deblocking strips (128 B per MB) per band of c->dbk_waves rows
-    const uint64_t strip_bytes = (uint64_t)((s.H + c->dbk_waves - 1) / c->dbk_waves) * s.W * 128;
+    const uint64_t strip_bytes = (uint64_t)((s.H + c->dbk_waves - 1) / c->dbk_waves) * s.W * 256;
     uint64_t need = s.slot_bytes * d->dpb_slots + (uint64_t)s.W * s.H * 4 + (uint64_t)s.H * 8 + strip_bytes;
     if (c->budget && c->used + need > c->budget)
         return h264g_set_error(H264G_E_BUDGET, "DPB budget: %llu + %llu > %llu", (unsigned long long)c->used,
@@ -745,6 +745,7 @@ int32_t h264g_stream_open(h264g_ctx *c, const h264g_stream_desc *d, uint32_t *id
         return h264g_set_error(H264G_E_NOMEM, "deblocking strips");
     }
     HIPCHK(hipMemset(s.rowprog, 0, (size_t)s.H * 8));
+    HIPCHK(hipMemset(s.gstrip, 0, strip_bytes));  // epoch 0 never matches a launch
     HIPCHK(hipMemset(s.dpb, 0, s.slot_bytes * d->dpb_slots));
     HIPCHK(hipMemset(s.flags, 0, (size_t)s.W * s.H * 4));
     s.bytes = need;

@@ -1124,27 +1124,29 @@ __device__ __forceinline__ void dbk_line_edge(uint32_t (&w)[5], bool act, bool l
 // only 4).  Lines of one direction touch disjoint samples, so this gives the reference's
 // edge-by-edge order (8.7, Deblock.cpp:76-635).  qq / eq: deblocking words (see k_bs) of
 // the MB and of the left / top MB.
+// cur: tile column of the MB's luma column 0 (16 or 0; chroma cur / 2); the left MB sits
+// in the other half of the tile, so a row's columns -4..-1 are at (16 - cur) + 12.
 template <int DIR>
 __device__ __forceinline__ void dbk_lines(const DevPic &P, uint32_t eq, bool edge, uint32_t qq,
-                                          const uint8_t *bsrec, WaveDbk *wd, const DbkTables *tb) {
+                                          const uint8_t *bsrec, WaveDbk *wd, const DbkTables *tb, int cur) {
     const int lane = wave_lane();
     if (lane >= 32) return;
     const int plane = lane < 16 ? 0 : (lane < 24 ? 1 : 2);
     const bool luma = plane == 0;
     const int k = luma ? lane : (lane & 7);
-    // base: tile row 0 at MB column -4; a row holds columns -4 .. 15 (chroma -4 .. 7)
-    uint8_t *base = luma ? &wd->L[0][DT_LX - 4] : &wd->C[plane - 1][0][DT_CX - 4];
-    const int stride = luma ? 32 : 16;
+    // tile row 0 of the plane; the MB's columns start at mc, the left MB's columns -4..-1 at lc
+    uint8_t *base = luma ? &wd->L[0][0] : &wd->C[plane - 1][0][0];
+    const int stride = luma ? 32 : 16, mc = luma ? cur : cur >> 1, lc = luma ? 28 - cur : 12 - (cur >> 1);
     uint32_t w[5];
     if (DIR == 0) {
-        const uint32_t *row = (const uint32_t *)(base + (k + 4) * stride);
-        w[0] = row[0];
-        w[1] = row[1];
-        w[2] = row[2];
-        w[3] = luma ? row[3] : 0u;
-        w[4] = luma ? row[4] : 0u;
+        uint8_t *row = base + (k + 4) * stride;
+        w[0] = *(const uint32_t *)(row + lc);
+        w[1] = *(const uint32_t *)(row + mc);
+        w[2] = *(const uint32_t *)(row + mc + 4);
+        w[3] = luma ? *(const uint32_t *)(row + mc + 8) : 0u;
+        w[4] = luma ? *(const uint32_t *)(row + mc + 12) : 0u;
     } else {
-        const uint8_t *c = base + k + 4;
+        const uint8_t *c = base + mc + k;
 #pragma unroll
         for (int i = 0; i < 3; i++)
             w[i] = (uint32_t)c[(4 * i) * stride] | ((uint32_t)c[(4 * i + 1) * stride] << 8) |
@@ -1177,16 +1179,16 @@ __device__ __forceinline__ void dbk_lines(const DevPic &P, uint32_t eq, bool edg
     dbk_line_edge<2>(w, on && luma, luma, bs2, ai, bi, t2);
     dbk_line_edge<3>(w, on && luma && !t8, luma, bs3, ai, bi, t3);
     if (DIR == 0) {
-        uint32_t *row = (uint32_t *)(base + (k + 4) * stride);
-        row[0] = w[0];
-        row[1] = w[1];
-        row[2] = w[2];
+        uint8_t *row = base + (k + 4) * stride;
+        *(uint32_t *)(row + lc) = w[0];
+        *(uint32_t *)(row + mc) = w[1];
+        *(uint32_t *)(row + mc + 4) = w[2];
         if (luma) {
-            row[3] = w[3];
-            row[4] = w[4];
+            *(uint32_t *)(row + mc + 8) = w[3];
+            *(uint32_t *)(row + mc + 12) = w[4];
         }
     } else {
-        uint8_t *c = base + k + 4;
+        uint8_t *c = base + mc + k;
 #pragma unroll
         for (int i = 1; i < 11; i++) c[i * stride] = (uint8_t)(w[i >> 2] >> (8 * (i & 3)));
         if (luma) {
@@ -1202,9 +1204,9 @@ __device__ __forceinline__ void dbk_lines(const DevPic &P, uint32_t eq, bool edg
 // tile; it is complete when this returns.
 __device__ __forceinline__ void dbk_filter(const DevPic &P, uint32_t qq, uint32_t lq, uint32_t tq, bool left, bool top,
                            const uint8_t *bsrec, WaveDbk *wd, const DbkTables *tb) {
-    dbk_lines<0>(P, lq, left, qq, bsrec, wd, tb);
+    dbk_lines<0>(P, lq, left, qq, bsrec, wd, tb, DT_LX);
     wave_sync();
-    dbk_lines<1>(P, tq, top, qq, bsrec, wd, tb);
+    dbk_lines<1>(P, tq, top, qq, bsrec, wd, tb, DT_LX);
     wave_sync();
 }
 
@@ -1424,13 +1426,12 @@ __global__ __launch_bounds__(WAVE) void k_intra(const DevPic *__restrict__ pics,
 // CU's caches).  A band's last row hands its results to the next band's first row, in
 // another workgroup and possibly another XCD, through device memory: it publishes its
 // progress word (P.rowprog[y], epoch << 32 | progress) with an agent-scope release every
-// few MBs (INTRA_PUBLISH_EVERY intra MBs, DBK_PUBLISH_EVERY MBs: each publish writes back
-// the L2, each read of a new value invalidates it, so the period trades that cost against
-// the next band's lag) and at the row end; the reader polls it relaxed and then
-// acquires at agent scope.  Work items are (picture, band) in band-major order and a
+// INTRA_PUBLISH_EVERY intra MBs and at the row end; the reader polls it relaxed and then
+// acquires at agent scope.  (Deblocking hands epoch-tagged strips across instead, see
+// strip_put_g.)  Work items are (picture, band) in band-major order and a
 // workgroup takes its item by ticket when it starts, so the band above always belongs to
 // a workgroup that started earlier and never waits on a later one.
-constexpr int INTRA_PUBLISH_EVERY = 1, DBK_PUBLISH_EVERY = 4;
+constexpr int INTRA_PUBLISH_EVERY = 1;
 
 // Takes the workgroup's ticket (into *slot) and clears `n` LDS counters.
 __device__ __forceinline__ uint32_t band_ticket(DevCtl *ctl, uint32_t *slot, uint32_t *ctr, int n) {
@@ -1641,10 +1642,10 @@ __device__ __forceinline__ DbkPrefetch dbk_fetch(const DevPic &P, const DbkPlane
     if (lane < 8) f.bs = *(const uint32_t *)&P.bs[(size_t)(mby * P.W + mbx) * 32 + lane * 4];
     return f;
 }
-__device__ __forceinline__ void dbk_place(const DbkPrefetch &f, WaveDbk *wd, uint8_t *bsrec) {
+__device__ __forceinline__ void dbk_place(const DbkPrefetch &f, WaveDbk *wd, uint8_t *bsrec, int cur) {
     const int lane = wave_lane();
-    *(uint32_t *)&wd->L[(lane >> 2) + 4][DT_LX + (lane & 3) * 4] = f.luma;
-    if (lane < 32) *(uint32_t *)&wd->C[lane >> 4][((lane >> 1) & 7) + 4][DT_CX + (lane & 1) * 4] = f.chroma;
+    *(uint32_t *)&wd->L[(lane >> 2) + 4][cur + (lane & 3) * 4] = f.luma;
+    if (lane < 32) *(uint32_t *)&wd->C[lane >> 4][((lane >> 1) & 7) + 4][(cur >> 1) + (lane & 1) * 4] = f.chroma;
     if (lane < 8) *(uint32_t *)&bsrec[lane * 4] = f.bs;
 }
 
@@ -1663,15 +1664,58 @@ __device__ __forceinline__ void dbk_store_mb(const DbkPlanes &d, int mbx, int mb
             *(uint2 *)&(c ? d.cr : d.cb)[(uint64_t)(mby * 8 + r) * d.wc + mbx * 8] = *(const uint2 *)&wd->C[c][r + 4][col >> 1];
     }
 }
-// Rows 13..15 (chroma 5..7) of the MB above the current one, final after the current MB's
-// top edge: lanes 0..2 luma, 32..37 chroma.
-__device__ __forceinline__ void dbk_store_top(const DbkPlanes &d, int mbx, int mby, const WaveDbk *wd) {
+// Rows 13..15 (chroma 5..7) of the MB above the current one (tile column cur), final after
+// the current MB's top edge: lanes 0..2 luma, 32..37 chroma.
+__device__ __forceinline__ void dbk_store_top(const DbkPlanes &d, int mbx, int mby, const WaveDbk *wd, int cur) {
     const int lane = wave_lane();
     if (lane < 3) {
-        *(uint4 *)&d.y[(uint64_t)(mby * 16 - 3 + lane) * d.wl + mbx * 16] = *(const uint4 *)&wd->L[lane + 1][DT_LX];
+        *(uint4 *)&d.y[(uint64_t)(mby * 16 - 3 + lane) * d.wl + mbx * 16] = *(const uint4 *)&wd->L[lane + 1][cur];
     } else if (lane >= 32 && lane < 38) {
         const int k = lane - 32, c = k >= 3, r = k - 3 * c;
-        *(uint2 *)&(c ? d.cr : d.cb)[(uint64_t)(mby * 8 - 3 + r) * d.wc + mbx * 8] = *(const uint2 *)&wd->C[c][r + 1][DT_CX];
+        *(uint2 *)&(c ? d.cr : d.cb)[(uint64_t)(mby * 8 - 3 + r) * d.wc + mbx * 8] = *(const uint2 *)&wd->C[c][r + 1][cur >> 1];
+    }
+}
+
+// Strips that cross from a band to the next (another workgroup, possibly another XCD) go
+// through device memory as 32 words of 64 bits: the strip's dword of lane j (the layout of
+// strip_put) in the low half, the launch epoch in the high half.  Each word is stored and
+// loaded as one agent-scope atomic, so a reader that finds the epoch in all 32 words holds
+// the whole strip; neither side needs a fence, whose L2 write-back / invalidation would
+// also hit every other kernel running on those XCDs.
+struct StripWords {
+    uint64_t w[32];
+};
+__device__ __forceinline__ void strip_put_g(StripWords *sw, const WaveDbk *wd, int col, uint32_t epoch) {
+    const int lane = wave_lane();
+    uint32_t v = 0;
+    if (lane < 16) {
+        v = *(const uint32_t *)&wd->L[16 + (lane >> 2)][col + (lane & 3) * 4];
+    } else if (lane < 32) {
+        const int k = lane - 16, c = k >> 3, r = (k >> 1) & 3, h = k & 1;
+        v = *(const uint32_t *)&wd->C[c][8 + r][(col >> 1) + h * 4];
+    }
+    if (lane < 32) __hip_atomic_store(&sw->w[lane], (uint64_t)epoch << 32 | v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// Waits for the strip of this launch and puts it into the tile rows -4..-1 above the
+// current MB (tile column cur).  Bounded like the other waits.
+__device__ void strip_get_g(const StripWords *sw, WaveDbk *wd, int cur, uint32_t epoch, uint32_t *err) {
+    const int lane = wave_lane();
+    uint64_t v = (uint64_t)epoch << 32;
+    for (uint32_t it = 0;; ++it) {
+        if (lane < 32) v = __hip_atomic_load(&sw->w[lane], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (__ballot((uint32_t)(v >> 32) != epoch) == 0) break;
+        if (it >= SPIN_LIMIT) {
+            atomicOr(err, DEVERR_TIMEOUT);
+            break;
+        }
+        if ((it & 255u) == 255u && uniform(__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) != 0u) break;
+        __builtin_amdgcn_s_sleep(1);
+    }
+    if (lane < 16) {
+        *(uint32_t *)&wd->L[lane >> 2][cur + (lane & 3) * 4] = (uint32_t)v;
+    } else if (lane < 32) {
+        const int k = lane - 16, c = k >> 3, r = (k >> 1) & 3, h = k & 1;
+        *(uint32_t *)&wd->C[c][r][(cur >> 1) + h * 4] = (uint32_t)v;
     }
 }
 
@@ -1685,30 +1729,18 @@ __device__ __forceinline__ void strip_put(StripSlot *sl, const WaveDbk *wd, int 
         *(uint32_t *)&sl->c[c][r][h * 4] = *(const uint32_t *)&wd->C[c][8 + r][(col >> 1) + h * 4];
     }
 }
-// The MB above's strip into the tile rows -4..-1 above the current MB.
-__device__ __forceinline__ void strip_get(const StripSlot *sl, WaveDbk *wd) {
+// The MB above's strip into the tile rows -4..-1 above the current MB (tile column cur).
+__device__ __forceinline__ void strip_get(const StripSlot *sl, WaveDbk *wd, int cur) {
     const int lane = wave_lane();
     if (lane < 16) {
-        *(uint32_t *)&wd->L[lane >> 2][DT_LX + (lane & 3) * 4] = *(const uint32_t *)&sl->y[lane >> 2][(lane & 3) * 4];
+        *(uint32_t *)&wd->L[lane >> 2][cur + (lane & 3) * 4] = *(const uint32_t *)&sl->y[lane >> 2][(lane & 3) * 4];
     } else if (lane < 32) {
         const int k = lane - 16, c = k >> 3, r = (k >> 1) & 3, h = k & 1;
-        *(uint32_t *)&wd->C[c][r][DT_CX + h * 4] = *(const uint32_t *)&sl->c[c][r][h * 4];
+        *(uint32_t *)&wd->C[c][r][(cur >> 1) + h * 4] = *(const uint32_t *)&sl->c[c][r][h * 4];
     }
 }
-// After MB x: it becomes the left MB of MB x + 1 (rows 0..15).
-__device__ __forceinline__ void dbk_carry_mb(WaveDbk *wd) {
-    const int lane = wave_lane();
-    wave_sync();
-    const uint32_t l = *(const uint32_t *)&wd->L[4 + (lane >> 2)][DT_LX + (lane & 3) * 4];
-    uint32_t c = 0;
-    if (lane < 32) c = *(const uint32_t *)&wd->C[lane >> 4][4 + ((lane >> 1) & 7)][DT_CX + (lane & 1) * 4];
-    *(uint32_t *)&wd->L[4 + (lane >> 2)][(lane & 3) * 4] = l;
-    if (lane < 32) *(uint32_t *)&wd->C[lane >> 4][4 + ((lane >> 1) & 7)][(lane & 1) * 4] = c;
-    wave_sync();
-}
-
 // MB row y of the band starting at row y0 (band index b, rows y0 .. y1 - 1) on wave w.
-__device__ void deblock_row_band(const DevPic &P, int y, int y0, int y1, int b, int w, DbkWgShared &sh, uint64_t tag,
+__device__ void deblock_row_band(const DevPic &P, int y, int y0, int y1, int b, int w, DbkWgShared &sh, uint32_t epoch,
                                  uint32_t *err) {
     const int W = P.W, H = P.H, ly = y - y0;
     const DbkPlanes d = dbk_planes(P);
@@ -1718,15 +1750,13 @@ __device__ void deblock_row_band(const DevPic &P, int y, int y0, int y1, int b, 
     uint8_t *bsrec = sh.bsrec[w];
     StripSlot *ring = sh.ring[ly];
     const StripSlot *above = ly > 0 ? sh.ring[ly - 1] : nullptr;
-    StripSlot *gout = (StripSlot *)P.gstrip + (size_t)b * W;              // this band -> next
-    const StripSlot *gin = (const StripSlot *)P.gstrip + (size_t)(b - 1) * W;  // previous band -> this
+    StripWords *gout = (StripWords *)P.gstrip + (size_t)b * W;              // this band -> next
+    const StripWords *gin = (const StripWords *)P.gstrip + (size_t)(b - 1) * W;  // previous band -> this
     const int ry = last ? 15 : 12, cy = last ? 7 : 4;  // rows this row stores of its own MBs
-    uint32_t seen = 0;
     // strip s of this row to the row below (slot s mod STRIP_SLOTS), then progress `done`
-    auto hand_down = [&](int s, int col, uint32_t done, bool end) {
+    auto hand_down = [&](int s, int col, uint32_t done) {
         if (to_g) {
-            strip_put(&gout[s], wd, col);
-            if (end || done % DBK_PUBLISH_EVERY == 0) band_publish(&P.rowprog[y], tag, done);
+            strip_put_g(&gout[s], wd, col, epoch);
         } else {
             if (s - STRIP_SLOTS >= 0) lds_wait(&sh.cons[ly], (uint32_t)(s - STRIP_SLOTS + 1), err);  // slot read?
             strip_put(&ring[s % STRIP_SLOTS], wd, col);
@@ -1739,43 +1769,45 @@ __device__ void deblock_row_band(const DevPic &P, int y, int y0, int y1, int b, 
     for (int x = 0; x < W; x++) {
         const int addr = y * W + x;
         const bool left = x > 0, rowend = x == W - 1;
+        const int cur = 16 * (x & 1), prev = 16 - cur;  // tile columns of MB x and MB x - 1
         const uint32_t lq = qq, tq = tn;
         qq = qn;
-        dbk_place(f, wd, bsrec);
+        dbk_place(f, wd, bsrec, cur);
         if (!rowend) {
             f = dbk_fetch(P, d, x + 1, y);  // next MB, in flight during this one
             qn = P.dbq[addr + 1];
             if (top) tn = P.dbq[addr - W + 1];
         }
         wave_sync();
-        dbk_lines<0>(P, lq, left, qq, bsrec, wd, &sh.tb);  // vertical edges: need nothing from above
+        dbk_lines<0>(P, lq, left, qq, bsrec, wd, &sh.tb, cur);  // vertical edges: need nothing from above
         wave_sync();
         // MB x - 1 is final now (its columns 13..15 took MB x's left edge) except rows 13..15,
         // which the next row's top edge filters: store it and hand its strip down before
         // waiting for the row above, so the row below runs about one step behind
         if (left) {
-            dbk_store_mb(d, x - 1, y, 0, ry, cy, wd);
-            if (!last) hand_down(x - 1, 0, (uint32_t)x, false);
+            dbk_store_mb(d, x - 1, y, prev, ry, cy, wd);
+            if (!last) hand_down(x - 1, prev, (uint32_t)x);
         }
         if (top) {
             if (from_g) {
-                band_wait(&P.rowprog[y - 1], tag, (uint32_t)x + 1, seen, err);
-                strip_get(&gin[x], wd);
+                strip_get_g(&gin[x], wd, cur, epoch, err);
             } else {
                 lds_wait(&sh.prog[ly - 1], (uint32_t)x + 1, err);
-                strip_get(&above[x % STRIP_SLOTS], wd);
+                strip_get(&above[x % STRIP_SLOTS], wd, cur);
                 lds_post(&sh.cons[ly - 1], (uint32_t)x + 1);
             }
             wave_sync();
         }
-        dbk_lines<1>(P, tq, top, qq, bsrec, wd, &sh.tb);  // horizontal edges
+        dbk_lines<1>(P, tq, top, qq, bsrec, wd, &sh.tb, cur);  // horizontal edges
         wave_sync();
-        if (top) dbk_store_top(d, x, y, wd);
+        if (top) dbk_store_top(d, x, y, wd, cur);
         if (rowend) {
-            dbk_store_mb(d, x, y, DT_LX, ry, cy, wd);
-            if (!last) hand_down(x, DT_LX, (uint32_t)W, true);
+            dbk_store_mb(d, x, y, cur, ry, cy, wd);
+            if (!last) hand_down(x, cur, (uint32_t)W);
         }
-        dbk_carry_mb(wd);
+        // MB x stays in its half of the tile as the left MB of MB x + 1, whose samples go
+        // to the other half: the next step's first tile writes follow this barrier
+        wave_sync();
     }
 }
 
@@ -1786,7 +1818,6 @@ __global__ __launch_bounds__(WAVE * DBK_WG_MAX_WAVES) void k_deblock_band(const 
     __shared__ DbkWgShared sh;
     __shared__ uint32_t s_ticket;
     const uint32_t epoch = launch_epoch(epoch_p);
-    const uint64_t tag = (uint64_t)epoch << 32;
     const int nw = blockDim.x / WAVE, w = (int)uniform(threadIdx.x / WAVE);
     if ((int)threadIdx.x < nw) sh.cons[threadIdx.x] = 0;
     dbk_tables_init(&sh.tb);
@@ -1795,7 +1826,7 @@ __global__ __launch_bounds__(WAVE * DBK_WG_MAX_WAVES) void k_deblock_band(const 
         const uint32_t it = items[t];
         const DevPic &P = pics[it >> WI_PIC_SHIFT];
         const int b = (int)(it & WI_MB_MASK), y0 = b * nw, y = y0 + w;
-        if (P.dbk_idc != 1 && y < P.H) deblock_row_band(P, y, y0, min(P.H, y0 + nw), b, w, sh, tag, err);
+        if (P.dbk_idc != 1 && y < P.H) deblock_row_band(P, y, y0, min(P.H, y0 + nw), b, w, sh, epoch, err);
     }
     band_retire(ctl, epoch_p, epoch);
 }
