@@ -47,7 +47,8 @@ struct Stream {
     uint64_t slot_bytes = 0;
     uint32_t *flags = nullptr;
     uint64_t *rowprog = nullptr;
-    uint8_t *gstrip = nullptr;     // deblocking strips between row bands
+    uint8_t *gstrip = nullptr;     // deblocking strips between row bands, then the intra ones
+    uint64_t *istrip = nullptr;    // the intra strips inside gstrip
     uint64_t bytes = 0;
     int lane = 0;                  // HIP stream (lane) of the context that runs its pictures
     bool open = false;
@@ -542,7 +543,7 @@ int stage_picture(h264g_ctx *ctx, Stream &st, const h264g_pic_desc &d, const h26
     DevPic &P = pic->hpic;
     std::memset(&P, 0, sizeof(P));
     P.mbs = pic->d_mbs; P.aux = pic->d_aux; P.levels = pic->d_levels; P.wt = pic->d_wt;
-    P.dpb = st.dpb; P.slot_bytes = st.slot_bytes; P.flags = st.flags; P.rowprog = st.rowprog; P.gstrip = st.gstrip; P.bs = d_bs; P.dbq = d_dbq; P.dst = d.dst_slot;
+    P.dpb = st.dpb; P.slot_bytes = st.slot_bytes; P.flags = st.flags; P.rowprog = st.rowprog; P.gstrip = st.gstrip; P.istrip = st.istrip; P.bs = d_bs; P.dbq = d_dbq; P.dst = d.dst_slot;
     P.ires = (int16_t *)(base + o_ires);
     P.irow = pic->d_items + irow_at; P.n_intra = pic->n_intra;
     P.W = W; P.H = H; P.offA = d.filter_offset_a; P.offB = d.filter_offset_b;
@@ -722,7 +723,9 @@ int32_t h264g_stream_open(h264g_ctx *c, const h264g_stream_desc *d, uint32_t *id
     s.lane = c->split_mbs > 0 && s.W * s.H >= c->split_mbs ? 1 : 0;
     // DPB slots, per-MB progress words, per-row progress words, and one row of
     // deblocking strips (128 B per MB) per band of c->dbk_waves rows
-    const uint64_t strip_bytes = (uint64_t)((s.H + c->dbk_waves - 1) / c->dbk_waves) * s.W * 256;
+    // strips between row bands: deblocking (256 B per MB column and band), then intra (64 B)
+    const uint64_t dstrip_bytes = (uint64_t)((s.H + c->dbk_waves - 1) / c->dbk_waves) * s.W * 256;
+    const uint64_t strip_bytes = dstrip_bytes + (uint64_t)((s.H + c->intra_waves - 1) / c->intra_waves) * s.W * 64;
     uint64_t need = s.slot_bytes * d->dpb_slots + (uint64_t)s.W * s.H * 4 + (uint64_t)s.H * 8 + strip_bytes;
     if (c->budget && c->used + need > c->budget)
         return h264g_set_error(H264G_E_BUDGET, "DPB budget: %llu + %llu > %llu", (unsigned long long)c->used,
@@ -746,6 +749,7 @@ int32_t h264g_stream_open(h264g_ctx *c, const h264g_stream_desc *d, uint32_t *id
     }
     HIPCHK(hipMemset(s.rowprog, 0, (size_t)s.H * 8));
     HIPCHK(hipMemset(s.gstrip, 0, strip_bytes));  // epoch 0 never matches a launch
+    s.istrip = (uint64_t *)(s.gstrip + dstrip_bytes);
     HIPCHK(hipMemset(s.dpb, 0, s.slot_bytes * d->dpb_slots));
     HIPCHK(hipMemset(s.flags, 0, (size_t)s.W * s.H * 4));
     s.bytes = need;

@@ -56,6 +56,7 @@ struct DevPic {
     int32_t  dbk_idc;         // disable_deblocking_filter_idc
     int32_t  cip;             // constrained_intra_pred_flag
     uint8_t  *gstrip;         // deblocking strips between bands of the stream: W x 256 B per band
+    uint64_t *istrip;         // intra strips between bands of the stream: W x 8 words per band
     uint32_t *dbq;            // per-MB deblocking word, written by k_bs
     int16_t  *ires;           // residual of each intra MB with levels (DevAux::ires): luma
                               // 16x16 then Cb, Cr 8x8, raster; written by the inter stage

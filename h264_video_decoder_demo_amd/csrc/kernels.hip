@@ -603,6 +603,7 @@ struct WaveIntra {
     int16_t pf[25];        // filtered 8x8 reference samples p'[-1,-1..7] then p'[0..15,-1]
     int16_t U[25];         // the same samples before filtering (-1: not available)
     int16_t X[2][16];      // neighbour lines of two 4x4 blocks (see pred4x4)
+    uint8_t cbot[2][8];    // the MB's bottom chroma rows (for the intra strips)
 };
 
 // Availability of the neighbouring MBs A, B, C, D for intra prediction as bits 0..3 of
@@ -759,7 +760,10 @@ __device__ __forceinline__ void intra_res_place(const IntraRes &r, WaveIntra *wi
 // Prediction and reconstruction of an intra MB whose residual intra_res_place left in
 // wi->res (res: the MB has levels); the neighbours it reads are final.  The LDS writes of
 // the residual are ordered before its first wave barrier.
-__device__ __forceinline__ void intra_mb(const DevPic &P, int addr, WaveIntra *wi, const Avail &av, bool res) {
+// above: neighbours (Avail bits: 2 = B, 4 = C, 8 = D) whose samples of row -1 the caller
+// already put into the tile, from the intra strips.
+__device__ __forceinline__ void intra_mb(const DevPic &P, int addr, WaveIntra *wi, const Avail &av, bool res,
+                                         uint32_t above = 0) {
     const int lane = wave_lane();
     const h264g_mb *mb = &P.mbs[addr];
     const int W = P.W, H = P.H, wl = W * 16, hl = H * 16, wc = W * 8, hc = H * 8;
@@ -770,13 +774,14 @@ __device__ __forceinline__ void intra_mb(const DevPic &P, int addr, WaveIntra *w
     if (lane < 25) {  // row -1, x = lane - 1
         int x = lane - 1;
         bool ok = x < 0 ? av.D() : (x < 16 ? av.B() : av.C());
+        ok = ok && !(above & (x < 0 ? 8u : (x < 16 ? 2u : 4u)));
         if (ok) wi->T[0][lane] = dy[(uint64_t)(mby * 16 - 1) * wl + mbx * 16 + x];
     } else if (lane < 41) {  // column -1
         int y = lane - 25;
         if (av.A()) wi->T[y + 1][0] = dy[(uint64_t)(mby * 16 + y) * wl + mbx * 16 - 1];
     } else if (lane < 59) {  // chroma row -1 (x = -1..7) for both planes
         int k = lane - 41, c = k / 9, x = k % 9 - 1;
-        bool ok = x < 0 ? av.D() : av.B();
+        bool ok = (x < 0 ? av.D() : av.B()) && !(above & (x < 0 ? 8u : 2u));
         const uint8_t *pl = c == 0 ? dcb : dcr;
         if (ok) wi->C[c][0][x + 1] = pl[(uint64_t)(mby * 8 - 1) * wc + mbx * 8 + x];
     }
@@ -934,6 +939,7 @@ __device__ __forceinline__ void intra_mb(const DevPic &P, int addr, WaveIntra *w
                 v = clip1((aa + bb * (x - 3) + cc * (y - 3) + 16) >> 5);
             }
             if (res) v = clip1(v + wi->res.rc[c][y * 8 + x]);
+            if (y == 7) wi->cbot[c][x] = (uint8_t)v;
             uint8_t *pl = c == 0 ? dcb : dcr;
             pl[(uint64_t)(mby * 8 + y) * wc + mbx * 8 + x] = (uint8_t)v;
         }
@@ -1423,15 +1429,13 @@ __global__ __launch_bounds__(WAVE) void k_intra(const DevPic *__restrict__ pics,
 // ---- bands of MB rows, one workgroup per band
 // A band is `waves` consecutive MB rows of one picture, one row per wave.  Inside the
 // band the rows synchronise through LDS counters (workgroup scope: the waves share the
-// CU's caches).  A band's last row hands its results to the next band's first row, in
-// another workgroup and possibly another XCD, through device memory: it publishes its
-// progress word (P.rowprog[y], epoch << 32 | progress) with an agent-scope release every
-// INTRA_PUBLISH_EVERY intra MBs and at the row end; the reader polls it relaxed and then
-// acquires at agent scope.  (Deblocking hands epoch-tagged strips across instead, see
-// strip_put_g.)  Work items are (picture, band) in band-major order and a
-// workgroup takes its item by ticket when it starts, so the band above always belongs to
-// a workgroup that started earlier and never waits on a later one.
-constexpr int INTRA_PUBLISH_EVERY = 1;
+// CU's caches).  A band's last row hands what the next band's first row reads (in
+// another workgroup, possibly on another XCD) through device memory as epoch-tagged
+// 64-bit words, one agent-scope atomic each (strip_put_g, istrip_put): the reader polls
+// the words themselves, and no agent-scope release / acquire (an L2 write-back or
+// invalidation of the whole XCD) is issued.  Work items are (picture, band) in band-major
+// order and a workgroup takes its item by ticket when it starts, so the band above always
+// belongs to a workgroup that started earlier and never waits on a later one.
 
 // Takes the workgroup's ticket (into *slot) and clears `n` LDS counters.
 __device__ __forceinline__ uint32_t band_ticket(DevCtl *ctl, uint32_t *slot, uint32_t *ctr, int n) {
@@ -1453,23 +1457,33 @@ __device__ __forceinline__ void band_retire(DevCtl *ctl, uint32_t *epoch, uint32
         }
     }
 }
-// Progress of a band's last row for the next band (every lane stores the same word, so
-// each lane's earlier stores are released).
-__device__ __forceinline__ void band_publish(uint64_t *w, uint64_t tag, uint32_t v) {
+// Intra strips: the bottom row of each intra MB of a band's last MB row (luma 0..15, Cb
+// 0..7, Cr 0..7) as 8 words of 64 bits, 4 samples and the launch epoch each, for the
+// next band's first row.  As with the deblocking strips (strip_put_g), each word is one
+// agent-scope atomic and the reader waits until the words carry its epoch, so the
+// samples cross workgroups (and XCDs) without a fence; the picture copy of those samples
+// is not read across bands.
+__device__ __forceinline__ void istrip_put(uint64_t *sw, WaveIntra *wi, uint32_t epoch) {
+    const int lane = wave_lane();
     wave_sync();
-    __hip_atomic_store(w, tag | v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    if (lane < 8) {
+        const uint8_t *src = lane < 4 ? &wi->T[16][1 + 4 * lane] : &wi->cbot[(lane - 4) >> 1][4 * (lane & 1)];
+        const uint32_t v = (uint32_t)src[0] | (uint32_t)src[1] << 8 | (uint32_t)src[2] << 16 | (uint32_t)src[3] << 24;
+        __hip_atomic_store(&sw[lane], (uint64_t)epoch << 32 | v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
 }
-// Waits until the previous band's last row published at least `need`; `seen` caches the
-// last value read, so most calls return without touching memory.
-__device__ void band_wait(const uint64_t *w, uint64_t tag, uint32_t need, uint32_t &seen, uint32_t *err) {
-    if (seen >= need) return;
+// The row -1 samples of the intra neighbours in `above` (2: B = MB x, 4: C = x + 1, 8: D =
+// x - 1) from the previous band's strips (sw: its row of W x 8 words) into the tile.
+// Lanes 0..23: MB x - 1 + (lane >> 3), word lane & 7 (D: the last sample of each plane,
+// words 3, 5, 7; B: all; C: luma 0..7, words 0, 1).  Bounded like the other waits.
+__device__ void istrip_get(const uint64_t *sw, int x, uint32_t above, WaveIntra *wi, uint32_t epoch, uint32_t *err) {
+    const int lane = wave_lane(), m = lane >> 3, k = lane & 7;
+    const uint32_t bit = m == 0 ? 8u : (m == 1 ? 2u : 4u);
+    const bool mine = lane < 24 && (above & bit) && (m == 1 || (m == 0 ? (k == 3 || k == 5 || k == 7) : k < 2));
+    uint64_t v = (uint64_t)epoch << 32;
     for (uint32_t it = 0;; ++it) {
-        const uint64_t v = __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const uint32_t lo = uniform((uint32_t)v), hi = uniform((uint32_t)(v >> 32));
-        if (((uint64_t)hi << 32 | lo) >= (tag | need)) {
-            seen = lo;
-            break;
-        }
+        if (mine) v = __hip_atomic_load(&sw[(size_t)(x - 1 + m) * 8 + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (__ballot(mine && (uint32_t)(v >> 32) != epoch) == 0) break;
         if (it >= SPIN_LIMIT) {
             atomicOr(err, DEVERR_TIMEOUT);
             break;
@@ -1477,14 +1491,20 @@ __device__ void band_wait(const uint64_t *w, uint64_t tag, uint32_t need, uint32
         if ((it & 255u) == 255u && uniform(__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) != 0u) break;
         __builtin_amdgcn_s_sleep(1);
     }
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    if (mine) {
+        const uint32_t d = (uint32_t)v;
+        if (m == 0) {
+            const uint8_t s = (uint8_t)(d >> 24);
+            if (k == 3) wi->T[0][0] = s;
+            else wi->C[(k - 5) >> 1][0][0] = s;
+        } else {
+            uint8_t *dst = m == 2 ? &wi->T[0][17 + 4 * k] : (k < 4 ? &wi->T[0][1 + 4 * k] : &wi->C[(k - 4) >> 1][0][1 + 4 * (k & 1)]);
+#pragma unroll
+            for (int t = 0; t < 4; t++) dst[t] = (uint8_t)(d >> (8 * t));
+        }
+    }
 }
 
-// Intra stage over bands: the wave of row y walks the row's intra MBs (P.irow: row
-// offsets, then columns).  Row progress p means "every intra MB of the row left of
-// column p is done"; MB (x, y) needs the row above at min(x + 2, W) when it reads its
-// intra neighbour C, at x + 1 when it reads only B / D (intra_avail).  Neighbour samples
-// pass through the picture in device memory.
 constexpr int INTRA_WG_MAX_WAVES = 8;
 __global__ __launch_bounds__(WAVE * INTRA_WG_MAX_WAVES) void k_intra_band(const DevPic *__restrict__ pics,
                                                                           const uint32_t *__restrict__ items, uint32_t n,
@@ -1493,27 +1513,24 @@ __global__ __launch_bounds__(WAVE * INTRA_WG_MAX_WAVES) void k_intra_band(const 
     __shared__ uint32_t prog[INTRA_WG_MAX_WAVES];
     __shared__ uint32_t s_ticket;
     const uint32_t epoch = launch_epoch(epoch_p);
-    const uint64_t tag = (uint64_t)epoch << 32;
     const int nw = blockDim.x / WAVE, w = (int)uniform(threadIdx.x / WAVE);
     const uint32_t t = band_ticket(ctl, &s_ticket, prog, nw);
     if (t < n) {
         const uint32_t it = items[t];
         const DevPic &P = pics[it >> WI_PIC_SHIFT];
-        const int W = P.W, H = P.H, y0 = (int)(it & WI_MB_MASK) * nw, y = y0 + w, y1 = min(H, y0 + nw);
+        const int band = (int)(it & WI_MB_MASK);
+        const int W = P.W, H = P.H, y0 = band * nw, y = y0 + w, y1 = min(H, y0 + nw);
         if (P.n_intra != 0 && y < H) {
             const bool from_g = y > 0 && y == y0, to_g = y < H - 1 && y == y1 - 1;
+            uint64_t *iout = P.istrip + (size_t)band * W * 8;                 // this band -> next
+            const uint64_t *iin = P.istrip + (size_t)(band - 1) * W * 8;      // previous band -> this
             RowSyncLds sync{prog, err};
-            uint32_t seen = 0;
             const uint32_t *off = P.irow, *col = P.irow + H + 1;
             const uint32_t b = uniform(off[y]), e = uniform(off[y + 1]);
-            auto post = [&](uint32_t done, bool force) {
-                if (to_g) {
-                    if (force) band_publish(&P.rowprog[y], tag, done);
-                } else if (y < H - 1) {
-                    sync.post(y - y0, (int)done);
-                }
+            auto post = [&](uint32_t done) {  // progress for the next row of the band
+                if (!to_g && y < H - 1) sync.post(y - y0, (int)done);
             };
-            post(b < e ? uniform(col[b]) : (uint32_t)W, true);
+            post(b < e ? uniform(col[b]) : (uint32_t)W);
             for (uint32_t i = b; i < e; i++) {
                 const int x = (int)uniform(col[i]), addr = y * W + x;
                 int deps[4];
@@ -1521,14 +1538,20 @@ __global__ __launch_bounds__(WAVE * INTRA_WG_MAX_WAVES) void k_intra_band(const 
                 const IntraRes r = intra_res_fetch(P, addr);  // in flight during the wait
                 // the row above up to C (x + 1) or up to B (x) when only B / D are read and intra
                 const int need = deps[2] >= 0 ? min(x + 2, W) : (deps[1] >= 0 || deps[3] >= 0 ? x + 1 : 0);
+                uint32_t above = 0;
                 if (need > 0) {
-                    if (from_g) band_wait(&P.rowprog[y - 1], tag, (uint32_t)need, seen, err);
-                    else sync.wait(y - 1 - y0, need);
+                    if (from_g) {
+                        above = (deps[1] >= 0 ? 2u : 0u) | (deps[2] >= 0 ? 4u : 0u) | (deps[3] >= 0 ? 8u : 0u);
+                        istrip_get(iin, x, above, &tiles[w], epoch, err);
+                    } else {
+                        sync.wait(y - 1 - y0, need);
+                    }
                 }
                 intra_res_place(r, &tiles[w]);
-                intra_mb(P, addr, &tiles[w], av, r.res);
+                intra_mb(P, addr, &tiles[w], av, r.res, above);
+                if (to_g) istrip_put(iout + (size_t)x * 8, &tiles[w], epoch);
                 const bool end = i + 1 == e;
-                post(end ? (uint32_t)W : uniform(col[i + 1]), end || (i - b) % INTRA_PUBLISH_EVERY == INTRA_PUBLISH_EVERY - 1);
+                post(end ? (uint32_t)W : uniform(col[i + 1]));
             }
         }
     }
