@@ -105,45 +105,82 @@ struct alignas(16) WaveRes {
 };
 static_assert(sizeof(WaveMc) <= 27 * 16 * 4 + 4 * 64 * 4, "interpolation buffer overlays the levels");
 
-__device__ int level_slot(int i, int i16, int cbp, int t8) {
-    int idx = 0;
-    if (i16) { if (i == idx) return 0; idx++; }
-    for (int q = 0; q < 4; q++)
-        if (cbp & (1 << q))
-            for (int b = 0; b < 4; b++) { if (i == idx) return 1 + 4 * q + b; idx++; }
-    int cc = cbp >> 4;
-    if (cc >= 1) for (int c = 0; c < 2; c++) { if (i == idx) return 17 + c; idx++; }
-    if (cc == 2) for (int k = 0; k < 8; k++) { if (i == idx) return 19 + k; idx++; }
-    (void)t8;
+// Slot (see WaveRes) of the MB's i-th level mask, in the order of the level stream: the
+// Intra16x16 DC block, the four 4x4 blocks (or 16-level chunks of the 8x8 block) of each
+// 8x8 quadrant whose cbp bit is set, the chroma DC blocks, the chroma AC blocks; -1 past
+// the end.
+__device__ __forceinline__ int level_slot(int i, int i16, int cbp) {
+    if (i16) {
+        if (i == 0) return 0;
+        i--;
+    }
+    const int lm = cbp & 15, nl = 4 * __popc(lm);
+    if (i < nl) {
+        int c = i >> 2, q = 0;  // q: the (i >> 2)-th set bit of lm
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            const int set = (lm >> t) & 1;
+            if (set && c == 0) q = t;
+            c -= set;
+        }
+        return 1 + 4 * q + (i & 3);
+    }
+    i -= nl;
+    const int cc = cbp >> 4;
+    if (cc >= 1) {
+        if (i < 2) return 17 + i;
+        i -= 2;
+    }
+    if (cc == 2 && i < 8) return 19 + i;
     return -1;
 }
 
-// 8.5.12 on one lane: scan-order c -> residual r (raster), dc_pass keeps c[0] as is.
-__device__ void residual4x4(const int32_t *scan, int qp, int dc_pass, int32_t r[16]) {
-    int32_t d[16];
-#pragma unroll
-    for (int k = 0; k < 16; k++) {
-        int pos = c_zz4[k], i = pos >> 2, j = pos & 3;
-        int32_t v = scan[k];
-        int32_t o;
-        if (k == 0 && dc_pass) o = v;
-        else if (qp >= 24) o = (v * ls4(qp % 6, i, j)) << (qp / 6 - 4);
-        else o = (v * ls4(qp % 6, i, j) + (1 << (3 - qp / 6))) >> (4 - qp / 6);
-        d[pos] = o;
-    }
-    int32_t f[16];
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-        int32_t e0 = d[i * 4] + d[i * 4 + 2], e1 = d[i * 4] - d[i * 4 + 2];
-        int32_t e2 = (d[i * 4 + 1] >> 1) - d[i * 4 + 3], e3 = d[i * 4 + 1] + (d[i * 4 + 3] >> 1);
-        f[i * 4] = e0 + e3; f[i * 4 + 1] = e1 + e2; f[i * 4 + 2] = e1 - e2; f[i * 4 + 3] = e0 - e3;
-    }
+// Scan positions of the samples (r, 0..3) of a 4x4 block, 4 bits each: the inverse of c_zz4.
+constexpr uint64_t izz4_table() {
+    uint64_t v = 0;
+    for (int k = 0; k < 16; k++) v |= (uint64_t)k << (4 * c_zz4[k]);
+    return v;
+}
+constexpr uint64_t IZZ4 = izz4_table();
+
+// 4x4 transpose over the four lanes of a quad (lane r holds row r in m[0..3], afterwards
+// column r): two exchange stages with DPP quad permutations.
+__device__ __forceinline__ int dpp_xor1(int v) { return __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, true); }  // [1,0,3,2]
+__device__ __forceinline__ int dpp_xor2(int v) { return __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, true); }  // [2,3,0,1]
+__device__ __forceinline__ void quad_transpose(int32_t (&m)[4], int r) {
+    const bool odd = r & 1, hi = r & 2;
+    int a = dpp_xor1(odd ? m[0] : m[1]), b = dpp_xor1(odd ? m[2] : m[3]);
+    if (odd) { m[0] = a; m[2] = b; } else { m[1] = a; m[3] = b; }
+    a = dpp_xor2(hi ? m[0] : m[2]);
+    b = dpp_xor2(hi ? m[1] : m[3]);
+    if (hi) { m[0] = a; m[1] = b; } else { m[2] = a; m[3] = b; }
+}
+
+// 8.5.12 for row r of a 4x4 block on the quad that holds the block (lane r of the quad):
+// dequantise the row's levels (scan order in `scan`), horizontal pass, transpose, vertical
+// pass; returns column r of the residual in out[0..3] (rows 0..3).  dc_pass: sample
+// (0, 0) is `dc` as it stands (Intra16x16 and chroma DC).
+__device__ __forceinline__ void residual4x4_quad(const int32_t *scan, int qp, bool dc_pass, int dc, int r, int32_t (&out)[4]) {
+    const int m = qp % 6, e = qp / 6;
+    const int s0 = 16 * c_v4[m][0], s1 = 16 * c_v4[m][1], s2 = 16 * c_v4[m][2];
+    const uint32_t iz = (uint32_t)(IZZ4 >> (16 * r)) & 0xFFFFu;
+    int32_t d[4];
 #pragma unroll
     for (int j = 0; j < 4; j++) {
-        int32_t g0 = f[j] + f[8 + j], g1 = f[j] - f[8 + j], g2 = (f[4 + j] >> 1) - f[12 + j], g3 = f[4 + j] + (f[12 + j] >> 1);
-        r[j] = (g0 + g3 + 32) >> 6; r[4 + j] = (g1 + g2 + 32) >> 6;
-        r[8 + j] = (g1 - g2 + 32) >> 6; r[12 + j] = (g0 - g3 + 32) >> 6;
+        const int32_t v = scan[(iz >> (4 * j)) & 15u];
+        const int ls = ((r | j) & 1) == 0 ? s0 : (((r & j) & 1) ? s1 : s2);
+        d[j] = e >= 4 ? (v * ls) << (e - 4) : (v * ls + (1 << (3 - e))) >> (4 - e);
     }
+    if (dc_pass && r == 0) d[0] = dc;
+    int32_t f[4];
+    {
+        const int32_t e0 = d[0] + d[2], e1 = d[0] - d[2], e2 = (d[1] >> 1) - d[3], e3 = d[1] + (d[3] >> 1);
+        f[0] = e0 + e3; f[1] = e1 + e2; f[2] = e1 - e2; f[3] = e0 - e3;
+    }
+    quad_transpose(f, r);
+    const int32_t g0 = f[0] + f[2], g1 = f[0] - f[2], g2 = (f[1] >> 1) - f[3], g3 = f[1] + (f[3] >> 1);
+    out[0] = (g0 + g3 + 32) >> 6; out[1] = (g1 + g2 + 32) >> 6;
+    out[2] = (g1 - g2 + 32) >> 6; out[3] = (g0 - g3 + 32) >> 6;
 }
 
 __device__ __forceinline__ void idct8(int32_t *v, int s) {  // 8.5.13.2, in place, stride s
@@ -180,7 +217,7 @@ __device__ __forceinline__ bool build_residual(const h264g_mb *mb, const DevAux 
     }
     pre -= cnt;
     if (lane < nmask) {
-        int slot = level_slot(lane, i16, cbp, t8);
+        int slot = level_slot(lane, i16, cbp);
         const int16_t *vals = lv + nmask + pre;
         int n = 0;
         while (mask) {
@@ -206,12 +243,11 @@ __device__ __forceinline__ bool build_residual(const h264g_mb *mb, const DevAux 
 #pragma unroll
             for (int j = 0; j < 8; j++) ws->t8[q][i * 8 + j] = d[j];
         }
-    } else if (lane < 16) {
-        int b = lane;
-        int32_t scan[16];
-#pragma unroll
-        for (int k = 0; k < 16; k++) scan[k] = ws->coef[1 + b][k];
-        int dcpass = 0;
+    } else if (i16 || (cbp & 15)) {
+        // 4x4 luma blocks: the quad of lanes 4b..4b+3 takes block luma4x4BlkIdx b, lane 4b + r
+        // its row r (then column r)
+        const int b = lane >> 2, r = lane & 3;
+        int dc = 0;
         if (i16) {  // 8.5.10: f = (H c H)[row][col] with H the 4x4 Hadamard, then scaling
             const int row = blk_y(b) >> 2, col = blk_x(b) >> 2;
             // H[r][i] is -1 where: r = 1, i >= 2; r = 2, i in {1, 2}; r = 3, i odd
@@ -230,31 +266,31 @@ __device__ __forceinline__ bool build_residual(const h264g_mb *mb, const DevAux 
                 }
                 f += hneg(row, i) ? -acc : acc;
             }
-            int s = ls4(qp % 6, 0, 0);
-            scan[0] = qp >= 36 ? (f * s) << (qp / 6 - 6) : (f * s + (1 << (5 - qp / 6))) >> (6 - qp / 6);
-            dcpass = 1;
+            const int sc = ls4(qp % 6, 0, 0);
+            dc = qp >= 36 ? (f * sc) << (qp / 6 - 6) : (f * sc + (1 << (5 - qp / 6))) >> (6 - qp / 6);
         }
-        int32_t r[16];
-        residual4x4(scan, qp, dcpass, r);
-        int x0 = blk_x(b), y0 = blk_y(b);
+        int32_t o[4];
+        residual4x4_quad(ws->coef[1 + b], qp, i16, dc, r, o);
+        const int x0 = blk_x(b), y0 = blk_y(b);
 #pragma unroll
-        for (int k = 0; k < 16; k++) ws->rl[(y0 + (k >> 2)) * 16 + x0 + (k & 3)] = (int16_t)r[k];
+        for (int i = 0; i < 4; i++) ws->rl[(y0 + i) * 16 + x0 + r] = (int16_t)o[i];
+    } else {  // no luma levels
+        ((uint2 *)ws->rl)[lane] = make_uint2(0u, 0u);
     }
-    if (lane >= 32 && lane < 40) {  // chroma: 8.5.11 DC then 8.5.12 per 4x4 block
-        int c = (lane - 32) >> 2, b = (lane - 32) & 3;
-        int qpc = ax.qpc[c];
-        const int32_t *dc = ws->coef[17 + c];
-        int32_t e00 = dc[0] + dc[2], e01 = dc[1] + dc[3], e10 = dc[0] - dc[2], e11 = dc[1] - dc[3];
-        int32_t f = b == 0 ? e00 + e01 : b == 1 ? e00 - e01 : b == 2 ? e10 + e11 : e10 - e11;
-        int32_t scan[16];
+    if (!(cbp >> 4)) {  // no chroma levels
+        ((uint32_t *)ws->rc)[lane] = 0u;
+    } else if (lane < 32) {  // chroma: 8.5.11 DC then 8.5.12 per 4x4 block, a quad per block
+        const int c = lane >> 4, b = (lane >> 2) & 3, r = lane & 3;
+        const int qpc = ax.qpc[c];
+        const int32_t *dcv = ws->coef[17 + c];
+        const int32_t e00 = dcv[0] + dcv[2], e01 = dcv[1] + dcv[3], e10 = dcv[0] - dcv[2], e11 = dcv[1] - dcv[3];
+        const int32_t f = b == 0 ? e00 + e01 : b == 1 ? e00 - e01 : b == 2 ? e10 + e11 : e10 - e11;
+        const int dc = ((f * ls4(qpc % 6, 0, 0)) << (qpc / 6)) >> 5;
+        int32_t o[4];
+        residual4x4_quad(ws->coef[19 + 4 * c + b], qpc, true, dc, r, o);
+        const int x0 = (b & 1) * 4, y0 = (b >> 1) * 4;
 #pragma unroll
-        for (int k = 0; k < 16; k++) scan[k] = ws->coef[19 + 4 * c + b][k];
-        scan[0] = ((f * ls4(qpc % 6, 0, 0)) << (qpc / 6)) >> 5;
-        int32_t r[16];
-        residual4x4(scan, qpc, 1, r);
-        int x0 = (b & 1) * 4, y0 = (b >> 1) * 4;
-#pragma unroll
-        for (int k = 0; k < 16; k++) ws->rc[c][(y0 + (k >> 2)) * 8 + x0 + (k & 3)] = (int16_t)r[k];
+        for (int i = 0; i < 4; i++) ws->rc[c][(y0 + i) * 8 + x0 + r] = (int16_t)o[i];
     }
     if (t8) {
         wave_sync();
@@ -631,47 +667,108 @@ __device__ __forceinline__ int tile_p(const WaveIntra *wi, const Avail &av, int 
     return wi->T[yy + 1][xx + 1];
 }
 
+// Flat index into WaveIntra::T of the sample tile_p reads (T[0][0] = p[-1, -1] of the MB),
+// or -1 when it is not available.
+__device__ __forceinline__ int tile_a(const Avail &av, int xO, int yO, int x, int y, int blk4) {
+    const int xx = xO + x, yy = yO + y;
+    if (yy < 0) {
+        if (xx < 0) return av.D() ? 0 : -1;
+        if (xx < 16) return av.B() ? xx + 1 : -1;
+        return av.C() ? xx + 1 : -1;
+    }
+    if (xx < 0) return av.A() ? (yy + 1) * 25 : -1;
+    if (xx >= 16) return -1;
+    if (blk4 >= 0 && x > 3 && (blk4 == 3 || blk4 == 11)) return -1;
+    return (yy + 1) * 25 + xx + 1;
+}
+
 // 8.3.1.2 for pixel (x, y) of a 4x4 block (H264PictureBase.cpp:1062-1419), from the
 // block's neighbour line X[0..14] = L3 L3 L2 L1 L0 TL T0 .. T7 T7 (L = p[-1, y],
 // T = p[x, -1], TL = p[-1, -1]; T4..T7 already substituted by T3 when unavailable;
-// -1 = not available).  Every directional mode reads a 2-tap (X[i] + X[i+1] + 1) >> 1
-// or a 3-tap (X[c-1] + 2 X[c] + X[c+1] + 2) >> 2 of this line: the equations of
-// 8.3.1.2.4 - 8.3.1.2.9 with p[-1, y] and p[x, -1] mapped to their X positions (the
-// duplicated end samples give the corner cases, e.g. Diagonal_Down_Left at (3, 3) and
-// Horizontal_Up at zHU = 5).
-__device__ __forceinline__ int xtap2(const int16_t *X, int i) { return (X[i] + X[i + 1] + 1) >> 1; }
-__device__ __forceinline__ int xtap3(const int16_t *X, int c) { return (X[c - 1] + 2 * X[c] + X[c + 1] + 2) >> 2; }
-__device__ int pred4x4(const int16_t *X, int mode, int x, int y) {
+// -1 = not available).  Every directional mode reads a copy X[i], a 2-tap
+// (X[i] + X[i+1] + 1) >> 1 or a 3-tap (X[c-1] + 2 X[c] + X[c+1] + 2) >> 2 of this line:
+// the equations of 8.3.1.2.4 - 8.3.1.2.9 with p[-1, y] and p[x, -1] mapped to their X
+// positions (the duplicated end samples give the corner cases, e.g. Diagonal_Down_Left
+// at (3, 3) and Horizontal_Up at zHU = 5).  All three are (X[i0] + X[i1] + X[i2] + X[i3]
+// + 2) >> 2: a copy repeats one index four times, a 2-tap equals (2 X[i] + 2 X[i+1] + 2)
+// >> 2 and a 3-tap repeats its middle index.  i4_taps gives the four indices (4 bits each,
+// i0 lowest) of sample (x, y) in mode m; DC (mode 2) is 0xFFFF.
+constexpr uint16_t i4_copy(int i) { return (uint16_t)(i | i << 4 | i << 8 | i << 12); }
+constexpr uint16_t i4_tap2(int i) { return (uint16_t)(i | i << 4 | (i + 1) << 8 | (i + 1) << 12); }
+constexpr uint16_t i4_tap3(int c) { return (uint16_t)((c - 1) | c << 4 | c << 8 | (c + 1) << 12); }
+constexpr uint16_t i4_taps(int mode, int x, int y) {
     switch (mode) {
-    case 0: return X[6 + x];  // vertical
-    case 1: return X[4 - y];  // horizontal
-    case 2: {                 // DC: the top / left samples are available all or none
-        const bool ta = X[6] >= 0, la = X[4] >= 0;
-        const int st = X[6] + X[7] + X[8] + X[9], sl = X[1] + X[2] + X[3] + X[4];
-        if (ta && la) return (st + sl + 4) >> 3;
-        if (la) return (sl + 2) >> 2;
-        if (ta) return (st + 2) >> 2;
-        return 128;
-    }
-    case 3: return xtap3(X, 7 + x + y);  // diagonal down left
-    case 4: return xtap3(X, 5 + x - y);  // diagonal down right
-    case 5: {                            // vertical right
+    case 0: return i4_copy(6 + x);
+    case 1: return i4_copy(4 - y);
+    case 2: return 0xFFFF;
+    case 3: return i4_tap3(7 + x + y);
+    case 4: return i4_tap3(5 + x - y);
+    case 5: {
         const int z = 2 * x - y;
-        if (z >= 0) return (z & 1) ? xtap3(X, 5 + x - (y >> 1)) : xtap2(X, 5 + x - (y >> 1));
-        return z == -1 ? xtap3(X, 5) : xtap3(X, 6 - y);
+        if (z >= 0) return (z & 1) ? i4_tap3(5 + x - (y >> 1)) : i4_tap2(5 + x - (y >> 1));
+        return z == -1 ? i4_tap3(5) : i4_tap3(6 - y);
     }
-    case 6: {                            // horizontal down
+    case 6: {
         const int z = 2 * y - x;
-        if (z >= 0) return (z & 1) ? xtap3(X, 5 - y + (x >> 1)) : xtap2(X, 4 - y + (x >> 1));
-        return z == -1 ? xtap3(X, 5) : xtap3(X, 4 + x);
+        if (z >= 0) return (z & 1) ? i4_tap3(5 - y + (x >> 1)) : i4_tap2(4 - y + (x >> 1));
+        return z == -1 ? i4_tap3(5) : i4_tap3(4 + x);
     }
-    case 7: return (y & 1) ? xtap3(X, 7 + x + (y >> 1)) : xtap2(X, 6 + x + (y >> 1));  // vertical left
-    default: {                                                                         // horizontal up
+    case 7: return (y & 1) ? i4_tap3(7 + x + (y >> 1)) : i4_tap2(6 + x + (y >> 1));
+    default: {
         const int z = x + 2 * y;
-        if (z < 5) return (z & 1) ? xtap3(X, 3 - y - (x >> 1)) : xtap2(X, 3 - y - (x >> 1));
-        return z == 5 ? xtap3(X, 1) : X[1];
+        if (z < 5) return (z & 1) ? i4_tap3(3 - y - (x >> 1)) : i4_tap2(3 - y - (x >> 1));
+        return z == 5 ? i4_tap3(1) : i4_copy(1);
     }
     }
+}
+struct I4Taps {
+    uint16_t v[9][16];
+};
+constexpr I4Taps make_i4taps() {
+    I4Taps t{};
+    for (int m = 0; m < 9; m++)
+        for (int k = 0; k < 16; k++) t.v[m][k] = i4_taps(m, k & 3, k >> 2);
+    return t;
+}
+__constant__ I4Taps c_i4taps = make_i4taps();
+
+// What an Intra_4x4 MB's wavefront steps need that does not depend on samples, gathered
+// before the MB waits for its neighbours: per step t (two 16-bit fields per word) the
+// flat tile index of neighbour-line sample j (lanes j < 15 of each 16-lane group; -1: not
+// available, T4..T7 already substituted by T3) and the taps of pixel j of the lane's
+// block (c_i4taps).
+struct I4Prep {
+    uint32_t xs[5], tp[5];
+};
+__device__ __forceinline__ I4Prep intra4_prep(const h264g_mb *mb, const Avail &av) {
+    const int lane = wave_lane(), g = lane >> 4, j = lane & 15;
+    I4Prep p;
+#pragma unroll
+    for (int t = 0; t < 10; t++) {
+        const int by = (t < 2 ? 0 : (t - 2) >> 1) + g, bx = t - 2 * by;
+        const bool has = lane < 32 && bx >= 0 && by < 4;
+        const int blk = 8 * (by >> 1) + 4 * (bx >> 1) + 2 * (by & 1) + (bx & 1), xO = bx * 4, yO = by * 4;
+        int a = -1;
+        uint32_t tp = 0;
+        if (has) {
+            if (j < 5) a = tile_a(av, xO, yO, -1, j == 0 ? 3 : 4 - j, blk);
+            else if (j == 5) a = tile_a(av, xO, yO, -1, -1, blk);
+            else if (j < 15) {
+                const int tx = j == 14 ? 7 : j - 6;
+                a = tile_a(av, xO, yO, tx, -1, blk);
+                if (tx >= 4 && a < 0) {  // p[4..7, -1] <- p[3, -1] (H264PictureBase.cpp:1161-1167)
+                    const int a3 = tile_a(av, xO, yO, 3, -1, blk);
+                    if (a3 >= 0) a = a3;
+                }
+            }
+            tp = c_i4taps.v[mb->intra_pred_modes[blk]][j];
+        }
+        const uint32_t sh = 16 * (t & 1);
+        if (!(t & 1)) { p.xs[t >> 1] = 0; p.tp[t >> 1] = 0; }
+        p.xs[t >> 1] |= ((uint32_t)a & 0xFFFFu) << sh;
+        p.tp[t >> 1] |= tp << sh;
+    }
+    return p;
 }
 
 // 8.3.2.2 for pixel (x, y) of the 8x8 block blk using the filtered samples in wi->pf:
@@ -762,8 +859,9 @@ __device__ __forceinline__ void intra_res_place(const IntraRes &r, WaveIntra *wi
 // the residual are ordered before its first wave barrier.
 // above: neighbours (Avail bits: 2 = B, 4 = C, 8 = D) whose samples of row -1 the caller
 // already put into the tile, from the intra strips.
+// p4: intra4_prep of the MB (used when it is Intra_4x4).
 __device__ __forceinline__ void intra_mb(const DevPic &P, int addr, WaveIntra *wi, const Avail &av, bool res,
-                                         uint32_t above = 0) {
+                                         const I4Prep &p4, uint32_t above = 0) {
     const int lane = wave_lane();
     const h264g_mb *mb = &P.mbs[addr];
     const int W = P.W, H = P.H, wl = W * 16, hl = H * 16, wc = W * 8, hc = H * 8;
@@ -796,30 +894,28 @@ __device__ __forceinline__ void intra_mb(const DevPic &P, int addr, WaveIntra *w
         // 4x4 blocks in wavefront order t = bx + 2 by, two blocks per step (lanes 0-15 and
         // 16-31): every neighbour a block reads has a smaller t, and a top-right block that
         // the reference's block order has not decoded yet (blocks 3, 11) counts as not
-        // available by rule (tile_p), so the result equals the block-index order.
-        const int g = lane >> 4, j = lane & 15;
+        // available by rule (tile_a), so the result equals the block-index order.  Each
+        // step gathers the two neighbour lines X (pred4x4) from the tile at the indices
+        // intra4_prep found, then evaluates every pixel with its four taps.
+        const int g = lane >> 4, j = lane & 15, x = j & 3, y = j >> 2;
+        const uint8_t *Tf = &wi->T[0][0];
+        const int16_t *X = wi->X[g];
+#pragma unroll
         for (int t = 0; t < 10; t++) {
             const int by = (t < 2 ? 0 : (t - 2) >> 1) + g, bx = t - 2 * by;
             const bool has = lane < 32 && bx >= 0 && by < 4;
-            const int blk = 8 * (by >> 1) + 4 * (bx >> 1) + 2 * (by & 1) + (bx & 1), xO = bx * 4, yO = by * 4;
-            if (has && j < 15) {  // neighbour line X (see pred4x4)
-                int v;
-                if (j < 5) v = tile_p(wi, av, xO, yO, -1, j == 0 ? 3 : 4 - j, blk);
-                else if (j == 5) v = tile_p(wi, av, xO, yO, -1, -1, blk);
-                else {
-                    const int tx = j == 14 ? 7 : j - 6;
-                    v = tile_p(wi, av, xO, yO, tx, -1, blk);
-                    if (tx >= 4 && v < 0) {  // p[4..7, -1] <- p[3, -1] (H264PictureBase.cpp:1161-1167)
-                        const int t3 = tile_p(wi, av, xO, yO, 3, -1, blk);
-                        if (t3 >= 0) v = t3;
-                    }
-                }
-                wi->X[g][j] = (int16_t)v;
-            }
+            const int xO = bx * 4, yO = by * 4;
+            const int xa = (int)(int16_t)(p4.xs[t >> 1] >> (16 * (t & 1)));
+            const uint32_t tp = (p4.tp[t >> 1] >> (16 * (t & 1))) & 0xFFFFu;
+            if (has && j < 15) wi->X[g][j] = (int16_t)(xa >= 0 ? (int)Tf[xa] : -1);
             wave_sync();
             if (has) {
-                const int x = j & 3, y = j >> 2;
-                int v = pred4x4(wi->X[g], mb->intra_pred_modes[blk], x, y);
+                int v = (X[tp & 15u] + X[(tp >> 4) & 15u] + X[(tp >> 8) & 15u] + X[tp >> 12] + 2) >> 2;
+                if (tp == 0xFFFFu) {  // DC: the top / left samples are available all or none
+                    const bool ta = X[6] >= 0, la = X[4] >= 0;
+                    const int st = X[6] + X[7] + X[8] + X[9], sl = X[1] + X[2] + X[3] + X[4];
+                    v = ta && la ? (st + sl + 4) >> 3 : la ? (sl + 2) >> 2 : ta ? (st + 2) >> 2 : 128;
+                }
                 if (res) v = clip1(v + wi->res.rl[(yO + y) * 16 + xO + x]);
                 wi->T[yO + y + 1][xO + x + 1] = (uint8_t)v;
             }
@@ -1418,9 +1514,10 @@ __global__ __launch_bounds__(WAVE) void k_intra(const DevPic *__restrict__ pics,
         int deps[4];
         const Avail av = intra_avail(P, addr, deps);
         const IntraRes r = intra_res_fetch(P, addr);
+        const I4Prep p4 = intra4_prep(&P.mbs[addr], av);
         wait_deps(P.flags, deps, 4, epoch, err);
         intra_res_place(r, &lds);
-        intra_mb(P, addr, &lds, av, r.res);
+        intra_mb(P, addr, &lds, av, r.res, p4);
         publish(&P.flags[addr], epoch);
     }
     retire(ctl, gridDim.x, epoch_p, epoch);
@@ -1536,6 +1633,7 @@ __global__ __launch_bounds__(WAVE * INTRA_WG_MAX_WAVES) void k_intra_band(const 
                 int deps[4];
                 const Avail av = intra_avail(P, addr, deps);
                 const IntraRes r = intra_res_fetch(P, addr);  // in flight during the wait
+                const I4Prep p4 = intra4_prep(&P.mbs[addr], av);
                 // the row above up to C (x + 1) or up to B (x) when only B / D are read and intra
                 const int need = deps[2] >= 0 ? min(x + 2, W) : (deps[1] >= 0 || deps[3] >= 0 ? x + 1 : 0);
                 uint32_t above = 0;
@@ -1548,7 +1646,7 @@ __global__ __launch_bounds__(WAVE * INTRA_WG_MAX_WAVES) void k_intra_band(const 
                     }
                 }
                 intra_res_place(r, &tiles[w]);
-                intra_mb(P, addr, &tiles[w], av, r.res, above);
+                intra_mb(P, addr, &tiles[w], av, r.res, p4, above);
                 if (to_g) istrip_put(iout + (size_t)x * 8, &tiles[w], epoch);
                 const bool end = i + 1 == e;
                 post(end ? (uint32_t)W : uniform(col[i + 1]));
