@@ -49,7 +49,7 @@ def stream_specs(frames):
     specs, sid = [], 0
     for n, W, H in MIX:
         for _ in range(n):
-            specs.append({"width_mbs": W, "height_mbs": H, "n_frames": frames, "seed": SEED4 + sid})
+            specs.append({"width_mbs": W, "height_mbs": H, "n_frames": frames, "seed": SEED4 + sid, "spare_slots": 2})
             sid += 1
     return specs
 

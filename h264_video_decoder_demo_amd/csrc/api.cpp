@@ -56,6 +56,30 @@ struct Stream {
 
 struct Batch;
 
+// The DPB slots one picture writes (dst) and reads (its reference pictures).
+struct SlotUse {
+    uint32_t stream;
+    int32_t dst;
+    std::vector<int32_t> refs;
+};
+SlotUse slot_use(uint32_t stream, const h264g_pic_desc &d) {
+    SlotUse u{stream, d.dst_slot, {}};
+    for (int l = 0; l < 2; l++)
+        for (int r = 0; r < 32; r++)
+            if (d.ref_slot[l][r] >= 0 && std::find(u.refs.begin(), u.refs.end(), d.ref_slot[l][r]) == u.refs.end())
+                u.refs.push_back(d.ref_slot[l][r]);
+    return u;
+}
+
+// Deblocking launches of one lane still possibly in flight on its deblocking stream: the
+// event recorded after each and the (stream, slot) pairs it writes.
+constexpr int kDbkHist = 4;
+struct DbkRec {
+    hipEvent_t ev = nullptr;
+    std::vector<std::pair<uint32_t, int32_t>> dst;
+    bool live = false;
+};
+
 }  // namespace
 
 struct h264g_pic {
@@ -90,6 +114,8 @@ struct Batch {
     uint32_t n_inter = 0, n_intra = 0, n_all = 0, n_rows = 0, n_ibands = 0, n_dbands = 0;
     bool dbk_off = false;            // no picture of the batch is deblocked
     hipGraphExec_t graph = nullptr;  // the captured stage sequence (kernel nodes only)
+    hipGraphExec_t graph_dbk = nullptr;  // its deblocking, when that runs on the lane's deblocking stream
+    std::vector<SlotUse> uses;       // DPB slots the pictures write and read
 };
 }  // namespace
 
@@ -104,15 +130,25 @@ struct h264g_ctx {
     // pictures overlap the other pictures' stages instead of ending every stage.
     hipStream_t stream = nullptr;  // lane 0
     hipStream_t aux = nullptr;     // lane 1
+    // With the band deblocking schedule, a lane's deblocking launches go to a stream of
+    // their own (dstream), after the lane's other stages of the same pictures.  The next
+    // pictures' stages wait for a deblocking launch only when they write or read a DPB
+    // slot it writes (hist), so they overlap the deblocking of independent pictures (e.g.
+    // a B picture not used for reference).
+    hipStream_t dstream[2] = {nullptr, nullptr};
+    hipEvent_t front[2] = {nullptr, nullptr};  // end of a lane's last pre-deblocking stages
+    DbkRec hist[2][kDbkHist];
+    int hist_next[2] = {0, 0};
     int split_mbs = 16384;         // 0: one lane
     uint32_t *d_err = nullptr;
-    DevCtl *d_ctl = nullptr;       // per lane L: [4L] intra, [4L + 1] deblock, [4L + 2].ticket = launch epoch
+    DevCtl *d_ctl = nullptr;       // per lane L: [4L] intra, [4L + 1] deblock, [4L + 2].ticket = intra launch
+                                   // epoch, [4L + 3].ticket = deblocking launch epoch
     int waves = 2048;
     uint64_t budget = 0, used = 0;
     std::vector<Stream> streams;
     // Device blocks of released pictures / batches, reusable once `ready` has completed on
     // the stream (so releasing never waits for the GPU and staging rarely allocates).
-    struct PoolBlock { void *ptr; size_t size; hipEvent_t ready[2]; };  // one event per lane
+    struct PoolBlock { void *ptr; size_t size; hipEvent_t ready[4]; };  // lanes, then their deblocking streams
     std::vector<PoolBlock> pool;
     size_t pool_bytes = 0;
     bool profile = false;
@@ -121,7 +157,7 @@ struct h264g_ctx {
     int intra_schedule = H264G_INTRA_BANDS;
     int intra_waves = 8;           // rows per band (waves per workgroup) of the intra band schedule
     int dbk_waves = 16;            // rows per band (waves per workgroup) of the deblocking band schedule
-    std::vector<hipEvent_t> ev;    // 5 events per profiled run: inter | intra | bS | deblock | end
+    std::vector<hipEvent_t> ev;    // 6 events per profiled run: inter | intra | bS | end (lane), deblock | end
     size_t ev_used = 0;            // events recorded since the last h264g_stage_times
     std::mutex mu;
 };
@@ -137,10 +173,21 @@ extern "C" int32_t h264g_set_error(int32_t code, const char *fmt, ...) {
 namespace {
 
 hipStream_t lane_stream(const h264g_ctx *c, int lane) { return lane ? c->aux : c->stream; }
+// The four streams: lane 0, lane 1, their deblocking streams.
+hipStream_t any_stream(const h264g_ctx *c, int i) { return i < 2 ? lane_stream(c, i) : c->dstream[i - 2]; }
 
 hipError_t sync_lanes(h264g_ctx *c) {
-    hipError_t e = hipStreamSynchronize(c->stream);
-    hipError_t e2 = hipStreamSynchronize(c->aux);
+    hipError_t r = hipSuccess;
+    for (int i = 0; i < 4; i++) {
+        const hipError_t e = hipStreamSynchronize(any_stream(c, i));
+        if (r == hipSuccess) r = e;
+    }
+    return r;
+}
+// A lane and its deblocking stream.
+hipError_t sync_lane(h264g_ctx *c, int lane) {
+    const hipError_t e = hipStreamSynchronize(lane_stream(c, lane));
+    const hipError_t e2 = hipStreamSynchronize(c->dstream[lane]);
     return e != hipSuccess ? e : e2;
 }
 
@@ -158,10 +205,11 @@ void *pool_get(h264g_ctx *c, size_t n) {
     const size_t want = pool_class(n);
     for (size_t i = 0; i < c->pool.size(); i++) {
         auto &b = c->pool[i];
-        if (b.size == want && hipEventQuery(b.ready[0]) == hipSuccess && hipEventQuery(b.ready[1]) == hipSuccess) {
+        bool ready = b.size == want;
+        for (int k = 0; k < 4 && ready; k++) ready = hipEventQuery(b.ready[k]) == hipSuccess;
+        if (ready) {
             void *p = b.ptr;
-            (void)hipEventDestroy(b.ready[0]);
-            (void)hipEventDestroy(b.ready[1]);
+            for (auto ev : b.ready) (void)hipEventDestroy(ev);
             c->pool_bytes -= b.size;
             c->pool.erase(c->pool.begin() + (long)i);
             return p;
@@ -172,22 +220,22 @@ void *pool_get(h264g_ctx *c, size_t n) {
     return p;
 }
 
-// Returns a block; it becomes reusable when the work enqueued so far on both lanes has
-// completed.
+// Returns a block; it becomes reusable when the work enqueued so far on all four streams
+// has completed.
 void pool_put(h264g_ctx *c, void *p, size_t n) {
     if (!p) return;
-    hipEvent_t e[2] = {nullptr, nullptr};
+    hipEvent_t e[4] = {nullptr, nullptr, nullptr, nullptr};
     bool ok = true;
-    for (int l = 0; l < 2; l++)
+    for (int l = 0; l < 4; l++)
         ok = ok && hipEventCreateWithFlags(&e[l], hipEventDisableTiming) == hipSuccess &&
-             hipEventRecord(e[l], lane_stream(c, l)) == hipSuccess;
+             hipEventRecord(e[l], any_stream(c, l)) == hipSuccess;
     if (!ok) {
         (void)sync_lanes(c);  // cannot track it: free it once idle
         for (auto ev : e) if (ev) (void)hipEventDestroy(ev);
         (void)hipFree(p);
         return;
     }
-    c->pool.push_back({p, pool_class(n), {e[0], e[1]}});
+    c->pool.push_back({p, pool_class(n), {e[0], e[1], e[2], e[3]}});
     c->pool_bytes += pool_class(n);
     while (c->pool_bytes > kPoolMaxBytes && !c->pool.empty()) {  // trim the oldest
         auto &b = c->pool.front();
@@ -568,59 +616,144 @@ struct StageSet {
     bool dbk_off;
 };
 
-int launch_stages(h264g_ctx *ctx, const StageSet &q, int lane) {
-    hipStream_t s = lane_stream(ctx, lane);
+// Parts of a stage set: inter / intra / bS, and deblocking.
+constexpr int kFront = 1, kBack = 2;
+
+// Launches the given parts on stream s.  ev: the run's 6 profiling events (inter, intra,
+// bS, end of those; deblocking, end) or null.
+int launch_stages(h264g_ctx *ctx, const StageSet &q, int lane, int parts, hipStream_t s, hipEvent_t *ev) {
     DevCtl *ctl = &ctx->d_ctl[4 * lane];
-    uint32_t *epoch = &ctl[2].ticket;
-    hipEvent_t *ev = nullptr;
-    if (ctx->profile) {
-        while (ctx->ev.size() < ctx->ev_used + 5) {
-            hipEvent_t e;
-            HIPCHK(hipEventCreate(&e));
-            ctx->ev.push_back(e);
+    int e = 0;
+    if (parts & kFront) {
+        if (ev) HIPCHK(hipEventRecord(ev[0], s));
+        e = h264g_launch_inter(q.pics, q.inter, q.n_inter, q.intra, q.n_intra, ctx->d_err, s);
+        if (e) return h264g_set_error(H264G_E_DEVICE, "inter launch: %s", hipGetErrorString((hipError_t)e));
+        if (ev) HIPCHK(hipEventRecord(ev[1], s));
+        if (q.n_intra)
+            e = ctx->intra_schedule == H264G_INTRA_BANDS
+                    ? h264g_launch_intra_band(q.pics, q.ibands, q.n_ibands, &ctl[0], &ctl[2].ticket, ctx->intra_waves,
+                                              ctx->d_err, s)
+                    : h264g_launch_intra(q.pics, q.intra, q.n_intra, &ctl[0], &ctl[2].ticket, ctx->d_err, ctx->waves, s);
+        if (e) return h264g_set_error(H264G_E_DEVICE, "intra launch: %s", hipGetErrorString((hipError_t)e));
+        if (ev) HIPCHK(hipEventRecord(ev[2], s));
+        if (!q.dbk_off) {
+            e = h264g_launch_bs(q.pics, q.all, q.n_all, s);
+            if (e) return h264g_set_error(H264G_E_DEVICE, "bS launch: %s", hipGetErrorString((hipError_t)e));
         }
-        ev = &ctx->ev[ctx->ev_used];
-        ctx->ev_used += 5;
-    }
-    if (ev) HIPCHK(hipEventRecord(ev[0], s));
-    int e = h264g_launch_inter(q.pics, q.inter, q.n_inter, q.intra, q.n_intra, ctx->d_err, s);
-    if (e) return h264g_set_error(H264G_E_DEVICE, "inter launch: %s", hipGetErrorString((hipError_t)e));
-    if (ev) HIPCHK(hipEventRecord(ev[1], s));
-    if (q.n_intra)
-        e = ctx->intra_schedule == H264G_INTRA_BANDS
-                ? h264g_launch_intra_band(q.pics, q.ibands, q.n_ibands, &ctl[0], epoch, ctx->intra_waves,
-                                          ctx->d_err, s)
-                : h264g_launch_intra(q.pics, q.intra, q.n_intra, &ctl[0], epoch, ctx->d_err, ctx->waves, s);
-    if (e) return h264g_set_error(H264G_E_DEVICE, "intra launch: %s", hipGetErrorString((hipError_t)e));
-    if (ev) HIPCHK(hipEventRecord(ev[2], s));
-    if (!q.dbk_off) {
-        e = h264g_launch_bs(q.pics, q.all, q.n_all, s);
-        if (e) return h264g_set_error(H264G_E_DEVICE, "bS launch: %s", hipGetErrorString((hipError_t)e));
         if (ev) HIPCHK(hipEventRecord(ev[3], s));
-        switch (ctx->dbk_schedule) {
-        case H264G_DEBLOCK_BANDS:
-            e = h264g_launch_deblock_band(q.pics, q.dbands, q.n_dbands, &ctl[1], epoch, ctx->dbk_waves,
-                                          ctx->d_err, s);
-            break;
-        case H264G_DEBLOCK_MB_WAVEFRONT:
-            e = h264g_launch_deblock(q.pics, q.all, q.n_all, &ctl[1], epoch, ctx->d_err, ctx->waves, s);
-            break;
-        default:
-            e = h264g_launch_deblock_rows(q.pics, q.rows, q.n_rows, &ctl[1], epoch, ctx->d_err, ctx->waves, s);
-        }
-        if (e) return h264g_set_error(H264G_E_DEVICE, "deblock launch: %s", hipGetErrorString((hipError_t)e));
     }
-    else if (ev) HIPCHK(hipEventRecord(ev[3], s));
-    if (ev) HIPCHK(hipEventRecord(ev[4], s));
+    if (parts & kBack) {
+        if (ev) HIPCHK(hipEventRecord(ev[4], s));
+        if (!q.dbk_off) {
+            uint32_t *epoch = &ctl[3].ticket;
+            switch (ctx->dbk_schedule) {
+            case H264G_DEBLOCK_BANDS:
+                e = h264g_launch_deblock_band(q.pics, q.dbands, q.n_dbands, &ctl[1], epoch, ctx->dbk_waves,
+                                              ctx->d_err, s);
+                break;
+            case H264G_DEBLOCK_MB_WAVEFRONT:
+                e = h264g_launch_deblock(q.pics, q.all, q.n_all, &ctl[1], epoch, ctx->d_err, ctx->waves, s);
+                break;
+            default:
+                e = h264g_launch_deblock_rows(q.pics, q.rows, q.n_rows, &ctl[1], epoch, ctx->d_err, ctx->waves, s);
+            }
+            if (e) return h264g_set_error(H264G_E_DEVICE, "deblock launch: %s", hipGetErrorString((hipError_t)e));
+        }
+        if (ev) HIPCHK(hipEventRecord(ev[5], s));
+    }
+    return 0;
+}
+
+hipEvent_t *profile_events(h264g_ctx *ctx) {
+    if (!ctx->profile) return nullptr;
+    while (ctx->ev.size() < ctx->ev_used + 6) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        ctx->ev.push_back(e);
+    }
+    hipEvent_t *ev = &ctx->ev[ctx->ev_used];
+    ctx->ev_used += 6;
+    return ev;
+}
+
+bool conflicts(const DbkRec &r, const std::vector<SlotUse> &uses) {
+    for (const auto &d : r.dst)
+        for (const SlotUse &u : uses)
+            if (u.stream == d.first && (u.dst == d.second || std::find(u.refs.begin(), u.refs.end(), d.second) != u.refs.end()))
+                return true;
+    return false;
+}
+
+// Stages of `q` (pictures `uses`) on `lane`.  sb: the batch whose captured graphs to replay
+// (captured on first use), or null for direct launches.  Order: the lane waits for the
+// newest deblocking launch in flight that writes a slot these pictures write or read
+// (deblocking launches on the lane's deblocking stream complete in order, so that one
+// covers the older ones); inter / intra / bS on the lane; deblocking on the deblocking
+// stream after them, with the band schedule (the other schedules share per-MB progress
+// words with the intra stage and stay on the lane).
+int enqueue(h264g_ctx *c, const StageSet &q, int lane, const std::vector<SlotUse> &uses, Batch *sb) {
+    hipStream_t s = lane_stream(c, lane), ds = c->dstream[lane];
+    for (int k = 1; k <= kDbkHist; k++) {
+        const DbkRec &r = c->hist[lane][(c->hist_next[lane] - k + kDbkHist) % kDbkHist];
+        if (r.live && conflicts(r, uses)) {
+            HIPCHK(hipStreamWaitEvent(s, r.ev, 0));
+            break;
+        }
+    }
+    const bool split = c->dbk_schedule == H264G_DEBLOCK_BANDS && !q.dbk_off;
+    hipEvent_t *ev = profile_events(c);
+    const bool graphs = sb && !c->profile && !c->no_graphs;
+    // capture of the launches `parts` on stream st into *g (kernel nodes only; every launch
+    // argument, including the wavefront epochs, which live in device memory, is the same
+    // on each replay)
+    auto capture = [&](int parts, hipStream_t st, hipGraphExec_t *g) -> int {
+        HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        int rc = launch_stages(c, q, lane, parts, st, nullptr);
+        hipGraph_t gr = nullptr;
+        hipError_t e = hipStreamEndCapture(st, &gr);
+        if (rc) { if (gr) (void)hipGraphDestroy(gr); return rc; }
+        if (e != hipSuccess) return h264g_set_error(H264G_E_DEVICE, "stream capture: %s", hipGetErrorString(e));
+        e = hipGraphInstantiate(g, gr, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(gr);
+        if (e != hipSuccess) { *g = nullptr; return h264g_set_error(H264G_E_DEVICE, "graph instantiate: %s", hipGetErrorString(e)); }
+        return 0;
+    };
+    const int first = split ? kFront : (kFront | kBack);
+    if (graphs) {
+        if (!sb->graph) { int rc = capture(first, s, &sb->graph); if (rc) return rc; }
+        HIPCHK(hipGraphLaunch(sb->graph, s));
+    } else {
+        int rc = launch_stages(c, q, lane, first, s, ev);
+        if (rc) return rc;
+    }
+    if (!split) return 0;
+    HIPCHK(hipEventRecord(c->front[lane], s));
+    HIPCHK(hipStreamWaitEvent(ds, c->front[lane], 0));
+    if (graphs) {
+        if (!sb->graph_dbk) { int rc = capture(kBack, ds, &sb->graph_dbk); if (rc) return rc; }
+        HIPCHK(hipGraphLaunch(sb->graph_dbk, ds));
+    } else {
+        int rc = launch_stages(c, q, lane, kBack, ds, ev);
+        if (rc) return rc;
+    }
+    DbkRec &r = c->hist[lane][c->hist_next[lane]];
+    if (r.live) HIPCHK(hipStreamWaitEvent(s, r.ev, 0));  // leaves the history: order the lane after it
+    if (!r.ev) HIPCHK(hipEventCreateWithFlags(&r.ev, hipEventDisableTiming));
+    HIPCHK(hipEventRecord(r.ev, ds));
+    r.dst.clear();
+    for (const SlotUse &u : uses) r.dst.push_back({u.stream, u.dst});
+    r.live = true;
+    c->hist_next[lane] = (c->hist_next[lane] + 1) % kDbkHist;
     return 0;
 }
 
 // Caller holds ctx->mu.  A batch with a graph is destroyed only when idle.
 int free_batch(h264g_ctx *c, h264g_batch *b) {
     if (!b) return 0;
-    if (b->sub[0].graph || b->sub[1].graph) (void)sync_lanes(c);
+    if (b->sub[0].graph || b->sub[1].graph || b->sub[0].graph_dbk || b->sub[1].graph_dbk) (void)sync_lanes(c);
     for (Batch &sb : b->sub) {
         if (sb.graph) (void)hipGraphExecDestroy(sb.graph);
+        if (sb.graph_dbk) (void)hipGraphExecDestroy(sb.graph_dbk);
         pool_put(c, sb.dmem, sb.dbytes);
     }
     delete b;
@@ -633,8 +766,11 @@ extern "C" {
 
 const char *h264g_last_error(void) { return g_err; }
 
-// Progress words start at 0, so the first wavefront launch uses epoch 1.
-static const uint32_t kFirstEpoch = 1;
+// Progress words start at 0, so the first wavefront launch uses epoch 1.  Intra and
+// deblocking launches count their epochs separately (they may run at the same time on a
+// lane and its deblocking stream); the deblocking count starts half the range away, so
+// the two never meet in the per-MB progress words the per-MB schedules share.
+static const uint32_t kFirstEpoch = 1, kFirstDbkEpoch = 0x80000001u;
 
 int32_t h264g_create(const h264g_config *cfg, h264g_ctx **out) {
     if (!out) return h264g_set_error(H264G_E_ARG, "out is NULL");
@@ -670,10 +806,16 @@ int32_t h264g_create(const h264g_config *cfg, h264g_ctx **out) {
     c->split_mbs = !cfg || cfg->lane_split_mbs == 0 ? 16384 : (cfg->lane_split_mbs < 0 ? 0 : cfg->lane_split_mbs);
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->dstream[0], hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->dstream[1], hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&c->front[0], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->front[1], hipEventDisableTiming) != hipSuccess ||
         hipMalloc(&c->d_err, sizeof(uint32_t)) != hipSuccess || hipMalloc(&c->d_ctl, 8 * sizeof(DevCtl)) != hipSuccess ||
         hipMemset(c->d_err, 0, sizeof(uint32_t)) != hipSuccess || hipMemset(c->d_ctl, 0, 8 * sizeof(DevCtl)) != hipSuccess ||
         hipMemcpy(&c->d_ctl[2].ticket, &kFirstEpoch, sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(&c->d_ctl[6].ticket, &kFirstEpoch, sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess) {
+        hipMemcpy(&c->d_ctl[3].ticket, &kFirstDbkEpoch, sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(&c->d_ctl[6].ticket, &kFirstEpoch, sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(&c->d_ctl[7].ticket, &kFirstDbkEpoch, sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess) {
         h264g_destroy(c);
         return h264g_set_error(H264G_E_DEVICE, "context allocation failed");
     }
@@ -684,8 +826,8 @@ int32_t h264g_create(const h264g_config *cfg, h264g_ctx **out) {
 int32_t h264g_destroy(h264g_ctx *c) {
     if (!c) return H264G_OK;
     (void)hipSetDevice(c->device);
-    if (c->stream) (void)hipStreamSynchronize(c->stream);
-    if (c->aux) (void)hipStreamSynchronize(c->aux);
+    for (int i = 0; i < 4; i++)
+        if (any_stream(c, i)) (void)hipStreamSynchronize(any_stream(c, i));
     for (auto &b : c->pool) {
         for (auto ev : b.ready) (void)hipEventDestroy(ev);
         (void)hipFree(b.ptr);
@@ -700,8 +842,12 @@ int32_t h264g_destroy(h264g_ctx *c) {
     for (auto &e : c->ev) if (e) (void)hipEventDestroy(e);
     if (c->d_err) (void)hipFree(c->d_err);
     if (c->d_ctl) (void)hipFree(c->d_ctl);
-    if (c->stream) (void)hipStreamDestroy(c->stream);
-    if (c->aux) (void)hipStreamDestroy(c->aux);
+    for (int l = 0; l < 2; l++) {
+        for (auto &r : c->hist[l]) if (r.ev) (void)hipEventDestroy(r.ev);
+        if (c->front[l]) (void)hipEventDestroy(c->front[l]);
+    }
+    for (int i = 0; i < 4; i++)
+        if (any_stream(c, i)) (void)hipStreamDestroy(any_stream(c, i));
     delete c;
     return H264G_OK;
 }
@@ -823,7 +969,7 @@ int32_t h264g_picture_submit(h264g_ctx *c, h264g_pic *p) {
     q.n_ibands = (uint32_t)((p->H + c->intra_waves - 1) / c->intra_waves);
     q.n_dbands = (uint32_t)((p->H + c->dbk_waves - 1) / c->dbk_waves);
     q.dbk_off = p->desc.disable_deblocking_filter_idc == 1;
-    return launch_stages(c, q, c->streams[p->stream].lane);
+    return enqueue(c, q, c->streams[p->stream].lane, {slot_use(p->stream, p->desc)}, nullptr);
 }
 
 int32_t h264g_picture_release(h264g_ctx *c, h264g_pic *p) {
@@ -849,9 +995,11 @@ int build_sub(h264g_ctx *c, const std::vector<const h264g_pic *> &pics, Batch *b
     std::vector<DevPic> hp(n);
     std::vector<uint32_t> inter, intra, dbk;
     int maxt = 0;
+    b->uses.clear();
     for (uint32_t i = 0; i < n; i++) {
         const h264g_pic *p = pics[i];
         hp[i] = p->hpic;
+        b->uses.push_back(slot_use(p->stream, p->desc));
         for (uint32_t a : p->inter_list) inter.push_back((i << WI_PIC_SHIFT) | a);
         maxt = std::max(maxt, (p->W - 1) + 2 * (p->H - 1));
     }
@@ -988,26 +1136,8 @@ int32_t h264g_batch_run(h264g_ctx *c, h264g_batch *b) {
     for (int l = 0; l < 2; l++) {
         Batch &sb = b->sub[l];
         if (!sb.n_pics) continue;
-        if (c->profile || c->no_graphs) {  // direct launches (profiled runs record events on the lane)
-            int rc = launch_stages(c, batch_stages(&sb), l);
-            if (rc) return rc;
-            continue;
-        }
-        hipStream_t s = lane_stream(c, l);
-        if (!sb.graph) {
-            // Capture the stage launches once; every launch argument (including the
-            // wavefront epoch, which lives in device memory) is the same on each replay.
-            HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-            int rc = launch_stages(c, batch_stages(&sb), l);
-            hipGraph_t g = nullptr;
-            hipError_t e = hipStreamEndCapture(s, &g);
-            if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
-            if (e != hipSuccess) return h264g_set_error(H264G_E_DEVICE, "stream capture: %s", hipGetErrorString(e));
-            e = hipGraphInstantiate(&sb.graph, g, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(g);
-            if (e != hipSuccess) { sb.graph = nullptr; return h264g_set_error(H264G_E_DEVICE, "graph instantiate: %s", hipGetErrorString(e)); }
-        }
-        HIPCHK(hipGraphLaunch(sb.graph, s));
+        int rc = enqueue(c, batch_stages(&sb), l, sb.uses, &sb);
+        if (rc) return rc;
     }
     return H264G_OK;
 }
@@ -1027,7 +1157,7 @@ int32_t h264g_batch_submit(h264g_ctx *c, h264g_pic *const *pics, uint32_t n) {
     std::lock_guard<std::mutex> lk(c->mu);
     HIPCHK(hipSetDevice(c->device));
     for (int l = 0; l < 2 && !rc; l++)  // used once: launched directly, not captured
-        if (b->sub[l].n_pics) rc = launch_stages(c, batch_stages(&b->sub[l]), l);
+        if (b->sub[l].n_pics) rc = enqueue(c, batch_stages(&b->sub[l]), l, b->sub[l].uses, nullptr);
     free_batch(c, b);                   // its blocks are reused after these launches
     return rc;
 }
@@ -1046,9 +1176,10 @@ int32_t h264g_sync(h264g_ctx *c) {
     return H264G_OK;
 }
 
-// A fence is one event per lane, recorded after the work enqueued so far on it.
+// A fence is one event per stream (lanes and their deblocking streams), recorded after the
+// work enqueued so far on it.
 struct h264g_fence {
-    hipEvent_t ev[2];
+    hipEvent_t ev[4];
 };
 
 int32_t h264g_fence_record(h264g_ctx *c, h264g_fence **out) {
@@ -1056,11 +1187,11 @@ int32_t h264g_fence_record(h264g_ctx *c, h264g_fence **out) {
     *out = nullptr;
     std::lock_guard<std::mutex> lk(c->mu);
     HIPCHK(hipSetDevice(c->device));
-    h264g_fence *f = new (std::nothrow) h264g_fence{{nullptr, nullptr}};
+    h264g_fence *f = new (std::nothrow) h264g_fence{{nullptr, nullptr, nullptr, nullptr}};
     if (!f) return h264g_set_error(H264G_E_NOMEM, "fence");
-    for (int l = 0; l < 2; l++) {
+    for (int l = 0; l < 4; l++) {
         if (hipEventCreateWithFlags(&f->ev[l], hipEventDisableTiming) != hipSuccess ||
-            hipEventRecord(f->ev[l], lane_stream(c, l)) != hipSuccess) {
+            hipEventRecord(f->ev[l], any_stream(c, l)) != hipSuccess) {
             for (auto e : f->ev) if (e) (void)hipEventDestroy(e);
             delete f;
             return h264g_set_error(H264G_E_DEVICE, "fence record");
@@ -1110,11 +1241,12 @@ int32_t h264g_stage_times(h264g_ctx *c, float out[4]) {
     HIPCHK(hipSetDevice(c->device));
     HIPCHK(sync_lanes(c));
     out[0] = out[1] = out[2] = out[3] = 0.f;
-    const size_t runs = c->ev_used / 5;
+    const size_t runs = c->ev_used / 6;
     for (size_t r = 0; r < runs; r++)
-        for (int i = 0; i < 4; i++) {
+        for (int i = 0; i < 4; i++) {  // inter, intra, bS: consecutive events; deblocking: events 4, 5
             float ms = 0.f;
-            HIPCHK(hipEventElapsedTime(&ms, c->ev[5 * r + i], c->ev[5 * r + i + 1]));
+            const size_t a = 6 * r + (i < 3 ? i : 4);
+            HIPCHK(hipEventElapsedTime(&ms, c->ev[a], c->ev[a + 1]));
             out[i] += ms;
         }
     c->ev_used = 0;
@@ -1147,7 +1279,7 @@ int32_t h264g_slot_readback(h264g_ctx *c, uint32_t sid, int32_t slot, uint8_t *y
     Stream &s = c->streams[sid];
     if (slot < 0 || slot >= s.desc.dpb_slots) return h264g_set_error(H264G_E_ARG, "slot %d", slot);
     HIPCHK(hipSetDevice(c->device));
-    HIPCHK(hipStreamSynchronize(lane_stream(c, s.lane)));  // the lane that writes the stream
+    HIPCHK(sync_lane(c, s.lane));  // the lane (and its deblocking stream) that writes the stream
     const uint8_t *base = s.dpb + (uint64_t)slot * s.slot_bytes;
     size_t ny = (size_t)s.W * s.H * 256, nc = (size_t)s.W * s.H * 64;
     HIPCHK(hipMemcpy(y, base, ny, hipMemcpyDeviceToHost));
@@ -1169,6 +1301,7 @@ int32_t h264g_slot_to_bgr24(h264g_ctx *c, uint32_t sid, int32_t slot, uint8_t *b
     uint8_t *d = (uint8_t *)pool_get(c, bytes);
     if (!d) return h264g_set_error(H264G_E_NOMEM, "conversion buffer");
     hipStream_t ls = lane_stream(c, s.lane);  // after the stream's pictures
+    HIPCHK(hipStreamSynchronize(c->dstream[s.lane]));  // and their deblocking
     int e = h264g_launch_bgr24(s.dpb + (uint64_t)slot * s.slot_bytes, wl, hl, d, stride, flip != 0, ls);
     hipError_t he = e ? (hipError_t)e : hipStreamSynchronize(ls);
     if (he == hipSuccess) he = hipMemcpy(bgr, d, bytes, hipMemcpyDeviceToHost);
@@ -1184,7 +1317,7 @@ int32_t h264g_slot_upload(h264g_ctx *c, uint32_t sid, int32_t slot, const uint8_
     Stream &s = c->streams[sid];
     if (slot < 0 || slot >= s.desc.dpb_slots) return h264g_set_error(H264G_E_ARG, "slot %d", slot);
     HIPCHK(hipSetDevice(c->device));
-    HIPCHK(hipStreamSynchronize(lane_stream(c, s.lane)));
+    HIPCHK(sync_lane(c, s.lane));
     uint8_t *base = s.dpb + (uint64_t)slot * s.slot_bytes;
     size_t ny = (size_t)s.W * s.H * 256, nc = (size_t)s.W * s.H * 64;
     HIPCHK(hipMemcpy(base, y, ny, hipMemcpyHostToDevice));

@@ -533,7 +533,8 @@ int32_t h264g_gen_create(const h264g_gen_params *params, h264g_gen **out) {
     if (!params || !out) return H264G_E_ARG;
     const h264g_gen_params &p = *params;
     if (p.width_mbs <= 0 || p.height_mbs <= 0 || p.n_frames <= 0 || p.max_refs < 1 || p.max_refs > 16 ||
-        p.qp_lo < 0 || p.qp_hi > 51 || p.qp_lo > p.qp_hi || p.qp_hi - p.qp_lo > 25)
+        p.qp_lo < 0 || p.qp_hi > 51 || p.qp_lo > p.qp_hi || p.qp_hi - p.qp_lo > 25 || p.spare_slots < 0 ||
+        p.spare_slots > 16)
         return h264g_set_error(H264G_E_ARG, "gen: bad params");
     h264g_gen *g = new h264g_gen();
     g->p = p;
@@ -542,7 +543,7 @@ int32_t h264g_gen_create(const h264g_gen_params *params, h264g_gen **out) {
     // DPB bookkeeping: reference frames in decoding order with their slot, POC, frame_num.
     struct RefEntry { int slot, poc, frame_num; };
     std::vector<RefEntry> refs;
-    int n_slots = p.max_refs + 1;
+    int n_slots = p.max_refs + 1 + p.spare_slots, last_dst = n_slots - 1;
     std::vector<int> slot_busy(n_slots, 0);
     g->dpb_slots = n_slots;
     g->pics.resize(plan.size());
@@ -616,9 +617,13 @@ int32_t h264g_gen_create(const h264g_gen_params *params, h264g_gen **out) {
         }
         // ---- destination slot
         int dst = -1;
-        for (int s = 0; s < n_slots; s++) if (!slot_busy[s]) { dst = s; break; }
+        for (int k = 0; k < n_slots; k++) {  // lowest free slot, or (spare slots) the next free one
+            const int s = p.spare_slots > 0 ? (last_dst + 1 + k) % n_slots : k;
+            if (!slot_busy[s]) { dst = s; break; }
+        }
         if (dst < 0) { delete g; return h264g_set_error(H264G_E_STATE, "gen: no free DPB slot"); }
         d.dst_slot = dst;
+        last_dst = dst;
 
         // ---- macroblocks
         PictureGen pg(p, rng, d, pic);

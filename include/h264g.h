@@ -346,6 +346,10 @@ typedef struct h264g_gen_params {
     int32_t  chroma_qp_index_offset, second_chroma_qp_index_offset;
     int32_t  ipcm_permille;
     int32_t  p_pattern;           /* 1 = config-1 P mix (60/30/10), 0 = general mix    */
+    int32_t  spare_slots;         /* DPB slots beyond max_refs + 1; with any, pictures
+                                     take the free slot after the last one used instead
+                                     of the lowest, so a picture does not overwrite the
+                                     slot of the picture decoded just before it        */
 } h264g_gen_params;
 
 int32_t h264g_gen_default_params(int32_t config, h264g_gen_params *out);

@@ -191,3 +191,43 @@ def test_wide_pictures_match_oracle(any_engine, width_mbs):
             assert plane_hashes(*got) == want[i], f"width {width_mbs} picture {i}"
     finally:
         engine.close_stream(sid)
+
+
+def test_overlapped_frames_match_oracle(any_engine):
+    """Frame indices of several streams batched and run back to back with no wait between
+    them.  The streams have spare DPB slots, so a picture does not overwrite the slot of
+    the picture before it and a B picture's deblocking overlaps the next picture's stages
+    (the engine orders a picture after a deblocking launch only when it reads or writes a
+    slot that launch writes).  At the end every slot of every stream equals the oracle's
+    DPB."""
+    from oracle.refcpu import Dpb
+    engine = any_engine
+    sizes = [(22, 18, 11), (40, 23, 12), (30, 20, 13)]  # the 920-MB stream runs on the second lane
+    gens = [Generator(2, width_mbs=w, height_mbs=h, n_frames=9, seed=0x0E4A0000 + s, spare_slots=2)
+            for w, h, s in sizes]
+    sids = [engine.open_stream(g.width_mbs, g.height_mbs, g.dpb_slots) for g in gens]
+    try:
+        frames = [[g.picture(i) for i in range(g.num_pictures)] for g in gens]
+        pics = [[engine.begin(sid, *f) for f in fr] for sid, fr in zip(sids, frames)]
+        batches = [engine.batch_create([pics[k][i] for k in range(len(gens)) if i < len(pics[k])])
+                   for i in range(max(len(p) for p in pics))]
+        for b in batches:
+            engine.batch_run(b)
+        engine.sync()
+        for k, g in enumerate(gens):
+            dpb = Dpb(g.width_mbs, g.height_mbs, g.dpb_slots)
+            for desc, mbs, levels in frames[k]:
+                assert dpb.decode(desc, mbs, levels, 2) == 0
+            written = {desc.dst_slot for desc, _, _ in frames[k]}
+            for slot in sorted(written):
+                got = engine.readback(sids[k], slot)
+                for gp, wp, pl in zip(got, dpb.split(slot), ("Y", "Cb", "Cr")):
+                    assert np.array_equal(gp, wp), f"stream {k} slot {slot} plane {pl}"
+        for b in batches:
+            engine.batch_destroy(b)
+        for row in pics:
+            for p in row:
+                engine.release(p)
+    finally:
+        for sid in sids:
+            engine.close_stream(sid)
