@@ -163,7 +163,7 @@ def main():
     ap.add_argument("--no-graphs", action="store_true", help="launch batch stages directly instead of hipGraph replay")
     ap.add_argument("--deblock", choices=["bands", "mb_wavefront", "mb_rows"], default="bands",
                     help="deblocking schedule")
-    ap.add_argument("--deblock-waves", type=int, default=0, help="MB rows per band of the bands schedule (0 = 16)")
+    ap.add_argument("--deblock-waves", type=int, default=0, help="MB rows per band of the bands schedule (0 = 12)")
     ap.add_argument("--intra", choices=["bands", "mb_wavefront"], default="bands", help="intra schedule")
     ap.add_argument("--intra-waves", type=int, default=0, help="MB rows per band of the bands intra schedule (0 = 8)")
     ap.add_argument("--serve-seconds", type=float, default=3.0,
@@ -342,7 +342,8 @@ def main():
         t = time.perf_counter()
         with cf.ThreadPoolExecutor(16) as ex:
             sgens = list(ex.map(lambda sp: Generator(2, width_mbs=sp.width_mbs, height_mbs=sp.height_mbs,
-                                                     n_frames=sp.n_frames, seed=sp.seed, gop=32), specs))
+                                                     n_frames=sp.n_frames, seed=sp.seed, gop=32, spare_slots=2),
+                                specs))
         log(f"serving: generated {sum(g.num_pictures for g in sgens)} frames of {len(specs)} streams in "
             f"{time.perf_counter() - t:.1f} s")
         seng = Engine(device=0, dpb_budget_bytes=200 * 10**9, waves=args.waves, deblock_schedule=schedule,

@@ -62,7 +62,7 @@ def test_serving_mix_equals_alone(budget_streams):
 
     budget = 0
     if budget_streams:  # room for about two of the largest DPBs: later streams must wait
-        # slots, per-MB and per-row progress words, strips of 2 deblocking bands of 16 rows
+        # slots, per-MB and per-row progress words, strips of 2 deblocking bands of 12 rows
         # and 3 intra bands of 8 rows
         budget = 2 * (22 * 18 * 384 * 5 + 22 * 18 * 4 + 18 * 8 + 2 * 22 * 256 + 3 * 22 * 64) + 1024
     # streams of 396 and 600 MBs run on the context's second HIP stream
