@@ -640,6 +640,7 @@ struct WaveIntra {
     int16_t U[25];         // the same samples before filtering (-1: not available)
     int16_t X[2][16];      // neighbour lines of two 4x4 blocks (see pred4x4)
     uint8_t cbot[2][8];    // the MB's bottom chroma rows (for the intra strips)
+    uint32_t p4[10][WAVE]; // Intra_4x4 wavefront step t, lane: see intra4_prep
 };
 
 // Availability of the neighbouring MBs A, B, C, D for intra prediction as bits 0..3 of
@@ -733,17 +734,14 @@ constexpr I4Taps make_i4taps() {
 __constant__ I4Taps c_i4taps = make_i4taps();
 
 // What an Intra_4x4 MB's wavefront steps need that does not depend on samples, gathered
-// before the MB waits for its neighbours: per step t (two 16-bit fields per word) the
-// flat tile index of neighbour-line sample j (lanes j < 15 of each 16-lane group; -1: not
-// available, T4..T7 already substituted by T3) and the taps of pixel j of the lane's
-// block (c_i4taps).
-struct I4Prep {
-    uint32_t xs[5], tp[5];
-};
-__device__ __forceinline__ I4Prep intra4_prep(const h264g_mb *mb, const Avail &av) {
+// into wi->p4 before the MB waits for its neighbours: per step t and lane the flat tile
+// index of neighbour-line sample j (lanes j < 15 of each 16-lane group; -1: not
+// available, T4..T7 already substituted by T3) in the low half and the taps of pixel j of
+// the lane's block (c_i4taps) in the high half.  The LDS writes are ordered before the
+// MB's first wave barrier in intra_mb.
+__device__ __forceinline__ void intra4_prep(const h264g_mb *mb, const Avail &av, WaveIntra *wi) {
     const int lane = wave_lane(), g = lane >> 4, j = lane & 15;
-    I4Prep p;
-#pragma unroll
+#pragma unroll 1
     for (int t = 0; t < 10; t++) {
         const int by = (t < 2 ? 0 : (t - 2) >> 1) + g, bx = t - 2 * by;
         const bool has = lane < 32 && bx >= 0 && by < 4;
@@ -763,12 +761,8 @@ __device__ __forceinline__ I4Prep intra4_prep(const h264g_mb *mb, const Avail &a
             }
             tp = c_i4taps.v[mb->intra_pred_modes[blk]][j];
         }
-        const uint32_t sh = 16 * (t & 1);
-        if (!(t & 1)) { p.xs[t >> 1] = 0; p.tp[t >> 1] = 0; }
-        p.xs[t >> 1] |= ((uint32_t)a & 0xFFFFu) << sh;
-        p.tp[t >> 1] |= tp << sh;
+        wi->p4[t][lane] = ((uint32_t)a & 0xFFFFu) | tp << 16;
     }
-    return p;
 }
 
 // 8.3.2.2 for pixel (x, y) of the 8x8 block blk using the filtered samples in wi->pf:
@@ -859,9 +853,9 @@ __device__ __forceinline__ void intra_res_place(const IntraRes &r, WaveIntra *wi
 // the residual are ordered before its first wave barrier.
 // above: neighbours (Avail bits: 2 = B, 4 = C, 8 = D) whose samples of row -1 the caller
 // already put into the tile, from the intra strips.
-// p4: intra4_prep of the MB (used when it is Intra_4x4).
+// An Intra_4x4 MB has run intra4_prep.
 __device__ __forceinline__ void intra_mb(const DevPic &P, int addr, WaveIntra *wi, const Avail &av, bool res,
-                                         const I4Prep &p4, uint32_t above = 0) {
+                                         uint32_t above = 0) {
     const int lane = wave_lane();
     const h264g_mb *mb = &P.mbs[addr];
     const int W = P.W, H = P.H, wl = W * 16, hl = H * 16, wc = W * 8, hc = H * 8;
@@ -900,13 +894,14 @@ __device__ __forceinline__ void intra_mb(const DevPic &P, int addr, WaveIntra *w
         const int g = lane >> 4, j = lane & 15, x = j & 3, y = j >> 2;
         const uint8_t *Tf = &wi->T[0][0];
         const int16_t *X = wi->X[g];
-#pragma unroll
+#pragma unroll 1
         for (int t = 0; t < 10; t++) {
             const int by = (t < 2 ? 0 : (t - 2) >> 1) + g, bx = t - 2 * by;
             const bool has = lane < 32 && bx >= 0 && by < 4;
             const int xO = bx * 4, yO = by * 4;
-            const int xa = (int)(int16_t)(p4.xs[t >> 1] >> (16 * (t & 1)));
-            const uint32_t tp = (p4.tp[t >> 1] >> (16 * (t & 1))) & 0xFFFFu;
+            const uint32_t e = wi->p4[t][lane];
+            const int xa = (int)(int16_t)(e & 0xFFFFu);
+            const uint32_t tp = e >> 16;
             if (has && j < 15) wi->X[g][j] = (int16_t)(xa >= 0 ? (int)Tf[xa] : -1);
             wave_sync();
             if (has) {
@@ -1514,10 +1509,10 @@ __global__ __launch_bounds__(WAVE) void k_intra(const DevPic *__restrict__ pics,
         int deps[4];
         const Avail av = intra_avail(P, addr, deps);
         const IntraRes r = intra_res_fetch(P, addr);
-        const I4Prep p4 = intra4_prep(&P.mbs[addr], av);
+        if (P.mbs[addr].mb_type == H264G_MB_I4x4) intra4_prep(&P.mbs[addr], av, &lds);
         wait_deps(P.flags, deps, 4, epoch, err);
         intra_res_place(r, &lds);
-        intra_mb(P, addr, &lds, av, r.res, p4);
+        intra_mb(P, addr, &lds, av, r.res);
         publish(&P.flags[addr], epoch);
     }
     retire(ctl, gridDim.x, epoch_p, epoch);
@@ -1633,7 +1628,7 @@ __global__ __launch_bounds__(WAVE * INTRA_WG_MAX_WAVES) void k_intra_band(const 
                 int deps[4];
                 const Avail av = intra_avail(P, addr, deps);
                 const IntraRes r = intra_res_fetch(P, addr);  // in flight during the wait
-                const I4Prep p4 = intra4_prep(&P.mbs[addr], av);
+                if (P.mbs[addr].mb_type == H264G_MB_I4x4) intra4_prep(&P.mbs[addr], av, &tiles[w]);
                 // the row above up to C (x + 1) or up to B (x) when only B / D are read and intra
                 const int need = deps[2] >= 0 ? min(x + 2, W) : (deps[1] >= 0 || deps[3] >= 0 ? x + 1 : 0);
                 uint32_t above = 0;
@@ -1646,7 +1641,7 @@ __global__ __launch_bounds__(WAVE * INTRA_WG_MAX_WAVES) void k_intra_band(const 
                     }
                 }
                 intra_res_place(r, &tiles[w]);
-                intra_mb(P, addr, &tiles[w], av, r.res, p4, above);
+                intra_mb(P, addr, &tiles[w], av, r.res, above);
                 if (to_g) istrip_put(iout + (size_t)x * 8, &tiles[w], epoch);
                 const bool end = i + 1 == e;
                 post(end ? (uint32_t)W : uniform(col[i + 1]));
