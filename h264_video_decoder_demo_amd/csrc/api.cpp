@@ -689,8 +689,9 @@ bool conflicts(const DbkRec &r, const std::vector<SlotUse> &uses) {
 // newest deblocking launch in flight that writes a slot these pictures write or read
 // (deblocking launches on the lane's deblocking stream complete in order, so that one
 // covers the older ones); inter / intra / bS on the lane; deblocking on the deblocking
-// stream after them, with the band schedule (the other schedules share per-MB progress
-// words with the intra stage and stay on the lane).
+// stream after them, with the band schedule in a two-lane context (the other schedules
+// share per-MB progress words with the intra stage, and a one-lane context is the
+// strictly sequential shape: there deblocking stays on the lane).
 int enqueue(h264g_ctx *c, const StageSet &q, int lane, const std::vector<SlotUse> &uses, Batch *sb) {
     hipStream_t s = lane_stream(c, lane), ds = c->dstream[lane];
     for (int k = 1; k <= kDbkHist; k++) {
@@ -700,7 +701,8 @@ int enqueue(h264g_ctx *c, const StageSet &q, int lane, const std::vector<SlotUse
             break;
         }
     }
-    const bool split = c->dbk_schedule == H264G_DEBLOCK_BANDS && !q.dbk_off;
+    // one-lane contexts (lane_split_mbs < 0) keep the strictly sequential shape
+    const bool split = c->dbk_schedule == H264G_DEBLOCK_BANDS && !q.dbk_off && c->split_mbs > 0;
     hipEvent_t *ev = profile_events(c);
     const bool graphs = sb && !c->profile && !c->no_graphs;
     // capture of the launches `parts` on stream st into *g (kernel nodes only; every launch

@@ -149,9 +149,11 @@ typedef struct h264g_config {
     int32_t  intra_waves;         /* rows per band = waves per workgroup of
                                      H264G_INTRA_BANDS, 1..8 (0 = 8)                 */
     int32_t  lane_split_mbs;      /* streams of at least this many MBs per picture run
-                                     on a second HIP stream, so their long intra /
+                                     on a second lane, so their long intra /
                                      deblocking chains overlap the other pictures'
-                                     stages (0 = 16384; negative = one stream)       */
+                                     stages (0 = 16384); negative = one HIP stream for
+                                     everything, every picture after the previous one
+                                     (no deblocking overlap either)                   */
 } h264g_config;
 
 /* Intra schedules: BANDS = pictures cut into bands of `intra_waves` MB rows, one
