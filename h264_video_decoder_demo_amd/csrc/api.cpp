@@ -566,7 +566,11 @@ int stage_picture(h264g_ctx *ctx, Stream &st, const h264g_pic_desc &d, const h26
     size_t o_bs = al(o_it + n_items * sizeof(uint32_t)), o_dbq = al(o_bs + (size_t)n_mbs * 32);
     size_t o_ires = al(o_dbq + (size_t)n_mbs * 4), o_pic = al(o_ires + (size_t)pic->n_intra * 768);
     size_t total = al(o_pic + sizeof(DevPic));
-    void *dm = pool_get(ctx, total);
+    void *dm = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(ctx->mu);  // the rest of the staging runs unlocked
+        dm = pool_get(ctx, total);
+    }
     if (!dm) return h264g_set_error(H264G_E_NOMEM, "picture device memory (%zu bytes)", total);
     pic->dmem = dm;
     pic->dbytes = total;
@@ -869,9 +873,8 @@ int32_t h264g_stream_open(h264g_ctx *c, const h264g_stream_desc *d, uint32_t *id
     s.H = d->frame_height_in_mbs;
     s.slot_bytes = (uint64_t)s.W * s.H * 384;
     s.lane = c->split_mbs > 0 && s.W * s.H >= c->split_mbs ? 1 : 0;
-    // DPB slots, per-MB progress words, per-row progress words, and one row of
-    // deblocking strips (128 B per MB) per band of c->dbk_waves rows
-    // strips between row bands: deblocking (256 B per MB column and band), then intra (64 B)
+    // DPB slots, per-MB progress words, per-row progress words, and the strips between
+    // row bands: deblocking (256 B per MB column and band), then intra (64 B)
     const uint64_t dstrip_bytes = (uint64_t)((s.H + c->dbk_waves - 1) / c->dbk_waves) * s.W * 256;
     const uint64_t strip_bytes = dstrip_bytes + (uint64_t)((s.H + c->intra_waves - 1) / c->intra_waves) * s.W * 64;
     uint64_t need = s.slot_bytes * d->dpb_slots + (uint64_t)s.W * s.H * 4 + (uint64_t)s.H * 8 + strip_bytes;
@@ -879,27 +882,27 @@ int32_t h264g_stream_open(h264g_ctx *c, const h264g_stream_desc *d, uint32_t *id
         return h264g_set_error(H264G_E_BUDGET, "DPB budget: %llu + %llu > %llu", (unsigned long long)c->used,
                                (unsigned long long)need, (unsigned long long)c->budget);
     HIPCHK(hipSetDevice(c->device));
-    HIPCHK(hipMalloc(&s.dpb, s.slot_bytes * d->dpb_slots));
-    if (hipMalloc(&s.flags, (size_t)s.W * s.H * 4) != hipSuccess) {
-        (void)hipFree(s.dpb);
-        return h264g_set_error(H264G_E_NOMEM, "flags");
+    // Blocks come from the event-guarded pool (a closed stream's blocks return there), and
+    // the clears run on the stream's lane ahead of its first picture, so opening a stream
+    // neither allocates in steady state nor waits for the GPU.
+    const size_t dpb_bytes = s.slot_bytes * d->dpb_slots, flag_bytes = (size_t)s.W * s.H * 4, row_bytes = (size_t)s.H * 8;
+    s.dpb = (uint8_t *)pool_get(c, dpb_bytes);
+    s.flags = (uint32_t *)pool_get(c, flag_bytes);
+    s.rowprog = (uint64_t *)pool_get(c, row_bytes);
+    s.gstrip = (uint8_t *)pool_get(c, strip_bytes);
+    if (!s.dpb || !s.flags || !s.rowprog || !s.gstrip) {
+        pool_put(c, s.dpb, dpb_bytes);
+        pool_put(c, s.flags, flag_bytes);
+        pool_put(c, s.rowprog, row_bytes);
+        pool_put(c, s.gstrip, strip_bytes);
+        return h264g_set_error(H264G_E_NOMEM, "stream device memory");
     }
-    if (hipMalloc(&s.rowprog, (size_t)s.H * 8) != hipSuccess) {
-        (void)hipFree(s.dpb);
-        (void)hipFree(s.flags);
-        return h264g_set_error(H264G_E_NOMEM, "row progress");
-    }
-    if (hipMalloc(&s.gstrip, strip_bytes) != hipSuccess) {
-        (void)hipFree(s.dpb);
-        (void)hipFree(s.flags);
-        (void)hipFree(s.rowprog);
-        return h264g_set_error(H264G_E_NOMEM, "deblocking strips");
-    }
-    HIPCHK(hipMemset(s.rowprog, 0, (size_t)s.H * 8));
-    HIPCHK(hipMemset(s.gstrip, 0, strip_bytes));  // epoch 0 never matches a launch
+    hipStream_t ls = lane_stream(c, s.lane);
+    HIPCHK(hipMemsetAsync(s.rowprog, 0, row_bytes, ls));
+    HIPCHK(hipMemsetAsync(s.gstrip, 0, strip_bytes, ls));  // epoch 0 never matches a launch
+    HIPCHK(hipMemsetAsync(s.dpb, 0, dpb_bytes, ls));
+    HIPCHK(hipMemsetAsync(s.flags, 0, flag_bytes, ls));
     s.istrip = (uint64_t *)(s.gstrip + dstrip_bytes);
-    HIPCHK(hipMemset(s.dpb, 0, s.slot_bytes * d->dpb_slots));
-    HIPCHK(hipMemset(s.flags, 0, (size_t)s.W * s.H * 4));
     s.bytes = need;
     s.open = true;
     c->used += need;
@@ -916,12 +919,13 @@ int32_t h264g_stream_close(h264g_ctx *c, uint32_t id) {
     std::lock_guard<std::mutex> lk(c->mu);
     if (id >= c->streams.size() || !c->streams[id].open) return h264g_set_error(H264G_E_ARG, "stream %u", id);
     (void)hipSetDevice(c->device);
-    (void)sync_lanes(c);
     Stream &s = c->streams[id];
-    (void)hipFree(s.dpb);
-    (void)hipFree(s.flags);
-    (void)hipFree(s.rowprog);
-    (void)hipFree(s.gstrip);
+    // back to the pool, reusable once the work enqueued so far has completed (no wait)
+    const uint64_t dstrip_bytes = (uint64_t)((s.H + c->dbk_waves - 1) / c->dbk_waves) * s.W * 256;
+    pool_put(c, s.dpb, s.slot_bytes * s.desc.dpb_slots);
+    pool_put(c, s.flags, (size_t)s.W * s.H * 4);
+    pool_put(c, s.rowprog, (size_t)s.H * 8);
+    pool_put(c, s.gstrip, dstrip_bytes + (uint64_t)((s.H + c->intra_waves - 1) / c->intra_waves) * s.W * 64);
     c->used -= s.bytes;
     s = Stream();
     return H264G_OK;
@@ -931,18 +935,26 @@ int32_t h264g_picture_begin(h264g_ctx *c, uint32_t sid, const h264g_pic_desc *d,
                             const int16_t *levels, uint64_t n_levels, h264g_pic **out) {
     if (!c || !d || !mbs || !out || (!levels && n_levels)) return h264g_set_error(H264G_E_ARG, "NULL argument");
     *out = nullptr;
-    std::lock_guard<std::mutex> lk(c->mu);
-    if (sid >= c->streams.size() || !c->streams[sid].open) return h264g_set_error(H264G_E_ARG, "stream %u", sid);
+    // Only the stream lookup and the device block take the context lock: the host work
+    // of staging (validation, per-MB records, work lists) and the uploads of several
+    // pictures can run on several threads at once.
+    Stream st;
+    {
+        std::lock_guard<std::mutex> lk(c->mu);
+        if (sid >= c->streams.size() || !c->streams[sid].open) return h264g_set_error(H264G_E_ARG, "stream %u", sid);
+        st = c->streams[sid];
+    }
     HIPCHK(hipSetDevice(c->device));
     h264g_pic *p = new (std::nothrow) h264g_pic();
     if (!p) return h264g_set_error(H264G_E_NOMEM, "pic");
     p->ctx = c;
     p->stream = sid;
     p->desc = *d;
-    p->W = c->streams[sid].W;
-    p->H = c->streams[sid].H;
-    int rc = stage_picture(c, c->streams[sid], *d, mbs, n_mbs, levels, n_levels, p);
+    p->W = st.W;
+    p->H = st.H;
+    int rc = stage_picture(c, st, *d, mbs, n_mbs, levels, n_levels, p);
     if (rc) {
+        std::lock_guard<std::mutex> lk(c->mu);
         pool_put(c, p->dmem, p->dbytes);
         delete p;
         return rc;

@@ -15,8 +15,10 @@ No counterpart in the reference, which decodes one file synchronously.
 from __future__ import annotations
 
 import collections
+import concurrent.futures
 import dataclasses
 import math
+import os
 import time
 
 import numpy as np
@@ -118,6 +120,7 @@ class ServeLoop:
         total = sum(s.gen.num_pictures for s in self.streams)
         t0 = time.perf_counter()
         next_window = 0.0
+        stagers = concurrent.futures.ThreadPoolExecutor(min(16, os.cpu_count() or 1))
         while frames < total:
             now = time.perf_counter() - t0
             # arrivals and admission
@@ -158,16 +161,18 @@ class ServeLoop:
                         s.sid = None
             # a batch window
             if now >= next_window:
-                batch = []
+                todo = []
                 for s in self.streams:
                     if s.sid is None or s.next_submit >= s.next_arrival:
                         continue
                     if self.on_frame is not None and s.next_submit > s.done:
                         continue  # read-back mode: one frame per stream in flight
-                    k = s.next_submit
-                    pic = eng.begin_raw(s.sid, *s.gen.picture_raw(k))
-                    batch.append((s, k, float(s.arrivals[k]), pic))
+                    todo.append((s, s.next_submit))
                     s.next_submit += 1
+                # host staging of the window's pictures on several threads (the library
+                # serialises only the device block allocation)
+                pics = list(stagers.map(lambda t: eng.begin_raw(t[0].sid, *t[0].gen.picture_raw(t[1])), todo))
+                batch = [(s, k, float(s.arrivals[k]), pic) for (s, k), pic in zip(todo, pics)]
                 if batch:
                     eng.batch_submit([p for _, _, _, p in batch])
                     inflight.append((eng.fence(), batch))
@@ -176,4 +181,5 @@ class ServeLoop:
             else:
                 time.sleep(min(2e-4, max(0.0, next_window - now)))
         eng.sync()
+        stagers.shutdown()
         return ServeResult(frames, mpix, time.perf_counter() - t0, lat, admitted_late, max_batch)

@@ -239,7 +239,8 @@ int32_t h264g_stream_close(h264g_ctx *ctx, uint32_t stream_id);
 /* Stage one picture's syntax: copies the records and levels to device memory
  * (input arrays stay owned by the caller).  Replaces the per-MB dispatch at
  * H264SliceData.cpp:386-487 (the host calls this where the reference called the
- * reconstruction functions, once per picture). */
+ * reconstruction functions, once per picture).  Several threads may stage pictures
+ * at once: only the stream lookup and the device block allocation are serialised. */
 int32_t h264g_picture_begin(h264g_ctx *ctx, uint32_t stream_id, const h264g_pic_desc *desc,
                             const h264g_mb *mbs, uint32_t n_mbs,
                             const int16_t *levels, uint64_t n_levels, h264g_pic **out);
