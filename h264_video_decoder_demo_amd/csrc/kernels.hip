@@ -640,7 +640,13 @@ struct WaveIntra {
     int16_t U[25];         // the same samples before filtering (-1: not available)
     int16_t X[2][16];      // neighbour lines of two 4x4 blocks (see pred4x4)
     uint8_t cbot[2][8];    // the MB's bottom chroma rows (for the intra strips)
-    uint32_t p4[10][WAVE]; // Intra_4x4 wavefront step t, lane: see intra4_prep
+    union {
+        uint32_t p4[10][WAVE];  // Intra_4x4 wavefront step t, lane: see intra4_prep
+        struct {
+            int32_t ua[4][32];      // Intra_8x8 block, reference sample: see intra8_prep
+            uint32_t tp[4][WAVE];   // Intra_8x8 block, pixel: taps on the filtered samples
+        } p8;
+    };
 };
 
 // Availability of the neighbouring MBs A, B, C, D for intra prediction as bits 0..3 of
@@ -653,23 +659,11 @@ struct Avail {
     __device__ bool D() const { return m & 8u; }
 };
 
-// Sample p[x, y] relative to the 4x4 / 8x8 block at (xO, yO) of the current MB, from the
-// tile, or -1 when not available for intra prediction (6.4.12.1, H264PictureBase.cpp:1099-1155).
-__device__ __forceinline__ int tile_p(const WaveIntra *wi, const Avail &av, int xO, int yO, int x, int y, int blk4) {
-    int xx = xO + x, yy = yO + y;
-    if (yy < 0) {
-        if (xx < 0) return av.D() ? wi->T[0][0] : -1;
-        if (xx < 16) return av.B() ? wi->T[0][xx + 1] : -1;
-        return av.C() ? wi->T[0][xx + 1] : -1;
-    }
-    if (xx < 0) return av.A() ? wi->T[yy + 1][0] : -1;
-    if (xx >= 16) return -1;
-    if (blk4 >= 0 && x > 3 && (blk4 == 3 || blk4 == 11)) return -1;  // H264PictureBase.cpp:1131
-    return wi->T[yy + 1][xx + 1];
-}
-
-// Flat index into WaveIntra::T of the sample tile_p reads (T[0][0] = p[-1, -1] of the MB),
-// or -1 when it is not available.
+// Flat index into the tile WaveIntra::T (T[0][0] = p[-1, -1] of the MB, T[1 + y][1 + x] =
+// sample (x, y)) of sample p[x, y] relative to the 4x4 / 8x8 block at (xO, yO) of the
+// current MB, or -1 when it is not available for intra prediction (6.4.12.1,
+// H264PictureBase.cpp:1099-1155; blk4: the 4x4 block, whose top-right samples inside the
+// MB are not available for blocks 3 and 11, H264PictureBase.cpp:1131).
 __device__ __forceinline__ int tile_a(const Avail &av, int xO, int yO, int x, int y, int blk4) {
     const int xx = xO + x, yy = yO + y;
     if (yy < 0) {
@@ -765,58 +759,91 @@ __device__ __forceinline__ void intra4_prep(const h264g_mb *mb, const Avail &av,
     }
 }
 
-// 8.3.2.2 for pixel (x, y) of the 8x8 block blk using the filtered samples in wi->pf:
-// pf[0] = p'[-1,-1], pf[1 + y] = p'[-1, y], pf[9 + x] = p'[x, -1] (H264PictureBase.cpp:1601-1843).
-__device__ int pred8x8(const WaveIntra *wi, int mode, int x, int y) {
-    const int16_t *pf = wi->pf;
-#define FT(i) ((i) < 0 ? pf[0] : pf[9 + (i)])
-#define FL(i) ((i) < 0 ? pf[0] : pf[1 + (i)])
-    int ta = pf[9] >= 0, la = pf[1] >= 0;  // all-or-nothing after filtering
+// Intra_8x8 sample (x, y) of mode m (8.3.2.2.2 - 8.3.2.2.10, H264PictureBase.cpp:1601-1843)
+// as (pf[i0] + pf[i1] + pf[i2] + pf[i3] + 2) >> 2 on the filtered reference samples
+// pf[0] = p'[-1, -1], pf[1 + y] = p'[-1, y], pf[9 + x] = p'[x, -1], like i4_taps (copy =
+// one index four times, 2-tap = both twice, 3-tap = middle twice; a weight 3 repeats an
+// index three times); 5 bits each, i0 lowest; DC (mode 2) is 0xFFFFFFFF.
+constexpr int i8_ft(int i) { return i < 0 ? 0 : 9 + i; }
+constexpr int i8_fl(int i) { return i < 0 ? 0 : 1 + i; }
+constexpr uint32_t i8_pack(int a, int b, int c, int d) { return (uint32_t)(a | b << 5 | c << 10 | d << 15); }
+constexpr uint32_t i8_tap2(int a, int b) { return i8_pack(a, a, b, b); }
+constexpr uint32_t i8_tap3(int a, int c, int b) { return i8_pack(a, c, c, b); }
+constexpr uint32_t i8_taps(int mode, int x, int y) {
     switch (mode) {
-    case 0: return FT(x);
-    case 1: return FL(y);
-    case 2: {
-        int st = 0, sl = 0;
-        for (int k = 0; k < 8; k++) { st += FT(k); sl += FL(k); }
-        if (ta && la) return (st + sl + 8) >> 4;
-        if (la) return (sl + 4) >> 3;
-        if (ta) return (st + 4) >> 3;
-        return 128;
-    }
+    case 0: return i8_pack(i8_ft(x), i8_ft(x), i8_ft(x), i8_ft(x));
+    case 1: return i8_pack(i8_fl(y), i8_fl(y), i8_fl(y), i8_fl(y));
+    case 2: return 0xFFFFFFFFu;
     case 3:
-        if (x == 7 && y == 7) return (FT(14) + 3 * FT(15) + 2) >> 2;
-        return (FT(x + y) + 2 * FT(x + y + 1) + FT(x + y + 2) + 2) >> 2;
+        if (x == 7 && y == 7) return i8_pack(i8_ft(14), i8_ft(15), i8_ft(15), i8_ft(15));
+        return i8_tap3(i8_ft(x + y), i8_ft(x + y + 1), i8_ft(x + y + 2));
     case 4:
-        if (x > y) return (FT(x - y - 2) + 2 * FT(x - y - 1) + FT(x - y) + 2) >> 2;
-        if (x < y) return (FL(y - x - 2) + 2 * FL(y - x - 1) + FL(y - x) + 2) >> 2;
-        return (FT(0) + 2 * pf[0] + FL(0) + 2) >> 2;
+        if (x > y) return i8_tap3(i8_ft(x - y - 2), i8_ft(x - y - 1), i8_ft(x - y));
+        if (x < y) return i8_tap3(i8_fl(y - x - 2), i8_fl(y - x - 1), i8_fl(y - x));
+        return i8_tap3(i8_ft(0), 0, i8_fl(0));
     case 5: {
-        int z = 2 * x - y;
-        if (z >= 0 && !(z & 1)) return (FT(x - (y >> 1) - 1) + FT(x - (y >> 1)) + 1) >> 1;
-        if (z >= 0) return (FT(x - (y >> 1) - 2) + 2 * FT(x - (y >> 1) - 1) + FT(x - (y >> 1)) + 2) >> 2;
-        if (z == -1) return (FL(0) + 2 * pf[0] + FT(0) + 2) >> 2;
-        return (FL(y - 2 * x - 1) + 2 * FL(y - 2 * x - 2) + FL(y - 2 * x - 3) + 2) >> 2;
+        const int z = 2 * x - y;
+        if (z >= 0 && !(z & 1)) return i8_tap2(i8_ft(x - (y >> 1) - 1), i8_ft(x - (y >> 1)));
+        if (z >= 0) return i8_tap3(i8_ft(x - (y >> 1) - 2), i8_ft(x - (y >> 1) - 1), i8_ft(x - (y >> 1)));
+        if (z == -1) return i8_tap3(i8_fl(0), 0, i8_ft(0));
+        return i8_tap3(i8_fl(y - 2 * x - 1), i8_fl(y - 2 * x - 2), i8_fl(y - 2 * x - 3));
     }
     case 6: {
-        int z = 2 * y - x;
-        if (z >= 0 && !(z & 1)) return (FL(y - (x >> 1) - 1) + FL(y - (x >> 1)) + 1) >> 1;
-        if (z >= 0) return (FL(y - (x >> 1) - 2) + 2 * FL(y - (x >> 1) - 1) + FL(y - (x >> 1)) + 2) >> 2;
-        if (z == -1) return (FL(0) + 2 * pf[0] + FT(0) + 2) >> 2;
-        return (FT(x - 2 * y - 1) + 2 * FT(x - 2 * y - 2) + FT(x - 2 * y - 3) + 2) >> 2;
+        const int z = 2 * y - x;
+        if (z >= 0 && !(z & 1)) return i8_tap2(i8_fl(y - (x >> 1) - 1), i8_fl(y - (x >> 1)));
+        if (z >= 0) return i8_tap3(i8_fl(y - (x >> 1) - 2), i8_fl(y - (x >> 1) - 1), i8_fl(y - (x >> 1)));
+        if (z == -1) return i8_tap3(i8_fl(0), 0, i8_ft(0));
+        return i8_tap3(i8_ft(x - 2 * y - 1), i8_ft(x - 2 * y - 2), i8_ft(x - 2 * y - 3));
     }
     case 7:
-        if (!(y & 1)) return (FT(x + (y >> 1)) + FT(x + (y >> 1) + 1) + 1) >> 1;
-        return (FT(x + (y >> 1)) + 2 * FT(x + (y >> 1) + 1) + FT(x + (y >> 1) + 2) + 2) >> 2;
+        if (!(y & 1)) return i8_tap2(i8_ft(x + (y >> 1)), i8_ft(x + (y >> 1) + 1));
+        return i8_tap3(i8_ft(x + (y >> 1)), i8_ft(x + (y >> 1) + 1), i8_ft(x + (y >> 1) + 2));
     default: {
-        int z = x + 2 * y;
-        if (z < 13 && !(z & 1)) return (FL(y + (x >> 1)) + FL(y + (x >> 1) + 1) + 1) >> 1;
-        if (z < 13) return (FL(y + (x >> 1)) + 2 * FL(y + (x >> 1) + 1) + FL(y + (x >> 1) + 2) + 2) >> 2;
-        if (z == 13) return (FL(6) + 3 * FL(7) + 2) >> 2;
-        return FL(7);
+        const int z = x + 2 * y;
+        if (z < 13 && !(z & 1)) return i8_tap2(i8_fl(y + (x >> 1)), i8_fl(y + (x >> 1) + 1));
+        if (z < 13) return i8_tap3(i8_fl(y + (x >> 1)), i8_fl(y + (x >> 1) + 1), i8_fl(y + (x >> 1) + 2));
+        if (z == 13) return i8_pack(i8_fl(6), i8_fl(7), i8_fl(7), i8_fl(7));
+        return i8_pack(i8_fl(7), i8_fl(7), i8_fl(7), i8_fl(7));
     }
     }
-#undef FT
-#undef FL
+}
+struct I8Taps {
+    uint32_t v[9][64];
+};
+constexpr I8Taps make_i8taps() {
+    I8Taps t{};
+    for (int m = 0; m < 9; m++)
+        for (int k = 0; k < 64; k++) t.v[m][k] = i8_taps(m, k & 7, k >> 3);
+    return t;
+}
+__constant__ I8Taps c_i8taps = make_i8taps();
+
+// What an Intra_8x8 MB's four blocks need that does not depend on samples, gathered into
+// wi->p8 before the MB waits for its neighbours: per block the flat tile index of each of
+// the 25 reference samples before filtering (U[0] = p[-1, -1], U[1 + y] = p[-1, y],
+// U[9 + x] = p[x, -1] with p[8..15, -1] <- p[7, -1] when not available; -1: not
+// available) and each pixel's taps (c_i8taps).
+__device__ __forceinline__ void intra8_prep(const h264g_mb *mb, const Avail &av, WaveIntra *wi) {
+    const int lane = wave_lane();
+#pragma unroll 1
+    for (int blk = 0; blk < 4; blk++) {
+        const int xO = (blk & 1) * 8, yO = (blk >> 1) * 8;
+        if (lane < 25) {
+            int a;
+            if (lane == 0) a = tile_a(av, xO, yO, -1, -1, -1);
+            else if (lane < 9) a = tile_a(av, xO, yO, -1, lane - 1, -1);
+            else {
+                const int x = lane - 9;
+                a = tile_a(av, xO, yO, x, -1, -1);
+                if (x >= 8 && a < 0) {
+                    const int a7 = tile_a(av, xO, yO, 7, -1, -1);
+                    if (a7 >= 0) a = a7;
+                }
+            }
+            wi->p8.ua[blk][lane] = a;
+        }
+        wi->p8.tp[blk][lane] = c_i8taps.v[mb->intra_pred_modes[blk]][lane];
+    }
 }
 
 // Residuals of intra MBs.  They need no neighbour samples, so the inter stage, which is
@@ -853,7 +880,7 @@ __device__ __forceinline__ void intra_res_place(const IntraRes &r, WaveIntra *wi
 // the residual are ordered before its first wave barrier.
 // above: neighbours (Avail bits: 2 = B, 4 = C, 8 = D) whose samples of row -1 the caller
 // already put into the tile, from the intra strips.
-// An Intra_4x4 MB has run intra4_prep.
+// An Intra_4x4 MB has run intra4_prep, an Intra_8x8 MB intra8_prep.
 __device__ __forceinline__ void intra_mb(const DevPic &P, int addr, WaveIntra *wi, const Avail &av, bool res,
                                          uint32_t above = 0) {
     const int lane = wave_lane();
@@ -917,23 +944,15 @@ __device__ __forceinline__ void intra_mb(const DevPic &P, int addr, WaveIntra *w
             wave_sync();
         }
     } else if (mb->mb_type == H264G_MB_I8x8) {
+        // each block: gather its 25 reference samples at the indices intra8_prep found,
+        // filter them, then one four-tap sum per pixel
+        const uint8_t *Tf = &wi->T[0][0];
+#pragma unroll 1
         for (int blk = 0; blk < 4; blk++) {
             const int xO = (blk & 1) * 8, yO = (blk >> 1) * 8;
-            // the 25 reference samples before filtering: U[0] = p[-1, -1], U[1 + y] = p[-1, y],
-            // U[9 + x] = p[x, -1] with p[8..15, -1] <- p[7, -1] when not available
             if (lane < 25) {
-                int v;
-                if (lane == 0) v = tile_p(wi, av, xO, yO, -1, -1, -1);
-                else if (lane < 9) v = tile_p(wi, av, xO, yO, -1, lane - 1, -1);
-                else {
-                    const int x = lane - 9;
-                    v = tile_p(wi, av, xO, yO, x, -1, -1);
-                    if (x >= 8 && v < 0) {
-                        const int t7 = tile_p(wi, av, xO, yO, 7, -1, -1);
-                        if (t7 >= 0) v = t7;
-                    }
-                }
-                wi->U[lane] = (int16_t)v;
+                const int a = wi->p8.ua[blk][lane];
+                wi->U[lane] = (int16_t)(a >= 0 ? (int)Tf[a] : -1);
             }
             wave_sync();
             // 8.3.2.2.1 reference sample filtering (H264PictureBase.cpp:1535-1599); the left
@@ -966,8 +985,17 @@ __device__ __forceinline__ void intra_mb(const DevPic &P, int addr, WaveIntra *w
                 wi->pf[lane] = (int16_t)v;
             }
             wave_sync();
-            int x = lane & 7, y = lane >> 3;
-            int v = pred8x8(wi, mb->intra_pred_modes[blk], x, y);
+            const int x = lane & 7, y = lane >> 3;
+            const uint32_t tp = wi->p8.tp[blk][lane];
+            const int16_t *pf = wi->pf;
+            int v = (pf[tp & 31u] + pf[(tp >> 5) & 31u] + pf[(tp >> 10) & 31u] + pf[(tp >> 15) & 31u] + 2) >> 2;
+            if (tp == 0xFFFFFFFFu) {  // DC (8.3.2.2.4): the filtered samples are available all or none
+                const bool ta = pf[9] >= 0, la = pf[1] >= 0;
+                int st = 0, sl = 0;
+#pragma unroll
+                for (int k = 0; k < 8; k++) { st += pf[9 + k]; sl += pf[1 + k]; }
+                v = ta && la ? (st + sl + 8) >> 4 : la ? (sl + 4) >> 3 : ta ? (st + 4) >> 3 : 128;
+            }
             if (res) v = clip1(v + wi->res.rl[(yO + y) * 16 + xO + x]);
             wave_sync();
             wi->T[yO + y + 1][xO + x + 1] = (uint8_t)v;
@@ -1510,6 +1538,7 @@ __global__ __launch_bounds__(WAVE) void k_intra(const DevPic *__restrict__ pics,
         const Avail av = intra_avail(P, addr, deps);
         const IntraRes r = intra_res_fetch(P, addr);
         if (P.mbs[addr].mb_type == H264G_MB_I4x4) intra4_prep(&P.mbs[addr], av, &lds);
+        else if (P.mbs[addr].mb_type == H264G_MB_I8x8) intra8_prep(&P.mbs[addr], av, &lds);
         wait_deps(P.flags, deps, 4, epoch, err);
         intra_res_place(r, &lds);
         intra_mb(P, addr, &lds, av, r.res);
@@ -1629,6 +1658,7 @@ __global__ __launch_bounds__(WAVE * INTRA_WG_MAX_WAVES) void k_intra_band(const 
                 const Avail av = intra_avail(P, addr, deps);
                 const IntraRes r = intra_res_fetch(P, addr);  // in flight during the wait
                 if (P.mbs[addr].mb_type == H264G_MB_I4x4) intra4_prep(&P.mbs[addr], av, &tiles[w]);
+                else if (P.mbs[addr].mb_type == H264G_MB_I8x8) intra8_prep(&P.mbs[addr], av, &tiles[w]);
                 // the row above up to C (x + 1) or up to B (x) when only B / D are read and intra
                 const int need = deps[2] >= 0 ? min(x + 2, W) : (deps[1] >= 0 || deps[3] >= 0 ? x + 1 : 0);
                 uint32_t above = 0;
