@@ -156,7 +156,7 @@ struct h264g_ctx {
     int dbk_schedule = H264G_DEBLOCK_BANDS;
     int intra_schedule = H264G_INTRA_BANDS;
     int intra_waves = 8;           // rows per band (waves per workgroup) of the intra band schedule
-    int dbk_waves = 12;            // rows per band (waves per workgroup) of the deblocking band schedule
+    int dbk_waves = 12;            // MB rows per band of the deblocking band schedule (two per wave)
     std::vector<hipEvent_t> ev;    // 6 events per profiled run: inter | intra | bS | end (lane), deblock | end
     size_t ev_used = 0;            // events recorded since the last h264g_stage_times
     std::mutex mu;
@@ -804,7 +804,7 @@ int32_t h264g_create(const h264g_config *cfg, h264g_ctx **out) {
         h264g_destroy(c);
         return h264g_set_error(H264G_E_ARG, "intra_schedule %d / intra_waves %d", cfg->intra_schedule, cfg->intra_waves);
     }
-    if (c->dbk_schedule < 0 || c->dbk_schedule > 2 || c->dbk_waves < 1 || c->dbk_waves > 16) {
+    if (c->dbk_schedule < 0 || c->dbk_schedule > 2 || c->dbk_waves < 1 || c->dbk_waves > 24) {
         h264g_destroy(c);
         return h264g_set_error(H264G_E_ARG, "deblock_schedule %d / deblock_waves %d", cfg->deblock_schedule, cfg->deblock_waves);
     }

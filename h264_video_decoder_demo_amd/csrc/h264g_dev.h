@@ -91,8 +91,9 @@ int h264g_launch_deblock_rows(const DevPic *pics, const uint32_t *items, uint32_
 // One workgroup of `waves` waves per picture (pics[0..n_pics)); pictures of up to 1024 MB rows.
 int h264g_launch_intra_band(const DevPic *pics, const uint32_t *bands, uint32_t n, DevCtl *ctl, uint32_t *epoch,
                             int waves, uint32_t *err, void *stream);
+// One workgroup of ceil(rows / 2) waves per band of `rows` MB rows (two rows per wave).
 int h264g_launch_deblock_band(const DevPic *pics, const uint32_t *bands, uint32_t n, DevCtl *ctl, uint32_t *epoch,
-                              int waves, uint32_t *err, void *stream);
+                              int rows, uint32_t *err, void *stream);
 int h264g_launch_bs(const DevPic *pics, const uint32_t *items, uint32_t n, void *stream);
 int h264g_launch_bgr24(const uint8_t *yuv, int wl, int hl, uint8_t *bgr, uint32_t stride, int flip, void *stream);
 int h264g_launch_copy(const uint8_t *src, uint8_t *dst, uint64_t bytes, void *stream);

@@ -1196,15 +1196,17 @@ __device__ void dbk_load(const DbkPlanes &d, int mbx, int mby, int region, WaveD
 }
 
 __device__ __forceinline__ int absdiff(int a, int b) { return (int)__builtin_amdgcn_sad_u16((uint32_t)a, (uint32_t)b, 0u); }
-__device__ __forceinline__ bool any_lane(bool v) { return __ballot(v) != 0; }
+// True when the condition holds on any active lane (the compare's lane mask, no VALU).
+__device__ __forceinline__ bool any_lane(bool v) { return __builtin_amdgcn_ballot_w64(v) != 0; }
 
 // Edge J of a line held as 20 bytes in w[0..4] (little-endian; the edge lies between
 // words J and J + 1); chroma lines hold 12 bytes and use J = 0, 1 (luma edges 0 and 8
 // for the bS).  8.7.2.2 - 8.7.2.4 (Deblock.cpp:1314-1522): alpha = alpha'(indexA), beta =
 // beta'(indexB), tc0 = tC0'(bS, indexA) (used for bS 1..3), with indexA / indexB =
 // Clip3(0, 51, qPav + filterOffsetA / B).  Every lane takes the same path: the wave
-// skips a filter that no lane applies (wave-uniform branches) and selects per lane
-// otherwise, so no branch waits on a per-line condition.
+// skips a filter that no lane applies (wave-uniform branches); inside, every candidate
+// value is computed on all lanes and chosen per lane with selects, so no per-line
+// condition turns into a divergent branch and its exec-mask bookkeeping.
 template <int J>
 __device__ __forceinline__ void dbk_line_edge(uint32_t (&w)[5], bool act, bool luma, int bS, int alpha, int beta, int tc0) {
     const bool on = act && bS != 0;
@@ -1215,27 +1217,35 @@ __device__ __forceinline__ void dbk_line_edge(uint32_t (&w)[5], bool act, bool l
     const int d0 = absdiff(p0, q0);
     const bool filt = on & (d0 < alpha) & (absdiff(p1, p0) < beta) & (absdiff(q1, q0) < beta);
     if (!any_lane(filt)) return;
-    const bool apl = absdiff(p2, p0) < beta, aql = absdiff(q2, q0) < beta;
+    // ap / aq < beta (8-460, 8-461), only ever used for luma lines
+    const bool apl = luma & (absdiff(p2, p0) < beta), aql = luma & (absdiff(q2, q0) < beta);
     int np0 = p0, np1 = p1, np2 = p2, nq0 = q0, nq1 = q1, nq2 = q2;
     const bool fn = filt & (bS < 4), fs = filt & (bS >= 4);
     if (any_lane(fn)) {  // bS < 4 (8.7.2.3)
-        const int tc = luma ? tc0 + (int)apl + (int)aql : tc0 + 1;
+        const int tc = tc0 + (int)apl + (int)aql + (int)!luma;  // luma tC0 + [ap] + [aq], chroma tC0 + 1
         const int delta = clip3(-tc, tc, (((q0 - p0) << 2) + (p1 - q1) + 4) >> 3);
         const int avg = (p0 + q0 + 1) >> 1;
-        np0 = fn ? clip1(p0 + delta) : np0;
-        nq0 = fn ? clip1(q0 - delta) : nq0;
-        np1 = fn & luma & apl ? (p1 + clip3(-tc0, tc0, (p2 + avg - (p1 << 1)) >> 1)) & 255 : np1;
-        nq1 = fn & luma & aql ? (q1 + clip3(-tc0, tc0, (q2 + avg - (q1 << 1)) >> 1)) & 255 : nq1;
+        const int fp0 = clip1(p0 + delta), fq0 = clip1(q0 - delta);
+        const int fp1 = (p1 + clip3(-tc0, tc0, (p2 + avg - (p1 << 1)) >> 1)) & 255;
+        const int fq1 = (q1 + clip3(-tc0, tc0, (q2 + avg - (q1 << 1)) >> 1)) & 255;
+        np0 = fn ? fp0 : np0;
+        nq0 = fn ? fq0 : nq0;
+        np1 = fn & apl ? fp1 : np1;
+        nq1 = fn & aql ? fq1 : nq1;
     }
     if (any_lane(fs)) {  // bS == 4 (8.7.2.4)
         const bool strong = d0 < ((alpha >> 2) + 2);
-        const bool sp = luma & apl & strong, sq = luma & aql & strong;
-        np0 = fs ? (sp ? (p2 + 2 * p1 + 2 * p0 + 2 * q0 + q1 + 4) >> 3 : (2 * p1 + p0 + q1 + 2) >> 2) : np0;
-        np1 = fs & sp ? (p2 + p1 + p0 + q0 + 2) >> 2 : np1;
-        np2 = fs & sp ? (2 * p3 + 3 * p2 + p1 + p0 + q0 + 4) >> 3 : np2;
-        nq0 = fs ? (sq ? (p1 + 2 * p0 + 2 * q0 + 2 * q1 + q2 + 4) >> 3 : (2 * q1 + q0 + p1 + 2) >> 2) : nq0;
-        nq1 = fs & sq ? (p0 + q0 + q1 + q2 + 2) >> 2 : nq1;
-        nq2 = fs & sq ? (2 * q3 + 3 * q2 + q1 + q0 + p0 + 4) >> 3 : nq2;
+        const bool sp = fs & apl & strong, sq = fs & aql & strong;
+        const int sp0 = (p2 + 2 * p1 + 2 * p0 + 2 * q0 + q1 + 4) >> 3, wp0 = (2 * p1 + p0 + q1 + 2) >> 2;
+        const int sq0 = (p1 + 2 * p0 + 2 * q0 + 2 * q1 + q2 + 4) >> 3, wq0 = (2 * q1 + q0 + p1 + 2) >> 2;
+        const int sp1 = (p2 + p1 + p0 + q0 + 2) >> 2, sq1 = (p0 + q0 + q1 + q2 + 2) >> 2;
+        const int sp2 = (2 * p3 + 3 * p2 + p1 + p0 + q0 + 4) >> 3, sq2 = (2 * q3 + 3 * q2 + q1 + q0 + p0 + 4) >> 3;
+        np0 = sp ? sp0 : (fs ? wp0 : np0);
+        nq0 = sq ? sq0 : (fs ? wq0 : nq0);
+        np1 = sp ? sp1 : np1;
+        nq1 = sq ? sq1 : nq1;
+        np2 = sp ? sp2 : np2;
+        nq2 = sq ? sq2 : nq2;
     }
     w[J] = __builtin_amdgcn_perm(pack4(0, np2, np1, np0), a, 0x07060500u);
     w[J + 1] = __builtin_amdgcn_perm(b, pack4(nq0, nq1, nq2, 0), 0x07020100u);
@@ -1251,11 +1261,14 @@ __device__ __forceinline__ void dbk_line_edge(uint32_t (&w)[5], bool act, bool l
 // the MB and of the left / top MB.
 // cur: tile column of the MB's luma column 0 (16 or 0; chroma cur / 2); the left MB sits
 // in the other half of the tile, so a row's columns -4..-1 are at (16 - cur) + 12.
-template <int DIR>
+// PAIRED: lanes 32-63 do the same for a second MB on a tile of their own (the caller's
+// per-lane wd / bsrec); act: the lane's MB is filtered in this step.
+template <int DIR, bool PAIRED = false>
 __device__ __forceinline__ void dbk_lines(const DevPic &P, uint32_t eq, bool edge, uint32_t qq,
-                                          const uint8_t *bsrec, WaveDbk *wd, const DbkTables *tb, int cur) {
-    const int lane = wave_lane();
-    if (lane >= 32) return;
+                                          const uint8_t *bsrec, WaveDbk *wd, const DbkTables *tb, int cur,
+                                          bool act = true) {
+    const int lane = PAIRED ? (wave_lane() & 31) : wave_lane();
+    if (!PAIRED && lane >= 32) return;
     const int plane = lane < 16 ? 0 : (lane < 24 ? 1 : 2);
     const bool luma = plane == 0;
     const int k = luma ? lane : (lane & 7);
@@ -1286,7 +1299,7 @@ __device__ __forceinline__ void dbk_lines(const DevPic &P, uint32_t eq, bool edg
     const int qav = (qpe + qpq + 1) >> 1;  // internal edges: (qpq + qpq + 1) >> 1 = qpq
     const int iA0 = clip3(0, 51, qav + P.offA), iB0 = clip3(0, 51, qav + P.offB);
     const int iAi = clip3(0, 51, qpq + P.offA), iBi = clip3(0, 51, qpq + P.offB);
-    const bool t8 = dbq_t8(qq), on = P.dbk_idc != 1;
+    const bool t8 = dbq_t8(qq), on = act && P.dbk_idc != 1;
     const int seg = luma ? (k >> 2) : (k >> 1);  // 4-line luma segment of the line
     // bS of this segment's edges 0..3 (k_bs stores them [direction][segment][edge]); chroma
     // edge j lies on luma edge 2j.  All table lookups are issued before the filtering.
@@ -1468,7 +1481,7 @@ __device__ void deblock_row(const DevPic &P, int mby, Sync &sync, WaveDbk *wd, c
 }
 
 // ------------------------------------------------------------ kernels
-constexpr int DBK_WG_MAX_WAVES = 16;
+constexpr int DBK_WG_MAX_WAVES = 12;  // deblocking bands: up to 24 MB rows
 // The LDS working sets are per wave and the per-MB code synchronises with wave barriers,
 // so a workgroup may hold one wave (most kernels) or several (the band kernels).
 // Workgroups are dispatched to the 8 XCDs round-robin (workgroup b on XCD b mod 8), each
@@ -1548,15 +1561,16 @@ __global__ __launch_bounds__(WAVE) void k_intra(const DevPic *__restrict__ pics,
 }
 
 // ---- bands of MB rows, one workgroup per band
-// A band is `waves` consecutive MB rows of one picture, one row per wave.  Inside the
-// band the rows synchronise through LDS counters (workgroup scope: the waves share the
-// CU's caches).  A band's last row hands what the next band's first row reads (in
-// another workgroup, possibly on another XCD) through device memory as epoch-tagged
-// 64-bit words, one agent-scope atomic each (strip_put_g, istrip_put): the reader polls
-// the words themselves, and no agent-scope release / acquire (an L2 write-back or
-// invalidation of the whole XCD) is issued.  Work items are (picture, band) in band-major
-// order and a workgroup takes its item by ticket when it starts, so the band above always
-// belongs to a workgroup that started earlier and never waits on a later one.
+// A band is consecutive MB rows of one picture, one row per wave (intra) or two
+// (deblocking, see deblock_pair_band).  Inside the band the rows synchronise through LDS
+// counters (workgroup scope: the waves share the CU's caches).  A band's last row hands
+// what the next band's first row reads (in another workgroup, possibly on another XCD)
+// through device memory as epoch-tagged 64-bit words, one agent-scope atomic each
+// (strip_put_g, istrip_put): the reader polls the words themselves, and no agent-scope
+// release / acquire (an L2 write-back or invalidation of the whole XCD) is issued.  Work
+// items are (picture, band) in band-major order and a workgroup takes its item by ticket
+// when it starts, so the band above always belongs to a workgroup that started earlier
+// and never waits on a later one.
 
 // Takes the workgroup's ticket (into *slot) and clears `n` LDS counters.
 __device__ __forceinline__ uint32_t band_ticket(DevCtl *ctl, uint32_t *slot, uint32_t *ctr, int n) {
@@ -1723,30 +1737,37 @@ __global__ __launch_bounds__(WAVE) void k_deblock_rows(const DevPic *__restrict_
     retire(ctl, gridDim.x, epoch_p, epoch);
 }
 
-// ---- deblocking over bands with LDS strips
-// Row y's wave hands the bottom 4 luma / chroma rows of each MB (the p samples of the
-// next row's top edge) to row y + 1's wave as a strip: through a ring of STRIP_SLOTS LDS
-// slots inside a band, through a full-width strip array in device memory (P.gstrip, one
-// row of W strips per band boundary) from a band's last row to the next band.  No sample
-// otherwise travels between waves: each sample is written to the picture by exactly one
-// wave, row y storing rows 0..12 (chroma 0..4) of its MBs and row y + 1 the rows 13..15
-// (chroma 5..7) after its top-edge filtering (the picture's last row stores all).  An MB
-// is stored, and its strip written, one step after its own filtering, when its right
-// neighbour's left edge has finished its columns 13..15; the row's last MB at its own
-// step.  Row y writes strip x into ring slot x mod STRIP_SLOTS only after row y + 1 has
-// read strip x - STRIP_SLOTS; every row of a band has its own wave, so the reader is
+// ---- deblocking over bands, two MB rows per wave
+// A band of R MB rows runs on ceil(R / 2) waves; wave w takes rows y0 + 2w (the upper
+// half: lanes 0-31) and y0 + 2w + 1 (the lower half: lanes 32-63).  The halves run the
+// per-MB step on 32 lanes each (luma lines on half-lanes 0-15, Cb 16-23, Cr 24-31), each
+// on a tile of its own, the lower half one MB behind the upper: one instruction stream
+// filters two MBs, and every lane of the wave has a line to filter.
+//
+// Row y hands the bottom 4 luma / chroma rows of each MB (the p samples of the next
+// row's top edge) to row y + 1 as a strip: inside a wave from the upper tile to the
+// lower tile, from a wave's lower row to the next wave's upper row through a ring of
+// STRIP_SLOTS LDS slots, and from a band's last row to the next band through a
+// full-width strip array in device memory (P.gstrip, one row of W strips per band
+// boundary).  No sample otherwise travels between rows: each sample is written to the
+// picture by exactly one half, row y storing rows 0..12 (chroma 0..4) of its MBs and row
+// y + 1 the rows 13..15 (chroma 5..7) after its top-edge filtering (the picture's last
+// row stores all).  An MB is stored, and its strip handed down, one step after its own
+// filtering, when its right neighbour's left edge has finished its columns 13..15 (the
+// row's last MB one step later as well, with nothing to filter in that step).  A lower
+// row writes strip x into ring slot x mod STRIP_SLOTS only after the next wave's upper
+// row has read strip x - STRIP_SLOTS; every wave of a band is resident, so the reader is
 // always running.
 constexpr int STRIP_SLOTS = 8;
 struct alignas(16) StripSlot {
-    uint8_t y[4][16];         // luma rows 12..15
-    uint8_t c[2][4][8];       // Cb / Cr rows 4..7
+    uint32_t w[32];           // strip dword of half-lane j (see strip_word)
 };
 struct DbkWgShared {
-    WaveDbk tiles[DBK_WG_MAX_WAVES];
-    StripSlot ring[DBK_WG_MAX_WAVES][STRIP_SLOTS];   // ring of local row r, read by row r + 1
-    alignas(16) uint8_t bsrec[DBK_WG_MAX_WAVES][32];
-    uint32_t prog[DBK_WG_MAX_WAVES];   // local row r: strips of MBs 0 .. prog - 1 complete
-    uint32_t cons[DBK_WG_MAX_WAVES];   // local row r: strips of MBs 0 .. cons - 1 read by row r + 1
+    WaveDbk tiles[2 * DBK_WG_MAX_WAVES];               // tile of row y0 + r
+    StripSlot ring[DBK_WG_MAX_WAVES][STRIP_SLOTS];    // wave w's lower row -> wave w + 1's upper row
+    alignas(16) uint8_t bsrec[2 * DBK_WG_MAX_WAVES][32];
+    uint32_t prog[DBK_WG_MAX_WAVES];   // wave w's lower row: strips of MBs 0 .. prog - 1 complete
+    uint32_t cons[DBK_WG_MAX_WAVES];   // wave w's lower row: strips of MBs 0 .. cons - 1 read by wave w + 1
     DbkTables tb;
 };
 
@@ -1771,85 +1792,100 @@ __device__ __forceinline__ void lds_post(uint32_t *w, uint32_t v) {
     __hip_atomic_store(w, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
-// Body of MB (mbx, mby) and its boundary strengths, one dword per lane (prefetch registers).
+__device__ __forceinline__ int half_lane() { return threadIdx.x & 31; }
+
+// Body of MB (mbx, mby) and its boundary strengths for one half (prefetch registers):
+// half-lane j holds luma row j / 2, bytes 8 (j & 1) .. + 7, the chroma dword j of the
+// two 8x8 planes and, for j < 8, boundary-strength dword j.
 struct DbkPrefetch {
-    uint32_t luma, chroma, bs;
+    uint2 luma;
+    uint32_t chroma, bs;
 };
-__device__ __forceinline__ DbkPrefetch dbk_fetch(const DevPic &P, const DbkPlanes &d, int mbx, int mby) {
-    const int lane = wave_lane();
-    DbkPrefetch f;
-    f.luma = *(const uint32_t *)&d.y[(uint64_t)(mby * 16 + (lane >> 2)) * d.wl + mbx * 16 + (lane & 3) * 4];
-    f.chroma = 0;
-    f.bs = 0;
-    if (lane < 32) {
-        const uint8_t *pl = (lane >> 4) ? d.cr : d.cb;
-        f.chroma = *(const uint32_t *)&pl[(uint64_t)(mby * 8 + ((lane >> 1) & 7)) * d.wc + mbx * 8 + (lane & 1) * 4];
+__device__ __forceinline__ DbkPrefetch dbk_fetch(const DevPic &P, const DbkPlanes &d, int mbx, int mby, bool on) {
+    const int hl = half_lane();
+    DbkPrefetch f{make_uint2(0u, 0u), 0u, 0u};
+    if (on) {
+        f.luma = *(const uint2 *)&d.y[(uint64_t)(mby * 16 + (hl >> 1)) * d.wl + mbx * 16 + (hl & 1) * 8];
+        const uint8_t *pl = (hl >> 4) ? d.cr : d.cb;
+        f.chroma = *(const uint32_t *)&pl[(uint64_t)(mby * 8 + ((hl >> 1) & 7)) * d.wc + mbx * 8 + (hl & 1) * 4];
+        if (hl < 8) f.bs = *(const uint32_t *)&P.bs[(size_t)(mby * P.W + mbx) * 32 + hl * 4];
     }
-    if (lane < 8) f.bs = *(const uint32_t *)&P.bs[(size_t)(mby * P.W + mbx) * 32 + lane * 4];
     return f;
 }
 __device__ __forceinline__ void dbk_place(const DbkPrefetch &f, WaveDbk *wd, uint8_t *bsrec, int cur) {
-    const int lane = wave_lane();
-    *(uint32_t *)&wd->L[(lane >> 2) + 4][cur + (lane & 3) * 4] = f.luma;
-    if (lane < 32) *(uint32_t *)&wd->C[lane >> 4][((lane >> 1) & 7) + 4][(cur >> 1) + (lane & 1) * 4] = f.chroma;
-    if (lane < 8) *(uint32_t *)&bsrec[lane * 4] = f.bs;
+    const int hl = half_lane();
+    *(uint2 *)&wd->L[(hl >> 1) + 4][cur + (hl & 1) * 8] = f.luma;
+    *(uint32_t *)&wd->C[hl >> 4][((hl >> 1) & 7) + 4][(cur >> 1) + (hl & 1) * 4] = f.chroma;
+    if (hl < 8) *(uint32_t *)&bsrec[hl * 4] = f.bs;
 }
 
-// Rows 0 .. ry (chroma 0 .. cy) of the MB whose column 0 sits at tile column `col` (0: the
-// left MB, DT_LX: the current one) to the picture: one aligned 16-byte luma row per lane
-// 0..15, one 8-byte chroma row per lane 32..47.
+// Rows 0 .. ry (chroma 0 .. cy) of the MB whose column 0 sits at tile column `col` to the
+// picture: one aligned 16-byte luma row per half-lane 0..15, one 8-byte chroma row per
+// half-lane 16..31.
 __device__ __forceinline__ void dbk_store_mb(const DbkPlanes &d, int mbx, int mby, int col, int ry, int cy,
                                              const WaveDbk *wd) {
-    const int lane = wave_lane();
-    if (lane < 16) {
-        if (lane <= ry)
-            *(uint4 *)&d.y[(uint64_t)(mby * 16 + lane) * d.wl + mbx * 16] = *(const uint4 *)&wd->L[lane + 4][col];
-    } else if (lane >= 32 && lane < 48) {
-        const int k = lane - 32, c = k >> 3, r = k & 7;
+    const int hl = half_lane();
+    if (hl < 16) {
+        if (hl <= ry)
+            *(uint4 *)&d.y[(uint64_t)(mby * 16 + hl) * d.wl + mbx * 16] = *(const uint4 *)&wd->L[hl + 4][col];
+    } else {
+        const int k = hl - 16, c = k >> 3, r = k & 7;
         if (r <= cy)
             *(uint2 *)&(c ? d.cr : d.cb)[(uint64_t)(mby * 8 + r) * d.wc + mbx * 8] = *(const uint2 *)&wd->C[c][r + 4][col >> 1];
     }
 }
 // Rows 13..15 (chroma 5..7) of the MB above the current one (tile column cur), final after
-// the current MB's top edge: lanes 0..2 luma, 32..37 chroma.
+// the current MB's top edge: half-lanes 0..2 luma, 3..8 chroma.
 __device__ __forceinline__ void dbk_store_top(const DbkPlanes &d, int mbx, int mby, const WaveDbk *wd, int cur) {
-    const int lane = wave_lane();
-    if (lane < 3) {
-        *(uint4 *)&d.y[(uint64_t)(mby * 16 - 3 + lane) * d.wl + mbx * 16] = *(const uint4 *)&wd->L[lane + 1][cur];
-    } else if (lane >= 32 && lane < 38) {
-        const int k = lane - 32, c = k >= 3, r = k - 3 * c;
+    const int hl = half_lane();
+    if (hl < 3) {
+        *(uint4 *)&d.y[(uint64_t)(mby * 16 - 3 + hl) * d.wl + mbx * 16] = *(const uint4 *)&wd->L[hl + 1][cur];
+    } else if (hl < 9) {
+        const int k = hl - 3, c = k >= 3, r = k - 3 * c;
         *(uint2 *)&(c ? d.cr : d.cb)[(uint64_t)(mby * 8 - 3 + r) * d.wc + mbx * 8] = *(const uint2 *)&wd->C[c][r + 1][cur >> 1];
     }
 }
 
+// A strip is one dword per half-lane: luma row 12 + j / 4, bytes 4 (j & 3) .. + 3 for
+// j < 16; for j = 16 + k the chroma plane k / 8, row 4 + (k / 2) % 4, bytes 4 (k & 1) .. + 3.
+// Strip dword of the MB at tile column `col` (rows 12..15 / chroma 4..7).
+__device__ __forceinline__ uint32_t strip_word(const WaveDbk *wd, int col) {
+    const int hl = half_lane();
+    if (hl < 16) return *(const uint32_t *)&wd->L[16 + (hl >> 2)][col + (hl & 3) * 4];
+    const int k = hl - 16, c = k >> 3, r = (k >> 1) & 3, h = k & 1;
+    return *(const uint32_t *)&wd->C[c][8 + r][(col >> 1) + h * 4];
+}
+// A strip dword into the tile rows -4..-1 above the MB at tile column cur.
+__device__ __forceinline__ void strip_place(WaveDbk *wd, int cur, uint32_t v) {
+    const int hl = half_lane();
+    if (hl < 16) {
+        *(uint32_t *)&wd->L[hl >> 2][cur + (hl & 3) * 4] = v;
+    } else {
+        const int k = hl - 16, c = k >> 3, r = (k >> 1) & 3, h = k & 1;
+        *(uint32_t *)&wd->C[c][r][(cur >> 1) + h * 4] = v;
+    }
+}
+
 // Strips that cross from a band to the next (another workgroup, possibly another XCD) go
-// through device memory as 32 words of 64 bits: the strip's dword of lane j (the layout of
-// strip_put) in the low half, the launch epoch in the high half.  Each word is stored and
-// loaded as one agent-scope atomic, so a reader that finds the epoch in all 32 words holds
-// the whole strip; neither side needs a fence, whose L2 write-back / invalidation would
-// also hit every other kernel running on those XCDs.
+// through device memory as 32 words of 64 bits: the strip dword of half-lane j in the low
+// half of word j, the launch epoch in the high half.  Each word is stored and loaded as
+// one agent-scope atomic, so a reader that finds the epoch in all 32 words holds the
+// whole strip; neither side needs a fence, whose L2 write-back / invalidation would also
+// hit every other kernel running on those XCDs.
 struct StripWords {
     uint64_t w[32];
 };
 __device__ __forceinline__ void strip_put_g(StripWords *sw, const WaveDbk *wd, int col, uint32_t epoch) {
-    const int lane = wave_lane();
-    uint32_t v = 0;
-    if (lane < 16) {
-        v = *(const uint32_t *)&wd->L[16 + (lane >> 2)][col + (lane & 3) * 4];
-    } else if (lane < 32) {
-        const int k = lane - 16, c = k >> 3, r = (k >> 1) & 3, h = k & 1;
-        v = *(const uint32_t *)&wd->C[c][8 + r][(col >> 1) + h * 4];
-    }
-    if (lane < 32) __hip_atomic_store(&sw->w[lane], (uint64_t)epoch << 32 | v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint32_t v = strip_word(wd, col);
+    __hip_atomic_store(&sw->w[half_lane()], (uint64_t)epoch << 32 | v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-// Waits for the strip of this launch and puts it into the tile rows -4..-1 above the
-// current MB (tile column cur).  Bounded like the other waits.
-__device__ void strip_get_g(const StripWords *sw, WaveDbk *wd, int cur, uint32_t epoch, uint32_t *err) {
-    const int lane = wave_lane();
+// Waits (the whole wave) for the strip of this launch and puts it into the tile of the
+// lanes where `mine` holds, above the MB at tile column cur.  Bounded like the other waits.
+__device__ void strip_get_g(const StripWords *sw, WaveDbk *wd, int cur, uint32_t epoch, bool mine, uint32_t *err) {
     uint64_t v = (uint64_t)epoch << 32;
     for (uint32_t it = 0;; ++it) {
-        if (lane < 32) v = __hip_atomic_load(&sw->w[lane], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (__ballot((uint32_t)(v >> 32) != epoch) == 0) break;
+        if (mine) v = __hip_atomic_load(&sw->w[half_lane()], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (!any_lane((uint32_t)(v >> 32) != epoch)) break;
         if (it >= SPIN_LIMIT) {
             atomicOr(err, DEVERR_TIMEOUT);
             break;
@@ -1857,110 +1893,95 @@ __device__ void strip_get_g(const StripWords *sw, WaveDbk *wd, int cur, uint32_t
         if ((it & 255u) == 255u && uniform(__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) != 0u) break;
         __builtin_amdgcn_s_sleep(1);
     }
-    if (lane < 16) {
-        *(uint32_t *)&wd->L[lane >> 2][cur + (lane & 3) * 4] = (uint32_t)v;
-    } else if (lane < 32) {
-        const int k = lane - 16, c = k >> 3, r = (k >> 1) & 3, h = k & 1;
-        *(uint32_t *)&wd->C[c][r][(cur >> 1) + h * 4] = (uint32_t)v;
-    }
+    if (mine) strip_place(wd, cur, (uint32_t)v);
 }
 
-// The strip (rows 12..15, chroma 4..7) of the MB at tile column `col`: one dword per lane.
-__device__ __forceinline__ void strip_put(StripSlot *sl, const WaveDbk *wd, int col) {
-    const int lane = wave_lane();
-    if (lane < 16) {
-        *(uint32_t *)&sl->y[lane >> 2][(lane & 3) * 4] = *(const uint32_t *)&wd->L[16 + (lane >> 2)][col + (lane & 3) * 4];
-    } else if (lane < 32) {
-        const int k = lane - 16, c = k >> 3, r = (k >> 1) & 3, h = k & 1;
-        *(uint32_t *)&sl->c[c][r][h * 4] = *(const uint32_t *)&wd->C[c][8 + r][(col >> 1) + h * 4];
-    }
-}
-// The MB above's strip into the tile rows -4..-1 above the current MB (tile column cur).
-__device__ __forceinline__ void strip_get(const StripSlot *sl, WaveDbk *wd, int cur) {
-    const int lane = wave_lane();
-    if (lane < 16) {
-        *(uint32_t *)&wd->L[lane >> 2][cur + (lane & 3) * 4] = *(const uint32_t *)&sl->y[lane >> 2][(lane & 3) * 4];
-    } else if (lane < 32) {
-        const int k = lane - 16, c = k >> 3, r = (k >> 1) & 3, h = k & 1;
-        *(uint32_t *)&wd->C[c][r][(cur >> 1) + h * 4] = *(const uint32_t *)&sl->c[c][r][h * 4];
-    }
-}
-// MB row y of the band starting at row y0 (band index b, rows y0 .. y1 - 1) on wave w.
-__device__ void deblock_row_band(const DevPic &P, int y, int y0, int y1, int b, int w, DbkWgShared &sh, uint32_t epoch,
-                                 uint32_t *err) {
-    const int W = P.W, H = P.H, ly = y - y0;
+// Wave w of the band of rows y0 .. y1 - 1 (band index b): rows y0 + 2w and, when it lies
+// in the band, y0 + 2w + 1.  Step s filters the upper row's MB s and the lower row's MB
+// s - 1, then stores (and hands down) the MBs one to the left of those.
+__device__ void deblock_pair_band(const DevPic &P, int w, int y0, int y1, int b, DbkWgShared &sh, uint32_t epoch,
+                                  uint32_t *err) {
+    const int W = P.W, H = P.H;
+    const int yu = y0 + 2 * w, yl = yu + 1;
+    const bool hasL = yl < y1, lower = wave_lane() >= 32;
+    const int y = lower ? yl : yu;
     const DbkPlanes d = dbk_planes(P);
-    const bool top = y > 0, last = y == H - 1;
-    const bool from_g = top && y == y0, to_g = !last && y == y1 - 1;
-    WaveDbk *wd = &sh.tiles[w];
-    uint8_t *bsrec = sh.bsrec[w];
-    StripSlot *ring = sh.ring[ly];
-    const StripSlot *above = ly > 0 ? sh.ring[ly - 1] : nullptr;
-    StripWords *gout = (StripWords *)P.gstrip + (size_t)b * W;              // this band -> next
-    const StripWords *gin = (const StripWords *)P.gstrip + (size_t)(b - 1) * W;  // previous band -> this
-    const int ry = last ? 15 : 12, cy = last ? 7 : 4;  // rows this row stores of its own MBs
-    // strip s of this row to the row below (slot s mod STRIP_SLOTS), then progress `done`
-    auto hand_down = [&](int s, int col, uint32_t done) {
-        if (to_g) {
-            strip_put_g(&gout[s], wd, col, epoch);
-        } else {
-            if (s - STRIP_SLOTS >= 0) lds_wait(&sh.cons[ly], (uint32_t)(s - STRIP_SLOTS + 1), err);  // slot read?
-            strip_put(&ring[s % STRIP_SLOTS], wd, col);
-            lds_post(&sh.prog[ly], done);
-        }
-    };
-    DbkPrefetch f = dbk_fetch(P, d, 0, y);
-    // deblocking words of the MB and the MB above, one MB ahead like the samples
-    uint32_t qq = 0u, qn = P.dbq[y * W], tn = top ? P.dbq[(y - 1) * W] : 0u;
-    for (int x = 0; x < W; x++) {
-        const int addr = y * W + x;
-        const bool left = x > 0, rowend = x == W - 1;
+    WaveDbk *const tu = &sh.tiles[2 * w], *const tl = &sh.tiles[2 * w + 1];
+    WaveDbk *const wd = lower ? tl : tu;
+    uint8_t *const bsrec = sh.bsrec[2 * w + (lower ? 1 : 0)];
+    const bool last = y == H - 1, topU = yu > 0, top = lower || topU;
+    const int ry = last ? 15 : 12, cy = last ? 7 : 4;  // rows a half stores of its own MBs
+    // the band's last row hands its strips to the next band (none after the picture's last)
+    const bool gU = !hasL && yu == y1 - 1 && yu != H - 1, gL = hasL && yl == y1 - 1 && yl != H - 1;
+    const bool ringL = hasL && yl + 1 < y1;  // the lower row hands down to wave w + 1
+    const bool from_g = topU && w == 0;      // the upper row reads the band above's strips
+    StripSlot *const ring = sh.ring[w];
+    const StripSlot *const above = w > 0 ? sh.ring[w - 1] : nullptr;
+    StripWords *const gout = (StripWords *)P.gstrip + (size_t)b * W;              // this band -> next
+    const StripWords *const gin = (const StripWords *)P.gstrip + (size_t)(b - 1) * W;  // previous band -> this
+    const int steps = W + (hasL ? 1 : 0);  // the lower row's last MB is stored in step W + 1
+    DbkPrefetch f = dbk_fetch(P, d, 0, yu, !lower);
+    // deblocking words of each half's MB and of the MB above it, one step ahead like the samples
+    uint32_t qq = 0u, qn = lower ? 0u : P.dbq[yu * W], tn = (!lower && topU) ? P.dbq[(yu - 1) * W] : 0u;
+    for (int s = 0; s <= steps; s++) {
+        const int x = lower ? s - 1 : s;  // this half's MB
+        const bool act = lower ? (hasL && s >= 1 && s <= W) : s < W;
         const int cur = 16 * (x & 1), prev = 16 - cur;  // tile columns of MB x and MB x - 1
+        const int prevU = 16 - 16 * (s & 1);              // the upper row's MB s - 1 (= the lower's MB column)
         const uint32_t lq = qq, tq = tn;
         qq = qn;
-        dbk_place(f, wd, bsrec, cur);
-        if (!rowend) {
-            f = dbk_fetch(P, d, x + 1, y);  // next MB, in flight during this one
-            qn = P.dbq[addr + 1];
-            if (top) tn = P.dbq[addr - W + 1];
+        if (act) dbk_place(f, wd, bsrec, cur);
+        {  // next step's MBs, in flight during this one
+            const bool nU = s + 1 < W, nL = hasL && s < W;
+            f = dbk_fetch(P, d, lower ? s : s + 1, y, lower ? nL : nU);
+            const uint32_t qu = nU ? P.dbq[yu * W + s + 1] : 0u, ql = nL ? P.dbq[yl * W + s] : 0u;
+            const uint32_t tu_ = nU && topU ? P.dbq[(yu - 1) * W + s + 1] : 0u, tl_ = nL ? P.dbq[yu * W + s] : 0u;
+            qn = lower ? ql : qu;
+            tn = lower ? tl_ : tu_;
         }
         wave_sync();
-        dbk_lines<0>(P, lq, left, qq, bsrec, wd, &sh.tb, cur);  // vertical edges: need nothing from above
+        dbk_lines<0, true>(P, lq, x > 0, qq, bsrec, wd, &sh.tb, cur, act);  // vertical edges
         wave_sync();
         // MB x - 1 is final now (its columns 13..15 took MB x's left edge) except rows 13..15,
         // which the next row's top edge filters: store it and hand its strip down before
         // waiting for the row above, so the row below runs about one step behind
-        if (left) {
-            dbk_store_mb(d, x - 1, y, prev, ry, cy, wd);
-            if (!last) hand_down(x - 1, prev, (uint32_t)x);
-        }
-        if (top) {
-            if (from_g) {
-                strip_get_g(&gin[x], wd, cur, epoch, err);
-            } else {
-                lds_wait(&sh.prog[ly - 1], (uint32_t)x + 1, err);
-                strip_get(&above[x % STRIP_SLOTS], wd, cur);
-                lds_post(&sh.cons[ly - 1], (uint32_t)x + 1);
+        const bool pU = s >= 1 && s <= W, pL = hasL && s >= 2;
+        if (lower ? pL : pU) dbk_store_mb(d, x - 1, y, prev, ry, cy, wd);
+        if (hasL && pU && lower) strip_place(tl, prevU, strip_word(tu, prevU));  // upper -> lower
+        if (gU && pU && !lower) strip_put_g(&gout[s - 1], tu, prevU, epoch);
+        if (gL && pL && lower) strip_put_g(&gout[s - 2], tl, prev, epoch);
+        if (ringL && pL) {
+            const int m = s - 2;
+            if (m - STRIP_SLOTS >= 0) lds_wait(&sh.cons[w], (uint32_t)(m - STRIP_SLOTS + 1), err);  // slot read?
+            if (lower) {
+                StripSlot *sl = &ring[m % STRIP_SLOTS];
+                sl->w[half_lane()] = strip_word(tl, prev);
             }
-            wave_sync();
+            lds_post(&sh.prog[w], (uint32_t)m + 1);
         }
-        dbk_lines<1>(P, tq, top, qq, bsrec, wd, &sh.tb, cur);  // horizontal edges
+        if (topU && s < W) {  // the upper row's strip from the row above
+            if (from_g) {
+                strip_get_g(&gin[s], tu, 16 * (s & 1), epoch, !lower, err);
+            } else {
+                lds_wait(&sh.prog[w - 1], (uint32_t)s + 1, err);
+                if (!lower) strip_place(tu, 16 * (s & 1), above[s % STRIP_SLOTS].w[half_lane()]);
+                lds_post(&sh.cons[w - 1], (uint32_t)s + 1);
+            }
+        }
         wave_sync();
-        if (top) dbk_store_top(d, x, y, wd, cur);
-        if (rowend) {
-            dbk_store_mb(d, x, y, cur, ry, cy, wd);
-            if (!last) hand_down(x, cur, (uint32_t)W);
-        }
-        // MB x stays in its half of the tile as the left MB of MB x + 1, whose samples go
-        // to the other half: the next step's first tile writes follow this barrier
+        dbk_lines<1, true>(P, tq, top, qq, bsrec, wd, &sh.tb, cur, act);  // horizontal edges
+        wave_sync();
+        if (act && top) dbk_store_top(d, x, y, wd, cur);
+        // the next step's tile writes (the other half of the tile) follow this barrier
         wave_sync();
     }
 }
 
-// One workgroup per band of `waves` MB rows (row y0 + w on wave w).
+// One workgroup per band of `rows` MB rows (rows y0 + 2w, y0 + 2w + 1 on wave w).
 __global__ __launch_bounds__(WAVE * DBK_WG_MAX_WAVES) void k_deblock_band(const DevPic *__restrict__ pics,
                                                                           const uint32_t *__restrict__ items, uint32_t n,
-                                                                          DevCtl *ctl, uint32_t *epoch_p, uint32_t *err) {
+                                                                          int rows, DevCtl *ctl, uint32_t *epoch_p,
+                                                                          uint32_t *err) {
     __shared__ DbkWgShared sh;
     __shared__ uint32_t s_ticket;
     const uint32_t epoch = launch_epoch(epoch_p);
@@ -1971,8 +1992,8 @@ __global__ __launch_bounds__(WAVE * DBK_WG_MAX_WAVES) void k_deblock_band(const 
     if (t < n) {
         const uint32_t it = items[t];
         const DevPic &P = pics[it >> WI_PIC_SHIFT];
-        const int b = (int)(it & WI_MB_MASK), y0 = b * nw, y = y0 + w;
-        if (P.dbk_idc != 1 && y < P.H) deblock_row_band(P, y, y0, min(P.H, y0 + nw), b, w, sh, epoch, err);
+        const int b = (int)(it & WI_MB_MASK), y0 = b * rows, y1 = min(P.H, y0 + rows);
+        if (P.dbk_idc != 1 && y0 + 2 * w < y1) deblock_pair_band(P, w, y0, y1, b, sh, epoch, err);
     }
     band_retire(ctl, epoch_p, epoch);
 }
@@ -2082,10 +2103,11 @@ int h264g_launch_intra_band(const DevPic *pics, const uint32_t *bands, uint32_t 
 }
 
 int h264g_launch_deblock_band(const DevPic *pics, const uint32_t *bands, uint32_t n, DevCtl *ctl, uint32_t *epoch,
-                              int waves, uint32_t *err, void *stream) {
+                              int rows, uint32_t *err, void *stream) {
     if (n == 0) return 0;
-    if (waves < 1 || waves > DBK_WG_MAX_WAVES) return (int)hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_deblock_band, dim3(n), dim3(WAVE * waves), 0, (hipStream_t)stream, pics, bands, n, ctl, epoch, err);
+    if (rows < 1 || rows > 2 * DBK_WG_MAX_WAVES) return (int)hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_deblock_band, dim3(n), dim3(WAVE * ((rows + 1) / 2)), 0, (hipStream_t)stream, pics, bands, n,
+                       rows, ctl, epoch, err);
     return (int)hipGetLastError();
 }
 

@@ -143,8 +143,8 @@ typedef struct h264g_config {
                                      batch's stage sequence as a captured hipGraph
                                      (profiled contexts always launch directly)       */
     int32_t  deblock_schedule;    /* H264G_DEBLOCK_*                                  */
-    int32_t  deblock_waves;       /* rows per band = waves per workgroup of
-                                     H264G_DEBLOCK_BANDS, 1..16 (0 = 12)             */
+    int32_t  deblock_waves;       /* MB rows per band of H264G_DEBLOCK_BANDS, 1..24
+                                     (0 = 12); two rows per wave                     */
     int32_t  intra_schedule;      /* H264G_INTRA_*                                    */
     int32_t  intra_waves;         /* rows per band = waves per workgroup of
                                      H264G_INTRA_BANDS, 1..8 (0 = 8)                 */
@@ -166,8 +166,9 @@ typedef struct h264g_config {
 
 /* Deblocking schedules (all give the raster-order result of 8.7):
  * BANDS        pictures cut into bands of `deblock_waves` MB rows, one workgroup per
- *              band and one wave per row; each MB's bottom rows go to the next row's
- *              wave through LDS inside a band and through device memory between bands
+ *              band and two rows per wave (the second one MB behind the first, on
+ *              the wave's other 32 lanes); each MB's bottom rows go to the next row
+ *              through LDS inside a band and through device memory between bands
  *              (default)
  * MB_WAVEFRONT persistent waves over single MBs ordered by t = x + 2y, progress words
  *              in device memory
