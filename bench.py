@@ -163,7 +163,7 @@ def main():
     ap.add_argument("--no-graphs", action="store_true", help="launch batch stages directly instead of hipGraph replay")
     ap.add_argument("--deblock", choices=["bands", "mb_wavefront", "mb_rows"], default="bands",
                     help="deblocking schedule")
-    ap.add_argument("--deblock-waves", type=int, default=0, help="MB rows per band of the bands schedule (0 = 12)")
+    ap.add_argument("--deblock-waves", type=int, default=0, help="MB rows per band of the bands schedule (0 = 8)")
     ap.add_argument("--intra", choices=["bands", "mb_wavefront"], default="bands", help="intra schedule")
     ap.add_argument("--intra-waves", type=int, default=0, help="MB rows per band of the bands intra schedule (0 = 8)")
     ap.add_argument("--serve-seconds", type=float, default=3.0,

@@ -156,7 +156,7 @@ struct h264g_ctx {
     int dbk_schedule = H264G_DEBLOCK_BANDS;
     int intra_schedule = H264G_INTRA_BANDS;
     int intra_waves = 8;           // rows per band (waves per workgroup) of the intra band schedule
-    int dbk_waves = 12;            // MB rows per band of the deblocking band schedule (two per wave)
+    int dbk_waves = 8;             // MB rows per band of the deblocking band schedule (two per wave)
     std::vector<hipEvent_t> ev;    // 6 events per profiled run: inter | intra | bS | end (lane), deblock | end
     size_t ev_used = 0;            // events recorded since the last h264g_stage_times
     std::mutex mu;
@@ -797,7 +797,7 @@ int32_t h264g_create(const h264g_config *cfg, h264g_ctx **out) {
     c->profile = cfg && cfg->profile;
     c->no_graphs = cfg && cfg->no_graphs;
     c->dbk_schedule = cfg ? cfg->deblock_schedule : H264G_DEBLOCK_BANDS;
-    c->dbk_waves = (cfg && cfg->deblock_waves > 0) ? cfg->deblock_waves : 12;
+    c->dbk_waves = (cfg && cfg->deblock_waves > 0) ? cfg->deblock_waves : 8;
     c->intra_schedule = cfg ? cfg->intra_schedule : H264G_INTRA_BANDS;
     c->intra_waves = (cfg && cfg->intra_waves > 0) ? cfg->intra_waves : 8;
     if (c->intra_schedule < 0 || c->intra_schedule > 1 || c->intra_waves > 8) {

@@ -127,7 +127,7 @@ class Engine:
                  intra_schedule: int = 0, intra_waves: int = 0, lane_split_mbs: int = 0):
         """deblock_schedule: _lib.DEBLOCK_BANDS (default), DEBLOCK_MB_WAVEFRONT or DEBLOCK_MB_ROWS;
         intra_schedule: _lib.INTRA_BANDS (default) or INTRA_MB_WAVEFRONT.  deblock_waves /
-        intra_waves: MB rows per band of the band schedules (0 = 12 / 8).  lane_split_mbs:
+        intra_waves: MB rows per band of the band schedules (0 = 8 / 8).  lane_split_mbs:
         streams with pictures of at least this many MBs run on the context's second HIP
         stream (0 = 16384, negative = one stream)."""
         cfg = Config(device=device, dpb_budget_bytes=dpb_budget_bytes, max_batch_pictures=max_batch_pictures,

@@ -144,7 +144,7 @@ typedef struct h264g_config {
                                      (profiled contexts always launch directly)       */
     int32_t  deblock_schedule;    /* H264G_DEBLOCK_*                                  */
     int32_t  deblock_waves;       /* MB rows per band of H264G_DEBLOCK_BANDS, 1..24
-                                     (0 = 12); two rows per wave                     */
+                                     (0 = 8); two rows per wave                      */
     int32_t  intra_schedule;      /* H264G_INTRA_*                                    */
     int32_t  intra_waves;         /* rows per band = waves per workgroup of
                                      H264G_INTRA_BANDS, 1..8 (0 = 8)                 */

@@ -62,9 +62,9 @@ def test_serving_mix_equals_alone(budget_streams):
 
     budget = 0
     if budget_streams:  # room for about two of the largest DPBs: later streams must wait
-        # slots, per-MB and per-row progress words, strips of 2 deblocking bands of 12 rows
-        # and 3 intra bands of 8 rows
-        budget = 2 * (22 * 18 * 384 * 5 + 22 * 18 * 4 + 18 * 8 + 2 * 22 * 256 + 3 * 22 * 64) + 1024
+        # slots, per-MB and per-row progress words, strips of 3 deblocking bands and 3 intra
+        # bands of 8 rows
+        budget = 2 * (22 * 18 * 384 * 5 + 22 * 18 * 4 + 18 * 8 + 3 * 22 * 256 + 3 * 22 * 64) + 1024
     # streams of 396 and 600 MBs run on the context's second HIP stream
     with Engine(device=0, dpb_budget_bytes=budget, lane_split_mbs=300) as eng:
         res = ServeLoop(eng, specs, window_ms=1.0, on_frame=on_frame).run()
