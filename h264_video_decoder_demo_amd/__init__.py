@@ -6,5 +6,5 @@ deblocking run as hand-written HIP kernels behind the C ABI of include/h264g.h;
 parsing stays on the host.  ``decoder.H264VideoDecoder`` mirrors the reference's
 decode/output-callback interface (H264VideoDecoder.h).
 """
-from .engine import Engine, Generator, StagedPicture, decode_stream_plain  # noqa: F401
+from .engine import Engine, Generator, MotionField, StagedPicture, decode_stream_plain  # noqa: F401
 from .decoder import H264VideoDecoder, H264Picture, H264_DECODE_ERROR_CODE_FILE_END  # noqa: F401

@@ -139,6 +139,10 @@ _SIGNATURES = {
                                     POINTER(c_void_p), POINTER(c_uint64)]),
     "h264g_gen_dpb_slots": (c_int32, [c_void_p]),
     "h264g_gen_digest": (c_uint64, [c_void_p]),
+    "h264g_mvfield_create": (c_int32, [c_int32, c_int32, POINTER(c_void_p)]),
+    "h264g_mvfield_destroy": (None, [c_void_p]),
+    "h264g_mvfield_reset": (c_int32, [c_void_p]),
+    "h264g_mvfield_derive": (c_int32, [c_void_p, c_uint32, c_void_p, c_void_p]),
 }
 
 _lib = None

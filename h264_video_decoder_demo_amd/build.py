@@ -1,7 +1,8 @@
 """In-tree build of libh264g.so for gfx950 (hipcc) -- run by __graft_entry__.build().
 
-The library holds the HIP kernels (csrc/kernels.hip), the C ABI (csrc/api.cpp) and
-the host-side syntax generator (csrc/gen.cpp).  It is built in the package folder so
+The library holds the HIP kernels (csrc/kernels.hip), the C ABI (csrc/api.cpp), the
+host motion-vector derivation (csrc/mvpred.cpp) and the host-side syntax generator
+(csrc/gen.cpp).  It is built in the package folder so
 that it travels with the source tree.
 """
 from __future__ import annotations
@@ -15,7 +16,7 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 OUT = os.path.join(PKG, "libh264g.so")
 
-SOURCES = ["api.cpp", "gen.cpp", "kernels.hip"]
+SOURCES = ["api.cpp", "gen.cpp", "mvpred.cpp", "kernels.hip"]
 
 HIPCC_FLAGS = [
     "-O3", "-std=c++17", "--offload-arch=gfx950", "-fPIC", "-shared",

@@ -1,0 +1,232 @@
+// mvpred.cpp -- host motion-vector derivation for frame pictures (8.4.1).
+//
+// A serial host parser reads mvd_l0 / mvd_l1 per (sub-)partition; the final motion
+// vector is mvp + mvd, with mvp predicted from the already decoded neighbouring
+// partitions (8.4.1.3), and P_Skip MBs carry no mvd at all (8.4.1.1).  The reference
+// derives both inside its reconstruction (P_Skip: H264InterPrediction.cpp:685-738;
+// prediction: Derivation_process_for_luma_motion_vector_prediction and its
+// neighbouring-partition process, H264InterPrediction.cpp:1660-2001), which this
+// library's device path does not have: the h264g_mb record carries final vectors.
+// h264g_mvfield restates that derivation on the host so that a parser can turn its
+// mvds into records.  Frame pictures only (no MBAFF field scaling); B_Skip / B_Direct
+// (8.4.1.2) are not derived, as the record cannot carry them.
+//
+// The field keeps, per 4x4 luma block of the picture, the reference index and vector
+// of each list and whether the block's partition has been decoded.  A neighbouring
+// partition (6.4.11.7) in another MB is available when its block has been decoded: MBs
+// are derived in decoding (raster) order and a picture is one slice, so an MB outside
+// the picture, the MB to the right and the MBs below are never decoded.
+//
+// Inside the current MB the reference departs from 6.4.11.7: its test that marks a
+// not yet decoded partition as not available is commented out
+// (H264InterPrediction.cpp:3164-3169), so such a partition counts as available, and its
+// motion is read from the MB's arrays as parsing left them: PredFlagLX 0, hence refIdx
+// -1 and mv 0 (H264InterPrediction.cpp:1862-1873, 593-603).  Only neighbour C can fall
+// there (in P_8x8 / B_8x8, a sub-partition of quadrant 0 or 2 whose C lies in
+// quadrant 1 or 3), and it is then not replaced by D.  The field reproduces that.
+#include "h264g.h"
+#include "h264g_internal.h"
+
+#include <algorithm>
+#include <cstring>
+#include <new>
+#include <vector>
+
+namespace {
+
+struct Blk {
+    int8_t ref[2];     // -1: the list is not used (or the MB is intra)
+    uint8_t decoded;
+    int16_t mv[2][2];
+};
+
+struct Nb {  // one neighbouring partition for one list (8.4.1.3.2)
+    bool avail;
+    int ref;
+    int mv[2];
+};
+
+inline int median3(int a, int b, int c) { return a + b + c - std::min(a, std::min(b, c)) - std::max(a, std::max(b, c)); }
+
+}  // namespace
+
+struct h264g_mvfield {
+    int W = 0, H = 0;  // in MBs
+    std::vector<Blk> b;  // (4W) x (4H) blocks, raster
+
+    Blk *at(int bx, int by) { return &b[(size_t)by * W * 4 + bx]; }
+
+    // Partition covering 4x4 block (mx*4 + x4, my*4 + y4), for list l.
+    Nb nb(int mx, int my, int x4, int y4, int l) {
+        Nb n{false, -1, {0, 0}};
+        const int bx = mx * 4 + x4, by = my * 4 + y4;
+        if (bx < 0 || by < 0 || bx >= W * 4 || by >= H * 4) return n;
+        const Blk *k = at(bx, by);
+        if (!k->decoded) {
+            n.avail = x4 >= 0 && x4 < 4 && y4 >= 0 && y4 < 4;  // current MB: available, no motion
+            return n;
+        }
+        n.avail = true;
+        n.ref = k->ref[l];
+        if (n.ref >= 0) {
+            n.mv[0] = k->mv[l][0];
+            n.mv[1] = k->mv[l][1];
+        }
+        return n;
+    }
+
+    // 8.4.1.3: mvp of the partition at 4x4 position (x4, y4), predPartWidth w4 (4x4
+    // units), for list l and reference index ref.  dir: 1 = 16x8 partition, 2 = 8x16
+    // partition (with part its index), 0 otherwise.
+    void mvp(int mx, int my, int x4, int y4, int w4, int l, int ref, int dir, int part, int out[2]) {
+        const Nb A = nb(mx, my, x4 - 1, y4, l), B = nb(mx, my, x4, y4 - 1, l);
+        Nb C = nb(mx, my, x4 + w4, y4 - 1, l);
+        if (!C.avail) C = nb(mx, my, x4 - 1, y4 - 1, l);  // D replaces an unavailable C
+        const Nb *dirn = nullptr;  // directional prediction of 16x8 / 8x16 (8.4.1.3)
+        if (dir == 1) dirn = part == 0 ? &B : &A;
+        else if (dir == 2) dirn = part == 0 ? &A : &C;
+        if (dirn && dirn->ref == ref) {
+            out[0] = dirn->mv[0];
+            out[1] = dirn->mv[1];
+            return;
+        }
+        // 8.4.1.3.1 median
+        Nb a = A, bb = B, c = C;
+        if (!bb.avail && !c.avail && a.avail) bb = c = a;
+        const int n = (a.ref == ref) + (bb.ref == ref) + (c.ref == ref);
+        if (n == 1) {
+            const Nb &m = a.ref == ref ? a : (bb.ref == ref ? bb : c);
+            out[0] = m.mv[0];
+            out[1] = m.mv[1];
+            return;
+        }
+        for (int k = 0; k < 2; k++) out[k] = median3(a.mv[k], bb.mv[k], c.mv[k]);
+    }
+
+    // Writes the motion of the w4 x h4 blocks at (x4, y4) of MB (mx, my) and marks them decoded.
+    void put(int mx, int my, int x4, int y4, int w4, int h4, const int ref[2], const int mv[2][2], h264g_mb *mb) {
+        for (int y = y4; y < y4 + h4; y++)
+            for (int x = x4; x < x4 + w4; x++) {
+                Blk *k = at(mx * 4 + x, my * 4 + y);
+                k->decoded = 1;
+                for (int l = 0; l < 2; l++) {
+                    k->ref[l] = (int8_t)ref[l];
+                    for (int c = 0; c < 2; c++) {
+                        k->mv[l][c] = (int16_t)(ref[l] >= 0 ? mv[l][c] : 0);
+                        if (mb) mb->mv[l][y * 4 + x][c] = k->mv[l][c];
+                    }
+                }
+            }
+    }
+};
+
+extern "C" {
+
+int32_t h264g_mvfield_create(int32_t width_mbs, int32_t height_mbs, h264g_mvfield **out) {
+    if (!out || width_mbs < 1 || height_mbs < 1 || width_mbs > 1024 || height_mbs > 1024)
+        return h264g_set_error(H264G_E_ARG, "mvfield %d x %d MBs", width_mbs, height_mbs);
+    h264g_mvfield *f = new (std::nothrow) h264g_mvfield;
+    if (!f) return h264g_set_error(H264G_E_NOMEM, "mvfield");
+    f->W = width_mbs;
+    f->H = height_mbs;
+    try {
+        f->b.assign((size_t)width_mbs * height_mbs * 16, Blk{{-1, -1}, 0, {{0, 0}, {0, 0}}});
+    } catch (...) {
+        delete f;
+        return h264g_set_error(H264G_E_NOMEM, "mvfield");
+    }
+    *out = f;
+    return 0;
+}
+
+void h264g_mvfield_destroy(h264g_mvfield *f) { delete f; }
+
+int32_t h264g_mvfield_reset(h264g_mvfield *f) {
+    if (!f) return h264g_set_error(H264G_E_ARG, "mvfield_reset: null field");
+    std::fill(f->b.begin(), f->b.end(), Blk{{-1, -1}, 0, {{0, 0}, {0, 0}}});
+    return 0;
+}
+
+int32_t h264g_mvfield_derive(h264g_mvfield *f, uint32_t mb_addr, h264g_mb *mb, const int16_t *mvd) {
+    if (!f || !mb) return h264g_set_error(H264G_E_ARG, "mvfield_derive: null argument");
+    if (mb_addr >= (uint32_t)(f->W * f->H)) return h264g_set_error(H264G_E_ARG, "mvfield_derive: MB %u", mb_addr);
+    const int mx = (int)(mb_addr % (uint32_t)f->W), my = (int)(mb_addr / (uint32_t)f->W);
+    for (int y = 0; y < 4; y++)  // a partition of this MB is available only once derived
+        for (int x = 0; x < 4; x++) f->at(mx * 4 + x, my * 4 + y)->decoded = 0;
+    const int none[2] = {-1, -1}, zero[2][2] = {{0, 0}, {0, 0}};
+    switch (mb->mb_type) {
+    case H264G_MB_I4x4:
+    case H264G_MB_I8x8:
+    case H264G_MB_I16x16:
+    case H264G_MB_IPCM:  // neighbours see an intra MB as refIdx -1, mv 0 (8.4.1.3.2)
+        f->put(mx, my, 0, 0, 4, 4, none, zero, nullptr);
+        return 0;
+    case H264G_MB_P_SKIP: {  // 8.4.1.1 (H264InterPrediction.cpp:685-738)
+        const Nb A = f->nb(mx, my, -1, 0, 0), B = f->nb(mx, my, 0, -1, 0);
+        int mv[2][2] = {{0, 0}, {0, 0}};
+        if (A.avail && B.avail && !(A.ref == 0 && A.mv[0] == 0 && A.mv[1] == 0) &&
+            !(B.ref == 0 && B.mv[0] == 0 && B.mv[1] == 0))
+            f->mvp(mx, my, 0, 0, 4, 0, 0, 0, 0, mv[0]);
+        for (int q = 0; q < 4; q++) {
+            mb->ref_idx[0][q] = 0;
+            mb->ref_idx[1][q] = -1;
+        }
+        const int ref[2] = {0, -1};
+        f->put(mx, my, 0, 0, 4, 4, ref, mv, mb);
+        return 0;
+    }
+    case H264G_MB_P:
+    case H264G_MB_B:
+        break;
+    default:
+        return h264g_set_error(H264G_E_ARG, "mvfield_derive: MB %u type %d", mb_addr, mb->mb_type);
+    }
+    if (!mvd) return h264g_set_error(H264G_E_ARG, "mvfield_derive: MB %u needs mvd", mb_addr);
+    const int shape = mb->part_shape;
+    if (shape > H264G_PART_8x8) return h264g_set_error(H264G_E_ARG, "mvfield_derive: MB %u shape %d", mb_addr, shape);
+    const int np = shape == H264G_PART_16x16 ? 1 : (shape == H264G_PART_8x8 ? 4 : 2);
+    for (int p = 0; p < np; p++) {
+        // partition p (mbPartIdx) in 4x4 units, then its sub-partitions in order
+        int px = 0, py = 0, pw = 4, ph = 4, ns = 1, sw = 4, sh = 4;
+        if (shape == H264G_PART_16x8) { py = 2 * p; ph = 2; }
+        else if (shape == H264G_PART_8x16) { px = 2 * p; pw = 2; }
+        else if (shape == H264G_PART_8x8) {
+            px = 2 * (p & 1); py = 2 * (p >> 1); pw = ph = 2;
+            const int ss = mb->sub_part_shape[p];
+            if (ss > H264G_SUB_4x4) return h264g_set_error(H264G_E_ARG, "mvfield_derive: MB %u sub shape %d", mb_addr, ss);
+            ns = ss == H264G_SUB_8x8 ? 1 : (ss == H264G_SUB_4x4 ? 4 : 2);
+        }
+        sw = pw;  // (sub-)partition size: predPartWidth is the sub-partition's width for P_8x8 / B_8x8
+        sh = ph;
+        if (shape == H264G_PART_8x8) {
+            const int ss = mb->sub_part_shape[p];
+            sw = (ss == H264G_SUB_4x8 || ss == H264G_SUB_4x4) ? 1 : 2;
+            sh = (ss == H264G_SUB_8x4 || ss == H264G_SUB_4x4) ? 1 : 2;
+        }
+        const int q = (py >> 1) * 2 + (px >> 1);  // 8x8 quadrant of the partition's first block
+        const int ref[2] = {mb->ref_idx[0][q], mb->ref_idx[1][q]};
+        if (ref[0] < 0 && ref[1] < 0)
+            return h264g_set_error(H264G_E_ARG, "mvfield_derive: MB %u partition %d uses no list", mb_addr, p);
+        const int dir = shape == H264G_PART_16x8 ? 1 : (shape == H264G_PART_8x16 ? 2 : 0);
+        for (int s = 0; s < ns; s++) {
+            int sx = px, sy = py;  // top-left 4x4 block of sub-partition s
+            if (shape == H264G_PART_8x8) {
+                if (sw == 1 && sh == 1) { sx += s & 1; sy += s >> 1; }
+                else if (sw == 1) sx += s;
+                else if (sh == 1) sy += s;
+            }
+            const int blk = sy * 4 + sx;
+            int mv[2][2] = {{0, 0}, {0, 0}};
+            for (int l = 0; l < 2; l++) {
+                if (ref[l] < 0) continue;
+                int pred[2];
+                f->mvp(mx, my, sx, sy, sw, l, ref[l], dir, p, pred);
+                for (int c = 0; c < 2; c++) mv[l][c] = pred[c] + mvd[(l * 16 + blk) * 2 + c];
+            }
+            f->put(mx, my, sx, sy, sw, sh, ref, mv, mb);
+        }
+    }
+    return 0;
+}
+
+}  // extern "C"

@@ -109,6 +109,47 @@ class Generator:
         return desc, mbs_p, nmb.value, lv_p, nlv.value
 
 
+class MotionField:
+    """Host motion-vector derivation for one picture being parsed (h264g_mvfield_*):
+    P_Skip vectors and mvp + mvd for P / B partitions of frame pictures, in the
+    reference's order (H264InterPrediction.cpp:685-738, 1660-2001)."""
+
+    def __init__(self, width_mbs: int, height_mbs: int):
+        h = c_void_p()
+        check(lib().h264g_mvfield_create(width_mbs, height_mbs, byref(h)))
+        self._h = h
+        self.width_mbs, self.height_mbs = width_mbs, height_mbs
+
+    def close(self):
+        if self._h:
+            lib().h264g_mvfield_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def reset(self):
+        """Start of a picture."""
+        check(lib().h264g_mvfield_reset(self._h))
+
+    def derive_picture(self, mbs: np.ndarray, mvd: np.ndarray) -> np.ndarray:
+        """Records with their mv filled in: mbs (MB_DTYPE, raster order; mv ignored),
+        mvd int16 [MB][list][16][2] (each partition's mvd at its top-left 4x4 block)."""
+        assert mbs.dtype == MB_DTYPE and mvd.dtype == np.int16 and mvd.shape == (len(mbs), 2, 16, 2)
+        assert len(mbs) == self.width_mbs * self.height_mbs
+        out = mbs.copy()
+        out["mv"] = 0
+        mvd = np.ascontiguousarray(mvd)
+        self.reset()
+        base = out.ctypes.data
+        for a in range(len(out)):
+            check(lib().h264g_mvfield_derive(self._h, a, base + a * MB_DTYPE.itemsize, mvd[a].ctypes.data))
+        return out
+
+
 # ------------------------------------------------------------------ engine
 @dataclass
 class StagedPicture:

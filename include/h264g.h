@@ -356,6 +356,28 @@ typedef struct h264g_gen_params {
                                      slot of the picture decoded just before it        */
 } h264g_gen_params;
 
+/* ------------------------------------------------ host motion-vector derivation
+ * The final motion vectors of an h264g_mb record from the parsed syntax, for frame
+ * pictures: P_Skip vectors (8.4.1.1) and mvp + mvd with the luma motion-vector
+ * prediction over the neighbouring partitions (8.4.1.3, 6.4.11.7).  Replaces what the
+ * reference derives inside reconstruction: P_Skip at H264InterPrediction.cpp:685-738,
+ * Derivation_process_for_luma_motion_vector_prediction and its neighbouring
+ * partitions at H264InterPrediction.cpp:1660-2001.  Host only, no device work; one
+ * field per picture being parsed, not thread-safe.  B_Skip / B_Direct (8.4.1.2) are
+ * not derived: the record cannot carry them. */
+typedef struct h264g_mvfield h264g_mvfield;
+int32_t h264g_mvfield_create(int32_t width_mbs, int32_t height_mbs, h264g_mvfield **out);
+void    h264g_mvfield_destroy(h264g_mvfield *field);
+/* Start of a picture: no MB decoded yet. */
+int32_t h264g_mvfield_reset(h264g_mvfield *field);
+/* MB mb_addr of the picture, called for every MB in decoding (raster) order, intra
+ * MBs included.  In: mb_type, part_shape, sub_part_shape, ref_idx (P_Skip: set to
+ * refIdxL0 = 0 here).  mvd: int16 [list][16][2], the mvd of each (sub-)partition and
+ * list at its top-left 4x4 block (raster index, the layout of mb->mv), read for the
+ * lists with ref_idx >= 0; NULL for intra and P_Skip MBs.  Out: mb->mv of every 4x4
+ * block of the used lists.  H264G_E_ARG for a type the field does not derive. */
+int32_t h264g_mvfield_derive(h264g_mvfield *field, uint32_t mb_addr, h264g_mb *mb, const int16_t *mvd);
+
 int32_t h264g_gen_default_params(int32_t config, h264g_gen_params *out);
 int32_t h264g_gen_create(const h264g_gen_params *params, h264g_gen **out);
 int32_t h264g_gen_destroy(h264g_gen *gen);

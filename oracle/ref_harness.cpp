@@ -25,9 +25,12 @@
 //
 // Usage: ref_harness --config N [--frames F --width W --height H --seed S]
 //                    [--ipcm P] [--wp P,B] [--cqp a,b] [--deblock idc]
-//                    --out DIR [--predeblock] [--bgr] [--time]
+//                    --out DIR [--predeblock] [--bgr] [--mvd] [--time]
 // --bgr also writes every picture converted by the reference's
 // CH264PictureBase::convertYuv420pToBgr24 (rows padded to 4 bytes) as bgr_NNN.raw.
+// --mvd also writes the solved mvd of every picture as mvd_NNN.bin: int16
+// [MB][list][16][2], each (sub-)partition's mvd at its top-left 4x4 block (raster index
+// inside the MB, the layout of h264g_mb::mv) for the lists it uses, zero elsewhere.
 #include "H264PicturesGOP.h"
 #include "H264Picture.h"
 #include "H264PictureBase.h"
@@ -138,6 +141,7 @@ struct Harness {
     CH264Picture *dpb[16];
     CH264SliceHeader sh;
     bool predeblock = false;
+    bool mvd = false;
     bool bgr = false;
     std::string out;
     double recon_seconds = 0;
@@ -508,6 +512,35 @@ struct Harness {
         }
     }
 
+    void write_mvd(const CH264MacroBlock *ms, const h264g_mb *mbs, uint32_t nmb, int idx) {
+        std::vector<int16_t> buf((size_t)nmb * 2 * 16 * 2, 0);
+        for (uint32_t a = 0; a < nmb; a++) {
+            const h264g_mb &r = mbs[a];
+            if (r.mb_type != H264G_MB_P && r.mb_type != H264G_MB_B) continue;
+            int np = r.part_shape == H264G_PART_16x16 ? 1 : (r.part_shape == H264G_PART_8x8 ? 4 : 2);
+            for (int p = 0; p < np; p++) {
+                int ns = 1;
+                if (r.part_shape == H264G_PART_8x8)
+                    ns = r.sub_part_shape[p] == H264G_SUB_8x8 ? 1 : (r.sub_part_shape[p] == H264G_SUB_4x4 ? 4 : 2);
+                for (int s = 0; s < ns; s++) {
+                    int blk = part_block(r, p, s);
+                    int q = (blk / 8) * 2 + (blk % 4) / 2;
+                    for (int l = 0; l < 2; l++) {
+                        if (r.ref_idx[l][q] < 0) continue;
+                        const int32_t (&md)[4][4][2] = l == 0 ? ms[a].mvd_l0 : ms[a].mvd_l1;
+                        for (int c = 0; c < 2; c++) buf[(((size_t)a * 2 + l) * 16 + blk) * 2 + c] = (int16_t)md[p][s][c];
+                    }
+                }
+            }
+        }
+        char path[1024];
+        snprintf(path, sizeof(path), "%s/mvd_%03d.bin", out.c_str(), idx);
+        FILE *f = fopen(path, "wb");
+        if (!f) fail("cannot write output");
+        fwrite(buf.data(), sizeof(int16_t), buf.size(), f);
+        fclose(f);
+    }
+
     void write_planes(CH264PictureBase &pic, const char *tag, int idx) {
         char path[1024];
         snprintf(path, sizeof(path), "%s/%s_%03d.yuv", out.c_str(), tag, idx);
@@ -620,6 +653,7 @@ struct Harness {
             recon_seconds += std::chrono::duration<double>(t1 - t0).count();
             for (uint32_t a = 0; a < nmb; a++) check_mb(pic, (int)a, mbs[a], i);
             write_planes(pic, "pic", i);
+            if (mvd) write_mvd(saved.data(), mbs, nmb, i);
             // reference marking state used by later pictures
             cur->m_picture_coded_type_marked_as_refrence = H264_PICTURE_CODED_TYPE_FRAME;
             pic.reference_marked_type = H264_PICTURE_MARKED_AS_used_for_short_term_reference;
@@ -653,6 +687,7 @@ int main(int argc, char **argv) {
         else if (a == "--bframes") p.b_frames = atoi(next());
         else if (a == "--out") hs.out = next();
         else if (a == "--predeblock") hs.predeblock = true;
+        else if (a == "--mvd") hs.mvd = true;
         else if (a == "--bgr") hs.bgr = true;
         else if (a == "--time") timing = true;
         else fail("unknown argument");

@@ -1,0 +1,69 @@
+"""Writes tests/golden/mvd_<case>.npz: the motion-vector differences that the
+reference decoder's own prediction needs to reproduce the generator's final motion
+vectors.
+
+oracle/ref_harness.cpp (the reference's reconstruction sources compiled unmodified by
+oracle/Makefile) solves, partition by partition in decoding order, mvd = mv - mvp with
+the reference's Derivation_process_for_luma_motion_vector_prediction
+(H264InterPrediction.cpp:1660-2001), decodes the picture again from those mvds and
+checks that every derived motion vector equals the generator's.  `--mvd` dumps them:
+int16 [picture][MB][list][16][2], each (sub-)partition's mvd at its top-left 4x4 block
+(raster index inside the MB) for the lists it uses.  These fixtures pin the library's
+host motion-vector derivation (h264g_mvfield_*) to the reference's.
+
+Run in a container that has /root/reference:  python tests/golden/make_mvd.py
+"""
+from __future__ import annotations
+
+import json
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, HERE)
+from make_golden import HARNESS, harness_args  # noqa: E402
+
+# name -> (generator config, overrides as generator parameter names)
+CASES = {
+    "cif_ip": (1, {}),                                              # P: 16x16, 8x8 with all sub shapes, P_Skip
+    "high_small": (2, {"width_mbs": 40, "height_mbs": 24}),         # IPBBPBBP, 4 refs, bi-prediction
+    "high_small_seed7": (2, {"width_mbs": 23, "height_mbs": 17, "seed": 7}),
+    "narrow": (2, {"width_mbs": 1, "height_mbs": 12, "max_refs": 1, "b_frames": 0}),
+    "flat": (2, {"width_mbs": 14, "height_mbs": 1, "max_refs": 2}),
+}
+
+
+def run_case(config: int, ov: dict):
+    from h264_video_decoder_demo_amd import Generator
+    g = Generator(config, **ov)
+    nmb = g.width_mbs * g.height_mbs
+    with tempfile.TemporaryDirectory() as d:
+        r = subprocess.run([HARNESS, *harness_args(config, ov), "--mvd", "--out", d],
+                           capture_output=True, text=True, check=True)
+        info = json.loads(r.stdout.strip().splitlines()[-1])
+        assert info["mv_mismatches"] == 0, info
+        assert int(info["digest"], 16) == g.digest, (info["digest"], g.digest)
+        n = info["pictures"]
+        mvd = np.stack([np.fromfile(os.path.join(d, f"mvd_{i:03d}.bin"), np.int16).reshape(nmb, 2, 16, 2)
+                        for i in range(n)])
+    return g.digest, mvd
+
+
+def main():
+    if not os.path.exists(HARNESS):
+        subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "ref", "-j8"], check=True)
+    sys.path.insert(0, ROOT)
+    for name, (config, ov) in CASES.items():
+        digest, mvd = run_case(config, ov)
+        np.savez_compressed(os.path.join(HERE, f"mvd_{name}.npz"), mvd=mvd, digest=np.uint64(digest),
+                            config=np.int32(config), overrides=json.dumps(ov, sort_keys=True))
+        print(name, hex(digest), mvd.shape, int(np.count_nonzero(mvd)), flush=True)
+
+
+if __name__ == "__main__":
+    main()
