@@ -166,6 +166,8 @@ def main():
     ap.add_argument("--deblock-waves", type=int, default=0, help="MB rows per band of the bands schedule (0 = 8)")
     ap.add_argument("--intra", choices=["bands", "mb_wavefront"], default="bands", help="intra schedule")
     ap.add_argument("--intra-waves", type=int, default=0, help="MB rows per band of the bands intra schedule (0 = 8)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default band heights instead of autotuning them on the workload's first pictures")
     ap.add_argument("--serve-seconds", type=float, default=3.0,
                     help="length of the serving run (Poisson arrivals, 128 streams, 400 frames/s); 0 = skip")
     ap.add_argument("--profile-run", action="store_true",
@@ -192,6 +194,18 @@ def main():
     with cf.ThreadPoolExecutor(16) as ex:
         gens = list(ex.map(lambda s: Generator(2, **s), specs))
     log(f"generated {len(gens)} streams in {time.perf_counter() - t:.1f} s")
+
+    # band heights: explicit, or picked by the autotuner on this workload (outside the
+    # timed region; cached per machine).  --profile-run only looks the choice up, so
+    # that the profiled kernels are the measured run's alone: run bench.py once first.
+    tune = None
+    if (args.deblock == "bands" and args.intra == "bands" and not args.deblock_waves and not args.intra_waves
+            and not args.no_autotune):
+        from h264_video_decoder_demo_amd.autotune import autotune
+        tune = autotune(gens, measure=not args.profile_run, engine_kwargs=dict(waves=args.waves, graphs=not args.no_graphs))
+        args.deblock_waves, args.intra_waves = tune["deblock_waves"], tune["intra_waves"]
+        log(f"autotune: bands of {args.deblock_waves} / {args.intra_waves} rows (deblocking / intra) in "
+            f"{tune['seconds']:.1f} s{' (cached)' if tune['cached'] else ''}: {tune.get('ms')}")
 
     schedule = {"bands": 0, "mb_wavefront": 1, "mb_rows": 2}[args.deblock]
     eng = Engine(device=0, profile=False, waves=args.waves, graphs=not args.no_graphs, deblock_schedule=schedule,
@@ -373,6 +387,10 @@ def main():
         "graphs": not args.no_graphs,
         "deblock_schedule": args.deblock,
         "intra_schedule": args.intra,
+        "band_rows": {"deblock": args.deblock_waves or 8, "intra": args.intra_waves or 8,
+                      "autotune": ({"ms_per_sample": tune["ms"], "seconds": round(tune["seconds"], 1),
+                                    "cached": tune["cached"]} if tune and not tune["cached"] else
+                                   ({"cached": True} if tune else None))},
         "batched_event_timed_value": round(mpix_step * args.steps / t_events, 1),
         "plain_value": round(plain, 1) if plain else None,
         "batched_over_plain": round(batched / plain, 2) if plain else None,

@@ -110,6 +110,7 @@ _SIGNATURES = {
     "h264g_destroy": (c_int32, [c_void_p]),
     "h264g_stream_open": (c_int32, [c_void_p, POINTER(StreamDesc), POINTER(c_uint32)]),
     "h264g_stream_close": (c_int32, [c_void_p, c_uint32]),
+    "h264g_set_bands": (c_int32, [c_void_p, c_int32, c_int32]),
     "h264g_picture_begin": (c_int32, [c_void_p, c_uint32, POINTER(PicDesc), c_void_p, c_uint32, c_void_p, c_uint64,
                                       POINTER(c_void_p)]),
     "h264g_picture_submit": (c_int32, [c_void_p, c_void_p]),

@@ -49,6 +49,8 @@ struct Stream {
     uint64_t *rowprog = nullptr;
     uint8_t *gstrip = nullptr;     // deblocking strips between row bands, then the intra ones
     uint64_t *istrip = nullptr;    // the intra strips inside gstrip
+    uint64_t strip_bytes = 0;      // size of gstrip, sized for bands of
+    int strip_dbk_rows = 0, strip_intra_rows = 0;  // at least these many rows
     uint64_t bytes = 0;
     int lane = 0;                  // HIP stream (lane) of the context that runs its pictures
     bool open = false;
@@ -121,6 +123,7 @@ struct Batch {
 
 struct h264g_batch {
     Batch sub[2];                  // pictures of lane 0 / lane 1
+    int dbk_rows = 0, intra_rows = 0;  // band heights its band lists were built for
 };
 
 struct h264g_ctx {
@@ -903,6 +906,9 @@ int32_t h264g_stream_open(h264g_ctx *c, const h264g_stream_desc *d, uint32_t *id
     HIPCHK(hipMemsetAsync(s.dpb, 0, dpb_bytes, ls));
     HIPCHK(hipMemsetAsync(s.flags, 0, flag_bytes, ls));
     s.istrip = (uint64_t *)(s.gstrip + dstrip_bytes);
+    s.strip_bytes = strip_bytes;
+    s.strip_dbk_rows = c->dbk_waves;
+    s.strip_intra_rows = c->intra_waves;
     s.bytes = need;
     s.open = true;
     c->used += need;
@@ -914,6 +920,22 @@ int32_t h264g_stream_open(h264g_ctx *c, const h264g_stream_desc *d, uint32_t *id
     return H264G_OK;
 }
 
+int32_t h264g_set_bands(h264g_ctx *c, int32_t deblock_rows, int32_t intra_rows) {
+    if (!c) return h264g_set_error(H264G_E_ARG, "ctx");
+    std::lock_guard<std::mutex> lk(c->mu);
+    const int d = deblock_rows > 0 ? deblock_rows : c->dbk_waves, i = intra_rows > 0 ? intra_rows : c->intra_waves;
+    if (d > 24 || i > 8) return h264g_set_error(H264G_E_ARG, "band heights %d / %d", d, i);
+    for (uint32_t k = 0; k < c->streams.size(); k++) {
+        const Stream &s = c->streams[k];
+        if (s.open && (d < s.strip_dbk_rows || i < s.strip_intra_rows))
+            return h264g_set_error(H264G_E_STATE, "stream %u has strips for bands of at least %d / %d rows", k,
+                                   s.strip_dbk_rows, s.strip_intra_rows);
+    }
+    c->dbk_waves = d;
+    c->intra_waves = i;
+    return H264G_OK;
+}
+
 int32_t h264g_stream_close(h264g_ctx *c, uint32_t id) {
     if (!c) return h264g_set_error(H264G_E_ARG, "ctx");
     std::lock_guard<std::mutex> lk(c->mu);
@@ -921,11 +943,10 @@ int32_t h264g_stream_close(h264g_ctx *c, uint32_t id) {
     (void)hipSetDevice(c->device);
     Stream &s = c->streams[id];
     // back to the pool, reusable once the work enqueued so far has completed (no wait)
-    const uint64_t dstrip_bytes = (uint64_t)((s.H + c->dbk_waves - 1) / c->dbk_waves) * s.W * 256;
     pool_put(c, s.dpb, s.slot_bytes * s.desc.dpb_slots);
     pool_put(c, s.flags, (size_t)s.W * s.H * 4);
     pool_put(c, s.rowprog, (size_t)s.H * 8);
-    pool_put(c, s.gstrip, dstrip_bytes + (uint64_t)((s.H + c->intra_waves - 1) / c->intra_waves) * s.W * 64);
+    pool_put(c, s.gstrip, s.strip_bytes);
     c->used -= s.bytes;
     s = Stream();
     return H264G_OK;
@@ -1109,6 +1130,8 @@ int32_t h264g_batch_create(h264g_ctx *c, h264g_pic *const *pics, uint32_t n, h26
     }
     h264g_batch *b = new (std::nothrow) h264g_batch();
     if (!b) return h264g_set_error(H264G_E_NOMEM, "batch");
+    b->dbk_rows = c->dbk_waves;
+    b->intra_rows = c->intra_waves;
     for (int l = 0; l < 2; l++) {
         if (lane[l].empty()) continue;
         int rc = build_sub(c, lane[l], &b->sub[l]);
@@ -1146,6 +1169,9 @@ StageSet batch_stages(const Batch *b) {
 int32_t h264g_batch_run(h264g_ctx *c, h264g_batch *b) {
     if (!c || !b) return h264g_set_error(H264G_E_ARG, "NULL argument");
     std::lock_guard<std::mutex> lk(c->mu);
+    if (b->dbk_rows != c->dbk_waves || b->intra_rows != c->intra_waves)
+        return h264g_set_error(H264G_E_STATE, "batch built for bands of %d / %d rows, the context now uses %d / %d: "
+                               "create it again", b->dbk_rows, b->intra_rows, c->dbk_waves, c->intra_waves);
     HIPCHK(hipSetDevice(c->device));
     for (int l = 0; l < 2; l++) {
         Batch &sb = b->sub[l];

@@ -210,6 +210,11 @@ class Engine:
         check(lib().h264g_stream_close(self._h, sid))
         self.streams.pop(sid, None)
 
+    def set_bands(self, deblock_rows: int = 0, intra_rows: int = 0):
+        """MB rows per band of the band schedules from now on (0 keeps one); batches
+        created before must be created again."""
+        check(lib().h264g_set_bands(self._h, deblock_rows, intra_rows))
+
     # ---- pictures
     def begin(self, sid: int, desc: PicDesc, mbs: np.ndarray, levels: np.ndarray) -> StagedPicture:
         mbs = np.ascontiguousarray(mbs, MB_DTYPE)

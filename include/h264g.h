@@ -236,6 +236,13 @@ int32_t h264g_destroy(h264g_ctx *ctx);
 int32_t h264g_stream_open(h264g_ctx *ctx, const h264g_stream_desc *desc, uint32_t *stream_id);
 int32_t h264g_stream_close(h264g_ctx *ctx, uint32_t stream_id);
 
+/* Band heights (MB rows per band) of the BANDS schedules from now on, for the
+ * autotuner; 0 keeps a height.  A stream's strips between bands are sized when it is
+ * opened, so heights below those of any open stream fail with H264G_E_STATE, and a
+ * batch created before the call fails in h264g_batch_run (H264G_E_STATE) until it is
+ * created again.  No counterpart in the reference (it has no GPU schedules). */
+int32_t h264g_set_bands(h264g_ctx *ctx, int32_t deblock_rows, int32_t intra_rows);
+
 /* -------------------------------------------------------- plain path */
 /* Stage one picture's syntax: copies the records and levels to device memory
  * (input arrays stay owned by the caller).  Replaces the per-MB dispatch at
