@@ -554,15 +554,36 @@ int stage_picture(h264g_ctx *ctx, Stream &st, const h264g_pic_desc &d, const h26
     size_t o_mbs = 0, o_aux = al(o_mbs + n_mbs * sizeof(h264g_mb)), o_lv = al(o_aux + n_mbs * sizeof(DevAux));
     size_t o_wt = al(o_lv + n_levels * sizeof(int16_t) + 16), o_it = al(o_wt + wt.size() * sizeof(DevWeight));
     pic->n_rows = (uint32_t)H;
-    // intra MBs per row: offsets, then columns (the per-picture intra schedule)
+    // intra MBs per row: offsets, then one entry per intra MB (the per-picture intra band
+    // schedule; an MB's residual sits at its entry's index in P.ires).  The entry carries
+    // what the band kernel would otherwise load from the neighbours' records before its
+    // wait (ICOL_* in h264g_dev.h): the column, the neighbour availability (A, B, C, D in
+    // bits 0..3; constrained_intra_pred drops inter neighbours), which available
+    // neighbours are intra MBs of this stage (C only when the MB reads its samples:
+    // Intra_8x8, or Intra_4x4 block 5 in modes 3 / 7), and whether it has levels.
     std::vector<uint32_t> irow(H + 1, 0), icol;
     for (int y = 0; y < H; y++) {
         irow[y] = (uint32_t)icol.size();
-        for (int xx = 0; xx < W; xx++)
-            if (aux[y * W + xx].kind & AUX_INTRA) {
-                aux[y * W + xx].ires = (uint32_t)icol.size();
-                icol.push_back((uint32_t)xx);
+        for (int xx = 0; xx < W; xx++) {
+            const uint32_t a = (uint32_t)(y * W + xx);
+            if (!(aux[a].kind & AUX_INTRA)) continue;
+            uint32_t av = (xx > 0 ? 1u : 0u) | (y > 0 ? 2u : 0u) | (y > 0 && xx < W - 1 ? 4u : 0u) |
+                          (xx > 0 && y > 0 ? 8u : 0u), dep = 0;
+            const int64_t nb[4] = {(int64_t)a - 1, (int64_t)a - W, (int64_t)a - W + 1, (int64_t)a - W - 1};
+            for (int k = 0; k < 4; k++) {
+                if (!(av & (1u << k))) continue;
+                const int c = mbs[nb[k]].mb_type;
+                if (d.constrained_intra_pred_flag && cls_inter(c)) av &= ~(1u << k);
+                if (cls_intra(c)) dep |= 1u << k;
             }
+            const h264g_mb &m = mbs[a];
+            if (!(m.mb_type == H264G_MB_I8x8 ||
+                  (m.mb_type == H264G_MB_I4x4 && (m.intra_pred_modes[5] == 3 || m.intra_pred_modes[5] == 7))))
+                dep &= ~4u;
+            aux[a].ires = (uint32_t)icol.size();
+            icol.push_back((uint32_t)xx | av << ICOL_AV_SHIFT | dep << ICOL_DEP_SHIFT |
+                           ((aux[a].kind & AUX_RESID) ? ICOL_RES : 0u));
+        }
     }
     irow[H] = (uint32_t)icol.size();
     size_t n_items = pic->n_inter + pic->n_intra + n_mbs + (size_t)H + irow.size() + icol.size();
@@ -869,6 +890,7 @@ int32_t h264g_stream_open(h264g_ctx *c, const h264g_stream_desc *d, uint32_t *id
     if (d->chroma_array_type != 1 || d->bit_depth != 8) return h264g_set_error(H264G_E_ARG, "only 8-bit 4:2:0 is supported");
     if (d->dpb_slots < 1 || d->dpb_slots > 17) return h264g_set_error(H264G_E_ARG, "dpb_slots");
     if (d->frame_height_in_mbs > 1024) return h264g_set_error(H264G_E_ARG, "frame_height_in_mbs > 1024");
+    if (d->pic_width_in_mbs > (int)ICOL_X_MASK + 1) return h264g_set_error(H264G_E_ARG, "pic_width_in_mbs > 2048");
     if (!d->frame_mbs_only || d->mbaff) return h264g_set_error(H264G_E_ARG, "field coding is not supported");
     Stream s;
     s.desc = *d;

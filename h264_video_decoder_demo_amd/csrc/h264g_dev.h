@@ -27,6 +27,14 @@ static_assert(sizeof(DevAux) == 28, "DevAux layout");
 #define AUX_T8      0x04u  // transform_size_8x8_flag
 #define AUX_INTER   0x08u
 #define AUX_RESID   0x10u  // has coefficient levels
+// Entry of an intra MB in the intra band schedule (DevPic::irow after the row offsets):
+// column in bits 0..10, neighbour availability (A, B, C, D) from ICOL_AV_SHIFT, the
+// available neighbours that are intra MBs the MB waits for from ICOL_DEP_SHIFT, and
+// ICOL_RES when the MB has levels (its residual is at the entry's index in DevPic::ires).
+#define ICOL_X_MASK    0x7FFu
+#define ICOL_AV_SHIFT  11
+#define ICOL_DEP_SHIFT 15
+#define ICOL_RES       (1u << 19)
 
 // Resolved weighted-prediction parameters of one (refIdxL0, refIdxL1) combination.
 struct DevWeight {
@@ -48,7 +56,7 @@ struct DevPic {
     uint64_t *rowprog;        // per-MB-row progress of row-pipelined deblocking:
                               // epoch << 32 | MBs of the row finished
     uint8_t  *bs;             // boundary strengths, 32 per MB: [dir][4-line segment][edge]
-    const uint32_t *irow;     // intra MBs per row: H + 1 offsets, then their columns
+    const uint32_t *irow;     // intra MBs per row: H + 1 offsets, then one ICOL entry per intra MB
     uint32_t n_intra;         // intra-predicted MBs of the picture
     int32_t  dst;             // destination slot
     int32_t  W, H;            // in macroblocks

@@ -1665,20 +1665,28 @@ __global__ __launch_bounds__(WAVE * INTRA_WG_MAX_WAVES) void k_intra_band(const 
             auto post = [&](uint32_t done) {  // progress for the next row of the band
                 if (!to_g && y < H - 1) sync.post(y - y0, (int)done);
             };
-            post(b < e ? uniform(col[b]) : (uint32_t)W);
+            post(b < e ? uniform(col[b]) & ICOL_X_MASK : (uint32_t)W);
             for (uint32_t i = b; i < e; i++) {
-                const int x = (int)uniform(col[i]), addr = y * W + x;
-                int deps[4];
-                const Avail av = intra_avail(P, addr, deps);
-                const IntraRes r = intra_res_fetch(P, addr);  // in flight during the wait
-                if (P.mbs[addr].mb_type == H264G_MB_I4x4) intra4_prep(&P.mbs[addr], av, &tiles[w]);
-                else if (P.mbs[addr].mb_type == H264G_MB_I8x8) intra8_prep(&P.mbs[addr], av, &tiles[w]);
+                // the MB's availability and intra neighbours come with its entry, so that
+                // nothing but its own record and residual is loaded before the wait
+                const uint32_t ent = uniform(col[i]);
+                const int x = (int)(ent & ICOL_X_MASK), addr = y * W + x;
+                Avail av;
+                av.m = (ent >> ICOL_AV_SHIFT) & 15u;
+                const uint32_t dep = (ent >> ICOL_DEP_SHIFT) & 15u;
+                IntraRes r;  // in flight during the wait
+                r.res = (ent & ICOL_RES) != 0;
+                r.v = make_uint4(0, 0, 0, 0);
+                if (r.res && wave_lane() < 48) r.v = ((const uint4 *)(P.ires + (size_t)i * 384))[wave_lane()];
+                const h264g_mb *mb = &P.mbs[addr];
+                if (mb->mb_type == H264G_MB_I4x4) intra4_prep(mb, av, &tiles[w]);
+                else if (mb->mb_type == H264G_MB_I8x8) intra8_prep(mb, av, &tiles[w]);
                 // the row above up to C (x + 1) or up to B (x) when only B / D are read and intra
-                const int need = deps[2] >= 0 ? min(x + 2, W) : (deps[1] >= 0 || deps[3] >= 0 ? x + 1 : 0);
+                const int need = (dep & 4u) ? min(x + 2, W) : ((dep & 10u) ? x + 1 : 0);
                 uint32_t above = 0;
                 if (need > 0) {
                     if (from_g) {
-                        above = (deps[1] >= 0 ? 2u : 0u) | (deps[2] >= 0 ? 4u : 0u) | (deps[3] >= 0 ? 8u : 0u);
+                        above = dep & 14u;
                         istrip_get(iin, x, above, &tiles[w], epoch, err);
                     } else {
                         sync.wait(y - 1 - y0, need);
@@ -1688,7 +1696,7 @@ __global__ __launch_bounds__(WAVE * INTRA_WG_MAX_WAVES) void k_intra_band(const 
                 intra_mb(P, addr, &tiles[w], av, r.res, above);
                 if (to_g) istrip_put(iout + (size_t)x * 8, &tiles[w], epoch);
                 const bool end = i + 1 == e;
-                post(end ? (uint32_t)W : uniform(col[i + 1]));
+                post(end ? (uint32_t)W : uniform(col[i + 1]) & ICOL_X_MASK);
             }
         }
     }
