@@ -75,3 +75,28 @@ def test_rejects_what_it_does_not_derive():
         _lib.check(_lib.lib().h264g_mvfield_derive(f._h, 0, mbs.ctypes.data, None))
     with pytest.raises(_lib.H264GError):  # MB address outside the picture
         _lib.check(_lib.lib().h264g_mvfield_derive(f._h, 1, mbs.ctypes.data, np.zeros(64, np.int16).ctypes.data))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["cif_ip", "high_small"])
+def test_pictures_from_derived_mvs_match_reference(golden, name):
+    """A parser's path: records whose motion vectors come from the mvds through the host
+    derivation (not from the generator), reconstructed on the GPU, give the reference's
+    pictures."""
+    from h264_video_decoder_demo_amd import Engine
+    from helpers import plane_hashes
+    z = np.load(os.path.join(GOLDEN, f"mvd_{name}.npz"), allow_pickle=False)
+    g = Generator(int(z["config"]), **json.loads(str(z["overrides"])))
+    assert g.digest == int(golden[name]["digest"], 16)  # the golden pictures are of this syntax
+    f = MotionField(g.width_mbs, g.height_mbs)
+    with Engine(device=0) as eng:
+        sid = eng.open_stream(g.width_mbs, g.height_mbs, g.dpb_slots)
+        for i in range(g.num_pictures):
+            desc, mbs, levels = g.picture(i)
+            derived = f.derive_picture(mbs, z["mvd"][i])
+            p = eng.begin(sid, desc, derived, levels)
+            eng.submit(p)
+            eng.sync()
+            assert plane_hashes(*eng.readback(sid, desc.dst_slot)) == golden[name]["pic"][i], (name, i)
+            eng.release(p)
+        eng.close_stream(sid)
