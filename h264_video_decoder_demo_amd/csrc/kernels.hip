@@ -202,28 +202,42 @@ __device__ __forceinline__ bool build_residual(const h264g_mb *mb, const DevAux 
     const int i16 = mb->mb_type == H264G_MB_I16x16;
     const int t8 = (ax.kind & AUX_T8) ? 1 : 0;
     const int qp = mb->qp_y;
-    int32_t *cf = &ws->coef[0][0];
-    for (int k = lane; k < 27 * 16; k += WAVE) cf[k] = 0;
-    wave_sync();
+    uint4 *cf = (uint4 *)&ws->coef[0][0];
+    for (int k = lane; k < 27 * 16 / 4; k += WAVE) cf[k] = make_uint4(0u, 0u, 0u, 0u);
     const int16_t *lv = levels + mb->level_offset;
-    const int nmask = ax.nmask;
+    const int nmask = ax.nmask;  // <= 27 masks of 16 bits: at most 432 values
     int mask = lane < nmask ? (uint16_t)lv[lane] : 0;
-    int cnt = __popc(mask);
-    int pre = cnt;  // inclusive prefix sum over lanes
+    const int cnt = __popc(mask);
+    // exclusive prefix sum of cnt (0..16) over the lanes, bit plane by bit plane (ballot +
+    // mbcnt, no cross-lane shuffles), and the total
+    int pre = 0, total = 0;
 #pragma unroll
-    for (int o = 1; o < 32; o <<= 1) {
-        int t = __shfl_up(pre, o, WAVE);
-        if (lane >= o) pre += t;
+    for (int bit = 0; bit < 5; bit++) {
+        const uint64_t bl = __builtin_amdgcn_ballot_w64((cnt >> bit) & 1);
+        pre += (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(bl >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bl, 0u)) << bit;
+        total += __popcll(bl) << bit;
     }
-    pre -= cnt;
+    // the values, with coalesced loads all in flight at once, into LDS (t8 is free until the
+    // 8x8 transform's first pass, after the scatter); the scatter then reads LDS
+    int16_t *stage = (int16_t *)&ws->t8[0][0];
+    static_assert(sizeof(ws->t8) >= 7 * WAVE * sizeof(int16_t), "value staging fits t8");
+    int16_t v[7];
+#pragma unroll
+    for (int t = 0; t < 7; t++) {
+        const int k = lane + t * WAVE;
+        v[t] = k < total ? lv[nmask + k] : (int16_t)0;
+    }
+#pragma unroll
+    for (int t = 0; t < 7; t++)
+        if (lane + t * WAVE < total) stage[lane + t * WAVE] = v[t];
+    wave_sync();
     if (lane < nmask) {
         int slot = level_slot(lane, i16, cbp);
-        const int16_t *vals = lv + nmask + pre;
-        int n = 0;
+        int n = pre;
         while (mask) {
             int k = __ffs(mask) - 1;
             mask &= mask - 1;
-            ws->coef[slot][k] = vals[n++];
+            ws->coef[slot][k] = stage[n++];
         }
     }
     wave_sync();
