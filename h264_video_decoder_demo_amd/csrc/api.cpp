@@ -447,6 +447,7 @@ int stage_picture(h264g_ctx *ctx, Stream &st, const h264g_pic_desc &d, const h26
             }
         }
         if (m.level_offset + nmask + nvals > n_levels) return h264g_set_error(H264G_E_ARG, "mb %u: values beyond levels", a);
+        x.ires = (uint32_t)nvals << AUX_NVALS_SHIFT;  // at most 27 x 16
         if (nmask) x.kind |= AUX_RESID;
         // ---- intra modes vs neighbour availability
         if (cls_intra(cls)) {
@@ -580,7 +581,7 @@ int stage_picture(h264g_ctx *ctx, Stream &st, const h264g_pic_desc &d, const h26
             if (!(m.mb_type == H264G_MB_I8x8 ||
                   (m.mb_type == H264G_MB_I4x4 && (m.intra_pred_modes[5] == 3 || m.intra_pred_modes[5] == 7))))
                 dep &= ~4u;
-            aux[a].ires = (uint32_t)icol.size();
+            aux[a].ires |= (uint32_t)icol.size();
             icol.push_back((uint32_t)xx | av << ICOL_AV_SHIFT | dep << ICOL_DEP_SHIFT |
                            ((aux[a].kind & AUX_RESID) ? ICOL_RES : 0u));
         }

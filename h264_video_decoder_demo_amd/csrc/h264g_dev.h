@@ -18,8 +18,11 @@ struct DevAux {
     uint8_t  qpc[2];          // QP'C of Cb / Cr for dequantisation
     uint8_t  qdb[3];          // deblocking qP: luma, Cb edges, Cr edges
     uint8_t  pad0;
-    uint32_t ires;            // intra MBs: index of the MB's residual in DevPic::ires
+    uint32_t ires;            // bits 0..21: intra MBs, index of the MB's residual in
+                              // DevPic::ires; bits 22..31: number of non-zero level values
 };
+#define AUX_IRES_MASK    0x3FFFFFu
+#define AUX_NVALS_SHIFT  22
 static_assert(sizeof(DevAux) == 28, "DevAux layout");
 
 #define AUX_INTRA   0x01u  // Intra_4x4 / Intra_8x8 / Intra_16x16 prediction (bS 3/4)

@@ -205,28 +205,29 @@ __device__ __forceinline__ bool build_residual(const h264g_mb *mb, const DevAux 
     uint4 *cf = (uint4 *)&ws->coef[0][0];
     for (int k = lane; k < 27 * 16 / 4; k += WAVE) cf[k] = make_uint4(0u, 0u, 0u, 0u);
     const int16_t *lv = levels + mb->level_offset;
-    const int nmask = ax.nmask;  // <= 27 masks of 16 bits: at most 432 values
+    // <= 27 masks of 16 bits, so at most 432 values; the host counted them, so the masks
+    // and the values load together (coalesced, all in flight at once)
+    const int nmask = ax.nmask, total = (int)(ax.ires >> AUX_NVALS_SHIFT);
     int mask = lane < nmask ? (uint16_t)lv[lane] : 0;
-    const int cnt = __popc(mask);
-    // exclusive prefix sum of cnt (0..16) over the lanes, bit plane by bit plane (ballot +
-    // mbcnt, no cross-lane shuffles), and the total
-    int pre = 0, total = 0;
-#pragma unroll
-    for (int bit = 0; bit < 5; bit++) {
-        const uint64_t bl = __builtin_amdgcn_ballot_w64((cnt >> bit) & 1);
-        pre += (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(bl >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bl, 0u)) << bit;
-        total += __popcll(bl) << bit;
-    }
-    // the values, with coalesced loads all in flight at once, into LDS (t8 is free until the
-    // 8x8 transform's first pass, after the scatter); the scatter then reads LDS
-    int16_t *stage = (int16_t *)&ws->t8[0][0];
-    static_assert(sizeof(ws->t8) >= 7 * WAVE * sizeof(int16_t), "value staging fits t8");
     int16_t v[7];
 #pragma unroll
     for (int t = 0; t < 7; t++) {
         const int k = lane + t * WAVE;
         v[t] = k < total ? lv[nmask + k] : (int16_t)0;
     }
+    const int cnt = __popc(mask);
+    // exclusive prefix sum of cnt (0..16) over the lanes, bit plane by bit plane (ballot +
+    // mbcnt, no cross-lane shuffles)
+    int pre = 0;
+#pragma unroll
+    for (int bit = 0; bit < 5; bit++) {
+        const uint64_t bl = __builtin_amdgcn_ballot_w64((cnt >> bit) & 1);
+        pre += (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(bl >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bl, 0u)) << bit;
+    }
+    // the values into LDS (t8 is free until the 8x8 transform's first pass, after the
+    // scatter); the scatter then reads LDS
+    int16_t *stage = (int16_t *)&ws->t8[0][0];
+    static_assert(sizeof(ws->t8) >= 7 * WAVE * sizeof(int16_t), "value staging fits t8");
 #pragma unroll
     for (int t = 0; t < 7; t++)
         if (lane + t * WAVE < total) stage[lane + t * WAVE] = v[t];
@@ -871,7 +872,7 @@ __device__ __forceinline__ void intra_res_build(const DevPic &P, int addr, WaveR
     const DevAux &ax = P.aux[addr];
     if (!build_residual(&P.mbs[addr], ax, P.levels, ws)) return;
     const int lane = wave_lane();
-    if (lane < 48) ((uint4 *)(P.ires + (size_t)ax.ires * 384))[lane] = ((const uint4 *)ws->rl)[lane];
+    if (lane < 48) ((uint4 *)(P.ires + (size_t)(ax.ires & AUX_IRES_MASK) * 384))[lane] = ((const uint4 *)ws->rl)[lane];
 }
 struct IntraRes {
     uint4 v;
@@ -882,7 +883,7 @@ __device__ __forceinline__ IntraRes intra_res_fetch(const DevPic &P, int addr) {
     IntraRes r;
     r.res = (ax.kind & AUX_RESID) != 0;
     r.v = make_uint4(0, 0, 0, 0);
-    if (r.res && wave_lane() < 48) r.v = ((const uint4 *)(P.ires + (size_t)ax.ires * 384))[wave_lane()];
+    if (r.res && wave_lane() < 48) r.v = ((const uint4 *)(P.ires + (size_t)(ax.ires & AUX_IRES_MASK) * 384))[wave_lane()];
     return r;
 }
 __device__ __forceinline__ void intra_res_place(const IntraRes &r, WaveIntra *wi) {
