@@ -100,3 +100,43 @@ def test_pictures_from_derived_mvs_match_reference(golden, name):
             assert plane_hashes(*eng.readback(sid, desc.dst_slot)) == golden[name]["pic"][i], (name, i)
             eng.release(p)
         eng.close_stream(sid)
+
+
+def _mvd_stream(tmp_path, name):
+    from h264_video_decoder_demo_amd.syntax_io import SyntaxFile, save_stream
+    z = np.load(os.path.join(GOLDEN, f"mvd_{name}.npz"), allow_pickle=False)
+    g = Generator(int(z["config"]), **json.loads(str(z["overrides"])))
+    path = str(tmp_path / f"{name}.h264g.npz")
+    save_stream(g, path, mvd=list(z["mvd"]))
+    return g, SyntaxFile(path)
+
+
+def test_syntax_file_with_mvd_derives_the_vectors(tmp_path):
+    """A stream stored with mvds and no final vectors reads back with the generator's
+    vectors (CPU)."""
+    g, f = _mvd_stream(tmp_path, "high_small_seed7")
+    for i in range(g.num_pictures):
+        _, want, _ = g.picture(i)
+        _, got, _ = f.picture(i)
+        used = np.repeat(np.repeat(want["ref_idx"].reshape(-1, 2, 2, 2), 2, axis=2), 2, axis=3).reshape(-1, 2, 16) >= 0
+        assert (got["mv"][used] == want["mv"][used]).all(), i
+
+
+@pytest.mark.gpu
+def test_decoder_on_mvd_stream_matches_reference(golden, tmp_path):
+    """The reference's decoder interface fed a stream with mvds (a parser that stops at
+    mvd) outputs the reference's pictures."""
+    from h264_video_decoder_demo_amd import H264VideoDecoder
+    from helpers import plane_hashes
+    g, _ = _mvd_stream(tmp_path, "cif_ip")
+    got = []
+
+    def on_frame(pic, user, err):
+        if pic is not None:
+            got.append(plane_hashes(pic.m_pic_buff_luma, pic.m_pic_buff_cb, pic.m_pic_buff_cr))
+        return 0
+
+    dec = H264VideoDecoder(device=0)
+    dec.set_output_frame_callback_functuin(on_frame, None)
+    assert dec.open(str(tmp_path / "cif_ip.h264g.npz")) == 0
+    assert sorted(got) == sorted(golden["cif_ip"]["pic"])
