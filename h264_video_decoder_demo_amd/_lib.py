@@ -132,6 +132,7 @@ _SIGNATURES = {
     "h264g_picture_alg_bytes": (c_int32, [c_void_p, POINTER(c_uint64)]),
     "h264g_calibration_copy": (c_int32, [c_void_p, c_uint64, c_int32]),
     "h264g_last_error": (c_char_p, []),
+    "h264g_device_describe": (c_int32, [c_int32, c_char_p, c_uint32]),
     "h264g_gen_default_params": (c_int32, [c_int32, POINTER(GenParams)]),
     "h264g_gen_create": (c_int32, [POINTER(GenParams), POINTER(c_void_p)]),
     "h264g_gen_destroy": (c_int32, [c_void_p]),

@@ -7,8 +7,8 @@ against band crossings through device memory; the best heights depend on the dev
 and on the mix of picture sizes that share a launch.  `autotune` measures candidate
 heights on the caller's own streams -- their pictures (all, or the first n_frames),
 staged once and batched per frame index as the engine batches them -- and caches the
-winner per device architecture and stream mix (a JSON file under the user's cache
-folder).
+winner per device (architecture, name, CU count, clock), library build, engine
+settings and stream mix (a JSON file under the user's cache folder).
 
 Every height gives the same bytes; only the time differs.
 """
@@ -30,9 +30,31 @@ def cache_path() -> str:
     return os.path.join(root, "h264g", "autotune.json")
 
 
-def _key(gens, n_frames: int, deblock_rows, intra_rows) -> str:
+def _library_id() -> str:
+    """Digest of the built library: a rebuilt kernel invalidates earlier measurements."""
+    from ._lib import LIB_PATH
+    h = hashlib.sha1()
+    with open(LIB_PATH, "rb") as f:
+        for chunk in iter(lambda: f.read(1 << 20), b""):
+            h.update(chunk)
+    return h.hexdigest()[:16]
+
+
+def _device_id(device: int) -> str:
+    """Architecture, name, CU count and clock of the HIP device."""
+    import ctypes
+    from ._lib import check, lib
+    buf = ctypes.create_string_buffer(256)
+    check(lib().h264g_device_describe(device, buf, len(buf)))
+    return buf.value.decode(errors="replace")
+
+
+def _key(gens, n_frames: int, deblock_rows, intra_rows, device: int = 0, engine_kwargs: dict | None = None) -> str:
+    """Cache key: the device, the library build, the engine settings that change the
+    schedules, the stream mix and the candidates."""
     mix = sorted(collections.Counter((g.width_mbs, g.height_mbs) for g in gens).items())
-    blob = json.dumps(["gfx950", mix, n_frames, list(deblock_rows), list(intra_rows)])
+    kw = sorted((k, v) for k, v in (engine_kwargs or {}).items() if k not in ("deblock_waves", "intra_waves"))
+    blob = json.dumps([_device_id(device), _library_id(), kw, mix, n_frames, list(deblock_rows), list(intra_rows)])
     return hashlib.sha1(blob.encode()).hexdigest()[:16]
 
 
@@ -51,7 +73,7 @@ def autotune(gens, n_frames: int = 0, reps: int = 2, deblock_rows=DEBLOCK_ROWS, 
     n = min(g.num_pictures for g in gens)
     if n_frames > 0:
         n = min(n, n_frames)
-    key = _key(gens, n, deblock_rows, intra_rows)
+    key = _key(gens, n, deblock_rows, intra_rows, device, engine_kwargs)
     path = cache_path()
     if use_cache and os.path.exists(path):
         try:

@@ -53,6 +53,9 @@ struct Stream {
     int strip_dbk_rows = 0, strip_intra_rows = 0;  // at least these many rows
     uint64_t bytes = 0;
     int lane = 0;                  // HIP stream (lane) of the context that runs its pictures
+    uint64_t gen = 0;              // open count of the context when this stream was opened:
+                                   // pictures and batches staged for an earlier stream with the
+                                   // same id carry another value and are refused
     bool open = false;
 };
 
@@ -87,6 +90,8 @@ struct DbkRec {
 struct h264g_pic {
     h264g_ctx *ctx = nullptr;
     uint32_t stream = 0;
+    uint64_t stream_gen = 0;       // Stream::gen at staging: its DPB and strips are valid only
+                                   // while the stream is open with that value
     h264g_pic_desc desc;
     int W = 0, H = 0;
     void *dmem = nullptr;          // one device block for everything below (pooled)
@@ -118,6 +123,7 @@ struct Batch {
     hipGraphExec_t graph = nullptr;  // the captured stage sequence (kernel nodes only)
     hipGraphExec_t graph_dbk = nullptr;  // its deblocking, when that runs on the lane's deblocking stream
     std::vector<SlotUse> uses;       // DPB slots the pictures write and read
+    std::vector<std::pair<uint32_t, uint64_t>> streams;  // (stream id, Stream::gen) of its pictures
 };
 }  // namespace
 
@@ -149,6 +155,7 @@ struct h264g_ctx {
     int waves = 2048;
     uint64_t budget = 0, used = 0;
     std::vector<Stream> streams;
+    uint64_t opened = 0;           // streams opened so far (Stream::gen)
     // Device blocks of released pictures / batches, reusable once `ready` has completed on
     // the stream (so releasing never waits for the GPU and staging rarely allocates).
     struct PoolBlock { void *ptr; size_t size; hipEvent_t ready[4]; };  // lanes, then their deblocking streams
@@ -192,6 +199,15 @@ hipError_t sync_lane(h264g_ctx *c, int lane) {
     const hipError_t e = hipStreamSynchronize(lane_stream(c, lane));
     const hipError_t e2 = hipStreamSynchronize(c->dstream[lane]);
     return e != hipSuccess ? e : e2;
+}
+
+// A picture staged for stream `id` opened as `gen` may still be launched: the stream is
+// open and is the same one (a closed stream's DPB, progress words and strips are back in
+// the pool, and its id may belong to a newer stream).  Caller holds ctx->mu.
+int check_stream(const h264g_ctx *c, uint32_t id, uint64_t gen) {
+    if (id >= c->streams.size() || !c->streams[id].open || c->streams[id].gen != gen)
+        return h264g_set_error(H264G_E_STATE, "stream %u was closed after the picture was staged", id);
+    return 0;
 }
 
 // ------------------------------------------------------------ block pool
@@ -356,8 +372,12 @@ int partitions(const h264g_mb &m, int out[16][4]) {
     return n;
 }
 
+// Validates and stages one picture.  Its device block comes from the context's pool, or,
+// when `fixed` is given, is the caller's block of `fixed_cap` bytes (pictures that need
+// more are refused).
 int stage_picture(h264g_ctx *ctx, Stream &st, const h264g_pic_desc &d, const h264g_mb *mbs, uint32_t n_mbs,
-                  const int16_t *levels, uint64_t n_levels, h264g_pic *pic) {
+                  const int16_t *levels, uint64_t n_levels, h264g_pic *pic, void *fixed = nullptr,
+                  size_t fixed_cap = 0) {
     const int W = st.W, H = st.H;
     if (d.structure != H264G_FRAME || d.mbaff) return h264g_set_error(H264G_E_ARG, "only progressive frame pictures are supported");
     if (d.slice_type < 0 || d.slice_type > 2) return h264g_set_error(H264G_E_ARG, "slice_type %d", d.slice_type);
@@ -591,14 +611,18 @@ int stage_picture(h264g_ctx *ctx, Stream &st, const h264g_pic_desc &d, const h26
     size_t o_bs = al(o_it + n_items * sizeof(uint32_t)), o_dbq = al(o_bs + (size_t)n_mbs * 32);
     size_t o_ires = al(o_dbq + (size_t)n_mbs * 4), o_pic = al(o_ires + (size_t)pic->n_intra * 768);
     size_t total = al(o_pic + sizeof(DevPic));
-    void *dm = nullptr;
-    {
+    void *dm = fixed;
+    if (fixed) {
+        if (total > fixed_cap)
+            return h264g_set_error(H264G_E_ARG, "picture needs %zu bytes of device staging, the fixed block holds %zu",
+                                   total, fixed_cap);
+    } else {
         std::lock_guard<std::mutex> lk(ctx->mu);  // the rest of the staging runs unlocked
         dm = pool_get(ctx, total);
+        if (!dm) return h264g_set_error(H264G_E_NOMEM, "picture device memory (%zu bytes)", total);
+        pic->dmem = dm;
+        pic->dbytes = total;
     }
-    if (!dm) return h264g_set_error(H264G_E_NOMEM, "picture device memory (%zu bytes)", total);
-    pic->dmem = dm;
-    pic->dbytes = total;
     char *base = (char *)dm;
     pic->d_mbs = (h264g_mb *)(base + o_mbs);
     pic->d_aux = (DevAux *)(base + o_aux);
@@ -797,6 +821,17 @@ extern "C" {
 
 const char *h264g_last_error(void) { return g_err; }
 
+int32_t h264g_device_describe(int32_t device, char *out, uint32_t len) {
+    if (!out || len == 0) return h264g_set_error(H264G_E_ARG, "NULL argument");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev)
+        return h264g_set_error(H264G_E_NODEV, "device %d", device);
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, device));
+    snprintf(out, len, "%s|%s|%d CUs|%d kHz", prop.gcnArchName, prop.name, prop.multiProcessorCount, prop.clockRate);
+    return H264G_OK;
+}
+
 // Progress words start at 0, so the first wavefront launch uses epoch 1.  Intra and
 // deblocking launches count their epochs separately (they may run at the same time on a
 // lane and its deblocking stream); the deblocking count starts half the range away, so
@@ -850,6 +885,13 @@ int32_t h264g_create(const h264g_config *cfg, h264g_ctx **out) {
         h264g_destroy(c);
         return h264g_set_error(H264G_E_DEVICE, "context allocation failed");
     }
+    // events of the deblocking history, so that enqueuing creates none
+    for (auto &lane : c->hist)
+        for (auto &r : lane)
+            if (hipEventCreateWithFlags(&r.ev, hipEventDisableTiming) != hipSuccess) {
+                h264g_destroy(c);
+                return h264g_set_error(H264G_E_DEVICE, "context events");
+            }
     *out = c;
     return H264G_OK;
 }
@@ -934,6 +976,7 @@ int32_t h264g_stream_open(h264g_ctx *c, const h264g_stream_desc *d, uint32_t *id
     s.strip_intra_rows = c->intra_waves;
     s.bytes = need;
     s.open = true;
+    s.gen = ++c->opened;
     c->used += need;
     uint32_t i = 0;
     for (; i < c->streams.size(); i++) if (!c->streams[i].open) break;
@@ -993,6 +1036,7 @@ int32_t h264g_picture_begin(h264g_ctx *c, uint32_t sid, const h264g_pic_desc *d,
     if (!p) return h264g_set_error(H264G_E_NOMEM, "pic");
     p->ctx = c;
     p->stream = sid;
+    p->stream_gen = st.gen;
     p->desc = *d;
     p->W = st.W;
     p->H = st.H;
@@ -1007,10 +1051,9 @@ int32_t h264g_picture_begin(h264g_ctx *c, uint32_t sid, const h264g_pic_desc *d,
     return H264G_OK;
 }
 
-int32_t h264g_picture_submit(h264g_ctx *c, h264g_pic *p) {
-    if (!c || !p || p->ctx != c) return h264g_set_error(H264G_E_ARG, "bad picture");
-    std::lock_guard<std::mutex> lk(c->mu);
-    HIPCHK(hipSetDevice(c->device));
+namespace {
+// The launches of one staged picture (caller holds ctx->mu).
+int enqueue_picture(h264g_ctx *c, const h264g_pic *p) {
     const uint32_t *it = p->d_items;
     StageSet q;
     q.pics = p->d_pic;
@@ -1028,6 +1071,15 @@ int32_t h264g_picture_submit(h264g_ctx *c, h264g_pic *p) {
     q.n_dbands = (uint32_t)((p->H + c->dbk_waves - 1) / c->dbk_waves);
     q.dbk_off = p->desc.disable_deblocking_filter_idc == 1;
     return enqueue(c, q, c->streams[p->stream].lane, {slot_use(p->stream, p->desc)}, nullptr);
+}
+}  // namespace
+
+int32_t h264g_picture_submit(h264g_ctx *c, h264g_pic *p) {
+    if (!c || !p || p->ctx != c) return h264g_set_error(H264G_E_ARG, "bad picture");
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (int rc = check_stream(c, p->stream, p->stream_gen)) return rc;
+    HIPCHK(hipSetDevice(c->device));
+    return enqueue_picture(c, p);
 }
 
 int32_t h264g_picture_release(h264g_ctx *c, h264g_pic *p) {
@@ -1054,10 +1106,12 @@ int build_sub(h264g_ctx *c, const std::vector<const h264g_pic *> &pics, Batch *b
     std::vector<uint32_t> inter, intra, dbk;
     int maxt = 0;
     b->uses.clear();
+    b->streams.clear();
     for (uint32_t i = 0; i < n; i++) {
         const h264g_pic *p = pics[i];
         hp[i] = p->hpic;
         b->uses.push_back(slot_use(p->stream, p->desc));
+        b->streams.push_back({p->stream, p->stream_gen});
         for (uint32_t a : p->inter_list) inter.push_back((i << WI_PIC_SHIFT) | a);
         maxt = std::max(maxt, (p->W - 1) + 2 * (p->H - 1));
     }
@@ -1146,6 +1200,7 @@ int32_t h264g_batch_create(h264g_ctx *c, h264g_pic *const *pics, uint32_t n, h26
     for (uint32_t i = 0; i < n; i++) {
         const h264g_pic *p = pics[i];
         if (!p || p->ctx != c) return h264g_set_error(H264G_E_ARG, "picture %u", i);
+        if (int rc = check_stream(c, p->stream, p->stream_gen)) return rc;
         if (std::find(seen.begin(), seen.end(), p->stream) != seen.end())
             return h264g_set_error(H264G_E_ARG, "two pictures of stream %u in one batch", p->stream);
         seen.push_back(p->stream);
@@ -1195,6 +1250,9 @@ int32_t h264g_batch_run(h264g_ctx *c, h264g_batch *b) {
     if (b->dbk_rows != c->dbk_waves || b->intra_rows != c->intra_waves)
         return h264g_set_error(H264G_E_STATE, "batch built for bands of %d / %d rows, the context now uses %d / %d: "
                                "create it again", b->dbk_rows, b->intra_rows, c->dbk_waves, c->intra_waves);
+    for (const Batch &sb : b->sub)  // every stream before any launch
+        for (const auto &sg : sb.streams)
+            if (int rc = check_stream(c, sg.first, sg.second)) return rc;
     HIPCHK(hipSetDevice(c->device));
     for (int l = 0; l < 2; l++) {
         Batch &sb = b->sub[l];

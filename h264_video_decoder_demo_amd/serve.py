@@ -48,6 +48,12 @@ class ServeResult:
     latencies_ms: list
     admitted_late: int      # streams that waited for DPB budget
     max_batch: int
+    arrival_span_s: float = 0.0  # first to last frame arrival
+
+    @property
+    def offered_fps(self) -> float:
+        """Frames that arrived over the span of their arrival times."""
+        return self.frames / self.arrival_span_s if self.arrival_span_s > 0 else 0.0
 
     @property
     def mpix_per_s(self) -> float:
@@ -59,15 +65,17 @@ class ServeResult:
 
 
 def make_specs(n_streams: int, total_fps: float, duration_s: float, seed: int = 0x5EED0005, gop: int = 32):
-    """Streams of the serving mix: resolution by the mix shares, start times uniform in
-    the first half of the run, frames arriving at total_fps / n_streams each."""
+    """Streams of the serving mix: resolution by the mix shares, frames arriving at
+    total_fps / n_streams each for the whole run.  Start times are uniform within one
+    mean frame gap of a stream (1 / rate), so every stream is active from then on and the
+    offered load is total_fps over the run instead of ramping up."""
     rng = np.random.default_rng(seed)
     shares = np.array([m[0] for m in SERVE_MIX])
     specs = []
     rate = total_fps / n_streams
     for i in range(n_streams):
         k = int(rng.choice(len(SERVE_MIX), p=shares / shares.sum()))
-        start = float(rng.uniform(0, duration_s / 2))
+        start = float(rng.uniform(0, min(1.0 / rate, duration_s)))
         n = max(1, int(math.ceil(rate * (duration_s - start))))
         specs.append(StreamSpec(SERVE_MIX[k][1], SERVE_MIX[k][2], seed + 1 + i, n, start, rate))
     del gop
@@ -96,13 +104,16 @@ class ServeLoop:
     on_frame(stream_index, frame_index, engine, stream_id, desc), if given, is called
     after the frame's batch has completed and before the stream's next frame is
     submitted (the loop then keeps one frame per stream in flight, so the frame's DPB
-    slot can be read back)."""
+    slot can be read back).  on_close(stream_index, engine, stream_id), if given, is
+    called after a stream's last frame has completed, before the stream is closed (its
+    DPB can be read back then)."""
 
     def __init__(self, engine: Engine, specs, gop: int = 32, window_ms: float = 2.0, seed: int = 1,
-                 on_frame=None, gens=None):
+                 on_frame=None, gens=None, on_close=None):
         self.eng = engine
         self.window = window_ms / 1e3
         self.on_frame = on_frame
+        self.on_close = on_close
         rng = np.random.default_rng(seed)
         self.streams = []
         for i, sp in enumerate(specs):
@@ -157,6 +168,8 @@ class ServeLoop:
                     if self.on_frame is not None:
                         self.on_frame(s.idx, k, eng, s.sid, desc)
                     if s.done == s.gen.num_pictures:
+                        if self.on_close is not None:
+                            self.on_close(s.idx, eng, s.sid)
                         eng.close_stream(s.sid)
                         s.sid = None
             # a batch window
@@ -182,4 +195,6 @@ class ServeLoop:
                 time.sleep(min(2e-4, max(0.0, next_window - now)))
         eng.sync()
         stagers.shutdown()
-        return ServeResult(frames, mpix, time.perf_counter() - t0, lat, admitted_late, max_batch)
+        arr = [float(s.arrivals[k]) for s in self.streams for k in range(s.gen.num_pictures)]
+        return ServeResult(frames, mpix, time.perf_counter() - t0, lat, admitted_late, max_batch,
+                           max(arr) - min(arr) if arr else 0.0)

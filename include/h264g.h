@@ -234,6 +234,12 @@ int32_t h264g_destroy(h264g_ctx *ctx);
  * would be exceeded (serving admission).  Replaces the per-frame my_malloc of
  * CH264PictureBase::init (H264PictureBase.cpp:160-182). */
 int32_t h264g_stream_open(h264g_ctx *ctx, const h264g_stream_desc *desc, uint32_t *stream_id);
+/* Returns the stream's device memory to the context (reused once the work enqueued so
+ * far has completed; the call does not wait).  Pictures staged for the stream, and
+ * batches holding them, can then no longer be launched: h264g_picture_submit,
+ * h264g_batch_create / _submit and h264g_batch_run return H264G_E_STATE before any
+ * launch, also when the id has been given to a newly opened stream.  Release and
+ * destroy them as usual. */
 int32_t h264g_stream_close(h264g_ctx *ctx, uint32_t stream_id);
 
 /* Band heights (MB rows per band) of the BANDS schedules from now on, for the
@@ -329,6 +335,10 @@ int32_t h264g_calibration_copy(h264g_ctx *ctx, uint64_t bytes, int32_t reps);
 
 /* Last error message of the calling thread (static storage). */
 const char *h264g_last_error(void);
+
+/* "<gcnArchName>|<device name>|<N> CUs|<clock> kHz" of a HIP device, for caches of
+ * measured choices (the band-height autotuner).  No counterpart in the reference. */
+int32_t h264g_device_describe(int32_t device, char *out, uint32_t len);
 
 /* ----------------------------------------------------- syntax generator */
 /* Deterministic synthetic syntax (configs of the benchmark).  Host-only. */

@@ -9,7 +9,8 @@ Cb and Cr planes, before and after deblocking, and of the BGR24 image the
 reference's convertYuv420pToBgr24 makes of it.  The reference has no tests or
 golden vectors of its own for this path; these are the pins.
 
-Run in a container that has /root/reference:  python tests/golden/make_golden.py
+Run in a container that has /root/reference:  python tests/golden/make_golden.py [case ...]
+(with case names, only those cases are recomputed and the others are kept).
 """
 from __future__ import annotations
 
@@ -48,6 +49,10 @@ CASES = {
     "flat": (2, {"width_mbs": 14, "height_mbs": 1, "max_refs": 2}),
     # the full 1920x1088 High-profile workload (8 pictures)
     "high_1080p": (2, {}),
+    # the other full picture sizes of the multi-stream and serving workloads, config-2
+    # syntax, I P B B
+    "high_720p": (2, {"width_mbs": 80, "height_mbs": 45, "n_frames": 4, "seed": 0x5EED0004 + 16}),
+    "high_2160p": (2, {"width_mbs": 240, "height_mbs": 135, "n_frames": 4, "seed": 0x5EED0004 + 56}),
 }
 
 _CLI = {"n_frames": "--frames", "width_mbs": "--width", "height_mbs": "--height", "seed": "--seed",
@@ -105,11 +110,18 @@ def main():
     if not os.path.exists(HARNESS):
         subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "ref", "-j8"], check=True)
     sys.path.insert(0, ROOT)
+    path = os.path.join(HERE, "golden.json")
+    only = sys.argv[1:]
     golden = {}
+    if only:
+        with open(path) as f:
+            golden = json.load(f)
     for name, (config, ov) in CASES.items():
+        if only and name not in only:
+            continue
         golden[name] = {"config": config, "overrides": ov, **run_case(config, ov)}
         print(name, golden[name]["digest"], golden[name]["pictures"], flush=True)
-    with open(os.path.join(HERE, "golden.json"), "w") as f:
+    with open(path, "w") as f:
         json.dump(golden, f, indent=1, sort_keys=True)
 
 

@@ -133,9 +133,28 @@ def test_autotune_search_on_cpu(monkeypatch, tmp_path):
     _FakeEngine.cost = {(d, i): abs(d - 10) + abs(i - 6) * 0.5 + 1 for d in at.DEBLOCK_ROWS for i in at.INTRA_ROWS}
     monkeypatch.setattr(engine_mod, "Engine", _FakeEngine)
     monkeypatch.setattr(at, "_clock", lambda: clock[0])
+    monkeypatch.setattr(at, "_device_id", lambda device: "gfx950|test device|256 CUs|2400000 kHz")
     monkeypatch.setenv("XDG_CACHE_HOME", str(tmp_path))
     r = at.autotune([_Gen(), _Gen()], n_frames=2, reps=2)
     assert (r["deblock_waves"], r["intra_waves"]) == (10, 6)
     assert len(r["ms"]) == len(at.DEBLOCK_ROWS) + len(at.INTRA_ROWS) - 1
     again = at.autotune([_Gen(), _Gen()], n_frames=2, reps=2)
     assert again["cached"] and (again["deblock_waves"], again["intra_waves"]) == (10, 6)
+
+
+def test_autotune_cache_key_names_device_build_and_settings(monkeypatch):
+    """A choice measured on another device, another library build or with other engine
+    settings is not reused."""
+    dev = ["gfx950|a|256 CUs|2400000 kHz"]
+    lib_id = ["0123"]
+    monkeypatch.setattr(at, "_device_id", lambda device: dev[0])
+    monkeypatch.setattr(at, "_library_id", lambda: lib_id[0])
+    gens = [_Gen(), _Gen()]
+    k0 = at._key(gens, 2, at.DEBLOCK_ROWS, at.INTRA_ROWS, 0, {"graphs": True})
+    assert k0 == at._key(gens, 2, at.DEBLOCK_ROWS, at.INTRA_ROWS, 0, {"graphs": True, "deblock_waves": 4})
+    assert k0 != at._key(gens, 2, at.DEBLOCK_ROWS, at.INTRA_ROWS, 0, {"graphs": False})
+    dev[0] = "gfx950|a|240 CUs|2400000 kHz"
+    assert k0 != at._key(gens, 2, at.DEBLOCK_ROWS, at.INTRA_ROWS, 0, {"graphs": True})
+    dev[0] = "gfx950|a|256 CUs|2400000 kHz"
+    lib_id[0] = "4567"
+    assert k0 != at._key(gens, 2, at.DEBLOCK_ROWS, at.INTRA_ROWS, 0, {"graphs": True})

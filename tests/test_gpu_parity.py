@@ -18,7 +18,7 @@ from helpers import first_diff, make_generator, oracle_decode, plane_hashes
 pytestmark = pytest.mark.gpu
 
 CASES = ["cif_ip", "cif_ip_seed2", "cif_ip_seed3", "cif_pcm_cqp", "high_small", "high_small_seed7",
-         "high_implicit", "high_default_nodbk", "narrow", "flat", "high_1080p"]
+         "high_implicit", "high_default_nodbk", "narrow", "flat", "high_1080p", "high_720p", "high_2160p"]
 
 
 def _diagnose(engine, sid, name, gen, i, desc, mbs, levels, got):
@@ -71,7 +71,8 @@ def test_batched_path_matches_reference(any_engine, golden):
     """Pictures of several streams of different sizes reconstructed in one batch per
     picture index give the same bytes as the reference."""
     engine = any_engine
-    names = ["cif_ip_seed2", "high_small", "high_implicit", "high_default_nodbk", "narrow", "flat", "cif_pcm_cqp"]
+    names = ["cif_ip_seed2", "high_small", "high_implicit", "high_default_nodbk", "narrow", "flat", "cif_pcm_cqp",
+             "high_720p", "high_2160p"]
     gens = {n: make_generator(golden[n]) for n in names}
     sids = {n: engine.open_stream(g.width_mbs, g.height_mbs, g.dpb_slots) for n, g in gens.items()}
     try:

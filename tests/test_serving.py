@@ -1,12 +1,13 @@
 """The serving loop: every frame decoded in the mix (continuous batching, DPB budget
-admission) is byte-identical to the same stream decoded alone."""
+admission) is byte-identical to the same stream decoded alone by the C oracle, which
+reproduces the reference decoder's pictures on the golden cases."""
 import numpy as np
 import pytest
 
 from h264_video_decoder_demo_amd import Engine, Generator
 from h264_video_decoder_demo_amd.serve import ServeLoop, StreamSpec, arrival_times, make_specs
 
-from helpers import plane_hashes
+from helpers import oracle_decode, plane_hashes
 
 
 def test_specs_follow_the_mix_and_rate():
@@ -32,29 +33,30 @@ def _specs():
     return [StreamSpec(w, h, 0x77 + i, 6, 0.002 * i, 400.0) for i, (w, h) in enumerate(sizes)]
 
 
-def _alone(specs):
+def _gen(sp, gop=32):
+    return Generator(2, width_mbs=sp.width_mbs, height_mbs=sp.height_mbs, n_frames=sp.n_frames, seed=sp.seed, gop=gop)
+
+
+def _alone(specs, gop=32):
+    """Every picture of every stream decoded alone, in decoding order, by the oracle."""
     out = {}
-    with Engine(device=0) as eng:
-        for i, sp in enumerate(specs):
-            g = Generator(2, width_mbs=sp.width_mbs, height_mbs=sp.height_mbs, n_frames=sp.n_frames, seed=sp.seed,
-                          gop=32)
-            sid = eng.open_stream(g.width_mbs, g.height_mbs, g.dpb_slots)
-            for k in range(g.num_pictures):
-                d, m, lv = g.picture(k)
-                p = eng.begin(sid, d, m, lv)
-                eng.submit(p)
-                eng.sync()
-                out[(i, k)] = plane_hashes(*eng.readback(sid, d.dst_slot))
-                eng.release(p)
-            eng.close_stream(sid)
+    for i, sp in enumerate(specs):
+        for k, _, planes in oracle_decode(_gen(sp, gop)):
+            out[(i, k)] = plane_hashes(*planes[2])
     return out
+
+
+def test_serving_streams_cross_a_gop_boundary():
+    """The oracle comparison covers an IDR after the first GOP: GOP 4 over 6 frames."""
+    g = _gen(StreamSpec(7, 5, 0x99, 6, 0.0, 1.0), gop=4)
+    assert [g.is_idr(k) for k in range(g.num_pictures)].count(True) >= 2
 
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("budget_streams", [0, 2])
 def test_serving_mix_equals_alone(budget_streams):
     specs = _specs()
-    want = _alone(specs)
+    want = _alone(specs, gop=4)
     got = {}
 
     def on_frame(i, k, eng, sid, desc):
@@ -67,7 +69,7 @@ def test_serving_mix_equals_alone(budget_streams):
         budget = 2 * (22 * 18 * 384 * 5 + 22 * 18 * 4 + 18 * 8 + 3 * 22 * 256 + 3 * 22 * 64) + 1024
     # streams of 396 and 600 MBs run on the context's second HIP stream
     with Engine(device=0, dpb_budget_bytes=budget, lane_split_mbs=300) as eng:
-        res = ServeLoop(eng, specs, window_ms=1.0, on_frame=on_frame).run()
+        res = ServeLoop(eng, specs, gop=4, window_ms=1.0, on_frame=on_frame).run()
     assert res.frames == sum(s.n_frames for s in specs)
     assert got == want
     assert len(res.latencies_ms) == res.frames
