@@ -114,6 +114,8 @@ _SIGNATURES = {
     "h264g_picture_begin": (c_int32, [c_void_p, c_uint32, POINTER(PicDesc), c_void_p, c_uint32, c_void_p, c_uint64,
                                       POINTER(c_void_p)]),
     "h264g_picture_submit": (c_int32, [c_void_p, c_void_p]),
+    "h264g_picture_decode_static": (c_int32, [c_void_p, c_uint32, POINTER(PicDesc), c_void_p, c_uint32, c_void_p,
+                                              c_uint64]),
     "h264g_batch_submit": (c_int32, [c_void_p, POINTER(c_void_p), c_uint32]),
     "h264g_batch_create": (c_int32, [c_void_p, POINTER(c_void_p), c_uint32, POINTER(c_void_p)]),
     "h264g_batch_run": (c_int32, [c_void_p, c_void_p]),

@@ -745,7 +745,8 @@ bool conflicts(const DbkRec &r, const std::vector<SlotUse> &uses) {
 // stream after them, with the band schedule in a two-lane context (the other schedules
 // share per-MB progress words with the intra stage, and a one-lane context is the
 // strictly sequential shape: there deblocking stays on the lane).
-int enqueue(h264g_ctx *c, const StageSet &q, int lane, const std::vector<SlotUse> &uses, Batch *sb) {
+int enqueue(h264g_ctx *c, const StageSet &q, int lane, const std::vector<SlotUse> &uses, Batch *sb,
+            bool events = true) {
     hipStream_t s = lane_stream(c, lane), ds = c->dstream[lane];
     for (int k = 1; k <= kDbkHist; k++) {
         const DbkRec &r = c->hist[lane][(c->hist_next[lane] - k + kDbkHist) % kDbkHist];
@@ -756,7 +757,7 @@ int enqueue(h264g_ctx *c, const StageSet &q, int lane, const std::vector<SlotUse
     }
     // one-lane contexts (lane_split_mbs < 0) keep the strictly sequential shape
     const bool split = c->dbk_schedule == H264G_DEBLOCK_BANDS && !q.dbk_off && c->split_mbs > 0;
-    hipEvent_t *ev = profile_events(c);
+    hipEvent_t *ev = events ? profile_events(c) : nullptr;
     const bool graphs = sb && !c->profile && !c->no_graphs;
     // capture of the launches `parts` on stream st into *g (kernel nodes only; every launch
     // argument, including the wavefront epochs, which live in device memory, is the same
@@ -1052,8 +1053,9 @@ int32_t h264g_picture_begin(h264g_ctx *c, uint32_t sid, const h264g_pic_desc *d,
 }
 
 namespace {
-// The launches of one staged picture (caller holds ctx->mu).
-int enqueue_picture(h264g_ctx *c, const h264g_pic *p) {
+// The launches of one staged picture (caller holds ctx->mu).  events: profiling contexts
+// record stage events (created on demand).
+int enqueue_picture(h264g_ctx *c, const h264g_pic *p, bool events = true) {
     const uint32_t *it = p->d_items;
     StageSet q;
     q.pics = p->d_pic;
@@ -1070,9 +1072,57 @@ int enqueue_picture(h264g_ctx *c, const h264g_pic *p) {
     q.n_ibands = (uint32_t)((p->H + c->intra_waves - 1) / c->intra_waves);
     q.n_dbands = (uint32_t)((p->H + c->dbk_waves - 1) / c->dbk_waves);
     q.dbk_off = p->desc.disable_deblocking_filter_idc == 1;
-    return enqueue(c, q, c->streams[p->stream].lane, {slot_use(p->stream, p->desc)}, nullptr);
+    return enqueue(c, q, c->streams[p->stream].lane, {slot_use(p->stream, p->desc)}, nullptr, events);
 }
+
+// State of the allocation-free entry point per device: the lock that serialises its calls
+// and the events recorded after a call's launches (on the lane, then on its deblocking
+// stream), which the next call on the device waits for before it overwrites the static
+// scratch array.  The events are created by the first call on the device.
+constexpr int kStaticDevices = 64;
+struct StaticState {
+    std::mutex mu;
+    hipEvent_t done[2] = {nullptr, nullptr};
+};
+StaticState g_static[kStaticDevices];
 }  // namespace
+
+int32_t h264g_picture_decode_static(h264g_ctx *c, uint32_t sid, const h264g_pic_desc *d, const h264g_mb *mbs,
+                                    uint32_t n_mbs, const int16_t *levels, uint64_t n_levels) {
+    if (!c || !d || !mbs || (!levels && n_levels)) return h264g_set_error(H264G_E_ARG, "NULL argument");
+    if (c->device >= kStaticDevices) return h264g_set_error(H264G_E_NODEV, "device %d", c->device);
+    Stream st;
+    {
+        std::lock_guard<std::mutex> lk(c->mu);
+        if (sid >= c->streams.size() || !c->streams[sid].open) return h264g_set_error(H264G_E_ARG, "stream %u", sid);
+        st = c->streams[sid];
+    }
+    HIPCHK(hipSetDevice(c->device));
+    StaticState &ss = g_static[c->device];
+    std::lock_guard<std::mutex> lk(ss.mu);
+    void *base = nullptr;
+    if (int e = h264g_scratch_address(c->device, &base))
+        return h264g_set_error(H264G_E_DEVICE, "static scratch address: %s", hipGetErrorString((hipError_t)e));
+    for (hipEvent_t &ev : ss.done) {
+        if (!ev) HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        else HIPCHK(hipEventSynchronize(ev));  // the previous call's launches have read the array
+    }
+    h264g_pic p;
+    p.ctx = c;
+    p.stream = sid;
+    p.stream_gen = st.gen;
+    p.desc = *d;
+    p.W = st.W;
+    p.H = st.H;
+    if (int rc = stage_picture(c, st, *d, mbs, n_mbs, levels, n_levels, &p, base, H264G_STATIC_SCRATCH_BYTES)) return rc;
+    std::lock_guard<std::mutex> lk2(c->mu);
+    if (int rc = check_stream(c, sid, st.gen)) return rc;
+    if (int rc = enqueue_picture(c, &p, false)) return rc;
+    const int lane = c->streams[sid].lane;
+    HIPCHK(hipEventRecord(ss.done[0], lane_stream(c, lane)));
+    HIPCHK(hipEventRecord(ss.done[1], c->dstream[lane]));
+    return H264G_OK;
+}
 
 int32_t h264g_picture_submit(h264g_ctx *c, h264g_pic *p) {
     if (!c || !p || p->ctx != c) return h264g_set_error(H264G_E_ARG, "bad picture");

@@ -109,3 +109,7 @@ int h264g_launch_bs(const DevPic *pics, const uint32_t *items, uint32_t n, void 
 int h264g_launch_bgr24(const uint8_t *yuv, int wl, int hl, uint8_t *bgr, uint32_t stride, int flip, void *stream);
 int h264g_launch_copy(const uint8_t *src, uint8_t *dst, uint64_t bytes, void *stream);
 }
+// Device address of the static scratch array (H264G_STATIC_SCRATCH_BYTES) on `device`,
+// which must be the current device; looked up on the first call per device.  Returns a
+// hipError_t.
+int h264g_scratch_address(int device, void **out);

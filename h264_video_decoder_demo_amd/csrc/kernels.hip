@@ -12,6 +12,9 @@
 //   deblock -- the same wavefront scheme over all MBs with the raster-order edge filter.
 // The queues hold MBs of any number of pictures (batching across streams).
 #include <hip/hip_runtime.h>
+
+#include <mutex>
+
 #include "h264g_dev.h"
 
 namespace {
@@ -2154,3 +2157,35 @@ int h264g_launch_copy(const uint8_t *src, uint8_t *dst, uint64_t bytes, void *st
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------ static scratch
+// Device memory of the allocation-free entry point (h264g_picture_decode_static): the
+// staged syntax of one picture and its stage buffers (records, levels, work lists,
+// boundary strengths, intra residuals).  Every device the library runs on holds its own
+// copy until the process ends.  In host code `scratch` names a host-side stand-in, so its
+// device address is looked up once per device and only that address is used.
+namespace {
+__device__ __align__(256) uint8_t scratch[H264G_STATIC_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_addr[kMaxDevices] = {};
+}  // namespace
+
+int h264g_scratch_address(int device, void **out) {
+    if (device < 0 || device >= kMaxDevices || !out) return (int)hipErrorInvalidValue;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_addr[device]) {
+        int cur = -1;
+        hipError_t e = hipGetDevice(&cur);
+        if (e != hipSuccess) return (int)e;
+        if (cur != device) return (int)hipErrorInvalidDevice;  // the caller makes `device` current
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return (int)e;
+        if (!p) return (int)hipErrorInvalidSymbol;
+        g_scratch_addr[device] = p;
+    }
+    *out = g_scratch_addr[device];
+    return 0;
+}

@@ -232,6 +232,14 @@ class Engine:
         W, H, _ = self.streams[sid]
         return StagedPicture(h, sid, desc, W, H)
 
+    def decode_static(self, sid: int, desc: PicDesc, mbs: np.ndarray, levels: np.ndarray):
+        """Stages and enqueues one picture without allocating device memory
+        (h264g_picture_decode_static: the device's static scratch array holds the staging)."""
+        mbs = np.ascontiguousarray(mbs, MB_DTYPE)
+        levels = np.ascontiguousarray(levels, np.int16)
+        check(lib().h264g_picture_decode_static(self._h, sid, byref(desc), _ptr(mbs), mbs.size, _ptr(levels),
+                                                levels.size))
+
     def submit(self, pic: StagedPicture):
         check(lib().h264g_picture_submit(self._h, pic.handle))
 

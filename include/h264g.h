@@ -263,6 +263,24 @@ int32_t h264g_picture_begin(h264g_ctx *ctx, uint32_t stream_id, const h264g_pic_
  * Replaces Deblocking_filter_process at H264PictureBase.cpp:707. */
 int32_t h264g_picture_submit(h264g_ctx *ctx, h264g_pic *pic);
 
+/* Allocation-free decode of one picture.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  The picture's staged syntax and stage buffers
+ * (records, levels, work lists, boundary strengths, intra residuals) go into a static
+ * device array of H264G_STATIC_SCRATCH_BYTES that every device the library runs on holds
+ * until the process ends, instead of the context's pooled device blocks; the launches and
+ * the result are those of h264g_picture_begin + h264g_picture_submit.  The call makes no
+ * hipMalloc / hipFree or other device allocation (the stream's DPB comes from
+ * h264g_stream_open as usual).  A picture whose staging needs more than the array holds
+ * fails with H264G_E_ARG before anything is copied (any 1920x1088 picture fits).  All
+ * calls on one device share its array: each call first waits for the launches of the
+ * previous call on that device, and concurrent calls are serialised, so a later call never
+ * overwrites what an earlier one's kernels still read.  Returns once the launches are
+ * enqueued (h264g_sync / fences wait for them). */
+#define H264G_STATIC_SCRATCH_BYTES (64u << 20)
+int32_t h264g_picture_decode_static(h264g_ctx *ctx, uint32_t stream_id, const h264g_pic_desc *desc,
+                                    const h264g_mb *mbs, uint32_t n_mbs, const int16_t *levels, uint64_t n_levels);
+
 /* Enqueue many staged pictures as one batched job: one persistent launch per
  * stage drains a device queue of macroblock tiles from all of them.  The
  * pictures must not reference each other (e.g. different streams).  No
