@@ -129,13 +129,16 @@ public:
     PictureGen(const h264g_gen_params &p, Rng &rng, h264g_pic_desc &desc, Picture &pic)
         : P(p), R(rng), D(desc), pic(pic), W(p.width_mbs), H(p.height_mbs) {
         motion.assign((size_t)W * 4 * H * 4, Motion());
-        mbclass.assign((size_t)W * H, 0);
     }
 
     void run() {
         int slice_qp = R.uni(P.qp_lo, P.qp_hi);
         qp_prev = slice_qp;
         pic.mbs.resize((size_t)W * H);
+        if (P.mbaff) {
+            run_mbaff();
+            return;
+        }
         for (int my = 0; my < H; my++)
             for (int mx = 0; mx < W; mx++) gen_mb(mx, my);
     }
@@ -147,8 +150,9 @@ private:
     Picture &pic;
     int W, H;
     int qp_prev = 26;
-    std::vector<Motion> motion;
-    std::vector<uint8_t> mbclass;  // 1 = intra
+    std::vector<Motion> motion;    // per 4x4 block: picture raster (frames), or [MB address][raster
+                                   // block] (MBAFF frames)
+    std::vector<uint8_t> fld;      // MBAFF: mb_field_decoding_flag per MB address
 
     Motion &mo(int bx, int by) { return motion[(size_t)by * W * 4 + bx]; }
     bool mb_exists(int mx, int my) const { return mx >= 0 && my >= 0 && mx < W && my < H; }
@@ -246,16 +250,25 @@ private:
     }
 
     // --- one macroblock -------------------------------------------------------
-    void gen_mb(int mx, int my) {
-        h264g_mb &mb = pic.mbs[(size_t)my * W + mx];
+    static void init_mb(h264g_mb &mb) {
         std::memset(&mb, 0, sizeof(mb));
         for (int l = 0; l < 2; l++)
             for (int q = 0; q < 4; q++) mb.ref_idx[l][q] = -1;
-        const int st = D.slice_type;
-        bool A = mb_exists(mx - 1, my), B = mb_exists(mx, my - 1);
-        bool Dn = mb_exists(mx - 1, my - 1);
+    }
 
-        // ---- choose class
+    void gen_mb(int mx, int my) {
+        h264g_mb &mb = pic.mbs[(size_t)my * W + mx];
+        init_mb(mb);
+        const bool A = mb_exists(mx - 1, my), B = mb_exists(mx, my - 1), Dn = mb_exists(mx - 1, my - 1);
+        const int cls = choose_class(mb);
+        gen_body(mb, cls, A, B, Dn, false, [&](int mv[2]) { pskip_mv(mx, my, mv); });
+        if (cls <= H264G_MB_IPCM) mark_intra(mx, my);
+        else store_motion(mx, my, mb);
+    }
+
+    // Macroblock class (and partition shape of P MBs of the config-1 mix).
+    int choose_class(h264g_mb &mb) {
+        const int st = D.slice_type;
         int cls;
         if (st == H264G_SLICE_I) {
             cls = R.permille(500) ? H264G_MB_I16x16 : (P.t8x8 && R.permille(500) ? H264G_MB_I8x8 : H264G_MB_I4x4);
@@ -277,16 +290,22 @@ private:
         }
         if (P.ipcm_permille && cls != H264G_MB_P_SKIP && R.permille(P.ipcm_permille)) cls = H264G_MB_IPCM;
         mb.mb_type = (uint8_t)cls;
-        bool intra = cls <= H264G_MB_IPCM;
-        mbclass[(size_t)my * W + mx] = intra ? 1 : 0;
+        return cls;
+    }
 
+    // Everything of an MB after its class: prediction syntax, levels, QP.  A / B / Dn:
+    // availability of the neighbouring MBs for intra prediction; field: a field MB of an
+    // MBAFF frame (its reference indices address the field list, twice as long);
+    // pskip(mv): the P_Skip motion vector.
+    template <class PSkip>
+    void gen_body(h264g_mb &mb, int cls, bool A, bool B, bool Dn, bool field, PSkip pskip) {
+        bool intra = cls <= H264G_MB_IPCM;
         LevelWriter lw;
         mb.level_offset = (uint32_t)pic.levels.size();
 
         if (cls == H264G_MB_IPCM) {
             for (int i = 0; i < 384; i++) pic.levels.push_back((int16_t)R.uni(0, 255));
             mb.qp_y = (uint8_t)qp_prev;  // mb_qp_delta not present: QPY = QPY,PREV
-            mark_intra(mx, my);
             return;
         }
 
@@ -310,14 +329,13 @@ private:
 
         if (cls == H264G_MB_P_SKIP) {
             int mv[2];
-            pskip_mv(mx, my, mv);
+            pskip(mv);
             for (int b = 0; b < 16; b++) { mb.mv[0][b][0] = (int16_t)mv[0]; mb.mv[0][b][1] = (int16_t)mv[1]; }
             for (int q = 0; q < 4; q++) mb.ref_idx[0][q] = 0;
             mb.qp_y = (uint8_t)qp_prev;
-            store_motion(mx, my, mb);
             return;
         }
-        if (cls == H264G_MB_P || cls == H264G_MB_B) gen_inter(mb);
+        if (cls == H264G_MB_P || cls == H264G_MB_B) gen_inter(mb, field);
 
         // ---- transform size for inter MBs (eligible: no sub-8x8 partitions)
         bool inter = cls == H264G_MB_P || cls == H264G_MB_B;
@@ -389,9 +407,6 @@ private:
             for (int c = 0; c < 2; c++)
                 for (int b = 0; b < 4; b++) lw.block(cac[c][b], 16, 1);
         lw.flush(pic.levels);
-
-        if (intra) mark_intra(mx, my);
-        else store_motion(mx, my, mb);
     }
 
     static int blk4x4_index(int x4, int y4) {  // luma4x4BlkIdx of the 4x4 at (x4, y4)
@@ -422,15 +437,164 @@ private:
             }
     }
 
-    // Draws partitions, prediction lists, reference indices and final MVs.
-    void gen_inter(h264g_mb &mb) {
+    // --- MBAFF frames ----------------------------------------------------------
+    // MB address 2 p + b is the top (b = 0) or bottom MB of pair p (pairs in raster order);
+    // both MBs of a pair share mb_field_decoding_flag.  Motion is kept per MB address.
+    Motion &mo_at(int addr, int blk) { return motion[(size_t)addr * 16 + blk]; }
+
+    // 6.4.12.2 (Table 6-4) for a luma location (xN, yN) relative to MB addr: the MB
+    // covering it (-1: not available) and the location (xW, yW) inside that MB.  Pairs A,
+    // B, C, D of 6.4.10 are available when inside the picture (one slice per picture).
+    int mbaff_nb(int addr, int xN, int yN, int &xW, int &yW) const {
+        const int p = addr >> 1, px = p % W, top = (addr & 1) == 0;
+        const int pA = px > 0 ? p - 1 : -1, pB = p >= W ? p - W : -1;
+        const int pC = (p >= W && px < W - 1) ? p - W + 1 : -1, pD = (p >= W && px > 0) ? p - W - 1 : -1;
+        const bool cur_field = fld[addr] != 0;
+        auto field_pair = [&](int q) { return fld[2 * q] != 0; };
+        int n = -1, yM = 0;
+        if (xN > 15 && yN >= 0) return -1;
+        if (yN > 15) return -1;
+        if (yN >= 0 && xN >= 0) {
+            n = addr;
+            yM = yN;
+        } else if (xN < 0 && yN >= 0) {  // left pair
+            if (pA < 0) return -1;
+            const bool af = field_pair(pA);
+            if (!cur_field) {
+                if (!af) { n = 2 * pA + (top ? 0 : 1); yM = yN; }
+                else { n = 2 * pA + (yN & 1); yM = (yN + (top ? 0 : 16)) >> 1; }
+            } else if (!af) {
+                const int r = 2 * yN + (top ? 0 : 1);  // row of the frame pair
+                n = 2 * pA + (r >= 16 ? 1 : 0);
+                yM = r & 15;
+            } else {
+                n = 2 * pA + (top ? 0 : 1);
+                yM = yN;
+            }
+        } else {  // yN < 0: above, above-left, above-right
+            const int q = xN < 0 ? -1 : (xN < 16 ? 0 : 1);
+            if (!cur_field && !top) {  // bottom frame MB: inside the pair, or the left pair
+                if (q == 1) return -1;
+                if (q == 0) { n = addr - 1; yM = yN; }
+                else {
+                    if (pA < 0) return -1;
+                    if (!field_pair(pA)) { n = 2 * pA; yM = yN; }
+                    else { n = 2 * pA + 1; yM = (yN + 16) >> 1; }
+                }
+            } else {
+                const int pn = q < 0 ? pD : (q == 0 ? pB : pC);
+                if (pn < 0) return -1;
+                if (!cur_field || !top) { n = 2 * pn + 1; yM = yN; }          // frame top MB, bottom field MB
+                else if (!field_pair(pn)) { n = 2 * pn + 1; yM = 2 * yN; }   // top field MB over a frame pair
+                else { n = 2 * pn; yM = yN; }
+            }
+        }
+        xW = (xN + 16) % 16;
+        yW = (yM + 16) % 16;
+        return n;
+    }
+
+    // Motion data of the partition covering luma location (xN, yN) of MB addr for list 0,
+    // scaled to the current MB's frame / field units (8.4.1.3.2 with 6.4.11.7,
+    // H264InterPrediction.cpp:1758-2001): field MB over a frame neighbour halves the
+    // vertical component (C division) and doubles refIdx, the other way round doubles
+    // the component and halves refIdx.
+    void neighbour_mbaff(int addr, int xN, int yN, bool &avail, int &ref, int mv[2]) {
+        mv[0] = mv[1] = 0;
+        ref = -1;
+        int xW = 0, yW = 0;
+        const int n = mbaff_nb(addr, xN, yN, xW, yW);
+        avail = n >= 0;
+        if (!avail) return;
+        const Motion &m = mo_at(n, (yW >> 2) * 4 + (xW >> 2));
+        if (m.intra || m.ref[0] < 0) return;
+        ref = m.ref[0];
+        mv[0] = m.mv[0][0];
+        mv[1] = m.mv[0][1];
+        if (fld[addr] && !fld[n]) { mv[1] = mv[1] / 2; ref = ref * 2; }
+        else if (!fld[addr] && fld[n]) { mv[1] = mv[1] * 2; ref = ref / 2; }
+    }
+
+    // 8.4.1.1 P_Skip motion vector in an MBAFF frame (H264InterPrediction.cpp:685-738).
+    void pskip_mv_mbaff(int addr, int out[2]) {
+        bool aA, aB, aC;
+        int rA, rB, rC, mA[2], mB[2], mC[2];
+        neighbour_mbaff(addr, -1, 0, aA, rA, mA);
+        neighbour_mbaff(addr, 0, -1, aB, rB, mB);
+        neighbour_mbaff(addr, 16, -1, aC, rC, mC);
+        if (!aC) neighbour_mbaff(addr, -1, -1, aC, rC, mC);
+        if (!aA || !aB || (rA == 0 && mA[0] == 0 && mA[1] == 0) || (rB == 0 && mB[0] == 0 && mB[1] == 0)) {
+            out[0] = out[1] = 0;
+            return;
+        }
+        const int cnt = (rA == 0) + (rB == 0) + (rC == 0);
+        if (cnt == 1) {
+            const int *m = rA == 0 ? mA : (rB == 0 ? mB : mC);
+            out[0] = m[0]; out[1] = m[1];
+            return;
+        }
+        for (int c = 0; c < 2; c++) {
+            const int a = mA[c], b = mB[c], cc = mC[c];
+            out[c] = a + b + cc - std::min(a, std::min(b, cc)) - std::max(a, std::max(b, cc));
+        }
+    }
+
+    void store_motion_mbaff(int addr, const h264g_mb &mb, bool intra) {
+        for (int blk = 0; blk < 16; blk++) {
+            Motion &m = mo_at(addr, blk);
+            m = Motion();
+            m.intra = intra ? 1 : 0;
+            const int q = ((blk >> 3) << 1) | ((blk & 3) >> 1);
+            for (int l = 0; l < 2; l++) {
+                m.ref[l] = intra ? -1 : mb.ref_idx[l][q];
+                m.mv[l][0] = intra ? 0 : mb.mv[l][blk][0];
+                m.mv[l][1] = intra ? 0 : mb.mv[l][blk][1];
+            }
+        }
+    }
+
+    // Pairs in raster order, top MB then bottom MB.  Both classes are drawn first: a pair
+    // of two P_Skip MBs carries no mb_field_decoding_flag, which is then inferred from the
+    // left pair, else the pair above, else frame (7.4.4, H264SliceData.cpp:166-370).
+    void run_mbaff() {
+        motion.assign((size_t)W * H * 16, Motion());
+        fld.assign((size_t)W * H, 0);
+        for (int py = 0; py < H / 2; py++)
+            for (int px = 0; px < W; px++) {
+                const int p = py * W + px, a0 = 2 * p;
+                int field = R.permille(500) ? 1 : 0;
+                h264g_mb &t = pic.mbs[a0], &b = pic.mbs[a0 + 1];
+                init_mb(t);
+                init_mb(b);
+                const int ct = choose_class(t), cb = choose_class(b);
+                if (ct == H264G_MB_P_SKIP && cb == H264G_MB_P_SKIP)
+                    field = px > 0 ? fld[2 * (p - 1)] : (py > 0 ? fld[2 * (p - W)] : 0);
+                fld[a0] = fld[a0 + 1] = (uint8_t)field;
+                for (int k = 0; k < 2; k++) {
+                    const int addr = a0 + k, cls = k ? cb : ct;
+                    h264g_mb &mb = k ? b : t;
+                    if (field) mb.flags |= H264G_MBF_FIELD;
+                    // intra neighbours (6.4.12.2): a frame bottom MB has its top MB above it and
+                    // the left pair above-left; every other MB the pairs around its own
+                    const bool A = px > 0, pb = py > 0;
+                    const bool B = (!field && k) ? true : pb;
+                    const bool Dn = (!field && k) ? A : (pb && px > 0);
+                    gen_body(mb, cls, A, B, Dn, field != 0, [&](int mv[2]) { pskip_mv_mbaff(addr, mv); });
+                    store_motion_mbaff(addr, mb, cls <= H264G_MB_IPCM);
+                }
+            }
+    }
+
+    // Draws partitions, prediction lists, reference indices and final MVs.  The reference
+    // indices of a field MB of an MBAFF frame range over the field list (2 x frames).
+    void gen_inter(h264g_mb &mb, bool field) {
         const bool isB = mb.mb_type == H264G_MB_B;
         auto draw_pred = [&]() -> int {  // 1 = L0, 2 = L1, 3 = Bi
             if (!isB) return 1;
             int u = R.uni(0, 99);
             return u < 40 ? 3 : (u < 70 ? 1 : 2);
         };
-        auto draw_ref = [&](int l) -> int { return R.uni(0, D.num_ref_idx_active[l] - 1); };
+        auto draw_ref = [&](int l) -> int { return R.uni(0, D.num_ref_idx_active[l] * (field ? 2 : 1) - 1); };
         // With implicit weights a bi-predicted partition must not use the same picture
         // from both lists: the reference divides by td = POC1 - POC0 = 0 there
         // (H264InterPrediction.cpp:2958) before its 32/32 fallback test.
@@ -515,7 +679,7 @@ int32_t h264g_gen_default_params(int32_t config, h264g_gen_params *o) {
         o->b_frames = 0; o->p_pattern = 1;
         return H264G_OK;
     }
-    if (config == 2 || config == 0) {
+    if (config == 2 || config == 0 || config == 3) {
         o->width_mbs = 120; o->height_mbs = 68; o->n_frames = 8;
         o->b_frames = 2; o->max_refs = 4; o->t8x8 = 1;
         o->mv_mode = 1; o->mv_range = 24;
@@ -524,6 +688,10 @@ int32_t h264g_gen_default_params(int32_t config, h264g_gen_params *o) {
         o->intra_in_inter_permille = 50;
         o->weighted_pred_flag = 0; o->weighted_bipred_idc = 1;
         o->filter_offsets = 1;
+        if (config == 3) {  // MBAFF frames, 4x4 transform only
+            o->t8x8 = 0;
+            o->mbaff = 1;
+        }
         return H264G_OK;
     }
     return H264G_E_ARG;
@@ -536,6 +704,9 @@ int32_t h264g_gen_create(const h264g_gen_params *params, h264g_gen **out) {
         p.qp_lo < 0 || p.qp_hi > 51 || p.qp_lo > p.qp_hi || p.qp_hi - p.qp_lo > 25 || p.spare_slots < 0 ||
         p.spare_slots > 16)
         return h264g_set_error(H264G_E_ARG, "gen: bad params");
+    // MBAFF: pairs of MB rows; implicit weights of field MBs would need field POCs
+    if (p.mbaff && (p.height_mbs % 2 || p.weighted_bipred_idc == 2))
+        return h264g_set_error(H264G_E_ARG, "gen: MBAFF needs an even height and no implicit weights");
     h264g_gen *g = new h264g_gen();
     g->p = p;
     std::vector<PicPlan> plan = plan_gop(p.n_frames, p.b_frames, p.gop);
@@ -556,6 +727,7 @@ int32_t h264g_gen_create(const h264g_gen_params *params, h264g_gen **out) {
         Rng rng(p.seed * 0x100000001B3ull + (uint64_t)i * 0x9E3779B97F4A7C15ull + 1);
         d.slice_type = pp.slice_type;
         d.structure = H264G_FRAME;
+        d.mbaff = p.mbaff ? 1 : 0;
         d.poc = 2 * pp.display;
         d.disable_deblocking_filter_idc = p.disable_deblocking_filter_idc;
         if (p.filter_offsets) {

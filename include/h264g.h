@@ -363,7 +363,9 @@ int32_t h264g_device_describe(int32_t device, char *out, uint32_t len);
 typedef struct h264g_gen h264g_gen;
 
 typedef struct h264g_gen_params {
-    int32_t  config;              /* 1 = CIF I+P, 2 = 1080p IPBBPBBP High, 0 = custom  */
+    int32_t  config;              /* 1 = CIF I+P, 2 = 1080p IPBBPBBP High, 3 = 1080p
+                                     MBAFF (config 2 syntax, 4x4 transform only),
+                                     0 = custom                                        */
     uint64_t seed;
     int32_t  width_mbs, height_mbs;
     int32_t  n_frames;
@@ -389,6 +391,9 @@ typedef struct h264g_gen_params {
                                      take the free slot after the last one used instead
                                      of the lowest, so a picture does not overwrite the
                                      slot of the picture decoded just before it        */
+    int32_t  mbaff;               /* 1 = MBAFF frames: MB pairs, each pair a field pair
+                                     with probability 1/2 (the inferred flag when both
+                                     MBs are skipped); height_mbs must be even          */
 } h264g_gen_params;
 
 /* ------------------------------------------------ host motion-vector derivation

@@ -38,6 +38,8 @@
 #include "H264SliceHeader.h"
 #include "h264g.h"
 
+#include <malloc.h>
+
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -163,8 +165,8 @@ struct Harness {
         sps.PicSizeInMapUnits = W * H;
         sps.PicWidthInSamplesL = W * 16;
         sps.PicWidthInSamplesC = W * 8;
-        sps.frame_mbs_only_flag = 1;
-        sps.mb_adaptive_frame_field_flag = 0;
+        sps.frame_mbs_only_flag = gp.mbaff ? 0 : 1;
+        sps.mb_adaptive_frame_field_flag = gp.mbaff ? 1 : 0;
         sps.direct_8x8_inference_flag = 1;
         sps.qpprime_y_zero_transform_bypass_flag = 0;
         sps.max_num_ref_frames = gp.max_refs;
@@ -179,7 +181,7 @@ struct Harness {
         }
         sh.field_pic_flag = 0;
         sh.bottom_field_flag = 0;
-        sh.MbaffFrameFlag = 0;
+        sh.MbaffFrameFlag = gp.mbaff ? 1 : 0;
         sh.PicHeightInMbs = H;
         sh.PicHeightInSamplesL = H * 16;
         sh.PicHeightInSamplesC = H * 8;
@@ -221,14 +223,26 @@ struct Harness {
         }
     }
 
+    // CurrMbAddr -> (mb_x, mb_y) as CH264SliceData::slice_data sets them (H264SliceData.cpp:
+    // 176-179): in MBAFF frames mb_y counts MB rows of pairs, 2 * pair row + bottom.
+    void set_position(CH264PictureBase &pic, int addr, const h264g_mb &r) {
+        const int mbaff = gp.mbaff ? 1 : 0, w2 = W * (1 + mbaff);
+        pic.CurrMbAddr = addr;
+        pic.mb_x = (addr % w2) / (1 + mbaff);
+        pic.mb_y = (addr / w2) * (1 + mbaff) + (addr % w2) % (1 + mbaff);
+        // the slice-level copy of mb_field_decoding_flag, read by the interpolation's
+        // effective reference height (H264InterPrediction.cpp:2351-2358, 2492-2499)
+        pic.m_h264_slice_data.mb_field_decoding_flag = (r.flags & H264G_MBF_FIELD) ? 1 : 0;
+    }
+
     // Fills the MB fields macroblock_layer() would set (without entropy decoding).
     void fill_mb(CH264PictureBase &pic, int addr, const h264g_mb &r, const DenseLevels &lv, int qp_prev) {
         CH264MacroBlock &m = pic.m_mbs[addr];
         m.field_pic_flag = 0;
         m.bottom_field_flag = 0;
         m.mb_skip_flag = 0;
-        m.mb_field_decoding_flag = 0;
-        m.MbaffFrameFlag = 0;
+        m.mb_field_decoding_flag = (r.flags & H264G_MBF_FIELD) ? 1 : 0;
+        m.MbaffFrameFlag = gp.mbaff ? 1 : 0;
         m.constrained_intra_pred_flag = sh.m_pps.constrained_intra_pred_flag;
         m.disable_deblocking_filter_idc = sh.disable_deblocking_filter_idc;
         m.CurrMbAddr = addr;
@@ -237,7 +251,8 @@ struct Harness {
         m.m_slice_type = sh.slice_type;
         m.FilterOffsetA = sh.FilterOffsetA;
         m.FilterOffsetB = sh.FilterOffsetB;
-        pic.Inverse_macroblock_scanning_process(0, addr, 0, m.m_mb_position_x, m.m_mb_position_y);
+        pic.Inverse_macroblock_scanning_process(m.MbaffFrameFlag, addr, m.mb_field_decoding_flag, m.m_mb_position_x,
+                                                m.m_mb_position_y);
 
         const int st = sh.slice_type;
         int t8 = (r.flags & H264G_MBF_T8x8) ? 1 : 0;
@@ -561,6 +576,26 @@ struct Harness {
         }
     }
 
+    // Field macroblocks of MBAFF frames clamp the horizontal reference coordinate to the
+    // field view's row pitch, twice the picture width (H264InterPrediction.cpp:2363,
+    // 2504-2507 with H264PictureBase.cpp:209-210), so on the bottom field's last chroma
+    // row the reference reads up to PicWidthInSamplesC - 1 bytes past the end of its
+    // frame buffer.  main() pins glibc's mmap threshold so that frame buffers are page
+    // mappings of their own; the bytes after a buffer up to the page end are then zero,
+    // and this checks that they cover the read.
+    void check_tail(const CH264PictureBase &pic) const {
+        const size_t n = (size_t)W * H * 384, need = (size_t)W * 8;
+        if (n < 256 * 1024) fail("MBAFF frame buffers below 256 KiB may come from the heap (glibc's top chunk)");
+        // glibc marks a chunk it mapped with mmap by bit 1 of the size word before it
+        const size_t chunk_size = ((const size_t *)pic.m_pic_buff_luma)[-1];
+        if (!(chunk_size & 2)) fail("frame buffer came from the heap, not a page mapping of its own");
+        const uint8_t *end = pic.m_pic_buff_luma + n;
+        const size_t slack = (size_t)(-(uintptr_t)end & 4095);
+        if (slack < need) fail("frame buffer ends too close to its page end for the field-MB overread");
+        for (size_t i = 0; i < need; i++)
+            if (end[i]) fail("bytes after the frame buffer are not zero");
+    }
+
     void run() {
         W = gp.width_mbs; H = gp.height_mbs;
         setup_headers();
@@ -568,6 +603,15 @@ struct Harness {
         slots.resize(nslots);
         memset(dpb, 0, sizeof(dpb));
         for (int s = 0; s < nslots; s++) { slots[s] = new CH264Picture; dpb[s] = slots[s]; }
+        // Every DPB entry's frame buffer is allocated up front by the reference's own
+        // init() (later inits keep it, H264PictureBase.cpp:154-157), before any other
+        // large allocation, so that each is a page mapping of its own (check_tail).
+        for (int s = 0; s < nslots; s++) {
+            CH264PictureBase &f = slots[s]->m_picture_frame;
+            f.m_picture_coded_type = H264_PICTURE_CODED_TYPE_FRAME;
+            if (f.init(sh) != 0) fail("picture init");
+            if (gp.mbaff) check_tail(f);
+        }
         int npics = h264g_gen_num_pictures(gen);
         for (int i = 0; i < npics; i++) {
             h264g_pic_desc d;
@@ -591,6 +635,7 @@ struct Harness {
             memcpy(pic.m_dpb, dpb, sizeof(dpb));
             cur->m_current_picture_ptr = &pic;
             if (pic.init(sh) != 0) fail("picture init");
+            if (gp.mbaff) check_tail(pic);
             cur->m_picture_top_filed.m_picture_coded_type = H264_PICTURE_CODED_TYPE_TOP_FIELD;
             cur->m_picture_top_filed.m_parent = cur;
             cur->m_picture_top_filed.init(sh);
@@ -614,9 +659,7 @@ struct Harness {
             int qp_prev = 0;
             for (uint32_t a = 0; a < nmb; a++) {
                 expand_levels(mbs[a], lv, dl[a]);
-                pic.CurrMbAddr = (int)a;
-                pic.mb_x = (int)a % W;
-                pic.mb_y = (int)a / W;
+                set_position(pic, (int)a, mbs[a]);
                 fill_mb(pic, (int)a, mbs[a], dl[a], a == 0 ? mbs[a].qp_y : qp_prev);
                 qp_prev = mbs[a].qp_y;
                 solve_mb(pic, (int)a, mbs[a]);
@@ -637,9 +680,7 @@ struct Harness {
             auto t0 = std::chrono::steady_clock::now();
             for (uint32_t a = 0; a < nmb; a++) {
                 pic.m_mbs[a] = saved[a];
-                pic.CurrMbAddr = (int)a;
-                pic.mb_x = (int)a % W;
-                pic.mb_y = (int)a / W;
+                set_position(pic, (int)a, mbs[a]);
                 if (decode_mb(pic, (int)a, mbs[a]) != 0) fail("decode failed in pass 2");
             }
             if (predeblock) {
@@ -665,6 +706,9 @@ struct Harness {
 }  // namespace
 
 int main(int argc, char **argv) {
+    // A fixed threshold (no dynamic adjustment): every frame buffer of 64 KiB or more is
+    // a page mapping of its own (see Harness::check_tail).
+    mallopt(M_MMAP_THRESHOLD, 64 * 1024);
     int config = 1;
     h264g_gen_params p;
     bool have = false;
@@ -685,6 +729,8 @@ int main(int argc, char **argv) {
         else if (a == "--deblock") p.disable_deblocking_filter_idc = atoi(next());
         else if (a == "--refs") p.max_refs = atoi(next());
         else if (a == "--bframes") p.b_frames = atoi(next());
+        else if (a == "--t8x8") p.t8x8 = atoi(next());
+        else if (a == "--mbaff") p.mbaff = atoi(next());
         else if (a == "--out") hs.out = next();
         else if (a == "--predeblock") hs.predeblock = true;
         else if (a == "--mvd") hs.mvd = true;

@@ -18,6 +18,21 @@
  *   I_PCM macroblocks are not "intra" for bS (m_mb_pred_mode is Intra_NA, so
  *       IS_INTRA_Prediction_Mode is false, Deblock.cpp:1022-1102; H264CommonFunc.h:272)
  *       and take RefIdxLX = 0 / PredFlagLX = 0 from the zeroed MB state.
+ * MBAFF frames (macroblock pairs, 6.4.12.2 / 8.7 with field macroblocks) follow the
+ * reference too, with its deviations there:
+ *   Q4  bS compares RefPicListX[RefIdxLX] of field MBs, whose refIdx addresses the
+ *       field list (Deblock.cpp:1175-1178), so the entry is refIdx, not refIdx / 2;
+ *   Q5  frame MB under a field pair: the first chroma top-edge pass takes its p-MB
+ *       data from the bottom field MB where luma takes the top one (Deblock.cpp:474-481
+ *       vs 264);
+ *   Q6  left edge of a frame MB beside a field pair: the p-MB alternates per line, but
+ *       the p block for bS 2 uses the frame line number (Deblock.cpp:755, 773); beside a
+ *       frame pair a field MB takes every line's p data from the top frame MB;
+ *   Q17 field-MB interpolation clamps x to the field view's row pitch, twice the width
+ *       (H264InterPrediction.cpp:2363, 2504-2507; H264PictureBase.cpp:209-210): samples
+ *       right of the picture come from the next row of the frame buffer, on the bottom
+ *       field's last row from the next plane or past the Cr plane (the caller's planes
+ *       carry a zeroed tail, see refcpu.h).
  * Each function cites the reference code it follows.  Used only by tests/, smoke()
  * and bench.py's CPU baseline, as the checker.
  */
@@ -133,10 +148,12 @@ void refcpu_residual8x8(const int32_t scan[64], int qp, int field, int32_t r[64]
     for (int k = 0; k < 64; k++) r[k] = (m[k] + 32) >> 6;
 }
 
-/* 8.5.10 (H264PictureBase.cpp:4993-5088) */
-void refcpu_luma_dc(const int32_t scan[16], int qp, int32_t dcY[16]) {
+/* 8.5.10 (H264PictureBase.cpp:4993-5088; the DC levels take the MB's scan,
+ * H264PictureBase.cpp:3519) */
+static void luma_dc_scan(const int32_t scan[16], int qp, int field, int32_t dcY[16]) {
     int32_t c[16], g[16], f[16];
-    for (int k = 0; k < 16; k++) c[ZZ4[k]] = scan[k];
+    const uint8_t *tab = field ? FLD4 : ZZ4;
+    for (int k = 0; k < 16; k++) c[tab[k]] = scan[k];
     for (int j = 0; j < 4; j++) { /* combine rows of c (first index) */
         int32_t a = c[j], b = c[4 + j], cc = c[8 + j], d = c[12 + j];
         g[j] = a + b + cc + d; g[4 + j] = a + b - cc - d; g[8 + j] = a - b - cc + d; g[12 + j] = a - b + cc - d;
@@ -151,6 +168,7 @@ void refcpu_luma_dc(const int32_t scan[16], int qp, int32_t dcY[16]) {
         else dcY[k] = (f[k] * s + (1 << (5 - qp / 6))) >> (6 - qp / 6);
     }
 }
+void refcpu_luma_dc(const int32_t scan[16], int qp, int32_t dcY[16]) { luma_dc_scan(scan, qp, 0, dcY); }
 
 /* 8.5.11 for 4:2:0 (H264PictureBase.cpp:3989-4043) */
 void refcpu_chroma_dc(const int32_t c[4], int qpc, int32_t dcC[4]) {
@@ -225,6 +243,7 @@ typedef struct {
     int refid[2][16];    /* reference picture identity per 4x4 (raster), -1 = none */
     int pred[2][16];     /* PredFlagLX per 4x4 */
     int mv[2][16][2];
+    int field;           /* mb_field_decoding_flag (MBAFF frames) */
 } MbInfo;
 
 typedef struct {
@@ -235,9 +254,92 @@ typedef struct {
     uint8_t *y, *cb, *cr;
     uint8_t *const *planes;
     MbInfo *info;
+    int mbaff;           /* MbaffFrameFlag: MB addresses run over pairs, top then bottom */
 } Ctx;
 
 static int is_intra_cls(int c) { return c == H264G_MB_I4x4 || c == H264G_MB_I8x8 || c == H264G_MB_I16x16; }
+
+/* ------------------------------------------------- MB addressing */
+static int mb_field(const Ctx *X, int addr) { return X->mbaff && (X->mbs[addr].flags & H264G_MBF_FIELD) ? 1 : 0; }
+
+/* 6.4.1 (H264PictureBase.cpp:2503-2530): luma position of the MB's first sample; a
+ * field MB of an MBAFF frame starts on row 32 * pairRow + bottom. */
+static void mb_luma_pos(const Ctx *X, int addr, int *x, int *y) {
+    if (!X->mbaff) {
+        *x = (addr % X->W) * 16;
+        *y = (addr / X->W) * 16;
+        return;
+    }
+    int pair = addr / 2;
+    *x = (pair % X->W) * 16;
+    *y = (pair / X->W) * 32 + (mb_field(X, addr) ? addr % 2 : (addr % 2) * 16);
+}
+
+/* Position and row pitch of the MB in a plane (n = 16 luma, 8 chroma): chroma rows of
+ * MB pairs as in cSC (H264PictureBase.cpp:2133): (yL >> 4) * 8 + yL % 2. */
+static void mb_plane_origin(const Ctx *X, int addr, int n, int stride, int *x0, int *y0, int *pitch) {
+    int xL, yL;
+    mb_luma_pos(X, addr, &xL, &yL);
+    *x0 = n == 16 ? xL : (xL >> 4) * 8;
+    *y0 = n == 16 ? yL : ((yL >> 4) * 8) + (yL % 2);
+    *pitch = stride * (mb_field(X, addr) ? 2 : 1);
+}
+
+/* 6.4.12.2, Table 6-4 (H264PictureBase.cpp:2984-3395): the MB covering (xN, yN) relative
+ * to MB addr of an MBAFF frame (-1: not available) and (xW, yW) inside it.  Pairs A, B,
+ * C, D of 6.4.10 exist when inside the picture (one slice per picture). */
+static int mbaff_nb(const Ctx *X, int addr, int xN, int yN, int maxW, int maxH, int *xW, int *yW) {
+    int W = X->W, pair = addr / 2, top = addr % 2 == 0, curFrame = !mb_field(X, addr);
+    int pA = pair % W ? 2 * (pair - 1) : -1, pB = pair >= W ? 2 * (pair - W) : -1;
+    int pC = (pair >= W && (pair + 1) % W) ? 2 * (pair - W + 1) : -1, pD = (pair >= W && pair % W) ? 2 * (pair - W - 1) : -1;
+    int n = -1, yM = 0;
+    if (yN > maxH - 1 || (xN > maxW - 1 && yN >= 0)) return -1;
+    if (xN >= 0 && yN >= 0) { n = addr; yM = yN; }
+    else if (xN < 0 && yN >= 0) {
+        if (pA < 0) return -1;
+        int aFrame = !mb_field(X, pA);
+        if (curFrame && aFrame) { n = pA + !top; yM = yN; }
+        else if (curFrame) { n = pA + (yN % 2); yM = (yN + (top ? 0 : maxH)) >> 1; }
+        else if (aFrame) {
+            int r = (yN << 1) + !top;            /* frame row of the left pair */
+            n = r < maxH ? pA : pA + 1;
+            yM = r < maxH ? r : r - maxH;
+        } else { n = pA + !top; yM = yN; }
+    } else { /* yN < 0 */
+        int pX = xN < 0 ? pD : (xN < maxW ? pB : pC);
+        if (curFrame && !top) {
+            if (xN > maxW - 1) return -1;
+            if (xN >= 0) { n = addr - 1; yM = yN; }
+            else if (pA < 0) return -1;
+            else if (!mb_field(X, pA)) { n = pA; yM = yN; }
+            else { n = pA + 1; yM = (yN + maxH) >> 1; }
+        } else {
+            if (pX < 0) return -1;
+            if (curFrame || !top) { n = pX + 1; yM = yN; }
+            else if (!mb_field(X, pX)) { n = pX + 1; yM = 2 * yN; }
+            else { n = pX; yM = yN; }
+        }
+    }
+    *xW = (xN + maxW) % maxW;
+    *yW = (yM + maxH) % maxH;
+    return n;
+}
+
+/* 6.4.12.1 (H264PictureBase.cpp:2878-2979) for non-MBAFF frames. */
+static int frame_nb(const Ctx *X, int addr, int xN, int yN, int n, int *xW, int *yW) {
+    int mx = addr % X->W, my = addr / X->W;
+    if (xN < 0 && yN < 0) { mx--; my--; }
+    else if (xN < 0 && yN < n) mx--;
+    else if (xN < 0) return -1;
+    else if (xN < n && yN < 0) my--;
+    else if (xN < n && yN < n) { /* current MB */ }
+    else if (yN < 0) { mx++; my--; }
+    else return -1;
+    if (mx < 0 || my < 0 || mx >= X->W || my >= X->H) return -1;
+    *xW = (xN + n) % n;
+    *yW = (yN + n) % n;
+    return my * X->W + mx;
+}
 
 /* ----------------------------------------------------------- levels */
 typedef struct {
@@ -338,49 +440,76 @@ static int weigh(const Weights *w, int c, int pf0, int pf1, int p0, int p1) {
 }
 
 /* -------------------------------------------------- inter MB (8.4) */
-static int inter_mb(Ctx *X, int mbx, int mby, const h264g_mb *mb, MbInfo *info) {
+/* Per 4x4 block: reference selection (8.4.2.1, H264InterPrediction.cpp:2117-2224),
+ * chroma vector (8.4.1.4, :2006-2047), interpolation (8.4.2.2) and weighting (8.4.2.3).
+ * A field MB of an MBAFF frame predicts from the field of RefPicListX[refIdx / 2] with
+ * the MB's parity when refIdx is even, the other one when odd; that field is the frame
+ * buffer seen from row `parity` with twice the row pitch, and the reference's field view
+ * reports that pitch as its width, which is also where it clamps x (Q17).  Its weights
+ * are looked up at refIdx >> 1 (H264InterPrediction.cpp:2993-3031). */
+static int inter_mb(Ctx *X, int addr, const h264g_mb *mb, MbInfo *info) {
     const h264g_pic_desc *d = X->d;
+    const int field = mb_field(X, addr), bottom = X->mbaff && addr % 2;
+    int x0, y0, pitch, cx0, cy0, cpitch;
+    mb_plane_origin(X, addr, 16, X->wl, &x0, &y0, &pitch);
+    mb_plane_origin(X, addr, 8, X->wc, &cx0, &cy0, &cpitch);
+    uint8_t *oy = X->y + y0 * X->wl + x0, *ocb = X->cb + cy0 * X->wc + cx0, *ocr = X->cr + cy0 * X->wc + cx0;
+    /* MB origin in the coordinates of the reference picture it predicts from: a field MB
+     * sits at field row 16 * pairRow (H264InterPrediction.cpp:553-557) */
+    const int xMb = x0, yMb = field ? (y0 - bottom) / 2 : y0;
     for (int b = 0; b < 16; b++) {
         int bx = b % 4, by = b / 4, q = (by / 2) * 2 + bx / 2;
         int ref[2], pf[2];
         for (int l = 0; l < 2; l++) {
             ref[l] = mb->ref_idx[l][q];
             pf[l] = ref[l] >= 0;
-            if (pf[l] && (ref[l] >= d->num_ref_idx_active[l] || d->ref_slot[l][ref[l]] < 0)) return H264G_E_ARG;
+            int frame_idx = field ? ref[l] >> 1 : ref[l];
+            if (pf[l] && (frame_idx >= d->num_ref_idx_active[l] || d->ref_slot[l][frame_idx] < 0)) return H264G_E_ARG;
         }
         if (!pf[0] && !pf[1]) return H264G_E_ARG;
+        if (field && d->slice_type == H264G_SLICE_B && d->weighted_bipred_idc == 2 && pf[0] && pf[1])
+            return H264G_E_ARG; /* implicit weights of field MBs need field POCs */
         Weights w;
-        int rc = derive_weights(d, ref[0], ref[1], pf[0], pf[1], &w);
+        int rc = derive_weights(d, field ? ref[0] >> 1 : ref[0], field ? ref[1] >> 1 : ref[1], pf[0], pf[1], &w);
         if (rc) return rc;
         int pl[2][16], pcb[2][4], pcr[2][4];
         for (int l = 0; l < 2; l++) {
             if (!pf[l]) continue;
-            const uint8_t *rp = X->planes[d->ref_slot[l][ref[l]]];
+            const uint8_t *rp = X->planes[d->ref_slot[l][field ? ref[l] >> 1 : ref[l]]];
             const uint8_t *ry = rp, *rcb = rp + X->wl * X->hl, *rcr = rcb + X->wc * X->hc;
+            int wl = X->wl, hl = X->hl, wc = X->wc, hc = X->hc, mvcy_off = 0;
             int mvx = mb->mv[l][b][0], mvy = mb->mv[l][b][1];
-            int xAL = mbx * 16 + bx * 4, yAL = mby * 16 + by * 4;
+            if (field) {
+                int parity = (ref[l] & 1) ^ bottom;  /* 0: top field, 1: bottom field */
+                ry += parity * wl; rcb += parity * wc; rcr += parity * wc;
+                wl *= 2; wc *= 2; hl /= 2; hc /= 2;
+                if (parity && !bottom) mvcy_off = -2;      /* Table 8-10 */
+                else if (!parity && bottom) mvcy_off = 2;
+            }
+            int xAL = xMb + bx * 4, yAL = yMb + by * 4;
             for (int yy = 0; yy < 4; yy++)
                 for (int xx = 0; xx < 4; xx++)
-                    pl[l][yy * 4 + xx] = refcpu_luma_sample(ry, X->wl, X->hl, xAL + (mvx >> 2) + xx, yAL + (mvy >> 2) + yy, mvx & 3, mvy & 3);
+                    pl[l][yy * 4 + xx] = refcpu_luma_sample(ry, wl, hl, xAL + (mvx >> 2) + xx, yAL + (mvy >> 2) + yy, mvx & 3, mvy & 3);
+            int mvcy = mvy + mvcy_off;
             for (int yy = 0; yy < 2; yy++)
                 for (int xx = 0; xx < 2; xx++) {
-                    int xi = xAL / 2 + (mvx >> 3) + xx, yi = yAL / 2 + (mvy >> 3) + yy;
-                    pcb[l][yy * 2 + xx] = refcpu_chroma_sample(rcb, X->wc, X->hc, xi, yi, mvx & 7, mvy & 7);
-                    pcr[l][yy * 2 + xx] = refcpu_chroma_sample(rcr, X->wc, X->hc, xi, yi, mvx & 7, mvy & 7);
+                    int xi = xAL / 2 + (mvx >> 3) + xx, yi = yAL / 2 + (mvcy >> 3) + yy;
+                    pcb[l][yy * 2 + xx] = refcpu_chroma_sample(rcb, wc, hc, xi, yi, mvx & 7, mvcy & 7);
+                    pcr[l][yy * 2 + xx] = refcpu_chroma_sample(rcr, wc, hc, xi, yi, mvx & 7, mvcy & 7);
                 }
         }
         for (int yy = 0; yy < 4; yy++)
             for (int xx = 0; xx < 4; xx++)
-                X->y[(mby * 16 + by * 4 + yy) * X->wl + mbx * 16 + bx * 4 + xx] =
-                    (uint8_t)weigh(&w, 0, pf[0], pf[1], pl[0][yy * 4 + xx], pl[1][yy * 4 + xx]);
+                oy[(by * 4 + yy) * pitch + bx * 4 + xx] = (uint8_t)weigh(&w, 0, pf[0], pf[1], pl[0][yy * 4 + xx], pl[1][yy * 4 + xx]);
         for (int yy = 0; yy < 2; yy++)
             for (int xx = 0; xx < 2; xx++) {
-                int o = (mby * 8 + by * 2 + yy) * X->wc + mbx * 8 + bx * 2 + xx;
-                X->cb[o] = (uint8_t)weigh(&w, 1, pf[0], pf[1], pcb[0][yy * 2 + xx], pcb[1][yy * 2 + xx]);
-                X->cr[o] = (uint8_t)weigh(&w, 2, pf[0], pf[1], pcr[0][yy * 2 + xx], pcr[1][yy * 2 + xx]);
+                int o = (by * 2 + yy) * cpitch + bx * 2 + xx;
+                ocb[o] = (uint8_t)weigh(&w, 1, pf[0], pf[1], pcb[0][yy * 2 + xx], pcb[1][yy * 2 + xx]);
+                ocr[o] = (uint8_t)weigh(&w, 2, pf[0], pf[1], pcr[0][yy * 2 + xx], pcr[1][yy * 2 + xx]);
             }
         for (int l = 0; l < 2; l++) {
             info->pred[l][b] = pf[l];
+            /* bS identity: RefPicListX[RefIdxLX], the field index for field MBs (Q4) */
             info->refid[l][b] = pf[l] ? d->ref_slot[l][ref[l]] : -1;
             info->mv[l][b][0] = mb->mv[l][b][0];
             info->mv[l][b][1] = mb->mv[l][b][1];
@@ -395,28 +524,29 @@ static void blk_xy(int b, int *x, int *y) {
     *y = ((b >> 3) & 1) * 8 + ((b >> 1) & 1) * 4;
 }
 
-static void add_residual(uint8_t *plane, int stride, int x0, int y0, const int32_t *r, int n) {
+static void add_residual(uint8_t *o, int pitch, int x, int y, const int32_t *r, int n) {
     for (int i = 0; i < n; i++)
         for (int j = 0; j < n; j++) {
-            uint8_t *s = &plane[(y0 + i) * stride + x0 + j];
+            uint8_t *s = &o[(y + i) * pitch + x + j];
             *s = (uint8_t)CLIP1(*s + r[i * n + j]);
         }
 }
 
-/* residual of the luma 4x4 block b for non-I16x16 MBs (8.5.1) */
-static void luma4x4_residual(const Levels *L, int b, int qp, int32_t r[16]) {
-    refcpu_residual4x4(L->l4[b], qp, 0, 0, r);
+/* residual of the luma 4x4 block b for non-I16x16 MBs (8.5.1); field MBs use the field
+ * scan (field_pic_flag | mb_field_decoding_flag, H264PictureBase.cpp:3419) */
+static void luma4x4_residual(const Levels *L, int b, int qp, int field, int32_t r[16]) {
+    refcpu_residual4x4(L->l4[b], qp, 0, field, r);
 }
 
 /* chroma residual of one component: rMb[8][8] raster (8.5.4) */
-static void chroma_residual(const Levels *L, int c, int qpc, int32_t rmb[64]) {
+static void chroma_residual(const Levels *L, int c, int qpc, int field, int32_t rmb[64]) {
     int32_t dcC[4];
     refcpu_chroma_dc(L->cdc[c], qpc, dcC);
     for (int b = 0; b < 4; b++) {
         int32_t list[16], r[16];
         memcpy(list, L->cac[c][b], sizeof(list));
         list[0] = dcC[b];
-        refcpu_residual4x4(list, qpc, 1, 0, r);
+        refcpu_residual4x4(list, qpc, 1, field, r);
         int x0 = (b % 2) * 4, y0 = (b / 2) * 4;
         for (int i = 0; i < 4; i++)
             for (int j = 0; j < 4; j++) rmb[(y0 + i) * 8 + x0 + j] = r[i * 4 + j];
@@ -425,36 +555,33 @@ static void chroma_residual(const Levels *L, int c, int qpc, int32_t rmb[64]) {
 
 /* ------------------------------------------------ intra (8.3) */
 /* Neighbouring sample (xN, yN) relative to the MB's top-left in a plane with MB size
- * n (16 luma, 8 chroma), per 6.4.12.1 (H264PictureBase.cpp:2878-2979): -1 if not
- * available (outside the picture, or an inter MB under constrained intra). */
-static int nb_sample(Ctx *X, const uint8_t *plane, int stride, int n, int mbx, int mby, int xN, int yN) {
-    int mx = mbx, my = mby;
-    if (xN < 0 && yN < 0) { mx--; my--; }
-    else if (xN < 0 && yN < n) mx--;
-    else if (xN < 0) return -1;
-    else if (xN < n && yN < 0) my--;
-    else if (xN < n && yN < n) { /* current MB */ }
-    else if (yN < 0) { mx++; my--; }
-    else return -1;
-    if (mx < 0 || my < 0 || mx >= X->W || my >= X->H) return -1;
+ * n (16 luma, 8 chroma): the MB covering it by 6.4.12 (Table 6-3 / 6-4), -1 if not
+ * available (outside the picture, or an inter MB under constrained intra), else the
+ * sample at that MB's position (6.4.1) plus (xW, yW), rows two apart in a field MB of
+ * an MBAFF frame (H264PictureBase.cpp:1099-1155, 1917-1925, 2172-2186). */
+static int nb_sample(Ctx *X, const uint8_t *plane, int stride, int n, int addr, int xN, int yN) {
+    int xW = 0, yW = 0;
+    int na = X->mbaff ? mbaff_nb(X, addr, xN, yN, n, n, &xW, &yW) : frame_nb(X, addr, xN, yN, n, &xW, &yW);
+    if (na < 0) return -1;
     if (X->d->constrained_intra_pred_flag) {
-        int c = X->info[my * X->W + mx].cls;
+        int c = X->mbs[na].mb_type;
         if (c == H264G_MB_P || c == H264G_MB_P_SKIP || c == H264G_MB_B) return -1;
     }
-    int xW = (xN + n) % n, yW = (yN + n) % n;
-    return plane[(my * n + yW) * stride + mx * n + xW];
+    int x0, y0, pitch;
+    mb_plane_origin(X, na, n, stride, &x0, &y0, &pitch);
+    return plane[y0 * stride + yW * pitch + x0 + xW];
 }
 
 #define P4(x, y) p[((y) + 1) * 9 + (x) + 1]
 /* 8.3.1.2 (H264PictureBase.cpp:1062-1419) for the 4x4 block at (xO, yO). */
-static int intra4x4_pred(Ctx *X, int mbx, int mby, int blk, int mode, uint8_t out[16]) {
+static int intra4x4_pred(Ctx *X, int addr, int blk, int mode, uint8_t out[16]) {
     int xO, yO, p[45];
     blk_xy(blk, &xO, &yO);
     for (int i = 0; i < 45; i++) p[i] = -1;
-    P4(-1, -1) = nb_sample(X, X->y, X->wl, 16, mbx, mby, xO - 1, yO - 1);
-    for (int y = 0; y < 4; y++) P4(-1, y) = nb_sample(X, X->y, X->wl, 16, mbx, mby, xO - 1, yO + y);
+    P4(-1, -1) = nb_sample(X, X->y, X->wl, 16, addr, xO - 1, yO - 1);
+    for (int y = 0; y < 4; y++) P4(-1, y) = nb_sample(X, X->y, X->wl, 16, addr, xO - 1, yO + y);
     for (int x = 0; x < 8; x++)
-        P4(x, -1) = (x > 3 && (blk == 3 || blk == 11)) ? -1 : nb_sample(X, X->y, X->wl, 16, mbx, mby, xO + x, yO - 1);
+        P4(x, -1) = (x > 3 && (blk == 3 || blk == 11)) ? -1 : nb_sample(X, X->y, X->wl, 16, addr, xO + x, yO - 1);
     if (P4(4, -1) < 0 && P4(5, -1) < 0 && P4(6, -1) < 0 && P4(7, -1) < 0 && P4(3, -1) >= 0)
         for (int x = 4; x < 8; x++) P4(x, -1) = P4(3, -1);
     int top = P4(0, -1) >= 0 && P4(1, -1) >= 0 && P4(2, -1) >= 0 && P4(3, -1) >= 0;
@@ -525,12 +652,12 @@ static int intra4x4_pred(Ctx *X, int mbx, int mby, int blk, int mode, uint8_t ou
 #define P8(x, y) p[((y) + 1) * 17 + (x) + 1]
 #define Q8(x, y) q[((y) + 1) * 17 + (x) + 1]
 /* 8.3.2.2 incl. reference sample filtering 8.3.2.2.1 (H264PictureBase.cpp:1423-1843) */
-static int intra8x8_pred(Ctx *X, int mbx, int mby, int blk, int mode, uint8_t out[64]) {
+static int intra8x8_pred(Ctx *X, int addr, int blk, int mode, uint8_t out[64]) {
     int xO = (blk % 2) * 8, yO = (blk / 2) * 8, p[9 * 17], q[9 * 17];
     for (int i = 0; i < 9 * 17; i++) p[i] = q[i] = -1;
-    P8(-1, -1) = nb_sample(X, X->y, X->wl, 16, mbx, mby, xO - 1, yO - 1);
-    for (int y = 0; y < 8; y++) P8(-1, y) = nb_sample(X, X->y, X->wl, 16, mbx, mby, xO - 1, yO + y);
-    for (int x = 0; x < 16; x++) P8(x, -1) = nb_sample(X, X->y, X->wl, 16, mbx, mby, xO + x, yO - 1);
+    P8(-1, -1) = nb_sample(X, X->y, X->wl, 16, addr, xO - 1, yO - 1);
+    for (int y = 0; y < 8; y++) P8(-1, y) = nb_sample(X, X->y, X->wl, 16, addr, xO - 1, yO + y);
+    for (int x = 0; x < 16; x++) P8(x, -1) = nb_sample(X, X->y, X->wl, 16, addr, xO + x, yO - 1);
     int up8 = 1;
     for (int x = 8; x < 16; x++) up8 &= P8(x, -1) < 0;
     if (up8 && P8(7, -1) >= 0) for (int x = 8; x < 16; x++) P8(x, -1) = P8(7, -1);
@@ -627,14 +754,14 @@ static int intra8x8_pred(Ctx *X, int mbx, int mby, int blk, int mode, uint8_t ou
 #undef Q8
 
 /* 8.3.3 (H264PictureBase.cpp:1847-2056) */
-static int intra16_pred(Ctx *X, int mbx, int mby, int mode, uint8_t out[256]) {
+static int intra16_pred(Ctx *X, int addr, int mode, uint8_t out[256]) {
     int top[16], left[16], tl, ta = 1, la = 1;
     for (int i = 0; i < 16; i++) {
-        top[i] = nb_sample(X, X->y, X->wl, 16, mbx, mby, i, -1);
-        left[i] = nb_sample(X, X->y, X->wl, 16, mbx, mby, -1, i);
+        top[i] = nb_sample(X, X->y, X->wl, 16, addr, i, -1);
+        left[i] = nb_sample(X, X->y, X->wl, 16, addr, -1, i);
         ta &= top[i] >= 0; la &= left[i] >= 0;
     }
-    tl = nb_sample(X, X->y, X->wl, 16, mbx, mby, -1, -1);
+    tl = nb_sample(X, X->y, X->wl, 16, addr, -1, -1);
     int dc = 128;
     if (mode == 2) {
         int st = 0, sl = 0;
@@ -663,13 +790,13 @@ static int intra16_pred(Ctx *X, int mbx, int mby, int mode, uint8_t out[256]) {
 }
 
 /* 8.3.4 for 4:2:0 (H264PictureBase.cpp:2076-2381); DC availability uses > 0 (Q1). */
-static int intra_chroma_pred(Ctx *X, const uint8_t *plane, int mbx, int mby, int mode, uint8_t out[64]) {
+static int intra_chroma_pred(Ctx *X, const uint8_t *plane, int addr, int mode, uint8_t out[64]) {
     int top[8], left[8], tl;
     for (int i = 0; i < 8; i++) {
-        top[i] = nb_sample(X, plane, X->wc, 8, mbx, mby, i, -1);
-        left[i] = nb_sample(X, plane, X->wc, 8, mbx, mby, -1, i);
+        top[i] = nb_sample(X, plane, X->wc, 8, addr, i, -1);
+        left[i] = nb_sample(X, plane, X->wc, 8, addr, -1, i);
     }
-    tl = nb_sample(X, plane, X->wc, 8, mbx, mby, -1, -1);
+    tl = nb_sample(X, plane, X->wc, 8, addr, -1, -1);
     if (mode == 0) {
         for (int blk = 0; blk < 4; blk++) {
             int xO = (blk % 2) * 4, yO = (blk / 2) * 4;
@@ -707,98 +834,130 @@ static int intra_chroma_pred(Ctx *X, const uint8_t *plane, int mbx, int mby, int
     return 0;
 }
 
-static void put_block(uint8_t *plane, int stride, int x0, int y0, const uint8_t *v, int n) {
-    for (int i = 0; i < n; i++) memcpy(&plane[(y0 + i) * stride + x0], &v[i * n], (size_t)n);
+/* Writes an n x n block at (x, y) of the MB whose first sample is o, rows `pitch` apart
+ * (8.5.14, H264PictureBase.cpp:4408-4538: field MBs of MBAFF frames every other row). */
+static void put_block(uint8_t *o, int pitch, int x, int y, const uint8_t *v, int n) {
+    for (int i = 0; i < n; i++) memcpy(&o[(y + i) * pitch + x], &v[i * n], (size_t)n);
 }
 
 /* Intra MB reconstruction in the reference's order (H264SliceData.cpp:400-453). */
-static int intra_mb(Ctx *X, int mbx, int mby, const h264g_mb *mb, const Levels *L) {
-    int qp = mb->qp_y, rc;
-    int x0 = mbx * 16, y0 = mby * 16;
+static int intra_mb(Ctx *X, int addr, const h264g_mb *mb, const Levels *L) {
+    int qp = mb->qp_y, rc, field = mb_field(X, addr);
+    int x0, y0, pitch;
+    mb_plane_origin(X, addr, 16, X->wl, &x0, &y0, &pitch);
+    uint8_t *o = X->y + y0 * X->wl + x0;
     if (mb->mb_type == H264G_MB_I4x4) {
         for (int b = 0; b < 16; b++) {
             int32_t r[16];
             uint8_t pred[16];
             int bx, by;
             blk_xy(b, &bx, &by);
-            luma4x4_residual(L, b, qp, r);
-            if ((rc = intra4x4_pred(X, mbx, mby, b, mb->intra_pred_modes[b], pred))) return rc;
-            put_block(X->y, X->wl, x0 + bx, y0 + by, pred, 4);
-            add_residual(X->y, X->wl, x0 + bx, y0 + by, r, 4);
+            luma4x4_residual(L, b, qp, field, r);
+            if ((rc = intra4x4_pred(X, addr, b, mb->intra_pred_modes[b], pred))) return rc;
+            put_block(o, pitch, bx, by, pred, 4);
+            add_residual(o, pitch, bx, by, r, 4);
         }
     } else if (mb->mb_type == H264G_MB_I8x8) {
         for (int b = 0; b < 4; b++) {
             int32_t r[64];
             uint8_t pred[64];
-            refcpu_residual8x8(L->l8[b], qp, 0, r);
-            if ((rc = intra8x8_pred(X, mbx, mby, b, mb->intra_pred_modes[b], pred))) return rc;
-            put_block(X->y, X->wl, x0 + (b % 2) * 8, y0 + (b / 2) * 8, pred, 8);
-            add_residual(X->y, X->wl, x0 + (b % 2) * 8, y0 + (b / 2) * 8, r, 8);
+            refcpu_residual8x8(L->l8[b], qp, field, r);
+            if ((rc = intra8x8_pred(X, addr, b, mb->intra_pred_modes[b], pred))) return rc;
+            put_block(o, pitch, (b % 2) * 8, (b / 2) * 8, pred, 8);
+            add_residual(o, pitch, (b % 2) * 8, (b / 2) * 8, r, 8);
         }
     } else {
         int32_t dcY[16], rmb[256];
         uint8_t pred[256];
-        refcpu_luma_dc(L->dc16, qp, dcY);
+        luma_dc_scan(L->dc16, qp, field, dcY);
         for (int b = 0; b < 16; b++) {
             int32_t list[16], r[16];
             int bx, by;
             blk_xy(b, &bx, &by);
             memcpy(list, L->l4[b], sizeof(list));
             list[0] = dcY[(by / 4) * 4 + bx / 4];  /* dcY raster index of block b */
-            refcpu_residual4x4(list, qp, 1, 0, r);
+            refcpu_residual4x4(list, qp, 1, field, r);
             for (int i = 0; i < 4; i++)
                 for (int j = 0; j < 4; j++) rmb[(by + i) * 16 + bx + j] = r[i * 4 + j];
         }
-        if ((rc = intra16_pred(X, mbx, mby, mb->intra16x16_pred_mode, pred))) return rc;
-        put_block(X->y, X->wl, x0, y0, pred, 16);
-        add_residual(X->y, X->wl, x0, y0, rmb, 16);
+        if ((rc = intra16_pred(X, addr, mb->intra16x16_pred_mode, pred))) return rc;
+        put_block(o, pitch, 0, 0, pred, 16);
+        add_residual(o, pitch, 0, 0, rmb, 16);
     }
+    mb_plane_origin(X, addr, 8, X->wc, &x0, &y0, &pitch);
     for (int c = 0; c < 2; c++) {
-        uint8_t *plane = c == 0 ? X->cb : X->cr;
+        uint8_t *plane = c == 0 ? X->cb : X->cr, *oc = plane + y0 * X->wc + x0;
         int off = c == 0 ? X->d->chroma_qp_index_offset : X->d->second_chroma_qp_index_offset;
         int32_t rmb[64];
         uint8_t pred[64];
-        chroma_residual(L, c, refcpu_chroma_qp(qp, off), rmb);
-        if ((rc = intra_chroma_pred(X, plane, mbx, mby, mb->intra_chroma_pred_mode, pred))) return rc;
-        put_block(plane, X->wc, mbx * 8, mby * 8, pred, 8);
-        add_residual(plane, X->wc, mbx * 8, mby * 8, rmb, 8);
+        chroma_residual(L, c, refcpu_chroma_qp(qp, off), field, rmb);
+        if ((rc = intra_chroma_pred(X, plane, addr, mb->intra_chroma_pred_mode, pred))) return rc;
+        put_block(oc, pitch, 0, 0, pred, 8);
+        add_residual(oc, pitch, 0, 0, rmb, 8);
     }
     return 0;
 }
 
 /* ---------------------------------------------- 8.7 deblocking */
-static int mv_far(const int *a, const int *b) { return iabs(a[0] - b[0]) >= 4 || iabs(a[1] - b[1]) >= 4; }
+/* |dx| >= 4 or |dy| >= ylim quarter samples; ylim is 2 where the q MB is a field MB of an
+ * MBAFF frame (Deblock.cpp:1157-1158) */
+static int mv_far(const int *a, const int *b, int ylim) { return iabs(a[0] - b[0]) >= 4 || iabs(a[1] - b[1]) >= ylim; }
 
-/* 8.7.2.1 (Deblock.cpp:994-1310) for luma positions (xp, yp) in MB p and (xq, yq) in MB q. */
-static int bs_luma(const MbInfo *P, const MbInfo *Q, int mb_edge, int xp, int yp, int xq, int yq) {
-    int pintra = is_intra_cls(P->cls), qintra = is_intra_cls(Q->cls);
-    if (mb_edge && (pintra || qintra)) return 4;
-    if (pintra || qintra) return 3;
+/* The bS 2 test of 8.7.2.1 (Deblock.cpp:1136-1148): non-zero levels in the transform
+ * block of p0 or q0. */
+static int bs_nonzero(const MbInfo *P, const MbInfo *Q, int xp, int yp, int xq, int yq) {
     int bp4 = 8 * (yp / 8) + 4 * (xp / 8) + 2 * ((yp % 8) / 4) + (xp % 8) / 4;
     int bq4 = 8 * (yq / 8) + 4 * (xq / 8) + 2 * ((yq % 8) / 4) + (xq % 8) / 4;
     int bp8 = 2 * (yp / 8) + xp / 8, bq8 = 2 * (yq / 8) + xq / 8;
-    if ((P->t8 && (P->nz8 >> bp8 & 1)) || (!P->t8 && (P->nz4 >> bp4 & 1)) ||
-        (Q->t8 && (Q->nz8 >> bq8 & 1)) || (!Q->t8 && (Q->nz4 >> bq4 & 1)))
-        return 2;
+    return (P->t8 && (P->nz8 >> bp8 & 1)) || (!P->t8 && (P->nz4 >> bp4 & 1)) ||
+           (Q->t8 && (Q->nz8 >> bq8 & 1)) || (!Q->t8 && (Q->nz4 >> bq4 & 1));
+}
+
+/* The bS 1 motion tests of 8.7.2.1 (Deblock.cpp:1170-1305) with quirk Q3. */
+static int bs_motion(const MbInfo *P, const MbInfo *Q, int xp, int yp, int xq, int yq, int ylim) {
     int rp = (yp / 4) * 4 + xp / 4, rq = (yq / 4) * 4 + xq / 4;  /* raster 4x4 */
     int p0 = P->refid[0][rp], p1 = P->refid[1][rp], q0 = Q->refid[0][rq], q1 = Q->refid[1][rq];
     int np = P->pred[0][rp] + P->pred[1][rp], nq = Q->pred[0][rq] + Q->pred[1][rq];
     if (!(((p0 == q0 && p1 == q1) || (p0 == q1 && p1 == q0)) && np == nq)) return 1;
     const int *mp0 = P->mv[0][rp], *mp1 = P->mv[1][rp], *mq0 = Q->mv[0][rq], *mq1 = Q->mv[1][rq];
     int fp0 = P->pred[0][rp], fp1 = P->pred[1][rp], fq0 = Q->pred[0][rq], fq1 = Q->pred[1][rq];
-    if (fp0 && !fp1 && fq0 && !fq1 && mv_far(mp0, mq0)) return 1;
-    if (fp0 && !fp1 && !fq0 && fq1 && mv_far(mp0, mq1)) return 1;
-    if (!fp0 && fp1 && fq0 && !fq1 && mv_far(mp1, mq0)) return 1;
-    if (!fp0 && fp1 && !fq0 && fq1 && mv_far(mp1, mq1)) return 1;
+    if (fp0 && !fp1 && fq0 && !fq1 && mv_far(mp0, mq0, ylim)) return 1;
+    if (fp0 && !fp1 && !fq0 && fq1 && mv_far(mp0, mq1, ylim)) return 1;
+    if (!fp0 && fp1 && fq0 && !fq1 && mv_far(mp1, mq0, ylim)) return 1;
+    if (!fp0 && fp1 && !fq0 && fq1 && mv_far(mp1, mq1, ylim)) return 1;
     if (fp0 && fp1 && p0 != p1 && fq0 && fq1 && ((q0 == p0 && q1 == p1) || (q0 == p1 && q1 == p0))) {
-        if (q0 == p0 && (mv_far(mp0, mq0) || mv_far(mp1, mq1))) return 1;
-        if (q0 == p1 && (mv_far(mp1, mq0) || mv_far(mp0, mq1))) return 1;
+        if (q0 == p0 && (mv_far(mp0, mq0, ylim) || mv_far(mp1, mq1, ylim))) return 1;
+        if (q0 == p1 && (mv_far(mp1, mq0, ylim) || mv_far(mp0, mq1, ylim))) return 1;
     }
     if (fp0 && fp1 && p0 == p1 && fq0 && fq1 && q0 == q1 && q0 == p0) {
         /* Q3: && binds tighter than || here (Deblock.cpp:1289-1299) */
-        if (mv_far(mp0, mq0) || (mv_far(mp1, mq1) && mv_far(mp0, mq1)) || mv_far(mp1, mq0)) return 1;
+        if (mv_far(mp0, mq0, ylim) || (mv_far(mp1, mq1, ylim) && mv_far(mp0, mq1, ylim)) || mv_far(mp1, mq0, ylim)) return 1;
     }
     return 0;
+}
+
+/* 8.7.2.1 (Deblock.cpp:994-1310) for luma positions (xp, yp) in MB p and (xq, yq) in MB q
+ * of a non-MBAFF frame. */
+static int bs_luma(const MbInfo *P, const MbInfo *Q, int mb_edge, int xp, int yp, int xq, int yq) {
+    int pintra = is_intra_cls(P->cls), qintra = is_intra_cls(Q->cls);
+    if (mb_edge && (pintra || qintra)) return 4;
+    if (pintra || qintra) return 3;
+    if (bs_nonzero(P, Q, xp, yp, xq, yq)) return 2;
+    return bs_motion(P, Q, xp, yp, xq, yq, 4);
+}
+
+/* 8.7.2.1 in an MBAFF frame (Deblock.cpp:994-1310): mixedModeEdgeFlag when p and q lie in
+ * a frame and a field MB; bS 4 for MB edges between frame MBs and for vertical MB edges,
+ * 3 for the other intra edges, 1 for mixed edges without levels. */
+static int bs_mbaff(const Ctx *X, int ap, int aq, int xp, int yp, int xq, int yq, int vertical) {
+    const MbInfo *P = &X->info[ap], *Q = &X->info[aq];
+    int intra = is_intra_cls(P->cls) || is_intra_cls(Q->cls);
+    int mixed = ap != aq && P->field != Q->field;
+    if (ap != aq && intra && ((!P->field && !Q->field) || vertical)) return 4;
+    if (intra && (!mixed || !vertical)) return 3;
+    if (bs_nonzero(P, Q, xp, yp, xq, yq)) return 2;
+    if (mixed) return 1;
+    return bs_motion(P, Q, xp, yp, xq, yq, Q->field ? 2 : 4);
 }
 
 /* 8.7.2.2 - 8.7.2.4 (Deblock.cpp:1314-1522) on p[0..3] / q[0..3] */
@@ -878,6 +1037,110 @@ static void filter_edge(Ctx *X, int mbx, int mby, int c, int vertical, int e) {
     }
 }
 
+/* One edge pass of MB addr in an MBAFF frame (Deblock.cpp:639-990): plane c; the lines
+ * E[k] = (e, k) (vertical) or (k, e) (horizontal); sample rows dy = 1 + fieldMode apart
+ * from the MB's position (yE % 2 selects the bottom field lines of a frame MB's top edge
+ * over a field pair); mbAddrN: the p-MB of an MB edge, alternating with the line's
+ * parity when leftMbEdge (frame MB beside a field pair).  The p0 position keeps the
+ * current MB's line numbering (Q6). */
+static void filter_edge_mbaff(Ctx *X, int addr, int c, int vertical, int e, int fieldMode, int leftMbEdge, int mbAddrN) {
+    const int chroma = c != 0, nE = chroma ? 8 : 16, dy = 1 + fieldMode, s = chroma ? 2 : 1;
+    uint8_t *plane = c == 0 ? X->y : (c == 1 ? X->cb : X->cr);
+    const int stride = chroma ? X->wc : X->wl;
+    int xI, yI;
+    mb_luma_pos(X, addr, &xI, &yI);
+    const int xP = chroma ? xI / 2 : xI, yP = chroma ? (yI + 1) / 2 : yI;
+    const MbInfo *Q = &X->info[addr];
+    /* Cb edges take the Cr offset and vice versa (Q16, see filter_edge) */
+    const int off = c == 1 ? X->d->second_chroma_qp_index_offset : X->d->chroma_qp_index_offset;
+    for (int k = 0; k < nE; k++) {
+        int xE = vertical ? e : k, yE = vertical ? k : e;
+        uint8_t pp[4], qq[4];
+        int qo[4], po[4];
+        for (int i = 0; i < 4; i++) {
+            if (vertical) {
+                qo[i] = (yP + dy * yE) * stride + xP + xE + i;
+                po[i] = (yP + dy * yE) * stride + xP + xE - i - 1;
+            } else {
+                qo[i] = (yP + dy * (yE + i) - yE % 2) * stride + xP + xE;
+                po[i] = (yP + dy * (yE - i - 1) - yE % 2) * stride + xP + xE;
+            }
+            qq[i] = plane[qo[i]];
+            pp[i] = plane[po[i]];
+        }
+        int ap = addr, xp, yp, xq, yq;
+        if (vertical) {
+            xp = xE - 1;
+            if (xp < 0) {
+                if (mbAddrN >= 0) ap = leftMbEdge ? mbAddrN + yE % 2 : mbAddrN;
+                xp += nE;
+            }
+            yp = yE; xq = xE; yq = yE;
+        } else {
+            xp = xE;
+            yp = (yE - 1) - yE % 2;
+            if (yp < 0) {
+                if (mbAddrN >= 0) ap = mbAddrN;
+                yp += nE;
+            }
+            xq = xE; yq = yE - yE % 2;
+        }
+        const MbInfo *P = &X->info[ap];
+        int bS = bs_mbaff(X, ap, addr, xp * s, yp * s, xq * s, yq * s, vertical);
+        int qpp = P->cls == H264G_MB_IPCM ? 0 : P->qp, qpq = Q->cls == H264G_MB_IPCM ? 0 : Q->qp;
+        if (chroma) { qpp = refcpu_chroma_qp(qpp, off); qpq = refcpu_chroma_qp(qpq, off); }
+        refcpu_filter_line(pp, qq, bS, chroma, (qpp + qpq + 1) >> 1, X->d->filter_offset_a, X->d->filter_offset_b);
+        for (int i = 0; i < 3; i++) {
+            plane[qo[i]] = qq[i];
+            plane[po[i]] = pp[i];
+        }
+    }
+}
+
+/* 8.7 for MBAFF frames, MBs in address order (Deblock.cpp:76-635): left / top edges
+ * when the pair has a left / upper neighbour (a frame MB's top edge inside its pair
+ * always); a frame MB under a field pair filters its top edge twice in field mode, top
+ * field lines against the pair's top MB, bottom field lines against its bottom MB
+ * (chroma: the bottom MB both times, Q5). */
+static void deblock_mbaff(Ctx *X) {
+    const int idc = X->d->disable_deblocking_filter_idc;
+    if (idc == 1) return;
+    for (int a = 0; a < X->W * X->H; a++) {
+        int xW, yW;
+        const int mbAddrA = mbaff_nb(X, a, -1, 0, 16, 16, &xW, &yW), mbAddrB = mbaff_nb(X, a, 0, -1, 16, 16, &xW, &yW);
+        const int field = mb_field(X, a), top = a % 2 == 0, row0 = (a >> 1) < X->W;
+        const int t8 = X->info[a].t8;
+        const int left = (a >> 1) % X->W != 0 && !(idc == 2 && mbAddrA < 0);
+        const int upper = !(row0 && (field || top)) && !(idc == 2 && mbAddrB < 0);
+        const int leftMix = a >= 2 && !field && mb_field(X, a - 2);
+        const int topMix = top && a >= 2 * X->W && !field && mb_field(X, a - 2 * X->W + 1);
+        if (left) filter_edge_mbaff(X, a, 0, 1, 0, field, leftMix, mbAddrA);
+        if (!t8) filter_edge_mbaff(X, a, 0, 1, 4, field, 0, mbAddrA);
+        filter_edge_mbaff(X, a, 0, 1, 8, field, 0, mbAddrA);
+        if (!t8) filter_edge_mbaff(X, a, 0, 1, 12, field, 0, mbAddrA);
+        if (upper && topMix) {
+            filter_edge_mbaff(X, a, 0, 0, 0, 1, 0, mbAddrB - 1);
+            filter_edge_mbaff(X, a, 0, 0, 1, 1, 0, mbAddrB);
+        } else if (upper) {
+            filter_edge_mbaff(X, a, 0, 0, 0, field, 0, mbAddrB);
+        }
+        if (!t8) filter_edge_mbaff(X, a, 0, 0, 4, field, 0, mbAddrB);
+        filter_edge_mbaff(X, a, 0, 0, 8, field, 0, mbAddrB);
+        if (!t8) filter_edge_mbaff(X, a, 0, 0, 12, field, 0, mbAddrB);
+        for (int c = 1; c <= 2; c++) {
+            if (left) filter_edge_mbaff(X, a, c, 1, 0, field, leftMix, mbAddrA);
+            filter_edge_mbaff(X, a, c, 1, 4, field, 0, mbAddrA);
+            if (upper && topMix) {
+                filter_edge_mbaff(X, a, c, 0, 0, 1, 0, mbAddrB);
+                filter_edge_mbaff(X, a, c, 0, 1, 1, 0, mbAddrB);
+            } else if (upper) {
+                filter_edge_mbaff(X, a, c, 0, 0, field, 0, mbAddrB);
+            }
+            filter_edge_mbaff(X, a, c, 0, 4, field, 0, mbAddrB);
+        }
+    }
+}
+
 /* 8.7 in MB raster order (Deblock.cpp:76-635) */
 static void deblock(Ctx *X) {
     if (X->d->disable_deblocking_filter_idc == 1) return;
@@ -905,11 +1168,12 @@ static void deblock(Ctx *X) {
 /* ------------------------------------------------------ picture */
 int refcpu_decode_picture(const h264g_pic_desc *d, const h264g_mb *mbs, int W, int H, const int16_t *levels,
                           uint8_t *const *planes, int stage) {
-    if (d->structure != H264G_FRAME || d->mbaff) return H264G_E_ARG;
+    if (d->structure != H264G_FRAME || (d->mbaff && (H % 2 || d->constrained_intra_pred_flag))) return H264G_E_ARG;
     Ctx X;
     X.d = d; X.mbs = mbs; X.W = W; X.H = H;
     X.wl = W * 16; X.hl = H * 16; X.wc = W * 8; X.hc = H * 8;
     X.planes = planes;
+    X.mbaff = d->mbaff ? 1 : 0;
     X.y = planes[d->dst_slot];
     X.cb = X.y + X.wl * X.hl;
     X.cr = X.cb + X.wc * X.hc;
@@ -920,10 +1184,13 @@ int refcpu_decode_picture(const h264g_pic_desc *d, const h264g_mb *mbs, int W, i
     for (int a = 0; a < W * H && !rc; a++) {
         const h264g_mb *mb = &mbs[a];
         MbInfo *info = &X.info[a];
-        int mbx = a % W, mby = a / W;
+        const int field = mb_field(&X, a);
+        if (X.mbaff && (mbs[a ^ 1].flags & H264G_MBF_FIELD) != (mb->flags & H264G_MBF_FIELD)) { rc = H264G_E_ARG; break; }
+        if (!X.mbaff && (mb->flags & H264G_MBF_FIELD)) { rc = H264G_E_ARG; break; }
         parse_levels(mb, levels, L);
         info->cls = mb->mb_type;
         info->qp = mb->qp_y;
+        info->field = field;
         info->t8 = (mb->flags & H264G_MBF_T8x8) && mb->mb_type != H264G_MB_IPCM ? 1 : 0;
         for (int b = 0; b < 16; b++) {
             int nz = 0;
@@ -937,47 +1204,55 @@ int refcpu_decode_picture(const h264g_pic_desc *d, const h264g_mb *mbs, int W, i
         }
         for (int l = 0; l < 2; l++)
             for (int b = 0; b < 16; b++) info->refid[l][b] = -1;
+        int x0, y0, pitch, cx0, cy0, cpitch;
+        mb_plane_origin(&X, a, 16, X.wl, &x0, &y0, &pitch);
+        mb_plane_origin(&X, a, 8, X.wc, &cx0, &cy0, &cpitch);
+        uint8_t *oy = X.y + y0 * X.wl + x0, *ocb = X.cb + cy0 * X.wc + cx0, *ocr = X.cr + cy0 * X.wc + cx0;
         if (mb->mb_type == H264G_MB_IPCM) {
-            /* RefIdxLX = 0 from the zeroed MB state (see file header) */
+            /* RefIdxLX = 0 from the zeroed MB state (see file header); samples in the MB's
+             * rows (H264PictureBase.cpp:2449-2499) */
             for (int l = 0; l < 2; l++)
                 for (int b = 0; b < 16; b++) info->refid[l][b] = d->num_ref_idx_active[l] > 0 ? d->ref_slot[l][0] : -1;
-            for (int i = 0; i < 256; i++) X.y[(mby * 16 + i / 16) * X.wl + mbx * 16 + i % 16] = (uint8_t)L->pcm[i];
+            for (int i = 0; i < 256; i++) oy[(i / 16) * pitch + i % 16] = (uint8_t)L->pcm[i];
             for (int i = 0; i < 64; i++) {
-                X.cb[(mby * 8 + i / 8) * X.wc + mbx * 8 + i % 8] = (uint8_t)L->pcm[256 + i];
-                X.cr[(mby * 8 + i / 8) * X.wc + mbx * 8 + i % 8] = (uint8_t)L->pcm[320 + i];
+                ocb[(i / 8) * cpitch + i % 8] = (uint8_t)L->pcm[256 + i];
+                ocr[(i / 8) * cpitch + i % 8] = (uint8_t)L->pcm[320 + i];
             }
         } else if (is_intra_cls(mb->mb_type)) {
-            rc = intra_mb(&X, mbx, mby, mb, L);
+            rc = intra_mb(&X, a, mb, L);
         } else {
-            rc = inter_mb(&X, mbx, mby, mb, info);
+            rc = inter_mb(&X, a, mb, info);
             if (!rc && mb->mb_type != H264G_MB_P_SKIP) {
                 /* residual added to the prediction (H264InterPrediction.cpp:22-407) */
                 int qp = mb->qp_y;
                 if (info->t8) {
                     for (int q = 0; q < 4; q++) {
                         int32_t r[64];
-                        refcpu_residual8x8(L->l8[q], qp, 0, r);
-                        add_residual(X.y, X.wl, mbx * 16 + (q % 2) * 8, mby * 16 + (q / 2) * 8, r, 8);
+                        refcpu_residual8x8(L->l8[q], qp, field, r);
+                        add_residual(oy, pitch, (q % 2) * 8, (q / 2) * 8, r, 8);
                     }
                 } else {
                     for (int b = 0; b < 16; b++) {
                         int32_t r[16];
                         int bx, by;
                         blk_xy(b, &bx, &by);
-                        luma4x4_residual(L, b, qp, r);
-                        add_residual(X.y, X.wl, mbx * 16 + bx, mby * 16 + by, r, 4);
+                        luma4x4_residual(L, b, qp, field, r);
+                        add_residual(oy, pitch, bx, by, r, 4);
                     }
                 }
                 for (int c = 0; c < 2; c++) {
                     int32_t rmb[64];
                     int off = c == 0 ? d->chroma_qp_index_offset : d->second_chroma_qp_index_offset;
-                    chroma_residual(L, c, refcpu_chroma_qp(qp, off), rmb);
-                    add_residual(c == 0 ? X.cb : X.cr, X.wc, mbx * 8, mby * 8, rmb, 8);
+                    chroma_residual(L, c, refcpu_chroma_qp(qp, off), field, rmb);
+                    add_residual(c == 0 ? ocb : ocr, cpitch, 0, 0, rmb, 8);
                 }
             }
         }
     }
-    if (!rc && stage >= 2) deblock(&X);
+    if (!rc && stage >= 2) {
+        if (X.mbaff) deblock_mbaff(&X);
+        else deblock(&X);
+    }
     free(X.info);
     free(L);
     (void)imin; (void)imax;

@@ -17,10 +17,13 @@ extern "C" {
 #endif
 
 /*
- * Reconstructs one picture (progressive frame, one slice) into planes[desc->dst_slot].
- * planes[s] points at slot s: Y (16W x 16H) followed by Cb and Cr (8W x 8H each),
- * stride = width.  stage: 1 = stop before deblocking, 2 = full reconstruction.
- * Returns 0, or a negative H264G_E_* code for syntax the reference rejects.
+ * Reconstructs one picture (a frame or an MBAFF frame, one slice) into
+ * planes[desc->dst_slot].  planes[s] points at slot s: Y (16W x 16H) followed by Cb and
+ * Cr (8W x 8H each), stride = width, then 8W zero bytes: field macroblocks of MBAFF
+ * frames read up to that far past the Cr plane, as the reference does past its frame
+ * buffer (quirk Q17 in refcpu.c).  stage: 1 = stop before deblocking, 2 = full
+ * reconstruction.  Returns 0, or a negative H264G_E_* code for syntax the reference
+ * rejects.
  */
 int refcpu_decode_picture(const h264g_pic_desc *desc, const h264g_mb *mbs, int width_mbs, int height_mbs,
                           const int16_t *levels, uint8_t *const *planes, int stage);

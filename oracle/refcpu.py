@@ -54,12 +54,14 @@ def _ptr(a: np.ndarray) -> int:
 
 
 class Dpb:
-    """Host DPB of one stream: ``slots`` pictures, each Y then Cb then Cr."""
+    """Host DPB of one stream: ``slots`` pictures, each Y then Cb then Cr, followed by a
+    zeroed tail of one chroma row that field macroblocks of MBAFF frames read past the
+    Cr plane (see refcpu.h)."""
 
     def __init__(self, width_mbs: int, height_mbs: int, slots: int):
         self.W, self.H = width_mbs, height_mbs
         self.slot_bytes = width_mbs * height_mbs * 384
-        self.planes = [np.zeros(self.slot_bytes, np.uint8) for _ in range(slots)]
+        self.planes = [np.zeros(self.slot_bytes + width_mbs * 8, np.uint8) for _ in range(slots)]
         self._ptrs = (c_void_p * slots)(*[p.ctypes.data for p in self.planes])
 
     def decode(self, desc, mbs: np.ndarray, levels: np.ndarray, stage: int = 2) -> int:
@@ -70,7 +72,7 @@ class Dpb:
 
     def split(self, slot: int):
         """(Y, Cb, Cr) views of one slot as 2-D arrays."""
-        p = self.planes[slot]
+        p = self.planes[slot][:self.slot_bytes]
         ny, nc = self.W * self.H * 256, self.W * self.H * 64
         return (p[:ny].reshape(self.H * 16, self.W * 16), p[ny:ny + nc].reshape(self.H * 8, self.W * 8),
                 p[ny + nc:].reshape(self.H * 8, self.W * 8))

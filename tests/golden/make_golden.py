@@ -53,11 +53,21 @@ CASES = {
     # syntax, I P B B
     "high_720p": (2, {"width_mbs": 80, "height_mbs": 45, "n_frames": 4, "seed": 0x5EED0004 + 16}),
     "high_2160p": (2, {"width_mbs": 240, "height_mbs": 135, "n_frames": 4, "seed": 0x5EED0004 + 56}),
+    # MBAFF frames (config 3 syntax: field / frame MB pairs 50/50, 4x4 transform): reduced
+    # size I P B B P B B P; 8x8 transforms (the 8x8 field scan) with I_PCM and idc 2; a
+    # two-pair-wide picture (edges on both sides of every pair); the full 1920x1088 stream
+    # (each frame buffer at least 256 KiB, which glibc always maps on pages of its own:
+    # ref_harness checks that the bytes the reference reads past it are zero)
+    "mbaff_small": (3, {"width_mbs": 32, "height_mbs": 22, "seed": 2}),
+    "mbaff_t8_pcm": (3, {"width_mbs": 28, "height_mbs": 26, "n_frames": 6, "seed": 4, "t8x8": 1,
+                         "ipcm_permille": 30, "disable_deblocking_filter_idc": 2}),
+    "mbaff_tall": (3, {"width_mbs": 2, "height_mbs": 344, "n_frames": 4, "seed": 5}),
+    "mbaff_1080p": (3, {}),
 }
 
 _CLI = {"n_frames": "--frames", "width_mbs": "--width", "height_mbs": "--height", "seed": "--seed",
         "ipcm_permille": "--ipcm", "disable_deblocking_filter_idc": "--deblock", "max_refs": "--refs",
-        "b_frames": "--bframes"}
+        "b_frames": "--bframes", "t8x8": "--t8x8", "mbaff": "--mbaff"}
 
 
 def harness_args(config: int, ov: dict) -> list:

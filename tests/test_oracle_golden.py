@@ -10,8 +10,9 @@ import pytest
 from helpers import make_generator, oracle_decode, plane_hashes
 
 CPU_CASES = ["cif_ip", "cif_ip_seed2", "cif_ip_seed3", "cif_pcm_cqp", "high_small", "high_small_seed7",
-             "high_implicit", "high_default_nodbk", "narrow", "flat", "high_1080p", "high_720p", "high_2160p"]
-BIG = ("high_1080p", "high_720p", "high_2160p")
+             "high_implicit", "high_default_nodbk", "narrow", "flat", "high_1080p", "high_720p", "high_2160p",
+             "mbaff_small", "mbaff_t8_pcm", "mbaff_tall", "mbaff_1080p"]
+BIG = ("high_1080p", "high_720p", "high_2160p", "mbaff_1080p")
 
 
 def test_golden_covers_cases(golden):
@@ -38,7 +39,7 @@ def test_oracle_matches_reference(golden, name):
         assert plane_hashes(*out[2]) == case["pic"][i], f"{name} picture {i}"
 
 
-@pytest.mark.parametrize("name", ["cif_ip", "high_small", "flat"])
+@pytest.mark.parametrize("name", ["cif_ip", "high_small", "flat", "mbaff_small"])
 def test_oracle_bgr24_matches_reference(golden, name):
     """YUV420P -> BGR24 output conversion against the reference's convertYuv420pToBgr24."""
     import hashlib

@@ -115,6 +115,9 @@ def test_dc_only_blocks_are_flat(qp):
 
 
 def test_residual_field_scan_differs_from_zigzag():
+    """A sanity check only: the field scans themselves are pinned end to end by the MBAFF
+    golden cases (field macroblocks; mbaff_t8_pcm has 8x8 transforms) in
+    test_oracle_golden.py."""
     s = np.zeros(16, np.int32)
     s[2] = 10  # zig-zag puts scan 2 at (1,0), field scan at (0,1) -> transposed residuals
     a = refcpu.residual4x4(s, 30, field=0)
