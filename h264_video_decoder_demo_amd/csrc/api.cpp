@@ -49,6 +49,8 @@ struct Stream {
     uint64_t *rowprog = nullptr;
     uint8_t *gstrip = nullptr;     // deblocking strips between row bands, then the intra ones
     uint64_t *istrip = nullptr;    // the intra strips inside gstrip
+    uint32_t *pairprog = nullptr;  // MBAFF streams: deblocking progress per MB pair
+    bool mbaff = false;            // MBAFF frames (pairs of field / frame MBs)
     uint64_t strip_bytes = 0;      // size of gstrip, sized for bands of
     int strip_dbk_rows = 0, strip_intra_rows = 0;  // at least these many rows
     uint64_t bytes = 0;
@@ -104,8 +106,13 @@ struct h264g_pic {
     DevPic *d_pic = nullptr;
     uint32_t n_inter = 0, n_intra = 0, n_mbs = 0, n_rows = 0;
     DevPic hpic;                   // host copy (batches re-pack it)
-    std::vector<uint32_t> intra_t, all_t;  // host copies of the t-ordered queues (MB addresses)
+    std::vector<uint32_t> intra_t, all_t;  // host copies of the t-ordered queues (MB addresses;
+                                           // MBAFF frames: intra MBs in pair t order, and
+                                           // all_t holds MB pair indices)
     std::vector<uint32_t> inter_list;
+    bool mbaff = false;            // an MBAFF frame: intra on the per-MB wavefront, deblocking
+                                   // over MB pairs (the band schedules take frame MBs only)
+    uint32_t n_pairs = 0;
     uint64_t alg[3] = {0, 0, 0};
 };
 
@@ -119,6 +126,8 @@ struct Batch {
     uint32_t *d_inter = nullptr, *d_intra = nullptr, *d_all = nullptr, *d_rows = nullptr;
     uint32_t *d_ibands = nullptr, *d_dbands = nullptr;   // (picture, band) items, band-major
     uint32_t n_inter = 0, n_intra = 0, n_all = 0, n_rows = 0, n_ibands = 0, n_dbands = 0;
+    uint32_t *d_mintra = nullptr, *d_mpairs = nullptr;   // MBAFF frames: intra MBs, MB pairs
+    uint32_t n_mintra = 0, n_mpairs = 0;
     bool dbk_off = false;            // no picture of the batch is deblocked
     hipGraphExec_t graph = nullptr;  // the captured stage sequence (kernel nodes only)
     hipGraphExec_t graph_dbk = nullptr;  // its deblocking, when that runs on the lane's deblocking stream
@@ -379,7 +388,12 @@ int stage_picture(h264g_ctx *ctx, Stream &st, const h264g_pic_desc &d, const h26
                   const int16_t *levels, uint64_t n_levels, h264g_pic *pic, void *fixed = nullptr,
                   size_t fixed_cap = 0) {
     const int W = st.W, H = st.H;
-    if (d.structure != H264G_FRAME || d.mbaff) return h264g_set_error(H264G_E_ARG, "only progressive frame pictures are supported");
+    if (d.structure != H264G_FRAME) return h264g_set_error(H264G_E_ARG, "field pictures are not supported");
+    if ((d.mbaff != 0) != st.mbaff)
+        return h264g_set_error(H264G_E_ARG, "picture mbaff %d on a stream opened with mbaff %d", d.mbaff, (int)st.mbaff);
+    const bool mbaff = st.mbaff;
+    if (mbaff && d.constrained_intra_pred_flag)
+        return h264g_set_error(H264G_E_ARG, "constrained intra prediction in MBAFF frames is not supported");
     if (d.slice_type < 0 || d.slice_type > 2) return h264g_set_error(H264G_E_ARG, "slice_type %d", d.slice_type);
     if ((int)n_mbs != W * H) return h264g_set_error(H264G_E_ARG, "n_mbs %u != %d", n_mbs, W * H);
     const int nslots = st.desc.dpb_slots;
@@ -394,6 +408,7 @@ int stage_picture(h264g_ctx *ctx, Stream &st, const h264g_pic_desc &d, const h26
         d.filter_offset_b < -12 || d.filter_offset_b > 12)
         return h264g_set_error(H264G_E_ARG, "deblocking parameters");
     std::vector<DevAux> aux(n_mbs);
+    std::vector<int8_t> bsref(mbaff ? (size_t)n_mbs * 8 : 0, (int8_t)-1);
     std::vector<DevWeight> wt;
     std::map<int, int> wmap;
     {
@@ -408,6 +423,7 @@ int stage_picture(h264g_ctx *ctx, Stream &st, const h264g_pic_desc &d, const h26
         std::memset(&x, 0, sizeof(x));
         for (int l = 0; l < 2; l++) for (int q = 0; q < 4; q++) x.refid[l][q] = -1;
         const int mbx = a % W, mby = a / W, cls = m.mb_type;
+        const bool field = mbaff && (m.flags & H264G_MBF_FIELD);
         if (cls > H264G_MB_B) return h264g_set_error(H264G_E_ARG, "mb %u: mb_type %d", a, cls);
         if (m.qp_y > 51) return h264g_set_error(H264G_E_ARG, "mb %u: qp %d", a, m.qp_y);
         if (m.level_offset > n_levels) return h264g_set_error(H264G_E_ARG, "mb %u: level_offset", a);
@@ -425,7 +441,9 @@ int stage_picture(h264g_ctx *ctx, Stream &st, const h264g_pic_desc &d, const h26
         if (cls_inter(cls) && t8 && ((cbp & 15) == 0 || (m.part_shape == H264G_PART_8x8 &&
                                                         (m.sub_part_shape[0] | m.sub_part_shape[1] | m.sub_part_shape[2] | m.sub_part_shape[3]))))
             return h264g_set_error(H264G_E_ARG, "mb %u: transform_size_8x8_flag without luma residual or with sub-8x8 partitions", a);
-        if (m.flags & H264G_MBF_FIELD) return h264g_set_error(H264G_E_ARG, "mb %u: field MB in a frame picture", a);
+        if (!mbaff && (m.flags & H264G_MBF_FIELD)) return h264g_set_error(H264G_E_ARG, "mb %u: field MB in a frame picture", a);
+        if (mbaff && ((m.flags ^ mbs[a ^ 1u].flags) & H264G_MBF_FIELD))
+            return h264g_set_error(H264G_E_ARG, "mb %u: the MBs of a pair differ in mb_field_decoding_flag", a);
         if (d.slice_type == H264G_SLICE_I && cls_inter(cls)) return h264g_set_error(H264G_E_ARG, "mb %u: inter MB in I slice", a);
         if (d.slice_type != H264G_SLICE_B && cls == H264G_MB_B) return h264g_set_error(H264G_E_ARG, "mb %u: B MB outside B slice", a);
         if (d.slice_type != H264G_SLICE_P && (cls == H264G_MB_P || cls == H264G_MB_P_SKIP)) return h264g_set_error(H264G_E_ARG, "mb %u: P MB outside P slice", a);
@@ -442,6 +460,7 @@ int stage_picture(h264g_ctx *ctx, Stream &st, const h264g_pic_desc &d, const h26
             // RefIdxLX[0] = 0 and PredFlagLX = 0 from the zeroed MB state (Deblock.cpp:1175-1183)
             for (int l = 0; l < 2; l++)
                 for (int q = 0; q < 4; q++) x.refid[l][q] = (int8_t)(d.num_ref_idx_active[l] > 0 ? d.ref_slot[l][0] : -1);
+            if (mbaff) std::memcpy(&bsref[(size_t)a * 8], x.refid, 8);
             alg_inter += 384 + 384 * 2 + 16;
             continue;
         }
@@ -478,6 +497,14 @@ int stage_picture(h264g_ctx *ctx, Stream &st, const h264g_pic_desc &d, const h26
                 return true;
             };
             bool A = avail(-1, 0), B = avail(0, -1), D = avail(-1, -1);
+            if (mbaff) {  // 6.4.10 / Table 6-4: a frame MB at the bottom of its pair has its top
+                          // MB above and the left pair above left; other MBs the pairs around
+                const int pair = (int)(a >> 1), px = pair % W, py = pair / W;
+                const bool in_pair = !field && (a & 1);
+                A = px > 0;
+                B = in_pair || py > 0;
+                D = in_pair ? px > 0 : (py > 0 && px > 0);
+            }
             if (cls == H264G_MB_I4x4 || cls == H264G_MB_I8x8) {
                 int n = cls == H264G_MB_I4x4 ? 4 : 2;
                 for (int by = 0; by < n; by++)
@@ -508,18 +535,29 @@ int stage_picture(h264g_ctx *ctx, Stream &st, const h264g_pic_desc &d, const h26
             int r[2] = {m.ref_idx[0][q], m.ref_idx[1][q]};
             if (r[0] < 0 && r[1] < 0) return h264g_set_error(H264G_E_ARG, "mb %u: quadrant %d predicts from no list", a, q);
             if (cls != H264G_MB_B && r[1] >= 0) return h264g_set_error(H264G_E_ARG, "mb %u: list 1 in a P MB", a);
+            // field MBs of MBAFF frames: refIdx addresses the field list, the fields of
+            // RefPicListX[refIdx / 2] (same parity as the MB when even), and weights sit at
+            // refIdx >> 1 (H264InterPrediction.cpp:2169-2197, 2993-3031)
+            const int nfield = field ? 2 : 1;
             for (int l = 0; l < 2; l++) {
-                if (r[l] < -1 || r[l] >= d.num_ref_idx_active[l]) return h264g_set_error(H264G_E_ARG, "mb %u: ref_idx[%d][%d] = %d", a, l, q, r[l]);
+                if (r[l] < -1 || r[l] >= d.num_ref_idx_active[l] * nfield)
+                    return h264g_set_error(H264G_E_ARG, "mb %u: ref_idx[%d][%d] = %d", a, l, q, r[l]);
                 if (r[l] >= 0) {
-                    x.refid[l][q] = (int8_t)d.ref_slot[l][r[l]];
+                    x.refid[l][q] = (int8_t)d.ref_slot[l][field ? r[l] >> 1 : r[l]];
                     x.pf |= (uint8_t)(1 << (4 * l + q));
+                    if (field && ((r[l] & 1) ^ (int)(a & 1))) x.fpar |= (uint8_t)(1 << (4 * l + q));
+                    // bS compares RefPicListX[RefIdxLX] even for field MBs (quirk Q4)
+                    if (mbaff) bsref[(size_t)a * 8 + 4 * l + q] = (int8_t)d.ref_slot[l][r[l]];
                 }
             }
-            int key = (r[0] + 1) * 64 + (r[1] + 1);
+            if (field && d.slice_type == H264G_SLICE_B && d.weighted_bipred_idc == 2 && r[0] >= 0 && r[1] >= 0)
+                return h264g_set_error(H264G_E_ARG, "mb %u: implicit weights of field MBs are not supported", a);
+            const int wr0 = field && r[0] >= 0 ? r[0] >> 1 : r[0], wr1 = field && r[1] >= 0 ? r[1] >> 1 : r[1];
+            int key = (wr0 + 1) * 64 + (wr1 + 1);
             auto it = wmap.find(key);
             if (it == wmap.end()) {
                 DevWeight w;
-                int rc = derive_weight(d, r[0], r[1], w);
+                int rc = derive_weight(d, wr0, wr1, w);
                 if (rc) return rc;
                 it = wmap.emplace(key, (int)wt.size()).first;
                 wt.push_back(w);
@@ -553,18 +591,37 @@ int stage_picture(h264g_ctx *ctx, Stream &st, const h264g_pic_desc &d, const h26
     pic->all_t.clear();
     for (uint32_t a = 0; a < n_mbs; a++)
         if (aux[a].kind & (AUX_INTER | AUX_PCM)) pic->inter_list.push_back(a);
-    for (int t = 0; t <= (W - 1) + 2 * (H - 1); t++)
-        for (int y = 0; y < H; y++) {
-            int xx = t - 2 * y;
-            if (xx < 0) break;
-            if (xx >= W) continue;
-            uint32_t a = (uint32_t)(y * W + xx);
-            pic->all_t.push_back(a);
-            if (aux[a].kind & AUX_INTRA) pic->intra_t.push_back(a);
-        }
+    pic->mbaff = mbaff;
+    if (mbaff) {  // pairs by t = x + 2 y, the top MB before the bottom one
+        const int HP = H / 2;
+        for (int t = 0; t <= (W - 1) + 2 * (HP - 1); t++)
+            for (int y = 0; y < HP; y++) {
+                const int xx = t - 2 * y;
+                if (xx < 0) break;
+                if (xx >= W) continue;
+                const uint32_t pair = (uint32_t)(y * W + xx);
+                pic->all_t.push_back(pair);
+                for (uint32_t a = 2 * pair; a < 2 * pair + 2; a++)
+                    if (aux[a].kind & AUX_INTRA) {
+                        aux[a].ires |= (uint32_t)pic->intra_t.size();  // residual slot
+                        pic->intra_t.push_back(a);
+                    }
+            }
+        pic->n_pairs = (uint32_t)pic->all_t.size();
+    } else {
+        for (int t = 0; t <= (W - 1) + 2 * (H - 1); t++)
+            for (int y = 0; y < H; y++) {
+                int xx = t - 2 * y;
+                if (xx < 0) break;
+                if (xx >= W) continue;
+                uint32_t a = (uint32_t)(y * W + xx);
+                pic->all_t.push_back(a);
+                if (aux[a].kind & AUX_INTRA) pic->intra_t.push_back(a);
+            }
+    }
     pic->n_inter = (uint32_t)pic->inter_list.size();
     pic->n_intra = (uint32_t)pic->intra_t.size();
-    pic->n_mbs = n_mbs;
+    pic->n_mbs = mbaff ? pic->n_pairs : n_mbs;  // entries of all_t
     pic->alg[0] = alg_inter;
     pic->alg[1] = alg_intra;
     // deblocking: one read and one write of the picture, the bS inputs (motion records,
@@ -574,7 +631,7 @@ int stage_picture(h264g_ctx *ctx, Stream &st, const h264g_pic_desc &d, const h26
     auto al = [](size_t v) { return (v + 255) & ~(size_t)255; };
     size_t o_mbs = 0, o_aux = al(o_mbs + n_mbs * sizeof(h264g_mb)), o_lv = al(o_aux + n_mbs * sizeof(DevAux));
     size_t o_wt = al(o_lv + n_levels * sizeof(int16_t) + 16), o_it = al(o_wt + wt.size() * sizeof(DevWeight));
-    pic->n_rows = (uint32_t)H;
+    pic->n_rows = mbaff ? 0u : (uint32_t)H;
     // intra MBs per row: offsets, then one entry per intra MB (the per-picture intra band
     // schedule; an MB's residual sits at its entry's index in P.ires).  The entry carries
     // what the band kernel would otherwise load from the neighbours' records before its
@@ -583,7 +640,7 @@ int stage_picture(h264g_ctx *ctx, Stream &st, const h264g_pic_desc &d, const h26
     // neighbours are intra MBs of this stage (C only when the MB reads its samples:
     // Intra_8x8, or Intra_4x4 block 5 in modes 3 / 7), and whether it has levels.
     std::vector<uint32_t> irow(H + 1, 0), icol;
-    for (int y = 0; y < H; y++) {
+    for (int y = 0; y < H && !mbaff; y++) {
         irow[y] = (uint32_t)icol.size();
         for (int xx = 0; xx < W; xx++) {
             const uint32_t a = (uint32_t)(y * W + xx);
@@ -607,9 +664,10 @@ int stage_picture(h264g_ctx *ctx, Stream &st, const h264g_pic_desc &d, const h26
         }
     }
     irow[H] = (uint32_t)icol.size();
-    size_t n_items = pic->n_inter + pic->n_intra + n_mbs + (size_t)H + irow.size() + icol.size();
+    size_t n_items = pic->n_inter + pic->n_intra + pic->all_t.size() + pic->n_rows + irow.size() + icol.size();
     size_t o_bs = al(o_it + n_items * sizeof(uint32_t)), o_dbq = al(o_bs + (size_t)n_mbs * 32);
-    size_t o_ires = al(o_dbq + (size_t)n_mbs * 4), o_pic = al(o_ires + (size_t)pic->n_intra * 768);
+    size_t o_ires = al(o_dbq + (size_t)n_mbs * 4), o_bsref = al(o_ires + (size_t)pic->n_intra * 768);
+    size_t o_pic = al(o_bsref + bsref.size());
     size_t total = al(o_pic + sizeof(DevPic));
     void *dm = fixed;
     if (fixed) {
@@ -637,7 +695,7 @@ int stage_picture(h264g_ctx *ctx, Stream &st, const h264g_pic_desc &d, const h26
     items.insert(items.end(), pic->inter_list.begin(), pic->inter_list.end());
     items.insert(items.end(), pic->intra_t.begin(), pic->intra_t.end());
     items.insert(items.end(), pic->all_t.begin(), pic->all_t.end());
-    for (int y = 0; y < H; y++) items.push_back((uint32_t)y);
+    for (uint32_t y = 0; y < pic->n_rows; y++) items.push_back(y);
     const size_t irow_at = items.size();
     items.insert(items.end(), irow.begin(), irow.end());
     items.insert(items.end(), icol.begin(), icol.end());
@@ -648,12 +706,16 @@ int stage_picture(h264g_ctx *ctx, Stream &st, const h264g_pic_desc &d, const h26
     P.ires = (int16_t *)(base + o_ires);
     P.irow = pic->d_items + irow_at; P.n_intra = pic->n_intra;
     P.W = W; P.H = H; P.offA = d.filter_offset_a; P.offB = d.filter_offset_b;
+    P.mbaff = mbaff ? 1 : 0;
+    P.bsref = mbaff ? (const int8_t *)(base + o_bsref) : nullptr;
+    P.pairprog = st.pairprog;
     P.dbk_idc = d.disable_deblocking_filter_idc; P.cip = d.constrained_intra_pred_flag;
     HIPCHK(hipMemcpy(pic->d_mbs, mbs, n_mbs * sizeof(h264g_mb), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(pic->d_aux, aux.data(), n_mbs * sizeof(DevAux), hipMemcpyHostToDevice));
     if (n_levels) HIPCHK(hipMemcpy(pic->d_levels, levels, n_levels * sizeof(int16_t), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(pic->d_wt, wt.data(), wt.size() * sizeof(DevWeight), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(pic->d_items, items.data(), items.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    if (mbaff) HIPCHK(hipMemcpy(base + o_bsref, bsref.data(), bsref.size(), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(pic->d_pic, &P, sizeof(DevPic), hipMemcpyHostToDevice));
     return 0;
 }
@@ -667,6 +729,9 @@ struct StageSet {
     const uint32_t *inter, *intra, *all, *rows, *ibands, *dbands;
     uint32_t n_inter, n_intra, n_all, n_rows, n_ibands, n_dbands;
     bool dbk_off;
+    // MBAFF frames: intra MBs for the per-MB wavefront, MB pairs for their deblocking
+    const uint32_t *mintra = nullptr, *mpairs = nullptr;
+    uint32_t n_mintra = 0, n_mpairs = 0;
 };
 
 // Parts of a stage set: inter / intra / bS, and deblocking.
@@ -681,6 +746,9 @@ int launch_stages(h264g_ctx *ctx, const StageSet &q, int lane, int parts, hipStr
         if (ev) HIPCHK(hipEventRecord(ev[0], s));
         e = h264g_launch_inter(q.pics, q.inter, q.n_inter, q.intra, q.n_intra, ctx->d_err, s);
         if (e) return h264g_set_error(H264G_E_DEVICE, "inter launch: %s", hipGetErrorString((hipError_t)e));
+        if (q.n_mintra)  // residuals of the intra MBs of MBAFF frames (their own list)
+            e = h264g_launch_inter(q.pics, q.inter, 0, q.mintra, q.n_mintra, ctx->d_err, s);
+        if (e) return h264g_set_error(H264G_E_DEVICE, "inter launch (MBAFF residuals): %s", hipGetErrorString((hipError_t)e));
         if (ev) HIPCHK(hipEventRecord(ev[1], s));
         if (q.n_intra)
             e = ctx->intra_schedule == H264G_INTRA_BANDS
@@ -688,6 +756,9 @@ int launch_stages(h264g_ctx *ctx, const StageSet &q, int lane, int parts, hipStr
                                               ctx->d_err, s)
                     : h264g_launch_intra(q.pics, q.intra, q.n_intra, &ctl[0], &ctl[2].ticket, ctx->d_err, ctx->waves, s);
         if (e) return h264g_set_error(H264G_E_DEVICE, "intra launch: %s", hipGetErrorString((hipError_t)e));
+        if (q.n_mintra)
+            e = h264g_launch_intra(q.pics, q.mintra, q.n_mintra, &ctl[0], &ctl[2].ticket, ctx->d_err, ctx->waves, s);
+        if (e) return h264g_set_error(H264G_E_DEVICE, "intra launch (MBAFF): %s", hipGetErrorString((hipError_t)e));
         if (ev) HIPCHK(hipEventRecord(ev[2], s));
         if (!q.dbk_off) {
             e = h264g_launch_bs(q.pics, q.all, q.n_all, s);
@@ -711,6 +782,8 @@ int launch_stages(h264g_ctx *ctx, const StageSet &q, int lane, int parts, hipStr
                 e = h264g_launch_deblock_rows(q.pics, q.rows, q.n_rows, &ctl[1], epoch, ctx->d_err, ctx->waves, s);
             }
             if (e) return h264g_set_error(H264G_E_DEVICE, "deblock launch: %s", hipGetErrorString((hipError_t)e));
+            e = h264g_launch_deblock_mbaff(q.pics, q.mpairs, q.n_mpairs, &ctl[1], epoch, ctx->d_err, ctx->waves, s);
+            if (e) return h264g_set_error(H264G_E_DEVICE, "deblock launch (MBAFF): %s", hipGetErrorString((hipError_t)e));
         }
         if (ev) HIPCHK(hipEventRecord(ev[5], s));
     }
@@ -912,6 +985,7 @@ int32_t h264g_destroy(h264g_ctx *c) {
         if (s.flags) (void)hipFree(s.flags);
         if (s.rowprog) (void)hipFree(s.rowprog);
         if (s.gstrip) (void)hipFree(s.gstrip);
+        if (s.pairprog) (void)hipFree(s.pairprog);
     }
     for (auto &e : c->ev) if (e) (void)hipEventDestroy(e);
     if (c->d_err) (void)hipFree(c->d_err);
@@ -935,18 +1009,25 @@ int32_t h264g_stream_open(h264g_ctx *c, const h264g_stream_desc *d, uint32_t *id
     if (d->dpb_slots < 1 || d->dpb_slots > 17) return h264g_set_error(H264G_E_ARG, "dpb_slots");
     if (d->frame_height_in_mbs > 1024) return h264g_set_error(H264G_E_ARG, "frame_height_in_mbs > 1024");
     if (d->pic_width_in_mbs > (int)ICOL_X_MASK + 1) return h264g_set_error(H264G_E_ARG, "pic_width_in_mbs > 2048");
-    if (!d->frame_mbs_only || d->mbaff) return h264g_set_error(H264G_E_ARG, "field coding is not supported");
+    if (d->frame_mbs_only && d->mbaff) return h264g_set_error(H264G_E_ARG, "mbaff needs frame_mbs_only = 0");
+    if (!d->frame_mbs_only && !d->mbaff) return h264g_set_error(H264G_E_ARG, "field pictures are not supported");
+    if (d->mbaff && d->frame_height_in_mbs % 2) return h264g_set_error(H264G_E_ARG, "MBAFF frames need an even height");
     Stream s;
     s.desc = *d;
     s.W = d->pic_width_in_mbs;
     s.H = d->frame_height_in_mbs;
-    s.slot_bytes = (uint64_t)s.W * s.H * 384;
+    s.mbaff = d->mbaff != 0;
+    // A slot is Y, Cb, Cr; MBAFF slots end in a zeroed chroma row: field MBs read up to one
+    // chroma row past the Cr plane, as the reference does past its frame buffer (kernels.hip
+    // luma_list), and find zeros there as the reference does
+    s.slot_bytes = (uint64_t)s.W * s.H * 384 + (s.mbaff ? ((uint64_t)s.W * 8 + 255) / 256 * 256 : 0);
     s.lane = c->split_mbs > 0 && s.W * s.H >= c->split_mbs ? 1 : 0;
     // DPB slots, per-MB progress words, per-row progress words, and the strips between
     // row bands: deblocking (256 B per MB column and band), then intra (64 B)
     const uint64_t dstrip_bytes = (uint64_t)((s.H + c->dbk_waves - 1) / c->dbk_waves) * s.W * 256;
     const uint64_t strip_bytes = dstrip_bytes + (uint64_t)((s.H + c->intra_waves - 1) / c->intra_waves) * s.W * 64;
-    uint64_t need = s.slot_bytes * d->dpb_slots + (uint64_t)s.W * s.H * 4 + (uint64_t)s.H * 8 + strip_bytes;
+    const size_t pair_bytes = s.mbaff ? (size_t)s.W * s.H / 2 * 4 : 0;
+    uint64_t need = s.slot_bytes * d->dpb_slots + (uint64_t)s.W * s.H * 4 + (uint64_t)s.H * 8 + strip_bytes + pair_bytes;
     if (c->budget && c->used + need > c->budget)
         return h264g_set_error(H264G_E_BUDGET, "DPB budget: %llu + %llu > %llu", (unsigned long long)c->used,
                                (unsigned long long)need, (unsigned long long)c->budget);
@@ -959,11 +1040,13 @@ int32_t h264g_stream_open(h264g_ctx *c, const h264g_stream_desc *d, uint32_t *id
     s.flags = (uint32_t *)pool_get(c, flag_bytes);
     s.rowprog = (uint64_t *)pool_get(c, row_bytes);
     s.gstrip = (uint8_t *)pool_get(c, strip_bytes);
-    if (!s.dpb || !s.flags || !s.rowprog || !s.gstrip) {
+    if (pair_bytes) s.pairprog = (uint32_t *)pool_get(c, pair_bytes);
+    if (!s.dpb || !s.flags || !s.rowprog || !s.gstrip || (pair_bytes && !s.pairprog)) {
         pool_put(c, s.dpb, dpb_bytes);
         pool_put(c, s.flags, flag_bytes);
         pool_put(c, s.rowprog, row_bytes);
         pool_put(c, s.gstrip, strip_bytes);
+        pool_put(c, s.pairprog, pair_bytes);
         return h264g_set_error(H264G_E_NOMEM, "stream device memory");
     }
     hipStream_t ls = lane_stream(c, s.lane);
@@ -971,6 +1054,7 @@ int32_t h264g_stream_open(h264g_ctx *c, const h264g_stream_desc *d, uint32_t *id
     HIPCHK(hipMemsetAsync(s.gstrip, 0, strip_bytes, ls));  // epoch 0 never matches a launch
     HIPCHK(hipMemsetAsync(s.dpb, 0, dpb_bytes, ls));
     HIPCHK(hipMemsetAsync(s.flags, 0, flag_bytes, ls));
+    if (pair_bytes) HIPCHK(hipMemsetAsync(s.pairprog, 0, pair_bytes, ls));
     s.istrip = (uint64_t *)(s.gstrip + dstrip_bytes);
     s.strip_bytes = strip_bytes;
     s.strip_dbk_rows = c->dbk_waves;
@@ -1014,6 +1098,7 @@ int32_t h264g_stream_close(h264g_ctx *c, uint32_t id) {
     pool_put(c, s.flags, (size_t)s.W * s.H * 4);
     pool_put(c, s.rowprog, (size_t)s.H * 8);
     pool_put(c, s.gstrip, s.strip_bytes);
+    if (s.pairprog) pool_put(c, s.pairprog, (size_t)s.W * s.H / 2 * 4);
     c->used -= s.bytes;
     s = Stream();
     return H264G_OK;
@@ -1072,6 +1157,13 @@ int enqueue_picture(h264g_ctx *c, const h264g_pic *p, bool events = true) {
     q.n_ibands = (uint32_t)((p->H + c->intra_waves - 1) / c->intra_waves);
     q.n_dbands = (uint32_t)((p->H + c->dbk_waves - 1) / c->dbk_waves);
     q.dbk_off = p->desc.disable_deblocking_filter_idc == 1;
+    if (p->mbaff) {  // items: inter MBs, intra MBs (pair t order), MB pairs (t order)
+        q.mintra = q.intra;
+        q.n_mintra = p->n_intra;
+        q.mpairs = q.all;
+        q.n_mpairs = q.dbk_off ? 0u : p->n_pairs;
+        q.n_intra = q.n_all = q.n_rows = q.n_ibands = q.n_dbands = 0;
+    }
     return enqueue(c, q, c->streams[p->stream].lane, {slot_use(p->stream, p->desc)}, nullptr, events);
 }
 
@@ -1166,24 +1258,30 @@ int build_sub(h264g_ctx *c, const std::vector<const h264g_pic *> &pics, Batch *b
         maxt = std::max(maxt, (p->W - 1) + 2 * (p->H - 1));
     }
     // Wavefront queues of all pictures merged by t = x + 2y (counting sort): every
-    // dependency of an entry has a smaller t and therefore an earlier ticket.
-    auto merge = [&](bool intra_only, std::vector<uint32_t> &outq) {
+    // dependency of an entry has a smaller t and therefore an earlier ticket.  MBAFF frames
+    // (mbaff = true) go by the t of their MB pairs, the top MB before the bottom one.
+    auto merge = [&](bool intra_only, bool mbaff, std::vector<uint32_t> &outq) {
         std::vector<std::vector<uint32_t>> byt(maxt + 1);
         for (uint32_t i = 0; i < n; i++) {
             const h264g_pic *p = pics[i];
+            if (p->mbaff != mbaff || (!intra_only && p->desc.disable_deblocking_filter_idc == 1)) continue;
             const std::vector<uint32_t> &q = intra_only ? p->intra_t : p->all_t;
             for (uint32_t a : q) {
-                int t = (int)(a % p->W) + 2 * (int)(a / p->W);
+                const uint32_t xy = mbaff && intra_only ? a >> 1 : a;  // MB pairs: all_t holds pairs
+                int t = (int)(xy % p->W) + 2 * (int)(xy / p->W);
                 byt[t].push_back((i << WI_PIC_SHIFT) | a);
             }
         }
         for (auto &v : byt) outq.insert(outq.end(), v.begin(), v.end());
     };
-    merge(true, intra);
+    merge(true, false, intra);
+    std::vector<uint32_t> mintra, mpairs;
+    merge(true, true, mintra);
+    merge(false, true, mpairs);
     bool any_dbk = false;
-    for (uint32_t i = 0; i < n; i++) any_dbk |= pics[i]->desc.disable_deblocking_filter_idc != 1;
+    for (uint32_t i = 0; i < n; i++) any_dbk |= !pics[i]->mbaff && pics[i]->desc.disable_deblocking_filter_idc != 1;
     std::vector<uint32_t> all, rows;
-    if (any_dbk) merge(false, all);
+    if (any_dbk) merge(false, false, all);
     int maxh = 0;
     for (uint32_t i = 0; i < n; i++) maxh = std::max(maxh, pics[i]->H);
     // bands of R rows of all pictures, band index major: the band above always comes first
@@ -1191,7 +1289,7 @@ int build_sub(h264g_ctx *c, const std::vector<const h264g_pic *> &pics, Batch *b
         for (int b = 0; b * R < maxh; b++)
             for (uint32_t i = 0; i < n; i++) {
                 const h264g_pic *p = pics[i];
-                if (b * R < p->H && (intra ? p->n_intra != 0 : p->desc.disable_deblocking_filter_idc != 1))
+                if (!p->mbaff && b * R < p->H && (intra ? p->n_intra != 0 : p->desc.disable_deblocking_filter_idc != 1))
                     out.push_back((i << WI_PIC_SHIFT) | (uint32_t)b);
             }
     };
@@ -1202,12 +1300,13 @@ int build_sub(h264g_ctx *c, const std::vector<const h264g_pic *> &pics, Batch *b
         // MB rows of all pictures, row index major: the row above always comes first.
         for (int y = 0; y < maxh; y++)
             for (uint32_t i = 0; i < n; i++)
-                if (y < pics[i]->H) rows.push_back((i << WI_PIC_SHIFT) | (uint32_t)y);
+                if (!pics[i]->mbaff && y < pics[i]->H) rows.push_back((i << WI_PIC_SHIFT) | (uint32_t)y);
     }
     auto al = [](size_t v) { return (v + 255) & ~(size_t)255; };
     size_t o_p = 0, o_i = al(n * sizeof(DevPic)), o_a = al(o_i + inter.size() * 4), o_l = al(o_a + intra.size() * 4);
     size_t o_r = al(o_l + all.size() * 4), o_ib = al(o_r + rows.size() * 4), o_db = al(o_ib + ibands.size() * 4);
-    size_t total = al(o_db + dbands.size() * 4 + 4);
+    size_t o_mi = al(o_db + dbands.size() * 4), o_mp = al(o_mi + mintra.size() * 4);
+    size_t total = al(o_mp + mpairs.size() * 4 + 4);
     b->dmem = pool_get(c, total);
     b->dbytes = total;
     if (!b->dmem) return h264g_set_error(H264G_E_NOMEM, "batch device memory");
@@ -1226,14 +1325,20 @@ int build_sub(h264g_ctx *c, const std::vector<const h264g_pic *> &pics, Batch *b
     b->n_rows = (uint32_t)rows.size();
     b->n_ibands = (uint32_t)ibands.size();
     b->n_dbands = (uint32_t)dbands.size();
-    b->dbk_off = !any_dbk;
+    b->d_mintra = (uint32_t *)(base + o_mi);
+    b->d_mpairs = (uint32_t *)(base + o_mp);
+    b->n_mintra = (uint32_t)mintra.size();
+    b->n_mpairs = (uint32_t)mpairs.size();
+    b->dbk_off = !any_dbk && mpairs.empty();
     if (hipMemcpy(b->d_pics, hp.data(), n * sizeof(DevPic), hipMemcpyHostToDevice) != hipSuccess ||
         (!inter.empty() && hipMemcpy(b->d_inter, inter.data(), inter.size() * 4, hipMemcpyHostToDevice) != hipSuccess) ||
         (!intra.empty() && hipMemcpy(b->d_intra, intra.data(), intra.size() * 4, hipMemcpyHostToDevice) != hipSuccess) ||
         (!all.empty() && hipMemcpy(b->d_all, all.data(), all.size() * 4, hipMemcpyHostToDevice) != hipSuccess) ||
         (!rows.empty() && hipMemcpy(b->d_rows, rows.data(), rows.size() * 4, hipMemcpyHostToDevice) != hipSuccess) ||
         (!ibands.empty() && hipMemcpy(b->d_ibands, ibands.data(), ibands.size() * 4, hipMemcpyHostToDevice) != hipSuccess) ||
-        (!dbands.empty() && hipMemcpy(b->d_dbands, dbands.data(), dbands.size() * 4, hipMemcpyHostToDevice) != hipSuccess))
+        (!dbands.empty() && hipMemcpy(b->d_dbands, dbands.data(), dbands.size() * 4, hipMemcpyHostToDevice) != hipSuccess) ||
+        (!mintra.empty() && hipMemcpy(b->d_mintra, mintra.data(), mintra.size() * 4, hipMemcpyHostToDevice) != hipSuccess) ||
+        (!mpairs.empty() && hipMemcpy(b->d_mpairs, mpairs.data(), mpairs.size() * 4, hipMemcpyHostToDevice) != hipSuccess))
         return h264g_set_error(H264G_E_DEVICE, "batch upload");
     return H264G_OK;
 }
@@ -1290,6 +1395,10 @@ StageSet batch_stages(const Batch *b) {
     q.dbands = b->d_dbands;
     q.n_dbands = b->n_dbands;
     q.dbk_off = b->dbk_off;
+    q.mintra = b->d_mintra;
+    q.n_mintra = b->n_mintra;
+    q.mpairs = b->d_mpairs;
+    q.n_mpairs = b->n_mpairs;
     return q;
 }
 }  // namespace

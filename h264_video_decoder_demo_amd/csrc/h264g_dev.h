@@ -7,8 +7,9 @@
 // everything the kernels would otherwise recompute per sample.
 struct DevAux {
     int8_t   refid[2][4];     // reference picture per list and 8x8 quadrant as a DPB slot,
-                              // -1 = none: the MC source when the list is used, and the
-                              // picture identity the bS derivation compares
+                              // -1 = none: the MC source when the list is used, and (frame
+                              // pictures) the picture identity the bS derivation compares;
+                              // field MBs of MBAFF frames: the frame of refIdx / 2
     uint8_t  pf;              // PredFlagLX of quadrant q at bit (4 * X + q)
     uint8_t  kind;            // AUX_* bits below
     uint16_t nz4;             // luma4x4BlkIdx b has non-zero levels (4x4-transform MBs)
@@ -17,7 +18,8 @@ struct DevAux {
     uint8_t  nmask;           // number of level masks of the MB
     uint8_t  qpc[2];          // QP'C of Cb / Cr for dequantisation
     uint8_t  qdb[3];          // deblocking qP: luma, Cb edges, Cr edges
-    uint8_t  pad0;
+    uint8_t  fpar;            // field MBs of MBAFF frames: bit (4 * X + q) = the referenced
+                              // field is the bottom field (refIdx odd flips the MB's parity)
     uint32_t ires;            // bits 0..21: intra MBs, index of the MB's residual in
                               // DevPic::ires; bits 22..31: number of non-zero level values
 };
@@ -71,6 +73,11 @@ struct DevPic {
     uint32_t *dbq;            // per-MB deblocking word, written by k_bs
     int16_t  *ires;           // residual of each intra MB with levels (DevAux::ires): luma
                               // 16x16 then Cb, Cr 8x8, raster; written by the inter stage
+    int32_t  mbaff;           // MbaffFrameFlag: MB address 2 p + b is the top / bottom MB of
+                              // pair p, pairs in raster order
+    const int8_t *bsref;      // MBAFF frames: per MB [list][quadrant] the picture identity
+                              // the bS derivation compares, RefPicListX[RefIdxLX] (-1: none)
+    uint32_t *pairprog;       // MBAFF streams: per-MB-pair deblocking progress words
 };
 
 // Work item: picture index in the batch and macroblock address.
@@ -106,6 +113,10 @@ int h264g_launch_intra_band(const DevPic *pics, const uint32_t *bands, uint32_t 
 int h264g_launch_deblock_band(const DevPic *pics, const uint32_t *bands, uint32_t n, DevCtl *ctl, uint32_t *epoch,
                               int rows, uint32_t *err, void *stream);
 int h264g_launch_bs(const DevPic *pics, const uint32_t *items, uint32_t n, void *stream);
+// Deblocking of MBAFF frames: persistent waves over MB pairs (items: picture index << 20 |
+// pair index, ordered by t = x + 2 y of the pair), one pair per wave at a time.
+int h264g_launch_deblock_mbaff(const DevPic *pics, const uint32_t *items, uint32_t n, DevCtl *ctl, uint32_t *epoch,
+                               uint32_t *err, int waves, void *stream);
 int h264g_launch_bgr24(const uint8_t *yuv, int wl, int hl, uint8_t *bgr, uint32_t stride, int flip, void *stream);
 int h264g_launch_copy(const uint8_t *src, uint8_t *dst, uint64_t bytes, void *stream);
 }

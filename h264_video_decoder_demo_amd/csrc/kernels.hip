@@ -25,6 +25,8 @@ constexpr int WAVE = 64;
 // Table 8-13 / 8-14 zig-zag (scan position -> raster index), H264PictureBase.cpp:4542-4744
 // (constexpr so that fully unrolled loops index register arrays with constants)
 __device__ constexpr uint8_t c_zz4[16] = {0, 1, 4, 8, 5, 2, 3, 6, 9, 12, 13, 10, 7, 11, 14, 15};
+// the field scan of Table 8-13 (field MBs of MBAFF frames)
+__device__ constexpr uint8_t c_fld4[16] = {0, 4, 1, 8, 12, 5, 9, 13, 2, 6, 10, 14, 3, 7, 11, 15};
 __constant__ uint8_t c_zz8[64] = {
     0, 1, 8, 16, 9, 2, 3, 10, 17, 24, 32, 25, 18, 11, 4, 5, 12, 19, 26, 33, 40, 48, 41, 34, 27, 20, 13, 6, 7, 14, 21, 28,
     35, 42, 49, 56, 57, 50, 43, 36, 29, 22, 15, 23, 30, 37, 44, 51, 58, 59, 52, 45, 38, 31, 39, 46, 53, 60, 61, 54, 47, 55, 62, 63};
@@ -32,6 +34,10 @@ __constant__ uint8_t c_zz8[64] = {
 __constant__ uint8_t c_izz8[64] = {
     0, 1, 5, 6, 14, 15, 27, 28, 2, 4, 7, 13, 16, 26, 29, 42, 3, 8, 12, 17, 25, 30, 41, 43, 9, 11, 18, 24, 31, 40, 44, 53,
     10, 19, 23, 32, 39, 45, 52, 54, 20, 22, 33, 38, 46, 51, 55, 60, 21, 34, 37, 47, 50, 56, 59, 61, 35, 36, 48, 49, 57, 58, 62, 63};
+// raster position -> 8x8 field scan index (inverse of Table 8-14's field scan)
+__constant__ uint8_t c_ifld8[64] = {
+    0, 3, 8, 15, 22, 30, 38, 52, 1, 4, 14, 21, 29, 37, 45, 53, 2, 7, 16, 23, 31, 39, 46, 58, 5, 9, 20, 28, 36, 44, 51, 59,
+    6, 13, 24, 32, 40, 47, 54, 60, 10, 17, 25, 33, 41, 48, 55, 61, 11, 18, 26, 34, 42, 49, 56, 62, 12, 19, 27, 35, 43, 50, 57, 63};
 // normAdjust values v (8-315, 8-318), H264PictureBase.cpp:4900-4950
 __constant__ uint8_t c_v4[6][3] = {{10, 16, 13}, {11, 18, 14}, {13, 20, 16}, {14, 23, 18}, {16, 25, 20}, {18, 29, 23}};
 __constant__ uint8_t c_v8[6][6] = {{20, 18, 32, 19, 25, 24}, {22, 19, 35, 21, 28, 26}, {26, 23, 42, 24, 33, 31},
@@ -138,13 +144,15 @@ __device__ __forceinline__ int level_slot(int i, int i16, int cbp) {
     return -1;
 }
 
-// Scan positions of the samples (r, 0..3) of a 4x4 block, 4 bits each: the inverse of c_zz4.
-constexpr uint64_t izz4_table() {
+// Scan positions of the samples (r, 0..3) of a 4x4 block, 4 bits each: the inverse of c_zz4
+// (IZZ4) and of c_fld4 (IFLD4).
+constexpr uint64_t iscan4_table(const uint8_t (&scan)[16]) {
     uint64_t v = 0;
-    for (int k = 0; k < 16; k++) v |= (uint64_t)k << (4 * c_zz4[k]);
+    for (int k = 0; k < 16; k++) v |= (uint64_t)k << (4 * scan[k]);
     return v;
 }
-constexpr uint64_t IZZ4 = izz4_table();
+constexpr uint64_t IZZ4 = iscan4_table(c_zz4);
+constexpr uint64_t IFLD4 = iscan4_table(c_fld4);
 
 // 4x4 transpose over the four lanes of a quad (lane r holds row r in m[0..3], afterwards
 // column r): two exchange stages with DPP quad permutations.
@@ -162,11 +170,13 @@ __device__ __forceinline__ void quad_transpose(int32_t (&m)[4], int r) {
 // 8.5.12 for row r of a 4x4 block on the quad that holds the block (lane r of the quad):
 // dequantise the row's levels (scan order in `scan`), horizontal pass, transpose, vertical
 // pass; returns column r of the residual in out[0..3] (rows 0..3).  dc_pass: sample
-// (0, 0) is `dc` as it stands (Intra16x16 and chroma DC).
-__device__ __forceinline__ void residual4x4_quad(const int32_t *scan, int qp, bool dc_pass, int dc, int r, int32_t (&out)[4]) {
+// (0, 0) is `dc` as it stands (Intra16x16 and chroma DC).  field: the levels follow the
+// field scan (field MBs, H264PictureBase.cpp:3419).
+__device__ __forceinline__ void residual4x4_quad(const int32_t *scan, int qp, bool dc_pass, int dc, int r, bool field,
+                                                 int32_t (&out)[4]) {
     const int m = qp % 6, e = qp / 6;
     const int s0 = 16 * c_v4[m][0], s1 = 16 * c_v4[m][1], s2 = 16 * c_v4[m][2];
-    const uint32_t iz = (uint32_t)(IZZ4 >> (16 * r)) & 0xFFFFu;
+    const uint32_t iz = (uint32_t)((field ? IFLD4 : IZZ4) >> (16 * r)) & 0xFFFFu;
     int32_t d[4];
 #pragma unroll
     for (int j = 0; j < 4; j++) {
@@ -198,9 +208,11 @@ __device__ __forceinline__ void idct8(int32_t *v, int s) {  // 8.5.13.2, in plac
 
 // Expands the MB's levels into ws->coef and computes all residual blocks into ws->rl /
 // ws->rc.  Must be called by the whole wave.  Returns false when the MB has no levels.
+// Field MBs of MBAFF frames take the field scans.
 __device__ __forceinline__ bool build_residual(const h264g_mb *mb, const DevAux &ax, const int16_t *levels, WaveRes *ws) {
     const int lane = wave_lane();
     if (!(ax.kind & AUX_RESID)) return false;
+    const bool field = (mb->flags & H264G_MBF_FIELD) != 0;
     const int cbp = mb->cbp;
     const int i16 = mb->mb_type == H264G_MB_I16x16;
     const int t8 = (ax.kind & AUX_T8) ? 1 : 0;
@@ -253,7 +265,7 @@ __device__ __forceinline__ bool build_residual(const h264g_mb *mb, const DevAux 
             int32_t d[8];
 #pragma unroll
             for (int j = 0; j < 8; j++) {
-                const int32_t v = src[c_izz8[i * 8 + j]];
+                const int32_t v = src[field ? c_ifld8[i * 8 + j] : c_izz8[i * 8 + j]];
                 d[j] = qp >= 36 ? (v * ls8(qp % 6, i, j)) << (qp / 6 - 6)
                                 : (v * ls8(qp % 6, i, j) + (1 << (5 - qp / 6))) >> (6 - qp / 6);
             }
@@ -276,10 +288,13 @@ __device__ __forceinline__ bool build_residual(const h264g_mb *mb, const DevAux 
                 int32_t acc = 0;
 #pragma unroll
                 for (int j = 0; j < 4; j++) {
-                    int32_t v = 0;
+                    int32_t vz = 0, vf = 0;
 #pragma unroll
-                    for (int k = 0; k < 16; k++)
-                        if (c_zz4[k] == i * 4 + j) v = ws->coef[0][k];  // resolved at compile time
+                    for (int k = 0; k < 16; k++) {  // resolved at compile time
+                        if (c_zz4[k] == i * 4 + j) vz = ws->coef[0][k];
+                        if (c_fld4[k] == i * 4 + j) vf = ws->coef[0][k];
+                    }
+                    const int32_t v = field ? vf : vz;
                     acc += hneg(col, j) ? -v : v;
                 }
                 f += hneg(row, i) ? -acc : acc;
@@ -288,7 +303,7 @@ __device__ __forceinline__ bool build_residual(const h264g_mb *mb, const DevAux 
             dc = qp >= 36 ? (f * sc) << (qp / 6 - 6) : (f * sc + (1 << (5 - qp / 6))) >> (6 - qp / 6);
         }
         int32_t o[4];
-        residual4x4_quad(ws->coef[1 + b], qp, i16, dc, r, o);
+        residual4x4_quad(ws->coef[1 + b], qp, i16, dc, r, field, o);
         const int x0 = blk_x(b), y0 = blk_y(b);
 #pragma unroll
         for (int i = 0; i < 4; i++) ws->rl[(y0 + i) * 16 + x0 + r] = (int16_t)o[i];
@@ -305,7 +320,7 @@ __device__ __forceinline__ bool build_residual(const h264g_mb *mb, const DevAux 
         const int32_t f = b == 0 ? e00 + e01 : b == 1 ? e00 - e01 : b == 2 ? e10 + e11 : e10 - e11;
         const int dc = ((f * ls4(qpc % 6, 0, 0)) << (qpc / 6)) >> 5;
         int32_t o[4];
-        residual4x4_quad(ws->coef[19 + 4 * c + b], qpc, true, dc, r, o);
+        residual4x4_quad(ws->coef[19 + 4 * c + b], qpc, true, dc, r, field, o);
         const int x0 = (b & 1) * 4, y0 = (b >> 1) * 4;
 #pragma unroll
         for (int i = 0; i < 4; i++) ws->rc[c][(y0 + i) * 8 + x0 + r] = (int16_t)o[i];
@@ -334,6 +349,39 @@ __device__ __forceinline__ int ldc(const uint8_t *p, int w, int h, int x, int y)
 }
 
 __device__ __forceinline__ uint8_t *slot_base(const DevPic &P, int s) { return P.dpb + (uint64_t)s * P.slot_bytes; }
+
+// Where an MB's samples lie (6.4.1, H264PictureBase.cpp:2503-2530): the picture row of its
+// first luma / chroma sample and the row step (2 for field MBs of MBAFF frames, whose rows
+// are every other picture row, 8.5.14), and its row origin in the coordinates of the
+// pictures it predicts from (frame rows, or field rows for field MBs,
+// H264InterPrediction.cpp:553-557).
+struct MbGeo {
+    int mbx, ly, cy, step, mcy, bot;
+    bool field;
+};
+__device__ __forceinline__ MbGeo mb_geo(const DevPic &P, int addr) {
+    MbGeo g;
+    if (!P.mbaff) {
+        const int y = addr / P.W;
+        g.mbx = addr % P.W;
+        g.ly = y * 16;
+        g.cy = y * 8;
+        g.step = 1;
+        g.mcy = g.ly;
+        g.bot = 0;
+        g.field = false;
+        return g;
+    }
+    const int pair = addr >> 1, py = pair / P.W;
+    g.mbx = pair % P.W;
+    g.bot = addr & 1;
+    g.field = (P.mbs[addr].flags & H264G_MBF_FIELD) != 0;
+    g.step = g.field ? 2 : 1;
+    g.ly = py * 32 + (g.field ? g.bot : 16 * g.bot);
+    g.cy = py * 16 + (g.field ? g.bot : 8 * g.bot);
+    g.mcy = g.field ? py * 16 : g.ly;
+    return g;
+}
 
 // (P1 + P2 + 1) >> 1 of every luma position (xF, yF), 8.4.2.2.1 (H264InterPrediction.cpp:2344-2480),
 // with P1, P2 among 0 = G (integer sample), 1 = b (horizontal half sample), 2 = h
@@ -380,10 +428,17 @@ __device__ __forceinline__ uint32_t avg4(uint32_t a, uint32_t b) { return (a | b
 // the lane's vertical 6-taps on window bytes (h, or m) and on b1 (j), and the pair of
 // the position's inputs.  Both stages are the same for every (xF, yF), so lanes with
 // different quarter-sample positions do not diverge; the wave skips what no lane uses.
-__device__ __forceinline__ uint32_t luma_list(const DevPic &P, const h264g_mb *mb, const DevAux &ax, int l, WaveRes *ws) {
+// A field MB of an MBAFF frame predicts from one field of the reference frame: its rows
+// are every other frame row from row `parity`, and the reference's field view reports
+// that row pitch, twice the width, as its width, which is also where it clamps x
+// (H264InterPrediction.cpp:2363 with H264PictureBase.cpp:209-210): right of the picture
+// the samples come from the next frame row, on the bottom field's last row from the next
+// plane (the DPB slots of MBAFF streams end in a zeroed tail for the Cr plane).
+__device__ __forceinline__ uint32_t luma_list(const DevPic &P, const h264g_mb *mb, const DevAux &ax, int l, const MbGeo &g,
+                                              WaveRes *ws) {
     const int lane = wave_lane();
-    const int W = P.W, wl = W * 16, hl = P.H * 16;
-    const int addr = (int)(mb - P.mbs), bx0 = (addr % W) * 16, by0 = (addr / W) * 16;
+    const int W = P.W, wl = W * 16 * g.step, hl = P.H * 16 / g.step;  // row pitch / clamp width, clamp height
+    const int bx0 = g.mbx * 16, by0 = g.mcy;
     const int b = lane & 15, r = lane >> 4, bx = b & 3, by = b >> 2, q = ((by >> 1) << 1) | (bx >> 1);
     const bool use = (ax.pf >> (4 * l + q)) & 1;
     const int mvx = mb->mv[l][b][0], mvy = mb->mv[l][b][1];
@@ -394,7 +449,7 @@ __device__ __forceinline__ uint32_t luma_list(const DevPic &P, const h264g_mb *m
     const int x0 = bx0 + bx * 4 + (mvx >> 2), y0 = by0 + by * 4 + (mvy >> 2);
     WaveMc &mc = ws->mc;
     if (use) {
-        const uint8_t *ref = slot_base(P, ax.refid[l][q]);
+        const uint8_t *ref = slot_base(P, ax.refid[l][q]) + (g.field && ((ax.fpar >> (4 * l + q)) & 1) ? wl / 2 : 0);
         const int xs = x0 - 2, xa = xs & ~3, off = xs & 3;
         const bool inside = xa >= 0 && xa + 12 <= wl;
 #pragma unroll
@@ -511,14 +566,15 @@ __device__ void inter_mb(const DevPic &P, int addr, WaveRes *ws) {
     const h264g_mb *mb = &P.mbs[addr];
     const DevAux &ax = P.aux[addr];  // per-lane fields (refid[q], wp[q]) are read from memory
     const int W = P.W, H = P.H, wl = W * 16, hl = H * 16, wc = W * 8, hc = H * 8;
-    const int mbx = addr % W, mby = addr / W;
+    const MbGeo g = mb_geo(P, addr);
+    const int mbx = g.mbx;
     uint8_t *dst = slot_base(P, P.dst);
     uint8_t *dy = dst, *dcb = dst + (uint64_t)wl * hl, *dcr = dcb + (uint64_t)wc * hc;
     if (ax.kind & AUX_PCM) {  // 8.3.5 (H264PictureBase.cpp:2449-2499)
         const int16_t *pcm = P.levels + mb->level_offset;
-        for (int i = lane; i < 256; i += WAVE) dy[(mby * 16 + (i >> 4)) * wl + mbx * 16 + (i & 15)] = (uint8_t)pcm[i];
-        dcb[(mby * 8 + (lane >> 3)) * wc + mbx * 8 + (lane & 7)] = (uint8_t)pcm[256 + lane];
-        dcr[(mby * 8 + (lane >> 3)) * wc + mbx * 8 + (lane & 7)] = (uint8_t)pcm[320 + lane];
+        for (int i = lane; i < 256; i += WAVE) dy[(g.ly + (i >> 4) * g.step) * wl + mbx * 16 + (i & 15)] = (uint8_t)pcm[i];
+        dcb[(g.cy + (lane >> 3) * g.step) * wc + mbx * 8 + (lane & 7)] = (uint8_t)pcm[256 + lane];
+        dcr[(g.cy + (lane >> 3) * g.step) * wc + mbx * 8 + (lane & 7)] = (uint8_t)pcm[320 + lane];
         return;
     }
     const bool res = build_residual(mb, ax, P.levels, ws);
@@ -530,7 +586,7 @@ __device__ void inter_mb(const DevPic &P, int addr, WaveRes *ws) {
 #pragma nounroll
         for (int l = 0; l < 2; l++) {  // one inlined copy of the interpolation for both lists
             if (!(ax.pf & (0x0F << (4 * l)))) continue;  // MB-uniform
-            const uint32_t v = luma_list(P, mb, ax, l, ws);
+            const uint32_t v = luma_list(P, mb, ax, l, g, ws);
             if (l) p1 = v; else p0 = v;
         }
         const DevWeight &w = P.wt[ax.wp[q]];
@@ -548,17 +604,30 @@ __device__ void inter_mb(const DevPic &P, int addr, WaveRes *ws) {
             packed = pack4(clip1((int)(packed & 255) + rl[0]), clip1((int)((packed >> 8) & 255) + rl[1]),
                            clip1((int)((packed >> 16) & 255) + rl[2]), clip1((int)(packed >> 24) + rl[3]));
         }
-        *(uint32_t *)&dy[(uint64_t)(mby * 16 + by * 4 + r) * wl + mbx * 16 + bx * 4] = packed;
+        *(uint32_t *)&dy[(uint64_t)(g.ly + (by * 4 + r) * g.step) * wl + mbx * 16 + bx * 4] = packed;
     }
     // ---- chroma: lane -> component c, 4x4 luma block b, chroma row r (2 samples)
     {
         int c = lane >> 5, i = lane & 31, b = i >> 1, r = i & 1, bx = b & 3, by = b >> 2, q = (by >> 1) * 2 + (bx >> 1);
         int pf0 = (ax.pf >> q) & 1, pf1 = (ax.pf >> (4 + q)) & 1;
-        int xC = mbx * 8 + bx * 2, yC = mby * 8 + by * 2 + r;
+        int xC = mbx * 8 + bx * 2, yC = g.mcy / 2 + by * 2 + r;
         int c00 = 0, c01 = 0, c10 = 0, c11 = 0;
         const uint64_t coff = (uint64_t)wl * hl + (uint64_t)c * wc * hc;
-        if (pf0) chroma2(slot_base(P, ax.refid[0][q]) + coff, wc, hc, xC, yC, mb->mv[0][b][0], mb->mv[0][b][1], c00, c01);
-        if (pf1) chroma2(slot_base(P, ax.refid[1][q]) + coff, wc, hc, xC, yC, mb->mv[1][b][0], mb->mv[1][b][1], c10, c11);
+        // field MBs: the parity's field of the reference (row pitch and clamp width 2 wc,
+        // see luma_list) and the vertical chroma vector offset of Table 8-10
+        // (H264InterPrediction.cpp:2024-2043)
+        const int fw = wc * g.step, fh = hc / g.step;
+        int mvy0 = mb->mv[0][b][1], mvy1 = mb->mv[1][b][1];
+        uint64_t o0 = coff, o1 = coff;
+        if (g.field) {
+            const int par0 = (ax.fpar >> q) & 1, par1 = (ax.fpar >> (4 + q)) & 1;
+            o0 += par0 ? wc : 0;
+            o1 += par1 ? wc : 0;
+            mvy0 += par0 == g.bot ? 0 : (par0 ? -2 : 2);
+            mvy1 += par1 == g.bot ? 0 : (par1 ? -2 : 2);
+        }
+        if (pf0) chroma2(slot_base(P, ax.refid[0][q]) + o0, fw, fh, xC, yC, mb->mv[0][b][0], mvy0, c00, c01);
+        if (pf1) chroma2(slot_base(P, ax.refid[1][q]) + o1, fw, fh, xC, yC, mb->mv[1][b][0], mvy1, c10, c11);
         const DevWeight &w = P.wt[ax.wp[q]];
         uint8_t *plane = c == 0 ? dcb : dcr;
         int v0 = weigh(w, 1 + c, pf0, pf1, c00, c10);
@@ -567,8 +636,9 @@ __device__ void inter_mb(const DevPic &P, int addr, WaveRes *ws) {
             v0 = clip1(v0 + ws->rc[c][(by * 2 + r) * 8 + bx * 2]);
             v1 = clip1(v1 + ws->rc[c][(by * 2 + r) * 8 + bx * 2 + 1]);
         }
-        *(uint16_t *)&plane[(uint64_t)yC * wc + xC] = (uint16_t)(v0 | (v1 << 8));
+        *(uint16_t *)&plane[(uint64_t)(g.cy + (by * 2 + r) * g.step) * wc + xC] = (uint16_t)(v0 | (v1 << 8));
     }
+    (void)H;
 }
 
 // ------------------------------------------------------------ wavefront helpers
@@ -593,20 +663,22 @@ __device__ __forceinline__ uint32_t uniform(uint32_t v) { return __builtin_amdgc
 // Waits until flags[i] == epoch for each listed neighbour.  Bounded: after SPIN_LIMIT
 // polls, or once another wave has raised the error word, the wave stops waiting (its
 // output is then wrong, and h264g_sync reports the error).
-__device__ void wait_deps(const uint32_t *flags, const int *deps, int nd, uint32_t epoch, uint32_t *err) {
-    for (int k = 0; k < nd; k++) {
-        const int d = (int)uniform((uint32_t)deps[k]);
-        if (d < 0) continue;
-        for (uint32_t it = 0;; ++it) {
-            if (uniform(ld_acquire(&flags[d])) == epoch) break;
-            if (it >= SPIN_LIMIT) {
-                atomicOr(err, DEVERR_TIMEOUT);
-                break;
-            }
-            if ((it & 255u) == 255u && uniform(__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) != 0u) break;
-            __builtin_amdgcn_s_sleep(1);
+// One of them (d < 0: none); the caller issues the acquire fence.
+__device__ __forceinline__ void wait_dep(const uint32_t *flags, int dep, uint32_t epoch, uint32_t *err) {
+    const int d = (int)uniform((uint32_t)dep);
+    if (d < 0) return;
+    for (uint32_t it = 0;; ++it) {
+        if (uniform(ld_acquire(&flags[d])) == epoch) break;
+        if (it >= SPIN_LIMIT) {
+            atomicOr(err, DEVERR_TIMEOUT);
+            break;
         }
+        if ((it & 255u) == 255u && uniform(__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) != 0u) break;
+        __builtin_amdgcn_s_sleep(1);
     }
+}
+__device__ void wait_deps(const uint32_t *flags, const int *deps, int nd, uint32_t epoch, uint32_t *err) {
+    for (int k = 0; k < nd; k++) wait_dep(flags, deps[k], epoch, err);
     __atomic_thread_fence(__ATOMIC_ACQUIRE);
 }
 
@@ -904,7 +976,12 @@ __device__ __forceinline__ void intra_mb(const DevPic &P, int addr, WaveIntra *w
     const int lane = wave_lane();
     const h264g_mb *mb = &P.mbs[addr];
     const int W = P.W, H = P.H, wl = W * 16, hl = H * 16, wc = W * 8, hc = H * 8;
-    const int mbx = addr % W, mby = addr / W;
+    // In an MBAFF frame every neighbouring sample that 6.4.12.2 (Table 6-4) gives an MB lies
+    // in the MB's own view of the picture: frame rows for frame MBs, the rows of its parity
+    // for field MBs (rows step apart from its first row), at the same columns; Table 6-4
+    // only decides availability (the caller's av).
+    const MbGeo g = mb_geo(P, addr);
+    const int mbx = g.mbx, st = g.step;
     uint8_t *dst = slot_base(P, P.dst);
     uint8_t *dy = dst, *dcb = dst + (uint64_t)wl * hl, *dcr = dcb + (uint64_t)wc * hc;
     // ---- neighbours into the tile
@@ -912,20 +989,20 @@ __device__ __forceinline__ void intra_mb(const DevPic &P, int addr, WaveIntra *w
         int x = lane - 1;
         bool ok = x < 0 ? av.D() : (x < 16 ? av.B() : av.C());
         ok = ok && !(above & (x < 0 ? 8u : (x < 16 ? 2u : 4u)));
-        if (ok) wi->T[0][lane] = dy[(uint64_t)(mby * 16 - 1) * wl + mbx * 16 + x];
+        if (ok) wi->T[0][lane] = dy[(uint64_t)(g.ly - st) * wl + mbx * 16 + x];
     } else if (lane < 41) {  // column -1
         int y = lane - 25;
-        if (av.A()) wi->T[y + 1][0] = dy[(uint64_t)(mby * 16 + y) * wl + mbx * 16 - 1];
+        if (av.A()) wi->T[y + 1][0] = dy[(uint64_t)(g.ly + y * st) * wl + mbx * 16 - 1];
     } else if (lane < 59) {  // chroma row -1 (x = -1..7) for both planes
         int k = lane - 41, c = k / 9, x = k % 9 - 1;
         bool ok = (x < 0 ? av.D() : av.B()) && !(above & (x < 0 ? 8u : 2u));
         const uint8_t *pl = c == 0 ? dcb : dcr;
-        if (ok) wi->C[c][0][x + 1] = pl[(uint64_t)(mby * 8 - 1) * wc + mbx * 8 + x];
+        if (ok) wi->C[c][0][x + 1] = pl[(uint64_t)(g.cy - st) * wc + mbx * 8 + x];
     }
     if (lane < 16) {  // chroma column -1
         int c = lane >> 3, y = lane & 7;
         const uint8_t *pl = c == 0 ? dcb : dcr;
-        if (av.A()) wi->C[c][y + 1][0] = pl[(uint64_t)(mby * 8 + y) * wc + mbx * 8 - 1];
+        if (av.A()) wi->C[c][y + 1][0] = pl[(uint64_t)(g.cy + y * st) * wc + mbx * 8 - 1];
     }
     wave_sync();
     // ---- luma
@@ -1078,13 +1155,13 @@ __device__ __forceinline__ void intra_mb(const DevPic &P, int addr, WaveIntra *w
             if (res) v = clip1(v + wi->res.rc[c][y * 8 + x]);
             if (y == 7) wi->cbot[c][x] = (uint8_t)v;
             uint8_t *pl = c == 0 ? dcb : dcr;
-            pl[(uint64_t)(mby * 8 + y) * wc + mbx * 8 + x] = (uint8_t)v;
+            pl[(uint64_t)(g.cy + y * st) * wc + mbx * 8 + x] = (uint8_t)v;
         }
     }
     // ---- luma tile -> picture
     for (int k = 0; k < 4; k++) {
         int i = lane + k * WAVE, x = i & 15, y = i >> 4;
-        dy[(uint64_t)(mby * 16 + y) * wl + mbx * 16 + x] = wi->T[y + 1][x + 1];
+        dy[(uint64_t)(g.ly + y * st) * wl + mbx * 16 + x] = wi->T[y + 1][x + 1];
     }
     (void)H;
 }
@@ -1555,6 +1632,37 @@ __device__ __forceinline__ Avail intra_avail(const DevPic &P, int addr, int deps
     return av;
 }
 
+// The same for an MB of an MBAFF frame (6.4.10 / Table 6-4; no constrained intra):
+// availability of the samples left (A), above (B), above right (C) and above left (D) of
+// the MB's view.  A frame MB at the bottom of its pair reads its top MB above it and the
+// left pair above left, and has no C; every other MB reads the pairs around its own.  It
+// waits for the intra MBs of the pairs whose samples it may read (both MBs of a pair: the
+// rows of a frame MB beside a field pair come from both, and the other way round), and
+// for the pair above right only when it reads C; deps holds up to 8 addresses.
+__device__ __forceinline__ Avail intra_avail_mbaff(const DevPic &P, int addr, int deps[8]) {
+    const int W = P.W, pair = addr >> 1, px = pair % W, py = pair / W, bot = addr & 1;
+    const bool field = (P.mbs[addr].flags & H264G_MBF_FIELD) != 0, in_pair = !field && bot;
+    const bool A = px > 0, B = py > 0, C = py > 0 && px < W - 1, D = py > 0 && px > 0;
+    Avail av;
+    av.m = (A ? 1u : 0u) | ((in_pair || B) ? 2u : 0u) | (!in_pair && C ? 4u : 0u) | ((in_pair ? A : D) ? 8u : 0u);
+    const h264g_mb &m = P.mbs[addr];
+    const bool uses_c = m.mb_type == H264G_MB_I8x8 ||
+                        (m.mb_type == H264G_MB_I4x4 && (m.intra_pred_modes[5] == 3 || m.intra_pred_modes[5] == 7));
+    // neighbouring pairs read: left; above, above-left, above-right (not for the bottom
+    // frame MB, whose row above is its own top MB, an earlier ticket of this launch)
+    const int pn[4] = {A ? pair - 1 : -1, !in_pair && B ? pair - W : -1, !in_pair && D ? pair - W - 1 : -1,
+                       !in_pair && C && uses_c ? pair - W + 1 : -1};
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const int a = pn[k] < 0 ? -1 : 2 * pn[k] + h;
+            deps[2 * k + h] = a >= 0 && cls_intra(P.mbs[a].mb_type) ? a : -1;
+        }
+    if (in_pair && cls_intra(P.mbs[addr - 1].mb_type)) deps[2] = addr - 1;  // slot of the unused above pair
+    return av;
+}
+
 __global__ __launch_bounds__(WAVE) void k_intra(const DevPic *__restrict__ pics, const uint32_t *__restrict__ items, uint32_t n,
                                                 DevCtl *ctl, uint32_t *epoch_p, uint32_t *err) {
     __shared__ WaveIntra lds;
@@ -1565,12 +1673,12 @@ __global__ __launch_bounds__(WAVE) void k_intra(const DevPic *__restrict__ pics,
         const uint32_t it = items[t];
         const DevPic &P = pics[it >> WI_PIC_SHIFT];
         const int addr = (int)(it & WI_MB_MASK);
-        int deps[4];
-        const Avail av = intra_avail(P, addr, deps);
+        int deps[8];
+        const Avail av = P.mbaff ? intra_avail_mbaff(P, addr, deps) : intra_avail(P, addr, deps);
         const IntraRes r = intra_res_fetch(P, addr);
         if (P.mbs[addr].mb_type == H264G_MB_I4x4) intra4_prep(&P.mbs[addr], av, &lds);
         else if (P.mbs[addr].mb_type == H264G_MB_I8x8) intra8_prep(&P.mbs[addr], av, &lds);
-        wait_deps(P.flags, deps, 4, epoch, err);
+        wait_deps(P.flags, deps, P.mbaff ? 8 : 4, epoch, err);
         intra_res_place(r, &lds);
         intra_mb(P, addr, &lds, av, r.res);
         publish(&P.flags[addr], epoch);
@@ -1738,6 +1846,220 @@ __global__ __launch_bounds__(WAVE) void k_deblock(const DevPic *__restrict__ pic
         wait_deps(P.flags, deps, 2, epoch, err);
         deblock_mb(P, addr, &lds, &tb);
         publish(&P.flags[addr], epoch);
+    }
+    retire(ctl, gridDim.x, epoch_p, epoch);
+}
+
+// ---- deblocking of MBAFF frames, one MB pair per wave
+// 8.7 in MB address order (Deblock.cpp:76-635): the pair's top MB, then its bottom MB.
+// Pair (x, y) follows pair (x - 1, y), whose right columns its left edges filter, and
+// pair (x + 1, y - 1), whose left edges filter the right columns of pair (x, y - 1) that
+// its top edges read: persistent waves over pairs in t = x + 2 y order, one progress word
+// per pair (P.pairprog).  The wave works on an LDS copy of the samples its edges touch:
+// luma rows -8..31 (a field MB's top edge reads four rows of its parity above the pair)
+// by columns -4..15, chroma rows -8..15 by columns -4..7, written back afterwards.
+constexpr int MR_LROWS = 40, MR_LCOLS = 20, MR_CROWS = 24, MR_CCOLS = 12, MR_ROW0 = 8, MR_COL0 = 4;
+struct alignas(16) MbaffRegion {
+    uint8_t L[MR_LROWS][MR_LCOLS];
+    uint8_t C[2][MR_CROWS][MR_CCOLS];
+};
+
+// Copies the pair's region between the picture and LDS (to_lds: load, else store).
+// Columns left of the picture and rows above it are skipped.
+__device__ void mbaff_region(const DevPic &P, int px, int py, MbaffRegion *rg, bool to_lds) {
+    const int lane = wave_lane(), wl = P.W * 16, wc = P.W * 8;
+    uint8_t *y = slot_base(P, P.dst), *cb = y + (uint64_t)wl * (P.H * 16), *cr = cb + (uint64_t)wc * (P.H * 8);
+    // dwords: luma rows x 5, then chroma planes x rows x 3
+    constexpr int NL = MR_LROWS * 5, NC = MR_CROWS * 3;
+    for (int i = lane; i < NL + 2 * NC; i += WAVE) {
+        uint8_t *pic, *lds;
+        int r, d;
+        if (i < NL) {
+            r = i / 5 - MR_ROW0;
+            d = i % 5;
+            if ((py == 0 && r < 0) || (px == 0 && d == 0)) continue;
+            pic = y + (uint64_t)(py * 32 + r) * wl + px * 16 - 4 + 4 * d;
+            lds = &rg->L[r + MR_ROW0][4 * d];
+        } else {
+            const int k = i - NL, c = k / NC;
+            r = (k % NC) / 3 - MR_ROW0;
+            d = (k % NC) % 3;
+            if ((py == 0 && r < 0) || (px == 0 && d == 0)) continue;
+            pic = (c ? cr : cb) + (uint64_t)(py * 16 + r) * wc + px * 8 - 4 + 4 * d;
+            lds = &rg->C[c][r + MR_ROW0][4 * d];
+        }
+        if (to_lds) *(uint32_t *)lds = *(const uint32_t *)pic;
+        else *(uint32_t *)pic = *(const uint32_t *)lds;
+    }
+}
+
+// 8.7.2.1 (Deblock.cpp:994-1310) in an MBAFF frame, for the luma positions (xp, yp) of p0
+// in MB ap and (xq, yq) of q0 in MB aq: mixedModeEdgeFlag between a frame and a field MB;
+// bS 4 at MB edges between frame MBs and at vertical MB edges, 3 at the other intra edges;
+// 2 for levels; 1 at mixed edges; then the motion tests with a vertical limit of 2
+// quarter samples when q is a field MB (:1157-1158), comparing RefPicListX[RefIdxLX]
+// (P.bsref, quirk Q4) with quirk Q3.  I_PCM counts as neither intra nor inter.
+__device__ __forceinline__ int bs_mbaff(const DevPic &P, int ap, int aq, int xp, int yp, int xq, int yq, bool vertical) {
+    const h264g_mb &mp = P.mbs[ap], &mq = P.mbs[aq];
+    const DevAux xa = P.aux[ap], xb = P.aux[aq];
+    const bool pf = mp.flags & H264G_MBF_FIELD, qf = mq.flags & H264G_MBF_FIELD;
+    const bool intra = ((xa.kind | xb.kind) & AUX_INTRA) != 0, mixed = ap != aq && pf != qf;
+    if (ap != aq && intra && ((!pf && !qf) || vertical)) return 4;
+    if (intra && (!mixed || !vertical)) return 3;
+    const int bp4 = 8 * (yp >> 3) + 4 * (xp >> 3) + 2 * ((yp & 7) >> 2) + ((xp & 7) >> 2);
+    const int bq4 = 8 * (yq >> 3) + 4 * (xq >> 3) + 2 * ((yq & 7) >> 2) + ((xq & 7) >> 2);
+    const int bp8 = 2 * (yp >> 3) + (xp >> 3), bq8 = 2 * (yq >> 3) + (xq >> 3);
+    const bool pt8 = xa.kind & AUX_T8, qt8 = xb.kind & AUX_T8;
+    if ((pt8 && ((xa.nz8 >> bp8) & 1)) || (!pt8 && ((xa.nz4 >> bp4) & 1)) || (qt8 && ((xb.nz8 >> bq8) & 1)) ||
+        (!qt8 && ((xb.nz4 >> bq4) & 1)))
+        return 2;
+    if (mixed) return 1;
+    const int ylim = qf ? 2 : 4;
+    const int qp = bp8, qq = bq8, rp = (yp >> 2) * 4 + (xp >> 2), rq = (yq >> 2) * 4 + (xq >> 2);
+    const int p0 = P.bsref[ap * 8 + qp], p1 = P.bsref[ap * 8 + 4 + qp], q0 = P.bsref[aq * 8 + qq], q1 = P.bsref[aq * 8 + 4 + qq];
+    const int fp0 = (xa.pf >> qp) & 1, fp1 = (xa.pf >> (4 + qp)) & 1, fq0 = (xb.pf >> qq) & 1, fq1 = (xb.pf >> (4 + qq)) & 1;
+    if (!(((p0 == q0 && p1 == q1) || (p0 == q1 && p1 == q0)) && fp0 + fp1 == fq0 + fq1)) return 1;
+    auto far = [&](int ax, int ay, int bx, int by) { return iabs(ax - bx) >= 4 || iabs(ay - by) >= ylim; };
+    const int mp0x = mp.mv[0][rp][0], mp0y = mp.mv[0][rp][1], mp1x = mp.mv[1][rp][0], mp1y = mp.mv[1][rp][1];
+    const int mq0x = mq.mv[0][rq][0], mq0y = mq.mv[0][rq][1], mq1x = mq.mv[1][rq][0], mq1y = mq.mv[1][rq][1];
+    if (fp0 && !fp1 && fq0 && !fq1 && far(mp0x, mp0y, mq0x, mq0y)) return 1;
+    if (fp0 && !fp1 && !fq0 && fq1 && far(mp0x, mp0y, mq1x, mq1y)) return 1;
+    if (!fp0 && fp1 && fq0 && !fq1 && far(mp1x, mp1y, mq0x, mq0y)) return 1;
+    if (!fp0 && fp1 && !fq0 && fq1 && far(mp1x, mp1y, mq1x, mq1y)) return 1;
+    if (fp0 && fp1 && p0 != p1 && fq0 && fq1 && ((q0 == p0 && q1 == p1) || (q0 == p1 && q1 == p0))) {
+        if (q0 == p0 && (far(mp0x, mp0y, mq0x, mq0y) || far(mp1x, mp1y, mq1x, mq1y))) return 1;
+        if (q0 == p1 && (far(mp1x, mp1y, mq0x, mq0y) || far(mp0x, mp0y, mq1x, mq1y))) return 1;
+    }
+    if (fp0 && fp1 && p0 == p1 && fq0 && fq1 && q0 == q1 && q0 == p0) {
+        if (far(mp0x, mp0y, mq0x, mq0y) || (far(mp1x, mp1y, mq1x, mq1y) && far(mp0x, mp0y, mq1x, mq1y)) ||
+            far(mp1x, mp1y, mq0x, mq0y))
+            return 1;
+    }
+    return 0;
+}
+
+// One edge pass of MB addr (8.7.1, Deblock.cpp:639-990) on the pair's LDS region: lane k
+// of a plane (luma lanes 0-15, Cb 16-23, Cr 24-31; act: the lane's plane takes part) owns
+// line k.  e: the edge's x (vertical) or y (horizontal) inside the MB; lines of a field-
+// mode pass are two rows apart (dy = 1 + fieldMode), and yE % 2 picks the bottom-field
+// lines of a frame MB's top edge over a field pair.  pn: the p-MB of an MB edge; with
+// leftMix (a frame MB beside a field pair) it alternates with the line's parity while the
+// p0 position keeps the current MB's line number (quirk Q6).
+__device__ __forceinline__ void mbaff_edge(const DevPic &P, MbaffRegion *rg, const DbkTables *tb, int addr, int lrow0, int crow0,
+                           bool act, bool vertical, int e, int fieldMode, int pn, bool leftMix) {
+    const int lane = wave_lane();
+    const int plane = lane < 16 ? 0 : (lane < 24 ? 1 : 2), chroma = plane != 0;
+    const int k = chroma ? (lane & 7) : (lane & 15), nE = chroma ? 8 : 16, dy = 1 + fieldMode, s = chroma ? 2 : 1;
+    act = act && lane < 32;
+    uint8_t *base = chroma ? &rg->C[plane - 1][0][0] : &rg->L[0][0];
+    const int pitch = chroma ? MR_CCOLS : MR_LCOLS, row0 = MR_ROW0 + (chroma ? crow0 : lrow0);
+    const int xE = vertical ? e : k, yE = vertical ? k : e;
+    int qi[4], pi[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        if (vertical) {
+            qi[i] = (row0 + dy * yE) * pitch + MR_COL0 + xE + i;
+            pi[i] = (row0 + dy * yE) * pitch + MR_COL0 + xE - i - 1;
+        } else {
+            qi[i] = (row0 + dy * (yE + i) - (yE & 1)) * pitch + MR_COL0 + xE;
+            pi[i] = (row0 + dy * (yE - i - 1) - (yE & 1)) * pitch + MR_COL0 + xE;
+        }
+    }
+    uint32_t w[5] = {0, 0, 0, 0, 0};
+    int ap = addr, xp, yp, xq, yq;
+    if (vertical) {
+        xp = xE - 1;
+        if (xp < 0) {
+            if (pn >= 0) ap = leftMix ? pn + (yE & 1) : pn;
+            xp += nE;
+        }
+        yp = yE; xq = xE; yq = yE;
+    } else {
+        xp = xE;
+        yp = (yE - 1) - (yE & 1);
+        if (yp < 0) {
+            if (pn >= 0) ap = pn;
+            yp += nE;
+        }
+        xq = xE; yq = yE - (yE & 1);
+    }
+    int bS = 0, alpha = 0, beta = 0, tc0 = 0;
+    if (act) {
+        w[0] = pack4(base[pi[3]], base[pi[2]], base[pi[1]], base[pi[0]]);
+        w[1] = pack4(base[qi[0]], base[qi[1]], base[qi[2]], base[qi[3]]);
+        bS = bs_mbaff(P, ap, addr, xp * s, yp * s, xq * s, yq * s, vertical);
+        const int qpp = P.aux[ap].qdb[plane], qpq = P.aux[addr].qdb[plane], qav = (qpp + qpq + 1) >> 1;
+        const int ia = clip3(0, 51, qav + P.offA), ib = clip3(0, 51, qav + P.offB);
+        alpha = tb->alpha[ia];
+        beta = tb->beta[ib];
+        tc0 = bS >= 1 && bS <= 3 ? tb->tc0[bS - 1][ia] : 0;
+    }
+    dbk_line_edge<0>(w, act, !chroma, bS, alpha, beta, tc0);
+    if (act) {
+        base[pi[0]] = (uint8_t)(w[0] >> 24);
+        base[pi[1]] = (uint8_t)(w[0] >> 16);
+        base[pi[2]] = (uint8_t)(w[0] >> 8);
+        base[qi[0]] = (uint8_t)w[1];
+        base[qi[1]] = (uint8_t)(w[1] >> 8);
+        base[qi[2]] = (uint8_t)(w[1] >> 16);
+    }
+    wave_sync();
+}
+
+// The edges of one MB of the pair in the reference's order (Deblock.cpp:76-635): luma
+// left, internal vertical, top (twice, in field mode, for a frame MB under a field pair:
+// its top field lines against the pair's top MB, its bottom field lines against the bottom
+// MB; chroma against the bottom MB both times, quirk Q5), internal horizontal; chroma in
+// the same order, each chroma pass beside a luma pass (different planes).
+__device__ __forceinline__ void deblock_mb_mbaff(const DevPic &P, int addr, MbaffRegion *rg, const DbkTables *tb) {
+    const int lane = wave_lane(), W = P.W, pair = addr >> 1, px = pair % W, py = pair / W, bot = addr & 1;
+    const bool field = (P.mbs[addr].flags & H264G_MBF_FIELD) != 0, t8 = (P.aux[addr].kind & AUX_T8) != 0;
+    const bool lfield = px > 0 && (P.mbs[2 * (pair - 1)].flags & H264G_MBF_FIELD);
+    const bool afield = py > 0 && (P.mbs[2 * (pair - W)].flags & H264G_MBF_FIELD);
+    // 6.4.11.1 with Table 6-4 at (-1, 0) and (0, -1)
+    const int mbA = px > 0 ? 2 * (pair - 1) + (field == lfield ? bot : 0) : -1;
+    const int mbB = !field && bot ? addr - 1 : (py > 0 ? 2 * (pair - W) + (field && !bot && afield ? 0 : 1) : -1);
+    const bool left = px > 0, upper = !(py == 0 && (field || !bot));
+    const bool leftMix = !field && lfield, topMix = !bot && py > 0 && !field && afield;
+    const int lrow0 = field ? bot : 16 * bot, crow0 = field ? bot : 8 * bot, fm = field ? 1 : 0;
+    const bool luma = lane < 16, chroma = lane >= 16;
+    mbaff_edge(P, rg, tb, addr, lrow0, crow0, left, true, 0, fm, mbA, leftMix);
+    mbaff_edge(P, rg, tb, addr, lrow0, crow0, chroma || !t8, true, 4, fm, mbA, false);
+    mbaff_edge(P, rg, tb, addr, lrow0, crow0, luma, true, 8, fm, mbA, false);
+    mbaff_edge(P, rg, tb, addr, lrow0, crow0, luma && !t8, true, 12, fm, mbA, false);
+    if (topMix) {
+        mbaff_edge(P, rg, tb, addr, lrow0, crow0, true, false, 0, 1, luma ? mbB - 1 : mbB, false);
+        mbaff_edge(P, rg, tb, addr, lrow0, crow0, true, false, 1, 1, mbB, false);
+    } else {
+        mbaff_edge(P, rg, tb, addr, lrow0, crow0, upper, false, 0, fm, mbB, false);
+    }
+    mbaff_edge(P, rg, tb, addr, lrow0, crow0, chroma || !t8, false, 4, fm, mbB, false);
+    mbaff_edge(P, rg, tb, addr, lrow0, crow0, luma, false, 8, fm, mbB, false);
+    mbaff_edge(P, rg, tb, addr, lrow0, crow0, luma && !t8, false, 12, fm, mbB, false);
+}
+
+__global__ __launch_bounds__(WAVE) void k_deblock_mbaff(const DevPic *__restrict__ pics, const uint32_t *__restrict__ items,
+                                                        uint32_t n, DevCtl *ctl, uint32_t *epoch_p, uint32_t *err) {
+    __shared__ MbaffRegion rg;
+    __shared__ DbkTables tb;
+    dbk_tables_init(&tb);
+    __syncthreads();
+    const uint32_t epoch = launch_epoch(epoch_p);
+    for (;;) {
+        int t = next_ticket(ctl, n);
+        if (t < 0) break;
+        const uint32_t it = items[t];
+        const DevPic &P = pics[it >> WI_PIC_SHIFT];
+        const int pair = (int)(it & WI_MB_MASK), W = P.W, px = pair % W, py = pair / W;
+        wait_dep(P.pairprog, px > 0 ? pair - 1 : -1, epoch, err);
+        wait_dep(P.pairprog, py > 0 ? (px < W - 1 ? pair - W + 1 : pair - W) : -1, epoch, err);
+        __atomic_thread_fence(__ATOMIC_ACQUIRE);
+        mbaff_region(P, px, py, &rg, true);
+        wave_sync();
+        deblock_mb_mbaff(P, 2 * pair, &rg, &tb);
+        deblock_mb_mbaff(P, 2 * pair + 1, &rg, &tb);
+        mbaff_region(P, px, py, &rg, false);
+        publish(&P.pairprog[pair], epoch);
     }
     retire(ctl, gridDim.x, epoch_p, epoch);
 }
@@ -2134,6 +2456,14 @@ int h264g_launch_deblock_band(const DevPic *pics, const uint32_t *bands, uint32_
     if (rows < 1 || rows > 2 * DBK_WG_MAX_WAVES) return (int)hipErrorInvalidValue;
     hipLaunchKernelGGL(k_deblock_band, dim3(n), dim3(WAVE * ((rows + 1) / 2)), 0, (hipStream_t)stream, pics, bands, n,
                        rows, ctl, epoch, err);
+    return (int)hipGetLastError();
+}
+
+int h264g_launch_deblock_mbaff(const DevPic *pics, const uint32_t *items, uint32_t n, DevCtl *ctl, uint32_t *epoch,
+                               uint32_t *err, int waves, void *stream) {
+    if (n == 0) return 0;
+    uint32_t g = (uint32_t)waves < n ? (uint32_t)waves : n;
+    hipLaunchKernelGGL(k_deblock_mbaff, dim3(g), dim3(WAVE), 0, (hipStream_t)stream, pics, items, n, ctl, epoch, err);
     return (int)hipGetLastError();
 }
 

@@ -58,6 +58,11 @@ class Generator:
         return self.params.height_mbs
 
     @property
+    def mbaff(self) -> bool:
+        """The pictures are MBAFF frames."""
+        return bool(self.params.mbaff)
+
+    @property
     def num_pictures(self) -> int:
         return check(lib().h264g_gen_num_pictures(self._h))
 
@@ -198,13 +203,18 @@ class Engine:
             pass
 
     # ---- streams
-    def open_stream(self, width_mbs: int, height_mbs: int, dpb_slots: int) -> int:
+    def open_stream(self, width_mbs: int, height_mbs: int, dpb_slots: int, mbaff: bool = False) -> int:
+        """mbaff: the stream's pictures are MBAFF frames (frame_mbs_only_flag 0)."""
         d = StreamDesc(pic_width_in_mbs=width_mbs, frame_height_in_mbs=height_mbs, chroma_array_type=1, bit_depth=8,
-                       frame_mbs_only=1, mbaff=0, dpb_slots=dpb_slots)
+                       frame_mbs_only=0 if mbaff else 1, mbaff=1 if mbaff else 0, dpb_slots=dpb_slots)
         sid = c_uint32()
         check(lib().h264g_stream_open(self._h, byref(d), byref(sid)))
         self.streams[sid.value] = (width_mbs, height_mbs, dpb_slots)
         return sid.value
+
+    def open_stream_for(self, gen: "Generator") -> int:
+        """A stream for the pictures of a generator (its size, DPB and frame coding)."""
+        return self.open_stream(gen.width_mbs, gen.height_mbs, gen.dpb_slots, gen.mbaff)
 
     def close_stream(self, sid: int):
         check(lib().h264g_stream_close(self._h, sid))
@@ -323,7 +333,7 @@ def decode_stream_plain(eng: Engine, gen: Generator, readback: bool = True):
     """Decodes a generated stream picture by picture (plain path).
 
     Returns the reconstructed (Y, Cb, Cr) of every picture in decoding order."""
-    sid = eng.open_stream(gen.width_mbs, gen.height_mbs, gen.dpb_slots)
+    sid = eng.open_stream_for(gen)
     out = []
     try:
         for i in range(gen.num_pictures):

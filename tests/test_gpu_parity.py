@@ -18,7 +18,8 @@ from helpers import first_diff, make_generator, oracle_decode, plane_hashes
 pytestmark = pytest.mark.gpu
 
 CASES = ["cif_ip", "cif_ip_seed2", "cif_ip_seed3", "cif_pcm_cqp", "high_small", "high_small_seed7",
-         "high_implicit", "high_default_nodbk", "narrow", "flat", "high_1080p", "high_720p", "high_2160p"]
+         "high_implicit", "high_default_nodbk", "narrow", "flat", "high_1080p", "high_720p", "high_2160p",
+         "mbaff_small", "mbaff_t8_pcm", "mbaff_tall", "mbaff_1080p"]
 
 
 def _diagnose(engine, sid, name, gen, i, desc, mbs, levels, got):
@@ -52,7 +53,7 @@ def test_plain_path_matches_reference(any_engine, golden, name):
     engine = any_engine
     case = golden[name]
     gen = make_generator(case)
-    sid = engine.open_stream(gen.width_mbs, gen.height_mbs, gen.dpb_slots)
+    sid = engine.open_stream_for(gen)
     try:
         for i in range(gen.num_pictures):
             desc, mbs, levels = gen.picture(i)
@@ -72,9 +73,9 @@ def test_batched_path_matches_reference(any_engine, golden):
     picture index give the same bytes as the reference."""
     engine = any_engine
     names = ["cif_ip_seed2", "high_small", "high_implicit", "high_default_nodbk", "narrow", "flat", "cif_pcm_cqp",
-             "high_720p", "high_2160p"]
+             "high_720p", "high_2160p", "mbaff_small", "mbaff_t8_pcm"]
     gens = {n: make_generator(golden[n]) for n in names}
-    sids = {n: engine.open_stream(g.width_mbs, g.height_mbs, g.dpb_slots) for n, g in gens.items()}
+    sids = {n: engine.open_stream_for(g) for n, g in gens.items()}
     try:
         npic = max(g.num_pictures for g in gens.values())
         for i in range(npic):
@@ -98,7 +99,7 @@ def test_batch_rerun_is_idempotent(engine, golden):
     """A prepared batch run twice over the same inputs gives the same output."""
     case = golden["high_small"]
     gen = make_generator(case)
-    sid = engine.open_stream(gen.width_mbs, gen.height_mbs, gen.dpb_slots)
+    sid = engine.open_stream_for(gen)
     try:
         for i in range(2):  # I and P picture
             desc, mbs, levels = gen.picture(i)
@@ -126,7 +127,7 @@ def test_graph_replays_match_eager_runs(engine, golden):
     launches and synchronisations in between, reproduces the reference every time."""
     names = ["high_small", "cif_ip_seed3"]
     gens = {n: make_generator(golden[n]) for n in names}
-    sids = {n: engine.open_stream(g.width_mbs, g.height_mbs, g.dpb_slots) for n, g in gens.items()}
+    sids = {n: engine.open_stream_for(g) for n, g in gens.items()}
     try:
         for i in range(2):  # I then P picture of both streams
             staged = [(n, g.picture(i)) for n, g in gens.items()]
@@ -150,13 +151,13 @@ def test_graph_replays_match_eager_runs(engine, golden):
             engine.close_stream(s)
 
 
-@pytest.mark.parametrize("name", ["cif_ip", "high_small", "flat"])
+@pytest.mark.parametrize("name", ["cif_ip", "high_small", "flat", "mbaff_small"])
 def test_bgr24_matches_reference(engine, golden, name):
     """Output conversion on the device against the reference's convertYuv420pToBgr24."""
     import hashlib
     case = golden[name]
     gen = make_generator(case)
-    sid = engine.open_stream(gen.width_mbs, gen.height_mbs, gen.dpb_slots)
+    sid = engine.open_stream_for(gen)
     try:
         for i in range(gen.num_pictures):
             desc, mbs, levels = gen.picture(i)
@@ -179,7 +180,7 @@ def test_wide_pictures_match_oracle(any_engine, width_mbs):
     engine = any_engine
     gen = Generator(2, width_mbs=width_mbs, height_mbs=5, n_frames=3, seed=11 + width_mbs)
     want = {i: plane_hashes(*out[2]) for i, _, out in oracle_decode(gen)}
-    sid = engine.open_stream(gen.width_mbs, gen.height_mbs, gen.dpb_slots)
+    sid = engine.open_stream_for(gen)
     try:
         for i in range(gen.num_pictures):
             desc, mbs, levels = gen.picture(i)
@@ -205,7 +206,8 @@ def test_overlapped_frames_match_oracle(any_engine):
     sizes = [(22, 18, 11), (40, 23, 12), (30, 20, 13)]  # the 920-MB stream runs on the second lane
     gens = [Generator(2, width_mbs=w, height_mbs=h, n_frames=9, seed=0x0E4A0000 + s, spare_slots=2)
             for w, h, s in sizes]
-    sids = [engine.open_stream(g.width_mbs, g.height_mbs, g.dpb_slots) for g in gens]
+    gens.append(Generator(3, width_mbs=30, height_mbs=22, n_frames=9, seed=0x0E4A0014, spare_slots=2))  # MBAFF
+    sids = [engine.open_stream_for(g) for g in gens]
     try:
         frames = [[g.picture(i) for i in range(g.num_pictures)] for g in gens]
         pics = [[engine.begin(sid, *f) for f in fr] for sid, fr in zip(sids, frames)]
@@ -231,3 +233,25 @@ def test_overlapped_frames_match_oracle(any_engine):
     finally:
         for sid in sids:
             engine.close_stream(sid)
+
+
+@pytest.mark.parametrize("shape", [(240, 6), (120, 12), (3, 40)])
+def test_mbaff_pictures_match_oracle(any_engine, shape):
+    """MBAFF frames of other shapes (as wide as 2160p, narrow and tall) against the
+    oracle, which reproduces the reference's MBAFF golden cases."""
+    engine = any_engine
+    w, h = shape
+    gen = Generator(3, width_mbs=w, height_mbs=h, n_frames=5, seed=0x3AFF0000 + w)
+    want = {i: plane_hashes(*out[2]) for i, _, out in oracle_decode(gen)}
+    sid = engine.open_stream_for(gen)
+    try:
+        for i in range(gen.num_pictures):
+            desc, mbs, levels = gen.picture(i)
+            pic = engine.begin(sid, desc, mbs, levels)
+            engine.submit(pic)
+            engine.sync()
+            got = engine.readback(sid, desc.dst_slot)
+            engine.release(pic)
+            assert plane_hashes(*got) == want[i], f"MBAFF {w}x{h} picture {i}"
+    finally:
+        engine.close_stream(sid)
