@@ -388,7 +388,12 @@ int stage_picture(h264g_ctx *ctx, Stream &st, const h264g_pic_desc &d, const h26
                   const int16_t *levels, uint64_t n_levels, h264g_pic *pic, void *fixed = nullptr,
                   size_t fixed_cap = 0) {
     const int W = st.W, H = st.H;
-    if (d.structure != H264G_FRAME) return h264g_set_error(H264G_E_ARG, "field pictures are not supported");
+    // Field pictures (PAFF): the reference decoder dereferences NULL building the
+    // reference list of every P / B field (H264RefPicList.cpp:1197-1230) and misplaces
+    // the MBs of I fields (field view at twice the pitch, H264PictureBase.cpp:205-208),
+    // so there is no reference output for them to equal.
+    if (d.structure != H264G_FRAME)
+        return h264g_set_error(H264G_E_ARG, "field pictures (structure %d) are not supported", d.structure);
     if ((d.mbaff != 0) != st.mbaff)
         return h264g_set_error(H264G_E_ARG, "picture mbaff %d on a stream opened with mbaff %d", d.mbaff, (int)st.mbaff);
     const bool mbaff = st.mbaff;

@@ -196,7 +196,9 @@ typedef struct h264g_stream_desc {
  */
 typedef struct h264g_pic_desc {
     int32_t  slice_type;                      /* H264G_SLICE_*                         */
-    int32_t  structure;                       /* H264G_FRAME / _TOP_FIELD / _BOTTOM_FIELD */
+    int32_t  structure;                       /* H264G_FRAME; field pictures (_TOP_FIELD / _BOTTOM_FIELD)
+                                                 are refused with H264G_E_ARG: the reference decoder
+                                                 has no defined output for them */
     int32_t  mbaff;                           /* MbaffFrameFlag                        */
     int32_t  constrained_intra_pred_flag;
     int32_t  disable_deblocking_filter_idc;

@@ -52,3 +52,19 @@ def test_closed_stream_pictures_are_refused():
             eng.release(p)
         eng.close_stream(sid2)
         assert sid2 not in eng.streams
+
+
+def test_field_pictures_are_refused():
+    """PAFF field pictures have no reference output (the reference crashes on P / B
+    fields and misplaces I-field MBs), so the ABI refuses them before any launch."""
+    from h264_video_decoder_demo_amd._lib import H264G_E_ARG
+    g = Generator(1, width_mbs=5, height_mbs=4, n_frames=1)
+    with Engine(device=0) as eng:
+        sid = eng.open_stream(g.width_mbs, g.height_mbs, g.dpb_slots)
+        d, m, lv = g.picture(0)
+        for structure in (1, 2):  # H264G_TOP_FIELD, H264G_BOTTOM_FIELD
+            d.structure = structure
+            with pytest.raises(H264GError) as e:
+                eng.begin(sid, d, m, lv)
+            assert e.value.code == H264G_E_ARG, e.value
+        eng.close_stream(sid)
