@@ -146,6 +146,36 @@ def serving_check(specs, gens, args):
     return {"streams": len(gens), "equal_dpb": equal, "frames": sum(g.num_pictures for g in gens)}
 
 
+def serving_capacity(args, rates, seconds, bound_ms):
+    """Serving capacity: the serving loop (128 streams of the serving mix, GOP 32, 2 ms
+    windows, 200 GB DPB budget) at rising offered rates, each for `seconds`; capacity is
+    the highest offered rate whose frame p99 latency stays at or under bound_ms.  Stops
+    at the first rate over the bound."""
+    from h264_video_decoder_demo_amd import Engine, Generator
+    from h264_video_decoder_demo_amd.serve import ServeLoop, make_specs
+    kw = dict(waves=args.waves, deblock_waves=args.deblock_waves, intra_waves=args.intra_waves)
+    runs = []
+    for fps in rates:
+        specs = make_specs(128, total_fps=float(fps), duration_s=seconds, seed=0x5EED0005 + int(fps))
+        with cf.ThreadPoolExecutor(16) as ex:
+            gens = list(ex.map(lambda sp: Generator(2, width_mbs=sp.width_mbs, height_mbs=sp.height_mbs,
+                                                    n_frames=sp.n_frames, seed=sp.seed, gop=32, spare_slots=2), specs))
+        with Engine(device=0, dpb_budget_bytes=200 * 10**9, **kw) as eng:
+            res = ServeLoop(eng, specs, window_ms=2.0, gens=gens).run()
+        r = {"target_fps": fps, "offered_fps": round(res.offered_fps, 1), "served_fps": round(res.served_fps, 1),
+             "frames": res.frames, "offered_mpix_s": round(res.mpix / seconds, 1), "p50_ms": round(res.pct(50), 3),
+             "p99_ms": round(res.pct(99), 3), "max_batch": res.max_batch}
+        r["ok"] = r["p99_ms"] <= bound_ms
+        runs.append(r)
+        log(f"capacity: {fps} fps offered -> p50 {r['p50_ms']} ms, p99 {r['p99_ms']} ms, max batch {r['max_batch']}")
+        if not r["ok"]:
+            break
+    ok = [r for r in runs if r["ok"]]
+    return {"p99_bound_ms": bound_ms, "seconds_per_rate": seconds,
+            "capacity_fps": ok[-1]["offered_fps"] if ok else None,
+            "capacity_mpix_s": ok[-1]["offered_mpix_s"] if ok else None, "runs": runs}
+
+
 def summarize_pmc(fetch_csv: str, write_csv: str, copy_bytes: int, out_json: str) -> dict:
     """HBM traffic per launch from two rocprofv3 --pmc runs of `bench.py --profile-run`
     (FETCH_SIZE, then WRITE_SIZE; KiB per dispatch), each scaled by what the counter
@@ -200,9 +230,17 @@ def main():
                     help="keep the default band heights instead of autotuning them on the workload's first pictures")
     ap.add_argument("--serve-seconds", type=float, default=3.0,
                     help="length of the serving run (Poisson arrivals, 128 streams, 400 frames/s); 0 = skip")
+    ap.add_argument("--capacity-rates", default="800,1600,3200,6400",
+                    help="offered rates (frames/s) of the serving capacity sweep, rising; empty = skip")
+    ap.add_argument("--capacity-seconds", type=float, default=1.0)
+    ap.add_argument("--capacity-p99-ms", type=float, default=50.0)
     ap.add_argument("--profile-run", action="store_true",
                     help="short run for rocprofv3: 1 timed step, no warmup, no latency / plain / CPU legs, "
                          "then the library's calibration copy of --copy-mib MiB")
+    ap.add_argument("--one-lane", action="store_true",
+                    help="batched run in a one-lane context (every stage, deblocking included, on one HIP "
+                         "stream): the isolated shape whose kernel durations add up to the step time")
+    ap.add_argument("--no-isolated", action="store_true", help="skip the isolated (one-lane) batched leg")
     ap.add_argument("--copy-mib", type=int, default=1024)
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="write profiles/traffic_config4.json from two rocprofv3 --pmc runs of --profile-run")
@@ -211,7 +249,7 @@ def main():
         print(json.dumps(summarize_pmc(*args.summarize_pmc, args.copy_mib << 20, TRAFFIC_FILE)))
         return
     if args.profile_run:
-        args.steps, args.warmup, args.no_plain, args.no_cpu_baseline = 1, 0, True, True
+        args.steps, args.warmup, args.no_plain, args.no_cpu_baseline, args.no_isolated = 1, 0, True, True, True
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this engine runs on one GPU; streams would be spread over GPUs by an outer "
               "scheduler (no collectives involved). Run with --gpus 1.", file=sys.stderr)
@@ -240,7 +278,7 @@ def main():
     schedule = {"bands": 0, "mb_wavefront": 1, "mb_rows": 2}[args.deblock]
     eng = Engine(device=0, profile=False, waves=args.waves, graphs=not args.no_graphs, deblock_schedule=schedule,
                  deblock_waves=args.deblock_waves, intra_schedule={"bands": 0, "mb_wavefront": 1}[args.intra],
-                 intra_waves=args.intra_waves)
+                 intra_waves=args.intra_waves, lane_split_mbs=-1 if args.one_lane else 0)
     sids, slots, pics = [], {}, []
     alg = [0, 0, 0, 0]
     for g in gens:
@@ -312,10 +350,12 @@ def main():
 
     plain = None
     lat_p = []
-    if not args.no_plain:
-        # ---- plain: the reference's shape -- one picture at a time, one launch per stage
-        # per picture, all on one HIP stream -- in a context of its own (same pictures,
-        # its own DPBs, which must end up identical to the batched context's)
+    isolated = None
+    peng = None
+    if not args.no_plain or not args.no_isolated:
+        # a one-lane context (every stage on one HIP stream, deblocking included) holding
+        # the same pictures with DPBs of its own, which must end up identical to the
+        # batched context's
         peng = Engine(device=0, waves=args.waves, deblock_schedule=schedule, deblock_waves=args.deblock_waves,
                       intra_schedule={"bands": 0, "mb_wavefront": 1}[args.intra], intra_waves=args.intra_waves,
                       lane_split_mbs=-1)
@@ -325,6 +365,35 @@ def main():
             psids.append(sid)
             pslots[sid] = g.dpb_slots
             ppics.append([peng.begin_raw(sid, *g.picture_raw(i)) for i in range(nframes)])
+    if not args.no_isolated:
+        # ---- isolated: the batched launches of the same job in the one-lane context, with
+        # HIP events around every stage.  Nothing runs beside a kernel, so each kernel's
+        # average duration is its own and the stage durations add up to at most the step.
+        pbatches = [peng.batch_create([row[f] for row in ppics]) for f in range(nframes)]
+        for b in pbatches:
+            peng.batch_run(b)
+        peng.sync()
+        peng.set_profile(True)
+        peng.stage_times()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            for b in pbatches:
+                peng.batch_run(b)
+        peng.sync()
+        t_iso = time.perf_counter() - t0
+        iso_runs, iso_ms = peng.stage_times()
+        peng.set_profile(False)
+        if dpb_hash(peng, psids, pslots) != h_batched:
+            raise SystemExit("bench.py: batched runs in the one-lane and two-lane contexts differ")
+        for b in pbatches:
+            peng.batch_destroy(b)
+        isolated = {"ms_per_step": 1e3 * t_iso / args.steps, "runs": iso_runs, "stage_ms": iso_ms}
+        log(f"isolated (one lane): {mpix_step * args.steps / t_iso:.1f} Mpix/s, {isolated['ms_per_step']:.2f} ms "
+            f"per step; stage ms per step {[round(x / args.steps, 3) for x in iso_ms]} "
+            f"(sum {sum(iso_ms) / args.steps:.2f})")
+    if not args.no_plain:
+        # ---- plain: the reference's shape -- one picture at a time, one launch per stage
+        # per picture, all on one HIP stream
         t0 = time.perf_counter()
         for _ in range(args.plain_steps):
             for f in range(nframes):
@@ -342,16 +411,30 @@ def main():
                 peng.submit(row[f])
                 peng.sync()
                 lat_p.append(1e3 * (time.perf_counter() - t1))
-        peng.close()
         log(f"plain: {plain:.1f} Mpix/s; batched/plain = {batched / plain:.1f}x")
+    if peng is not None:
+        peng.close()
 
     # ---- roofline of the dominant kernel (algorithmic bytes / event-timed duration)
     # Each frame index launches one stage sequence per lane with pictures (runs counts
     # them); a kernel's algorithmic bytes over its summed launch durations is its rate.
-    dom = max(range(4), key=lambda k: stage_ms[k])
-    launches = max(1, runs)
+    # Durations come from the isolated leg when it ran (one lane: nothing beside the
+    # kernel), else from the two-lane run (durations include the other lane's kernels).
+    def stage_table(runs_, ms_):
+        launches_ = max(1, runs_)
+        out_ = {}
+        for k in range(4):
+            ms = ms_[k] / launches_
+            out_[STAGES[k]] = {"avg_launch_ms": round(ms, 4),
+                               "alg_bytes_per_launch": int(alg[k] * args.steps / launches_),
+                               "alg_GBs": round(alg[k] * args.steps / (ms_[k] * 1e-3) / 1e9, 1) if ms > 0 else None}
+        return launches_, out_
+
+    src_runs, src_ms = (isolated["runs"], isolated["stage_ms"]) if isolated else (runs, stage_ms)
+    dom = max(range(4), key=lambda k: src_ms[k])
+    launches, stages = stage_table(src_runs, src_ms)
     per_launch_bytes = alg[dom] * args.steps / launches
-    avg_ms = stage_ms[dom] / launches
+    avg_ms = src_ms[dom] / launches
     achieved = per_launch_bytes / (avg_ms * 1e-3) / 1e9
     kname = {0: "k_inter", 1: "k_intra_band" if args.intra == "bands" else "k_intra", 2: "k_bs",
              3: {"bands": "k_deblock_band", "mb_wavefront": "k_deblock", "mb_rows": "k_deblock_rows"}[args.deblock]}[dom]
@@ -361,13 +444,10 @@ def main():
             traffic = json.load(f).get(kname)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": kname,
-                "alg_bytes_per_launch": int(per_launch_bytes), "avg_launch_ms": round(avg_ms, 4)}
-    stages = {}
-    for k in range(4):
-        ms = stage_ms[k] / launches
-        stages[STAGES[k]] = {"avg_launch_ms": round(ms, 4), "alg_bytes_per_launch": int(alg[k] * args.steps / launches),
-                             "alg_GBs": round(alg[k] * args.steps / (stage_ms[k] * 1e-3) / 1e9, 1) if ms > 0 else None}
-    mc_dbk = (alg[0] + alg[3]) * args.steps / ((stage_ms[0] + stage_ms[3]) * 1e-3) / 1e9
+                "alg_bytes_per_launch": int(per_launch_bytes), "avg_launch_ms": round(avg_ms, 4),
+                "timing": "isolated: one-lane context, one HIP stream" if isolated else "two lanes (concurrent)"}
+    mc_dbk = (alg[0] + alg[3]) * args.steps / ((src_ms[0] + src_ms[3]) * 1e-3) / 1e9
+    _, stages_mix = stage_table(runs, stage_ms)
 
     serve = None
     if args.serve_seconds > 0:
@@ -396,7 +476,7 @@ def main():
         res = ServeLoop(seng, specs, window_ms=2.0, gens=sgens).run()
         seng.close()
         serve = {"streams": len(specs), "target_fps": 400.0, "offered_fps": round(res.offered_fps, 1),
-                 "frames": res.frames, "seconds": round(res.wall_s, 3),
+                 "served_fps": round(res.served_fps, 1), "frames": res.frames, "seconds": round(res.wall_s, 3),
                  "mpix_s": round(res.mpix_per_s, 1), "p50_ms": round(res.pct(50), 3), "p99_ms": round(res.pct(99), 3),
                  "max_batch": res.max_batch, "window_ms": 2.0, "dpb_budget_gb": 200,
                  "note": "mpix_s is the offered load served (the loop keeps up), not capacity"}
@@ -404,6 +484,10 @@ def main():
             f"p99 {serve['p99_ms']} ms")
         serve["bitexact_vs_plain"] = serving_check(specs, sgens, args)
         log(f"serving check: {serve['bitexact_vs_plain']}")
+        rates = [int(x) for x in args.capacity_rates.split(",") if x.strip()]
+        if rates:
+            serve["capacity"] = serving_capacity(args, rates, args.capacity_seconds, args.capacity_p99_ms)
+            log(f"serving capacity: {serve['capacity']['capacity_fps']} fps at p99 <= {args.capacity_p99_ms} ms")
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -439,6 +523,13 @@ def main():
         "stage_launches": launches,
         "stages": stages,
         "mc_deblock_alg_GBs": round(mc_dbk, 1),
+        "isolated": ({"ms_per_step": round(isolated["ms_per_step"], 3),
+                      "stage_ms_per_step": [round(x / args.steps, 3) for x in isolated["stage_ms"]],
+                      "sum_stage_ms_per_step": round(sum(isolated["stage_ms"]) / args.steps, 3),
+                      "value": round(mpix_step / (isolated["ms_per_step"] * 1e-3), 1),
+                      "mc_deblock_alg_GBs_per_step": round((alg[0] + alg[3]) / (isolated["ms_per_step"] * 1e-3) / 1e9, 1)}
+                     if isolated else None),
+        "stages_two_lanes": stages_mix,
         # both GPU paths against each other; the reference's pictures pin them in the GPU tests
         # (tests/test_gpu_parity.py: every golden case, including full 1280x720, 1920x1088 and
         # 3840x2160 pictures, on the plain and the batched path)

@@ -126,8 +126,9 @@ struct Batch {
     uint32_t *d_inter = nullptr, *d_intra = nullptr, *d_all = nullptr, *d_rows = nullptr;
     uint32_t *d_ibands = nullptr, *d_dbands = nullptr;   // (picture, band) items, band-major
     uint32_t n_inter = 0, n_intra = 0, n_all = 0, n_rows = 0, n_ibands = 0, n_dbands = 0;
-    uint32_t *d_mintra = nullptr, *d_mpairs = nullptr;   // MBAFF frames: intra MBs, MB pairs
-    uint32_t n_mintra = 0, n_mpairs = 0;
+    // MBAFF frames: inter MBs (after the frame pictures' in d_inter), intra MBs, MB pairs
+    uint32_t *d_minter = nullptr, *d_mintra = nullptr, *d_mpairs = nullptr;
+    uint32_t n_minter = 0, n_mintra = 0, n_mpairs = 0;
     bool dbk_off = false;            // no picture of the batch is deblocked
     hipGraphExec_t graph = nullptr;  // the captured stage sequence (kernel nodes only)
     hipGraphExec_t graph_dbk = nullptr;  // its deblocking, when that runs on the lane's deblocking stream
@@ -734,9 +735,10 @@ struct StageSet {
     const uint32_t *inter, *intra, *all, *rows, *ibands, *dbands;
     uint32_t n_inter, n_intra, n_all, n_rows, n_ibands, n_dbands;
     bool dbk_off;
-    // MBAFF frames: intra MBs for the per-MB wavefront, MB pairs for their deblocking
-    const uint32_t *mintra = nullptr, *mpairs = nullptr;
-    uint32_t n_mintra = 0, n_mpairs = 0;
+    // MBAFF frames: inter MBs, intra MBs for the per-MB wavefront, MB pairs for their
+    // deblocking
+    const uint32_t *minter = nullptr, *mintra = nullptr, *mpairs = nullptr;
+    uint32_t n_minter = 0, n_mintra = 0, n_mpairs = 0;
 };
 
 // Parts of a stage set: inter / intra / bS, and deblocking.
@@ -749,20 +751,21 @@ int launch_stages(h264g_ctx *ctx, const StageSet &q, int lane, int parts, hipStr
     int e = 0;
     if (parts & kFront) {
         if (ev) HIPCHK(hipEventRecord(ev[0], s));
-        e = h264g_launch_inter(q.pics, q.inter, q.n_inter, q.intra, q.n_intra, ctx->d_err, s);
+        e = h264g_launch_inter(q.pics, q.inter, q.n_inter, q.intra, q.n_intra, 0, ctx->d_err, s);
         if (e) return h264g_set_error(H264G_E_DEVICE, "inter launch: %s", hipGetErrorString((hipError_t)e));
-        if (q.n_mintra)  // residuals of the intra MBs of MBAFF frames (their own list)
-            e = h264g_launch_inter(q.pics, q.inter, 0, q.mintra, q.n_mintra, ctx->d_err, s);
-        if (e) return h264g_set_error(H264G_E_DEVICE, "inter launch (MBAFF residuals): %s", hipGetErrorString((hipError_t)e));
+        // MBAFF frames: inter MBs and the residuals of their intra MBs (lists of their own)
+        e = h264g_launch_inter(q.pics, q.minter, q.n_minter, q.mintra, q.n_mintra, 1, ctx->d_err, s);
+        if (e) return h264g_set_error(H264G_E_DEVICE, "inter launch (MBAFF): %s", hipGetErrorString((hipError_t)e));
         if (ev) HIPCHK(hipEventRecord(ev[1], s));
         if (q.n_intra)
             e = ctx->intra_schedule == H264G_INTRA_BANDS
                     ? h264g_launch_intra_band(q.pics, q.ibands, q.n_ibands, &ctl[0], &ctl[2].ticket, ctx->intra_waves,
                                               ctx->d_err, s)
-                    : h264g_launch_intra(q.pics, q.intra, q.n_intra, &ctl[0], &ctl[2].ticket, ctx->d_err, ctx->waves, s);
+                    : h264g_launch_intra(q.pics, q.intra, q.n_intra, &ctl[0], &ctl[2].ticket, ctx->d_err, ctx->waves, 0,
+                                         s);
         if (e) return h264g_set_error(H264G_E_DEVICE, "intra launch: %s", hipGetErrorString((hipError_t)e));
         if (q.n_mintra)
-            e = h264g_launch_intra(q.pics, q.mintra, q.n_mintra, &ctl[0], &ctl[2].ticket, ctx->d_err, ctx->waves, s);
+            e = h264g_launch_intra(q.pics, q.mintra, q.n_mintra, &ctl[0], &ctl[2].ticket, ctx->d_err, ctx->waves, 1, s);
         if (e) return h264g_set_error(H264G_E_DEVICE, "intra launch (MBAFF): %s", hipGetErrorString((hipError_t)e));
         if (ev) HIPCHK(hipEventRecord(ev[2], s));
         if (!q.dbk_off) {
@@ -1163,6 +1166,9 @@ int enqueue_picture(h264g_ctx *c, const h264g_pic *p, bool events = true) {
     q.n_dbands = (uint32_t)((p->H + c->dbk_waves - 1) / c->dbk_waves);
     q.dbk_off = p->desc.disable_deblocking_filter_idc == 1;
     if (p->mbaff) {  // items: inter MBs, intra MBs (pair t order), MB pairs (t order)
+        q.minter = q.inter;
+        q.n_minter = q.n_inter;
+        q.n_inter = 0;
         q.mintra = q.intra;
         q.n_mintra = p->n_intra;
         q.mpairs = q.all;
@@ -1250,7 +1256,7 @@ namespace {
 int build_sub(h264g_ctx *c, const std::vector<const h264g_pic *> &pics, Batch *b) {
     const uint32_t n = (uint32_t)pics.size();
     std::vector<DevPic> hp(n);
-    std::vector<uint32_t> inter, intra, dbk;
+    std::vector<uint32_t> inter, intra, dbk, minter;
     int maxt = 0;
     b->uses.clear();
     b->streams.clear();
@@ -1259,9 +1265,12 @@ int build_sub(h264g_ctx *c, const std::vector<const h264g_pic *> &pics, Batch *b
         hp[i] = p->hpic;
         b->uses.push_back(slot_use(p->stream, p->desc));
         b->streams.push_back({p->stream, p->stream_gen});
-        for (uint32_t a : p->inter_list) inter.push_back((i << WI_PIC_SHIFT) | a);
+        for (uint32_t a : p->inter_list) (p->mbaff ? minter : inter).push_back((i << WI_PIC_SHIFT) | a);
         maxt = std::max(maxt, (p->W - 1) + 2 * (p->H - 1));
     }
+    // the frame pictures' inter MBs, then the MBAFF frames' (each set is one launch)
+    const uint32_t n_frame_inter = (uint32_t)inter.size();
+    inter.insert(inter.end(), minter.begin(), minter.end());
     // Wavefront queues of all pictures merged by t = x + 2y (counting sort): every
     // dependency of an entry has a smaller t and therefore an earlier ticket.  MBAFF frames
     // (mbaff = true) go by the t of their MB pairs, the top MB before the bottom one.
@@ -1324,7 +1333,9 @@ int build_sub(h264g_ctx *c, const std::vector<const h264g_pic *> &pics, Batch *b
     b->d_rows = (uint32_t *)(base + o_r);
     b->d_ibands = (uint32_t *)(base + o_ib);
     b->d_dbands = (uint32_t *)(base + o_db);
-    b->n_inter = (uint32_t)inter.size();
+    b->n_inter = n_frame_inter;
+    b->d_minter = b->d_inter + n_frame_inter;
+    b->n_minter = (uint32_t)minter.size();
     b->n_intra = (uint32_t)intra.size();
     b->n_all = (uint32_t)all.size();
     b->n_rows = (uint32_t)rows.size();
@@ -1400,6 +1411,8 @@ StageSet batch_stages(const Batch *b) {
     q.dbands = b->d_dbands;
     q.n_dbands = b->n_dbands;
     q.dbk_off = b->dbk_off;
+    q.minter = b->d_minter;
+    q.n_minter = b->n_minter;
     q.mintra = b->d_mintra;
     q.n_mintra = b->n_mintra;
     q.mpairs = b->d_mpairs;

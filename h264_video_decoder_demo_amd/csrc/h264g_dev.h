@@ -96,12 +96,13 @@ struct DevCtl {
 
 extern "C" {
 // Kernel launchers (kernels.hip); all enqueue on `stream` and return a hipError_t.
+// mbaff: every picture of the launch is an MBAFF frame (else none is).
 int h264g_launch_inter(const DevPic *pics, const uint32_t *items, uint32_t n, const uint32_t *intra_items, uint32_t n_intra,
-                       uint32_t *err, void *stream);
+                       int mbaff, uint32_t *err, void *stream);
 // `epoch` points at the device word shared by all wavefront launches of a context
 // (initially 1; each launch's last wave advances it).
 int h264g_launch_intra(const DevPic *pics, const uint32_t *items, uint32_t n, DevCtl *ctl, uint32_t *epoch,
-                       uint32_t *err, int waves, void *stream);
+                       uint32_t *err, int waves, int mbaff, void *stream);
 int h264g_launch_deblock(const DevPic *pics, const uint32_t *items, uint32_t n, DevCtl *ctl, uint32_t *epoch,
                          uint32_t *err, int waves, void *stream);
 int h264g_launch_deblock_rows(const DevPic *pics, const uint32_t *items, uint32_t n, DevCtl *ctl, uint32_t *epoch,

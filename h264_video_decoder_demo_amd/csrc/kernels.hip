@@ -209,10 +209,11 @@ __device__ __forceinline__ void idct8(int32_t *v, int s) {  // 8.5.13.2, in plac
 // Expands the MB's levels into ws->coef and computes all residual blocks into ws->rl /
 // ws->rc.  Must be called by the whole wave.  Returns false when the MB has no levels.
 // Field MBs of MBAFF frames take the field scans.
+template <bool MBAFF>
 __device__ __forceinline__ bool build_residual(const h264g_mb *mb, const DevAux &ax, const int16_t *levels, WaveRes *ws) {
     const int lane = wave_lane();
     if (!(ax.kind & AUX_RESID)) return false;
-    const bool field = (mb->flags & H264G_MBF_FIELD) != 0;
+    const bool field = MBAFF && (mb->flags & H264G_MBF_FIELD) != 0;
     const int cbp = mb->cbp;
     const int i16 = mb->mb_type == H264G_MB_I16x16;
     const int t8 = (ax.kind & AUX_T8) ? 1 : 0;
@@ -359,9 +360,10 @@ struct MbGeo {
     int mbx, ly, cy, step, mcy, bot;
     bool field;
 };
+template <bool MBAFF>
 __device__ __forceinline__ MbGeo mb_geo(const DevPic &P, int addr) {
     MbGeo g;
-    if (!P.mbaff) {
+    if (!MBAFF || !P.mbaff) {
         const int y = addr / P.W;
         g.mbx = addr % P.W;
         g.ly = y * 16;
@@ -561,12 +563,13 @@ __device__ __forceinline__ int weigh(const DevWeight &w, int c, int pf0, int pf1
 
 
 // Inter MB (or I_PCM copy): prediction + residual, written to the destination slot.
+template <bool MBAFF>
 __device__ void inter_mb(const DevPic &P, int addr, WaveRes *ws) {
     const int lane = wave_lane();
     const h264g_mb *mb = &P.mbs[addr];
     const DevAux &ax = P.aux[addr];  // per-lane fields (refid[q], wp[q]) are read from memory
     const int W = P.W, H = P.H, wl = W * 16, hl = H * 16, wc = W * 8, hc = H * 8;
-    const MbGeo g = mb_geo(P, addr);
+    const MbGeo g = mb_geo<MBAFF>(P, addr);
     const int mbx = g.mbx;
     uint8_t *dst = slot_base(P, P.dst);
     uint8_t *dy = dst, *dcb = dst + (uint64_t)wl * hl, *dcr = dcb + (uint64_t)wc * hc;
@@ -577,7 +580,7 @@ __device__ void inter_mb(const DevPic &P, int addr, WaveRes *ws) {
         dcr[(g.cy + (lane >> 3) * g.step) * wc + mbx * 8 + (lane & 7)] = (uint8_t)pcm[320 + lane];
         return;
     }
-    const bool res = build_residual(mb, ax, P.levels, ws);
+    const bool res = build_residual<MBAFF>(mb, ax, P.levels, ws);
     // ---- luma: lane -> 4x4 block b = lane & 15 (raster), row r = lane >> 4
     {
         const int b = lane & 15, r = lane >> 4, bx = b & 3, by = b >> 2, q = (by >> 1) * 2 + (bx >> 1);
@@ -943,9 +946,10 @@ __device__ __forceinline__ void intra8_prep(const h264g_mb *mb, const Avail &av,
 // (intra_res_place).
 static_assert(offsetof(WaveRes, rc) == offsetof(WaveRes, rl) + 512 && offsetof(WaveRes, rl) % 16 == 0,
               "luma and chroma residual planes are one 768-byte run");
+template <bool MBAFF>
 __device__ __forceinline__ void intra_res_build(const DevPic &P, int addr, WaveRes *ws) {
     const DevAux &ax = P.aux[addr];
-    if (!build_residual(&P.mbs[addr], ax, P.levels, ws)) return;
+    if (!build_residual<MBAFF>(&P.mbs[addr], ax, P.levels, ws)) return;
     const int lane = wave_lane();
     if (lane < 48) ((uint4 *)(P.ires + (size_t)(ax.ires & AUX_IRES_MASK) * 384))[lane] = ((const uint4 *)ws->rl)[lane];
 }
@@ -971,6 +975,7 @@ __device__ __forceinline__ void intra_res_place(const IntraRes &r, WaveIntra *wi
 // above: neighbours (Avail bits: 2 = B, 4 = C, 8 = D) whose samples of row -1 the caller
 // already put into the tile, from the intra strips.
 // An Intra_4x4 MB has run intra4_prep, an Intra_8x8 MB intra8_prep.
+template <bool MBAFF>
 __device__ __forceinline__ void intra_mb(const DevPic &P, int addr, WaveIntra *wi, const Avail &av, bool res,
                                          uint32_t above = 0) {
     const int lane = wave_lane();
@@ -980,7 +985,7 @@ __device__ __forceinline__ void intra_mb(const DevPic &P, int addr, WaveIntra *w
     // in the MB's own view of the picture: frame rows for frame MBs, the rows of its parity
     // for field MBs (rows step apart from its first row), at the same columns; Table 6-4
     // only decides availability (the caller's av).
-    const MbGeo g = mb_geo(P, addr);
+    const MbGeo g = mb_geo<MBAFF>(P, addr);
     const int mbx = g.mbx, st = g.step;
     uint8_t *dst = slot_base(P, P.dst);
     uint8_t *dy = dst, *dcb = dst + (uint64_t)wl * hl, *dcr = dcb + (uint64_t)wc * hc;
@@ -1591,6 +1596,9 @@ __device__ __forceinline__ uint32_t xcd_item(uint32_t b, uint32_t n) {
 
 // Workgroups 0 .. n - 1 reconstruct the inter (and I_PCM) MBs of `items`, the following
 // ni build the residuals of the intra MBs of `iitems`.
+// MBAFF: the instance for MBAFF frames (field MBs: geometry, field scans, field references);
+// frame pictures run the instance without any of it.
+template <bool MBAFF>
 __global__ __launch_bounds__(WAVE) void k_inter(const DevPic *__restrict__ pics, const uint32_t *__restrict__ items, uint32_t n,
                                                 const uint32_t *__restrict__ iitems, uint32_t ni, uint32_t *err) {
     __shared__ WaveRes lds;
@@ -1599,8 +1607,8 @@ __global__ __launch_bounds__(WAVE) void k_inter(const DevPic *__restrict__ pics,
     const uint32_t i = xcd_item(blockIdx.x, total);
     const uint32_t it = i < n ? items[i] : iitems[i - n];
     const DevPic &P = pics[it >> WI_PIC_SHIFT];
-    if (i < n) inter_mb(P, (int)(it & WI_MB_MASK), &lds);
-    else intra_res_build(P, (int)(it & WI_MB_MASK), &lds);
+    if (i < n) inter_mb<MBAFF>(P, (int)(it & WI_MB_MASK), &lds);
+    else intra_res_build<MBAFF>(P, (int)(it & WI_MB_MASK), &lds);
     (void)err;
 }
 
@@ -1663,6 +1671,7 @@ __device__ __forceinline__ Avail intra_avail_mbaff(const DevPic &P, int addr, in
     return av;
 }
 
+template <bool MBAFF>
 __global__ __launch_bounds__(WAVE) void k_intra(const DevPic *__restrict__ pics, const uint32_t *__restrict__ items, uint32_t n,
                                                 DevCtl *ctl, uint32_t *epoch_p, uint32_t *err) {
     __shared__ WaveIntra lds;
@@ -1673,14 +1682,14 @@ __global__ __launch_bounds__(WAVE) void k_intra(const DevPic *__restrict__ pics,
         const uint32_t it = items[t];
         const DevPic &P = pics[it >> WI_PIC_SHIFT];
         const int addr = (int)(it & WI_MB_MASK);
-        int deps[8];
-        const Avail av = P.mbaff ? intra_avail_mbaff(P, addr, deps) : intra_avail(P, addr, deps);
+        int deps[MBAFF ? 8 : 4];
+        const Avail av = MBAFF ? intra_avail_mbaff(P, addr, deps) : intra_avail(P, addr, deps);
         const IntraRes r = intra_res_fetch(P, addr);
         if (P.mbs[addr].mb_type == H264G_MB_I4x4) intra4_prep(&P.mbs[addr], av, &lds);
         else if (P.mbs[addr].mb_type == H264G_MB_I8x8) intra8_prep(&P.mbs[addr], av, &lds);
-        wait_deps(P.flags, deps, P.mbaff ? 8 : 4, epoch, err);
+        wait_deps(P.flags, deps, MBAFF ? 8 : 4, epoch, err);
         intra_res_place(r, &lds);
-        intra_mb(P, addr, &lds, av, r.res);
+        intra_mb<MBAFF>(P, addr, &lds, av, r.res);
         publish(&P.flags[addr], epoch);
     }
     retire(ctl, gridDim.x, epoch_p, epoch);
@@ -1819,7 +1828,7 @@ __global__ __launch_bounds__(WAVE * INTRA_WG_MAX_WAVES) void k_intra_band(const 
                     }
                 }
                 intra_res_place(r, &tiles[w]);
-                intra_mb(P, addr, &tiles[w], av, r.res, above);
+                intra_mb<false>(P, addr, &tiles[w], av, r.res, above);
                 if (to_g) istrip_put(iout + (size_t)x * 8, &tiles[w], epoch);
                 const bool end = i + 1 == e;
                 post(end ? (uint32_t)W : uniform(col[i + 1]) & ICOL_X_MASK);
@@ -2411,18 +2420,25 @@ __global__ void k_copy(const uint4 *src, uint4 *dst, uint64_t n16) {
 extern "C" {
 
 int h264g_launch_inter(const DevPic *pics, const uint32_t *items, uint32_t n, const uint32_t *intra_items, uint32_t n_intra,
-                       uint32_t *err, void *stream) {
+                       int mbaff, uint32_t *err, void *stream) {
     if (n + n_intra == 0) return 0;
-    hipLaunchKernelGGL(k_inter, dim3(n + n_intra), dim3(WAVE), 0, (hipStream_t)stream, pics, items, n, intra_items, n_intra,
-                       err);
+    if (mbaff)
+        hipLaunchKernelGGL(k_inter<true>, dim3(n + n_intra), dim3(WAVE), 0, (hipStream_t)stream, pics, items, n, intra_items,
+                           n_intra, err);
+    else
+        hipLaunchKernelGGL(k_inter<false>, dim3(n + n_intra), dim3(WAVE), 0, (hipStream_t)stream, pics, items, n,
+                           intra_items, n_intra, err);
     return (int)hipGetLastError();
 }
 
 int h264g_launch_intra(const DevPic *pics, const uint32_t *items, uint32_t n, DevCtl *ctl, uint32_t *epoch, uint32_t *err,
-                       int waves, void *stream) {
+                       int waves, int mbaff, void *stream) {
     if (n == 0) return 0;
     uint32_t g = (uint32_t)waves < n ? (uint32_t)waves : n;
-    hipLaunchKernelGGL(k_intra, dim3(g), dim3(WAVE), 0, (hipStream_t)stream, pics, items, n, ctl, epoch, err);
+    if (mbaff)
+        hipLaunchKernelGGL(k_intra<true>, dim3(g), dim3(WAVE), 0, (hipStream_t)stream, pics, items, n, ctl, epoch, err);
+    else
+        hipLaunchKernelGGL(k_intra<false>, dim3(g), dim3(WAVE), 0, (hipStream_t)stream, pics, items, n, ctl, epoch, err);
     return (int)hipGetLastError();
 }
 

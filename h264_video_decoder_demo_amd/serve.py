@@ -17,7 +17,6 @@ from __future__ import annotations
 import collections
 import concurrent.futures
 import dataclasses
-import math
 import os
 import time
 
@@ -38,6 +37,7 @@ class StreamSpec:
     n_frames: int
     start_s: float          # first arrival
     rate_fps: float         # Poisson rate of this stream's frames
+    arrivals: tuple = ()    # the frames' arrival times, when fixed in advance (else drawn)
 
 
 @dataclasses.dataclass
@@ -56,6 +56,11 @@ class ServeResult:
         return self.frames / self.arrival_span_s if self.arrival_span_s > 0 else 0.0
 
     @property
+    def served_fps(self) -> float:
+        """Frames completed over the loop's wall time (first arrival to last completion)."""
+        return self.frames / self.wall_s if self.wall_s > 0 else 0.0
+
+    @property
     def mpix_per_s(self) -> float:
         return self.mpix / self.wall_s
 
@@ -65,24 +70,33 @@ class ServeResult:
 
 
 def make_specs(n_streams: int, total_fps: float, duration_s: float, seed: int = 0x5EED0005, gop: int = 32):
-    """Streams of the serving mix: resolution by the mix shares, frames arriving at
-    total_fps / n_streams each for the whole run.  Start times are uniform within one
-    mean frame gap of a stream (1 / rate), so every stream is active from then on and the
-    offered load is total_fps over the run instead of ramping up."""
+    """Streams of the serving mix: resolution by the mix shares; each stream's frames are
+    a Poisson process of rate total_fps / n_streams on [0, duration_s), so all frames
+    together arrive at total_fps over the whole window (no ramp at the start, no tail
+    after it).  A stream whose process has no arrival in the window is left out."""
     rng = np.random.default_rng(seed)
     shares = np.array([m[0] for m in SERVE_MIX])
     specs = []
     rate = total_fps / n_streams
     for i in range(n_streams):
         k = int(rng.choice(len(SERVE_MIX), p=shares / shares.sum()))
-        start = float(rng.uniform(0, min(1.0 / rate, duration_s)))
-        n = max(1, int(math.ceil(rate * (duration_s - start))))
-        specs.append(StreamSpec(SERVE_MIX[k][1], SERVE_MIX[k][2], seed + 1 + i, n, start, rate))
+        arr, t = [], 0.0
+        while True:
+            t += float(rng.exponential(1.0 / rate))
+            if t >= duration_s:
+                break
+            arr.append(t)
+        if arr:
+            specs.append(StreamSpec(SERVE_MIX[k][1], SERVE_MIX[k][2], seed + 1 + i, len(arr), arr[0], rate,
+                                    tuple(arr)))
     del gop
     return specs
 
 
 def arrival_times(spec: StreamSpec, rng: np.random.Generator):
+    """The spec's fixed arrival times, or n_frames Poisson arrivals from start_s."""
+    if spec.arrivals:
+        return np.asarray(spec.arrivals, dtype=np.float64)
     gaps = rng.exponential(1.0 / spec.rate_fps, size=spec.n_frames)
     gaps[0] = 0.0
     return spec.start_s + np.cumsum(gaps)

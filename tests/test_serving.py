@@ -19,6 +19,18 @@ def test_specs_follow_the_mix_and_rate():
     assert specs == make_specs(400, total_fps=400.0, duration_s=4.0)  # deterministic
 
 
+def test_offered_load_is_constant_over_the_window():
+    """All arrivals fall inside the window, at the target rate in every part of it."""
+    T, fps = 3.0, 400.0
+    specs = make_specs(128, total_fps=fps, duration_s=T)
+    arr = np.sort(np.concatenate([np.asarray(s.arrivals) for s in specs]))
+    assert arr.min() >= 0.0 and arr.max() < T
+    assert all(s.n_frames == len(s.arrivals) and s.start_s == s.arrivals[0] for s in specs)
+    assert len(arr) == pytest.approx(fps * T, rel=0.1)
+    thirds = np.histogram(arr, bins=3, range=(0.0, T))[0]
+    assert thirds.min() > 0.85 * fps * T / 3 and thirds.max() < 1.15 * fps * T / 3
+
+
 def test_arrivals_are_poisson_with_the_stream_rate():
     spec = StreamSpec(4, 4, 1, 20000, 0.5, 50.0)
     t = arrival_times(spec, np.random.default_rng(3))
