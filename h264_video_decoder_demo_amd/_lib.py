@@ -100,7 +100,7 @@ class GenParams(ctypes.Structure):
         ("weighted_pred_flag", c_int32), ("weighted_bipred_idc", c_int32), ("filter_offsets", c_int32),
         ("disable_deblocking_filter_idc", c_int32), ("chroma_qp_index_offset", c_int32),
         ("second_chroma_qp_index_offset", c_int32), ("ipcm_permille", c_int32), ("p_pattern", c_int32),
-        ("spare_slots", c_int32), ("mbaff", c_int32),
+        ("spare_slots", c_int32), ("mbaff", c_int32), ("constrained_intra_pred", c_int32),
     ]
 
 
@@ -147,6 +147,7 @@ _SIGNATURES = {
     "h264g_mvfield_destroy": (None, [c_void_p]),
     "h264g_mvfield_reset": (c_int32, [c_void_p]),
     "h264g_mvfield_derive": (c_int32, [c_void_p, c_uint32, c_void_p, c_void_p]),
+    "h264g_mvfield_intra_modes": (c_int32, [c_void_p, c_uint32, c_void_p, c_void_p, c_int32]),
 }
 
 _lib = None

@@ -32,6 +32,15 @@ int chroma_qp(int qpy, int offset) {  // 8.5.8, Table 8-15 (H264PictureBase.cpp:
 }
 
 bool cls_inter(int c) { return c == H264G_MB_P || c == H264G_MB_P_SKIP || c == H264G_MB_B; }
+// An MB that constrained intra prediction hides from intra MBs.  The reference tests
+// IS_INTER_Prediction_Mode(m_mb_pred_mode) (H264PictureBase.cpp:1129, 1492, 1905, 2158,
+// 836, 973; H264CommonFunc.h:273), which holds for Pred_L0 / Pred_L1 / BiPred only: P_8x8
+// and B_8x8 MBs have m_mb_pred_mode Pred_NA (H264MacroBlock.cpp:99-100, 142) and stay
+// available to intra prediction; P_Skip is Pred_L0.
+bool cip_hides(const h264g_mb &m) {
+    return m.mb_type == H264G_MB_P_SKIP ||
+           ((m.mb_type == H264G_MB_P || m.mb_type == H264G_MB_B) && m.part_shape != H264G_PART_8x8);
+}
 bool cls_intra(int c) { return c == H264G_MB_I4x4 || c == H264G_MB_I8x8 || c == H264G_MB_I16x16; }
 
 #define HIPCHK(expr)                                                                     \
@@ -499,7 +508,7 @@ int stage_picture(h264g_ctx *ctx, Stream &st, const h264g_pic_desc &d, const h26
             auto avail = [&](int dx, int dy) {
                 int nx = mbx + dx, ny = mby + dy;
                 if (nx < 0 || ny < 0 || nx >= W) return false;
-                if (d.constrained_intra_pred_flag && cls_inter(mbs[ny * W + nx].mb_type)) return false;
+                if (d.constrained_intra_pred_flag && cip_hides(mbs[ny * W + nx])) return false;
                 return true;
             };
             bool A = avail(-1, 0), B = avail(0, -1), D = avail(-1, -1);
@@ -642,7 +651,7 @@ int stage_picture(h264g_ctx *ctx, Stream &st, const h264g_pic_desc &d, const h26
     // schedule; an MB's residual sits at its entry's index in P.ires).  The entry carries
     // what the band kernel would otherwise load from the neighbours' records before its
     // wait (ICOL_* in h264g_dev.h): the column, the neighbour availability (A, B, C, D in
-    // bits 0..3; constrained_intra_pred drops inter neighbours), which available
+    // bits 0..3; constrained_intra_pred drops the inter neighbours cip_hides names), which available
     // neighbours are intra MBs of this stage (C only when the MB reads its samples:
     // Intra_8x8, or Intra_4x4 block 5 in modes 3 / 7), and whether it has levels.
     std::vector<uint32_t> irow(H + 1, 0), icol;
@@ -657,7 +666,7 @@ int stage_picture(h264g_ctx *ctx, Stream &st, const h264g_pic_desc &d, const h26
             for (int k = 0; k < 4; k++) {
                 if (!(av & (1u << k))) continue;
                 const int c = mbs[nb[k]].mb_type;
-                if (d.constrained_intra_pred_flag && cls_inter(c)) av &= ~(1u << k);
+                if (d.constrained_intra_pred_flag && cip_hides(mbs[nb[k]])) av &= ~(1u << k);
                 if (cls_intra(c)) dep |= 1u << k;
             }
             const h264g_mb &m = mbs[a];

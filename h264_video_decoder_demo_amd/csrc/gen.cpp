@@ -259,7 +259,18 @@ private:
     void gen_mb(int mx, int my) {
         h264g_mb &mb = pic.mbs[(size_t)my * W + mx];
         init_mb(mb);
-        const bool A = mb_exists(mx - 1, my), B = mb_exists(mx, my - 1), Dn = mb_exists(mx - 1, my - 1);
+        // neighbours for intra prediction: inside the picture and, under constrained intra
+        // prediction, not an MB whose prediction mode the reference counts as inter
+        // (Pred_L0 / Pred_L1 / BiPred: P_Skip and the P / B MBs not split into 8x8; P_8x8
+        // and B_8x8 have Pred_NA and stay available)
+        auto avail = [&](int x, int y) {
+            if (!mb_exists(x, y)) return false;
+            const h264g_mb &n = pic.mbs[(size_t)y * W + x];
+            const bool hides = n.mb_type == H264G_MB_P_SKIP ||
+                               ((n.mb_type == H264G_MB_P || n.mb_type == H264G_MB_B) && n.part_shape != H264G_PART_8x8);
+            return !P.constrained_intra_pred || !hides;
+        };
+        const bool A = avail(mx - 1, my), B = avail(mx, my - 1), Dn = avail(mx - 1, my - 1);
         const int cls = choose_class(mb);
         gen_body(mb, cls, A, B, Dn, false, [&](int mv[2]) { pskip_mv(mx, my, mv); });
         if (cls <= H264G_MB_IPCM) mark_intra(mx, my);
@@ -705,8 +716,9 @@ int32_t h264g_gen_create(const h264g_gen_params *params, h264g_gen **out) {
         p.spare_slots > 16)
         return h264g_set_error(H264G_E_ARG, "gen: bad params");
     // MBAFF: pairs of MB rows; implicit weights of field MBs would need field POCs
-    if (p.mbaff && (p.height_mbs % 2 || p.weighted_bipred_idc == 2))
-        return h264g_set_error(H264G_E_ARG, "gen: MBAFF needs an even height and no implicit weights");
+    if (p.mbaff && (p.height_mbs % 2 || p.weighted_bipred_idc == 2 || p.constrained_intra_pred))
+        return h264g_set_error(H264G_E_ARG,
+                               "gen: MBAFF needs an even height, no implicit weights and no constrained intra");
     h264g_gen *g = new h264g_gen();
     g->p = p;
     std::vector<PicPlan> plan = plan_gop(p.n_frames, p.b_frames, p.gop);
@@ -728,6 +740,7 @@ int32_t h264g_gen_create(const h264g_gen_params *params, h264g_gen **out) {
         d.slice_type = pp.slice_type;
         d.structure = H264G_FRAME;
         d.mbaff = p.mbaff ? 1 : 0;
+        d.constrained_intra_pred_flag = p.constrained_intra_pred ? 1 : 0;
         d.poc = 2 * pp.display;
         d.disable_deblocking_filter_idc = p.disable_deblocking_filter_idc;
         if (p.filter_offsets) {

@@ -349,7 +349,24 @@ __device__ __forceinline__ int ldc(const uint8_t *p, int w, int h, int x, int y)
     return p[clip3(0, h - 1, y) * w + clip3(0, w - 1, x)];
 }
 
-__device__ __forceinline__ uint8_t *slot_base(const DevPic &P, int s) { return P.dpb + (uint64_t)s * P.slot_bytes; }
+// Device memory seen through global-address-space pointers: loads and stores through
+// them are global_* instructions, which count on the vector-memory counter only.  Through
+// generic pointers they would be flat_* instructions, which also count on the LDS counter,
+// so every LDS wait would wait for them too (a prefetch from HBM would stall the next LDS
+// access).
+// (The host pass of this file parses the device code too; it sees plain pointers.)
+#if defined(__HIP_DEVICE_COMPILE__)
+#define GAS __attribute__((address_space(1)))
+#else
+#define GAS
+#endif
+typedef GAS uint8_t gu8;
+template <typename T>
+__device__ __forceinline__ GAS T *G(T *p) { return (GAS T *)p; }
+template <typename T>
+__device__ __forceinline__ const GAS T *G(const T *p) { return (const GAS T *)p; }
+
+__device__ __forceinline__ gu8 *slot_base(const DevPic &P, int s) { return G(P.dpb) + (uint64_t)s * P.slot_bytes; }
 
 // Where an MB's samples lie (6.4.1, H264PictureBase.cpp:2503-2530): the picture row of its
 // first luma / chroma sample and the row step (2 for field MBs of MBAFF frames, whose rows
@@ -377,7 +394,7 @@ __device__ __forceinline__ MbGeo mb_geo(const DevPic &P, int addr) {
     const int pair = addr >> 1, py = pair / P.W;
     g.mbx = pair % P.W;
     g.bot = addr & 1;
-    g.field = (P.mbs[addr].flags & H264G_MBF_FIELD) != 0;
+    g.field = (G(P.mbs)[addr].flags & H264G_MBF_FIELD) != 0;
     g.step = g.field ? 2 : 1;
     g.ly = py * 32 + (g.field ? g.bot : 16 * g.bot);
     g.cy = py * 16 + (g.field ? g.bot : 8 * g.bot);
@@ -566,21 +583,21 @@ __device__ __forceinline__ int weigh(const DevWeight &w, int c, int pf0, int pf1
 template <bool MBAFF>
 __device__ void inter_mb(const DevPic &P, int addr, WaveRes *ws) {
     const int lane = wave_lane();
-    const h264g_mb *mb = &P.mbs[addr];
-    const DevAux &ax = P.aux[addr];  // per-lane fields (refid[q], wp[q]) are read from memory
+    const h264g_mb *mb = &G(P.mbs)[addr];
+    const DevAux &ax = G(P.aux)[addr];  // per-lane fields (refid[q], wp[q]) are read from memory
     const int W = P.W, H = P.H, wl = W * 16, hl = H * 16, wc = W * 8, hc = H * 8;
     const MbGeo g = mb_geo<MBAFF>(P, addr);
     const int mbx = g.mbx;
     uint8_t *dst = slot_base(P, P.dst);
     uint8_t *dy = dst, *dcb = dst + (uint64_t)wl * hl, *dcr = dcb + (uint64_t)wc * hc;
     if (ax.kind & AUX_PCM) {  // 8.3.5 (H264PictureBase.cpp:2449-2499)
-        const int16_t *pcm = P.levels + mb->level_offset;
+        const int16_t *pcm = G(P.levels) + mb->level_offset;
         for (int i = lane; i < 256; i += WAVE) dy[(g.ly + (i >> 4) * g.step) * wl + mbx * 16 + (i & 15)] = (uint8_t)pcm[i];
         dcb[(g.cy + (lane >> 3) * g.step) * wc + mbx * 8 + (lane & 7)] = (uint8_t)pcm[256 + lane];
         dcr[(g.cy + (lane >> 3) * g.step) * wc + mbx * 8 + (lane & 7)] = (uint8_t)pcm[320 + lane];
         return;
     }
-    const bool res = build_residual<MBAFF>(mb, ax, P.levels, ws);
+    const bool res = build_residual<MBAFF>(mb, ax, G(P.levels), ws);
     // ---- luma: lane -> 4x4 block b = lane & 15 (raster), row r = lane >> 4
     {
         const int b = lane & 15, r = lane >> 4, bx = b & 3, by = b >> 2, q = (by >> 1) * 2 + (bx >> 1);
@@ -592,7 +609,7 @@ __device__ void inter_mb(const DevPic &P, int addr, WaveRes *ws) {
             const uint32_t v = luma_list(P, mb, ax, l, g, ws);
             if (l) p1 = v; else p0 = v;
         }
-        const DevWeight &w = P.wt[ax.wp[q]];
+        const DevWeight &w = G(P.wt)[ax.wp[q]];
         uint32_t packed;
         if (!w.mode) {
             packed = pf0 && pf1 ? avg4(p0, p1) : (pf0 ? p0 : p1);
@@ -631,7 +648,7 @@ __device__ void inter_mb(const DevPic &P, int addr, WaveRes *ws) {
         }
         if (pf0) chroma2(slot_base(P, ax.refid[0][q]) + o0, fw, fh, xC, yC, mb->mv[0][b][0], mvy0, c00, c01);
         if (pf1) chroma2(slot_base(P, ax.refid[1][q]) + o1, fw, fh, xC, yC, mb->mv[1][b][0], mvy1, c10, c11);
-        const DevWeight &w = P.wt[ax.wp[q]];
+        const DevWeight &w = G(P.wt)[ax.wp[q]];
         uint8_t *plane = c == 0 ? dcb : dcr;
         int v0 = weigh(w, 1 + c, pf0, pf1, c00, c10);
         int v1 = weigh(w, 1 + c, pf0, pf1, c01, c11);
@@ -719,6 +736,13 @@ __device__ void retire(DevCtl *ctl, uint32_t total_waves, uint32_t *epoch, uint3
 }
 
 __device__ __forceinline__ bool cls_inter(int c) { return c == H264G_MB_P || c == H264G_MB_P_SKIP || c == H264G_MB_B; }
+// An MB that constrained intra prediction hides from intra MBs: the reference tests
+// IS_INTER_Prediction_Mode(m_mb_pred_mode) (H264PictureBase.cpp:1129; H264CommonFunc.h:273),
+// false for P_8x8 / B_8x8 (Pred_NA, H264MacroBlock.cpp:99-100, 142), which stay available.
+__device__ __forceinline__ bool cip_hides(const h264g_mb &m) {
+    return m.mb_type == H264G_MB_P_SKIP ||
+           ((m.mb_type == H264G_MB_P || m.mb_type == H264G_MB_B) && m.part_shape != H264G_PART_8x8);
+}
 __device__ __forceinline__ bool cls_intra(int c) { return c == H264G_MB_I4x4 || c == H264G_MB_I8x8 || c == H264G_MB_I16x16; }
 
 // ------------------------------------------------------------ intra (8.3)
@@ -940,7 +964,7 @@ __device__ __forceinline__ void intra8_prep(const h264g_mb *mb, const Avail &av,
 }
 
 // Residuals of intra MBs.  They need no neighbour samples, so the inter stage, which is
-// parallel over MBs, builds them (intra_res_build) and leaves them in P.ires; the intra
+// parallel over MBs, builds them (intra_res_build) and leaves them in G(P.ires); the intra
 // schedules load an MB's residual (intra_res_fetch, lanes 0-47 one 16-byte piece each)
 // before they wait for its neighbours and put it in the tile after the wait
 // (intra_res_place).
@@ -948,21 +972,21 @@ static_assert(offsetof(WaveRes, rc) == offsetof(WaveRes, rl) + 512 && offsetof(W
               "luma and chroma residual planes are one 768-byte run");
 template <bool MBAFF>
 __device__ __forceinline__ void intra_res_build(const DevPic &P, int addr, WaveRes *ws) {
-    const DevAux &ax = P.aux[addr];
-    if (!build_residual<MBAFF>(&P.mbs[addr], ax, P.levels, ws)) return;
+    const DevAux &ax = G(P.aux)[addr];
+    if (!build_residual<MBAFF>(&G(P.mbs)[addr], ax, G(P.levels), ws)) return;
     const int lane = wave_lane();
-    if (lane < 48) ((uint4 *)(P.ires + (size_t)(ax.ires & AUX_IRES_MASK) * 384))[lane] = ((const uint4 *)ws->rl)[lane];
+    if (lane < 48) ((uint4 *)(G(P.ires) + (size_t)(ax.ires & AUX_IRES_MASK) * 384))[lane] = ((const uint4 *)ws->rl)[lane];
 }
 struct IntraRes {
     uint4 v;
     bool res;
 };
 __device__ __forceinline__ IntraRes intra_res_fetch(const DevPic &P, int addr) {
-    const DevAux &ax = P.aux[addr];
+    const DevAux &ax = G(P.aux)[addr];
     IntraRes r;
     r.res = (ax.kind & AUX_RESID) != 0;
     r.v = make_uint4(0, 0, 0, 0);
-    if (r.res && wave_lane() < 48) r.v = ((const uint4 *)(P.ires + (size_t)(ax.ires & AUX_IRES_MASK) * 384))[wave_lane()];
+    if (r.res && wave_lane() < 48) r.v = ((const uint4 *)(G(P.ires) + (size_t)(ax.ires & AUX_IRES_MASK) * 384))[wave_lane()];
     return r;
 }
 __device__ __forceinline__ void intra_res_place(const IntraRes &r, WaveIntra *wi) {
@@ -979,7 +1003,7 @@ template <bool MBAFF>
 __device__ __forceinline__ void intra_mb(const DevPic &P, int addr, WaveIntra *wi, const Avail &av, bool res,
                                          uint32_t above = 0) {
     const int lane = wave_lane();
-    const h264g_mb *mb = &P.mbs[addr];
+    const h264g_mb *mb = &G(P.mbs)[addr];
     const int W = P.W, H = P.H, wl = W * 16, hl = H * 16, wc = W * 8, hc = H * 8;
     // In an MBAFF frame every neighbouring sample that 6.4.12.2 (Table 6-4) gives an MB lies
     // in the MB's own view of the picture: frame rows for frame MBs, the rows of its parity
@@ -1184,8 +1208,8 @@ __device__ __forceinline__ int dbq_qp(uint32_t q, int plane) { return (q >> (8 *
 __device__ __forceinline__ bool dbq_t8(uint32_t q) { return (q >> 24) & 1u; }
 
 __device__ __forceinline__ void load_view(const DevPic &P, int addr, MbView &v) {
-    const DevAux ax = P.aux[addr];
-    v.cls = P.mbs[addr].mb_type;
+    const DevAux ax = G(P.aux)[addr];
+    v.cls = G(P.mbs)[addr].mb_type;
     v.kind = ax.kind;
     v.nz4 = ax.nz4;
     v.nz8 = ax.nz8;
@@ -1195,7 +1219,7 @@ __device__ __forceinline__ void load_view(const DevPic &P, int addr, MbView &v) 
     v.ref1 = (uint32_t)(uint8_t)ax.refid[1][0] | ((uint32_t)(uint8_t)ax.refid[1][1] << 8) |
              ((uint32_t)(uint8_t)ax.refid[1][2] << 16) | ((uint32_t)(uint8_t)ax.refid[1][3] << 24);
 }
-// Deblocking word of an MB (P.dbq, written by k_bs): qP of luma, Cb edges, Cr edges in
+// Deblocking word of an MB (G(P.dbq), written by k_bs): qP of luma, Cb edges, Cr edges in
 // bytes 0..2, transform_size_8x8_flag in bit 24.
 
 __device__ __forceinline__ bool mv_far(int ax, int ay, int bx, int by) { return iabs(ax - bx) >= 4 || iabs(ay - by) >= 4; }
@@ -1217,7 +1241,7 @@ __device__ __forceinline__ int bs_luma(const DevPic &P, int ap, const MbView &Pv
     int p0 = view_ref(Pv, 0, qp), p1 = view_ref(Pv, 1, qp), q0 = view_ref(Qv, 0, qq), q1 = view_ref(Qv, 1, qq);
     int fp0 = (Pv.pf >> qp) & 1, fp1 = (Pv.pf >> (4 + qp)) & 1, fq0 = (Qv.pf >> qq) & 1, fq1 = (Qv.pf >> (4 + qq)) & 1;
     if (!(((p0 == q0 && p1 == q1) || (p0 == q1 && p1 == q0)) && fp0 + fp1 == fq0 + fq1)) return 1;
-    const h264g_mb *mp = &P.mbs[ap], *mq = &P.mbs[aq];
+    const h264g_mb *mp = &G(P.mbs)[ap], *mq = &G(P.mbs)[aq];
     int mp0x = mp->mv[0][rp][0], mp0y = mp->mv[0][rp][1], mp1x = mp->mv[1][rp][0], mp1y = mp->mv[1][rp][1];
     int mq0x = mq->mv[0][rq][0], mq0y = mq->mv[0][rq][1], mq1x = mq->mv[1][rq][0], mq1y = mq->mv[1][rq][1];
     if (fp0 && !fp1 && fq0 && !fq1 && mv_far(mp0x, mp0y, mq0x, mq0y)) return 1;
@@ -1471,9 +1495,9 @@ __device__ void deblock_mb(const DevPic &P, int addr, WaveDbk *wd, const DbkTabl
     const DbkPlanes d = dbk_planes(P);
     const bool left = mbx > 0, top = mby > 0;
     dbk_load(d, mbx, mby, 1 | (top ? 2 : 0) | (left ? 4 : 0), wd);
-    const uint32_t qq = P.dbq[addr], lq = left ? P.dbq[addr - 1] : 0u, tq = top ? P.dbq[addr - W] : 0u;
+    const uint32_t qq = G(P.dbq)[addr], lq = left ? G(P.dbq)[addr - 1] : 0u, tq = top ? G(P.dbq)[addr - W] : 0u;
     wave_sync();
-    dbk_filter(P, qq, lq, tq, left, top, P.bs + (size_t)addr * 32, wd, tb);
+    dbk_filter(P, qq, lq, tq, left, top, G(P.bs) + (size_t)addr * 32, wd, tb);
     dbk_store(d, mbx, mby, left, top, wd);
 }
 
@@ -1515,7 +1539,7 @@ __device__ void wait_row(const uint64_t *w, uint64_t want, uint32_t *err) {
 
 // Row progress across CUs: epoch-tagged 64-bit words in global memory (agent scope).
 struct RowSyncGlobal {
-    uint64_t *prog;
+    GAS uint64_t *prog;
     uint64_t tag;
     uint32_t *err;
     __device__ void wait(int row, int need) { wait_row(&prog[row], tag | (uint32_t)need, err); }
@@ -1566,14 +1590,14 @@ __device__ void deblock_row(const DevPic &P, int mby, Sync &sync, WaveDbk *wd, c
         const bool left = mbx > 0;
         dbk_load(d, mbx, mby, 1, wd);  // the MB's own samples: final before this launch
         lq = qq;
-        qq = P.dbq[addr];
+        qq = G(P.dbq)[addr];
         if (top) {
-            tq = P.dbq[addr - W];
+            tq = G(P.dbq)[addr - W];
             sync.wait(mby - 1, min(mbx + 2, W));
             dbk_load(d, mbx, mby, 2, wd);
         }
         wave_sync();
-        dbk_filter(P, qq, lq, tq, left, top, P.bs + (size_t)addr * 32, wd, tb);
+        dbk_filter(P, qq, lq, tq, left, top, G(P.bs) + (size_t)addr * 32, wd, tb);
         dbk_store(d, mbx, mby, left, top, wd);
         sync.post(mby, mbx + 1);
         dbk_carry_left(wd);
@@ -1625,15 +1649,15 @@ __device__ __forceinline__ Avail intra_avail(const DevPic &P, int addr, int deps
     for (int k = 0; k < 4; k++) {
         deps[k] = -1;
         if (!(av.m & (1u << k))) continue;
-        const int c = P.mbs[nb[k]].mb_type;
-        if (P.cip && cls_inter(c)) av.m &= ~(1u << k);
+        const int c = G(P.mbs)[nb[k]].mb_type;
+        if (P.cip && cip_hides(G(P.mbs)[nb[k]])) av.m &= ~(1u << k);
         if (cls_intra(c)) deps[k] = nb[k];
     }
     // Samples of C (above right) are read only by Intra_8x8 (the top reference samples
     // p[8..15, -1] of block 1 and their filtering, 8.3.2.2) and by Intra_4x4 block 5 in
     // the modes that use p[4..7, -1] (Diagonal_Down_Left 3, Vertical_Left 7); without
     // them the MB does not wait for C.
-    const h264g_mb &m = P.mbs[addr];
+    const h264g_mb &m = G(P.mbs)[addr];
     const bool uses_c = m.mb_type == H264G_MB_I8x8 ||
                         (m.mb_type == H264G_MB_I4x4 && (m.intra_pred_modes[5] == 3 || m.intra_pred_modes[5] == 7));
     if (!uses_c) deps[2] = -1;
@@ -1649,11 +1673,11 @@ __device__ __forceinline__ Avail intra_avail(const DevPic &P, int addr, int deps
 // for the pair above right only when it reads C; deps holds up to 8 addresses.
 __device__ __forceinline__ Avail intra_avail_mbaff(const DevPic &P, int addr, int deps[8]) {
     const int W = P.W, pair = addr >> 1, px = pair % W, py = pair / W, bot = addr & 1;
-    const bool field = (P.mbs[addr].flags & H264G_MBF_FIELD) != 0, in_pair = !field && bot;
+    const bool field = (G(P.mbs)[addr].flags & H264G_MBF_FIELD) != 0, in_pair = !field && bot;
     const bool A = px > 0, B = py > 0, C = py > 0 && px < W - 1, D = py > 0 && px > 0;
     Avail av;
     av.m = (A ? 1u : 0u) | ((in_pair || B) ? 2u : 0u) | (!in_pair && C ? 4u : 0u) | ((in_pair ? A : D) ? 8u : 0u);
-    const h264g_mb &m = P.mbs[addr];
+    const h264g_mb &m = G(P.mbs)[addr];
     const bool uses_c = m.mb_type == H264G_MB_I8x8 ||
                         (m.mb_type == H264G_MB_I4x4 && (m.intra_pred_modes[5] == 3 || m.intra_pred_modes[5] == 7));
     // neighbouring pairs read: left; above, above-left, above-right (not for the bottom
@@ -1665,9 +1689,9 @@ __device__ __forceinline__ Avail intra_avail_mbaff(const DevPic &P, int addr, in
 #pragma unroll
         for (int h = 0; h < 2; h++) {
             const int a = pn[k] < 0 ? -1 : 2 * pn[k] + h;
-            deps[2 * k + h] = a >= 0 && cls_intra(P.mbs[a].mb_type) ? a : -1;
+            deps[2 * k + h] = a >= 0 && cls_intra(G(P.mbs)[a].mb_type) ? a : -1;
         }
-    if (in_pair && cls_intra(P.mbs[addr - 1].mb_type)) deps[2] = addr - 1;  // slot of the unused above pair
+    if (in_pair && cls_intra(G(P.mbs)[addr - 1].mb_type)) deps[2] = addr - 1;  // slot of the unused above pair
     return av;
 }
 
@@ -1685,12 +1709,12 @@ __global__ __launch_bounds__(WAVE) void k_intra(const DevPic *__restrict__ pics,
         int deps[MBAFF ? 8 : 4];
         const Avail av = MBAFF ? intra_avail_mbaff(P, addr, deps) : intra_avail(P, addr, deps);
         const IntraRes r = intra_res_fetch(P, addr);
-        if (P.mbs[addr].mb_type == H264G_MB_I4x4) intra4_prep(&P.mbs[addr], av, &lds);
-        else if (P.mbs[addr].mb_type == H264G_MB_I8x8) intra8_prep(&P.mbs[addr], av, &lds);
-        wait_deps(P.flags, deps, MBAFF ? 8 : 4, epoch, err);
+        if (G(P.mbs)[addr].mb_type == H264G_MB_I4x4) intra4_prep(&G(P.mbs)[addr], av, &lds);
+        else if (G(P.mbs)[addr].mb_type == H264G_MB_I8x8) intra8_prep(&G(P.mbs)[addr], av, &lds);
+        wait_deps(G(P.flags), deps, MBAFF ? 8 : 4, epoch, err);
         intra_res_place(r, &lds);
         intra_mb<MBAFF>(P, addr, &lds, av, r.res);
-        publish(&P.flags[addr], epoch);
+        publish(&G(P.flags)[addr], epoch);
     }
     retire(ctl, gridDim.x, epoch_p, epoch);
 }
@@ -1792,10 +1816,10 @@ __global__ __launch_bounds__(WAVE * INTRA_WG_MAX_WAVES) void k_intra_band(const 
         const int W = P.W, H = P.H, y0 = band * nw, y = y0 + w, y1 = min(H, y0 + nw);
         if (P.n_intra != 0 && y < H) {
             const bool from_g = y > 0 && y == y0, to_g = y < H - 1 && y == y1 - 1;
-            uint64_t *iout = P.istrip + (size_t)band * W * 8;                 // this band -> next
-            const uint64_t *iin = P.istrip + (size_t)(band - 1) * W * 8;      // previous band -> this
+            uint64_t *iout = G(P.istrip) + (size_t)band * W * 8;                 // this band -> next
+            const uint64_t *iin = G(P.istrip) + (size_t)(band - 1) * W * 8;      // previous band -> this
             RowSyncLds sync{prog, err};
-            const uint32_t *off = P.irow, *col = P.irow + H + 1;
+            const uint32_t *off = G(P.irow), *col = G(P.irow) + H + 1;
             const uint32_t b = uniform(off[y]), e = uniform(off[y + 1]);
             auto post = [&](uint32_t done) {  // progress for the next row of the band
                 if (!to_g && y < H - 1) sync.post(y - y0, (int)done);
@@ -1812,8 +1836,8 @@ __global__ __launch_bounds__(WAVE * INTRA_WG_MAX_WAVES) void k_intra_band(const 
                 IntraRes r;  // in flight during the wait
                 r.res = (ent & ICOL_RES) != 0;
                 r.v = make_uint4(0, 0, 0, 0);
-                if (r.res && wave_lane() < 48) r.v = ((const uint4 *)(P.ires + (size_t)i * 384))[wave_lane()];
-                const h264g_mb *mb = &P.mbs[addr];
+                if (r.res && wave_lane() < 48) r.v = ((const uint4 *)(G(P.ires) + (size_t)i * 384))[wave_lane()];
+                const h264g_mb *mb = &G(P.mbs)[addr];
                 if (mb->mb_type == H264G_MB_I4x4) intra4_prep(mb, av, &tiles[w]);
                 else if (mb->mb_type == H264G_MB_I8x8) intra8_prep(mb, av, &tiles[w]);
                 // the row above up to C (x + 1) or up to B (x) when only B / D are read and intra
@@ -1852,9 +1876,9 @@ __global__ __launch_bounds__(WAVE) void k_deblock(const DevPic *__restrict__ pic
         const DevPic &P = pics[it >> WI_PIC_SHIFT];
         const int addr = (int)(it & WI_MB_MASK), W = P.W, mbx = addr % W, mby = addr / W;
         int deps[2] = {mbx > 0 ? addr - 1 : -1, mby > 0 ? (mbx < W - 1 ? addr - W + 1 : addr - W) : -1};
-        wait_deps(P.flags, deps, 2, epoch, err);
+        wait_deps(G(P.flags), deps, 2, epoch, err);
         deblock_mb(P, addr, &lds, &tb);
-        publish(&P.flags[addr], epoch);
+        publish(&G(P.flags)[addr], epoch);
     }
     retire(ctl, gridDim.x, epoch_p, epoch);
 }
@@ -1864,7 +1888,7 @@ __global__ __launch_bounds__(WAVE) void k_deblock(const DevPic *__restrict__ pic
 // Pair (x, y) follows pair (x - 1, y), whose right columns its left edges filter, and
 // pair (x + 1, y - 1), whose left edges filter the right columns of pair (x, y - 1) that
 // its top edges read: persistent waves over pairs in t = x + 2 y order, one progress word
-// per pair (P.pairprog).  The wave works on an LDS copy of the samples its edges touch:
+// per pair (G(P.pairprog)).  The wave works on an LDS copy of the samples its edges touch:
 // luma rows -8..31 (a field MB's top edge reads four rows of its parity above the pair)
 // by columns -4..15, chroma rows -8..15 by columns -4..7, written back afterwards.
 constexpr int MR_LROWS = 40, MR_LCOLS = 20, MR_CROWS = 24, MR_CCOLS = 12, MR_ROW0 = 8, MR_COL0 = 4;
@@ -1907,10 +1931,10 @@ __device__ void mbaff_region(const DevPic &P, int px, int py, MbaffRegion *rg, b
 // bS 4 at MB edges between frame MBs and at vertical MB edges, 3 at the other intra edges;
 // 2 for levels; 1 at mixed edges; then the motion tests with a vertical limit of 2
 // quarter samples when q is a field MB (:1157-1158), comparing RefPicListX[RefIdxLX]
-// (P.bsref, quirk Q4) with quirk Q3.  I_PCM counts as neither intra nor inter.
+// (G(P.bsref), quirk Q4) with quirk Q3.  I_PCM counts as neither intra nor inter.
 __device__ __forceinline__ int bs_mbaff(const DevPic &P, int ap, int aq, int xp, int yp, int xq, int yq, bool vertical) {
-    const h264g_mb &mp = P.mbs[ap], &mq = P.mbs[aq];
-    const DevAux xa = P.aux[ap], xb = P.aux[aq];
+    const h264g_mb &mp = G(P.mbs)[ap], &mq = G(P.mbs)[aq];
+    const DevAux xa = G(P.aux)[ap], xb = G(P.aux)[aq];
     const bool pf = mp.flags & H264G_MBF_FIELD, qf = mq.flags & H264G_MBF_FIELD;
     const bool intra = ((xa.kind | xb.kind) & AUX_INTRA) != 0, mixed = ap != aq && pf != qf;
     if (ap != aq && intra && ((!pf && !qf) || vertical)) return 4;
@@ -1925,7 +1949,7 @@ __device__ __forceinline__ int bs_mbaff(const DevPic &P, int ap, int aq, int xp,
     if (mixed) return 1;
     const int ylim = qf ? 2 : 4;
     const int qp = bp8, qq = bq8, rp = (yp >> 2) * 4 + (xp >> 2), rq = (yq >> 2) * 4 + (xq >> 2);
-    const int p0 = P.bsref[ap * 8 + qp], p1 = P.bsref[ap * 8 + 4 + qp], q0 = P.bsref[aq * 8 + qq], q1 = P.bsref[aq * 8 + 4 + qq];
+    const int p0 = G(P.bsref)[ap * 8 + qp], p1 = G(P.bsref)[ap * 8 + 4 + qp], q0 = G(P.bsref)[aq * 8 + qq], q1 = G(P.bsref)[aq * 8 + 4 + qq];
     const int fp0 = (xa.pf >> qp) & 1, fp1 = (xa.pf >> (4 + qp)) & 1, fq0 = (xb.pf >> qq) & 1, fq1 = (xb.pf >> (4 + qq)) & 1;
     if (!(((p0 == q0 && p1 == q1) || (p0 == q1 && p1 == q0)) && fp0 + fp1 == fq0 + fq1)) return 1;
     auto far = [&](int ax, int ay, int bx, int by) { return iabs(ax - bx) >= 4 || iabs(ay - by) >= ylim; };
@@ -1997,7 +2021,7 @@ __device__ __forceinline__ void mbaff_edge(const DevPic &P, MbaffRegion *rg, con
         w[0] = pack4(base[pi[3]], base[pi[2]], base[pi[1]], base[pi[0]]);
         w[1] = pack4(base[qi[0]], base[qi[1]], base[qi[2]], base[qi[3]]);
         bS = bs_mbaff(P, ap, addr, xp * s, yp * s, xq * s, yq * s, vertical);
-        const int qpp = P.aux[ap].qdb[plane], qpq = P.aux[addr].qdb[plane], qav = (qpp + qpq + 1) >> 1;
+        const int qpp = G(P.aux)[ap].qdb[plane], qpq = G(P.aux)[addr].qdb[plane], qav = (qpp + qpq + 1) >> 1;
         const int ia = clip3(0, 51, qav + P.offA), ib = clip3(0, 51, qav + P.offB);
         alpha = tb->alpha[ia];
         beta = tb->beta[ib];
@@ -2022,9 +2046,9 @@ __device__ __forceinline__ void mbaff_edge(const DevPic &P, MbaffRegion *rg, con
 // the same order, each chroma pass beside a luma pass (different planes).
 __device__ __forceinline__ void deblock_mb_mbaff(const DevPic &P, int addr, MbaffRegion *rg, const DbkTables *tb) {
     const int lane = wave_lane(), W = P.W, pair = addr >> 1, px = pair % W, py = pair / W, bot = addr & 1;
-    const bool field = (P.mbs[addr].flags & H264G_MBF_FIELD) != 0, t8 = (P.aux[addr].kind & AUX_T8) != 0;
-    const bool lfield = px > 0 && (P.mbs[2 * (pair - 1)].flags & H264G_MBF_FIELD);
-    const bool afield = py > 0 && (P.mbs[2 * (pair - W)].flags & H264G_MBF_FIELD);
+    const bool field = (G(P.mbs)[addr].flags & H264G_MBF_FIELD) != 0, t8 = (G(P.aux)[addr].kind & AUX_T8) != 0;
+    const bool lfield = px > 0 && (G(P.mbs)[2 * (pair - 1)].flags & H264G_MBF_FIELD);
+    const bool afield = py > 0 && (G(P.mbs)[2 * (pair - W)].flags & H264G_MBF_FIELD);
     // 6.4.11.1 with Table 6-4 at (-1, 0) and (0, -1)
     const int mbA = px > 0 ? 2 * (pair - 1) + (field == lfield ? bot : 0) : -1;
     const int mbB = !field && bot ? addr - 1 : (py > 0 ? 2 * (pair - W) + (field && !bot && afield ? 0 : 1) : -1);
@@ -2060,15 +2084,15 @@ __global__ __launch_bounds__(WAVE) void k_deblock_mbaff(const DevPic *__restrict
         const uint32_t it = items[t];
         const DevPic &P = pics[it >> WI_PIC_SHIFT];
         const int pair = (int)(it & WI_MB_MASK), W = P.W, px = pair % W, py = pair / W;
-        wait_dep(P.pairprog, px > 0 ? pair - 1 : -1, epoch, err);
-        wait_dep(P.pairprog, py > 0 ? (px < W - 1 ? pair - W + 1 : pair - W) : -1, epoch, err);
+        wait_dep(G(P.pairprog), px > 0 ? pair - 1 : -1, epoch, err);
+        wait_dep(G(P.pairprog), py > 0 ? (px < W - 1 ? pair - W + 1 : pair - W) : -1, epoch, err);
         __atomic_thread_fence(__ATOMIC_ACQUIRE);
         mbaff_region(P, px, py, &rg, true);
         wave_sync();
         deblock_mb_mbaff(P, 2 * pair, &rg, &tb);
         deblock_mb_mbaff(P, 2 * pair + 1, &rg, &tb);
         mbaff_region(P, px, py, &rg, false);
-        publish(&P.pairprog[pair], epoch);
+        publish(&G(P.pairprog)[pair], epoch);
     }
     retire(ctl, gridDim.x, epoch_p, epoch);
 }
@@ -2088,7 +2112,7 @@ __global__ __launch_bounds__(WAVE) void k_deblock_rows(const DevPic *__restrict_
         if (t < 0) break;
         const uint32_t it = items[t];
         const DevPic &P = pics[it >> WI_PIC_SHIFT];
-        RowSyncGlobal sync{P.rowprog, tag, err};
+        RowSyncGlobal sync{G(P.rowprog), tag, err};
         deblock_row(P, (int)(it & WI_MB_MASK), sync, &lds, &tb);
     }
     retire(ctl, gridDim.x, epoch_p, epoch);
@@ -2105,7 +2129,7 @@ __global__ __launch_bounds__(WAVE) void k_deblock_rows(const DevPic *__restrict_
 // row's top edge) to row y + 1 as a strip: inside a wave from the upper tile to the
 // lower tile, from a wave's lower row to the next wave's upper row through a ring of
 // STRIP_SLOTS LDS slots, and from a band's last row to the next band through a
-// full-width strip array in device memory (P.gstrip, one row of W strips per band
+// full-width strip array in device memory (G(P.gstrip), one row of W strips per band
 // boundary).  No sample otherwise travels between rows: each sample is written to the
 // picture by exactly one half, row y storing rows 0..12 (chroma 0..4) of its MBs and row
 // y + 1 the rows 13..15 (chroma 5..7) after its top-edge filtering (the picture's last
@@ -2165,7 +2189,7 @@ __device__ __forceinline__ DbkPrefetch dbk_fetch(const DevPic &P, const DbkPlane
         f.luma = *(const uint2 *)&d.y[(uint64_t)(mby * 16 + (hl >> 1)) * d.wl + mbx * 16 + (hl & 1) * 8];
         const uint8_t *pl = (hl >> 4) ? d.cr : d.cb;
         f.chroma = *(const uint32_t *)&pl[(uint64_t)(mby * 8 + ((hl >> 1) & 7)) * d.wc + mbx * 8 + (hl & 1) * 4];
-        if (hl < 8) f.bs = *(const uint32_t *)&P.bs[(size_t)(mby * P.W + mbx) * 32 + hl * 4];
+        if (hl < 8) f.bs = *(const uint32_t *)&G(P.bs)[(size_t)(mby * P.W + mbx) * 32 + hl * 4];
     }
     return f;
 }
@@ -2274,12 +2298,12 @@ __device__ void deblock_pair_band(const DevPic &P, int w, int y0, int y1, int b,
     const bool from_g = topU && w == 0;      // the upper row reads the band above's strips
     StripSlot *const ring = sh.ring[w];
     const StripSlot *const above = w > 0 ? sh.ring[w - 1] : nullptr;
-    StripWords *const gout = (StripWords *)P.gstrip + (size_t)b * W;              // this band -> next
-    const StripWords *const gin = (const StripWords *)P.gstrip + (size_t)(b - 1) * W;  // previous band -> this
+    StripWords *const gout = (StripWords *)G(P.gstrip) + (size_t)b * W;              // this band -> next
+    const StripWords *const gin = (const StripWords *)G(P.gstrip) + (size_t)(b - 1) * W;  // previous band -> this
     const int steps = W + (hasL ? 1 : 0);  // the lower row's last MB is stored in step W + 1
     DbkPrefetch f = dbk_fetch(P, d, 0, yu, !lower);
     // deblocking words of each half's MB and of the MB above it, one step ahead like the samples
-    uint32_t qq = 0u, qn = lower ? 0u : P.dbq[yu * W], tn = (!lower && topU) ? P.dbq[(yu - 1) * W] : 0u;
+    uint32_t qq = 0u, qn = lower ? 0u : G(P.dbq)[yu * W], tn = (!lower && topU) ? G(P.dbq)[(yu - 1) * W] : 0u;
     for (int s = 0; s <= steps; s++) {
         const int x = lower ? s - 1 : s;  // this half's MB
         const bool act = lower ? (hasL && s >= 1 && s <= W) : s < W;
@@ -2291,8 +2315,8 @@ __device__ void deblock_pair_band(const DevPic &P, int w, int y0, int y1, int b,
         {  // next step's MBs, in flight during this one
             const bool nU = s + 1 < W, nL = hasL && s < W;
             f = dbk_fetch(P, d, lower ? s : s + 1, y, lower ? nL : nU);
-            const uint32_t qu = nU ? P.dbq[yu * W + s + 1] : 0u, ql = nL ? P.dbq[yl * W + s] : 0u;
-            const uint32_t tu_ = nU && topU ? P.dbq[(yu - 1) * W + s + 1] : 0u, tl_ = nL ? P.dbq[yu * W + s] : 0u;
+            const uint32_t qu = nU ? G(P.dbq)[yu * W + s + 1] : 0u, ql = nL ? G(P.dbq)[yl * W + s] : 0u;
+            const uint32_t tu_ = nU && topU ? G(P.dbq)[(yu - 1) * W + s + 1] : 0u, tl_ = nL ? G(P.dbq)[yu * W + s] : 0u;
             qn = lower ? ql : qu;
             tn = lower ? tl_ : tu_;
         }
@@ -2357,7 +2381,7 @@ __global__ __launch_bounds__(WAVE * DBK_WG_MAX_WAVES) void k_deblock_band(const 
 
 // Boundary strengths of every edge segment of every MB (8.7.2.1, Deblock.cpp:994-1310),
 // stored [direction][4-line segment][edge] so that a deblocking line reads its four edges
-// as one dword, and each MB's deblocking word (P.dbq: qP of luma / Cb edges / Cr edges in
+// as one dword, and each MB's deblocking word (G(P.dbq): qP of luma / Cb edges / Cr edges in
 // bytes 0..2, transform_size_8x8_flag in bit 24); fully parallel: they depend on syntax
 // only, not on samples.  Half a wave per MB; lane j = 16 * dir + 4 * edge + segment.
 __global__ __launch_bounds__(WAVE) void k_bs(const DevPic *__restrict__ pics, const uint32_t *__restrict__ items, uint32_t n) {
@@ -2378,10 +2402,10 @@ __global__ __launch_bounds__(WAVE) void k_bs(const DevPic *__restrict__ pics, co
         const int xp = dir == 0 ? (e ? e - 1 : 15) : k, yp = dir == 0 ? k : (e ? e - 1 : 15);
         bS = bs_luma(P, ap, Pv, addr, Qv, e == 0, xp, yp, xq, yq);
     }
-    P.bs[(size_t)addr * 32 + (j & 16) + ((j & 3) << 2) + ((j >> 2) & 3)] = (uint8_t)bS;  // [dir][segment][edge]
+    G(P.bs)[(size_t)addr * 32 + (j & 16) + ((j & 3) << 2) + ((j >> 2) & 3)] = (uint8_t)bS;  // [dir][segment][edge]
     if (j == 0) {
-        const DevAux &ax = P.aux[addr];
-        P.dbq[addr] = (uint32_t)ax.qdb[0] | ((uint32_t)ax.qdb[1] << 8) | ((uint32_t)ax.qdb[2] << 16) |
+        const DevAux &ax = G(P.aux)[addr];
+        G(P.dbq)[addr] = (uint32_t)ax.qdb[0] | ((uint32_t)ax.qdb[1] << 8) | ((uint32_t)ax.qdb[2] << 16) |
                       ((ax.kind & AUX_T8) ? 1u << 24 : 0u);
     }
 }

@@ -24,6 +24,12 @@
 // -1 and mv 0 (H264InterPrediction.cpp:1862-1873, 593-603).  Only neighbour C can fall
 // there (in P_8x8 / B_8x8, a sub-partition of quadrant 0 or 2 whose C lies in
 // quadrant 1 or 3), and it is then not replaced by D.  The field reproduces that.
+//
+// The field also predicts the luma intra modes of I_NxN MBs (8.3.1.1 / 8.3.2.1), which a
+// parser reads as prev_intra*_pred_mode_flag / rem_intra*_pred_mode: per MB it keeps the
+// prediction class and the sixteen 4x4 modes (an I8x8 MB's four modes, each on its four
+// 4x4 blocks), as the reference keeps m_mb_pred_mode and Intra4x4PredMode /
+// Intra8x8PredMode (H264PictureBase.cpp:773-1058).
 #include "h264g.h"
 #include "h264g_internal.h"
 
@@ -50,12 +56,53 @@ inline int median3(int a, int b, int c) { return a + b + c - std::min(a, std::mi
 
 }  // namespace
 
+// Prediction class of an MB for intra-mode prediction (the reference's m_mb_pred_mode).
+// CLS_INTER: Pred_L0 / Pred_L1 / BiPred, which dcPredModePredictedFlag tests under
+// constrained intra prediction; P_8x8 / B_8x8 have Pred_NA (H264MacroBlock.cpp:99-100,
+// 142) and are CLS_OTHER like Intra_16x16 and I_PCM.
+enum : uint8_t { CLS_NONE = 0, CLS_INTER, CLS_I4, CLS_I8, CLS_OTHER };
+
+// luma4x4BlkIdx of the 4x4 block holding luma sample (x, y) of an MB (6.4.13.1).
+inline int blk4_at(int x, int y) { return 4 * ((y >> 3) * 2 + (x >> 3)) + ((y >> 2) & 1) * 2 + ((x >> 2) & 1); }
+
 struct h264g_mvfield {
     int W = 0, H = 0;  // in MBs
     std::vector<Blk> b;  // (4W) x (4H) blocks, raster
+    std::vector<uint8_t> cls;    // per MB: CLS_*, CLS_NONE until derived in this picture
+    std::vector<uint8_t> imode;  // per MB: 16 modes by luma4x4BlkIdx (I4x4 / I8x8 MBs)
 
     Blk *at(int bx, int by) { return &b[(size_t)by * W * 4 + bx]; }
 
+    // intraMxMPredModeN of the neighbour holding luma sample (x, y) of MB (mx, my), with x
+    // or y possibly -1 (the MB left / above); `cur` holds the current MB's modes derived
+    // so far.  blk8: the 8x8 derivation, whose I4x4 neighbours give their 4x4 block n of
+    // the 8x8 block holding the sample (n = 1 left, 2 above; H264PictureBase.cpp:1010-1037).
+    // dc: dcPredModePredictedFlag (unavailable neighbour, or inter neighbour under
+    // constrained intra prediction).
+    int intra_nb(int mx, int my, int x, int y, bool blk8, int n, uint8_t cur_cls, const uint8_t *cur, bool cip,
+                 bool &dc) {
+        int ax = mx, ay = my;
+        if (x < 0) { ax--; x += 16; }
+        if (y < 0) { ay--; y += 16; }
+        uint8_t c;
+        const uint8_t *modes;
+        if (ax == mx && ay == my) {
+            c = cur_cls;
+            modes = cur;
+        } else {
+            if (ax < 0 || ay < 0 || ax >= W || cls[(size_t)ay * W + ax] == CLS_NONE) {
+                dc = true;
+                return 2;
+            }
+            c = cls[(size_t)ay * W + ax];
+            modes = &imode[((size_t)ay * W + ax) * 16];
+        }
+        if (cip && c == CLS_INTER) dc = true;
+        if (c != CLS_I4 && c != CLS_I8) return 2;
+        const int k = blk4_at(x, y);
+        if (blk8 && c == CLS_I4) return modes[(k & ~3) + n];
+        return modes[k];  // I8x8 modes sit on every 4x4 block of their 8x8 block
+    }
     // Partition covering 4x4 block (mx*4 + x4, my*4 + y4), for list l.
     Nb nb(int mx, int my, int x4, int y4, int l) {
         Nb n{false, -1, {0, 0}};
@@ -131,6 +178,8 @@ int32_t h264g_mvfield_create(int32_t width_mbs, int32_t height_mbs, h264g_mvfiel
     f->H = height_mbs;
     try {
         f->b.assign((size_t)width_mbs * height_mbs * 16, Blk{{-1, -1}, 0, {{0, 0}, {0, 0}}});
+        f->cls.assign((size_t)width_mbs * height_mbs, CLS_NONE);
+        f->imode.assign((size_t)width_mbs * height_mbs * 16, 2);
     } catch (...) {
         delete f;
         return h264g_set_error(H264G_E_NOMEM, "mvfield");
@@ -144,6 +193,37 @@ void h264g_mvfield_destroy(h264g_mvfield *f) { delete f; }
 int32_t h264g_mvfield_reset(h264g_mvfield *f) {
     if (!f) return h264g_set_error(H264G_E_ARG, "mvfield_reset: null field");
     std::fill(f->b.begin(), f->b.end(), Blk{{-1, -1}, 0, {{0, 0}, {0, 0}}});
+    std::fill(f->cls.begin(), f->cls.end(), (uint8_t)CLS_NONE);
+    return 0;
+}
+
+int32_t h264g_mvfield_intra_modes(h264g_mvfield *f, uint32_t mb_addr, h264g_mb *mb, const uint8_t *syntax,
+                                  int32_t constrained_intra_pred) {
+    if (!f || !mb || !syntax) return h264g_set_error(H264G_E_ARG, "mvfield_intra_modes: null argument");
+    if (mb_addr >= (uint32_t)(f->W * f->H)) return h264g_set_error(H264G_E_ARG, "mvfield_intra_modes: MB %u", mb_addr);
+    const bool i8 = mb->mb_type == H264G_MB_I8x8;
+    if (!i8 && mb->mb_type != H264G_MB_I4x4)
+        return h264g_set_error(H264G_E_ARG, "mvfield_intra_modes: MB %u type %d is not I_NxN", mb_addr, mb->mb_type);
+    const int mx = (int)(mb_addr % (uint32_t)f->W), my = (int)(mb_addr / (uint32_t)f->W);
+    const bool cip = constrained_intra_pred != 0;
+    uint8_t cur[16];
+    std::memset(cur, 2, sizeof(cur));
+    const int nblk = i8 ? 4 : 16;
+    for (int k = 0; k < nblk; k++) {
+        // the block's top-left sample (6.4.3 / 6.4.5), its neighbours A (left) and B (above)
+        const int x = i8 ? (k & 1) * 8 : ((k >> 2) & 1) * 8 + (k & 1) * 4;
+        const int y = i8 ? (k >> 1) * 8 : ((k >> 3) & 1) * 8 + ((k >> 1) & 1) * 4;
+        bool dc = false;
+        const int a = f->intra_nb(mx, my, x - 1, y, i8, 1, i8 ? CLS_I8 : CLS_I4, cur, cip, dc);
+        const int b = f->intra_nb(mx, my, x, y - 1, i8, 2, i8 ? CLS_I8 : CLS_I4, cur, cip, dc);
+        const int pred = dc ? 2 : std::min(a, b);
+        const int prev = syntax[2 * k], rem = syntax[2 * k + 1];
+        if (!prev && rem > 7) return h264g_set_error(H264G_E_ARG, "mvfield_intra_modes: MB %u rem %d", mb_addr, rem);
+        const int mode = prev ? pred : (rem < pred ? rem : rem + 1);
+        mb->intra_pred_modes[k] = (uint8_t)mode;
+        if (i8) std::memset(&cur[4 * k], mode, 4);
+        else cur[k] = (uint8_t)mode;
+    }
     return 0;
 }
 
@@ -153,6 +233,15 @@ int32_t h264g_mvfield_derive(h264g_mvfield *f, uint32_t mb_addr, h264g_mb *mb, c
     const int mx = (int)(mb_addr % (uint32_t)f->W), my = (int)(mb_addr / (uint32_t)f->W);
     for (int y = 0; y < 4; y++)  // a partition of this MB is available only once derived
         for (int x = 0; x < 4; x++) f->at(mx * 4 + x, my * 4 + y)->decoded = 0;
+    {  // the MB's class and intra modes, for the intra-mode prediction of later MBs
+        const int t = mb->mb_type;
+        uint8_t *im = &f->imode[(size_t)mb_addr * 16];
+        const bool split = (t == H264G_MB_P || t == H264G_MB_B) && mb->part_shape == H264G_PART_8x8;
+        f->cls[mb_addr] = t == H264G_MB_I4x4 ? CLS_I4 : t == H264G_MB_I8x8 ? CLS_I8
+                        : (t == H264G_MB_I16x16 || t == H264G_MB_IPCM || split) ? CLS_OTHER : CLS_INTER;
+        for (int k = 0; k < 16; k++)
+            im[k] = t == H264G_MB_I4x4 ? mb->intra_pred_modes[k] : t == H264G_MB_I8x8 ? mb->intra_pred_modes[k >> 2] : 2;
+    }
     const int none[2] = {-1, -1}, zero[2][2] = {{0, 0}, {0, 0}};
     switch (mb->mb_type) {
     case H264G_MB_I4x4:

@@ -13,7 +13,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _lib
-from ._lib import MB_DTYPE, Config, GenParams, PicDesc, StreamDesc, check, lib
+from ._lib import MB_DTYPE, MB_I4x4, MB_I8x8, Config, GenParams, PicDesc, StreamDesc, check, lib
 
 
 def _ptr(a: np.ndarray):
@@ -140,18 +140,29 @@ class MotionField:
         """Start of a picture."""
         check(lib().h264g_mvfield_reset(self._h))
 
-    def derive_picture(self, mbs: np.ndarray, mvd: np.ndarray) -> np.ndarray:
+    def derive_picture(self, mbs: np.ndarray, mvd: np.ndarray, imode: np.ndarray = None,
+                       constrained_intra_pred: bool = False) -> np.ndarray:
         """Records with their mv filled in: mbs (MB_DTYPE, raster order; mv ignored),
-        mvd int16 [MB][list][16][2] (each partition's mvd at its top-left 4x4 block)."""
+        mvd int16 [MB][list][16][2] (each partition's mvd at its top-left 4x4 block).
+        imode, when given: uint8 [MB][16][2] intra-mode syntax (prev flag, rem) of the
+        I_NxN MBs, whose intra_pred_modes are then derived too (8.3.1.1 / 8.3.2.1)."""
         assert mbs.dtype == MB_DTYPE and mvd.dtype == np.int16 and mvd.shape == (len(mbs), 2, 16, 2)
         assert len(mbs) == self.width_mbs * self.height_mbs
         out = mbs.copy()
         out["mv"] = 0
         mvd = np.ascontiguousarray(mvd)
+        if imode is not None:
+            assert imode.dtype == np.uint8 and imode.shape == (len(mbs), 16, 2)
+            imode = np.ascontiguousarray(imode)
         self.reset()
         base = out.ctypes.data
         for a in range(len(out)):
-            check(lib().h264g_mvfield_derive(self._h, a, base + a * MB_DTYPE.itemsize, mvd[a].ctypes.data))
+            rec = base + a * MB_DTYPE.itemsize
+            if imode is not None and out["mb_type"][a] in (MB_I4x4, MB_I8x8):
+                out["intra_pred_modes"][a] = 0
+                check(lib().h264g_mvfield_intra_modes(self._h, a, rec, imode[a].ctypes.data,
+                                                      1 if constrained_intra_pred else 0))
+            check(lib().h264g_mvfield_derive(self._h, a, rec, mvd[a].ctypes.data))
         return out
 
 

@@ -396,6 +396,8 @@ typedef struct h264g_gen_params {
     int32_t  mbaff;               /* 1 = MBAFF frames: MB pairs, each pair a field pair
                                      with probability 1/2 (the inferred flag when both
                                      MBs are skipped); height_mbs must be even          */
+    int32_t  constrained_intra_pred; /* constrained_intra_pred_flag: intra MBs predict
+                                     only from intra neighbours (frames only)           */
 } h264g_gen_params;
 
 /* ------------------------------------------------ host motion-vector derivation
@@ -419,6 +421,18 @@ int32_t h264g_mvfield_reset(h264g_mvfield *field);
  * lists with ref_idx >= 0; NULL for intra and P_Skip MBs.  Out: mb->mv of every 4x4
  * block of the used lists.  H264G_E_ARG for a type the field does not derive. */
 int32_t h264g_mvfield_derive(h264g_mvfield *field, uint32_t mb_addr, h264g_mb *mb, const int16_t *mvd);
+/* Intra4x4PredMode / Intra8x8PredMode of I_NxN MB mb_addr from its parsed syntax
+ * (8.3.1.1 / 8.3.2.1; replaces getIntra4x4PredMode / getIntra8x8PredMode,
+ * H264PictureBase.cpp:773-1058): predIntraNxNPredMode = Min(mode of neighbour A, of B),
+ * DC (2) when a neighbour is not available, is inter under constrained intra prediction
+ * or is not I_NxN; an I4x4 neighbour of an 8x8 block gives its 4x4 block 1 (left) or 2
+ * (above) of the neighbouring 8x8 block.  Call it before h264g_mvfield_derive of the same
+ * MB (which records the modes for the MBs after it).  In: mb_type (I4x4 or I8x8);
+ * syntax: uint8 [16][2] (prev_intra*_pred_mode_flag, rem_intra*_pred_mode) per
+ * luma4x4BlkIdx, or per luma8x8BlkIdx in the first four entries for I8x8;
+ * constrained_intra_pred: the PPS flag.  Out: mb->intra_pred_modes. */
+int32_t h264g_mvfield_intra_modes(h264g_mvfield *field, uint32_t mb_addr, h264g_mb *mb, const uint8_t *syntax,
+                                  int32_t constrained_intra_pred);
 
 int32_t h264g_gen_default_params(int32_t config, h264g_gen_params *out);
 int32_t h264g_gen_create(const h264g_gen_params *params, h264g_gen **out);

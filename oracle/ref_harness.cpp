@@ -554,6 +554,28 @@ struct Harness {
         if (!f) fail("cannot write output");
         fwrite(buf.data(), sizeof(int16_t), buf.size(), f);
         fclose(f);
+        // the solved intra-mode syntax: uint8 [MB][16][2] (prev flag, rem) per
+        // luma4x4BlkIdx of I4x4 MBs, per luma8x8BlkIdx (first four) of I8x8 MBs, else 0
+        std::vector<uint8_t> im((size_t)nmb * 16 * 2, 0);
+        for (uint32_t a = 0; a < nmb; a++) {
+            uint8_t *o = &im[(size_t)a * 32];
+            if (mbs[a].mb_type == H264G_MB_I4x4) {
+                for (int b = 0; b < 16; b++) {
+                    o[2 * b] = (uint8_t)ms[a].prev_intra4x4_pred_mode_flag[b];
+                    o[2 * b + 1] = ms[a].prev_intra4x4_pred_mode_flag[b] ? 0 : (uint8_t)ms[a].rem_intra4x4_pred_mode[b];
+                }
+            } else if (mbs[a].mb_type == H264G_MB_I8x8) {
+                for (int b = 0; b < 4; b++) {
+                    o[2 * b] = (uint8_t)ms[a].prev_intra8x8_pred_mode_flag[b];
+                    o[2 * b + 1] = ms[a].prev_intra8x8_pred_mode_flag[b] ? 0 : (uint8_t)ms[a].rem_intra8x8_pred_mode[b];
+                }
+            }
+        }
+        snprintf(path, sizeof(path), "%s/imode_%03d.bin", out.c_str(), idx);
+        f = fopen(path, "wb");
+        if (!f) fail("cannot write output");
+        fwrite(im.data(), 1, im.size(), f);
+        fclose(f);
     }
 
     void write_planes(CH264PictureBase &pic, const char *tag, int idx) {
@@ -731,6 +753,7 @@ int main(int argc, char **argv) {
         else if (a == "--bframes") p.b_frames = atoi(next());
         else if (a == "--t8x8") p.t8x8 = atoi(next());
         else if (a == "--mbaff") p.mbaff = atoi(next());
+        else if (a == "--cip") p.constrained_intra_pred = atoi(next());
         else if (a == "--out") hs.out = next();
         else if (a == "--predeblock") hs.predeblock = true;
         else if (a == "--mvd") hs.mvd = true;

@@ -33,6 +33,10 @@
  *       right of the picture come from the next row of the frame buffer, on the bottom
  *       field's last row from the next plane or past the Cr plane (the caller's planes
  *       carry a zeroed tail, see refcpu.h).
+ *   Q18 constrained intra prediction hides an inter MB only when its m_mb_pred_mode is
+ *       Pred_L0 / Pred_L1 / BiPred (IS_INTER_Prediction_Mode, H264CommonFunc.h:273;
+ *       H264PictureBase.cpp:1129, 1492, 1905, 2158): P_8x8 / B_8x8 MBs (Pred_NA,
+ *       H264MacroBlock.cpp:99-100, 142) stay available to intra prediction.
  * Each function cites the reference code it follows.  Used only by tests/, smoke()
  * and bench.py's CPU baseline, as the checker.
  */
@@ -563,9 +567,11 @@ static int nb_sample(Ctx *X, const uint8_t *plane, int stride, int n, int addr, 
     int xW = 0, yW = 0;
     int na = X->mbaff ? mbaff_nb(X, addr, xN, yN, n, n, &xW, &yW) : frame_nb(X, addr, xN, yN, n, &xW, &yW);
     if (na < 0) return -1;
-    if (X->d->constrained_intra_pred_flag) {
-        int c = X->mbs[na].mb_type;
-        if (c == H264G_MB_P || c == H264G_MB_P_SKIP || c == H264G_MB_B) return -1;
+    if (X->d->constrained_intra_pred_flag) {  /* quirk Q18 */
+        const h264g_mb *m = &X->mbs[na];
+        if (m->mb_type == H264G_MB_P_SKIP ||
+            ((m->mb_type == H264G_MB_P || m->mb_type == H264G_MB_B) && m->part_shape != H264G_PART_8x8))
+            return -1;
     }
     int x0, y0, pitch;
     mb_plane_origin(X, na, n, stride, &x0, &y0, &pitch);

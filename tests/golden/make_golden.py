@@ -63,11 +63,15 @@ CASES = {
                          "ipcm_permille": 30, "disable_deblocking_filter_idc": 2}),
     "mbaff_tall": (3, {"width_mbs": 2, "height_mbs": 344, "n_frames": 4, "seed": 5}),
     "mbaff_1080p": (3, {}),
+    # constrained intra prediction: intra MBs predict from intra neighbours only, and an
+    # inter neighbour makes the intra-mode prediction DC
+    "high_small_cip": (2, {"width_mbs": 40, "height_mbs": 24, "constrained_intra_pred": 1, "seed": 12}),
+    "high_odd_cip": (2, {"width_mbs": 23, "height_mbs": 17, "constrained_intra_pred": 1, "seed": 13}),
 }
 
 _CLI = {"n_frames": "--frames", "width_mbs": "--width", "height_mbs": "--height", "seed": "--seed",
         "ipcm_permille": "--ipcm", "disable_deblocking_filter_idc": "--deblock", "max_refs": "--refs",
-        "b_frames": "--bframes", "t8x8": "--t8x8", "mbaff": "--mbaff"}
+        "b_frames": "--bframes", "t8x8": "--t8x8", "mbaff": "--mbaff", "constrained_intra_pred": "--cip"}
 
 
 def harness_args(config: int, ov: dict) -> list:

@@ -8,8 +8,13 @@ the reference's Derivation_process_for_luma_motion_vector_prediction
 (H264InterPrediction.cpp:1660-2001), decodes the picture again from those mvds and
 checks that every derived motion vector equals the generator's.  `--mvd` dumps them:
 int16 [picture][MB][list][16][2], each (sub-)partition's mvd at its top-left 4x4 block
-(raster index inside the MB) for the lists it uses.  These fixtures pin the library's
-host motion-vector derivation (h264g_mvfield_*) to the reference's.
+(raster index inside the MB) for the lists it uses.  The harness also solves the
+intra-mode syntax of I_NxN MBs with the reference's getIntra4x4PredMode /
+getIntra8x8PredMode (H264PictureBase.cpp:773-1058) and decodes the picture again from it:
+`imode` holds it, uint8 [picture][MB][16][2] (prev_intra*_pred_mode_flag,
+rem_intra*_pred_mode) per luma4x4BlkIdx (I4x4) or in the first four entries per
+luma8x8BlkIdx (I8x8).  These fixtures pin the library's host derivation
+(h264g_mvfield_*) to the reference's.
 
 Run in a container that has /root/reference:  python tests/golden/make_mvd.py
 """
@@ -35,6 +40,8 @@ CASES = {
     "high_small_seed7": (2, {"width_mbs": 23, "height_mbs": 17, "seed": 7}),
     "narrow": (2, {"width_mbs": 1, "height_mbs": 12, "max_refs": 1, "b_frames": 0}),
     "flat": (2, {"width_mbs": 14, "height_mbs": 1, "max_refs": 2}),
+    "high_small_cip": (2, {"width_mbs": 40, "height_mbs": 24, "constrained_intra_pred": 1, "seed": 12}),
+    "high_odd_cip": (2, {"width_mbs": 23, "height_mbs": 17, "constrained_intra_pred": 1, "seed": 13}),
 }
 
 
@@ -51,7 +58,9 @@ def run_case(config: int, ov: dict):
         n = info["pictures"]
         mvd = np.stack([np.fromfile(os.path.join(d, f"mvd_{i:03d}.bin"), np.int16).reshape(nmb, 2, 16, 2)
                         for i in range(n)])
-    return g.digest, mvd
+        imode = np.stack([np.fromfile(os.path.join(d, f"imode_{i:03d}.bin"), np.uint8).reshape(nmb, 16, 2)
+                          for i in range(n)])
+    return g.digest, mvd, imode
 
 
 def main():
@@ -59,8 +68,8 @@ def main():
         subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "ref", "-j8"], check=True)
     sys.path.insert(0, ROOT)
     for name, (config, ov) in CASES.items():
-        digest, mvd = run_case(config, ov)
-        np.savez_compressed(os.path.join(HERE, f"mvd_{name}.npz"), mvd=mvd, digest=np.uint64(digest),
+        digest, mvd, imode = run_case(config, ov)
+        np.savez_compressed(os.path.join(HERE, f"mvd_{name}.npz"), mvd=mvd, imode=imode, digest=np.uint64(digest),
                             config=np.int32(config), overrides=json.dumps(ov, sort_keys=True))
         print(name, hex(digest), mvd.shape, int(np.count_nonzero(mvd)), flush=True)
 

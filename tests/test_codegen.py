@@ -41,3 +41,11 @@ def test_kernels_use_no_scratch(device_asm):
     sizes = [int(line.split(":")[1]) for line in device_asm.splitlines()
              if line.strip().startswith(".private_segment_fixed_size:")]
     assert sizes and all(s == 0 for s in sizes), sizes
+
+
+def test_kernels_use_no_flat_memory_instructions(device_asm):
+    """Device memory is reached through global-address-space pointers: flat_* loads and
+    stores would also count on the LDS counter, so every LDS wait would wait for them."""
+    found = sorted({line.split()[0] for line in device_asm.splitlines()
+                    if line.strip().startswith(("flat_load", "flat_store", "flat_atomic"))})
+    assert not found, f"kernels contain {found}"

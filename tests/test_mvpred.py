@@ -48,6 +48,32 @@ def test_derived_mvs_equal_generator(path):
     assert checked[4] > 0 and checked[5] + checked[6] > 0, checked
 
 
+@pytest.mark.parametrize("path", CASES, ids=[os.path.basename(p)[4:-4] for p in CASES])
+def test_derived_intra_modes_equal_generator(path):
+    """The intra modes of I_NxN MBs derived from the reference-solved prev / rem syntax
+    (8.3.1.1 / 8.3.2.1, getIntra4x4PredMode / getIntra8x8PredMode) are the generator's,
+    I4x4 and I8x8 MBs beside every kind of neighbour, and with constrained intra
+    prediction (inter neighbours predict DC) in the *_cip streams."""
+    z = np.load(path, allow_pickle=False)
+    g = Generator(int(z["config"]), **json.loads(str(z["overrides"])))
+    assert g.digest == int(z["digest"]), "generator changed: regenerate with tests/golden/make_mvd.py"
+    f = MotionField(g.width_mbs, g.height_mbs)
+    n4 = n8 = rem = 0
+    for i in range(g.num_pictures):
+        desc, mbs, _ = g.picture(i)
+        got = f.derive_picture(mbs, z["mvd"][i], z["imode"][i], bool(desc.constrained_intra_pred_flag))
+        for a in np.nonzero(mbs["mb_type"] == _lib.MB_I4x4)[0]:
+            assert (got["intra_pred_modes"][a] == mbs["intra_pred_modes"][a]).all(), (i, int(a))
+            n4 += 1
+        for a in np.nonzero(mbs["mb_type"] == _lib.MB_I8x8)[0]:
+            assert (got["intra_pred_modes"][a, :4] == mbs["intra_pred_modes"][a, :4]).all(), (i, int(a))
+            n8 += 1
+        rem += int((z["imode"][i][..., 0] == 0).sum())
+    assert n4 > 0 and rem > 0
+    if int(z["config"]) == 2:
+        assert n8 > 0
+
+
 def test_isolated_mb_predicts_zero():
     """With no decoded neighbour every prediction is 0 (8.4.1.3.2): mv = mvd."""
     mbs = np.zeros(1, dtype=_lib.MB_DTYPE)
