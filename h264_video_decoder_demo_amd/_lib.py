@@ -27,7 +27,7 @@ H264G_E_TIMEOUT = -8
 
 SLICE_P, SLICE_B, SLICE_I = 0, 1, 2
 MB_I4x4, MB_I8x8, MB_I16x16, MB_IPCM, MB_P, MB_P_SKIP, MB_B = range(7)
-MBF_T8x8, MBF_FIELD = 0x01, 0x02
+MBF_T8x8, MBF_FIELD, MBF_SKIP, MBF_DIRECT_ALL = 0x01, 0x02, 0x04, 0xF0
 
 
 class H264GError(RuntimeError):
@@ -101,6 +101,7 @@ class GenParams(ctypes.Structure):
         ("disable_deblocking_filter_idc", c_int32), ("chroma_qp_index_offset", c_int32),
         ("second_chroma_qp_index_offset", c_int32), ("ipcm_permille", c_int32), ("p_pattern", c_int32),
         ("spare_slots", c_int32), ("mbaff", c_int32), ("constrained_intra_pred", c_int32),
+        ("direct_permille", c_int32),
     ]
 
 
@@ -148,6 +149,7 @@ _SIGNATURES = {
     "h264g_mvfield_reset": (c_int32, [c_void_p]),
     "h264g_mvfield_derive": (c_int32, [c_void_p, c_uint32, c_void_p, c_void_p]),
     "h264g_mvfield_intra_modes": (c_int32, [c_void_p, c_uint32, c_void_p, c_void_p, c_int32]),
+    "h264g_mvfield_colocated": (c_int32, [c_void_p, c_void_p, c_int32]),
 }
 
 _lib = None

@@ -126,8 +126,9 @@ struct LevelWriter {
 
 class PictureGen {
 public:
-    PictureGen(const h264g_gen_params &p, Rng &rng, h264g_pic_desc &desc, Picture &pic)
-        : P(p), R(rng), D(desc), pic(pic), W(p.width_mbs), H(p.height_mbs) {
+    // col: the records of RefPicList1[0] (B pictures with direct MBs), else null
+    PictureGen(const h264g_gen_params &p, Rng &rng, h264g_pic_desc &desc, Picture &pic, const h264g_mb *col = nullptr)
+        : P(p), R(rng), D(desc), pic(pic), W(p.width_mbs), H(p.height_mbs), col(col) {
         motion.assign((size_t)W * 4 * H * 4, Motion());
     }
 
@@ -149,6 +150,7 @@ private:
     h264g_pic_desc &D;
     Picture &pic;
     int W, H;
+    const h264g_mb *col;
     int qp_prev = 26;
     std::vector<Motion> motion;    // per 4x4 block: picture raster (frames), or [MB address][raster
                                    // block] (MBAFF frames)
@@ -272,7 +274,8 @@ private:
         };
         const bool A = avail(mx - 1, my), B = avail(mx, my - 1), Dn = avail(mx - 1, my - 1);
         const int cls = choose_class(mb);
-        gen_body(mb, cls, A, B, Dn, false, [&](int mv[2]) { pskip_mv(mx, my, mv); });
+        gen_body(mb, cls, A, B, Dn, false, [&](int mv[2]) { pskip_mv(mx, my, mv); },
+                 [&](h264g_mb &m) { direct_spatial(mx, my, m); });
         if (cls <= H264G_MB_IPCM) mark_intra(mx, my);
         else store_motion(mx, my, mb);
     }
@@ -293,13 +296,23 @@ private:
                 cls = u == 0 ? H264G_MB_I16x16 : (u == 1 ? H264G_MB_I4x4 : (P.t8x8 ? H264G_MB_I8x8 : H264G_MB_I4x4));
             } else if (st == H264G_SLICE_P && R.permille(P.skip_permille)) {
                 cls = H264G_MB_P_SKIP;
+            } else if (st == H264G_SLICE_B && P.direct_permille > 0 && R.permille(P.direct_permille)) {
+                // B_Skip, B_Direct_16x16, or B_8x8 with B_Direct_8x8 sub-macroblocks
+                cls = H264G_MB_B;
+                mb.part_shape = H264G_PART_8x8;
+                const int u = R.uni(0, 2);
+                mb.flags |= u == 2 ? (uint8_t)(R.uni(1, 14) << 4) : (uint8_t)H264G_MBF_DIRECT_ALL;
+                if (u == 0) mb.flags |= H264G_MBF_SKIP;
             } else {
                 cls = st == H264G_SLICE_P ? H264G_MB_P : H264G_MB_B;
                 int u = R.uni(0, 99);
                 mb.part_shape = u < 40 ? H264G_PART_16x16 : (u < 55 ? H264G_PART_16x8 : (u < 70 ? H264G_PART_8x16 : H264G_PART_8x8));
             }
         }
-        if (P.ipcm_permille && cls != H264G_MB_P_SKIP && R.permille(P.ipcm_permille)) cls = H264G_MB_IPCM;
+        if (P.ipcm_permille && cls != H264G_MB_P_SKIP && R.permille(P.ipcm_permille)) {
+            cls = H264G_MB_IPCM;
+            mb.flags &= (uint8_t)~(H264G_MBF_SKIP | H264G_MBF_DIRECT_ALL);  // no longer a direct B MB
+        }
         mb.mb_type = (uint8_t)cls;
         return cls;
     }
@@ -308,8 +321,8 @@ private:
     // availability of the neighbouring MBs for intra prediction; field: a field MB of an
     // MBAFF frame (its reference indices address the field list, twice as long);
     // pskip(mv): the P_Skip motion vector.
-    template <class PSkip>
-    void gen_body(h264g_mb &mb, int cls, bool A, bool B, bool Dn, bool field, PSkip pskip) {
+    template <class PSkip, class Direct>
+    void gen_body(h264g_mb &mb, int cls, bool A, bool B, bool Dn, bool field, PSkip pskip, Direct direct) {
         bool intra = cls <= H264G_MB_IPCM;
         LevelWriter lw;
         mb.level_offset = (uint32_t)pic.levels.size();
@@ -347,6 +360,11 @@ private:
             return;
         }
         if (cls == H264G_MB_P || cls == H264G_MB_B) gen_inter(mb, field);
+        if (mb.flags & H264G_MBF_DIRECT_ALL) direct(mb);
+        if (mb.flags & H264G_MBF_SKIP) {  // B_Skip: no residual, mb_qp_delta absent
+            mb.qp_y = (uint8_t)qp_prev;
+            return;
+        }
 
         // ---- transform size for inter MBs (eligible: no sub-8x8 partitions)
         bool inter = cls == H264G_MB_P || cls == H264G_MB_B;
@@ -446,6 +464,82 @@ private:
                     m.mv[l][1] = mb.mv[l][y * 4 + x][1];
                 }
             }
+    }
+
+    // --- 8.4.1.2.2 spatial direct prediction (frames, direct_8x8_inference_flag 1) --
+    // (H264InterPrediction.cpp:1259-1432.)  Reference indices: MinPositive over the
+    // neighbouring partitions A, B and C (D when C is not available) of the MB as one
+    // 16-wide partition; both negative: 0 / 0 with zero vectors.  Vectors: the 16x16
+    // median prediction of each list, zero where the list's index is 0 and the quadrant's
+    // co-located block (the corner 4x4 block of the same quadrant in RefPicList1[0],
+    // 8.4.1.2.1) has reference index 0 and a vector within +-1.  As in the reference
+    // (H264InterPrediction.cpp:1226-1249) an intra co-located MB has index -1, except
+    // I_PCM, which is not intra there and reads its zeroed motion: index 0, vector 0.
+    void nb_list(int bx, int by, int l, bool &avail, int &ref, int mv[2]) {
+        mv[0] = mv[1] = 0;
+        ref = -1;
+        avail = bx >= 0 && by >= 0 && bx < W * 4 && by < H * 4;
+        if (!avail) return;
+        const Motion &m = mo(bx, by);
+        if (m.intra || m.ref[l] < 0) return;
+        ref = m.ref[l];
+        mv[0] = m.mv[l][0];
+        mv[1] = m.mv[l][1];
+    }
+
+    void direct_spatial(int mx, int my, h264g_mb &mb) {
+        const int x0 = mx * 4, y0 = my * 4;
+        const bool c_exists = mb_exists(mx + 1, my - 1);
+        bool av[3][2];
+        int rf[3][2], mv[3][2][2];
+        for (int l = 0; l < 2; l++) {
+            nb_list(x0 - 1, y0, l, av[0][l], rf[0][l], mv[0][l]);
+            nb_list(x0, y0 - 1, l, av[1][l], rf[1][l], mv[1][l]);
+            if (c_exists) nb_list(x0 + 4, y0 - 1, l, av[2][l], rf[2][l], mv[2][l]);
+            else nb_list(x0 - 1, y0 - 1, l, av[2][l], rf[2][l], mv[2][l]);
+        }
+        auto min_pos = [](int a, int b) { return a >= 0 && b >= 0 ? std::min(a, b) : std::max(a, b); };
+        int ref[2], mvp[2][2] = {{0, 0}, {0, 0}};
+        for (int l = 0; l < 2; l++) ref[l] = min_pos(rf[0][l], min_pos(rf[1][l], rf[2][l]));
+        const bool zero = ref[0] < 0 && ref[1] < 0;
+        if (zero) ref[0] = ref[1] = 0;
+        for (int l = 0; l < 2 && !zero; l++) {
+            if (ref[l] < 0) continue;
+            int r[3], m[3][2];
+            for (int k = 0; k < 3; k++) { r[k] = rf[k][l]; m[k][0] = mv[k][l][0]; m[k][1] = mv[k][l][1]; }
+            if (!av[1][l] && !av[2][l] && av[0][l]) {  // 8.4.1.3.1: B and C take A
+                for (int k = 1; k < 3; k++) { r[k] = r[0]; m[k][0] = m[0][0]; m[k][1] = m[0][1]; }
+            }
+            const int n = (r[0] == ref[l]) + (r[1] == ref[l]) + (r[2] == ref[l]);
+            for (int c = 0; c < 2; c++) {
+                if (n == 1) mvp[l][c] = r[0] == ref[l] ? m[0][c] : (r[1] == ref[l] ? m[1][c] : m[2][c]);
+                else mvp[l][c] = m[0][c] + m[1][c] + m[2][c] - std::min(m[0][c], std::min(m[1][c], m[2][c])) -
+                                 std::max(m[0][c], std::max(m[1][c], m[2][c]));
+            }
+        }
+        const h264g_mb &cm = col[(size_t)my * W + mx];
+        static const int corner[4] = {0, 3, 12, 15};
+        for (int q = 0; q < 4; q++) {
+            if (!(mb.flags & H264G_MBF_DIRECT(q))) continue;
+            bool colzero;
+            if (cm.mb_type == H264G_MB_I4x4 || cm.mb_type == H264G_MB_I8x8 || cm.mb_type == H264G_MB_I16x16) {
+                colzero = false;
+            } else if (cm.mb_type == H264G_MB_IPCM) {
+                colzero = true;
+            } else {
+                const int lc = cm.ref_idx[0][q] >= 0 ? 0 : 1;
+                const int16_t *v = cm.mv[lc][corner[q]];
+                colzero = cm.ref_idx[lc][q] == 0 && v[0] >= -1 && v[0] <= 1 && v[1] >= -1 && v[1] <= 1;
+            }
+            const int qx = (q & 1) * 2, qy = (q >> 1) * 2;
+            for (int l = 0; l < 2; l++) {
+                mb.ref_idx[l][q] = (int8_t)ref[l];
+                const bool z = zero || ref[l] < 0 || (ref[l] == 0 && colzero);
+                for (int y = qy; y < qy + 2; y++)
+                    for (int x = qx; x < qx + 2; x++)
+                        for (int c = 0; c < 2; c++) mb.mv[l][y * 4 + x][c] = ref[l] < 0 ? 0 : (int16_t)(z ? 0 : mvp[l][c]);
+            }
+        }
     }
 
     // --- MBAFF frames ----------------------------------------------------------
@@ -590,7 +684,8 @@ private:
                     const bool A = px > 0, pb = py > 0;
                     const bool B = (!field && k) ? true : pb;
                     const bool Dn = (!field && k) ? A : (pb && px > 0);
-                    gen_body(mb, cls, A, B, Dn, field != 0, [&](int mv[2]) { pskip_mv_mbaff(addr, mv); });
+                    gen_body(mb, cls, A, B, Dn, field != 0, [&](int mv[2]) { pskip_mv_mbaff(addr, mv); },
+                             [](h264g_mb &) {});
                     store_motion_mbaff(addr, mb, cls <= H264G_MB_IPCM);
                 }
             }
@@ -644,6 +739,10 @@ private:
             }
         } else {
             for (int q = 0; q < 4; q++) {
+                if (mb.flags & H264G_MBF_DIRECT(q)) {  // direct: derived after the others' syntax
+                    mb.sub_part_shape[q] = H264G_SUB_8x8;
+                    continue;
+                }
                 int sub = R.uni(0, 3);
                 mb.sub_part_shape[q] = (uint8_t)sub;
                 int pred, refs[2];
@@ -716,15 +815,20 @@ int32_t h264g_gen_create(const h264g_gen_params *params, h264g_gen **out) {
         p.spare_slots > 16)
         return h264g_set_error(H264G_E_ARG, "gen: bad params");
     // MBAFF: pairs of MB rows; implicit weights of field MBs would need field POCs
-    if (p.mbaff && (p.height_mbs % 2 || p.weighted_bipred_idc == 2 || p.constrained_intra_pred))
+    if (p.mbaff && (p.height_mbs % 2 || p.weighted_bipred_idc == 2 || p.constrained_intra_pred || p.direct_permille))
         return h264g_set_error(H264G_E_ARG,
-                               "gen: MBAFF needs an even height, no implicit weights and no constrained intra");
+                               "gen: MBAFF needs an even height, no implicit weights, no constrained intra, no direct");
+    // direct MBs take both lists' indices from their neighbours: with implicit weights one
+    // picture in both lists would make the reference divide by td = 0
+    // (H264InterPrediction.cpp:2958)
+    if (p.direct_permille < 0 || p.direct_permille > 1000 || (p.direct_permille && p.weighted_bipred_idc == 2))
+        return h264g_set_error(H264G_E_ARG, "gen: direct MBs need direct_permille in 0..1000 and no implicit weights");
     h264g_gen *g = new h264g_gen();
     g->p = p;
     std::vector<PicPlan> plan = plan_gop(p.n_frames, p.b_frames, p.gop);
 
     // DPB bookkeeping: reference frames in decoding order with their slot, POC, frame_num.
-    struct RefEntry { int slot, poc, frame_num; };
+    struct RefEntry { int slot, poc, frame_num, pic; };
     std::vector<RefEntry> refs;
     int n_slots = p.max_refs + 1 + p.spare_slots, last_dst = n_slots - 1;
     std::vector<int> slot_busy(n_slots, 0);
@@ -811,7 +915,12 @@ int32_t h264g_gen_create(const h264g_gen_params *params, h264g_gen **out) {
         last_dst = dst;
 
         // ---- macroblocks
-        PictureGen pg(p, rng, d, pic);
+        const h264g_mb *col = pp.slice_type == H264G_SLICE_B && !l1.empty() ? g->pics[l1[0].pic].mbs.data() : nullptr;
+        if (p.direct_permille && pp.slice_type == H264G_SLICE_B && !col) {
+            delete g;
+            return h264g_set_error(H264G_E_STATE, "gen: B picture without RefPicList1[0]");
+        }
+        PictureGen pg(p, rng, d, pic, col);
         pg.run();
 
         // ---- reference marking: sliding window over max_refs frames
@@ -821,7 +930,7 @@ int32_t h264g_gen_create(const h264g_gen_params *params, h264g_gen **out) {
                 slot_busy[oldest->slot] = 0;
                 refs.erase(oldest);
             }
-            refs.push_back({dst, d.poc, pp.frame_num});
+            refs.push_back({dst, d.poc, pp.frame_num, (int)i});
             slot_busy[dst] = 1;
         }
     }

@@ -30,6 +30,17 @@
 // prediction class and the sixteen 4x4 modes (an I8x8 MB's four modes, each on its four
 // 4x4 blocks), as the reference keeps m_mb_pred_mode and Intra4x4PredMode /
 // Intra8x8PredMode (H264PictureBase.cpp:773-1058).
+//
+// Spatial direct prediction (8.4.1.2.2, H264InterPrediction.cpp:1259-1432) of B_Skip,
+// B_Direct_16x16 and B_Direct_8x8 (records with H264G_MBF_DIRECT bits): reference
+// indices MinPositive over the MB's neighbours A, B and C (or D), the 16x16 median
+// prediction per list, and zero vectors where a list's index is 0 and the quadrant's
+// co-located block in RefPicList1[0] (set by h264g_mvfield_colocated; its corner 4x4
+// block under direct_8x8_inference_flag 1) has index 0 and a vector within +-1.  As in
+// the reference (H264InterPrediction.cpp:1226-1249) an intra co-located MB has index -1
+// but an I_PCM one, which is not "intra" there, reads its zeroed motion: index 0, vector
+// 0 (quirk Q20).  Temporal direct prediction is not derived: the reference's copy
+// divides by zero (quirk Q19).
 #include "h264g.h"
 #include "h264g_internal.h"
 
@@ -70,6 +81,8 @@ struct h264g_mvfield {
     std::vector<Blk> b;  // (4W) x (4H) blocks, raster
     std::vector<uint8_t> cls;    // per MB: CLS_*, CLS_NONE until derived in this picture
     std::vector<uint8_t> imode;  // per MB: 16 modes by luma4x4BlkIdx (I4x4 / I8x8 MBs)
+    std::vector<uint8_t> colzero;  // per MB: bit q = colZeroFlag of quadrant q (spatial direct)
+    bool have_col = false;
 
     Blk *at(int bx, int by) { return &b[(size_t)by * W * 4 + bx]; }
 
@@ -180,6 +193,7 @@ int32_t h264g_mvfield_create(int32_t width_mbs, int32_t height_mbs, h264g_mvfiel
         f->b.assign((size_t)width_mbs * height_mbs * 16, Blk{{-1, -1}, 0, {{0, 0}, {0, 0}}});
         f->cls.assign((size_t)width_mbs * height_mbs, CLS_NONE);
         f->imode.assign((size_t)width_mbs * height_mbs * 16, 2);
+        f->colzero.assign((size_t)width_mbs * height_mbs, 0);
     } catch (...) {
         delete f;
         return h264g_set_error(H264G_E_NOMEM, "mvfield");
@@ -194,6 +208,32 @@ int32_t h264g_mvfield_reset(h264g_mvfield *f) {
     if (!f) return h264g_set_error(H264G_E_ARG, "mvfield_reset: null field");
     std::fill(f->b.begin(), f->b.end(), Blk{{-1, -1}, 0, {{0, 0}, {0, 0}}});
     std::fill(f->cls.begin(), f->cls.end(), (uint8_t)CLS_NONE);
+    return 0;
+}
+
+int32_t h264g_mvfield_colocated(h264g_mvfield *f, const h264g_mb *col_mbs, int32_t col_short_term) {
+    if (!f) return h264g_set_error(H264G_E_ARG, "mvfield_colocated: null field");
+    f->have_col = col_mbs != nullptr;
+    if (!col_mbs) return 0;
+    static const int corner[4] = {0, 3, 12, 15};  // luma4x4BlkIdx 0, 5, 10, 15 in raster order
+    for (size_t a = 0; a < f->colzero.size(); a++) {
+        const h264g_mb &c = col_mbs[a];
+        uint8_t z = 0;
+        for (int q = 0; q < 4 && col_short_term; q++) {
+            bool cz;
+            if (c.mb_type == H264G_MB_I4x4 || c.mb_type == H264G_MB_I8x8 || c.mb_type == H264G_MB_I16x16) {
+                cz = false;  // refIdxCol -1
+            } else if (c.mb_type == H264G_MB_IPCM) {
+                cz = true;   // Q20: refIdxCol 0, mvCol 0
+            } else {         // predFlagL0Col ? L0 : L1 of the partition holding the corner block
+                const int lc = c.ref_idx[0][q] >= 0 ? 0 : 1;
+                const int16_t *v = c.mv[lc][corner[q]];
+                cz = c.ref_idx[lc][q] == 0 && v[0] >= -1 && v[0] <= 1 && v[1] >= -1 && v[1] <= 1;
+            }
+            if (cz) z |= (uint8_t)(1u << q);
+        }
+        f->colzero[a] = z;
+    }
     return 0;
 }
 
@@ -270,7 +310,26 @@ int32_t h264g_mvfield_derive(h264g_mvfield *f, uint32_t mb_addr, h264g_mb *mb, c
     default:
         return h264g_set_error(H264G_E_ARG, "mvfield_derive: MB %u type %d", mb_addr, mb->mb_type);
     }
-    if (!mvd) return h264g_set_error(H264G_E_ARG, "mvfield_derive: MB %u needs mvd", mb_addr);
+    const uint8_t direct = mb->mb_type == H264G_MB_B ? (uint8_t)(mb->flags & H264G_MBF_DIRECT_ALL) >> 4 : 0;
+    if (direct && (mb->part_shape != H264G_PART_8x8 || !f->have_col))
+        return h264g_set_error(H264G_E_ARG, "mvfield_derive: MB %u: direct needs part_shape 8x8 and a co-located picture",
+                               mb_addr);
+    int dref[2] = {-1, -1}, dmv[2][2] = {{0, 0}, {0, 0}};
+    bool dzero = false;
+    if (direct) {  // 8.4.1.2.2, once for the MB: its neighbours lie outside it
+        for (int l = 0; l < 2; l++) {
+            const Nb A = f->nb(mx, my, -1, 0, l), B = f->nb(mx, my, 0, -1, l);
+            Nb C = f->nb(mx, my, 4, -1, l);
+            if (!C.avail) C = f->nb(mx, my, -1, -1, l);
+            auto min_pos = [](int a, int b) { return a >= 0 && b >= 0 ? std::min(a, b) : std::max(a, b); };
+            dref[l] = min_pos(A.ref, min_pos(B.ref, C.ref));
+        }
+        dzero = dref[0] < 0 && dref[1] < 0;
+        if (dzero) dref[0] = dref[1] = 0;
+        for (int l = 0; l < 2 && !dzero; l++)
+            if (dref[l] >= 0) f->mvp(mx, my, 0, 0, 4, l, dref[l], 0, 0, dmv[l]);
+    }
+    if (!mvd && direct != 0xF) return h264g_set_error(H264G_E_ARG, "mvfield_derive: MB %u needs mvd", mb_addr);
     const int shape = mb->part_shape;
     if (shape > H264G_PART_8x8) return h264g_set_error(H264G_E_ARG, "mvfield_derive: MB %u shape %d", mb_addr, shape);
     const int np = shape == H264G_PART_16x16 ? 1 : (shape == H264G_PART_8x8 ? 4 : 2);
@@ -293,6 +352,16 @@ int32_t h264g_mvfield_derive(h264g_mvfield *f, uint32_t mb_addr, h264g_mb *mb, c
             sh = (ss == H264G_SUB_8x4 || ss == H264G_SUB_4x4) ? 1 : 2;
         }
         const int q = (py >> 1) * 2 + (px >> 1);  // 8x8 quadrant of the partition's first block
+        if (direct & (1u << p)) {  // a direct quadrant: indices and vectors from the MB's prediction
+            const bool cz = (f->colzero[mb_addr] >> p) & 1;
+            int mv[2][2] = {{0, 0}, {0, 0}};
+            for (int l = 0; l < 2; l++) {
+                mb->ref_idx[l][q] = (int8_t)dref[l];
+                if (dref[l] >= 0 && !dzero && !(dref[l] == 0 && cz)) { mv[l][0] = dmv[l][0]; mv[l][1] = dmv[l][1]; }
+            }
+            f->put(mx, my, px, py, 2, 2, dref, mv, mb);
+            continue;
+        }
         const int ref[2] = {mb->ref_idx[0][q], mb->ref_idx[1][q]};
         if (ref[0] < 0 && ref[1] < 0)
             return h264g_set_error(H264G_E_ARG, "mvfield_derive: MB %u partition %d uses no list", mb_addr, p);

@@ -141,11 +141,14 @@ class MotionField:
         check(lib().h264g_mvfield_reset(self._h))
 
     def derive_picture(self, mbs: np.ndarray, mvd: np.ndarray, imode: np.ndarray = None,
-                       constrained_intra_pred: bool = False) -> np.ndarray:
+                       constrained_intra_pred: bool = False, col: np.ndarray = None,
+                       col_short_term: bool = True) -> np.ndarray:
         """Records with their mv filled in: mbs (MB_DTYPE, raster order; mv ignored),
         mvd int16 [MB][list][16][2] (each partition's mvd at its top-left 4x4 block).
         imode, when given: uint8 [MB][16][2] intra-mode syntax (prev flag, rem) of the
-        I_NxN MBs, whose intra_pred_modes are then derived too (8.3.1.1 / 8.3.2.1)."""
+        I_NxN MBs, whose intra_pred_modes are then derived too (8.3.1.1 / 8.3.2.1).
+        col: the final records of RefPicList1[0] for the spatial-direct MBs of a B
+        picture (their ref_idx and mv are then derived as well)."""
         assert mbs.dtype == MB_DTYPE and mvd.dtype == np.int16 and mvd.shape == (len(mbs), 2, 16, 2)
         assert len(mbs) == self.width_mbs * self.height_mbs
         out = mbs.copy()
@@ -155,6 +158,12 @@ class MotionField:
             assert imode.dtype == np.uint8 and imode.shape == (len(mbs), 16, 2)
             imode = np.ascontiguousarray(imode)
         self.reset()
+        if col is not None:
+            assert col.dtype == MB_DTYPE and len(col) == len(mbs)
+            col = np.ascontiguousarray(col)
+            check(lib().h264g_mvfield_colocated(self._h, col.ctypes.data, 1 if col_short_term else 0))
+        else:
+            check(lib().h264g_mvfield_colocated(self._h, None, 0))
         base = out.ctypes.data
         for a in range(len(out)):
             rec = base + a * MB_DTYPE.itemsize

@@ -63,7 +63,8 @@ extern "C" {
 #define H264G_MB_IPCM     3   /* I_PCM                              */
 #define H264G_MB_P        4   /* P_L0_16x16 / P_L0_L0_16x8 / P_L0_L0_8x16 / P_8x8 */
 #define H264G_MB_P_SKIP   5   /* P_Skip (final MV supplied by the host)        */
-#define H264G_MB_B        6   /* B_{L0,L1,Bi}_*, B_8x8 with non-direct subtypes */
+#define H264G_MB_B        6   /* B_{L0,L1,Bi}_*, B_8x8; with the DIRECT flags also
+                                 B_Skip, B_Direct_16x16 and B_Direct_8x8 sub-MBs  */
 
 /* Partition shapes (MbPartWidth x MbPartHeight) and sub-partition shapes. */
 #define H264G_PART_16x16  0
@@ -78,6 +79,16 @@ extern "C" {
 /* h264g_mb.flags */
 #define H264G_MBF_T8x8    0x01u   /* transform_size_8x8_flag                 */
 #define H264G_MBF_FIELD   0x02u   /* mb_field_decoding_flag (MBAFF pairs)    */
+/* Direct prediction (8.4.1.2, direct_8x8_inference_flag = 1) in B pictures: an H264G_MB_B
+ * record with part_shape 8x8 whose quadrant q is a direct-predicted 8x8 block carries
+ * H264G_MBF_DIRECT(q), sub_part_shape[q] = H264G_SUB_8x8 and the derived ref_idx / mv of
+ * the quadrant (direct motion is uniform over each 8x8 quadrant under the inference).
+ * All four bits: B_Direct_16x16, or with H264G_MBF_SKIP B_Skip (no residual, cbp 0,
+ * QPY = QPY,PREV).  Some bits: B_8x8 with B_Direct_8x8 sub-macroblocks.  The device path
+ * reads only ref_idx and mv; the host derivation (h264g_mvfield_*) fills them. */
+#define H264G_MBF_SKIP    0x04u
+#define H264G_MBF_DIRECT(q) (0x10u << (q))
+#define H264G_MBF_DIRECT_ALL 0xF0u
 
 /* ----------------------------------------------------------------- records */
 /*
@@ -398,6 +409,9 @@ typedef struct h264g_gen_params {
                                      MBs are skipped); height_mbs must be even          */
     int32_t  constrained_intra_pred; /* constrained_intra_pred_flag: intra MBs predict
                                      only from intra neighbours (frames only)           */
+    int32_t  direct_permille;     /* B pictures: share of spatial-direct MBs (B_Skip,
+                                     B_Direct_16x16, B_8x8 with B_Direct_8x8 sub-MBs in
+                                     equal parts; frames, no implicit weights)          */
 } h264g_gen_params;
 
 /* ------------------------------------------------ host motion-vector derivation
@@ -406,9 +420,9 @@ typedef struct h264g_gen_params {
  * prediction over the neighbouring partitions (8.4.1.3, 6.4.11.7).  Replaces what the
  * reference derives inside reconstruction: P_Skip at H264InterPrediction.cpp:685-738,
  * Derivation_process_for_luma_motion_vector_prediction and its neighbouring
- * partitions at H264InterPrediction.cpp:1660-2001.  Host only, no device work; one
- * field per picture being parsed, not thread-safe.  B_Skip / B_Direct (8.4.1.2) are
- * not derived: the record cannot carry them. */
+ * partitions at H264InterPrediction.cpp:1660-2001; spatial direct prediction after
+ * h264g_mvfield_colocated.  Host only, no device work; one field per picture being
+ * parsed, not thread-safe. */
 typedef struct h264g_mvfield h264g_mvfield;
 int32_t h264g_mvfield_create(int32_t width_mbs, int32_t height_mbs, h264g_mvfield **out);
 void    h264g_mvfield_destroy(h264g_mvfield *field);
@@ -433,6 +447,15 @@ int32_t h264g_mvfield_derive(h264g_mvfield *field, uint32_t mb_addr, h264g_mb *m
  * constrained_intra_pred: the PPS flag.  Out: mb->intra_pred_modes. */
 int32_t h264g_mvfield_intra_modes(h264g_mvfield *field, uint32_t mb_addr, h264g_mb *mb, const uint8_t *syntax,
                                   int32_t constrained_intra_pred);
+/* The co-located picture of the B picture being parsed, for spatial direct prediction
+ * (8.4.1.2.1 / 8.4.1.2.2; replaces the reference's colocated lookup and spatial direct
+ * derivation, H264InterPrediction.cpp:922-1432): col_mbs are the final records of
+ * RefPicList1[0] (W x H, raster), col_short_term whether it is marked "used for
+ * short-term reference"; NULL clears it.  Frame pictures, direct_8x8_inference_flag 1,
+ * direct_spatial_mv_pred_flag 1 only (the reference's temporal direct divides by zero).
+ * Afterwards h264g_mvfield_derive fills ref_idx and mv of the quadrants of H264G_MB_B
+ * records that carry H264G_MBF_DIRECT bits (mvd may then be NULL when all four do). */
+int32_t h264g_mvfield_colocated(h264g_mvfield *field, const h264g_mb *col_mbs, int32_t col_short_term);
 
 int32_t h264g_gen_default_params(int32_t config, h264g_gen_params *out);
 int32_t h264g_gen_create(const h264g_gen_params *params, h264g_gen **out);

@@ -204,6 +204,7 @@ struct Harness {
         sh.m_pps.weighted_bipred_idc = d.weighted_bipred_idc;
         sh.num_ref_idx_l0_active_minus1 = d.num_ref_idx_active[0] > 0 ? d.num_ref_idx_active[0] - 1 : 0;
         sh.num_ref_idx_l1_active_minus1 = d.num_ref_idx_active[1] > 0 ? d.num_ref_idx_active[1] - 1 : 0;
+        sh.direct_spatial_mv_pred_flag = 1;  // spatial direct (B_Skip / B_Direct_* records)
         // pred_weight_table as left by parsing (H264SliceHeader.cpp:586-665)
         sh.luma_log2_weight_denom = d.luma_log2_weight_denom;
         sh.chroma_log2_weight_denom = d.chroma_log2_weight_denom;
@@ -270,6 +271,8 @@ struct Harness {
             raw = 5;
         } else if (r.mb_type == H264G_MB_P) {
             raw = r.part_shape;  // 0 16x16, 1 16x8, 2 8x16, 3 P_8x8
+        } else if ((r.flags & H264G_MBF_DIRECT_ALL) == H264G_MBF_DIRECT_ALL) {
+            raw = (r.flags & H264G_MBF_SKIP) ? 23 : 0;  // B_Skip (inferred, as H264MacroBlock.cpp:1176) / B_Direct_16x16
         } else {
             if (r.part_shape == H264G_PART_16x16) raw = quad_pred(r, 0);                 // 1..3
             else if (r.part_shape == H264G_PART_8x8) raw = 22;
@@ -278,10 +281,10 @@ struct Harness {
         }
         m.mb_type = raw;
         m.transform_size_8x8_flag = t8;
-        if (r.mb_type == H264G_MB_P_SKIP) {
+        if (r.mb_type == H264G_MB_P_SKIP || (r.flags & H264G_MBF_SKIP)) {
             // macroblock_layer_mb_skip (H264MacroBlock.cpp:1142-1204)
             m.m_slice_type_fixed = st;
-            m.m_mb_type_fixed = 5;
+            m.m_mb_type_fixed = raw;
             m.mb_qp_delta = 0;
         } else {
             CH264MacroBlock::fix_mb_type(st, raw, m.m_slice_type_fixed, m.m_mb_type_fixed);
@@ -313,8 +316,10 @@ struct Harness {
             m.CodedBlockPatternLuma = cbp_l;
             m.CodedBlockPatternChroma = cbp_c;
         }
-        // sub-macroblock types (sub_mb_pred, H264MacroBlock.cpp:1425-1473)
-        if (r.part_shape == H264G_PART_8x8 && (r.mb_type == H264G_MB_P || r.mb_type == H264G_MB_B)) {
+        // sub-macroblock types (sub_mb_pred, H264MacroBlock.cpp:1425-1473); none for B_Skip /
+        // B_Direct_16x16, B_Direct_8x8 (0) for the direct quadrants of B_8x8
+        const bool all_direct = (r.flags & H264G_MBF_DIRECT_ALL) == H264G_MBF_DIRECT_ALL;
+        if (r.part_shape == H264G_PART_8x8 && (r.mb_type == H264G_MB_P || r.mb_type == H264G_MB_B) && !all_direct) {
             for (int q = 0; q < 4; q++) {
                 int sub = r.sub_part_shape[q];
                 if (r.mb_type == H264G_MB_P) {
@@ -326,7 +331,7 @@ struct Harness {
                     m.SubMbPartWidth[q] = t.SubMbPartWidth;
                     m.SubMbPartHeight[q] = t.SubMbPartHeight;
                 } else {
-                    int s = b_sub_type(sub, quad_pred(r, q));
+                    int s = (r.flags & H264G_MBF_DIRECT(q)) ? 0 : b_sub_type(sub, quad_pred(r, q));
                     m.sub_mb_type[q] = s;
                     const SUB_MB_TYPE_B_MBS_T &t = sub_mb_type_B_mbs_define[s];
                     m.m_name_of_sub_mb_type[q] = t.name_of_sub_mb_type;
@@ -427,7 +432,7 @@ struct Harness {
         int ret = 0;
         uint8_t *Y = pic.m_pic_buff_luma, *Cb = pic.m_pic_buff_cb, *Cr = pic.m_pic_buff_cr;
         int wl = pic.PicWidthInSamplesL, wc = pic.PicWidthInSamplesC;
-        if (r.mb_type == H264G_MB_P_SKIP) return pic.Inter_prediction_process();
+        if (r.mb_type == H264G_MB_P_SKIP || (r.flags & H264G_MBF_SKIP)) return pic.Inter_prediction_process();
         switch (m.m_mb_pred_mode) {
         case Intra_4x4:
             ret |= pic.transform_decoding_process_for_4x4_luma_residual_blocks(0, 0, 8, wl, Y);
@@ -485,6 +490,7 @@ struct Harness {
             memset(m.mvd_l1, 0, sizeof(m.mvd_l1));
             int np = r.part_shape == H264G_PART_16x16 ? 1 : (r.part_shape == H264G_PART_8x8 ? 4 : 2);
             for (int p = 0; p < np; p++) {
+                if (r.flags & H264G_MBF_DIRECT(p)) continue;  // direct: no mvd, the reference derives it
                 int ns = r.part_shape == H264G_PART_8x8 ? m.NumSubMbPart[p] : 1;
                 for (int s = 0; s < ns; s++) {
                     clear_motion(m);
@@ -507,7 +513,29 @@ struct Harness {
         if (r.mb_type != H264G_MB_P && r.mb_type != H264G_MB_B && r.mb_type != H264G_MB_P_SKIP) return;
         CH264MacroBlock &m = pic.m_mbs[addr];
         int np = r.mb_type == H264G_MB_P_SKIP ? 1 : (r.part_shape == H264G_PART_16x16 ? 1 : (r.part_shape == H264G_PART_8x8 ? 4 : 2));
+        for (int p = 0; p < np && r.mb_type == H264G_MB_B; p++) {
+            // a direct quadrant: the reference derived its indices and its four 4x4 vectors
+            if (!(r.flags & H264G_MBF_DIRECT(p))) continue;
+            for (int l = 0; l < 2; l++) {
+                const int pf = l == 0 ? m.m_PredFlagL0[p] : m.m_PredFlagL1[p];
+                const int ref = l == 0 ? m.m_RefIdxL0[p] : m.m_RefIdxL1[p];
+                const bool used = r.ref_idx[l][p] >= 0;
+                bool bad = pf != (used ? 1 : 0) || (used && ref != r.ref_idx[l][p]);
+                for (int s = 0; s < 4 && used; s++) {
+                    const int blk = ((p >> 1) * 2 + (s >> 1)) * 4 + (p & 1) * 2 + (s & 1);
+                    int32_t (&mv)[4][4][2] = l == 0 ? m.m_MvL0 : m.m_MvL1;
+                    bad |= mv[p][s][0] != r.mv[l][blk][0] || mv[p][s][1] != r.mv[l][blk][1];
+                }
+                if (bad) {
+                    if (mismatches < 20)
+                        fprintf(stderr, "direct mismatch pic %d mb %d quadrant %d list %d: ref pf %d idx %d, gen idx %d\n",
+                                pic_idx, addr, p, l, pf, ref, r.ref_idx[l][p]);
+                    mismatches++;
+                }
+            }
+        }
         for (int p = 0; p < np; p++) {
+            if (r.mb_type == H264G_MB_B && (r.flags & H264G_MBF_DIRECT(p))) continue;
             int ns = (r.mb_type != H264G_MB_P_SKIP && r.part_shape == H264G_PART_8x8) ? m.NumSubMbPart[p] : 1;
             for (int s = 0; s < ns; s++) {
                 int blk = r.mb_type == H264G_MB_P_SKIP ? 0 : part_block(r, p, s);
@@ -534,6 +562,7 @@ struct Harness {
             if (r.mb_type != H264G_MB_P && r.mb_type != H264G_MB_B) continue;
             int np = r.part_shape == H264G_PART_16x16 ? 1 : (r.part_shape == H264G_PART_8x8 ? 4 : 2);
             for (int p = 0; p < np; p++) {
+                if (r.flags & H264G_MBF_DIRECT(p)) continue;  // direct: no mvd
                 int ns = 1;
                 if (r.part_shape == H264G_PART_8x8)
                     ns = r.sub_part_shape[p] == H264G_SUB_8x8 ? 1 : (r.sub_part_shape[p] == H264G_SUB_4x4 ? 4 : 2);
@@ -717,7 +746,11 @@ struct Harness {
             for (uint32_t a = 0; a < nmb; a++) check_mb(pic, (int)a, mbs[a], i);
             write_planes(pic, "pic", i);
             if (mvd) write_mvd(saved.data(), mbs, nmb, i);
-            // reference marking state used by later pictures
+            // end of the picture (H264PictureBase.cpp:694-700) and the reference marking
+            // state used by later pictures (a B picture's co-located picture: 8.4.1.2.1,
+            // H264InterPrediction.cpp:1004-1011)
+            pic.m_is_decode_finished = 1;
+            cur->m_is_decode_finished = 1;
             cur->m_picture_coded_type_marked_as_refrence = H264_PICTURE_CODED_TYPE_FRAME;
             pic.reference_marked_type = H264_PICTURE_MARKED_AS_used_for_short_term_reference;
             cur->reference_marked_type = H264_PICTURE_MARKED_AS_used_for_short_term_reference;
@@ -754,6 +787,7 @@ int main(int argc, char **argv) {
         else if (a == "--t8x8") p.t8x8 = atoi(next());
         else if (a == "--mbaff") p.mbaff = atoi(next());
         else if (a == "--cip") p.constrained_intra_pred = atoi(next());
+        else if (a == "--direct") p.direct_permille = atoi(next());
         else if (a == "--out") hs.out = next();
         else if (a == "--predeblock") hs.predeblock = true;
         else if (a == "--mvd") hs.mvd = true;

@@ -1,11 +1,13 @@
-// paff_probe.cpp -- TEST INFRASTRUCTURE ONLY: a CPU probe of how the reference decoder
-// stores field pictures (PAFF).  Nothing on the product path links or runs it.
+// stream_probe.cpp -- TEST INFRASTRUCTURE ONLY: a CPU probe of how the reference decoder
+// handles field pictures (PAFF) and temporal direct prediction.  Nothing on the product
+// path links or runs it.
 //
 // It writes a small Annex-B stream of its own (CAVLC, Main profile, frame_mbs_only_flag 0,
 // no MBAFF): eight frames in the decode order of the 8-frame, 2-B-frame test sequence
 // (I0 P3 B1 B2 P6 B4 B5 P7), then a top/bottom field pair (two P fields or two I
-// fields), then one more P frame so that the pair is closed by the decoder's normal
-// end-of-picture path.  I frames
+// fields, or none), then one more P frame so that the pair is closed by the decoder's
+// normal end-of-picture path.  The B pictures' skipped MBs are B_Skip: spatial direct,
+// or temporal direct with TEMPORAL.  I frames
 // are all I_PCM with seeded samples, P and B pictures are one skip run.  The reference's
 // own CH264VideoDecoder (all of its translation units, built unchanged) decodes the
 // stream; the output callback reports, per decoded picture, the DPB entry it was decoded
@@ -15,11 +17,13 @@
 // the buffer behind it.  Built with AddressSanitizer (host code only), the run also shows
 // whether a field decoded into a never-used entry writes past that entry's buffer.
 //
-// usage: paff_probe W H REORDER FIELDS [stream.h264]
-//   W, H     frame size in macroblocks (H even)
-//   REORDER  max_num_reorder_frames written into the VUI, or -1 for no VUI (the
-//            reference then infers it from level_idc, H264SPS.cpp:365-395)
-//   FIELDS   P: the field pair is two P fields; I: two I fields
+// usage: stream_probe W H REORDER FIELDS [stream.h264] [TEMPORAL]
+//   W, H      frame size in macroblocks (H even)
+//   REORDER   max_num_reorder_frames written into the VUI, or -1 for no VUI (the
+//             reference then infers it from level_idc, H264SPS.cpp:365-395)
+//   FIELDS    P: the field pair is two P fields; I: two I fields; N: no field pair
+//   TEMPORAL  1: direct_spatial_mv_pred_flag 0 in the B pictures (temporal direct);
+//             2: the same with two active references per list in the B pictures
 #include <malloc.h>
 #include <stdint.h>
 #include <stdio.h>
@@ -92,7 +96,7 @@ uint8_t next_sample() {
     return (uint8_t)(16 + ((g_rng >> 16) % 220));  // 16..235: no zero bytes
 }
 
-std::vector<uint8_t> make_stream(int W, int H, int reorder, char fields) {
+std::vector<uint8_t> make_stream(int W, int H, int reorder, char fields, int temporal) {
     std::vector<uint8_t> out;
     {   // seq_parameter_set_rbsp (7.3.2.1.1)
         BitWriter b;
@@ -155,6 +159,7 @@ std::vector<uint8_t> make_stream(int W, int H, int reorder, char fields) {
         {fields, 1, 0, 4, 16, 1}, {fields, 1, 0, 4, 17, 2}, {'P', 1, 0, 5, 18, 0},
     };
     for (const Pic &p : pics) {
+        if (p.kind == 'N') continue;
         BitWriter b;
         const int mbs = W * H / (p.field ? 2 : 1);
         b.ue(0);  // first_mb_in_slice
@@ -165,8 +170,14 @@ std::vector<uint8_t> make_stream(int W, int H, int reorder, char fields) {
         if (p.field) b.u(1, p.field == 2);   // bottom_field_flag
         if (p.idr) b.ue(0);                  // idr_pic_id
         b.u(6, (uint32_t)p.poc_lsb);
-        if (p.kind == 'B') b.u(1, 1);        // direct_spatial_mv_pred_flag
-        if (p.kind != 'I') b.u(1, 0);        // num_ref_idx_active_override_flag
+        if (p.kind == 'B') b.u(1, temporal ? 0 : 1);  // direct_spatial_mv_pred_flag
+        if (p.kind == 'B' && temporal == 2) {  // num_ref_idx_active_override_flag: 2 + 2
+            b.u(1, 1);
+            b.ue(1);
+            b.ue(1);
+        } else if (p.kind != 'I') {
+            b.u(1, 0);                       // num_ref_idx_active_override_flag
+        }
         if (p.kind != 'I') b.u(1, 0);        // ref_pic_list_modification_flag_l0
         if (p.kind == 'B') b.u(1, 0);        // ref_pic_list_modification_flag_l1
         if (p.ref) {                          // dec_ref_pic_marking
@@ -237,7 +248,7 @@ const char *type_name(int t) {
 
 int main(int argc, char **argv) {
     if (argc < 5) {
-        fprintf(stderr, "usage: %s W H REORDER FIELDS [stream.h264]\n", argv[0]);
+        fprintf(stderr, "usage: %s W H REORDER FIELDS [stream.h264] [TEMPORAL]\n", argv[0]);
         return 2;
     }
     setvbuf(stdout, nullptr, _IOLBF, 0);  // keep the decoder's log in order with sanitizer reports
@@ -245,9 +256,10 @@ int main(int argc, char **argv) {
     g_H = atoi(argv[2]);
     const int reorder = atoi(argv[3]);
     const char fields = argv[4][0];
-    const char *path = argc > 5 ? argv[5] : "/tmp/paff_probe.h264";
-    if (g_W <= 0 || g_H <= 0 || g_H % 2 || (fields != 'P' && fields != 'I')) return 2;
-    std::vector<uint8_t> s = make_stream(g_W, g_H, reorder, fields);
+    const char *path = argc > 5 ? argv[5] : "/tmp/stream_probe.h264";
+    const int temporal = argc > 6 ? atoi(argv[6]) : 0;
+    if (g_W <= 0 || g_H <= 0 || g_H % 2 || (fields != 'P' && fields != 'I' && fields != 'N')) return 2;
+    std::vector<uint8_t> s = make_stream(g_W, g_H, reorder, fields, temporal);
     FILE *fp = fopen(path, "wb");
     if (!fp || fwrite(s.data(), 1, s.size(), fp) != s.size()) return 3;
     fclose(fp);
@@ -259,8 +271,8 @@ int main(int argc, char **argv) {
     std::sort(g_seen.begin(), g_seen.end(), [](const Seen &a, const Seen &b) { return a.num < b.num; });
     const size_t frame_bytes = (size_t)g_W * g_H * 384;
     std::map<int, int> first_use;
-    printf("PAFF_PROBE W=%d H=%d reorder=%d fields=%c decoder_rc=%d frame_buffer_bytes=%zu\n", g_W, g_H, reorder,
-           fields, rc, frame_bytes);
+    printf("STREAM_PROBE W=%d H=%d reorder=%d fields=%c temporal=%d decoder_rc=%d frame_buffer_bytes=%zu\n", g_W, g_H,
+           reorder, fields, temporal, rc, frame_bytes);
     printf("%-4s %-6s %-5s %-10s %-10s %-12s %-5s %-12s %-12s\n", "num", "type", "entry", "reused", "frame_rows",
            "buffer", "poc", "nz_even_rows", "nz_odd_rows");
     for (const Seen &e : g_seen) {

@@ -67,11 +67,19 @@ CASES = {
     # inter neighbour makes the intra-mode prediction DC
     "high_small_cip": (2, {"width_mbs": 40, "height_mbs": 24, "constrained_intra_pred": 1, "seed": 12}),
     "high_odd_cip": (2, {"width_mbs": 23, "height_mbs": 17, "constrained_intra_pred": 1, "seed": 13}),
+    # spatial direct prediction in the B pictures: B_Skip, B_Direct_16x16 and B_8x8 with
+    # B_Direct_8x8 sub-macroblocks beside every other kind of MB
+    "high_direct": (2, {"width_mbs": 40, "height_mbs": 24, "direct_permille": 400, "seed": 21}),
+    "high_direct_odd": (2, {"width_mbs": 23, "height_mbs": 17, "direct_permille": 600, "seed": 22}),
+    # I_PCM co-located MBs: the reference reads their zeroed motion as index 0, vector 0
+    "high_direct_pcm": (2, {"width_mbs": 30, "height_mbs": 20, "direct_permille": 500, "ipcm_permille": 150,
+                            "seed": 23}),
 }
 
 _CLI = {"n_frames": "--frames", "width_mbs": "--width", "height_mbs": "--height", "seed": "--seed",
         "ipcm_permille": "--ipcm", "disable_deblocking_filter_idc": "--deblock", "max_refs": "--refs",
-        "b_frames": "--bframes", "t8x8": "--t8x8", "mbaff": "--mbaff", "constrained_intra_pred": "--cip"}
+        "b_frames": "--bframes", "t8x8": "--t8x8", "mbaff": "--mbaff", "constrained_intra_pred": "--cip",
+        "direct_permille": "--direct"}
 
 
 def harness_args(config: int, ov: dict) -> list:

@@ -42,6 +42,13 @@ CASES = {
     "flat": (2, {"width_mbs": 14, "height_mbs": 1, "max_refs": 2}),
     "high_small_cip": (2, {"width_mbs": 40, "height_mbs": 24, "constrained_intra_pred": 1, "seed": 12}),
     "high_odd_cip": (2, {"width_mbs": 23, "height_mbs": 17, "constrained_intra_pred": 1, "seed": 13}),
+    # spatial direct prediction in the B pictures: B_Skip, B_Direct_16x16 and B_8x8 with
+    # B_Direct_8x8 sub-macroblocks beside every other kind of MB
+    "high_direct": (2, {"width_mbs": 40, "height_mbs": 24, "direct_permille": 400, "seed": 21}),
+    "high_direct_odd": (2, {"width_mbs": 23, "height_mbs": 17, "direct_permille": 600, "seed": 22}),
+    # I_PCM co-located MBs: the reference reads their zeroed motion as index 0, vector 0
+    "high_direct_pcm": (2, {"width_mbs": 30, "height_mbs": 20, "direct_permille": 500, "ipcm_permille": 150,
+                            "seed": 23}),
 }
 
 

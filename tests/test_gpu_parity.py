@@ -19,7 +19,8 @@ pytestmark = pytest.mark.gpu
 
 CASES = ["cif_ip", "cif_ip_seed2", "cif_ip_seed3", "cif_pcm_cqp", "high_small", "high_small_seed7",
          "high_implicit", "high_default_nodbk", "narrow", "flat", "high_1080p", "high_720p", "high_2160p",
-         "mbaff_small", "mbaff_t8_pcm", "mbaff_tall", "mbaff_1080p", "high_small_cip", "high_odd_cip"]
+         "mbaff_small", "mbaff_t8_pcm", "mbaff_tall", "mbaff_1080p", "high_small_cip", "high_odd_cip",
+         "high_direct", "high_direct_odd", "high_direct_pcm"]
 
 
 def _diagnose(engine, sid, name, gen, i, desc, mbs, levels, got):
@@ -73,7 +74,8 @@ def test_batched_path_matches_reference(any_engine, golden):
     picture index give the same bytes as the reference."""
     engine = any_engine
     names = ["cif_ip_seed2", "high_small", "high_implicit", "high_default_nodbk", "narrow", "flat", "cif_pcm_cqp",
-             "high_720p", "high_2160p", "mbaff_small", "mbaff_t8_pcm", "high_odd_cip"]
+             "high_720p", "high_2160p", "mbaff_small", "mbaff_t8_pcm", "high_odd_cip",
+             "high_direct_odd"]
     gens = {n: make_generator(golden[n]) for n in names}
     sids = {n: engine.open_stream_for(g) for n, g in gens.items()}
     try:

@@ -4,8 +4,11 @@ The fixtures under tests/golden/mvd_*.npz hold, for generated streams, the mvd o
 (sub-)partition that the reference decoder's prediction (H264InterPrediction.cpp:
 1660-2001, driven by oracle/ref_harness.cpp) needs to reproduce the generator's final
 motion vectors; the harness decoded each stream again from those mvds and found every
-vector equal.  Deriving the vectors from the same mvds must give the generator's
-vectors back, partition by partition, P_Skip MBs included.  No GPU is used.
+vector equal, and in the *_direct streams every reference index and vector that the
+reference derived itself for B_Skip, B_Direct_16x16 and B_Direct_8x8 (spatial direct,
+H264InterPrediction.cpp:892-1432) equal to the generator's.  Deriving the vectors from
+the same mvds must give the generator's vectors back, partition by partition, P_Skip
+and direct MBs included.  No GPU is used.
 """
 import glob
 import json
@@ -33,9 +36,19 @@ def test_derived_mvs_equal_generator(path):
     mvd = z["mvd"]
     f = MotionField(g.width_mbs, g.height_mbs)
     checked = {4: 0, 5: 0, 6: 0}
+    slot_pic = {}  # DPB slot -> the records last decoded into it
+    direct = 0
     for i in range(g.num_pictures):
-        _, mbs, _ = g.picture(i)
-        got = f.derive_picture(mbs, mvd[i])
+        desc, mbs, _ = g.picture(i)
+        col = slot_pic[desc.ref_slot[1][0]] if desc.slice_type == _lib.SLICE_B else None
+        inp = mbs.copy()
+        dq = (inp["flags"][:, None] >> (4 + np.arange(4))) & 1  # direct quadrants: indices are derived
+        inp["ref_idx"][:, 0][dq == 1] = 99
+        inp["ref_idx"][:, 1][dq == 1] = 99
+        got = f.derive_picture(inp, mvd[i], col=col)
+        slot_pic[desc.dst_slot] = got
+        assert (got["ref_idx"] == mbs["ref_idx"]).all(), i
+        direct += int(dq.sum())
         for a in np.nonzero(np.isin(mbs["mb_type"], INTER))[0]:
             for blk in range(16):
                 q = _quadrant(blk)
@@ -46,6 +59,8 @@ def test_derived_mvs_equal_generator(path):
                         (i, int(a), int(mbs["mb_type"][a]), l, blk)
             checked[int(mbs["mb_type"][a])] += 1
     assert checked[4] > 0 and checked[5] + checked[6] > 0, checked
+    if json.loads(str(z["overrides"])).get("direct_permille"):
+        assert direct > 0
 
 
 @pytest.mark.parametrize("path", CASES, ids=[os.path.basename(p)[4:-4] for p in CASES])
@@ -59,9 +74,12 @@ def test_derived_intra_modes_equal_generator(path):
     assert g.digest == int(z["digest"]), "generator changed: regenerate with tests/golden/make_mvd.py"
     f = MotionField(g.width_mbs, g.height_mbs)
     n4 = n8 = rem = 0
+    slot_pic = {}
     for i in range(g.num_pictures):
         desc, mbs, _ = g.picture(i)
-        got = f.derive_picture(mbs, z["mvd"][i], z["imode"][i], bool(desc.constrained_intra_pred_flag))
+        col = slot_pic[desc.ref_slot[1][0]] if desc.slice_type == _lib.SLICE_B else None
+        got = f.derive_picture(mbs, z["mvd"][i], z["imode"][i], bool(desc.constrained_intra_pred_flag), col=col)
+        slot_pic[desc.dst_slot] = got
         for a in np.nonzero(mbs["mb_type"] == _lib.MB_I4x4)[0]:
             assert (got["intra_pred_modes"][a] == mbs["intra_pred_modes"][a]).all(), (i, int(a))
             n4 += 1
@@ -104,7 +122,7 @@ def test_rejects_what_it_does_not_derive():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("name", ["cif_ip", "high_small"])
+@pytest.mark.parametrize("name", ["cif_ip", "high_small", "high_direct_pcm"])
 def test_pictures_from_derived_mvs_match_reference(golden, name):
     """A parser's path: records whose motion vectors come from the mvds through the host
     derivation (not from the generator), reconstructed on the GPU, give the reference's
@@ -115,11 +133,14 @@ def test_pictures_from_derived_mvs_match_reference(golden, name):
     g = Generator(int(z["config"]), **json.loads(str(z["overrides"])))
     assert g.digest == int(golden[name]["digest"], 16)  # the golden pictures are of this syntax
     f = MotionField(g.width_mbs, g.height_mbs)
+    slot_pic = {}
     with Engine(device=0) as eng:
         sid = eng.open_stream(g.width_mbs, g.height_mbs, g.dpb_slots)
         for i in range(g.num_pictures):
             desc, mbs, levels = g.picture(i)
-            derived = f.derive_picture(mbs, z["mvd"][i])
+            col = slot_pic[desc.ref_slot[1][0]] if desc.slice_type == _lib.SLICE_B else None
+            derived = f.derive_picture(mbs, z["mvd"][i], z["imode"][i], col=col)
+            slot_pic[desc.dst_slot] = derived
             p = eng.begin(sid, desc, derived, levels)
             eng.submit(p)
             eng.sync()

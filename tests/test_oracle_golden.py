@@ -11,7 +11,8 @@ from helpers import make_generator, oracle_decode, plane_hashes
 
 CPU_CASES = ["cif_ip", "cif_ip_seed2", "cif_ip_seed3", "cif_pcm_cqp", "high_small", "high_small_seed7",
              "high_implicit", "high_default_nodbk", "narrow", "flat", "high_1080p", "high_720p", "high_2160p",
-             "mbaff_small", "mbaff_t8_pcm", "mbaff_tall", "mbaff_1080p", "high_small_cip", "high_odd_cip"]
+             "mbaff_small", "mbaff_t8_pcm", "mbaff_tall", "mbaff_1080p", "high_small_cip", "high_odd_cip",
+             "high_direct", "high_direct_odd", "high_direct_pcm"]
 BIG = ("high_1080p", "high_720p", "high_2160p", "mbaff_1080p")
 
 
