@@ -1323,56 +1323,77 @@ __device__ __forceinline__ int absdiff(int a, int b) { return (int)__builtin_amd
 // True when the condition holds on any active lane (the compare's lane mask, no VALU).
 __device__ __forceinline__ bool any_lane(bool v) { return __builtin_amdgcn_ballot_w64(v) != 0; }
 
+// Packed 16-bit pairs: the p-side value of a line in the low half, the q-side value in
+// the high half.  The p and q sides of 8.7.2.3 / 8.7.2.4 are mirror images, so one packed
+// integer operation (v_pk_*_i16) computes both.
+typedef short s16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ s16x2 as_pair(uint32_t v) { return __builtin_bit_cast(s16x2, v); }
+__device__ __forceinline__ uint32_t as_bits(s16x2 v) { return __builtin_bit_cast(uint32_t, v); }
+__device__ __forceinline__ s16x2 pair(int lo, int hi) { return (s16x2){(short)lo, (short)hi}; }
+__device__ __forceinline__ s16x2 pair_clip(s16x2 v, s16x2 lo, s16x2 hi) {
+    return __builtin_elementwise_min(__builtin_elementwise_max(v, lo), hi);
+}
+__device__ __forceinline__ s16x2 pair_swap(s16x2 v) { return __builtin_shufflevector(v, v, 1, 0); }
+// n where the half's condition holds (hp: p half, hq: q half), else o.
+__device__ __forceinline__ s16x2 pair_sel(bool hp, bool hq, s16x2 n, s16x2 o) {
+    const uint32_t m = (hp ? 0x0000ffffu : 0u) | (hq ? 0xffff0000u : 0u);
+    return as_pair((as_bits(n) & m) | (as_bits(o) & ~m));
+}
+
 // Edge J of a line held as 20 bytes in w[0..4] (little-endian; the edge lies between
 // words J and J + 1); chroma lines hold 12 bytes and use J = 0, 1 (luma edges 0 and 8
 // for the bS).  8.7.2.2 - 8.7.2.4 (Deblock.cpp:1314-1522): alpha = alpha'(indexA), beta =
 // beta'(indexB), tc0 = tC0'(bS, indexA) (used for bS 1..3), with indexA / indexB =
-// Clip3(0, 51, qPav + filterOffsetA / B).  Every lane takes the same path: the wave
-// skips a filter that no lane applies (wave-uniform branches); inside, every candidate
-// value is computed on all lanes and chosen per lane with selects, so no per-line
-// condition turns into a divergent branch and its exec-mask bookkeeping.
+// Clip3(0, 51, qPav + filterOffsetA / B).  Every lane takes the same path: the wave skips
+// a filter that no lane applies (wave-uniform branches); inside, every candidate value is
+// computed on all lanes and chosen per lane with selects, so no per-line condition turns
+// into a divergent branch and its exec-mask bookkeeping.  The samples travel as (p, q)
+// pairs: P0 = (p0, q0) .. P3 = (p3, q3).
 template <int J>
 __device__ __forceinline__ void dbk_line_edge(uint32_t (&w)[5], bool act, bool luma, int bS, int alpha, int beta, int tc0) {
     const bool on = act && bS != 0;
     if (!any_lane(on)) return;
-    const uint32_t a = w[J], b = w[J + 1];
-    const int p0 = (int)(a >> 24), p1 = (int)((a >> 16) & 255u), p2 = (int)((a >> 8) & 255u), p3 = (int)(a & 255u);
-    const int q0 = (int)(b & 255u), q1 = (int)((b >> 8) & 255u), q2 = (int)((b >> 16) & 255u), q3 = (int)(b >> 24);
+    const uint32_t a = w[J], b = w[J + 1];  // a: p3 p2 p1 p0 (bytes 0..3), b: q0 q1 q2 q3
+    const s16x2 P0 = as_pair(__builtin_amdgcn_perm(b, a, 0x0c040c03u)), P1 = as_pair(__builtin_amdgcn_perm(b, a, 0x0c050c02u));
+    const s16x2 P2 = as_pair(__builtin_amdgcn_perm(b, a, 0x0c060c01u)), P3 = as_pair(__builtin_amdgcn_perm(b, a, 0x0c070c00u));
+    const int p0 = P0.x, q0 = P0.y, p1 = P1.x, q1 = P1.y, p2 = P2.x, q2 = P2.y;
     const int d0 = absdiff(p0, q0);
     const bool filt = on & (d0 < alpha) & (absdiff(p1, p0) < beta) & (absdiff(q1, q0) < beta);
     if (!any_lane(filt)) return;
     // ap / aq < beta (8-460, 8-461), only ever used for luma lines
     const bool apl = luma & (absdiff(p2, p0) < beta), aql = luma & (absdiff(q2, q0) < beta);
-    int np0 = p0, np1 = p1, np2 = p2, nq0 = q0, nq1 = q1, nq2 = q2;
+    s16x2 N0 = P0, N1 = P1, N2 = P2;
     const bool fn = filt & (bS < 4), fs = filt & (bS >= 4);
+    const s16x2 X0 = pair_swap(P0), X1 = pair_swap(P1);  // (q0, p0), (q1, p1)
     if (any_lane(fn)) {  // bS < 4 (8.7.2.3)
         const int tc = tc0 + (int)apl + (int)aql + (int)!luma;  // luma tC0 + [ap] + [aq], chroma tC0 + 1
         const int delta = clip3(-tc, tc, (((q0 - p0) << 2) + (p1 - q1) + 4) >> 3);
-        const int avg = (p0 + q0 + 1) >> 1;
-        const int fp0 = clip1(p0 + delta), fq0 = clip1(q0 - delta);
-        const int fp1 = (p1 + clip3(-tc0, tc0, (p2 + avg - (p1 << 1)) >> 1)) & 255;
-        const int fq1 = (q1 + clip3(-tc0, tc0, (q2 + avg - (q1 << 1)) >> 1)) & 255;
-        np0 = fn ? fp0 : np0;
-        nq0 = fn ? fq0 : nq0;
-        np1 = fn & apl ? fp1 : np1;
-        nq1 = fn & aql ? fq1 : nq1;
+        const short avg = (short)((p0 + q0 + 1) >> 1);
+        // p'0 = Clip1(p0 + delta), q'0 = Clip1(q0 - delta)
+        const s16x2 f0 = pair_clip(P0 + pair(delta, -delta), pair(0, 0), pair(255, 255));
+        // p'1 = p1 + Clip3(-tC0, tC0, (p2 + avg - 2 p1) >> 1), q'1 alike, kept to 8 bits
+        const s16x2 t = (P2 + (s16x2){avg, avg} - (P1 << 1)) >> 1;
+        const s16x2 f1 = (P1 + pair_clip(t, pair(-tc0, -tc0), pair(tc0, tc0))) & (s16x2){255, 255};
+        N0 = pair_sel(fn, fn, f0, N0);
+        N1 = pair_sel(fn & apl, fn & aql, f1, N1);
     }
     if (any_lane(fs)) {  // bS == 4 (8.7.2.4)
         const bool strong = d0 < ((alpha >> 2) + 2);
         const bool sp = fs & apl & strong, sq = fs & aql & strong;
-        const int sp0 = (p2 + 2 * p1 + 2 * p0 + 2 * q0 + q1 + 4) >> 3, wp0 = (2 * p1 + p0 + q1 + 2) >> 2;
-        const int sq0 = (p1 + 2 * p0 + 2 * q0 + 2 * q1 + q2 + 4) >> 3, wq0 = (2 * q1 + q0 + p1 + 2) >> 2;
-        const int sp1 = (p2 + p1 + p0 + q0 + 2) >> 2, sq1 = (p0 + q0 + q1 + q2 + 2) >> 2;
-        const int sp2 = (2 * p3 + 3 * p2 + p1 + p0 + q0 + 4) >> 3, sq2 = (2 * q3 + 3 * q2 + q1 + q0 + p0 + 4) >> 3;
-        np0 = sp ? sp0 : (fs ? wp0 : np0);
-        nq0 = sq ? sq0 : (fs ? wq0 : nq0);
-        np1 = sp ? sp1 : np1;
-        nq1 = sq ? sq1 : nq1;
-        np2 = sp ? sp2 : np2;
-        nq2 = sq ? sq2 : nq2;
+        const s16x2 m = P1 + P0 + X0;                                  // p1 + p0 + q0
+        const s16x2 S0 = ((m << 1) + P2 + X1 + (s16x2){4, 4}) >> 3;    // p2 + 2p1 + 2p0 + 2q0 + q1 + 4
+        const s16x2 W0 = ((P1 << 1) + P0 + X1 + (s16x2){2, 2}) >> 2;   // 2p1 + p0 + q1 + 2
+        const s16x2 S1 = (m + P2 + (s16x2){2, 2}) >> 2;                // p2 + p1 + p0 + q0 + 2
+        const s16x2 S2 = ((P3 << 1) + P2 * (s16x2){3, 3} + m + (s16x2){4, 4}) >> 3;  // 2p3 + 3p2 + p1 + p0 + q0 + 4
+        N0 = pair_sel(sp, sq, S0, pair_sel(fs, fs, W0, N0));
+        N1 = pair_sel(sp, sq, S1, N1);
+        N2 = pair_sel(sp, sq, S2, N2);
     }
-    w[J] = __builtin_amdgcn_perm(pack4(0, np2, np1, np0), a, 0x07060500u);
-    w[J + 1] = __builtin_amdgcn_perm(b, pack4(nq0, nq1, nq2, 0), 0x07020100u);
+    // back to bytes: t01 = np0 np1 nq0 nq1, u = np0 np1 np2 nq2
+    const uint32_t t01 = __builtin_amdgcn_perm(as_bits(N1), as_bits(N0), 0x06020400u);
+    const uint32_t u = __builtin_amdgcn_perm(as_bits(N2), t01, 0x06040100u);
+    w[J] = __builtin_amdgcn_perm(u, a, 0x04050600u);                                                  // p3 np2 np1 np0
+    w[J + 1] = __builtin_amdgcn_perm(b, __builtin_amdgcn_perm(u, t01, 0x0c070302u), 0x07020100u);   // nq0 nq1 nq2 q3
 }
 
 // The edges of one direction (DIR 0: vertical edges, 1: horizontal) of the MB in the
