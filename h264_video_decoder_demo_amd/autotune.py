@@ -6,7 +6,7 @@ workgroup and so the waves per CU, and trades dependency-chain steps inside a ba
 against band crossings through device memory; the best heights depend on the device
 and on the mix of picture sizes that share a launch.  `autotune` measures candidate
 heights on the caller's own streams -- their pictures (all, or the first n_frames),
-staged once and batched per frame index as the engine batches them -- and caches the
+staged once and batched by runs of independent frames (engine.independent_runs) -- and caches the
 winner per device (architecture, name, CU count, clock), library build, engine
 settings and stream mix (a JSON file under the user's cache folder).
 
@@ -67,7 +67,7 @@ def autotune(gens, n_frames: int = 0, reps: int = 2, deblock_rows=DEBLOCK_ROWS, 
     intra height.  measure=False only looks the choice up (defaults 8 / 8 when it is
     not cached), so that nothing is launched.  Returns {"deblock_waves",
     "intra_waves", "ms", "cached", "seconds"}."""
-    from .engine import Engine
+    from .engine import Engine, independent_runs
 
     t_start = _clock()
     n = min(g.num_pictures for g in gens)
@@ -96,7 +96,7 @@ def autotune(gens, n_frames: int = 0, reps: int = 2, deblock_rows=DEBLOCK_ROWS, 
 
         def measure(d: int, i: int) -> float:
             eng.set_bands(d, i)
-            batches = [eng.batch_create([r[f] for r in rows]) for f in range(n)]
+            batches = [eng.batch_create([r[f] for f in run for r in rows]) for run in independent_runs(rows)]
             try:
                 for b in batches:  # graph capture
                     eng.batch_run(b)

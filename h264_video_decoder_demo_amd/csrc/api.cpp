@@ -54,14 +54,13 @@ struct Stream {
     int W = 0, H = 0;
     uint8_t *dpb = nullptr;
     uint64_t slot_bytes = 0;
-    uint32_t *flags = nullptr;
-    uint64_t *rowprog = nullptr;
-    uint8_t *gstrip = nullptr;     // deblocking strips between row bands, then the intra ones
-    uint64_t *istrip = nullptr;    // the intra strips inside gstrip
-    uint32_t *pairprog = nullptr;  // MBAFF streams: deblocking progress per MB pair
+    // H264G_MAX_STREAM_PICS progress sets, one for each picture of the stream in one
+    // launch, each: per-MB progress words | per-row progress words | deblocking strips
+    // between row bands | intra strips | MBAFF streams: deblocking progress per MB pair
+    uint8_t *prog = nullptr;
+    uint64_t set_bytes = 0, o_rowprog = 0, o_gstrip = 0, o_istrip = 0, o_pair = 0;
     bool mbaff = false;            // MBAFF frames (pairs of field / frame MBs)
-    uint64_t strip_bytes = 0;      // size of gstrip, sized for bands of
-    int strip_dbk_rows = 0, strip_intra_rows = 0;  // at least these many rows
+    int strip_dbk_rows = 0, strip_intra_rows = 0;  // strips sized for bands of at least these many rows
     uint64_t bytes = 0;
     int lane = 0;                  // HIP stream (lane) of the context that runs its pictures
     uint64_t gen = 0;              // open count of the context when this stream was opened:
@@ -69,6 +68,16 @@ struct Stream {
                                    // same id carry another value and are refused
     bool open = false;
 };
+
+// Points a picture's progress words and strips at progress set k of its stream.
+void use_set(const Stream &s, int k, DevPic &P) {
+    uint8_t *b = s.prog + (uint64_t)k * s.set_bytes;
+    P.flags = (uint32_t *)b;
+    P.rowprog = (uint64_t *)(b + s.o_rowprog);
+    P.gstrip = b + s.o_gstrip;
+    P.istrip = (uint64_t *)(b + s.o_istrip);
+    P.pairprog = s.mbaff ? (uint32_t *)(b + s.o_pair) : nullptr;
+}
 
 struct Batch;
 
@@ -717,13 +726,13 @@ int stage_picture(h264g_ctx *ctx, Stream &st, const h264g_pic_desc &d, const h26
     DevPic &P = pic->hpic;
     std::memset(&P, 0, sizeof(P));
     P.mbs = pic->d_mbs; P.aux = pic->d_aux; P.levels = pic->d_levels; P.wt = pic->d_wt;
-    P.dpb = st.dpb; P.slot_bytes = st.slot_bytes; P.flags = st.flags; P.rowprog = st.rowprog; P.gstrip = st.gstrip; P.istrip = st.istrip; P.bs = d_bs; P.dbq = d_dbq; P.dst = d.dst_slot;
+    P.dpb = st.dpb; P.slot_bytes = st.slot_bytes; P.bs = d_bs; P.dbq = d_dbq; P.dst = d.dst_slot;
     P.ires = (int16_t *)(base + o_ires);
     P.irow = pic->d_items + irow_at; P.n_intra = pic->n_intra;
     P.W = W; P.H = H; P.offA = d.filter_offset_a; P.offB = d.filter_offset_b;
     P.mbaff = mbaff ? 1 : 0;
     P.bsref = mbaff ? (const int8_t *)(base + o_bsref) : nullptr;
-    P.pairprog = st.pairprog;
+    use_set(st, 0, P);
     P.dbk_idc = d.disable_deblocking_filter_idc; P.cip = d.constrained_intra_pred_flag;
     HIPCHK(hipMemcpy(pic->d_mbs, mbs, n_mbs * sizeof(h264g_mb), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(pic->d_aux, aux.data(), n_mbs * sizeof(DevAux), hipMemcpyHostToDevice));
@@ -999,10 +1008,7 @@ int32_t h264g_destroy(h264g_ctx *c) {
     c->pool.clear();
     for (auto &s : c->streams) {
         if (s.dpb) (void)hipFree(s.dpb);
-        if (s.flags) (void)hipFree(s.flags);
-        if (s.rowprog) (void)hipFree(s.rowprog);
-        if (s.gstrip) (void)hipFree(s.gstrip);
-        if (s.pairprog) (void)hipFree(s.pairprog);
+        if (s.prog) (void)hipFree(s.prog);
     }
     for (auto &e : c->ev) if (e) (void)hipEventDestroy(e);
     if (c->d_err) (void)hipFree(c->d_err);
@@ -1039,12 +1045,17 @@ int32_t h264g_stream_open(h264g_ctx *c, const h264g_stream_desc *d, uint32_t *id
     // luma_list), and find zeros there as the reference does
     s.slot_bytes = (uint64_t)s.W * s.H * 384 + (s.mbaff ? ((uint64_t)s.W * 8 + 255) / 256 * 256 : 0);
     s.lane = c->split_mbs > 0 && s.W * s.H >= c->split_mbs ? 1 : 0;
-    // DPB slots, per-MB progress words, per-row progress words, and the strips between
-    // row bands: deblocking (256 B per MB column and band), then intra (64 B)
-    const uint64_t dstrip_bytes = (uint64_t)((s.H + c->dbk_waves - 1) / c->dbk_waves) * s.W * 256;
-    const uint64_t strip_bytes = dstrip_bytes + (uint64_t)((s.H + c->intra_waves - 1) / c->intra_waves) * s.W * 64;
-    const size_t pair_bytes = s.mbaff ? (size_t)s.W * s.H / 2 * 4 : 0;
-    uint64_t need = s.slot_bytes * d->dpb_slots + (uint64_t)s.W * s.H * 4 + (uint64_t)s.H * 8 + strip_bytes + pair_bytes;
+    // DPB slots, and the progress sets: per-MB progress words, per-row progress words,
+    // the strips between row bands (deblocking: 256 B per MB column and band; intra: 64 B)
+    // and the MB pair progress words of MBAFF streams
+    auto al = [](uint64_t v) { return (v + 255) & ~(uint64_t)255; };
+    s.o_rowprog = al((uint64_t)s.W * s.H * 4);
+    s.o_gstrip = al(s.o_rowprog + (uint64_t)s.H * 8);
+    s.o_istrip = s.o_gstrip + (uint64_t)((s.H + c->dbk_waves - 1) / c->dbk_waves) * s.W * 256;
+    s.o_pair = al(s.o_istrip + (uint64_t)((s.H + c->intra_waves - 1) / c->intra_waves) * s.W * 64);
+    s.set_bytes = al(s.o_pair + (s.mbaff ? (uint64_t)s.W * s.H / 2 * 4 : 0));
+    const size_t dpb_bytes = s.slot_bytes * d->dpb_slots, prog_bytes = s.set_bytes * H264G_MAX_STREAM_PICS;
+    const uint64_t need = dpb_bytes + prog_bytes;
     if (c->budget && c->used + need > c->budget)
         return h264g_set_error(H264G_E_BUDGET, "DPB budget: %llu + %llu > %llu", (unsigned long long)c->used,
                                (unsigned long long)need, (unsigned long long)c->budget);
@@ -1052,28 +1063,16 @@ int32_t h264g_stream_open(h264g_ctx *c, const h264g_stream_desc *d, uint32_t *id
     // Blocks come from the event-guarded pool (a closed stream's blocks return there), and
     // the clears run on the stream's lane ahead of its first picture, so opening a stream
     // neither allocates in steady state nor waits for the GPU.
-    const size_t dpb_bytes = s.slot_bytes * d->dpb_slots, flag_bytes = (size_t)s.W * s.H * 4, row_bytes = (size_t)s.H * 8;
     s.dpb = (uint8_t *)pool_get(c, dpb_bytes);
-    s.flags = (uint32_t *)pool_get(c, flag_bytes);
-    s.rowprog = (uint64_t *)pool_get(c, row_bytes);
-    s.gstrip = (uint8_t *)pool_get(c, strip_bytes);
-    if (pair_bytes) s.pairprog = (uint32_t *)pool_get(c, pair_bytes);
-    if (!s.dpb || !s.flags || !s.rowprog || !s.gstrip || (pair_bytes && !s.pairprog)) {
+    s.prog = (uint8_t *)pool_get(c, prog_bytes);
+    if (!s.dpb || !s.prog) {
         pool_put(c, s.dpb, dpb_bytes);
-        pool_put(c, s.flags, flag_bytes);
-        pool_put(c, s.rowprog, row_bytes);
-        pool_put(c, s.gstrip, strip_bytes);
-        pool_put(c, s.pairprog, pair_bytes);
+        pool_put(c, s.prog, prog_bytes);
         return h264g_set_error(H264G_E_NOMEM, "stream device memory");
     }
     hipStream_t ls = lane_stream(c, s.lane);
-    HIPCHK(hipMemsetAsync(s.rowprog, 0, row_bytes, ls));
-    HIPCHK(hipMemsetAsync(s.gstrip, 0, strip_bytes, ls));  // epoch 0 never matches a launch
+    HIPCHK(hipMemsetAsync(s.prog, 0, prog_bytes, ls));  // epoch 0 never matches a launch
     HIPCHK(hipMemsetAsync(s.dpb, 0, dpb_bytes, ls));
-    HIPCHK(hipMemsetAsync(s.flags, 0, flag_bytes, ls));
-    if (pair_bytes) HIPCHK(hipMemsetAsync(s.pairprog, 0, pair_bytes, ls));
-    s.istrip = (uint64_t *)(s.gstrip + dstrip_bytes);
-    s.strip_bytes = strip_bytes;
     s.strip_dbk_rows = c->dbk_waves;
     s.strip_intra_rows = c->intra_waves;
     s.bytes = need;
@@ -1112,10 +1111,7 @@ int32_t h264g_stream_close(h264g_ctx *c, uint32_t id) {
     Stream &s = c->streams[id];
     // back to the pool, reusable once the work enqueued so far has completed (no wait)
     pool_put(c, s.dpb, s.slot_bytes * s.desc.dpb_slots);
-    pool_put(c, s.flags, (size_t)s.W * s.H * 4);
-    pool_put(c, s.rowprog, (size_t)s.H * 8);
-    pool_put(c, s.gstrip, s.strip_bytes);
-    if (s.pairprog) pool_put(c, s.pairprog, (size_t)s.W * s.H / 2 * 4);
+    pool_put(c, s.prog, s.set_bytes * H264G_MAX_STREAM_PICS);
     c->used -= s.bytes;
     s = Stream();
     return H264G_OK;
@@ -1272,6 +1268,11 @@ int build_sub(h264g_ctx *c, const std::vector<const h264g_pic *> &pics, Batch *b
     for (uint32_t i = 0; i < n; i++) {
         const h264g_pic *p = pics[i];
         hp[i] = p->hpic;
+        // the k-th picture of a stream in the batch (a stream's pictures share its lane)
+        // waits on and publishes to progress set k
+        int k = 0;
+        for (uint32_t j = 0; j < i; j++) k += pics[j]->stream == p->stream;
+        use_set(c->streams[p->stream], k, hp[i]);
         b->uses.push_back(slot_use(p->stream, p->desc));
         b->streams.push_back({p->stream, p->stream_gen});
         for (uint32_t a : p->inter_list) (p->mbaff ? minter : inter).push_back((i << WI_PIC_SHIFT) | a);
@@ -1375,15 +1376,26 @@ int32_t h264g_batch_create(h264g_ctx *c, h264g_pic *const *pics, uint32_t n, h26
     if (n >= (1u << (32 - WI_PIC_SHIFT))) return h264g_set_error(H264G_E_ARG, "too many pictures in one batch");
     std::lock_guard<std::mutex> lk(c->mu);
     HIPCHK(hipSetDevice(c->device));
-    std::vector<uint32_t> seen;
     std::vector<const h264g_pic *> lane[2];
     for (uint32_t i = 0; i < n; i++) {
         const h264g_pic *p = pics[i];
         if (!p || p->ctx != c) return h264g_set_error(H264G_E_ARG, "picture %u", i);
         if (int rc = check_stream(c, p->stream, p->stream_gen)) return rc;
-        if (std::find(seen.begin(), seen.end(), p->stream) != seen.end())
-            return h264g_set_error(H264G_E_ARG, "two pictures of stream %u in one batch", p->stream);
-        seen.push_back(p->stream);
+        // pictures of one stream: at most H264G_MAX_STREAM_PICS (one progress set each),
+        // none writing a slot another writes or reads
+        const SlotUse u = slot_use(p->stream, p->desc);
+        int same = 0;
+        for (uint32_t j = 0; j < i; j++) {
+            if (!pics[j] || pics[j]->stream != p->stream) continue;
+            const SlotUse v = slot_use(pics[j]->stream, pics[j]->desc);
+            if (u.dst == v.dst || std::find(u.refs.begin(), u.refs.end(), v.dst) != u.refs.end() ||
+                std::find(v.refs.begin(), v.refs.end(), u.dst) != v.refs.end())
+                return h264g_set_error(H264G_E_ARG, "pictures %u and %u of stream %u in one batch depend on each other "
+                                       "(slots %d and %d)", j, i, p->stream, v.dst, u.dst);
+            if (++same >= H264G_MAX_STREAM_PICS)
+                return h264g_set_error(H264G_E_ARG, "more than %d pictures of stream %u in one batch",
+                                       H264G_MAX_STREAM_PICS, p->stream);
+        }
         lane[c->streams[p->stream].lane].push_back(p);
     }
     h264g_batch *b = new (std::nothrow) h264g_batch();

@@ -349,6 +349,46 @@ class Engine:
         check(lib().h264g_slot_upload(self._h, sid, slot, _ptr(y), _ptr(cb), _ptr(cr)))
 
 
+MAX_STREAM_PICS = 4  # H264G_MAX_STREAM_PICS: pictures of one stream in one batch
+
+
+def slot_use(desc: PicDesc):
+    """(DPB slot the picture writes, set of slots it reads)."""
+    refs = {desc.ref_slot[l][r] for l in range(2) for r in range(32) if desc.ref_slot[l][r] >= 0}
+    return desc.dst_slot, refs
+
+
+def independent_runs(rows, max_per_stream: int = MAX_STREAM_PICS):
+    """Groups the frame indices of per-stream picture lists (StagedPicture or PicDesc, in
+    decoding order) into runs of consecutive indices that can share one batch: in no
+    stream does a picture of a run write a DPB slot that another picture of the run writes
+    or reads (so none depends on another), and a run holds at most max_per_stream pictures
+    of a stream.  With I0 P3 B1 B2 P6 B4 B5 P9 ... in decoding order and non-reference B
+    pictures, the runs are {I0} {P3} {B1 B2 P6} {B4 B5 P9} ...: the batched launches of
+    one stream's pictures follow its dependency levels instead of its decoding order."""
+    n = min(len(r) for r in rows) if rows else 0
+    uses = [[slot_use(getattr(p, "desc", p)) for p in r[:n]] for r in rows]
+    runs, cur = [], []
+    for f in range(n):
+        ok = len(cur) < max_per_stream
+        for su in uses:
+            if not ok:
+                break
+            dst, refs = su[f]
+            for g in cur:
+                d2, r2 = su[g]
+                if dst == d2 or d2 in refs or dst in r2:
+                    ok = False
+                    break
+        if cur and not ok:
+            runs.append(cur)
+            cur = []
+        cur.append(f)
+    if cur:
+        runs.append(cur)
+    return runs
+
+
 def decode_stream_plain(eng: Engine, gen: Generator, readback: bool = True):
     """Decodes a generated stream picture by picture (plain path).
 
@@ -369,4 +409,5 @@ def decode_stream_plain(eng: Engine, gen: Generator, readback: bool = True):
     return out
 
 
-__all__ = ["Generator", "Engine", "StagedPicture", "decode_stream_plain", "_lib"]
+__all__ = ["Generator", "Engine", "StagedPicture", "decode_stream_plain", "independent_runs", "slot_use",
+           "MAX_STREAM_PICS", "_lib"]

@@ -296,8 +296,13 @@ int32_t h264g_picture_decode_static(h264g_ctx *ctx, uint32_t stream_id, const h2
 
 /* Enqueue many staged pictures as one batched job: one persistent launch per
  * stage drains a device queue of macroblock tiles from all of them.  The
- * pictures must not reference each other (e.g. different streams).  No
- * counterpart in the reference, which decodes one picture at a time. */
+ * pictures must not depend on each other: pictures of different streams, and
+ * up to H264G_MAX_STREAM_PICS pictures of one stream with distinct dst slots
+ * none of which reads a slot another of them writes (e.g. the B pictures
+ * between two P pictures together with the next P picture); H264G_E_ARG
+ * otherwise.  No counterpart in the reference, which decodes one picture at a
+ * time. */
+#define H264G_MAX_STREAM_PICS 4
 int32_t h264g_batch_submit(h264g_ctx *ctx, h264g_pic *const *pics, uint32_t n_pics);
 
 /* The same in two steps: build the batch's device work queues once

@@ -92,8 +92,15 @@ class _FakeEngine:
     def open_stream(self, w, h, slots):
         return 0
 
-    def begin_raw(self, sid, *a):
-        return object()
+    def begin_raw(self, sid, i):
+        from h264_video_decoder_demo_amd._lib import PicDesc
+        d = PicDesc()  # picture i writes slot i and reads slot i - 1: one batch per picture
+        d.dst_slot = i
+        for l in range(2):
+            for r in range(32):
+                d.ref_slot[l][r] = -1
+        d.ref_slot[0][0] = i - 1
+        return d
 
     def set_bands(self, d, i):
         assert d >= self.bands[0] and i >= self.bands[1]  # never below the strips' sizing
@@ -120,7 +127,7 @@ class _Gen:
     width_mbs, height_mbs, dpb_slots, num_pictures = 4, 3, 2, 4
 
     def picture_raw(self, i):
-        return ()
+        return (i,)
 
 
 def test_autotune_search_on_cpu(monkeypatch, tmp_path):

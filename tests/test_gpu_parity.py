@@ -237,6 +237,70 @@ def test_overlapped_frames_match_oracle(any_engine):
             engine.close_stream(sid)
 
 
+def test_dependency_level_batches_match_oracle(any_engine):
+    """Batches of several pictures of one stream: the frame indices of four streams
+    (one MBAFF, one on the second lane) grouped into runs of mutually independent
+    pictures (I0 | P3 | B1 B2 P6 | B4 B5 P9 ...), so each batch holds up to three pictures
+    of every stream, each with a progress set of its own.  Run twice back to back with
+    no wait (the second pass re-decodes the same DPB contents); at the end every slot of
+    every stream equals the oracle's DPB."""
+    from oracle.refcpu import Dpb
+    from h264_video_decoder_demo_amd.engine import independent_runs
+    engine = any_engine
+    sizes = [(22, 18, 21), (40, 23, 22), (17, 9, 23)]  # the 920-MB stream runs on the second lane
+    gens = [Generator(2, width_mbs=w, height_mbs=h, n_frames=11, seed=0x0E4A0100 + s, spare_slots=2)
+            for w, h, s in sizes]
+    gens.append(Generator(3, width_mbs=30, height_mbs=22, n_frames=11, seed=0x0E4A0114, spare_slots=2))  # MBAFF
+    sids = [engine.open_stream_for(g) for g in gens]
+    try:
+        frames = [[g.picture(i) for i in range(g.num_pictures)] for g in gens]
+        pics = [[engine.begin(sid, *f) for f in fr] for sid, fr in zip(sids, frames)]
+        runs = independent_runs(pics)
+        assert max(len(r) for r in runs) == 3, runs
+        batches = [engine.batch_create([row[f] for f in run for row in pics]) for run in runs]
+        for _ in range(2):
+            for b in batches:
+                engine.batch_run(b)
+        engine.sync()
+        n = min(len(r) for r in pics)
+        for k, g in enumerate(gens):
+            dpb = Dpb(g.width_mbs, g.height_mbs, g.dpb_slots)
+            for desc, mbs, levels in frames[k][:n]:
+                assert dpb.decode(desc, mbs, levels, 2) == 0
+            written = {desc.dst_slot for desc, _, _ in frames[k][:n]}
+            for slot in sorted(written):
+                got = engine.readback(sids[k], slot)
+                for gp, wp, pl in zip(got, dpb.split(slot), ("Y", "Cb", "Cr")):
+                    assert np.array_equal(gp, wp), f"stream {k} slot {slot} plane {pl}"
+        for b in batches:
+            engine.batch_destroy(b)
+        for row in pics:
+            for p in row:
+                engine.release(p)
+    finally:
+        for sid in sids:
+            engine.close_stream(sid)
+
+
+def test_dependent_pictures_of_one_stream_are_refused(engine):
+    """A batch refuses two pictures of one stream that depend on each other (one reads or
+    writes the slot the other writes), before anything is launched."""
+    from h264_video_decoder_demo_amd._lib import H264GError
+    gen = Generator(2, width_mbs=6, height_mbs=4, n_frames=9, seed=0x0E4A0200, spare_slots=2)
+    sid = engine.open_stream_for(gen)
+    try:
+        pics = [engine.begin(sid, *gen.picture(i)) for i in range(gen.num_pictures)]
+        for pair in ([0, 1], [1, 2], [2, 3, 4, 5]):  # P3 reads I0; B1 reads P3; B4 reads P6
+            with pytest.raises(H264GError, match="depend on each other"):
+                engine.batch_create([pics[i] for i in pair])
+        b = engine.batch_create([pics[i] for i in (2, 3, 4)])  # B1 B2 P6
+        engine.batch_destroy(b)
+        for p in pics:
+            engine.release(p)
+    finally:
+        engine.close_stream(sid)
+
+
 @pytest.mark.parametrize("shape", [(240, 6), (120, 12), (3, 40)])
 def test_mbaff_pictures_match_oracle(any_engine, shape):
     """MBAFF frames of other shapes (as wide as 2160p, narrow and tall) against the

@@ -110,3 +110,45 @@ def test_decoder_stops_when_the_callback_returns_nonzero(engine):
     dec.set_output_frame_callback_functuin(cb, None)
     dec.open(gen)
     assert calls == [0, 2, None]
+
+
+def _desc(dst, refs):
+    from h264_video_decoder_demo_amd._lib import PicDesc
+    d = PicDesc()
+    d.dst_slot = dst
+    for l in range(2):
+        for r in range(32):
+            d.ref_slot[l][r] = -1
+    for r, slot in enumerate(refs):
+        d.ref_slot[r % 2][r // 2] = slot
+    return d
+
+
+def test_independent_runs_follow_dependency_levels():
+    """Runs of frame indices that may share a batch: I0 | P3 | B1 B2 P6 | B4 B5 P9 ... for the
+    generator's IPBB streams, every pair inside a run independent in every stream, and
+    each run ended by a picture that depends on one inside it (or by the per-stream cap)."""
+    from h264_video_decoder_demo_amd.engine import MAX_STREAM_PICS, independent_runs, slot_use
+    gens = [Generator(2, width_mbs=6, height_mbs=4, n_frames=16, seed=0x1D0 + k, spare_slots=2) for k in range(2)]
+    rows = [[g.picture_raw(i)[0] for i in range(g.num_pictures)] for g in gens]
+    runs = independent_runs(rows)
+    assert runs == [[0], [1], [2, 3, 4], [5, 6, 7], [8, 9, 10], [11, 12, 13], [14, 15]]
+    assert [f for r in runs for f in r] == list(range(16))
+
+    def dep(a, b):
+        (da, ra), (db, rb) = slot_use(a), slot_use(b)
+        return da == db or da in rb or db in ra
+
+    for run, nxt in zip(runs, runs[1:] + [None]):
+        for row in rows:
+            assert not any(dep(row[f], row[g]) for f in run for g in run if f < g)
+        if nxt is not None:
+            assert len(run) == MAX_STREAM_PICS or any(dep(row[f], row[nxt[0]]) for row in rows for f in run)
+    # a dependency in one stream splits the run for all; the cap splits independent pictures
+    a = [_desc(0, []), _desc(1, []), _desc(2, [])]
+    b = [_desc(0, []), _desc(1, [0]), _desc(2, [])]
+    assert independent_runs([a]) == [[0, 1, 2]]
+    assert independent_runs([a, b]) == [[0], [1, 2]]
+    wide = [_desc(k, [9]) for k in range(6)]
+    assert independent_runs([wide]) == [[0, 1, 2, 3], [4, 5]]
+    assert independent_runs([wide], max_per_stream=1) == [[k] for k in range(6)]
