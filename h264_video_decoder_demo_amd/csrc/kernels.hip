@@ -88,9 +88,10 @@ __device__ __forceinline__ void wave_sync() {
 }
 
 // ------------------------------------------------------------ residuals
-// LDS scratch of one wave: dense scan-order levels of the MB (slots: 0 = Intra16x16 DC,
-// 1 + luma4x4BlkIdx (8x8 block q = slots 1+4q..4+4q), 17/18 = chroma DC Cb/Cr,
-// 19 + 4c + b = chroma AC), then the residual planes.
+// LDS scratch of one wave: the MB's levels as they arrive -- per slot (0 = Intra16x16
+// DC, 1 + luma4x4BlkIdx (8x8 block q = slots 1+4q..4+4q), 17/18 = chroma DC Cb/Cr,
+// 19 + 4c + b = chroma AC) its 16-bit mask of non-zero scan positions and the index of
+// its first value, and the values in stream order -- then the residual planes.
 // Luma interpolation of one list (inter MBs, after the residual is built).  Window row rr
 // of 4x4 block b holds reference row y0 - 2 + rr at columns x0 - 2 .. x0 + 5, where
 // (x0, y0) is the block's integer sample position; b1 is the unrounded horizontal 6-tap
@@ -104,15 +105,17 @@ struct WaveMc {
 struct alignas(16) WaveRes {
     union {
         struct {
-            int32_t coef[27][16];
-            int32_t t8[4][64];    // 8x8 transform: rows after the horizontal pass
+            uint32_t sm[28];          // slot: mask (bits 0..15) | index of its first value << 16
+            int16_t vals[7 * WAVE];   // level values in stream order (at most 27 * 16)
+            int32_t dense[4][64];     // scan-order levels: the 8x8 blocks, or [0] = Intra16x16 DC
+            int32_t cdc[2][4];        // scan-order levels: chroma DC
+            int32_t t8[4][64];        // 8x8 transform: rows after the horizontal pass
         };
-        WaveMc mc;                // live only after the residual planes are built
+        WaveMc mc;                    // live only after the residual planes are built
     };
     int16_t rl[256];      // luma residual, raster 16x16
     int16_t rc[2][64];    // chroma residual, raster 8x8
 };
-static_assert(sizeof(WaveMc) <= 27 * 16 * 4 + 4 * 64 * 4, "interpolation buffer overlays the levels");
 
 // Slot (see WaveRes) of the MB's i-th level mask, in the order of the level stream: the
 // Intra16x16 DC block, the four 4x4 blocks (or 16-level chunks of the 8x8 block) of each
@@ -167,20 +170,27 @@ __device__ __forceinline__ void quad_transpose(int32_t (&m)[4], int r) {
     if (hi) { m[0] = a; m[1] = b; } else { m[2] = a; m[3] = b; }
 }
 
+// Level at scan position k (< 16) of the block with slot entry e: 0 unless mask bit k is
+// set, else the value at the block's first index plus the number of mask bits below k.
+__device__ __forceinline__ int32_t level_at(uint32_t e, const int16_t *vals, int k) {
+    const uint32_t below = __popc(e & ((1u << k) - 1u));
+    return ((e >> k) & 1u) ? (int32_t)vals[(e >> 16) + below] : 0;
+}
+
 // 8.5.12 for row r of a 4x4 block on the quad that holds the block (lane r of the quad):
-// dequantise the row's levels (scan order in `scan`), horizontal pass, transpose, vertical
-// pass; returns column r of the residual in out[0..3] (rows 0..3).  dc_pass: sample
-// (0, 0) is `dc` as it stands (Intra16x16 and chroma DC).  field: the levels follow the
-// field scan (field MBs, H264PictureBase.cpp:3419).
-__device__ __forceinline__ void residual4x4_quad(const int32_t *scan, int qp, bool dc_pass, int dc, int r, bool field,
-                                                 int32_t (&out)[4]) {
+// dequantise the row's levels (slot entry e), horizontal pass, transpose, vertical pass;
+// returns column r of the residual in out[0..3] (rows 0..3).  dc_pass: sample (0, 0) is
+// `dc` as it stands (Intra16x16 and chroma DC).  field: the levels follow the field scan
+// (field MBs, H264PictureBase.cpp:3419).
+__device__ __forceinline__ void residual4x4_quad(uint32_t se, const int16_t *vals, int qp, bool dc_pass, int dc, int r,
+                                                 bool field, int32_t (&out)[4]) {
     const int m = qp % 6, e = qp / 6;
     const int s0 = 16 * c_v4[m][0], s1 = 16 * c_v4[m][1], s2 = 16 * c_v4[m][2];
     const uint32_t iz = (uint32_t)((field ? IFLD4 : IZZ4) >> (16 * r)) & 0xFFFFu;
     int32_t d[4];
 #pragma unroll
     for (int j = 0; j < 4; j++) {
-        const int32_t v = scan[(iz >> (4 * j)) & 15u];
+        const int32_t v = level_at(se, vals, (int)((iz >> (4 * j)) & 15u));
         const int ls = ((r | j) & 1) == 0 ? s0 : (((r & j) & 1) ? s1 : s2);
         d[j] = e >= 4 ? (v * ls) << (e - 4) : (v * ls + (1 << (3 - e))) >> (4 - e);
     }
@@ -206,8 +216,7 @@ __device__ __forceinline__ void idct8(int32_t *v, int s) {  // 8.5.13.2, in plac
     v[4 * s] = f6 - f1; v[5 * s] = f4 - f3; v[6 * s] = f2 - f5; v[7 * s] = f0 - f7;
 }
 
-// Expands the MB's levels into ws->coef and computes all residual blocks into ws->rl /
-// ws->rc.  Must be called by the whole wave.  Returns false when the MB has no levels.
+// Places the MB's levels in LDS and computes all residual blocks into ws->rl / ws->rc.  Must be called by the whole wave.  Returns false when the MB has no levels.
 // Field MBs of MBAFF frames take the field scans.
 template <bool MBAFF>
 __device__ __forceinline__ bool build_residual(const h264g_mb *mb, const DevAux &ax, const int16_t *levels, WaveRes *ws) {
@@ -218,13 +227,11 @@ __device__ __forceinline__ bool build_residual(const h264g_mb *mb, const DevAux 
     const int i16 = mb->mb_type == H264G_MB_I16x16;
     const int t8 = (ax.kind & AUX_T8) ? 1 : 0;
     const int qp = mb->qp_y;
-    uint4 *cf = (uint4 *)&ws->coef[0][0];
-    for (int k = lane; k < 27 * 16 / 4; k += WAVE) cf[k] = make_uint4(0u, 0u, 0u, 0u);
     const int16_t *lv = levels + mb->level_offset;
     // <= 27 masks of 16 bits, so at most 432 values; the host counted them, so the masks
     // and the values load together (coalesced, all in flight at once)
     const int nmask = ax.nmask, total = (int)(ax.ires >> AUX_NVALS_SHIFT);
-    int mask = lane < nmask ? (uint16_t)lv[lane] : 0;
+    const int mask = lane < nmask ? (uint16_t)lv[lane] : 0;
     int16_t v[7];
 #pragma unroll
     for (int t = 0; t < 7; t++) {
@@ -240,29 +247,29 @@ __device__ __forceinline__ bool build_residual(const h264g_mb *mb, const DevAux 
         const uint64_t bl = __builtin_amdgcn_ballot_w64((cnt >> bit) & 1);
         pre += (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(bl >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bl, 0u)) << bit;
     }
-    // the values into LDS (t8 is free until the 8x8 transform's first pass, after the
-    // scatter); the scatter then reads LDS
-    int16_t *stage = (int16_t *)&ws->t8[0][0];
-    static_assert(sizeof(ws->t8) >= 7 * WAVE * sizeof(int16_t), "value staging fits t8");
+    // slot entries (absent slots 0; the wave's LDS writes land in program order) and values
+    if (lane < 28) ws->sm[lane] = 0u;
+    if (lane < nmask) ws->sm[level_slot(lane, i16, cbp)] = (uint32_t)mask | (uint32_t)pre << 16;
 #pragma unroll
     for (int t = 0; t < 7; t++)
-        if (lane + t * WAVE < total) stage[lane + t * WAVE] = v[t];
+        if (lane + t * WAVE < total) ws->vals[lane + t * WAVE] = v[t];
     wave_sync();
-    if (lane < nmask) {
-        int slot = level_slot(lane, i16, cbp);
-        int n = pre;
-        while (mask) {
-            int k = __ffs(mask) - 1;
-            mask &= mask - 1;
-            ws->coef[slot][k] = stage[n++];
-        }
+    // scan-order levels where a pass reads positions of other rows: the 8x8 blocks (lane
+    // -> position lane of each), the Intra16x16 DC block (lanes 0..15) and the chroma DC
+    // blocks (lanes 56..63)
+    if (t8) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) ws->dense[q][lane] = level_at(ws->sm[1 + 4 * q + (lane >> 4)], ws->vals, lane & 15);
+    } else if (i16 && lane < 16) {
+        ws->dense[0][lane] = level_at(ws->sm[0], ws->vals, lane);
     }
-    wave_sync();
+    if ((cbp >> 4) && lane >= 56) ws->cdc[(lane - 56) >> 2][lane & 3] = level_at(ws->sm[17 + ((lane - 56) >> 2)], ws->vals, lane & 3);
+    if (t8 || i16 || (cbp >> 4)) wave_sync();
     if (t8) {
         // 8.5.13: lanes 0-31 = (8x8 block q, row i): dequantise + horizontal pass ...
         if (lane < 32) {
             const int q = lane >> 3, i = lane & 7;
-            const int32_t *src = &ws->coef[1 + 4 * q][0];
+            const int32_t *src = ws->dense[q];
             int32_t d[8];
 #pragma unroll
             for (int j = 0; j < 8; j++) {
@@ -292,8 +299,8 @@ __device__ __forceinline__ bool build_residual(const h264g_mb *mb, const DevAux 
                     int32_t vz = 0, vf = 0;
 #pragma unroll
                     for (int k = 0; k < 16; k++) {  // resolved at compile time
-                        if (c_zz4[k] == i * 4 + j) vz = ws->coef[0][k];
-                        if (c_fld4[k] == i * 4 + j) vf = ws->coef[0][k];
+                        if (c_zz4[k] == i * 4 + j) vz = ws->dense[0][k];
+                        if (c_fld4[k] == i * 4 + j) vf = ws->dense[0][k];
                     }
                     const int32_t v = field ? vf : vz;
                     acc += hneg(col, j) ? -v : v;
@@ -304,7 +311,7 @@ __device__ __forceinline__ bool build_residual(const h264g_mb *mb, const DevAux 
             dc = qp >= 36 ? (f * sc) << (qp / 6 - 6) : (f * sc + (1 << (5 - qp / 6))) >> (6 - qp / 6);
         }
         int32_t o[4];
-        residual4x4_quad(ws->coef[1 + b], qp, i16, dc, r, field, o);
+        residual4x4_quad(ws->sm[1 + b], ws->vals, qp, i16, dc, r, field, o);
         const int x0 = blk_x(b), y0 = blk_y(b);
 #pragma unroll
         for (int i = 0; i < 4; i++) ws->rl[(y0 + i) * 16 + x0 + r] = (int16_t)o[i];
@@ -316,12 +323,12 @@ __device__ __forceinline__ bool build_residual(const h264g_mb *mb, const DevAux 
     } else if (lane < 32) {  // chroma: 8.5.11 DC then 8.5.12 per 4x4 block, a quad per block
         const int c = lane >> 4, b = (lane >> 2) & 3, r = lane & 3;
         const int qpc = ax.qpc[c];
-        const int32_t *dcv = ws->coef[17 + c];
+        const int32_t *dcv = ws->cdc[c];
         const int32_t e00 = dcv[0] + dcv[2], e01 = dcv[1] + dcv[3], e10 = dcv[0] - dcv[2], e11 = dcv[1] - dcv[3];
         const int32_t f = b == 0 ? e00 + e01 : b == 1 ? e00 - e01 : b == 2 ? e10 + e11 : e10 - e11;
         const int dc = ((f * ls4(qpc % 6, 0, 0)) << (qpc / 6)) >> 5;
         int32_t o[4];
-        residual4x4_quad(ws->coef[19 + 4 * c + b], qpc, true, dc, r, field, o);
+        residual4x4_quad(ws->sm[19 + 4 * c + b], ws->vals, qpc, true, dc, r, field, o);
         const int x0 = (b & 1) * 4, y0 = (b >> 1) * 4;
 #pragma unroll
         for (int i = 0; i < 4; i++) ws->rc[c][(y0 + i) * 8 + x0 + r] = (int16_t)o[i];
@@ -345,10 +352,6 @@ __device__ __forceinline__ bool build_residual(const h264g_mb *mb, const DevAux 
 }
 
 // ------------------------------------------------------------ inter MC
-__device__ __forceinline__ int ldc(const uint8_t *p, int w, int h, int x, int y) {
-    return p[clip3(0, h - 1, y) * w + clip3(0, w - 1, x)];
-}
-
 // Device memory seen through global-address-space pointers: loads and stores through
 // them are global_* instructions, which count on the vector-memory counter only.  Through
 // generic pointers they would be flat_* instructions, which also count on the LDS counter,
@@ -553,31 +556,63 @@ __device__ __forceinline__ uint32_t luma_list(const DevPic &P, const h264g_mb *m
 }
 
 // Two horizontally adjacent chroma prediction samples at (xC, yC) with a 1/8-sample
-// MV, 8.4.2.2.2 (H264InterPrediction.cpp:2485-2522), coordinates clamped.
-__device__ __forceinline__ void chroma2(const uint8_t *ref, int wc, int hc, int xC, int yC, int mvx, int mvy, int &o0, int &o1) {
+// MV, 8.4.2.2.2 (H264InterPrediction.cpp:2485-2522), coordinates clamped: samples
+// A, B (row yi) and C, D (row yi + 1) weighted (8 - xF)(8 - yF), xF(8 - yF), (8 - xF)yF,
+// xF yF.  The three columns of each row arrive as one dword (two aligned dword loads
+// where the columns lie inside the row, clamped byte loads at the picture's edges), and
+// each output is one v_dot4 of the bytes (A, B, C, D) with the four weights.
+__device__ __forceinline__ void chroma2(const gu8 *ref, int wc, int hc, int xC, int yC, int mvx, int mvy, int &o0, int &o1) {
     const int xi = xC + (mvx >> 3), yi = yC + (mvy >> 3), xF = mvx & 7, yF = mvy & 7;
-    const int A0 = ldc(ref, wc, hc, xi, yi), A1 = ldc(ref, wc, hc, xi + 1, yi), A2 = ldc(ref, wc, hc, xi + 2, yi);
-    const int B0 = ldc(ref, wc, hc, xi, yi + 1), B1 = ldc(ref, wc, hc, xi + 1, yi + 1), B2 = ldc(ref, wc, hc, xi + 2, yi + 1);
-    o0 = ((8 - xF) * (8 - yF) * A0 + xF * (8 - yF) * A1 + (8 - xF) * yF * B0 + xF * yF * B1 + 32) >> 6;
-    o1 = ((8 - xF) * (8 - yF) * A1 + xF * (8 - yF) * A2 + (8 - xF) * yF * B1 + xF * yF * B2 + 32) >> 6;
-}
-
-// 8.4.2.3.1 / 8.4.2.3.2 (H264InterPrediction.cpp:2526-2829); the host resolved the
-// weights, including the L1-only luma entries of quirk Q2, into w0/o0.
-__device__ __forceinline__ int weigh(const DevWeight &w, int c, int pf0, int pf1, int p0, int p1) {
-    if (!w.mode) {
-        if (pf0 && pf1) return (p0 + p1 + 1) >> 1;
-        return pf0 ? p0 : p1;
+    const gu8 *r0 = ref + (uint64_t)clip3(0, hc - 1, yi) * wc, *r1 = ref + (uint64_t)clip3(0, hc - 1, yi + 1) * wc;
+    const int xa = xi & ~3;
+    uint32_t a, b;  // bytes 0..2: columns xi .. xi + 2 of rows yi, yi + 1
+    if (xa >= 0 && xa + 8 <= wc) {
+        const GAS uint32_t *p0 = (const GAS uint32_t *)(r0 + xa), *p1 = (const GAS uint32_t *)(r1 + xa);
+        a = align8(p0[1], p0[0], (uint32_t)(xi & 3));
+        b = align8(p1[1], p1[0], (uint32_t)(xi & 3));
+    } else {
+        const int x0 = clip3(0, wc - 1, xi), x1 = clip3(0, wc - 1, xi + 1), x2 = clip3(0, wc - 1, xi + 2);
+        a = pack4(r0[x0], r0[x1], r0[x2], 0);
+        b = pack4(r1[x0], r1[x1], r1[x2], 0);
     }
-    int lw = w.logWD[c];
-    if (pf0 && pf1) return clip1(((p0 * w.w0[c] + p1 * w.w1[c] + (1 << lw)) >> (lw + 1)) + ((w.o0[c] + w.o1[c] + 1) >> 1));
-    int p = pf0 ? p0 : p1;
-    int ww = (pf0 || c == 0) ? w.w0[c] : w.w1[c];
-    int oo = (pf0 || c == 0) ? w.o0[c] : w.o1[c];
-    if (lw >= 1) return clip1(((p * ww + (1 << (lw - 1))) >> lw) + oo);
-    return clip1(p * ww + oo);
+    const uint32_t w = (uint32_t)((8 - xF) * (8 - yF)) | (uint32_t)(xF * (8 - yF)) << 8 | (uint32_t)((8 - xF) * yF) << 16 |
+                       (uint32_t)(xF * yF) << 24;
+    o0 = (int)(__builtin_amdgcn_udot4(__builtin_amdgcn_perm(b, a, 0x05040100u), w, 32u, false) >> 6);
+    o1 = (int)(__builtin_amdgcn_udot4(__builtin_amdgcn_perm(b, a, 0x06050201u), w, 32u, false) >> 6);
 }
 
+// Explicit weighted prediction, 8.4.2.3.2 (H264InterPrediction.cpp:2526-2829), as one
+// v_dot2 per sample: the two lists' samples as the 16-bit halves of a dword (list 0
+// low), weighted (w0, w1) -- one of them 0 for a single list -- plus the rounding term,
+// then shifted, offset and clipped.  The host resolved the weights, including the
+// L1-only luma entries of quirk Q2, into w0 / o0.  (Default weighting, 8.4.2.3.1, is
+// the rounded average and stays outside.)
+struct WpTerms {
+    uint32_t w;          // (w0, w1) as 16-bit halves
+    int rnd, sh, off;
+};
+__device__ __forceinline__ WpTerms wp_terms(const DevWeight &w, int c, int pf0, int pf1) {
+    const int lw = w.logWD[c];
+    WpTerms t;
+    if (pf0 && pf1) {
+        t.w = (uint16_t)w.w0[c] | (uint32_t)(uint16_t)w.w1[c] << 16;
+        t.rnd = 1 << lw;
+        t.sh = lw + 1;
+        t.off = (w.o0[c] + w.o1[c] + 1) >> 1;
+    } else {
+        const int ww = (pf0 || c == 0) ? w.w0[c] : w.w1[c];
+        t.w = pf0 ? (uint32_t)(uint16_t)ww : (uint32_t)(uint16_t)ww << 16;
+        t.rnd = lw >= 1 ? 1 << (lw - 1) : 0;
+        t.sh = lw;
+        t.off = (pf0 || c == 0) ? w.o0[c] : w.o1[c];
+    }
+    return t;
+}
+__device__ __forceinline__ int wp_apply(const WpTerms &t, uint32_t pair) {
+    typedef short short2_t __attribute__((ext_vector_type(2)));
+    const int v = __builtin_amdgcn_sdot2(__builtin_bit_cast(short2_t, pair), __builtin_bit_cast(short2_t, t.w), t.rnd, false);
+    return clip1((v >> t.sh) + t.off);
+}
 
 // Inter MB (or I_PCM copy): prediction + residual, written to the destination slot.
 template <bool MBAFF>
@@ -613,11 +648,12 @@ __device__ void inter_mb(const DevPic &P, int addr, WaveRes *ws) {
         uint32_t packed;
         if (!w.mode) {
             packed = pf0 && pf1 ? avg4(p0, p1) : (pf0 ? p0 : p1);
-        } else {
-            packed = 0;
-#pragma unroll
-            for (int i = 0; i < 4; i++)
-                packed |= (uint32_t)weigh(w, 0, pf0, pf1, (p0 >> (8 * i)) & 255, (p1 >> (8 * i)) & 255) << (8 * i);
+        } else {  // byte i of p0 / p1 as the 16-bit halves of one dword (v_perm, 0x0c = zero)
+            const WpTerms t = wp_terms(w, 0, pf0, pf1);
+            packed = pack4(wp_apply(t, __builtin_amdgcn_perm(p1, p0, 0x0c040c00u)),
+                           wp_apply(t, __builtin_amdgcn_perm(p1, p0, 0x0c050c01u)),
+                           wp_apply(t, __builtin_amdgcn_perm(p1, p0, 0x0c060c02u)),
+                           wp_apply(t, __builtin_amdgcn_perm(p1, p0, 0x0c070c03u)));
         }
         if (res) {
             const int16_t *rl = &ws->rl[(by * 4 + r) * 16 + bx * 4];
@@ -650,8 +686,15 @@ __device__ void inter_mb(const DevPic &P, int addr, WaveRes *ws) {
         if (pf1) chroma2(slot_base(P, ax.refid[1][q]) + o1, fw, fh, xC, yC, mb->mv[1][b][0], mvy1, c10, c11);
         const DevWeight &w = G(P.wt)[ax.wp[q]];
         uint8_t *plane = c == 0 ? dcb : dcr;
-        int v0 = weigh(w, 1 + c, pf0, pf1, c00, c10);
-        int v1 = weigh(w, 1 + c, pf0, pf1, c01, c11);
+        int v0, v1;
+        if (!w.mode) {
+            v0 = pf0 && pf1 ? (c00 + c10 + 1) >> 1 : (pf0 ? c00 : c10);
+            v1 = pf0 && pf1 ? (c01 + c11 + 1) >> 1 : (pf0 ? c01 : c11);
+        } else {
+            const WpTerms t = wp_terms(w, 1 + c, pf0, pf1);
+            v0 = wp_apply(t, (uint32_t)c00 | (uint32_t)c10 << 16);
+            v1 = wp_apply(t, (uint32_t)c01 | (uint32_t)c11 << 16);
+        }
         if (res) {
             v0 = clip1(v0 + ws->rc[c][(by * 2 + r) * 8 + bx * 2]);
             v1 = clip1(v1 + ws->rc[c][(by * 2 + r) * 8 + bx * 2 + 1]);

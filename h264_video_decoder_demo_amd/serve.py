@@ -5,7 +5,8 @@ The loop admits a stream when its first frame arrives and its DPB fits the HBM
 budget (h264g_stream_open refuses otherwise; a refused stream waits until another
 stream closes), keeps each stream's reference pictures resident in its device DPB,
 and every `window_ms` enqueues one batched launch per stage holding the next
-arrived frame of every admitted stream.  Batches run back to back on the
+arrived frames of every admitted stream (those of a stream that do not depend on
+each other, e.g. two B pictures and the next P picture).  Batches run back to back on the
 context's HIP stream while the host stages the next one; completion of each batch
 is observed through a fence.  A frame's latency is its batch's completion time
 minus its arrival time.
@@ -23,7 +24,7 @@ import time
 import numpy as np
 
 from ._lib import H264G_E_BUDGET, H264GError
-from .engine import Engine, Generator
+from .engine import MAX_STREAM_PICS, Engine, Generator, slot_use
 
 # resolution mix of the serving workload: (share, width MBs, height MBs)
 SERVE_MIX = ((0.25, 22, 18), (0.35, 80, 45), (0.30, 120, 68), (0.10, 240, 135))
@@ -190,12 +191,18 @@ class ServeLoop:
             if now >= next_window:
                 todo = []
                 for s in self.streams:
-                    if s.sid is None or s.next_submit >= s.next_arrival:
-                        continue
-                    if self.on_frame is not None and s.next_submit > s.done:
-                        continue  # read-back mode: one frame per stream in flight
-                    todo.append((s, s.next_submit))
-                    s.next_submit += 1
+                    # the stream's arrived frames that do not depend on each other (up to
+                    # MAX_STREAM_PICS, e.g. B1 B2 P6) share the window's batch; read-back
+                    # mode keeps one frame per stream in flight
+                    taken = []
+                    while (s.sid is not None and s.next_submit < s.next_arrival and len(taken) < MAX_STREAM_PICS
+                           and (self.on_frame is None or (not taken and s.next_submit == s.done))):
+                        dst, refs = slot_use(s.gen.picture_raw(s.next_submit)[0])
+                        if any(dst == d or d in refs or dst in r for d, r in taken):
+                            break
+                        taken.append((dst, refs))
+                        todo.append((s, s.next_submit))
+                        s.next_submit += 1
                 # host staging of the window's pictures on several threads (the library
                 # serialises only the device block allocation)
                 pics = list(stagers.map(lambda t: eng.begin_raw(t[0].sid, *t[0].gen.picture_raw(t[1])), todo))
