@@ -191,7 +191,7 @@ def summarize_pmc(fetch_csv: str, write_csv: str, copy_bytes: int, out_json: str
                     continue
                 name = row["Kernel_Name"]
                 for k in KERNELS + ("k_copy", "k_bs", "k_deblock_band", "k_intra_band", "k_deblock_rows"):
-                    if f"::{k}(" in name:
+                    if f"::{k}(" in name or f"::{k}<false>(" in name:  # frame-picture instance
                         acc[k].append(float(row["Counter_Value"]))
         return {k: sum(v) / len(v) for k, v in acc.items()}
 
@@ -302,7 +302,8 @@ def main():
     # each stream's pictures are launched by dependency level, not one at a time
     runs_f = independent_runs(pics)
     batches = [eng.batch_create([row[f] for f in run for row in pics]) for run in runs_f]
-    log(f"batches per step: {len(batches)} (frame index runs {runs_f})")
+    n_batches = len(batches)
+    log(f"batches per step: {n_batches} (frame index runs {runs_f})")
     log(f"staged {sum(len(r) for r in pics)} pictures ({mpix_step:.1f} Mpix per step) in "
         f"{time.perf_counter() - t:.1f} s")
 
@@ -507,7 +508,7 @@ def main():
         "config": {"workload": "64 streams x 16 frames, mixed 352x288/1280x720/1920x1088/3840x2160 "
                                "High-profile IPBB syntax, batched launch per stage per run of independent frames",
                    "streams": len(gens), "frames_per_stream": nframes, "pictures_per_step": len(gens) * nframes,
-                   "batches_per_step": len(batches),
+                   "batches_per_step": n_batches,
                    "mpix_per_step": round(mpix_step, 3)},
         "graphs": not args.no_graphs,
         "deblock_schedule": args.deblock,
