@@ -150,7 +150,6 @@ struct Batch {
     bool dbk_off = false;            // no picture of the batch is deblocked
     hipGraphExec_t graph = nullptr;  // the captured stage sequence (kernel nodes only)
     hipGraphExec_t graph_dbk = nullptr;  // its deblocking, when that runs on the lane's deblocking stream
-    hipGraphExec_t graph_bs = nullptr;   // its bS pass, when that runs there too
     std::vector<SlotUse> uses;       // DPB slots the pictures write and read
     std::vector<std::pair<uint32_t, uint64_t>> streams;  // (stream id, Stream::gen) of its pictures
 };
@@ -760,8 +759,8 @@ struct StageSet {
     uint32_t n_minter = 0, n_mintra = 0, n_mpairs = 0;
 };
 
-// Parts of a stage set: inter / intra, deblocking, bS.
-constexpr int kFront = 1, kBack = 2, kBs = 4;
+// Parts of a stage set: inter / intra / bS, and deblocking.
+constexpr int kFront = 1, kBack = 2;
 
 // Launches the given parts on stream s.  ev: the run's 6 profiling events (inter, intra,
 // bS, end of those; deblocking, end) or null.
@@ -787,8 +786,6 @@ int launch_stages(h264g_ctx *ctx, const StageSet &q, int lane, int parts, hipStr
             e = h264g_launch_intra(q.pics, q.mintra, q.n_mintra, &ctl[0], &ctl[2].ticket, ctx->d_err, ctx->waves, 1, s);
         if (e) return h264g_set_error(H264G_E_DEVICE, "intra launch (MBAFF): %s", hipGetErrorString((hipError_t)e));
         if (ev) HIPCHK(hipEventRecord(ev[2], s));
-    }
-    if (parts & kBs) {
         if (!q.dbk_off) {
             e = h264g_launch_bs(q.pics, q.all, q.n_all, s);
             if (e) return h264g_set_error(H264G_E_DEVICE, "bS launch: %s", hipGetErrorString((hipError_t)e));
@@ -846,10 +843,7 @@ bool conflicts(const DbkRec &r, const std::vector<SlotUse> &uses) {
 // covers the older ones); inter / intra / bS on the lane; deblocking on the deblocking
 // stream after them, with the band schedule in a two-lane context (the other schedules
 // share per-MB progress words with the intra stage, and a one-lane context is the
-// strictly sequential shape: there deblocking stays on the lane).  There the bS pass,
-// which reads only the MB records, goes to the deblocking stream ahead of the lane's
-// inter / intra stages, so that it runs beside them instead of after them (except in
-// profiling runs, whose stage events time it on the lane).
+// strictly sequential shape: there deblocking stays on the lane).
 int enqueue(h264g_ctx *c, const StageSet &q, int lane, const std::vector<SlotUse> &uses, Batch *sb,
             bool events = true) {
     hipStream_t s = lane_stream(c, lane), ds = c->dstream[lane];
@@ -879,17 +873,7 @@ int enqueue(h264g_ctx *c, const StageSet &q, int lane, const std::vector<SlotUse
         if (e != hipSuccess) { *g = nullptr; return h264g_set_error(H264G_E_DEVICE, "graph instantiate: %s", hipGetErrorString(e)); }
         return 0;
     };
-    const bool bs_early = split && !ev;
-    if (bs_early) {
-        if (graphs) {
-            if (!sb->graph_bs) { int rc = capture(kBs, ds, &sb->graph_bs); if (rc) return rc; }
-            HIPCHK(hipGraphLaunch(sb->graph_bs, ds));
-        } else {
-            int rc = launch_stages(c, q, lane, kBs, ds, nullptr);
-            if (rc) return rc;
-        }
-    }
-    const int first = split ? (bs_early ? kFront : kFront | kBs) : (kFront | kBs | kBack);
+    const int first = split ? kFront : (kFront | kBack);
     if (graphs) {
         if (!sb->graph) { int rc = capture(first, s, &sb->graph); if (rc) return rc; }
         HIPCHK(hipGraphLaunch(sb->graph, s));
@@ -921,13 +905,10 @@ int enqueue(h264g_ctx *c, const StageSet &q, int lane, const std::vector<SlotUse
 // Caller holds ctx->mu.  A batch with a graph is destroyed only when idle.
 int free_batch(h264g_ctx *c, h264g_batch *b) {
     if (!b) return 0;
-    if (b->sub[0].graph || b->sub[1].graph || b->sub[0].graph_dbk || b->sub[1].graph_dbk || b->sub[0].graph_bs ||
-        b->sub[1].graph_bs)
-        (void)sync_lanes(c);
+    if (b->sub[0].graph || b->sub[1].graph || b->sub[0].graph_dbk || b->sub[1].graph_dbk) (void)sync_lanes(c);
     for (Batch &sb : b->sub) {
         if (sb.graph) (void)hipGraphExecDestroy(sb.graph);
         if (sb.graph_dbk) (void)hipGraphExecDestroy(sb.graph_dbk);
-        if (sb.graph_bs) (void)hipGraphExecDestroy(sb.graph_bs);
         pool_put(c, sb.dmem, sb.dbytes);
     }
     delete b;
