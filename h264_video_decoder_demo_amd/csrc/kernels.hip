@@ -60,16 +60,6 @@ __device__ __forceinline__ int ls4(int m, int i, int j) {
     int cls = ((i | j) & 1) == 0 ? 0 : (((i & j) & 1) ? 1 : 2);
     return 16 * c_v4[m][cls];
 }
-__device__ __forceinline__ int ls8(int m, int i, int j) {
-    int cls;
-    if ((i & 3) == 0 && (j & 3) == 0) cls = 0;
-    else if ((i & 1) && (j & 1)) cls = 1;
-    else if ((i & 3) == 2 && (j & 3) == 2) cls = 2;
-    else if (((i & 3) == 0 && (j & 1)) || ((i & 1) && (j & 3) == 0)) cls = 3;
-    else if (((i & 3) == 0 && (j & 3) == 2) || ((i & 3) == 2 && (j & 3) == 0)) cls = 4;
-    else cls = 5;
-    return 16 * c_v8[m][cls];
-}
 
 // luma4x4BlkIdx -> (x, y) in the MB (6.4.3)
 __device__ __forceinline__ int blk_x(int b) { return ((b >> 2) & 1) * 8 + (b & 1) * 4; }
@@ -107,7 +97,8 @@ struct alignas(16) WaveRes {
         struct {
             uint32_t sm[28];          // slot: mask (bits 0..15) | index of its first value << 16
             int16_t vals[7 * WAVE];   // level values in stream order (at most 27 * 16)
-            int32_t dense[4][64];     // scan-order levels: the 8x8 blocks, or [0] = Intra16x16 DC
+            int32_t dense[4][64];     // levels: the 8x8 blocks in raster order, or [0] = the
+                                      // Intra16x16 DC block in scan order
             int32_t cdc[2][4];        // scan-order levels: chroma DC
             int32_t t8[4][64];        // 8x8 transform: rows after the horizontal pass
         };
@@ -186,13 +177,16 @@ __device__ __forceinline__ void residual4x4_quad(uint32_t se, const int16_t *val
                                                  bool field, int32_t (&out)[4]) {
     const int m = qp % 6, e = qp / 6;
     const int s0 = 16 * c_v4[m][0], s1 = 16 * c_v4[m][1], s2 = 16 * c_v4[m][2];
+    // LevelScale4x4 of row r: s0 at (even, even), s1 at (odd, odd), s2 elsewhere
+    const int ls_even = (r & 1) ? s2 : s0, ls_odd = (r & 1) ? s1 : s2;
+    // 8-336 / 8-337 as one expression: qP / 6 >= 4 shifts left, else rounds and shifts right
+    const int lsh = e >= 4 ? e - 4 : 0, rsh = e >= 4 ? 0 : 4 - e, rnd = e >= 4 ? 0 : 1 << (3 - e);
     const uint32_t iz = (uint32_t)((field ? IFLD4 : IZZ4) >> (16 * r)) & 0xFFFFu;
     int32_t d[4];
 #pragma unroll
     for (int j = 0; j < 4; j++) {
         const int32_t v = level_at(se, vals, (int)((iz >> (4 * j)) & 15u));
-        const int ls = ((r | j) & 1) == 0 ? s0 : (((r & j) & 1) ? s1 : s2);
-        d[j] = e >= 4 ? (v * ls) << (e - 4) : (v * ls + (1 << (3 - e))) >> (4 - e);
+        d[j] = (int32_t)(((uint32_t)(v * ((j & 1) ? ls_odd : ls_even)) << lsh) + (uint32_t)rnd) >> rsh;
     }
     if (dc_pass && r == 0) d[0] = dc;
     int32_t f[4];
@@ -254,12 +248,12 @@ __device__ __forceinline__ bool build_residual(const h264g_mb *mb, const DevAux 
     for (int t = 0; t < 7; t++)
         if (lane + t * WAVE < total) ws->vals[lane + t * WAVE] = v[t];
     wave_sync();
-    // scan-order levels where a pass reads positions of other rows: the 8x8 blocks (lane
-    // -> position lane of each), the Intra16x16 DC block (lanes 0..15) and the chroma DC
-    // blocks (lanes 56..63)
-    if (t8) {
+    // dense levels where a pass reads positions of other rows: the 8x8 blocks, the
+    // Intra16x16 DC block (lanes 0..15) and the chroma DC blocks (lanes 56..63)
+    if (t8) {  // lane = raster position in the 8x8 blocks, p its scan position
+        const int p = field ? c_ifld8[lane] : c_izz8[lane];
 #pragma unroll
-        for (int q = 0; q < 4; q++) ws->dense[q][lane] = level_at(ws->sm[1 + 4 * q + (lane >> 4)], ws->vals, lane & 15);
+        for (int q = 0; q < 4; q++) ws->dense[q][lane] = level_at(ws->sm[1 + 4 * q + (p >> 4)], ws->vals, p & 15);
     } else if (i16 && lane < 16) {
         ws->dense[0][lane] = level_at(ws->sm[0], ws->vals, lane);
     }
@@ -268,14 +262,22 @@ __device__ __forceinline__ bool build_residual(const h264g_mb *mb, const DevAux 
     if (t8) {
         // 8.5.13: lanes 0-31 = (8x8 block q, row i): dequantise + horizontal pass ...
         if (lane < 32) {
-            const int q = lane >> 3, i = lane & 7;
-            const int32_t *src = ws->dense[q];
+            const int q = lane >> 3, i = lane & 7, m = qp % 6, e = qp / 6;
+            const int32_t *src = &ws->dense[q][i * 8];
+            // LevelScale8x8 of row i by column class (j % 4 == 0, j odd, j % 4 == 2): the
+            // six normAdjust8x8 classes of 8-318
+            const int v0 = 16 * c_v8[m][0], v1 = 16 * c_v8[m][1], v2 = 16 * c_v8[m][2], v3 = 16 * c_v8[m][3],
+                      v4 = 16 * c_v8[m][4], v5 = 16 * c_v8[m][5];
+            const int ls_0 = (i & 1) ? v3 : (i & 2) ? v4 : v0;
+            const int ls_1 = (i & 1) ? v1 : (i & 2) ? v5 : v3;
+            const int ls_2 = (i & 1) ? v5 : (i & 2) ? v2 : v4;
+            // 8-338 / 8-339 as one expression (qP / 6 >= 6 shifts left)
+            const int lsh = e >= 6 ? e - 6 : 0, rsh = e >= 6 ? 0 : 6 - e, rnd = e >= 6 ? 0 : 1 << (5 - e);
             int32_t d[8];
 #pragma unroll
             for (int j = 0; j < 8; j++) {
-                const int32_t v = src[field ? c_ifld8[i * 8 + j] : c_izz8[i * 8 + j]];
-                d[j] = qp >= 36 ? (v * ls8(qp % 6, i, j)) << (qp / 6 - 6)
-                                : (v * ls8(qp % 6, i, j) + (1 << (5 - qp / 6))) >> (6 - qp / 6);
+                const int ls = (j & 1) ? ls_1 : (j & 2) ? ls_2 : ls_0;
+                d[j] = (int32_t)(((uint32_t)(src[j] * ls) << lsh) + (uint32_t)rnd) >> rsh;
             }
             idct8(d, 1);
 #pragma unroll
