@@ -105,6 +105,15 @@ class GenParams(ctypes.Structure):
     ]
 
 
+class ParsedPicture(ctypes.Structure):
+    _fields_ = [
+        ("desc", PicDesc), ("mbs", c_void_p), ("n_mbs", c_uint32), ("levels", c_void_p), ("n_levels", c_uint64),
+        ("width_mbs", c_int32), ("height_mbs", c_int32), ("dpb_slots", c_int32), ("max_num_reorder_frames", c_int32),
+        ("idr", c_int32), ("nal_ref_idc", c_int32), ("frame_num", c_int32),
+        ("profile_idc", c_int32), ("constraint_set3", c_int32), ("level_idc", c_int32),
+    ]
+
+
 # Every entry point include/h264g.h declares, with its ctypes signature.
 _SIGNATURES = {
     "h264g_create": (c_int32, [POINTER(Config), POINTER(c_void_p)]),
@@ -150,6 +159,9 @@ _SIGNATURES = {
     "h264g_mvfield_derive": (c_int32, [c_void_p, c_uint32, c_void_p, c_void_p]),
     "h264g_mvfield_intra_modes": (c_int32, [c_void_p, c_uint32, c_void_p, c_void_p, c_int32]),
     "h264g_mvfield_colocated": (c_int32, [c_void_p, c_void_p, c_int32]),
+    "h264g_parser_create": (c_int32, [c_void_p, c_uint64, POINTER(c_void_p)]),
+    "h264g_parser_destroy": (c_int32, [c_void_p]),
+    "h264g_parser_next": (c_int32, [c_void_p, POINTER(ParsedPicture)]),
 }
 
 _lib = None

@@ -16,7 +16,7 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 OUT = os.path.join(PKG, "libh264g.so")
 
-SOURCES = ["api.cpp", "gen.cpp", "mvpred.cpp", "kernels.hip"]
+SOURCES = ["api.cpp", "gen.cpp", "mvpred.cpp", "bitstream.cpp", "kernels.hip"]
 
 HIPCC_FLAGS = [
     "-O3", "-std=c++17", "--offload-arch=gfx950", "-fPIC", "-shared",

@@ -1,0 +1,1388 @@
+// Annex B byte-stream front end for frame pictures coded with CAVLC: NAL units, SPS / PPS,
+// slice headers, slice data (macroblock_layer, mb_pred / sub_mb_pred, residual_block_cavlc)
+// and the decoded picture buffer (POC, reference marking, reference picture lists, DPB
+// slots), producing the h264g_pic_desc / h264g_mb / level records the GPU stages consume.
+// It replaces the reference's parsing and picture management on the path to
+// reconstruction: CH264NalUnit / CH264SPS / CH264PPS / CH264SliceHeader
+// (H264NalUnit.cpp:122, H264SPS.cpp, H264PPS.cpp, H264SliceHeader.cpp:289-509),
+// CH264SliceData::slice_data and CH264MacroBlock::macroblock_layer
+// (H264SliceData.cpp:386-487, H264MacroBlock.cpp:912-1139), CH264ResidualBlockCavlc
+// (H264ResidualBlockCavlc.cpp) and the reference list / marking code (H264RefPicList.cpp).
+// Motion vectors and intra prediction modes are derived by the motion field
+// (h264g_mvfield_*, mvpred.cpp) from the parsed mvd and prev / rem syntax.
+//
+// Refused with H264G_E_ARG (none occur in the reference's CAVLC sample streams of one
+// slice per picture): CABAC, field pictures and MBAFF, several slices per picture or
+// slice groups, scaling matrices, chroma formats other than 4:2:0, bit depths above 8,
+// SP / SI slices, temporal direct prediction, POC type 1, gaps in frame_num.
+#include <algorithm>
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "h264g.h"
+
+extern "C" int32_t h264g_set_error(int32_t code, const char *fmt, ...);
+
+namespace {
+
+// ------------------------------------------------------------------ bit reader
+struct Bits {
+    const uint8_t *d = nullptr;
+    size_t nbits = 0, pos = 0;
+    bool over = false;  // read past the end
+    uint32_t bit() {
+        if (pos >= nbits) { over = true; return 0; }
+        const uint32_t b = (d[pos >> 3] >> (7 - (pos & 7))) & 1u;
+        pos++;
+        return b;
+    }
+    uint32_t u(int n) {
+        uint32_t v = 0;
+        for (int i = 0; i < n; i++) v = (v << 1) | bit();
+        return v;
+    }
+    // the next n (<= 32) bits without consuming them, zeros past the end
+    uint32_t peek(int n) const {
+        uint32_t v = 0;
+        for (int i = 0; i < n; i++) {
+            const size_t p = pos + i;
+            v = (v << 1) | (p < nbits ? (d[p >> 3] >> (7 - (p & 7))) & 1u : 0u);
+        }
+        return v;
+    }
+    void skip(int n) { pos += n; if (pos > nbits) over = true; }
+    uint32_t ue() {  // 9.1
+        int z = 0;
+        while (!bit()) {
+            if (++z > 31 || over) { over = true; return 0; }
+        }
+        return z ? ((1u << z) - 1u) + u(z) : 0u;
+    }
+    int32_t se() {  // 9.1.1
+        const uint32_t k = ue();
+        return (k & 1u) ? (int32_t)((k + 1) >> 1) : -(int32_t)(k >> 1);
+    }
+    int32_t te(int range) { return range > 1 ? (int32_t)ue() : (int32_t)!bit(); }  // 9.1.2 (range >= 1)
+    bool aligned() const { return (pos & 7) == 0; }
+    // more_rbsp_data(): anything before the rbsp_stop_one_bit (the last 1 bit)
+    bool more_rbsp_data() const {
+        if (pos >= nbits) return false;
+        size_t last = nbits;
+        while (last > pos) {
+            const size_t p = last - 1;
+            if ((d[p >> 3] >> (7 - (p & 7))) & 1u) break;
+            last--;
+        }
+        return last > 0 && pos < last - 1;
+    }
+};
+
+// NAL payload without its header byte and emulation-prevention bytes (7.3.1, 7.4.1)
+void unescape(const uint8_t *p, size_t n, std::vector<uint8_t> &out) {
+    out.clear();
+    out.reserve(n);
+    int zeros = 0;
+    for (size_t i = 0; i < n; i++) {
+        if (zeros >= 2 && p[i] == 3) { zeros = 0; continue; }
+        out.push_back(p[i]);
+        zeros = p[i] == 0 ? zeros + 1 : 0;
+    }
+}
+
+// ------------------------------------------------------------------ CAVLC code tables
+// The standard's code tables, each entry a codeword as written in the standard.
+struct Code {
+    const char *bits;
+    int8_t a, b;  // coeff_token: TrailingOnes, TotalCoeff; total_zeros / run_before: value
+};
+
+// Table 9-5, 0 <= nC < 2
+const Code kCoeffToken0[] = {
+    {"1", 0, 0}, {"000101", 0, 1}, {"01", 1, 1}, {"00000111", 0, 2}, {"000100", 1, 2}, {"001", 2, 2},
+    {"000000111", 0, 3}, {"00000110", 1, 3}, {"0000101", 2, 3}, {"00011", 3, 3},
+    {"0000000111", 0, 4}, {"000000110", 1, 4}, {"00000101", 2, 4}, {"000011", 3, 4},
+    {"00000000111", 0, 5}, {"0000000110", 1, 5}, {"000000101", 2, 5}, {"0000100", 3, 5},
+    {"0000000001111", 0, 6}, {"00000000110", 1, 6}, {"0000000101", 2, 6}, {"00000100", 3, 6},
+    {"0000000001011", 0, 7}, {"0000000001110", 1, 7}, {"00000000101", 2, 7}, {"000000100", 3, 7},
+    {"0000000001000", 0, 8}, {"0000000001010", 1, 8}, {"0000000001101", 2, 8}, {"0000000100", 3, 8},
+    {"00000000001111", 0, 9}, {"00000000001110", 1, 9}, {"0000000001001", 2, 9}, {"00000000100", 3, 9},
+    {"00000000001011", 0, 10}, {"00000000001010", 1, 10}, {"00000000001101", 2, 10}, {"0000000001100", 3, 10},
+    {"000000000001111", 0, 11}, {"000000000001110", 1, 11}, {"00000000001001", 2, 11}, {"00000000001100", 3, 11},
+    {"000000000001011", 0, 12}, {"000000000001010", 1, 12}, {"000000000001101", 2, 12}, {"00000000001000", 3, 12},
+    {"0000000000001111", 0, 13}, {"000000000000001", 1, 13}, {"000000000001001", 2, 13}, {"000000000001100", 3, 13},
+    {"0000000000001011", 0, 14}, {"0000000000001110", 1, 14}, {"0000000000001101", 2, 14}, {"000000000001000", 3, 14},
+    {"0000000000000111", 0, 15}, {"0000000000001010", 1, 15}, {"0000000000001001", 2, 15}, {"0000000000001100", 3, 15},
+    {"0000000000000100", 0, 16}, {"0000000000000110", 1, 16}, {"0000000000000101", 2, 16}, {"0000000000001000", 3, 16},
+};
+// Table 9-5, 2 <= nC < 4
+const Code kCoeffToken1[] = {
+    {"11", 0, 0}, {"001011", 0, 1}, {"10", 1, 1}, {"000111", 0, 2}, {"00111", 1, 2}, {"011", 2, 2},
+    {"0000111", 0, 3}, {"001010", 1, 3}, {"001001", 2, 3}, {"0101", 3, 3},
+    {"00000111", 0, 4}, {"000110", 1, 4}, {"000101", 2, 4}, {"0100", 3, 4},
+    {"00000100", 0, 5}, {"0000110", 1, 5}, {"0000101", 2, 5}, {"00110", 3, 5},
+    {"000000111", 0, 6}, {"00000110", 1, 6}, {"00000101", 2, 6}, {"001000", 3, 6},
+    {"00000001111", 0, 7}, {"000000110", 1, 7}, {"000000101", 2, 7}, {"000100", 3, 7},
+    {"00000001011", 0, 8}, {"00000001110", 1, 8}, {"00000001101", 2, 8}, {"0000100", 3, 8},
+    {"000000001111", 0, 9}, {"00000001010", 1, 9}, {"00000001001", 2, 9}, {"000000100", 3, 9},
+    {"000000001011", 0, 10}, {"000000001110", 1, 10}, {"000000001101", 2, 10}, {"00000001100", 3, 10},
+    {"000000001000", 0, 11}, {"000000001010", 1, 11}, {"000000001001", 2, 11}, {"00000001000", 3, 11},
+    {"0000000001111", 0, 12}, {"0000000001110", 1, 12}, {"0000000001101", 2, 12}, {"000000001100", 3, 12},
+    {"0000000001011", 0, 13}, {"0000000001010", 1, 13}, {"0000000001001", 2, 13}, {"0000000001100", 3, 13},
+    {"0000000000111", 0, 14}, {"00000000001011", 1, 14}, {"0000000000110", 2, 14}, {"0000000001000", 3, 14},
+    {"00000000001001", 0, 15}, {"00000000001000", 1, 15}, {"00000000001010", 2, 15}, {"0000000000001", 3, 15},
+    {"00000000000111", 0, 16}, {"00000000000110", 1, 16}, {"00000000000101", 2, 16}, {"00000000000100", 3, 16},
+};
+// Table 9-5, 4 <= nC < 8
+const Code kCoeffToken2[] = {
+    {"1111", 0, 0}, {"001111", 0, 1}, {"1110", 1, 1}, {"001011", 0, 2}, {"01111", 1, 2}, {"1101", 2, 2},
+    {"001000", 0, 3}, {"01100", 1, 3}, {"01110", 2, 3}, {"1100", 3, 3},
+    {"0001111", 0, 4}, {"01010", 1, 4}, {"01011", 2, 4}, {"1011", 3, 4},
+    {"0001011", 0, 5}, {"01000", 1, 5}, {"01001", 2, 5}, {"1010", 3, 5},
+    {"0001001", 0, 6}, {"001110", 1, 6}, {"001101", 2, 6}, {"1001", 3, 6},
+    {"0001000", 0, 7}, {"001010", 1, 7}, {"001001", 2, 7}, {"1000", 3, 7},
+    {"00001111", 0, 8}, {"0001110", 1, 8}, {"0001101", 2, 8}, {"01101", 3, 8},
+    {"00001011", 0, 9}, {"00001110", 1, 9}, {"0001010", 2, 9}, {"001100", 3, 9},
+    {"000001111", 0, 10}, {"00001010", 1, 10}, {"00001101", 2, 10}, {"0001100", 3, 10},
+    {"000001011", 0, 11}, {"000001110", 1, 11}, {"00001001", 2, 11}, {"00001100", 3, 11},
+    {"000001000", 0, 12}, {"000001010", 1, 12}, {"000001101", 2, 12}, {"00001000", 3, 12},
+    {"0000001101", 0, 13}, {"000000111", 1, 13}, {"000001001", 2, 13}, {"000001100", 3, 13},
+    {"0000001001", 0, 14}, {"0000001100", 1, 14}, {"0000001011", 2, 14}, {"0000001010", 3, 14},
+    {"0000000101", 0, 15}, {"0000001000", 1, 15}, {"0000000111", 2, 15}, {"0000000110", 3, 15},
+    {"0000000001", 0, 16}, {"0000000100", 1, 16}, {"0000000011", 2, 16}, {"0000000010", 3, 16},
+};
+// Table 9-5, nC == -1 (chroma DC, 4:2:0)
+const Code kCoeffTokenChromaDc[] = {
+    {"01", 0, 0}, {"000111", 0, 1}, {"1", 1, 1}, {"000100", 0, 2}, {"000110", 1, 2}, {"001", 2, 2},
+    {"000011", 0, 3}, {"0000011", 1, 3}, {"0000010", 2, 3}, {"000101", 3, 3},
+    {"000010", 0, 4}, {"00000011", 1, 4}, {"00000010", 2, 4}, {"0000000", 3, 4},
+};
+// Tables 9-7 / 9-8: total_zeros of 4x4 blocks, tzVlcIndex 1..15 (one string per value)
+const char *const kTotalZeros[15][16] = {
+    {"1", "011", "010", "0011", "0010", "00011", "00010", "000011", "000010", "0000011", "0000010", "00000011",
+     "00000010", "000000011", "000000010", "000000001"},
+    {"111", "110", "101", "100", "011", "0101", "0100", "0011", "0010", "00011", "00010", "000011", "000010",
+     "000001", "000000"},
+    {"0101", "111", "110", "101", "0100", "0011", "100", "011", "0010", "00011", "00010", "000001", "00001",
+     "000000"},
+    {"00011", "111", "0101", "0100", "110", "101", "100", "0011", "011", "0010", "00010", "00001", "00000"},
+    {"0101", "0100", "0011", "111", "110", "101", "100", "011", "0010", "00001", "0001", "00000"},
+    {"000001", "00001", "111", "110", "101", "100", "011", "010", "0001", "001", "000000"},
+    {"000001", "00001", "101", "100", "011", "11", "010", "0001", "001", "000000"},
+    {"000001", "0001", "00001", "011", "11", "10", "010", "001", "000000"},
+    {"000001", "000000", "0001", "11", "10", "001", "01", "00001"},
+    {"00001", "00000", "001", "11", "10", "01", "0001"},
+    {"0000", "0001", "001", "010", "1", "011"},
+    {"0000", "0001", "01", "1", "001"},
+    {"000", "001", "1", "01"},
+    {"00", "01", "1"},
+    {"0", "1"},
+};
+// Table 9-9a: total_zeros of chroma DC (4:2:0), tzVlcIndex 1..3
+const char *const kTotalZerosChromaDc[3][4] = {{"1", "01", "001", "000"}, {"1", "01", "00"}, {"1", "0"}};
+// Table 9-10: run_before, zerosLeft 1..6 and > 6
+const char *const kRunBefore[7][15] = {
+    {"1", "0"},
+    {"1", "01", "00"},
+    {"11", "10", "01", "00"},
+    {"11", "10", "01", "001", "000"},
+    {"11", "10", "011", "010", "001", "000"},
+    {"11", "000", "001", "011", "010", "101", "100"},
+    {"111", "110", "101", "100", "011", "010", "001", "0001", "00001", "000001", "0000001", "00000001", "000000001",
+     "0000000001", "00000000001"},
+};
+// Table 9-4 (ChromaArrayType 1 or 2): codeNum of coded_block_pattern -> value, for
+// Intra_4x4 / Intra_8x8 MBs and for inter MBs
+const uint8_t kCbpIntra[48] = {47, 31, 15, 0,  23, 27, 29, 30, 7,  11, 13, 14, 39, 43, 45, 46,
+                               16, 3,  5,  10, 12, 19, 21, 26, 28, 35, 37, 42, 44, 1,  2,  4,
+                               8,  17, 18, 20, 24, 6,  9,  22, 25, 32, 33, 34, 36, 40, 38, 41};
+const uint8_t kCbpInter[48] = {0,  16, 1,  2,  4,  8,  32, 3,  5,  10, 12, 15, 47, 7,  11, 13,
+                               14, 6,  9,  31, 35, 37, 42, 44, 33, 34, 36, 40, 39, 43, 45, 46,
+                               17, 18, 20, 24, 19, 21, 26, 28, 23, 27, 29, 30, 22, 25, 38, 41};
+
+// A codeword table compiled for lookup: (code, length) pairs matched against the next bits.
+struct Vlc {
+    struct E { uint32_t code; int len; int a, b; };
+    std::vector<E> e;
+    void add(const char *s, int a, int b) {
+        uint32_t c = 0;
+        int n = 0;
+        for (; s[n]; n++) c = (c << 1) | (uint32_t)(s[n] == '1');
+        e.push_back({c, n, a, b});
+    }
+    // the entry whose codeword is next in the stream (consumed), or null
+    const E *read(Bits &bs) const {
+        const uint32_t w = bs.peek(16);
+        for (const E &x : e)
+            if ((w >> (16 - x.len)) == x.code) {
+                bs.skip(x.len);
+                return &x;
+            }
+        return nullptr;
+    }
+};
+
+struct Tables {
+    Vlc coeff_token[4];  // nC 0-1, 2-3, 4-7, chroma DC
+    Vlc total_zeros[15], total_zeros_cdc[3], run_before[7];
+    Tables() {
+        for (const Code &c : kCoeffToken0) coeff_token[0].add(c.bits, c.a, c.b);
+        for (const Code &c : kCoeffToken1) coeff_token[1].add(c.bits, c.a, c.b);
+        for (const Code &c : kCoeffToken2) coeff_token[2].add(c.bits, c.a, c.b);
+        for (const Code &c : kCoeffTokenChromaDc) coeff_token[3].add(c.bits, c.a, c.b);
+        for (int t = 0; t < 15; t++)
+            for (int v = 0; v < 16 && kTotalZeros[t][v]; v++) total_zeros[t].add(kTotalZeros[t][v], v, 0);
+        for (int t = 0; t < 3; t++)
+            for (int v = 0; v < 4 && kTotalZerosChromaDc[t][v]; v++) total_zeros_cdc[t].add(kTotalZerosChromaDc[t][v], v, 0);
+        for (int t = 0; t < 7; t++)
+            for (int v = 0; v < 15 && kRunBefore[t][v]; v++) run_before[t].add(kRunBefore[t][v], v, 0);
+    }
+};
+const Tables &tables() {
+    static const Tables t;
+    return t;
+}
+
+// ------------------------------------------------------------------ parameter sets
+struct Sps {
+    bool valid = false;
+    int profile_idc = 0, constraint3 = 0, level_idc = 0;
+    int chroma_format_idc = 1, log2_max_frame_num = 4, poc_type = 0, log2_max_poc_lsb = 4;
+    int max_num_ref_frames = 0, gaps = 0, width_mbs = 0, height_map_units = 0, frame_mbs_only = 1;
+    int direct_8x8_inference = 0;
+    int max_num_reorder_frames = -1;  // VUI bitstream_restriction, -1 when absent
+    int max_dec_frame_buffering = -1;
+};
+struct Pps {
+    bool valid = false;
+    int sps_id = 0, cabac = 0, bottom_field_pic_order = 0, num_slice_groups = 1;
+    int num_ref_idx_default[2] = {1, 1}, weighted_pred = 0, weighted_bipred_idc = 0, pic_init_qp = 26;
+    int chroma_qp_index_offset = 0, second_chroma_qp_index_offset = 0, deblocking_filter_control = 0;
+    int constrained_intra_pred = 0, redundant_pic_cnt_present = 0, transform_8x8_mode = 0;
+};
+
+int skip_scaling_list(Bits &bs, int n) {  // only to reject them with a clear message
+    (void)bs;
+    (void)n;
+    return h264g_set_error(H264G_E_ARG, "scaling matrices are not supported");
+}
+
+void skip_hrd(Bits &bs) {  // E.1.2
+    const uint32_t cpb_cnt = bs.ue() + 1;
+    bs.u(4);
+    bs.u(4);
+    for (uint32_t i = 0; i < cpb_cnt && !bs.over; i++) {
+        bs.ue();
+        bs.ue();
+        bs.u(1);
+    }
+    bs.u(5);
+    bs.u(5);
+    bs.u(5);
+    bs.u(5);
+}
+
+int parse_sps(Bits &bs, Sps *table) {  // 7.3.2.1.1
+    Sps s;
+    s.profile_idc = (int)bs.u(8);
+    const uint32_t cflags = bs.u(8);
+    s.constraint3 = (cflags >> 4) & 1;
+    s.level_idc = (int)bs.u(8);
+    const uint32_t id = bs.ue();
+    if (id > 31) return h264g_set_error(H264G_E_ARG, "seq_parameter_set_id %u", id);
+    const int p = s.profile_idc;
+    if (p == 100 || p == 110 || p == 122 || p == 244 || p == 44 || p == 83 || p == 86 || p == 118 || p == 128 ||
+        p == 138 || p == 139 || p == 134 || p == 135) {
+        s.chroma_format_idc = (int)bs.ue();
+        if (s.chroma_format_idc == 3) bs.u(1);
+        const uint32_t bdl = bs.ue(), bdc = bs.ue();
+        bs.u(1);  // qpprime_y_zero_transform_bypass_flag
+        if (bdl || bdc) return h264g_set_error(H264G_E_ARG, "only 8-bit samples are supported");
+        if (bs.u(1)) return skip_scaling_list(bs, 0);
+    }
+    if (s.chroma_format_idc != 1) return h264g_set_error(H264G_E_ARG, "only 4:2:0 is supported");
+    s.log2_max_frame_num = (int)bs.ue() + 4;
+    s.poc_type = (int)bs.ue();
+    if (s.poc_type == 0) s.log2_max_poc_lsb = (int)bs.ue() + 4;
+    else if (s.poc_type == 1) return h264g_set_error(H264G_E_ARG, "pic_order_cnt_type 1 is not supported");
+    s.max_num_ref_frames = (int)bs.ue();
+    s.gaps = (int)bs.u(1);
+    s.width_mbs = (int)bs.ue() + 1;
+    s.height_map_units = (int)bs.ue() + 1;
+    s.frame_mbs_only = (int)bs.u(1);
+    if (!s.frame_mbs_only) return h264g_set_error(H264G_E_ARG, "field coding (frame_mbs_only_flag 0) is not supported");
+    s.direct_8x8_inference = (int)bs.u(1);
+    if (bs.u(1)) {  // frame cropping: the planes stay whole, as in the reference
+        bs.ue();
+        bs.ue();
+        bs.ue();
+        bs.ue();
+    }
+    if (bs.u(1)) {  // vui_parameters (E.1.1), read for max_num_reorder_frames
+        if (bs.u(1)) {
+            if (bs.u(8) == 255) {
+                bs.u(16);
+                bs.u(16);
+            }
+        }
+        if (bs.u(1)) bs.u(1);
+        if (bs.u(1)) {
+            bs.u(3);
+            bs.u(1);
+            if (bs.u(1)) {
+                bs.u(8);
+                bs.u(8);
+                bs.u(8);
+            }
+        }
+        if (bs.u(1)) {
+            bs.ue();
+            bs.ue();
+        }
+        if (bs.u(1)) {
+            bs.u(32);
+            bs.u(32);
+            bs.u(1);
+        }
+        const uint32_t nal_hrd = bs.u(1);
+        if (nal_hrd) skip_hrd(bs);
+        const uint32_t vcl_hrd = bs.u(1);
+        if (vcl_hrd) skip_hrd(bs);
+        if (nal_hrd || vcl_hrd) bs.u(1);
+        bs.u(1);  // pic_struct_present_flag
+        if (bs.u(1)) {  // bitstream_restriction_flag
+            bs.u(1);
+            bs.ue();
+            bs.ue();
+            bs.ue();
+            bs.ue();
+            s.max_num_reorder_frames = (int)bs.ue();
+            s.max_dec_frame_buffering = (int)bs.ue();
+        }
+    }
+    if (bs.over) return h264g_set_error(H264G_E_ARG, "truncated SPS");
+    s.valid = true;
+    table[id] = s;
+    return 0;
+}
+
+int parse_pps(Bits &bs, Pps *table, const Sps *sps) {  // 7.3.2.2
+    Pps q;
+    const uint32_t id = bs.ue();
+    if (id > 255) return h264g_set_error(H264G_E_ARG, "pic_parameter_set_id %u", id);
+    q.sps_id = (int)bs.ue();
+    if (q.sps_id > 31 || !sps[q.sps_id].valid) return h264g_set_error(H264G_E_ARG, "PPS refers to a missing SPS");
+    q.cabac = (int)bs.u(1);
+    q.bottom_field_pic_order = (int)bs.u(1);
+    q.num_slice_groups = (int)bs.ue() + 1;
+    if (q.num_slice_groups != 1) return h264g_set_error(H264G_E_ARG, "slice groups are not supported");
+    q.num_ref_idx_default[0] = (int)bs.ue() + 1;
+    q.num_ref_idx_default[1] = (int)bs.ue() + 1;
+    q.weighted_pred = (int)bs.u(1);
+    q.weighted_bipred_idc = (int)bs.u(2);
+    q.pic_init_qp = 26 + bs.se();
+    bs.se();  // pic_init_qs
+    q.chroma_qp_index_offset = bs.se();
+    q.deblocking_filter_control = (int)bs.u(1);
+    q.constrained_intra_pred = (int)bs.u(1);
+    q.redundant_pic_cnt_present = (int)bs.u(1);
+    q.second_chroma_qp_index_offset = q.chroma_qp_index_offset;
+    if (bs.more_rbsp_data()) {
+        q.transform_8x8_mode = (int)bs.u(1);
+        if (bs.u(1)) return skip_scaling_list(bs, 0);
+        q.second_chroma_qp_index_offset = bs.se();
+    }
+    if (bs.over) return h264g_set_error(H264G_E_ARG, "truncated PPS");
+    q.valid = true;
+    table[id] = q;
+    return 0;
+}
+
+// ------------------------------------------------------------------ slice header
+struct Mmco {
+    int op, diff, lt_pic_num, lt_idx, max_lt_idx_plus1;
+};
+struct Slice {
+    int nal_ref_idc = 0, idr = 0;
+    int first_mb = 0, slice_type = 0, pps_id = 0, frame_num = 0, idr_pic_id = 0, poc_lsb = 0, delta_poc_bottom = 0;
+    int redundant_pic_cnt = 0, direct_spatial = 1, num_ref_idx[2] = {0, 0};
+    std::vector<std::pair<int, int>> mods[2];  // (modification_of_pic_nums_idc, value)
+    int luma_log2_wd = 0, chroma_log2_wd = 0, luma_flag_last[2] = {0, 0};
+    int16_t lw[2][32], lo[2][32], cw[2][32][2], co[2][32][2];
+    int no_output_of_prior_pics = 0, long_term_reference = 0, adaptive_marking = 0;
+    std::vector<Mmco> mmco;
+    int slice_qp = 26, disable_dbk = 0, alpha_div2 = 0, beta_div2 = 0;
+};
+
+int parse_pred_weight_table(Bits &bs, Slice &s) {  // 7.3.3.2
+    s.luma_log2_wd = (int)bs.ue();
+    s.chroma_log2_wd = (int)bs.ue();
+    for (int l = 0; l < (s.slice_type == H264G_SLICE_B ? 2 : 1); l++) {
+        for (int i = 0; i < s.num_ref_idx[l]; i++) {
+            const int lf = (int)bs.u(1);
+            s.luma_flag_last[l] = lf;
+            s.lw[l][i] = (int16_t)(1 << s.luma_log2_wd);
+            s.lo[l][i] = 0;
+            if (lf) {
+                s.lw[l][i] = (int16_t)bs.se();
+                s.lo[l][i] = (int16_t)bs.se();
+            }
+            const int cf = (int)bs.u(1);
+            for (int c = 0; c < 2; c++) {
+                s.cw[l][i][c] = (int16_t)(1 << s.chroma_log2_wd);
+                s.co[l][i][c] = 0;
+                if (cf) {
+                    s.cw[l][i][c] = (int16_t)bs.se();
+                    s.co[l][i][c] = (int16_t)bs.se();
+                }
+            }
+        }
+    }
+    return 0;
+}
+
+int parse_slice_header(Bits &bs, Slice &s, int nal_type, int nal_ref_idc, const Sps *sps_t, const Pps *pps_t,
+                       const Sps **sps_out, const Pps **pps_out) {  // 7.3.3
+    s.nal_ref_idc = nal_ref_idc;
+    s.idr = nal_type == 5;
+    s.first_mb = (int)bs.ue();
+    const uint32_t st = bs.ue() % 5;
+    if (st == 3 || st == 4) return h264g_set_error(H264G_E_ARG, "SP / SI slices are not supported");
+    s.slice_type = st == 0 ? H264G_SLICE_P : st == 1 ? H264G_SLICE_B : H264G_SLICE_I;
+    s.pps_id = (int)bs.ue();
+    if (s.pps_id > 255 || !pps_t[s.pps_id].valid) return h264g_set_error(H264G_E_ARG, "slice refers to a missing PPS");
+    const Pps &pps = pps_t[s.pps_id];
+    const Sps &sps = sps_t[pps.sps_id];
+    *sps_out = &sps;
+    *pps_out = &pps;
+    if (pps.cabac) return h264g_set_error(H264G_E_ARG, "CABAC (entropy_coding_mode_flag 1) is not supported");
+    s.frame_num = (int)bs.u(sps.log2_max_frame_num);
+    if (s.idr) s.idr_pic_id = (int)bs.ue();
+    if (sps.poc_type == 0) {
+        s.poc_lsb = (int)bs.u(sps.log2_max_poc_lsb);
+        if (pps.bottom_field_pic_order) s.delta_poc_bottom = bs.se();
+    }
+    if (pps.redundant_pic_cnt_present) s.redundant_pic_cnt = (int)bs.ue();
+    if (s.slice_type == H264G_SLICE_B) s.direct_spatial = (int)bs.u(1);
+    s.num_ref_idx[0] = s.slice_type == H264G_SLICE_I ? 0 : pps.num_ref_idx_default[0];
+    s.num_ref_idx[1] = s.slice_type == H264G_SLICE_B ? pps.num_ref_idx_default[1] : 0;
+    if (s.slice_type != H264G_SLICE_I && bs.u(1)) {
+        s.num_ref_idx[0] = (int)bs.ue() + 1;
+        if (s.slice_type == H264G_SLICE_B) s.num_ref_idx[1] = (int)bs.ue() + 1;
+    }
+    if (s.num_ref_idx[0] > 32 || s.num_ref_idx[1] > 32) return h264g_set_error(H264G_E_ARG, "num_ref_idx_active > 32");
+    for (int l = 0; l < (s.slice_type == H264G_SLICE_B ? 2 : s.slice_type == H264G_SLICE_P ? 1 : 0); l++) {
+        if (bs.u(1)) {  // ref_pic_list_modification_flag_lX (7.3.3.1)
+            for (int guard = 0; guard < 100; guard++) {
+                const int idc = (int)bs.ue();
+                if (idc == 3 || bs.over) break;
+                if (idc > 2) return h264g_set_error(H264G_E_ARG, "modification_of_pic_nums_idc %d", idc);
+                s.mods[l].push_back({idc, (int)bs.ue()});
+            }
+        }
+    }
+    if ((pps.weighted_pred && s.slice_type == H264G_SLICE_P) || (pps.weighted_bipred_idc == 1 && s.slice_type == H264G_SLICE_B))
+        parse_pred_weight_table(bs, s);
+    if (nal_ref_idc) {  // dec_ref_pic_marking (7.3.3.3)
+        if (s.idr) {
+            s.no_output_of_prior_pics = (int)bs.u(1);
+            s.long_term_reference = (int)bs.u(1);
+        } else if ((s.adaptive_marking = (int)bs.u(1))) {
+            for (int guard = 0; guard < 100; guard++) {
+                Mmco m{(int)bs.ue(), 0, 0, 0, 0};
+                if (m.op == 0 || bs.over) break;
+                if (m.op == 1 || m.op == 3) m.diff = (int)bs.ue() + 1;
+                if (m.op == 2) m.lt_pic_num = (int)bs.ue();
+                if (m.op == 3 || m.op == 6) m.lt_idx = (int)bs.ue();
+                if (m.op == 4) m.max_lt_idx_plus1 = (int)bs.ue();
+                if (m.op > 6) return h264g_set_error(H264G_E_ARG, "memory_management_control_operation %d", m.op);
+                s.mmco.push_back(m);
+            }
+        }
+    }
+    s.slice_qp = pps.pic_init_qp + bs.se();
+    if (s.slice_qp < 0 || s.slice_qp > 51) return h264g_set_error(H264G_E_ARG, "SliceQPY %d", s.slice_qp);
+    if (pps.deblocking_filter_control) {
+        s.disable_dbk = (int)bs.ue();
+        if (s.disable_dbk != 1) {
+            s.alpha_div2 = bs.se();
+            s.beta_div2 = bs.se();
+        }
+    }
+    if (bs.over) return h264g_set_error(H264G_E_ARG, "truncated slice header");
+    return 0;
+}
+
+// ------------------------------------------------------------------ macroblock layer
+// Per-MB state the parse of later MBs reads: the TotalCoeff of each 4x4 block (luma in
+// raster order of the MB's 4x4 grid, chroma [c][raster of the 2x2 grid]) for nC.
+struct MbNz {
+    uint8_t avail = 0;  // parsed in this picture
+    uint8_t skip = 0, pcm = 0;
+    uint8_t luma[16];
+    uint8_t chroma[2][4];
+};
+
+// luma4x4BlkIdx -> raster index of the MB's 4x4 grid
+inline int blk_raster(int b) { return (((b >> 3) & 1) * 2 + ((b >> 1) & 1)) * 4 + ((b >> 2) & 1) * 2 + (b & 1); }
+
+struct Residual {
+    // parsed coefficient blocks in scan order
+    int16_t dc[16];            // Intra16x16DCLevel
+    int16_t luma[16][16];      // by luma4x4BlkIdx (AC blocks: index 0..14 = scan 1..15)
+    int16_t luma8[4][64];      // 8x8 blocks (CAVLC: interleaved 4x4 parses)
+    int16_t cdc[2][4];
+    int16_t cac[2][4][15];
+};
+
+struct Parser;
+
+}  // namespace
+
+struct h264g_parser {
+    std::vector<uint8_t> data;  // the byte stream (a copy)
+    size_t pos = 0;             // next unread byte
+    Sps sps[32];
+    Pps pps[256];
+    const Sps *cur_sps = nullptr;
+    int width_mbs = 0, height_mbs = 0, dpb_slots = 0;
+    // DPB: frames kept for reference
+    struct Frame {
+        int slot = -1, frame_num = 0, poc = 0, long_term_idx = -1;
+        bool short_ref = false, long_ref = false;
+        std::vector<h264g_mb> mbs;  // final records (co-located picture of direct prediction)
+    };
+    std::vector<Frame> dpb;
+    int max_long_term_idx = -1;  // "no long-term frame indices"
+    // POC type 0 / 2 state (8.2.1)
+    int prev_poc_msb = 0, prev_poc_lsb = 0, prev_frame_num_offset = 0, prev_frame_num = 0;
+    bool prev_mmco5 = false;
+    int prev_ref_top_poc = 0;
+    // the picture handed out by the last h264g_parser_next
+    std::vector<h264g_mb> mbs;
+    std::vector<int16_t> levels;
+    h264g_mvfield *field = nullptr;
+    int pictures = 0;
+    ~h264g_parser() {
+        if (field) h264g_mvfield_destroy(field);
+    }
+};
+
+namespace {
+
+// ------------------------------------------------------------------ CAVLC residual block
+// residual_block_cavlc (7.3.5.3.2, 9.2): coeffLevel[startIdx..endIdx] of maxNumCoeff.
+int residual_block(Bits &bs, int nC, int start, int end, int max_num, int16_t *out, int *total) {
+    const Tables &T = tables();
+    for (int i = 0; i < max_num; i++) out[i] = 0;
+    int t1, tc;
+    if (nC >= 8) {  // 6-bit fixed-length code: 0000 11 = (0, 0), else TotalCoeff - 1 (4 bits), TrailingOnes (2 bits)
+        const uint32_t c = bs.u(6);
+        t1 = c == 3 ? 0 : (int)(c & 3);
+        tc = c == 3 ? 0 : (int)(c >> 2) + 1;
+    } else {
+        const Vlc::E *ct = T.coeff_token[nC == -1 ? 3 : nC < 2 ? 0 : nC < 4 ? 1 : 2].read(bs);
+        if (!ct) return h264g_set_error(H264G_E_ARG, "invalid coeff_token");
+        t1 = ct->a;
+        tc = ct->b;
+    }
+    *total = tc;
+    if (tc == 0) return 0;
+    if (tc > max_num || t1 > tc) return h264g_set_error(H264G_E_ARG, "coeff_token beyond the block");
+    int level[16];
+    int suffix_length = (tc > 10 && t1 < 3) ? 1 : 0;
+    for (int i = 0; i < tc; i++) {
+        if (i < t1) {
+            level[i] = bs.bit() ? -1 : 1;
+            continue;
+        }
+        int prefix = 0;  // level_prefix (9.2.2.1)
+        while (!bs.bit()) {
+            if (++prefix > 31 || bs.over) return h264g_set_error(H264G_E_ARG, "invalid level_prefix");
+        }
+        int suffix_size = suffix_length;
+        if (prefix == 14 && suffix_length == 0) suffix_size = 4;
+        if (prefix >= 15) suffix_size = prefix - 3;
+        int code = (std::min(15, prefix) << suffix_length) + (suffix_size ? (int)bs.u(suffix_size) : 0);
+        if (prefix >= 15 && suffix_length == 0) code += 15;
+        if (prefix >= 16) code += (1 << (prefix - 3)) - 4096;
+        if (i == t1 && t1 < 3) code += 2;
+        level[i] = (code & 1) ? (-code - 1) >> 1 : (code + 2) >> 1;
+        if (suffix_length == 0) suffix_length = 1;
+        if (std::abs(level[i]) > (3 << (suffix_length - 1)) && suffix_length < 6) suffix_length++;
+    }
+    int zeros_left = 0;
+    if (tc < end - start + 1) {
+        const Vlc::E *tz = max_num == 4 ? T.total_zeros_cdc[tc - 1].read(bs) : T.total_zeros[tc - 1].read(bs);
+        if (!tz) return h264g_set_error(H264G_E_ARG, "invalid total_zeros");
+        zeros_left = tz->a;
+    }
+    int run[16];
+    for (int i = 0; i < tc - 1; i++) {
+        run[i] = 0;
+        if (zeros_left > 0) {
+            const Vlc::E *rb = T.run_before[std::min(zeros_left, 7) - 1].read(bs);
+            if (!rb) return h264g_set_error(H264G_E_ARG, "invalid run_before");
+            run[i] = rb->a;
+            zeros_left -= run[i];
+            if (zeros_left < 0) return h264g_set_error(H264G_E_ARG, "run_before beyond total_zeros");
+        }
+    }
+    run[tc - 1] = zeros_left;
+    int coeff = -1;
+    for (int i = tc - 1; i >= 0; i--) {
+        coeff += run[i] + 1;
+        if (start + coeff > end) return h264g_set_error(H264G_E_ARG, "coefficients beyond the block");
+        out[start + coeff] = (int16_t)level[i];
+    }
+    return bs.over ? h264g_set_error(H264G_E_ARG, "truncated residual") : 0;
+}
+
+// mb_type tables (7-11, 7-13, 7-14): partition shape and prediction of each partition
+// (bit 0 = list 0, bit 1 = list 1) of P and B inter MB types
+struct InterType { int shape; int pred[2]; };
+const InterType kPType[5] = {{H264G_PART_16x16, {1, 0}}, {H264G_PART_16x8, {1, 1}}, {H264G_PART_8x16, {1, 1}},
+                             {H264G_PART_8x8, {0, 0}}, {H264G_PART_8x8, {0, 0}}};
+const InterType kBType[23] = {
+    {H264G_PART_8x8, {0, 0}},   // B_Direct_16x16
+    {H264G_PART_16x16, {1, 0}}, {H264G_PART_16x16, {2, 0}}, {H264G_PART_16x16, {3, 0}},
+    {H264G_PART_16x8, {1, 1}},  {H264G_PART_8x16, {1, 1}},  {H264G_PART_16x8, {2, 2}}, {H264G_PART_8x16, {2, 2}},
+    {H264G_PART_16x8, {1, 2}},  {H264G_PART_8x16, {1, 2}},  {H264G_PART_16x8, {2, 1}}, {H264G_PART_8x16, {2, 1}},
+    {H264G_PART_16x8, {1, 3}},  {H264G_PART_8x16, {1, 3}},  {H264G_PART_16x8, {2, 3}}, {H264G_PART_8x16, {2, 3}},
+    {H264G_PART_16x8, {3, 1}},  {H264G_PART_8x16, {3, 1}},  {H264G_PART_16x8, {3, 2}}, {H264G_PART_8x16, {3, 2}},
+    {H264G_PART_16x8, {3, 3}},  {H264G_PART_8x16, {3, 3}},
+    {H264G_PART_8x8, {0, 0}},   // B_8x8
+};
+// sub_mb_type (7-17, 7-18): shape and prediction; B index 0 is B_Direct_8x8
+const int kPSubShape[4] = {H264G_SUB_8x8, H264G_SUB_8x4, H264G_SUB_4x8, H264G_SUB_4x4};
+const int kBSubShape[13] = {H264G_SUB_8x8, H264G_SUB_8x8, H264G_SUB_8x8, H264G_SUB_8x8, H264G_SUB_8x4,
+                            H264G_SUB_4x8, H264G_SUB_8x4, H264G_SUB_4x8, H264G_SUB_8x4, H264G_SUB_4x8,
+                            H264G_SUB_4x4, H264G_SUB_4x4, H264G_SUB_4x4};
+const int kBSubPred[13] = {0, 1, 2, 3, 1, 1, 2, 2, 3, 3, 1, 2, 3};
+
+// 4x4 raster offsets (x, y in 4x4 units) of the partitions of a shape: quadrant q of
+// 16x8 / 8x16 and the sub-partitions of a sub shape
+inline int sub_parts(int shape) { return shape == H264G_SUB_8x8 ? 1 : shape == H264G_SUB_4x4 ? 4 : 2; }
+inline void sub_origin(int shape, int k, int &x, int &y) {
+    x = shape == H264G_SUB_4x8 ? k : shape == H264G_SUB_4x4 ? (k & 1) : 0;
+    y = shape == H264G_SUB_8x4 ? k : shape == H264G_SUB_4x4 ? (k >> 1) : 0;
+}
+
+struct PicState {
+    h264g_parser *p;
+    const Sps *sps;
+    const Pps *pps;
+    Slice *sl;
+    int W, H;
+    std::vector<MbNz> nz;
+    std::vector<int16_t> *levels;
+};
+
+// TotalCoeff of the 4x4 luma block at (x, y) in 4x4 units of MB mb, or -1 if unavailable
+// (outside the picture or not yet parsed); the neighbour MB is located from the
+// current one (single-slice pictures: every parsed MB is available)
+int nz_luma(const PicState &st, int mb, int bx, int by) {
+    int mx = mb % st.W, my = mb / st.W;
+    if (bx < 0) { bx += 4; mx--; }
+    if (by < 0) { by += 4; my--; }
+    if (mx < 0 || my < 0) return -1;
+    const MbNz &n = st.nz[my * st.W + mx];
+    if (!n.avail) return -1;
+    if (n.skip) return 0;
+    if (n.pcm) return 16;
+    return n.luma[by * 4 + bx];
+}
+int nz_chroma(const PicState &st, int mb, int c, int bx, int by) {
+    int mx = mb % st.W, my = mb / st.W;
+    if (bx < 0) { bx += 2; mx--; }
+    if (by < 0) { by += 2; my--; }
+    if (mx < 0 || my < 0) return -1;
+    const MbNz &n = st.nz[my * st.W + mx];
+    if (!n.avail) return -1;
+    if (n.skip) return 0;
+    if (n.pcm) return 16;
+    return n.chroma[c][by * 2 + bx];
+}
+inline int nc_of(int a, int b) {  // 9.2.1
+    if (a >= 0 && b >= 0) return (a + b + 1) >> 1;
+    if (a >= 0) return a;
+    if (b >= 0) return b;
+    return 0;
+}
+
+// Appends the masks and values of the MB's blocks in the order h264g.h documents.
+void emit_levels(const Residual &r, bool i16, bool t8, int cbp, std::vector<int16_t> &lv) {
+    std::vector<uint16_t> masks;
+    std::vector<int16_t> vals;
+    auto block = [&](const int16_t *c, int n, int first_pos) {  // n coefficients at scan first_pos..
+        uint16_t m = 0;
+        for (int i = 0; i < n; i++)
+            if (c[i]) {
+                m |= (uint16_t)(1u << (first_pos + i));
+                vals.push_back(c[i]);
+            }
+        masks.push_back(m);
+    };
+    if (i16) block(r.dc, 16, 0);
+    for (int q = 0; q < 4; q++) {
+        if (!((cbp >> q) & 1)) continue;
+        if (t8) {
+            for (int k = 0; k < 4; k++) block(&r.luma8[q][16 * k], 16, 0);
+        } else {
+            for (int k = 0; k < 4; k++) {
+                if (i16) block(r.luma[4 * q + k], 15, 1);
+                else block(r.luma[4 * q + k], 16, 0);
+            }
+        }
+    }
+    if ((cbp >> 4) >= 1)
+        for (int c = 0; c < 2; c++) block(r.cdc[c], 4, 0);
+    if ((cbp >> 4) == 2)
+        for (int c = 0; c < 2; c++)
+            for (int b = 0; b < 4; b++) block(r.cac[c][b], 15, 1);
+    for (uint16_t m : masks) lv.push_back((int16_t)m);
+    lv.insert(lv.end(), vals.begin(), vals.end());
+}
+
+// residual() (7.3.5.3) for CAVLC, 4:2:0
+int parse_residual(Bits &bs, PicState &st, int mb, bool i16, bool t8, int cbp, Residual &r, MbNz &nzc) {
+    memset(&r, 0, sizeof(r));
+    memset(nzc.luma, 0, sizeof(nzc.luma));
+    memset(nzc.chroma, 0, sizeof(nzc.chroma));
+    int total = 0;
+    auto luma_nc = [&](int blk) {
+        const int rr = blk_raster(blk), bx = rr & 3, by = rr >> 2;
+        const int a = bx > 0 ? nzc.luma[by * 4 + bx - 1] : nz_luma(st, mb, -1, by);
+        const int b = by > 0 ? nzc.luma[(by - 1) * 4 + bx] : nz_luma(st, mb, bx, -1);
+        return nc_of(a, b);
+    };
+    if (i16) {
+        if (int rc = residual_block(bs, luma_nc(0), 0, 15, 16, r.dc, &total)) return rc;
+    }
+    for (int q = 0; q < 4; q++) {
+        for (int k = 0; k < 4; k++) {
+            const int blk = 4 * q + k;
+            if (!((cbp >> q) & 1)) continue;
+            int16_t c[16];
+            const int nc = luma_nc(blk);
+            if (i16) {
+                if (int rc = residual_block(bs, nc, 0, 14, 15, c, &total)) return rc;
+                memcpy(r.luma[blk], c, sizeof(c));
+            } else {
+                if (int rc = residual_block(bs, nc, 0, 15, 16, c, &total)) return rc;
+                memcpy(r.luma[blk], c, sizeof(c));
+                if (t8)
+                    for (int i = 0; i < 16; i++) r.luma8[q][4 * i + k] = c[i];
+            }
+            nzc.luma[blk_raster(blk)] = (uint8_t)total;
+        }
+    }
+    const int cc = cbp >> 4;
+    for (int c = 0; c < 2 && cc; c++) {
+        if (int rc = residual_block(bs, -1, 0, 3, 4, r.cdc[c], &total)) return rc;
+    }
+    for (int c = 0; c < 2; c++)
+        for (int b = 0; b < 4; b++) {
+            if (cc != 2) continue;
+            const int bx = b & 1, by = b >> 1;
+            const int a = bx > 0 ? nzc.chroma[c][by * 2 + bx - 1] : nz_chroma(st, mb, c, -1, by);
+            const int bb = by > 0 ? nzc.chroma[c][(by - 1) * 2 + bx] : nz_chroma(st, mb, c, bx, -1);
+            int16_t co[16];
+            if (int rc = residual_block(bs, nc_of(a, bb), 0, 14, 15, co, &total)) return rc;
+            memcpy(r.cac[c][b], co, 15 * sizeof(int16_t));
+            nzc.chroma[c][b] = (uint8_t)total;
+        }
+    return 0;
+}
+
+// One macroblock_layer (7.3.5) of a P / B / I slice into m (mvd and intra syntax beside it).
+int parse_mb(Bits &bs, PicState &st, int mb, int &qp, h264g_mb &m, int16_t (&mvd)[2][16][2], uint8_t (&isyn)[16][2],
+             bool &has_mvd) {
+    const Slice &sl = *st.sl;
+    const Pps &pps = *st.pps;
+    MbNz &nzc = st.nz[mb];
+    memset(&m, 0, sizeof(m));
+    memset(mvd, 0, sizeof(mvd));
+    has_mvd = false;
+    for (int l = 0; l < 2; l++)
+        for (int q = 0; q < 4; q++) m.ref_idx[l][q] = -1;
+    m.level_offset = (uint32_t)st.levels->size();
+    int mb_type = (int)bs.ue();
+    bool intra = sl.slice_type == H264G_SLICE_I;
+    if (sl.slice_type == H264G_SLICE_P) {
+        if (mb_type >= 5) { intra = true; mb_type -= 5; }
+    } else if (sl.slice_type == H264G_SLICE_B) {
+        if (mb_type >= 23) { intra = true; mb_type -= 23; }
+    }
+    if (intra && mb_type > 25) return h264g_set_error(H264G_E_ARG, "mb_type %d", mb_type);
+    if (!intra && sl.slice_type == H264G_SLICE_P && mb_type > 4) return h264g_set_error(H264G_E_ARG, "mb_type %d", mb_type);
+    nzc.avail = 1;
+    nzc.skip = 0;
+    nzc.pcm = 0;
+    if (intra && mb_type == 25) {  // I_PCM
+        while (!bs.aligned()) bs.bit();
+        m.mb_type = H264G_MB_IPCM;
+        m.qp_y = (uint8_t)qp;
+        for (int i = 0; i < 384; i++) st.levels->push_back((int16_t)bs.u(8));
+        nzc.pcm = 1;
+        return bs.over ? h264g_set_error(H264G_E_ARG, "truncated I_PCM") : 0;
+    }
+    bool t8 = false, i16 = false;
+    int cbp = 0;
+    bool no_sub_lt8 = true;
+    const int nref[2] = {sl.num_ref_idx[0], sl.num_ref_idx[1]};
+    if (intra) {
+        if (mb_type == 0) {  // I_NxN
+            if (pps.transform_8x8_mode) t8 = bs.u(1);
+            const int n = t8 ? 4 : 16;
+            for (int b = 0; b < n; b++) {
+                isyn[b][0] = (uint8_t)bs.u(1);
+                isyn[b][1] = isyn[b][0] ? 0 : (uint8_t)bs.u(3);
+            }
+            m.mb_type = t8 ? H264G_MB_I8x8 : H264G_MB_I4x4;
+        } else {
+            i16 = true;
+            m.mb_type = H264G_MB_I16x16;
+            m.intra16x16_pred_mode = (uint8_t)((mb_type - 1) % 4);
+            cbp = (((mb_type - 1) / 4) % 3) << 4 | (mb_type >= 13 ? 15 : 0);
+        }
+        m.intra_chroma_pred_mode = (uint8_t)bs.ue();
+        if (m.intra_chroma_pred_mode > 3) return h264g_set_error(H264G_E_ARG, "intra_chroma_pred_mode");
+    } else if (sl.slice_type == H264G_SLICE_P || mb_type != 0) {
+        const bool is_b = sl.slice_type == H264G_SLICE_B;
+        const InterType &it = is_b ? kBType[mb_type] : kPType[mb_type];
+        m.mb_type = is_b ? H264G_MB_B : H264G_MB_P;
+        m.part_shape = (uint8_t)it.shape;
+        has_mvd = true;
+        if (it.shape == H264G_PART_8x8) {  // sub_mb_pred (7.3.5.2)
+            int sub[4];
+            for (int q = 0; q < 4; q++) {
+                sub[q] = (int)bs.ue();
+                if (sub[q] > (is_b ? 12 : 3)) return h264g_set_error(H264G_E_ARG, "sub_mb_type %d", sub[q]);
+                m.sub_part_shape[q] = (uint8_t)(is_b ? kBSubShape[sub[q]] : kPSubShape[sub[q]]);
+                if (is_b && sub[q] == 0) {
+                    m.flags |= H264G_MBF_DIRECT(q);
+                    if (!st.sps->direct_8x8_inference) no_sub_lt8 = false;
+                } else if (sub_parts(m.sub_part_shape[q]) > 1) {
+                    no_sub_lt8 = false;
+                }
+            }
+            auto pred = [&](int q) { return is_b ? kBSubPred[sub[q]] : 1; };
+            for (int l = 0; l < 2; l++)
+                for (int q = 0; q < 4; q++) {
+                    if (!(pred(q) & (1 << l))) continue;
+                    m.ref_idx[l][q] = 0;
+                    if (nref[l] > 1 && !(mb_type == 4 && !is_b)) m.ref_idx[l][q] = (int8_t)bs.te(nref[l] - 1);
+                    if (m.ref_idx[l][q] >= nref[l]) return h264g_set_error(H264G_E_ARG, "ref_idx out of range");
+                }
+            for (int l = 0; l < 2; l++)
+                for (int q = 0; q < 4; q++) {
+                    if (!(pred(q) & (1 << l))) continue;
+                    const int shape = m.sub_part_shape[q];
+                    for (int k = 0; k < sub_parts(shape); k++) {
+                        int x, y;
+                        sub_origin(shape, k, x, y);
+                        const int r = ((q >> 1) * 2 + y) * 4 + (q & 1) * 2 + x;
+                        mvd[l][r][0] = (int16_t)bs.se();
+                        mvd[l][r][1] = (int16_t)bs.se();
+                    }
+                }
+        } else {  // mb_pred of an inter MB (7.3.5.1)
+            const int np = it.shape == H264G_PART_16x16 ? 1 : 2;
+            auto quads = [&](int k, int *q) -> int {  // quadrants of partition k
+                if (np == 1) { q[0] = 0; q[1] = 1; q[2] = 2; q[3] = 3; return 4; }
+                if (it.shape == H264G_PART_16x8) { q[0] = 2 * k; q[1] = 2 * k + 1; return 2; }
+                q[0] = k; q[1] = k + 2;
+                return 2;
+            };
+            for (int l = 0; l < 2; l++)
+                for (int k = 0; k < np; k++) {
+                    if (!(it.pred[k] & (1 << l))) continue;
+                    int ri = 0;
+                    if (nref[l] > 1) ri = bs.te(nref[l] - 1);
+                    if (ri >= nref[l]) return h264g_set_error(H264G_E_ARG, "ref_idx out of range");
+                    int q[4];
+                    const int nq = quads(k, q);
+                    for (int j = 0; j < nq; j++) m.ref_idx[l][q[j]] = (int8_t)ri;
+                }
+            for (int l = 0; l < 2; l++)
+                for (int k = 0; k < np; k++) {
+                    if (!(it.pred[k] & (1 << l))) continue;
+                    const int r = np == 1 ? 0 : it.shape == H264G_PART_16x8 ? 8 * k : 2 * k;
+                    mvd[l][r][0] = (int16_t)bs.se();
+                    mvd[l][r][1] = (int16_t)bs.se();
+                }
+        }
+    } else {  // B_Direct_16x16
+        m.mb_type = H264G_MB_B;
+        m.part_shape = H264G_PART_8x8;
+        m.flags |= H264G_MBF_DIRECT_ALL;
+        if (!st.sps->direct_8x8_inference) no_sub_lt8 = false;
+    }
+    if (!i16) {
+        const uint32_t code = bs.ue();
+        if (code > 47) return h264g_set_error(H264G_E_ARG, "coded_block_pattern");
+        const bool nxn = m.mb_type == H264G_MB_I4x4 || m.mb_type == H264G_MB_I8x8;
+        cbp = nxn ? kCbpIntra[code] : kCbpInter[code];  // CodedBlockPatternLuma | Chroma << 4
+        if ((cbp & 15) && pps.transform_8x8_mode && !nxn && no_sub_lt8 && !intra) t8 = bs.u(1);
+    }
+    if (t8) m.flags |= H264G_MBF_T8x8;
+    m.cbp = (uint8_t)cbp;
+    if ((cbp & 15) || (cbp >> 4) || i16) {
+        // out-of-range deltas are clipped, as the reference does (H264MacroBlock.cpp:1116-1119)
+        const int dq = std::min(25, std::max(-26, bs.se()));
+        qp = (qp + dq + 52) % 52;  // 7-37
+        Residual r;
+        if (int rc = parse_residual(bs, st, mb, i16, t8, cbp, r, nzc)) return rc;
+        emit_levels(r, i16, t8, cbp, *st.levels);
+    } else {
+        memset(nzc.luma, 0, sizeof(nzc.luma));
+        memset(nzc.chroma, 0, sizeof(nzc.chroma));
+    }
+    m.qp_y = (uint8_t)qp;
+    return bs.over ? h264g_set_error(H264G_E_ARG, "truncated macroblock") : 0;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ DPB and pictures
+namespace {
+
+using Frame = h264g_parser::Frame;
+
+int max_frame_num(const Sps &s) { return 1 << s.log2_max_frame_num; }
+
+// FrameNumWrap (8-27) of a short-term frame relative to the current frame_num
+int frame_num_wrap(const Frame &f, int cur_frame_num, int max_fn) {
+    return f.frame_num > cur_frame_num ? f.frame_num - max_fn : f.frame_num;
+}
+
+// 8.2.1: PicOrderCnt of the current frame (types 0 and 2)
+int decode_poc(h264g_parser *p, const Sps &sps, const Slice &sl) {
+    if (sps.poc_type == 0) {
+        if (sl.idr) {
+            p->prev_poc_msb = 0;
+            p->prev_poc_lsb = 0;
+        } else if (p->prev_mmco5) {
+            p->prev_poc_msb = 0;
+            p->prev_poc_lsb = p->prev_ref_top_poc;
+        }
+        const int max_lsb = 1 << sps.log2_max_poc_lsb;
+        int msb;
+        if (sl.poc_lsb < p->prev_poc_lsb && p->prev_poc_lsb - sl.poc_lsb >= max_lsb / 2) msb = p->prev_poc_msb + max_lsb;
+        else if (sl.poc_lsb > p->prev_poc_lsb && sl.poc_lsb - p->prev_poc_lsb > max_lsb / 2) msb = p->prev_poc_msb - max_lsb;
+        else msb = p->prev_poc_msb;
+        const int top = msb + sl.poc_lsb, bottom = top + sl.delta_poc_bottom;
+        if (sl.nal_ref_idc) {  // prevPicOrderCntMsb / Lsb come from the previous reference picture
+            p->prev_poc_msb = msb;
+            p->prev_poc_lsb = sl.poc_lsb;
+        }
+        return std::min(top, bottom);
+    }
+    // type 2
+    int offset;
+    if (sl.idr) offset = 0;
+    else if (p->prev_frame_num > sl.frame_num) offset = p->prev_frame_num_offset + max_frame_num(sps);
+    else offset = p->prev_frame_num_offset;
+    if (p->prev_mmco5 && !sl.idr) offset = p->prev_frame_num > sl.frame_num ? max_frame_num(sps) : 0;
+    const int poc = sl.idr ? 0 : (sl.nal_ref_idc ? 2 * (offset + sl.frame_num) : 2 * (offset + sl.frame_num) - 1);
+    p->prev_frame_num_offset = offset;
+    return poc;
+}
+
+// 8.2.4.2.1 / 8.2.4.2.3 / 8.2.4.2.4 (frames): initial lists of DPB indices
+void init_lists(h264g_parser *p, const Sps &sps, const Slice &sl, int cur_poc, std::vector<int> lists[2]) {
+    const int max_fn = max_frame_num(sps);
+    std::vector<int> st, lt;
+    for (int i = 0; i < (int)p->dpb.size(); i++) {
+        if (p->dpb[i].short_ref) st.push_back(i);
+        else if (p->dpb[i].long_ref) lt.push_back(i);
+    }
+    std::sort(lt.begin(), lt.end(), [&](int a, int b) { return p->dpb[a].long_term_idx < p->dpb[b].long_term_idx; });
+    lists[0].clear();
+    lists[1].clear();
+    if (sl.slice_type == H264G_SLICE_P) {
+        std::sort(st.begin(), st.end(), [&](int a, int b) {
+            return frame_num_wrap(p->dpb[a], sl.frame_num, max_fn) > frame_num_wrap(p->dpb[b], sl.frame_num, max_fn);
+        });
+        lists[0] = st;
+        lists[0].insert(lists[0].end(), lt.begin(), lt.end());
+    } else if (sl.slice_type == H264G_SLICE_B) {
+        std::vector<int> before, after;
+        for (int i : st) (p->dpb[i].poc < cur_poc ? before : after).push_back(i);
+        std::sort(before.begin(), before.end(), [&](int a, int b) { return p->dpb[a].poc > p->dpb[b].poc; });
+        std::sort(after.begin(), after.end(), [&](int a, int b) { return p->dpb[a].poc < p->dpb[b].poc; });
+        lists[0] = before;
+        lists[0].insert(lists[0].end(), after.begin(), after.end());
+        lists[0].insert(lists[0].end(), lt.begin(), lt.end());
+        lists[1] = after;
+        lists[1].insert(lists[1].end(), before.begin(), before.end());
+        lists[1].insert(lists[1].end(), lt.begin(), lt.end());
+        if (lists[1].size() > 1 && lists[0] == lists[1]) std::swap(lists[1][0], lists[1][1]);
+    }
+}
+
+// 8.2.4.3: modification of a list (entries are DPB indices, -1 = no reference picture)
+int modify_list(h264g_parser *p, const Sps &sps, const Slice &sl, int l, std::vector<int> &list) {
+    const int max_fn = max_frame_num(sps), n = sl.num_ref_idx[l];
+    list.resize(n, -1);
+    int pred = sl.frame_num;
+    int ref_idx = 0;
+    for (const auto &md : sl.mods[l]) {
+        int target = -1;
+        if (md.first < 2) {
+            const int abs_diff = md.second + 1;
+            int no_wrap = md.first == 0 ? pred - abs_diff : pred + abs_diff;
+            if (no_wrap < 0) no_wrap += max_fn;
+            if (no_wrap >= max_fn) no_wrap -= max_fn;
+            pred = no_wrap;
+            const int pic_num = no_wrap > sl.frame_num ? no_wrap - max_fn : no_wrap;
+            for (int i = 0; i < (int)p->dpb.size(); i++)
+                if (p->dpb[i].short_ref && frame_num_wrap(p->dpb[i], sl.frame_num, max_fn) == pic_num) target = i;
+        } else {
+            for (int i = 0; i < (int)p->dpb.size(); i++)
+                if (p->dpb[i].long_ref && p->dpb[i].long_term_idx == md.second) target = i;
+        }
+        if (target < 0) return h264g_set_error(H264G_E_ARG, "reference list modification names a missing picture");
+        if (ref_idx >= n) return h264g_set_error(H264G_E_ARG, "too many reference list modifications");
+        list.insert(list.begin() + ref_idx, target);
+        ref_idx++;
+        int w = ref_idx;
+        for (int r = ref_idx; r < (int)list.size(); r++)
+            if (list[r] != target) list[w++] = list[r];
+        list.resize(w);
+        list.resize(n, -1);
+    }
+    list.resize(n, -1);
+    return 0;
+}
+
+// 8.2.5: marking after decoding the current picture (which becomes DPB entry `cur`)
+void mark(h264g_parser *p, const Sps &sps, const Slice &sl, Frame &cur) {
+    const int max_fn = max_frame_num(sps);
+    p->prev_mmco5 = false;
+    if (sl.idr) {
+        for (Frame &f : p->dpb) f.short_ref = f.long_ref = false;
+        if (sl.long_term_reference) {
+            cur.long_ref = true;
+            cur.long_term_idx = 0;
+            p->max_long_term_idx = 0;
+        } else {
+            cur.short_ref = true;
+            p->max_long_term_idx = -1;
+        }
+        return;
+    }
+    bool cur_long = false;
+    if (sl.adaptive_marking) {
+        for (const Mmco &m : sl.mmco) {
+            switch (m.op) {
+            case 1: {
+                const int pn = sl.frame_num - m.diff;
+                for (Frame &f : p->dpb)
+                    if (f.short_ref && frame_num_wrap(f, sl.frame_num, max_fn) == pn) f.short_ref = false;
+                break;
+            }
+            case 2:
+                for (Frame &f : p->dpb)
+                    if (f.long_ref && f.long_term_idx == m.lt_pic_num) f.long_ref = false;
+                break;
+            case 3: {
+                const int pn = sl.frame_num - m.diff;
+                for (Frame &f : p->dpb)
+                    if (f.long_ref && f.long_term_idx == m.lt_idx) f.long_ref = false;
+                for (Frame &f : p->dpb)
+                    if (f.short_ref && frame_num_wrap(f, sl.frame_num, max_fn) == pn) {
+                        f.short_ref = false;
+                        f.long_ref = true;
+                        f.long_term_idx = m.lt_idx;
+                    }
+                break;
+            }
+            case 4:
+                p->max_long_term_idx = m.max_lt_idx_plus1 - 1;
+                for (Frame &f : p->dpb)
+                    if (f.long_ref && f.long_term_idx > p->max_long_term_idx) f.long_ref = false;
+                break;
+            case 5:
+                for (Frame &f : p->dpb) f.short_ref = f.long_ref = false;
+                p->max_long_term_idx = -1;
+                p->prev_mmco5 = true;
+                break;
+            case 6:
+                for (Frame &f : p->dpb)
+                    if (f.long_ref && f.long_term_idx == m.lt_idx) f.long_ref = false;
+                cur.long_ref = true;
+                cur.long_term_idx = m.lt_idx;
+                cur_long = true;
+                break;
+            }
+        }
+    } else {  // sliding window (8.2.5.3)
+        int n_short = 0, n_long = 0;
+        for (const Frame &f : p->dpb) {
+            n_short += f.short_ref;
+            n_long += f.long_ref;
+        }
+        if (n_short + n_long >= std::max(sps.max_num_ref_frames, 1) && n_short > 0) {
+            int oldest = -1;
+            for (int i = 0; i < (int)p->dpb.size(); i++)
+                if (p->dpb[i].short_ref &&
+                    (oldest < 0 || frame_num_wrap(p->dpb[i], sl.frame_num, max_fn) <
+                                       frame_num_wrap(p->dpb[oldest], sl.frame_num, max_fn)))
+                    oldest = i;
+            p->dpb[oldest].short_ref = false;
+        }
+    }
+    if (!cur_long) cur.short_ref = true;
+}
+
+// the next NAL unit: [begin, end) of its bytes (header included), or false at the end
+bool next_nal(h264g_parser *p, size_t &begin, size_t &end) {
+    const std::vector<uint8_t> &d = p->data;
+    size_t i = p->pos;
+    // start code prefix 0x000001
+    while (i + 3 <= d.size() && !(d[i] == 0 && d[i + 1] == 0 && d[i + 2] == 1)) i++;
+    if (i + 3 > d.size()) {
+        p->pos = d.size();
+        return false;
+    }
+    begin = i + 3;
+    size_t j = begin;
+    while (j + 3 <= d.size() && !(d[j] == 0 && d[j + 1] == 0 && (d[j + 2] == 1 || (d[j + 2] == 0 && j + 3 < d.size() && d[j + 3] == 1))))
+        j++;
+    end = j + 3 <= d.size() ? j : d.size();
+    while (end > begin && d[end - 1] == 0) end--;  // trailing_zero_8bits
+    p->pos = j + 3 <= d.size() ? j : d.size();
+    return true;
+}
+
+}  // namespace
+
+extern "C" int32_t h264g_parser_create(const uint8_t *data, uint64_t len, h264g_parser **out) {
+    if (!data || !out) return h264g_set_error(H264G_E_ARG, "NULL argument");
+    *out = nullptr;
+    h264g_parser *p = new (std::nothrow) h264g_parser();
+    if (!p) return h264g_set_error(H264G_E_NOMEM, "parser");
+    p->data.assign(data, data + len);
+    *out = p;
+    return H264G_OK;
+}
+
+extern "C" int32_t h264g_parser_destroy(h264g_parser *p) {
+    delete p;
+    return H264G_OK;
+}
+
+extern "C" int32_t h264g_parser_next(h264g_parser *p, h264g_parsed_picture *out) {
+    if (!p || !out) return h264g_set_error(H264G_E_ARG, "NULL argument");
+    memset(out, 0, sizeof(*out));
+    std::vector<uint8_t> rbsp;
+    size_t begin, end;
+    while (next_nal(p, begin, end)) {
+        if (end <= begin) continue;
+        const uint8_t hdr = p->data[begin];
+        const int nal_type = hdr & 31, nal_ref_idc = (hdr >> 5) & 3;
+        if (nal_type != 1 && nal_type != 5 && nal_type != 7 && nal_type != 8) {
+            if (nal_type >= 2 && nal_type <= 4) return h264g_set_error(H264G_E_ARG, "data partitioning is not supported");
+            continue;  // SEI, AUD, end of sequence / stream, filler ...
+        }
+        unescape(&p->data[begin + 1], end - begin - 1, rbsp);
+        Bits bs;
+        bs.d = rbsp.data();
+        bs.nbits = rbsp.size() * 8;
+        if (nal_type == 7) {
+            if (int rc = parse_sps(bs, p->sps)) return rc;
+            continue;
+        }
+        if (nal_type == 8) {
+            if (int rc = parse_pps(bs, p->pps, p->sps)) return rc;
+            continue;
+        }
+        // ---- a slice: the whole picture (one slice per picture)
+        Slice sl;
+        const Sps *sps = nullptr;
+        const Pps *pps = nullptr;
+        if (int rc = parse_slice_header(bs, sl, nal_type, nal_ref_idc, p->sps, p->pps, &sps, &pps)) return rc;
+        if (sl.first_mb != 0) return h264g_set_error(H264G_E_ARG, "several slices per picture are not supported");
+        if (sl.redundant_pic_cnt) continue;
+        if (sl.slice_type == H264G_SLICE_B && !sl.direct_spatial)
+            return h264g_set_error(H264G_E_ARG, "temporal direct prediction is not supported");
+        const int W = sps->width_mbs, H = sps->height_map_units;
+        if (p->cur_sps && (p->width_mbs != W || p->height_mbs != H))
+            return h264g_set_error(H264G_E_ARG, "the picture size changes inside the stream");
+        if (!p->cur_sps) {
+            p->width_mbs = W;
+            p->height_mbs = H;
+            p->dpb_slots = std::max(sps->max_num_ref_frames, 1) + 1;
+            if (h264g_mvfield_create(W, H, &p->field)) return h264g_set_error(H264G_E_NOMEM, "motion field");
+        }
+        p->cur_sps = sps;
+        const int max_fn = max_frame_num(*sps);
+        if (sl.idr) {
+            p->dpb.clear();
+            p->prev_frame_num = 0;
+        } else if (sl.frame_num != p->prev_frame_num && sl.frame_num != (p->prev_frame_num + 1) % max_fn) {
+            return h264g_set_error(H264G_E_ARG, "gaps in frame_num are not supported");
+        }
+        const int poc = decode_poc(p, *sps, sl);
+        // reference lists (DPB indices), then slots
+        std::vector<int> lists[2];
+        init_lists(p, *sps, sl, poc, lists);
+        for (int l = 0; l < (sl.slice_type == H264G_SLICE_B ? 2 : sl.slice_type == H264G_SLICE_P ? 1 : 0); l++) {
+            if (int rc = modify_list(p, *sps, sl, l, lists[l])) return rc;
+        }
+        h264g_pic_desc &d = out->desc;
+        d.slice_type = sl.slice_type;
+        d.structure = H264G_FRAME;
+        d.constrained_intra_pred_flag = pps->constrained_intra_pred;
+        d.disable_deblocking_filter_idc = sl.disable_dbk;
+        d.filter_offset_a = sl.alpha_div2 * 2;
+        d.filter_offset_b = sl.beta_div2 * 2;
+        d.chroma_qp_index_offset = pps->chroma_qp_index_offset;
+        d.second_chroma_qp_index_offset = pps->second_chroma_qp_index_offset;
+        d.weighted_pred_flag = pps->weighted_pred;
+        d.weighted_bipred_idc = pps->weighted_bipred_idc;
+        d.luma_log2_weight_denom = sl.luma_log2_wd;
+        d.chroma_log2_weight_denom = sl.chroma_log2_wd;
+        d.poc = poc;
+        for (int l = 0; l < 2; l++) {
+            d.num_ref_idx_active[l] = sl.num_ref_idx[l];
+            d.luma_weight_flag_last[l] = sl.luma_flag_last[l];
+            for (int r = 0; r < 32; r++) {
+                d.ref_slot[l][r] = -1;
+                d.ref_poc[l][r] = 0;
+                d.ref_long_term[l][r] = 0;
+                if (r < (int)lists[l].size() && lists[l][r] >= 0) {
+                    const Frame &f = p->dpb[lists[l][r]];
+                    d.ref_slot[l][r] = f.slot;
+                    d.ref_poc[l][r] = f.poc;
+                    d.ref_long_term[l][r] = f.long_ref ? 1 : 0;
+                }
+                const bool explicit_wp = (pps->weighted_pred && sl.slice_type == H264G_SLICE_P) ||
+                                         (pps->weighted_bipred_idc == 1 && sl.slice_type == H264G_SLICE_B);
+                if (explicit_wp && r < sl.num_ref_idx[l]) {
+                    d.luma_weight[l][r] = sl.lw[l][r];
+                    d.luma_offset[l][r] = sl.lo[l][r];
+                    for (int c = 0; c < 2; c++) {
+                        d.chroma_weight[l][r][c] = sl.cw[l][r][c];
+                        d.chroma_offset[l][r][c] = sl.co[l][r][c];
+                    }
+                }
+            }
+        }
+        // a free slot: not held by a reference frame
+        int slot = -1;
+        for (int s = 0; s < p->dpb_slots && slot < 0; s++) {
+            bool used = false;
+            for (const Frame &f : p->dpb) used |= (f.short_ref || f.long_ref) && f.slot == s;
+            if (!used) slot = s;
+        }
+        if (slot < 0) return h264g_set_error(H264G_E_ARG, "DPB full: more reference frames than max_num_ref_frames");
+        d.dst_slot = slot;
+        // ---- slice data
+        PicState st{p, sps, pps, &sl, W, H, std::vector<MbNz>((size_t)W * H), &p->levels};
+        p->mbs.assign((size_t)W * H, h264g_mb());
+        p->levels.clear();
+        h264g_mvfield_reset(p->field);
+        if (sl.slice_type == H264G_SLICE_B) {
+            if (lists[1].empty() || lists[1][0] < 0) return h264g_set_error(H264G_E_ARG, "B slice without RefPicList1[0]");
+            const Frame &col = p->dpb[lists[1][0]];
+            if (int rc = h264g_mvfield_colocated(p->field, col.mbs.data(), col.short_ref ? 1 : 0)) return rc;
+        } else {
+            h264g_mvfield_colocated(p->field, nullptr, 0);
+        }
+        int qp = sl.slice_qp, mb = 0;
+        const int n_mbs = W * H;
+        bool more = true;
+        int16_t mvd[2][16][2];
+        uint8_t isyn[16][2];
+        while (more && mb < n_mbs) {
+            if (sl.slice_type != H264G_SLICE_I) {
+                const uint32_t run = bs.ue();
+                if (run > (uint32_t)(n_mbs - mb)) return h264g_set_error(H264G_E_ARG, "mb_skip_run past the picture");
+                for (uint32_t k = 0; k < run; k++, mb++) {
+                    h264g_mb &m = p->mbs[mb];
+                    memset(&m, 0, sizeof(m));
+                    for (int l = 0; l < 2; l++)
+                        for (int q = 0; q < 4; q++) m.ref_idx[l][q] = -1;
+                    m.qp_y = (uint8_t)qp;
+                    m.level_offset = (uint32_t)p->levels.size();
+                    if (sl.slice_type == H264G_SLICE_P) {
+                        m.mb_type = H264G_MB_P_SKIP;
+                        m.ref_idx[0][0] = m.ref_idx[0][1] = m.ref_idx[0][2] = m.ref_idx[0][3] = 0;
+                    } else {
+                        m.mb_type = H264G_MB_B;
+                        m.part_shape = H264G_PART_8x8;
+                        m.flags = H264G_MBF_SKIP | H264G_MBF_DIRECT_ALL;
+                    }
+                    MbNz &n = st.nz[mb];
+                    n.avail = 1;
+                    n.skip = 1;
+                    if (int rc = h264g_mvfield_derive(p->field, (uint32_t)mb, &m, nullptr)) return rc;
+                }
+                if (run > 0) more = bs.more_rbsp_data();
+                if (!more || mb >= n_mbs) break;
+            }
+            h264g_mb &m = p->mbs[mb];
+            bool has_mvd = false;
+            if (int rc = parse_mb(bs, st, mb, qp, m, mvd, isyn, has_mvd)) {
+                char msg[512];
+                snprintf(msg, sizeof(msg), "%s", h264g_last_error());
+                return h264g_set_error(rc, "picture %d MB %d: %s", p->pictures, mb, msg);
+            }
+            if (m.mb_type == H264G_MB_I4x4 || m.mb_type == H264G_MB_I8x8) {
+                if (int rc = h264g_mvfield_intra_modes(p->field, (uint32_t)mb, &m, &isyn[0][0], pps->constrained_intra_pred))
+                    return rc;
+            }
+            const bool all_direct = (m.flags & H264G_MBF_DIRECT_ALL) == H264G_MBF_DIRECT_ALL;
+            if (int rc = h264g_mvfield_derive(p->field, (uint32_t)mb, &m, has_mvd && !all_direct ? &mvd[0][0][0] : nullptr))
+                return rc;
+            mb++;
+            more = bs.more_rbsp_data();
+        }
+        if (mb != n_mbs) return h264g_set_error(H264G_E_ARG, "picture %d: the slice covers %d of %d MBs", p->pictures, mb, n_mbs);
+        if (bs.over) return h264g_set_error(H264G_E_ARG, "picture %d: slice data truncated", p->pictures);
+        // ---- DPB bookkeeping: the current frame, then marking
+        Frame cur;
+        cur.slot = slot;
+        cur.frame_num = sl.frame_num;
+        cur.poc = poc;
+        if (sl.nal_ref_idc) {
+            mark(p, *sps, sl, cur);
+            if (p->prev_mmco5) {  // 8.2.1: after memory_management_control_operation 5
+                cur.frame_num = 0;
+                p->prev_ref_top_poc = 0;
+                cur.poc = 0;
+            }
+            cur.mbs = p->mbs;
+            p->dpb.erase(std::remove_if(p->dpb.begin(), p->dpb.end(), [](const Frame &f) { return !f.short_ref && !f.long_ref; }),
+                         p->dpb.end());
+            p->dpb.push_back(cur);
+            p->prev_frame_num = cur.frame_num;
+        } else {
+            p->prev_frame_num = sl.frame_num;
+        }
+        out->mbs = p->mbs.data();
+        out->n_mbs = (uint32_t)p->mbs.size();
+        out->levels = p->levels.data();
+        out->n_levels = p->levels.size();
+        out->width_mbs = W;
+        out->height_mbs = H;
+        out->dpb_slots = p->dpb_slots;
+        out->max_num_reorder_frames = sps->max_num_reorder_frames;
+        out->idr = sl.idr;
+        out->nal_ref_idc = sl.nal_ref_idc;
+        out->frame_num = sl.frame_num;
+        out->profile_idc = sps->profile_idc;
+        out->constraint_set3 = sps->constraint3;
+        out->level_idc = sps->level_idc;
+        p->pictures++;
+        return 1;
+    }
+    return 0;
+}

@@ -10,9 +10,10 @@ H264_DECODE_ERROR_CODE_FILE_END)`` ends the stream (H264VideoDecoder.cpp:340-436
 The output reorder follows CH264PicturesGOP::getOneOutPicture
 (H264PicturesGOP.cpp:89-165).
 
-Reconstruction runs on the GPU through libh264g; ``source`` supplies the parsed
-per-picture syntax (the generator, or a syntax file written by ``syntax_io``), in
-place of the reference's bitstream parser.
+Reconstruction runs on the GPU through libh264g; ``source`` is an H.264 Annex B
+byte stream (a ``.h264`` / ``.264`` path, bytes, or a ``bitstream.Bitstream``:
+CAVLC frame pictures, parsed on the host by the library's front end), or parsed
+per-picture syntax (the generator, or a syntax file written by ``syntax_io``).
 """
 from __future__ import annotations
 
@@ -20,6 +21,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
+from ._lib import PicDesc
 from .engine import Engine
 
 # H264_DECODE_ERROR_CODE (H264CommonFunc.h:477-484)
@@ -81,6 +83,16 @@ class _OutputReorder:
         return self.buf.pop(i)
 
 
+def _with_last(items):
+    """(item, is_last) for each item of an iterator."""
+    it = iter(items)
+    cur = next(it, None)
+    while cur is not None:
+        nxt = next(it, None)
+        yield cur, nxt is None
+        cur = nxt
+
+
 class H264VideoDecoder:
     def __init__(self, engine: Engine | None = None, device: int = 0):
         self._engine = engine
@@ -105,10 +117,21 @@ class H264VideoDecoder:
         return -1 if self.m_output_frame_callback(pic, self.m_userData, 0) != 0 else 0
 
     def open(self, source) -> int:
-        """Decodes every picture of ``source`` (an object with width_mbs, height_mbs,
-        dpb_slots, num_pictures, num_reorder_frames and picture(i) -> (desc, mbs,
-        levels), e.g. Generator or syntax_io.SyntaxFile, or a path to a syntax file)."""
-        if isinstance(source, str):
+        """Decodes every picture of ``source``: an Annex B byte stream (path ending in
+        .h264 / .264 / .26l, bytes, or a bitstream.Bitstream), or an object with
+        width_mbs, height_mbs, dpb_slots, num_pictures, num_reorder_frames and
+        picture(i) -> (desc, mbs, levels), e.g. Generator or syntax_io.SyntaxFile, or a
+        path to a syntax file."""
+        from .bitstream import Bitstream
+        if isinstance(source, (bytes, bytearray)) or (
+                isinstance(source, str) and source.lower().endswith((".h264", ".264", ".26l"))):
+            try:
+                source = Bitstream(source)
+            except OSError:
+                if self.m_output_frame_callback is not None:
+                    return self.m_output_frame_callback(None, self.m_userData, H264_DECODE_ERROR_CODE_OPEN_FILE_FAILED)
+                return -1
+        elif isinstance(source, str):
             from .syntax_io import SyntaxFile
             try:
                 source = SyntaxFile(source)
@@ -123,15 +146,26 @@ class H264VideoDecoder:
         reorder = _OutputReorder(getattr(source, "num_reorder_frames", 0))
         ret = 0
         stopped = False
+        byte_stream = isinstance(source, Bitstream)
+        if byte_stream:
+            items = _with_last((p.desc, p.mbs, p.levels, p.idr) for p in source.pictures())
+        else:
+            items = ((source.picture(i) + (source.is_idr(i),), False) for i in range(source.num_pictures))
         try:
-            for i in range(source.num_pictures):
-                desc, mbs, levels = source.picture(i)
+            for i, ((desc, mbs, levels, idr), last) in enumerate(items):
+                if last and byte_stream:
+                    # the reference hands the stream's last picture to the callback without
+                    # finishing it (H264VideoDecoder.cpp:353-361 skips
+                    # end_decode_the_picture_and_get_a_new_empty_picture, which runs the
+                    # deblocking filter, H264PictureBase.cpp:707): it is output unfiltered
+                    desc = PicDesc.from_buffer_copy(desc)
+                    desc.disable_deblocking_filter_idc = 1
                 pic = eng.begin(sid, desc, mbs, levels)
                 eng.submit(pic)
                 eng.sync()
                 y, cb, cr = eng.readback(sid, desc.dst_slot)  # host-visible before any callback
                 eng.release(pic)
-                if i > 0 and source.is_idr(i):
+                if i > 0 and idr:
                     while True:  # an IDR flushes the reorder buffer (H264VideoDecoder.cpp:383-410)
                         out = reorder.flush_one()
                         if out is None:

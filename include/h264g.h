@@ -462,6 +462,39 @@ int32_t h264g_mvfield_intra_modes(h264g_mvfield *field, uint32_t mb_addr, h264g_
  * records that carry H264G_MBF_DIRECT bits (mvd may then be NULL when all four do). */
 int32_t h264g_mvfield_colocated(h264g_mvfield *field, const h264g_mb *col_mbs, int32_t col_short_term);
 
+/* ------------------------------------------------------------ bitstream front end
+ * Annex B byte stream of frame pictures coded with CAVLC, one slice per picture (the
+ * reference's CAVLC sample streams): NAL units, SPS / PPS, slice headers, slice data and
+ * the decoded picture buffer (POC types 0 and 2, sliding-window and adaptive reference
+ * marking, reference list initialisation and modification, DPB slots).  Replaces the
+ * reference's parsing and picture management on the way to reconstruction:
+ * CH264NalUnit / CH264SPS / CH264PPS / CH264SliceHeader (H264NalUnit.cpp:122,
+ * H264SliceHeader.cpp:289-509), CH264SliceData::slice_data (H264SliceData.cpp:386-487),
+ * CH264MacroBlock::macroblock_layer (H264MacroBlock.cpp:912-1139),
+ * CH264ResidualBlockCavlc (H264ResidualBlockCavlc.cpp) and H264RefPicList.cpp.
+ * Refused with H264G_E_ARG: CABAC, field pictures and MBAFF, several slices per picture,
+ * slice groups, scaling matrices, formats other than 8-bit 4:2:0, SP / SI slices,
+ * temporal direct prediction, POC type 1, gaps in frame_num.  Host only. */
+typedef struct h264g_parser h264g_parser;
+typedef struct h264g_parsed_picture {
+    h264g_pic_desc desc;          /* ready for h264g_picture_begin (dst_slot, ref_slot ...) */
+    const h264g_mb *mbs;          /* final records (motion vectors and intra modes derived) */
+    uint32_t n_mbs;
+    const int16_t *levels;
+    uint64_t n_levels;            /* the three stay valid until the next h264g_parser_next */
+    int32_t width_mbs, height_mbs;
+    int32_t dpb_slots;            /* slots the stream needs: max_num_ref_frames + 1        */
+    int32_t max_num_reorder_frames;  /* from the VUI, -1 when absent                       */
+    int32_t idr, nal_ref_idc, frame_num;
+    int32_t profile_idc, constraint_set3, level_idc;
+} h264g_parsed_picture;
+/* The parser keeps a copy of the byte stream. */
+int32_t h264g_parser_create(const uint8_t *data, uint64_t len, h264g_parser **out);
+int32_t h264g_parser_destroy(h264g_parser *parser);
+/* The next picture in decoding order: 1 with *out filled, 0 at the end of the stream,
+ * negative on an error (h264g_last_error names the picture and MB). */
+int32_t h264g_parser_next(h264g_parser *parser, h264g_parsed_picture *out);
+
 int32_t h264g_gen_default_params(int32_t config, h264g_gen_params *out);
 int32_t h264g_gen_create(const h264g_gen_params *params, h264g_gen **out);
 int32_t h264g_gen_destroy(h264g_gen *gen);
