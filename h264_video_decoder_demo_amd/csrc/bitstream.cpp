@@ -1,5 +1,6 @@
-// Annex B byte-stream front end for frame pictures coded with CAVLC: NAL units, SPS / PPS,
-// slice headers, slice data (macroblock_layer, mb_pred / sub_mb_pred, residual_block_cavlc)
+// Annex B byte-stream front end for frame pictures coded with CAVLC or CABAC: NAL units,
+// SPS / PPS, slice headers, slice data (macroblock_layer, mb_pred / sub_mb_pred,
+// residual_block_cavlc / residual_block_cabac)
 // and the decoded picture buffer (POC, reference marking, reference picture lists, DPB
 // slots), producing the h264g_pic_desc / h264g_mb / level records the GPU stages consume.
 // It replaces the reference's parsing and picture management on the path to
@@ -7,12 +8,13 @@
 // (H264NalUnit.cpp:122, H264SPS.cpp, H264PPS.cpp, H264SliceHeader.cpp:289-509),
 // CH264SliceData::slice_data and CH264MacroBlock::macroblock_layer
 // (H264SliceData.cpp:386-487, H264MacroBlock.cpp:912-1139), CH264ResidualBlockCavlc
-// (H264ResidualBlockCavlc.cpp) and the reference list / marking code (H264RefPicList.cpp).
+// (H264ResidualBlockCavlc.cpp), CH264Cabac (H264Cabac.cpp) and the reference list /
+// marking code (H264RefPicList.cpp).
 // Motion vectors and intra prediction modes are derived by the motion field
 // (h264g_mvfield_*, mvpred.cpp) from the parsed mvd and prev / rem syntax.
 //
-// Refused with H264G_E_ARG (none occur in the reference's CAVLC sample streams of one
-// slice per picture): CABAC, field pictures and MBAFF, several slices per picture or
+// Refused with H264G_E_ARG (none occur in the reference's progressive sample streams of
+// one slice per picture): field pictures and MBAFF, several slices per picture or
 // slice groups, scaling matrices, chroma formats other than 4:2:0, bit depths above 8,
 // SP / SI slices, temporal direct prediction, POC type 1, gaps in frame_num.
 #include <algorithm>
@@ -415,7 +417,7 @@ struct Slice {
     int16_t lw[2][32], lo[2][32], cw[2][32][2], co[2][32][2];
     int no_output_of_prior_pics = 0, long_term_reference = 0, adaptive_marking = 0;
     std::vector<Mmco> mmco;
-    int slice_qp = 26, disable_dbk = 0, alpha_div2 = 0, beta_div2 = 0;
+    int cabac_init_idc = 0, slice_qp = 26, disable_dbk = 0, alpha_div2 = 0, beta_div2 = 0;
 };
 
 int parse_pred_weight_table(Bits &bs, Slice &s) {  // 7.3.3.2
@@ -459,7 +461,6 @@ int parse_slice_header(Bits &bs, Slice &s, int nal_type, int nal_ref_idc, const 
     const Sps &sps = sps_t[pps.sps_id];
     *sps_out = &sps;
     *pps_out = &pps;
-    if (pps.cabac) return h264g_set_error(H264G_E_ARG, "CABAC (entropy_coding_mode_flag 1) is not supported");
     s.frame_num = (int)bs.u(sps.log2_max_frame_num);
     if (s.idr) s.idr_pic_id = (int)bs.ue();
     if (sps.poc_type == 0) {
@@ -503,6 +504,10 @@ int parse_slice_header(Bits &bs, Slice &s, int nal_type, int nal_ref_idc, const 
                 s.mmco.push_back(m);
             }
         }
+    }
+    if (pps.cabac && s.slice_type != H264G_SLICE_I) {
+        s.cabac_init_idc = (int)bs.ue();
+        if (s.cabac_init_idc > 2) return h264g_set_error(H264G_E_ARG, "cabac_init_idc %d", s.cabac_init_idc);
     }
     s.slice_qp = pps.pic_init_qp + bs.se();
     if (s.slice_qp < 0 || s.slice_qp > 51) return h264g_set_error(H264G_E_ARG, "SliceQPY %d", s.slice_qp);
@@ -947,6 +952,701 @@ int parse_mb(Bits &bs, PicState &st, int mb, int &qp, h264g_mb &m, int16_t (&mvd
     return bs.over ? h264g_set_error(H264G_E_ARG, "truncated macroblock") : 0;
 }
 
+// A skipped MB's record (P_Skip, or B_Skip: direct prediction of all four quadrants).
+void skip_record(h264g_mb &m, int slice_type, int qp, uint32_t level_offset) {
+    memset(&m, 0, sizeof(m));
+    for (int l = 0; l < 2; l++)
+        for (int q = 0; q < 4; q++) m.ref_idx[l][q] = -1;
+    m.qp_y = (uint8_t)qp;
+    m.level_offset = level_offset;
+    if (slice_type == H264G_SLICE_P) {
+        m.mb_type = H264G_MB_P_SKIP;
+        m.ref_idx[0][0] = m.ref_idx[0][1] = m.ref_idx[0][2] = m.ref_idx[0][3] = 0;
+    } else {
+        m.mb_type = H264G_MB_B;
+        m.part_shape = H264G_PART_8x8;
+        m.flags = H264G_MBF_SKIP | H264G_MBF_DIRECT_ALL;
+    }
+}
+
+// Intra prediction modes and motion vectors of a parsed MB (the motion field, in MB order).
+int finish_mb(PicState &st, int mb, h264g_mb &m, const int16_t *mvd, const uint8_t *isyn) {
+    h264g_parser *p = st.p;
+    if (m.mb_type == H264G_MB_I4x4 || m.mb_type == H264G_MB_I8x8) {
+        if (int rc = h264g_mvfield_intra_modes(p->field, (uint32_t)mb, &m, isyn, st.pps->constrained_intra_pred)) return rc;
+    }
+    const bool all_direct = (m.flags & H264G_MBF_DIRECT_ALL) == H264G_MBF_DIRECT_ALL;
+    return h264g_mvfield_derive(p->field, (uint32_t)mb, &m, all_direct ? nullptr : mvd);
+}
+
+int mb_error(const PicState &st, int mb, int rc) {
+    char msg[512];
+    snprintf(msg, sizeof(msg), "%s", h264g_last_error());
+    return h264g_set_error(rc, "picture %d MB %d: %s", st.p->pictures, mb, msg);
+}
+
+// slice_data() (7.3.4) with CAVLC: mb_skip_run, then macroblock_layer; mb = MBs covered.
+int parse_slice_data_cavlc(Bits &bs, PicState &st, int &mb) {
+    h264g_parser *p = st.p;
+    const Slice &sl = *st.sl;
+    const int n_mbs = st.W * st.H;
+    int qp = sl.slice_qp;
+    bool more = true;
+    int16_t mvd[2][16][2];
+    uint8_t isyn[16][2];
+    while (more && mb < n_mbs) {
+        if (sl.slice_type != H264G_SLICE_I) {
+            const uint32_t run = bs.ue();
+            if (run > (uint32_t)(n_mbs - mb)) return h264g_set_error(H264G_E_ARG, "mb_skip_run past the picture");
+            for (uint32_t k = 0; k < run; k++, mb++) {
+                h264g_mb &m = p->mbs[mb];
+                skip_record(m, sl.slice_type, qp, (uint32_t)p->levels.size());
+                st.nz[mb].avail = 1;
+                st.nz[mb].skip = 1;
+                if (int rc = finish_mb(st, mb, m, nullptr, nullptr)) return mb_error(st, mb, rc);
+            }
+            if (run > 0) more = bs.more_rbsp_data();
+            if (!more || mb >= n_mbs) break;
+        }
+        h264g_mb &m = p->mbs[mb];
+        bool has_mvd = false;
+        if (int rc = parse_mb(bs, st, mb, qp, m, mvd, isyn, has_mvd)) return mb_error(st, mb, rc);
+        if (int rc = finish_mb(st, mb, m, has_mvd ? &mvd[0][0][0] : nullptr, &isyn[0][0])) return mb_error(st, mb, rc);
+        mb++;
+        more = bs.more_rbsp_data();
+    }
+    return 0;
+}
+
+// ------------------------------------------------------------------ CABAC (9.3)
+#include "cabac_init.inc"
+
+// Table 9-44 and the state transitions of 9.3.3.2.1.1
+const uint8_t kRangeLps[64][4] = {
+    {128, 176, 208, 240}, {128, 167, 197, 227}, {128, 158, 187, 216}, {123, 150, 178, 205}, {116, 142, 169, 195},
+    {111, 135, 160, 185}, {105, 128, 152, 175}, {100, 122, 144, 166}, {95, 116, 137, 158}, {90, 110, 130, 150},
+    {85, 104, 123, 142},  {81, 99, 117, 135},   {77, 94, 111, 128},   {73, 89, 105, 122},   {69, 85, 100, 116},
+    {66, 80, 95, 110},    {62, 76, 90, 104},    {59, 72, 86, 99},     {56, 69, 81, 94},     {53, 65, 77, 89},
+    {51, 62, 73, 85},     {48, 59, 69, 80},     {46, 56, 66, 76},     {43, 53, 63, 72},     {41, 50, 59, 69},
+    {39, 48, 56, 65},     {37, 45, 54, 62},     {35, 43, 51, 59},     {33, 41, 48, 56},     {32, 39, 46, 53},
+    {30, 37, 43, 50},     {29, 35, 41, 48},     {27, 33, 39, 45},     {26, 31, 37, 43},     {24, 30, 35, 41},
+    {23, 28, 33, 39},     {22, 27, 32, 37},     {21, 26, 30, 35},     {20, 24, 29, 33},     {19, 23, 27, 31},
+    {18, 22, 26, 30},     {17, 21, 25, 28},     {16, 20, 23, 27},     {15, 19, 22, 25},     {14, 18, 21, 24},
+    {14, 17, 20, 23},     {13, 16, 19, 22},     {12, 15, 18, 21},     {12, 14, 17, 20},     {11, 14, 16, 19},
+    {11, 13, 15, 18},     {10, 12, 15, 17},     {10, 12, 14, 16},     {9, 11, 13, 15},      {9, 11, 12, 14},
+    {8, 10, 12, 14},      {8, 9, 11, 13},       {7, 9, 11, 12},       {7, 9, 10, 12},       {7, 8, 10, 11},
+    {6, 8, 9, 11},        {6, 7, 9, 10},        {6, 7, 8, 9},         {2, 2, 2, 2}};
+const uint8_t kTransLps[64] = {0,  0,  1,  2,  2,  4,  4,  5,  6,  7,  8,  9,  9,  11, 11, 12, 13, 13, 15, 15, 16, 16,
+                               18, 18, 19, 19, 21, 21, 22, 22, 23, 24, 24, 25, 26, 26, 27, 27, 28, 29, 29, 30, 30, 30,
+                               31, 32, 32, 33, 33, 33, 34, 34, 35, 35, 35, 36, 36, 36, 37, 37, 37, 38, 38, 63};
+// ctxIdxInc of significant_coeff_flag / last_significant_coeff_flag of 8x8 blocks in frame
+// MBs by scan position (Table 9-43)
+const uint8_t kSig8x8[63] = {0,  1,  2,  3,  4,  5,  5,  4,  4,  3,  3,  4,  4,  4,  5,  5,  4,  4,  4,  4,  3,
+                             3,  6,  7,  7,  7,  8,  9,  10, 9,  8,  7,  7,  6,  11, 12, 13, 11, 6,  7,  8,  9,
+                             14, 10, 9,  8,  6,  11, 12, 13, 11, 6,  9,  14, 10, 9,  11, 12, 13, 11, 14, 10, 12};
+const uint8_t kLast8x8[63] = {0, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2,
+                              3, 3, 3, 3, 3, 3, 3, 3, 4, 4, 4, 4, 4, 4, 4, 4, 5, 5, 5, 5, 6, 6, 6, 6, 7, 7, 7, 7, 8, 8, 8};
+
+// The arithmetic decoding engine (9.3.1.2, 9.3.3.2), one bit of the stream at a time.
+struct Cabac {
+    Bits *bs = nullptr;
+    uint32_t range = 0, offset = 0;
+    uint8_t state[460], mps[460];
+    void init_contexts(int slice_type, int idc, int qp) {  // 9.3.1.1
+        const int col = slice_type == H264G_SLICE_I ? 0 : 1 + idc;
+        const int q = std::min(51, std::max(0, qp));
+        for (int c = 0; c < 460; c++) {
+            const int m = kCabacInit[c][col][0], n = kCabacInit[c][col][1];
+            const int pre = std::min(126, std::max(1, ((m * q) >> 4) + n));
+            state[c] = (uint8_t)(pre <= 63 ? 63 - pre : pre - 64);
+            mps[c] = (uint8_t)(pre <= 63 ? 0 : 1);
+        }
+    }
+    void init_engine() {
+        range = 510;
+        offset = bs->u(9);
+    }
+    int decision(int c) {  // 9.3.3.2.1
+        const uint32_t lps = kRangeLps[state[c]][(range >> 6) & 3];
+        range -= lps;
+        int bin;
+        if (offset >= range) {
+            bin = !mps[c];
+            offset -= range;
+            range = lps;
+            if (state[c] == 0) mps[c] = (uint8_t)(1 - mps[c]);
+            state[c] = kTransLps[state[c]];
+        } else {
+            bin = mps[c];
+            if (state[c] < 62) state[c]++;
+        }
+        while (range < 256) {  // RenormD
+            range <<= 1;
+            offset = (offset << 1) | bs->bit();
+        }
+        return bin;
+    }
+    int bypass() {  // 9.3.3.2.3
+        offset = (offset << 1) | bs->bit();
+        if (offset >= range) {
+            offset -= range;
+            return 1;
+        }
+        return 0;
+    }
+    int terminate() {  // 9.3.3.2.2.3
+        range -= 2;
+        if (offset >= range) return 1;
+        while (range < 256) {
+            range <<= 1;
+            offset = (offset << 1) | bs->bit();
+        }
+        return 0;
+    }
+};
+
+// What later MBs' context derivation reads of an MB (9.3.3.1.1).
+struct CMb {
+    uint8_t avail, skip, intra, pcm, i16, nxn, bdirect16, t8, cbp, chroma_mode, qp_delta_nz;
+    uint8_t cbf_dc, cbf_cdc;   // Intra16x16 DC; chroma DC bit c
+    uint16_t cbf_luma;         // 4x4 luma blocks, raster bit
+    uint8_t cbf_cac[2];        // chroma AC, raster bit of the 2x2 grid
+    int8_t ref[2][16];         // refIdx of each 4x4 block, -1 where the list is not used
+    uint8_t direct[16];        // predicted by direct prediction (B_Skip / B_Direct / direct 8x8)
+    uint8_t amvd[2][16][2];    // absolute mvd components (clipped to 255)
+};
+
+struct CabacPic {
+    Cabac c;
+    std::vector<CMb> mbs;
+    int W = 0, H = 0;
+    const CMb *mb_at(int mb, int dx, int dy) const {  // neighbour MB (dx, dy in {-1, 0}) or null
+        const int x = mb % W + dx, y = mb / W + dy;
+        if (x < 0 || y < 0 || x >= W) return nullptr;
+        const CMb &n = mbs[y * W + x];
+        return n.avail ? &n : nullptr;
+    }
+};
+
+// The MB and raster 4x4 block holding 4x4 block (bx, by) of MB mb, bx / by in -1..3
+// (the left / above neighbours); null when outside the picture or not yet decoded.
+const CMb *blk_at(const CabacPic &cp, int mb, const CMb &cur, int bx, int by, int &r) {
+    int dx = 0, dy = 0;
+    if (bx < 0) { bx += 4; dx = -1; }
+    if (by < 0) { by += 4; dy = -1; }
+    r = by * 4 + bx;
+    if (!dx && !dy) return &cur;
+    return cp.mb_at(mb, dx, dy);
+}
+
+int cabac_mb_skip(CabacPic &cp, int mb, int slice_type) {
+    const CMb *a = cp.mb_at(mb, -1, 0), *b = cp.mb_at(mb, 0, -1);
+    const int inc = (a && !a->skip) + (b && !b->skip);
+    return cp.c.decision((slice_type == H264G_SLICE_P ? 11 : 24) + inc);
+}
+
+// mb_type of I MBs, ctxIdxOffset 3 (I slices) or the intra suffix 17 / 32 (P / B slices);
+// returns 0 (I_NxN), 1..24 (I_16x16) or 25 (I_PCM)
+int cabac_mb_type_intra(CabacPic &cp, int mb, int base, bool islice) {
+    Cabac &c = cp.c;
+    if (islice) {
+        const CMb *a = cp.mb_at(mb, -1, 0), *b = cp.mb_at(mb, 0, -1);
+        const int inc = (a && !a->nxn) + (b && !b->nxn);
+        if (!c.decision(base + inc)) return 0;
+        base += 2;
+    } else if (!c.decision(base)) {
+        return 0;
+    }
+    if (c.terminate()) return 25;
+    int t = 1 + 12 * c.decision(base + 1);
+    if (c.decision(base + 2)) t += 4 + 4 * c.decision(base + 2 + (islice ? 1 : 0));
+    t += 2 * c.decision(base + 3 + (islice ? 1 : 0));
+    t += c.decision(base + 3 + (islice ? 2 : 0));
+    return t;
+}
+
+int cabac_mb_type_p(CabacPic &cp, int mb, bool &intra) {
+    Cabac &c = cp.c;
+    intra = false;
+    if (!c.decision(14)) {
+        if (!c.decision(15)) return 3 * c.decision(16);  // P_L0_16x16 (0) / P_8x8 (3)
+        return 2 - c.decision(17);                       // P_L0_L0_8x16 (2) / P_L0_L0_16x8 (1)
+    }
+    intra = true;
+    return cabac_mb_type_intra(cp, mb, 17, false);
+}
+
+int cabac_mb_type_b(CabacPic &cp, int mb, bool &intra) {
+    Cabac &c = cp.c;
+    intra = false;
+    const CMb *a = cp.mb_at(mb, -1, 0), *b = cp.mb_at(mb, 0, -1);
+    const int inc = (a && !a->bdirect16) + (b && !b->bdirect16);
+    if (!c.decision(27 + inc)) return 0;  // B_Direct_16x16
+    if (!c.decision(27 + 3)) return 1 + c.decision(27 + 5);
+    int bits = c.decision(27 + 4) << 3;
+    bits |= c.decision(27 + 5) << 2;
+    bits |= c.decision(27 + 5) << 1;
+    bits |= c.decision(27 + 5);
+    if (bits < 8) return bits + 3;
+    if (bits == 13) {
+        intra = true;
+        return cabac_mb_type_intra(cp, mb, 32, false);
+    }
+    if (bits == 14) return 11;
+    if (bits == 15) return 22;
+    bits = (bits << 1) | c.decision(27 + 5);
+    return bits - 4;
+}
+
+int cabac_sub_p(Cabac &c) {
+    if (c.decision(21)) return 0;
+    if (!c.decision(22)) return 1;
+    return c.decision(23) ? 2 : 3;
+}
+
+int cabac_sub_b(Cabac &c) {
+    if (!c.decision(36)) return 0;
+    if (!c.decision(37)) return 1 + c.decision(39);
+    int t = 3;
+    if (c.decision(38)) {
+        if (c.decision(39)) return 11 + c.decision(39);
+        t += 4;
+    }
+    t += 2 * c.decision(39);
+    t += c.decision(39);
+    return t;
+}
+
+// ref_idx_lX of the partition whose top-left 4x4 block is (bx, by) (9.3.3.1.1.6)
+int cabac_ref_idx(CabacPic &cp, int mb, const CMb &cur, int l, int bx, int by, bool bslice) {
+    int inc = 0;
+    for (int k = 0; k < 2; k++) {
+        int r;
+        const CMb *n = blk_at(cp, mb, cur, bx - (k == 0), by - (k == 1), r);
+        if (!n || n->skip || n->intra) continue;
+        if (bslice && n->direct[r]) continue;
+        if (n->ref[l][r] > 0) inc += k ? 2 : 1;
+    }
+    int ref = 0;
+    while (cp.c.decision(54 + inc)) {
+        ref++;
+        inc = ref == 1 ? 4 : 5;
+        if (ref > 31) return -1;
+    }
+    return ref;
+}
+
+// one mvd component (comp 0: x, ctxIdxOffset 40; 1: y, 47) (9.3.3.1.1.7, UEG3 suffix)
+int cabac_mvd(CabacPic &cp, int mb, const CMb &cur, int l, int bx, int by, int comp) {
+    int sum = 0;
+    for (int k = 0; k < 2; k++) {
+        int r;
+        const CMb *n = blk_at(cp, mb, cur, bx - (k == 0), by - (k == 1), r);
+        if (n) sum += n->amvd[l][r][comp];
+    }
+    const int base = comp ? 47 : 40;
+    Cabac &c = cp.c;
+    if (!c.decision(base + (sum < 3 ? 0 : sum > 32 ? 2 : 1))) return 0;
+    int v = 1, ctx = base + 3;
+    while (v < 9 && c.decision(ctx)) {
+        v++;
+        if (ctx < base + 6) ctx++;
+    }
+    if (v >= 9) {
+        int k = 3;
+        while (c.bypass()) {
+            v += 1 << k;
+            if (++k > 24) return 0x7fffffff;
+        }
+        while (k--) v += c.bypass() << k;
+    }
+    return c.bypass() ? -v : v;
+}
+
+// coded_block_flag neighbour term (9.3.3.1.1.9); kind: 0 luma DC, 1 luma 4x4 (raster r),
+// 2 chroma DC (component cc), 3 chroma AC (component cc, raster r of the 2x2 grid)
+int cbf_term(const CMb *n, bool cur_intra, int kind, int r, int cc) {
+    if (!n) return cur_intra ? 1 : 0;
+    if (n->pcm) return 1;
+    if (n->skip) return 0;
+    switch (kind) {
+    case 0:
+        return n->i16 ? n->cbf_dc : 0;
+    case 1: {
+        const int b8 = ((r >> 3) & 1) * 2 + ((r >> 1) & 1);
+        if (!((n->cbp >> b8) & 1)) return 0;
+        if (n->t8) return 1;  // an 8x8 block's coded_block_flag is inferred to be 1
+        return (n->cbf_luma >> r) & 1;
+    }
+    case 2:
+        return (n->cbp >> 4) ? (n->cbf_cdc >> cc) & 1 : 0;
+    default:
+        return (n->cbp >> 4) == 2 ? (n->cbf_cac[cc] >> r) & 1 : 0;
+    }
+}
+
+// residual_block_cabac (7.3.5.3.3) of block category cat into coef (maxNumCoeff entries, scan
+// order); with_cbf: coded_block_flag present, cbf_inc its ctxIdxInc.  Returns 1 when coded.
+int cabac_block(Cabac &c, int cat, int max_num, bool with_cbf, int cbf_inc, int16_t *coef) {
+    static const int kCbfOff[5] = {0, 4, 8, 12, 16};
+    static const int kSigOff[5] = {0, 15, 29, 44, 47};
+    static const int kAbsOff[5] = {0, 10, 20, 30, 39};
+    for (int i = 0; i < max_num; i++) coef[i] = 0;
+    if (with_cbf && !c.decision(85 + kCbfOff[cat] + cbf_inc)) return 0;
+    const int sig_base = cat == 5 ? 402 : 105 + kSigOff[cat], last_base = cat == 5 ? 417 : 166 + kSigOff[cat];
+    const int abs_base = cat == 5 ? 426 : 227 + kAbsOff[cat];
+    int idx[64], n = 0, i = 0;
+    for (; i < max_num - 1; i++) {
+        const int si = cat == 5 ? kSig8x8[i] : cat == 3 ? std::min(i, 2) : i;
+        if (c.decision(sig_base + si)) {
+            idx[n++] = i;
+            const int li = cat == 5 ? kLast8x8[i] : cat == 3 ? std::min(i, 2) : i;
+            if (c.decision(last_base + li)) break;
+        }
+    }
+    if (i == max_num - 1) idx[n++] = i;
+    int eq1 = 0, gt1 = 0;
+    for (int k = n - 1; k >= 0; k--) {
+        int v;
+        if (!c.decision(abs_base + (gt1 ? 0 : std::min(4, 1 + eq1)))) {
+            v = 1;
+            eq1++;
+        } else {
+            const int ctx = abs_base + 5 + std::min(4 - (cat == 3 ? 1 : 0), gt1);
+            v = 2;
+            while (v < 15 && c.decision(ctx)) v++;
+            if (v >= 15) {  // UEG0 suffix
+                int j = 0;
+                while (c.bypass()) {
+                    if (++j > 24) return -1;
+                }
+                int s = 1;
+                while (j--) s = 2 * s + c.bypass();
+                v = s + 14;
+            }
+            gt1++;
+        }
+        coef[idx[k]] = (int16_t)(c.bypass() ? -v : v);
+    }
+    return 1;
+}
+
+// macroblock_layer (7.3.5) with CABAC
+int cabac_mb(CabacPic &cp, PicState &st, int mb, int &qp, bool &prev_qp_nz, h264g_mb &m, int16_t (&mvd)[2][16][2],
+             uint8_t (&isyn)[16][2]) {
+    Cabac &c = cp.c;
+    const Slice &sl = *st.sl;
+    const Pps &pps = *st.pps;
+    CMb &cur = cp.mbs[mb];
+    memset(&cur, 0, sizeof(cur));
+    memset(cur.ref, -1, sizeof(cur.ref));
+    memset(&m, 0, sizeof(m));
+    memset(mvd, 0, sizeof(mvd));
+    for (int l = 0; l < 2; l++)
+        for (int q = 0; q < 4; q++) m.ref_idx[l][q] = -1;
+    m.level_offset = (uint32_t)st.levels->size();
+    bool intra = sl.slice_type == H264G_SLICE_I;
+    int mb_type;
+    if (intra) mb_type = cabac_mb_type_intra(cp, mb, 3, true);
+    else if (sl.slice_type == H264G_SLICE_P) mb_type = cabac_mb_type_p(cp, mb, intra);
+    else mb_type = cabac_mb_type_b(cp, mb, intra);
+    cur.avail = 1;
+    cur.intra = intra;
+    if (intra && mb_type == 25) {  // I_PCM: the samples follow byte-aligned, then the engine restarts
+        m.mb_type = H264G_MB_IPCM;
+        m.qp_y = (uint8_t)qp;
+        cur.pcm = 1;
+        cur.cbp = 0x2f;
+        Bits &bs = *c.bs;
+        while (!bs.aligned()) bs.bit();
+        for (int i = 0; i < 384; i++) st.levels->push_back((int16_t)bs.u(8));
+        c.init_engine();
+        prev_qp_nz = false;
+        return bs.over ? h264g_set_error(H264G_E_ARG, "truncated I_PCM") : 0;
+    }
+    bool t8 = false, i16 = false, no_sub_lt8 = true;
+    int cbp = 0;
+    const bool is_b = sl.slice_type == H264G_SLICE_B;
+    const int nref[2] = {sl.num_ref_idx[0], sl.num_ref_idx[1]};
+    auto set_ref = [&](int l, int q, int v) {  // quadrant q of list l
+        m.ref_idx[l][q] = (int8_t)v;
+        for (int k = 0; k < 4; k++) cur.ref[l][((q >> 1) * 2 + (k >> 1)) * 4 + (q & 1) * 2 + (k & 1)] = (int8_t)v;
+    };
+    auto put_amvd = [&](int l, int bx, int by, int w, int h, int dx, int dy) {
+        for (int y = by; y < by + h; y++)
+            for (int x = bx; x < bx + w; x++) {
+                cur.amvd[l][y * 4 + x][0] = (uint8_t)std::min(255, std::abs(dx));
+                cur.amvd[l][y * 4 + x][1] = (uint8_t)std::min(255, std::abs(dy));
+            }
+    };
+    auto read_mvd = [&](int l, int bx, int by, int w, int h) -> int {
+        const int dx = cabac_mvd(cp, mb, cur, l, bx, by, 0);
+        const int dy = cabac_mvd(cp, mb, cur, l, bx, by, 1);
+        if (dx < -32768 || dx > 32767 || dy < -32768 || dy > 32767) return h264g_set_error(H264G_E_ARG, "mvd out of range");
+        mvd[l][by * 4 + bx][0] = (int16_t)dx;
+        mvd[l][by * 4 + bx][1] = (int16_t)dy;
+        put_amvd(l, bx, by, w, h, dx, dy);
+        return 0;
+    };
+    if (intra) {
+        if (mb_type == 0) {
+            if (pps.transform_8x8_mode) {
+                const CMb *a = cp.mb_at(mb, -1, 0), *b = cp.mb_at(mb, 0, -1);
+                t8 = c.decision(399 + (a && a->t8) + (b && b->t8));
+            }
+            const int n = t8 ? 4 : 16;
+            for (int k = 0; k < n; k++) {
+                isyn[k][0] = (uint8_t)c.decision(68);
+                isyn[k][1] = 0;
+                if (!isyn[k][0]) {  // rem_intra_pred_mode: 3 bins, least significant first
+                    int v = c.decision(69);
+                    v |= c.decision(69) << 1;
+                    v |= c.decision(69) << 2;
+                    isyn[k][1] = (uint8_t)v;
+                }
+            }
+            m.mb_type = t8 ? H264G_MB_I8x8 : H264G_MB_I4x4;
+            cur.nxn = 1;
+        } else {
+            i16 = true;
+            m.mb_type = H264G_MB_I16x16;
+            m.intra16x16_pred_mode = (uint8_t)((mb_type - 1) % 4);
+            cbp = (((mb_type - 1) / 4) % 3) << 4 | (mb_type >= 13 ? 15 : 0);
+            cur.i16 = 1;
+        }
+        {  // intra_chroma_pred_mode (9.3.3.1.1.8)
+            const CMb *a = cp.mb_at(mb, -1, 0), *b = cp.mb_at(mb, 0, -1);
+            auto term = [](const CMb *n) { return n && n->intra && !n->pcm && n->chroma_mode != 0; };
+            int v = 0;
+            if (c.decision(64 + term(a) + term(b))) {
+                v = 1;
+                while (v < 3 && c.decision(64 + 3)) v++;
+            }
+            m.intra_chroma_pred_mode = (uint8_t)v;
+            cur.chroma_mode = (uint8_t)v;
+        }
+    } else if (!is_b || mb_type != 0) {
+        const InterType &it = is_b ? kBType[mb_type] : kPType[mb_type];
+        m.mb_type = is_b ? H264G_MB_B : H264G_MB_P;
+        m.part_shape = (uint8_t)it.shape;
+        if (it.shape == H264G_PART_8x8) {
+            int sub[4];
+            for (int q = 0; q < 4; q++) {
+                sub[q] = is_b ? cabac_sub_b(c) : cabac_sub_p(c);
+                m.sub_part_shape[q] = (uint8_t)(is_b ? kBSubShape[sub[q]] : kPSubShape[sub[q]]);
+                if (is_b && sub[q] == 0) {
+                    m.flags |= H264G_MBF_DIRECT(q);
+                    for (int k = 0; k < 4; k++) cur.direct[((q >> 1) * 2 + (k >> 1)) * 4 + (q & 1) * 2 + (k & 1)] = 1;
+                    if (!st.sps->direct_8x8_inference) no_sub_lt8 = false;
+                } else if (sub_parts(m.sub_part_shape[q]) > 1) {
+                    no_sub_lt8 = false;
+                }
+            }
+            auto pred = [&](int q) { return is_b ? kBSubPred[sub[q]] : 1; };
+            for (int l = 0; l < 2; l++)
+                for (int q = 0; q < 4; q++) {
+                    if (!(pred(q) & (1 << l))) continue;
+                    int v = 0;
+                    if (nref[l] > 1) v = cabac_ref_idx(cp, mb, cur, l, (q & 1) * 2, (q >> 1) * 2, is_b);
+                    if (v < 0 || v >= nref[l]) return h264g_set_error(H264G_E_ARG, "ref_idx out of range");
+                    set_ref(l, q, v);
+                }
+            for (int l = 0; l < 2; l++)
+                for (int q = 0; q < 4; q++) {
+                    if (!(pred(q) & (1 << l))) continue;
+                    const int shape = m.sub_part_shape[q];
+                    const int w = shape == H264G_SUB_8x8 || shape == H264G_SUB_8x4 ? 2 : 1;
+                    const int h = shape == H264G_SUB_8x8 || shape == H264G_SUB_4x8 ? 2 : 1;
+                    for (int k = 0; k < sub_parts(shape); k++) {
+                        int x, y;
+                        sub_origin(shape, k, x, y);
+                        if (int rc = read_mvd(l, (q & 1) * 2 + x, (q >> 1) * 2 + y, w, h)) return rc;
+                    }
+                }
+        } else {
+            const int np = it.shape == H264G_PART_16x16 ? 1 : 2;
+            auto origin = [&](int k, int &bx, int &by, int &w, int &h) {
+                bx = it.shape == H264G_PART_8x16 ? 2 * k : 0;
+                by = it.shape == H264G_PART_16x8 ? 2 * k : 0;
+                w = it.shape == H264G_PART_8x16 ? 2 : 4;
+                h = it.shape == H264G_PART_16x8 ? 2 : 4;
+            };
+            for (int l = 0; l < 2; l++)
+                for (int k = 0; k < np; k++) {
+                    if (!(it.pred[k] & (1 << l))) continue;
+                    int bx, by, w, h;
+                    origin(k, bx, by, w, h);
+                    int v = 0;
+                    if (nref[l] > 1) v = cabac_ref_idx(cp, mb, cur, l, bx, by, is_b);
+                    if (v < 0 || v >= nref[l]) return h264g_set_error(H264G_E_ARG, "ref_idx out of range");
+                    for (int q = 0; q < 4; q++) {
+                        const int qx = (q & 1) * 2, qy = (q >> 1) * 2;
+                        if (qx >= bx && qx < bx + w && qy >= by && qy < by + h) set_ref(l, q, v);
+                    }
+                }
+            for (int l = 0; l < 2; l++)
+                for (int k = 0; k < np; k++) {
+                    if (!(it.pred[k] & (1 << l))) continue;
+                    int bx, by, w, h;
+                    origin(k, bx, by, w, h);
+                    if (int rc = read_mvd(l, bx, by, w, h)) return rc;
+                }
+        }
+    } else {  // B_Direct_16x16
+        m.mb_type = H264G_MB_B;
+        m.part_shape = H264G_PART_8x8;
+        m.flags |= H264G_MBF_DIRECT_ALL;
+        cur.bdirect16 = 1;
+        memset(cur.direct, 1, sizeof(cur.direct));
+        if (!st.sps->direct_8x8_inference) no_sub_lt8 = false;
+    }
+    if (!i16) {  // coded_block_pattern (9.3.2.6, 9.3.3.1.1.4)
+        const CMb *a = cp.mb_at(mb, -1, 0), *b = cp.mb_at(mb, 0, -1);
+        for (int b8 = 0; b8 < 4; b8++) {
+            const int bx = b8 & 1, by = b8 >> 1;
+            int ta, tb;
+            if (bx) ta = !((cbp >> (b8 - 1)) & 1);
+            else ta = a && !a->pcm && (a->skip || !((a->cbp >> (b8 + 1)) & 1));
+            if (by) tb = !((cbp >> (b8 - 2)) & 1);
+            else tb = b && !b->pcm && (b->skip || !((b->cbp >> (b8 + 2)) & 1));
+            cbp |= c.decision(73 + ta + 2 * tb) << b8;
+        }
+        auto cterm = [](const CMb *n, int need) { return n && (n->pcm || (!n->skip && (n->cbp >> 4) >= need)); };
+        if (c.decision(77 + cterm(a, 1) + 2 * cterm(b, 1))) cbp |= (1 + c.decision(77 + 4 + cterm(a, 2) + 2 * cterm(b, 2))) << 4;
+        const bool nxn = m.mb_type == H264G_MB_I4x4 || m.mb_type == H264G_MB_I8x8;
+        if ((cbp & 15) && pps.transform_8x8_mode && !nxn && no_sub_lt8 && !intra) {
+            const CMb *aa = cp.mb_at(mb, -1, 0), *bb = cp.mb_at(mb, 0, -1);
+            t8 = c.decision(399 + (aa && aa->t8) + (bb && bb->t8));
+        }
+    }
+    if (t8) m.flags |= H264G_MBF_T8x8;
+    cur.t8 = t8;
+    cur.cbp = (uint8_t)cbp;
+    m.cbp = (uint8_t)cbp;
+    if ((cbp & 15) || (cbp >> 4) || i16) {
+        // mb_qp_delta (9.3.3.1.1.5): the previous MB in decoding order had a non-zero one
+        int k = 0;
+        if (c.decision(60 + (prev_qp_nz ? 1 : 0))) {
+            k = 1;
+            int ctx = 60 + 2;
+            while (c.decision(ctx)) {
+                if (++k > 104) return h264g_set_error(H264G_E_ARG, "mb_qp_delta out of range");
+                ctx = 60 + 3;
+            }
+        }
+        const int dq = std::min(25, std::max(-26, (k & 1) ? (k + 1) / 2 : -(k / 2)));
+        qp = (qp + dq + 52) % 52;
+        prev_qp_nz = dq != 0;
+        cur.qp_delta_nz = dq != 0;
+        // residual (7.3.5.3) with residual_block_cabac
+        Residual r;
+        memset(&r, 0, sizeof(r));
+        const CMb *ma = cp.mb_at(mb, -1, 0), *mbb = cp.mb_at(mb, 0, -1);
+        if (i16) {
+            const int inc = cbf_term(ma, true, 0, 0, 0) + 2 * cbf_term(mbb, true, 0, 0, 0);
+            cur.cbf_dc = (uint8_t)cabac_block(c, 0, 16, true, inc, r.dc);
+        }
+        for (int q = 0; q < 4; q++) {
+            if (!((cbp >> q) & 1)) continue;
+            if (t8) {
+                const int got = cabac_block(c, 5, 64, false, 0, r.luma8[q]);
+                if (got < 0) return h264g_set_error(H264G_E_ARG, "invalid coeff_abs_level_minus1");
+                continue;
+            }
+            for (int k2 = 0; k2 < 4; k2++) {
+                const int blk = 4 * q + k2, rr = blk_raster(blk), bx = rr & 3, by = rr >> 2;
+                int ra, rb;
+                const CMb *na = blk_at(cp, mb, cur, bx - 1, by, ra), *nb = blk_at(cp, mb, cur, bx, by - 1, rb);
+                // inside the current MB the neighbour block is one of its own: its flag so far
+                const int ta = na == &cur ? (cur.cbf_luma >> ra) & 1 : cbf_term(na, intra, 1, ra, 0);
+                const int tb = nb == &cur ? (cur.cbf_luma >> rb) & 1 : cbf_term(nb, intra, 1, rb, 0);
+                int16_t co[16];
+                const int got = i16 ? cabac_block(c, 1, 15, true, ta + 2 * tb, co) : cabac_block(c, 2, 16, true, ta + 2 * tb, co);
+                if (got < 0) return h264g_set_error(H264G_E_ARG, "invalid coeff_abs_level_minus1");
+                memcpy(r.luma[blk], co, sizeof(co));
+                if (got) cur.cbf_luma |= (uint16_t)(1u << rr);
+            }
+        }
+        if (cbp >> 4) {
+            for (int cc = 0; cc < 2; cc++) {
+                const int inc = cbf_term(ma, intra, 2, 0, cc) + 2 * cbf_term(mbb, intra, 2, 0, cc);
+                const int got = cabac_block(c, 3, 4, true, inc, r.cdc[cc]);
+                if (got < 0) return h264g_set_error(H264G_E_ARG, "invalid coeff_abs_level_minus1");
+                if (got) cur.cbf_cdc |= (uint8_t)(1u << cc);
+            }
+        }
+        if ((cbp >> 4) == 2) {
+            for (int cc = 0; cc < 2; cc++)
+                for (int b4 = 0; b4 < 4; b4++) {
+                    const int bx = b4 & 1, by = b4 >> 1;
+                    int ta, tb;
+                    if (bx) ta = (cur.cbf_cac[cc] >> (b4 - 1)) & 1;
+                    else ta = cbf_term(ma, intra, 3, b4 + 1, cc);
+                    if (by) tb = (cur.cbf_cac[cc] >> (b4 - 2)) & 1;
+                    else tb = cbf_term(mbb, intra, 3, b4 + 2, cc);
+                    int16_t co[16];
+                    const int got = cabac_block(c, 4, 15, true, ta + 2 * tb, co);
+                    if (got < 0) return h264g_set_error(H264G_E_ARG, "invalid coeff_abs_level_minus1");
+                    memcpy(r.cac[cc][b4], co, 15 * sizeof(int16_t));
+                    if (got) cur.cbf_cac[cc] |= (uint8_t)(1u << b4);
+                }
+        }
+        emit_levels(r, i16, t8, cbp, *st.levels);
+    } else {
+        prev_qp_nz = false;
+    }
+    m.qp_y = (uint8_t)qp;
+    return c.bs->over ? h264g_set_error(H264G_E_ARG, "truncated macroblock") : 0;
+}
+
+// slice_data() (7.3.4) with CABAC: cabac_alignment_one_bit, then mb_skip_flag /
+// macroblock_layer / end_of_slice_flag per MB
+int parse_slice_data_cabac(Bits &bs, PicState &st, int &mb) {
+    h264g_parser *p = st.p;
+    const Slice &sl = *st.sl;
+    const int n_mbs = st.W * st.H;
+    while (!bs.aligned()) {
+        if (!bs.bit()) return h264g_set_error(H264G_E_ARG, "cabac_alignment_one_bit is 0");
+    }
+    CabacPic cp;
+    cp.W = st.W;
+    cp.H = st.H;
+    cp.mbs.assign((size_t)n_mbs, CMb());
+    cp.c.bs = &bs;
+    cp.c.init_contexts(sl.slice_type, sl.cabac_init_idc, sl.slice_qp);
+    cp.c.init_engine();
+    int qp = sl.slice_qp;
+    bool prev_qp_nz = false;
+    int16_t mvd[2][16][2];
+    uint8_t isyn[16][2];
+    while (mb < n_mbs) {
+        h264g_mb &m = p->mbs[mb];
+        if (sl.slice_type != H264G_SLICE_I && cabac_mb_skip(cp, mb, sl.slice_type)) {
+            skip_record(m, sl.slice_type, qp, (uint32_t)p->levels.size());
+            CMb &cm = cp.mbs[mb];
+            memset(&cm, 0, sizeof(cm));
+            memset(cm.ref, -1, sizeof(cm.ref));
+            cm.avail = 1;
+            cm.skip = 1;
+            if (sl.slice_type == H264G_SLICE_B) {
+                cm.bdirect16 = 1;
+                memset(cm.direct, 1, sizeof(cm.direct));
+            } else {
+                memset(cm.ref[0], 0, sizeof(cm.ref[0]));
+            }
+            prev_qp_nz = false;
+            if (int rc = finish_mb(st, mb, m, nullptr, nullptr)) return mb_error(st, mb, rc);
+        } else {
+            if (int rc = cabac_mb(cp, st, mb, qp, prev_qp_nz, m, mvd, isyn)) return mb_error(st, mb, rc);
+            if (int rc = finish_mb(st, mb, m, &mvd[0][0][0], &isyn[0][0])) return mb_error(st, mb, rc);
+        }
+        mb++;
+        if (cp.c.terminate()) break;  // end_of_slice_flag
+        if (bs.over) return h264g_set_error(H264G_E_ARG, "picture %d: slice data truncated", p->pictures);
+    }
+    return 0;
+}
+
 }  // namespace
 
 // ------------------------------------------------------------------ DPB and pictures
@@ -1296,55 +1996,9 @@ extern "C" int32_t h264g_parser_next(h264g_parser *p, h264g_parsed_picture *out)
         } else {
             h264g_mvfield_colocated(p->field, nullptr, 0);
         }
-        int qp = sl.slice_qp, mb = 0;
+        int mb = 0;
         const int n_mbs = W * H;
-        bool more = true;
-        int16_t mvd[2][16][2];
-        uint8_t isyn[16][2];
-        while (more && mb < n_mbs) {
-            if (sl.slice_type != H264G_SLICE_I) {
-                const uint32_t run = bs.ue();
-                if (run > (uint32_t)(n_mbs - mb)) return h264g_set_error(H264G_E_ARG, "mb_skip_run past the picture");
-                for (uint32_t k = 0; k < run; k++, mb++) {
-                    h264g_mb &m = p->mbs[mb];
-                    memset(&m, 0, sizeof(m));
-                    for (int l = 0; l < 2; l++)
-                        for (int q = 0; q < 4; q++) m.ref_idx[l][q] = -1;
-                    m.qp_y = (uint8_t)qp;
-                    m.level_offset = (uint32_t)p->levels.size();
-                    if (sl.slice_type == H264G_SLICE_P) {
-                        m.mb_type = H264G_MB_P_SKIP;
-                        m.ref_idx[0][0] = m.ref_idx[0][1] = m.ref_idx[0][2] = m.ref_idx[0][3] = 0;
-                    } else {
-                        m.mb_type = H264G_MB_B;
-                        m.part_shape = H264G_PART_8x8;
-                        m.flags = H264G_MBF_SKIP | H264G_MBF_DIRECT_ALL;
-                    }
-                    MbNz &n = st.nz[mb];
-                    n.avail = 1;
-                    n.skip = 1;
-                    if (int rc = h264g_mvfield_derive(p->field, (uint32_t)mb, &m, nullptr)) return rc;
-                }
-                if (run > 0) more = bs.more_rbsp_data();
-                if (!more || mb >= n_mbs) break;
-            }
-            h264g_mb &m = p->mbs[mb];
-            bool has_mvd = false;
-            if (int rc = parse_mb(bs, st, mb, qp, m, mvd, isyn, has_mvd)) {
-                char msg[512];
-                snprintf(msg, sizeof(msg), "%s", h264g_last_error());
-                return h264g_set_error(rc, "picture %d MB %d: %s", p->pictures, mb, msg);
-            }
-            if (m.mb_type == H264G_MB_I4x4 || m.mb_type == H264G_MB_I8x8) {
-                if (int rc = h264g_mvfield_intra_modes(p->field, (uint32_t)mb, &m, &isyn[0][0], pps->constrained_intra_pred))
-                    return rc;
-            }
-            const bool all_direct = (m.flags & H264G_MBF_DIRECT_ALL) == H264G_MBF_DIRECT_ALL;
-            if (int rc = h264g_mvfield_derive(p->field, (uint32_t)mb, &m, has_mvd && !all_direct ? &mvd[0][0][0] : nullptr))
-                return rc;
-            mb++;
-            more = bs.more_rbsp_data();
-        }
+        if (int rc = pps->cabac ? parse_slice_data_cabac(bs, st, mb) : parse_slice_data_cavlc(bs, st, mb)) return rc;
         if (mb != n_mbs) return h264g_set_error(H264G_E_ARG, "picture %d: the slice covers %d of %d MBs", p->pictures, mb, n_mbs);
         if (bs.over) return h264g_set_error(H264G_E_ARG, "picture %d: slice data truncated", p->pictures);
         // ---- DPB bookkeeping: the current frame, then marking

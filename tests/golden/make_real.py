@@ -1,11 +1,13 @@
-"""Writes the real-stream fixtures: a prefix of one of the reference's own sample
-streams and the reference decoder's output for it.
+"""Writes the real-stream fixtures: prefixes of the reference's own sample streams and
+the reference decoder's output for each.
 
   tests/golden/heavyhand_cavlc_15.h264   the first 15 pictures (whole NAL units) of
       data/HeavyHand_1080p.B_frames_4.no_cabac.no_tff.h264: High profile, CAVLC,
       1920x1088 frames, B pyramids, explicit (P) and implicit (B) weighted prediction,
       4 reference frames, reference list modification
-  tests/golden/heavyhand_cavlc_15.json   per output picture of the reference decoder
+  tests/golden/heavyhand_cabac_10.h264   the first 10 pictures of
+      data/HeavyHand_1080p.B_frames.cabac.h264: the same content coded with CABAC
+  tests/golden/<name>.json               per output picture of the reference decoder
       (oracle/_ref/ref_decode: every reference translation unit but main.cpp, compiled
       unmodified by `make -C oracle decode`), in its output order: PicOrderCnt, slice
       type, decoding index, SHA-256 of the Y / Cb / Cr planes
@@ -19,9 +21,9 @@ import sys
 
 ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
 HERE = os.path.dirname(os.path.abspath(__file__))
-SRC = "/root/reference/data/HeavyHand_1080p.B_frames_4.no_cabac.no_tff.h264"
-N_PICTURES = 15
-NAME = "heavyhand_cavlc_15"
+DATA = "/root/reference/data"
+STREAMS = [("HeavyHand_1080p.B_frames_4.no_cabac.no_tff.h264", 15, "heavyhand_cavlc_15"),
+           ("HeavyHand_1080p.B_frames.cabac.h264", 10, "heavyhand_cabac_10")]
 
 
 def prefix(data: bytes, n_pictures: int) -> bytes:
@@ -45,19 +47,20 @@ def prefix(data: bytes, n_pictures: int) -> bytes:
 
 def main():
     subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "decode"], check=True, capture_output=True)
-    with open(SRC, "rb") as f:
-        data = prefix(f.read(), N_PICTURES)
-    path = os.path.join(HERE, NAME + ".h264")
-    with open(path, "wb") as f:
-        f.write(data)
-    out = subprocess.run([os.path.join(ROOT, "oracle", "_ref", "ref_decode"), path], check=True, capture_output=True,
-                         text=True, errors="replace").stdout
-    pics = [json.loads(line[len("REFPIC "):]) for line in out.splitlines() if line.startswith("REFPIC ")]
-    if len(pics) != N_PICTURES:
-        sys.exit(f"the reference output {len(pics)} pictures, expected {N_PICTURES}")
-    with open(os.path.join(HERE, NAME + ".json"), "w") as f:
-        json.dump({"source": os.path.basename(SRC), "pictures": pics}, f, indent=1)
-    print(f"{path}: {len(data)} bytes, {len(pics)} output pictures")
+    for src, n_pictures, name in STREAMS:
+        with open(os.path.join(DATA, src), "rb") as f:
+            data = prefix(f.read(), n_pictures)
+        path = os.path.join(HERE, name + ".h264")
+        with open(path, "wb") as f:
+            f.write(data)
+        out = subprocess.run([os.path.join(ROOT, "oracle", "_ref", "ref_decode"), path], check=True,
+                             capture_output=True, text=True, errors="replace").stdout
+        pics = [json.loads(line[len("REFPIC "):]) for line in out.splitlines() if line.startswith("REFPIC ")]
+        if len(pics) != n_pictures:
+            sys.exit(f"the reference output {len(pics)} pictures for {src}, expected {n_pictures}")
+        with open(os.path.join(HERE, name + ".json"), "w") as f:
+            json.dump({"source": src, "pictures": pics}, f, indent=1)
+        print(f"{path}: {len(data)} bytes, {len(pics)} output pictures")
 
 
 if __name__ == "__main__":

@@ -1,7 +1,7 @@
-"""The bitstream front end (h264g_parser_*) on a prefix of the reference's own CAVLC
-sample stream (tests/golden/heavyhand_cavlc_15.*, written by make_real.py from the
-reference decoder's output): parse on the CPU, reconstruct on the GPU, and compare
-every output picture with the reference decoder's."""
+"""The bitstream front end (h264g_parser_*) on prefixes of the reference's own sample
+streams (tests/golden/heavyhand_*, written by make_real.py from the reference decoder's
+output): parse on the CPU, reconstruct on the GPU, and compare every output picture with
+the reference decoder's."""
 import hashlib
 import json
 import os
@@ -14,22 +14,47 @@ from h264_video_decoder_demo_amd.bitstream import Bitstream
 from h264_video_decoder_demo_amd.decoder import H264_DECODE_ERROR_CODE_FILE_END, H264VideoDecoder
 
 HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
-STREAM = os.path.join(HERE, "heavyhand_cavlc_15.h264")
-REF = json.load(open(os.path.join(HERE, "heavyhand_cavlc_15.json")))["pictures"]
+STREAMS = ["heavyhand_cavlc_15", "heavyhand_cabac_10"]  # CAVLC and CABAC codings of one clip
 
 
-def _decode_order(ref):
-    return [p for p in sorted(ref, key=lambda p: p["decode_index"])]
+def _stream(name):
+    return os.path.join(HERE, name + ".h264")
 
 
-def test_parser_follows_the_reference_picture_order():
+def _ref(name):
+    with open(os.path.join(HERE, name + ".json")) as f:
+        return json.load(f)["pictures"]
+
+
+class _BitWriter:
+    def __init__(self):
+        self.bits = []
+
+    def u(self, n, v):
+        self.bits += [(v >> (n - 1 - i)) & 1 for i in range(n)]
+
+    def ue(self, v):
+        k = (v + 1).bit_length()
+        self.u(k - 1, 0)
+        self.u(k, v + 1)
+
+    def nal(self, header):
+        bits = self.bits + [1]
+        bits += [0] * (-len(bits) % 8)
+        body = bytes(int("".join(map(str, bits[i:i + 8])), 2) for i in range(0, len(bits), 8))
+        return b"\x00\x00\x00\x01" + bytes([header]) + body
+
+
+@pytest.mark.parametrize("name", STREAMS)
+def test_parser_follows_the_reference_picture_order(name):
     """Every slice parses to its end (the parser refuses a slice whose MBs do not cover
-    the picture or whose data runs out), in the reference's decoding order, with the
-    reference's PicOrderCnt and slice type for each picture."""
-    bs = Bitstream(STREAM)
+    the picture or whose data runs out; CABAC slices end on end_of_slice_flag), in the
+    reference's decoding order, with the reference's PicOrderCnt and slice type for
+    each picture."""
+    bs = Bitstream(_stream(name))
     assert (bs.width_mbs, bs.height_mbs) == (120, 68)
     pics = list(bs.pictures())
-    want = _decode_order(REF)
+    want = sorted(_ref(name), key=lambda p: p["decode_index"])
     assert [(p.desc.poc, p.desc.slice_type) for p in pics] == [(w["poc"], w["slice_type"]) for w in want]
     assert pics[0].idr and not any(p.idr for p in pics[1:])
     # reference lists name slots of earlier reference pictures only, never the destination
@@ -38,41 +63,72 @@ def test_parser_follows_the_reference_picture_order():
             refs = [p.desc.ref_slot[l][i] for i in range(p.desc.num_ref_idx_active[l])]
             assert all(0 <= r < bs.dpb_slots and r != p.desc.dst_slot for r in refs), (p.desc.poc, refs)
     # a second pass from the start gives the same pictures
-    again = list(Bitstream(STREAM).pictures())
+    again = list(Bitstream(_stream(name)).pictures())
     for a, b in zip(pics, again):
         assert np.array_equal(a.mbs, b.mbs) and np.array_equal(a.levels, b.levels)
 
 
-def test_parser_refuses_cabac_and_damaged_streams():
-    """A CABAC PPS is refused by name; damaged slice data ends in an error or a parse
-    that still stays inside its buffers (the library never reads past the stream)."""
-    data = bytearray(open(STREAM, "rb").read())
-    # entropy_coding_mode_flag is the third bit of the PPS payload (pps_id, sps_id: ue 0)
-    k = data.find(b"\x00\x00\x01\x68")
-    data[k + 4] |= 0x20
-    with pytest.raises(_lib.H264GError, match="CABAC"):
-        list(Bitstream(bytes(data)).pictures())
+def test_cavlc_and_cabac_codings_parse_to_the_same_kinds_of_mbs():
+    """The two codings of the clip (different encoder runs, so different decisions) both
+    use every MB class the engine takes: I_NxN with both transform sizes, I_16x16, P and
+    B partitions, skips and direct prediction."""
+    for name in STREAMS:
+        kinds = set()
+        for p in Bitstream(_stream(name)).pictures():
+            kinds |= {int(t) for t in np.unique(p.mbs["mb_type"])}
+            if p.desc.slice_type == 1:
+                assert np.any(p.mbs["flags"] & _lib.MBF_DIRECT_ALL), name
+        assert {_lib.MB_I4x4, _lib.MB_I8x8, _lib.MB_I16x16, _lib.MB_P, _lib.MB_P_SKIP, _lib.MB_B} <= kinds, name
+
+
+def test_parser_refuses_field_coding_and_damaged_streams():
+    """A field-coded SPS (frame_mbs_only_flag 0) is refused by name; damaged slice data
+    ends in an error or a parse that stays inside its buffers (the library never reads
+    past the stream)."""
+    w = _BitWriter()
+    w.u(8, 77)   # profile_idc (Main)
+    w.u(8, 0)    # constraint flags
+    w.u(8, 30)   # level_idc
+    w.ue(0)      # seq_parameter_set_id
+    w.ue(0)      # log2_max_frame_num_minus4
+    w.ue(0)      # pic_order_cnt_type
+    w.ue(0)      # log2_max_pic_order_cnt_lsb_minus4
+    w.ue(1)      # max_num_ref_frames
+    w.u(1, 0)    # gaps_in_frame_num_value_allowed_flag
+    w.ue(1)      # pic_width_in_mbs_minus1
+    w.ue(1)      # pic_height_in_map_units_minus1
+    w.u(1, 0)    # frame_mbs_only_flag
+    w.u(1, 0)    # mb_adaptive_frame_field_flag
+    w.u(1, 1)    # direct_8x8_inference_flag
+    w.u(1, 0)    # frame_cropping_flag
+    w.u(1, 0)    # vui_parameters_present_flag
+    with pytest.raises(_lib.H264GError, match="field coding"):
+        Bitstream(w.nal(0x67))
     rng = np.random.default_rng(5)
-    clean = open(STREAM, "rb").read()
-    first_slice = clean.find(b"\x00\x00\x01\x65")
-    for _ in range(12):
-        d = bytearray(clean)
-        for pos in rng.integers(first_slice + 16, len(d), 4):
-            d[pos] ^= int(rng.integers(1, 256))
-        try:
-            for _p in Bitstream(bytes(d)).pictures():
+    for name in STREAMS:
+        clean = open(_stream(name), "rb").read()
+        first_slice = clean.find(b"\x00\x00\x01\x65")
+        for _ in range(8):
+            d = bytearray(clean)
+            for pos in rng.integers(first_slice + 16, len(d), 4):
+                d[pos] ^= int(rng.integers(1, 256))
+            try:
+                for _p in Bitstream(bytes(d)).pictures():
+                    pass
+            except (_lib.H264GError, ValueError):
                 pass
-        except (_lib.H264GError, ValueError):
-            pass
     with pytest.raises(ValueError):
         Bitstream(b"\x00\x00\x01\x09\xf0")  # an access unit delimiter, no picture
 
 
 @pytest.mark.gpu
-def test_real_stream_equals_reference_decoder(engine):
-    """The 15 pictures decoded through H264VideoDecoder (host parse, GPU reconstruction)
-    are handed to the callback in the reference decoder's output order, and each one's
-    planes equal the reference decoder's byte for byte."""
+@pytest.mark.parametrize("name", STREAMS)
+def test_real_stream_equals_reference_decoder(engine, name):
+    """The pictures decoded through H264VideoDecoder (host parse, GPU reconstruction) are
+    handed to the callback in the reference decoder's output order, and each one's planes
+    equal the reference decoder's byte for byte (its last picture unfiltered, as the
+    reference outputs it)."""
+    ref = _ref(name)
     got = []
 
     def callback(pic, user, err):
@@ -86,7 +142,7 @@ def test_real_stream_equals_reference_decoder(engine):
 
     dec = H264VideoDecoder(engine)
     dec.set_output_frame_callback_functuin(callback, None)
-    assert dec.open(STREAM) == 0
-    assert [g[0] for g in got] == [r["poc"] for r in REF]
-    for g, r in zip(got, REF):
-        assert g[1:] == (r["y"], r["cb"], r["cr"]), f"POC {r['poc']} (decoding index {r['decode_index']})"
+    assert dec.open(_stream(name)) == 0
+    assert [g[0] for g in got] == [r["poc"] for r in ref]
+    for g, r in zip(got, ref):
+        assert g[1:] == (r["y"], r["cb"], r["cr"]), f"{name}: POC {r['poc']} (decoding index {r['decode_index']})"
