@@ -27,7 +27,8 @@ H264G_E_TIMEOUT = -8
 
 SLICE_P, SLICE_B, SLICE_I = 0, 1, 2
 MB_I4x4, MB_I8x8, MB_I16x16, MB_IPCM, MB_P, MB_P_SKIP, MB_B = range(7)
-MBF_T8x8, MBF_FIELD, MBF_SKIP, MBF_DIRECT_ALL = 0x01, 0x02, 0x04, 0xF0
+MBF_T8x8, MBF_FIELD, MBF_SKIP, MBF_LOST, MBF_DIRECT_ALL = 0x01, 0x02, 0x04, 0x08, 0xF0
+SLICE_NONE = 0xFFFFFFFF
 
 
 class H264GError(RuntimeError):
@@ -42,7 +43,7 @@ class MbRecord(ctypes.Structure):
         ("intra16x16_pred_mode", c_uint8), ("intra_chroma_pred_mode", c_uint8), ("part_shape", c_uint8),
         ("reserved0", c_uint8), ("sub_part_shape", c_uint8 * 4), ("intra_pred_modes", c_uint8 * 16),
         ("ref_idx", (c_int8 * 4) * 2), ("mv", ((c_int16 * 2) * 16) * 2), ("level_offset", c_uint32),
-        ("reserved1", c_uint32),
+        ("slice_num", c_uint32),
     ]
 
 
@@ -50,7 +51,7 @@ MB_DTYPE = np.dtype([
     ("mb_type", "u1"), ("flags", "u1"), ("qp_y", "u1"), ("cbp", "u1"),
     ("intra16x16_pred_mode", "u1"), ("intra_chroma_pred_mode", "u1"), ("part_shape", "u1"),
     ("reserved0", "u1"), ("sub_part_shape", "u1", (4,)), ("intra_pred_modes", "u1", (16,)),
-    ("ref_idx", "i1", (2, 4)), ("mv", "<i2", (2, 16, 2)), ("level_offset", "<u4"), ("reserved1", "<u4"),
+    ("ref_idx", "i1", (2, 4)), ("mv", "<i2", (2, 16, 2)), ("level_offset", "<u4"), ("slice_num", "<u4"),
 ])
 assert MB_DTYPE.itemsize == ctypes.sizeof(MbRecord) == 172
 
@@ -101,7 +102,7 @@ class GenParams(ctypes.Structure):
         ("disable_deblocking_filter_idc", c_int32), ("chroma_qp_index_offset", c_int32),
         ("second_chroma_qp_index_offset", c_int32), ("ipcm_permille", c_int32), ("p_pattern", c_int32),
         ("spare_slots", c_int32), ("mbaff", c_int32), ("constrained_intra_pred", c_int32),
-        ("direct_permille", c_int32),
+        ("direct_permille", c_int32), ("slices", c_int32),
     ]
 
 
@@ -111,6 +112,7 @@ class ParsedPicture(ctypes.Structure):
         ("width_mbs", c_int32), ("height_mbs", c_int32), ("dpb_slots", c_int32), ("max_num_reorder_frames", c_int32),
         ("idr", c_int32), ("nal_ref_idc", c_int32), ("frame_num", c_int32),
         ("profile_idc", c_int32), ("constraint_set3", c_int32), ("level_idc", c_int32),
+        ("n_slices", c_int32), ("lost_mbs", c_int32), ("first_lost_mb", c_int32),
     ]
 
 
@@ -162,6 +164,7 @@ _SIGNATURES = {
     "h264g_parser_create": (c_int32, [c_void_p, c_uint64, POINTER(c_void_p)]),
     "h264g_parser_destroy": (c_int32, [c_void_p]),
     "h264g_parser_next": (c_int32, [c_void_p, POINTER(ParsedPicture)]),
+    "h264g_parser_damage": (ctypes.c_char_p, [c_void_p]),
 }
 
 _lib = None

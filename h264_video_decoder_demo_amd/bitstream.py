@@ -31,6 +31,10 @@ class Picture:
     idr: bool
     nal_ref_idc: int
     frame_num: int
+    n_slices: int = 1
+    lost_mbs: int = 0         # MBs no slice decoded (a damaged slice stopped early): zero samples
+    first_lost_mb: int = -1
+    damage: str = ""          # the parse error that stopped the damaged slice
 
 
 class Bitstream:
@@ -83,7 +87,9 @@ class Bitstream:
         lv = (np.ctypeslib.as_array(ctypes.cast(pp.levels, ctypes.POINTER(ctypes.c_int16)), shape=(pp.n_levels,)).copy()
               if pp.n_levels else np.zeros(0, np.int16))
         desc = PicDesc.from_buffer_copy(pp.desc)
-        return Picture(desc, mbs, lv, bool(pp.idr), pp.nal_ref_idc, pp.frame_num)
+        damage = lib().h264g_parser_damage(self._h).decode() if pp.lost_mbs else ""
+        return Picture(desc, mbs, lv, bool(pp.idr), pp.nal_ref_idc, pp.frame_num, pp.n_slices, pp.lost_mbs,
+                       pp.first_lost_mb, damage)
 
     def pictures(self):
         """Every picture in decoding order (parses the stream from its start)."""

@@ -431,6 +431,26 @@ int stage_picture(h264g_ctx *ctx, Stream &st, const h264g_pic_desc &d, const h26
     if (d.disable_deblocking_filter_idc < 0 || d.disable_deblocking_filter_idc > 2 || d.filter_offset_a < -12 || d.filter_offset_a > 12 ||
         d.filter_offset_b < -12 || d.filter_offset_b > 12)
         return h264g_set_error(H264G_E_ARG, "deblocking parameters");
+    // Slices.  Deblocking stops at the first MB of no slice (H264G_SLICE_NONE), where the
+    // reference's bS derivation fails on the MB's reset type and the failure is ignored
+    // (H264InterPrediction.cpp:3182-3186, H264PictureBase.cpp:707-708).  With
+    // disable_deblocking_filter_idc 0 the MBs before it must form one slice: the reference
+    // filters an edge between slices with the current MB's data on both sides (it takes
+    // the unavailable neighbour's address as the current one, Deblock.cpp:733-750), which
+    // is not followed here; MBAFF frames take one slice.
+    uint32_t dbk_stop = n_mbs, first_sn = H264G_SLICE_NONE;
+    for (uint32_t a = 0; a < n_mbs; a++) {
+        const uint32_t sn = mbs[a].slice_num;
+        if (sn == H264G_SLICE_NONE) {
+            if (mbaff) return h264g_set_error(H264G_E_ARG, "mb %u: MBs of no slice in an MBAFF frame", a);
+            if (dbk_stop == n_mbs) dbk_stop = a;
+            continue;
+        }
+        if (first_sn == H264G_SLICE_NONE) first_sn = sn;
+        if (sn != first_sn && (mbaff || (d.disable_deblocking_filter_idc == 0 && a < dbk_stop)))
+            return h264g_set_error(H264G_E_ARG, "mb %u: several slices %s", a,
+                                   mbaff ? "in an MBAFF frame" : "with disable_deblocking_filter_idc 0");
+    }
     std::vector<DevAux> aux(n_mbs);
     std::vector<int8_t> bsref(mbaff ? (size_t)n_mbs * 8 : 0, (int8_t)-1);
     std::vector<DevWeight> wt;
@@ -517,6 +537,7 @@ int stage_picture(h264g_ctx *ctx, Stream &st, const h264g_pic_desc &d, const h26
             auto avail = [&](int dx, int dy) {
                 int nx = mbx + dx, ny = mby + dy;
                 if (nx < 0 || ny < 0 || nx >= W) return false;
+                if (mbs[ny * W + nx].slice_num != m.slice_num) return false;  // another slice (6.4.8)
                 if (d.constrained_intra_pred_flag && cip_hides(mbs[ny * W + nx])) return false;
                 return true;
             };
@@ -675,6 +696,10 @@ int stage_picture(h264g_ctx *ctx, Stream &st, const h264g_pic_desc &d, const h26
             for (int k = 0; k < 4; k++) {
                 if (!(av & (1u << k))) continue;
                 const int c = mbs[nb[k]].mb_type;
+                if (mbs[nb[k]].slice_num != mbs[a].slice_num) {  // another slice: not available (6.4.8)
+                    av &= ~(1u << k);
+                    continue;
+                }
                 if (d.constrained_intra_pred_flag && cip_hides(mbs[nb[k]])) av &= ~(1u << k);
                 if (cls_intra(c)) dep |= 1u << k;
             }
@@ -734,6 +759,7 @@ int stage_picture(h264g_ctx *ctx, Stream &st, const h264g_pic_desc &d, const h26
     P.bsref = mbaff ? (const int8_t *)(base + o_bsref) : nullptr;
     use_set(st, 0, P);
     P.dbk_idc = d.disable_deblocking_filter_idc; P.cip = d.constrained_intra_pred_flag;
+    P.dbk_stop = dbk_stop;
     HIPCHK(hipMemcpy(pic->d_mbs, mbs, n_mbs * sizeof(h264g_mb), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(pic->d_aux, aux.data(), n_mbs * sizeof(DevAux), hipMemcpyHostToDevice));
     if (n_levels) HIPCHK(hipMemcpy(pic->d_levels, levels, n_levels * sizeof(int16_t), hipMemcpyHostToDevice));

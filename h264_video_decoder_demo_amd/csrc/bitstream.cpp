@@ -13,10 +13,14 @@
 // Motion vectors and intra prediction modes are derived by the motion field
 // (h264g_mvfield_*, mvpred.cpp) from the parsed mvd and prev / rem syntax.
 //
-// Refused with H264G_E_ARG (none occur in the reference's progressive sample streams of
-// one slice per picture): field pictures and MBAFF, several slices per picture or
-// slice groups, scaling matrices, chroma formats other than 4:2:0, bit depths above 8,
-// SP / SI slices, temporal direct prediction, POC type 1, gaps in frame_num.
+// Pictures may have several slices (in order, sharing one reference list, weight table
+// and deblocking setting, with disable_deblocking_filter_idc 1 or 2); each MB record
+// carries its slice number, which makes neighbours in other slices unavailable.
+// Refused with H264G_E_ARG (none occur in the reference's progressive sample streams):
+// field pictures and MBAFF, slice edges deblocked with idc 0 (the reference's quirk Q7),
+// slice groups, arbitrary slice order, scaling matrices, chroma formats other than
+// 4:2:0, bit depths above 8, SP / SI slices, temporal direct prediction, POC type 1,
+// gaps in frame_num.
 #include <algorithm>
 #include <cstdarg>
 #include <cstdint>
@@ -572,6 +576,7 @@ struct h264g_parser {
     std::vector<int16_t> levels;
     h264g_mvfield *field = nullptr;
     int pictures = 0;
+    char damage[512] = {0};     // why the last picture's damaged slice stopped ("" if none)
     ~h264g_parser() {
         if (field) h264g_mvfield_destroy(field);
     }
@@ -631,7 +636,9 @@ int residual_block(Bits &bs, int nC, int start, int end, int max_num, int16_t *o
         run[i] = 0;
         if (zeros_left > 0) {
             const Vlc::E *rb = T.run_before[std::min(zeros_left, 7) - 1].read(bs);
-            if (!rb) return h264g_set_error(H264G_E_ARG, "invalid run_before");
+            if (!rb)
+                return h264g_set_error(H264G_E_ARG, "invalid run_before (nC %d, TotalCoeff %d, zerosLeft %d)", nC, tc,
+                                       zeros_left);
             run[i] = rb->a;
             zeros_left -= run[i];
             if (zeros_left < 0) return h264g_set_error(H264G_E_ARG, "run_before beyond total_zeros");
@@ -683,8 +690,12 @@ struct PicState {
     const Pps *pps;
     Slice *sl;
     int W, H;
-    std::vector<MbNz> nz;
+    std::vector<MbNz> nz;          // this slice's MBs (the others: not available)
     std::vector<int16_t> *levels;
+    uint32_t slice_num;            // the slice's index in its picture
+    size_t mark = 0;               // levels->size() when the current MB began
+    bool luma_done = false;        // the current MB's luma residual has parsed
+    bool partial = false;          // the MB that failed holds a record of what it parsed
 };
 
 // TotalCoeff of the 4x4 luma block at (x, y) in 4x4 units of MB mb, or -1 if unavailable
@@ -786,6 +797,7 @@ int parse_residual(Bits &bs, PicState &st, int mb, bool i16, bool t8, int cbp, R
             nzc.luma[blk_raster(blk)] = (uint8_t)total;
         }
     }
+    st.luma_done = true;
     const int cc = cbp >> 4;
     for (int c = 0; c < 2 && cc; c++) {
         if (int rc = residual_block(bs, -1, 0, 3, 4, r.cdc[c], &total)) return rc;
@@ -942,7 +954,23 @@ int parse_mb(Bits &bs, PicState &st, int mb, int &qp, h264g_mb &m, int16_t (&mvd
         const int dq = std::min(25, std::max(-26, bs.se()));
         qp = (qp + dq + 52) % 52;  // 7-37
         Residual r;
-        if (int rc = parse_residual(bs, st, mb, i16, t8, cbp, r, nzc)) return rc;
+        st.luma_done = false;
+        if (int rc = parse_residual(bs, st, mb, i16, t8, cbp, r, nzc)) {
+            // The reference reconstructs an MB whose residual fails to parse from the syntax
+            // it holds (H264SliceData.cpp:375-379 only logs the failure): QPY keeps its reset
+            // value 0, as it is set after residual() (H264MacroBlock.cpp:1110-1124); luma
+            // levels are kept only once every luma block parsed (H264MacroBlock.cpp:1616-1622),
+            // chroma blocks one by one as they parsed (H264MacroBlock.cpp:1626-1700).
+            if (!st.luma_done) {
+                memset(r.dc, 0, sizeof(r.dc));
+                memset(r.luma, 0, sizeof(r.luma));
+                memset(r.luma8, 0, sizeof(r.luma8));
+            }
+            m.qp_y = 0;
+            emit_levels(r, i16, t8, cbp, *st.levels);
+            st.partial = true;
+            return rc;
+        }
         emit_levels(r, i16, t8, cbp, *st.levels);
     } else {
         memset(nzc.luma, 0, sizeof(nzc.luma));
@@ -972,11 +1000,29 @@ void skip_record(h264g_mb &m, int slice_type, int qp, uint32_t level_offset) {
 // Intra prediction modes and motion vectors of a parsed MB (the motion field, in MB order).
 int finish_mb(PicState &st, int mb, h264g_mb &m, const int16_t *mvd, const uint8_t *isyn) {
     h264g_parser *p = st.p;
+    m.slice_num = st.slice_num;
     if (m.mb_type == H264G_MB_I4x4 || m.mb_type == H264G_MB_I8x8) {
         if (int rc = h264g_mvfield_intra_modes(p->field, (uint32_t)mb, &m, isyn, st.pps->constrained_intra_pred)) return rc;
     }
     const bool all_direct = (m.flags & H264G_MBF_DIRECT_ALL) == H264G_MBF_DIRECT_ALL;
     return h264g_mvfield_derive(p->field, (uint32_t)mb, &m, all_direct ? nullptr : mvd);
+}
+
+// An MB left undecoded (H264G_MBF_LOST): zero samples as I_PCM with QPY 0, the reset
+// state the reference leaves such MBs in (H264PictureBase.cpp:45-67).  An MB where slice
+// data failed stays in its slice; MBs no slice reached belong to no slice
+// (H264G_SLICE_NONE).
+int undecoded_mb(h264g_parser *p, int mb, uint32_t slice_num) {
+    h264g_mb &m = p->mbs[mb];
+    memset(&m, 0, sizeof(m));
+    for (int l = 0; l < 2; l++)
+        for (int q = 0; q < 4; q++) m.ref_idx[l][q] = -1;
+    m.mb_type = H264G_MB_IPCM;
+    m.flags = H264G_MBF_LOST;
+    m.slice_num = slice_num;
+    m.level_offset = (uint32_t)p->levels.size();
+    p->levels.resize(p->levels.size() + 384, 0);
+    return h264g_mvfield_derive(p->field, (uint32_t)mb, &m, nullptr);
 }
 
 int mb_error(const PicState &st, int mb, int rc) {
@@ -995,6 +1041,7 @@ int parse_slice_data_cavlc(Bits &bs, PicState &st, int &mb) {
     int16_t mvd[2][16][2];
     uint8_t isyn[16][2];
     while (more && mb < n_mbs) {
+        st.mark = p->levels.size();
         if (sl.slice_type != H264G_SLICE_I) {
             const uint32_t run = bs.ue();
             if (run > (uint32_t)(n_mbs - mb)) return h264g_set_error(H264G_E_ARG, "mb_skip_run past the picture");
@@ -1010,7 +1057,14 @@ int parse_slice_data_cavlc(Bits &bs, PicState &st, int &mb) {
         }
         h264g_mb &m = p->mbs[mb];
         bool has_mvd = false;
-        if (int rc = parse_mb(bs, st, mb, qp, m, mvd, isyn, has_mvd)) return mb_error(st, mb, rc);
+        if (int rc = parse_mb(bs, st, mb, qp, m, mvd, isyn, has_mvd)) {
+            const int err = mb_error(st, mb, rc);
+            if (st.partial) {  // reconstructed from what parsed; the slice ends with it
+                m.flags |= H264G_MBF_LOST;
+                if (finish_mb(st, mb, m, has_mvd ? &mvd[0][0][0] : nullptr, &isyn[0][0])) st.partial = false;
+            }
+            return err;
+        }
         if (int rc = finish_mb(st, mb, m, has_mvd ? &mvd[0][0][0] : nullptr, &isyn[0][0])) return mb_error(st, mb, rc);
         mb++;
         more = bs.more_rbsp_data();
@@ -1620,6 +1674,7 @@ int parse_slice_data_cabac(Bits &bs, PicState &st, int &mb) {
     int16_t mvd[2][16][2];
     uint8_t isyn[16][2];
     while (mb < n_mbs) {
+        st.mark = p->levels.size();
         h264g_mb &m = p->mbs[mb];
         if (sl.slice_type != H264G_SLICE_I && cabac_mb_skip(cp, mb, sl.slice_type)) {
             skip_record(m, sl.slice_type, qp, (uint32_t)p->levels.size());
@@ -1878,165 +1933,271 @@ extern "C" int32_t h264g_parser_destroy(h264g_parser *p) {
     return H264G_OK;
 }
 
+namespace {
+
+// The slice's reference lists as DPB indices: initialisation, then modification.
+int slice_lists(h264g_parser *p, const Sps &sps, const Slice &sl, int poc, std::vector<int> lists[2]) {
+    init_lists(p, sps, sl, poc, lists);
+    for (int l = 0; l < (sl.slice_type == H264G_SLICE_B ? 2 : sl.slice_type == H264G_SLICE_P ? 1 : 0); l++) {
+        if (int rc = modify_list(p, sps, sl, l, lists[l])) return rc;
+    }
+    return 0;
+}
+
+bool explicit_wp(const Pps &pps, const Slice &sl) {
+    return (pps.weighted_pred && sl.slice_type == H264G_SLICE_P) || (pps.weighted_bipred_idc == 1 && sl.slice_type == H264G_SLICE_B);
+}
+
+// Fills the picture's descriptor from its first slice (except dst_slot).
+void fill_desc(const h264g_parser *p, const Pps &pps, const Slice &sl, int poc, const std::vector<int> lists[2],
+               h264g_pic_desc &d) {
+    memset(&d, 0, sizeof(d));
+    d.slice_type = sl.slice_type;
+    d.structure = H264G_FRAME;
+    d.constrained_intra_pred_flag = pps.constrained_intra_pred;
+    d.disable_deblocking_filter_idc = sl.disable_dbk;
+    d.filter_offset_a = sl.alpha_div2 * 2;
+    d.filter_offset_b = sl.beta_div2 * 2;
+    d.chroma_qp_index_offset = pps.chroma_qp_index_offset;
+    d.second_chroma_qp_index_offset = pps.second_chroma_qp_index_offset;
+    d.weighted_pred_flag = pps.weighted_pred;
+    d.weighted_bipred_idc = pps.weighted_bipred_idc;
+    d.luma_log2_weight_denom = sl.luma_log2_wd;
+    d.chroma_log2_weight_denom = sl.chroma_log2_wd;
+    d.poc = poc;
+    for (int l = 0; l < 2; l++) {
+        d.num_ref_idx_active[l] = sl.num_ref_idx[l];
+        d.luma_weight_flag_last[l] = sl.luma_flag_last[l];
+        for (int r = 0; r < 32; r++) {
+            d.ref_slot[l][r] = -1;
+            if (r < (int)lists[l].size() && lists[l][r] >= 0) {
+                const Frame &f = p->dpb[lists[l][r]];
+                d.ref_slot[l][r] = f.slot;
+                d.ref_poc[l][r] = f.poc;
+                d.ref_long_term[l][r] = f.long_ref ? 1 : 0;
+            }
+            if (explicit_wp(pps, sl) && r < sl.num_ref_idx[l]) {
+                d.luma_weight[l][r] = sl.lw[l][r];
+                d.luma_offset[l][r] = sl.lo[l][r];
+                for (int c = 0; c < 2; c++) {
+                    d.chroma_weight[l][r][c] = sl.cw[l][r][c];
+                    d.chroma_offset[l][r][c] = sl.co[l][r][c];
+                }
+            }
+        }
+    }
+}
+
+// Why a later slice of a picture cannot share the picture's descriptor, or null.
+const char *slice_mismatch(const Pps &pps, const Slice &a, const Slice &b, const std::vector<int> la[2],
+                           const std::vector<int> lb[2]) {
+    if (a.slice_type != b.slice_type) return "slice types differ";
+    if (a.frame_num != b.frame_num || a.pps_id != b.pps_id || a.idr != b.idr || (a.nal_ref_idc != 0) != (b.nal_ref_idc != 0))
+        return "frame_num / PPS / IDR / reference flag differ";
+    if (a.disable_dbk != b.disable_dbk || a.alpha_div2 != b.alpha_div2 || a.beta_div2 != b.beta_div2)
+        return "deblocking parameters differ";
+    for (int l = 0; l < 2; l++) {
+        if (a.num_ref_idx[l] != b.num_ref_idx[l] || la[l] != lb[l]) return "reference lists differ";
+        if (!explicit_wp(pps, a)) continue;
+        if (a.luma_log2_wd != b.luma_log2_wd || a.chroma_log2_wd != b.chroma_log2_wd || a.luma_flag_last[l] != b.luma_flag_last[l])
+            return "weight tables differ";
+        for (int r = 0; r < a.num_ref_idx[l]; r++)
+            if (a.lw[l][r] != b.lw[l][r] || a.lo[l][r] != b.lo[l][r] || memcmp(a.cw[l][r], b.cw[l][r], sizeof(a.cw[l][r])) ||
+                memcmp(a.co[l][r], b.co[l][r], sizeof(a.co[l][r])))
+                return "weight tables differ";
+    }
+    return nullptr;
+}
+
+}  // namespace
+
+extern "C" const char *h264g_parser_damage(const h264g_parser *p) { return p ? p->damage : ""; }
+
 extern "C" int32_t h264g_parser_next(h264g_parser *p, h264g_parsed_picture *out) {
     if (!p || !out) return h264g_set_error(H264G_E_ARG, "NULL argument");
     memset(out, 0, sizeof(*out));
     std::vector<uint8_t> rbsp;
     size_t begin, end;
-    while (next_nal(p, begin, end)) {
+    // the picture being assembled from its slices
+    bool started = false;
+    Slice first;
+    const Sps *sps = nullptr;
+    const Pps *pps = nullptr;
+    std::vector<int> lists[2];
+    int poc = 0, covered = 0, n_slices = 0, W = 0, H = 0, lost = 0, first_lost = -1;
+    p->damage[0] = 0;
+    for (;;) {
+        const size_t nal_pos = p->pos;
+        if (!next_nal(p, begin, end)) break;
         if (end <= begin) continue;
         const uint8_t hdr = p->data[begin];
         const int nal_type = hdr & 31, nal_ref_idc = (hdr >> 5) & 3;
-        if (nal_type != 1 && nal_type != 5 && nal_type != 7 && nal_type != 8) {
+        if (nal_type != 1 && nal_type != 5) {
             if (nal_type >= 2 && nal_type <= 4) return h264g_set_error(H264G_E_ARG, "data partitioning is not supported");
+            // parameter sets, SEI, access unit delimiters and sequence ends come before a
+            // picture's first slice: one of them ends the picture being assembled (7.4.1.2.3)
+            if (started && nal_type >= 6 && nal_type <= 11) {
+                p->pos = nal_pos;
+                break;
+            }
+            if (nal_type == 7 || nal_type == 8) {
+                unescape(&p->data[begin + 1], end - begin - 1, rbsp);
+                Bits bs;
+                bs.d = rbsp.data();
+                bs.nbits = rbsp.size() * 8;
+                if (int rc = nal_type == 7 ? parse_sps(bs, p->sps) : parse_pps(bs, p->pps, p->sps)) return rc;
+            }
             continue;  // SEI, AUD, end of sequence / stream, filler ...
         }
         unescape(&p->data[begin + 1], end - begin - 1, rbsp);
         Bits bs;
         bs.d = rbsp.data();
         bs.nbits = rbsp.size() * 8;
-        if (nal_type == 7) {
-            if (int rc = parse_sps(bs, p->sps)) return rc;
-            continue;
-        }
-        if (nal_type == 8) {
-            if (int rc = parse_pps(bs, p->pps, p->sps)) return rc;
-            continue;
-        }
-        // ---- a slice: the whole picture (one slice per picture)
         Slice sl;
-        const Sps *sps = nullptr;
-        const Pps *pps = nullptr;
-        if (int rc = parse_slice_header(bs, sl, nal_type, nal_ref_idc, p->sps, p->pps, &sps, &pps)) return rc;
-        if (sl.first_mb != 0) return h264g_set_error(H264G_E_ARG, "several slices per picture are not supported");
+        const Sps *ssps = nullptr;
+        const Pps *spps = nullptr;
+        if (int rc = parse_slice_header(bs, sl, nal_type, nal_ref_idc, p->sps, p->pps, &ssps, &spps)) return rc;
         if (sl.redundant_pic_cnt) continue;
+        if (started && sl.first_mb == 0) {  // the next picture's first slice
+            p->pos = nal_pos;
+            break;
+        }
         if (sl.slice_type == H264G_SLICE_B && !sl.direct_spatial)
             return h264g_set_error(H264G_E_ARG, "temporal direct prediction is not supported");
-        const int W = sps->width_mbs, H = sps->height_map_units;
-        if (p->cur_sps && (p->width_mbs != W || p->height_mbs != H))
-            return h264g_set_error(H264G_E_ARG, "the picture size changes inside the stream");
-        if (!p->cur_sps) {
-            p->width_mbs = W;
-            p->height_mbs = H;
-            p->dpb_slots = std::max(sps->max_num_ref_frames, 1) + 1;
-            if (h264g_mvfield_create(W, H, &p->field)) return h264g_set_error(H264G_E_NOMEM, "motion field");
-        }
-        p->cur_sps = sps;
-        const int max_fn = max_frame_num(*sps);
-        if (sl.idr) {
-            p->dpb.clear();
-            p->prev_frame_num = 0;
-        } else if (sl.frame_num != p->prev_frame_num && sl.frame_num != (p->prev_frame_num + 1) % max_fn) {
-            return h264g_set_error(H264G_E_ARG, "gaps in frame_num are not supported");
-        }
-        const int poc = decode_poc(p, *sps, sl);
-        // reference lists (DPB indices), then slots
-        std::vector<int> lists[2];
-        init_lists(p, *sps, sl, poc, lists);
-        for (int l = 0; l < (sl.slice_type == H264G_SLICE_B ? 2 : sl.slice_type == H264G_SLICE_P ? 1 : 0); l++) {
-            if (int rc = modify_list(p, *sps, sl, l, lists[l])) return rc;
-        }
-        h264g_pic_desc &d = out->desc;
-        d.slice_type = sl.slice_type;
-        d.structure = H264G_FRAME;
-        d.constrained_intra_pred_flag = pps->constrained_intra_pred;
-        d.disable_deblocking_filter_idc = sl.disable_dbk;
-        d.filter_offset_a = sl.alpha_div2 * 2;
-        d.filter_offset_b = sl.beta_div2 * 2;
-        d.chroma_qp_index_offset = pps->chroma_qp_index_offset;
-        d.second_chroma_qp_index_offset = pps->second_chroma_qp_index_offset;
-        d.weighted_pred_flag = pps->weighted_pred;
-        d.weighted_bipred_idc = pps->weighted_bipred_idc;
-        d.luma_log2_weight_denom = sl.luma_log2_wd;
-        d.chroma_log2_weight_denom = sl.chroma_log2_wd;
-        d.poc = poc;
-        for (int l = 0; l < 2; l++) {
-            d.num_ref_idx_active[l] = sl.num_ref_idx[l];
-            d.luma_weight_flag_last[l] = sl.luma_flag_last[l];
-            for (int r = 0; r < 32; r++) {
-                d.ref_slot[l][r] = -1;
-                d.ref_poc[l][r] = 0;
-                d.ref_long_term[l][r] = 0;
-                if (r < (int)lists[l].size() && lists[l][r] >= 0) {
-                    const Frame &f = p->dpb[lists[l][r]];
-                    d.ref_slot[l][r] = f.slot;
-                    d.ref_poc[l][r] = f.poc;
-                    d.ref_long_term[l][r] = f.long_ref ? 1 : 0;
-                }
-                const bool explicit_wp = (pps->weighted_pred && sl.slice_type == H264G_SLICE_P) ||
-                                         (pps->weighted_bipred_idc == 1 && sl.slice_type == H264G_SLICE_B);
-                if (explicit_wp && r < sl.num_ref_idx[l]) {
-                    d.luma_weight[l][r] = sl.lw[l][r];
-                    d.luma_offset[l][r] = sl.lo[l][r];
-                    for (int c = 0; c < 2; c++) {
-                        d.chroma_weight[l][r][c] = sl.cw[l][r][c];
-                        d.chroma_offset[l][r][c] = sl.co[l][r][c];
-                    }
-                }
+        if (!started) {
+            // ---- a new picture
+            if (sl.first_mb != 0) return h264g_set_error(H264G_E_ARG, "a picture starts with a slice at MB %d", sl.first_mb);
+            sps = ssps;
+            pps = spps;
+            W = sps->width_mbs;
+            H = sps->height_map_units;
+            if (p->cur_sps && (p->width_mbs != W || p->height_mbs != H))
+                return h264g_set_error(H264G_E_ARG, "the picture size changes inside the stream");
+            if (!p->cur_sps) {
+                p->width_mbs = W;
+                p->height_mbs = H;
+                p->dpb_slots = std::max(sps->max_num_ref_frames, 1) + 1;
+                if (h264g_mvfield_create(W, H, &p->field)) return h264g_set_error(H264G_E_NOMEM, "motion field");
             }
-        }
-        // a free slot: not held by a reference frame
-        int slot = -1;
-        for (int s = 0; s < p->dpb_slots && slot < 0; s++) {
-            bool used = false;
-            for (const Frame &f : p->dpb) used |= (f.short_ref || f.long_ref) && f.slot == s;
-            if (!used) slot = s;
-        }
-        if (slot < 0) return h264g_set_error(H264G_E_ARG, "DPB full: more reference frames than max_num_ref_frames");
-        d.dst_slot = slot;
-        // ---- slice data
-        PicState st{p, sps, pps, &sl, W, H, std::vector<MbNz>((size_t)W * H), &p->levels};
-        p->mbs.assign((size_t)W * H, h264g_mb());
-        p->levels.clear();
-        h264g_mvfield_reset(p->field);
-        if (sl.slice_type == H264G_SLICE_B) {
-            if (lists[1].empty() || lists[1][0] < 0) return h264g_set_error(H264G_E_ARG, "B slice without RefPicList1[0]");
-            const Frame &col = p->dpb[lists[1][0]];
-            if (int rc = h264g_mvfield_colocated(p->field, col.mbs.data(), col.short_ref ? 1 : 0)) return rc;
-        } else {
-            h264g_mvfield_colocated(p->field, nullptr, 0);
-        }
-        int mb = 0;
-        const int n_mbs = W * H;
-        if (int rc = pps->cabac ? parse_slice_data_cabac(bs, st, mb) : parse_slice_data_cavlc(bs, st, mb)) return rc;
-        if (mb != n_mbs) return h264g_set_error(H264G_E_ARG, "picture %d: the slice covers %d of %d MBs", p->pictures, mb, n_mbs);
-        if (bs.over) return h264g_set_error(H264G_E_ARG, "picture %d: slice data truncated", p->pictures);
-        // ---- DPB bookkeeping: the current frame, then marking
-        Frame cur;
-        cur.slot = slot;
-        cur.frame_num = sl.frame_num;
-        cur.poc = poc;
-        if (sl.nal_ref_idc) {
-            mark(p, *sps, sl, cur);
-            if (p->prev_mmco5) {  // 8.2.1: after memory_management_control_operation 5
-                cur.frame_num = 0;
-                p->prev_ref_top_poc = 0;
-                cur.poc = 0;
+            p->cur_sps = sps;
+            const int max_fn = max_frame_num(*sps);
+            if (sl.idr) {
+                p->dpb.clear();
+                p->prev_frame_num = 0;
+            } else if (sl.frame_num != p->prev_frame_num && sl.frame_num != (p->prev_frame_num + 1) % max_fn) {
+                return h264g_set_error(H264G_E_ARG, "gaps in frame_num are not supported");
             }
-            cur.mbs = p->mbs;
-            p->dpb.erase(std::remove_if(p->dpb.begin(), p->dpb.end(), [](const Frame &f) { return !f.short_ref && !f.long_ref; }),
-                         p->dpb.end());
-            p->dpb.push_back(cur);
-            p->prev_frame_num = cur.frame_num;
+            poc = decode_poc(p, *sps, sl);
+            if (int rc = slice_lists(p, *sps, sl, poc, lists)) return rc;
+            fill_desc(p, *pps, sl, poc, lists, out->desc);
+            int slot = -1;  // a slot not held by a reference frame
+            for (int k = 0; k < p->dpb_slots && slot < 0; k++) {
+                bool used = false;
+                for (const Frame &f : p->dpb) used |= (f.short_ref || f.long_ref) && f.slot == k;
+                if (!used) slot = k;
+            }
+            if (slot < 0) return h264g_set_error(H264G_E_ARG, "DPB full: more reference frames than max_num_ref_frames");
+            out->desc.dst_slot = slot;
+            p->mbs.assign((size_t)W * H, h264g_mb());
+            p->levels.clear();
+            h264g_mvfield_reset(p->field);
+            if (sl.slice_type == H264G_SLICE_B) {
+                if (lists[1].empty() || lists[1][0] < 0) return h264g_set_error(H264G_E_ARG, "B slice without RefPicList1[0]");
+                const Frame &col = p->dpb[lists[1][0]];
+                if (int rc = h264g_mvfield_colocated(p->field, col.mbs.data(), col.short_ref ? 1 : 0)) return rc;
+            } else {
+                h264g_mvfield_colocated(p->field, nullptr, 0);
+            }
+            first = sl;
+            started = true;
         } else {
-            p->prev_frame_num = sl.frame_num;
+            // ---- a later slice of the same picture: it must fit the picture's descriptor
+            if (spps != pps) return h264g_set_error(H264G_E_ARG, "picture %d: slices refer to different PPSs", p->pictures);
+            std::vector<int> sl_lists[2];
+            if (int rc = slice_lists(p, *sps, sl, poc, sl_lists)) return rc;
+            if (const char *why = slice_mismatch(*pps, first, sl, lists, sl_lists))
+                return h264g_set_error(H264G_E_ARG, "picture %d: its slices do not share one descriptor (%s)", p->pictures, why);
+            if (sl.disable_dbk == 0)  // the reference filters slice edges from the current MB's data (quirk Q7)
+                return h264g_set_error(H264G_E_ARG, "several slices per picture with deblocking across their edges "
+                                       "(disable_deblocking_filter_idc 0) are not supported");
         }
-        out->mbs = p->mbs.data();
-        out->n_mbs = (uint32_t)p->mbs.size();
-        out->levels = p->levels.data();
-        out->n_levels = p->levels.size();
-        out->width_mbs = W;
-        out->height_mbs = H;
-        out->dpb_slots = p->dpb_slots;
-        out->max_num_reorder_frames = sps->max_num_reorder_frames;
-        out->idr = sl.idr;
-        out->nal_ref_idc = sl.nal_ref_idc;
-        out->frame_num = sl.frame_num;
-        out->profile_idc = sps->profile_idc;
-        out->constraint_set3 = sps->constraint3;
-        out->level_idc = sps->level_idc;
-        p->pictures++;
-        return 1;
+        if (sl.first_mb < covered || sl.first_mb >= W * H)
+            return h264g_set_error(H264G_E_ARG, "picture %d: slice at MB %d after MB %d", p->pictures, sl.first_mb, covered);
+        for (; covered < sl.first_mb; covered++) {  // MBs the slices left out
+            if (first_lost < 0) first_lost = covered;
+            lost++;
+            if (int rc = undecoded_mb(p, covered, H264G_SLICE_NONE)) return rc;
+        }
+        // ---- slice data: neighbours of other slices are not available (fresh per-slice state)
+        PicState st{p, sps, pps, &sl, W, H, std::vector<MbNz>((size_t)W * H), &p->levels, (uint32_t)n_slices};
+        st.mark = p->levels.size();
+        int mb = sl.first_mb;
+        int rc = pps->cabac ? parse_slice_data_cabac(bs, st, mb) : parse_slice_data_cavlc(bs, st, mb);
+        if (!rc && bs.over) rc = h264g_set_error(H264G_E_ARG, "picture %d: slice data truncated", p->pictures);
+        if (rc && mb < W * H) {
+            // Damaged slice data: the slice ends with the MB that failed, and decoding goes on
+            // with the next slice, as the reference's does (H264VideoDecoder.cpp:135-136,
+            // CONTINUE_IF_FAILED).  An MB whose residual failed (CAVLC) is reconstructed from
+            // what parsed; any other failed MB stays undecoded.
+            snprintf(p->damage, sizeof(p->damage), "%s", h264g_last_error());
+            if (first_lost < 0) first_lost = mb;
+            lost++;
+            if (!st.partial) {
+                p->levels.resize(st.mark);
+                if (int rc2 = undecoded_mb(p, mb, st.slice_num)) return rc2;
+            }
+            mb++;
+        } else if (rc) {
+            return rc;
+        }
+        covered = mb;
+        n_slices++;
     }
-    return 0;
+    if (!started) return 0;
+    const int n_mbs = W * H;
+    for (; covered < n_mbs; covered++) {  // MBs after the last slice
+        if (first_lost < 0) first_lost = covered;
+        lost++;
+        if (int rc = undecoded_mb(p, covered, H264G_SLICE_NONE)) return rc;
+    }
+    // ---- DPB bookkeeping: the current frame, then marking
+    const Slice &sl = first;
+    Frame cur;
+    cur.slot = out->desc.dst_slot;
+    cur.frame_num = sl.frame_num;
+    cur.poc = poc;
+    if (sl.nal_ref_idc) {
+        mark(p, *sps, sl, cur);
+        if (p->prev_mmco5) {  // 8.2.1: after memory_management_control_operation 5
+            cur.frame_num = 0;
+            p->prev_ref_top_poc = 0;
+            cur.poc = 0;
+        }
+        cur.mbs = p->mbs;
+        p->dpb.erase(std::remove_if(p->dpb.begin(), p->dpb.end(), [](const Frame &f) { return !f.short_ref && !f.long_ref; }),
+                     p->dpb.end());
+        p->dpb.push_back(cur);
+        p->prev_frame_num = cur.frame_num;
+    } else {
+        p->prev_frame_num = sl.frame_num;
+    }
+    out->mbs = p->mbs.data();
+    out->n_mbs = (uint32_t)p->mbs.size();
+    out->levels = p->levels.data();
+    out->n_levels = p->levels.size();
+    out->width_mbs = W;
+    out->height_mbs = H;
+    out->dpb_slots = p->dpb_slots;
+    out->max_num_reorder_frames = sps->max_num_reorder_frames;
+    out->idr = sl.idr;
+    out->nal_ref_idc = sl.nal_ref_idc;
+    out->frame_num = sl.frame_num;
+    out->profile_idc = sps->profile_idc;
+    out->constraint_set3 = sps->constraint3;
+    out->level_idc = sps->level_idc;
+    out->n_slices = n_slices;
+    out->lost_mbs = lost;
+    out->first_lost_mb = first_lost;
+    p->pictures++;
+    return 1;
 }

@@ -261,6 +261,10 @@ private:
     void gen_mb(int mx, int my) {
         h264g_mb &mb = pic.mbs[(size_t)my * W + mx];
         init_mb(mb);
+        if (P.slices > 1) {
+            const int run = (W * H + P.slices - 1) / P.slices;
+            mb.slice_num = (uint32_t)((my * W + mx) / run);
+        }
         // neighbours for intra prediction: inside the picture and, under constrained intra
         // prediction, not an MB whose prediction mode the reference counts as inter
         // (Pred_L0 / Pred_L1 / BiPred: P_Skip and the P / B MBs not split into 8x8; P_8x8
@@ -268,6 +272,7 @@ private:
         auto avail = [&](int x, int y) {
             if (!mb_exists(x, y)) return false;
             const h264g_mb &n = pic.mbs[(size_t)y * W + x];
+            if (n.slice_num != mb.slice_num) return false;  // another slice
             const bool hides = n.mb_type == H264G_MB_P_SKIP ||
                                ((n.mb_type == H264G_MB_P || n.mb_type == H264G_MB_B) && n.part_shape != H264G_PART_8x8);
             return !P.constrained_intra_pred || !hides;
@@ -823,6 +828,10 @@ int32_t h264g_gen_create(const h264g_gen_params *params, h264g_gen **out) {
     // (H264InterPrediction.cpp:2958)
     if (p.direct_permille < 0 || p.direct_permille > 1000 || (p.direct_permille && p.weighted_bipred_idc == 2))
         return h264g_set_error(H264G_E_ARG, "gen: direct MBs need direct_permille in 0..1000 and no implicit weights");
+    // several slices: the engine filters no edge between slices (idc 2) or none at all (1)
+    if (p.slices < 0 || p.slices > p.width_mbs * p.height_mbs ||
+        (p.slices > 1 && (p.mbaff || (p.disable_deblocking_filter_idc != 1 && p.disable_deblocking_filter_idc != 2))))
+        return h264g_set_error(H264G_E_ARG, "gen: several slices need frames and disable_deblocking_filter_idc 1 or 2");
     h264g_gen *g = new h264g_gen();
     g->p = p;
     std::vector<PicPlan> plan = plan_gop(p.n_frames, p.b_frames, p.gop);

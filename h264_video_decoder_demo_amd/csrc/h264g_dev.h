@@ -78,6 +78,8 @@ struct DevPic {
     const int8_t *bsref;      // MBAFF frames: per MB [list][quadrant] the picture identity
                               // the bS derivation compares, RefPicListX[RefIdxLX] (-1: none)
     uint32_t *pairprog;       // MBAFF streams: per-MB-pair deblocking progress words
+    uint32_t dbk_stop;        // MBs from this address on filter none of their edges (the
+                              // first MB of no slice; W * H when every MB has one)
 };
 
 // Work item: picture index in the batch and macroblock address.

@@ -1716,6 +1716,10 @@ __device__ __forceinline__ Avail intra_avail(const DevPic &P, int addr, int deps
         deps[k] = -1;
         if (!(av.m & (1u << k))) continue;
         const int c = G(P.mbs)[nb[k]].mb_type;
+        if (G(P.mbs)[nb[k]].slice_num != G(P.mbs)[addr].slice_num) {  // another slice (6.4.8)
+            av.m &= ~(1u << k);
+            continue;
+        }
         if (P.cip && cip_hides(G(P.mbs)[nb[k]])) av.m &= ~(1u << k);
         if (cls_intra(c)) deps[k] = nb[k];
     }
@@ -2459,8 +2463,14 @@ __global__ __launch_bounds__(WAVE) void k_bs(const DevPic *__restrict__ pics, co
     const int addr = (int)(it & WI_MB_MASK), W = P.W, mbx = addr % W, mby = addr / W;
     const int j = lane & 31, dir = j >> 4, e = ((j >> 2) & 3) * 4, k = (j & 3) * 4;
     int bS = 0;
-    if (e > 0 || (dir == 0 ? mbx > 0 : mby > 0)) {
-        const int ap = e ? addr : (dir == 0 ? addr - 1 : addr - W);
+    const int ap = e ? addr : (dir == 0 ? addr - 1 : addr - W);
+    // an MB edge at the picture border is not filtered, nor, with disable_deblocking_filter_idc
+    // 2, one between two slices (8.7: filterLeftMbEdgeFlag / filterTopMbEdgeFlag); no edge
+    // of an MB at or after the deblocking stop is
+    const bool edge = (uint32_t)addr < P.dbk_stop &&
+                      (e > 0 || ((dir == 0 ? mbx > 0 : mby > 0) &&
+                                 !(P.dbk_idc == 2 && G(P.mbs)[ap].slice_num != G(P.mbs)[addr].slice_num)));
+    if (edge) {
         MbView Qv, Pv;
         load_view(P, addr, Qv);
         load_view(P, ap, Pv);

@@ -13,9 +13,10 @@
 //
 // The field keeps, per 4x4 luma block of the picture, the reference index and vector
 // of each list and whether the block's partition has been decoded.  A neighbouring
-// partition (6.4.11.7) in another MB is available when its block has been decoded: MBs
-// are derived in decoding (raster) order and a picture is one slice, so an MB outside
-// the picture, the MB to the right and the MBs below are never decoded.
+// partition (6.4.11.7) in another MB is available when its block has been decoded and
+// its MB lies in the current MB's slice (h264g_mb.slice_num, 6.4.8): MBs are derived in
+// decoding (raster) order, so an MB outside the picture, the MB to the right and the MBs
+// below are never decoded.
 //
 // Inside the current MB the reference departs from 6.4.11.7: its test that marks a
 // not yet decoded partition as not available is commented out
@@ -82,6 +83,8 @@ struct h264g_mvfield {
     std::vector<uint8_t> cls;    // per MB: CLS_*, CLS_NONE until derived in this picture
     std::vector<uint8_t> imode;  // per MB: 16 modes by luma4x4BlkIdx (I4x4 / I8x8 MBs)
     std::vector<uint8_t> colzero;  // per MB: bit q = colZeroFlag of quadrant q (spatial direct)
+    std::vector<uint32_t> slice;   // per MB: slice_num of the derived MBs
+    uint32_t cur_slice = 0;        // slice_num of the MB being derived
     bool have_col = false;
 
     Blk *at(int bx, int by) { return &b[(size_t)by * W * 4 + bx]; }
@@ -103,7 +106,8 @@ struct h264g_mvfield {
             c = cur_cls;
             modes = cur;
         } else {
-            if (ax < 0 || ay < 0 || ax >= W || cls[(size_t)ay * W + ax] == CLS_NONE) {
+            if (ax < 0 || ay < 0 || ax >= W || cls[(size_t)ay * W + ax] == CLS_NONE ||
+                slice[(size_t)ay * W + ax] != cur_slice) {
                 dc = true;
                 return 2;
             }
@@ -122,10 +126,12 @@ struct h264g_mvfield {
         const int bx = mx * 4 + x4, by = my * 4 + y4;
         if (bx < 0 || by < 0 || bx >= W * 4 || by >= H * 4) return n;
         const Blk *k = at(bx, by);
+        const bool in_cur = x4 >= 0 && x4 < 4 && y4 >= 0 && y4 < 4;
         if (!k->decoded) {
-            n.avail = x4 >= 0 && x4 < 4 && y4 >= 0 && y4 < 4;  // current MB: available, no motion
+            n.avail = in_cur;  // current MB: available, no motion
             return n;
         }
+        if (!in_cur && slice[(size_t)(by >> 2) * W + (bx >> 2)] != cur_slice) return n;  // another slice
         n.avail = true;
         n.ref = k->ref[l];
         if (n.ref >= 0) {
@@ -194,6 +200,7 @@ int32_t h264g_mvfield_create(int32_t width_mbs, int32_t height_mbs, h264g_mvfiel
         f->cls.assign((size_t)width_mbs * height_mbs, CLS_NONE);
         f->imode.assign((size_t)width_mbs * height_mbs * 16, 2);
         f->colzero.assign((size_t)width_mbs * height_mbs, 0);
+        f->slice.assign((size_t)width_mbs * height_mbs, 0);
     } catch (...) {
         delete f;
         return h264g_set_error(H264G_E_NOMEM, "mvfield");
@@ -246,6 +253,7 @@ int32_t h264g_mvfield_intra_modes(h264g_mvfield *f, uint32_t mb_addr, h264g_mb *
         return h264g_set_error(H264G_E_ARG, "mvfield_intra_modes: MB %u type %d is not I_NxN", mb_addr, mb->mb_type);
     const int mx = (int)(mb_addr % (uint32_t)f->W), my = (int)(mb_addr / (uint32_t)f->W);
     const bool cip = constrained_intra_pred != 0;
+    f->cur_slice = mb->slice_num;
     uint8_t cur[16];
     std::memset(cur, 2, sizeof(cur));
     const int nblk = i8 ? 4 : 16;
@@ -271,6 +279,8 @@ int32_t h264g_mvfield_derive(h264g_mvfield *f, uint32_t mb_addr, h264g_mb *mb, c
     if (!f || !mb) return h264g_set_error(H264G_E_ARG, "mvfield_derive: null argument");
     if (mb_addr >= (uint32_t)(f->W * f->H)) return h264g_set_error(H264G_E_ARG, "mvfield_derive: MB %u", mb_addr);
     const int mx = (int)(mb_addr % (uint32_t)f->W), my = (int)(mb_addr / (uint32_t)f->W);
+    f->cur_slice = mb->slice_num;
+    f->slice[mb_addr] = mb->slice_num;
     for (int y = 0; y < 4; y++)  // a partition of this MB is available only once derived
         for (int x = 0; x < 4; x++) f->at(mx * 4 + x, my * 4 + y)->decoded = 0;
     {  // the MB's class and intra modes, for the intra-mode prediction of later MBs

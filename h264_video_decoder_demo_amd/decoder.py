@@ -17,6 +17,7 @@ per-picture syntax (the generator, or a syntax file written by ``syntax_io``).
 """
 from __future__ import annotations
 
+import warnings
 from dataclasses import dataclass
 
 import numpy as np
@@ -148,7 +149,15 @@ class H264VideoDecoder:
         stopped = False
         byte_stream = isinstance(source, Bitstream)
         if byte_stream:
-            items = _with_last((p.desc, p.mbs, p.levels, p.idr) for p in source.pictures())
+            def parsed():
+                for n, p in enumerate(source.pictures()):
+                    if p.lost_mbs:
+                        # the reference abandons a slice whose data fails and decodes on
+                        # (H264VideoDecoder.cpp:135-136); its MBs from there stay zero
+                        warnings.warn(f"picture {n} (POC {p.desc.poc}): {p.lost_mbs} MBs from MB {p.first_lost_mb} "
+                                      f"not decoded ({p.damage})", RuntimeWarning, stacklevel=3)
+                    yield p.desc, p.mbs, p.levels, p.idr
+            items = _with_last(parsed())
         else:
             items = ((source.picture(i) + (source.is_idr(i),), False) for i in range(source.num_pictures))
         try:

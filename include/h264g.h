@@ -87,6 +87,18 @@ extern "C" {
  * QPY = QPY,PREV).  Some bits: B_8x8 with B_Direct_8x8 sub-macroblocks.  The device path
  * reads only ref_idx and mv; the host derivation (h264g_mvfield_*) fills them. */
 #define H264G_MBF_SKIP    0x04u
+/* Marks (for the caller; reconstruction reads the record as it stands) an MB of a
+ * damaged slice: the MB whose slice data failed to parse, reconstructed from the syntax
+ * that parsed before the failure with QPY 0 (the reference's reset value), or, when no
+ * syntax of it is usable, an I_PCM of zero samples; and every MB that no slice reached, an
+ * I_PCM of zero samples with QPY 0 and slice_num H264G_SLICE_NONE, as the reference leaves
+ * such MBs as each picture's reset zeroed them (H264PictureBase.cpp:45-67).
+ * Deblocking stops at the first MB (address order) whose slice_num is H264G_SLICE_NONE:
+ * that MB and every later one filter none of their edges, as the reference's bS
+ * derivation fails on the reset MB type (H264InterPrediction.cpp:3182-3186) and the
+ * failure ends the picture's deblocking (H264PictureBase.cpp:707-708). */
+#define H264G_MBF_LOST    0x08u
+#define H264G_SLICE_NONE  0xFFFFFFFFu
 #define H264G_MBF_DIRECT(q) (0x10u << (q))
 #define H264G_MBF_DIRECT_ALL 0xF0u
 
@@ -120,7 +132,12 @@ typedef struct h264g_mb {
     int8_t   ref_idx[2][4];
     int16_t  mv[2][16][2];
     uint32_t level_offset;            /* first int16 of this MB's level block        */
-    uint32_t reserved1;
+    uint32_t slice_num;               /* the MB's slice: neighbours with another value
+                                         are not available for intra prediction and
+                                         motion vector prediction, and with
+                                         disable_deblocking_filter_idc 2 their common
+                                         edge is not filtered (0 for every MB of a
+                                         picture of one slice)                       */
 } h264g_mb;
 
 /*
@@ -417,6 +434,10 @@ typedef struct h264g_gen_params {
     int32_t  direct_permille;     /* B pictures: share of spatial-direct MBs (B_Skip,
                                      B_Direct_16x16, B_8x8 with B_Direct_8x8 sub-MBs in
                                      equal parts; frames, no implicit weights)          */
+    int32_t  slices;              /* slices per picture (0 or 1: one), runs of
+                                     ceil(W * H / slices) MBs in address order; intra
+                                     prediction sees no neighbour of another slice.
+                                     Frames, disable_deblocking_filter_idc 1 or 2       */
 } h264g_gen_params;
 
 /* ------------------------------------------------ host motion-vector derivation
@@ -435,7 +456,7 @@ void    h264g_mvfield_destroy(h264g_mvfield *field);
 int32_t h264g_mvfield_reset(h264g_mvfield *field);
 /* MB mb_addr of the picture, called for every MB in decoding (raster) order, intra
  * MBs included.  In: mb_type, part_shape, sub_part_shape, ref_idx (P_Skip: set to
- * refIdxL0 = 0 here).  mvd: int16 [list][16][2], the mvd of each (sub-)partition and
+ * refIdxL0 = 0 here), slice_num (neighbours of another slice are not available).  mvd: int16 [list][16][2], the mvd of each (sub-)partition and
  * list at its top-left 4x4 block (raster index, the layout of mb->mv), read for the
  * lists with ref_idx >= 0; NULL for intra and P_Skip MBs.  Out: mb->mv of every 4x4
  * block of the used lists.  H264G_E_ARG for a type the field does not derive. */
@@ -463,18 +484,23 @@ int32_t h264g_mvfield_intra_modes(h264g_mvfield *field, uint32_t mb_addr, h264g_
 int32_t h264g_mvfield_colocated(h264g_mvfield *field, const h264g_mb *col_mbs, int32_t col_short_term);
 
 /* ------------------------------------------------------------ bitstream front end
- * Annex B byte stream of frame pictures coded with CAVLC, one slice per picture (the
- * reference's CAVLC sample streams): NAL units, SPS / PPS, slice headers, slice data and
- * the decoded picture buffer (POC types 0 and 2, sliding-window and adaptive reference
- * marking, reference list initialisation and modification, DPB slots).  Replaces the
- * reference's parsing and picture management on the way to reconstruction:
- * CH264NalUnit / CH264SPS / CH264PPS / CH264SliceHeader (H264NalUnit.cpp:122,
- * H264SliceHeader.cpp:289-509), CH264SliceData::slice_data (H264SliceData.cpp:386-487),
- * CH264MacroBlock::macroblock_layer (H264MacroBlock.cpp:912-1139),
- * CH264ResidualBlockCavlc (H264ResidualBlockCavlc.cpp) and H264RefPicList.cpp.
- * Refused with H264G_E_ARG: CABAC, field pictures and MBAFF, several slices per picture,
- * slice groups, scaling matrices, formats other than 8-bit 4:2:0, SP / SI slices,
- * temporal direct prediction, POC type 1, gaps in frame_num.  Host only. */
+ * Annex B byte stream of frame pictures coded with CAVLC or CABAC, in one slice or several
+ * per picture: NAL units, SPS / PPS, slice headers, slice data and the decoded picture
+ * buffer (POC types 0 and 2, sliding-window and adaptive reference marking, reference
+ * list initialisation and modification, DPB slots).  Replaces the reference's parsing and
+ * picture management on the way to reconstruction: CH264NalUnit / CH264SPS / CH264PPS /
+ * CH264SliceHeader (H264NalUnit.cpp:122, H264SliceHeader.cpp:289-509),
+ * CH264SliceData::slice_data (H264SliceData.cpp:386-487), CH264MacroBlock::macroblock_layer
+ * (H264MacroBlock.cpp:912-1139), CH264ResidualBlockCavlc (H264ResidualBlockCavlc.cpp),
+ * CH264Cabac (H264Cabac.cpp) and H264RefPicList.cpp.
+ * Slices of one picture must share one descriptor (slice type, reference lists, weights,
+ * deblocking parameters); several slices with disable_deblocking_filter_idc 0 are
+ * refused.  Slice data that fails to parse ends its slice where it fails, and MBs that no
+ * slice decodes are H264G_MBF_LOST records, as the reference skips the rest of such a
+ * slice and decodes on (H264VideoDecoder.cpp:135-136); lost_mbs counts them.
+ * Refused with H264G_E_ARG: field pictures and MBAFF, slice groups, data partitioning,
+ * scaling matrices, formats other than 8-bit 4:2:0, SP / SI slices, temporal direct
+ * prediction, POC type 1, gaps in frame_num.  Host only. */
 typedef struct h264g_parser h264g_parser;
 typedef struct h264g_parsed_picture {
     h264g_pic_desc desc;          /* ready for h264g_picture_begin (dst_slot, ref_slot ...) */
@@ -487,6 +513,9 @@ typedef struct h264g_parsed_picture {
     int32_t max_num_reorder_frames;  /* from the VUI, -1 when absent                       */
     int32_t idr, nal_ref_idc, frame_num;
     int32_t profile_idc, constraint_set3, level_idc;
+    int32_t n_slices;             /* slices that made the picture                          */
+    int32_t lost_mbs;             /* H264G_MBF_LOST records (0 for an intact picture)       */
+    int32_t first_lost_mb;        /* address of the first of them, -1 when none             */
 } h264g_parsed_picture;
 /* The parser keeps a copy of the byte stream. */
 int32_t h264g_parser_create(const uint8_t *data, uint64_t len, h264g_parser **out);
@@ -494,6 +523,9 @@ int32_t h264g_parser_destroy(h264g_parser *parser);
 /* The next picture in decoding order: 1 with *out filled, 0 at the end of the stream,
  * negative on an error (h264g_last_error names the picture and MB). */
 int32_t h264g_parser_next(h264g_parser *parser, h264g_parsed_picture *out);
+/* Why the last picture's damaged slice stopped (the parse error at its first_lost_mb),
+ * or "" when its slices were intact; valid until the next h264g_parser_next. */
+const char *h264g_parser_damage(const h264g_parser *parser);
 
 int32_t h264g_gen_default_params(int32_t config, h264g_gen_params *out);
 int32_t h264g_gen_create(const h264g_gen_params *params, h264g_gen **out);

@@ -2,7 +2,7 @@
  * refcpu.c -- TEST INFRASTRUCTURE ONLY: the oracle.
  *
  * Plain-C restatement of the picture reconstruction of jfu222/h264_video_decoder_demo
- * for progressive frames (one slice per picture): dequantisation + inverse
+ * for progressive frames (one slice or several per picture): dequantisation + inverse
  * transforms (8.5), inter prediction with fractional-sample interpolation and
  * weighted prediction (8.4.2 / 8.4.3), intra prediction (8.3) and the deblocking
  * filter (8.7), including the reference's deterministic deviations from the spec:
@@ -37,6 +37,14 @@
  *       Pred_L0 / Pred_L1 / BiPred (IS_INTER_Prediction_Mode, H264CommonFunc.h:273;
  *       H264PictureBase.cpp:1129, 1492, 1905, 2158): P_8x8 / B_8x8 MBs (Pred_NA,
  *       H264MacroBlock.cpp:99-100, 142) stay available to intra prediction.
+ * Slices: a neighbour MB of another slice (h264g_mb.slice_num) is not available
+ * (H264PictureBase.cpp:2890-2937); with disable_deblocking_filter_idc 2 MB edges
+ * between slices are not filtered (Deblock.cpp:128-153).
+ *   Q22 the deblocking of a picture stops at its first MB of no slice
+ *       (H264G_SLICE_NONE, an MB that no slice decoded): the bS derivation fails on the
+ *       MB's reset type MB_TYPE_NA (H264InterPrediction.cpp:3182-3186) and
+ *       end_decode_the_picture ignores the failure (H264PictureBase.cpp:707-708), so
+ *       that MB and every later one filter none of their edges.
  * Each function cites the reference code it follows.  Used only by tests/, smoke()
  * and bench.py's CPU baseline, as the checker.
  */
@@ -340,6 +348,7 @@ static int frame_nb(const Ctx *X, int addr, int xN, int yN, int n, int *xW, int 
     else if (yN < 0) { mx++; my--; }
     else return -1;
     if (mx < 0 || my < 0 || mx >= X->W || my >= X->H) return -1;
+    if (X->mbs[my * X->W + mx].slice_num != X->mbs[addr].slice_num) return -1;  /* another slice */
     *xW = (xN + n) % n;
     *yW = (yN + n) % n;
     return my * X->W + mx;
@@ -1147,13 +1156,18 @@ static void deblock_mbaff(Ctx *X) {
     }
 }
 
-/* 8.7 in MB raster order (Deblock.cpp:76-635) */
+/* 8.7 in MB raster order (Deblock.cpp:76-635), up to the first MB of no slice (Q22) */
 static void deblock(Ctx *X) {
-    if (X->d->disable_deblocking_filter_idc == 1) return;
+    const int idc = X->d->disable_deblocking_filter_idc;
+    if (idc == 1) return;
     for (int mby = 0; mby < X->H; mby++)
         for (int mbx = 0; mbx < X->W; mbx++) {
-            const MbInfo *I = &X->info[mby * X->W + mbx];
-            int left = mbx > 0, topf = mby > 0;
+            const int a = mby * X->W + mbx;
+            const MbInfo *I = &X->info[a];
+            const uint32_t sn = X->mbs[a].slice_num;
+            if (sn == H264G_SLICE_NONE) return;
+            int left = mbx > 0 && !(idc == 2 && X->mbs[a - 1].slice_num != sn);
+            int topf = mby > 0 && !(idc == 2 && X->mbs[a - X->W].slice_num != sn);
             if (left) filter_edge(X, mbx, mby, 0, 1, 0);
             if (!I->t8) filter_edge(X, mbx, mby, 0, 1, 4);
             filter_edge(X, mbx, mby, 0, 1, 8);

@@ -1,7 +1,8 @@
 """The bitstream front end (h264g_parser_*) on prefixes of the reference's own sample
-streams (tests/golden/heavyhand_*, written by make_real.py from the reference decoder's
-output): parse on the CPU, reconstruct on the GPU, and compare every output picture with
-the reference decoder's."""
+streams (tests/golden/heavyhand_* and gop121_12, written by make_real.py from the reference
+decoder's output): parse on the CPU, reconstruct with the oracle on the CPU and on the
+GPU, and compare every output picture with the reference decoder's."""
+import contextlib
 import hashlib
 import json
 import os
@@ -12,9 +13,12 @@ import pytest
 from h264_video_decoder_demo_amd import _lib
 from h264_video_decoder_demo_amd.bitstream import Bitstream
 from h264_video_decoder_demo_amd.decoder import H264_DECODE_ERROR_CODE_FILE_END, H264VideoDecoder
+from oracle.refcpu import Dpb
 
 HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
-STREAMS = ["heavyhand_cavlc_15", "heavyhand_cabac_10"]  # CAVLC and CABAC codings of one clip
+CODINGS = ["heavyhand_cavlc_15", "heavyhand_cabac_10"]  # CAVLC and CABAC codings of one clip
+# + Baseline, 9 slices per picture with disable_deblocking_filter_idc 2, a damaged slice
+STREAMS = CODINGS + ["gop121_12"]
 
 
 def _stream(name):
@@ -72,7 +76,7 @@ def test_cavlc_and_cabac_codings_parse_to_the_same_kinds_of_mbs():
     """The two codings of the clip (different encoder runs, so different decisions) both
     use every MB class the engine takes: I_NxN with both transform sizes, I_16x16, P and
     B partitions, skips and direct prediction."""
-    for name in STREAMS:
+    for name in CODINGS:
         kinds = set()
         for p in Bitstream(_stream(name)).pictures():
             kinds |= {int(t) for t in np.unique(p.mbs["mb_type"])}
@@ -121,6 +125,53 @@ def test_parser_refuses_field_coding_and_damaged_streams():
         Bitstream(b"\x00\x00\x01\x09\xf0")  # an access unit delimiter, no picture
 
 
+def test_damaged_slice_ends_where_the_reference_abandons_it():
+    """gop121's first picture: nine slices, the last of which fails at MB 7713 (its
+    run_before code is invalid: the reference's parse fails at the same MB).  That MB keeps
+    what parsed before the failure (Intra16x16 DC prediction, no levels, QPY 0 as the
+    reference leaves it); the MBs after it belong to no slice and are zero I_PCM records;
+    every later picture is whole."""
+    pics = list(Bitstream(_stream("gop121_12")).pictures())
+    p0 = pics[0]
+    assert (p0.n_slices, p0.lost_mbs, p0.first_lost_mb) == (9, 447, 7713)
+    assert "invalid run_before" in p0.damage
+    m = p0.mbs[7713]
+    assert (m["mb_type"], m["intra16x16_pred_mode"], m["qp_y"], m["slice_num"]) == (_lib.MB_I16x16, 2, 0, 8)
+    assert m["flags"] & _lib.MBF_LOST
+    nmask = 1 + 16  # Intra16x16DCLevel + 16 AC blocks (cbp luma 15), no chroma
+    assert not np.any(p0.levels[m["level_offset"]:m["level_offset"] + nmask])
+    rest = p0.mbs[7714:]
+    assert np.all(rest["mb_type"] == _lib.MB_IPCM) and np.all(rest["slice_num"] == _lib.SLICE_NONE)
+    assert np.all(rest["flags"] & _lib.MBF_LOST) and np.all(rest["qp_y"] == 0)
+    assert not np.any(p0.mbs[:7713]["flags"] & _lib.MBF_LOST)
+    # slices in order: slice_num rises with the address up to the damage
+    assert np.all(np.diff(p0.mbs[:7714]["slice_num"].astype(np.int64)) >= 0)
+    assert sorted(set(p0.mbs[:7714]["slice_num"].tolist())) == list(range(9))
+    for p in pics[1:]:
+        assert (p.n_slices, p.lost_mbs, p.first_lost_mb, p.damage) == (9, 0, -1, "")
+
+
+@pytest.mark.parametrize("name", STREAMS)
+def test_oracle_reproduces_the_reference_decoder(name):
+    """The oracle (refcpu.c) reconstructs every parsed picture of the reference's own
+    sample streams to the reference decoder's planes, byte for byte: multi-slice pictures,
+    the damaged slice and the deblocking stop after it included (the stream's last picture
+    unfiltered, as the reference outputs it).  This pins the oracle on real data; the GPU
+    test below then holds the engine to the same digests."""
+    bs = Bitstream(_stream(name))
+    ref = {r["poc"]: r for r in _ref(name)}
+    dpb = Dpb(bs.width_mbs, bs.height_mbs, bs.dpb_slots)
+    pics = list(bs.pictures())
+    for i, p in enumerate(pics):
+        d = p.desc
+        if i == len(pics) - 1:
+            d.disable_deblocking_filter_idc = 1
+        assert dpb.decode(d, p.mbs, p.levels) == 0
+        got = tuple(hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest() for a in dpb.split(d.dst_slot))
+        r = ref[d.poc]
+        assert got == (r["y"], r["cb"], r["cr"]), f"{name}: POC {d.poc}"
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("name", STREAMS)
 def test_real_stream_equals_reference_decoder(engine, name):
@@ -142,7 +193,9 @@ def test_real_stream_equals_reference_decoder(engine, name):
 
     dec = H264VideoDecoder(engine)
     dec.set_output_frame_callback_functuin(callback, None)
-    assert dec.open(_stream(name)) == 0
+    damaged = name.startswith("gop121")
+    with pytest.warns(RuntimeWarning, match="not decoded") if damaged else contextlib.nullcontext():
+        assert dec.open(_stream(name)) == 0
     assert [g[0] for g in got] == [r["poc"] for r in ref]
     for g, r in zip(got, ref):
         assert g[1:] == (r["y"], r["cb"], r["cr"]), f"{name}: POC {r['poc']} (decoding index {r['decode_index']})"

@@ -301,6 +301,94 @@ def test_dependent_pictures_of_one_stream_are_refused(engine):
         engine.close_stream(sid)
 
 
+def _decode_all(engine, gen, pictures):
+    """Every (desc, mbs, levels) of pictures through the engine, compared with the oracle
+    decoding the same records on its own DPB."""
+    from oracle.refcpu import Dpb
+    dpb = Dpb(gen.width_mbs, gen.height_mbs, gen.dpb_slots)
+    sid = engine.open_stream_for(gen)
+    try:
+        for i, (desc, mbs, levels) in enumerate(pictures):
+            assert dpb.decode(desc, mbs, levels, 2) == 0
+            pic = engine.begin(sid, desc, mbs, levels)
+            engine.submit(pic)
+            engine.sync()
+            got = engine.readback(sid, desc.dst_slot)
+            engine.release(pic)
+            for gp, wp, pl in zip(got, dpb.split(desc.dst_slot), ("Y", "Cb", "Cr")):
+                d = first_diff(gp, wp, 16 if pl == "Y" else 8)
+                assert d is None, f"picture {i} {pl}: {d}"
+    finally:
+        engine.close_stream(sid)
+
+
+@pytest.mark.parametrize("slices,idc", [(7, 2), (23, 2), (3, 1)])
+def test_multi_slice_pictures_match_oracle(any_engine, slices, idc):
+    """Pictures of several slices (runs of MBs that end mid-row): intra prediction sees no
+    neighbour of another slice (6.4.8) and, with disable_deblocking_filter_idc 2, MB edges
+    between slices are not filtered; I, P and B pictures with a third of the inter
+    pictures' MBs intra, against the oracle."""
+    gen = Generator(2, width_mbs=23, height_mbs=9, n_frames=5, seed=0x51CE0000 + slices, slices=slices,
+                    disable_deblocking_filter_idc=idc, intra_in_inter_permille=330)
+    pics = [gen.picture(i) for i in range(gen.num_pictures)]
+    for _, mbs, _ in pics:
+        assert len(np.unique(mbs["slice_num"])) == slices
+    _decode_all(any_engine, gen, pics)
+
+
+def _lose_tail(desc, mbs, levels, first):
+    """The picture with MBs first.. replaced by what a damaged slice leaves: zero I_PCM
+    records of no slice."""
+    from h264_video_decoder_demo_amd import _lib
+    mbs = mbs.copy()
+    tail = mbs[first:]
+    tail[...] = 0
+    tail["mb_type"] = _lib.MB_IPCM
+    tail["flags"] = _lib.MBF_LOST
+    tail["slice_num"] = _lib.SLICE_NONE
+    tail["ref_idx"] = -1
+    tail["level_offset"] = len(levels) + 384 * np.arange(len(tail))
+    return desc, mbs, np.concatenate([levels, np.zeros(384 * len(tail), np.int16)])
+
+
+def test_mbs_of_no_slice_stop_deblocking(any_engine):
+    """MBs that no slice decoded (zero I_PCM records, slice H264G_SLICE_NONE) from a
+    mid-row MB on: deblocking stops at the first of them, so neither its edges nor any
+    later MB's are filtered, in I, P and B pictures (single-slice pictures with
+    disable_deblocking_filter_idc 0 and 2-slice ones with 2), against the oracle."""
+    for idc, slices in ((0, 1), (2, 2)):
+        gen = Generator(2, width_mbs=21, height_mbs=8, n_frames=4, seed=0x51CE0100 + idc, slices=slices,
+                        disable_deblocking_filter_idc=idc)
+        n = gen.width_mbs * gen.height_mbs
+        pics = [_lose_tail(*gen.picture(i), first=n - 1 - (37 * i + 50) % (n // 2)) for i in range(gen.num_pictures)]
+        _decode_all(any_engine, gen, pics)
+
+
+def test_slice_layouts_the_engine_does_not_take_are_refused(engine):
+    """Several slices with disable_deblocking_filter_idc 0 (the reference filters such
+    edges with the current MB's data on both sides) and MBs of no slice in an MBAFF frame
+    are refused before anything runs."""
+    from h264_video_decoder_demo_amd import _lib
+    gen = Generator(2, width_mbs=8, height_mbs=4, n_frames=1, seed=0x51CE0200, slices=2, disable_deblocking_filter_idc=2)
+    desc, mbs, levels = gen.picture(0)
+    sid = engine.open_stream_for(gen)
+    try:
+        engine.release(engine.begin(sid, desc, mbs, levels))
+        desc.disable_deblocking_filter_idc = 0
+        with pytest.raises(_lib.H264GError, match="several slices"):
+            engine.begin(sid, desc, mbs, levels)
+    finally:
+        engine.close_stream(sid)
+    gen = Generator(3, width_mbs=6, height_mbs=4, n_frames=1, seed=0x51CE0201)
+    desc, mbs, levels = _lose_tail(*gen.picture(0), first=20)
+    sid = engine.open_stream_for(gen)
+    try:
+        with pytest.raises(_lib.H264GError, match="no slice in an MBAFF"):
+            engine.begin(sid, desc, mbs, levels)
+    finally:
+        engine.close_stream(sid)
+
+
 @pytest.mark.parametrize("shape", [(240, 6), (120, 12), (3, 40)])
 def test_mbaff_pictures_match_oracle(any_engine, shape):
     """MBAFF frames of other shapes (as wide as 2160p, narrow and tall) against the
