@@ -161,6 +161,19 @@ __device__ __forceinline__ void quad_transpose(int32_t (&m)[4], int r) {
     if (hi) { m[0] = a; m[1] = b; } else { m[2] = a; m[3] = b; }
 }
 
+// 4x4 byte transpose over the four lanes of a quad: lane j holds row j of a block of bytes
+// (byte c = column c) and gets column j (byte r = row r).  Two exchange stages, each a DPP
+// quad permutation and one byte permute with a lane-dependent selector; applied twice it
+// gives the rows back.
+__device__ __forceinline__ uint32_t quad_transpose_bytes(uint32_t v, int j) {
+    // even j: (own c0, partner c0, own c2, partner c2); odd j: (partner c1, own c1, partner c3, own c3)
+    const uint32_t x = (uint32_t)dpp_xor1((int)v);
+    const uint32_t t = __builtin_amdgcn_perm(x, v, (j & 1) ? 0x03070105u : 0x06020400u);
+    // j < 2: own low half, then the low half of lane j + 2; j >= 2: the high half of lane j - 2, then own
+    const uint32_t y = (uint32_t)dpp_xor2((int)t);
+    return __builtin_amdgcn_perm(y, t, (j & 2) ? 0x03020706u : 0x05040100u);
+}
+
 // Level at scan position k (< 16) of the block with slot entry e: 0 unless mask bit k is
 // set, else the value at the block's first index plus the number of mask bits below k.
 __device__ __forceinline__ int32_t level_at(uint32_t e, const int16_t *vals, int k) {
@@ -1474,16 +1487,13 @@ __device__ __forceinline__ void dbk_lines(const DevPic &P, uint32_t eq, bool edg
         w[3] = luma ? *(const uint32_t *)(row + mc + 8) : 0u;
         w[4] = luma ? *(const uint32_t *)(row + mc + 12) : 0u;
     } else {
-        const uint8_t *c = base + mc + k;
+        // the quad of lanes that owns columns 4m..4m+3 reads rows 4i..4i+3 of them as dwords
+        // (its lane j row 4i + j) and transposes them into the four columns; quads never
+        // mix planes, so a quad agrees on `luma`
+        const uint8_t *c = base + mc + (k & ~3) + (k & 3) * stride;
 #pragma unroll
-        for (int i = 0; i < 3; i++)
-            w[i] = (uint32_t)c[(4 * i) * stride] | ((uint32_t)c[(4 * i + 1) * stride] << 8) |
-                   ((uint32_t)c[(4 * i + 2) * stride] << 16) | ((uint32_t)c[(4 * i + 3) * stride] << 24);
-#pragma unroll
-        for (int i = 3; i < 5; i++)
-            w[i] = luma ? ((uint32_t)c[(4 * i) * stride] | ((uint32_t)c[(4 * i + 1) * stride] << 8) |
-                           ((uint32_t)c[(4 * i + 2) * stride] << 16) | ((uint32_t)c[(4 * i + 3) * stride] << 24))
-                        : 0u;
+        for (int i = 0; i < 5; i++)
+            w[i] = quad_transpose_bytes((i < 3 || luma) ? *(const uint32_t *)(c + 4 * i * stride) : 0u, k & 3);
     }
     const int qpq = dbq_qp(qq, plane), qpe = edge ? dbq_qp(eq, plane) : qpq;
     const int qav = (qpe + qpq + 1) >> 1;  // internal edges: (qpq + qpq + 1) >> 1 = qpq
@@ -1516,12 +1526,12 @@ __device__ __forceinline__ void dbk_lines(const DevPic &P, uint32_t eq, bool edg
             *(uint32_t *)(row + mc + 12) = w[4];
         }
     } else {
-        uint8_t *c = base + mc + k;
+        // back to rows (the first and last row, p3 / q3 of the outer edges, keep their values)
+        uint8_t *c = base + mc + (k & ~3) + (k & 3) * stride;
 #pragma unroll
-        for (int i = 1; i < 11; i++) c[i * stride] = (uint8_t)(w[i >> 2] >> (8 * (i & 3)));
-        if (luma) {
-#pragma unroll
-            for (int i = 11; i < 19; i++) c[i * stride] = (uint8_t)(w[i >> 2] >> (8 * (i & 3)));
+        for (int i = 0; i < 5; i++) {
+            const uint32_t v = quad_transpose_bytes(w[i], k & 3);
+            if (i < 3 || luma) *(uint32_t *)(c + 4 * i * stride) = v;
         }
     }
 }
