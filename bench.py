@@ -14,8 +14,9 @@ The same pictures are also run through the plain per-picture path (one launch pe
 stage per picture), and the batched output is checked byte-for-byte against it.
 
 Prints one JSON line (metric, value = batched Mpix/s, plain Mpix/s, p50/p99 frame
-latency, the roofline of the dominant kernel, and the reference decoder's CPU
-reconstruction timed on this host's cores).
+latency, the roofline of the dominant kernel, the reference decoder's CPU
+reconstruction timed on this host's cores, and end-to-end decoding of the reference's
+own sample-stream prefixes, byte stream to host planes, beside the reference decoder).
 """
 from __future__ import annotations
 
@@ -146,6 +147,72 @@ def serving_check(specs, gens, args):
     return {"streams": len(gens), "equal_dpb": equal, "frames": sum(g.num_pictures for g in gens)}
 
 
+BYTE_STREAMS = ("heavyhand_cavlc_15", "heavyhand_cabac_10", "gop121_12")
+
+
+def byte_streams(reps):
+    """End-to-end decoding of the reference's own sample streams (the prefixes committed
+    under tests/golden): H264VideoDecoder.open on the Annex B file -- host parsing (NAL,
+    slice headers, CAVLC / CABAC, POC, lists, MV derivation), GPU reconstruction, read-back
+    of every output picture to host planes as the reference's callback receives them --
+    best of `reps` runs after one warm-up, each output picture checked against the
+    reference decoder's digests outside the timing; beside it the host parse alone and the
+    reference decoder (oracle/_ref/ref_decode, every reference translation unit but
+    main.cpp, compiled unmodified) on the same file, one process."""
+    import warnings
+    from h264_video_decoder_demo_amd import Engine
+    from h264_video_decoder_demo_amd.bitstream import Bitstream
+    from h264_video_decoder_demo_amd.decoder import H264VideoDecoder
+    ref_bin = os.path.join(ROOT, "oracle", "_ref", "ref_decode")
+    out = {}
+    with Engine(device=0) as eng, warnings.catch_warnings():
+        warnings.simplefilter("ignore", RuntimeWarning)  # gop121's damaged slice, decoded as the reference does
+        for name in BYTE_STREAMS:
+            path = os.path.join(ROOT, "tests", "golden", name + ".h264")
+            with open(os.path.join(ROOT, "tests", "golden", name + ".json")) as f:
+                ref = json.load(f)["pictures"]
+            got = []
+
+            def keep(pic, user, err):
+                if pic is not None:
+                    got.append(pic)
+                return 0
+
+            def decode():
+                got.clear()
+                dec = H264VideoDecoder(eng)
+                dec.set_output_frame_callback_functuin(keep, None)
+                t = time.perf_counter()
+                rc = dec.open(path)
+                return time.perf_counter() - t, rc
+
+            decode()
+            best = min(decode()[0] for _ in range(reps))
+            same = len(got) == len(ref) and all(
+                (p.PicOrderCnt, hashlib.sha256(p.m_pic_buff_luma.tobytes()).hexdigest(),
+                 hashlib.sha256(p.m_pic_buff_cb.tobytes()).hexdigest(),
+                 hashlib.sha256(p.m_pic_buff_cr.tobytes()).hexdigest()) == (r["poc"], r["y"], r["cb"], r["cr"])
+                for p, r in zip(got, ref))
+            if not same:
+                raise SystemExit(f"bench.py: {name} decoded differently from the reference decoder")
+            t = time.perf_counter()
+            n = sum(1 for _ in Bitstream(path).pictures())
+            parse = time.perf_counter() - t
+            mpix = sum(r["width"] * r["height"] for r in ref) / 1e6
+            row = {"pictures": n, "mpix": round(mpix, 2), "decode_ms": round(1e3 * best, 1),
+                   "fps": round(n / best, 1), "mpix_s": round(mpix / best, 1), "host_parse_ms": round(1e3 * parse, 1),
+                   "bitexact_vs_reference_decoder": same}
+            if os.path.exists(ref_bin):
+                t = time.perf_counter()
+                subprocess.run([ref_bin, path], check=True, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+                tr = time.perf_counter() - t
+                row.update({"reference_decoder_ms": round(1e3 * tr, 1), "reference_decoder_fps": round(n / tr, 2),
+                            "speedup": round(tr / best, 1)})
+            out[name] = row
+            log(f"byte stream {name}: {row}")
+    return out
+
+
 def serving_capacity(args, rates, seconds, bound_ms):
     """Serving capacity: the serving loop (128 streams of the serving mix, GOP 32, 2 ms
     windows, 200 GB DPB budget) at rising offered rates, each for `seconds`; capacity is
@@ -241,6 +308,8 @@ def main():
                     help="batched run in a one-lane context (every stage, deblocking included, on one HIP "
                          "stream): the isolated shape whose kernel durations add up to the step time")
     ap.add_argument("--no-isolated", action="store_true", help="skip the isolated (one-lane) batched leg")
+    ap.add_argument("--no-byte-streams", action="store_true",
+                    help="skip the end-to-end decoding of the reference's sample-stream prefixes")
     ap.add_argument("--copy-mib", type=int, default=1024)
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="write profiles/traffic_config4.json from two rocprofv3 --pmc runs of --profile-run")
@@ -250,6 +319,7 @@ def main():
         return
     if args.profile_run:
         args.steps, args.warmup, args.no_plain, args.no_cpu_baseline, args.no_isolated = 1, 0, True, True, True
+        args.no_byte_streams = True
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this engine runs on one GPU; streams would be spread over GPUs by an outer "
               "scheduler (no collectives involved). Run with --gpus 1.", file=sys.stderr)
@@ -495,6 +565,8 @@ def main():
             serve["capacity"] = serving_capacity(args, rates, args.capacity_seconds, args.capacity_p99_ms)
             log(f"serving capacity: {serve['capacity']['capacity_fps']} fps at p99 <= {args.capacity_p99_ms} ms")
 
+    streams_e2e = None if args.no_byte_streams else byte_streams(3)
+
     cpu = None
     if not args.no_cpu_baseline:
         t = time.perf_counter()
@@ -544,6 +616,7 @@ def main():
                      "serving_dpb_equals_plain_dpb": serve["bitexact_vs_plain"] if serve else None},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "byte_streams": streams_e2e,
     }
     for b in batches:
         eng.batch_destroy(b)

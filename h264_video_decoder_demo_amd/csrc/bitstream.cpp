@@ -48,25 +48,28 @@ struct Bits {
         return b;
     }
     uint32_t u(int n) {
-        uint32_t v = 0;
-        for (int i = 0; i < n; i++) v = (v << 1) | bit();
+        if (n <= 0) return 0;
+        const uint32_t v = peek(n);
+        skip(n);
         return v;
     }
-    // the next n (<= 32) bits without consuming them, zeros past the end
+    // the next n (1..32) bits without consuming them, zeros past the end (nbits is a
+    // whole number of bytes): five bytes from the current one, shifted into place
     uint32_t peek(int n) const {
-        uint32_t v = 0;
-        for (int i = 0; i < n; i++) {
-            const size_t p = pos + i;
-            v = (v << 1) | (p < nbits ? (d[p >> 3] >> (7 - (p & 7))) & 1u : 0u);
-        }
-        return v;
+        const size_t nb = nbits >> 3, b0 = pos >> 3;
+        uint64_t v = 0;
+        for (size_t i = 0; i < 5; i++) v = (v << 8) | (b0 + i < nb ? d[b0 + i] : 0u);
+        return (uint32_t)(((v << (pos & 7)) & 0xFFFFFFFFFFull) >> (40 - n));
     }
-    void skip(int n) { pos += n; if (pos > nbits) over = true; }
-    uint32_t ue() {  // 9.1
-        int z = 0;
-        while (!bit()) {
-            if (++z > 31 || over) { over = true; return 0; }
-        }
+    void skip(int n) {  // past the end: flagged, and the position stays at the end
+        pos += n;
+        if (pos > nbits) { over = true; pos = nbits; }
+    }
+    uint32_t ue() {  // 9.1: leading zeros, a 1, as many more bits
+        const uint32_t w = peek(32);
+        if (!w) { skip(32); over = true; return 0; }  // more than 31 leading zeros
+        const int z = __builtin_clz(w);
+        skip(z + 1);
         return z ? ((1u << z) - 1u) + u(z) : 0u;
     }
     int32_t se() {  // 9.1.1
@@ -210,25 +213,55 @@ const uint8_t kCbpInter[48] = {0,  16, 1,  2,  4,  8,  32, 3,  5,  10, 12, 15, 4
                                14, 6,  9,  31, 35, 37, 42, 44, 33, 34, 36, 40, 39, 43, 45, 46,
                                17, 18, 20, 24, 19, 21, 26, 28, 23, 27, 29, 30, 22, 25, 38, 41};
 
-// A codeword table compiled for lookup: (code, length) pairs matched against the next bits.
+// A codeword table compiled for lookup.  The codes (at most 16 bits, prefix-free) are
+// grouped by their number z of leading zeros; within a group an entry is found by the
+// m(z) bits after the first 1, m(z) the longest such tail in the group (shorter codes fill
+// every completion of their tail).  A code of zeros only, when the table has one, is
+// matched first.
 struct Vlc {
-    struct E { uint32_t code; int len; int a, b; };
-    std::vector<E> e;
+    struct E { uint8_t len; int8_t a, b; };  // len 0: no codeword
+    struct Code { uint32_t code; int len, a, b; };
+    std::vector<Code> codes;
+    int8_t m[17] = {};
+    uint16_t base[18] = {};
+    std::vector<E> lut;
+    E zero{0, 0, 0};
     void add(const char *s, int a, int b) {
         uint32_t c = 0;
         int n = 0;
         for (; s[n]; n++) c = (c << 1) | (uint32_t)(s[n] == '1');
-        e.push_back({c, n, a, b});
+        codes.push_back({c, n, a, b});
+    }
+    static int zeros_of(const Code &x) { return x.len - (32 - __builtin_clz(x.code)); }
+    void build() {
+        for (const Code &x : codes)
+            if (x.code) m[zeros_of(x)] = (int8_t)std::max<int>(m[zeros_of(x)], x.len - zeros_of(x) - 1);
+        for (int z = 0; z < 17; z++) base[z + 1] = (uint16_t)(base[z] + (1 << m[z]));
+        lut.assign(base[17], E{0, 0, 0});
+        for (const Code &x : codes) {
+            if (!x.code) {
+                zero = E{(uint8_t)x.len, (int8_t)x.a, (int8_t)x.b};
+                continue;
+            }
+            const int z = zeros_of(x), tail = x.len - z - 1, fill = m[z] - tail;
+            const uint32_t bits = x.code & ((1u << tail) - 1u);
+            for (uint32_t k = 0; k < (1u << fill); k++)
+                lut[base[z] + (bits << fill) + k] = E{(uint8_t)x.len, (int8_t)x.a, (int8_t)x.b};
+        }
     }
     // the entry whose codeword is next in the stream (consumed), or null
     const E *read(Bits &bs) const {
         const uint32_t w = bs.peek(16);
-        for (const E &x : e)
-            if ((w >> (16 - x.len)) == x.code) {
-                bs.skip(x.len);
-                return &x;
-            }
-        return nullptr;
+        if (zero.len && (w >> (16 - zero.len)) == 0) {
+            bs.skip(zero.len);
+            return &zero;
+        }
+        if (!w) return nullptr;
+        const int z = __builtin_clz(w) - 16;
+        const E &x = lut[base[z] + (((w << (z + 1)) & 0xFFFFu) >> (16 - m[z]))];
+        if (!x.len) return nullptr;
+        bs.skip(x.len);
+        return &x;
     }
 };
 
@@ -246,6 +279,10 @@ struct Tables {
             for (int v = 0; v < 4 && kTotalZerosChromaDc[t][v]; v++) total_zeros_cdc[t].add(kTotalZerosChromaDc[t][v], v, 0);
         for (int t = 0; t < 7; t++)
             for (int v = 0; v < 15 && kRunBefore[t][v]; v++) run_before[t].add(kRunBefore[t][v], v, 0);
+        for (Vlc &v : coeff_token) v.build();
+        for (Vlc &v : total_zeros) v.build();
+        for (Vlc &v : total_zeros_cdc) v.build();
+        for (Vlc &v : run_before) v.build();
     }
 };
 const Tables &tables() {
@@ -732,16 +769,17 @@ inline int nc_of(int a, int b) {  // 9.2.1
 
 // Appends the masks and values of the MB's blocks in the order h264g.h documents.
 void emit_levels(const Residual &r, bool i16, bool t8, int cbp, std::vector<int16_t> &lv) {
-    std::vector<uint16_t> masks;
-    std::vector<int16_t> vals;
+    uint16_t masks[27];
+    int16_t vals[27 * 16];
+    int nm = 0, nv = 0;
     auto block = [&](const int16_t *c, int n, int first_pos) {  // n coefficients at scan first_pos..
         uint16_t m = 0;
         for (int i = 0; i < n; i++)
             if (c[i]) {
                 m |= (uint16_t)(1u << (first_pos + i));
-                vals.push_back(c[i]);
+                vals[nv++] = c[i];
             }
-        masks.push_back(m);
+        masks[nm++] = m;
     };
     if (i16) block(r.dc, 16, 0);
     for (int q = 0; q < 4; q++) {
@@ -760,8 +798,8 @@ void emit_levels(const Residual &r, bool i16, bool t8, int cbp, std::vector<int1
     if ((cbp >> 4) == 2)
         for (int c = 0; c < 2; c++)
             for (int b = 0; b < 4; b++) block(r.cac[c][b], 15, 1);
-    for (uint16_t m : masks) lv.push_back((int16_t)m);
-    lv.insert(lv.end(), vals.begin(), vals.end());
+    for (int i = 0; i < nm; i++) lv.push_back((int16_t)masks[i]);
+    lv.insert(lv.end(), vals, vals + nv);
 }
 
 // residual() (7.3.5.3) for CAVLC, 4:2:0
@@ -1134,11 +1172,15 @@ struct Cabac {
             bin = mps[c];
             if (state[c] < 62) state[c]++;
         }
-        while (range < 256) {  // RenormD
-            range <<= 1;
-            offset = (offset << 1) | bs->bit();
-        }
+        renorm();
         return bin;
+    }
+    void renorm() {  // RenormD: shift range back to 9 bits, the same number of bits into offset
+        if (range < 256) {
+            const int n = __builtin_clz(range) - 23;
+            range <<= n;
+            offset = (offset << n) | bs->u(n);
+        }
     }
     int bypass() {  // 9.3.3.2.3
         offset = (offset << 1) | bs->bit();
@@ -1151,10 +1193,7 @@ struct Cabac {
     int terminate() {  // 9.3.3.2.2.3
         range -= 2;
         if (offset >= range) return 1;
-        while (range < 256) {
-            range <<= 1;
-            offset = (offset << 1) | bs->bit();
-        }
+        renorm();
         return 0;
     }
 };

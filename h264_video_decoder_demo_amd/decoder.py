@@ -17,6 +17,8 @@ per-picture syntax (the generator, or a syntax file written by ``syntax_io``).
 """
 from __future__ import annotations
 
+import queue
+import threading
 import warnings
 from dataclasses import dataclass
 
@@ -94,6 +96,54 @@ def _with_last(items):
         cur = nxt
 
 
+class _Prefetch:
+    """The items of an iterator produced on a thread of their own, up to `depth` ahead of
+    the consumer (the byte-stream parser runs in C with the GIL released, so parsing the
+    next pictures overlaps the reconstruction of the current one).  An exception of the
+    producer is raised at the consumer; close() stops the producer."""
+
+    _END = object()
+
+    def __init__(self, items, depth: int = 4):
+        self._q = queue.Queue(maxsize=depth)
+        self._stop = threading.Event()
+        self._t = threading.Thread(target=self._run, args=(iter(items),), daemon=True)
+        self._t.start()
+
+    def _run(self, it):
+        try:
+            for x in it:
+                while not self._stop.is_set():
+                    try:
+                        self._q.put((True, x), timeout=0.05)
+                        break
+                    except queue.Full:
+                        continue
+                if self._stop.is_set():
+                    return
+            self._q.put((True, self._END))
+        except BaseException as e:  # handed to the consumer
+            self._q.put((False, e))
+
+    def __iter__(self):
+        while True:
+            ok, x = self._q.get()
+            if not ok:
+                raise x
+            if x is self._END:
+                return
+            yield x
+
+    def close(self):
+        self._stop.set()
+        while self._t.is_alive():
+            try:
+                self._q.get(timeout=0.05)
+            except queue.Empty:
+                pass
+        self._t.join()
+
+
 class H264VideoDecoder:
     def __init__(self, engine: Engine | None = None, device: int = 0):
         self._engine = engine
@@ -148,20 +198,45 @@ class H264VideoDecoder:
         ret = 0
         stopped = False
         byte_stream = isinstance(source, Bitstream)
+        prefetch = None
         if byte_stream:
-            def parsed():
-                for n, p in enumerate(source.pictures()):
+            prefetch = _Prefetch(source.pictures())
+            items = _with_last(prefetch)
+        else:
+            items = ((source.picture(i), False) for i in range(source.num_pictures))
+        inflight = None  # (staged picture, index, desc, idr): submitted, not yet read back
+
+        def finish():
+            """Waits for the picture in flight, reads it back (host-visible before any
+            callback) and hands it to the reorder buffer; True when the callback stops."""
+            nonlocal inflight
+            pic, i, desc, idr = inflight
+            inflight = None
+            eng.sync()
+            y, cb, cr = eng.readback(sid, desc.dst_slot)
+            eng.release(pic)
+            if i > 0 and idr:
+                while True:  # an IDR flushes the reorder buffer (H264VideoDecoder.cpp:383-410)
+                    out = reorder.flush_one()
+                    if out is None:
+                        break
+                    if self._emit(out):
+                        return True
+            out = reorder.push(H264Picture(y, cb, cr, W * 16, H * 16, desc.poc, desc.slice_type, i))
+            return out is not None and self._emit(out)
+
+        try:
+            for i, (item, last) in enumerate(items):
+                if byte_stream:
+                    p = item
                     if p.lost_mbs:
                         # the reference abandons a slice whose data fails and decodes on
                         # (H264VideoDecoder.cpp:135-136); its MBs from there stay zero
-                        warnings.warn(f"picture {n} (POC {p.desc.poc}): {p.lost_mbs} MBs from MB {p.first_lost_mb} "
-                                      f"not decoded ({p.damage})", RuntimeWarning, stacklevel=3)
-                    yield p.desc, p.mbs, p.levels, p.idr
-            items = _with_last(parsed())
-        else:
-            items = ((source.picture(i) + (source.is_idr(i),), False) for i in range(source.num_pictures))
-        try:
-            for i, ((desc, mbs, levels, idr), last) in enumerate(items):
+                        warnings.warn(f"picture {i} (POC {p.desc.poc}): {p.lost_mbs} MBs from MB {p.first_lost_mb} "
+                                      f"not decoded ({p.damage})", RuntimeWarning, stacklevel=2)
+                    desc, mbs, levels, idr = p.desc, p.mbs, p.levels, p.idr
+                else:
+                    (desc, mbs, levels), idr = item, source.is_idr(i)
                 if last and byte_stream:
                     # the reference hands the stream's last picture to the callback without
                     # finishing it (H264VideoDecoder.cpp:353-361 skips
@@ -169,25 +244,18 @@ class H264VideoDecoder:
                     # deblocking filter, H264PictureBase.cpp:707): it is output unfiltered
                     desc = PicDesc.from_buffer_copy(desc)
                     desc.disable_deblocking_filter_idc = 1
+                # staging this picture on the host overlaps the reconstruction of the one in
+                # flight; it is submitted once that one is read back, since a picture may
+                # write the slot the previous one was output from
                 pic = eng.begin(sid, desc, mbs, levels)
-                eng.submit(pic)
-                eng.sync()
-                y, cb, cr = eng.readback(sid, desc.dst_slot)  # host-visible before any callback
-                eng.release(pic)
-                if i > 0 and idr:
-                    while True:  # an IDR flushes the reorder buffer (H264VideoDecoder.cpp:383-410)
-                        out = reorder.flush_one()
-                        if out is None:
-                            break
-                        if self._emit(out):
-                            stopped = True
-                            break
-                    if stopped:
-                        break
-                out = reorder.push(H264Picture(y, cb, cr, W * 16, H * 16, desc.poc, desc.slice_type, i))
-                if out is not None and self._emit(out):
+                if inflight is not None and finish():
+                    eng.release(pic)
                     stopped = True
                     break
+                eng.submit(pic)
+                inflight = (pic, i, desc, idr)
+            if not stopped and inflight is not None:
+                stopped = finish()
             if not stopped:
                 while True:
                     out = reorder.flush_one()
@@ -197,6 +265,11 @@ class H264VideoDecoder:
                         stopped = True
                         break
         finally:
+            if inflight is not None:
+                eng.sync()
+                eng.release(inflight[0])
+            if prefetch is not None:
+                prefetch.close()
             eng.close_stream(sid)
             if own:
                 eng.close()

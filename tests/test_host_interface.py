@@ -152,3 +152,27 @@ def test_independent_runs_follow_dependency_levels():
     wide = [_desc(k, [9]) for k in range(6)]
     assert independent_runs([wide]) == [[0, 1, 2, 3], [4, 5]]
     assert independent_runs([wide], max_per_stream=1) == [[k] for k in range(6)]
+
+
+def test_prefetch_thread_hands_items_and_errors_to_the_consumer():
+    """The parser runs ahead on a thread of its own: items arrive in order, an exception
+    of the producer is raised at the consumer after the items before it, and close()
+    stops a producer the consumer no longer reads."""
+    import itertools
+    from h264_video_decoder_demo_amd.decoder import _Prefetch
+    assert list(_Prefetch(range(100), depth=3)) == list(range(100))
+
+    def failing():
+        yield 1
+        yield 2
+        raise ValueError("damaged")
+
+    got = []
+    with pytest.raises(ValueError, match="damaged"):
+        for x in _Prefetch(failing()):
+            got.append(x)
+    assert got == [1, 2]
+    p = _Prefetch(itertools.count(), depth=2)
+    assert next(iter(p)) == 0
+    p.close()
+    assert not p._t.is_alive()
