@@ -147,7 +147,7 @@ def serving_check(specs, gens, args):
     return {"streams": len(gens), "equal_dpb": equal, "frames": sum(g.num_pictures for g in gens)}
 
 
-BYTE_STREAMS = ("heavyhand_cavlc_15", "heavyhand_cabac_10", "gop121_12")
+BYTE_STREAMS = ("heavyhand_cavlc_15", "heavyhand_cabac_10", "gop121_12", "gop121_idc0_12")
 
 
 def byte_streams(reps):

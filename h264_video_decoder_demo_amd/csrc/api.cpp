@@ -433,11 +433,8 @@ int stage_picture(h264g_ctx *ctx, Stream &st, const h264g_pic_desc &d, const h26
         return h264g_set_error(H264G_E_ARG, "deblocking parameters");
     // Slices.  Deblocking stops at the first MB of no slice (H264G_SLICE_NONE), where the
     // reference's bS derivation fails on the MB's reset type and the failure is ignored
-    // (H264InterPrediction.cpp:3182-3186, H264PictureBase.cpp:707-708).  With
-    // disable_deblocking_filter_idc 0 the MBs before it must form one slice: the reference
-    // filters an edge between slices with the current MB's data on both sides (it takes
-    // the unavailable neighbour's address as the current one, Deblock.cpp:733-750), which
-    // is not followed here; MBAFF frames take one slice.
+    // (H264InterPrediction.cpp:3182-3186, H264PictureBase.cpp:707-708).  MBAFF frames take
+    // one slice.
     uint32_t dbk_stop = n_mbs, first_sn = H264G_SLICE_NONE;
     for (uint32_t a = 0; a < n_mbs; a++) {
         const uint32_t sn = mbs[a].slice_num;
@@ -447,9 +444,7 @@ int stage_picture(h264g_ctx *ctx, Stream &st, const h264g_pic_desc &d, const h26
             continue;
         }
         if (first_sn == H264G_SLICE_NONE) first_sn = sn;
-        if (sn != first_sn && (mbaff || (d.disable_deblocking_filter_idc == 0 && a < dbk_stop)))
-            return h264g_set_error(H264G_E_ARG, "mb %u: several slices %s", a,
-                                   mbaff ? "in an MBAFF frame" : "with disable_deblocking_filter_idc 0");
+        if (sn != first_sn && mbaff) return h264g_set_error(H264G_E_ARG, "mb %u: several slices in an MBAFF frame", a);
     }
     std::vector<DevAux> aux(n_mbs);
     std::vector<int8_t> bsref(mbaff ? (size_t)n_mbs * 8 : 0, (int8_t)-1);

@@ -2156,9 +2156,6 @@ extern "C" int32_t h264g_parser_next(h264g_parser *p, h264g_parsed_picture *out)
             if (int rc = slice_lists(p, *sps, sl, poc, sl_lists)) return rc;
             if (const char *why = slice_mismatch(*pps, first, sl, lists, sl_lists))
                 return h264g_set_error(H264G_E_ARG, "picture %d: its slices do not share one descriptor (%s)", p->pictures, why);
-            if (sl.disable_dbk == 0)  // the reference filters slice edges from the current MB's data (quirk Q7)
-                return h264g_set_error(H264G_E_ARG, "several slices per picture with deblocking across their edges "
-                                       "(disable_deblocking_filter_idc 0) are not supported");
         }
         if (sl.first_mb < covered || sl.first_mb >= W * H)
             return h264g_set_error(H264G_E_ARG, "picture %d: slice at MB %d after MB %d", p->pictures, sl.first_mb, covered);

@@ -828,10 +828,8 @@ int32_t h264g_gen_create(const h264g_gen_params *params, h264g_gen **out) {
     // (H264InterPrediction.cpp:2958)
     if (p.direct_permille < 0 || p.direct_permille > 1000 || (p.direct_permille && p.weighted_bipred_idc == 2))
         return h264g_set_error(H264G_E_ARG, "gen: direct MBs need direct_permille in 0..1000 and no implicit weights");
-    // several slices: the engine filters no edge between slices (idc 2) or none at all (1)
-    if (p.slices < 0 || p.slices > p.width_mbs * p.height_mbs ||
-        (p.slices > 1 && (p.mbaff || (p.disable_deblocking_filter_idc != 1 && p.disable_deblocking_filter_idc != 2))))
-        return h264g_set_error(H264G_E_ARG, "gen: several slices need frames and disable_deblocking_filter_idc 1 or 2");
+    if (p.slices < 0 || p.slices > p.width_mbs * p.height_mbs || (p.slices > 1 && p.mbaff))
+        return h264g_set_error(H264G_E_ARG, "gen: several slices need frame MBs (no MBAFF)");
     h264g_gen *g = new h264g_gen();
     g->p = p;
     std::vector<PicPlan> plan = plan_gop(p.n_frames, p.b_frames, p.gop);

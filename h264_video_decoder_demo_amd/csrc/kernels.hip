@@ -1264,6 +1264,9 @@ struct MbView {
 __device__ __forceinline__ int view_ref(const MbView &v, int l, int q) { return (int8_t)((l ? v.ref1 : v.ref0) >> (8 * q)); }
 __device__ __forceinline__ int dbq_qp(uint32_t q, int plane) { return (q >> (8 * plane)) & 255; }
 __device__ __forceinline__ bool dbq_t8(uint32_t q) { return (q >> 24) & 1u; }
+// the MB's left / top MB edge lies between slices under disable_deblocking_filter_idc 0:
+// its p side takes the MB's own qP (see k_bs)
+constexpr uint32_t DBQ_OWN_LEFT = 1u << 25, DBQ_OWN_TOP = 1u << 26;
 
 __device__ __forceinline__ void load_view(const DevPic &P, int addr, MbView &v) {
     const DevAux ax = G(P.aux)[addr];
@@ -1278,7 +1281,7 @@ __device__ __forceinline__ void load_view(const DevPic &P, int addr, MbView &v) 
              ((uint32_t)(uint8_t)ax.refid[1][2] << 16) | ((uint32_t)(uint8_t)ax.refid[1][3] << 24);
 }
 // Deblocking word of an MB (G(P.dbq), written by k_bs): qP of luma, Cb edges, Cr edges in
-// bytes 0..2, transform_size_8x8_flag in bit 24.
+// bytes 0..2, transform_size_8x8_flag in bit 24, DBQ_OWN_LEFT / DBQ_OWN_TOP in bits 25, 26.
 
 __device__ __forceinline__ bool mv_far(int ax, int ay, int bx, int by) { return iabs(ax - bx) >= 4 || iabs(ay - by) >= 4; }
 
@@ -1495,7 +1498,8 @@ __device__ __forceinline__ void dbk_lines(const DevPic &P, uint32_t eq, bool edg
         for (int i = 0; i < 5; i++)
             w[i] = quad_transpose_bytes((i < 3 || luma) ? *(const uint32_t *)(c + 4 * i * stride) : 0u, k & 3);
     }
-    const int qpq = dbq_qp(qq, plane), qpe = edge ? dbq_qp(eq, plane) : qpq;
+    const bool own = qq & (DIR == 0 ? DBQ_OWN_LEFT : DBQ_OWN_TOP);
+    const int qpq = dbq_qp(qq, plane), qpe = edge && !own ? dbq_qp(eq, plane) : qpq;
     const int qav = (qpe + qpq + 1) >> 1;  // internal edges: (qpq + qpq + 1) >> 1 = qpq
     const int iA0 = clip3(0, 51, qav + P.offA), iB0 = clip3(0, 51, qav + P.offB);
     const int iAi = clip3(0, 51, qpq + P.offA), iBi = clip3(0, 51, qpq + P.offB);
@@ -2462,7 +2466,8 @@ __global__ __launch_bounds__(WAVE * DBK_WG_MAX_WAVES) void k_deblock_band(const 
 // Boundary strengths of every edge segment of every MB (8.7.2.1, Deblock.cpp:994-1310),
 // stored [direction][4-line segment][edge] so that a deblocking line reads its four edges
 // as one dword, and each MB's deblocking word (G(P.dbq): qP of luma / Cb edges / Cr edges in
-// bytes 0..2, transform_size_8x8_flag in bit 24); fully parallel: they depend on syntax
+// bytes 0..2, transform_size_8x8_flag in bit 24, the own-qP flags of its MB edges in bits
+// 25 / 26); fully parallel: they depend on syntax
 // only, not on samples.  Half a wave per MB; lane j = 16 * dir + 4 * edge + segment.
 __global__ __launch_bounds__(WAVE) void k_bs(const DevPic *__restrict__ pics, const uint32_t *__restrict__ items, uint32_t n) {
     const int lane = wave_lane();
@@ -2473,26 +2478,33 @@ __global__ __launch_bounds__(WAVE) void k_bs(const DevPic *__restrict__ pics, co
     const int addr = (int)(it & WI_MB_MASK), W = P.W, mbx = addr % W, mby = addr / W;
     const int j = lane & 31, dir = j >> 4, e = ((j >> 2) & 3) * 4, k = (j & 3) * 4;
     int bS = 0;
-    const int ap = e ? addr : (dir == 0 ? addr - 1 : addr - W);
-    // an MB edge at the picture border is not filtered, nor, with disable_deblocking_filter_idc
+    const uint32_t sq = G(P.mbs)[addr].slice_num;
+    const bool has_nb = dir == 0 ? mbx > 0 : mby > 0;
+    const bool other = has_nb && G(P.mbs)[dir == 0 ? addr - 1 : addr - W].slice_num != sq;
+    // An MB edge at the picture border is not filtered, nor, with disable_deblocking_filter_idc
     // 2, one between two slices (8.7: filterLeftMbEdgeFlag / filterTopMbEdgeFlag); no edge
-    // of an MB at or after the deblocking stop is
-    const bool edge = (uint32_t)addr < P.dbk_stop &&
-                      (e > 0 || ((dir == 0 ? mbx > 0 : mby > 0) &&
-                                 !(P.dbk_idc == 2 && G(P.mbs)[ap].slice_num != G(P.mbs)[addr].slice_num)));
+    // of an MB at or after the deblocking stop is.  With idc 0 the reference filters an
+    // edge between slices with the current MB in the neighbour's place (its neighbour
+    // derivation reports the other slice's MB as unavailable and the p-MB address stays
+    // the current one, Deblock.cpp:747-760, 784-791): the bS of an internal edge between
+    // the current MB's blocks at x (or y) 15 and 0, and the current MB's qP on both sides.
+    const bool own = e == 0 && other && P.dbk_idc == 0;
+    const int ap = (e || own) ? addr : (dir == 0 ? addr - 1 : addr - W);
+    const bool edge = (uint32_t)addr < P.dbk_stop && (e > 0 || (has_nb && !(P.dbk_idc == 2 && other)));
     if (edge) {
         MbView Qv, Pv;
         load_view(P, addr, Qv);
         load_view(P, ap, Pv);
         const int xq = dir == 0 ? e : k, yq = dir == 0 ? k : e;
         const int xp = dir == 0 ? (e ? e - 1 : 15) : k, yp = dir == 0 ? k : (e ? e - 1 : 15);
-        bS = bs_luma(P, ap, Pv, addr, Qv, e == 0, xp, yp, xq, yq);
+        bS = bs_luma(P, ap, Pv, addr, Qv, e == 0 && !own, xp, yp, xq, yq);
     }
     G(P.bs)[(size_t)addr * 32 + (j & 16) + ((j & 3) << 2) + ((j >> 2) & 3)] = (uint8_t)bS;  // [dir][segment][edge]
     if (j == 0) {
         const DevAux &ax = G(P.aux)[addr];
+        const bool own_t = P.dbk_idc == 0 && mby > 0 && G(P.mbs)[addr - W].slice_num != sq;
         G(P.dbq)[addr] = (uint32_t)ax.qdb[0] | ((uint32_t)ax.qdb[1] << 8) | ((uint32_t)ax.qdb[2] << 16) |
-                      ((ax.kind & AUX_T8) ? 1u << 24 : 0u);
+                         ((ax.kind & AUX_T8) ? 1u << 24 : 0u) | (own ? DBQ_OWN_LEFT : 0u) | (own_t ? DBQ_OWN_TOP : 0u);
     }
 }
 

@@ -134,10 +134,13 @@ typedef struct h264g_mb {
     uint32_t level_offset;            /* first int16 of this MB's level block        */
     uint32_t slice_num;               /* the MB's slice: neighbours with another value
                                          are not available for intra prediction and
-                                         motion vector prediction, and with
-                                         disable_deblocking_filter_idc 2 their common
-                                         edge is not filtered (0 for every MB of a
-                                         picture of one slice)                       */
+                                         motion vector prediction; their common edge
+                                         is not filtered with
+                                         disable_deblocking_filter_idc 2, and with 0
+                                         it is filtered as the reference does, with
+                                         the current MB in the neighbour's place for
+                                         bS and qP (0 for every MB of a picture of
+                                         one slice)                                  */
 } h264g_mb;
 
 /*
@@ -437,7 +440,7 @@ typedef struct h264g_gen_params {
     int32_t  slices;              /* slices per picture (0 or 1: one), runs of
                                      ceil(W * H / slices) MBs in address order; intra
                                      prediction sees no neighbour of another slice.
-                                     Frames, disable_deblocking_filter_idc 1 or 2       */
+                                     Frames only (no MBAFF)                              */
 } h264g_gen_params;
 
 /* ------------------------------------------------ host motion-vector derivation
@@ -494,8 +497,7 @@ int32_t h264g_mvfield_colocated(h264g_mvfield *field, const h264g_mb *col_mbs, i
  * (H264MacroBlock.cpp:912-1139), CH264ResidualBlockCavlc (H264ResidualBlockCavlc.cpp),
  * CH264Cabac (H264Cabac.cpp) and H264RefPicList.cpp.
  * Slices of one picture must share one descriptor (slice type, reference lists, weights,
- * deblocking parameters); several slices with disable_deblocking_filter_idc 0 are
- * refused.  Slice data that fails to parse ends its slice where it fails, and MBs that no
+ * deblocking parameters).  Slice data that fails to parse ends its slice where it fails, and MBs that no
  * slice decodes are H264G_MBF_LOST records, as the reference skips the rest of such a
  * slice and decodes on (H264VideoDecoder.cpp:135-136); lost_mbs counts them.
  * Refused with H264G_E_ARG: field pictures and MBAFF, slice groups, data partitioning,

@@ -39,7 +39,11 @@
  *       H264MacroBlock.cpp:99-100, 142) stay available to intra prediction.
  * Slices: a neighbour MB of another slice (h264g_mb.slice_num) is not available
  * (H264PictureBase.cpp:2890-2937); with disable_deblocking_filter_idc 2 MB edges
- * between slices are not filtered (Deblock.cpp:128-153).
+ * between slices are not filtered (Deblock.cpp:128-153);
+ *   Q7  with disable_deblocking_filter_idc 0 such an edge is filtered with the current MB
+ *       as its p-MB (the unavailable neighbour leaves the p-MB address at the current one,
+ *       Deblock.cpp:747-760, 784-791): bS of an internal edge between the MB's blocks at
+ *       x (or y) 15 and 0, the MB's own qP on both sides, the neighbour's samples.
  *   Q22 the deblocking of a picture stops at its first MB of no slice
  *       (H264G_SLICE_NONE, an MB that no slice decoded): the bS derivation fails on the
  *       MB's reset type MB_TYPE_NA (H264InterPrediction.cpp:3182-3186) and
@@ -1010,14 +1014,16 @@ void refcpu_filter_line(uint8_t p[4], uint8_t q[4], int bS, int chroma, int qp_a
 }
 
 /* One edge of MB (mbx, mby): plane c (0 luma, 1 Cb, 2 Cr), vertical or horizontal, at
- * offset e inside the MB (Deblock.cpp:639-840). */
-static void filter_edge(Ctx *X, int mbx, int mby, int c, int vertical, int e) {
+ * offset e inside the MB (Deblock.cpp:639-840).  own: an MB edge whose p-MB is the current
+ * one (Q7): the bS of an internal edge between the MB's blocks at 15 and 0, the MB's qP on
+ * both sides, the neighbour's samples. */
+static void filter_edge(Ctx *X, int mbx, int mby, int c, int vertical, int e, int own) {
     int n = c ? 8 : 16;
     uint8_t *plane = c == 0 ? X->y : (c == 1 ? X->cb : X->cr);
     int stride = c ? X->wc : X->wl;
     const MbInfo *Q = &X->info[mby * X->W + mbx];
     const MbInfo *P = Q;
-    int mb_edge = e == 0;
+    int mb_edge = e == 0 && !own;
     if (mb_edge) P = vertical ? &X->info[mby * X->W + mbx - 1] : &X->info[(mby - 1) * X->W + mbx];
     int scale = c ? 2 : 1;
     int qpp, qpq;
@@ -1166,21 +1172,23 @@ static void deblock(Ctx *X) {
             const MbInfo *I = &X->info[a];
             const uint32_t sn = X->mbs[a].slice_num;
             if (sn == H264G_SLICE_NONE) return;
-            int left = mbx > 0 && !(idc == 2 && X->mbs[a - 1].slice_num != sn);
-            int topf = mby > 0 && !(idc == 2 && X->mbs[a - X->W].slice_num != sn);
-            if (left) filter_edge(X, mbx, mby, 0, 1, 0);
-            if (!I->t8) filter_edge(X, mbx, mby, 0, 1, 4);
-            filter_edge(X, mbx, mby, 0, 1, 8);
-            if (!I->t8) filter_edge(X, mbx, mby, 0, 1, 12);
-            if (topf) filter_edge(X, mbx, mby, 0, 0, 0);
-            if (!I->t8) filter_edge(X, mbx, mby, 0, 0, 4);
-            filter_edge(X, mbx, mby, 0, 0, 8);
-            if (!I->t8) filter_edge(X, mbx, mby, 0, 0, 12);
+            const int otherL = mbx > 0 && X->mbs[a - 1].slice_num != sn;
+            const int otherT = mby > 0 && X->mbs[a - X->W].slice_num != sn;
+            int left = mbx > 0 && !(idc == 2 && otherL), topf = mby > 0 && !(idc == 2 && otherT);
+            const int ownL = idc == 0 && otherL, ownT = idc == 0 && otherT;  /* Q7 */
+            if (left) filter_edge(X, mbx, mby, 0, 1, 0, ownL);
+            if (!I->t8) filter_edge(X, mbx, mby, 0, 1, 4, 0);
+            filter_edge(X, mbx, mby, 0, 1, 8, 0);
+            if (!I->t8) filter_edge(X, mbx, mby, 0, 1, 12, 0);
+            if (topf) filter_edge(X, mbx, mby, 0, 0, 0, ownT);
+            if (!I->t8) filter_edge(X, mbx, mby, 0, 0, 4, 0);
+            filter_edge(X, mbx, mby, 0, 0, 8, 0);
+            if (!I->t8) filter_edge(X, mbx, mby, 0, 0, 12, 0);
             for (int c = 1; c <= 2; c++) {
-                if (left) filter_edge(X, mbx, mby, c, 1, 0);
-                filter_edge(X, mbx, mby, c, 1, 4);
-                if (topf) filter_edge(X, mbx, mby, c, 0, 0);
-                filter_edge(X, mbx, mby, c, 0, 4);
+                if (left) filter_edge(X, mbx, mby, c, 1, 0, ownL);
+                filter_edge(X, mbx, mby, c, 1, 4, 0);
+                if (topf) filter_edge(X, mbx, mby, c, 0, 0, ownT);
+                filter_edge(X, mbx, mby, c, 0, 4, 0);
             }
         }
 }

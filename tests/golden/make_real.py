@@ -11,6 +11,10 @@ the reference decoder's output for each.
       Baseline profile, CAVLC, 9 slices per picture, disable_deblocking_filter_idc 2; the
       last slice of its first picture is damaged (its data fails at MB 7713), which the
       reference decoder survives by abandoning the slice there
+  tests/golden/gop121_idc0_12.h264       the same 12 pictures with every slice header's
+      disable_deblocking_filter_idc rewritten from 2 to 0 (deblocking across the slice
+      edges, which the reference filters with the current MB's data on both sides); the
+      rest of each slice is moved by the two bits the shorter code saves
   tests/golden/<name>.json               per output picture of the reference decoder
       (oracle/_ref/ref_decode: every reference translation unit but main.cpp, compiled
       unmodified by `make -C oracle decode`), in its output order: PicOrderCnt, slice
@@ -26,9 +30,138 @@ import sys
 ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
 HERE = os.path.dirname(os.path.abspath(__file__))
 DATA = "/root/reference/data"
-STREAMS = [("HeavyHand_1080p.B_frames_4.no_cabac.no_tff.h264", 15, "heavyhand_cavlc_15"),
-           ("HeavyHand_1080p.B_frames.cabac.h264", 10, "heavyhand_cabac_10"),
-           ("gop121.naluCnt453.h264", 12, "gop121_12")]
+STREAMS = [("HeavyHand_1080p.B_frames_4.no_cabac.no_tff.h264", 15, "heavyhand_cavlc_15", None),
+           ("HeavyHand_1080p.B_frames.cabac.h264", 10, "heavyhand_cabac_10", None),
+           ("gop121.naluCnt453.h264", 12, "gop121_12", None),
+           ("gop121.naluCnt453.h264", 12, "gop121_idc0_12", 0)]
+
+
+class _Bits:
+    """Reads an RBSP bit by bit (7.2: u(n), ue(v), se(v)) and copies what it read."""
+
+    def __init__(self, rbsp: bytes):
+        self.bits = "".join(f"{b:08b}" for b in rbsp)
+        self.pos = 0
+
+    def u(self, n):
+        v = int(self.bits[self.pos:self.pos + n] or "0", 2)
+        self.pos += n
+        return v
+
+    def ue(self):
+        z = 0
+        while self.bits[self.pos + z] == "0":
+            z += 1
+        self.pos += z + 1
+        return (1 << z) - 1 + self.u(z)
+
+    def se(self):
+        k = self.ue()
+        return (k + 1) // 2 if k & 1 else -(k // 2)
+
+
+def _unescape(nal: bytes) -> bytes:
+    out, zeros = bytearray(), 0
+    for b in nal:
+        if zeros >= 2 and b == 3:
+            zeros = 0
+            continue
+        out.append(b)
+        zeros = zeros + 1 if b == 0 else 0
+    return bytes(out)
+
+
+def _escape(rbsp: bytes) -> bytes:
+    out, zeros = bytearray(), 0
+    for b in rbsp:
+        if zeros >= 2 and b <= 3:
+            out.append(3)
+            zeros = 0
+        out.append(b)
+        zeros = zeros + 1 if b == 0 else 0
+    return bytes(out)
+
+
+def _ue_bits(v: int) -> str:
+    b = bin(v + 1)[2:]
+    return "0" * (len(b) - 1) + b
+
+
+def with_deblocking_idc(data: bytes, idc: int) -> bytes:
+    """The stream with disable_deblocking_filter_idc of every slice set to idc, for the
+    syntax gop121 uses (CAVLC I / P slices, POC type 0, no weighted prediction): SPS and
+    PPS fields are read as far as the slice header needs them (7.3.2.1, 7.3.2.2), the
+    slice header up to the field (7.3.3), and the slice's bits after it are kept."""
+    starts, i = [], 0
+    while (j := data.find(b"\x00\x00\x01", i)) >= 0:
+        starts.append(j)
+        i = j + 3
+    out, sps, pps = bytearray(data[:starts[0]]), {}, {}
+    for k, st in enumerate(starts):
+        end = starts[k + 1] if k + 1 < len(starts) else len(data)
+        nal = data[st + 3:end]
+        while k + 1 < len(starts) and nal.endswith(b"\x00"):  # the next start code's leading zero
+            nal = nal[:-1]
+        trail = data[st + 3 + len(nal):end]
+        hdr, typ = nal[0], nal[0] & 31
+        b = _Bits(_unescape(nal[1:]))
+        if typ == 7:
+            prof = b.u(8); b.u(16); b.ue()
+            assert prof in (66, 77), "the rewrite reads Baseline / Main parameter sets"
+            sps = {"log2_fn": b.ue() + 4, "poc_type": b.ue()}
+            assert sps["poc_type"] == 0
+            sps["log2_poc"] = b.ue() + 4
+        elif typ == 8:
+            b.ue(); b.ue()
+            assert b.u(1) == 0, "CAVLC"
+            pps["poc_present"] = b.u(1)
+            assert b.ue() == 0, "one slice group"
+            pps["nref"] = (b.ue() + 1, b.ue() + 1)
+            assert b.u(1) == 0 and b.u(2) == 0, "no weighted prediction"
+            b.se(); b.se(); b.se()
+            pps["dbk_present"] = b.u(1)
+            b.u(1)
+            pps["redundant"] = b.u(1)
+        elif typ in (1, 5) and pps.get("dbk_present"):
+            b.ue()
+            st_type = b.ue() % 5
+            assert st_type in (0, 2), "I and P slices"
+            b.ue(); b.u(sps["log2_fn"])
+            if typ == 5:
+                b.ue()
+            b.u(sps["log2_poc"])
+            if pps["poc_present"]:
+                b.se()
+            if pps["redundant"]:
+                b.ue()
+            if st_type == 0:
+                if b.u(1):
+                    b.ue()
+                if b.u(1):  # ref_pic_list_modification_flag_l0
+                    while (op := b.ue()) != 3:
+                        b.ue()
+            if hdr >> 5:
+                if typ == 5:
+                    b.u(2)
+                elif b.u(1):
+                    while (op := b.ue()) != 0:
+                        if op in (1, 3):
+                            b.ue()
+                        if op in (2, 3, 6):
+                            b.ue()
+                        if op == 4:
+                            b.ue()
+            b.se()  # slice_qp_delta
+            at = b.pos
+            old = b.ue()
+            body = b.bits.rstrip("0")
+            pad = (len(b.bits) - len(body)) // 8  # whole zero bytes after the stop bit's byte
+            rest = body[b.pos:-1]  # the bits before rbsp_stop_one_bit
+            bits = b.bits[:at] + _ue_bits(idc if old != 1 else old) + rest + "1"
+            bits += "0" * (-len(bits) % 8 + 8 * pad)
+            nal = bytes([hdr]) + _escape(bytes(int(bits[i:i + 8], 2) for i in range(0, len(bits), 8)))
+        out += data[st:st + 3] + nal + trail
+    return bytes(out)
 
 
 def prefix(data: bytes, n_pictures: int) -> bytes:
@@ -57,9 +190,11 @@ def prefix(data: bytes, n_pictures: int) -> bytes:
 
 def main():
     subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "decode"], check=True, capture_output=True)
-    for src, n_pictures, name in STREAMS:
+    for src, n_pictures, name, idc in STREAMS:
         with open(os.path.join(DATA, src), "rb") as f:
             data = prefix(f.read(), n_pictures)
+        if idc is not None:
+            data = with_deblocking_idc(data, idc)
         path = os.path.join(HERE, name + ".h264")
         with open(path, "wb") as f:
             f.write(data)

@@ -17,8 +17,9 @@ from oracle.refcpu import Dpb
 
 HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 CODINGS = ["heavyhand_cavlc_15", "heavyhand_cabac_10"]  # CAVLC and CABAC codings of one clip
-# + Baseline, 9 slices per picture with disable_deblocking_filter_idc 2, a damaged slice
-STREAMS = CODINGS + ["gop121_12"]
+# + Baseline, 9 slices per picture with disable_deblocking_filter_idc 2, a damaged slice;
+# the same with every slice's idc rewritten to 0 (deblocking across slice edges, quirk Q7)
+STREAMS = CODINGS + ["gop121_12", "gop121_idc0_12"]
 
 
 def _stream(name):
@@ -149,6 +150,10 @@ def test_damaged_slice_ends_where_the_reference_abandons_it():
     assert sorted(set(p0.mbs[:7714]["slice_num"].tolist())) == list(range(9))
     for p in pics[1:]:
         assert (p.n_slices, p.lost_mbs, p.first_lost_mb, p.damage) == (9, 0, -1, "")
+    # the idc 0 rewrite (tests/golden/make_real.py) changes the slice headers only
+    for a, b in zip(pics, Bitstream(_stream("gop121_idc0_12")).pictures()):
+        assert (a.desc.disable_deblocking_filter_idc, b.desc.disable_deblocking_filter_idc) == (2, 0)
+        assert np.array_equal(a.mbs, b.mbs) and np.array_equal(a.levels, b.levels)
 
 
 @pytest.mark.parametrize("name", STREAMS)
@@ -193,7 +198,7 @@ def test_real_stream_equals_reference_decoder(engine, name):
 
     dec = H264VideoDecoder(engine)
     dec.set_output_frame_callback_functuin(callback, None)
-    damaged = name.startswith("gop121")
+    damaged = name.startswith("gop121")  # both codings of it keep the damaged slice
     with pytest.warns(RuntimeWarning, match="not decoded") if damaged else contextlib.nullcontext():
         assert dec.open(_stream(name)) == 0
     assert [g[0] for g in got] == [r["poc"] for r in ref]

@@ -322,12 +322,13 @@ def _decode_all(engine, gen, pictures):
         engine.close_stream(sid)
 
 
-@pytest.mark.parametrize("slices,idc", [(7, 2), (23, 2), (3, 1)])
+@pytest.mark.parametrize("slices,idc", [(7, 2), (23, 2), (3, 1), (7, 0), (23, 0)])
 def test_multi_slice_pictures_match_oracle(any_engine, slices, idc):
     """Pictures of several slices (runs of MBs that end mid-row): intra prediction sees no
-    neighbour of another slice (6.4.8) and, with disable_deblocking_filter_idc 2, MB edges
-    between slices are not filtered; I, P and B pictures with a third of the inter
-    pictures' MBs intra, against the oracle."""
+    neighbour of another slice (6.4.8); with disable_deblocking_filter_idc 2 MB edges
+    between slices are not filtered, with 0 they are, with the current MB in the
+    neighbour's place for bS and qP (quirk Q7); I, P and B pictures with a third of the
+    inter pictures' MBs intra, against the oracle."""
     gen = Generator(2, width_mbs=23, height_mbs=9, n_frames=5, seed=0x51CE0000 + slices, slices=slices,
                     disable_deblocking_filter_idc=idc, intra_in_inter_permille=330)
     pics = [gen.picture(i) for i in range(gen.num_pictures)]
@@ -365,26 +366,19 @@ def test_mbs_of_no_slice_stop_deblocking(any_engine):
 
 
 def test_slice_layouts_the_engine_does_not_take_are_refused(engine):
-    """Several slices with disable_deblocking_filter_idc 0 (the reference filters such
-    edges with the current MB's data on both sides) and MBs of no slice in an MBAFF frame
-    are refused before anything runs."""
+    """Several slices, or MBs of no slice, in an MBAFF frame are refused before anything
+    runs."""
     from h264_video_decoder_demo_amd import _lib
-    gen = Generator(2, width_mbs=8, height_mbs=4, n_frames=1, seed=0x51CE0200, slices=2, disable_deblocking_filter_idc=2)
+    gen = Generator(3, width_mbs=6, height_mbs=4, n_frames=1, seed=0x51CE0201)
     desc, mbs, levels = gen.picture(0)
     sid = engine.open_stream_for(gen)
     try:
-        engine.release(engine.begin(sid, desc, mbs, levels))
-        desc.disable_deblocking_filter_idc = 0
-        with pytest.raises(_lib.H264GError, match="several slices"):
-            engine.begin(sid, desc, mbs, levels)
-    finally:
-        engine.close_stream(sid)
-    gen = Generator(3, width_mbs=6, height_mbs=4, n_frames=1, seed=0x51CE0201)
-    desc, mbs, levels = _lose_tail(*gen.picture(0), first=20)
-    sid = engine.open_stream_for(gen)
-    try:
+        two = mbs.copy()
+        two["slice_num"][12:] = 1
+        with pytest.raises(_lib.H264GError, match="several slices in an MBAFF"):
+            engine.begin(sid, desc, two, levels)
         with pytest.raises(_lib.H264GError, match="no slice in an MBAFF"):
-            engine.begin(sid, desc, mbs, levels)
+            engine.begin(sid, *_lose_tail(desc, mbs, levels, first=20))
     finally:
         engine.close_stream(sid)
 
