@@ -68,6 +68,7 @@ class PicDesc(ctypes.Structure):
         ("ref_slot", (c_int32 * 32) * 2), ("ref_poc", (c_int32 * 32) * 2), ("ref_long_term", (c_int32 * 32) * 2),
         ("luma_weight_flag_last", c_int32 * 2), ("luma_weight", (c_int16 * 32) * 2), ("luma_offset", (c_int16 * 32) * 2),
         ("chroma_weight", ((c_int16 * 2) * 32) * 2), ("chroma_offset", ((c_int16 * 2) * 32) * 2),
+        ("scaling_matrix", c_int32), ("weight_scale4x4", (c_uint8 * 16) * 6), ("weight_scale8x8", (c_uint8 * 64) * 2),
     ]
 
 

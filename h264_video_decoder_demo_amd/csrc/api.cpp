@@ -446,6 +446,31 @@ int stage_picture(h264g_ctx *ctx, Stream &st, const h264g_pic_desc &d, const h26
         if (first_sn == H264G_SLICE_NONE) first_sn = sn;
         if (sn != first_sn && mbaff) return h264g_set_error(H264G_E_ARG, "mb %u: several slices in an MBAFF frame", a);
     }
+    // Scaling matrices: LevelScale4x4 / 8x8 (8-315 / 8-318) of the two luma lists the
+    // reference dequantises with (chroma takes the MB's luma list, quirk Q13).
+    std::vector<uint16_t> lsv;
+    if (d.scaling_matrix) {
+        // the reference maps a field MB's lists with the field scan (H264PictureBase.cpp:4896-4897)
+        if (mbaff) return h264g_set_error(H264G_E_ARG, "scaling matrices in MBAFF frames are not supported");
+        static const int v4[6][3] = {{10, 16, 13}, {11, 18, 14}, {13, 20, 16}, {14, 23, 18}, {16, 25, 20}, {18, 29, 23}};
+        static const int v8[6][6] = {{20, 18, 32, 19, 25, 24}, {22, 19, 35, 21, 28, 26}, {26, 23, 42, 24, 33, 31},
+                                     {28, 25, 45, 26, 35, 33}, {32, 28, 51, 30, 40, 38}, {36, 32, 58, 34, 46, 43}};
+        lsv.assign(LS_ENTRIES, 0);
+        for (int l = 0; l < 2; l++)
+            for (int m = 0; m < 6; m++) {
+                for (int k = 0; k < 16; k++) {
+                    const int i = k >> 2, j = k & 3, c = ((i | j) & 1) == 0 ? 0 : ((i & j) & 1) ? 1 : 2;
+                    lsv[l * LS4_LIST + m * 16 + k] = (uint16_t)(d.weight_scale4x4[3 * l][k] * v4[m][c]);
+                }
+                for (int k = 0; k < 64; k++) {
+                    const int i = k >> 3, j = k & 7;
+                    const int c = (i % 4 == 0 && j % 4 == 0) ? 0 : (i % 2 == 1 && j % 2 == 1) ? 1
+                                : (i % 4 == 2 && j % 4 == 2) ? 2 : ((i % 4 == 0 && j % 2 == 1) || (i % 2 == 1 && j % 4 == 0)) ? 3
+                                : ((i % 4 == 0 && j % 4 == 2) || (i % 4 == 2 && j % 4 == 0)) ? 4 : 5;
+                    lsv[LS8_AT + l * LS8_LIST + m * 64 + k] = (uint16_t)(d.weight_scale8x8[l][k] * v8[m][c]);
+                }
+            }
+    }
     std::vector<DevAux> aux(n_mbs);
     std::vector<int8_t> bsref(mbaff ? (size_t)n_mbs * 8 : 0, (int8_t)-1);
     std::vector<DevWeight> wt;
@@ -711,7 +736,7 @@ int stage_picture(h264g_ctx *ctx, Stream &st, const h264g_pic_desc &d, const h26
     size_t n_items = pic->n_inter + pic->n_intra + pic->all_t.size() + pic->n_rows + irow.size() + icol.size();
     size_t o_bs = al(o_it + n_items * sizeof(uint32_t)), o_dbq = al(o_bs + (size_t)n_mbs * 32);
     size_t o_ires = al(o_dbq + (size_t)n_mbs * 4), o_bsref = al(o_ires + (size_t)pic->n_intra * 768);
-    size_t o_pic = al(o_bsref + bsref.size());
+    size_t o_ls = al(o_bsref + bsref.size()), o_pic = al(o_ls + lsv.size() * sizeof(uint16_t));
     size_t total = al(o_pic + sizeof(DevPic));
     void *dm = fixed;
     if (fixed) {
@@ -755,7 +780,9 @@ int stage_picture(h264g_ctx *ctx, Stream &st, const h264g_pic_desc &d, const h26
     use_set(st, 0, P);
     P.dbk_idc = d.disable_deblocking_filter_idc; P.cip = d.constrained_intra_pred_flag;
     P.dbk_stop = dbk_stop;
+    P.ls = lsv.empty() ? nullptr : (const uint16_t *)(base + o_ls);
     HIPCHK(hipMemcpy(pic->d_mbs, mbs, n_mbs * sizeof(h264g_mb), hipMemcpyHostToDevice));
+    if (!lsv.empty()) HIPCHK(hipMemcpy(base + o_ls, lsv.data(), lsv.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(pic->d_aux, aux.data(), n_mbs * sizeof(DevAux), hipMemcpyHostToDevice));
     if (n_levels) HIPCHK(hipMemcpy(pic->d_levels, levels, n_levels * sizeof(int16_t), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(pic->d_wt, wt.data(), wt.size() * sizeof(DevWeight), hipMemcpyHostToDevice));

@@ -14,13 +14,12 @@
 // (h264g_mvfield_*, mvpred.cpp) from the parsed mvd and prev / rem syntax.
 //
 // Pictures may have several slices (in order, sharing one reference list, weight table
-// and deblocking setting, with disable_deblocking_filter_idc 1 or 2); each MB record
-// carries its slice number, which makes neighbours in other slices unavailable.
-// Refused with H264G_E_ARG (none occur in the reference's progressive sample streams):
-// field pictures and MBAFF, slice edges deblocked with idc 0 (the reference's quirk Q7),
-// slice groups, arbitrary slice order, scaling matrices, chroma formats other than
-// 4:2:0, bit depths above 8, SP / SI slices, temporal direct prediction, POC type 1,
-// gaps in frame_num.
+// and deblocking setting); each MB record carries its slice number, which makes
+// neighbours in other slices unavailable.  SPS / PPS scaling matrices become the
+// picture's lists as the reference derives them.  Refused with H264G_E_ARG (none occur
+// in the reference's progressive sample streams): field pictures and MBAFF, slice groups,
+// arbitrary slice order, chroma formats other than 4:2:0, bit depths above 8, lossless
+// coding, SP / SI slices, temporal direct prediction, POC type 1, gaps in frame_num.
 #include <algorithm>
 #include <cstdarg>
 #include <cstdint>
@@ -291,6 +290,11 @@ const Tables &tables() {
 }
 
 // ------------------------------------------------------------------ parameter sets
+struct ScalingLists {
+    bool present_matrix = false;                 // seq_ / pic_scaling_matrix_present_flag
+    uint8_t present[8] = {}, use_default[8] = {};  // per list: *_scaling_list_present_flag, useDefault
+    uint8_t l4[6][16] = {}, l8[2][64] = {};      // as parsed, scan order; zero when not sent
+};
 struct Sps {
     bool valid = false;
     int profile_idc = 0, constraint3 = 0, level_idc = 0;
@@ -299,19 +303,44 @@ struct Sps {
     int direct_8x8_inference = 0;
     int max_num_reorder_frames = -1;  // VUI bitstream_restriction, -1 when absent
     int max_dec_frame_buffering = -1;
+    ScalingLists sl;
 };
 struct Pps {
     bool valid = false;
+    ScalingLists sl;
     int sps_id = 0, cabac = 0, bottom_field_pic_order = 0, num_slice_groups = 1;
     int num_ref_idx_default[2] = {1, 1}, weighted_pred = 0, weighted_bipred_idc = 0, pic_init_qp = 26;
     int chroma_qp_index_offset = 0, second_chroma_qp_index_offset = 0, deblocking_filter_control = 0;
     int constrained_intra_pred = 0, redundant_pic_cnt_present = 0, transform_8x8_mode = 0;
 };
 
-int skip_scaling_list(Bits &bs, int n) {  // only to reject them with a clear message
-    (void)bs;
-    (void)n;
-    return h264g_set_error(H264G_E_ARG, "scaling matrices are not supported");
+// scaling_list() (7.3.2.1.1.1) as the reference parses it (H264CommonFunc.cpp:48-67): every
+// entry is stored, useDefaultScalingMatrixFlag when the first delta makes nextScale 0.
+int parse_scaling_list(Bits &bs, uint8_t *list, int n, uint8_t &use_default) {
+    int last = 8, next = 8;
+    for (int j = 0; j < n; j++) {
+        if (next != 0) {
+            const int delta = bs.se();
+            if (delta < -128 || delta > 127) return h264g_set_error(H264G_E_ARG, "delta_scale %d", delta);
+            next = (last + delta + 256) % 256;
+            use_default = j == 0 && next == 0;
+        }
+        list[j] = (uint8_t)(next == 0 ? last : next);
+        last = list[j];
+    }
+    return 0;
+}
+// The scaling_list_present flags and lists of an SPS or PPS (4:2:0: six 4x4, two 8x8).
+int parse_scaling_lists(Bits &bs, ScalingLists &sl, int n) {
+    sl.present_matrix = true;
+    for (int i = 0; i < n; i++) {
+        sl.present[i] = (uint8_t)bs.u(1);
+        if (!sl.present[i]) continue;
+        if (int rc = i < 6 ? parse_scaling_list(bs, sl.l4[i], 16, sl.use_default[i])
+                           : parse_scaling_list(bs, sl.l8[i - 6], 64, sl.use_default[i]))
+            return rc;
+    }
+    return 0;
 }
 
 void skip_hrd(Bits &bs) {  // E.1.2
@@ -341,11 +370,12 @@ int parse_sps(Bits &bs, Sps *table) {  // 7.3.2.1.1
     if (p == 100 || p == 110 || p == 122 || p == 244 || p == 44 || p == 83 || p == 86 || p == 118 || p == 128 ||
         p == 138 || p == 139 || p == 134 || p == 135) {
         s.chroma_format_idc = (int)bs.ue();
-        if (s.chroma_format_idc == 3) bs.u(1);
+        if (s.chroma_format_idc != 1) return h264g_set_error(H264G_E_ARG, "only 4:2:0 is supported");
         const uint32_t bdl = bs.ue(), bdc = bs.ue();
-        bs.u(1);  // qpprime_y_zero_transform_bypass_flag
+        if (bs.u(1)) return h264g_set_error(H264G_E_ARG, "lossless coding (qpprime_y_zero_transform_bypass_flag) is not supported");
         if (bdl || bdc) return h264g_set_error(H264G_E_ARG, "only 8-bit samples are supported");
-        if (bs.u(1)) return skip_scaling_list(bs, 0);
+        if (bs.u(1))
+            if (int rc = parse_scaling_lists(bs, s.sl, 8)) return rc;
     }
     if (s.chroma_format_idc != 1) return h264g_set_error(H264G_E_ARG, "only 4:2:0 is supported");
     s.log2_max_frame_num = (int)bs.ue() + 4;
@@ -436,8 +466,12 @@ int parse_pps(Bits &bs, Pps *table, const Sps *sps) {  // 7.3.2.2
     q.second_chroma_qp_index_offset = q.chroma_qp_index_offset;
     if (bs.more_rbsp_data()) {
         q.transform_8x8_mode = (int)bs.u(1);
-        if (bs.u(1)) return skip_scaling_list(bs, 0);
-        q.second_chroma_qp_index_offset = bs.se();
+        // the reference reads second_chroma_qp_index_offset only inside the scaling-matrix
+        // branch (H264PPS.cpp:207-227): without a matrix the PPS keeps chroma_qp_index_offset
+        if (bs.u(1)) {
+            if (int rc = parse_scaling_lists(bs, q.sl, 6 + 2 * q.transform_8x8_mode)) return rc;
+            q.second_chroma_qp_index_offset = bs.se();
+        }
     }
     if (bs.over) return h264g_set_error(H264G_E_ARG, "truncated PPS");
     q.valid = true;
@@ -1988,9 +2022,79 @@ bool explicit_wp(const Pps &pps, const Slice &sl) {
 }
 
 // Fills the picture's descriptor from its first slice (except dst_slot).
+// Default_4x4_Intra / _Inter and Default_8x8_Intra / _Inter (Table 7-3, 7-4), scan order.
+const uint8_t kDefault4[2][16] = {{6, 13, 13, 20, 20, 20, 28, 28, 28, 28, 32, 32, 32, 37, 37, 42},
+                                  {10, 14, 14, 20, 20, 20, 24, 24, 24, 24, 27, 27, 27, 30, 30, 34}};
+const uint8_t kDefault8[2][64] = {
+    {6,  10, 10, 13, 11, 13, 16, 16, 16, 16, 18, 18, 18, 18, 18, 23, 23, 23, 23, 23, 23, 25,
+     25, 25, 25, 25, 25, 25, 27, 27, 27, 27, 27, 27, 27, 27, 29, 29, 29, 29, 29, 29, 29, 31,
+     31, 31, 31, 31, 31, 33, 33, 33, 33, 33, 36, 36, 36, 36, 38, 38, 38, 40, 40, 42},
+    {9,  13, 13, 15, 13, 15, 17, 17, 17, 17, 19, 19, 19, 19, 19, 21, 21, 21, 21, 21, 21, 22,
+     22, 22, 22, 22, 22, 22, 24, 24, 24, 24, 24, 24, 24, 24, 25, 25, 25, 25, 25, 25, 25, 27,
+     27, 27, 27, 27, 27, 28, 28, 28, 28, 28, 30, 30, 30, 30, 32, 32, 32, 33, 33, 35}};
+
+// Raster position of each zig-zag scan position of an n x n block (Tables 8-12, 8-13):
+// anti-diagonals in turn, odd ones walked down-left, even ones up-right.
+template <int N>
+struct ZigZag {
+    uint8_t at[N * N];
+    constexpr ZigZag() : at() {
+        int k = 0;
+        for (int d = 0; d < 2 * N - 1; d++)
+            for (int t = 0; t <= d; t++) {
+                const int i = d & 1 ? t : d - t, j = d - i;  // row i, column j
+                if (i < N && j < N) at[k++] = (uint8_t)(i * N + j);
+            }
+    }
+};
+constexpr ZigZag<4> kZz4;
+constexpr ZigZag<8> kZz8;
+
+// The picture's scaling lists as the reference's set_scaling_lists_values leaves them
+// (H264SliceHeader.cpp:928-1158): Table 7-2 fall-back rules A (SPS) then B (PPS), in raster
+// order.  Quirk: a list the PPS sends explicitly is taken from the SPS's parsed list of
+// that index (zero when the SPS did not send it), not from the PPS (:1093-1106, 1136-1149).
+void scaling_desc(const Sps &sps, const Pps &pps, h264g_pic_desc &d) {
+    const ScalingLists &a = sps.sl, &b = pps.sl;
+    if (!a.present_matrix && !b.present_matrix) return;  // flat
+    uint8_t l4[6][16] = {}, l8[2][64] = {};
+    if (a.present_matrix) {
+        for (int i = 0; i < 6; i++) {
+            const uint8_t *src = !a.present[i] ? (i == 0 || i == 3 ? kDefault4[i / 3] : l4[i - 1])
+                               : a.use_default[i] ? kDefault4[i / 3] : a.l4[i];
+            memcpy(l4[i], src, 16);
+        }
+        for (int i = 0; i < 2; i++)
+            memcpy(l8[i], !a.present[6 + i] || a.use_default[6 + i] ? kDefault8[i] : a.l8[i], 64);
+    }
+    if (b.present_matrix) {
+        for (int i = 0; i < 6; i++) {
+            if (!b.present[i]) {
+                if (i != 0 && i != 3) memcpy(l4[i], l4[i - 1], 16);
+                else if (!a.present_matrix) memcpy(l4[i], kDefault4[i / 3], 16);
+            } else {
+                memcpy(l4[i], b.use_default[i] ? kDefault4[i / 3] : a.l4[i], 16);
+            }
+        }
+        for (int i = 0; i < 2; i++) {
+            if (!b.present[6 + i]) {
+                if (!a.present_matrix) memcpy(l8[i], kDefault8[i], 64);
+            } else {
+                memcpy(l8[i], b.use_default[6 + i] ? kDefault8[i] : a.l8[i], 64);
+            }
+        }
+    }
+    d.scaling_matrix = 1;
+    for (int i = 0; i < 6; i++)
+        for (int k = 0; k < 16; k++) d.weight_scale4x4[i][kZz4.at[k]] = l4[i][k];
+    for (int i = 0; i < 2; i++)
+        for (int k = 0; k < 64; k++) d.weight_scale8x8[i][kZz8.at[k]] = l8[i][k];
+}
+
 void fill_desc(const h264g_parser *p, const Pps &pps, const Slice &sl, int poc, const std::vector<int> lists[2],
                h264g_pic_desc &d) {
     memset(&d, 0, sizeof(d));
+    scaling_desc(*p->cur_sps, pps, d);
     d.slice_type = sl.slice_type;
     d.structure = H264G_FRAME;
     d.constrained_intra_pred_flag = pps.constrained_intra_pred;

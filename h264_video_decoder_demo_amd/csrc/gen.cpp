@@ -15,6 +15,7 @@
 #include "h264g_internal.h"
 
 #include <cmath>
+#include <cstddef>
 #include <cstring>
 #include <vector>
 #include <algorithm>
@@ -943,7 +944,9 @@ int32_t h264g_gen_create(const h264g_gen_params *params, h264g_gen **out) {
     }
     uint64_t h = 0xCBF29CE484222325ull;
     for (auto &pic : g->pics) {
-        h = fnv1a(h, &pic.desc, sizeof(pic.desc));
+        // the descriptor up to the scaling matrices, which the generator leaves flat (the
+        // digest of a stream stays what it was before the descriptor carried them)
+        h = fnv1a(h, &pic.desc, offsetof(h264g_pic_desc, scaling_matrix));
         h = fnv1a(h, pic.mbs.data(), pic.mbs.size() * sizeof(h264g_mb));
         h = fnv1a(h, pic.levels.data(), pic.levels.size() * sizeof(int16_t));
     }

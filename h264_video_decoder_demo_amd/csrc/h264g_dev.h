@@ -80,7 +80,14 @@ struct DevPic {
     uint32_t *pairprog;       // MBAFF streams: per-MB-pair deblocking progress words
     uint32_t dbk_stop;        // MBs from this address on filter none of their edges (the
                               // first MB of no slice; W * H when every MB has one)
+    const uint16_t *ls;       // scaling matrices: LevelScale4x4 [intra, inter luma list][qP % 6]
+                              // [raster 16] at 0, LevelScale8x8 [list][qP % 6][raster 64] at
+                              // LS8_AT; null with flat lists
 };
+#define LS4_LIST 96           // entries per 4x4 list (6 x 16)
+#define LS8_AT   192          // first 8x8 entry
+#define LS8_LIST 384          // entries per 8x8 list (6 x 64)
+#define LS_ENTRIES 960
 
 // Work item: picture index in the batch and macroblock address.
 #define WI_PIC_SHIFT 20u

@@ -17,6 +17,24 @@
 
 #include "h264g_dev.h"
 
+// Device memory seen through global-address-space pointers: loads and stores through
+// them are global_* instructions, which count on the vector-memory counter only.  Through
+// generic pointers they would be flat_* instructions, which also count on the LDS counter,
+// so every LDS wait would wait for them too (a prefetch from HBM would stall the next LDS
+// access).
+// (The host pass of this file parses the device code too; it sees plain pointers.)
+#if defined(__HIP_DEVICE_COMPILE__)
+#define GAS __attribute__((address_space(1)))
+#else
+#define GAS
+#endif
+typedef GAS uint8_t gu8;
+template <typename T>
+__device__ __forceinline__ GAS T *G(T *p) { return (GAS T *)p; }
+template <typename T>
+__device__ __forceinline__ const GAS T *G(const T *p) { return (const GAS T *)p; }
+
+
 namespace {
 
 constexpr int WAVE = 64;
@@ -187,11 +205,19 @@ __device__ __forceinline__ int32_t level_at(uint32_t e, const int16_t *vals, int
 // `dc` as it stands (Intra16x16 and chroma DC).  field: the levels follow the field scan
 // (field MBs, H264PictureBase.cpp:3419).
 __device__ __forceinline__ void residual4x4_quad(uint32_t se, const int16_t *vals, int qp, bool dc_pass, int dc, int r,
-                                                 bool field, int32_t (&out)[4]) {
+                                                 bool field, int32_t (&out)[4], const GAS uint16_t *lsm = nullptr) {
     const int m = qp % 6, e = qp / 6;
     const int s0 = 16 * c_v4[m][0], s1 = 16 * c_v4[m][1], s2 = 16 * c_v4[m][2];
-    // LevelScale4x4 of row r: s0 at (even, even), s1 at (odd, odd), s2 elsewhere
-    const int ls_even = (r & 1) ? s2 : s0, ls_odd = (r & 1) ? s1 : s2;
+    // LevelScale4x4 of row r: with flat lists s0 at (even, even), s1 at (odd, odd), s2
+    // elsewhere; with scaling matrices lsm holds LevelScale4x4(qP % 6, i, j), raster
+    int ls[4];
+    if (lsm) {
+        const uint2 t = *(const GAS uint2 *)(lsm + 4 * r);
+        ls[0] = (int)(t.x & 0xFFFFu); ls[1] = (int)(t.x >> 16); ls[2] = (int)(t.y & 0xFFFFu); ls[3] = (int)(t.y >> 16);
+    } else {
+        ls[0] = ls[2] = (r & 1) ? s2 : s0;
+        ls[1] = ls[3] = (r & 1) ? s1 : s2;
+    }
     // 8-336 / 8-337 as one expression: qP / 6 >= 4 shifts left, else rounds and shifts right
     const int lsh = e >= 4 ? e - 4 : 0, rsh = e >= 4 ? 0 : 4 - e, rnd = e >= 4 ? 0 : 1 << (3 - e);
     const uint32_t iz = (uint32_t)((field ? IFLD4 : IZZ4) >> (16 * r)) & 0xFFFFu;
@@ -199,7 +225,7 @@ __device__ __forceinline__ void residual4x4_quad(uint32_t se, const int16_t *val
 #pragma unroll
     for (int j = 0; j < 4; j++) {
         const int32_t v = level_at(se, vals, (int)((iz >> (4 * j)) & 15u));
-        d[j] = (int32_t)(((uint32_t)(v * ((j & 1) ? ls_odd : ls_even)) << lsh) + (uint32_t)rnd) >> rsh;
+        d[j] = (int32_t)(((uint32_t)(v * ls[j]) << lsh) + (uint32_t)rnd) >> rsh;
     }
     if (dc_pass && r == 0) d[0] = dc;
     int32_t f[4];
@@ -224,9 +250,11 @@ __device__ __forceinline__ void idct8(int32_t *v, int s) {  // 8.5.13.2, in plac
 }
 
 // Places the MB's levels in LDS and computes all residual blocks into ws->rl / ws->rc.  Must be called by the whole wave.  Returns false when the MB has no levels.
-// Field MBs of MBAFF frames take the field scans.
+// Field MBs of MBAFF frames take the field scans.  lstab: the picture's LevelScale tables
+// when it has scaling matrices (DevPic::ls), else null (flat lists).
 template <bool MBAFF>
-__device__ __forceinline__ bool build_residual(const h264g_mb *mb, const DevAux &ax, const int16_t *levels, WaveRes *ws) {
+__device__ __forceinline__ bool build_residual(const h264g_mb *mb, const DevAux &ax, const int16_t *levels, WaveRes *ws,
+                                               const GAS uint16_t *lstab) {
     const int lane = wave_lane();
     if (!(ax.kind & AUX_RESID)) return false;
     const bool field = MBAFF && (mb->flags & H264G_MBF_FIELD) != 0;
@@ -234,6 +262,10 @@ __device__ __forceinline__ bool build_residual(const h264g_mb *mb, const DevAux 
     const int i16 = mb->mb_type == H264G_MB_I16x16;
     const int t8 = (ax.kind & AUX_T8) ? 1 : 0;
     const int qp = mb->qp_y;
+    // the MB's luma lists (intra or inter), which serve its chroma blocks too (quirk Q13)
+    const bool inter = !(ax.kind & AUX_INTRA);
+    const GAS uint16_t *ls4t = lstab ? lstab + (inter ? LS4_LIST : 0) : nullptr;
+    const GAS uint16_t *ls8t = lstab ? lstab + LS8_AT + (inter ? LS8_LIST : 0) : nullptr;
     const int16_t *lv = levels + mb->level_offset;
     // <= 27 masks of 16 bits, so at most 432 values; the host counted them, so the masks
     // and the values load together (coalesced, all in flight at once)
@@ -284,14 +316,20 @@ __device__ __forceinline__ bool build_residual(const h264g_mb *mb, const DevAux 
             const int ls_0 = (i & 1) ? v3 : (i & 2) ? v4 : v0;
             const int ls_1 = (i & 1) ? v1 : (i & 2) ? v5 : v3;
             const int ls_2 = (i & 1) ? v5 : (i & 2) ? v2 : v4;
+            int lsr[8];  // LevelScale8x8 of row i: the classes above, or the matrix row
+#pragma unroll
+            for (int j = 0; j < 8; j++) lsr[j] = (j & 1) ? ls_1 : (j & 2) ? ls_2 : ls_0;
+            if (ls8t) {
+                const uint4 t = *(const GAS uint4 *)(ls8t + 64 * m + 8 * i);
+                const uint32_t w[4] = {t.x, t.y, t.z, t.w};
+#pragma unroll
+                for (int j = 0; j < 8; j++) lsr[j] = (int)((w[j >> 1] >> (16 * (j & 1))) & 0xFFFFu);
+            }
             // 8-338 / 8-339 as one expression (qP / 6 >= 6 shifts left)
             const int lsh = e >= 6 ? e - 6 : 0, rsh = e >= 6 ? 0 : 6 - e, rnd = e >= 6 ? 0 : 1 << (5 - e);
             int32_t d[8];
 #pragma unroll
-            for (int j = 0; j < 8; j++) {
-                const int ls = (j & 1) ? ls_1 : (j & 2) ? ls_2 : ls_0;
-                d[j] = (int32_t)(((uint32_t)(src[j] * ls) << lsh) + (uint32_t)rnd) >> rsh;
-            }
+            for (int j = 0; j < 8; j++) d[j] = (int32_t)(((uint32_t)(src[j] * lsr[j]) << lsh) + (uint32_t)rnd) >> rsh;
             idct8(d, 1);
 #pragma unroll
             for (int j = 0; j < 8; j++) ws->t8[q][i * 8 + j] = d[j];
@@ -322,11 +360,11 @@ __device__ __forceinline__ bool build_residual(const h264g_mb *mb, const DevAux 
                 }
                 f += hneg(row, i) ? -acc : acc;
             }
-            const int sc = ls4(qp % 6, 0, 0);
+            const int sc = ls4t ? ls4t[16 * (qp % 6)] : ls4(qp % 6, 0, 0);
             dc = qp >= 36 ? (f * sc) << (qp / 6 - 6) : (f * sc + (1 << (5 - qp / 6))) >> (6 - qp / 6);
         }
         int32_t o[4];
-        residual4x4_quad(ws->sm[1 + b], ws->vals, qp, i16, dc, r, field, o);
+        residual4x4_quad(ws->sm[1 + b], ws->vals, qp, i16, dc, r, field, o, ls4t ? ls4t + 16 * (qp % 6) : nullptr);
         const int x0 = blk_x(b), y0 = blk_y(b);
 #pragma unroll
         for (int i = 0; i < 4; i++) ws->rl[(y0 + i) * 16 + x0 + r] = (int16_t)o[i];
@@ -341,9 +379,9 @@ __device__ __forceinline__ bool build_residual(const h264g_mb *mb, const DevAux 
         const int32_t *dcv = ws->cdc[c];
         const int32_t e00 = dcv[0] + dcv[2], e01 = dcv[1] + dcv[3], e10 = dcv[0] - dcv[2], e11 = dcv[1] - dcv[3];
         const int32_t f = b == 0 ? e00 + e01 : b == 1 ? e00 - e01 : b == 2 ? e10 + e11 : e10 - e11;
-        const int dc = ((f * ls4(qpc % 6, 0, 0)) << (qpc / 6)) >> 5;
+        const int dc = ((f * (ls4t ? ls4t[16 * (qpc % 6)] : ls4(qpc % 6, 0, 0))) << (qpc / 6)) >> 5;
         int32_t o[4];
-        residual4x4_quad(ws->sm[19 + 4 * c + b], ws->vals, qpc, true, dc, r, field, o);
+        residual4x4_quad(ws->sm[19 + 4 * c + b], ws->vals, qpc, true, dc, r, field, o, ls4t ? ls4t + 16 * (qpc % 6) : nullptr);
         const int x0 = (b & 1) * 4, y0 = (b >> 1) * 4;
 #pragma unroll
         for (int i = 0; i < 4; i++) ws->rc[c][(y0 + i) * 8 + x0 + r] = (int16_t)o[i];
@@ -367,23 +405,6 @@ __device__ __forceinline__ bool build_residual(const h264g_mb *mb, const DevAux 
 }
 
 // ------------------------------------------------------------ inter MC
-// Device memory seen through global-address-space pointers: loads and stores through
-// them are global_* instructions, which count on the vector-memory counter only.  Through
-// generic pointers they would be flat_* instructions, which also count on the LDS counter,
-// so every LDS wait would wait for them too (a prefetch from HBM would stall the next LDS
-// access).
-// (The host pass of this file parses the device code too; it sees plain pointers.)
-#if defined(__HIP_DEVICE_COMPILE__)
-#define GAS __attribute__((address_space(1)))
-#else
-#define GAS
-#endif
-typedef GAS uint8_t gu8;
-template <typename T>
-__device__ __forceinline__ GAS T *G(T *p) { return (GAS T *)p; }
-template <typename T>
-__device__ __forceinline__ const GAS T *G(const T *p) { return (const GAS T *)p; }
-
 __device__ __forceinline__ gu8 *slot_base(const DevPic &P, int s) { return G(P.dpb) + (uint64_t)s * P.slot_bytes; }
 
 // Where an MB's samples lie (6.4.1, H264PictureBase.cpp:2503-2530): the picture row of its
@@ -647,7 +668,7 @@ __device__ void inter_mb(const DevPic &P, int addr, WaveRes *ws) {
         dcr[(g.cy + (lane >> 3) * g.step) * wc + mbx * 8 + (lane & 7)] = (uint8_t)pcm[320 + lane];
         return;
     }
-    const bool res = build_residual<MBAFF>(mb, ax, G(P.levels), ws);
+    const bool res = build_residual<MBAFF>(mb, ax, G(P.levels), ws, G(P.ls));
     // ---- luma: lane -> 4x4 block b = lane & 15 (raster), row r = lane >> 4
     {
         const int b = lane & 15, r = lane >> 4, bx = b & 3, by = b >> 2, q = (by >> 1) * 2 + (bx >> 1);
@@ -1031,7 +1052,7 @@ static_assert(offsetof(WaveRes, rc) == offsetof(WaveRes, rl) + 512 && offsetof(W
 template <bool MBAFF>
 __device__ __forceinline__ void intra_res_build(const DevPic &P, int addr, WaveRes *ws) {
     const DevAux &ax = G(P.aux)[addr];
-    if (!build_residual<MBAFF>(&G(P.mbs)[addr], ax, G(P.levels), ws)) return;
+    if (!build_residual<MBAFF>(&G(P.mbs)[addr], ax, G(P.levels), ws, G(P.ls))) return;
     const int lane = wave_lane();
     if (lane < 48) ((uint4 *)(G(P.ires) + (size_t)(ax.ires & AUX_IRES_MASK) * 384))[lane] = ((const uint4 *)ws->rl)[lane];
 }

@@ -252,6 +252,16 @@ typedef struct h264g_pic_desc {
     int16_t  luma_offset[2][32];
     int16_t  chroma_weight[2][32][2];
     int16_t  chroma_offset[2][32][2];
+    /* Scaling matrices (7.4.2.1.1.1, Table 7-2), the lists the reference's
+     * set_scaling_lists_values leaves for the picture (H264SliceHeader.cpp:928-1158),
+     * in raster order (row * 4 + col, row * 8 + col: the frame inverse scan of 8.5.6 /
+     * 8.5.7).  0 in scaling_matrix: flat lists (16), the arrays are not read.  As in the
+     * reference, dequantisation uses only the luma lists: chroma blocks take the
+     * LevelScale4x4 of the MB's luma list (quirk Q13, H264PictureBase.cpp:3752-3929).
+     * Frame pictures only (MBAFF frames would take the field scan for field MBs). */
+    int32_t  scaling_matrix;
+    uint8_t  weight_scale4x4[6][16];          /* Intra Y, Cb, Cr, Inter Y, Cb, Cr      */
+    uint8_t  weight_scale8x8[2][64];          /* Intra Y, Inter Y                      */
 } h264g_pic_desc;
 
 typedef struct h264g_ctx h264g_ctx;
@@ -500,8 +510,9 @@ int32_t h264g_mvfield_colocated(h264g_mvfield *field, const h264g_mb *col_mbs, i
  * deblocking parameters).  Slice data that fails to parse ends its slice where it fails, and MBs that no
  * slice decodes are H264G_MBF_LOST records, as the reference skips the rest of such a
  * slice and decodes on (H264VideoDecoder.cpp:135-136); lost_mbs counts them.
+ * SPS / PPS scaling matrices give the descriptor's lists (h264g_pic_desc.scaling_matrix).
  * Refused with H264G_E_ARG: field pictures and MBAFF, slice groups, data partitioning,
- * scaling matrices, formats other than 8-bit 4:2:0, SP / SI slices, temporal direct
+ * formats other than 8-bit 4:2:0, lossless coding, SP / SI slices, temporal direct
  * prediction, POC type 1, gaps in frame_num.  Host only. */
 typedef struct h264g_parser h264g_parser;
 typedef struct h264g_parsed_picture {

@@ -13,6 +13,9 @@
  *       3011-3013 with the member layout of H264SliceHeader.h:78-80);
  *   Q3  bS = 1 test for bi-predicted partitions with one reference picture parses as
  *       L0L0 || (L1L1 && L0pL1q) || L1pL0q (Deblock.cpp:1289-1299);
+ *   Q13 with scaling matrices, chroma blocks dequantise with the LevelScale4x4 of the
+ *       MB's luma list (Intra Y or Inter Y), the one its luma driver derived last: the
+ *       chroma drivers never derive their own (H264PictureBase.cpp:3752-3929, 4852-4989);
  *   Q16 deblocking derives the Cb edge QPs with second_chroma_qp_index_offset and the
  *       Cr edge QPs with chroma_qp_index_offset (argument mix-up, Deblock.cpp:813-815);
  *   I_PCM macroblocks are not "intra" for bS (m_mb_pred_mode is Intra_NA, so
@@ -92,11 +95,14 @@ static const uint8_t TC0[3][52] = {
     {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 2, 2, 2, 2, 3, 3, 3, 4, 4, 5, 5, 6, 7, 8, 8, 10, 11, 12, 13, 15, 17},
     {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 2, 2, 2, 2, 3, 3, 3, 4, 4, 4, 5, 6, 6, 7, 8, 9, 10, 11, 13, 14, 16, 18, 20, 23, 25}};
 
-static int ls4(int m, int i, int j) { /* LevelScale4x4 with flat lists (weightScale = 16) */
+/* LevelScale4x4 / 8x8 (8-315 / 8-318, H264PictureBase.cpp:4852-4989): weightScale (raster,
+ * NULL = the flat 16) times normAdjust */
+static int ls4w(const uint8_t *w, int m, int i, int j) {
     int cls = (i % 2 == 0 && j % 2 == 0) ? 0 : ((i % 2 == 1 && j % 2 == 1) ? 1 : 2);
-    return 16 * V4[m][cls];
+    return (w ? w[i * 4 + j] : 16) * V4[m][cls];
 }
-static int ls8(int m, int i, int j) {
+static int ls4(int m, int i, int j) { return ls4w(NULL, m, i, j); }
+static int ls8w(const uint8_t *w, int m, int i, int j) {
     int cls;
     if (i % 4 == 0 && j % 4 == 0) cls = 0;
     else if (i % 2 == 1 && j % 2 == 1) cls = 1;
@@ -104,7 +110,7 @@ static int ls8(int m, int i, int j) {
     else if ((i % 4 == 0 && j % 2 == 1) || (i % 2 == 1 && j % 4 == 0)) cls = 3;
     else if ((i % 4 == 0 && j % 4 == 2) || (i % 4 == 2 && j % 4 == 0)) cls = 4;
     else cls = 5;
-    return 16 * V8[m][cls];
+    return (w ? w[i * 8 + j] : 16) * V8[m][cls];
 }
 
 /* ------------------------------------------------- 8.5 transforms */
@@ -113,8 +119,9 @@ int refcpu_chroma_qp(int qpy, int offset) { /* H264PictureBase.cpp:4748-4813 */
     return qpi < 30 ? qpi : QPC_TAB[qpi - 30];
 }
 
-/* 8.5.12 (H264PictureBase.cpp:4106-4266); r is raster row-major. */
-void refcpu_residual4x4(const int32_t scan[16], int qp, int dc_passthrough, int field, int32_t r[16]) {
+/* 8.5.12 (H264PictureBase.cpp:4106-4266); r is raster row-major; w: weightScale4x4 or NULL. */
+static void residual4x4_w(const int32_t scan[16], int qp, int dc_passthrough, int field, const uint8_t *w,
+                          int32_t r[16]) {
     int32_t c[16], d[16], f[16];
     const uint8_t *tab = field ? FLD4 : ZZ4;
     for (int k = 0; k < 16; k++) c[tab[k]] = scan[k];
@@ -122,8 +129,8 @@ void refcpu_residual4x4(const int32_t scan[16], int qp, int dc_passthrough, int 
         for (int j = 0; j < 4; j++) {
             int32_t v = c[i * 4 + j];
             if (i == 0 && j == 0 && dc_passthrough) d[0] = v;
-            else if (qp >= 24) d[i * 4 + j] = (v * ls4(qp % 6, i, j)) << (qp / 6 - 4);
-            else d[i * 4 + j] = (v * ls4(qp % 6, i, j) + (1 << (3 - qp / 6))) >> (4 - qp / 6);
+            else if (qp >= 24) d[i * 4 + j] = (v * ls4w(w, qp % 6, i, j)) << (qp / 6 - 4);
+            else d[i * 4 + j] = (v * ls4w(w, qp % 6, i, j) + (1 << (3 - qp / 6))) >> (4 - qp / 6);
         }
     for (int i = 0; i < 4; i++) { /* rows */
         int32_t *q = &d[i * 4];
@@ -135,6 +142,9 @@ void refcpu_residual4x4(const int32_t scan[16], int qp, int dc_passthrough, int 
         r[j] = (g0 + g3 + 32) >> 6; r[4 + j] = (g1 + g2 + 32) >> 6;
         r[8 + j] = (g1 - g2 + 32) >> 6; r[12 + j] = (g0 - g3 + 32) >> 6;
     }
+}
+void refcpu_residual4x4(const int32_t scan[16], int qp, int dc_passthrough, int field, int32_t r[16]) {
+    residual4x4_w(scan, qp, dc_passthrough, field, NULL, r);
 }
 
 static void idct8_1d(const int32_t *in, int stride, int32_t *out, int ostride) { /* 8.5.13.2 */
@@ -148,25 +158,26 @@ static void idct8_1d(const int32_t *in, int stride, int32_t *out, int ostride) {
     out[4 * ostride] = f6 - f1; out[5 * ostride] = f4 - f3; out[6 * ostride] = f2 - f5; out[7 * ostride] = f0 - f7;
 }
 
-/* 8.5.13 (H264PictureBase.cpp:4270-4403) */
-void refcpu_residual8x8(const int32_t scan[64], int qp, int field, int32_t r[64]) {
+/* 8.5.13 (H264PictureBase.cpp:4270-4403); w: weightScale8x8 or NULL */
+static void residual8x8_w(const int32_t scan[64], int qp, int field, const uint8_t *w, int32_t r[64]) {
     int32_t c[64], d[64], g[64], m[64];
     const uint8_t *tab = field ? FLD8 : ZZ8;
     for (int k = 0; k < 64; k++) c[tab[k]] = scan[k];
     for (int i = 0; i < 8; i++)
         for (int j = 0; j < 8; j++) {
             int32_t v = c[i * 8 + j];
-            if (qp >= 36) d[i * 8 + j] = (v * ls8(qp % 6, i, j)) << (qp / 6 - 6);
-            else d[i * 8 + j] = (v * ls8(qp % 6, i, j) + (1 << (5 - qp / 6))) >> (6 - qp / 6);
+            if (qp >= 36) d[i * 8 + j] = (v * ls8w(w, qp % 6, i, j)) << (qp / 6 - 6);
+            else d[i * 8 + j] = (v * ls8w(w, qp % 6, i, j) + (1 << (5 - qp / 6))) >> (6 - qp / 6);
         }
     for (int i = 0; i < 8; i++) idct8_1d(&d[i * 8], 1, &g[i * 8], 1);
     for (int j = 0; j < 8; j++) idct8_1d(&g[j], 8, &m[j], 8);
     for (int k = 0; k < 64; k++) r[k] = (m[k] + 32) >> 6;
 }
+void refcpu_residual8x8(const int32_t scan[64], int qp, int field, int32_t r[64]) { residual8x8_w(scan, qp, field, NULL, r); }
 
 /* 8.5.10 (H264PictureBase.cpp:4993-5088; the DC levels take the MB's scan,
  * H264PictureBase.cpp:3519) */
-static void luma_dc_scan(const int32_t scan[16], int qp, int field, int32_t dcY[16]) {
+static void luma_dc_scan(const int32_t scan[16], int qp, int field, const uint8_t *w, int32_t dcY[16]) {
     int32_t c[16], g[16], f[16];
     const uint8_t *tab = field ? FLD4 : ZZ4;
     for (int k = 0; k < 16; k++) c[tab[k]] = scan[k];
@@ -178,20 +189,21 @@ static void luma_dc_scan(const int32_t scan[16], int qp, int field, int32_t dcY[
         int32_t a = g[i * 4], b = g[i * 4 + 1], cc = g[i * 4 + 2], d = g[i * 4 + 3];
         f[i * 4] = a + b + cc + d; f[i * 4 + 1] = a + b - cc - d; f[i * 4 + 2] = a - b - cc + d; f[i * 4 + 3] = a - b + cc - d;
     }
-    int s = ls4(qp % 6, 0, 0);
+    int s = ls4w(w, qp % 6, 0, 0);
     for (int k = 0; k < 16; k++) {
         if (qp >= 36) dcY[k] = (f[k] * s) << (qp / 6 - 6);
         else dcY[k] = (f[k] * s + (1 << (5 - qp / 6))) >> (6 - qp / 6);
     }
 }
-void refcpu_luma_dc(const int32_t scan[16], int qp, int32_t dcY[16]) { luma_dc_scan(scan, qp, 0, dcY); }
+void refcpu_luma_dc(const int32_t scan[16], int qp, int32_t dcY[16]) { luma_dc_scan(scan, qp, 0, NULL, dcY); }
 
 /* 8.5.11 for 4:2:0 (H264PictureBase.cpp:3989-4043) */
-void refcpu_chroma_dc(const int32_t c[4], int qpc, int32_t dcC[4]) {
+static void chroma_dc_w(const int32_t c[4], int qpc, const uint8_t *w, int32_t dcC[4]) {
     int32_t e00 = c[0] + c[2], e01 = c[1] + c[3], e10 = c[0] - c[2], e11 = c[1] - c[3];
     int32_t f[4] = {e00 + e01, e00 - e01, e10 + e11, e10 - e11};
-    for (int k = 0; k < 4; k++) dcC[k] = ((f[k] * ls4(qpc % 6, 0, 0)) << (qpc / 6)) >> 5;
+    for (int k = 0; k < 4; k++) dcC[k] = ((f[k] * ls4w(w, qpc % 6, 0, 0)) << (qpc / 6)) >> 5;
 }
+void refcpu_chroma_dc(const int32_t c[4], int qpc, int32_t dcC[4]) { chroma_dc_w(c, qpc, NULL, dcC); }
 
 /* ---------------------------------------------- 8.4.2.2 interpolation */
 static int px(const uint8_t *ref, int w, int h, int x, int y) {
@@ -551,19 +563,21 @@ static void add_residual(uint8_t *o, int pitch, int x, int y, const int32_t *r, 
 
 /* residual of the luma 4x4 block b for non-I16x16 MBs (8.5.1); field MBs use the field
  * scan (field_pic_flag | mb_field_decoding_flag, H264PictureBase.cpp:3419) */
-static void luma4x4_residual(const Levels *L, int b, int qp, int field, int32_t r[16]) {
-    refcpu_residual4x4(L->l4[b], qp, 0, field, r);
+static void luma4x4_residual(const Levels *L, int b, int qp, int field, const uint8_t *w, int32_t r[16]) {
+    residual4x4_w(L->l4[b], qp, 0, field, w, r);
 }
 
-/* chroma residual of one component: rMb[8][8] raster (8.5.4) */
-static void chroma_residual(const Levels *L, int c, int qpc, int field, int32_t rmb[64]) {
+/* chroma residual of one component: rMb[8][8] raster (8.5.4); w: the MB's luma 4x4 list
+ * (quirk Q13: the chroma drivers never derive their own LevelScale4x4,
+ * H264PictureBase.cpp:3752-3929) */
+static void chroma_residual(const Levels *L, int c, int qpc, int field, const uint8_t *w, int32_t rmb[64]) {
     int32_t dcC[4];
-    refcpu_chroma_dc(L->cdc[c], qpc, dcC);
+    chroma_dc_w(L->cdc[c], qpc, w, dcC);
     for (int b = 0; b < 4; b++) {
         int32_t list[16], r[16];
         memcpy(list, L->cac[c][b], sizeof(list));
         list[0] = dcC[b];
-        refcpu_residual4x4(list, qpc, 1, field, r);
+        residual4x4_w(list, qpc, 1, field, w, r);
         int x0 = (b % 2) * 4, y0 = (b / 2) * 4;
         for (int i = 0; i < 4; i++)
             for (int j = 0; j < 4; j++) rmb[(y0 + i) * 8 + x0 + j] = r[i * 4 + j];
@@ -862,6 +876,9 @@ static void put_block(uint8_t *o, int pitch, int x, int y, const uint8_t *v, int
 /* Intra MB reconstruction in the reference's order (H264SliceData.cpp:400-453). */
 static int intra_mb(Ctx *X, int addr, const h264g_mb *mb, const Levels *L) {
     int qp = mb->qp_y, rc, field = mb_field(X, addr);
+    /* scaling matrices: the intra luma lists, chroma included (Q13) */
+    const uint8_t *w4 = X->d->scaling_matrix ? X->d->weight_scale4x4[0] : NULL;
+    const uint8_t *w8 = X->d->scaling_matrix ? X->d->weight_scale8x8[0] : NULL;
     int x0, y0, pitch;
     mb_plane_origin(X, addr, 16, X->wl, &x0, &y0, &pitch);
     uint8_t *o = X->y + y0 * X->wl + x0;
@@ -871,7 +888,7 @@ static int intra_mb(Ctx *X, int addr, const h264g_mb *mb, const Levels *L) {
             uint8_t pred[16];
             int bx, by;
             blk_xy(b, &bx, &by);
-            luma4x4_residual(L, b, qp, field, r);
+            luma4x4_residual(L, b, qp, field, w4, r);
             if ((rc = intra4x4_pred(X, addr, b, mb->intra_pred_modes[b], pred))) return rc;
             put_block(o, pitch, bx, by, pred, 4);
             add_residual(o, pitch, bx, by, r, 4);
@@ -880,7 +897,7 @@ static int intra_mb(Ctx *X, int addr, const h264g_mb *mb, const Levels *L) {
         for (int b = 0; b < 4; b++) {
             int32_t r[64];
             uint8_t pred[64];
-            refcpu_residual8x8(L->l8[b], qp, field, r);
+            residual8x8_w(L->l8[b], qp, field, w8, r);
             if ((rc = intra8x8_pred(X, addr, b, mb->intra_pred_modes[b], pred))) return rc;
             put_block(o, pitch, (b % 2) * 8, (b / 2) * 8, pred, 8);
             add_residual(o, pitch, (b % 2) * 8, (b / 2) * 8, r, 8);
@@ -888,14 +905,14 @@ static int intra_mb(Ctx *X, int addr, const h264g_mb *mb, const Levels *L) {
     } else {
         int32_t dcY[16], rmb[256];
         uint8_t pred[256];
-        luma_dc_scan(L->dc16, qp, field, dcY);
+        luma_dc_scan(L->dc16, qp, field, w4, dcY);
         for (int b = 0; b < 16; b++) {
             int32_t list[16], r[16];
             int bx, by;
             blk_xy(b, &bx, &by);
             memcpy(list, L->l4[b], sizeof(list));
             list[0] = dcY[(by / 4) * 4 + bx / 4];  /* dcY raster index of block b */
-            refcpu_residual4x4(list, qp, 1, field, r);
+            residual4x4_w(list, qp, 1, field, w4, r);
             for (int i = 0; i < 4; i++)
                 for (int j = 0; j < 4; j++) rmb[(by + i) * 16 + bx + j] = r[i * 4 + j];
         }
@@ -909,7 +926,7 @@ static int intra_mb(Ctx *X, int addr, const h264g_mb *mb, const Levels *L) {
         int off = c == 0 ? X->d->chroma_qp_index_offset : X->d->second_chroma_qp_index_offset;
         int32_t rmb[64];
         uint8_t pred[64];
-        chroma_residual(L, c, refcpu_chroma_qp(qp, off), field, rmb);
+        chroma_residual(L, c, refcpu_chroma_qp(qp, off), field, w4, rmb);
         if ((rc = intra_chroma_pred(X, plane, addr, mb->intra_chroma_pred_mode, pred))) return rc;
         put_block(oc, pitch, 0, 0, pred, 8);
         add_residual(oc, pitch, 0, 0, rmb, 8);
@@ -1202,6 +1219,7 @@ int refcpu_decode_picture(const h264g_pic_desc *d, const h264g_mb *mbs, int W, i
     X.wl = W * 16; X.hl = H * 16; X.wc = W * 8; X.hc = H * 8;
     X.planes = planes;
     X.mbaff = d->mbaff ? 1 : 0;
+    if (X.mbaff && d->scaling_matrix) return H264G_E_ARG;  /* field MBs would map the lists with the field scan */
     X.y = planes[d->dst_slot];
     X.cb = X.y + X.wl * X.hl;
     X.cr = X.cb + X.wc * X.hc;
@@ -1251,12 +1269,15 @@ int refcpu_decode_picture(const h264g_pic_desc *d, const h264g_mb *mbs, int W, i
         } else {
             rc = inter_mb(&X, a, mb, info);
             if (!rc && mb->mb_type != H264G_MB_P_SKIP) {
-                /* residual added to the prediction (H264InterPrediction.cpp:22-407) */
+                /* residual added to the prediction (H264InterPrediction.cpp:22-407); scaling
+                 * matrices: the inter luma lists, chroma included (Q13) */
                 int qp = mb->qp_y;
+                const uint8_t *w4 = d->scaling_matrix ? d->weight_scale4x4[3] : NULL;
+                const uint8_t *w8 = d->scaling_matrix ? d->weight_scale8x8[1] : NULL;
                 if (info->t8) {
                     for (int q = 0; q < 4; q++) {
                         int32_t r[64];
-                        refcpu_residual8x8(L->l8[q], qp, field, r);
+                        residual8x8_w(L->l8[q], qp, field, w8, r);
                         add_residual(oy, pitch, (q % 2) * 8, (q / 2) * 8, r, 8);
                     }
                 } else {
@@ -1264,14 +1285,14 @@ int refcpu_decode_picture(const h264g_pic_desc *d, const h264g_mb *mbs, int W, i
                         int32_t r[16];
                         int bx, by;
                         blk_xy(b, &bx, &by);
-                        luma4x4_residual(L, b, qp, field, r);
+                        luma4x4_residual(L, b, qp, field, w4, r);
                         add_residual(oy, pitch, bx, by, r, 4);
                     }
                 }
                 for (int c = 0; c < 2; c++) {
                     int32_t rmb[64];
                     int off = c == 0 ? d->chroma_qp_index_offset : d->second_chroma_qp_index_offset;
-                    chroma_residual(L, c, refcpu_chroma_qp(qp, off), field, rmb);
+                    chroma_residual(L, c, refcpu_chroma_qp(qp, off), field, w4, rmb);
                     add_residual(c == 0 ? ocb : ocr, cpitch, 0, 0, rmb, 8);
                 }
             }

@@ -15,6 +15,10 @@ the reference decoder's output for each.
       disable_deblocking_filter_idc rewritten from 2 to 0 (deblocking across the slice
       edges, which the reference filters with the current MB's data on both sides); the
       rest of each slice is moved by the two bits the shorter code saves
+  tests/golden/heavyhand_cavlc_sl_8.h264  the first 8 pictures of the CAVLC clip with scaling
+      matrices written into its SPS and PPS (with_scaling_matrices below: explicit lists,
+      lists left to fall-back rules A and B, lists sent as "use the default", and PPS
+      lists the reference takes from the SPS instead)
   tests/golden/<name>.json               per output picture of the reference decoder
       (oracle/_ref/ref_decode: every reference translation unit but main.cpp, compiled
       unmodified by `make -C oracle decode`), in its output order: PicOrderCnt, slice
@@ -33,7 +37,8 @@ DATA = "/root/reference/data"
 STREAMS = [("HeavyHand_1080p.B_frames_4.no_cabac.no_tff.h264", 15, "heavyhand_cavlc_15", None),
            ("HeavyHand_1080p.B_frames.cabac.h264", 10, "heavyhand_cabac_10", None),
            ("gop121.naluCnt453.h264", 12, "gop121_12", None),
-           ("gop121.naluCnt453.h264", 12, "gop121_idc0_12", 0)]
+           ("gop121.naluCnt453.h264", 12, "gop121_idc0_12", 0),
+           ("HeavyHand_1080p.B_frames_4.no_cabac.no_tff.h264", 8, "heavyhand_cavlc_sl_8", "scaling")]
 
 
 class _Bits:
@@ -85,6 +90,84 @@ def _escape(rbsp: bytes) -> bytes:
 def _ue_bits(v: int) -> str:
     b = bin(v + 1)[2:]
     return "0" * (len(b) - 1) + b
+
+
+def _se_bits(v: int) -> str:
+    return _ue_bits(2 * v - 1 if v > 0 else -2 * v)
+
+
+def _list_bits(values, stop_at=None, default=False) -> str:
+    """scaling_list() syntax for `values` (scan order): delta_scale per entry; `default`
+    sends the first delta that makes nextScale 0 (useDefaultScalingMatrixFlag), `stop_at`
+    ends the list there (the remaining entries repeat the last value)."""
+    if default:
+        return _se_bits(-8)
+    bits, last = "", 8
+    for j, v in enumerate(values):
+        if j == stop_at:
+            return bits + _se_bits(((0 - last + 128) % 256) - 128)
+        bits += _se_bits(((v - last + 128) % 256) - 128)
+        last = v
+    return bits
+
+
+# (present, bits) of the 8 scaling lists sent by the SPS and by the PPS
+SPS_LISTS = [(1, _list_bits([8 + 2 * j for j in range(16)])),        # 0 Intra Y: a ramp
+             (0, ""),                                                 # 1: rule A, copy of 0
+             (1, _list_bits(None, default=True)),                     # 2: Default_4x4_Intra
+             (0, ""),                                                 # 3: rule A, Default_4x4_Inter
+             (1, _list_bits([12, 20] + [0] * 14, stop_at=2)),         # 4: 12, 20, 20, ...
+             (0, ""),                                                 # 5: copy of 4
+             (1, _list_bits([16 + j % 7 for j in range(64)])),        # 6 Intra Y 8x8
+             (1, _list_bits(None, default=True))]                     # 7: Default_8x8_Inter
+PPS_LISTS = [(1, _list_bits([20] * 16)),                             # 0: taken from the SPS's list 0
+             (0, ""), (0, ""),                                        # 1, 2: copies (rule B)
+             (1, _list_bits(None, default=True)),                     # 3: Default_4x4_Inter
+             (1, _list_bits([30] * 16)),                              # 4: taken from the SPS's list 4
+             (0, ""),                                                 # 5: copy of 4
+             (0, ""),                                                 # 6: rule B, the SPS-level list
+             (1, _list_bits([40] * 64))]                              # 7: the SPS's parsed list 7 (all 8)
+
+
+def with_scaling_matrices(data: bytes) -> bytes:
+    """The stream with seq_scaling_matrix_present_flag and pic_scaling_matrix_present_flag
+    set and the lists above written after them (7.3.2.1.1, 7.3.2.2), for a High-profile
+    4:2:0 SPS and a PPS with transform_8x8_mode_flag; everything else is kept."""
+    starts, i = [], 0
+    while (j := data.find(b"\x00\x00\x01", i)) >= 0:
+        starts.append(j)
+        i = j + 3
+    out = bytearray(data[:starts[0]])
+    for k, st in enumerate(starts):
+        end = starts[k + 1] if k + 1 < len(starts) else len(data)
+        nal = data[st + 3:end]
+        while k + 1 < len(starts) and nal.endswith(b"\x00"):
+            nal = nal[:-1]
+        trail = data[st + 3 + len(nal):end]
+        hdr, typ = nal[0], nal[0] & 31
+        if typ in (7, 8):
+            b = _Bits(_unescape(nal[1:]))
+            if typ == 7:
+                assert b.u(8) == 100, "High profile"
+                b.u(16); b.ue()
+                assert b.ue() == 1, "4:2:0"
+                b.ue(); b.ue(); b.u(1)
+                lists = SPS_LISTS
+            else:
+                b.ue(); b.ue(); b.u(2)
+                assert b.ue() == 0
+                b.ue(); b.ue(); b.u(3); b.se(); b.se(); b.se(); b.u(3)
+                assert b.u(1) == 1, "transform_8x8_mode_flag"
+                lists = PPS_LISTS
+            at = b.pos
+            assert b.u(1) == 0, "no scaling matrix yet"
+            body = b.bits.rstrip("0")
+            pad = (len(b.bits) - len(body)) // 8
+            bits = b.bits[:at] + "1" + "".join(f"{p}{x}" for p, x in lists) + body[b.pos:-1] + "1"
+            bits += "0" * (-len(bits) % 8 + 8 * pad)
+            nal = bytes([hdr]) + _escape(bytes(int(bits[i:i + 8], 2) for i in range(0, len(bits), 8)))
+        out += data[st:st + 3] + nal + trail
+    return bytes(out)
 
 
 def with_deblocking_idc(data: bytes, idc: int) -> bytes:
@@ -193,7 +276,9 @@ def main():
     for src, n_pictures, name, idc in STREAMS:
         with open(os.path.join(DATA, src), "rb") as f:
             data = prefix(f.read(), n_pictures)
-        if idc is not None:
+        if idc == "scaling":
+            data = with_scaling_matrices(data)
+        elif idc is not None:
             data = with_deblocking_idc(data, idc)
         path = os.path.join(HERE, name + ".h264")
         with open(path, "wb") as f:

@@ -18,8 +18,9 @@ from oracle.refcpu import Dpb
 HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 CODINGS = ["heavyhand_cavlc_15", "heavyhand_cabac_10"]  # CAVLC and CABAC codings of one clip
 # + Baseline, 9 slices per picture with disable_deblocking_filter_idc 2, a damaged slice;
-# the same with every slice's idc rewritten to 0 (deblocking across slice edges, quirk Q7)
-STREAMS = CODINGS + ["gop121_12", "gop121_idc0_12"]
+# the same with every slice's idc rewritten to 0 (deblocking across slice edges, quirk Q7);
+# the CAVLC clip with scaling matrices written into its SPS and PPS (quirks Q13, Q23)
+STREAMS = CODINGS + ["gop121_12", "gop121_idc0_12", "heavyhand_cavlc_sl_8"]
 
 
 def _stream(name):
@@ -154,6 +155,27 @@ def test_damaged_slice_ends_where_the_reference_abandons_it():
     for a, b in zip(pics, Bitstream(_stream("gop121_idc0_12")).pictures()):
         assert (a.desc.disable_deblocking_filter_idc, b.desc.disable_deblocking_filter_idc) == (2, 0)
         assert np.array_equal(a.mbs, b.mbs) and np.array_equal(a.levels, b.levels)
+
+
+def test_scaling_lists_follow_the_reference_fall_back():
+    """The scaling matrices written into the CAVLC clip (make_real.py): the picture's lists
+    in raster order as the reference leaves them.  Intra Y is the SPS's ramp (the PPS's
+    explicit list 0 is replaced by the SPS's list 0, quirk Q23), Inter Y the default the PPS
+    asks for, list 4 the SPS's 12, 20, 20, ... (the PPS sent 30s), the 8x8 intra list the
+    SPS's (rule B), the 8x8 inter list the SPS's parsed "use default" list, all 8 (Q23)."""
+    d = next(iter(Bitstream(_stream("heavyhand_cavlc_sl_8")).pictures())).desc
+    assert d.scaling_matrix == 1
+    w4 = np.array([list(d.weight_scale4x4[i]) for i in range(6)])
+    w8 = np.array([list(d.weight_scale8x8[i]) for i in range(2)])
+    zz = [0, 1, 4, 8, 5, 2, 3, 6, 9, 12, 13, 10, 7, 11, 14, 15]  # Table 8-13, zig-zag
+    ramp = np.zeros(16, int)
+    ramp[zz] = [8 + 2 * j for j in range(16)]
+    assert np.array_equal(w4[0], ramp) and np.array_equal(w4[1], ramp) and np.array_equal(w4[2], ramp)
+    inter_default = np.zeros(16, int)
+    inter_default[zz] = [10, 14, 14, 20, 20, 20, 24, 24, 24, 24, 27, 27, 27, 30, 30, 34]  # Table 7-3
+    assert np.array_equal(w4[3], inter_default)
+    assert w4[4][0] == 12 and w4[4][1] == 20 and set(w4[4][2:]) == {20} and np.array_equal(w4[5], w4[4])
+    assert sorted(set(w8[0])) == list(range(16, 23)) and set(w8[1]) == {8}
 
 
 @pytest.mark.parametrize("name", STREAMS)

@@ -337,6 +337,27 @@ def test_multi_slice_pictures_match_oracle(any_engine, slices, idc):
     _decode_all(any_engine, gen, pics)
 
 
+def test_scaling_matrices_match_oracle(any_engine):
+    """Pictures dequantised with scaling matrices (random weights 1..255 per picture and
+    list): 4x4 and 8x8 luma blocks of intra and inter MBs, Intra16x16 DC, chroma DC and AC
+    (which take the MB's luma list, quirk Q13), against the oracle."""
+    gen = Generator(2, width_mbs=19, height_mbs=7, n_frames=5, seed=0x5CA1E000, intra_in_inter_permille=300)
+    rng = np.random.default_rng(0x5CA1E)
+    pics = []
+    for i in range(gen.num_pictures):
+        desc, mbs, levels = gen.picture(i)
+        desc.scaling_matrix = 1
+        w4, w8 = rng.integers(1, 256, (6, 16)), rng.integers(1, 256, (2, 64))
+        for l in range(6):
+            for k in range(16):
+                desc.weight_scale4x4[l][k] = int(w4[l, k])
+        for l in range(2):
+            for k in range(64):
+                desc.weight_scale8x8[l][k] = int(w8[l, k])
+        pics.append((desc, mbs, levels))
+    _decode_all(any_engine, gen, pics)
+
+
 def _lose_tail(desc, mbs, levels, first):
     """The picture with MBs first.. replaced by what a damaged slice leaves: zero I_PCM
     records of no slice."""
