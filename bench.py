@@ -147,7 +147,7 @@ def serving_check(specs, gens, args):
     return {"streams": len(gens), "equal_dpb": equal, "frames": sum(g.num_pictures for g in gens)}
 
 
-BYTE_STREAMS = ("heavyhand_cavlc_15", "heavyhand_cabac_10", "gop121_12", "gop121_idc0_12")
+BYTE_STREAMS = ("heavyhand_cavlc_15", "heavyhand_cabac_10", "gop121_12", "gop121_idc0_12", "heavyhand_cavlc_sl_8")
 
 
 def byte_streams(reps):
@@ -258,7 +258,8 @@ def summarize_pmc(fetch_csv: str, write_csv: str, copy_bytes: int, out_json: str
                     continue
                 name = row["Kernel_Name"]
                 for k in KERNELS + ("k_copy", "k_bs", "k_deblock_band", "k_intra_band", "k_deblock_rows"):
-                    if f"::{k}(" in name or f"::{k}<false>(" in name:  # frame-picture instance
+                    # the frame-picture instance (k_inter<false, false>: no MBAFF, flat lists)
+                    if f"::{k}(" in name or f"::{k}<false>(" in name or f"::{k}<false, false>(" in name:
                         acc[k].append(float(row["Counter_Value"]))
         return {k: sum(v) / len(v) for k, v in acc.items()}
 

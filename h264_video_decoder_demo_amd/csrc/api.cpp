@@ -148,6 +148,7 @@ struct Batch {
     uint32_t *d_minter = nullptr, *d_mintra = nullptr, *d_mpairs = nullptr;
     uint32_t n_minter = 0, n_mintra = 0, n_mpairs = 0;
     bool dbk_off = false;            // no picture of the batch is deblocked
+    bool scaled = false;             // a picture of the batch has scaling matrices
     hipGraphExec_t graph = nullptr;  // the captured stage sequence (kernel nodes only)
     hipGraphExec_t graph_dbk = nullptr;  // its deblocking, when that runs on the lane's deblocking stream
     std::vector<SlotUse> uses;       // DPB slots the pictures write and read
@@ -436,6 +437,7 @@ int stage_picture(h264g_ctx *ctx, Stream &st, const h264g_pic_desc &d, const h26
     // (H264InterPrediction.cpp:3182-3186, H264PictureBase.cpp:707-708).  MBAFF frames take
     // one slice.
     uint32_t dbk_stop = n_mbs, first_sn = H264G_SLICE_NONE;
+    bool multi_slice = false;
     for (uint32_t a = 0; a < n_mbs; a++) {
         const uint32_t sn = mbs[a].slice_num;
         if (sn == H264G_SLICE_NONE) {
@@ -445,6 +447,7 @@ int stage_picture(h264g_ctx *ctx, Stream &st, const h264g_pic_desc &d, const h26
         }
         if (first_sn == H264G_SLICE_NONE) first_sn = sn;
         if (sn != first_sn && mbaff) return h264g_set_error(H264G_E_ARG, "mb %u: several slices in an MBAFF frame", a);
+        multi_slice |= sn != first_sn;
     }
     // Scaling matrices: LevelScale4x4 / 8x8 (8-315 / 8-318) of the two luma lists the
     // reference dequantises with (chroma takes the MB's luma list, quirk Q13).
@@ -780,6 +783,7 @@ int stage_picture(h264g_ctx *ctx, Stream &st, const h264g_pic_desc &d, const h26
     use_set(st, 0, P);
     P.dbk_idc = d.disable_deblocking_filter_idc; P.cip = d.constrained_intra_pred_flag;
     P.dbk_stop = dbk_stop;
+    P.multi_slice = multi_slice ? 1 : 0;
     P.ls = lsv.empty() ? nullptr : (const uint16_t *)(base + o_ls);
     HIPCHK(hipMemcpy(pic->d_mbs, mbs, n_mbs * sizeof(h264g_mb), hipMemcpyHostToDevice));
     if (!lsv.empty()) HIPCHK(hipMemcpy(base + o_ls, lsv.data(), lsv.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
@@ -801,6 +805,7 @@ struct StageSet {
     const uint32_t *inter, *intra, *all, *rows, *ibands, *dbands;
     uint32_t n_inter, n_intra, n_all, n_rows, n_ibands, n_dbands;
     bool dbk_off;
+    bool scaled = false;  // a picture has scaling matrices: the inter stage takes the kernel that reads them
     // MBAFF frames: inter MBs, intra MBs for the per-MB wavefront, MB pairs for their
     // deblocking
     const uint32_t *minter = nullptr, *mintra = nullptr, *mpairs = nullptr;
@@ -817,10 +822,10 @@ int launch_stages(h264g_ctx *ctx, const StageSet &q, int lane, int parts, hipStr
     int e = 0;
     if (parts & kFront) {
         if (ev) HIPCHK(hipEventRecord(ev[0], s));
-        e = h264g_launch_inter(q.pics, q.inter, q.n_inter, q.intra, q.n_intra, 0, ctx->d_err, s);
+        e = h264g_launch_inter(q.pics, q.inter, q.n_inter, q.intra, q.n_intra, 0, q.scaled ? 1 : 0, ctx->d_err, s);
         if (e) return h264g_set_error(H264G_E_DEVICE, "inter launch: %s", hipGetErrorString((hipError_t)e));
         // MBAFF frames: inter MBs and the residuals of their intra MBs (lists of their own)
-        e = h264g_launch_inter(q.pics, q.minter, q.n_minter, q.mintra, q.n_mintra, 1, ctx->d_err, s);
+        e = h264g_launch_inter(q.pics, q.minter, q.n_minter, q.mintra, q.n_mintra, 1, 0, ctx->d_err, s);
         if (e) return h264g_set_error(H264G_E_DEVICE, "inter launch (MBAFF): %s", hipGetErrorString((hipError_t)e));
         if (ev) HIPCHK(hipEventRecord(ev[1], s));
         if (q.n_intra)
@@ -1218,6 +1223,7 @@ int enqueue_picture(h264g_ctx *c, const h264g_pic *p, bool events = true) {
     q.n_ibands = (uint32_t)((p->H + c->intra_waves - 1) / c->intra_waves);
     q.n_dbands = (uint32_t)((p->H + c->dbk_waves - 1) / c->dbk_waves);
     q.dbk_off = p->desc.disable_deblocking_filter_idc == 1;
+    q.scaled = p->hpic.ls != nullptr;
     if (p->mbaff) {  // items: inter MBs, intra MBs (pair t order), MB pairs (t order)
         q.minter = q.inter;
         q.n_minter = q.n_inter;
@@ -1404,6 +1410,7 @@ int build_sub(h264g_ctx *c, const std::vector<const h264g_pic *> &pics, Batch *b
     b->n_mintra = (uint32_t)mintra.size();
     b->n_mpairs = (uint32_t)mpairs.size();
     b->dbk_off = !any_dbk && mpairs.empty();
+    for (uint32_t i = 0; i < n; i++) b->scaled |= hp[i].ls != nullptr;
     if (hipMemcpy(b->d_pics, hp.data(), n * sizeof(DevPic), hipMemcpyHostToDevice) != hipSuccess ||
         (!inter.empty() && hipMemcpy(b->d_inter, inter.data(), inter.size() * 4, hipMemcpyHostToDevice) != hipSuccess) ||
         (!intra.empty() && hipMemcpy(b->d_intra, intra.data(), intra.size() * 4, hipMemcpyHostToDevice) != hipSuccess) ||
@@ -1480,6 +1487,7 @@ StageSet batch_stages(const Batch *b) {
     q.dbands = b->d_dbands;
     q.n_dbands = b->n_dbands;
     q.dbk_off = b->dbk_off;
+    q.scaled = b->scaled;
     q.minter = b->d_minter;
     q.n_minter = b->n_minter;
     q.mintra = b->d_mintra;

@@ -80,6 +80,8 @@ struct DevPic {
     uint32_t *pairprog;       // MBAFF streams: per-MB-pair deblocking progress words
     uint32_t dbk_stop;        // MBs from this address on filter none of their edges (the
                               // first MB of no slice; W * H when every MB has one)
+    int32_t  multi_slice;     // the MBs carry more than one slice number (MBs of no slice
+                              // aside): k_bs reads them
     const uint16_t *ls;       // scaling matrices: LevelScale4x4 [intra, inter luma list][qP % 6]
                               // [raster 16] at 0, LevelScale8x8 [list][qP % 6][raster 64] at
                               // LS8_AT; null with flat lists
@@ -106,8 +108,9 @@ struct DevCtl {
 extern "C" {
 // Kernel launchers (kernels.hip); all enqueue on `stream` and return a hipError_t.
 // mbaff: every picture of the launch is an MBAFF frame (else none is).
+// scaled: some picture of the launch has scaling matrices (DevPic::ls).
 int h264g_launch_inter(const DevPic *pics, const uint32_t *items, uint32_t n, const uint32_t *intra_items, uint32_t n_intra,
-                       int mbaff, uint32_t *err, void *stream);
+                       int mbaff, int scaled, uint32_t *err, void *stream);
 // `epoch` points at the device word shared by all wavefront launches of a context
 // (initially 1; each launch's last wave advances it).
 int h264g_launch_intra(const DevPic *pics, const uint32_t *items, uint32_t n, DevCtl *ctl, uint32_t *epoch,

@@ -204,17 +204,19 @@ __device__ __forceinline__ int32_t level_at(uint32_t e, const int16_t *vals, int
 // returns column r of the residual in out[0..3] (rows 0..3).  dc_pass: sample (0, 0) is
 // `dc` as it stands (Intra16x16 and chroma DC).  field: the levels follow the field scan
 // (field MBs, H264PictureBase.cpp:3419).
+template <bool SCALED = false>
 __device__ __forceinline__ void residual4x4_quad(uint32_t se, const int16_t *vals, int qp, bool dc_pass, int dc, int r,
                                                  bool field, int32_t (&out)[4], const GAS uint16_t *lsm = nullptr) {
     const int m = qp % 6, e = qp / 6;
-    const int s0 = 16 * c_v4[m][0], s1 = 16 * c_v4[m][1], s2 = 16 * c_v4[m][2];
-    // LevelScale4x4 of row r: with flat lists s0 at (even, even), s1 at (odd, odd), s2
-    // elsewhere; with scaling matrices lsm holds LevelScale4x4(qP % 6, i, j), raster
+    // LevelScale4x4 of row r: with flat lists 16 v0 at (even, even), 16 v1 at (odd, odd),
+    // 16 v2 elsewhere; with scaling matrices (SCALED) lsm holds LevelScale4x4(qP % 6, i, j),
+    // raster
     int ls[4];
-    if (lsm) {
+    if (SCALED) {
         const uint2 t = *(const GAS uint2 *)(lsm + 4 * r);
         ls[0] = (int)(t.x & 0xFFFFu); ls[1] = (int)(t.x >> 16); ls[2] = (int)(t.y & 0xFFFFu); ls[3] = (int)(t.y >> 16);
     } else {
+        const int s0 = 16 * c_v4[m][0], s1 = 16 * c_v4[m][1], s2 = 16 * c_v4[m][2];
         ls[0] = ls[2] = (r & 1) ? s2 : s0;
         ls[1] = ls[3] = (r & 1) ? s1 : s2;
     }
@@ -252,7 +254,7 @@ __device__ __forceinline__ void idct8(int32_t *v, int s) {  // 8.5.13.2, in plac
 // Places the MB's levels in LDS and computes all residual blocks into ws->rl / ws->rc.  Must be called by the whole wave.  Returns false when the MB has no levels.
 // Field MBs of MBAFF frames take the field scans.  lstab: the picture's LevelScale tables
 // when it has scaling matrices (DevPic::ls), else null (flat lists).
-template <bool MBAFF>
+template <bool MBAFF, bool SCALED>
 __device__ __forceinline__ bool build_residual(const h264g_mb *mb, const DevAux &ax, const int16_t *levels, WaveRes *ws,
                                                const GAS uint16_t *lstab) {
     const int lane = wave_lane();
@@ -264,8 +266,8 @@ __device__ __forceinline__ bool build_residual(const h264g_mb *mb, const DevAux 
     const int qp = mb->qp_y;
     // the MB's luma lists (intra or inter), which serve its chroma blocks too (quirk Q13)
     const bool inter = !(ax.kind & AUX_INTRA);
-    const GAS uint16_t *ls4t = lstab ? lstab + (inter ? LS4_LIST : 0) : nullptr;
-    const GAS uint16_t *ls8t = lstab ? lstab + LS8_AT + (inter ? LS8_LIST : 0) : nullptr;
+    const GAS uint16_t *ls4t = SCALED ? lstab + (inter ? LS4_LIST : 0) : nullptr;
+    const GAS uint16_t *ls8t = SCALED ? lstab + LS8_AT + (inter ? LS8_LIST : 0) : nullptr;
     const int16_t *lv = levels + mb->level_offset;
     // <= 27 masks of 16 bits, so at most 432 values; the host counted them, so the masks
     // and the values load together (coalesced, all in flight at once)
@@ -319,7 +321,7 @@ __device__ __forceinline__ bool build_residual(const h264g_mb *mb, const DevAux 
             int lsr[8];  // LevelScale8x8 of row i: the classes above, or the matrix row
 #pragma unroll
             for (int j = 0; j < 8; j++) lsr[j] = (j & 1) ? ls_1 : (j & 2) ? ls_2 : ls_0;
-            if (ls8t) {
+            if (SCALED) {
                 const uint4 t = *(const GAS uint4 *)(ls8t + 64 * m + 8 * i);
                 const uint32_t w[4] = {t.x, t.y, t.z, t.w};
 #pragma unroll
@@ -360,11 +362,11 @@ __device__ __forceinline__ bool build_residual(const h264g_mb *mb, const DevAux 
                 }
                 f += hneg(row, i) ? -acc : acc;
             }
-            const int sc = ls4t ? ls4t[16 * (qp % 6)] : ls4(qp % 6, 0, 0);
+            const int sc = SCALED ? ls4t[16 * (qp % 6)] : ls4(qp % 6, 0, 0);
             dc = qp >= 36 ? (f * sc) << (qp / 6 - 6) : (f * sc + (1 << (5 - qp / 6))) >> (6 - qp / 6);
         }
         int32_t o[4];
-        residual4x4_quad(ws->sm[1 + b], ws->vals, qp, i16, dc, r, field, o, ls4t ? ls4t + 16 * (qp % 6) : nullptr);
+        residual4x4_quad<SCALED>(ws->sm[1 + b], ws->vals, qp, i16, dc, r, field, o, ls4t + 16 * (qp % 6));
         const int x0 = blk_x(b), y0 = blk_y(b);
 #pragma unroll
         for (int i = 0; i < 4; i++) ws->rl[(y0 + i) * 16 + x0 + r] = (int16_t)o[i];
@@ -379,9 +381,9 @@ __device__ __forceinline__ bool build_residual(const h264g_mb *mb, const DevAux 
         const int32_t *dcv = ws->cdc[c];
         const int32_t e00 = dcv[0] + dcv[2], e01 = dcv[1] + dcv[3], e10 = dcv[0] - dcv[2], e11 = dcv[1] - dcv[3];
         const int32_t f = b == 0 ? e00 + e01 : b == 1 ? e00 - e01 : b == 2 ? e10 + e11 : e10 - e11;
-        const int dc = ((f * (ls4t ? ls4t[16 * (qpc % 6)] : ls4(qpc % 6, 0, 0))) << (qpc / 6)) >> 5;
+        const int dc = ((f * (SCALED ? ls4t[16 * (qpc % 6)] : ls4(qpc % 6, 0, 0))) << (qpc / 6)) >> 5;
         int32_t o[4];
-        residual4x4_quad(ws->sm[19 + 4 * c + b], ws->vals, qpc, true, dc, r, field, o, ls4t ? ls4t + 16 * (qpc % 6) : nullptr);
+        residual4x4_quad<SCALED>(ws->sm[19 + 4 * c + b], ws->vals, qpc, true, dc, r, field, o, ls4t + 16 * (qpc % 6));
         const int x0 = (b & 1) * 4, y0 = (b >> 1) * 4;
 #pragma unroll
         for (int i = 0; i < 4; i++) ws->rc[c][(y0 + i) * 8 + x0 + r] = (int16_t)o[i];
@@ -651,7 +653,9 @@ __device__ __forceinline__ int wp_apply(const WpTerms &t, uint32_t pair) {
 }
 
 // Inter MB (or I_PCM copy): prediction + residual, written to the destination slot.
-template <bool MBAFF>
+// SCALED: the launch holds pictures with scaling matrices (their residual code reads the
+// LevelScale tables; the other pictures' the flat one).
+template <bool MBAFF, bool SCALED>
 __device__ void inter_mb(const DevPic &P, int addr, WaveRes *ws) {
     const int lane = wave_lane();
     const h264g_mb *mb = &G(P.mbs)[addr];
@@ -668,7 +672,8 @@ __device__ void inter_mb(const DevPic &P, int addr, WaveRes *ws) {
         dcr[(g.cy + (lane >> 3) * g.step) * wc + mbx * 8 + (lane & 7)] = (uint8_t)pcm[320 + lane];
         return;
     }
-    const bool res = build_residual<MBAFF>(mb, ax, G(P.levels), ws, G(P.ls));
+    const bool res = SCALED && P.ls ? build_residual<MBAFF, SCALED>(mb, ax, G(P.levels), ws, G(P.ls))
+                                    : build_residual<MBAFF, false>(mb, ax, G(P.levels), ws, nullptr);
     // ---- luma: lane -> 4x4 block b = lane & 15 (raster), row r = lane >> 4
     {
         const int b = lane & 15, r = lane >> 4, bx = b & 3, by = b >> 2, q = (by >> 1) * 2 + (bx >> 1);
@@ -1049,10 +1054,12 @@ __device__ __forceinline__ void intra8_prep(const h264g_mb *mb, const Avail &av,
 // (intra_res_place).
 static_assert(offsetof(WaveRes, rc) == offsetof(WaveRes, rl) + 512 && offsetof(WaveRes, rl) % 16 == 0,
               "luma and chroma residual planes are one 768-byte run");
-template <bool MBAFF>
+template <bool MBAFF, bool SCALED>
 __device__ __forceinline__ void intra_res_build(const DevPic &P, int addr, WaveRes *ws) {
     const DevAux &ax = G(P.aux)[addr];
-    if (!build_residual<MBAFF>(&G(P.mbs)[addr], ax, G(P.levels), ws, G(P.ls))) return;
+    if (!(SCALED && P.ls ? build_residual<MBAFF, SCALED>(&G(P.mbs)[addr], ax, G(P.levels), ws, G(P.ls))
+                         : build_residual<MBAFF, false>(&G(P.mbs)[addr], ax, G(P.levels), ws, nullptr)))
+        return;
     const int lane = wave_lane();
     if (lane < 48) ((uint4 *)(G(P.ires) + (size_t)(ax.ires & AUX_IRES_MASK) * 384))[lane] = ((const uint4 *)ws->rl)[lane];
 }
@@ -1723,7 +1730,7 @@ __device__ __forceinline__ uint32_t xcd_item(uint32_t b, uint32_t n) {
 // ni build the residuals of the intra MBs of `iitems`.
 // MBAFF: the instance for MBAFF frames (field MBs: geometry, field scans, field references);
 // frame pictures run the instance without any of it.
-template <bool MBAFF>
+template <bool MBAFF, bool SCALED>
 __global__ __launch_bounds__(WAVE) void k_inter(const DevPic *__restrict__ pics, const uint32_t *__restrict__ items, uint32_t n,
                                                 const uint32_t *__restrict__ iitems, uint32_t ni, uint32_t *err) {
     __shared__ WaveRes lds;
@@ -1732,8 +1739,8 @@ __global__ __launch_bounds__(WAVE) void k_inter(const DevPic *__restrict__ pics,
     const uint32_t i = xcd_item(blockIdx.x, total);
     const uint32_t it = i < n ? items[i] : iitems[i - n];
     const DevPic &P = pics[it >> WI_PIC_SHIFT];
-    if (i < n) inter_mb<MBAFF>(P, (int)(it & WI_MB_MASK), &lds);
-    else intra_res_build<MBAFF>(P, (int)(it & WI_MB_MASK), &lds);
+    if (i < n) inter_mb<MBAFF, SCALED>(P, (int)(it & WI_MB_MASK), &lds);
+    else intra_res_build<MBAFF, SCALED>(P, (int)(it & WI_MB_MASK), &lds);
     (void)err;
 }
 
@@ -1751,7 +1758,7 @@ __device__ __forceinline__ Avail intra_avail(const DevPic &P, int addr, int deps
         deps[k] = -1;
         if (!(av.m & (1u << k))) continue;
         const int c = G(P.mbs)[nb[k]].mb_type;
-        if (G(P.mbs)[nb[k]].slice_num != G(P.mbs)[addr].slice_num) {  // another slice (6.4.8)
+        if (P.multi_slice && G(P.mbs)[nb[k]].slice_num != G(P.mbs)[addr].slice_num) {  // another slice (6.4.8)
             av.m &= ~(1u << k);
             continue;
         }
@@ -2499,9 +2506,9 @@ __global__ __launch_bounds__(WAVE) void k_bs(const DevPic *__restrict__ pics, co
     const int addr = (int)(it & WI_MB_MASK), W = P.W, mbx = addr % W, mby = addr / W;
     const int j = lane & 31, dir = j >> 4, e = ((j >> 2) & 3) * 4, k = (j & 3) * 4;
     int bS = 0;
-    const uint32_t sq = G(P.mbs)[addr].slice_num;
     const bool has_nb = dir == 0 ? mbx > 0 : mby > 0;
-    const bool other = has_nb && G(P.mbs)[dir == 0 ? addr - 1 : addr - W].slice_num != sq;
+    const uint32_t sq = P.multi_slice ? G(P.mbs)[addr].slice_num : 0u;
+    const bool other = P.multi_slice && e == 0 && has_nb && G(P.mbs)[dir == 0 ? addr - 1 : addr - W].slice_num != sq;
     // An MB edge at the picture border is not filtered, nor, with disable_deblocking_filter_idc
     // 2, one between two slices (8.7: filterLeftMbEdgeFlag / filterTopMbEdgeFlag); no edge
     // of an MB at or after the deblocking stop is.  With idc 0 the reference filters an
@@ -2523,7 +2530,7 @@ __global__ __launch_bounds__(WAVE) void k_bs(const DevPic *__restrict__ pics, co
     G(P.bs)[(size_t)addr * 32 + (j & 16) + ((j & 3) << 2) + ((j >> 2) & 3)] = (uint8_t)bS;  // [dir][segment][edge]
     if (j == 0) {
         const DevAux &ax = G(P.aux)[addr];
-        const bool own_t = P.dbk_idc == 0 && mby > 0 && G(P.mbs)[addr - W].slice_num != sq;
+        const bool own_t = P.multi_slice && P.dbk_idc == 0 && mby > 0 && G(P.mbs)[addr - W].slice_num != sq;
         G(P.dbq)[addr] = (uint32_t)ax.qdb[0] | ((uint32_t)ax.qdb[1] << 8) | ((uint32_t)ax.qdb[2] << 16) |
                          ((ax.kind & AUX_T8) ? 1u << 24 : 0u) | (own ? DBQ_OWN_LEFT : 0u) | (own_t ? DBQ_OWN_TOP : 0u);
     }
@@ -2563,13 +2570,16 @@ __global__ void k_copy(const uint4 *src, uint4 *dst, uint64_t n16) {
 extern "C" {
 
 int h264g_launch_inter(const DevPic *pics, const uint32_t *items, uint32_t n, const uint32_t *intra_items, uint32_t n_intra,
-                       int mbaff, uint32_t *err, void *stream) {
+                       int mbaff, int scaled, uint32_t *err, void *stream) {
     if (n + n_intra == 0) return 0;
-    if (mbaff)
-        hipLaunchKernelGGL(k_inter<true>, dim3(n + n_intra), dim3(WAVE), 0, (hipStream_t)stream, pics, items, n, intra_items,
-                           n_intra, err);
+    if (mbaff)  // (scaling matrices are refused in MBAFF frames)
+        hipLaunchKernelGGL((k_inter<true, false>), dim3(n + n_intra), dim3(WAVE), 0, (hipStream_t)stream, pics, items, n,
+                           intra_items, n_intra, err);
+    else if (scaled)
+        hipLaunchKernelGGL((k_inter<false, true>), dim3(n + n_intra), dim3(WAVE), 0, (hipStream_t)stream, pics, items, n,
+                           intra_items, n_intra, err);
     else
-        hipLaunchKernelGGL(k_inter<false>, dim3(n + n_intra), dim3(WAVE), 0, (hipStream_t)stream, pics, items, n,
+        hipLaunchKernelGGL((k_inter<false, false>), dim3(n + n_intra), dim3(WAVE), 0, (hipStream_t)stream, pics, items, n,
                            intra_items, n_intra, err);
     return (int)hipGetLastError();
 }
