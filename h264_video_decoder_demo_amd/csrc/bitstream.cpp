@@ -19,7 +19,7 @@
 // picture's lists as the reference derives them.  Refused with H264G_E_ARG (none occur
 // in the reference's progressive sample streams): field pictures and MBAFF, slice groups,
 // arbitrary slice order, chroma formats other than 4:2:0, bit depths above 8, lossless
-// coding, SP / SI slices, temporal direct prediction, POC type 1, gaps in frame_num.
+// coding, SP / SI slices, temporal direct prediction, gaps in frame_num.
 #include <algorithm>
 #include <cstdarg>
 #include <cstdint>
@@ -299,6 +299,9 @@ struct Sps {
     bool valid = false;
     int profile_idc = 0, constraint3 = 0, level_idc = 0;
     int chroma_format_idc = 1, log2_max_frame_num = 4, poc_type = 0, log2_max_poc_lsb = 4;
+    // pic_order_cnt_type 1 (7.4.2.1.1)
+    int delta_poc_always_zero = 0, offset_non_ref = 0, offset_top_to_bottom = 0, n_cycle = 0;
+    int offset_ref[256] = {}, expected_delta_cycle = 0;
     int max_num_ref_frames = 0, gaps = 0, width_mbs = 0, height_map_units = 0, frame_mbs_only = 1;
     int direct_8x8_inference = 0;
     int max_num_reorder_frames = -1;  // VUI bitstream_restriction, -1 when absent
@@ -380,8 +383,22 @@ int parse_sps(Bits &bs, Sps *table) {  // 7.3.2.1.1
     if (s.chroma_format_idc != 1) return h264g_set_error(H264G_E_ARG, "only 4:2:0 is supported");
     s.log2_max_frame_num = (int)bs.ue() + 4;
     s.poc_type = (int)bs.ue();
-    if (s.poc_type == 0) s.log2_max_poc_lsb = (int)bs.ue() + 4;
-    else if (s.poc_type == 1) return h264g_set_error(H264G_E_ARG, "pic_order_cnt_type 1 is not supported");
+    if (s.poc_type == 0) {
+        s.log2_max_poc_lsb = (int)bs.ue() + 4;
+    } else if (s.poc_type == 1) {
+        s.delta_poc_always_zero = (int)bs.u(1);
+        s.offset_non_ref = bs.se();
+        s.offset_top_to_bottom = bs.se();
+        const uint32_t n = bs.ue();
+        if (n > 255) return h264g_set_error(H264G_E_ARG, "num_ref_frames_in_pic_order_cnt_cycle %u", n);
+        s.n_cycle = (int)n;
+        for (int i = 0; i < s.n_cycle; i++) {
+            s.offset_ref[i] = bs.se();
+            s.expected_delta_cycle += s.offset_ref[i];  // ExpectedDeltaPerPicOrderCntCycle (7-12)
+        }
+    } else if (s.poc_type != 2) {
+        return h264g_set_error(H264G_E_ARG, "pic_order_cnt_type %d", s.poc_type);
+    }
     s.max_num_ref_frames = (int)bs.ue();
     s.gaps = (int)bs.u(1);
     s.width_mbs = (int)bs.ue() + 1;
@@ -486,6 +503,7 @@ struct Mmco {
 struct Slice {
     int nal_ref_idc = 0, idr = 0;
     int first_mb = 0, slice_type = 0, pps_id = 0, frame_num = 0, idr_pic_id = 0, poc_lsb = 0, delta_poc_bottom = 0;
+    int delta_poc[2] = {0, 0};  // delta_pic_order_cnt[0..1] (pic_order_cnt_type 1)
     int redundant_pic_cnt = 0, direct_spatial = 1, num_ref_idx[2] = {0, 0};
     std::vector<std::pair<int, int>> mods[2];  // (modification_of_pic_nums_idc, value)
     int luma_log2_wd = 0, chroma_log2_wd = 0, luma_flag_last[2] = {0, 0};
@@ -538,6 +556,10 @@ int parse_slice_header(Bits &bs, Slice &s, int nal_type, int nal_ref_idc, const 
     *pps_out = &pps;
     s.frame_num = (int)bs.u(sps.log2_max_frame_num);
     if (s.idr) s.idr_pic_id = (int)bs.ue();
+    if (sps.poc_type == 1 && !sps.delta_poc_always_zero) {
+        s.delta_poc[0] = bs.se();
+        if (pps.bottom_field_pic_order) s.delta_poc[1] = bs.se();
+    }
     if (sps.poc_type == 0) {
         s.poc_lsb = (int)bs.u(sps.log2_max_poc_lsb);
         if (pps.bottom_field_pic_order) s.delta_poc_bottom = bs.se();
@@ -640,6 +662,8 @@ struct h264g_parser {
     int max_long_term_idx = -1;  // "no long-term frame indices"
     // POC type 0 / 2 state (8.2.1)
     int prev_poc_msb = 0, prev_poc_lsb = 0, prev_frame_num_offset = 0, prev_frame_num = 0;
+    int prev_raw_frame_num = 0;  // frame_num of the previous picture as its slices sent it
+    bool last_pic_mmco5 = false; // the previous picture (in decoding order) had MMCO 5
     bool prev_mmco5 = false;
     int prev_ref_top_poc = 0;
     // the picture handed out by the last h264g_parser_next
@@ -1811,6 +1835,23 @@ int decode_poc(h264g_parser *p, const Sps &sps, const Slice &sl) {
         }
         return std::min(top, bottom);
     }
+    if (sps.poc_type == 1) {  // 8.2.1.2, as the reference derives it (H264RefPicList.cpp:118-215)
+        // prevFrameNumOffset and the frame_num of the previous picture in decoding order
+        const int prev_off = p->last_pic_mmco5 ? 0 : p->prev_frame_num_offset;
+        const int offset = sl.idr ? 0 : (p->prev_raw_frame_num > sl.frame_num ? prev_off + max_frame_num(sps) : prev_off);
+        int abs_fn = sps.n_cycle ? offset + sl.frame_num : 0;
+        if (!sl.nal_ref_idc && abs_fn > 0) abs_fn--;
+        int expected = 0;
+        if (abs_fn > 0) {
+            const int cycle = (abs_fn - 1) / sps.n_cycle, in_cycle = (abs_fn - 1) % sps.n_cycle;
+            expected = cycle * sps.expected_delta_cycle;
+            for (int i = 0; i <= in_cycle; i++) expected += sps.offset_ref[i];
+        }
+        if (!sl.nal_ref_idc) expected += sps.offset_non_ref;
+        const int top = expected + sl.delta_poc[0], bottom = top + sps.offset_top_to_bottom + sl.delta_poc[1];
+        p->prev_frame_num_offset = offset;
+        return std::min(top, bottom);
+    }
     // type 2
     int offset;
     if (sl.idr) offset = 0;
@@ -2321,6 +2362,8 @@ extern "C" int32_t h264g_parser_next(h264g_parser *p, h264g_parsed_picture *out)
     } else {
         p->prev_frame_num = sl.frame_num;
     }
+    p->prev_raw_frame_num = sl.frame_num;
+    p->last_pic_mmco5 = sl.nal_ref_idc && p->prev_mmco5;
     out->mbs = p->mbs.data();
     out->n_mbs = (uint32_t)p->mbs.size();
     out->levels = p->levels.data();

@@ -499,7 +499,7 @@ int32_t h264g_mvfield_colocated(h264g_mvfield *field, const h264g_mb *col_mbs, i
 /* ------------------------------------------------------------ bitstream front end
  * Annex B byte stream of frame pictures coded with CAVLC or CABAC, in one slice or several
  * per picture: NAL units, SPS / PPS, slice headers, slice data and the decoded picture
- * buffer (POC types 0 and 2, sliding-window and adaptive reference marking, reference
+ * buffer (POC types 0, 1 and 2, sliding-window and adaptive reference marking, reference
  * list initialisation and modification, DPB slots).  Replaces the reference's parsing and
  * picture management on the way to reconstruction: CH264NalUnit / CH264SPS / CH264PPS /
  * CH264SliceHeader (H264NalUnit.cpp:122, H264SliceHeader.cpp:289-509),
@@ -513,7 +513,7 @@ int32_t h264g_mvfield_colocated(h264g_mvfield *field, const h264g_mb *col_mbs, i
  * SPS / PPS scaling matrices give the descriptor's lists (h264g_pic_desc.scaling_matrix).
  * Refused with H264G_E_ARG: field pictures and MBAFF, slice groups, data partitioning,
  * formats other than 8-bit 4:2:0, lossless coding, SP / SI slices, temporal direct
- * prediction, POC type 1, gaps in frame_num.  Host only. */
+ * prediction, gaps in frame_num.  Host only. */
 typedef struct h264g_parser h264g_parser;
 typedef struct h264g_parsed_picture {
     h264g_pic_desc desc;          /* ready for h264g_picture_begin (dst_slot, ref_slot ...) */

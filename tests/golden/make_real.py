@@ -15,6 +15,10 @@ the reference decoder's output for each.
       disable_deblocking_filter_idc rewritten from 2 to 0 (deblocking across the slice
       edges, which the reference filters with the current MB's data on both sides); the
       rest of each slice is moved by the two bits the shorter code saves
+  tests/golden/gop121_poc1_12.h264       gop121's 12 pictures with pic_order_cnt_type 1
+      (with_poc_type1 below: SPS cycle of two reference-frame offsets, an offset for
+      non-reference pictures, delta_pic_order_cnt[0] in every slice header in place of
+      pic_order_cnt_lsb)
   tests/golden/heavyhand_cavlc_sl_8.h264  the first 8 pictures of the CAVLC clip with scaling
       matrices written into its SPS and PPS (with_scaling_matrices below: explicit lists,
       lists left to fall-back rules A and B, lists sent as "use the default", and PPS
@@ -38,7 +42,8 @@ STREAMS = [("HeavyHand_1080p.B_frames_4.no_cabac.no_tff.h264", 15, "heavyhand_ca
            ("HeavyHand_1080p.B_frames.cabac.h264", 10, "heavyhand_cabac_10", None),
            ("gop121.naluCnt453.h264", 12, "gop121_12", None),
            ("gop121.naluCnt453.h264", 12, "gop121_idc0_12", 0),
-           ("HeavyHand_1080p.B_frames_4.no_cabac.no_tff.h264", 8, "heavyhand_cavlc_sl_8", "scaling")]
+           ("HeavyHand_1080p.B_frames_4.no_cabac.no_tff.h264", 8, "heavyhand_cavlc_sl_8", "scaling"),
+           ("gop121.naluCnt453.h264", 12, "gop121_poc1_12", "poc1")]
 
 
 class _Bits:
@@ -170,6 +175,67 @@ def with_scaling_matrices(data: bytes) -> bytes:
     return bytes(out)
 
 
+# pic_order_cnt_type 1 fields written into the SPS: delta_pic_order_always_zero_flag 0,
+# offset_for_non_ref_pic -1, offset_for_top_to_bottom_field 0, a cycle of two
+# offset_for_ref_frame (3, 1); the k-th picture's slices send delta_pic_order_cnt[0] k % 2
+POC1_SPS = "0" + _se_bits(-1) + _se_bits(0) + _ue_bits(2) + _se_bits(3) + _se_bits(1)
+
+
+def _nals(data: bytes):
+    """(start code offset, NAL bytes, bytes after the NAL up to the next start code)."""
+    starts, i = [], 0
+    while (j := data.find(b"\x00\x00\x01", i)) >= 0:
+        starts.append(j)
+        i = j + 3
+    for k, st in enumerate(starts):
+        end = starts[k + 1] if k + 1 < len(starts) else len(data)
+        nal = data[st + 3:end]
+        while k + 1 < len(starts) and nal.endswith(b"\x00"):
+            nal = nal[:-1]
+        yield st, nal, data[st + 3 + len(nal):end]
+
+
+def _repack(hdr: int, bits: str, pad: int) -> bytes:
+    """A NAL from its header byte and its RBSP bits up to the stop bit (exclusive)."""
+    bits += "1"
+    bits += "0" * (-len(bits) % 8 + 8 * pad)
+    return bytes([hdr]) + _escape(bytes(int(bits[i:i + 8], 2) for i in range(0, len(bits), 8)))
+
+
+def with_poc_type1(data: bytes) -> bytes:
+    """gop121 (Baseline, POC type 0, CAVLC I / P slices) with pic_order_cnt_type 1: the
+    SPS's log2_max_pic_order_cnt_lsb_minus4 becomes the type-1 fields above, and each
+    slice's pic_order_cnt_lsb becomes delta_pic_order_cnt[0] (7.3.2.1.1, 7.3.3)."""
+    out, sps, pps, pic = bytearray(data[:data.find(b"\x00\x00\x01")]), {}, {}, -1
+    for st, nal, trail in _nals(data):
+        hdr, typ = nal[0], nal[0] & 31
+        b = _Bits(_unescape(nal[1:]))
+        body = b.bits.rstrip("0")
+        pad = (len(b.bits) - len(body)) // 8
+        if typ == 7:
+            assert b.u(8) == 66, "Baseline"
+            b.u(16); b.ue()
+            sps["log2_fn"] = b.ue() + 4
+            at = b.pos
+            assert b.ue() == 0, "POC type 0"
+            sps["log2_poc"] = b.ue() + 4
+            nal = _repack(hdr, body[:at] + _ue_bits(1) + POC1_SPS + body[b.pos:-1], pad)
+        elif typ == 8:
+            b.ue(); b.ue(); b.u(1)
+            assert b.u(1) == 0, "no bottom_field_pic_order_in_frame_present_flag"
+        elif typ in (1, 5):
+            if b.bits[b.pos] == "1":  # first_mb_in_slice 0: a new picture
+                pic += 1
+            b.ue(); b.ue(); b.ue(); b.u(sps["log2_fn"])
+            if typ == 5:
+                b.ue()
+            at = b.pos
+            b.u(sps["log2_poc"])
+            nal = _repack(hdr, body[:at] + _se_bits(pic % 2) + body[b.pos:-1], pad)
+        out += data[st:st + 3] + nal + trail
+    return bytes(out)
+
+
 def with_deblocking_idc(data: bytes, idc: int) -> bytes:
     """The stream with disable_deblocking_filter_idc of every slice set to idc, for the
     syntax gop121 uses (CAVLC I / P slices, POC type 0, no weighted prediction): SPS and
@@ -278,6 +344,8 @@ def main():
             data = prefix(f.read(), n_pictures)
         if idc == "scaling":
             data = with_scaling_matrices(data)
+        elif idc == "poc1":
+            data = with_poc_type1(data)
         elif idc is not None:
             data = with_deblocking_idc(data, idc)
         path = os.path.join(HERE, name + ".h264")
