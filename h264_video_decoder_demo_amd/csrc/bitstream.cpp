@@ -19,7 +19,7 @@
 // picture's lists as the reference derives them.  Refused with H264G_E_ARG (none occur
 // in the reference's progressive sample streams): field pictures and MBAFF, slice groups,
 // arbitrary slice order, chroma formats other than 4:2:0, bit depths above 8, lossless
-// coding, SP / SI slices, temporal direct prediction, gaps in frame_num.
+// coding, SP / SI slices, temporal direct prediction.
 #include <algorithm>
 #include <cstdarg>
 #include <cstdint>
@@ -2264,13 +2264,13 @@ extern "C" int32_t h264g_parser_next(h264g_parser *p, h264g_parsed_picture *out)
                 if (h264g_mvfield_create(W, H, &p->field)) return h264g_set_error(H264G_E_NOMEM, "motion field");
             }
             p->cur_sps = sps;
-            const int max_fn = max_frame_num(*sps);
             if (sl.idr) {
                 p->dpb.clear();
                 p->prev_frame_num = 0;
-            } else if (sl.frame_num != p->prev_frame_num && sl.frame_num != (p->prev_frame_num + 1) % max_fn) {
-                return h264g_set_error(H264G_E_ARG, "gaps in frame_num are not supported");
             }
+            // A gap in frame_num goes without the "non-existing" frames of 8.2.5.2: the
+            // reference's Decoding_process_for_gaps_in_frame_num does nothing
+            // (H264RefPicList.cpp:1598-1601), so the pictures that remain are the references.
             poc = decode_poc(p, *sps, sl);
             if (int rc = slice_lists(p, *sps, sl, poc, lists)) return rc;
             fill_desc(p, *pps, sl, poc, lists, out->desc);

@@ -513,7 +513,8 @@ int32_t h264g_mvfield_colocated(h264g_mvfield *field, const h264g_mb *col_mbs, i
  * SPS / PPS scaling matrices give the descriptor's lists (h264g_pic_desc.scaling_matrix).
  * Refused with H264G_E_ARG: field pictures and MBAFF, slice groups, data partitioning,
  * formats other than 8-bit 4:2:0, lossless coding, SP / SI slices, temporal direct
- * prediction, gaps in frame_num.  Host only. */
+ * prediction.  A gap in frame_num inserts no frames (the reference's gap process is
+ * empty).  Host only. */
 typedef struct h264g_parser h264g_parser;
 typedef struct h264g_parsed_picture {
     h264g_pic_desc desc;          /* ready for h264g_picture_begin (dst_slot, ref_slot ...) */

@@ -19,6 +19,9 @@ the reference decoder's output for each.
       (with_poc_type1 below: SPS cycle of two reference-frame offsets, an offset for
       non-reference pictures, delta_pic_order_cnt[0] in every slice header in place of
       pic_order_cnt_lsb)
+  tests/golden/gop121_gap_12.h264        gop121's 12 pictures with frame_num raised by 2 from
+      the 6th picture on (with_frame_num_gap below): a gap in frame_num, which the
+      reference passes over without inserting frames
   tests/golden/heavyhand_cavlc_sl_8.h264  the first 8 pictures of the CAVLC clip with scaling
       matrices written into its SPS and PPS (with_scaling_matrices below: explicit lists,
       lists left to fall-back rules A and B, lists sent as "use the default", and PPS
@@ -43,7 +46,8 @@ STREAMS = [("HeavyHand_1080p.B_frames_4.no_cabac.no_tff.h264", 15, "heavyhand_ca
            ("gop121.naluCnt453.h264", 12, "gop121_12", None),
            ("gop121.naluCnt453.h264", 12, "gop121_idc0_12", 0),
            ("HeavyHand_1080p.B_frames_4.no_cabac.no_tff.h264", 8, "heavyhand_cavlc_sl_8", "scaling"),
-           ("gop121.naluCnt453.h264", 12, "gop121_poc1_12", "poc1")]
+           ("gop121.naluCnt453.h264", 12, "gop121_poc1_12", "poc1"),
+           ("gop121.naluCnt453.h264", 12, "gop121_gap_12", "gap")]
 
 
 class _Bits:
@@ -236,6 +240,30 @@ def with_poc_type1(data: bytes) -> bytes:
     return bytes(out)
 
 
+def with_frame_num_gap(data: bytes, first: int, by: int) -> bytes:
+    """gop121 with frame_num + `by` (modulo MaxFrameNum) in the slices of its pictures from
+    index `first` on: the field keeps its width, so only those bits change (7.3.3)."""
+    out, sps, pic = bytearray(data[:data.find(b"\x00\x00\x01")]), {}, -1
+    for st, nal, trail in _nals(data):
+        hdr, typ = nal[0], nal[0] & 31
+        b = _Bits(_unescape(nal[1:]))
+        if typ == 7:
+            b.u(24); b.ue()
+            sps["log2_fn"] = b.ue() + 4
+        elif typ in (1, 5):
+            if b.bits[b.pos] == "1":
+                pic += 1
+            if pic >= first:
+                b.ue(); b.ue(); b.ue()
+                at, n = b.pos, sps["log2_fn"]
+                fn = (b.u(n) + by) % (1 << n)
+                body = b.bits.rstrip("0")
+                pad = (len(b.bits) - len(body)) // 8
+                nal = _repack(hdr, body[:at] + format(fn, f"0{n}b") + body[at + n:-1], pad)
+        out += data[st:st + 3] + nal + trail
+    return bytes(out)
+
+
 def with_deblocking_idc(data: bytes, idc: int) -> bytes:
     """The stream with disable_deblocking_filter_idc of every slice set to idc, for the
     syntax gop121 uses (CAVLC I / P slices, POC type 0, no weighted prediction): SPS and
@@ -346,6 +374,8 @@ def main():
             data = with_scaling_matrices(data)
         elif idc == "poc1":
             data = with_poc_type1(data)
+        elif idc == "gap":
+            data = with_frame_num_gap(data, 5, 2)
         elif idc is not None:
             data = with_deblocking_idc(data, idc)
         path = os.path.join(HERE, name + ".h264")

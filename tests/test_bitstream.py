@@ -20,8 +20,9 @@ CODINGS = ["heavyhand_cavlc_15", "heavyhand_cabac_10"]  # CAVLC and CABAC coding
 # + Baseline, 9 slices per picture with disable_deblocking_filter_idc 2, a damaged slice;
 # the same with every slice's idc rewritten to 0 (deblocking across slice edges, quirk Q7);
 # the CAVLC clip with scaling matrices written into its SPS and PPS (quirks Q13, Q23);
-# gop121 with pic_order_cnt_type 1 (its POCs repeat, which exercises the output order's ties)
-STREAMS = CODINGS + ["gop121_12", "gop121_idc0_12", "heavyhand_cavlc_sl_8", "gop121_poc1_12"]
+# gop121 with pic_order_cnt_type 1 (its POCs repeat, which exercises the output order's ties);
+# gop121 with a gap in frame_num (no frames inserted, as in the reference)
+STREAMS = CODINGS + ["gop121_12", "gop121_idc0_12", "heavyhand_cavlc_sl_8", "gop121_poc1_12", "gop121_gap_12"]
 
 
 def _stream(name):
