@@ -831,13 +831,10 @@ void emit_levels(const Residual &r, bool i16, bool t8, int cbp, std::vector<int1
     int16_t vals[27 * 16];
     int nm = 0, nv = 0;
     auto block = [&](const int16_t *c, int n, int first_pos) {  // n coefficients at scan first_pos..
-        uint16_t m = 0;
-        for (int i = 0; i < n; i++)
-            if (c[i]) {
-                m |= (uint16_t)(1u << (first_pos + i));
-                vals[nv++] = c[i];
-            }
-        masks[nm++] = m;
+        uint32_t m = 0;
+        for (int i = 0; i < n; i++) m |= (uint32_t)(c[i] != 0) << i;  // branch-free, vectorisable
+        for (uint32_t t = m; t; t &= t - 1) vals[nv++] = c[__builtin_ctz(t)];
+        masks[nm++] = (uint16_t)(m << first_pos);
     };
     if (i16) block(r.dc, 16, 0);
     for (int q = 0; q < 4; q++) {
