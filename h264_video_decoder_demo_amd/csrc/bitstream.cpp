@@ -24,9 +24,13 @@
 #include <cstdarg>
 #include <cstdint>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <deque>
+#include <memory>
 #include <new>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "h264g.h"
@@ -641,7 +645,41 @@ struct Residual {
     int16_t cac[2][4][15];
 };
 
-struct Parser;
+// What deriving a parsed MB's motion vectors and intra modes reads of its syntax, kept
+// when that derivation runs after the picture's slice data has parsed.
+struct FinMb {
+    uint8_t has_mvd;
+    int16_t mvd[2][16][2];
+    uint8_t isyn[16][2];
+};
+
+// One picture's slices as the stream sends them (header parsed, RBSP kept), the parameter
+// sets in force for it, and its records.  Its slice data may be parsed ahead, on a thread
+// of its own, into records whose motion vectors and modes wait for the in-order pass.
+struct PicJob {
+    struct SliceNal {
+        Slice sl;
+        std::vector<uint8_t> rbsp;
+        size_t data_bit = 0;  // first bit of slice_data()
+        bool over = false;    // the header read past the end of the RBSP
+        int mb_end = 0;       // MBs covered once parsed ahead
+        int damaged_mb = -1;  // the MB where its slice data failed, parsed ahead
+        std::string damage;   // and why
+    };
+    std::vector<SliceNal> slices;
+    Sps sps;
+    Pps pps;
+    int32_t rc = 0;           // an error met gathering the picture: returned in its place
+    std::string err;
+    std::vector<h264g_mb> mbs;
+    std::vector<int16_t> levels;
+    std::vector<FinMb> fin;
+    bool ahead_ok = false;    // the slice data parsed ahead without an error
+    std::thread worker;
+    ~PicJob() {
+        if (worker.joinable()) worker.join();
+    }
+};
 
 }  // namespace
 
@@ -650,7 +688,6 @@ struct h264g_parser {
     size_t pos = 0;             // next unread byte
     Sps sps[32];
     Pps pps[256];
-    const Sps *cur_sps = nullptr;
     int width_mbs = 0, height_mbs = 0, dpb_slots = 0;
     // DPB: frames kept for reference
     struct Frame {
@@ -666,11 +703,16 @@ struct h264g_parser {
     bool last_pic_mmco5 = false; // the previous picture (in decoding order) had MMCO 5
     bool prev_mmco5 = false;
     int prev_ref_top_poc = 0;
-    // the picture handed out by the last h264g_parser_next
-    std::vector<h264g_mb> mbs;
-    std::vector<int16_t> levels;
     h264g_mvfield *field = nullptr;
-    int pictures = 0;
+    int pictures = 0;              // pictures handed out
+    // pictures gathered ahead of the one handed out (at most `threads`), the slice data of
+    // each parsing on its own thread, and the picture handed out by the last
+    // h264g_parser_next (its records stay valid until the next call)
+    int threads = 1;
+    int gathered = 0;
+    bool gather_end = false;       // the stream (or an error) ended gathering
+    std::deque<std::unique_ptr<PicJob>> ahead;
+    std::unique_ptr<PicJob> out_job;
     char damage[512] = {0};     // why the last picture's damaged slice stopped ("" if none)
     ~h264g_parser() {
         if (field) h264g_mvfield_destroy(field);
@@ -780,12 +822,15 @@ inline void sub_origin(int shape, int k, int &x, int &y) {
 }
 
 struct PicState {
-    h264g_parser *p;
+    h264g_mvfield *field;          // motion field the MBs' vectors and modes go into, or
+    std::vector<FinMb> *fin;       // (if non-null) where their syntax waits for it
+    int picture;                   // the picture's index in the stream (messages)
     const Sps *sps;
     const Pps *pps;
-    Slice *sl;
+    const Slice *sl;
     int W, H;
     std::vector<MbNz> nz;          // this slice's MBs (the others: not available)
+    h264g_mb *mbs;                 // the picture's records
     std::vector<int16_t> *levels;
     uint32_t slice_num;            // the slice's index in its picture
     size_t mark = 0;               // levels->size() when the current MB began
@@ -1091,42 +1136,57 @@ void skip_record(h264g_mb &m, int slice_type, int qp, uint32_t level_offset) {
 }
 
 // Intra prediction modes and motion vectors of a parsed MB (the motion field, in MB order).
-int finish_mb(PicState &st, int mb, h264g_mb &m, const int16_t *mvd, const uint8_t *isyn) {
-    h264g_parser *p = st.p;
-    m.slice_num = st.slice_num;
+int derive_mb(h264g_mvfield *field, int cip, int mb, h264g_mb &m, const int16_t *mvd, const uint8_t *isyn) {
     if (m.mb_type == H264G_MB_I4x4 || m.mb_type == H264G_MB_I8x8) {
-        if (int rc = h264g_mvfield_intra_modes(p->field, (uint32_t)mb, &m, isyn, st.pps->constrained_intra_pred)) return rc;
+        if (int rc = h264g_mvfield_intra_modes(field, (uint32_t)mb, &m, isyn, cip)) return rc;
     }
     const bool all_direct = (m.flags & H264G_MBF_DIRECT_ALL) == H264G_MBF_DIRECT_ALL;
-    return h264g_mvfield_derive(p->field, (uint32_t)mb, &m, all_direct ? nullptr : mvd);
+    return h264g_mvfield_derive(field, (uint32_t)mb, &m, all_direct ? nullptr : mvd);
+}
+
+// A parsed MB joins its slice; its modes and vectors are derived now, or its syntax is
+// kept for the pass that derives them in MB order.
+int finish_mb(PicState &st, int mb, h264g_mb &m, const int16_t *mvd, const uint8_t *isyn) {
+    m.slice_num = st.slice_num;
+    if (st.fin) {
+        FinMb &f = (*st.fin)[mb];
+        f.has_mvd = mvd != nullptr;
+        if (mvd) memcpy(f.mvd, mvd, sizeof(f.mvd));
+        if (isyn && (m.mb_type == H264G_MB_I4x4 || m.mb_type == H264G_MB_I8x8)) memcpy(f.isyn, isyn, sizeof(f.isyn));
+        return 0;
+    }
+    return derive_mb(st.field, st.pps->constrained_intra_pred, mb, m, mvd, isyn);
 }
 
 // An MB left undecoded (H264G_MBF_LOST): zero samples as I_PCM with QPY 0, the reset
 // state the reference leaves such MBs in (H264PictureBase.cpp:45-67).  An MB where slice
 // data failed stays in its slice; MBs no slice reached belong to no slice
 // (H264G_SLICE_NONE).
-int undecoded_mb(h264g_parser *p, int mb, uint32_t slice_num) {
-    h264g_mb &m = p->mbs[mb];
+void undecoded_record(h264g_mb *mbs, std::vector<int16_t> &levels, int mb, uint32_t slice_num) {
+    h264g_mb &m = mbs[mb];
     memset(&m, 0, sizeof(m));
     for (int l = 0; l < 2; l++)
         for (int q = 0; q < 4; q++) m.ref_idx[l][q] = -1;
     m.mb_type = H264G_MB_IPCM;
     m.flags = H264G_MBF_LOST;
     m.slice_num = slice_num;
-    m.level_offset = (uint32_t)p->levels.size();
-    p->levels.resize(p->levels.size() + 384, 0);
-    return h264g_mvfield_derive(p->field, (uint32_t)mb, &m, nullptr);
+    m.level_offset = (uint32_t)levels.size();
+    levels.resize(levels.size() + 384, 0);
+}
+
+int undecoded_mb(h264g_mb *mbs, std::vector<int16_t> &levels, h264g_mvfield *field, int mb, uint32_t slice_num) {
+    undecoded_record(mbs, levels, mb, slice_num);
+    return h264g_mvfield_derive(field, (uint32_t)mb, &mbs[mb], nullptr);
 }
 
 int mb_error(const PicState &st, int mb, int rc) {
     char msg[512];
     snprintf(msg, sizeof(msg), "%s", h264g_last_error());
-    return h264g_set_error(rc, "picture %d MB %d: %s", st.p->pictures, mb, msg);
+    return h264g_set_error(rc, "picture %d MB %d: %s", st.picture, mb, msg);
 }
 
 // slice_data() (7.3.4) with CAVLC: mb_skip_run, then macroblock_layer; mb = MBs covered.
 int parse_slice_data_cavlc(Bits &bs, PicState &st, int &mb) {
-    h264g_parser *p = st.p;
     const Slice &sl = *st.sl;
     const int n_mbs = st.W * st.H;
     int qp = sl.slice_qp;
@@ -1134,13 +1194,13 @@ int parse_slice_data_cavlc(Bits &bs, PicState &st, int &mb) {
     int16_t mvd[2][16][2];
     uint8_t isyn[16][2];
     while (more && mb < n_mbs) {
-        st.mark = p->levels.size();
+        st.mark = st.levels->size();
         if (sl.slice_type != H264G_SLICE_I) {
             const uint32_t run = bs.ue();
             if (run > (uint32_t)(n_mbs - mb)) return h264g_set_error(H264G_E_ARG, "mb_skip_run past the picture");
             for (uint32_t k = 0; k < run; k++, mb++) {
-                h264g_mb &m = p->mbs[mb];
-                skip_record(m, sl.slice_type, qp, (uint32_t)p->levels.size());
+                h264g_mb &m = st.mbs[mb];
+                skip_record(m, sl.slice_type, qp, (uint32_t)st.levels->size());
                 st.nz[mb].avail = 1;
                 st.nz[mb].skip = 1;
                 if (int rc = finish_mb(st, mb, m, nullptr, nullptr)) return mb_error(st, mb, rc);
@@ -1148,7 +1208,7 @@ int parse_slice_data_cavlc(Bits &bs, PicState &st, int &mb) {
             if (run > 0) more = bs.more_rbsp_data();
             if (!more || mb >= n_mbs) break;
         }
-        h264g_mb &m = p->mbs[mb];
+        h264g_mb &m = st.mbs[mb];
         bool has_mvd = false;
         if (int rc = parse_mb(bs, st, mb, qp, m, mvd, isyn, has_mvd)) {
             const int err = mb_error(st, mb, rc);
@@ -1750,7 +1810,6 @@ int cabac_mb(CabacPic &cp, PicState &st, int mb, int &qp, bool &prev_qp_nz, h264
 // slice_data() (7.3.4) with CABAC: cabac_alignment_one_bit, then mb_skip_flag /
 // macroblock_layer / end_of_slice_flag per MB
 int parse_slice_data_cabac(Bits &bs, PicState &st, int &mb) {
-    h264g_parser *p = st.p;
     const Slice &sl = *st.sl;
     const int n_mbs = st.W * st.H;
     while (!bs.aligned()) {
@@ -1768,10 +1827,10 @@ int parse_slice_data_cabac(Bits &bs, PicState &st, int &mb) {
     int16_t mvd[2][16][2];
     uint8_t isyn[16][2];
     while (mb < n_mbs) {
-        st.mark = p->levels.size();
-        h264g_mb &m = p->mbs[mb];
+        st.mark = st.levels->size();
+        h264g_mb &m = st.mbs[mb];
         if (sl.slice_type != H264G_SLICE_I && cabac_mb_skip(cp, mb, sl.slice_type)) {
-            skip_record(m, sl.slice_type, qp, (uint32_t)p->levels.size());
+            skip_record(m, sl.slice_type, qp, (uint32_t)st.levels->size());
             CMb &cm = cp.mbs[mb];
             memset(&cm, 0, sizeof(cm));
             memset(cm.ref, -1, sizeof(cm.ref));
@@ -1791,7 +1850,7 @@ int parse_slice_data_cabac(Bits &bs, PicState &st, int &mb) {
         }
         mb++;
         if (cp.c.terminate()) break;  // end_of_slice_flag
-        if (bs.over) return h264g_set_error(H264G_E_ARG, "picture %d: slice data truncated", p->pictures);
+        if (bs.over) return h264g_set_error(H264G_E_ARG, "picture %d: slice data truncated", st.picture);
     }
     return 0;
 }
@@ -2035,6 +2094,12 @@ extern "C" int32_t h264g_parser_create(const uint8_t *data, uint64_t len, h264g_
     h264g_parser *p = new (std::nothrow) h264g_parser();
     if (!p) return h264g_set_error(H264G_E_NOMEM, "parser");
     p->data.assign(data, data + len);
+    // pictures whose slice data parses ahead, each on a thread of its own: the environment's
+    // H264G_PARSE_THREADS (1: every picture parses in order on the calling thread), else
+    // the host's hardware threads, at most 8
+    const char *env = getenv("H264G_PARSE_THREADS");
+    const int n = env && *env ? atoi(env) : (int)std::thread::hardware_concurrency();
+    p->threads = std::max(1, std::min(n, env && *env ? 64 : 8));
     *out = p;
     return H264G_OK;
 }
@@ -2129,10 +2194,10 @@ void scaling_desc(const Sps &sps, const Pps &pps, h264g_pic_desc &d) {
         for (int k = 0; k < 64; k++) d.weight_scale8x8[i][kZz8.at[k]] = l8[i][k];
 }
 
-void fill_desc(const h264g_parser *p, const Pps &pps, const Slice &sl, int poc, const std::vector<int> lists[2],
-               h264g_pic_desc &d) {
+void fill_desc(const h264g_parser *p, const Sps &sps, const Pps &pps, const Slice &sl, int poc,
+               const std::vector<int> lists[2], h264g_pic_desc &d) {
     memset(&d, 0, sizeof(d));
-    scaling_desc(*p->cur_sps, pps, d);
+    scaling_desc(sps, pps, d);
     d.slice_type = sl.slice_type;
     d.structure = H264G_FRAME;
     d.constrained_intra_pred_flag = pps.constrained_intra_pred;
@@ -2194,29 +2259,34 @@ const char *slice_mismatch(const Pps &pps, const Slice &a, const Slice &b, const
 
 extern "C" const char *h264g_parser_damage(const h264g_parser *p) { return p ? p->damage : ""; }
 
-extern "C" int32_t h264g_parser_next(h264g_parser *p, h264g_parsed_picture *out) {
-    if (!p || !out) return h264g_set_error(H264G_E_ARG, "NULL argument");
-    memset(out, 0, sizeof(*out));
+namespace {
+
+// Scans the next picture's NAL units: parameter sets update the tables, slice headers are
+// parsed and each slice's RBSP is kept, up to the first NAL unit of the next picture.
+// Null at the end of the stream; an error is kept in the picture's job, to be returned
+// once the pictures before it have been handed out, and ends gathering.
+std::unique_ptr<PicJob> gather(h264g_parser *p) {
+    std::unique_ptr<PicJob> job(new PicJob());
+    auto fail = [&](int32_t rc) {
+        job->rc = rc;
+        job->err = h264g_last_error();
+        p->gather_end = true;
+        return std::move(job);
+    };
     std::vector<uint8_t> rbsp;
     size_t begin, end;
-    // the picture being assembled from its slices
-    bool started = false;
-    Slice first;
-    const Sps *sps = nullptr;
     const Pps *pps = nullptr;
-    std::vector<int> lists[2];
-    int poc = 0, covered = 0, n_slices = 0, W = 0, H = 0, lost = 0, first_lost = -1;
-    p->damage[0] = 0;
     for (;;) {
         const size_t nal_pos = p->pos;
         if (!next_nal(p, begin, end)) break;
         if (end <= begin) continue;
         const uint8_t hdr = p->data[begin];
         const int nal_type = hdr & 31, nal_ref_idc = (hdr >> 5) & 3;
+        const bool started = !job->slices.empty();
         if (nal_type != 1 && nal_type != 5) {
-            if (nal_type >= 2 && nal_type <= 4) return h264g_set_error(H264G_E_ARG, "data partitioning is not supported");
+            if (nal_type >= 2 && nal_type <= 4) return fail(h264g_set_error(H264G_E_ARG, "data partitioning is not supported"));
             // parameter sets, SEI, access unit delimiters and sequence ends come before a
-            // picture's first slice: one of them ends the picture being assembled (7.4.1.2.3)
+            // picture's first slice: one of them ends the picture being gathered (7.4.1.2.3)
             if (started && nal_type >= 6 && nal_type <= 11) {
                 p->pos = nal_pos;
                 break;
@@ -2226,132 +2296,251 @@ extern "C" int32_t h264g_parser_next(h264g_parser *p, h264g_parsed_picture *out)
                 Bits bs;
                 bs.d = rbsp.data();
                 bs.nbits = rbsp.size() * 8;
-                if (int rc = nal_type == 7 ? parse_sps(bs, p->sps) : parse_pps(bs, p->pps, p->sps)) return rc;
+                if (int rc = nal_type == 7 ? parse_sps(bs, p->sps) : parse_pps(bs, p->pps, p->sps)) return fail(rc);
             }
             continue;  // SEI, AUD, end of sequence / stream, filler ...
         }
-        unescape(&p->data[begin + 1], end - begin - 1, rbsp);
+        PicJob::SliceNal s;
+        unescape(&p->data[begin + 1], end - begin - 1, s.rbsp);
         Bits bs;
-        bs.d = rbsp.data();
-        bs.nbits = rbsp.size() * 8;
-        Slice sl;
+        bs.d = s.rbsp.data();
+        bs.nbits = s.rbsp.size() * 8;
         const Sps *ssps = nullptr;
         const Pps *spps = nullptr;
-        if (int rc = parse_slice_header(bs, sl, nal_type, nal_ref_idc, p->sps, p->pps, &ssps, &spps)) return rc;
-        if (sl.redundant_pic_cnt) continue;
-        if (started && sl.first_mb == 0) {  // the next picture's first slice
+        if (int rc = parse_slice_header(bs, s.sl, nal_type, nal_ref_idc, p->sps, p->pps, &ssps, &spps)) return fail(rc);
+        if (s.sl.redundant_pic_cnt) continue;
+        if (started && s.sl.first_mb == 0) {  // the next picture's first slice
             p->pos = nal_pos;
             break;
         }
-        if (sl.slice_type == H264G_SLICE_B && !sl.direct_spatial)
-            return h264g_set_error(H264G_E_ARG, "temporal direct prediction is not supported");
+        if (s.sl.slice_type == H264G_SLICE_B && !s.sl.direct_spatial)
+            return fail(h264g_set_error(H264G_E_ARG, "temporal direct prediction is not supported"));
         if (!started) {
-            // ---- a new picture
-            if (sl.first_mb != 0) return h264g_set_error(H264G_E_ARG, "a picture starts with a slice at MB %d", sl.first_mb);
-            sps = ssps;
+            if (s.sl.first_mb != 0)
+                return fail(h264g_set_error(H264G_E_ARG, "a picture starts with a slice at MB %d", s.sl.first_mb));
+            job->sps = *ssps;
+            job->pps = *spps;
             pps = spps;
-            W = sps->width_mbs;
-            H = sps->height_map_units;
-            if (p->cur_sps && (p->width_mbs != W || p->height_mbs != H))
-                return h264g_set_error(H264G_E_ARG, "the picture size changes inside the stream");
-            if (!p->cur_sps) {
-                p->width_mbs = W;
-                p->height_mbs = H;
-                p->dpb_slots = std::max(sps->max_num_ref_frames, 1) + 1;
-                if (h264g_mvfield_create(W, H, &p->field)) return h264g_set_error(H264G_E_NOMEM, "motion field");
+        } else if (spps != pps) {
+            return fail(h264g_set_error(H264G_E_ARG, "picture %d: slices refer to different PPSs", p->gathered));
+        }
+        s.data_bit = bs.pos;
+        s.over = bs.over;
+        job->slices.push_back(std::move(s));
+    }
+    if (job->slices.empty()) {
+        p->gather_end = true;
+        return nullptr;
+    }
+    p->gathered++;
+    return job;
+}
+
+Bits slice_bits(const PicJob::SliceNal &s) {
+    Bits bs;
+    bs.d = s.rbsp.data();
+    bs.nbits = s.rbsp.size() * 8;
+    bs.pos = s.data_bit;
+    bs.over = s.over;
+    return bs;
+}
+
+// Parses a gathered picture's slice data ahead of its turn (on a thread of its own): the
+// records and levels, with each MB's vector and mode syntax kept for the in-order pass.
+// A damaged slice ends as the in-order parse ends it; anything else out of the ordinary
+// (slices out of order, an error at the picture's end) is left to the in-order pass,
+// which parses the picture again and reports it.
+void parse_ahead(PicJob *job, int picture) {
+    const int W = job->sps.width_mbs, H = job->sps.height_map_units, n_mbs = W * H;
+    job->mbs.assign((size_t)n_mbs, h264g_mb());
+    job->levels.clear();
+    job->fin.resize((size_t)n_mbs);
+    int covered = 0;
+    uint32_t n_slices = 0;
+    for (PicJob::SliceNal &s : job->slices) {
+        if (s.sl.first_mb < covered || s.sl.first_mb >= n_mbs) return;
+        for (; covered < s.sl.first_mb; covered++)  // MBs the slices left out
+            undecoded_record(job->mbs.data(), job->levels, covered, H264G_SLICE_NONE);
+        Bits bs = slice_bits(s);
+        PicState st{nullptr, &job->fin, picture, &job->sps, &job->pps, &s.sl, W, H, std::vector<MbNz>((size_t)n_mbs),
+                    job->mbs.data(), &job->levels, n_slices};
+        int mb = s.sl.first_mb;
+        st.mark = job->levels.size();
+        int rc = job->pps.cabac ? parse_slice_data_cabac(bs, st, mb) : parse_slice_data_cavlc(bs, st, mb);
+        if (!rc && bs.over) rc = h264g_set_error(H264G_E_ARG, "picture %d: slice data truncated", picture);
+        if (rc && mb >= n_mbs) return;
+        if (rc) {  // the slice ends with the MB that failed (see assemble_slices)
+            s.damaged_mb = mb;
+            s.damage = h264g_last_error();
+            if (!st.partial) {
+                job->levels.resize(st.mark);
+                undecoded_record(job->mbs.data(), job->levels, mb, n_slices);
+                job->fin[mb].has_mvd = 0;
             }
-            p->cur_sps = sps;
-            if (sl.idr) {
-                p->dpb.clear();
-                p->prev_frame_num = 0;
-            }
-            // A gap in frame_num goes without the "non-existing" frames of 8.2.5.2: the
-            // reference's Decoding_process_for_gaps_in_frame_num does nothing
-            // (H264RefPicList.cpp:1598-1601), so the pictures that remain are the references.
-            poc = decode_poc(p, *sps, sl);
-            if (int rc = slice_lists(p, *sps, sl, poc, lists)) return rc;
-            fill_desc(p, *pps, sl, poc, lists, out->desc);
-            int slot = -1;  // a slot not held by a reference frame
-            for (int k = 0; k < p->dpb_slots && slot < 0; k++) {
-                bool used = false;
-                for (const Frame &f : p->dpb) used |= (f.short_ref || f.long_ref) && f.slot == k;
-                if (!used) slot = k;
-            }
-            if (slot < 0) return h264g_set_error(H264G_E_ARG, "DPB full: more reference frames than max_num_ref_frames");
-            out->desc.dst_slot = slot;
-            p->mbs.assign((size_t)W * H, h264g_mb());
-            p->levels.clear();
-            h264g_mvfield_reset(p->field);
-            if (sl.slice_type == H264G_SLICE_B) {
-                if (lists[1].empty() || lists[1][0] < 0) return h264g_set_error(H264G_E_ARG, "B slice without RefPicList1[0]");
-                const Frame &col = p->dpb[lists[1][0]];
-                if (int rc = h264g_mvfield_colocated(p->field, col.mbs.data(), col.short_ref ? 1 : 0)) return rc;
-            } else {
-                h264g_mvfield_colocated(p->field, nullptr, 0);
-            }
-            first = sl;
-            started = true;
-        } else {
-            // ---- a later slice of the same picture: it must fit the picture's descriptor
-            if (spps != pps) return h264g_set_error(H264G_E_ARG, "picture %d: slices refer to different PPSs", p->pictures);
+            mb++;
+        }
+        s.mb_end = mb;
+        covered = mb;
+        n_slices++;
+    }
+    for (; covered < n_mbs; covered++) undecoded_record(job->mbs.data(), job->levels, covered, H264G_SLICE_NONE);
+    job->ahead_ok = true;
+}
+
+constexpr int kParseAgain = 1;  // the syntax parsed ahead failed to derive: parse in order
+
+// The picture's slices in order, into the records: slice data parsed now (damaged slices
+// end where the reference abandons them), or, when it parsed ahead, the vectors and modes
+// of its MBs derived in MB order.  MBs no slice covers are left undecoded.
+int assemble_slices(h264g_parser *p, PicJob &job, bool ahead, int poc, const std::vector<int> lists[2],
+                    h264g_parsed_picture *out) {
+    const Sps &sps = job.sps;
+    const Pps &pps = job.pps;
+    const int W = sps.width_mbs, H = sps.height_map_units, n_mbs = W * H;
+    int covered = 0, n_slices = 0, lost = 0, first_lost = -1;
+    p->damage[0] = 0;
+    if (!ahead) {
+        job.mbs.assign((size_t)n_mbs, h264g_mb());
+        job.levels.clear();
+    }
+    h264g_mb *mbs = job.mbs.data();
+    for (size_t i = 0; i < job.slices.size(); i++) {
+        PicJob::SliceNal &s = job.slices[i];
+        const Slice &sl = s.sl;
+        if (i > 0) {  // a later slice of the picture must fit the picture's descriptor
             std::vector<int> sl_lists[2];
-            if (int rc = slice_lists(p, *sps, sl, poc, sl_lists)) return rc;
-            if (const char *why = slice_mismatch(*pps, first, sl, lists, sl_lists))
+            if (int rc = slice_lists(p, sps, sl, poc, sl_lists)) return rc;
+            if (const char *why = slice_mismatch(pps, job.slices[0].sl, sl, lists, sl_lists))
                 return h264g_set_error(H264G_E_ARG, "picture %d: its slices do not share one descriptor (%s)", p->pictures, why);
         }
-        if (sl.first_mb < covered || sl.first_mb >= W * H)
+        if (sl.first_mb < covered || sl.first_mb >= n_mbs)
             return h264g_set_error(H264G_E_ARG, "picture %d: slice at MB %d after MB %d", p->pictures, sl.first_mb, covered);
         for (; covered < sl.first_mb; covered++) {  // MBs the slices left out
             if (first_lost < 0) first_lost = covered;
             lost++;
-            if (int rc = undecoded_mb(p, covered, H264G_SLICE_NONE)) return rc;
+            if (int rc = ahead ? h264g_mvfield_derive(p->field, (uint32_t)covered, &mbs[covered], nullptr)
+                               : undecoded_mb(mbs, job.levels, p->field, covered, H264G_SLICE_NONE))
+                return rc;
         }
-        // ---- slice data: neighbours of other slices are not available (fresh per-slice state)
-        PicState st{p, sps, pps, &sl, W, H, std::vector<MbNz>((size_t)W * H), &p->levels, (uint32_t)n_slices};
-        st.mark = p->levels.size();
         int mb = sl.first_mb;
-        int rc = pps->cabac ? parse_slice_data_cabac(bs, st, mb) : parse_slice_data_cavlc(bs, st, mb);
-        if (!rc && bs.over) rc = h264g_set_error(H264G_E_ARG, "picture %d: slice data truncated", p->pictures);
-        if (rc && mb < W * H) {
-            // Damaged slice data: the slice ends with the MB that failed, and decoding goes on
-            // with the next slice, as the reference's does (H264VideoDecoder.cpp:135-136,
-            // CONTINUE_IF_FAILED).  An MB whose residual failed (CAVLC) is reconstructed from
-            // what parsed; any other failed MB stays undecoded.
-            snprintf(p->damage, sizeof(p->damage), "%s", h264g_last_error());
-            if (first_lost < 0) first_lost = mb;
-            lost++;
-            if (!st.partial) {
-                p->levels.resize(st.mark);
-                if (int rc2 = undecoded_mb(p, mb, st.slice_num)) return rc2;
+        if (ahead) {
+            if (s.damaged_mb >= 0) {
+                snprintf(p->damage, sizeof(p->damage), "%s", s.damage.c_str());
+                if (first_lost < 0) first_lost = s.damaged_mb;
+                lost++;
             }
-            mb++;
-        } else if (rc) {
-            return rc;
+            for (; mb < s.mb_end; mb++) {
+                const FinMb &f = job.fin[mb];
+                if (derive_mb(p->field, pps.constrained_intra_pred, mb, mbs[mb], f.has_mvd ? &f.mvd[0][0][0] : nullptr,
+                              &f.isyn[0][0]))
+                    return kParseAgain;
+            }
+        } else {
+            // slice data: neighbours of other slices are not available (fresh per-slice state)
+            Bits bs = slice_bits(s);
+            PicState st{p->field, nullptr, p->pictures, &sps, &pps, &sl, W, H, std::vector<MbNz>((size_t)n_mbs),
+                        mbs, &job.levels, (uint32_t)n_slices};
+            st.mark = job.levels.size();
+            int rc = pps.cabac ? parse_slice_data_cabac(bs, st, mb) : parse_slice_data_cavlc(bs, st, mb);
+            if (!rc && bs.over) rc = h264g_set_error(H264G_E_ARG, "picture %d: slice data truncated", p->pictures);
+            if (rc && mb < n_mbs) {
+                // Damaged slice data: the slice ends with the MB that failed, and decoding goes
+                // on with the next slice, as the reference's does (H264VideoDecoder.cpp:135-136,
+                // CONTINUE_IF_FAILED).  An MB whose residual failed (CAVLC) is reconstructed
+                // from what parsed; any other failed MB stays undecoded.
+                snprintf(p->damage, sizeof(p->damage), "%s", h264g_last_error());
+                if (first_lost < 0) first_lost = mb;
+                lost++;
+                if (!st.partial) {
+                    job.levels.resize(st.mark);
+                    if (int rc2 = undecoded_mb(mbs, job.levels, p->field, mb, st.slice_num)) return rc2;
+                }
+                mb++;
+            } else if (rc) {
+                return rc;
+            }
         }
         covered = mb;
         n_slices++;
     }
-    if (!started) return 0;
-    const int n_mbs = W * H;
     for (; covered < n_mbs; covered++) {  // MBs after the last slice
         if (first_lost < 0) first_lost = covered;
         lost++;
-        if (int rc = undecoded_mb(p, covered, H264G_SLICE_NONE)) return rc;
+        if (int rc = ahead ? h264g_mvfield_derive(p->field, (uint32_t)covered, &mbs[covered], nullptr)
+                           : undecoded_mb(mbs, job.levels, p->field, covered, H264G_SLICE_NONE))
+            return rc;
     }
+    out->n_slices = n_slices;
+    out->lost_mbs = lost;
+    out->first_lost_mb = first_lost;
+    return 0;
+}
+
+// The motion field, fresh for the picture, with its co-located picture for direct prediction.
+int start_field(h264g_parser *p, const Slice &sl, const std::vector<int> lists[2]) {
+    h264g_mvfield_reset(p->field);
+    if (sl.slice_type == H264G_SLICE_B) {
+        if (lists[1].empty() || lists[1][0] < 0) return h264g_set_error(H264G_E_ARG, "B slice without RefPicList1[0]");
+        const Frame &col = p->dpb[lists[1][0]];
+        return h264g_mvfield_colocated(p->field, col.mbs.data(), col.short_ref ? 1 : 0);
+    }
+    return h264g_mvfield_colocated(p->field, nullptr, 0);
+}
+
+// A gathered picture in decoding order: POC, reference lists, descriptor and DPB slot,
+// then its slices, then reference marking.
+int assemble(h264g_parser *p, PicJob &job, h264g_parsed_picture *out) {
+    const Sps &sps = job.sps;
+    const Pps &pps = job.pps;
+    const Slice &sl = job.slices[0].sl;
+    const int W = sps.width_mbs, H = sps.height_map_units;
+    if (p->field && (p->width_mbs != W || p->height_mbs != H))
+        return h264g_set_error(H264G_E_ARG, "the picture size changes inside the stream");
+    if (!p->field) {
+        p->width_mbs = W;
+        p->height_mbs = H;
+        p->dpb_slots = std::max(sps.max_num_ref_frames, 1) + 1;
+        if (h264g_mvfield_create(W, H, &p->field)) return h264g_set_error(H264G_E_NOMEM, "motion field");
+    }
+    if (sl.idr) {
+        p->dpb.clear();
+        p->prev_frame_num = 0;
+    }
+    // A gap in frame_num goes without the "non-existing" frames of 8.2.5.2: the
+    // reference's Decoding_process_for_gaps_in_frame_num does nothing
+    // (H264RefPicList.cpp:1598-1601), so the pictures that remain are the references.
+    const int poc = decode_poc(p, sps, sl);
+    std::vector<int> lists[2];
+    if (int rc = slice_lists(p, sps, sl, poc, lists)) return rc;
+    fill_desc(p, sps, pps, sl, poc, lists, out->desc);
+    int slot = -1;  // a slot not held by a reference frame
+    for (int k = 0; k < p->dpb_slots && slot < 0; k++) {
+        bool used = false;
+        for (const Frame &f : p->dpb) used |= (f.short_ref || f.long_ref) && f.slot == k;
+        if (!used) slot = k;
+    }
+    if (slot < 0) return h264g_set_error(H264G_E_ARG, "DPB full: more reference frames than max_num_ref_frames");
+    out->desc.dst_slot = slot;
+    if (int rc = start_field(p, sl, lists)) return rc;
+    int rc = assemble_slices(p, job, job.ahead_ok, poc, lists, out);
+    if (rc == kParseAgain) {
+        if ((rc = start_field(p, sl, lists))) return rc;
+        rc = assemble_slices(p, job, false, poc, lists, out);
+    }
+    if (rc) return rc;
     // ---- DPB bookkeeping: the current frame, then marking
-    const Slice &sl = first;
     Frame cur;
     cur.slot = out->desc.dst_slot;
     cur.frame_num = sl.frame_num;
     cur.poc = poc;
     if (sl.nal_ref_idc) {
-        mark(p, *sps, sl, cur);
+        mark(p, sps, sl, cur);
         if (p->prev_mmco5) {  // 8.2.1: after memory_management_control_operation 5
             cur.frame_num = 0;
             p->prev_ref_top_poc = 0;
             cur.poc = 0;
         }
-        cur.mbs = p->mbs;
+        cur.mbs = job.mbs;
         p->dpb.erase(std::remove_if(p->dpb.begin(), p->dpb.end(), [](const Frame &f) { return !f.short_ref && !f.long_ref; }),
                      p->dpb.end());
         p->dpb.push_back(cur);
@@ -2361,23 +2550,49 @@ extern "C" int32_t h264g_parser_next(h264g_parser *p, h264g_parsed_picture *out)
     }
     p->prev_raw_frame_num = sl.frame_num;
     p->last_pic_mmco5 = sl.nal_ref_idc && p->prev_mmco5;
-    out->mbs = p->mbs.data();
-    out->n_mbs = (uint32_t)p->mbs.size();
-    out->levels = p->levels.data();
-    out->n_levels = p->levels.size();
+    out->mbs = job.mbs.data();
+    out->n_mbs = (uint32_t)job.mbs.size();
+    out->levels = job.levels.data();
+    out->n_levels = job.levels.size();
     out->width_mbs = W;
     out->height_mbs = H;
     out->dpb_slots = p->dpb_slots;
-    out->max_num_reorder_frames = sps->max_num_reorder_frames;
+    out->max_num_reorder_frames = sps.max_num_reorder_frames;
     out->idr = sl.idr;
     out->nal_ref_idc = sl.nal_ref_idc;
     out->frame_num = sl.frame_num;
-    out->profile_idc = sps->profile_idc;
-    out->constraint_set3 = sps->constraint3;
-    out->level_idc = sps->level_idc;
-    out->n_slices = n_slices;
-    out->lost_mbs = lost;
-    out->first_lost_mb = first_lost;
+    out->profile_idc = sps.profile_idc;
+    out->constraint_set3 = sps.constraint3;
+    out->level_idc = sps.level_idc;
     p->pictures++;
     return 1;
+}
+
+}  // namespace
+
+extern "C" int32_t h264g_parser_next(h264g_parser *p, h264g_parsed_picture *out) {
+    if (!p || !out) return h264g_set_error(H264G_E_ARG, "NULL argument");
+    memset(out, 0, sizeof(*out));
+    p->damage[0] = 0;
+    p->out_job.reset();
+    // keep up to `threads` pictures gathered, the slice data of each parsing on its own thread
+    while (!p->gather_end && (int)p->ahead.size() < p->threads) {
+        std::unique_ptr<PicJob> job = gather(p);
+        if (!job) break;
+        if (p->threads > 1 && job->rc == 0) {
+            try {
+                job->worker = std::thread(parse_ahead, job.get(), p->gathered - 1);
+            } catch (const std::exception &) {  // no thread: the picture parses in order
+            }
+        }
+        p->ahead.push_back(std::move(job));
+    }
+    if (p->ahead.empty()) return 0;
+    std::unique_ptr<PicJob> job = std::move(p->ahead.front());
+    p->ahead.pop_front();
+    if (job->worker.joinable()) job->worker.join();
+    if (job->rc) return h264g_set_error(job->rc, "%s", job->err.c_str());
+    const int rc = assemble(p, *job, out);
+    if (rc == 1) p->out_job = std::move(job);
+    return rc;
 }

@@ -531,7 +531,12 @@ typedef struct h264g_parsed_picture {
     int32_t lost_mbs;             /* H264G_MBF_LOST records (0 for an intact picture)       */
     int32_t first_lost_mb;        /* address of the first of them, -1 when none             */
 } h264g_parsed_picture;
-/* The parser keeps a copy of the byte stream. */
+/* The parser keeps a copy of the byte stream.  Pictures after the one handed out are
+ * gathered ahead and their slice data parsed on worker threads, one per picture: as many
+ * as the environment variable H264G_PARSE_THREADS names when the parser is created (1:
+ * every picture parses on the calling thread), else the host's hardware threads, at most
+ * 8.  Their vectors and modes are derived on the calling thread in decoding order, so
+ * the pictures, errors and damage reports are those of the in-order parse. */
 int32_t h264g_parser_create(const uint8_t *data, uint64_t len, h264g_parser **out);
 int32_t h264g_parser_destroy(h264g_parser *parser);
 /* The next picture in decoding order: 1 with *out filled, 0 at the end of the stream,

@@ -6,6 +6,7 @@ import contextlib
 import hashlib
 import json
 import os
+import re
 
 import numpy as np
 import pytest
@@ -157,6 +158,68 @@ def test_damaged_slice_ends_where_the_reference_abandons_it():
     for a, b in zip(pics, Bitstream(_stream("gop121_idc0_12")).pictures()):
         assert (a.desc.disable_deblocking_filter_idc, b.desc.disable_deblocking_filter_idc) == (2, 0)
         assert np.array_equal(a.mbs, b.mbs) and np.array_equal(a.levels, b.levels)
+
+
+def _parse_with(threads, data):
+    """Every picture of the stream, then the error that ended it (if any), parsed with
+    H264G_PARSE_THREADS = threads (the library reads it when a parser is created)."""
+    old = os.environ.get("H264G_PARSE_THREADS")
+    os.environ["H264G_PARSE_THREADS"] = str(threads)
+    try:
+        out = []
+        try:
+            for p in Bitstream(data).pictures():
+                out.append(p)
+        except (_lib.H264GError, ValueError) as e:
+            out.append(str(e))
+        return out
+    finally:
+        if old is None:
+            del os.environ["H264G_PARSE_THREADS"]
+        else:
+            os.environ["H264G_PARSE_THREADS"] = old
+
+
+def _drop_slice(data, k):
+    """The stream without its k-th slice that does not start a picture (first_mb_in_slice
+    > 0: the first bit of the slice header is 0), or unchanged if it has fewer."""
+    starts = [m.start() for m in re.finditer(b"\x00\x00\x01", data)] + [len(data)]
+    later = [i for i in range(len(starts) - 1)
+             if data[starts[i] + 3] & 31 in (1, 5) and not data[starts[i] + 4] & 0x80]
+    if len(later) <= k:
+        return data
+    i = later[k]
+    return data[:starts[i]] + data[starts[i + 1]:]
+
+
+@pytest.mark.parametrize("name", STREAMS)
+def test_pictures_parsed_ahead_equal_pictures_parsed_in_order(name):
+    """Slice data parsed ahead on worker threads, with the vectors and modes derived
+    afterwards in MB order, gives exactly the records, levels, descriptors and damage
+    reports of the in-order parse (H264G_PARSE_THREADS 1): clean streams, a stream with
+    a slice dropped (MBs of no slice mid-picture), and streams with random bytes flipped
+    (damaged slices, refused pictures, errors after some pictures)."""
+    clean = open(_stream(name), "rb").read()
+    variants = [clean, _drop_slice(clean, 3)]
+    rng = np.random.default_rng(0xA11EAD)
+    first_slice = clean.find(b"\x00\x00\x01\x65")
+    for _ in range(3):
+        d = bytearray(clean)
+        for pos in rng.integers(first_slice + 16, len(d), 3):
+            d[pos] ^= int(rng.integers(1, 256))
+        variants.append(bytes(d))
+    for data in variants:
+        a, b = _parse_with(1, data), _parse_with(4, data)
+        assert len(a) == len(b)
+        for x, y in zip(a, b):
+            if isinstance(x, str) or isinstance(y, str):
+                assert x == y
+                continue
+            assert bytes(x.desc) == bytes(y.desc)
+            assert np.array_equal(x.mbs, y.mbs) and np.array_equal(x.levels, y.levels)
+            assert (x.n_slices, x.lost_mbs, x.first_lost_mb, x.damage, x.idr, x.nal_ref_idc, x.frame_num) == \
+                   (y.n_slices, y.lost_mbs, y.first_lost_mb, y.damage, y.idr, y.nal_ref_idc, y.frame_num)
+    assert (variants[1] != clean) == name.startswith("gop121")  # the multi-slice streams
 
 
 def test_scaling_lists_follow_the_reference_fall_back():
