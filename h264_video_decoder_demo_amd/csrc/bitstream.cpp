@@ -2350,7 +2350,7 @@ Bits slice_bits(const PicJob::SliceNal &s) {
 // A damaged slice ends as the in-order parse ends it; anything else out of the ordinary
 // (slices out of order, an error at the picture's end) is left to the in-order pass,
 // which parses the picture again and reports it.
-void parse_ahead(PicJob *job, int picture) {
+void parse_ahead_slices(PicJob *job, int picture) {
     const int W = job->sps.width_mbs, H = job->sps.height_map_units, n_mbs = W * H;
     job->mbs.assign((size_t)n_mbs, h264g_mb());
     job->levels.clear();
@@ -2385,6 +2385,14 @@ void parse_ahead(PicJob *job, int picture) {
     }
     for (; covered < n_mbs; covered++) undecoded_record(job->mbs.data(), job->levels, covered, H264G_SLICE_NONE);
     job->ahead_ok = true;
+}
+
+void parse_ahead(PicJob *job, int picture) {
+    try {
+        parse_ahead_slices(job, picture);
+    } catch (const std::exception &) {  // out of memory: the picture parses in order
+        job->ahead_ok = false;
+    }
 }
 
 constexpr int kParseAgain = 1;  // the syntax parsed ahead failed to derive: parse in order
