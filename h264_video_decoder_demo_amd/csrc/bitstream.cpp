@@ -13,6 +13,12 @@
 // Motion vectors and intra prediction modes are derived by the motion field
 // (h264g_mvfield_*, mvpred.cpp) from the parsed mvd and prev / rem syntax.
 //
+// Pictures are gathered in stream order on the calling thread (parameter sets, slice
+// headers, each slice's RBSP); the slice data of the pictures after the one handed out is
+// parsed ahead, one worker thread per picture, into records whose vectors and modes the
+// calling thread derives in decoding order (entropy decoding reads only parsed syntax of
+// its neighbours, never derived vectors), so the output is that of the in-order parse.
+//
 // Pictures may have several slices (in order, sharing one reference list, weight table
 // and deblocking setting); each MB record carries its slice number, which makes
 // neighbours in other slices unavailable.  SPS / PPS scaling matrices become the
