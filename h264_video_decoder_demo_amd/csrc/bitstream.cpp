@@ -681,9 +681,12 @@ struct PicJob {
     std::vector<int16_t> levels;
     std::vector<FinMb> fin;
     bool ahead_ok = false;    // the slice data parsed ahead without an error
+    bool derived = false;     // and its vectors and modes too (a picture without B slices)
+    h264g_mvfield *field = nullptr;
     std::thread worker;
     ~PicJob() {
         if (worker.joinable()) worker.join();
+        if (field) h264g_mvfield_destroy(field);
     }
 };
 
@@ -2360,16 +2363,28 @@ void parse_ahead_slices(PicJob *job, int picture) {
     const int W = job->sps.width_mbs, H = job->sps.height_map_units, n_mbs = W * H;
     job->mbs.assign((size_t)n_mbs, h264g_mb());
     job->levels.clear();
-    job->fin.resize((size_t)n_mbs);
+    // Without B slices nothing is predicted from another picture: the worker derives the
+    // vectors and modes itself, into a motion field of its own in the state the in-order
+    // pass starts a picture with.
+    bool own = true;
+    for (const PicJob::SliceNal &s : job->slices) own &= s.sl.slice_type != H264G_SLICE_B;
+    if (own && (h264g_mvfield_create(W, H, &job->field) || h264g_mvfield_reset(job->field) ||
+                h264g_mvfield_colocated(job->field, nullptr, 0)))
+        own = false;
+    h264g_mvfield *field = own ? job->field : nullptr;
+    h264g_mb *mbs = job->mbs.data();
+    if (!own) job->fin.resize((size_t)n_mbs);
     int covered = 0;
     uint32_t n_slices = 0;
     for (PicJob::SliceNal &s : job->slices) {
         if (s.sl.first_mb < covered || s.sl.first_mb >= n_mbs) return;
-        for (; covered < s.sl.first_mb; covered++)  // MBs the slices left out
-            undecoded_record(job->mbs.data(), job->levels, covered, H264G_SLICE_NONE);
+        for (; covered < s.sl.first_mb; covered++) {  // MBs the slices left out
+            if (!own) undecoded_record(mbs, job->levels, covered, H264G_SLICE_NONE);
+            else if (undecoded_mb(mbs, job->levels, field, covered, H264G_SLICE_NONE)) return;
+        }
         Bits bs = slice_bits(s);
-        PicState st{nullptr, &job->fin, picture, &job->sps, &job->pps, &s.sl, W, H, std::vector<MbNz>((size_t)n_mbs),
-                    job->mbs.data(), &job->levels, n_slices};
+        PicState st{field, own ? nullptr : &job->fin, picture, &job->sps, &job->pps, &s.sl, W, H,
+                    std::vector<MbNz>((size_t)n_mbs), mbs, &job->levels, n_slices};
         int mb = s.sl.first_mb;
         st.mark = job->levels.size();
         int rc = job->pps.cabac ? parse_slice_data_cabac(bs, st, mb) : parse_slice_data_cavlc(bs, st, mb);
@@ -2380,8 +2395,12 @@ void parse_ahead_slices(PicJob *job, int picture) {
             s.damage = h264g_last_error();
             if (!st.partial) {
                 job->levels.resize(st.mark);
-                undecoded_record(job->mbs.data(), job->levels, mb, n_slices);
-                job->fin[mb].has_mvd = 0;
+                if (!own) {
+                    undecoded_record(mbs, job->levels, mb, n_slices);
+                    job->fin[mb].has_mvd = 0;
+                } else if (undecoded_mb(mbs, job->levels, field, mb, n_slices)) {
+                    return;
+                }
             }
             mb++;
         }
@@ -2389,7 +2408,11 @@ void parse_ahead_slices(PicJob *job, int picture) {
         covered = mb;
         n_slices++;
     }
-    for (; covered < n_mbs; covered++) undecoded_record(job->mbs.data(), job->levels, covered, H264G_SLICE_NONE);
+    for (; covered < n_mbs; covered++) {
+        if (!own) undecoded_record(mbs, job->levels, covered, H264G_SLICE_NONE);
+        else if (undecoded_mb(mbs, job->levels, field, covered, H264G_SLICE_NONE)) return;
+    }
+    job->derived = own;
     job->ahead_ok = true;
 }
 
@@ -2432,8 +2455,8 @@ int assemble_slices(h264g_parser *p, PicJob &job, bool ahead, int poc, const std
         for (; covered < sl.first_mb; covered++) {  // MBs the slices left out
             if (first_lost < 0) first_lost = covered;
             lost++;
-            if (int rc = ahead ? h264g_mvfield_derive(p->field, (uint32_t)covered, &mbs[covered], nullptr)
-                               : undecoded_mb(mbs, job.levels, p->field, covered, H264G_SLICE_NONE))
+            if (int rc = !ahead ? undecoded_mb(mbs, job.levels, p->field, covered, H264G_SLICE_NONE)
+                         : job.derived ? 0 : h264g_mvfield_derive(p->field, (uint32_t)covered, &mbs[covered], nullptr))
                 return rc;
         }
         int mb = sl.first_mb;
@@ -2443,6 +2466,7 @@ int assemble_slices(h264g_parser *p, PicJob &job, bool ahead, int poc, const std
                 if (first_lost < 0) first_lost = s.damaged_mb;
                 lost++;
             }
+            if (job.derived) mb = s.mb_end;
             for (; mb < s.mb_end; mb++) {
                 const FinMb &f = job.fin[mb];
                 if (derive_mb(p->field, pps.constrained_intra_pred, mb, mbs[mb], f.has_mvd ? &f.mvd[0][0][0] : nullptr,
@@ -2480,8 +2504,8 @@ int assemble_slices(h264g_parser *p, PicJob &job, bool ahead, int poc, const std
     for (; covered < n_mbs; covered++) {  // MBs after the last slice
         if (first_lost < 0) first_lost = covered;
         lost++;
-        if (int rc = ahead ? h264g_mvfield_derive(p->field, (uint32_t)covered, &mbs[covered], nullptr)
-                           : undecoded_mb(mbs, job.levels, p->field, covered, H264G_SLICE_NONE))
+        if (int rc = !ahead ? undecoded_mb(mbs, job.levels, p->field, covered, H264G_SLICE_NONE)
+                     : job.derived ? 0 : h264g_mvfield_derive(p->field, (uint32_t)covered, &mbs[covered], nullptr))
             return rc;
     }
     out->n_slices = n_slices;
@@ -2535,7 +2559,9 @@ int assemble(h264g_parser *p, PicJob &job, h264g_parsed_picture *out) {
     }
     if (slot < 0) return h264g_set_error(H264G_E_ARG, "DPB full: more reference frames than max_num_ref_frames");
     out->desc.dst_slot = slot;
-    if (int rc = start_field(p, sl, lists)) return rc;
+    if (!job.derived) {  // a picture derived ahead never reads the parser's motion field
+        if (int rc = start_field(p, sl, lists)) return rc;
+    }
     int rc = assemble_slices(p, job, job.ahead_ok, poc, lists, out);
     if (rc == kParseAgain) {
         if ((rc = start_field(p, sl, lists))) return rc;
