@@ -15,9 +15,11 @@
 //
 // Pictures are gathered in stream order on the calling thread (parameter sets, slice
 // headers, each slice's RBSP); the slice data of the pictures after the one handed out is
-// parsed ahead, one worker thread per picture, into records whose vectors and modes the
-// calling thread derives in decoding order (entropy decoding reads only parsed syntax of
-// its neighbours, never derived vectors), so the output is that of the in-order parse.
+// parsed ahead, one worker thread per picture.  A picture without B slices is derived
+// there too (vectors and modes, in a motion field of its own); a B picture's records wait
+// for the calling thread, which derives them in decoding order, after the co-located
+// picture (entropy decoding reads only parsed syntax of its neighbours, never derived
+// vectors).  The output is that of the in-order parse.
 //
 // Pictures may have several slices (in order, sharing one reference list, weight table
 // and deblocking setting); each MB record carries its slice number, which makes

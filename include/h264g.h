@@ -535,8 +535,9 @@ typedef struct h264g_parsed_picture {
  * gathered ahead and their slice data parsed on worker threads, one per picture: as many
  * as the environment variable H264G_PARSE_THREADS names when the parser is created (1:
  * every picture parses on the calling thread), else the host's hardware threads, at most
- * 8.  Their vectors and modes are derived on the calling thread in decoding order, so
- * the pictures, errors and damage reports are those of the in-order parse. */
+ * 8.  A worker derives the vectors and modes of a picture without B slices; those of B
+ * pictures (co-located direct prediction) are derived on the calling thread in decoding
+ * order.  The pictures, errors and damage reports are those of the in-order parse. */
 int32_t h264g_parser_create(const uint8_t *data, uint64_t len, h264g_parser **out);
 int32_t h264g_parser_destroy(h264g_parser *parser);
 /* The next picture in decoding order: 1 with *out filled, 0 at the end of the stream,
